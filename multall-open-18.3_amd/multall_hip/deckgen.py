@@ -1,0 +1,381 @@
+"""Synthetic-cascade deck generator in Multall's NEW input format.
+
+The writer emits the label-line / value-line pairs in the order the
+reference's list-directed reader consumes them (``NEW_READIN``,
+multall-open-18.3.f:148-2158; row block at :593-1550, section block at
+:832-1000, inlet/exit block at :1838-1940, mixing-length limits at
+:2100-2125).  Only the options this build supports are emitted: no cooling,
+no bleed, no shroud, ``ISHIFT=0`` (the generator lays the inter-row grid
+out itself, with the two J-lines of a mixing plane a fraction of a
+spacing apart), no
+throughflow or Q3D.
+
+Geometry: circular-arc camber line between the inlet and exit metal
+angles, NACA-4-digit thickness distribution (closed trailing edge form),
+straight camber extensions upstream and downstream, constant hub and tip
+radii, and a seeded uniform jitter of +/-``jitter`` (relative) applied to
+the streamwise grid spacing and to the tangential camber coordinate.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+
+@dataclass
+class RowSpec:
+    name: str = "STATOR"
+    nblades: int = 60
+    jm: int = 97
+    jle: int = 26
+    jte: int = 72
+    chord_x: float = 0.04          # axial chord, m
+    alpha_in: float = 0.0          # camber angle at the leading edge, degrees
+    alpha_out: float = 70.0        # camber angle at the trailing edge, degrees
+    tmax: float = 0.20             # maximum thickness / chord
+    rpm: float = 0.0
+    up_len: float = 1.0            # upstream length in axial chords
+    dn_len: float = 1.0            # downstream length in axial chords
+    ktips: int = 0                 # tip-gap start K (0 = no gap)
+    ktipe: int = 0
+    fractip: float = 0.0           # tip gap as a fraction of span
+    pguess: tuple = (1.0e5, 1.0e5, 0.8e5, 0.8e5)   # PUP, PLE, PTE, PDN
+
+
+@dataclass
+class DeckSpec:
+    title: str = "SYNTHETIC CASCADE"
+    im: int = 37
+    km: int = 33
+    rows: List[RowSpec] = field(default_factory=lambda: [RowSpec()])
+    r_hub: float = 0.30
+    r_tip: float = 0.35
+    cp: float = 1004.5
+    ga: float = 1.4
+    itimst: int = 3
+    cfl: float = 0.4
+    damp: float = 10.0
+    machlim: float = 2.0
+    f_pdown: float = 0.0
+    nsteps: int = 2000
+    conlim: float = 0.005
+    sfx: float = 0.005
+    sft: float = 0.005
+    fac_4th: float = 0.8
+    nchange: int = 500
+    mg: tuple = (3, 3, 3, 9, 9, 9)
+    fblk: tuple = (0.4, 0.2, 0.1)
+    ifmix: int = 1
+    rfmix: float = 0.025
+    fextrap: float = 0.8
+    fsmthb: float = 1.0
+    fangle: float = 0.8
+    in_press: int = 0
+    in_vtan: int = 0
+    in_vr: int = 1
+    rfin: float = 0.1
+    ipout: int = 0
+    sfexit: float = 0.0
+    nsfexit: int = 0
+    ilos: int = 10
+    nlos: int = 5
+    reyno: float = 5.0e5
+    rf_vis: float = 0.5
+    ftrans: float = 0.0001
+    turbvis_lim: float = 3000.0
+    prandtl: float = 1.0
+    yplam: float = 5.0
+    ypturb: float = 25.0
+    facmixup: float = 2.0
+    nmixup: int = 1000
+    sa_factors: tuple = (0.0, 1.0, 1.0, 1.0, 1.0, 2.0, 0.0, 0.0)
+    po1: float = 1.0e5
+    to1: float = 300.0
+    vm1: float = 100.0
+    pdown_hub: float = 0.8e5
+    pdown_tip: float = 0.8e5
+    fp_ratio: float = 1.0          # geometric expansion of FP towards mid-pitch
+    fr_ratio: float = 1.0
+    seed: int = 0
+    jitter: float = 0.002
+    nsecs: int = 2
+
+
+def _geometric(n: int, ratio: float) -> np.ndarray:
+    """Symmetric spacing weights growing by ``ratio`` towards the middle."""
+    w = np.ones(n)
+    for i in range(1, n):
+        w[i] = w[i - 1] * ratio
+    return np.minimum(w, w[::-1])
+
+
+def _camber(xf: np.ndarray, a1: float, a2: float, cx: float):
+    """Circular-arc camber line: rtheta offset and slope angle at axial
+    fraction ``xf`` in [0, 1] of an arc turning from ``a1`` to ``a2`` (rad)."""
+    if abs(a2 - a1) < 1e-9:
+        return xf * cx * math.tan(a1), np.full_like(xf, a1)
+    rad = cx / (math.sin(a2) - math.sin(a1))
+    s = np.clip(xf * cx / rad + math.sin(a1), -1.0, 1.0)
+    phi = np.arcsin(s)
+    y = -rad * (np.cos(phi) - math.cos(a1))
+    return y, phi
+
+
+def _naca_thickness(xf: np.ndarray, tmax: float) -> np.ndarray:
+    x = np.clip(xf, 0.0, 1.0)
+    return 5.0 * tmax * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2
+                         + 0.2843 * x ** 3 - 0.1036 * x ** 4)
+
+
+def row_sections(spec: DeckSpec, row: RowSpec, x_start: float, rng):
+    """Return (x, rt_upp, rt_thick) along J for one row, and the row's end x."""
+    cx = row.chord_x
+    nup = row.jle - 1                       # intervals upstream of the LE
+    nbl = row.jte - row.jle                 # intervals on the blade
+    ndn = row.jm - row.jte                  # intervals downstream of the TE
+    x_le = x_start + row.up_len * cx
+    x_te = x_le + cx
+    x_end = x_te + row.dn_len * cx
+    # upstream: spacing shrinking towards the LE; blade: cosine clustering.
+    fu = _geometric(2 * nup, 1.08)[:nup][::-1] if nup > 0 else np.zeros(0)
+    fu = fu[::-1]
+    xu = x_start + np.concatenate([[0.0], np.cumsum(fu[::-1])]) / max(fu.sum(), 1e-30) * (x_le - x_start)
+    tb = 0.5 * (1.0 - np.cos(np.linspace(0.0, math.pi, nbl + 1)))
+    xb = x_le + tb * cx
+    fd = _geometric(2 * ndn, 1.08)[:ndn] if ndn > 0 else np.zeros(0)
+    xd = x_te + np.concatenate([[0.0], np.cumsum(fd)]) / max(fd.sum(), 1e-30) * (x_end - x_te)
+    x = np.concatenate([xu[:-1], xb, xd[1:]])
+    assert x.size == row.jm, (x.size, row.jm)
+    # relative jitter of the interior streamwise spacing
+    dx = np.diff(x)
+    jit = 1.0 + spec.jitter * rng.uniform(-1.0, 1.0, size=dx.size)
+    jit[row.jle - 2:row.jle] = 1.0        # keep the LE/TE points exactly on the blade
+    jit[row.jte - 2:row.jte] = 1.0
+    x = x[0] + np.concatenate([[0.0], np.cumsum(dx * jit)])
+    x *= (x_end - x_start) / (x[-1] - x[0])
+    x += x_start - x[0]
+    a1 = math.radians(row.alpha_in)
+    a2 = math.radians(row.alpha_out)
+    xf = (x - x[row.jle - 1]) / (x[row.jte - 1] - x[row.jle - 1])
+    yb, phi = _camber(np.clip(xf, 0.0, 1.0), a1, a2, cx)
+    y_te = yb[row.jte - 1]
+    y = np.where(xf < 0.0, (x - x[row.jle - 1]) * math.tan(a1),
+                 np.where(xf > 1.0, y_te + (x - x[row.jte - 1]) * math.tan(a2), yb))
+    y = y + spec.jitter * cx * 0.01 * rng.uniform(-1.0, 1.0, size=y.size) * (xf > 0.0) * (xf < 1.0)
+    t = _naca_thickness(xf, row.tmax) * cx
+    t = np.where((xf <= 0.0) | (xf >= 1.0), 0.0, t / np.cos(phi))
+    rt_upp = y + 0.5 * t
+    return x, rt_upp, t, x_end
+
+
+def _fmt(vals, per_line=8, fmt="{:12.6f}"):
+    out = []
+    vals = list(vals)
+    for i in range(0, len(vals), per_line):
+        out.append("".join(fmt.format(v) for v in vals[i:i + per_line]))
+    return "\n".join(out)
+
+
+def _ints(vals):
+    return " ".join(str(int(v)) for v in vals)
+
+
+def write_deck(spec: DeckSpec) -> str:
+    """Render ``spec`` as NEW-format deck text."""
+    rng = np.random.default_rng(spec.seed)
+    L: List[str] = []
+    a = L.append
+    a(spec.title[:72])
+    a("    CP   and   GAMMA")
+    a(f"{spec.cp:12.4f}{spec.ga:12.4f}")
+    a("       ITIMST")
+    a(f"{spec.itimst:10d}")
+    a("     CFL,    DAMP,    MACHLIM,    F_PDOWN")
+    a(f"{spec.cfl:12.6f}{spec.damp:12.6f}{spec.machlim:12.6f}{spec.f_pdown:12.6f}")
+    a("  IF_RESTART")
+    a("         0")
+    a("  NSTEPS_MAX, CONLIM")
+    a(f"{spec.nsteps:10d}  {spec.conlim:.9g}")
+    a("   SFX,      SFT,      FAC_4TH,     NCHANGE")
+    a(f"{spec.sfx:12.6f}{spec.sft:12.6f}{spec.fac_4th:12.6f}{spec.nchange:10d}")
+    a("       NUMBER OF BLADE ROWS")
+    a(f"{len(spec.rows):10d}")
+    a("        IM        KM")
+    a(f"{spec.im:10d}{spec.km:10d}")
+    a("  FP(I),I=1,IMM1")
+    a(_fmt(_geometric(spec.im - 1, spec.fp_ratio)))
+    a("  FR(K),K=1,KMM1")
+    a(_fmt(_geometric(spec.km - 1, spec.fr_ratio)))
+    a("        IR        JR        KR        IRBB      JRBB      KRBB")
+    a("".join(f"{v:10d}" for v in spec.mg))
+    a("   FBLK1,     FBLK2,     FBLK3")
+    a("".join(f"{v:12.6f}" for v in spec.fblk))
+    a("       IFMIX")
+    a(f"{spec.ifmix:10d}")
+    if spec.ifmix != 0:
+        a("   RFMIX,    FEXTRAP,   FSMTHB,    FANGLE")
+        a(f"{spec.rfmix:12.6f}{spec.fextrap:12.6f}{spec.fsmthb:12.6f}{spec.fangle:12.6f}")
+    a("      IFCOOL    IFBLEED    IF_ROUGH")
+    a("         0         0         0")
+    a("       NSECS_IN")
+    a(f"{spec.nsecs:10d}")
+    a("       IN_PRESS  IN_VTAN   IN_VR    IN_FLOW  IF_REPEAT  RFIN")
+    a(f"{spec.in_press:10d}{spec.in_vtan:10d}{spec.in_vr:10d}         0         0{spec.rfin:12.5f}")
+    a("  IPOUT SFEXIT  NSFEXIT")
+    a(f"{spec.ipout:5d}{spec.sfexit:12.6f}{spec.nsfexit:5d}")
+    a("  PLATE_LOSS THROTTLE_EXIT")
+    a("  0.000000  0.000000")
+    a("        ILOS      NLOS      IBOUND")
+    a(f"{spec.ilos:10d}{spec.nlos:10d}         0")
+    if spec.ilos != 0:
+        a("   REYNO,     RF_VIS,   FTRANS, TURBVIS_LIM, PRANDTL, YPLUSWALL")
+        a(f"{spec.reyno:12.1f}{spec.rf_vis:10.3f}{spec.ftrans:10.5f}{spec.turbvis_lim:10.3f}"
+          f"{spec.prandtl:10.3f}     0.000")
+    if spec.ilos >= 200:
+        a("  FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD")
+        a("".join(f"{v:10.4f}" for v in spec.sa_factors))
+    if spec.ilos != 0:
+        a("   YPLAM      YPTURB")
+        a(f"{spec.yplam:12.6f}{spec.ypturb:12.6f}")
+    a("      ISHIFT    NEXTRAP_LE  NEXTRAP_TE")
+    a("         0        10        10")
+    a("  NSTG(N),N=1,NROWS")
+    a("".join(f"{1 + n // 2:10d}" for n in range(len(spec.rows))))
+    a("  5  TIME STEPS FOR PRINTOUT")
+    big = spec.nsteps + 10
+    a("".join(f"{big:10d}" for _ in range(5)))
+    a("  MARKER FOR VARIABLES TO BE SENT TO THE OUTPUT FILE.")
+    a(" " + _ints([0] * 13))
+    a("  STREAM SURFACES ON WHICH RESULTS ARE TO BE SENT TO   THE OUTPUT FILE")
+    a(" " + _ints([0] * spec.km))
+
+    x_start = 0.0
+    span = spec.r_tip - spec.r_hub
+    for nr, row in enumerate(spec.rows):
+        x, rtu, rtt, x_end = row_sections(spec, row, x_start, rng)
+        # the next row starts a small fraction of a spacing downstream, so the
+        # two J-lines of the mixing plane are nearly but not exactly coincident
+        x_start = x_end + 0.02 * (x[-1] - x[-2])
+        a(" ******************************************************************************")
+        a(" ***********STARTING THE INPUT FOR EACH BLADE ROW******************************")
+        a(f"{row.name:<72s}")
+        a("    NUMBER OF BLADES IN ROW")
+        a(f"{row.nblades:10d}")
+        a("        JM        JLE       JTE")
+        a(f"{row.jm:10d}{row.jle:10d}{row.jte:10d}")
+        a("      KTIPSTART  KTIPEND")
+        a(f"{row.ktips:10d}{row.ktipe:10d}")
+        if row.ktips > 0:
+            a("  TIP GAPS")
+            a(f"{row.fractip:12.6f}{row.fractip:12.6f}")
+            a("  BLADE THINNING")
+            a(_fmt([1.0] * spec.km, 16, "{:5.1f}"))
+        a("       BOUNDARY LAYER TRANSITION POINTS")
+        a("         0         0         0         0")
+        a("  SET NEWGRID = 1 TO GENERATE A NEW GRID")
+        a("         0")
+        a("   RPMROW,    RPMHUB")
+        hub_rpm = row.rpm
+        a(f"{row.rpm:10.1f}{hub_rpm:10.1f}")
+        a("       JROTHS    JROTHE    JROTTS    JROTTE")
+        a("         0         0         0         0")
+        a("   PUPROW    PLEROW   PTEROW    PDROW")
+        a("".join(f"{v:10.1f}" for v in row.pguess))
+        a("      NSECS_ROW   INSURF")
+        a(f"{spec.nsecs:10d}         0")
+        a("  IF_CUSP   IFANGLES")
+        a("         0         0")
+        for ks in range(spec.nsecs):
+            fr = ks / (spec.nsecs - 1)
+            r = spec.r_hub + fr * span
+            a(" ******************************************************************************")
+            a(f"  ROW NUMBER {nr + 1:11d}  SECTION NUMBER {ks + 1:11d}")
+            a("           0           0           0  IF_DESIGN etc")
+            a("   1.00000   0.00000")
+            a(_fmt(x))
+            a("   1.00000   0.00000")
+            a(_fmt(rtu))
+            a("   1.00000")
+            a(_fmt(rtt))
+            a("   1.00000   0.00000")
+            a(_fmt(np.full(row.jm, r)))
+    kin = 2
+    a("  STARTING INLET BOUNDARY CONDITION DATA .")
+    a("  NUMBER OF POINTS FOR INLET BOUNDARY CONDITIONS")
+    a(f"{kin:10d}")
+    a("  SPACING OF INLET BOUNDARY CONDITION POINTS")
+    a(_fmt([1.0] * (kin - 1)))
+    a("   INLET STAGNATION PRESSURES")
+    a(_fmt([spec.po1] * kin, 8, "{:12.1f}"))
+    a("   INLET STAGNATION TEMPERATURES")
+    a(_fmt([spec.to1] * kin, 8, "{:12.3f}"))
+    a("   INLET TANGENTIAL VELOCITY")
+    a(_fmt([0.0] * kin, 8, "{:12.4f}"))
+    a("   INLET MERIDIONAL VELOCITY")
+    a(_fmt([spec.vm1] * kin, 8, "{:12.4f}"))
+    a("   INLET YAW ANGLE")
+    a(_fmt([0.0] * kin, 8, "{:12.4f}"))
+    a("   INLET PITCH ANGLE")
+    a(_fmt([0.0] * kin, 8, "{:12.4f}"))
+    a("   PDOWN_HUB   PDOWN_TIP")
+    a(f"{spec.pdown_hub:12.1f}{spec.pdown_tip:12.1f}")
+    if spec.ilos != 0:
+        a(" MIXING LENGTH LIMITS ON ALL BLADE ROWS")
+        for _ in spec.rows:
+            if spec.ilos == 10:
+                a("  0.030000  0.030000  0.030000  0.030000  0.030000  0.020000")
+            else:
+                a("  0.020000  0.030000  0.040000  0.050000  1.000000  0.500000")
+        a("  FACTOR TO INCREASE THE TURBULENT VISCOSITY OVER THE FIRST NMIXUP STEPS")
+        a(f"{spec.facmixup:10.5f}{spec.nmixup:6d}")
+    return "\n".join(L) + "\n"
+
+
+# --------------------------------------------------------------------------
+# Named workloads.  Sizes and operating points follow the task definitions.
+
+def deck_tiny(ilos: int = 0, nrows: int = 1, nsteps: int = 100) -> DeckSpec:
+    """A small cascade for exact-parity tests: IM=9, KM=9, 41 J-points/row."""
+    rows = []
+    for n in range(nrows):
+        rotor = (n % 2 == 1)
+        rows.append(RowSpec(name="ROTOR" if rotor else "STATOR", nblades=48 if rotor else 40,
+                            jm=41, jle=11, jte=31,
+                            alpha_in=-20.0 if rotor else 0.0, alpha_out=-60.0 if rotor else 65.0,
+                            rpm=6000.0 if rotor else 0.0,
+                            pguess=(1.0e5, 0.97e5, 0.85e5, 0.83e5) if n == 0 else (0.83e5, 0.82e5, 0.8e5, 0.8e5)))
+    return DeckSpec(title=f"TINY CASCADE ILOS={ilos} ROWS={nrows}", im=9, km=9, rows=rows,
+                    ilos=ilos, nsteps=nsteps, conlim=1e-9, nchange=50, nmixup=50,
+                    pdown_hub=0.8e5, pdown_tip=0.8e5, seed=1)
+
+
+def deck_config0(nsteps: int = 2000) -> DeckSpec:
+    """Single axial turbine stator row, 37x33x97, 60 blades, hub/tip 0.30/0.35 m,
+    camber 0->70 deg, 20% NACA thickness, p0=1e5 Pa, T0=300 K, p_exit=0.8e5 Pa,
+    mixing-length model, scree, multigrid 3^3/9^3, FP32."""
+    row = RowSpec(name="STATOR", nblades=60, jm=97, jle=26, jte=72, chord_x=0.04,
+                  alpha_in=0.0, alpha_out=70.0, tmax=0.20,
+                  pguess=(1.0e5, 0.97e5, 0.82e5, 0.80e5))
+    return DeckSpec(title="CONFIG0 TURBINE STATOR 37X33X97", im=37, km=33, rows=[row],
+                    ilos=10, nsteps=nsteps, conlim=0.005, nchange=500, seed=0,
+                    po1=1.0e5, to1=300.0, vm1=100.0, pdown_hub=0.8e5, pdown_tip=0.8e5)
+
+
+def deck_config1(nsteps: int = 5000, ilos: int = 200) -> DeckSpec:
+    """1.5-stage axial turbine, 3 rows of 64x64x193, blades 48/60/48, rotor at
+    12000 rpm with a 0.5 mm tip gap over 4 K-cells, p0=2e5 Pa, T0=1000 K,
+    p_exit=1.1e5 Pa, Spalart-Allmaras (ILOS=200)."""
+    span = 0.05
+    s1 = RowSpec(name="STATOR 1", nblades=48, jm=193, jle=49, jte=145, alpha_in=0.0, alpha_out=70.0,
+                 pguess=(2.0e5, 1.95e5, 1.55e5, 1.5e5))
+    r1 = RowSpec(name="ROTOR 1", nblades=60, jm=193, jle=49, jte=145, alpha_in=-20.0, alpha_out=-65.0,
+                 rpm=12000.0, ktips=60, ktipe=64, fractip=0.0005 / span,
+                 pguess=(1.5e5, 1.48e5, 1.25e5, 1.22e5))
+    s2 = RowSpec(name="STATOR 2", nblades=48, jm=193, jle=49, jte=145, alpha_in=0.0, alpha_out=70.0,
+                 pguess=(1.22e5, 1.2e5, 1.12e5, 1.1e5))
+    return DeckSpec(title="CONFIG1 1.5-STAGE TURBINE 3X64X64X193", im=64, km=64, rows=[s1, r1, s2],
+                    ilos=ilos, nsteps=nsteps, conlim=0.005, nchange=1000, seed=1,
+                    po1=2.0e5, to1=1000.0, vm1=150.0, pdown_hub=1.1e5, pdown_tip=1.1e5)
