@@ -1,0 +1,1219 @@
+/* TEST INFRASTRUCTURE ONLY (the oracle).  Never linked into the product.
+ *
+ * Sequential FP32 restatement of the reference's time-marching loop; see
+ * mh_oracle.h.  Every routine cites the reference lines it restates
+ * (multall-open-18.3.f, "mf:" below).  Index macros take the reference's
+ * 1-based (I,J,K) so loop bounds read as in the reference; all literals are
+ * float so the arithmetic stays in single precision, statement by
+ * statement, as the reference's does (build with -ffp-contract=off).
+ *
+ * Supported options (others are rejected by orc_start): ideal gas
+ * (IFGAS=0), ITIMST=3 (scree; SSS and residual smoothing through F1/F2/F3
+ * and NRSMTH), IM>2, KM>2, no cooling/bleed/shroud, IN_FLOW=0,
+ * IF_REPEAT=0, no exit throttle or plate loss, IF_CUSP=0, IF_RESTART=0,
+ * ILOS=0 (inviscid) or ILOS=10 (mixing-length LOSS with YPLUSWALL=0).
+ */
+#include "mh_oracle.h"
+
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ----------------------------------------------------------------------- */
+/* state                                                                    */
+
+#define F3D(X) X(RO) X(ROVX) X(ROVR) X(RORVT) X(ROE) X(ROVT) X(ROWT) X(P) X(HO) X(PEFF) \
+  X(VX) X(VR) X(VT) X(WT) X(T_STATIC) X(DRO) X(DROVX) X(DROVR) X(DRORVT) X(DROE)          \
+  X(AQX) X(AQR) X(ABX) X(ABR) X(ASX) X(ASR) X(VOL) X(VOLOR) X(STEP) X(BSTEP) X(RSTEP)     \
+  X(XFORCE) X(RFORCE) X(TFORCE) X(QSOURCE) X(FLOWX) X(FLOWT) X(FLOWR) X(XFLUX) X(TFLUX)  \
+  X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN)
+#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID)
+#define FIK(X) X(FTL) X(FBL) X(FTR) X(FBR)
+#define F1D(X) X(FP) X(FU) X(FD) X(FR) X(FKU) X(FKD) X(PO1) X(TO1) X(RPO1) X(BSSIN) X(BSCOS) \
+  X(BRSIN) X(BRCOS) X(VTIN) X(PD) X(PLIM) X(VM1) X(FACUP) X(FACDWN) X(WRAD) X(WHUB)         \
+  X(WTIP) X(VLIMIT) X(ROLIMIT) X(AVG_CHG) X(AVG_BLK) X(STEP1) X(STEP2) X(STEPSBK)           \
+  X(XLLIM_I1) X(XLLIM_IM) X(XLLIM_K1) X(XLLIM_KM) X(XLLIM_DWN) X(XLLIM_UP) X(CHORD)         \
+  X(ROUGH_H) X(ROUGH_T) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
+  X(REYNOLDS) X(ROVEXIT)
+#define I1D(X) X(IB1) X(IB2) X(KB1) X(KB2) X(NBLADE) X(IND) X(INDLE) X(INDMIX) X(INDTE)   \
+  X(NROW) X(JSBLK) X(JB1) X(JB2) X(JSTART) X(JMIX) X(JLE) X(JTE) X(JLED) X(KTIPS) X(KTIPE) \
+  X(KS1) X(KS2) X(NOUT) X(JTRAN_I1) X(JTRAN_IM) X(JTRAN_K1) X(JTRAN_KM)
+
+#define SCAL(X) X(CFL) X(DAMP) X(DAMPIN) X(MACHLIM) X(F_PDOWN) X(CONLIM) X(SFXIN) X(SFTIN)    \
+  X(SFX) X(SFT) X(FAC_4TH) X(F1) X(F2) X(F3) X(RSMTH) X(SFEXIT) X(FSMTHB) X(RFIN) X(RFIN1)    \
+  X(RFIN2) X(PDOWN_HUB) X(PDOWN_TIP) X(FEXTRAP) X(RFMIX) X(FANGLE) X(CP) X(GA) X(RGAS) X(CV)  \
+  X(FGA) X(GA1) X(RCP) X(VSOUND) X(FACMIXUP) X(FMIXUP) X(REYNO) X(RF_VIS) X(FTRANS)           \
+  X(PRANDTL) X(YPLUSWALL) X(TURBVIS_LIM) X(VISLAM) X(FTCOND) X(PREFF) X(ROLIM) X(ROMIN)       \
+  X(RO_REF) X(EAVG) X(EMAX) X(ECONT) X(VRMS) X(VMAX) X(VREF) X(TCOOL_MIN) X(SFEX1) X(SFEXH)   \
+  X(FLOWRAT)
+#define ISCAL(X) X(IM) X(JM) X(KM) X(NROWS) X(IMID) X(KMID) X(NSTEP) X(NSTEPS_MAX) X(NCHANGE) \
+  X(NRSMTH) X(NSFEXIT) X(IN_PRESS) X(IN_VTAN) X(IN_VR) X(IPOUT) X(IFMIX) X(IR) X(JR) X(KR)    \
+  X(ILOS) X(NLOS) X(NMIXUP) X(NIB1) X(NJB1) X(NKB1) X(NIB2) X(NJB2) X(NKB2) X(NSBLK) X(IFEND) \
+  X(IFPRINT) X(ITIMST) X(IF_RESTART) X(IFGAS) X(IN_FLOW) X(IF_REPEAT) X(IBOUND)
+
+struct orc {
+#define X(n) float *n;
+  F3D(X) FJK(X) FIK(X) F1D(X)
+#undef X
+#define X(n) int *n;
+  I1D(X)
+#undef X
+#define X(n) float n;
+  SCAL(X)
+#undef X
+#define X(n) int n;
+  ISCAL(X)
+#undef X
+  float TCOND;
+  float *CHECK_FLOW, *BLADE_FLOW; /* LOOP's SAVEd locals (mf:3672-3673) */
+  float *B1CHG, *B2CHG, *SBCHG;   /* TSTEP's block accumulators (mf:5482) */
+};
+
+orc_t *orc_create(void) { return (orc_t *)calloc(1, sizeof(orc_t)); }
+void orc_destroy(orc_t *o) {
+  if (!o) return;
+  free(o->CHECK_FLOW); free(o->BLADE_FLOW); free(o->B1CHG); free(o->B2CHG); free(o->SBCHG);
+  free(o);
+}
+
+int orc_bind(orc_t *o, const char *name, void *ptr) {
+#define X(n) if (!strcmp(name, #n)) { o->n = (float *)ptr; return 0; }
+  F3D(X) FJK(X) FIK(X) F1D(X)
+#undef X
+#define X(n) if (!strcmp(name, #n)) { o->n = (int *)ptr; return 0; }
+  I1D(X)
+#undef X
+  return -1;
+}
+
+int orc_scalar(orc_t *o, const char *name, double v) {
+#define X(n) if (!strcmp(name, #n)) { o->n = (float)v; return 0; }
+  SCAL(X)
+#undef X
+#define X(n) if (!strcmp(name, #n)) { o->n = (int)v; return 0; }
+  ISCAL(X)
+#undef X
+  return -1;
+}
+
+double orc_get_scalar(orc_t *o, const char *name) {
+#define X(n) if (!strcmp(name, #n)) return (double)o->n;
+  SCAL(X) ISCAL(X)
+#undef X
+  return NAN;
+}
+
+/* 1-based index helpers, (I,J,K) as in the reference */
+#define IM_ (o->IM)
+#define JM_ (o->JM)
+#define KM_ (o->KM)
+#define A3(a, i, j, k) (o->a[((i) - 1) + (size_t)IM_ * (((j) - 1) + (size_t)JM_ * ((k) - 1))])
+#define D3(p, i, j, k) ((p)[((i) - 1) + (size_t)IM_ * (((j) - 1) + (size_t)JM_ * ((k) - 1))])
+#define AJK(a, j, k) (o->a[((j) - 1) + (size_t)JM_ * ((k) - 1)])
+#define AIK(a, i, k) (o->a[((i) - 1) + (size_t)IM_ * ((k) - 1)])
+#define A1(a, n) (o->a[(n) - 1])
+#define NRSZ 21 /* leading dimension of the (NRS,MAXKI) LOSS tables, mf:commall :73-75 */
+#define ANR(a, nr, i) (o->a[((nr) - 1) + (size_t)NRSZ * ((i) - 1)])
+#define STEP1_(a, b, c) (o->STEP1[((a) - 1) + (size_t)o->NIB1 * (((b) - 1) + (size_t)(o->NJB1 + 1) * ((c) - 1))])
+#define STEP2_(a, b, c) (o->STEP2[((a) - 1) + (size_t)o->NIB2 * (((b) - 1) + (size_t)(o->NJB2 + 1) * ((c) - 1))])
+#define B1_(a, b, c) (o->B1CHG[((a) - 1) + (size_t)o->NIB1 * (((b) - 1) + (size_t)(o->NJB1 + 1) * ((c) - 1))])
+#define B2_(a, b, c) (o->B2CHG[((a) - 1) + (size_t)o->NIB2 * (((b) - 1) + (size_t)(o->NJB2 + 1) * ((c) - 1))])
+
+/* ----------------------------------------------------------------------- */
+/* set-up at the top of LOOP (mf:3674-3714)                                 */
+
+int orc_start(orc_t *o) {
+#define X(n) if (!o->n) return -100;
+  F3D(X) FJK(X) FIK(X)
+#undef X
+  if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 || o->IN_FLOW != 0 ||
+      o->IF_REPEAT != 0 || o->IF_RESTART != 0)
+    return -2;
+  if (o->ILOS != 0 && o->ILOS != 10) return -3;
+  if (o->ILOS == 10 && o->YPLUSWALL < 0.0f) return -3;
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMID = o->IMID, KMID = o->KMID;
+  (void)IM;
+  o->RO_REF = A1(PO1, KMID) / (o->RGAS * A1(TO1, KMID));
+  o->ROMIN = fminf(A3(RO, IMID, 1, KMID), A3(RO, IMID, JM, KMID));
+  o->ROLIM = 0.25f * o->ROMIN;
+  for (int K = 1; K <= KM; K++) A1(PLIM, K) = 0.9f * A1(PO1, K) + 0.1f * A3(P, 1, 1, K);
+  o->PREFF = 0.5f * (A3(P, IMID, 1, KMID) + A3(P, IMID, JM, KMID));
+  o->SFXIN = o->SFXIN * o->CFL / 0.5f;
+  o->SFTIN = o->SFTIN * o->CFL / 0.5f;
+  o->RFIN1 = 1.0f - o->RFIN;
+  o->RFIN2 = 2.f * o->RFIN;
+  o->ECONT = 1.f;
+  o->EAVG = 1.0e06f;
+  o->EMAX = 1.0e06f;
+  o->TCOOL_MIN = 1.0e6f;
+  o->IFEND = 0;
+  o->NSTEP = 0;
+  o->SFEX1 = 1.f - o->SFEXIT;
+  o->SFEXH = 0.5f * o->SFEXIT;
+  free(o->CHECK_FLOW); free(o->BLADE_FLOW); free(o->B1CHG); free(o->B2CHG); free(o->SBCHG);
+  o->CHECK_FLOW = (float *)calloc(JM + 1, sizeof(float));
+  o->BLADE_FLOW = (float *)calloc(JM + 1, sizeof(float));
+  o->B1CHG = (float *)calloc((size_t)o->NIB1 * (o->NJB1 + 1) * o->NKB1 + 1, sizeof(float));
+  o->B2CHG = (float *)calloc((size_t)o->NIB2 * (o->NJB2 + 1) * o->NKB2 + 1, sizeof(float));
+  o->SBCHG = (float *)calloc(o->NSBLK + 1, sizeof(float));
+  return 0;
+}
+
+/* ----------------------------------------------------------------------- */
+/* primitive variables, mf:3826-3897                                        */
+
+void orc_primitives(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, JMM1 = JM - 1;
+  for (int K = 1; K <= KM; K++)
+    for (int J = 2; J <= JM; J++) {
+      float RRNOW = 1 / AJK(R, J, K);
+      for (int I = 1; I <= IM; I++) {
+        float RECIP = 1.0f / A3(RO, I, J, K);
+        A3(VX, I, J, K) = A3(ROVX, I, J, K) * RECIP;
+        A3(VR, I, J, K) = A3(ROVR, I, J, K) * RECIP;
+        A3(ROVT, I, J, K) = A3(RORVT, I, J, K) * RRNOW;
+        A3(VT, I, J, K) = A3(ROVT, I, J, K) * RECIP;
+      }
+    }
+  if (o->NSTEP % 5 == 0 || o->IFEND == 1) {
+    float VRMS = 0.0f, VMAX = 0.0f;
+    for (int K = 1; K <= KM; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IM; I++) {
+          float EKE = A3(VR, I, J, K) * A3(VR, I, J, K) + A3(WT, I, J, K) * A3(WT, I, J, K) +
+                      A3(VX, I, J, K) * A3(VX, I, J, K);
+          if (EKE > VMAX) VMAX = EKE;
+          VRMS = VRMS + EKE;
+        }
+    o->VMAX = sqrtf(VMAX);
+    o->VRMS = VRMS;
+  }
+  for (int K = 1; K <= KM; K++)
+    for (int J = 2; J <= JMM1; J++)
+      for (int I = 1; I <= IM; I++) {
+        float RONOW = A3(RO, I, J, K);
+        float EKE = .5f * (A3(VR, I, J, K) * A3(VR, I, J, K) + A3(VT, I, J, K) * A3(VT, I, J, K) +
+                           A3(VX, I, J, K) * A3(VX, I, J, K));
+        float TSTATIC = (A3(ROE, I, J, K) / RONOW - EKE) / o->CV;
+        A3(P, I, J, K) = RONOW * o->RGAS * TSTATIC;
+        A3(HO, I, J, K) = o->CP * TSTATIC + EKE;
+        A3(T_STATIC, I, J, K) = TSTATIC;
+      }
+}
+
+/* ----------------------------------------------------------------------- */
+/* SMOOTH_RESID, mf:14833-14917                                             */
+
+static void smooth_resid(orc_t *o, float *var, float SMFAC, int NSMTH) {
+  const int IM = o->IM, KM = o->KM, IMM1 = IM - 1, IMM2 = IM - 2, KMM1 = KM - 1, KMM2 = KM - 2;
+  float *TEMPVAR = (float *)malloc(sizeof(float) * (size_t)(o->JM + IM + KM + 8));
+  float SFAC1 = 1.0f - SMFAC, SFACH = 0.5f * SMFAC;
+  for (int NSM = 1; NSM <= NSMTH; NSM++) {
+    for (int NR = 1; NR <= o->NROWS; NR++) {
+      int J1 = A1(JSTART, NR), J1P1 = J1 + 1, J1P2 = J1 + 2, J2 = A1(JMIX, NR), J2M1 = J2 - 1;
+      for (int K = 1; K <= KMM1; K++)
+        for (int I = 1; I <= IMM1; I++) {
+          for (int J = J1P2; J <= J2M1; J++)
+            TEMPVAR[J] = SFAC1 * D3(var, I, J, K) + SFACH * (D3(var, I, J + 1, K) + D3(var, I, J - 1, K));
+          TEMPVAR[J1P1] = D3(var, I, J1P1, K) + SFACH * (D3(var, I, J1P2, K) - D3(var, I, J1P1, K));
+          TEMPVAR[J2] = D3(var, I, J2, K) + SFACH * (D3(var, I, J2M1, K) - D3(var, I, J2, K));
+          for (int J = J1P1; J <= J2; J++) D3(var, I, J, K) = TEMPVAR[J];
+        }
+      for (int K = 1; K <= KMM1; K++)
+        for (int J = J1P1; J <= J2; J++) {
+          for (int I = 2; I <= IMM2; I++)
+            TEMPVAR[I] = SFAC1 * D3(var, I, J, K) + SFACH * (D3(var, I + 1, J, K) + D3(var, I - 1, J, K));
+          TEMPVAR[1] = D3(var, 1, J, K) + SFACH * (D3(var, 2, J, K) - D3(var, 1, J, K));
+          TEMPVAR[IMM1] = D3(var, IMM1, J, K) + SFACH * (D3(var, IMM2, J, K) - D3(var, IMM1, J, K));
+          for (int I = 1; I <= IMM1; I++) D3(var, I, J, K) = TEMPVAR[I];
+        }
+      for (int I = 1; I <= IMM1; I++)
+        for (int J = J1P1; J <= J2; J++) {
+          for (int K = 2; K <= KMM2; K++)
+            TEMPVAR[K] = SFAC1 * D3(var, I, J, K) + SFACH * (D3(var, I, J, K + 1) + D3(var, I, J, K - 1));
+          TEMPVAR[1] = D3(var, I, J, 1) + SFACH * (D3(var, I, J, 2) - D3(var, I, J, 1));
+          TEMPVAR[KMM1] = D3(var, I, J, KMM1) + SFACH * (D3(var, I, J, KMM2) - D3(var, I, J, KMM1));
+          for (int K = 1; K <= KMM1; K++) D3(var, I, J, K) = TEMPVAR[K];
+        }
+    }
+  }
+  free(TEMPVAR);
+}
+
+/* ----------------------------------------------------------------------- */
+/* SMOOTH_VAR, mf:15352-15594                                               */
+
+void orc_smooth_var(orc_t *o, float *D) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, IMM2 = IM - 2, KMM1 = KM - 1,
+            KMM2 = KM - 2;
+  const int NUM3 = 3;
+  const float SFX = o->SFX, SFX1 = 1.f - SFX, FAC_4TH = o->FAC_4TH;
+  const float SFXB = o->FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
+  const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
+  const float SFT = o->SFT, SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
+  int nmax = (IM > KM ? IM : KM) + 2;
+  float *AVG = (float *)malloc(sizeof(float) * nmax), *CURVE = (float *)malloc(sizeof(float) * nmax),
+        *SCURVE = (float *)malloc(sizeof(float) * nmax);
+  float *STORE = o->STORE;
+  if (!(SFX < 0.0001f)) {
+    for (int NR = 1; NR <= o->NROWS; NR++) {
+      int J1 = A1(JSTART, NR), J2 = A1(JMIX, NR);
+      for (int J = J1 + 3; J <= J2 - 3; J++)
+        for (int K = 1; K <= KM; K++)
+          for (int I = 1; I <= IM; I++) {
+            float AVERG = 0.5f * (D3(D, I, J + 1, K) + D3(D, I, J - 1, K));
+            float CURV = AVERG - 0.5f * D3(D, I, J, K) - 0.25f * (D3(D, I, J - 2, K) + D3(D, I, J + 2, K));
+            D3(STORE, I, J, K) = SFX1 * D3(D, I, J, K) + SFX * (AVERG + FAC_4TH * CURV);
+          }
+      for (int K = 1; K <= KM; K++)
+        for (int I = 1; I <= IM; I++) {
+          D3(STORE, I, J2, K) = D3(D, I, J2, K);
+          D3(STORE, I, J1, K) = D3(D, I, J1, K);
+        }
+      for (int K = 1; K <= KM; K++)
+        for (int I = 1; I <= IM; I++) {
+          D3(STORE, I, J1 + 2, K) = SFXB1 * D3(D, I, J1 + 2, K) + SFXBH * (D3(D, I, J1 + 3, K) + D3(D, I, J1 + 1, K));
+          D3(STORE, I, J1 + 1, K) = SFXB1 * D3(D, I, J1 + 1, K) +
+                                    SFXB * (1.25f * D3(D, I, J1 + 2, K) + 0.5f * D3(D, I, J1 + 3, K) -
+                                            0.75f * D3(D, I, J1 + 4, K));
+          D3(STORE, I, J2 - 2, K) = SFXB1 * D3(D, I, J2 - 2, K) + SFXBH * (D3(D, I, J2 - 1, K) + D3(D, I, J2 - 3, K));
+          D3(STORE, I, J2 - 1, K) = SFXB1 * D3(D, I, J2 - 1, K) +
+                                    SFXB * (1.25f * D3(D, I, J2 - 2, K) + 0.5f * D3(D, I, J2 - 3, K) -
+                                            0.75f * D3(D, I, J2 - 4, K));
+        }
+    }
+    for (int NR = 1; NR <= o->NROWS; NR++) {
+      int JS = A1(JLED, NR) - 2, JE = JS + 1;
+      for (int J = JS; J <= JE; J++)
+        for (int K = 1; K <= KM; K++) {
+          D3(STORE, 1, J, K) = SFXHM1 * D3(D, 1, J, K) + SFX14 * (D3(D, 2, J, K) + D3(D, IMM1, J, K));
+          D3(STORE, IM, J, K) = SFXHM1 * D3(D, IM, J, K) + SFX14 * (D3(D, 2, J, K) + D3(D, IMM1, J, K));
+        }
+      int J = A1(JLED, NR);
+      for (int K = 1; K <= KM; K++) {
+        D3(STORE, 1, J, K) = SFXHM1 * D3(D, 1, J, K) + SFX14 * (D3(D, 1, J + 1, K) + D3(D, IM, J, K));
+        D3(STORE, IM, J, K) = SFXHM1 * D3(D, IM, J, K) + SFX14 * (D3(D, IM, J + 1, K) + D3(D, 1, J, K));
+      }
+    }
+    for (int NR = 1; NR <= o->NROWS; NR++) {
+      int J1 = A1(JSTART, NR), J2 = A1(JMIX, NR);
+      for (int J = J1; J <= J2; J++)
+        for (int K = 1; K <= KM; K++)
+          for (int I = 1; I <= IM; I++) D3(D, I, J, K) = D3(STORE, I, J, K);
+    }
+  }
+  if (!(SFT < 0.0001f)) {
+    for (int J = 2; J <= JM; J++) {
+      for (int K = 1; K <= KM; K++) {
+        for (int I = 2; I <= IMM1; I++) {
+          AVG[I] = A1(FD, I) * D3(D, I + 1, J, K) + A1(FU, I) * D3(D, I - 1, J, K);
+          CURVE[I] = D3(D, I, J, K) - AVG[I];
+        }
+        if (!(A1(IND, J) == 1 || A1(INDLE, J) == 1)) {
+          AVG[1] = 0.5f * (D3(D, 2, J, K) + D3(D, IMM1, J, K));
+          AVG[IM] = AVG[1];
+          CURVE[1] = D3(D, 1, J, K) - AVG[1];
+          CURVE[IM] = CURVE[1];
+          SCURVE[1] = 0.5f * (CURVE[2] + CURVE[IMM1]);
+          SCURVE[IM] = SCURVE[1];
+        } else {
+          AVG[1] = D3(D, 2, J, K) + A1(FU, 1) * (D3(D, 2, J, K) - D3(D, NUM3, J, K));
+          AVG[IM] = D3(D, IMM1, J, K) + A1(FU, IM) * (D3(D, IMM1, J, K) - D3(D, IMM2, J, K));
+          CURVE[1] = 0.0f;
+          CURVE[IM] = 0.0f;
+          SCURVE[1] = 0.0f;
+          SCURVE[IM] = 0.0f;
+        }
+        for (int I = 2; I <= IMM1; I++) SCURVE[I] = A1(FU, I) * CURVE[I - 1] + A1(FD, I) * CURVE[I + 1];
+        for (int I = 1; I <= IM; I++)
+          D3(D, I, J, K) = SFT1 * D3(D, I, J, K) + SFT * (AVG[I] + FAC_4TH * SCURVE[I]);
+      }
+      for (int I = 1; I <= IM; I++) {
+        for (int K = 2; K <= KMM1; K++) {
+          AVG[K] = A1(FKD, K) * D3(D, I, J, K + 1) + A1(FKU, K) * D3(D, I, J, K - 1);
+          CURVE[K] = D3(D, I, J, K) - AVG[K];
+        }
+        AVG[1] = D3(D, I, J, 2) + 0.5f * A1(FKU, 1) * (D3(D, I, J, 2) - D3(D, I, J, 3));
+        AVG[KM] = D3(D, I, J, KMM1) + 0.5f * A1(FKU, KM) * (D3(D, I, J, KMM1) - D3(D, I, J, KMM2));
+        CURVE[1] = 0.0f;
+        CURVE[KM] = 0.0f;
+        SCURVE[1] = 0.0f;
+        SCURVE[KM] = 0.0f;
+        for (int K = 2; K <= KMM1; K++) SCURVE[K] = A1(FKU, K) * CURVE[K - 1] + A1(FKD, K) * CURVE[K + 1];
+        for (int K = 1; K <= KM; K++)
+          D3(D, I, J, K) = SFT1 * D3(D, I, J, K) + SFT * (AVG[K] + FAC_4TH * SCURVE[K]);
+      }
+      D3(D, 1, J, 1) = SFT1 * D3(D, 1, J, 1) + SFTH * (D3(D, 2, J, 1) + D3(D, 1, J, 2));
+      D3(D, IM, J, 1) = SFT1 * D3(D, IM, J, 1) + SFTH * (D3(D, IMM1, J, 1) + D3(D, IM, J, 2));
+      D3(D, 1, J, KM) = SFT1 * D3(D, 1, J, KM) + SFTH * (D3(D, 1, J, KMM1) + D3(D, 2, J, KM));
+      D3(D, IM, J, KM) = SFT1 * D3(D, IM, J, KM) + SFTH * (D3(D, IMM1, J, KM) + D3(D, IM, J, KMM1));
+    }
+  }
+  if (!(o->SFEXIT < 0.0001f)) {
+    int JSSTART = JM + 1 - o->NSFEXIT;
+    float SFEX1 = o->SFEX1, SFEXH = o->SFEXH, SFEXIT = o->SFEXIT;
+    for (int J = JSSTART; J <= JM; J++) {
+      for (int K = 1; K <= KM; K++) {
+        for (int I = 2; I <= IMM1; I++)
+          D3(D, I, J, K) = SFEX1 * D3(D, I, J, K) + SFEXH * (D3(D, I + 1, J, K) + D3(D, I - 1, J, K));
+        D3(D, 1, J, K) = D3(D, 1, J, K) + SFEXIT * (2.f * D3(D, 2, J, K) - D3(D, NUM3, J, K) - D3(D, 1, J, K));
+        D3(D, IM, J, K) = D3(D, IM, J, K) + SFEXIT * (2.f * D3(D, IMM1, J, K) - D3(D, IMM2, J, K) - D3(D, IM, J, K));
+      }
+      for (int I = 1; I <= IM; I++) {
+        for (int K = 2; K <= KMM1; K++)
+          D3(D, I, J, K) = SFEX1 * D3(D, I, J, K) + SFEXH * (D3(D, I, J, K - 1) + D3(D, I, J, K + 1));
+        D3(D, I, J, 1) = D3(D, I, J, 1) + SFEXIT * (2.f * D3(D, I, J, 2) - D3(D, I, J, 3) - D3(D, I, J, 1));
+        D3(D, I, J, KM) = D3(D, I, J, KM) + SFEXIT * (2.f * D3(D, I, J, KMM1) - D3(D, I, J, KMM2) - D3(D, I, J, KM));
+      }
+    }
+  }
+  free(AVG); free(CURVE); free(SCURVE);
+}
+
+/* ----------------------------------------------------------------------- */
+/* TSTEP, mf:5470-5932 (NCALL=2, the SA branch, is not restated here)      */
+
+void orc_tstep(orc_t *o, float *D, float *DIFF, int NCALL) {
+  (void)NCALL;
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID;
+  const int NRWSM1 = o->NROWS - 1;
+  float *STORE = o->STORE;
+  for (int K = 1; K <= o->NKB1; K++)
+    for (int J = 1; J <= o->NJB1 + 1; J++)
+      for (int I = 1; I <= o->NIB1; I++) B1_(I, J, K) = 0.0f;
+  for (int K = 1; K <= o->NKB2; K++)
+    for (int J = 1; J <= o->NJB2 + 1; J++)
+      for (int I = 1; I <= o->NIB2; I++) B2_(I, J, K) = 0.0f;
+  for (int J = 1; J <= o->NSBLK; J++) o->SBCHG[J] = 0.0f;
+  /* tip-gap flux balance, mf:5503-5510 */
+  for (int J = 2; J <= JM; J++) {
+    int NR = A1(NROW, J);
+    if (A1(KTIPS, NR) <= 0) continue;
+    for (int K = A1(KTIPS, NR); K <= A1(KTIPE, NR) - 1; K++) {
+      A3(TFLUX, 1, J, K) = 0.5f * (A3(TFLUX, 1, J, K) + A3(TFLUX, IM, J, K));
+      A3(TFLUX, IM, J, K) = A3(TFLUX, 1, J, K);
+    }
+  }
+  /* upstream-face flux extrapolation at mixing planes, mf:5516-5555 */
+  if (!(o->FEXTRAP < 0.001f)) {
+    for (int NR = 1; NR <= NRWSM1; NR++) {
+      int J = A1(JMIX, NR), JP1 = J + 1, JM1 = J - 1, JP2 = J + 2;
+      for (int K = 1; K <= KMM1; K++) {
+        float FLOWDIRN = -A3(FLOWX, IMID, J, K);
+        float AVFLXJM1 = 0.0f, AVFLXJP2 = 0.0f;
+        for (int I = 1; I <= IMM1; I++) {
+          AVFLXJM1 = AVFLXJM1 + A3(XFLUX, I, JM1, K);
+          AVFLXJP2 = AVFLXJP2 + A3(XFLUX, I, JP2, K);
+        }
+        for (int I = 1; I <= IMM1; I++) {
+          if (FLOWDIRN > 0.0f) {
+            float DFLUXJM1 = A3(XFLUX, I, JM1, K) - AVFLXJM1 * A1(FP, I);
+            float DFLUX = DFLUXJM1 * o->FEXTRAP;
+            A3(SOURCE, I, J, K) = A3(SOURCE, I, J, K) - DFLUX;
+          } else {
+            float DFLUXJP2 = A3(XFLUX, I, JP2, K) - AVFLXJP2 * A1(FP, I);
+            float DFLUX = DFLUXJP2 * o->FEXTRAP;
+            A3(SOURCE, I, JP2, K) = A3(SOURCE, I, JP2, K) + DFLUX;
+          }
+          A3(SOURCE, I, JP1, K) = 0.0f;
+        }
+      }
+    }
+  }
+  /* residual and scree/SSS history, mf:5560-5570 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++) {
+      float RATPITCH = (float)A1(NBLADE, J - 1) / (float)A1(NBLADE, J);
+      for (int I = 1; I <= IMM1; I++) {
+        float DELTA = A3(XFLUX, I, J, K) - A3(XFLUX, I, J - 1, K) * RATPITCH + A3(RFLUX, I, J, K) -
+                      A3(RFLUX, I, J, K + 1) + A3(TFLUX, I, J, K) - A3(TFLUX, I + 1, J, K) - A3(SOURCE, I, J, K);
+        D3(STORE, I, J, K) = o->F1 * DELTA + o->F2 * D3(DIFF, I, J, K);
+        D3(DIFF, I, J, K) = DELTA + o->F3 * D3(DIFF, I, J, K);
+      }
+    }
+  /* pitchwise averaging of the change at mixing planes, mf:5578-5607 */
+  for (int NR = 1; NR <= NRWSM1; NR++) {
+    int J = A1(JMIX, NR);
+    for (int K = 1; K <= KMM1; K++) {
+      float FLOWDIRN = -A3(FLOWX, IMID, J, K);
+      if ((FLOWDIRN > 0.0f) || (o->FEXTRAP < 0.001f)) {
+        int JMX = J + 2;
+        float SUM_STORE = 0.0f;
+        for (int I = 1; I <= IMM1; I++) SUM_STORE = SUM_STORE + D3(STORE, I, JMX, K);
+        for (int I = 1; I <= IMM1; I++) D3(STORE, I, JMX, K) = SUM_STORE * A1(FP, I);
+      }
+      if ((FLOWDIRN < 0.0f) || (o->FEXTRAP < 0.001f)) {
+        float SUM_STORE = 0.0f;
+        for (int I = 1; I <= IMM1; I++) SUM_STORE = SUM_STORE + D3(STORE, I, J, K);
+        for (int I = 1; I <= IMM1; I++) D3(STORE, I, J, K) = SUM_STORE * A1(FP, I);
+      }
+    }
+  }
+  /* multigrid block sums and update, mf:5612-5660 */
+  int mg = !(o->IR <= 1 && o->JR <= 1 && o->KR <= 1);
+  if (mg) {
+    for (int K = 1; K <= KMM1; K++) {
+      int K1 = A1(KB1, K), K2 = A1(KB2, K);
+      for (int J = 2; J <= JM; J++) {
+        int JSB = A1(JSBLK, J), J1 = A1(JB1, J), J2 = A1(JB2, J);
+        for (int I = 1; I <= IMM1; I++) {
+          int I1 = A1(IB1, I), I2 = A1(IB2, I);
+          float DELTA = D3(STORE, I, J, K);
+          B1_(I1, J1, K1) = B1_(I1, J1, K1) + DELTA;
+          B2_(I2, J2, K2) = B2_(I2, J2, K2) + DELTA;
+          o->SBCHG[JSB] = o->SBCHG[JSB] + DELTA;
+        }
+      }
+    }
+  }
+  for (int K = 1; K <= KMM1; K++) {
+    int K1 = A1(KB1, K), K2 = A1(KB2, K);
+    for (int J = 2; J <= JM; J++) {
+      int JSB = A1(JSBLK, J), J1 = A1(JB1, J), J2 = A1(JB2, J);
+      for (int I = 1; I <= IMM1; I++) {
+        int I1 = A1(IB1, I), I2 = A1(IB2, I);
+        float DELTA = D3(STORE, I, J, K) * A3(STEP, I, J, K) +
+                      (B1_(I1, J1, K1) * STEP1_(I1, J1, K1) + B2_(I2, J2, K2) * STEP2_(I2, J2, K2) +
+                       o->SBCHG[JSB] * A1(STEPSBK, JSB)) *
+                          A3(RSTEP, I, J, K);
+        D3(STORE, I, J, K) = DELTA;
+      }
+    }
+  }
+  if (o->NRSMTH > 0) smooth_resid(o, STORE, o->RSMTH, o->NRSMTH);
+  /* negative feedback, mf:5677-5724 */
+  if (!(o->DAMP < 2.0f || o->DAMP > 100)) {
+    for (int NR = 1; NR <= o->NROWS; NR++) {
+      float SUMCHG = 0.0f;
+      int JST = A1(JSTART, NR) + 1, JEN = A1(JMIX, NR) - 1;
+      int JCHANGE = JEN - JST + 1;
+      int NSUM = IMM1 * KMM1 * JCHANGE;
+      for (int K = 1; K <= KMM1; K++)
+        for (int J = JST; J <= JEN; J++)
+          for (int I = 1; I <= IMM1; I++) SUMCHG = SUMCHG + fabsf(D3(STORE, I, J, K));
+      A1(AVG_CHG, NR) = SUMCHG / (float)NSUM;
+    }
+    int NROWS = o->NROWS;
+    if (NROWS == 1) A1(AVG_BLK, 1) = A1(AVG_CHG, 1);
+    if (NROWS == 2) {
+      A1(AVG_BLK, 2) = 0.5f * (A1(AVG_CHG, 1) + A1(AVG_CHG, 2));
+      A1(AVG_BLK, 1) = A1(AVG_BLK, 2);
+    }
+    if (NROWS > 2) {
+      for (int NR = 2; NR <= NROWS - 1; NR++)
+        A1(AVG_BLK, NR) = 0.25f * (A1(AVG_CHG, NR - 1) + A1(AVG_CHG, NR + 1)) + 0.5f * A1(AVG_CHG, NR);
+      A1(AVG_BLK, NROWS) = 0.5f * (A1(AVG_CHG, NROWS - 1) + A1(AVG_CHG, NROWS));
+      A1(AVG_BLK, 1) = 0.5f * (A1(AVG_CHG, 1) + A1(AVG_CHG, 2));
+    }
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++) {
+        int NR = A1(NROW, J);
+        for (int I = 1; I <= IMM1; I++) {
+          float DELTA = D3(STORE, I, J, K);
+          float ABSCHG = fabsf(DELTA);
+          float FDAMP = ABSCHG / A1(AVG_BLK, NR);
+          D3(STORE, I, J, K) = DELTA / (1.f + FDAMP / o->DAMP);
+        }
+      }
+  }
+  /* second pitchwise averaging at mixing planes, mf:5786-5818 */
+  for (int NR = 1; NR <= NRWSM1; NR++) {
+    int J = A1(JMIX, NR);
+    for (int K = 1; K <= KMM1; K++) {
+      float FLOWDIRN = -A3(FLOWX, IMID, J, K);
+      if ((FLOWDIRN > 0.0f) || (o->FEXTRAP < 0.001f)) {
+        int JMX = J + 2;
+        float SUM_STORE = 0.0f;
+        for (int I = 1; I <= IMM1; I++) SUM_STORE = SUM_STORE + D3(STORE, I, JMX, K);
+        SUM_STORE = SUM_STORE / (float)IMM1;
+        for (int I = 1; I <= IMM1; I++) D3(STORE, I, JMX, K) = SUM_STORE;
+      }
+      if ((FLOWDIRN < 0.0f) || (o->FEXTRAP < 0.001f)) {
+        float SUM_STORE = 0.0f;
+        for (int I = 1; I <= IMM1; I++) SUM_STORE = SUM_STORE + D3(STORE, I, J, K);
+        SUM_STORE = SUM_STORE / (float)IMM1;
+        for (int I = 1; I <= IMM1; I++) D3(STORE, I, J, K) = SUM_STORE;
+      }
+    }
+  }
+  /* distribute the cell change to its 8 corners, mf:5828-5842 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float ADD = D3(STORE, I, J, K);
+        D3(D, I, J, K) = D3(D, I, J, K) + ADD * AIK(FBL, I, K) * A1(FACDWN, J);
+        D3(D, I + 1, J, K) = D3(D, I + 1, J, K) + ADD * AIK(FBR, I, K) * A1(FACDWN, J);
+        D3(D, I, J, K + 1) = D3(D, I, J, K + 1) + ADD * AIK(FTL, I, K) * A1(FACDWN, J);
+        D3(D, I + 1, J, K + 1) = D3(D, I + 1, J, K + 1) + ADD * AIK(FTR, I, K) * A1(FACDWN, J);
+        D3(D, I, J - 1, K) = D3(D, I, J - 1, K) + ADD * AIK(FBL, I, K) * A1(FACUP, J);
+        D3(D, I + 1, J - 1, K) = D3(D, I + 1, J - 1, K) + ADD * AIK(FBR, I, K) * A1(FACUP, J);
+        D3(D, I, J - 1, K + 1) = D3(D, I, J - 1, K + 1) + ADD * AIK(FTL, I, K) * A1(FACUP, J);
+        D3(D, I + 1, J - 1, K + 1) = D3(D, I + 1, J - 1, K + 1) + ADD * AIK(FTR, I, K) * A1(FACUP, J);
+      }
+  orc_smooth_var(o, D);
+  /* periodicity, tip gap and mixing-plane averages of D, mf:5858-5926 */
+  int ILAST = IM;
+  for (int J = 1; J <= JM; J++) {
+    if ((A1(IND, J) == 1) || (A1(INDLE, J) == 1)) continue;
+    for (int K = 1; K <= KM; K++) {
+      D3(D, 1, J, K) = 0.5f * (D3(D, 1, J, K) + D3(D, ILAST, J, K));
+      D3(D, ILAST, J, K) = D3(D, 1, J, K);
+    }
+  }
+  for (int J = 1; J <= JM; J++) {
+    int NR = A1(NROW, J);
+    if (A1(KTIPS, NR) <= 0) continue;
+    int K1 = A1(KTIPS, NR), K2 = A1(KTIPE, NR);
+    if (K1 == 1) K2 = K2 - 1;
+    if (K2 == KM) K1 = K1 + 1;
+    for (int K = K1; K <= K2; K++) {
+      D3(D, 1, J, K) = 0.5f * (D3(D, 1, J, K) + D3(D, ILAST, J, K));
+      D3(D, ILAST, J, K) = D3(D, 1, J, K);
+    }
+  }
+  if (o->IFMIX != 0) {
+    for (int NR = 1; NR <= NRWSM1; NR++) {
+      int J = A1(JMIX, NR), JP1 = J + 1;
+      for (int K = 1; K <= KM; K++) {
+        float AVGVARJ = 0.0f, AVGVARJP1 = 0.0f;
+        for (int I = 1; I <= IMM1; I++) {
+          AVGVARJ = AVGVARJ + A1(FP, I) * (D3(D, I, J, K) + D3(D, I + 1, J, K));
+          AVGVARJP1 = AVGVARJP1 + A1(FP, I) * (D3(D, I, JP1, K) + D3(D, I + 1, JP1, K));
+        }
+        AVGVARJ = 0.5f * AVGVARJ;
+        AVGVARJP1 = 0.5f * AVGVARJP1;
+        for (int I = 1; I <= IM; I++) {
+          D3(D, I, J, K) = AVGVARJ;
+          D3(D, I, JP1, K) = AVGVARJP1;
+        }
+      }
+    }
+  }
+}
+
+/* ----------------------------------------------------------------------- */
+/* STEPUP, mf:7916-8002 (ideal gas)                                         */
+
+void orc_stepup(orc_t *o, float RELAX) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, KMID = o->KMID;
+  float TLIM = 0.1f * A1(TO1, KMID);
+  float VLIM = 0.01f * A1(VM1, KMID) * A1(VM1, KMID);
+  float RELAX1 = 1.0f - RELAX;
+  float *TEMP1 = o->TEMP1;
+  for (int K = 1; K <= KM; K++)
+    for (int J = 1; J <= JM; J++)
+      for (int I = 1; I <= IM; I++) {
+        float VSQ = A3(VX, I, J, K) * A3(VX, I, J, K) + A3(VT, I, J, K) * A3(VT, I, J, K) +
+                    A3(VR, I, J, K) * A3(VR, I, J, K);
+        float HSTAT = A3(HO, I, J, K) - 0.5f * VSQ;
+        float TSTATIC = HSTAT * o->RCP;
+        float GAMNOW = o->GA;
+        if (TSTATIC < TLIM) TSTATIC = TLIM;
+        float WTREL = A3(WT, I, J, K);
+        float WSQ = VSQ - (A3(VT, I, J, K) * A3(VT, I, J, K) - WTREL * WTREL);
+        if (WSQ < VLIM) WSQ = VLIM;
+        float V_SONIC = sqrtf(GAMNOW * o->RGAS * TSTATIC);
+        float VPLUSC = sqrtf(WSQ) + V_SONIC;
+        D3(TEMP1, I, J, K) = o->VSOUND / VPLUSC;
+      }
+  for (int I = 1; I <= IMM1; I++) {
+    int IP1 = I + 1;
+    for (int K = 1; K <= KMM1; K++) {
+      int KP1 = K + 1;
+      float AMACHP = D3(TEMP1, I, 1, K) + D3(TEMP1, IP1, 1, K) + D3(TEMP1, I, 1, KP1) + D3(TEMP1, IP1, 1, KP1);
+      for (int J = 2; J <= JM; J++) {
+        float AMACHL = D3(TEMP1, I, J, K) + D3(TEMP1, I, J, KP1) + D3(TEMP1, IP1, J, K) + D3(TEMP1, IP1, J, KP1);
+        float AVMACH = 0.125f * (AMACHL + AMACHP);
+        AMACHP = AMACHL;
+        float STEPNEW = A3(BSTEP, I, J, K) * AVMACH;
+        A3(STEP, I, J, K) = RELAX * STEPNEW + RELAX1 * A3(STEP, I, J, K);
+        A3(RSTEP, I, J, K) = A3(STEP, I, J, K) / A3(BSTEP, I, J, K);
+      }
+    }
+  }
+  for (int NR = 1; NR <= o->NROWS - 1; NR++) {
+    int J = A1(JMIX, NR), JP1 = J + 1, JP2 = J + 2;
+    for (int K = 1; K <= KMM1; K++)
+      for (int I = 1; I <= IMM1; I++) {
+        A3(STEP, I, JP1, K) = A3(STEP, I, JP2, K);
+        A3(RSTEP, I, JP1, K) = 0.0f;
+      }
+  }
+}
+
+/* ----------------------------------------------------------------------- */
+/* NEW_MIXPLAN, mf:14922-15251 (ideal gas, ITIMST<5)                        */
+
+void orc_new_mixplan(orc_t *o) {
+  const int IM = o->IM, KM = o->KM, IMID = o->IMID;
+  float *ALPHAA = (float *)calloc(IM + 2, sizeof(float)), *GAMA = (float *)calloc(IM + 2, sizeof(float));
+  float *WREF_JP2 = (float *)calloc(IM + 2, sizeof(float)), *ROREF_JP2 = (float *)calloc(IM + 2, sizeof(float));
+  float *PISENT = (float *)calloc(IM + 2, sizeof(float)), *ROISENT = (float *)calloc(IM + 2, sizeof(float));
+  float *WMACH_JP2 = (float *)calloc(IM + 2, sizeof(float)), *TISENT = (float *)calloc(IM + 2, sizeof(float));
+  float *WIS_JP2 = (float *)calloc(IM + 2, sizeof(float));
+  float RFMIX = o->RFMIX, RFMIX1 = 1.0f - RFMIX;
+  for (int NR = 1; NR <= o->NROWS - 1; NR++) {
+    for (int K = 1; K <= KM; K++) {
+      int KSUB = K;
+      if (K == KM) KSUB = KM - 1;
+      int J = A1(JMIX, NR), JP1, JP2, JP3;
+      float F_ANGLE;
+      if (A3(FLOWX, IMID, J, KSUB) < 0.0f) {
+        JP1 = J + 1; JP2 = J + 2; JP3 = J + 3; F_ANGLE = o->FANGLE;
+      } else {
+        J = A1(JMIX, NR) + 1;
+        JP1 = J - 1; JP2 = J - 2; JP3 = J - 3; F_ANGLE = 0.0f;
+      }
+      float F_ANGLE1 = 1.0f - F_ANGLE;
+      float RJP1 = AJK(R, JP1, K), UJP1 = AJK(UBLADE, JP1, K);
+      float ROMIX = A3(RO, IMID, JP1, K);
+      float VXMIX = A3(ROVX, IMID, JP1, K) / ROMIX;
+      float VRMIX = A3(ROVR, IMID, JP1, K) / ROMIX;
+      float VTMIX = A3(RORVT, IMID, JP1, K) / ROMIX / RJP1;
+      float ROEMIX = A3(ROE, IMID, JP1, K);
+      float VMSQ = VXMIX * VXMIX + VRMIX * VRMIX;
+      float VABSQ = VMSQ + VTMIX * VTMIX;
+      float WTMIX = VTMIX - UJP1;
+      float WMIXSQ = VMSQ + WTMIX * WTMIX;
+      float WMIX = sqrtf(WMIXSQ);
+      float ALPHAMIX = asinf(WTMIX / WMIX);
+      float GAMAMIX = atan2f(VRMIX, VXMIX);
+      float TMIX = (ROEMIX / ROMIX - 0.5f * VABSQ) / o->CV;
+      float ROTHALPY = o->CP * TMIX + 0.5f * (WMIXSQ - UJP1 * UJP1);
+      float GAGAS = o->GA;
+      float PMIX = ROMIX * o->RGAS * TMIX;
+      int NMACH = 0;
+      float UNOWJP2 = 0.0f, RJP2 = 0.0f;
+      for (int I = 1; I <= IM; I++) {
+        RJP2 = AJK(R, JP2, K);
+        UNOWJP2 = AJK(UBLADE, JP2, K);
+        float ROJP2 = A3(RO, I, JP2, K);
+        float XVELJP2 = A3(ROVX, I, JP2, K) / ROJP2;
+        float RVELJP2 = A3(ROVR, I, JP2, K) / ROJP2;
+        float TVELJP2 = A3(RORVT, I, JP2, K) / RJP2 / ROJP2;
+        float WTJP2 = TVELJP2 - UNOWJP2;
+        float VMSQJP2 = XVELJP2 * XVELJP2 + RVELJP2 * RVELJP2;
+        float VELSQJP2 = VMSQJP2 + TVELJP2 * TVELJP2;
+        float WACT_SQ_JP2 = VMSQJP2 + WTJP2 * WTJP2;
+        float WACT_JP2 = sqrtf(WACT_SQ_JP2);
+        float RO_ACT_JP2 = ROJP2;
+        float T_ACT_JP2 = (A3(ROE, I, JP2, K) / ROJP2 - 0.5f * VELSQJP2) / o->CV;
+        float RJP3 = AJK(R, JP3, K), UNOWJP3 = AJK(UBLADE, JP3, K);
+        float ROJP3 = A3(RO, I, JP3, K);
+        float ROVXJP3 = A3(ROVX, I, JP3, K), ROVRJP3 = A3(ROVR, I, JP3, K);
+        float ROVTJP3 = A3(RORVT, I, JP3, K) / RJP3;
+        float ROWTJP3 = ROVTJP3 - UNOWJP3 * ROJP3;
+        float ROVMSQJP3 = ROVXJP3 * ROVXJP3 + ROVRJP3 * ROVRJP3;
+        float ROWRELSQJP3 = ROVMSQJP3 + ROWTJP3 * ROWTJP3;
+        float ROWRELJP3 = sqrtf(ROWRELSQJP3);
+        float ALPHAJP3 = asinf(ROWTJP3 / ROWRELJP3);
+        float GAMAJP3 = atan2f(ROVRJP3, ROVXJP3);
+        /* the JMIX+4 angles (mf:15065-15076) are computed but unused */
+        float PSTAT = ROJP2 * o->RGAS * T_ACT_JP2;
+        float TISENT_JP2 = TMIX * powf(PSTAT / PMIX, o->FGA);
+        float WIS_SQ_JP2 = 2 * (ROTHALPY - o->CP * TISENT_JP2 + 0.5f * UNOWJP2 * UNOWJP2);
+        float ROISENT_JP2 = PSTAT / o->RGAS / TISENT_JP2;
+        ROISENT[I] = ROISENT_JP2;
+        PISENT[I] = PSTAT;
+        TISENT[I] = TISENT_JP2;
+        if (WIS_SQ_JP2 < 1.0f) WIS_SQ_JP2 = 1.0f;
+        WIS_JP2[I] = sqrtf(WIS_SQ_JP2);
+        WREF_JP2[I] = WACT_JP2;
+        ROREF_JP2[I] = RO_ACT_JP2;
+        WMACH_JP2[I] = WACT_JP2 / sqrtf(GAGAS * o->RGAS * T_ACT_JP2);
+        if (WMACH_JP2[I] > 1.0f) NMACH = NMACH + 1;
+        ALPHAA[I] = F_ANGLE1 * ALPHAMIX + F_ANGLE * ALPHAJP3;
+        GAMA[I] = F_ANGLE1 * GAMAMIX + F_ANGLE * GAMAJP3;
+      }
+      if (NMACH != 0) {
+        float FROTN = 1.0f;
+        if (WTMIX < 0.0f) FROTN = -1.0f;
+        for (int I = IMID + 1; I <= IM; I++) {
+          int IM1 = I - 1;
+          if (WMACH_JP2[I] > 1.001f && WMACH_JP2[IM1] > 1.001f) {
+            float DELP = PISENT[I] - PISENT[IM1];
+            float DELTAI = sqrtf(WMACH_JP2[I] * WMACH_JP2[I] - 1) / (ROREF_JP2[I] * WREF_JP2[I] * WREF_JP2[I]);
+            float DELTAIM1 = sqrtf(WMACH_JP2[IM1] * WMACH_JP2[IM1] - 1) / (ROREF_JP2[IM1] * WREF_JP2[IM1] * WREF_JP2[IM1]);
+            ALPHAA[I] = ALPHAA[IM1] + FROTN * DELP * 0.5f * (DELTAI + DELTAIM1);
+          }
+        }
+        for (int I = IMID - 1; I >= 1; I--) {
+          int IP1 = I + 1;
+          if (WMACH_JP2[I] > 1.001f && WMACH_JP2[IP1] > 1.001f) {
+            float DELP = PISENT[I] - PISENT[IP1];
+            float DELTAI = sqrtf(WMACH_JP2[I] * WMACH_JP2[I] - 1) / (ROREF_JP2[I] * WREF_JP2[I] * WREF_JP2[I]);
+            float DELTAIP1 = sqrtf(WMACH_JP2[IP1] * WMACH_JP2[IP1] - 1) / (ROREF_JP2[IP1] * WREF_JP2[IP1] * WREF_JP2[IP1]);
+            ALPHAA[I] = ALPHAA[IP1] + FROTN * DELP * 0.5f * (DELTAI + DELTAIP1);
+          }
+        }
+        float ROTN = ALPHAA[IM] - ALPHAA[1];
+        float SUMFP = 0.0f;
+        for (int I = 1; I <= IM; I++) {
+          ALPHAA[I] = ALPHAA[I] - ROTN * (SUMFP - 0.5f);
+          SUMFP = SUMFP + A1(FP, I);
+        }
+      }
+      for (int I = 1; I <= IM; I++) {
+        float RO_JP2 = ROISENT[I];
+        float WT_JP2 = WIS_JP2[I] * sinf(ALPHAA[I]);
+        float VM_JP2 = WIS_JP2[I] * cosf(ALPHAA[I]);
+        float VX_JP2 = VM_JP2 * cosf(GAMA[I]);
+        float VR_JP2 = VM_JP2 * sinf(GAMA[I]);
+        float VT_JP2 = WT_JP2 + UNOWJP2;
+        float V_JP2_SQ = VM_JP2 * VM_JP2 + VT_JP2 * VT_JP2;
+        float ROE_JP2 = ROISENT[I] * (o->CV * TISENT[I] + 0.5f * V_JP2_SQ);
+        A3(RO, I, JP2, K) = RFMIX1 * A3(RO, I, JP2, K) + RFMIX * RO_JP2;
+        A3(ROVX, I, JP2, K) = RFMIX1 * A3(ROVX, I, JP2, K) + RFMIX * RO_JP2 * VX_JP2;
+        A3(ROVR, I, JP2, K) = RFMIX1 * A3(ROVR, I, JP2, K) + RFMIX * RO_JP2 * VR_JP2;
+        A3(RORVT, I, JP2, K) = RFMIX1 * A3(RORVT, I, JP2, K) + RFMIX * RO_JP2 * VT_JP2 * RJP2;
+        A3(ROE, I, JP2, K) = RFMIX1 * A3(ROE, I, JP2, K) + RFMIX * ROE_JP2;
+      }
+    }
+  }
+  free(ALPHAA); free(GAMA); free(WREF_JP2); free(ROREF_JP2); free(PISENT); free(ROISENT);
+  free(WMACH_JP2); free(TISENT); free(WIS_JP2);
+}
+
+void orc_loss(orc_t *o) { (void)o; }
+
+/* ----------------------------------------------------------------------- */
+/* one time step: the body of LOOP from label 5000 to the GO TO 5000        */
+
+static void step_once(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, JMM1 = JM - 1;
+  const int IMID = o->IMID, KMID = o->KMID;
+  /* step bookkeeping and smoothing/damping ramp, mf:3753-3824 */
+  o->IFPRINT = 0;
+  o->NSTEP = o->NSTEP + 1;
+  int N = o->NSTEP;
+  if (o->NSTEP == o->NSTEPS_MAX) o->IFEND = 1;
+  if (o->EAVG < o->CONLIM && o->ECONT < 0.01f) o->IFEND = 1;
+  if (fabsf(o->EMAX) < (4 * o->CONLIM) && o->ECONT < 0.01f) o->IFEND = 1;
+  if (o->NOUT)
+    for (int I = 1; I <= 5; I++)
+      if (o->NSTEP == A1(NOUT, I)) o->IFPRINT = 1;
+  (void)N;
+  float FCHANGE = 1.0f - (float)(o->NSTEP - 1) / (float)o->NCHANGE;
+  if (FCHANGE > 0.0f) {
+    o->SFT = o->SFTIN + 0.02f * FCHANGE;
+    o->SFX = o->SFXIN + 0.02f * FCHANGE;
+    o->DAMP = o->DAMPIN * (1.f - 0.75f * FCHANGE);
+  }
+  float FMIXUP = 1.0f + (o->FACMIXUP - 1.f) * (float)(o->NMIXUP - o->NSTEP) / (float)o->NMIXUP;
+  if (FMIXUP < 1.0f) FMIXUP = 1.0f;
+  o->FMIXUP = FMIXUP * FMIXUP;
+
+  orc_primitives(o);
+  if (o->ILOS == 10 && o->NSTEP % o->NLOS == 0) orc_loss(o);
+
+  /* mass fluxes, mf:3960-4190 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 1; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGRVX = A3(ROVX, I, J, K) + A3(ROVX, I, J, K + 1) + A3(ROVX, I + 1, J, K) + A3(ROVX, I + 1, J, K + 1);
+        float AVGRVR = A3(ROVR, I, J, K) + A3(ROVR, I, J, K + 1) + A3(ROVR, I + 1, J, K) + A3(ROVR, I + 1, J, K + 1);
+        A3(FLOWX, I, J, K) = 0.25f * (AVGRVX * A3(AQX, I, J, K) + AVGRVR * A3(AQR, I, J, K));
+        A3(SOURCE, I, J, K) = 0.0f;
+        A3(XFLUX, I, J, K) = A3(FLOWX, I, J, K);
+      }
+  for (int K = 1; K <= KM; K++)
+    for (int J = 1; J <= JM; J++)
+      for (int I = 1; I <= IM; I++) {
+        A3(ROWT, I, J, K) = A3(ROVT, I, J, K) - AJK(UBLADE, J, K) * A3(RO, I, J, K);
+        A3(WT, I, J, K) = A3(ROWT, I, J, K) / A3(RO, I, J, K);
+      }
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IM; I++) {
+        float AVGRVX = A3(ROVX, I, J, K) + A3(ROVX, I, J - 1, K) + A3(ROVX, I, J - 1, K + 1) + A3(ROVX, I, J, K + 1);
+        float AVGRVT = A3(ROWT, I, J, K) + A3(ROWT, I, J - 1, K) + A3(ROWT, I, J - 1, K + 1) + A3(ROWT, I, J, K + 1);
+        float AVGRVR = A3(ROVR, I, J, K) + A3(ROVR, I, J - 1, K) + A3(ROVR, I, J - 1, K + 1) + A3(ROVR, I, J, K + 1);
+        A3(FLOWT, I, J, K) = (AVGRVX * A3(ABX, I, J, K) + AVGRVT * AJK(ABT, J, K) + AVGRVR * A3(ABR, I, J, K)) * 0.25f;
+        A3(TFLUX, I, J, K) = A3(FLOWT, I, J, K);
+      }
+  for (int J = 2; J <= JM; J++) {
+    if (A1(IND, J) == 1) continue;
+    for (int K = 1; K <= KMM1; K++) {
+      float AVGFLUX = 0.5f * (A3(FLOWT, 1, J, K) + A3(FLOWT, IM, J, K));
+      A3(FLOWT, 1, J, K) = AVGFLUX;
+      A3(FLOWT, IM, J, K) = AVGFLUX;
+      A3(TFLUX, 1, J, K) = AVGFLUX;
+      A3(TFLUX, IM, J, K) = AVGFLUX;
+    }
+  }
+  for (int J = 2; J <= JM; J++) {
+    if (A1(IND, J) != 1) continue;
+    int NR = A1(NROW, J);
+    for (int K = A1(KS1, NR); K <= A1(KS2, NR); K++) {
+      A3(TFLUX, 1, J, K) = 0.0f;
+      A3(TFLUX, IM, J, K) = 0.0f;
+      A3(FLOWT, 1, J, K) = 0.0f;
+      A3(FLOWT, IM, J, K) = 0.0f;
+    }
+  }
+  for (int K = 2; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGRVX = A3(ROVX, I, J, K) + A3(ROVX, I, J - 1, K) + A3(ROVX, I + 1, J - 1, K) + A3(ROVX, I + 1, J, K);
+        float AVGRVR = A3(ROVR, I, J, K) + A3(ROVR, I, J - 1, K) + A3(ROVR, I + 1, J - 1, K) + A3(ROVR, I + 1, J, K);
+        A3(FLOWR, I, J, K) = 0.25f * (AVGRVX * A3(ASX, I, J, K) + AVGRVR * A3(ASR, I, J, K));
+        A3(RFLUX, I, J, K) = A3(FLOWR, I, J, K);
+      }
+  for (int J = 2; J <= JM; J++)
+    for (int I = 1; I <= IMM1; I++) {
+      A3(FLOWR, I, J, 1) = 0.0f;
+      A3(FLOWR, I, J, KM) = 0.0f;
+      A3(RFLUX, I, J, 1) = 0.0f;
+      A3(RFLUX, I, J, KM) = 0.0f;
+    }
+  orc_tstep(o, o->RO, o->DRO, 1);
+
+  /* energy fluxes, mf:4357-4456 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 1; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGHO = A3(HO, I, J, K) + A3(HO, I, J, K + 1) + A3(HO, I + 1, J, K) + A3(HO, I + 1, J, K + 1);
+        A3(XFLUX, I, J, K) = A3(FLOWX, I, J, K) * AVGHO * 0.25f;
+      }
+  for (int J = 2; J <= JM; J++)
+    for (int K = 1; K <= KMM1; K++)
+      for (int I = 1; I <= IMM1; I++) A3(SOURCE, I, J, K) = A3(QSOURCE, I, J, K);
+  for (int K = 1; K <= KM; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGHO = 0.25f * (A3(HO, I, J, K) + A3(HO, I, J - 1, K) + A3(HO, I + 1, J - 1, K) + A3(HO, I + 1, J, K));
+        A3(RFLUX, I, J, K) = A3(FLOWR, I, J, K) * AVGHO;
+      }
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IM; I++) {
+        float AVGHO = A3(HO, I, J, K) + A3(HO, I, J - 1, K) + A3(HO, I, J - 1, K + 1) + A3(HO, I, J, K + 1);
+        float AVGPB = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I, J - 1, K + 1) + A3(PEFF, I, J, K + 1);
+        A3(TFLUX, I, J, K) = (A3(FLOWT, I, J, K) * AVGHO + AJK(WRABT, J, K) * AVGPB) * 0.25f;
+      }
+  for (int J = 2; J <= JM; J++) {
+    if (A1(IND, J) == 1) continue;
+    for (int K = 1; K <= KMM1; K++) {
+      A3(TFLUX, 1, J, K) = 0.5f * (A3(TFLUX, 1, J, K) + A3(TFLUX, IM, J, K));
+      A3(TFLUX, IM, J, K) = A3(TFLUX, 1, J, K);
+    }
+  }
+  orc_tstep(o, o->ROE, o->DROE, 3);
+
+  /* inlet and exit boundary conditions, mf:4476-4757 */
+  {
+    float PDIFF = 0.0f, SUMP = 0.0f;
+    if (o->IPOUT == -1) PDIFF = o->PDOWN_TIP - A1(PD, KM);
+    for (int K = 1; K <= KM; K++) {
+      float PAVG2 = 0.0f, PAVG3 = 0.0f, PAVGOUT = 0.0f, SUM = 0.0f;
+      for (int I = 1; I <= IMM1; I++) {
+        PAVG2 = PAVG2 + 0.5f * (A3(P, I, 2, K) + A3(P, I + 1, 2, K)) * A1(FP, I);
+        PAVG3 = PAVG3 + 0.5f * (A3(P, I, 3, K) + A3(P, I + 1, 3, K)) * A1(FP, I);
+        PAVGOUT = PAVGOUT + 0.5f * (A3(P, I, JMM1, K) + A3(P, I + 1, JMM1, K)) * A1(FP, I);
+        SUM = SUM + 0.5f * A1(FP, I) *
+                        (A3(ROVT, I, JM, K) * A3(VT, I, JM, K) + A3(ROVT, I + 1, JM, K) * A3(VT, I + 1, JM, K)) /
+                        AJK(R, JM, K);
+      }
+      if (o->IPOUT <= 0) {
+        A1(PD, 1) = o->PDOWN_HUB;
+        if (K != 1) {
+          float DP = (SUM + SUMP) * 0.5f * (AJK(R, JM, K) - AJK(R, JM, K - 1));
+          A1(PD, K) = 0.9f * A1(PD, K) + 0.1f * (A1(PD, K - 1) + DP);
+        }
+        SUMP = SUM;
+        if (o->IPOUT == -1) A1(PD, K) = A1(PD, K) + PDIFF;
+      }
+      for (int I = 1; I <= IM; I++) {
+        float RO_STAG = A1(PO1, K) / o->RGAS / A1(TO1, K);
+        float RO_IN = A3(RO, I, 1, K);
+        if (RO_IN > 0.9999f * RO_STAG) RO_IN = 0.9999f * RO_STAG;
+        float PNEW = A1(PO1, K) * powf(RO_IN / RO_STAG, o->GA);
+        if (o->IN_PRESS == 0) A3(P, I, 1, K) = o->RFIN1 * A3(P, I, 1, K) + o->RFIN * PNEW;
+        if (abs(o->IN_PRESS) == 1)
+          A3(P, I, 1, K) = o->RFIN1 * A3(P, I, 1, K) + o->RFIN2 * A3(P, I, 2, K) - o->RFIN * A3(P, I, 3, K);
+        if (abs(o->IN_PRESS) == 3) A3(P, I, 1, K) = o->RFIN1 * A3(P, I, 1, K) + o->RFIN2 * PAVG2 - o->RFIN * PAVG3;
+        if (abs(o->IN_PRESS) == 4)
+          A3(P, I, 1, K) = o->RFIN1 * A3(P, I, 1, K) + o->RFIN2 * A3(P, IMID, 2, KMID) - o->RFIN * A3(P, IMID, 3, KMID);
+        if (A3(P, I, 1, K) > A1(PLIM, K)) A3(P, I, 1, K) = 0.9f * A3(P, I, 1, K) + 0.1f * A1(PLIM, K);
+        if (A3(P, I, 1, K) > 0.99999f * A1(PO1, K)) A3(P, I, 1, K) = A1(PO1, K) * 0.99999f;
+        float TSTATIC = A1(TO1, K) * powf(A3(P, I, 1, K) * A1(RPO1, K), o->FGA);
+        float VABSQ = 2 * o->CP * (A1(TO1, K) - TSTATIC);
+        A3(T_STATIC, I, 1, K) = TSTATIC;
+        float VABS = sqrtf(VABSQ);
+        float VMER = 0.0f;
+        if (o->IN_VTAN == 0) {
+          VMER = VABS * A1(BSCOS, K);
+          A3(VT, I, 1, K) = VABS * A1(BSSIN, K);
+        }
+        if (o->IN_VTAN == 1) {
+          float VMERSQ = VABSQ - A1(VTIN, K) * A1(VTIN, K);
+          if (VMERSQ < 0.01f) VMERSQ = 0.01f;
+          VMER = sqrtf(VMERSQ);
+          A3(VT, I, 1, K) = A1(VTIN, K);
+        }
+        if (o->IN_VTAN == 2) {
+          float WR = AJK(UBLADE, 1, K);
+          float VRELSQ = VABSQ - WR * WR * A1(BSCOS, K) * A1(BSCOS, K);
+          if (VRELSQ < 0.01f) VRELSQ = 0.01f;
+          float VREL = sqrtf(VRELSQ) - WR * A1(BSSIN, K);
+          VMER = VREL * A1(BSCOS, K);
+          A3(VT, I, 1, K) = VREL * A1(BSSIN, K) + WR;
+        }
+        if (o->IN_VR > 0) A3(VX, I, 1, K) = VMER * A1(BRCOS, K);
+        float VXSQ = VMER * VMER - A3(VR, I, 2, K) * A3(VR, I, 2, K);
+        if (VXSQ < 0.0000001f * A1(TO1, K) * o->CP) VXSQ = 0.0000001f * A1(TO1, K) * o->CP;
+        if (o->IN_VR <= 0) A3(VX, I, 1, K) = sqrtf(VXSQ);
+        if (o->IN_VR > 0) A3(VR, I, 1, K) = VMER * A1(BRSIN, K);
+        if (o->IN_VR <= 0) A3(VR, I, 1, K) = A3(VR, I, 2, K);
+        A3(RO, I, 1, K) = A3(P, I, 1, K) / (o->RGAS * TSTATIC);
+        A3(ROVT, I, 1, K) = A3(RO, I, 1, K) * A3(VT, I, 1, K);
+        A3(ROVR, I, 1, K) = A3(RO, I, 1, K) * A3(VR, I, 1, K);
+        A3(RORVT, I, 1, K) = A3(ROVT, I, 1, K) * AJK(R, 1, K);
+        A3(ROVX, I, 1, K) = A3(RO, I, 1, K) * A3(VX, I, 1, K);
+        A3(ROE, I, 1, K) = A3(P, I, 1, K) / o->GA1 + A3(RO, I, 1, K) * 0.5f * VABSQ;
+        A3(HO, I, 1, K) = o->CP * A1(TO1, K);
+        /* exit, mf:4690-4730 */
+        float EKE = .5f * (A3(VR, I, JM, K) * A3(VR, I, JM, K) + A3(VT, I, JM, K) * A3(VT, I, JM, K) +
+                           A3(VX, I, JM, K) * A3(VX, I, JM, K));
+        A3(P, I, JM, K) = A1(PD, K) + o->FEXTRAP * (A3(P, I, JMM1, K) - PAVGOUT);
+        A3(HO, I, JM, K) = 2.0f * A3(HO, I, JMM1, K) - A3(HO, I, JM - 2, K);
+        float TEXIT = (A3(HO, I, JM, K) - EKE) / o->CP;
+        A3(RO, I, JM, K) = A3(P, I, JM, K) / o->RGAS / TEXIT;
+        A3(ROE, I, JM, K) = A3(RO, I, JM, K) * (o->CV * TEXIT + EKE);
+      }
+    }
+  }
+  for (int K = 1; K <= KM; K++)
+    for (int J = 1; J <= JM; J++) {
+      int JP1 = J + 1;
+      if (J == JM) JP1 = JM;
+      for (int I = 1; I <= IM; I++)
+        A3(PEFF, I, J, K) = A3(P, I, J, K) - o->PREFF + o->F_PDOWN * (A3(P, I, JP1, K) - A3(P, I, J, K));
+    }
+
+  /* axial momentum, mf:4818-4906 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 1; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        A3(SOURCE, I, J, K) = A3(XFORCE, I, J, K);
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J, K + 1) + A3(PEFF, I + 1, J, K + 1) + A3(PEFF, I + 1, J, K);
+        float AVGVX = A3(VX, I, J, K) + A3(VX, I, J, K + 1) + A3(VX, I + 1, J, K + 1) + A3(VX, I + 1, J, K);
+        A3(XFLUX, I, J, K) = 0.25f * (AVGP * A3(AQX, I, J, K) + A3(FLOWX, I, J, K) * AVGVX);
+      }
+  for (int K = 1; K <= KM; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I + 1, J - 1, K) + A3(PEFF, I + 1, J, K);
+        float AVGVX = A3(VX, I, J, K) + A3(VX, I, J - 1, K) + A3(VX, I + 1, J - 1, K) + A3(VX, I + 1, J, K);
+        A3(RFLUX, I, J, K) = 0.25f * (AVGP * A3(ASX, I, J, K) + A3(FLOWR, I, J, K) * AVGVX);
+      }
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IM; I++) {
+        float AVGVX = A3(VX, I, J, K) + A3(VX, I, J - 1, K) + A3(VX, I, J - 1, K + 1) + A3(VX, I, J, K + 1);
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I, J - 1, K + 1) + A3(PEFF, I, J, K + 1);
+        A3(TFLUX, I, J, K) = 0.25f * (A3(FLOWT, I, J, K) * AVGVX + A3(ABX, I, J, K) * AVGP);
+      }
+  for (int J = 2; J <= JM; J++) {
+    if (A1(IND, J) == 1) continue;
+    for (int K = 1; K <= KMM1; K++) {
+      float AVGPCUSP = A3(PEFF, 1, J, K) + A3(PEFF, 1, J - 1, K) + A3(PEFF, 1, J, K + 1) + A3(PEFF, 1, J - 1, K + 1) +
+                       A3(PEFF, IM, J, K) + A3(PEFF, IM, J - 1, K) + A3(PEFF, IM, J, K + 1) + A3(PEFF, IM, J - 1, K + 1);
+      float AVGVXCUSP = A3(VX, 1, J, K) + A3(VX, 1, J - 1, K) + A3(VX, 1, J, K + 1) + A3(VX, 1, J - 1, K + 1) +
+                        A3(VX, IM, J, K) + A3(VX, IM, J - 1, K) + A3(VX, IM, J, K + 1) + A3(VX, IM, J - 1, K + 1);
+      A3(TFLUX, 1, J, K) = 0.125f * (A3(FLOWT, 1, J, K) * AVGVXCUSP + AVGPCUSP * A3(ABX, 1, J, K));
+      A3(TFLUX, IM, J, K) = 0.125f * (A3(FLOWT, IM, J, K) * AVGVXCUSP + AVGPCUSP * A3(ABX, IM, J, K));
+    }
+  }
+  orc_tstep(o, o->ROVX, o->DROVX, 4);
+
+  /* angular momentum, mf:4913-4996 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 1; J <= JM; J++) {
+      float AVGR = 0.125f * (AJK(R, J, K) + AJK(R, J, K + 1));
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGVT = A3(VT, I, J, K) + A3(VT, I, J, K + 1) + A3(VT, I + 1, J, K) + A3(VT, I + 1, J, K + 1);
+        A3(SOURCE, I, J, K) = A3(TFORCE, I, J, K);
+        A3(XFLUX, I, J, K) = A3(FLOWX, I, J, K) * AVGR * AVGVT;
+      }
+    }
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++) {
+      float AVGR = 0.25f * AJK(RAVG_CELL, J, K);
+      for (int I = 1; I <= IM; I++) {
+        float AVGVT = A3(VT, I, J, K) + A3(VT, I, J - 1, K) + A3(VT, I, J - 1, K + 1) + A3(VT, I, J, K + 1);
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I, J - 1, K + 1) + A3(PEFF, I, J, K + 1);
+        A3(TFLUX, I, J, K) = (A3(FLOWT, I, J, K) * AVGVT + AJK(ABT, J, K) * AVGP) * AVGR;
+      }
+    }
+  for (int K = 1; K <= KM; K++)
+    for (int J = 2; J <= JM; J++) {
+      float AVGR = (AJK(R, J, K) + AJK(R, J - 1, K)) * 0.125f;
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGVT = A3(VT, I, J, K) + A3(VT, I, J - 1, K) + A3(VT, I + 1, J - 1, K) + A3(VT, I + 1, J, K);
+        A3(RFLUX, I, J, K) = A3(FLOWR, I, J, K) * AVGVT * AVGR;
+      }
+    }
+  for (int J = 2; J <= JM; J++) {
+    if (A1(IND, J) == 1) continue;
+    for (int K = 1; K <= KMM1; K++) {
+      A3(TFLUX, 1, J, K) = 0.5f * (A3(TFLUX, 1, J, K) + A3(TFLUX, IM, J, K));
+      A3(TFLUX, IM, J, K) = A3(TFLUX, 1, J, K);
+    }
+  }
+  orc_tstep(o, o->RORVT, o->DRORVT, 5);
+
+  /* radial momentum, mf:5003-5105 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 1; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGVR = A3(VR, I, J, K) + A3(VR, I, J, K + 1) + A3(VR, I + 1, J, K) + A3(VR, I + 1, J, K + 1);
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J, K + 1) + A3(PEFF, I + 1, J, K + 1) + A3(PEFF, I + 1, J, K);
+        A3(XFLUX, I, J, K) = 0.25f * (A3(FLOWX, I, J, K) * AVGVR + A3(AQR, I, J, K) * AVGP);
+      }
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IM; I++) {
+        float AVGVR = A3(VR, I, J, K) + A3(VR, I, J - 1, K) + A3(VR, I, J - 1, K + 1) + A3(VR, I, J, K + 1);
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I, J - 1, K + 1) + A3(PEFF, I, J, K + 1);
+        A3(TFLUX, I, J, K) = .25f * (A3(FLOWT, I, J, K) * AVGVR + AVGP * A3(ABR, I, J, K));
+      }
+  for (int K = 1; K <= KM; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGVR = A3(VR, I, J, K) + A3(VR, I, J - 1, K) + A3(VR, I + 1, J - 1, K) + A3(VR, I + 1, J, K);
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I + 1, J - 1, K) + A3(PEFF, I + 1, J, K);
+        A3(RFLUX, I, J, K) = 0.25f * (A3(FLOWR, I, J, K) * AVGVR + A3(ASR, I, J, K) * AVGP);
+      }
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        float AVGRVT = A3(ROVT, I, J, K) + A3(ROVT, I, J, K + 1) + A3(ROVT, I, J - 1, K) + A3(ROVT, I, J - 1, K + 1) +
+                       A3(ROVT, I + 1, J, K) + A3(ROVT, I + 1, J, K + 1) + A3(ROVT, I + 1, J - 1, K) +
+                       A3(ROVT, I + 1, J - 1, K + 1);
+        float AVGVT = 0.125f * (A3(VT, I, J, K) + A3(VT, I, J - 1, K) + A3(VT, I, J - 1, K + 1) + A3(VT, I, J, K + 1) +
+                                A3(VT, I + 1, J, K) + A3(VT, I + 1, J - 1, K) + A3(VT, I + 1, J - 1, K + 1) +
+                                A3(VT, I + 1, J, K + 1));
+        float AVGP = A3(PEFF, I, J, K) + A3(PEFF, I, J - 1, K) + A3(PEFF, I, J - 1, K + 1) + A3(PEFF, I, J, K + 1) +
+                     A3(PEFF, I + 1, J, K) + A3(PEFF, I + 1, J - 1, K) + A3(PEFF, I + 1, J - 1, K + 1) +
+                     A3(PEFF, I + 1, J, K + 1);
+        A3(SOURCE, I, J, K) = A3(RFORCE, I, J, K) - (AVGRVT * AVGVT + AVGP) * A3(VOLOR, I, J, K);
+      }
+  for (int J = 2; J <= JM; J++) {
+    if (A1(IND, J) == 1) continue;
+    for (int K = 1; K <= KMM1; K++) {
+      float AVGP1 = 0.25f * (A3(PEFF, 1, J, K) + A3(PEFF, 1, J - 1, K) + A3(PEFF, 1, J, K + 1) + A3(PEFF, 1, J - 1, K + 1));
+      float AVGPM = 0.25f * (A3(PEFF, IM, J, K) + A3(PEFF, IM, J - 1, K) + A3(PEFF, IM, J, K + 1) + A3(PEFF, IM, J - 1, K + 1));
+      float T1 = A3(TFLUX, 1, J, K) - AVGP1 * A3(ABR, 1, J, K);
+      float T2 = A3(TFLUX, IM, J, K) - AVGPM * A3(ABR, IM, J, K);
+      A3(TFLUX, 1, J, K) = 0.5f * ((T1 + T2) + (AVGP1 + AVGPM) * A3(ABR, 1, J, K));
+      A3(TFLUX, IM, J, K) = 0.5f * ((T1 + T2) + (AVGP1 + AVGPM) * A3(ABR, IM, J, K));
+    }
+  }
+  orc_tstep(o, o->ROVR, o->DROVR, 6);
+
+  /* density and Mach limiters, mf:5115-5168 */
+  for (int K = 1; K <= KM; K++)
+    for (int J = 3; J <= JM - 3; J++)
+      for (int I = 1; I <= IM; I++) {
+        float ROAVG = 0.25f * (A3(RO, I, J - 2, K) + A3(RO, I, J - 1, K) + A3(RO, I, J + 1, K) + A3(RO, I, J + 2, K));
+        if (A3(RO, I, J, K) < 0.6f * ROAVG) A3(RO, I, J, K) = 0.5f * (A3(RO, I, J, K) + 0.6f * ROAVG);
+        if (A3(RO, I, J, K) > 1.5f * ROAVG) A3(RO, I, J, K) = 0.5f * (A3(RO, I, J, K) + 1.5f * ROAVG);
+        if (A3(RO, I, J, K) < o->ROLIM) A3(RO, I, J, K) = 0.5f * (A3(RO, I, J, K) + o->ROLIM);
+      }
+  {
+    float GM1 = o->GA - 1.0f;
+    float TRATIO = 1.0f + o->MACHLIM * o->MACHLIM * 0.5f * GM1;
+    float RFGM1 = 1.0f / GM1;
+    float RORATIO = powf(TRATIO, RFGM1);
+    for (int NR = 1; NR <= o->NROWS; NR++) {
+      int JREF = A1(JLE, NR);
+      float TREF = A3(T_STATIC, IMID, JREF, KMID);
+      float WREFSQ = A3(VX, IMID, JREF, KMID) * A3(VX, IMID, JREF, KMID) +
+                     A3(VR, IMID, JREF, KMID) * A3(VR, IMID, JREF, KMID) +
+                     A3(WT, IMID, JREF, KMID) * A3(WT, IMID, JREF, KMID);
+      float TOREL_REF = TREF + 0.5f * WREFSQ / o->CP;
+      float TLIMIT = TOREL_REF / TRATIO;
+      if (TLIMIT > o->TCOOL_MIN) TLIMIT = o->TCOOL_MIN;
+      A1(VLIMIT, NR) = sqrtf(2 * o->CP * (TOREL_REF - TLIMIT));
+      float ROSTAG_REL = A3(RO, IMID, JREF, KMID) * powf(TOREL_REF / TREF, RFGM1);
+      A1(ROLIMIT, NR) = ROSTAG_REL / RORATIO;
+    }
+    for (int K = 1; K <= KM; K++)
+      for (int J = 1; J <= JM; J++) {
+        int NR = A1(NROW, J);
+        float V_LIMIT = A1(VLIMIT, NR), RO_LIMIT = A1(ROLIMIT, NR);
+        for (int I = 1; I <= IM; I++) {
+          if (A3(RO, I, J, K) < RO_LIMIT) A3(RO, I, J, K) = RO_LIMIT;
+          float RONOW = A3(RO, I, J, K);
+          float WRELSQ = A3(VX, I, J, K) * A3(VX, I, J, K) + A3(VR, I, J, K) * A3(VR, I, J, K) +
+                         A3(WT, I, J, K) * A3(WT, I, J, K);
+          float WREL = sqrtf(WRELSQ);
+          float FACSAFE = V_LIMIT / WREL;
+          if (FACSAFE < 1.0f) {
+            A3(ROVX, I, J, K) = RONOW * A3(VX, I, J, K) * FACSAFE;
+            A3(ROVR, I, J, K) = RONOW * A3(VR, I, J, K) * FACSAFE;
+            float ROVTREL = RONOW * A3(WT, I, J, K) * FACSAFE;
+            A3(ROWT, I, J, K) = ROVTREL;
+            A3(ROVT, I, J, K) = ROVTREL + RONOW * AJK(UBLADE, J, K);
+            A3(RORVT, I, J, K) = A3(ROVT, I, J, K) * AJK(R, J, K);
+          }
+        }
+      }
+  }
+  if (o->IFMIX > 0 && o->RFMIX > 1.0e-3f) orc_new_mixplan(o);
+
+  /* every 5 steps: convergence checks and the time-step update, mf:5252-5349 */
+  if (o->NSTEP % 5 == 0 || o->IFEND == 1) {
+    for (int J = 1; J <= JM; J++) {
+      float SUMAS = 0;
+      int NB = A1(NBLADE, J);
+      for (int I = 1; I <= IMM1; I++)
+        for (int K = 1; K <= KMM1; K++) SUMAS = SUMAS - A3(FLOWX, I, J, K);
+      o->BLADE_FLOW[J] = SUMAS * (float)NB;
+      o->CHECK_FLOW[J] = (SUMAS + 0.0f) * (float)NB + 0.0f - 0.0f;
+    }
+    float ECONT = 0.0f;
+    for (int J = 1; J <= JM; J++) {
+      float RATIO = o->CHECK_FLOW[J] / o->CHECK_FLOW[1];
+      float EMASS = fabsf(1.f - RATIO);
+      if (EMASS > ECONT) ECONT = EMASS;
+    }
+    o->ECONT = ECONT;
+    o->FLOWRAT = o->BLADE_FLOW[1];
+    float EMAX = 0.0f, EAVG = 0.0f;
+    float RIJKM = 2.f / (float)(IM * JM * KM);
+    float VREF = sqrtf(o->VRMS * RIJKM);
+    float RVEF = 100.f / VREF;
+    for (int K = 1; K <= KM; K++)
+      for (int J = 2; J <= JM; J++) {
+        float XD = AJK(DX, J, K), RD = AJK(DR, J, K), SD = AJK(DS, J, K);
+        for (int I = 1; I <= IM; I++) {
+          float VM_START = (XD * A3(VX, I, J, K) + RD * A3(VR, I, J, K)) / SD;
+          float VM_END = (XD * A3(ROVX, I, J, K) + RD * A3(ROVR, I, J, K)) / SD / A3(RO, I, J, K);
+          float DVMER = (VM_END - VM_START) * RVEF;
+          A3(STORE, I, J, K) = DVMER;
+          EAVG = EAVG + fabsf(DVMER);
+          if (fabsf(DVMER) > fabsf(EMAX)) EMAX = DVMER;
+        }
+      }
+    o->EAVG = EAVG / (float)(IM * JMM1 * KM);
+    o->EMAX = EMAX;
+    o->VREF = VREF;
+    if (o->IFPRINT == 1 || o->IFEND == 1) {
+      if (o->ILOS == 10) orc_loss(o);
+    }
+    orc_stepup(o, 0.25f);
+  }
+}
+
+int orc_step(orc_t *o, int n) {
+  int done = 0;
+  for (int s = 0; s < n; s++) {
+    if (o->IFEND) break;
+    step_once(o);
+    done++;
+  }
+  return done;
+}
