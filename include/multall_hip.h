@@ -1,0 +1,149 @@
+/* multall_hip.h -- C ABI of the MI355X time-marching library.
+ *
+ * Drop-in replacement for the reference solver's main iteration loop
+ * (Multall-open-18.3/multall-18.3/multall-open-18.3.f):
+ *
+ *   - mh_start + mh_step replace `CALL LOOP` (mf:137, body mf:3664-5463):
+ *     the prologue at mf:3674-3714 and the time steps from label 5000, each
+ *     step doing the reference's primitive recovery, face fluxes, the five
+ *     `TSTEP(D,DIFF,NCALL)` updates (mf:4251-5105, TSTEP mf:5470-5932 with
+ *     SMOOTH_VAR mf:15352-15594 and SMOOTH_RESID mf:14833-14917), the
+ *     inlet/exit boundary conditions (mf:4476-4757), the limiters
+ *     (mf:5115-5168), NEW_MIXPLAN (mf:14922-15251), the every-5-step
+ *     convergence checks (mf:5252-5317) and STEPUP(0.25) (mf:7916-8002);
+ *   - mh_set_array / mh_get_array stand for the COMMON blocks the loop
+ *     reads and writes (commall-open-18.3:13-203);
+ *   - mh_integrals replaces SUMFLX (mf:7873-7912) as called by EFICOOL
+ *     (mf:10741), accumulating in FP64.
+ *
+ * Conventions: arrays use the reference's names and its I-fastest,
+ * column-major order.  Dense arrays are packed to the logical extents
+ * (IM, JM, KM); mh_set_array_strided accepts the padded COMMON arrays
+ * (leading dimensions ID, JD).  Integer index maps keep the reference's
+ * 1-based values.  Every entry point returns MH_OK (0) or a negative error
+ * code; mh_last_error() gives the message.  A context is not thread-safe;
+ * it owns its device buffers and HIP stream.  No host pointer is retained
+ * after a call returns.
+ */
+#ifndef MULTALL_HIP_H
+#define MULTALL_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK 0
+#define MH_E_ARG -1      /* bad argument, unknown id, size mismatch */
+#define MH_E_HIP -2      /* a HIP runtime call failed */
+#define MH_E_UNSUPPORTED -3 /* an option the library does not implement */
+#define MH_E_STATE -4    /* call out of order (e.g. mh_step before mh_start) */
+#define MH_E_NAN -5      /* NaN/Inf detected in the convergence reductions */
+
+typedef struct mh_ctx mh_ctx;
+
+/* Problem extents (reference names).  nib1.. are the multigrid block counts
+ * NIB1, NJB1, NKB1, NIB2, NJB2, NKB2 and the superblock count NSBLK
+ * (commall-open-18.3:89-93), as set by the reference's SETUP (mf:7067-7180). */
+typedef struct mh_dims {
+  int im, jm, km, nrows;
+  int nib1, njb1, nkb1, nib2, njb2, nkb2, nsblk;
+} mh_dims;
+
+/* Convergence summary of the last every-5-step check (mf:5252-5317). */
+typedef struct mh_resid {
+  int nstep;        /* steps taken since mh_start (NSTEP) */
+  int ifend;        /* 1 once the run has ended (NSTEPS_MAX, converged) */
+  float emax, eavg, econt, vref, vmax;
+  float flow_in, flow_out; /* BLADE_FLOW(1), BLADE_FLOW(JM), kg/s */
+} mh_resid;
+
+/* Array ids.  Shapes: N = (IM,JM,KM) node/cell arrays; JK = (JM,KM);
+ * IK = (IM,KM); I, K, J = vectors; R = per row (NROWS+1); B1/B2/SB =
+ * multigrid step arrays STEP1 (NIB1,NJB1+1,NKB1), STEP2, STEPSBK. */
+enum mh_array_id {
+  /* conserved variables and SSS history (BKPRIM, BKDIFF) -- N */
+  MH_RO = 0, MH_ROVX, MH_ROVR, MH_RORVT, MH_ROE,
+  MH_DRO, MH_DROVX, MH_DROVR, MH_DRORVT, MH_DROE,
+  /* secondary variables (BKSEC, BKPRIM) -- N */
+  MH_ROVT, MH_ROWT, MH_P, MH_HO, MH_PEFF, MH_VX, MH_VR, MH_VT, MH_WT, MH_T_STATIC,
+  /* face areas, volumes, time steps (BKAREA, BKFLUX) -- N */
+  MH_AQX, MH_AQR, MH_ABX, MH_ABR, MH_ASX, MH_ASR, MH_VOL, MH_VOLOR,
+  MH_STEP, MH_BSTEP, MH_RSTEP,
+  /* body forces (BKLOSS) -- N */
+  MH_XFORCE, MH_RFORCE, MH_TFORCE, MH_QSOURCE,
+  /* face mass fluxes (BKFLOW) -- N */
+  MH_FLOWX, MH_FLOWT, MH_FLOWR,
+  /* (J,K) geometry -- JK */
+  MH_R, MH_X, MH_RAVG_CELL, MH_ABT, MH_WRABT, MH_UBLADE, MH_DX, MH_DR, MH_DS, MH_SMERID,
+  /* corner distribution weights (BKDIST) -- IK */
+  MH_FTL, MH_FBL, MH_FTR, MH_FBR,
+  /* pitchwise vectors -- I */
+  MH_FP, MH_FU, MH_FD,
+  /* spanwise vectors -- K */
+  MH_FR, MH_FKU, MH_FKD, MH_PO1, MH_TO1, MH_RPO1, MH_BSSIN, MH_BSCOS, MH_BRSIN, MH_BRCOS,
+  MH_VTIN, MH_PD, MH_VM1,
+  /* streamwise vectors -- J */
+  MH_FACUP, MH_FACDWN, MH_WRAD,
+  /* multigrid block time steps -- B1, B2, SB */
+  MH_STEP1, MH_STEP2, MH_STEPSBK,
+  /* integer maps (1-based values) */
+  MH_IB1, MH_IB2,                                   /* I */
+  MH_KB1, MH_KB2,                                   /* K */
+  MH_NBLADE, MH_IND, MH_INDLE, MH_INDMIX, MH_INDTE, MH_NROW, MH_JSBLK, MH_JB1, MH_JB2, /* J */
+  MH_JSTART, MH_JMIX, MH_JLE, MH_JTE, MH_JLED, MH_KTIPS, MH_KTIPE, MH_KS1, MH_KS2,  /* R */
+  MH_NARRAYS
+};
+
+/* Scalar ids (reference names; BKSMTH, BKCNTL, BKGAS, BKINPT, BKSTAG ...). */
+enum mh_scalar_id {
+  MH_S_CFL = 0, MH_S_DAMPIN, MH_S_MACHLIM, MH_S_F_PDOWN, MH_S_CONLIM, MH_S_SFXIN, MH_S_SFTIN,
+  MH_S_FAC_4TH, MH_S_F1, MH_S_F2, MH_S_F3, MH_S_RSMTH, MH_S_SFEXIT, MH_S_FSMTHB, MH_S_RFIN,
+  MH_S_PDOWN_HUB, MH_S_PDOWN_TIP, MH_S_FEXTRAP, MH_S_RFMIX, MH_S_FANGLE, MH_S_CP, MH_S_GA,
+  MH_S_RGAS, MH_S_CV, MH_S_FGA, MH_S_GA1, MH_S_RCP, MH_S_VSOUND, MH_S_FACMIXUP,
+  MH_S_IMID, MH_S_KMID, MH_S_NSTEPS_MAX, MH_S_NCHANGE, MH_S_NRSMTH, MH_S_NSFEXIT, MH_S_IN_PRESS,
+  MH_S_IN_VTAN, MH_S_IN_VR, MH_S_IPOUT, MH_S_IFMIX, MH_S_IR, MH_S_JR, MH_S_KR, MH_S_ILOS,
+  MH_S_NLOS, MH_S_NMIXUP, MH_S_ITIMST, MH_S_IF_RESTART, MH_S_IFGAS, MH_S_IN_FLOW,
+  MH_S_IF_REPEAT, MH_S_IRBB, MH_S_KRBB,
+  /* library controls (no reference counterpart) */
+  MH_S_EXACT_ORDER,   /* 1: reductions in the reference's sequential order */
+  MH_NSCALARS
+};
+
+/* Create a context for one machine (all rows of one deck). */
+int mh_create(const mh_dims *dims, mh_ctx **out);
+int mh_destroy(mh_ctx *ctx);
+const char *mh_last_error(const mh_ctx *ctx);
+
+/* Name lookups ("RO", "CFL" ...); return -1 if unknown. */
+int mh_array_id(const char *name);
+int mh_scalar_id(const char *name);
+/* Number of elements of an array for this context's extents. */
+long long mh_array_len(const mh_ctx *ctx, int id);
+/* 1 for int32 arrays, 0 for float arrays, -1 for a bad id. */
+int mh_array_is_int(int id);
+
+/* Copy a dense host array (float or int32 per mh_array_is_int) in/out. */
+int mh_set_array(mh_ctx *ctx, int id, const void *host, long long n);
+int mh_get_array(mh_ctx *ctx, int id, void *host, long long n);
+/* Same, from/to a padded COMMON array with leading dimensions ld_i (ID) and
+ * ld_j (JD) for N arrays, ld_j (JD) for JK arrays, ld_i (ID) for IK arrays. */
+int mh_set_array_strided(mh_ctx *ctx, int id, const void *host, int ld_i, int ld_j);
+int mh_get_array_strided(mh_ctx *ctx, int id, void *host, int ld_i, int ld_j);
+int mh_set_scalar(mh_ctx *ctx, int id, double value);
+double mh_get_scalar(const mh_ctx *ctx, int id);
+
+/* LOOP prologue (mf:3674-3714).  Call once after all inputs are set. */
+int mh_start(mh_ctx *ctx);
+/* Run up to nsteps time steps (fewer if the run ends).  *out (optional)
+ * receives the latest convergence summary. */
+int mh_step(mh_ctx *ctx, int nsteps, mh_resid *out);
+/* Per-J mass-flux-weighted sums of SUMFLX (mf:7883-7910) in FP64:
+ * out[7*JM] = {flow, sum_po, sum_to, sum_rvt, sum_entropy, sum_tstat,
+ * sum_pstat} per J, with p0/po1 and T0 from the current state. */
+int mh_integrals(mh_ctx *ctx, double *out, long long n);
+/* Wait for all queued work; returns MH_E_HIP on an asynchronous error. */
+int mh_sync(mh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,660 @@
+// C ABI and step orchestration of libmultall_hip (see include/multall_hip.h).
+//
+// mh_step mirrors the control flow of the reference's LOOP
+// (multall-open-18.3.f:3753-5454): host-side bookkeeping of the step counter,
+// the smoothing/damping ramp and the stop tests, and one stream of kernel
+// launches per step in the reference's phase order.  The every-5-step
+// convergence figures come back to the host (one small copy) because the
+// next step's stop test needs them, exactly when the reference evaluates it.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+// kernels (mh_kernels.hip)
+__global__ void k_prim(Geo, Dev, Params);
+__global__ void k_massflux(Geo, Dev);
+__global__ void k_flux_energy(Geo, Dev);
+__global__ void k_flux_axial(Geo, Dev);
+__global__ void k_flux_angular(Geo, Dev);
+__global__ void k_flux_radial(Geo, Dev);
+__global__ void k_bc(Geo, Dev, Params, StepScalars);
+__global__ void k_peff(Geo, Dev, Params, StepScalars);
+__global__ void k_fextrap(Geo, Dev, Params);
+__global__ void k_residual(Geo, Dev, float *, float *, Params);
+__global__ void k_mpavg(Geo, Dev, Params, int);
+__global__ void k_mg_sum_seq(Geo, Dev, const int *, const int *, int, int, int, int, int, float *);
+__global__ void k_mg_sum_wave(Geo, Dev, const int *, const int *, int, int, int, int, int, float *);
+__global__ void k_line_sums(Geo, Dev, double *, int);
+__global__ void k_range_sums(Geo, const double *, const int *, const int *, int, float *, int);
+__global__ void k_range_sums_seq(Geo, Dev, const int *, const int *, int, float *, int, int);
+__global__ void k_mg_apply(Geo, Dev);
+__global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int);
+__global__ void k_avg_blk(Geo, Dev);
+__global__ void k_damp_apply(Geo, Dev, float);
+__global__ void k_scatter(Geo, Dev, const float *, float *);
+__global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int);
+__global__ void k_fill_int(int *, int, int);
+__global__ void k_rolim_check(Geo, Dev, StepScalars);
+__global__ void k_rolim_fix(Geo, Dev, StepScalars);
+__global__ void k_machref(Geo, Dev, Params, StepScalars);
+__global__ void k_mach_apply(Geo, Dev);
+__global__ void k_mixplan(Geo, Dev, Params, float *);
+__global__ void k_vrms_part(Geo, Dev, double *);
+__global__ void k_vrms_final(Geo, Dev, const double *, int, int);
+__global__ void k_checkflow(Geo, Dev, int);
+__global__ void k_econt(Geo, Dev);
+__global__ void k_emax_part(Geo, Dev, double *);
+__global__ void k_emax_final(Geo, Dev, const double *, int, int);
+__global__ void k_stepup_temp(Geo, Dev, Params);
+__global__ void k_stepup_step(Geo, Dev, float);
+__global__ void k_stepup_mix(Geo, Dev);
+__global__ void k_integrals(Geo, Dev, Params);
+
+const ArrayInfo kArrays[MH_NARRAYS] = {
+    {"RO", K_N, false}, {"ROVX", K_N, false}, {"ROVR", K_N, false}, {"RORVT", K_N, false}, {"ROE", K_N, false},
+    {"DRO", K_N, false}, {"DROVX", K_N, false}, {"DROVR", K_N, false}, {"DRORVT", K_N, false}, {"DROE", K_N, false},
+    {"ROVT", K_N, false}, {"ROWT", K_N, false}, {"P", K_N, false}, {"HO", K_N, false}, {"PEFF", K_N, false},
+    {"VX", K_N, false}, {"VR", K_N, false}, {"VT", K_N, false}, {"WT", K_N, false}, {"T_STATIC", K_N, false},
+    {"AQX", K_N, false}, {"AQR", K_N, false}, {"ABX", K_N, false}, {"ABR", K_N, false}, {"ASX", K_N, false},
+    {"ASR", K_N, false}, {"VOL", K_N, false}, {"VOLOR", K_N, false}, {"STEP", K_N, false}, {"BSTEP", K_N, false},
+    {"RSTEP", K_N, false}, {"XFORCE", K_N, false}, {"RFORCE", K_N, false}, {"TFORCE", K_N, false},
+    {"QSOURCE", K_N, false}, {"FLOWX", K_N, false}, {"FLOWT", K_N, false}, {"FLOWR", K_N, false},
+    {"R", K_JK, false}, {"X", K_JK, false}, {"RAVG_CELL", K_JK, false}, {"ABT", K_JK, false},
+    {"WRABT", K_JK, false}, {"UBLADE", K_JK, false}, {"DX", K_JK, false}, {"DR", K_JK, false},
+    {"DS", K_JK, false}, {"SMERID", K_JK, false},
+    {"FTL", K_IK, false}, {"FBL", K_IK, false}, {"FTR", K_IK, false}, {"FBR", K_IK, false},
+    {"FP", K_I, false}, {"FU", K_I, false}, {"FD", K_I, false},
+    {"FR", K_K, false}, {"FKU", K_K, false}, {"FKD", K_K, false}, {"PO1", K_K, false}, {"TO1", K_K, false},
+    {"RPO1", K_K, false}, {"BSSIN", K_K, false}, {"BSCOS", K_K, false}, {"BRSIN", K_K, false},
+    {"BRCOS", K_K, false}, {"VTIN", K_K, false}, {"PD", K_K, false}, {"VM1", K_K, false},
+    {"FACUP", K_J, false}, {"FACDWN", K_J, false}, {"WRAD", K_J, false},
+    {"STEP1", K_B1, false}, {"STEP2", K_B2, false}, {"STEPSBK", K_SB, false},
+    {"IB1", K_I, true}, {"IB2", K_I, true}, {"KB1", K_K, true}, {"KB2", K_K, true},
+    {"NBLADE", K_J, true}, {"IND", K_J, true}, {"INDLE", K_J, true}, {"INDMIX", K_J, true}, {"INDTE", K_J, true},
+    {"NROW", K_J, true}, {"JSBLK", K_J, true}, {"JB1", K_J, true}, {"JB2", K_J, true},
+    {"JSTART", K_R, true}, {"JMIX", K_R, true}, {"JLE", K_R, true}, {"JTE", K_R, true}, {"JLED", K_R, true},
+    {"KTIPS", K_R, true}, {"KTIPE", K_R, true}, {"KS1", K_R, true}, {"KS2", K_R, true},
+};
+
+const char *const kScalarNames[MH_NSCALARS] = {
+    "CFL", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "SFTIN", "FAC_4TH", "F1", "F2", "F3", "RSMTH",
+    "SFEXIT", "FSMTHB", "RFIN", "PDOWN_HUB", "PDOWN_TIP", "FEXTRAP", "RFMIX", "FANGLE", "CP", "GA", "RGAS", "CV",
+    "FGA", "GA1", "RCP", "VSOUND", "FACMIXUP", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
+    "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS", "NMIXUP", "ITIMST",
+    "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "EXACT_ORDER",
+};
+
+}  // namespace mh
+
+using namespace mh;
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+
+long long kind_len(const mh_dims &d, Kind k) {
+  switch (k) {
+    case K_N: return (long long)d.im * d.jm * d.km;
+    case K_JK: return (long long)d.jm * d.km;
+    case K_IK: return (long long)d.im * d.km;
+    case K_I: return d.im;
+    case K_K: return d.km;
+    case K_J: return d.jm;
+    case K_R: return d.nrows + 1;
+    case K_B1: return (long long)d.nib1 * (d.njb1 + 1) * d.nkb1;
+    case K_B2: return (long long)d.nib2 * (d.njb2 + 1) * d.nkb2;
+    case K_SB: return d.nsblk;
+  }
+  return 0;
+}
+
+void **slot_of(Dev &d, int id) {
+  void **base[MH_NARRAYS] = {
+      (void **)&d.RO, (void **)&d.ROVX, (void **)&d.ROVR, (void **)&d.RORVT, (void **)&d.ROE, (void **)&d.DRO,
+      (void **)&d.DROVX, (void **)&d.DROVR, (void **)&d.DRORVT, (void **)&d.DROE, (void **)&d.ROVT,
+      (void **)&d.ROWT, (void **)&d.P, (void **)&d.HO, (void **)&d.PEFF, (void **)&d.VX, (void **)&d.VR,
+      (void **)&d.VT, (void **)&d.WT, (void **)&d.T_STATIC, (void **)&d.AQX, (void **)&d.AQR, (void **)&d.ABX,
+      (void **)&d.ABR, (void **)&d.ASX, (void **)&d.ASR, (void **)&d.VOL, (void **)&d.VOLOR, (void **)&d.STEP,
+      (void **)&d.BSTEP, (void **)&d.RSTEP, (void **)&d.XFORCE, (void **)&d.RFORCE, (void **)&d.TFORCE,
+      (void **)&d.QSOURCE, (void **)&d.FLOWX, (void **)&d.FLOWT, (void **)&d.FLOWR, (void **)&d.R, (void **)&d.X,
+      (void **)&d.RAVG_CELL, (void **)&d.ABT, (void **)&d.WRABT, (void **)&d.UBLADE, (void **)&d.DX,
+      (void **)&d.DR, (void **)&d.DS, (void **)&d.SMERID, (void **)&d.FTL, (void **)&d.FBL, (void **)&d.FTR,
+      (void **)&d.FBR, (void **)&d.FP, (void **)&d.FU, (void **)&d.FD, (void **)&d.FR, (void **)&d.FKU,
+      (void **)&d.FKD, (void **)&d.PO1, (void **)&d.TO1, (void **)&d.RPO1, (void **)&d.BSSIN, (void **)&d.BSCOS,
+      (void **)&d.BRSIN, (void **)&d.BRCOS, (void **)&d.VTIN, (void **)&d.PD, (void **)&d.VM1, (void **)&d.FACUP,
+      (void **)&d.FACDWN, (void **)&d.WRAD, (void **)&d.STEP1, (void **)&d.STEP2, (void **)&d.STEPSBK,
+      (void **)&d.IB1, (void **)&d.IB2, (void **)&d.KB1, (void **)&d.KB2, (void **)&d.NBLADE, (void **)&d.IND,
+      (void **)&d.INDLE, (void **)&d.INDMIX, (void **)&d.INDTE, (void **)&d.NROW, (void **)&d.JSBLK,
+      (void **)&d.JB1, (void **)&d.JB2, (void **)&d.JSTART, (void **)&d.JMIX, (void **)&d.JLE, (void **)&d.JTE,
+      (void **)&d.JLED, (void **)&d.KTIPS, (void **)&d.KTIPE, (void **)&d.KS1, (void **)&d.KS2,
+  };
+  return base[id];
+}
+
+int fail(mh_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) return fail(c, MH_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(mh_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, n * sizeof(T) + 256));
+  c->owned.push_back(q);
+  HIPCHK(hipMemsetAsync(q, 0, n * sizeof(T) + 256, c->stream));
+  *p = (T *)q;
+  return MH_OK;
+}
+
+float sf(const mh_ctx *c, int id) { return (float)c->scal[id]; }
+int si(const mh_ctx *c, int id) { return (int)c->scal[id]; }
+
+Params params_of(const mh_ctx *c) {
+  Params p{};
+  p.CFL = sf(c, MH_S_CFL); p.DAMPIN = sf(c, MH_S_DAMPIN); p.MACHLIM = sf(c, MH_S_MACHLIM);
+  p.F_PDOWN = sf(c, MH_S_F_PDOWN); p.CONLIM = sf(c, MH_S_CONLIM); p.SFXIN = c->SFXIN_s; p.SFTIN = c->SFTIN_s;
+  p.FAC_4TH = sf(c, MH_S_FAC_4TH); p.F1 = sf(c, MH_S_F1); p.F2 = sf(c, MH_S_F2); p.F3 = sf(c, MH_S_F3);
+  p.RSMTH = sf(c, MH_S_RSMTH); p.SFEXIT = sf(c, MH_S_SFEXIT); p.FSMTHB = sf(c, MH_S_FSMTHB);
+  p.RFIN = sf(c, MH_S_RFIN); p.PDOWN_HUB = sf(c, MH_S_PDOWN_HUB); p.PDOWN_TIP = sf(c, MH_S_PDOWN_TIP);
+  p.FEXTRAP = sf(c, MH_S_FEXTRAP); p.RFMIX = sf(c, MH_S_RFMIX); p.FANGLE = sf(c, MH_S_FANGLE);
+  p.CP = sf(c, MH_S_CP); p.GA = sf(c, MH_S_GA); p.RGAS = sf(c, MH_S_RGAS); p.CV = sf(c, MH_S_CV);
+  p.FGA = sf(c, MH_S_FGA); p.GA1 = sf(c, MH_S_GA1); p.RCP = sf(c, MH_S_RCP); p.VSOUND = sf(c, MH_S_VSOUND);
+  p.FACMIXUP = sf(c, MH_S_FACMIXUP);
+  p.IMID = si(c, MH_S_IMID); p.KMID = si(c, MH_S_KMID); p.NSTEPS_MAX = si(c, MH_S_NSTEPS_MAX);
+  p.NCHANGE = si(c, MH_S_NCHANGE); p.NRSMTH = si(c, MH_S_NRSMTH); p.NSFEXIT = si(c, MH_S_NSFEXIT);
+  p.IN_PRESS = si(c, MH_S_IN_PRESS); p.IN_VTAN = si(c, MH_S_IN_VTAN); p.IN_VR = si(c, MH_S_IN_VR);
+  p.IPOUT = si(c, MH_S_IPOUT); p.IFMIX = si(c, MH_S_IFMIX); p.IR = si(c, MH_S_IR); p.JR = si(c, MH_S_JR);
+  p.KR = si(c, MH_S_KR); p.ILOS = si(c, MH_S_ILOS); p.NLOS = si(c, MH_S_NLOS); p.NMIXUP = si(c, MH_S_NMIXUP);
+  p.ITIMST = si(c, MH_S_ITIMST); p.IF_RESTART = si(c, MH_S_IF_RESTART); p.IFGAS = si(c, MH_S_IFGAS);
+  p.IN_FLOW = si(c, MH_S_IN_FLOW); p.IF_REPEAT = si(c, MH_S_IF_REPEAT); p.IRBB = si(c, MH_S_IRBB);
+  p.KRBB = si(c, MH_S_KRBB); p.EXACT = si(c, MH_S_EXACT_ORDER);
+  return p;
+}
+
+}  // namespace
+
+namespace {
+
+dim3 node_grid(const Geo &g) { return dim3((g.IM + 63) / 64, (g.JM + 3) / 4, g.KM); }
+const dim3 kNodeBlock(64, 4, 1);
+
+int last_launch(mh_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, MH_E_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e));
+  return MH_OK;
+}
+#define LCHK(what)                          \
+  do {                                      \
+    int r_ = last_launch(c, what);          \
+    if (r_ != MH_OK) return r_;             \
+  } while (0)
+
+StepScalars step_scalars(const mh_ctx *c) {
+  StepScalars s{};
+  s.SFT = c->SFT; s.SFX = c->SFX; s.DAMP = c->DAMP; s.FMIXUP = c->FMIXUP; s.PREFF = c->PREFF;
+  s.ROLIM = c->ROLIM; s.TCOOL_MIN = 1.0e6f;
+  s.SFEX1 = 1.f - sf(c, MH_S_SFEXIT); s.SFEXH = 0.5f * sf(c, MH_S_SFEXIT);
+  s.RFIN1 = 1.0f - sf(c, MH_S_RFIN); s.RFIN2 = 2.f * sf(c, MH_S_RFIN);
+  s.NSTEP = c->nstep; s.IFEND = c->ifend;
+  return s;
+}
+
+// Build the CSR J-lists on the host from the integer maps.
+int build_lists(mh_ctx *c) {
+  const mh_dims &dm = c->dims;
+  std::vector<int> jb1(dm.jm), jb2(dm.jm), jsb(dm.jm), jstart(dm.nrows + 1), jmix(dm.nrows + 1);
+  HIPCHK(hipMemcpy(jb1.data(), c->d.JB1, dm.jm * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(jb2.data(), c->d.JB2, dm.jm * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(jsb.data(), c->d.JSBLK, dm.jm * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(jstart.data(), c->d.JSTART, (dm.nrows + 1) * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(jmix.data(), c->d.JMIX, (dm.nrows + 1) * 4, hipMemcpyDeviceToHost));
+  auto csr = [&](const std::vector<int> &map, int nb, std::vector<int> &ptr, std::vector<int> &list) -> bool {
+    ptr.assign(nb + 1, 0);
+    list.clear();
+    for (int b = 1; b <= nb; b++) {
+      ptr[b - 1] = (int)list.size();
+      for (int J = 2; J <= dm.jm; J++)
+        if (map[J - 1] == b) list.push_back(J);
+    }
+    ptr[nb] = (int)list.size();
+    for (int J = 2; J <= dm.jm; J++)
+      if (map[J - 1] < 1 || map[J - 1] > nb) return false;
+    return true;
+  };
+  std::vector<int> p1, l1, p2, l2, ps, ls, pr, lr;
+  if (!csr(jb1, dm.njb1 + 1, p1, l1)) return fail(c, MH_E_ARG, "JB1 out of range");
+  if (!csr(jb2, dm.njb2 + 1, p2, l2)) return fail(c, MH_E_ARG, "JB2 out of range");
+  if (!csr(jsb, dm.nsblk, ps, ls)) return fail(c, MH_E_ARG, "JSBLK out of range");
+  pr.assign(dm.nrows + 1, 0);
+  for (int nr = 1; nr <= dm.nrows; nr++) {
+    pr[nr - 1] = (int)lr.size();
+    for (int J = jstart[nr - 1] + 1; J <= jmix[nr - 1] - 1; J++) lr.push_back(J);
+    if (jmix[nr - 1] - jstart[nr - 1] < 9) return fail(c, MH_E_UNSUPPORTED, "rows need at least 10 J points");
+  }
+  pr[dm.nrows] = (int)lr.size();
+  auto up = [&](std::vector<int> &v, int **dst) -> int {
+    int r = dalloc(c, dst, v.size() + 1);
+    if (r) return r;
+    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+    return MH_OK;
+  };
+  Lists &L = c->lists;
+  int r;
+  if ((r = up(p1, &L.b1_ptr)) || (r = up(l1, &L.b1_list)) || (r = up(p2, &L.b2_ptr)) || (r = up(l2, &L.b2_list)) ||
+      (r = up(ps, &L.sb_ptr)) || (r = up(ls, &L.sb_list)) || (r = up(pr, &L.row_ptr)) || (r = up(lr, &L.row_list)))
+    return r;
+  return MH_OK;
+}
+
+// TSTEP (mf:5470-5932) for one conserved variable.
+int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
+  const Geo &g = c->g;
+  Dev d = c->d;
+  const Params p = params_of(c);
+  const StepScalars s = step_scalars(c);
+  const dim3 grid = node_grid(g);
+  const int nmp = (g.NROWS - 1) * g.KMM1;
+  if (!(p.FEXTRAP < 0.001f) && nmp > 0) {
+    hipLaunchKernelGGL(k_fextrap, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p);
+    LCHK("fextrap");
+  }
+  hipLaunchKernelGGL(k_residual, grid, kNodeBlock, 0, c->stream, g, d, D, DIFF, p);
+  LCHK("residual");
+  if (nmp > 0) {
+    hipLaunchKernelGGL(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 1);
+    LCHK("mpavg1");
+  }
+  const bool mg = !(p.IR <= 1 && p.JR <= 1 && p.KR <= 1);
+  const Lists &L = c->lists;
+  if (mg) {
+    const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
+    hipLaunchKernelGGL(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list,
+                       g.NIB1, g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
+    LCHK("mg1");
+    if (p.EXACT) {
+      hipLaunchKernelGGL(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list,
+                         g.NIB2, g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
+      LCHK("mg2s");
+      hipLaunchKernelGGL(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr,
+                         L.sb_list, g.NSBLK, d.SBCHG, 0, 0);
+      LCHK("sbs");
+    } else {
+      hipLaunchKernelGGL(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
+                         g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
+      LCHK("mg2w");
+      hipLaunchKernelGGL(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 0);
+      LCHK("lines");
+      hipLaunchKernelGGL(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
+                         L.sb_list, g.NSBLK, d.SBCHG, 0);
+      LCHK("sbw");
+    }
+  }
+  hipLaunchKernelGGL(k_mg_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("mgapply");
+  if (p.NRSMTH > 0) {
+    for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
+      hipLaunchKernelGGL(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0);
+      hipLaunchKernelGGL(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.TMP, d.STORE, p.RSMTH, 1);
+      hipLaunchKernelGGL(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 2);
+      std::swap(c->d.STORE, c->d.TMP);
+      d = c->d;
+    }
+    LCHK("smooth_resid");
+  }
+  if (!(s.DAMP < 2.0f || s.DAMP > 100)) {
+    if (p.EXACT) {
+      hipLaunchKernelGGL(k_range_sums_seq, dim3(1), dim3(64), 0, c->stream, g, d, L.row_ptr, L.row_list, g.NROWS,
+                         d.SC + SC_AVG_CHG, 1, 1);
+    } else {
+      hipLaunchKernelGGL(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 1);
+      hipLaunchKernelGGL(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
+                         L.row_list, g.NROWS, d.SC + SC_AVG_CHG, 1);
+    }
+    hipLaunchKernelGGL(k_avg_blk, dim3(1), dim3(64), 0, c->stream, g, d);
+    hipLaunchKernelGGL(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
+    LCHK("damp");
+  }
+  if (nmp > 0) {
+    hipLaunchKernelGGL(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 2);
+    LCHK("mpavg2");
+  }
+  hipLaunchKernelGGL(k_scatter, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.DSC);
+  LCHK("scatter");
+  hipLaunchKernelGGL(k_smooth_plane, dim3(g.JM), dim3(256), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
+                     s, ncall);
+  LCHK("smooth_plane");
+  return MH_OK;
+}
+
+int stepup(mh_ctx *c, float relax) {
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const Params p = params_of(c);
+  hipLaunchKernelGGL(k_stepup_temp, node_grid(g), kNodeBlock, 0, c->stream, g, d, p);
+  hipLaunchKernelGGL(k_stepup_step, node_grid(g), kNodeBlock, 0, c->stream, g, d, relax);
+  const int nmix = (g.NROWS - 1) * g.IMM1 * g.KMM1;
+  if (nmix > 0) hipLaunchKernelGGL(k_stepup_mix, dim3((nmix + 255) / 256), dim3(256), 0, c->stream, g, d);
+  LCHK("stepup");
+  return MH_OK;
+}
+
+// One time step: the body of LOOP from label 5000 (mf:3753-5454).
+int step_once(mh_ctx *c) {
+  const Geo &g = c->g;
+  const Params p = params_of(c);
+  const dim3 grid = node_grid(g);
+  int r;
+  // bookkeeping, stop tests and the smoothing/damping ramp, mf:3760-3824
+  c->nstep = c->nstep + 1;
+  if (c->nstep == p.NSTEPS_MAX) c->ifend = 1;
+  if (c->EAVG < p.CONLIM && c->ECONT < 0.01f) c->ifend = 1;
+  if (fabsf(c->EMAX) < (4 * p.CONLIM) && c->ECONT < 0.01f) c->ifend = 1;
+  float FCHANGE = 1.0f - (float)(c->nstep - 1) / (float)p.NCHANGE;
+  if (FCHANGE > 0.0f) {
+    c->SFT = c->SFTIN_s + 0.02f * FCHANGE;
+    c->SFX = c->SFXIN_s + 0.02f * FCHANGE;
+    c->DAMP = p.DAMPIN * (1.f - 0.75f * FCHANGE);
+  }
+  float FMIXUP = 1.0f + (p.FACMIXUP - 1.f) * (float)(p.NMIXUP - c->nstep) / (float)p.NMIXUP;
+  if (FMIXUP < 1.0f) FMIXUP = 1.0f;
+  c->FMIXUP = FMIXUP * FMIXUP;
+  const bool check5 = (c->nstep % 5 == 0) || c->ifend == 1;
+  Dev d = c->d;
+  const StepScalars s = step_scalars(c);
+
+  hipLaunchKernelGGL(k_prim, grid, kNodeBlock, 0, c->stream, g, d, p);
+  LCHK("prim");
+  if (check5) {
+    hipLaunchKernelGGL(k_vrms_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
+    hipLaunchKernelGGL(k_vrms_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LCHK("vrms");
+  }
+  hipLaunchKernelGGL(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("massflux");
+  if ((r = tstep(c, d.RO, d.DRO, 1))) return r;
+  hipLaunchKernelGGL(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("flux_energy");
+  if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
+  hipLaunchKernelGGL(k_bc, dim3(1), dim3(256), 4 * g.KM * sizeof(float), c->stream, g, d, p, s);
+  LCHK("bc");
+  hipLaunchKernelGGL(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
+  hipLaunchKernelGGL(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("flux_axial");
+  if ((r = tstep(c, d.ROVX, d.DROVX, 4))) return r;
+  hipLaunchKernelGGL(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("flux_angular");
+  if ((r = tstep(c, d.RORVT, d.DRORVT, 5))) return r;
+  hipLaunchKernelGGL(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("flux_radial");
+  if ((r = tstep(c, d.ROVR, d.DROVR, 6))) return r;
+  // limiters, mf:5115-5168
+  d = c->d;
+  const int nline = g.IM * g.KM;
+  hipLaunchKernelGGL(k_fill_int, dim3((nline + 255) / 256), dim3(256), 0, c->stream, d.FLAGS, nline, 0);
+  hipLaunchKernelGGL(k_rolim_check, grid, kNodeBlock, 0, c->stream, g, d, s);
+  hipLaunchKernelGGL(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s);
+  hipLaunchKernelGGL(k_machref, dim3(1), dim3(64), 0, c->stream, g, d, p, s);
+  hipLaunchKernelGGL(k_mach_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  LCHK("limiters");
+  if (g.IM > 2 && p.IFMIX > 0 && p.RFMIX > 1.0e-3f && g.NROWS > 1) {
+    const int nl = (g.NROWS - 1) * g.KM;
+    hipLaunchKernelGGL(k_mixplan, dim3((nl + 63) / 64), dim3(64), 0, c->stream, g, d, p, c->lists.mixscratch);
+    LCHK("mixplan");
+  }
+  if (check5) {
+    hipLaunchKernelGGL(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, d, p.EXACT);
+    hipLaunchKernelGGL(k_econt, dim3(1), dim3(64), 0, c->stream, g, d);
+    hipLaunchKernelGGL(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
+    hipLaunchKernelGGL(k_emax_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LCHK("convergence");
+    if ((r = stepup(c, 0.25f))) return r;
+    float sc[8];
+    HIPCHK(hipMemcpyAsync(sc, d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->EMAX = sc[SC_EMAX];
+    c->EAVG = sc[SC_EAVG];
+    c->ECONT = sc[SC_ECONT];
+    c->last.emax = sc[SC_EMAX];
+    c->last.eavg = sc[SC_EAVG];
+    c->last.econt = sc[SC_ECONT];
+    c->last.vref = sc[SC_VREF];
+    c->last.vmax = sc[SC_VMAX];
+    c->last.flow_in = sc[SC_FLOWRAT];
+    c->last.flow_out = sc[SC_FLOWOUT];
+    if (!std::isfinite(sc[SC_VMAX]) || !std::isfinite(sc[SC_FLOWRAT]))
+      return fail(c, MH_E_NAN, "non-finite flow field at step " + std::to_string(c->nstep));
+  }
+  c->last.nstep = c->nstep;
+  c->last.ifend = c->ifend;
+  return MH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int mh_array_id(const char *name) {
+  if (!name) return -1;
+  for (int i = 0; i < MH_NARRAYS; i++)
+    if (!strcmp(name, kArrays[i].name)) return i;
+  return -1;
+}
+
+int mh_scalar_id(const char *name) {
+  if (!name) return -1;
+  for (int i = 0; i < MH_NSCALARS; i++)
+    if (!strcmp(name, kScalarNames[i])) return i;
+  return -1;
+}
+
+int mh_array_is_int(int id) { return (id < 0 || id >= MH_NARRAYS) ? -1 : (kArrays[id].is_int ? 1 : 0); }
+
+long long mh_array_len(const mh_ctx *c, int id) {
+  if (!c || id < 0 || id >= MH_NARRAYS) return -1;
+  return kind_len(c->dims, kArrays[id].kind);
+}
+
+const char *mh_last_error(const mh_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mh_create(const mh_dims *dims, mh_ctx **out) {
+  if (!dims || !out) return MH_E_ARG;
+  *out = nullptr;
+  const mh_dims &dm = *dims;
+  if (dm.im < 4 || dm.km < 4 || dm.jm < 10 || dm.nrows < 1 || dm.nrows > 64 || dm.nib1 < 1 || dm.njb1 < 1 ||
+      dm.nkb1 < 1 || dm.nib2 < 1 || dm.njb2 < 1 || dm.nkb2 < 1 || dm.nsblk < 1)
+    return MH_E_ARG;
+  mh_ctx *c = new mh_ctx();
+  c->dims = dm;
+  Geo &g = c->g;
+  g.IM = dm.im; g.JM = dm.jm; g.KM = dm.km; g.IMM1 = dm.im - 1; g.JMM1 = dm.jm - 1; g.KMM1 = dm.km - 1;
+  g.NROWS = dm.nrows; g.NIB1 = dm.nib1; g.NJB1 = dm.njb1; g.NKB1 = dm.nkb1; g.NIB2 = dm.nib2; g.NJB2 = dm.njb2;
+  g.NKB2 = dm.nkb2; g.NSBLK = dm.nsblk;
+  auto bail = [&](int r) { mh_destroy(c); return r; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(MH_E_HIP);
+  int r;
+  for (int id = 0; id < MH_NARRAYS; id++) {
+    const long long n = kind_len(dm, kArrays[id].kind);
+    void **slot = slot_of(c->d, id);
+    if (kArrays[id].is_int) r = dalloc(c, (int **)slot, (size_t)n);
+    else r = dalloc(c, (float **)slot, (size_t)n);
+    if (r) return bail(r);
+  }
+  const size_t N = (size_t)dm.im * dm.jm * dm.km;
+  Dev &d = c->d;
+  if ((r = dalloc(c, &d.XFLUX, N)) || (r = dalloc(c, &d.TFLUX, N)) || (r = dalloc(c, &d.RFLUX, N)) ||
+      (r = dalloc(c, &d.SOURCE, N)) || (r = dalloc(c, &d.STORE, N)) || (r = dalloc(c, &d.DSC, N)) ||
+      (r = dalloc(c, &d.TMP, N)) || (r = dalloc(c, &d.TEMP1, N)) ||
+      (r = dalloc(c, &d.B1CHG, kind_len(dm, K_B1))) || (r = dalloc(c, &d.B2CHG, kind_len(dm, K_B2))) ||
+      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) ||
+      (r = dalloc(c, &d.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))) ||
+      (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
+      (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
+      (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) ||
+      (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
+    return bail(r);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
+  c->lds_plane = (size_t)3 * dm.im * dm.km * sizeof(float);
+  if (c->lds_plane > 160 * 1024) return bail(MH_E_UNSUPPORTED);
+  if (hipFuncSetAttribute((const void *)k_smooth_plane, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)c->lds_plane) != hipSuccess)
+    return bail(MH_E_HIP);
+  *out = c;
+  return MH_OK;
+}
+
+int mh_destroy(mh_ctx *c) {
+  if (!c) return MH_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->owned) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MH_OK;
+}
+
+static int copy_array(mh_ctx *c, int id, void *host, long long n, int ld_i, int ld_j, bool to_dev) {
+  if (!c || id < 0 || id >= MH_NARRAYS || !host) return fail(c, MH_E_ARG, "bad argument");
+  const mh_dims &dm = c->dims;
+  const Kind k = kArrays[id].kind;
+  const long long len = kind_len(dm, k);
+  char *dev = (char *)*slot_of(c->d, id);
+  const hipMemcpyKind dir = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (ld_i <= 0 && ld_j <= 0) {  // dense
+    if (n != len) return fail(c, MH_E_ARG, std::string("size mismatch for ") + kArrays[id].name);
+    if (to_dev) HIPCHK(hipMemcpy(dev, host, (size_t)len * 4, dir));
+    else HIPCHK(hipMemcpy(host, dev, (size_t)len * 4, dir));
+    return MH_OK;
+  }
+  // padded FORTRAN array: rows of the first logical extent at pitch ld_i
+  size_t w = 0, h = 0, planes = 1, spitch = 0, splane = 0;
+  switch (k) {
+    case K_N: w = dm.im; h = dm.jm; planes = dm.km; spitch = ld_i; splane = (size_t)ld_i * ld_j; break;
+    case K_B1: w = dm.nib1; h = dm.njb1 + 1; planes = dm.nkb1; spitch = ld_i; splane = (size_t)ld_i * ld_j; break;
+    case K_B2: w = dm.nib2; h = dm.njb2 + 1; planes = dm.nkb2; spitch = ld_i; splane = (size_t)ld_i * ld_j; break;
+    case K_JK: w = dm.jm; h = dm.km; spitch = ld_j; break;
+    case K_IK: w = dm.im; h = dm.km; spitch = ld_i; break;
+    default: w = (size_t)len; h = 1; spitch = w; break;
+  }
+  if (spitch < w || (planes > 1 && (size_t)ld_j < h)) return fail(c, MH_E_ARG, "leading dimensions too small");
+  for (size_t pz = 0; pz < planes; pz++) {
+    char *dptr = dev + pz * w * h * 4;
+    char *hptr = (char *)host + pz * splane * 4;
+    if (to_dev) HIPCHK(hipMemcpy2D(dptr, w * 4, hptr, spitch * 4, w * 4, h, dir));
+    else HIPCHK(hipMemcpy2D(hptr, spitch * 4, dptr, w * 4, w * 4, h, dir));
+  }
+  return MH_OK;
+}
+
+int mh_set_array(mh_ctx *c, int id, const void *host, long long n) {
+  return copy_array(c, id, const_cast<void *>(host), n, 0, 0, true);
+}
+int mh_get_array(mh_ctx *c, int id, void *host, long long n) { return copy_array(c, id, host, n, 0, 0, false); }
+int mh_set_array_strided(mh_ctx *c, int id, const void *host, int ld_i, int ld_j) {
+  return copy_array(c, id, const_cast<void *>(host), 0, ld_i > 0 ? ld_i : 1, ld_j > 0 ? ld_j : 1, true);
+}
+int mh_get_array_strided(mh_ctx *c, int id, void *host, int ld_i, int ld_j) {
+  return copy_array(c, id, host, 0, ld_i > 0 ? ld_i : 1, ld_j > 0 ? ld_j : 1, false);
+}
+
+int mh_set_scalar(mh_ctx *c, int id, double v) {
+  if (!c || id < 0 || id >= MH_NSCALARS) return fail(c, MH_E_ARG, "bad scalar id");
+  c->scal[id] = v;
+  c->scal_set[id] = true;
+  return MH_OK;
+}
+
+double mh_get_scalar(const mh_ctx *c, int id) {
+  if (!c || id < 0 || id >= MH_NSCALARS) return NAN;
+  return c->scal[id];
+}
+
+int mh_start(mh_ctx *c) {
+  if (!c) return MH_E_ARG;
+  for (int i = 0; i < MH_NSCALARS; i++)
+    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER)
+      return fail(c, MH_E_STATE, std::string("scalar not set: ") + kScalarNames[i]);
+  const Params p = params_of(c);
+  if (p.IFGAS != 0) return fail(c, MH_E_UNSUPPORTED, "real-gas properties (IFGAS=1)");
+  if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
+  if (p.IN_FLOW != 0 || p.IF_REPEAT != 0 || p.IF_RESTART != 0)
+    return fail(c, MH_E_UNSUPPORTED, "IN_FLOW, IF_REPEAT or IF_RESTART set");
+  if (p.ILOS != 0) return fail(c, MH_E_UNSUPPORTED, "viscous models (ILOS != 0)");
+  if (p.IMID < 1 || p.IMID > c->g.IM || p.KMID < 1 || p.KMID > c->g.KM) return fail(c, MH_E_ARG, "IMID/KMID");
+  c->g.IMID = p.IMID;
+  c->g.KMID = p.KMID;
+  int r = build_lists(c);
+  if (r) return r;
+  // LOOP prologue, mf:3674-3714
+  const Geo &g = c->g;
+  std::vector<float> po1(g.KM), p1(g.KM);
+  float pa = 0, pb = 0, ra = 0, rb = 0;
+  const size_t nA = (size_t)(p.IMID - 1) + (size_t)g.IM * (size_t)g.JM * (p.KMID - 1);
+  const size_t nB = nA + (size_t)g.IM * (g.JM - 1);
+  HIPCHK(hipMemcpy(po1.data(), c->d.PO1, g.KM * 4, hipMemcpyDeviceToHost));
+  for (int K = 1; K <= g.KM; K++)
+    HIPCHK(hipMemcpy(&p1[K - 1], c->d.P + (size_t)g.IM * g.JM * (K - 1), 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&pa, c->d.P + nA, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&pb, c->d.P + nB, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&ra, c->d.RO + nA, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&rb, c->d.RO + nB, 4, hipMemcpyDeviceToHost));
+  c->ROMIN = fminf(ra, rb);
+  c->ROLIM = 0.25f * c->ROMIN;
+  std::vector<float> plim(g.KM);
+  for (int K = 1; K <= g.KM; K++) plim[K - 1] = 0.9f * po1[K - 1] + 0.1f * p1[K - 1];
+  HIPCHK(hipMemcpy(c->d.SC + SC_PLIM, plim.data(), g.KM * 4, hipMemcpyHostToDevice));
+  c->PREFF = 0.5f * (pa + pb);
+  c->SFXIN_s = sf(c, MH_S_SFXIN) * sf(c, MH_S_CFL) / 0.5f;
+  c->SFTIN_s = sf(c, MH_S_SFTIN) * sf(c, MH_S_CFL) / 0.5f;
+  c->ECONT = 1.f;
+  c->EAVG = 1.0e06f;
+  c->EMAX = 1.0e06f;
+  c->ifend = 0;
+  c->nstep = 0;
+  c->last = mh_resid{};
+  c->started = true;
+  return MH_OK;
+}
+
+int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
+  if (!c) return MH_E_ARG;
+  if (!c->started) return fail(c, MH_E_STATE, "mh_step before mh_start");
+  for (int i = 0; i < nsteps && !c->ifend; i++) {
+    int r = step_once(c);
+    if (r) return r;
+  }
+  if (out) *out = c->last;
+  return MH_OK;
+}
+
+int mh_integrals(mh_ctx *c, double *out, long long n) {
+  if (!c || !out || n != 7LL * c->g.JM) return fail(c, MH_E_ARG, "integrals: need 7*JM doubles");
+  const Params p = params_of(c);
+  hipLaunchKernelGGL(k_integrals, dim3(c->g.JM), dim3(256), 0, c->stream, c->g, c->d, p);
+  LCHK("integrals");
+  HIPCHK(hipMemcpyAsync(out, c->d.INTEG, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MH_OK;
+}
+
+int mh_sync(mh_ctx *c) {
+  if (!c) return MH_E_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MH_OK;
+}
+
+}  // extern "C"

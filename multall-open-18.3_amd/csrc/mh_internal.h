@@ -1,0 +1,118 @@
+// Internal definitions shared by the HIP translation units of libmultall_hip.
+//
+// Layout in HBM: every node/cell array is one dense FP32 buffer [KM][JM][IM]
+// (I fastest, so a pitchwise line of IM floats is one coalesced segment);
+// (J,K) arrays are [KM][JM]; (I,K) arrays are [KM][IM].  Cell (I,J,K) spans
+// nodes I..I+1, J-1..J, K..K+1 exactly as in the reference, and cell arrays
+// share the node arrays' shape (the unused I=IM, K=KM, J=1 slots are kept so
+// that indices match the reference one-for-one).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "multall_hip.h"
+
+namespace mh {
+
+enum Kind : int { K_N, K_JK, K_IK, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB };
+
+struct ArrayInfo {
+  const char *name;
+  Kind kind;
+  bool is_int;
+};
+
+extern const ArrayInfo kArrays[MH_NARRAYS];
+extern const char *const kScalarNames[MH_NSCALARS];
+
+// Extents and derived counts passed by value to every kernel.
+struct Geo {
+  int IM, JM, KM, IMM1, JMM1, KMM1, IMID, KMID, NROWS;
+  int NIB1, NJB1, NKB1, NIB2, NJB2, NKB2, NSBLK;
+};
+
+// Device pointers of every array, by reference name, plus scratch buffers.
+struct Dev {
+  float *RO, *ROVX, *ROVR, *RORVT, *ROE, *DRO, *DROVX, *DROVR, *DRORVT, *DROE;
+  float *ROVT, *ROWT, *P, *HO, *PEFF, *VX, *VR, *VT, *WT, *T_STATIC;
+  float *AQX, *AQR, *ABX, *ABR, *ASX, *ASR, *VOL, *VOLOR, *STEP, *BSTEP, *RSTEP;
+  float *XFORCE, *RFORCE, *TFORCE, *QSOURCE, *FLOWX, *FLOWT, *FLOWR;
+  float *R, *X, *RAVG_CELL, *ABT, *WRABT, *UBLADE, *DX, *DR, *DS, *SMERID;
+  float *FTL, *FBL, *FTR, *FBR;
+  float *FP, *FU, *FD;
+  float *FR, *FKU, *FKD, *PO1, *TO1, *RPO1, *BSSIN, *BSCOS, *BRSIN, *BRCOS, *VTIN, *PD, *VM1;
+  float *FACUP, *FACDWN, *WRAD;
+  float *STEP1, *STEP2, *STEPSBK;
+  int *IB1, *IB2, *KB1, *KB2, *NBLADE, *IND, *INDLE, *INDMIX, *INDTE, *NROW, *JSBLK, *JB1, *JB2;
+  int *JSTART, *JMIX, *JLE, *JTE, *JLED, *KTIPS, *KTIPE, *KS1, *KS2;
+  // scratch (not part of the public array set)
+  float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
+  float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
+  double *RED;                       // FP64 partial sums of the reductions
+  float *SC;                         // device-side scalars (see SC_* below)
+  int *FLAGS;                        // per (I,K) line flags for the density limiter
+  float *CHECK_FLOW, *BLADE_FLOW;    // per J
+  double *INTEG;                     // 7*JM FP64 integrals
+};
+
+// Slots of Dev::SC, the step's device-resident scalars.
+enum {
+  SC_VRMS = 0, SC_VMAX, SC_EMAX, SC_EAVG, SC_ECONT, SC_VREF, SC_FLOWRAT, SC_FLOWOUT,
+  SC_AVG_CHG = 16,          // [NROWS]
+  SC_AVG_BLK = 16 + 64,     // [NROWS]
+  SC_VLIMIT = 16 + 128,     // [NROWS]
+  SC_ROLIMIT = 16 + 192,    // [NROWS]
+  SC_PLIM = 16 + 256,       // [KM], PLIM(K) fixed at the start of LOOP (mf:3684-3686)
+  SC_COUNT = 16 + 256       // + KM
+};
+
+// Per-step host scalars handed to the kernels.
+struct StepScalars {
+  float SFT, SFX, DAMP, FMIXUP, PREFF, ROLIM, TCOOL_MIN, SFEX1, SFEXH, RFIN1, RFIN2;
+  int NSTEP, IFEND;
+};
+
+// Copy of the scalar table used by the kernels (floats and ints).
+struct Params {
+  float CFL, DAMPIN, MACHLIM, F_PDOWN, CONLIM, SFXIN, SFTIN, FAC_4TH, F1, F2, F3, RSMTH, SFEXIT,
+      FSMTHB, RFIN, PDOWN_HUB, PDOWN_TIP, FEXTRAP, RFMIX, FANGLE, CP, GA, RGAS, CV, FGA, GA1, RCP,
+      VSOUND, FACMIXUP;
+  int IMID, KMID, NSTEPS_MAX, NCHANGE, NRSMTH, NSFEXIT, IN_PRESS, IN_VTAN, IN_VR, IPOUT, IFMIX, IR,
+      JR, KR, ILOS, NLOS, NMIXUP, ITIMST, IF_RESTART, IFGAS, IN_FLOW, IF_REPEAT, IRBB, KRBB, EXACT;
+};
+
+// Host-built J membership lists (CSR, 1-based J values) of the multigrid
+// blocks (JB1, JB2 incl. the mixing-plane dummy block), the superblocks
+// (JSBLK) and the row interiors used by the damping average.
+struct Lists {
+  int *b1_ptr = nullptr, *b1_list = nullptr, *b2_ptr = nullptr, *b2_list = nullptr;
+  int *sb_ptr = nullptr, *sb_list = nullptr, *row_ptr = nullptr, *row_list = nullptr;
+  float *mixscratch = nullptr;
+};
+
+}  // namespace mh
+
+struct mh_ctx {
+  mh::Lists lists{};
+  size_t lds_plane = 0;
+  mh_dims dims{};
+  mh::Geo g{};
+  mh::Dev d{};
+  hipStream_t stream = nullptr;
+  std::vector<void *> owned;                // every device allocation
+  double scal[MH_NSCALARS] = {};
+  bool scal_set[MH_NSCALARS] = {};
+  bool started = false;
+  // loop state (mf:3674-3714 and the per-step bookkeeping)
+  int nstep = 0, ifend = 0;
+  float SFT = 0, SFX = 0, DAMP = 0, FMIXUP = 1, PREFF = 0, ROLIM = 0, ROMIN = 0;
+  float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
+  float EAVG = 1e6f, EMAX = 1e6f, ECONT = 1.f;
+  mh_resid last{};
+  std::string err;
+};

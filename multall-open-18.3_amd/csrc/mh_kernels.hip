@@ -1,0 +1,1319 @@
+// HIP kernels for one time step of the reference's LOOP (gfx950).
+//
+// Every kernel names the reference lines it implements
+// (Multall-open-18.3/multall-18.3/multall-open-18.3.f, "mf:").  The
+// statement-level arithmetic follows the reference's FP32 expression order
+// (the library is built with -ffp-contract=off), so pointwise and stencil
+// phases reproduce the reference bit-for-bit; reductions either follow the
+// reference's sequential order (EXACT mode) or use deterministic FP64 trees.
+//
+// Index helpers take the reference's 1-based (I,J,K).  Node-range kernels
+// use blocks of 64 (I) x 4 (J) threads and one grid z-slice per K, so a
+// wavefront covers one pitchwise line: coalesced 256-byte rows of the
+// I-fastest layout.
+#include "mh_internal.h"
+
+namespace mh {
+
+#define IX(I, J, K) ((size_t)((I) - 1) + (size_t)g.IM * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((K) - 1)))
+#define JKX(J, K) ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((K) - 1))
+#define IKX(I, K) ((size_t)((I) - 1) + (size_t)g.IM * (size_t)((K) - 1))
+#define NODE_IJK                                           \
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1; \
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 1; \
+  const int K = blockIdx.z + 1;                            \
+  if (I > g.IM || J > g.JM) return;
+
+// ---------------------------------------------------------------------------
+// primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
+
+__global__ void k_prim(Geo g, Dev d, Params p) {
+  NODE_IJK
+  if (J < 2) return;
+  const size_t n = IX(I, J, K);
+  float RRNOW = 1 / d.R[JKX(J, K)];
+  float RONOW = d.RO[n];
+  float RECIP = 1.0f / RONOW;
+  float vx = d.ROVX[n] * RECIP;
+  float vr = d.ROVR[n] * RECIP;
+  float rovt = d.RORVT[n] * RRNOW;
+  float vt = rovt * RECIP;
+  d.VX[n] = vx;
+  d.VR[n] = vr;
+  d.ROVT[n] = rovt;
+  d.VT[n] = vt;
+  if (J <= g.JMM1) {
+    float EKE = .5f * (vr * vr + vt * vt + vx * vx);
+    float TSTATIC = (d.ROE[n] / RONOW - EKE) / p.CV;
+    d.P[n] = RONOW * p.RGAS * TSTATIC;
+    d.HO[n] = p.CP * TSTATIC + EKE;
+    d.T_STATIC[n] = TSTATIC;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// mass fluxes, relative swirl, periodic and blade-surface flux conditions,
+// mf:3960-4190 (no bleed, cooling or shroud terms)
+
+__device__ __forceinline__ float rowt_at(const Geo &g, const Dev &d, int I, int J, int K) {
+  const size_t n = IX(I, J, K);
+  return d.ROVT[n] - d.UBLADE[JKX(J, K)] * d.RO[n];
+}
+
+__device__ __forceinline__ float flowt_raw(const Geo &g, const Dev &d, int I, int J, int K) {
+  float AVGRVX = d.ROVX[IX(I, J, K)] + d.ROVX[IX(I, J - 1, K)] + d.ROVX[IX(I, J - 1, K + 1)] + d.ROVX[IX(I, J, K + 1)];
+  float AVGRVT = rowt_at(g, d, I, J, K) + rowt_at(g, d, I, J - 1, K) + rowt_at(g, d, I, J - 1, K + 1) +
+                 rowt_at(g, d, I, J, K + 1);
+  float AVGRVR = d.ROVR[IX(I, J, K)] + d.ROVR[IX(I, J - 1, K)] + d.ROVR[IX(I, J - 1, K + 1)] + d.ROVR[IX(I, J, K + 1)];
+  return (AVGRVX * d.ABX[IX(I, J, K)] + AVGRVT * d.ABT[JKX(J, K)] + AVGRVR * d.ABR[IX(I, J, K)]) * 0.25f;
+}
+
+__global__ void k_massflux(Geo g, Dev d) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  if (I <= g.IMM1 && K <= g.KMM1) {
+    float AVGRVX = d.ROVX[IX(I, J, K)] + d.ROVX[IX(I, J, K + 1)] + d.ROVX[IX(I + 1, J, K)] + d.ROVX[IX(I + 1, J, K + 1)];
+    float AVGRVR = d.ROVR[IX(I, J, K)] + d.ROVR[IX(I, J, K + 1)] + d.ROVR[IX(I + 1, J, K)] + d.ROVR[IX(I + 1, J, K + 1)];
+    float fx = 0.25f * (AVGRVX * d.AQX[n] + AVGRVR * d.AQR[n]);
+    d.FLOWX[n] = fx;
+    d.SOURCE[n] = 0.0f;
+    d.XFLUX[n] = fx;
+  }
+  {
+    float rowt = rowt_at(g, d, I, J, K);
+    d.ROWT[n] = rowt;
+    d.WT[n] = rowt / d.RO[n];
+  }
+  if (J >= 2 && K <= g.KMM1) {
+    float ft;
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
+      ft = 0.5f * (flowt_raw(g, d, 1, J, K) + flowt_raw(g, d, g.IM, J, K));
+    } else {
+      ft = flowt_raw(g, d, I, J, K);
+      if (d.IND[J - 1] == 1 && (I == 1 || I == g.IM)) {
+        int NR = d.NROW[J - 1];
+        if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) ft = 0.0f;
+      }
+    }
+    d.FLOWT[n] = ft;
+    d.TFLUX[n] = ft;
+  }
+  if (J >= 2 && I <= g.IMM1) {
+    float fr = 0.0f;
+    if (K >= 2 && K <= g.KMM1) {
+      float AVGRVX = d.ROVX[IX(I, J, K)] + d.ROVX[IX(I, J - 1, K)] + d.ROVX[IX(I + 1, J - 1, K)] + d.ROVX[IX(I + 1, J, K)];
+      float AVGRVR = d.ROVR[IX(I, J, K)] + d.ROVR[IX(I, J - 1, K)] + d.ROVR[IX(I + 1, J - 1, K)] + d.ROVR[IX(I + 1, J, K)];
+      fr = 0.25f * (AVGRVX * d.ASX[n] + AVGRVR * d.ASR[n]);
+    }
+    d.FLOWR[n] = fr;
+    d.RFLUX[n] = fr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// energy fluxes, mf:4357-4456
+
+__device__ __forceinline__ float tflux_energy(const Geo &g, const Dev &d, int I, int J, int K) {
+  float AVGHO = d.HO[IX(I, J, K)] + d.HO[IX(I, J - 1, K)] + d.HO[IX(I, J - 1, K + 1)] + d.HO[IX(I, J, K + 1)];
+  float AVGPB = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
+  return (d.FLOWT[IX(I, J, K)] * AVGHO + d.WRABT[JKX(J, K)] * AVGPB) * 0.25f;
+}
+
+__global__ void k_flux_energy(Geo g, Dev d) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  if (I <= g.IMM1 && K <= g.KMM1) {
+    float AVGHO = d.HO[IX(I, J, K)] + d.HO[IX(I, J, K + 1)] + d.HO[IX(I + 1, J, K)] + d.HO[IX(I + 1, J, K + 1)];
+    d.XFLUX[n] = d.FLOWX[n] * AVGHO * 0.25f;
+    if (J >= 2) d.SOURCE[n] = d.QSOURCE[n];
+  }
+  if (J >= 2 && I <= g.IMM1) {
+    float AVGHO = 0.25f * (d.HO[IX(I, J, K)] + d.HO[IX(I, J - 1, K)] + d.HO[IX(I + 1, J - 1, K)] + d.HO[IX(I + 1, J, K)]);
+    d.RFLUX[n] = d.FLOWR[n] * AVGHO;
+  }
+  if (J >= 2 && K <= g.KMM1) {
+    float tf;
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM))
+      tf = 0.5f * (tflux_energy(g, d, 1, J, K) + tflux_energy(g, d, g.IM, J, K));
+    else
+      tf = tflux_energy(g, d, I, J, K);
+    d.TFLUX[n] = tf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// axial momentum fluxes, mf:4818-4896
+
+__device__ __forceinline__ float tflux_axial(const Geo &g, const Dev &d, int I, int J, int K) {
+  float AVGVX = d.VX[IX(I, J, K)] + d.VX[IX(I, J - 1, K)] + d.VX[IX(I, J - 1, K + 1)] + d.VX[IX(I, J, K + 1)];
+  float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
+  return 0.25f * (d.FLOWT[IX(I, J, K)] * AVGVX + d.ABX[IX(I, J, K)] * AVGP);
+}
+
+__global__ void k_flux_axial(Geo g, Dev d) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  if (I <= g.IMM1 && K <= g.KMM1) {
+    d.SOURCE[n] = d.XFORCE[n];
+    float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J, K + 1)] + d.PEFF[IX(I + 1, J, K + 1)] + d.PEFF[IX(I + 1, J, K)];
+    float AVGVX = d.VX[IX(I, J, K)] + d.VX[IX(I, J, K + 1)] + d.VX[IX(I + 1, J, K + 1)] + d.VX[IX(I + 1, J, K)];
+    d.XFLUX[n] = 0.25f * (AVGP * d.AQX[n] + d.FLOWX[n] * AVGVX);
+  }
+  if (J >= 2 && I <= g.IMM1) {
+    float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I + 1, J - 1, K)] + d.PEFF[IX(I + 1, J, K)];
+    float AVGVX = d.VX[IX(I, J, K)] + d.VX[IX(I, J - 1, K)] + d.VX[IX(I + 1, J - 1, K)] + d.VX[IX(I + 1, J, K)];
+    d.RFLUX[n] = 0.25f * (AVGP * d.ASX[n] + d.FLOWR[n] * AVGVX);
+  }
+  if (J >= 2 && K <= g.KMM1) {
+    float tf;
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
+      const int IM = g.IM;
+      float AVGPCUSP = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)] +
+                       d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)];
+      float AVGVXCUSP = d.VX[IX(1, J, K)] + d.VX[IX(1, J - 1, K)] + d.VX[IX(1, J, K + 1)] + d.VX[IX(1, J - 1, K + 1)] +
+                        d.VX[IX(IM, J, K)] + d.VX[IX(IM, J - 1, K)] + d.VX[IX(IM, J, K + 1)] + d.VX[IX(IM, J - 1, K + 1)];
+      tf = 0.125f * (d.FLOWT[n] * AVGVXCUSP + AVGPCUSP * d.ABX[n]);
+    } else {
+      tf = tflux_axial(g, d, I, J, K);
+    }
+    d.TFLUX[n] = tf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// angular momentum fluxes, mf:4913-4956
+
+__device__ __forceinline__ float tflux_angular(const Geo &g, const Dev &d, int I, int J, int K) {
+  float AVGR = 0.25f * d.RAVG_CELL[JKX(J, K)];
+  float AVGVT = d.VT[IX(I, J, K)] + d.VT[IX(I, J - 1, K)] + d.VT[IX(I, J - 1, K + 1)] + d.VT[IX(I, J, K + 1)];
+  float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
+  return (d.FLOWT[IX(I, J, K)] * AVGVT + d.ABT[JKX(J, K)] * AVGP) * AVGR;
+}
+
+__global__ void k_flux_angular(Geo g, Dev d) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  if (I <= g.IMM1 && K <= g.KMM1) {
+    float AVGR = 0.125f * (d.R[JKX(J, K)] + d.R[JKX(J, K + 1)]);
+    float AVGVT = d.VT[IX(I, J, K)] + d.VT[IX(I, J, K + 1)] + d.VT[IX(I + 1, J, K)] + d.VT[IX(I + 1, J, K + 1)];
+    d.SOURCE[n] = d.TFORCE[n];
+    d.XFLUX[n] = d.FLOWX[n] * AVGR * AVGVT;
+  }
+  if (J >= 2 && K <= g.KMM1) {
+    float tf;
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM))
+      tf = 0.5f * (tflux_angular(g, d, 1, J, K) + tflux_angular(g, d, g.IM, J, K));
+    else
+      tf = tflux_angular(g, d, I, J, K);
+    d.TFLUX[n] = tf;
+  }
+  if (J >= 2 && I <= g.IMM1) {
+    float AVGR = (d.R[JKX(J, K)] + d.R[JKX(J - 1, K)]) * 0.125f;
+    float AVGVT = d.VT[IX(I, J, K)] + d.VT[IX(I, J - 1, K)] + d.VT[IX(I + 1, J - 1, K)] + d.VT[IX(I + 1, J, K)];
+    d.RFLUX[n] = d.FLOWR[n] * AVGVT * AVGR;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// radial momentum fluxes and source, mf:5003-5095
+
+__global__ void k_flux_radial(Geo g, Dev d) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  if (I <= g.IMM1 && K <= g.KMM1) {
+    float AVGVR = d.VR[IX(I, J, K)] + d.VR[IX(I, J, K + 1)] + d.VR[IX(I + 1, J, K)] + d.VR[IX(I + 1, J, K + 1)];
+    float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J, K + 1)] + d.PEFF[IX(I + 1, J, K + 1)] + d.PEFF[IX(I + 1, J, K)];
+    d.XFLUX[n] = 0.25f * (d.FLOWX[n] * AVGVR + d.AQR[n] * AVGP);
+  }
+  if (J >= 2 && K <= g.KMM1) {
+    float AVGVR = d.VR[IX(I, J, K)] + d.VR[IX(I, J - 1, K)] + d.VR[IX(I, J - 1, K + 1)] + d.VR[IX(I, J, K + 1)];
+    float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
+    float tf = .25f * (d.FLOWT[n] * AVGVR + AVGP * d.ABR[n]);
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
+      const int IM = g.IM;
+      float AVGVR1 = d.VR[IX(1, J, K)] + d.VR[IX(1, J - 1, K)] + d.VR[IX(1, J - 1, K + 1)] + d.VR[IX(1, J, K + 1)];
+      float AVGPB1 = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J - 1, K + 1)] + d.PEFF[IX(1, J, K + 1)];
+      float TF1 = .25f * (d.FLOWT[IX(1, J, K)] * AVGVR1 + AVGPB1 * d.ABR[IX(1, J, K)]);
+      float AVGVRM = d.VR[IX(IM, J, K)] + d.VR[IX(IM, J - 1, K)] + d.VR[IX(IM, J - 1, K + 1)] + d.VR[IX(IM, J, K + 1)];
+      float AVGPBM = d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J - 1, K + 1)] + d.PEFF[IX(IM, J, K + 1)];
+      float TFM = .25f * (d.FLOWT[IX(IM, J, K)] * AVGVRM + AVGPBM * d.ABR[IX(IM, J, K)]);
+      float AVGP1 = 0.25f * (d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)]);
+      float AVGPM = 0.25f * (d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)]);
+      float T1 = TF1 - AVGP1 * d.ABR[IX(1, J, K)];
+      float T2 = TFM - AVGPM * d.ABR[IX(IM, J, K)];
+      tf = 0.5f * ((T1 + T2) + (AVGP1 + AVGPM) * d.ABR[n]);
+    }
+    d.TFLUX[n] = tf;
+  }
+  if (J >= 2 && I <= g.IMM1) {
+    float AVGVR = d.VR[IX(I, J, K)] + d.VR[IX(I, J - 1, K)] + d.VR[IX(I + 1, J - 1, K)] + d.VR[IX(I + 1, J, K)];
+    float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I + 1, J - 1, K)] + d.PEFF[IX(I + 1, J, K)];
+    d.RFLUX[n] = 0.25f * (d.FLOWR[n] * AVGVR + d.ASR[n] * AVGP);
+    if (K <= g.KMM1) {
+      float AVGRVT = d.ROVT[IX(I, J, K)] + d.ROVT[IX(I, J, K + 1)] + d.ROVT[IX(I, J - 1, K)] + d.ROVT[IX(I, J - 1, K + 1)] +
+                     d.ROVT[IX(I + 1, J, K)] + d.ROVT[IX(I + 1, J, K + 1)] + d.ROVT[IX(I + 1, J - 1, K)] +
+                     d.ROVT[IX(I + 1, J - 1, K + 1)];
+      float AVGVT = 0.125f * (d.VT[IX(I, J, K)] + d.VT[IX(I, J - 1, K)] + d.VT[IX(I, J - 1, K + 1)] + d.VT[IX(I, J, K + 1)] +
+                              d.VT[IX(I + 1, J, K)] + d.VT[IX(I + 1, J - 1, K)] + d.VT[IX(I + 1, J - 1, K + 1)] +
+                              d.VT[IX(I + 1, J, K + 1)]);
+      float AVGP8 = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)] +
+                    d.PEFF[IX(I + 1, J, K)] + d.PEFF[IX(I + 1, J - 1, K)] + d.PEFF[IX(I + 1, J - 1, K + 1)] +
+                    d.PEFF[IX(I + 1, J, K + 1)];
+      d.SOURCE[n] = d.RFORCE[n] - (AVGRVT * AVGVT + AVGP8) * d.VOLOR[n];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inlet and exit boundary conditions, mf:4476-4734 (IFGAS=0, ITIMST<5,
+// no throttle or plate loss).  One workgroup: the exit-pressure recursion
+// PD(K) is serial in K; the per-node updates are independent.
+
+__global__ void k_bc(Geo g, Dev d, Params p, StepScalars s) {
+  extern __shared__ float sh[];
+  float *PAVG2 = sh, *PAVG3 = sh + g.KM, *PAVGOUT = sh + 2 * g.KM, *SUMK = sh + 3 * g.KM;
+  const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
+  for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) {
+    float a2 = 0.0f, a3 = 0.0f, ao = 0.0f, sm = 0.0f;
+    for (int I = 1; I <= IMM1; I++) {
+      float fp = d.FP[I - 1];
+      a2 = a2 + 0.5f * (d.P[IX(I, 2, K)] + d.P[IX(I + 1, 2, K)]) * fp;
+      a3 = a3 + 0.5f * (d.P[IX(I, 3, K)] + d.P[IX(I + 1, 3, K)]) * fp;
+      ao = ao + 0.5f * (d.P[IX(I, JMM1, K)] + d.P[IX(I + 1, JMM1, K)]) * fp;
+      sm = sm + 0.5f * fp * (d.ROVT[IX(I, JM, K)] * d.VT[IX(I, JM, K)] + d.ROVT[IX(I + 1, JM, K)] * d.VT[IX(I + 1, JM, K)]) /
+                    d.R[JKX(JM, K)];
+    }
+    PAVG2[K - 1] = a2;
+    PAVG3[K - 1] = a3;
+    PAVGOUT[K - 1] = ao;
+    SUMK[K - 1] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && p.IPOUT <= 0) {
+    float PDIFF = 0.0f, SUMP = 0.0f;
+    if (p.IPOUT == -1) PDIFF = p.PDOWN_TIP - d.PD[g.KM - 1];
+    for (int K = 1; K <= g.KM; K++) {
+      d.PD[0] = p.PDOWN_HUB;
+      if (K != 1) {
+        float DP = (SUMK[K - 1] + SUMP) * 0.5f * (d.R[JKX(JM, K)] - d.R[JKX(JM, K - 1)]);
+        d.PD[K - 1] = 0.9f * d.PD[K - 1] + 0.1f * (d.PD[K - 2] + DP);
+      }
+      SUMP = SUMK[K - 1];
+      if (p.IPOUT == -1) d.PD[K - 1] = d.PD[K - 1] + PDIFF;
+    }
+  }
+  __syncthreads();
+  const int IM = g.IM;
+  for (int t = threadIdx.x; t < IM * g.KM; t += blockDim.x) {
+    const int I = t % IM + 1, K = t / IM + 1;
+    const size_t n1 = IX(I, 1, K);
+    const float PO1 = d.PO1[K - 1], TO1 = d.TO1[K - 1];
+    float RO_STAG = PO1 / p.RGAS / TO1;
+    float RO_IN = d.RO[n1];
+    if (RO_IN > 0.9999f * RO_STAG) RO_IN = 0.9999f * RO_STAG;
+    float PNEW = PO1 * powf(RO_IN / RO_STAG, p.GA);
+    float P1 = d.P[n1];
+    if (p.IN_PRESS == 0) P1 = s.RFIN1 * P1 + p.RFIN * PNEW;
+    if (abs(p.IN_PRESS) == 1) P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(I, 2, K)] - p.RFIN * d.P[IX(I, 3, K)];
+    if (abs(p.IN_PRESS) == 3) P1 = s.RFIN1 * P1 + s.RFIN2 * PAVG2[K - 1] - p.RFIN * PAVG3[K - 1];
+    if (abs(p.IN_PRESS) == 4)
+      P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(p.IMID, 2, p.KMID)] - p.RFIN * d.P[IX(p.IMID, 3, p.KMID)];
+    const float plim = d.SC[SC_PLIM + K - 1];
+    if (P1 > plim) P1 = 0.9f * P1 + 0.1f * plim;
+    if (P1 > 0.99999f * PO1) P1 = PO1 * 0.99999f;
+    d.P[n1] = P1;
+    float TSTATIC = TO1 * powf(P1 * d.RPO1[K - 1], p.FGA);
+    float VABSQ = 2 * p.CP * (TO1 - TSTATIC);
+    d.T_STATIC[n1] = TSTATIC;
+    float VABS = sqrtf(VABSQ);
+    float VMER = 0.0f, vt1 = d.VT[n1];
+    if (p.IN_VTAN == 0) {
+      VMER = VABS * d.BSCOS[K - 1];
+      vt1 = VABS * d.BSSIN[K - 1];
+    }
+    if (p.IN_VTAN == 1) {
+      float VMERSQ = VABSQ - d.VTIN[K - 1] * d.VTIN[K - 1];
+      if (VMERSQ < 0.01f) VMERSQ = 0.01f;
+      VMER = sqrtf(VMERSQ);
+      vt1 = d.VTIN[K - 1];
+    }
+    if (p.IN_VTAN == 2) {
+      float WR = d.UBLADE[JKX(1, K)];
+      float VRELSQ = VABSQ - WR * WR * d.BSCOS[K - 1] * d.BSCOS[K - 1];
+      if (VRELSQ < 0.01f) VRELSQ = 0.01f;
+      float VREL = sqrtf(VRELSQ) - WR * d.BSSIN[K - 1];
+      VMER = VREL * d.BSCOS[K - 1];
+      vt1 = VREL * d.BSSIN[K - 1] + WR;
+    }
+    float vx1 = d.VX[n1], vr1 = d.VR[n1];
+    const float vr2 = d.VR[IX(I, 2, K)];
+    if (p.IN_VR > 0) vx1 = VMER * d.BRCOS[K - 1];
+    float VXSQ = VMER * VMER - vr2 * vr2;
+    if (VXSQ < 0.0000001f * TO1 * p.CP) VXSQ = 0.0000001f * TO1 * p.CP;
+    if (p.IN_VR <= 0) vx1 = sqrtf(VXSQ);
+    if (p.IN_VR > 0) vr1 = VMER * d.BRSIN[K - 1];
+    if (p.IN_VR <= 0) vr1 = vr2;
+    float ro1 = P1 / (p.RGAS * TSTATIC);
+    float rovt1 = ro1 * vt1;
+    d.VX[n1] = vx1;
+    d.VR[n1] = vr1;
+    d.VT[n1] = vt1;
+    d.RO[n1] = ro1;
+    d.ROVT[n1] = rovt1;
+    d.ROVR[n1] = ro1 * vr1;
+    d.RORVT[n1] = rovt1 * d.R[JKX(1, K)];
+    d.ROVX[n1] = ro1 * vx1;
+    d.ROE[n1] = P1 / p.GA1 + ro1 * 0.5f * VABSQ;
+    d.HO[n1] = p.CP * TO1;
+    // exit, mf:4690-4730
+    const size_t nm = IX(I, JM, K);
+    float vr = d.VR[nm], vt = d.VT[nm], vx = d.VX[nm];
+    float EKE = .5f * (vr * vr + vt * vt + vx * vx);
+    float PEX = d.PD[K - 1] + p.FEXTRAP * (d.P[IX(I, JMM1, K)] - PAVGOUT[K - 1]);
+    float HOEX = 2.0f * d.HO[IX(I, JMM1, K)] - d.HO[IX(I, JM - 2, K)];
+    float TEXIT = (HOEX - EKE) / p.CP;
+    float roex = PEX / p.RGAS / TEXIT;
+    d.P[nm] = PEX;
+    d.HO[nm] = HOEX;
+    d.RO[nm] = roex;
+    d.ROE[nm] = roex * (p.CV * TEXIT + EKE);
+  }
+}
+
+// PEFF, mf:4751-4757
+__global__ void k_peff(Geo g, Dev d, Params p, StepScalars s) {
+  NODE_IJK
+  int JP1 = J + 1;
+  if (J == g.JM) JP1 = g.JM;
+  const size_t n = IX(I, J, K);
+  d.PEFF[n] = d.P[n] - s.PREFF + p.F_PDOWN * (d.P[IX(I, JP1, K)] - d.P[n]);
+}
+
+// ---------------------------------------------------------------------------
+// TSTEP phases (mf:5470-5932)
+
+// upstream-face flux extrapolation at mixing planes, mf:5516-5555
+__global__ void k_fextrap(Geo g, Dev d, Params p) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
+  if (NR > g.NROWS - 1) return;
+  const int J = d.JMIX[NR - 1], JP1 = J + 1, JM1 = J - 1, JP2 = J + 2;
+  float FLOWDIRN = -d.FLOWX[IX(p.IMID, J, K)];
+  float AVFLXJM1 = 0.0f, AVFLXJP2 = 0.0f;
+  for (int I = 1; I <= g.IMM1; I++) {
+    AVFLXJM1 = AVFLXJM1 + d.XFLUX[IX(I, JM1, K)];
+    AVFLXJP2 = AVFLXJP2 + d.XFLUX[IX(I, JP2, K)];
+  }
+  for (int I = 1; I <= g.IMM1; I++) {
+    if (FLOWDIRN > 0.0f) {
+      float DFLUX = (d.XFLUX[IX(I, JM1, K)] - AVFLXJM1 * d.FP[I - 1]) * p.FEXTRAP;
+      d.SOURCE[IX(I, J, K)] = d.SOURCE[IX(I, J, K)] - DFLUX;
+    } else {
+      float DFLUX = (d.XFLUX[IX(I, JP2, K)] - AVFLXJP2 * d.FP[I - 1]) * p.FEXTRAP;
+      d.SOURCE[IX(I, JP2, K)] = d.SOURCE[IX(I, JP2, K)] + DFLUX;
+    }
+    d.SOURCE[IX(I, JP1, K)] = 0.0f;
+  }
+}
+
+// flux sum, scree/SSS history (mf:5560-5570), with the tip-gap TFLUX
+// balance of mf:5503-5510 applied to the two periodic faces it touches.
+__global__ void k_residual(Geo g, Dev d, float *D, float *DIFF, Params p) {
+  NODE_IJK
+  if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
+  const size_t n = IX(I, J, K);
+  float RATPITCH = (float)d.NBLADE[J - 2] / (float)d.NBLADE[J - 1];
+  float TF = d.TFLUX[n], TFP = d.TFLUX[IX(I + 1, J, K)];
+  const int NR = d.NROW[J - 1];
+  const int kts = d.KTIPS[NR - 1];
+  if (kts > 0 && K >= kts && K <= d.KTIPE[NR - 1] - 1) {
+    float tavg = 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(g.IM, J, K)]);
+    if (I == 1) TF = tavg;
+    if (I + 1 == g.IM) TFP = tavg;
+  }
+  float DELTA = d.XFLUX[n] - d.XFLUX[IX(I, J - 1, K)] * RATPITCH + d.RFLUX[n] - d.RFLUX[IX(I, J, K + 1)] + TF - TFP -
+                d.SOURCE[n];
+  float df = DIFF[n];
+  d.STORE[n] = p.F1 * DELTA + p.F2 * df;
+  DIFF[n] = DELTA + p.F3 * df;
+}
+
+// pitchwise averaging of STORE at the mixing planes, mf:5578-5607 (pass 1,
+// FP-weighted) and mf:5786-5818 (pass 2, plain mean)
+__global__ void k_mpavg(Geo g, Dev d, Params p, int pass) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
+  if (NR > g.NROWS - 1) return;
+  const int J = d.JMIX[NR - 1];
+  float FLOWDIRN = -d.FLOWX[IX(p.IMID, J, K)];
+  for (int side = 0; side < 2; side++) {
+    bool on = side == 0 ? ((FLOWDIRN > 0.0f) || (p.FEXTRAP < 0.001f)) : ((FLOWDIRN < 0.0f) || (p.FEXTRAP < 0.001f));
+    if (!on) continue;
+    const int JMX = side == 0 ? J + 2 : J;
+    float SUM_STORE = 0.0f;
+    for (int I = 1; I <= g.IMM1; I++) SUM_STORE = SUM_STORE + d.STORE[IX(I, JMX, K)];
+    if (pass == 1) {
+      for (int I = 1; I <= g.IMM1; I++) d.STORE[IX(I, JMX, K)] = SUM_STORE * d.FP[I - 1];
+    } else {
+      SUM_STORE = SUM_STORE / (float)g.IMM1;
+      for (int I = 1; I <= g.IMM1; I++) d.STORE[IX(I, JMX, K)] = SUM_STORE;
+    }
+  }
+}
+
+// Multigrid block sums, mf:5621-5635.  The J members of each block come
+// from a host-built CSR list (jptr/jlist) because blocks stop at mixing
+// planes and the mixing-plane cells form a separate dummy block.
+// Level 1 (3x3x3 cells): one thread per block, reference order K, J, I.
+__global__ void k_mg_sum_seq(Geo g, Dev d, const int *jptr, const int *jlist, int ni, int nj, int nk, int ir, int kr,
+                             float *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int ib = t % ni + 1, jb = (t / ni) % nj + 1, kb = t / (ni * nj) + 1;
+  const int i0 = (ib - 1) * ir + 1, i1 = min(ib * ir, g.IMM1);
+  const int k0 = (kb - 1) * kr + 1, k1 = min(kb * kr, g.KMM1);
+  float s = 0.0f;
+  for (int K = k0; K <= k1; K++)
+    for (int q = jptr[jb - 1]; q < jptr[jb]; q++) {
+      const int J = jlist[q];
+      for (int I = i0; I <= i1; I++) s = s + d.STORE[IX(I, J, K)];
+    }
+  out[t] = s;
+}
+
+// Level 2 (9x9x9) in FP64: one wavefront per block, lanes stride over the
+// block's cells, fixed-order tree at the end (deterministic).
+__global__ void k_mg_sum_wave(Geo g, Dev d, const int *jptr, const int *jlist, int ni, int nj, int nk, int ir, int kr,
+                              float *out) {
+  const int t = blockIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int ib = t % ni + 1, jb = (t / ni) % nj + 1, kb = t / (ni * nj) + 1;
+  const int i0 = (ib - 1) * ir + 1, i1 = min(ib * ir, g.IMM1);
+  const int k0 = (kb - 1) * kr + 1, k1 = min(kb * kr, g.KMM1);
+  const int ni_ = i1 - i0 + 1, nk_ = k1 - k0 + 1, q0 = jptr[jb - 1], nj_ = jptr[jb] - q0;
+  const int tot = ni_ * nj_ * nk_;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < tot; c += 64) {
+    const int I = i0 + c % ni_, J = jlist[q0 + (c / ni_) % nj_], K = k0 + c / (ni_ * nj_);
+    s += (double)d.STORE[IX(I, J, K)];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) out[t] = (float)s;
+}
+
+// Superblock and per-row sums over whole J ranges: stage 1 sums each
+// (J,K) line over I in FP64 (optionally of |STORE|).
+__global__ void k_line_sums(Geo g, Dev d, double *lines, int absval) {
+  const int J = blockIdx.x * blockDim.y + threadIdx.y + 1;
+  const int K = blockIdx.y + 1;
+  if (J > g.JM || K > g.KMM1) return;
+  double s = 0.0;
+  if (J >= 2)
+    for (int I = threadIdx.x + 1; I <= g.IMM1; I += 64) {
+      float v = d.STORE[IX(I, J, K)];
+      s += (double)(absval ? fabsf(v) : v);
+    }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) lines[JKX(J, K)] = s;
+}
+
+// stage 2: one wavefront per J range [jlo, jhi] (1-based, from CSR lists)
+__global__ void k_range_sums(Geo g, const double *lines, const int *jptr, const int *jlist, int nranges, float *out,
+                             int divide_by_cells) {
+  const int r = blockIdx.x;
+  if (r >= nranges) return;
+  const int q0 = jptr[r], nj = jptr[r + 1] - q0;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < nj * g.KMM1; c += 64) {
+    const int J = jlist[q0 + c % nj], K = c / nj + 1;
+    s += lines[JKX(J, K)];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) {
+    float f = (float)s;
+    if (divide_by_cells) f = f / (float)(g.IMM1 * g.KMM1 * nj);
+    out[r] = f;
+  }
+}
+
+// sequential-order versions for EXACT mode (reference order K, J, I)
+__global__ void k_range_sums_seq(Geo g, Dev d, const int *jptr, const int *jlist, int nranges, float *out,
+                                 int absval, int divide_by_cells) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nranges) return;
+  float s = 0.0f;
+  for (int K = 1; K <= g.KMM1; K++)
+    for (int q = jptr[r]; q < jptr[r + 1]; q++) {
+      const int J = jlist[q];
+      for (int I = 1; I <= g.IMM1; I++) {
+        float v = d.STORE[IX(I, J, K)];
+        s = s + (absval ? fabsf(v) : v);
+      }
+    }
+  if (divide_by_cells) s = s / (float)(g.IMM1 * g.KMM1 * (jptr[r + 1] - jptr[r]));
+  out[r] = s;
+}
+
+// multigrid update, mf:5645-5660
+__global__ void k_mg_apply(Geo g, Dev d) {
+  NODE_IJK
+  if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
+  const size_t n = IX(I, J, K);
+  const int I1 = d.IB1[I - 1], I2 = d.IB2[I - 1], K1 = d.KB1[K - 1], K2 = d.KB2[K - 1];
+  const int J1 = d.JB1[J - 1], J2 = d.JB2[J - 1], JSB = d.JSBLK[J - 1];
+  const size_t b1 = (size_t)(I1 - 1) + (size_t)g.NIB1 * ((size_t)(J1 - 1) + (size_t)(g.NJB1 + 1) * (K1 - 1));
+  const size_t b2 = (size_t)(I2 - 1) + (size_t)g.NIB2 * ((size_t)(J2 - 1) + (size_t)(g.NJB2 + 1) * (K2 - 1));
+  float DELTA = d.STORE[n] * d.STEP[n] +
+                (d.B1CHG[b1] * d.STEP1[b1] + d.B2CHG[b2] * d.STEP2[b2] + d.SBCHG[JSB - 1] * d.STEPSBK[JSB - 1]) * d.RSTEP[n];
+  d.STORE[n] = DELTA;
+}
+
+// SMOOTH_RESID, mf:14833-14917: one pass in J, then I, then K, within each
+// row (rows never share cells).  dir 0/1/2 = J/I/K; src -> dst.
+__global__ void k_smooth_resid(Geo g, Dev d, const float *src, float *dst, float SMFAC, int dir) {
+  NODE_IJK
+  if (I > g.IMM1 || K > g.KMM1) return;
+  const int NR = d.NROW[J - 1];
+  const int J1 = d.JSTART[NR - 1], J1P1 = J1 + 1, J1P2 = J1 + 2, J2 = d.JMIX[NR - 1], J2M1 = J2 - 1;
+  const size_t n = IX(I, J, K);
+  if (J < J1P1) { dst[n] = src[n]; return; }
+  const float SFAC1 = 1.0f - SMFAC, SFACH = 0.5f * SMFAC;
+  float v;
+  if (dir == 0) {
+    if (J == J1P1) v = src[n] + SFACH * (src[IX(I, J1P2, K)] - src[n]);
+    else if (J == J2) v = src[n] + SFACH * (src[IX(I, J2M1, K)] - src[n]);
+    else v = SFAC1 * src[n] + SFACH * (src[IX(I, J + 1, K)] + src[IX(I, J - 1, K)]);
+  } else if (dir == 1) {
+    const int IMM1 = g.IMM1, IMM2 = g.IM - 2;
+    if (I == 1) v = src[n] + SFACH * (src[IX(2, J, K)] - src[n]);
+    else if (I == IMM1) v = src[n] + SFACH * (src[IX(IMM2, J, K)] - src[n]);
+    else v = SFAC1 * src[n] + SFACH * (src[IX(I + 1, J, K)] + src[IX(I - 1, J, K)]);
+  } else {
+    const int KMM1 = g.KMM1, KMM2 = g.KM - 2;
+    if (K == 1) v = src[n] + SFACH * (src[IX(I, J, 2)] - src[n]);
+    else if (K == KMM1) v = src[n] + SFACH * (src[IX(I, J, KMM2)] - src[n]);
+    else v = SFAC1 * src[n] + SFACH * (src[IX(I, J, K + 1)] + src[IX(I, J, K - 1)]);
+  }
+  dst[n] = v;
+}
+
+// row-averaged change smoothing between neighbouring rows, mf:5699-5710
+__global__ void k_avg_blk(Geo g, Dev d) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int NROWS = g.NROWS;
+  float *AC = d.SC + SC_AVG_CHG, *AB = d.SC + SC_AVG_BLK;
+  if (NROWS == 1) AB[0] = AC[0];
+  if (NROWS == 2) {
+    AB[1] = 0.5f * (AC[0] + AC[1]);
+    AB[0] = AB[1];
+  }
+  if (NROWS > 2) {
+    for (int NR = 2; NR <= NROWS - 1; NR++) AB[NR - 1] = 0.25f * (AC[NR - 2] + AC[NR]) + 0.5f * AC[NR - 1];
+    AB[NROWS - 1] = 0.5f * (AC[NROWS - 2] + AC[NROWS - 1]);
+    AB[0] = 0.5f * (AC[0] + AC[1]);
+  }
+}
+
+// negative feedback, mf:5716-5724
+__global__ void k_damp_apply(Geo g, Dev d, float DAMP) {
+  NODE_IJK
+  if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
+  const size_t n = IX(I, J, K);
+  const int NR = d.NROW[J - 1];
+  float DELTA = d.STORE[n];
+  float ABSCHG = fabsf(DELTA);
+  float FDAMP = ABSCHG / d.SC[SC_AVG_BLK + NR - 1];
+  d.STORE[n] = DELTA / (1.f + FDAMP / DAMP);
+}
+
+// Corner distribution of the cell changes, mf:5828-5842, as a node-centred
+// gather: node (I,J,K) receives from cells I-1..I, J..J+1, K-1..K, summed in
+// the order the reference's cell loop (K, J, I) visits them.
+__global__ void k_scatter(Geo g, Dev d, const float *D, float *DOUT) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  float v = D[n];
+  for (int kc = K - 1; kc <= K; kc++) {
+    if (kc < 1 || kc > g.KMM1) continue;
+    const bool top = (kc == K - 1);  // node is the cell's K+1 corner
+    for (int jc = J; jc <= J + 1; jc++) {
+      if (jc < 2 || jc > g.JM) continue;
+      const float fac = (jc == J) ? d.FACDWN[jc - 1] : d.FACUP[jc - 1];
+      for (int ic = I - 1; ic <= I; ic++) {
+        if (ic < 1 || ic > g.IMM1) continue;
+        const bool right = (ic == I - 1);  // node is the cell's I+1 corner
+        const float *w = top ? (right ? d.FTR : d.FTL) : (right ? d.FBR : d.FBL);
+        float ADD = d.STORE[IX(ic, jc, kc)];
+        v = v + ADD * w[IKX(ic, kc)] * fac;
+      }
+    }
+  }
+  DOUT[n] = v;
+}
+
+// SMOOTH_VAR (mf:15352-15594) followed by TSTEP's periodic, tip-gap and
+// mixing-plane conditions on D (mf:5858-5926).  One workgroup per J plane:
+// the streamwise stencil reads planes J-2..J+2 of the pre-smoothing field
+// SRC (global memory, L2-resident); the rest of the work is plane-local and
+// runs in LDS.  Writes the finished plane to DST.
+__global__ void k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Params p, StepScalars s, int ncall) {
+  extern __shared__ float lds[];
+  const int J = blockIdx.x + 1;
+  const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
+  const int npl = IM * KM;
+  float *PL = lds;          // the plane, [K][I]
+  float *AV = lds + npl;    // AVG scratch
+  float *CU = lds + 2 * npl;  // CURVE scratch
+#define PLX(I, K) PL[((I) - 1) + IM * ((K) - 1)]
+#define AVX(I, K) AV[((I) - 1) + IM * ((K) - 1)]
+#define CUX(I, K) CU[((I) - 1) + IM * ((K) - 1)]
+  const float SFX = s.SFX, SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
+  const float SFXB = p.FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
+  const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
+  const float SFT = s.SFT, SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
+  const int NR = d.NROW[J - 1];
+  const int J1 = d.JSTART[NR - 1], J2 = d.JMIX[NR - 1], JLED = d.JLED[NR - 1];
+  // streamwise smoothing, mf:15369-15445
+  for (int t = threadIdx.x; t < npl; t += blockDim.x) {
+    const int I = t % IM + 1, K = t / IM + 1;
+#define S(JJ) SRC[IX(I, (JJ), K)]
+    float v = S(J);
+    if (!(SFX < 0.0001f)) {
+      if (J >= J1 + 3 && J <= J2 - 3) {
+        float AVERG = 0.5f * (S(J + 1) + S(J - 1));
+        float CURV = AVERG - 0.5f * S(J) - 0.25f * (S(J - 2) + S(J + 2));
+        v = SFX1 * S(J) + SFX * (AVERG + FAC_4TH * CURV);
+      }
+      if (J == J1 + 2) v = SFXB1 * S(J1 + 2) + SFXBH * (S(J1 + 3) + S(J1 + 1));
+      if (J == J1 + 1) v = SFXB1 * S(J1 + 1) + SFXB * (1.25f * S(J1 + 2) + 0.5f * S(J1 + 3) - 0.75f * S(J1 + 4));
+      if (J == J2 - 2) v = SFXB1 * S(J2 - 2) + SFXBH * (S(J2 - 1) + S(J2 - 3));
+      if (J == J2 - 1) v = SFXB1 * S(J2 - 1) + SFXB * (1.25f * S(J2 - 2) + 0.5f * S(J2 - 3) - 0.75f * S(J2 - 4));
+      if (I == 1 || I == IM) {
+        const int Io = (I == 1) ? IM : 1;
+        if (J == JLED - 2 || J == JLED - 1)
+          v = SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
+        if (J == JLED) v = SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)]);
+      }
+    }
+#undef S
+    PLX(I, K) = v;
+  }
+  __syncthreads();
+  // pitchwise and spanwise smoothing and corners, mf:15460-15541 (J >= 2)
+  if (J >= 2 && !(SFT < 0.0001f)) {
+    const bool blade = (d.IND[J - 1] == 1 || d.INDLE[J - 1] == 1);
+    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
+      const int I = t % IM + 1, K = t / IM + 1;
+      float avg, cur;
+      if (I >= 2 && I <= IMM1) {
+        avg = d.FD[I - 1] * PLX(I + 1, K) + d.FU[I - 1] * PLX(I - 1, K);
+        cur = PLX(I, K) - avg;
+      } else if (!blade) {
+        avg = 0.5f * (PLX(2, K) + PLX(IMM1, K));
+        cur = PLX(1, K) - avg;
+      } else {
+        avg = (I == 1) ? PLX(2, K) + d.FU[0] * (PLX(2, K) - PLX(3, K))
+                       : PLX(IMM1, K) + d.FU[IM - 1] * (PLX(IMM1, K) - PLX(IMM2, K));
+        cur = 0.0f;
+      }
+      AVX(I, K) = avg;
+      CUX(I, K) = cur;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
+      const int I = t % IM + 1, K = t / IM + 1;
+      float sc;
+      if (I >= 2 && I <= IMM1) sc = d.FU[I - 1] * CUX(I - 1, K) + d.FD[I - 1] * CUX(I + 1, K);
+      else if (!blade) sc = 0.5f * (CUX(2, K) + CUX(IMM1, K));
+      else sc = 0.0f;
+      // AVG(I,K) is read only by this thread, so the result can replace it
+      AVX(I, K) = SFT1 * PLX(I, K) + SFT * (AVX(I, K) + FAC_4TH * sc);
+    }
+    __syncthreads();
+    { float *t = PL; PL = AV; AV = t; }
+    // spanwise
+    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
+      const int I = t % IM + 1, K = t / IM + 1;
+      float avg, cur;
+      if (K >= 2 && K <= KMM1) {
+        avg = d.FKD[K - 1] * PLX(I, K + 1) + d.FKU[K - 1] * PLX(I, K - 1);
+        cur = PLX(I, K) - avg;
+      } else if (K == 1) {
+        avg = PLX(I, 2) + 0.5f * d.FKU[0] * (PLX(I, 2) - PLX(I, 3));
+        cur = 0.0f;
+      } else {
+        avg = PLX(I, KMM1) + 0.5f * d.FKU[KM - 1] * (PLX(I, KMM1) - PLX(I, KMM2));
+        cur = 0.0f;
+      }
+      AVX(I, K) = avg;
+      CUX(I, K) = cur;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
+      const int I = t % IM + 1, K = t / IM + 1;
+      float sc = (K >= 2 && K <= KMM1) ? d.FKU[K - 1] * CUX(I, K - 1) + d.FKD[K - 1] * CUX(I, K + 1) : 0.0f;
+      AVX(I, K) = SFT1 * PLX(I, K) + SFT * (AVX(I, K) + FAC_4TH * sc);
+    }
+    __syncthreads();
+    { float *t = PL; PL = AV; AV = t; }
+    if (threadIdx.x == 0) {
+      PLX(1, 1) = SFT1 * PLX(1, 1) + SFTH * (PLX(2, 1) + PLX(1, 2));
+      PLX(IM, 1) = SFT1 * PLX(IM, 1) + SFTH * (PLX(IMM1, 1) + PLX(IM, 2));
+      PLX(1, KM) = SFT1 * PLX(1, KM) + SFTH * (PLX(1, KMM1) + PLX(2, KM));
+      PLX(IM, KM) = SFT1 * PLX(IM, KM) + SFTH * (PLX(IMM1, KM) + PLX(IM, KMM1));
+    }
+    __syncthreads();
+  }
+  // exit smoothing, mf:15553-15588: Gauss-Seidel sweeps in I, then in K
+  if (!(p.SFEXIT < 0.0001f) && J >= g.JM + 1 - p.NSFEXIT) {
+    const float SFEX1 = s.SFEX1, SFEXH = s.SFEXH, SFEXIT = p.SFEXIT;
+    for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
+      for (int I = 2; I <= IMM1; I++) PLX(I, K) = SFEX1 * PLX(I, K) + SFEXH * (PLX(I + 1, K) + PLX(I - 1, K));
+      PLX(1, K) = PLX(1, K) + SFEXIT * (2.f * PLX(2, K) - PLX(3, K) - PLX(1, K));
+      PLX(IM, K) = PLX(IM, K) + SFEXIT * (2.f * PLX(IMM1, K) - PLX(IMM2, K) - PLX(IM, K));
+    }
+    __syncthreads();
+    for (int I = threadIdx.x + 1; I <= IM; I += blockDim.x) {
+      for (int K = 2; K <= KMM1; K++) PLX(I, K) = SFEX1 * PLX(I, K) + SFEXH * (PLX(I, K - 1) + PLX(I, K + 1));
+      PLX(I, 1) = PLX(I, 1) + SFEXIT * (2.f * PLX(I, 2) - PLX(I, 3) - PLX(I, 1));
+      PLX(I, KM) = PLX(I, KM) + SFEXIT * (2.f * PLX(I, KMM1) - PLX(I, KMM2) - PLX(I, KM));
+    }
+    __syncthreads();
+  }
+  // periodicity off the blade and across the tip gap, mf:5858-5891
+  const int ILAST = (ncall == 2) ? IMM1 : IM;
+  {
+    const bool per = !((d.IND[J - 1] == 1) || (d.INDLE[J - 1] == 1));
+    int K1 = 1, K2 = 0;
+    if (d.KTIPS[NR - 1] > 0) {
+      K1 = d.KTIPS[NR - 1];
+      K2 = d.KTIPE[NR - 1];
+      if (K1 == 1) K2 = K2 - 1;
+      if (K2 == KM) K1 = K1 + 1;
+    }
+    for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
+      if (per) {
+        PLX(1, K) = 0.5f * (PLX(1, K) + PLX(ILAST, K));
+        PLX(ILAST, K) = PLX(1, K);
+      }
+      if (K >= K1 && K <= K2) {
+        PLX(1, K) = 0.5f * (PLX(1, K) + PLX(ILAST, K));
+        PLX(ILAST, K) = PLX(1, K);
+      }
+    }
+  }
+  __syncthreads();
+  // pitchwise-uniform D on both faces of a mixing plane, mf:5901-5926
+  if (p.IFMIX != 0) {
+    bool mp = (J == J2 && NR < g.NROWS) || (NR > 1 && J == J1);
+    if (mp) {
+      for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
+        float AVG = 0.0f;
+        for (int I = 1; I <= IMM1; I++) AVG = AVG + d.FP[I - 1] * (PLX(I, K) + PLX(I + 1, K));
+        AVG = 0.5f * AVG;
+        for (int I = 1; I <= IM; I++) PLX(I, K) = AVG;
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < npl; t += blockDim.x) {
+    const int I = t % IM + 1, K = t / IM + 1;
+    DST[IX(I, J, K)] = PLX(I, K);
+  }
+#undef PLX
+#undef AVX
+#undef CUX
+}
+
+// ---------------------------------------------------------------------------
+// limiters, mf:5115-5168
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__device__ __forceinline__ float rolim_one(float ro, float rom2, float rom1, float rop1, float rop2, float ROLIM) {
+  float ROAVG = 0.25f * (rom2 + rom1 + rop1 + rop2);
+  if (ro < 0.6f * ROAVG) ro = 0.5f * (ro + 0.6f * ROAVG);
+  if (ro > 1.5f * ROAVG) ro = 0.5f * (ro + 1.5f * ROAVG);
+  if (ro < ROLIM) ro = 0.5f * (ro + ROLIM);
+  return ro;
+}
+
+// Density limiter (mf:5115-5122) in two passes.  The reference updates RO in
+// place along J, so a change at J is seen by J+1 and J+2.  Pass 1 evaluates
+// every node with the unmodified field and flags the (I,K) lines where any
+// node would change; pass 2 replays only those lines serially in J.
+__global__ void k_rolim_check(Geo g, Dev d, StepScalars s) {
+  NODE_IJK
+  if (J < 3 || J > g.JM - 3) return;
+  float ro = d.RO[IX(I, J, K)];
+  float nv = rolim_one(ro, d.RO[IX(I, J - 2, K)], d.RO[IX(I, J - 1, K)], d.RO[IX(I, J + 1, K)], d.RO[IX(I, J + 2, K)], s.ROLIM);
+  if (nv != ro) d.FLAGS[IKX(I, K)] = 1;
+}
+
+__global__ void k_rolim_fix(Geo g, Dev d, StepScalars s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.IM * g.KM) return;
+  if (d.FLAGS[t] == 0) return;
+  const int I = t % g.IM + 1, K = t / g.IM + 1;
+  for (int J = 3; J <= g.JM - 3; J++)
+    d.RO[IX(I, J, K)] = rolim_one(d.RO[IX(I, J, K)], d.RO[IX(I, J - 2, K)], d.RO[IX(I, J - 1, K)], d.RO[IX(I, J + 1, K)],
+                                  d.RO[IX(I, J + 2, K)], s.ROLIM);
+}
+
+// reference relative Mach conditions per row, mf:5127-5145
+__global__ void k_machref(Geo g, Dev d, Params p, StepScalars s) {
+  const int NR = threadIdx.x + 1;
+  if (NR > g.NROWS) return;
+  const int IMID = p.IMID, KMID = p.KMID;
+  float GM1 = p.GA - 1.0f;
+  float TRATIO = 1.0f + p.MACHLIM * p.MACHLIM * 0.5f * GM1;
+  float RFGM1 = 1.0f / GM1;
+  float RORATIO = powf(TRATIO, RFGM1);
+  const int JREF = d.JLE[NR - 1];
+  const size_t n = IX(IMID, JREF, KMID);
+  float TREF = d.T_STATIC[n];
+  float WREFSQ = d.VX[n] * d.VX[n] + d.VR[n] * d.VR[n] + d.WT[n] * d.WT[n];
+  float TOREL_REF = TREF + 0.5f * WREFSQ / p.CP;
+  float TLIMIT = TOREL_REF / TRATIO;
+  if (TLIMIT > s.TCOOL_MIN) TLIMIT = s.TCOOL_MIN;
+  d.SC[SC_VLIMIT + NR - 1] = sqrtf(2 * p.CP * (TOREL_REF - TLIMIT));
+  float ROSTAG_REL = d.RO[n] * powf(TOREL_REF / TREF, RFGM1);
+  d.SC[SC_ROLIMIT + NR - 1] = ROSTAG_REL / RORATIO;
+}
+
+// relative Mach number and mass-flux limiter, mf:5150-5168
+__global__ void k_mach_apply(Geo g, Dev d) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  const int NR = d.NROW[J - 1];
+  const float V_LIMIT = d.SC[SC_VLIMIT + NR - 1], RO_LIMIT = d.SC[SC_ROLIMIT + NR - 1];
+  float ro = d.RO[n];
+  if (ro < RO_LIMIT) {
+    ro = RO_LIMIT;
+    d.RO[n] = ro;
+  }
+  const float vx = d.VX[n], vr = d.VR[n], wt = d.WT[n];
+  float WRELSQ = vx * vx + vr * vr + wt * wt;
+  float WREL = sqrtf(WRELSQ);
+  float FACSAFE = V_LIMIT / WREL;
+  if (FACSAFE < 1.0f) {
+    d.ROVX[n] = ro * vx * FACSAFE;
+    d.ROVR[n] = ro * vr * FACSAFE;
+    float ROVTREL = ro * wt * FACSAFE;
+    d.ROWT[n] = ROVTREL;
+    float rovt = ROVTREL + ro * d.UBLADE[JKX(J, K)];
+    d.ROVT[n] = rovt;
+    d.RORVT[n] = rovt * d.R[JKX(J, K)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NEW_MIXPLAN, mf:14922-15251 (ideal gas, ITIMST<5): one thread per
+// (mixing plane, K) line; the supersonic Prandtl-Meyer sweeps are serial in I.
+
+__global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NR = t / g.KM + 1, K = t % g.KM + 1;
+  if (NR > g.NROWS - 1) return;
+  const int IM = g.IM, IMID = p.IMID;
+  float *ALPHAA = scratch + (size_t)t * 8 * IM;
+  float *GAMA = ALPHAA + IM, *WREF = GAMA + IM, *ROREF = WREF + IM, *PISENT = ROREF + IM, *ROISENT = PISENT + IM,
+        *WMACH = ROISENT + IM, *TISENT = WMACH + IM;
+  float WIS[1];
+  (void)WIS;
+  const float RFMIX = p.RFMIX, RFMIX1 = 1.0f - RFMIX;
+  int KSUB = K;
+  if (K == g.KM) KSUB = g.KM - 1;
+  int J = d.JMIX[NR - 1], JP1, JP2, JP3;
+  float F_ANGLE;
+  if (d.FLOWX[IX(IMID, J, KSUB)] < 0.0f) {
+    JP1 = J + 1; JP2 = J + 2; JP3 = J + 3; F_ANGLE = p.FANGLE;
+  } else {
+    J = d.JMIX[NR - 1] + 1;
+    JP1 = J - 1; JP2 = J - 2; JP3 = J - 3; F_ANGLE = 0.0f;
+  }
+  float F_ANGLE1 = 1.0f - F_ANGLE;
+  float RJP1 = d.R[JKX(JP1, K)], UJP1 = d.UBLADE[JKX(JP1, K)];
+  const size_t nm = IX(IMID, JP1, K);
+  float ROMIX = d.RO[nm];
+  float VXMIX = d.ROVX[nm] / ROMIX;
+  float VRMIX = d.ROVR[nm] / ROMIX;
+  float VTMIX = d.RORVT[nm] / ROMIX / RJP1;
+  float ROEMIX = d.ROE[nm];
+  float VMSQ = VXMIX * VXMIX + VRMIX * VRMIX;
+  float VABSQ = VMSQ + VTMIX * VTMIX;
+  float WTMIX = VTMIX - UJP1;
+  float WMIXSQ = VMSQ + WTMIX * WTMIX;
+  float WMIX = sqrtf(WMIXSQ);
+  float ALPHAMIX = asinf(WTMIX / WMIX);
+  float GAMAMIX = atan2f(VRMIX, VXMIX);
+  float TMIX = (ROEMIX / ROMIX - 0.5f * VABSQ) / p.CV;
+  float ROTHALPY = p.CP * TMIX + 0.5f * (WMIXSQ - UJP1 * UJP1);
+  float GAGAS = p.GA;
+  float PMIX = ROMIX * p.RGAS * TMIX;
+  int NMACH = 0;
+  const float RJP2 = d.R[JKX(JP2, K)], UNOWJP2 = d.UBLADE[JKX(JP2, K)];
+  const float RJP3 = d.R[JKX(JP3, K)], UNOWJP3 = d.UBLADE[JKX(JP3, K)];
+  float *WISV = scratch + (size_t)(g.NROWS - 1) * g.KM * 8 * IM + (size_t)t * IM;
+  for (int I = 1; I <= IM; I++) {
+    const size_t n2 = IX(I, JP2, K), n3 = IX(I, JP3, K);
+    float ROJP2 = d.RO[n2];
+    float XVELJP2 = d.ROVX[n2] / ROJP2;
+    float RVELJP2 = d.ROVR[n2] / ROJP2;
+    float TVELJP2 = d.RORVT[n2] / RJP2 / ROJP2;
+    float WTJP2 = TVELJP2 - UNOWJP2;
+    float VMSQJP2 = XVELJP2 * XVELJP2 + RVELJP2 * RVELJP2;
+    float VELSQJP2 = VMSQJP2 + TVELJP2 * TVELJP2;
+    float WACT_SQ_JP2 = VMSQJP2 + WTJP2 * WTJP2;
+    float WACT_JP2 = sqrtf(WACT_SQ_JP2);
+    float T_ACT_JP2 = (d.ROE[n2] / ROJP2 - 0.5f * VELSQJP2) / p.CV;
+    float ROJP3 = d.RO[n3];
+    float ROVXJP3 = d.ROVX[n3], ROVRJP3 = d.ROVR[n3];
+    float ROVTJP3 = d.RORVT[n3] / RJP3;
+    float ROWTJP3 = ROVTJP3 - UNOWJP3 * ROJP3;
+    float ROVMSQJP3 = ROVXJP3 * ROVXJP3 + ROVRJP3 * ROVRJP3;
+    float ROWRELSQJP3 = ROVMSQJP3 + ROWTJP3 * ROWTJP3;
+    float ROWRELJP3 = sqrtf(ROWRELSQJP3);
+    float ALPHAJP3 = asinf(ROWTJP3 / ROWRELJP3);
+    float GAMAJP3 = atan2f(ROVRJP3, ROVXJP3);
+    float PSTAT = ROJP2 * p.RGAS * T_ACT_JP2;
+    float TISENT_JP2 = TMIX * powf(PSTAT / PMIX, p.FGA);
+    float WIS_SQ_JP2 = 2 * (ROTHALPY - p.CP * TISENT_JP2 + 0.5f * UNOWJP2 * UNOWJP2);
+    float ROISENT_JP2 = PSTAT / p.RGAS / TISENT_JP2;
+    ROISENT[I - 1] = ROISENT_JP2;
+    PISENT[I - 1] = PSTAT;
+    TISENT[I - 1] = TISENT_JP2;
+    if (WIS_SQ_JP2 < 1.0f) WIS_SQ_JP2 = 1.0f;
+    WISV[I - 1] = sqrtf(WIS_SQ_JP2);
+    WREF[I - 1] = WACT_JP2;
+    ROREF[I - 1] = ROJP2;
+    WMACH[I - 1] = WACT_JP2 / sqrtf(GAGAS * p.RGAS * T_ACT_JP2);
+    if (WMACH[I - 1] > 1.0f) NMACH = NMACH + 1;
+    ALPHAA[I - 1] = F_ANGLE1 * ALPHAMIX + F_ANGLE * ALPHAJP3;
+    GAMA[I - 1] = F_ANGLE1 * GAMAMIX + F_ANGLE * GAMAJP3;
+  }
+  if (NMACH != 0) {
+    float FROTN = 1.0f;
+    if (WTMIX < 0.0f) FROTN = -1.0f;
+    for (int I = IMID + 1; I <= IM; I++) {
+      const int a = I - 1, b = I - 2;
+      if (WMACH[a] > 1.001f && WMACH[b] > 1.001f) {
+        float DELP = PISENT[a] - PISENT[b];
+        float DELTAI = sqrtf(WMACH[a] * WMACH[a] - 1) / (ROREF[a] * WREF[a] * WREF[a]);
+        float DELTAIM1 = sqrtf(WMACH[b] * WMACH[b] - 1) / (ROREF[b] * WREF[b] * WREF[b]);
+        ALPHAA[a] = ALPHAA[b] + FROTN * DELP * 0.5f * (DELTAI + DELTAIM1);
+      }
+    }
+    for (int I = IMID - 1; I >= 1; I--) {
+      const int a = I - 1, b = I;
+      if (WMACH[a] > 1.001f && WMACH[b] > 1.001f) {
+        float DELP = PISENT[a] - PISENT[b];
+        float DELTAI = sqrtf(WMACH[a] * WMACH[a] - 1) / (ROREF[a] * WREF[a] * WREF[a]);
+        float DELTAIP1 = sqrtf(WMACH[b] * WMACH[b] - 1) / (ROREF[b] * WREF[b] * WREF[b]);
+        ALPHAA[a] = ALPHAA[b] + FROTN * DELP * 0.5f * (DELTAI + DELTAIP1);
+      }
+    }
+    float ROTN = ALPHAA[IM - 1] - ALPHAA[0];
+    float SUMFP = 0.0f;
+    for (int I = 1; I <= IM; I++) {
+      ALPHAA[I - 1] = ALPHAA[I - 1] - ROTN * (SUMFP - 0.5f);
+      SUMFP = SUMFP + d.FP[I - 1];
+    }
+  }
+  for (int I = 1; I <= IM; I++) {
+    const size_t n2 = IX(I, JP2, K);
+    float RO_JP2 = ROISENT[I - 1];
+    float WT_JP2 = WISV[I - 1] * sinf(ALPHAA[I - 1]);
+    float VM_JP2 = WISV[I - 1] * cosf(ALPHAA[I - 1]);
+    float VX_JP2 = VM_JP2 * cosf(GAMA[I - 1]);
+    float VR_JP2 = VM_JP2 * sinf(GAMA[I - 1]);
+    float VT_JP2 = WT_JP2 + UNOWJP2;
+    float V_JP2_SQ = VM_JP2 * VM_JP2 + VT_JP2 * VT_JP2;
+    float ROE_JP2 = ROISENT[I - 1] * (p.CV * TISENT[I - 1] + 0.5f * V_JP2_SQ);
+    d.RO[n2] = RFMIX1 * d.RO[n2] + RFMIX * RO_JP2;
+    d.ROVX[n2] = RFMIX1 * d.ROVX[n2] + RFMIX * RO_JP2 * VX_JP2;
+    d.ROVR[n2] = RFMIX1 * d.ROVR[n2] + RFMIX * RO_JP2 * VR_JP2;
+    d.RORVT[n2] = RFMIX1 * d.RORVT[n2] + RFMIX * RO_JP2 * VT_JP2 * RJP2;
+    d.ROE[n2] = RFMIX1 * d.ROE[n2] + RFMIX * ROE_JP2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// every-5-step checks: VRMS/VMAX (mf:3846-3864), CHECK_FLOW/ECONT
+// (mf:5264-5283), EMAX/EAVG (mf:5291-5317)
+
+// Sum and max of W^2 over J>=2: stage 1 per block in FP64.
+__global__ void k_vrms_part(Geo g, Dev d, double *part) {
+  __shared__ double ssum[256];
+  __shared__ float smax[256];
+  const int tid = threadIdx.x;
+  const size_t N = (size_t)g.IM * g.JM * g.KM;
+  double s = 0.0;
+  float m = 0.0f;
+  for (size_t n = (size_t)blockIdx.x * blockDim.x + tid; n < N; n += (size_t)gridDim.x * blockDim.x) {
+    const int J = (int)((n / g.IM) % g.JM) + 1;
+    if (J < 2) continue;
+    float EKE = d.VR[n] * d.VR[n] + d.WT[n] * d.WT[n] + d.VX[n] * d.VX[n];
+    s += (double)EKE;
+    if (EKE > m) m = EKE;
+  }
+  ssum[tid] = s;
+  smax[tid] = m;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      ssum[tid] += ssum[tid + off];
+      smax[tid] = fmaxf(smax[tid], smax[tid + off]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[2 * blockIdx.x] = ssum[0];
+    part[2 * blockIdx.x + 1] = (double)smax[0];
+  }
+}
+
+__global__ void k_vrms_final(Geo g, Dev d, const double *part, int nparts, int exact) {
+  if (threadIdx.x != 0) return;
+  float VRMS = 0.0f, VMAX = 0.0f;
+  if (exact) {
+    for (int K = 1; K <= g.KM; K++)
+      for (int J = 2; J <= g.JM; J++)
+        for (int I = 1; I <= g.IM; I++) {
+          const size_t n = IX(I, J, K);
+          float EKE = d.VR[n] * d.VR[n] + d.WT[n] * d.WT[n] + d.VX[n] * d.VX[n];
+          if (EKE > VMAX) VMAX = EKE;
+          VRMS = VRMS + EKE;
+        }
+  } else {
+    double s = 0.0, m = 0.0;
+    for (int b = 0; b < nparts; b++) {
+      s += part[2 * b];
+      m = fmax(m, part[2 * b + 1]);
+    }
+    VRMS = (float)s;
+    VMAX = (float)m;
+  }
+  d.SC[SC_VRMS] = VRMS;
+  d.SC[SC_VMAX] = sqrtf(VMAX);
+}
+
+// per-J mass flow, reference order over I then K (exact) or FP64 (fast)
+__global__ void k_checkflow(Geo g, Dev d, int exact) {
+  const int J = blockIdx.x + 1;
+  if (J > g.JM) return;
+  float SUMAS;
+  if (exact) {
+    if (threadIdx.x != 0) return;
+    SUMAS = 0;
+    for (int I = 1; I <= g.IMM1; I++)
+      for (int K = 1; K <= g.KMM1; K++) SUMAS = SUMAS - d.FLOWX[IX(I, J, K)];
+  } else {
+    double s = 0.0;
+    for (int c = threadIdx.x; c < g.IMM1 * g.KMM1; c += 64) s -= (double)d.FLOWX[IX(c % g.IMM1 + 1, J, c / g.IMM1 + 1)];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x != 0) return;
+    SUMAS = (float)s;
+  }
+  const float NB = (float)d.NBLADE[J - 1];
+  d.BLADE_FLOW[J - 1] = SUMAS * NB;
+  d.CHECK_FLOW[J - 1] = (SUMAS + 0.0f) * NB + 0.0f - 0.0f;
+}
+
+__global__ void k_econt(Geo g, Dev d) {
+  if (threadIdx.x != 0) return;
+  float ECONT = 0.0f;
+  for (int J = 1; J <= g.JM; J++) {
+    float RATIO = d.CHECK_FLOW[J - 1] / d.CHECK_FLOW[0];
+    float EMASS = fabsf(1.f - RATIO);
+    if (EMASS > ECONT) ECONT = EMASS;
+  }
+  d.SC[SC_ECONT] = ECONT;
+  d.SC[SC_FLOWRAT] = d.BLADE_FLOW[0];
+  d.SC[SC_FLOWOUT] = d.BLADE_FLOW[g.JM - 1];
+}
+
+__device__ __forceinline__ float dvmer_at(const Geo &g, const Dev &d, int I, int J, int K, float RVEF) {
+  const size_t n = IX(I, J, K);
+  float XD = d.DX[JKX(J, K)], RD = d.DR[JKX(J, K)], SD = d.DS[JKX(J, K)];
+  float VM_START = (XD * d.VX[n] + RD * d.VR[n]) / SD;
+  float VM_END = (XD * d.ROVX[n] + RD * d.ROVR[n]) / SD / d.RO[n];
+  return (VM_END - VM_START) * RVEF;
+}
+
+// stage 1: per block FP64 sum of |DVMER| and the first max-|DVMER| node
+__global__ void k_emax_part(Geo g, Dev d, double *part) {
+  __shared__ double ssum[256];
+  __shared__ float sabs[256];
+  __shared__ long long sidx[256];
+  __shared__ float sval[256];
+  const int tid = threadIdx.x;
+  const float RIJKM = 2.f / (float)(g.IM * g.JM * g.KM);
+  const float VREF = sqrtf(d.SC[SC_VRMS] * RIJKM);
+  const float RVEF = 100.f / VREF;
+  const size_t N = (size_t)g.IM * g.JM * g.KM;
+  double s = 0.0;
+  float ba = -1.0f, bv = 0.0f;
+  long long bi = -1;
+  for (size_t n = (size_t)blockIdx.x * blockDim.x + tid; n < N; n += (size_t)gridDim.x * blockDim.x) {
+    const int I = (int)(n % g.IM) + 1, J = (int)((n / g.IM) % g.JM) + 1, K = (int)(n / ((size_t)g.IM * g.JM)) + 1;
+    if (J < 2) continue;
+    float v = dvmer_at(g, d, I, J, K, RVEF);
+    s += (double)fabsf(v);
+    if (fabsf(v) > ba) { ba = fabsf(v); bv = v; bi = (long long)n; }
+  }
+  ssum[tid] = s; sabs[tid] = ba; sval[tid] = bv; sidx[tid] = bi;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      ssum[tid] += ssum[tid + off];
+      const float a2 = sabs[tid + off];
+      const long long i2 = sidx[tid + off];
+      if (a2 > sabs[tid] || (a2 == sabs[tid] && i2 >= 0 && (sidx[tid] < 0 || i2 < sidx[tid]))) {
+        sabs[tid] = a2; sval[tid] = sval[tid + off]; sidx[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[4 * blockIdx.x] = ssum[0];
+    part[4 * blockIdx.x + 1] = (double)sabs[0];
+    part[4 * blockIdx.x + 2] = (double)sval[0];
+    part[4 * blockIdx.x + 3] = (double)sidx[0];
+  }
+}
+
+__global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int exact) {
+  if (threadIdx.x != 0) return;
+  const float RIJKM = 2.f / (float)(g.IM * g.JM * g.KM);
+  const float VREF = sqrtf(d.SC[SC_VRMS] * RIJKM);
+  const float RVEF = 100.f / VREF;
+  float EMAX = 0.0f, EAVG = 0.0f;
+  if (exact) {
+    for (int K = 1; K <= g.KM; K++)
+      for (int J = 2; J <= g.JM; J++)
+        for (int I = 1; I <= g.IM; I++) {
+          float DVMER = dvmer_at(g, d, I, J, K, RVEF);
+          EAVG = EAVG + fabsf(DVMER);
+          if (fabsf(DVMER) > fabsf(EMAX)) EMAX = DVMER;
+        }
+  } else {
+    double s = 0.0, ba = -1.0, bi = -1.0;
+    float bv = 0.0f;
+    for (int b = 0; b < nparts; b++) {
+      s += part[4 * b];
+      const double a = part[4 * b + 1], i = part[4 * b + 3];
+      if (i >= 0 && (a > ba || (a == ba && i < bi))) { ba = a; bi = i; bv = (float)part[4 * b + 2]; }
+    }
+    EAVG = (float)s;
+    EMAX = bv;
+  }
+  d.SC[SC_EAVG] = EAVG / (float)(g.IM * g.JMM1 * g.KM);
+  d.SC[SC_EMAX] = EMAX;
+  d.SC[SC_VREF] = VREF;
+}
+
+// ---------------------------------------------------------------------------
+// STEPUP, mf:7916-8002 (ideal gas, ITIMST<5)
+
+__global__ void k_stepup_temp(Geo g, Dev d, Params p) {
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  float TLIM = 0.1f * d.TO1[p.KMID - 1];
+  float VLIM = 0.01f * d.VM1[p.KMID - 1] * d.VM1[p.KMID - 1];
+  const float vx = d.VX[n], vt = d.VT[n], vr = d.VR[n];
+  float VSQ = vx * vx + vt * vt + vr * vr;
+  float HSTAT = d.HO[n] - 0.5f * VSQ;
+  float TSTATIC = HSTAT * p.RCP;
+  float GAMNOW = p.GA;
+  if (TSTATIC < TLIM) TSTATIC = TLIM;
+  float WTREL = d.WT[n];
+  float WSQ = VSQ - (vt * vt - WTREL * WTREL);
+  if (WSQ < VLIM) WSQ = VLIM;
+  float V_SONIC = sqrtf(GAMNOW * p.RGAS * TSTATIC);
+  float VPLUSC = sqrtf(WSQ) + V_SONIC;
+  d.TEMP1[n] = p.VSOUND / VPLUSC;
+}
+
+__global__ void k_stepup_step(Geo g, Dev d, float RELAX) {
+  NODE_IJK
+  if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
+  const int IP1 = I + 1, KP1 = K + 1;
+  const float RELAX1 = 1.0f - RELAX;
+  const float *T = d.TEMP1;
+  float AMACHP = T[IX(I, J - 1, K)] + T[IX(I, J - 1, KP1)] + T[IX(IP1, J - 1, K)] + T[IX(IP1, J - 1, KP1)];
+  if (J == 2) AMACHP = T[IX(I, 1, K)] + T[IX(IP1, 1, K)] + T[IX(I, 1, KP1)] + T[IX(IP1, 1, KP1)];
+  float AMACHL = T[IX(I, J, K)] + T[IX(I, J, KP1)] + T[IX(IP1, J, K)] + T[IX(IP1, J, KP1)];
+  float AVMACH = 0.125f * (AMACHL + AMACHP);
+  const size_t n = IX(I, J, K);
+  float STEPNEW = d.BSTEP[n] * AVMACH;
+  float st = RELAX * STEPNEW + RELAX1 * d.STEP[n];
+  d.STEP[n] = st;
+  d.RSTEP[n] = st / d.BSTEP[n];
+}
+
+__global__ void k_stepup_mix(Geo g, Dev d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = g.IMM1 * g.KMM1;
+  const int NR = t / per + 1, c = t % per;
+  if (NR > g.NROWS - 1) return;
+  const int I = c % g.IMM1 + 1, K = c / g.IMM1 + 1;
+  const int J = d.JMIX[NR - 1];
+  d.STEP[IX(I, J + 1, K)] = d.STEP[IX(I, J + 2, K)];
+  d.RSTEP[IX(I, J + 1, K)] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// mass-flux-weighted station integrals (EFICOOL mf:10685-10739 + SUMFLX
+// mf:7883-7910), FP64 per-J sums: one workgroup per J
+
+__global__ void k_integrals(Geo g, Dev d, Params p) {
+  const int J = blockIdx.x + 1;
+  if (J > g.JM) return;
+  __shared__ double sh[7][256];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const float TO1M = d.TO1[p.KMID - 1], PO1M = d.PO1[p.KMID - 1];
+  const float NB = (float)d.NBLADE[J - 1];
+  auto temps = [&](int I, int K, float &tstag, float &prat0, float &rvt, float &ent, float &ts, float &ps) {
+    const size_t n = IX(I, J, K);
+    float EKE = 0.5f * (d.VX[n] * d.VX[n] + d.VT[n] * d.VT[n] + d.VR[n] * d.VR[n]);
+    float TS = (d.HO[n] - EKE) * p.RCP;
+    float TSTAG = d.HO[n] * p.RCP;
+    if (TSTAG < 0.1f * TO1M) TSTAG = 0.1f * TO1M;
+    if (TS < 0.1f * TO1M) TS = 0.1f * TO1M;
+    float PSTAT = d.P[n];
+    if (PSTAT < 0.02f * PO1M) PSTAT = 0.02f * PO1M;
+    float PSTAG = PSTAT * powf(TSTAG / TS, 1.0f / p.FGA);
+    float TRAT = TS / TO1M, PRAT = PSTAT / PO1M, ENTRPY = 0.0f;
+    if (!((PRAT > 0.99999f && PRAT < 1.00001f) || (TRAT > 0.99999f && TRAT < 1.00001f)))
+      ENTRPY = p.CP * logf(TRAT) - p.RGAS * logf(PRAT);
+    tstag = TSTAG; prat0 = PSTAG / PO1M; rvt = d.RORVT[n] / d.RO[n]; ent = ENTRPY; ts = TS; ps = d.P[n];
+  };
+  for (int c = threadIdx.x; c < g.IMM1 * g.KMM1; c += blockDim.x) {
+    const int I = c % g.IMM1 + 1, K = c / g.IMM1 + 1;
+    double DFLOW = -(double)d.FLOWX[IX(I, J, K)] * (double)NB;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    const int ci[4] = {I, I + 1, I + 1, I}, ck[4] = {K, K, K + 1, K + 1};
+    for (int q = 0; q < 4; q++) {
+      float a, b, c2, e, f, h;
+      temps(ci[q], ck[q], a, b, c2, e, f, h);
+      s[0] += b; s[1] += a; s[2] += c2; s[3] += e; s[4] += f; s[5] += h;
+    }
+    acc[0] += DFLOW;
+    for (int q = 0; q < 6; q++) acc[q + 1] += 0.25 * DFLOW * s[q];
+  }
+  for (int q = 0; q < 7; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int q = 0; q < 7; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 7; q++) d.INTEG[(size_t)q * g.JM + J - 1] = sh[q][0];
+}
+
+}  // namespace mh

@@ -45,7 +45,7 @@ SCALARS_F = ["CFL", "DAMP", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "
 SCALARS_I = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
              "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS",
              "NMIXUP", "NIB1", "NJB1", "NKB1", "NIB2", "NJB2", "NKB2", "NSBLK", "ITIMST",
-             "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IBOUND"]
+             "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IBOUND", "IRBB", "JRBB", "KRBB"]
 
 
 def build() -> str:
@@ -126,8 +126,7 @@ class Oracle:
             if L.orc_bind(self.h, k.encode(), arr.ctypes.data) != 0:
                 raise KeyError(k)
         for k, v in state["scalars"].items():
-            if L.orc_scalar(self.h, k.encode(), float(v)) != 0:
-                raise KeyError(k)
+            L.orc_scalar(self.h, k.encode(), float(v))   # names the oracle does not use are ignored
 
     def start(self):
         rc = self.lib.orc_start(self.h)

@@ -1,0 +1,83 @@
+"""Generate the golden fixtures under tests/golden/ from the reference solver.
+
+Runs the reference built by oracle/build_ref.sh (this container only; the
+reference is not available where the tests run) on the synthetic decks and
+stores, per case:
+
+  <case>.dat          the NEW-format input deck
+  <case>_setup.npz    COMMON state after the reference's SETUP (dense arrays
+                      plus a JSON list of scalars): the input of every test
+  <case>_ref<N>.npz   conserved variables, SSS histories and time steps after
+                      N time steps of the reference (N = 1 and 100)
+
+Usage: python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "multall-open-18.3_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from multall_hip import deckgen  # noqa: E402
+
+OUT_VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP",
+            "XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
+
+CASES = {
+    "tiny1_inv": lambda n: deckgen.deck_tiny(ilos=0, nrows=1, nsteps=n),
+    "tiny3_inv": lambda n: deckgen.deck_tiny(ilos=0, nrows=3, nsteps=n),
+    "tiny3_ml": lambda n: deckgen.deck_tiny(ilos=10, nrows=3, nsteps=n),
+}
+STEPS = (1, 100)
+
+
+def save_state(path, st):
+    arrays = {k: v for k, v in st.items() if k != "scalars"}
+    np.savez_compressed(path, __scalars__=np.array(json.dumps(st["scalars"])), **arrays)
+
+
+def load_state(path):
+    z = np.load(path, allow_pickle=False)
+    st = {k: z[k] for k in z.files if k != "__scalars__"}
+    st["scalars"] = json.loads(str(z["__scalars__"]))
+    return st
+
+
+def make(case):
+    import refdump, orc
+    setup_ref = None
+    for n in STEPS:
+        spec = CASES[case](n)
+        txt = deckgen.write_deck(spec)
+        setup, final, _ = refdump.run_reference(txt, "fp32")
+        st = orc.from_dump(setup)
+        st["scalars"].pop("NSTEPS_MAX")
+        st.pop("NOUT")
+        if setup_ref is None:
+            setup_ref = st
+            with open(os.path.join(HERE, "decks", f"{case}.dat"), "w") as f:
+                f.write(deckgen.write_deck(CASES[case](max(STEPS))))
+            save_state(os.path.join(HERE, f"{case}_setup.npz"), st)
+        else:
+            for k, v in st.items():
+                if k == "scalars":
+                    assert v == setup_ref["scalars"], case
+                else:
+                    assert np.array_equal(v, setup_ref[k]), (case, k)
+        out = {k: final.node(k).astype(np.float32) for k in OUT_VARS}
+        out["NSTEP"] = np.array(int(final["NSTEP"]))
+        np.savez_compressed(os.path.join(HERE, f"{case}_ref{n}.npz"), **out)
+        print(f"{case}: {n} steps done")
+
+
+if __name__ == "__main__":
+    os.makedirs(os.path.join(HERE, "decks"), exist_ok=True)
+    for c in (sys.argv[1:] or list(CASES)):
+        make(c)
