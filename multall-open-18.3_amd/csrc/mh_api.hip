@@ -158,6 +158,19 @@ int dalloc(mh_ctx *c, T **p, size_t n) {
   return MH_OK;
 }
 
+// All host<->device traffic goes through the context's stream, so it is
+// ordered after the zero-fills and kernels already queued there.
+int h2d(mh_ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MH_OK;
+}
+int d2h(mh_ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MH_OK;
+}
+
 float sf(const mh_ctx *c, int id) { return (float)c->scal[id]; }
 int si(const mh_ctx *c, int id) { return (int)c->scal[id]; }
 
@@ -215,11 +228,12 @@ StepScalars step_scalars(const mh_ctx *c) {
 int build_lists(mh_ctx *c) {
   const mh_dims &dm = c->dims;
   std::vector<int> jb1(dm.jm), jb2(dm.jm), jsb(dm.jm), jstart(dm.nrows + 1), jmix(dm.nrows + 1);
-  HIPCHK(hipMemcpy(jb1.data(), c->d.JB1, dm.jm * 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(jb2.data(), c->d.JB2, dm.jm * 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(jsb.data(), c->d.JSBLK, dm.jm * 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(jstart.data(), c->d.JSTART, (dm.nrows + 1) * 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(jmix.data(), c->d.JMIX, (dm.nrows + 1) * 4, hipMemcpyDeviceToHost));
+  int r0;
+  if ((r0 = d2h(c, jb1.data(), c->d.JB1, dm.jm * 4)) || (r0 = d2h(c, jb2.data(), c->d.JB2, dm.jm * 4)) ||
+      (r0 = d2h(c, jsb.data(), c->d.JSBLK, dm.jm * 4)) ||
+      (r0 = d2h(c, jstart.data(), c->d.JSTART, (dm.nrows + 1) * 4)) ||
+      (r0 = d2h(c, jmix.data(), c->d.JMIX, (dm.nrows + 1) * 4)))
+    return r0;
   auto csr = [&](const std::vector<int> &map, int nb, std::vector<int> &ptr, std::vector<int> &list) -> bool {
     ptr.assign(nb + 1, 0);
     list.clear();
@@ -247,8 +261,7 @@ int build_lists(mh_ctx *c) {
   auto up = [&](std::vector<int> &v, int **dst) -> int {
     int r = dalloc(c, dst, v.size() + 1);
     if (r) return r;
-    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * 4, hipMemcpyHostToDevice));
-    return MH_OK;
+    return h2d(c, *dst, v.data(), v.size() * 4);
   };
   Lists &L = c->lists;
   int r;
@@ -533,12 +546,9 @@ static int copy_array(mh_ctx *c, int id, void *host, long long n, int ld_i, int 
   const long long len = kind_len(dm, k);
   char *dev = (char *)*slot_of(c->d, id);
   const hipMemcpyKind dir = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
-  HIPCHK(hipStreamSynchronize(c->stream));
   if (ld_i <= 0 && ld_j <= 0) {  // dense
     if (n != len) return fail(c, MH_E_ARG, std::string("size mismatch for ") + kArrays[id].name);
-    if (to_dev) HIPCHK(hipMemcpy(dev, host, (size_t)len * 4, dir));
-    else HIPCHK(hipMemcpy(host, dev, (size_t)len * 4, dir));
-    return MH_OK;
+    return to_dev ? h2d(c, dev, host, (size_t)len * 4) : d2h(c, host, dev, (size_t)len * 4);
   }
   // padded FORTRAN array: rows of the first logical extent at pitch ld_i
   size_t w = 0, h = 0, planes = 1, spitch = 0, splane = 0;
@@ -554,9 +564,10 @@ static int copy_array(mh_ctx *c, int id, void *host, long long n, int ld_i, int 
   for (size_t pz = 0; pz < planes; pz++) {
     char *dptr = dev + pz * w * h * 4;
     char *hptr = (char *)host + pz * splane * 4;
-    if (to_dev) HIPCHK(hipMemcpy2D(dptr, w * 4, hptr, spitch * 4, w * 4, h, dir));
-    else HIPCHK(hipMemcpy2D(hptr, spitch * 4, dptr, w * 4, w * 4, h, dir));
+    if (to_dev) HIPCHK(hipMemcpy2DAsync(dptr, w * 4, hptr, spitch * 4, w * 4, h, dir, c->stream));
+    else HIPCHK(hipMemcpy2DAsync(hptr, spitch * 4, dptr, w * 4, w * 4, h, dir, c->stream));
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
   return MH_OK;
 }
 
@@ -605,18 +616,17 @@ int mh_start(mh_ctx *c) {
   float pa = 0, pb = 0, ra = 0, rb = 0;
   const size_t nA = (size_t)(p.IMID - 1) + (size_t)g.IM * (size_t)g.JM * (p.KMID - 1);
   const size_t nB = nA + (size_t)g.IM * (g.JM - 1);
-  HIPCHK(hipMemcpy(po1.data(), c->d.PO1, g.KM * 4, hipMemcpyDeviceToHost));
+  if ((r = d2h(c, po1.data(), c->d.PO1, g.KM * 4))) return r;
   for (int K = 1; K <= g.KM; K++)
-    HIPCHK(hipMemcpy(&p1[K - 1], c->d.P + (size_t)g.IM * g.JM * (K - 1), 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(&pa, c->d.P + nA, 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(&pb, c->d.P + nB, 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(&ra, c->d.RO + nA, 4, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(&rb, c->d.RO + nB, 4, hipMemcpyDeviceToHost));
+    if ((r = d2h(c, &p1[K - 1], c->d.P + (size_t)g.IM * g.JM * (K - 1), 4))) return r;
+  if ((r = d2h(c, &pa, c->d.P + nA, 4)) || (r = d2h(c, &pb, c->d.P + nB, 4)) || (r = d2h(c, &ra, c->d.RO + nA, 4)) ||
+      (r = d2h(c, &rb, c->d.RO + nB, 4)))
+    return r;
   c->ROMIN = fminf(ra, rb);
   c->ROLIM = 0.25f * c->ROMIN;
   std::vector<float> plim(g.KM);
   for (int K = 1; K <= g.KM; K++) plim[K - 1] = 0.9f * po1[K - 1] + 0.1f * p1[K - 1];
-  HIPCHK(hipMemcpy(c->d.SC + SC_PLIM, plim.data(), g.KM * 4, hipMemcpyHostToDevice));
+  if ((r = h2d(c, c->d.SC + SC_PLIM, plim.data(), g.KM * 4))) return r;
   c->PREFF = 0.5f * (pa + pb);
   c->SFXIN_s = sf(c, MH_S_SFXIN) * sf(c, MH_S_CFL) / 0.5f;
   c->SFTIN_s = sf(c, MH_S_SFTIN) * sf(c, MH_S_CFL) / 0.5f;
