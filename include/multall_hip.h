@@ -143,6 +143,13 @@ int mh_integrals(mh_ctx *ctx, double *out, long long n);
 /* Wait for all queued work; returns MH_E_HIP on an asynchronous error. */
 int mh_sync(mh_ctx *ctx);
 
+/* Kernel timing (no counterpart in the reference, whose only timing is the
+ * MCLOCK CPU time per step, mf:5404-5414).  When enabled, every kernel launch
+ * is bracketed by HIP events recorded on the context's stream.  The report
+ * holds one line per kernel: "<name> <launches> <total milliseconds>". */
+int mh_profile_enable(mh_ctx *ctx, int on);
+int mh_profile_report(mh_ctx *ctx, char *buf, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -203,6 +203,64 @@ namespace {
 dim3 node_grid(const Geo &g) { return dim3((g.IM + 63) / 64, (g.JM + 3) / 4, g.KM); }
 const dim3 kNodeBlock(64, 4, 1);
 
+hipEvent_t prof_event(mh_ctx *c) {
+  if (!c->prof_pool.empty()) {
+    hipEvent_t e = c->prof_pool.back();
+    c->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one kernel launch with events on the context stream when
+// profiling is on (mh_profile_enable).
+struct ProfScope {
+  mh_ctx *c;
+  mh_prof_rec rec{};
+  ProfScope(mh_ctx *c_, const char *name) : c(c_) {
+    if (!c->prof_on) return;
+    rec.name = name;
+    rec.t0 = prof_event(c);
+    rec.t1 = prof_event(c);
+    if (rec.t0) (void)hipEventRecord(rec.t0, c->stream);
+  }
+  ~ProfScope() {
+    if (!c->prof_on || !rec.t0 || !rec.t1) return;
+    (void)hipEventRecord(rec.t1, c->stream);
+    c->prof_pending.push_back(rec);
+  }
+};
+#define LAUNCH(kern, ...)                   \
+  do {                                      \
+    ProfScope ps_(c, #kern);                \
+    hipLaunchKernelGGL(kern, __VA_ARGS__);  \
+  } while (0)
+
+// Reduce finished launch timings into the per-kernel totals.
+int prof_collect(mh_ctx *c) {
+  if (c->prof_pending.empty()) return MH_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto &r : c->prof_pending) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, r.t0, r.t1));
+    bool found = false;
+    for (auto &t : c->prof_tot)
+      if (t.first == r.name) {
+        t.second.first += 1;
+        t.second.second += ms;
+        found = true;
+        break;
+      }
+    if (!found) c->prof_tot.push_back({r.name, {1, (double)ms}});
+    c->prof_pool.push_back(r.t0);
+    c->prof_pool.push_back(r.t1);
+  }
+  c->prof_pending.clear();
+  return MH_OK;
+}
+
 int last_launch(mh_ctx *c, const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, MH_E_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e));
@@ -280,47 +338,47 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   const dim3 grid = node_grid(g);
   const int nmp = (g.NROWS - 1) * g.KMM1;
   if (!(p.FEXTRAP < 0.001f) && nmp > 0) {
-    hipLaunchKernelGGL(k_fextrap, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p);
+    LAUNCH(k_fextrap, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p);
     LCHK("fextrap");
   }
-  hipLaunchKernelGGL(k_residual, grid, kNodeBlock, 0, c->stream, g, d, D, DIFF, p);
+  LAUNCH(k_residual, grid, kNodeBlock, 0, c->stream, g, d, D, DIFF, p);
   LCHK("residual");
   if (nmp > 0) {
-    hipLaunchKernelGGL(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 1);
+    LAUNCH(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 1);
     LCHK("mpavg1");
   }
   const bool mg = !(p.IR <= 1 && p.JR <= 1 && p.KR <= 1);
   const Lists &L = c->lists;
   if (mg) {
     const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
-    hipLaunchKernelGGL(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list,
+    LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list,
                        g.NIB1, g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
     LCHK("mg1");
     if (p.EXACT) {
-      hipLaunchKernelGGL(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list,
+      LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list,
                          g.NIB2, g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
       LCHK("mg2s");
-      hipLaunchKernelGGL(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr,
+      LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr,
                          L.sb_list, g.NSBLK, d.SBCHG, 0, 0);
       LCHK("sbs");
     } else {
-      hipLaunchKernelGGL(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
+      LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
                          g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
       LCHK("mg2w");
-      hipLaunchKernelGGL(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 0);
+      LAUNCH(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 0);
       LCHK("lines");
-      hipLaunchKernelGGL(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
+      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
                          L.sb_list, g.NSBLK, d.SBCHG, 0);
       LCHK("sbw");
     }
   }
-  hipLaunchKernelGGL(k_mg_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_mg_apply, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("mgapply");
   if (p.NRSMTH > 0) {
     for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
-      hipLaunchKernelGGL(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0);
-      hipLaunchKernelGGL(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.TMP, d.STORE, p.RSMTH, 1);
-      hipLaunchKernelGGL(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 2);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.TMP, d.STORE, p.RSMTH, 1);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 2);
       std::swap(c->d.STORE, c->d.TMP);
       d = c->d;
     }
@@ -328,24 +386,24 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   }
   if (!(s.DAMP < 2.0f || s.DAMP > 100)) {
     if (p.EXACT) {
-      hipLaunchKernelGGL(k_range_sums_seq, dim3(1), dim3(64), 0, c->stream, g, d, L.row_ptr, L.row_list, g.NROWS,
+      LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, c->stream, g, d, L.row_ptr, L.row_list, g.NROWS,
                          d.SC + SC_AVG_CHG, 1, 1);
     } else {
-      hipLaunchKernelGGL(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 1);
-      hipLaunchKernelGGL(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
+      LAUNCH(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 1);
+      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
                          L.row_list, g.NROWS, d.SC + SC_AVG_CHG, 1);
     }
-    hipLaunchKernelGGL(k_avg_blk, dim3(1), dim3(64), 0, c->stream, g, d);
-    hipLaunchKernelGGL(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
+    LAUNCH(k_avg_blk, dim3(1), dim3(64), 0, c->stream, g, d);
+    LAUNCH(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
     LCHK("damp");
   }
   if (nmp > 0) {
-    hipLaunchKernelGGL(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 2);
+    LAUNCH(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 2);
     LCHK("mpavg2");
   }
-  hipLaunchKernelGGL(k_scatter, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.DSC);
+  LAUNCH(k_scatter, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.DSC);
   LCHK("scatter");
-  hipLaunchKernelGGL(k_smooth_plane, dim3(g.JM), dim3(256), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
+  LAUNCH(k_smooth_plane, dim3(g.JM), dim3(256), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
                      s, ncall);
   LCHK("smooth_plane");
   return MH_OK;
@@ -355,10 +413,10 @@ int stepup(mh_ctx *c, float relax) {
   const Geo &g = c->g;
   const Dev &d = c->d;
   const Params p = params_of(c);
-  hipLaunchKernelGGL(k_stepup_temp, node_grid(g), kNodeBlock, 0, c->stream, g, d, p);
-  hipLaunchKernelGGL(k_stepup_step, node_grid(g), kNodeBlock, 0, c->stream, g, d, relax);
+  LAUNCH(k_stepup_temp, node_grid(g), kNodeBlock, 0, c->stream, g, d, p);
+  LAUNCH(k_stepup_step, node_grid(g), kNodeBlock, 0, c->stream, g, d, relax);
   const int nmix = (g.NROWS - 1) * g.IMM1 * g.KMM1;
-  if (nmix > 0) hipLaunchKernelGGL(k_stepup_mix, dim3((nmix + 255) / 256), dim3(256), 0, c->stream, g, d);
+  if (nmix > 0) LAUNCH(k_stepup_mix, dim3((nmix + 255) / 256), dim3(256), 0, c->stream, g, d);
   LCHK("stepup");
   return MH_OK;
 }
@@ -387,50 +445,50 @@ int step_once(mh_ctx *c) {
   Dev d = c->d;
   const StepScalars s = step_scalars(c);
 
-  hipLaunchKernelGGL(k_prim, grid, kNodeBlock, 0, c->stream, g, d, p);
+  LAUNCH(k_prim, grid, kNodeBlock, 0, c->stream, g, d, p);
   LCHK("prim");
   if (check5) {
-    hipLaunchKernelGGL(k_vrms_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
-    hipLaunchKernelGGL(k_vrms_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LAUNCH(k_vrms_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
+    LAUNCH(k_vrms_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("vrms");
   }
-  hipLaunchKernelGGL(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
   if ((r = tstep(c, d.RO, d.DRO, 1))) return r;
-  hipLaunchKernelGGL(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_energy");
   if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
-  hipLaunchKernelGGL(k_bc, dim3(1), dim3(256), 4 * g.KM * sizeof(float), c->stream, g, d, p, s);
+  LAUNCH(k_bc, dim3(1), dim3(256), 4 * g.KM * sizeof(float), c->stream, g, d, p, s);
   LCHK("bc");
-  hipLaunchKernelGGL(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
-  hipLaunchKernelGGL(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
+  LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_axial");
   if ((r = tstep(c, d.ROVX, d.DROVX, 4))) return r;
-  hipLaunchKernelGGL(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_angular");
   if ((r = tstep(c, d.RORVT, d.DRORVT, 5))) return r;
-  hipLaunchKernelGGL(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_radial");
   if ((r = tstep(c, d.ROVR, d.DROVR, 6))) return r;
   // limiters, mf:5115-5168
   d = c->d;
   const int nline = g.IM * g.KM;
-  hipLaunchKernelGGL(k_fill_int, dim3((nline + 255) / 256), dim3(256), 0, c->stream, d.FLAGS, nline, 0);
-  hipLaunchKernelGGL(k_rolim_check, grid, kNodeBlock, 0, c->stream, g, d, s);
-  hipLaunchKernelGGL(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s);
-  hipLaunchKernelGGL(k_machref, dim3(1), dim3(64), 0, c->stream, g, d, p, s);
-  hipLaunchKernelGGL(k_mach_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_fill_int, dim3((nline + 255) / 256), dim3(256), 0, c->stream, d.FLAGS, nline, 0);
+  LAUNCH(k_rolim_check, grid, kNodeBlock, 0, c->stream, g, d, s);
+  LAUNCH(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s);
+  LAUNCH(k_machref, dim3(1), dim3(64), 0, c->stream, g, d, p, s);
+  LAUNCH(k_mach_apply, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("limiters");
   if (g.IM > 2 && p.IFMIX > 0 && p.RFMIX > 1.0e-3f && g.NROWS > 1) {
     const int nl = (g.NROWS - 1) * g.KM;
-    hipLaunchKernelGGL(k_mixplan, dim3((nl + 63) / 64), dim3(64), 0, c->stream, g, d, p, c->lists.mixscratch);
+    LAUNCH(k_mixplan, dim3((nl + 63) / 64), dim3(64), 0, c->stream, g, d, p, c->lists.mixscratch);
     LCHK("mixplan");
   }
   if (check5) {
-    hipLaunchKernelGGL(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, d, p.EXACT);
-    hipLaunchKernelGGL(k_econt, dim3(1), dim3(64), 0, c->stream, g, d);
-    hipLaunchKernelGGL(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
-    hipLaunchKernelGGL(k_emax_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LAUNCH(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, d, p.EXACT);
+    LAUNCH(k_econt, dim3(1), dim3(64), 0, c->stream, g, d);
+    LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
+    LAUNCH(k_emax_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("convergence");
     if ((r = stepup(c, 0.25f))) return r;
     float sc[8];
@@ -533,6 +591,11 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
 int mh_destroy(mh_ctx *c) {
   if (!c) return MH_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &r : c->prof_pending) {
+    (void)hipEventDestroy(r.t0);
+    (void)hipEventDestroy(r.t1);
+  }
+  for (auto e : c->prof_pool) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -646,6 +709,7 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
   for (int i = 0; i < nsteps && !c->ifend; i++) {
     int r = step_once(c);
     if (r) return r;
+    if (c->prof_on && c->prof_pending.size() > 4096 && (r = prof_collect(c))) return r;
   }
   if (out) *out = c->last;
   return MH_OK;
@@ -654,11 +718,36 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
 int mh_integrals(mh_ctx *c, double *out, long long n) {
   if (!c || !out || n != 7LL * c->g.JM) return fail(c, MH_E_ARG, "integrals: need 7*JM doubles");
   const Params p = params_of(c);
-  hipLaunchKernelGGL(k_integrals, dim3(c->g.JM), dim3(256), 0, c->stream, c->g, c->d, p);
+  LAUNCH(k_integrals, dim3(c->g.JM), dim3(256), 0, c->stream, c->g, c->d, p);
   LCHK("integrals");
   HIPCHK(hipMemcpyAsync(out, c->d.INTEG, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MH_OK;
+}
+
+int mh_profile_enable(mh_ctx *c, int on) {
+  if (!c) return MH_E_ARG;
+  int r = prof_collect(c);
+  if (r) return r;
+  c->prof_on = on != 0;
+  c->prof_tot.clear();
+  return MH_OK;
+}
+
+int mh_profile_report(mh_ctx *c, char *buf, int cap) {
+  if (!c || !buf || cap <= 0) return MH_E_ARG;
+  int r = prof_collect(c);
+  if (r) return r;
+  std::string out;
+  for (auto &t : c->prof_tot) {
+    char line[160];
+    snprintf(line, sizeof line, "%s %lld %.6f\n", t.first.c_str(), t.second.first, t.second.second);
+    out += line;
+  }
+  int n = (int)std::min<size_t>(out.size(), (size_t)cap - 1);
+  memcpy(buf, out.data(), n);
+  buf[n] = 0;
+  return n;
 }
 
 int mh_sync(mh_ctx *c) {

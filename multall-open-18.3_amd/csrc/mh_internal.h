@@ -97,7 +97,17 @@ struct Lists {
 
 }  // namespace mh
 
+// One timed launch: HIP events recorded on the context's stream around it.
+struct mh_prof_rec {
+  const char *name;
+  hipEvent_t t0, t1;
+};
+
 struct mh_ctx {
+  bool prof_on = false;
+  std::vector<mh_prof_rec> prof_pending;      // launches not yet reduced
+  std::vector<hipEvent_t> prof_pool;          // recycled events
+  std::vector<std::pair<std::string, std::pair<long long, double>>> prof_tot;  // name -> (calls, ms)
   mh::Lists lists{};
   size_t lds_plane = 0;
   mh_dims dims{};

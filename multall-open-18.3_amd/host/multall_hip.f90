@@ -1,0 +1,190 @@
+!> Driver program: the reference's main program (multall-open-18.3.f:89-142)
+!> with `CALL LOOP` replaced by the MI355X library.  Reads a NEW-format deck
+!> on standard input, builds the grid with mh_host, uploads it, and marches
+!> in chunks of 5 steps -- writing the reference's convergence line to
+!> stage.log (mf:5437-5438) and polling the `stopit` file every 10 steps
+!> (mf:3774-3779) -- until NSTEPS_MAX or convergence.  Writes `flow_out` in
+!> the reference's record layout (mf:5951-5976) at the end.
+program multall_hip
+  use iso_c_binding
+  use mh_host
+  use mh_hipif
+  implicit none
+  type(mh_dims) :: dims
+  type(mh_resid) :: res
+  type(c_ptr) :: ctx
+  integer :: rc, ifstop, ios, nstep_done
+  character(len=1) :: ansin
+  real, allocatable, target :: buf(:,:,:)
+
+  open (unit=13, file='intype', status='old', iostat=ios)
+  if (ios == 0) then
+    read (13, *, iostat=ios) ansin
+    close (13)
+    if (ansin == 'O' .or. ansin == 'o') then
+      write (*, *) ' OLD_READIN decks are not supported; use the NEW format.'
+      stop 2
+    end if
+  end if
+  open (unit=12, file='stopit')
+  write (12, *) 0
+  close (12)
+  if (mh_host_load_unit(5)) then
+    write (*, *) ' DECK/SETUP FAILED: ', trim(mh_host_err)
+    stop 2
+  end if
+  write (*, '(a,3i6,a,i3)') ' GRID IM,JM,KM =', IM, JM, KM, '   ROWS =', NROWS
+  dims = mh_dims(IM, JM, KM, NROWS, NIB1, NJB1, NKB1, NIB2, NJB2, NKB2, NSBLK)
+  rc = mh_create(dims, ctx)
+  call check(rc, 'mh_create')
+  call upload()
+  rc = mh_start(ctx)
+  call check(rc, 'mh_start')
+  open (unit=4, file='stage.log')
+  write (*, *) ' STARTING THE MAIN TIME STEPPING LOOP ON THE GPU'
+  nstep_done = 0
+  do
+    rc = mh_step(ctx, 5, res)
+    call check(rc, 'mh_step')
+    if (res%nstep == nstep_done) exit
+    nstep_done = res%nstep
+    write (4, '(a,3e12.5,a,e12.5,a,i5)') 'EMAX,EAVG,ECONT', res%emax, res%eavg, res%econt, ' FLOW', res%flow_in, &
+      'STEP', res%nstep
+    if (mod(res%nstep, 50) == 0 .or. res%ifend == 1) &
+      write (*, '(i6,3f9.4,2f10.2,2f12.4)') res%nstep, res%emax, res%eavg, res%econt, res%vref, res%vmax, &
+      res%flow_in, res%flow_out
+    if (res%ifend == 1) exit
+    if (mod(res%nstep, 10) == 0) then
+      open (unit=12, file='stopit')
+      read (12, *, iostat=ios) ifstop
+      close (12)
+      if (ios == 0 .and. ifstop >= 1) then
+        rc = mh_set_scalar(ctx, mh_scalar_id(cname('NSTEPS_MAX')), dble(res%nstep + 1))
+      end if
+    end if
+  end do
+  close (4)
+  call write_flow_out()
+  rc = mh_destroy(ctx)
+  write (*, *) ' STOPPED AFTER ', res%nstep, ' STEPS'
+
+contains
+
+  subroutine check(r, what)
+    integer, intent(in) :: r
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (r == 0) return
+    call c_f_pointer(mh_last_error(ctx), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write (*, *) ' LIBRARY ERROR in ', what, ' rc=', r, ': ', msg(1:n)
+    stop 3
+  end subroutine check
+
+  subroutine put3(name, a)
+    character(len=*), intent(in) :: name
+    real, intent(in), target, contiguous :: a(:,:,:)
+    call check(mh_set_array(ctx, mh_array_id(cname(name)), c_loc(a), int(size(a), c_long_long)), name)
+  end subroutine put3
+  subroutine put2(name, a)
+    character(len=*), intent(in) :: name
+    real, intent(in), target, contiguous :: a(:,:)
+    call check(mh_set_array(ctx, mh_array_id(cname(name)), c_loc(a), int(size(a), c_long_long)), name)
+  end subroutine put2
+  subroutine put1(name, a)
+    character(len=*), intent(in) :: name
+    real, intent(in), target, contiguous :: a(:)
+    call check(mh_set_array(ctx, mh_array_id(cname(name)), c_loc(a), int(size(a), c_long_long)), name)
+  end subroutine put1
+  subroutine puti(name, a)
+    character(len=*), intent(in) :: name
+    integer, intent(in), target, contiguous :: a(:)
+    call check(mh_set_array(ctx, mh_array_id(cname(name)), c_loc(a), int(size(a), c_long_long)), name)
+  end subroutine puti
+  subroutine sets(name, v)
+    character(len=*), intent(in) :: name
+    real(c_double), intent(in) :: v
+    call check(mh_set_scalar(ctx, mh_scalar_id(cname(name)), v), name)
+  end subroutine sets
+
+  subroutine upload()
+    call put3('RO', RO); call put3('ROVX', ROVX); call put3('ROVR', ROVR); call put3('RORVT', RORVT)
+    call put3('ROE', ROE); call put3('DRO', DRO); call put3('DROVX', DROVX); call put3('DROVR', DROVR)
+    call put3('DRORVT', DRORVT); call put3('DROE', DROE); call put3('ROVT', ROVT); call put3('ROWT', ROWT)
+    call put3('P', P); call put3('HO', HO); call put3('PEFF', PEFF); call put3('VX', VX); call put3('VR', VR)
+    call put3('VT', VT); call put3('WT', WT); call put3('T_STATIC', T_STATIC); call put3('AQX', AQX)
+    call put3('AQR', AQR); call put3('ABX', ABX); call put3('ABR', ABR); call put3('ASX', ASX)
+    call put3('ASR', ASR); call put3('VOL', VOL); call put3('VOLOR', VOLOR); call put3('STEP', STEP)
+    call put3('BSTEP', BSTEP); call put3('RSTEP', RSTEP); call put3('XFORCE', XFORCE)
+    call put3('RFORCE', RFORCE); call put3('TFORCE', TFORCE); call put3('QSOURCE', QSOURCE)
+    call put3('FLOWX', FLOWX); call put3('FLOWT', FLOWT); call put3('FLOWR', FLOWR)
+    call put2('R', R); call put2('X', X); call put2('RAVG_CELL', RAVG_CELL); call put2('ABT', ABT)
+    call put2('WRABT', WRABT); call put2('UBLADE', UBLADE); call put2('DX', DX); call put2('DR', DR)
+    call put2('DS', DS); call put2('SMERID', SMERID); call put2('FTL', FTL); call put2('FBL', FBL)
+    call put2('FTR', FTR); call put2('FBR', FBR)
+    call put1('FP', FP(1:IM)); call put1('FU', FU(1:IM)); call put1('FD', FD(1:IM))
+    call put1('FR', FR(1:KM)); call put1('FKU', FKU(1:KM)); call put1('FKD', FKD(1:KM))
+    call put1('PO1', PO1(1:KM)); call put1('TO1', TO1(1:KM)); call put1('RPO1', RPO1(1:KM))
+    call put1('BSSIN', BSSIN(1:KM)); call put1('BSCOS', BSCOS(1:KM)); call put1('BRSIN', BRSIN(1:KM))
+    call put1('BRCOS', BRCOS(1:KM)); call put1('VTIN', VTIN(1:KM)); call put1('PD', PD(1:KM))
+    call put1('VM1', VM1(1:KM)); call put1('FACUP', FACUP); call put1('FACDWN', FACDWN)
+    call put1('WRAD', WRAD); call put3('STEP1', STEP1); call put3('STEP2', STEP2); call put1('STEPSBK', STEPSBK)
+    call puti('IB1', IB1); call puti('IB2', IB2); call puti('KB1', KB1); call puti('KB2', KB2)
+    call puti('NBLADE', NBLADE); call puti('IND', IND); call puti('INDLE', INDLE); call puti('INDMIX', INDMIX)
+    call puti('INDTE', INDTE); call puti('NROW', NROW); call puti('JSBLK', JSBLK); call puti('JB1', JB1)
+    call puti('JB2', JB2); call puti('JSTART', JSTART(1:NROWS + 1)); call puti('JMIX', JMIX(1:NROWS + 1))
+    call puti('JLE', JLE(1:NROWS + 1)); call puti('JTE', JTE(1:NROWS + 1)); call puti('JLED', JLED(1:NROWS + 1))
+    call puti('KTIPS', KTIPS(1:NROWS + 1)); call puti('KTIPE', KTIPE(1:NROWS + 1))
+    call puti('KS1', KS1(1:NROWS + 1)); call puti('KS2', KS2(1:NROWS + 1))
+    call sets('CFL', dble(CFL)); call sets('DAMPIN', dble(DAMPIN)); call sets('MACHLIM', dble(MACHLIM))
+    call sets('F_PDOWN', dble(F_PDOWN)); call sets('CONLIM', dble(CONLIM)); call sets('SFXIN', dble(SFXIN))
+    call sets('SFTIN', dble(SFTIN)); call sets('FAC_4TH', dble(FAC_4TH)); call sets('F1', dble(F1))
+    call sets('F2', dble(F2)); call sets('F3', dble(F3)); call sets('RSMTH', dble(RSMTH))
+    call sets('SFEXIT', dble(SFEXIT)); call sets('FSMTHB', dble(FSMTHB)); call sets('RFIN', dble(RFIN))
+    call sets('PDOWN_HUB', dble(PDOWN_HUB)); call sets('PDOWN_TIP', dble(PDOWN_TIP))
+    call sets('FEXTRAP', dble(FEXTRAP)); call sets('RFMIX', dble(RFMIX)); call sets('FANGLE', dble(FANGLE))
+    call sets('CP', dble(CP)); call sets('GA', dble(GA)); call sets('RGAS', dble(RGAS)); call sets('CV', dble(CV))
+    call sets('FGA', dble(FGA)); call sets('GA1', dble(GA1)); call sets('RCP', dble(RCP))
+    call sets('VSOUND', dble(VSOUND)); call sets('FACMIXUP', dble(FACMIXUP)); call sets('IMID', dble(IMID))
+    call sets('KMID', dble(KMID)); call sets('NSTEPS_MAX', dble(NSTEPS_MAX)); call sets('NCHANGE', dble(NCHANGE))
+    call sets('NRSMTH', dble(NRSMTH)); call sets('NSFEXIT', dble(NSFEXIT)); call sets('IN_PRESS', dble(IN_PRESS))
+    call sets('IN_VTAN', dble(IN_VTAN)); call sets('IN_VR', dble(IN_VR)); call sets('IPOUT', dble(IPOUT))
+    call sets('IFMIX', dble(IFMIX)); call sets('IR', dble(IR)); call sets('JR', dble(JR)); call sets('KR', dble(KR))
+    call sets('ILOS', dble(ILOS)); call sets('NLOS', dble(NLOS)); call sets('NMIXUP', dble(NMIXUP))
+    call sets('ITIMST', dble(ITIMST)); call sets('IF_RESTART', dble(IF_RESTART)); call sets('IFGAS', dble(IFGAS))
+    call sets('IN_FLOW', dble(IN_FLOW)); call sets('IF_REPEAT', dble(IF_REPEAT)); call sets('IRBB', dble(IRBB))
+    call sets('KRBB', dble(KRBB))
+  end subroutine upload
+
+  subroutine get3(name, a)
+    character(len=*), intent(in) :: name
+    real, intent(inout), target, contiguous :: a(:,:,:)
+    call check(mh_get_array(ctx, mh_array_id(cname(name)), c_loc(a), int(size(a), c_long_long)), name)
+  end subroutine get3
+
+  ! flow_out, mf:5951-5976 (ITIMST<5: ROE as marched, ROSUB unset = 0)
+  subroutine write_flow_out()
+    integer :: u
+    call get3('RO', RO); call get3('ROVX', ROVX); call get3('ROVR', ROVR); call get3('ROVT', ROVT)
+    call get3('ROE', ROE); call get3('PEFF', PEFF)
+    open (newunit=u, file='flow_out', form='unformatted')
+    write (u) res%nstep
+    write (u) RO
+    write (u) ROVX
+    write (u) ROVR
+    write (u) ROVT
+    write (u) ROE
+    allocate (buf(IM, JM, KM))
+    buf = 0.0
+    write (u) buf
+    write (u) buf
+    write (u) PEFF + PREFF
+    close (u)
+  end subroutine write_flow_out
+
+end program multall_hip

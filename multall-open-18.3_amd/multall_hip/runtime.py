@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmultall_hip.so")
 EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar_id", "mh_array_len",
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
            "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_start", "mh_step",
-           "mh_integrals", "mh_sync"]
+           "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report"]
 
 
 class MhDims(ctypes.Structure):
@@ -73,6 +73,8 @@ def load(path: str = LIB_PATH):
     L.mh_step.argtypes = [vp, i, ctypes.POINTER(MhResid)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
     L.mh_sync.argtypes = [vp]
+    L.mh_profile_enable.argtypes = [vp, i]
+    L.mh_profile_report.argtypes = [vp, ctypes.c_char_p, i]
     _lib = L
     return L
 
@@ -147,6 +149,21 @@ class Solver:
         self._check(self.L.mh_integrals(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 7 * jm),
                     "mh_integrals")
         return out.reshape(7, jm)
+
+    def profile(self, on: bool):
+        self._check(self.L.mh_profile_enable(self.h, 1 if on else 0), "mh_profile_enable")
+
+    def profile_report(self) -> Dict[str, tuple]:
+        """{kernel: (launches, total_ms)} since profiling was enabled."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        n = self.L.mh_profile_report(self.h, buf, len(buf))
+        if n < 0:
+            self._check(n, "mh_profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, calls, ms = line.split()
+            out[name] = (int(calls), float(ms))
+        return out
 
     def sync(self):
         self._check(self.L.mh_sync(self.h), "mh_sync")

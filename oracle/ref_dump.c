@@ -7,6 +7,7 @@
  *   "MHDUMP1\0"  int32 real_size  int32 nblocks  double loop_seconds
  *   per block:   char name[16]  int64 nbytes  bytes...
  *
+ * With MH_NODUMP set in the environment only the header is written.
  * loop_seconds is the wall time from the MHATEXIT call (just before LOOP)
  * to the exit hook, i.e. the reference's own time-marching loop.
  */
@@ -50,12 +51,13 @@ void mhblk_(const char *name, const int64_t *first, const int64_t *end, long len
 static void dump_to(const char *path, double secs) {
   FILE *f = fopen(path, "wb");
   if (!f) { perror(path); exit(3); }
-  int32_t n = nblk;
+  /* MH_NODUMP=1: timing runs keep only the header (loop_seconds) */
+  int32_t n = getenv("MH_NODUMP") ? 0 : nblk;
   fwrite("MHDUMP1", 1, 8, f);
   fwrite(&real_size, 4, 1, f);
   fwrite(&n, 4, 1, f);
   fwrite(&secs, 8, 1, f);
-  for (int i = 0; i < nblk; i++) {
+  for (int i = 0; i < n; i++) {
     fwrite(blk[i].name, 1, 16, f);
     fwrite(&blk[i].nbytes, 8, 1, f);
     fwrite(blk[i].base, 1, (size_t)blk[i].nbytes, f);
