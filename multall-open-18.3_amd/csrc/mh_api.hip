@@ -35,15 +35,13 @@ __global__ void k_range_sums(Geo, const double *, const int *, const int *, int,
 __global__ void k_range_sums_seq(Geo, Dev, const int *, const int *, int, float *, int, int);
 __global__ void k_mg_apply(Geo, Dev);
 __global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int);
-__global__ void k_avg_blk(Geo, Dev);
 __global__ void k_damp_apply(Geo, Dev, float);
 __global__ void k_scatter(Geo, Dev, const float *, float *);
 __global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int);
 __global__ void k_fill_int(int *, int, int);
 __global__ void k_rolim_check(Geo, Dev, StepScalars);
 __global__ void k_rolim_fix(Geo, Dev, StepScalars);
-__global__ void k_machref(Geo, Dev, Params, StepScalars);
-__global__ void k_mach_apply(Geo, Dev);
+__global__ void k_mach_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_mixplan(Geo, Dev, Params, float *);
 __global__ void k_vrms_part(Geo, Dev, double *);
 __global__ void k_vrms_final(Geo, Dev, const double *, int, int);
@@ -338,13 +336,13 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   const dim3 grid = node_grid(g);
   const int nmp = (g.NROWS - 1) * g.KMM1;
   if (!(p.FEXTRAP < 0.001f) && nmp > 0) {
-    LAUNCH(k_fextrap, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p);
+    LAUNCH(k_fextrap, dim3(nmp), dim3(64), 0, c->stream, g, d, p);
     LCHK("fextrap");
   }
   LAUNCH(k_residual, grid, kNodeBlock, 0, c->stream, g, d, D, DIFF, p);
   LCHK("residual");
   if (nmp > 0) {
-    LAUNCH(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 1);
+    LAUNCH(k_mpavg, dim3(nmp), dim3(64), 0, c->stream, g, d, p, 1);
     LCHK("mpavg1");
   }
   const bool mg = !(p.IR <= 1 && p.JR <= 1 && p.KR <= 1);
@@ -367,7 +365,7 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
       LCHK("mg2w");
       LAUNCH(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 0);
       LCHK("lines");
-      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
+      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(256), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
                          L.sb_list, g.NSBLK, d.SBCHG, 0);
       LCHK("sbw");
     }
@@ -390,15 +388,14 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
                          d.SC + SC_AVG_CHG, 1, 1);
     } else {
       LAUNCH(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 1);
-      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
+      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(256), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
                          L.row_list, g.NROWS, d.SC + SC_AVG_CHG, 1);
     }
-    LAUNCH(k_avg_blk, dim3(1), dim3(64), 0, c->stream, g, d);
     LAUNCH(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
     LCHK("damp");
   }
   if (nmp > 0) {
-    LAUNCH(k_mpavg, dim3((nmp + 63) / 64), dim3(64), 0, c->stream, g, d, p, 2);
+    LAUNCH(k_mpavg, dim3(nmp), dim3(64), 0, c->stream, g, d, p, 2);
     LCHK("mpavg2");
   }
   LAUNCH(k_scatter, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.DSC);
@@ -449,7 +446,7 @@ int step_once(mh_ctx *c) {
   LCHK("prim");
   if (check5) {
     LAUNCH(k_vrms_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
-    LAUNCH(k_vrms_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LAUNCH(k_vrms_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("vrms");
   }
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
@@ -458,7 +455,7 @@ int step_once(mh_ctx *c) {
   LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_energy");
   if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
-  LAUNCH(k_bc, dim3(1), dim3(256), 4 * g.KM * sizeof(float), c->stream, g, d, p, s);
+  LAUNCH(k_bc, dim3(1), dim3(1024), 4 * g.KM * sizeof(float), c->stream, g, d, p, s);
   LCHK("bc");
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
@@ -473,22 +470,20 @@ int step_once(mh_ctx *c) {
   // limiters, mf:5115-5168
   d = c->d;
   const int nline = g.IM * g.KM;
-  LAUNCH(k_fill_int, dim3((nline + 255) / 256), dim3(256), 0, c->stream, d.FLAGS, nline, 0);
   LAUNCH(k_rolim_check, grid, kNodeBlock, 0, c->stream, g, d, s);
   LAUNCH(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s);
-  LAUNCH(k_machref, dim3(1), dim3(64), 0, c->stream, g, d, p, s);
-  LAUNCH(k_mach_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  LAUNCH(k_mach_apply, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   LCHK("limiters");
   if (g.IM > 2 && p.IFMIX > 0 && p.RFMIX > 1.0e-3f && g.NROWS > 1) {
     const int nl = (g.NROWS - 1) * g.KM;
-    LAUNCH(k_mixplan, dim3((nl + 63) / 64), dim3(64), 0, c->stream, g, d, p, c->lists.mixscratch);
+    LAUNCH(k_mixplan, dim3(nl), dim3(64), 9 * g.IM * sizeof(float), c->stream, g, d, p, c->lists.mixscratch);
     LCHK("mixplan");
   }
   if (check5) {
     LAUNCH(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, d, p.EXACT);
-    LAUNCH(k_econt, dim3(1), dim3(64), 0, c->stream, g, d);
+    LAUNCH(k_econt, dim3(1), dim3(256), 0, c->stream, g, d);
     LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
-    LAUNCH(k_emax_final, dim3(1), dim3(64), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("convergence");
     if ((r = stepup(c, 0.25f))) return r;
     float sc[8];
@@ -546,7 +541,8 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   if (!dims || !out) return MH_E_ARG;
   *out = nullptr;
   const mh_dims &dm = *dims;
-  if (dm.im < 4 || dm.km < 4 || dm.jm < 10 || dm.nrows < 1 || dm.nrows > 64 || dm.nib1 < 1 || dm.njb1 < 1 ||
+  // pitchwise lines are summed on one wavefront in up to 4 chunks (IM-1 <= 256)
+  if (dm.im < 4 || dm.im > 257 || dm.km < 4 || dm.jm < 10 || dm.nrows < 1 || dm.nrows > 64 || dm.nib1 < 1 || dm.njb1 < 1 ||
       dm.nkb1 < 1 || dm.nib2 < 1 || dm.njb2 < 1 || dm.nkb2 < 1 || dm.nsblk < 1)
     return MH_E_ARG;
   mh_ctx *c = new mh_ctx();

@@ -24,6 +24,20 @@ namespace mh {
   const int K = blockIdx.z + 1;                            \
   if (I > g.IM || J > g.JM) return;
 
+
+// Left-to-right FP32 sum of x[1..n] (n <= 256) on one wavefront, in the
+// reference's sequential order: lane l holds x[l+1+64c] in v[c].  Every lane
+// returns the same value.  The shuffles are independent; only the adds chain.
+__device__ __forceinline__ float wave_ordered_sum(const float (&v)[4], int n) {
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int m = min(64, n - 64 * c);
+    for (int l = 0; l < m; l++) s = s + __shfl(v[c], l, 64);
+  }
+  return s;
+}
+
 // ---------------------------------------------------------------------------
 // primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
 
@@ -273,20 +287,33 @@ __global__ void k_bc(Geo g, Dev d, Params p, StepScalars s) {
   extern __shared__ float sh[];
   float *PAVG2 = sh, *PAVG3 = sh + g.KM, *PAVGOUT = sh + 2 * g.KM, *SUMK = sh + 3 * g.KM;
   const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
-  for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) {
-    float a2 = 0.0f, a3 = 0.0f, ao = 0.0f, sm = 0.0f;
-    for (int I = 1; I <= IMM1; I++) {
-      float fp = d.FP[I - 1];
-      a2 = a2 + 0.5f * (d.P[IX(I, 2, K)] + d.P[IX(I + 1, 2, K)]) * fp;
-      a3 = a3 + 0.5f * (d.P[IX(I, 3, K)] + d.P[IX(I + 1, 3, K)]) * fp;
-      ao = ao + 0.5f * (d.P[IX(I, JMM1, K)] + d.P[IX(I + 1, JMM1, K)]) * fp;
-      sm = sm + 0.5f * fp * (d.ROVT[IX(I, JM, K)] * d.VT[IX(I, JM, K)] + d.ROVT[IX(I + 1, JM, K)] * d.VT[IX(I + 1, JM, K)]) /
-                    d.R[JKX(JM, K)];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  // pitchwise sums per K in the reference's order over I, mf:4512-4521
+  for (int K = wave + 1; K <= g.KM; K += nwave) {
+    float v2[4], v3[4], vo[4], vs[4];
+    const float rjm = d.R[JKX(JM, K)];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      if (I <= IMM1) {
+        const float fp = d.FP[I - 1];
+        v2[c] = 0.5f * (d.P[IX(I, 2, K)] + d.P[IX(I + 1, 2, K)]) * fp;
+        v3[c] = 0.5f * (d.P[IX(I, 3, K)] + d.P[IX(I + 1, 3, K)]) * fp;
+        vo[c] = 0.5f * (d.P[IX(I, JMM1, K)] + d.P[IX(I + 1, JMM1, K)]) * fp;
+        vs[c] = 0.5f * fp *
+                (d.ROVT[IX(I, JM, K)] * d.VT[IX(I, JM, K)] + d.ROVT[IX(I + 1, JM, K)] * d.VT[IX(I + 1, JM, K)]) / rjm;
+      } else {
+        v2[c] = v3[c] = vo[c] = vs[c] = 0.0f;
+      }
     }
-    PAVG2[K - 1] = a2;
-    PAVG3[K - 1] = a3;
-    PAVGOUT[K - 1] = ao;
-    SUMK[K - 1] = sm;
+    const float a2 = wave_ordered_sum(v2, IMM1), a3 = wave_ordered_sum(v3, IMM1);
+    const float ao = wave_ordered_sum(vo, IMM1), sm = wave_ordered_sum(vs, IMM1);
+    if (lane == 0) {
+      PAVG2[K - 1] = a2;
+      PAVG3[K - 1] = a3;
+      PAVGOUT[K - 1] = ao;
+      SUMK[K - 1] = sm;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0 && p.IPOUT <= 0) {
@@ -394,22 +421,30 @@ __global__ void k_peff(Geo g, Dev d, Params p, StepScalars s) {
 
 // upstream-face flux extrapolation at mixing planes, mf:5516-5555
 __global__ void k_fextrap(Geo g, Dev d, Params p) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x;  // one wavefront per (mixing plane, K) line
   const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
   if (NR > g.NROWS - 1) return;
+  const int lane = threadIdx.x;
   const int J = d.JMIX[NR - 1], JP1 = J + 1, JM1 = J - 1, JP2 = J + 2;
-  float FLOWDIRN = -d.FLOWX[IX(p.IMID, J, K)];
-  float AVFLXJM1 = 0.0f, AVFLXJP2 = 0.0f;
-  for (int I = 1; I <= g.IMM1; I++) {
-    AVFLXJM1 = AVFLXJM1 + d.XFLUX[IX(I, JM1, K)];
-    AVFLXJP2 = AVFLXJP2 + d.XFLUX[IX(I, JP2, K)];
+  const float FLOWDIRN = -d.FLOWX[IX(p.IMID, J, K)];
+  float a[4], b[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    a[c] = (I <= g.IMM1) ? d.XFLUX[IX(I, JM1, K)] : 0.0f;
+    b[c] = (I <= g.IMM1) ? d.XFLUX[IX(I, JP2, K)] : 0.0f;
   }
-  for (int I = 1; I <= g.IMM1; I++) {
+  const float AVFLXJM1 = wave_ordered_sum(a, g.IMM1);
+  const float AVFLXJP2 = wave_ordered_sum(b, g.IMM1);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    if (I > g.IMM1) continue;
     if (FLOWDIRN > 0.0f) {
-      float DFLUX = (d.XFLUX[IX(I, JM1, K)] - AVFLXJM1 * d.FP[I - 1]) * p.FEXTRAP;
+      float DFLUX = (a[c] - AVFLXJM1 * d.FP[I - 1]) * p.FEXTRAP;
       d.SOURCE[IX(I, J, K)] = d.SOURCE[IX(I, J, K)] - DFLUX;
     } else {
-      float DFLUX = (d.XFLUX[IX(I, JP2, K)] - AVFLXJP2 * d.FP[I - 1]) * p.FEXTRAP;
+      float DFLUX = (b[c] - AVFLXJP2 * d.FP[I - 1]) * p.FEXTRAP;
       d.SOURCE[IX(I, JP2, K)] = d.SOURCE[IX(I, JP2, K)] + DFLUX;
     }
     d.SOURCE[IX(I, JP1, K)] = 0.0f;
@@ -441,22 +476,29 @@ __global__ void k_residual(Geo g, Dev d, float *D, float *DIFF, Params p) {
 // pitchwise averaging of STORE at the mixing planes, mf:5578-5607 (pass 1,
 // FP-weighted) and mf:5786-5818 (pass 2, plain mean)
 __global__ void k_mpavg(Geo g, Dev d, Params p, int pass) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x;  // one wavefront per (mixing plane, K) line
   const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
   if (NR > g.NROWS - 1) return;
+  const int lane = threadIdx.x;
   const int J = d.JMIX[NR - 1];
-  float FLOWDIRN = -d.FLOWX[IX(p.IMID, J, K)];
+  const float FLOWDIRN = -d.FLOWX[IX(p.IMID, J, K)];
   for (int side = 0; side < 2; side++) {
-    bool on = side == 0 ? ((FLOWDIRN > 0.0f) || (p.FEXTRAP < 0.001f)) : ((FLOWDIRN < 0.0f) || (p.FEXTRAP < 0.001f));
+    const bool on = side == 0 ? ((FLOWDIRN > 0.0f) || (p.FEXTRAP < 0.001f))
+                              : ((FLOWDIRN < 0.0f) || (p.FEXTRAP < 0.001f));
     if (!on) continue;
     const int JMX = side == 0 ? J + 2 : J;
-    float SUM_STORE = 0.0f;
-    for (int I = 1; I <= g.IMM1; I++) SUM_STORE = SUM_STORE + d.STORE[IX(I, JMX, K)];
-    if (pass == 1) {
-      for (int I = 1; I <= g.IMM1; I++) d.STORE[IX(I, JMX, K)] = SUM_STORE * d.FP[I - 1];
-    } else {
-      SUM_STORE = SUM_STORE / (float)g.IMM1;
-      for (int I = 1; I <= g.IMM1; I++) d.STORE[IX(I, JMX, K)] = SUM_STORE;
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      v[c] = (I <= g.IMM1) ? d.STORE[IX(I, JMX, K)] : 0.0f;
+    }
+    float SUM_STORE = wave_ordered_sum(v, g.IMM1);
+    if (pass != 1) SUM_STORE = SUM_STORE / (float)g.IMM1;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      if (I <= g.IMM1) d.STORE[IX(I, JMX, K)] = (pass == 1) ? SUM_STORE * d.FP[I - 1] : SUM_STORE;
     }
   }
 }
@@ -517,20 +559,26 @@ __global__ void k_line_sums(Geo g, Dev d, double *lines, int absval) {
   if (threadIdx.x == 0) lines[JKX(J, K)] = s;
 }
 
-// stage 2: one wavefront per J range [jlo, jhi] (1-based, from CSR lists)
+// stage 2: one 256-thread workgroup per J range, fixed-order tree
 __global__ void k_range_sums(Geo g, const double *lines, const int *jptr, const int *jlist, int nranges, float *out,
                              int divide_by_cells) {
+  __shared__ double sh[256];
   const int r = blockIdx.x;
   if (r >= nranges) return;
   const int q0 = jptr[r], nj = jptr[r + 1] - q0;
   double s = 0.0;
-  for (int c = threadIdx.x; c < nj * g.KMM1; c += 64) {
+  for (int c = threadIdx.x; c < nj * g.KMM1; c += 256) {
     const int J = jlist[q0 + c % nj], K = c / nj + 1;
     s += lines[JKX(J, K)];
   }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
-    float f = (float)s;
+    float f = (float)sh[0];
     if (divide_by_cells) f = f / (float)(g.IMM1 * g.KMM1 * nj);
     out[r] = f;
   }
@@ -597,24 +645,17 @@ __global__ void k_smooth_resid(Geo g, Dev d, const float *src, float *dst, float
   dst[n] = v;
 }
 
-// row-averaged change smoothing between neighbouring rows, mf:5699-5710
-__global__ void k_avg_blk(Geo g, Dev d) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int NROWS = g.NROWS;
-  float *AC = d.SC + SC_AVG_CHG, *AB = d.SC + SC_AVG_BLK;
-  if (NROWS == 1) AB[0] = AC[0];
-  if (NROWS == 2) {
-    AB[1] = 0.5f * (AC[0] + AC[1]);
-    AB[0] = AB[1];
-  }
-  if (NROWS > 2) {
-    for (int NR = 2; NR <= NROWS - 1; NR++) AB[NR - 1] = 0.25f * (AC[NR - 2] + AC[NR]) + 0.5f * AC[NR - 1];
-    AB[NROWS - 1] = 0.5f * (AC[NROWS - 2] + AC[NROWS - 1]);
-    AB[0] = 0.5f * (AC[0] + AC[1]);
-  }
+// row-averaged change smoothing between neighbouring rows (mf:5699-5710)
+// evaluated per thread for its own row, then the negative feedback
+// (mf:5716-5724)
+__device__ __forceinline__ float avg_blk(const float *AC, int NR, int NROWS) {
+  if (NROWS == 1) return AC[0];
+  if (NROWS == 2) return 0.5f * (AC[0] + AC[1]);
+  if (NR == 1) return 0.5f * (AC[0] + AC[1]);
+  if (NR == NROWS) return 0.5f * (AC[NROWS - 2] + AC[NROWS - 1]);
+  return 0.25f * (AC[NR - 2] + AC[NR]) + 0.5f * AC[NR - 1];
 }
 
-// negative feedback, mf:5716-5724
 __global__ void k_damp_apply(Geo g, Dev d, float DAMP) {
   NODE_IJK
   if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
@@ -622,7 +663,7 @@ __global__ void k_damp_apply(Geo g, Dev d, float DAMP) {
   const int NR = d.NROW[J - 1];
   float DELTA = d.STORE[n];
   float ABSCHG = fabsf(DELTA);
-  float FDAMP = ABSCHG / d.SC[SC_AVG_BLK + NR - 1];
+  float FDAMP = ABSCHG / avg_blk(d.SC + SC_AVG_CHG, NR, g.NROWS);
   d.STORE[n] = DELTA / (1.f + FDAMP / DAMP);
 }
 
@@ -857,16 +898,16 @@ __global__ void k_rolim_fix(Geo g, Dev d, StepScalars s) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.IM * g.KM) return;
   if (d.FLAGS[t] == 0) return;
+  d.FLAGS[t] = 0;  // ready for the next step's check
   const int I = t % g.IM + 1, K = t / g.IM + 1;
   for (int J = 3; J <= g.JM - 3; J++)
     d.RO[IX(I, J, K)] = rolim_one(d.RO[IX(I, J, K)], d.RO[IX(I, J - 2, K)], d.RO[IX(I, J - 1, K)], d.RO[IX(I, J + 1, K)],
                                   d.RO[IX(I, J + 2, K)], s.ROLIM);
 }
 
-// reference relative Mach conditions per row, mf:5127-5145
-__global__ void k_machref(Geo g, Dev d, Params p, StepScalars s) {
-  const int NR = threadIdx.x + 1;
-  if (NR > g.NROWS) return;
+// reference relative Mach conditions of row NR, mf:5127-5145
+__device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Params &p, float TCOOL_MIN, int NR,
+                                         float &vlimit, float &rolimit) {
   const int IMID = p.IMID, KMID = p.KMID;
   float GM1 = p.GA - 1.0f;
   float TRATIO = 1.0f + p.MACHLIM * p.MACHLIM * 0.5f * GM1;
@@ -878,18 +919,23 @@ __global__ void k_machref(Geo g, Dev d, Params p, StepScalars s) {
   float WREFSQ = d.VX[n] * d.VX[n] + d.VR[n] * d.VR[n] + d.WT[n] * d.WT[n];
   float TOREL_REF = TREF + 0.5f * WREFSQ / p.CP;
   float TLIMIT = TOREL_REF / TRATIO;
-  if (TLIMIT > s.TCOOL_MIN) TLIMIT = s.TCOOL_MIN;
-  d.SC[SC_VLIMIT + NR - 1] = sqrtf(2 * p.CP * (TOREL_REF - TLIMIT));
+  if (TLIMIT > TCOOL_MIN) TLIMIT = TCOOL_MIN;
+  vlimit = sqrtf(2 * p.CP * (TOREL_REF - TLIMIT));
   float ROSTAG_REL = d.RO[n] * powf(TOREL_REF / TREF, RFGM1);
-  d.SC[SC_ROLIMIT + NR - 1] = ROSTAG_REL / RORATIO;
+  rolimit = ROSTAG_REL / RORATIO;
 }
 
-// relative Mach number and mass-flux limiter, mf:5150-5168
-__global__ void k_mach_apply(Geo g, Dev d) {
+// relative Mach number and mass-flux limiter, mf:5150-5168; each workgroup
+// evaluates the per-row limits (mf:5127-5145) into LDS first.
+__global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
+  __shared__ float lim[2 * 64];
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid < g.NROWS) mach_ref(g, d, p, st.TCOOL_MIN, tid + 1, lim[tid], lim[64 + tid]);
+  __syncthreads();
   NODE_IJK
   const size_t n = IX(I, J, K);
   const int NR = d.NROW[J - 1];
-  const float V_LIMIT = d.SC[SC_VLIMIT + NR - 1], RO_LIMIT = d.SC[SC_ROLIMIT + NR - 1];
+  const float V_LIMIT = lim[NR - 1], RO_LIMIT = lim[64 + NR - 1];
   float ro = d.RO[n];
   if (ro < RO_LIMIT) {
     ro = RO_LIMIT;
@@ -915,15 +961,14 @@ __global__ void k_mach_apply(Geo g, Dev d) {
 // (mixing plane, K) line; the supersonic Prandtl-Meyer sweeps are serial in I.
 
 __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  extern __shared__ float ml[];  // 9 * IM floats: per-I work arrays of this line
+  const int t = blockIdx.x;      // one wavefront per (mixing plane, K) line
   const int NR = t / g.KM + 1, K = t % g.KM + 1;
   if (NR > g.NROWS - 1) return;
-  const int IM = g.IM, IMID = p.IMID;
-  float *ALPHAA = scratch + (size_t)t * 8 * IM;
-  float *GAMA = ALPHAA + IM, *WREF = GAMA + IM, *ROREF = WREF + IM, *PISENT = ROREF + IM, *ROISENT = PISENT + IM,
-        *WMACH = ROISENT + IM, *TISENT = WMACH + IM;
-  float WIS[1];
-  (void)WIS;
+  const int IM = g.IM, IMID = p.IMID, lane = threadIdx.x;
+  float *ALPHAA = ml, *GAMA = ml + IM, *WREF = ml + 2 * IM, *ROREF = ml + 3 * IM, *PISENT = ml + 4 * IM,
+        *ROISENT = ml + 5 * IM, *WMACH = ml + 6 * IM, *TISENT = ml + 7 * IM, *WISV = ml + 8 * IM;
+  (void)scratch;
   const float RFMIX = p.RFMIX, RFMIX1 = 1.0f - RFMIX;
   int KSUB = K;
   if (K == g.KM) KSUB = g.KM - 1;
@@ -935,30 +980,31 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
     J = d.JMIX[NR - 1] + 1;
     JP1 = J - 1; JP2 = J - 2; JP3 = J - 3; F_ANGLE = 0.0f;
   }
-  float F_ANGLE1 = 1.0f - F_ANGLE;
-  float RJP1 = d.R[JKX(JP1, K)], UJP1 = d.UBLADE[JKX(JP1, K)];
+  const float F_ANGLE1 = 1.0f - F_ANGLE;
+  // mixed-out state from the mid-pitch point of the plane, mf:14968-14995
+  const float RJP1 = d.R[JKX(JP1, K)], UJP1 = d.UBLADE[JKX(JP1, K)];
   const size_t nm = IX(IMID, JP1, K);
-  float ROMIX = d.RO[nm];
-  float VXMIX = d.ROVX[nm] / ROMIX;
-  float VRMIX = d.ROVR[nm] / ROMIX;
-  float VTMIX = d.RORVT[nm] / ROMIX / RJP1;
-  float ROEMIX = d.ROE[nm];
-  float VMSQ = VXMIX * VXMIX + VRMIX * VRMIX;
-  float VABSQ = VMSQ + VTMIX * VTMIX;
-  float WTMIX = VTMIX - UJP1;
-  float WMIXSQ = VMSQ + WTMIX * WTMIX;
-  float WMIX = sqrtf(WMIXSQ);
-  float ALPHAMIX = asinf(WTMIX / WMIX);
-  float GAMAMIX = atan2f(VRMIX, VXMIX);
-  float TMIX = (ROEMIX / ROMIX - 0.5f * VABSQ) / p.CV;
-  float ROTHALPY = p.CP * TMIX + 0.5f * (WMIXSQ - UJP1 * UJP1);
-  float GAGAS = p.GA;
-  float PMIX = ROMIX * p.RGAS * TMIX;
-  int NMACH = 0;
+  const float ROMIX = d.RO[nm];
+  const float VXMIX = d.ROVX[nm] / ROMIX;
+  const float VRMIX = d.ROVR[nm] / ROMIX;
+  const float VTMIX = d.RORVT[nm] / ROMIX / RJP1;
+  const float ROEMIX = d.ROE[nm];
+  const float VMSQ = VXMIX * VXMIX + VRMIX * VRMIX;
+  const float VABSQ = VMSQ + VTMIX * VTMIX;
+  const float WTMIX = VTMIX - UJP1;
+  const float WMIXSQ = VMSQ + WTMIX * WTMIX;
+  const float WMIX = sqrtf(WMIXSQ);
+  const float ALPHAMIX = asinf(WTMIX / WMIX);
+  const float GAMAMIX = atan2f(VRMIX, VXMIX);
+  const float TMIX = (ROEMIX / ROMIX - 0.5f * VABSQ) / p.CV;
+  const float ROTHALPY = p.CP * TMIX + 0.5f * (WMIXSQ - UJP1 * UJP1);
+  const float GAGAS = p.GA;
+  const float PMIX = ROMIX * p.RGAS * TMIX;
   const float RJP2 = d.R[JKX(JP2, K)], UNOWJP2 = d.UBLADE[JKX(JP2, K)];
   const float RJP3 = d.R[JKX(JP3, K)], UNOWJP3 = d.UBLADE[JKX(JP3, K)];
-  float *WISV = scratch + (size_t)(g.NROWS - 1) * g.KM * 8 * IM + (size_t)t * IM;
-  for (int I = 1; I <= IM; I++) {
+  // isentropic state and flow angles at JP2 for every I, mf:15000-15125
+  int nmach = 0;
+  for (int I = lane + 1; I <= IM; I += 64) {
     const size_t n2 = IX(I, JP2, K), n3 = IX(I, JP3, K);
     float ROJP2 = d.RO[n2];
     float XVELJP2 = d.ROVX[n2] / ROJP2;
@@ -990,12 +1036,17 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
     WISV[I - 1] = sqrtf(WIS_SQ_JP2);
     WREF[I - 1] = WACT_JP2;
     ROREF[I - 1] = ROJP2;
-    WMACH[I - 1] = WACT_JP2 / sqrtf(GAGAS * p.RGAS * T_ACT_JP2);
-    if (WMACH[I - 1] > 1.0f) NMACH = NMACH + 1;
+    float wm = WACT_JP2 / sqrtf(GAGAS * p.RGAS * T_ACT_JP2);
+    WMACH[I - 1] = wm;
+    if (wm > 1.0f) nmach++;
     ALPHAA[I - 1] = F_ANGLE1 * ALPHAMIX + F_ANGLE * ALPHAJP3;
     GAMA[I - 1] = F_ANGLE1 * GAMAMIX + F_ANGLE * GAMAJP3;
   }
-  if (NMACH != 0) {
+  const bool supersonic = __any(nmach != 0);
+  __syncthreads();
+  // Prandtl-Meyer angle variation for supersonic relative flow, serial in I,
+  // mf:15139-15173, then periodicity of the angle, mf:15177-15182
+  if (supersonic && lane == 0) {
     float FROTN = 1.0f;
     if (WTMIX < 0.0f) FROTN = -1.0f;
     for (int I = IMID + 1; I <= IM; I++) {
@@ -1023,7 +1074,9 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
       SUMFP = SUMFP + d.FP[I - 1];
     }
   }
-  for (int I = 1; I <= IM; I++) {
+  __syncthreads();
+  // relax the JP2 state towards the isentropic one, mf:15199-15246
+  for (int I = lane + 1; I <= IM; I += 64) {
     const size_t n2 = IX(I, JP2, K);
     float RO_JP2 = ROISENT[I - 1];
     float WT_JP2 = WISV[I - 1] * sinf(ALPHAA[I - 1]);
@@ -1077,9 +1130,11 @@ __global__ void k_vrms_part(Geo g, Dev d, double *part) {
 }
 
 __global__ void k_vrms_final(Geo g, Dev d, const double *part, int nparts, int exact) {
-  if (threadIdx.x != 0) return;
-  float VRMS = 0.0f, VMAX = 0.0f;
+  __shared__ double ss[256];
+  __shared__ double sm[256];
   if (exact) {
+    if (threadIdx.x != 0) return;
+    float VRMS = 0.0f, VMAX = 0.0f;
     for (int K = 1; K <= g.KM; K++)
       for (int J = 2; J <= g.JM; J++)
         for (int I = 1; I <= g.IM; I++) {
@@ -1088,17 +1143,29 @@ __global__ void k_vrms_final(Geo g, Dev d, const double *part, int nparts, int e
           if (EKE > VMAX) VMAX = EKE;
           VRMS = VRMS + EKE;
         }
-  } else {
-    double s = 0.0, m = 0.0;
-    for (int b = 0; b < nparts; b++) {
-      s += part[2 * b];
-      m = fmax(m, part[2 * b + 1]);
-    }
-    VRMS = (float)s;
-    VMAX = (float)m;
+    d.SC[SC_VRMS] = VRMS;
+    d.SC[SC_VMAX] = sqrtf(VMAX);
+    return;
   }
-  d.SC[SC_VRMS] = VRMS;
-  d.SC[SC_VMAX] = sqrtf(VMAX);
+  double s = 0.0, m = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+    s += part[2 * b];
+    m = fmax(m, part[2 * b + 1]);
+  }
+  ss[threadIdx.x] = s;
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ss[threadIdx.x] += ss[threadIdx.x + off];
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.SC[SC_VRMS] = (float)ss[0];
+    d.SC[SC_VMAX] = sqrtf((float)sm[0]);
+  }
 }
 
 // per-J mass flow, reference order over I then K (exact) or FP64 (fast)
@@ -1124,16 +1191,24 @@ __global__ void k_checkflow(Geo g, Dev d, int exact) {
 }
 
 __global__ void k_econt(Geo g, Dev d) {
-  if (threadIdx.x != 0) return;
-  float ECONT = 0.0f;
-  for (int J = 1; J <= g.JM; J++) {
-    float RATIO = d.CHECK_FLOW[J - 1] / d.CHECK_FLOW[0];
-    float EMASS = fabsf(1.f - RATIO);
-    if (EMASS > ECONT) ECONT = EMASS;
+  __shared__ float sm[256];
+  float e = 0.0f;
+  const float cf1 = d.CHECK_FLOW[0];
+  for (int J = threadIdx.x + 1; J <= g.JM; J += 256) {
+    float RATIO = d.CHECK_FLOW[J - 1] / cf1;
+    e = fmaxf(e, fabsf(1.f - RATIO));
   }
-  d.SC[SC_ECONT] = ECONT;
-  d.SC[SC_FLOWRAT] = d.BLADE_FLOW[0];
-  d.SC[SC_FLOWOUT] = d.BLADE_FLOW[g.JM - 1];
+  sm[threadIdx.x] = e;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.SC[SC_ECONT] = sm[0];
+    d.SC[SC_FLOWRAT] = d.BLADE_FLOW[0];
+    d.SC[SC_FLOWOUT] = d.BLADE_FLOW[g.JM - 1];
+  }
 }
 
 __device__ __forceinline__ float dvmer_at(const Geo &g, const Dev &d, int I, int J, int K, float RVEF) {
@@ -1187,12 +1262,14 @@ __global__ void k_emax_part(Geo g, Dev d, double *part) {
 }
 
 __global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int exact) {
-  if (threadIdx.x != 0) return;
+  __shared__ double ssum[256], sabs[256], sidx[256];
+  __shared__ float sval[256];
   const float RIJKM = 2.f / (float)(g.IM * g.JM * g.KM);
   const float VREF = sqrtf(d.SC[SC_VRMS] * RIJKM);
   const float RVEF = 100.f / VREF;
-  float EMAX = 0.0f, EAVG = 0.0f;
   if (exact) {
+    if (threadIdx.x != 0) return;
+    float EMAX = 0.0f, EAVG = 0.0f;
     for (int K = 1; K <= g.KM; K++)
       for (int J = 2; J <= g.JM; J++)
         for (int I = 1; I <= g.IM; I++) {
@@ -1200,20 +1277,36 @@ __global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int e
           EAVG = EAVG + fabsf(DVMER);
           if (fabsf(DVMER) > fabsf(EMAX)) EMAX = DVMER;
         }
-  } else {
-    double s = 0.0, ba = -1.0, bi = -1.0;
-    float bv = 0.0f;
-    for (int b = 0; b < nparts; b++) {
-      s += part[4 * b];
-      const double a = part[4 * b + 1], i = part[4 * b + 3];
-      if (i >= 0 && (a > ba || (a == ba && i < bi))) { ba = a; bi = i; bv = (float)part[4 * b + 2]; }
-    }
-    EAVG = (float)s;
-    EMAX = bv;
+    d.SC[SC_EAVG] = EAVG / (float)(g.IM * g.JMM1 * g.KM);
+    d.SC[SC_EMAX] = EMAX;
+    d.SC[SC_VREF] = VREF;
+    return;
   }
-  d.SC[SC_EAVG] = EAVG / (float)(g.IM * g.JMM1 * g.KM);
-  d.SC[SC_EMAX] = EMAX;
-  d.SC[SC_VREF] = VREF;
+  double s = 0.0, ba = -1.0, bi = -1.0;
+  float bv = 0.0f;
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+    s += part[4 * b];
+    const double a = part[4 * b + 1], i = part[4 * b + 3];
+    if (i >= 0 && (a > ba || (a == ba && i < bi))) { ba = a; bi = i; bv = (float)part[4 * b + 2]; }
+  }
+  ssum[threadIdx.x] = s; sabs[threadIdx.x] = ba; sidx[threadIdx.x] = bi; sval[threadIdx.x] = bv;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    const int t = threadIdx.x;
+    if (t < off) {
+      ssum[t] += ssum[t + off];
+      const double a = sabs[t + off], i = sidx[t + off];
+      if (i >= 0 && (a > sabs[t] || (a == sabs[t] && (sidx[t] < 0 || i < sidx[t])))) {
+        sabs[t] = a; sidx[t] = i; sval[t] = sval[t + off];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.SC[SC_EAVG] = (float)ssum[0] / (float)(g.IM * g.JMM1 * g.KM);
+    d.SC[SC_EMAX] = sval[0];
+    d.SC[SC_VREF] = VREF;
+  }
 }
 
 // ---------------------------------------------------------------------------
