@@ -40,7 +40,7 @@ KERNEL_BYTES_PER_NODE = {
     "k_flux_axial": 4 * (9 + 4), "k_flux_angular": 4 * (6 + 4), "k_flux_radial": 4 * (12 + 4),
     "k_residual": 4 * (5 + 2), "k_mg_apply": 4 * (3 + 1), "k_damp_apply": 4 * 2,
     "k_scatter": 4 * (2 + 1), "k_smooth_plane": 4 * 2, "k_peff": 4 * 2, "k_mg_sum_seq": 4,
-    "k_mg_sum_wave": 4, "k_line_sums": 4, "k_rolim_check": 4, "k_mach_apply": 4 * 4,
+    "k_mg_sum_wave": 4, "k_jplane_sums": 4, "k_rolim_check": 4, "k_mach_apply": 4 * 4,
     "k_stepup_temp": 4 * 6, "k_stepup_step": 4 * 5, "k_smooth_resid": 4 * 2,
 }
 

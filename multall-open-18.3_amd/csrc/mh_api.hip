@@ -23,14 +23,15 @@ __global__ void k_flux_energy(Geo, Dev);
 __global__ void k_flux_axial(Geo, Dev);
 __global__ void k_flux_angular(Geo, Dev);
 __global__ void k_flux_radial(Geo, Dev);
-__global__ void k_bc(Geo, Dev, Params, StepScalars);
+__global__ void k_bc_sums(Geo, Dev, Params);
+__global__ void k_bc_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_peff(Geo, Dev, Params, StepScalars);
 __global__ void k_fextrap(Geo, Dev, Params);
 __global__ void k_residual(Geo, Dev, float *, float *, Params);
 __global__ void k_mpavg(Geo, Dev, Params, int);
 __global__ void k_mg_sum_seq(Geo, Dev, const int *, const int *, int, int, int, int, int, float *);
 __global__ void k_mg_sum_wave(Geo, Dev, const int *, const int *, int, int, int, int, int, float *);
-__global__ void k_line_sums(Geo, Dev, double *, int);
+__global__ void k_jplane_sums(Geo, Dev, double *, int);
 __global__ void k_range_sums(Geo, const double *, const int *, const int *, int, float *, int);
 __global__ void k_range_sums_seq(Geo, Dev, const int *, const int *, int, float *, int, int);
 __global__ void k_mg_apply(Geo, Dev);
@@ -363,9 +364,9 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
       LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
                          g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
       LCHK("mg2w");
-      LAUNCH(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 0);
-      LCHK("lines");
-      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(256), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
+      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, c->stream, g, d, d.RED, 0);
+      LCHK("jplanes");
+      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
                          L.sb_list, g.NSBLK, d.SBCHG, 0);
       LCHK("sbw");
     }
@@ -387,8 +388,8 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
       LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, c->stream, g, d, L.row_ptr, L.row_list, g.NROWS,
                          d.SC + SC_AVG_CHG, 1, 1);
     } else {
-      LAUNCH(k_line_sums, dim3((g.JM + 3) / 4, g.KMM1), dim3(64, 4), 0, c->stream, g, d, d.RED, 1);
-      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(256), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
+      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, c->stream, g, d, d.RED, 1);
+      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
                          L.row_list, g.NROWS, d.SC + SC_AVG_CHG, 1);
     }
     LAUNCH(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
@@ -400,7 +401,7 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   }
   LAUNCH(k_scatter, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.DSC);
   LCHK("scatter");
-  LAUNCH(k_smooth_plane, dim3(g.JM), dim3(256), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
+  LAUNCH(k_smooth_plane, dim3(g.JM), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
                      s, ncall);
   LCHK("smooth_plane");
   return MH_OK;
@@ -455,7 +456,8 @@ int step_once(mh_ctx *c) {
   LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_energy");
   if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
-  LAUNCH(k_bc, dim3(1), dim3(1024), 4 * g.KM * sizeof(float), c->stream, g, d, p, s);
+  LAUNCH(k_bc_sums, dim3(1), dim3(1024), 6 * g.KM * sizeof(float), c->stream, g, d, p);
+  LAUNCH(k_bc_apply, dim3((g.IM * g.KM + 255) / 256), dim3(256), 0, c->stream, g, d, p, s);
   LCHK("bc");
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
@@ -567,7 +569,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.SOURCE, N)) || (r = dalloc(c, &d.STORE, N)) || (r = dalloc(c, &d.DSC, N)) ||
       (r = dalloc(c, &d.TMP, N)) || (r = dalloc(c, &d.TEMP1, N)) ||
       (r = dalloc(c, &d.B1CHG, kind_len(dm, K_B1))) || (r = dalloc(c, &d.B2CHG, kind_len(dm, K_B2))) ||
-      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) ||
+      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)3 * dm.km)) ||
       (r = dalloc(c, &d.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))) ||
       (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||

@@ -53,6 +53,7 @@ struct Dev {
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
+  float *BCW;                        // exit/inlet pitch averages per K (3*KM)
   double *RED;                       // FP64 partial sums of the reductions
   float *SC;                         // device-side scalars (see SC_* below)
   int *FLAGS;                        // per (I,K) line flags for the density limiter

@@ -38,6 +38,17 @@ __device__ __forceinline__ float wave_ordered_sum(const float (&v)[4], int n) {
   return s;
 }
 
+// Sum of x[1..n] (n <= 256) held as in wave_ordered_sum.  EXACT mode keeps
+// the reference's order; otherwise a deterministic FP64 butterfly (every
+// lane ends with the same bits: each level adds the same two operands).
+__device__ __forceinline__ float line_sum(const float (&v)[4], int n, int exact) {
+  if (exact) return wave_ordered_sum(v, n);
+  double s = ((double)v[0] + (double)v[1]) + ((double)v[2] + (double)v[3]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  return (float)s;
+}
+
 // ---------------------------------------------------------------------------
 // primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
 
@@ -280,12 +291,14 @@ __global__ void k_flux_radial(Geo g, Dev d) {
 
 // ---------------------------------------------------------------------------
 // inlet and exit boundary conditions, mf:4476-4734 (IFGAS=0, ITIMST<5,
-// no throttle or plate loss).  One workgroup: the exit-pressure recursion
-// PD(K) is serial in K; the per-node updates are independent.
+// no throttle or plate loss).  k_bc_sums (one workgroup) forms the pitchwise
+// sums per K and the exit-pressure recursion PD(K), which is serial in K;
+// k_bc_apply updates the inlet and exit nodes in parallel.  d.BCW carries
+// PAVG2, PAVG3 and PAVGOUT (KM each) from the first to the second.
 
-__global__ void k_bc(Geo g, Dev d, Params p, StepScalars s) {
+__global__ void k_bc_sums(Geo g, Dev d, Params p) {
   extern __shared__ float sh[];
-  float *PAVG2 = sh, *PAVG3 = sh + g.KM, *PAVGOUT = sh + 2 * g.KM, *SUMK = sh + 3 * g.KM;
+  float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *PAVGOUT = d.BCW + 2 * g.KM, *SUMK = sh + 3 * g.KM;
   const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   // pitchwise sums per K in the reference's order over I, mf:4512-4521
@@ -306,8 +319,8 @@ __global__ void k_bc(Geo g, Dev d, Params p, StepScalars s) {
         v2[c] = v3[c] = vo[c] = vs[c] = 0.0f;
       }
     }
-    const float a2 = wave_ordered_sum(v2, IMM1), a3 = wave_ordered_sum(v3, IMM1);
-    const float ao = wave_ordered_sum(vo, IMM1), sm = wave_ordered_sum(vs, IMM1);
+    const float a2 = line_sum(v2, IMM1, p.EXACT), a3 = line_sum(v3, IMM1, p.EXACT);
+    const float ao = line_sum(vo, IMM1, p.EXACT), sm = line_sum(vs, IMM1, p.EXACT);
     if (lane == 0) {
       PAVG2[K - 1] = a2;
       PAVG3[K - 1] = a3;
@@ -316,22 +329,36 @@ __global__ void k_bc(Geo g, Dev d, Params p, StepScalars s) {
     }
   }
   __syncthreads();
+  // exit pressure from radial equilibrium, serial in K, mf:4531-4542
+  float *PDL = sh + 4 * g.KM, *RJM = sh + 5 * g.KM;
+  for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) {
+    PDL[K - 1] = d.PD[K - 1];
+    RJM[K - 1] = d.R[JKX(JM, K)];
+  }
+  __syncthreads();
   if (threadIdx.x == 0 && p.IPOUT <= 0) {
     float PDIFF = 0.0f, SUMP = 0.0f;
-    if (p.IPOUT == -1) PDIFF = p.PDOWN_TIP - d.PD[g.KM - 1];
+    if (p.IPOUT == -1) PDIFF = p.PDOWN_TIP - PDL[g.KM - 1];
     for (int K = 1; K <= g.KM; K++) {
-      d.PD[0] = p.PDOWN_HUB;
+      PDL[0] = p.PDOWN_HUB;
       if (K != 1) {
-        float DP = (SUMK[K - 1] + SUMP) * 0.5f * (d.R[JKX(JM, K)] - d.R[JKX(JM, K - 1)]);
-        d.PD[K - 1] = 0.9f * d.PD[K - 1] + 0.1f * (d.PD[K - 2] + DP);
+        float DP = (SUMK[K - 1] + SUMP) * 0.5f * (RJM[K - 1] - RJM[K - 2]);
+        PDL[K - 1] = 0.9f * PDL[K - 1] + 0.1f * (PDL[K - 2] + DP);
       }
       SUMP = SUMK[K - 1];
-      if (p.IPOUT == -1) d.PD[K - 1] = d.PD[K - 1] + PDIFF;
+      if (p.IPOUT == -1) PDL[K - 1] = PDL[K - 1] + PDIFF;
     }
   }
   __syncthreads();
-  const int IM = g.IM;
-  for (int t = threadIdx.x; t < IM * g.KM; t += blockDim.x) {
+  for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) d.PD[K - 1] = PDL[K - 1];
+}
+
+__global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
+  const float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *PAVGOUT = d.BCW + 2 * g.KM, *PDL = d.PD;
+  const int JM = g.JM, JMM1 = g.JMM1, IM = g.IM;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= IM * g.KM) return;
+  {
     const int I = t % IM + 1, K = t / IM + 1;
     const size_t n1 = IX(I, 1, K);
     const float PO1 = d.PO1[K - 1], TO1 = d.TO1[K - 1];
@@ -396,7 +423,7 @@ __global__ void k_bc(Geo g, Dev d, Params p, StepScalars s) {
     const size_t nm = IX(I, JM, K);
     float vr = d.VR[nm], vt = d.VT[nm], vx = d.VX[nm];
     float EKE = .5f * (vr * vr + vt * vt + vx * vx);
-    float PEX = d.PD[K - 1] + p.FEXTRAP * (d.P[IX(I, JMM1, K)] - PAVGOUT[K - 1]);
+    float PEX = PDL[K - 1] + p.FEXTRAP * (d.P[IX(I, JMM1, K)] - PAVGOUT[K - 1]);
     float HOEX = 2.0f * d.HO[IX(I, JMM1, K)] - d.HO[IX(I, JM - 2, K)];
     float TEXIT = (HOEX - EKE) / p.CP;
     float roex = PEX / p.RGAS / TEXIT;
@@ -434,8 +461,8 @@ __global__ void k_fextrap(Geo g, Dev d, Params p) {
     a[c] = (I <= g.IMM1) ? d.XFLUX[IX(I, JM1, K)] : 0.0f;
     b[c] = (I <= g.IMM1) ? d.XFLUX[IX(I, JP2, K)] : 0.0f;
   }
-  const float AVFLXJM1 = wave_ordered_sum(a, g.IMM1);
-  const float AVFLXJP2 = wave_ordered_sum(b, g.IMM1);
+  const float AVFLXJM1 = line_sum(a, g.IMM1, p.EXACT);
+  const float AVFLXJP2 = line_sum(b, g.IMM1, p.EXACT);
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const int I = 64 * c + lane + 1;
@@ -493,7 +520,7 @@ __global__ void k_mpavg(Geo g, Dev d, Params p, int pass) {
       const int I = 64 * c + lane + 1;
       v[c] = (I <= g.IMM1) ? d.STORE[IX(I, JMX, K)] : 0.0f;
     }
-    float SUM_STORE = wave_ordered_sum(v, g.IMM1);
+    float SUM_STORE = line_sum(v, g.IMM1, p.EXACT);
     if (pass != 1) SUM_STORE = SUM_STORE / (float)g.IMM1;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -543,33 +570,20 @@ __global__ void k_mg_sum_wave(Geo g, Dev d, const int *jptr, const int *jlist, i
   if (threadIdx.x == 0) out[t] = (float)s;
 }
 
-// Superblock and per-row sums over whole J ranges: stage 1 sums each
-// (J,K) line over I in FP64 (optionally of |STORE|).
-__global__ void k_line_sums(Geo g, Dev d, double *lines, int absval) {
-  const int J = blockIdx.x * blockDim.y + threadIdx.y + 1;
-  const int K = blockIdx.y + 1;
-  if (J > g.JM || K > g.KMM1) return;
+// Superblock and per-row sums over whole J ranges.  Stage 1: one
+// workgroup per J plane sums its cells (optionally |STORE|) in FP64 with a
+// fixed-order tree.
+__global__ void k_jplane_sums(Geo g, Dev d, double *jpart, int absval) {
+  __shared__ double sh[256];
+  const int J = blockIdx.x + 1;
   double s = 0.0;
-  if (J >= 2)
-    for (int I = threadIdx.x + 1; I <= g.IMM1; I += 64) {
-      float v = d.STORE[IX(I, J, K)];
+  if (J >= 2) {
+    const int n = g.IMM1 * g.KMM1;
+    for (int c = threadIdx.x; c < n; c += 256) {
+      const int I = c % g.IMM1 + 1, K = c / g.IMM1 + 1;
+      const float v = d.STORE[IX(I, J, K)];
       s += (double)(absval ? fabsf(v) : v);
     }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (threadIdx.x == 0) lines[JKX(J, K)] = s;
-}
-
-// stage 2: one 256-thread workgroup per J range, fixed-order tree
-__global__ void k_range_sums(Geo g, const double *lines, const int *jptr, const int *jlist, int nranges, float *out,
-                             int divide_by_cells) {
-  __shared__ double sh[256];
-  const int r = blockIdx.x;
-  if (r >= nranges) return;
-  const int q0 = jptr[r], nj = jptr[r + 1] - q0;
-  double s = 0.0;
-  for (int c = threadIdx.x; c < nj * g.KMM1; c += 256) {
-    const int J = jlist[q0 + c % nj], K = c / nj + 1;
-    s += lines[JKX(J, K)];
   }
   sh[threadIdx.x] = s;
   __syncthreads();
@@ -577,8 +591,20 @@ __global__ void k_range_sums(Geo g, const double *lines, const int *jptr, const 
     if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
     __syncthreads();
   }
+  if (threadIdx.x == 0) jpart[J - 1] = sh[0];
+}
+
+// stage 2: one wavefront per J range sums the per-J partials
+__global__ void k_range_sums(Geo g, const double *jpart, const int *jptr, const int *jlist, int nranges, float *out,
+                             int divide_by_cells) {
+  const int r = blockIdx.x;
+  if (r >= nranges) return;
+  const int q0 = jptr[r], nj = jptr[r + 1] - q0;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < nj; c += 64) s += jpart[jlist[q0 + c] - 1];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (threadIdx.x == 0) {
-    float f = (float)sh[0];
+    float f = (float)s;
     if (divide_by_cells) f = f / (float)(g.IMM1 * g.KMM1 * nj);
     out[r] = f;
   }

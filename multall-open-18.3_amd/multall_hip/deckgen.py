@@ -90,6 +90,10 @@ class DeckSpec:
     yplam: float = 5.0
     ypturb: float = 25.0
     facmixup: float = 2.0
+    ibound: int = 0
+    ypluswall: float = 0.0
+    if_rough: int = 0
+    rough: tuple = (0.0, 0.0, 0.0, 0.0)   # hub, casing, I=1 and I=IM surfaces, microns
     nmixup: int = 1000
     sa_factors: tuple = (0.0, 1.0, 1.0, 1.0, 1.0, 2.0, 0.0, 0.0)
     po1: float = 1.0e5
@@ -219,7 +223,7 @@ def write_deck(spec: DeckSpec) -> str:
         a("   RFMIX,    FEXTRAP,   FSMTHB,    FANGLE")
         a(f"{spec.rfmix:12.6f}{spec.fextrap:12.6f}{spec.fsmthb:12.6f}{spec.fangle:12.6f}")
     a("      IFCOOL    IFBLEED    IF_ROUGH")
-    a("         0         0         0")
+    a(f"         0         0{spec.if_rough:10d}")
     a("       NSECS_IN")
     a(f"{spec.nsecs:10d}")
     a("       IN_PRESS  IN_VTAN   IN_VR    IN_FLOW  IF_REPEAT  RFIN")
@@ -229,11 +233,11 @@ def write_deck(spec: DeckSpec) -> str:
     a("  PLATE_LOSS THROTTLE_EXIT")
     a("  0.000000  0.000000")
     a("        ILOS      NLOS      IBOUND")
-    a(f"{spec.ilos:10d}{spec.nlos:10d}         0")
+    a(f"{spec.ilos:10d}{spec.nlos:10d}{spec.ibound:10d}")
     if spec.ilos != 0:
         a("   REYNO,     RF_VIS,   FTRANS, TURBVIS_LIM, PRANDTL, YPLUSWALL")
         a(f"{spec.reyno:12.1f}{spec.rf_vis:10.3f}{spec.ftrans:10.5f}{spec.turbvis_lim:10.3f}"
-          f"{spec.prandtl:10.3f}     0.000")
+          f"{spec.prandtl:10.3f}{spec.ypluswall:10.3f}")
     if spec.ilos >= 200:
         a("  FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD")
         a("".join(f"{v:10.4f}" for v in spec.sa_factors))
@@ -331,6 +335,10 @@ def write_deck(spec: DeckSpec) -> str:
                 a("  0.020000  0.030000  0.040000  0.050000  1.000000  0.500000")
         a("  FACTOR TO INCREASE THE TURBULENT VISCOSITY OVER THE FIRST NMIXUP STEPS")
         a(f"{spec.facmixup:10.5f}{spec.nmixup:6d}")
+    if spec.if_rough > 0:
+        a("  SURFACE ROUGHNESS IN MICRONS, HUB, CASING, I=1 AND I=IM SURFACES, EACH ROW")
+        for _ in spec.rows:
+            a("".join(f"{v:10.3f}" for v in spec.rough))
     return "\n".join(L) + "\n"
 
 
@@ -350,6 +358,30 @@ def deck_tiny(ilos: int = 0, nrows: int = 1, nsteps: int = 100) -> DeckSpec:
     return DeckSpec(title=f"TINY CASCADE ILOS={ilos} ROWS={nrows}", im=9, km=9, rows=rows,
                     ilos=ilos, nsteps=nsteps, conlim=1e-9, nchange=50, nmixup=50,
                     pdown_hub=0.8e5, pdown_tip=0.8e5, seed=1)
+
+
+def deck_tiny_ml_variant(nsteps: int = 100) -> DeckSpec:
+    """Three-row tiny cascade with the mixing-length model's other branches:
+    power-law laminar viscosity (REYNO < 0), rough walls, and a rotor tip
+    gap over the two outer K-cells."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=10 POWER-LAW VISCOSITY, ROUGH, TIP GAP"
+    spec.reyno = -2.0
+    spec.if_rough = 1
+    spec.rough = (20.0, 30.0, 10.0, 15.0)
+    r = spec.rows[1]
+    r.ktips, r.ktipe, r.fractip = 7, 9, 0.02
+    return spec
+
+
+def deck_tiny_wall_variant(nsteps: int = 100) -> DeckSpec:
+    """One-row tiny cascade with wall stresses from a fixed wall y+
+    (YPLUSWALL > 5) and a shear-free hub (IBOUND = 1)."""
+    spec = deck_tiny(ilos=10, nrows=1, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=10 YPLUSWALL=12 IBOUND=1"
+    spec.ypluswall = 12.0
+    spec.ibound = 1
+    return spec
 
 
 def deck_config0(nsteps: int = 2000) -> DeckSpec:

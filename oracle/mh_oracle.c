@@ -16,6 +16,7 @@
 #include "mh_oracle.h"
 
 #include <math.h>
+#include <stdbool.h>
 #include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,14 +28,14 @@
   X(VX) X(VR) X(VT) X(WT) X(T_STATIC) X(DRO) X(DROVX) X(DROVR) X(DRORVT) X(DROE)          \
   X(AQX) X(AQR) X(ABX) X(ABR) X(ASX) X(ASR) X(VOL) X(VOLOR) X(STEP) X(BSTEP) X(RSTEP)     \
   X(XFORCE) X(RFORCE) X(TFORCE) X(QSOURCE) X(FLOWX) X(FLOWT) X(FLOWR) X(XFLUX) X(TFLUX)  \
-  X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN)
+  X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN) X(DWALLSQ) X(Y_PLUS)
 #define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID)
 #define FIK(X) X(FTL) X(FBL) X(FTR) X(FBR)
 #define F1D(X) X(FP) X(FU) X(FD) X(FR) X(FKU) X(FKD) X(PO1) X(TO1) X(RPO1) X(BSSIN) X(BSCOS) \
   X(BRSIN) X(BRCOS) X(VTIN) X(PD) X(PLIM) X(VM1) X(FACUP) X(FACDWN) X(WRAD) X(WHUB)         \
   X(WTIP) X(VLIMIT) X(ROLIMIT) X(AVG_CHG) X(AVG_BLK) X(STEP1) X(STEP2) X(STEPSBK)           \
   X(XLLIM_I1) X(XLLIM_IM) X(XLLIM_K1) X(XLLIM_KM) X(XLLIM_DWN) X(XLLIM_UP) X(CHORD)         \
-  X(ROUGH_H) X(ROUGH_T) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
+  X(ROUGH_H) X(ROUGH_T) X(ROUGH_L) X(ROUGH_U) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
   X(REYNOLDS) X(ROVEXIT)
 #define I1D(X) X(IB1) X(IB2) X(KB1) X(KB2) X(NBLADE) X(IND) X(INDLE) X(INDMIX) X(INDTE)   \
   X(NROW) X(JSBLK) X(JB1) X(JB2) X(JSTART) X(JMIX) X(JLE) X(JTE) X(JLED) X(KTIPS) X(KTIPE) \
@@ -46,7 +47,7 @@
   X(FGA) X(GA1) X(RCP) X(VSOUND) X(FACMIXUP) X(FMIXUP) X(REYNO) X(RF_VIS) X(FTRANS)           \
   X(PRANDTL) X(YPLUSWALL) X(TURBVIS_LIM) X(VISLAM) X(FTCOND) X(PREFF) X(ROLIM) X(ROMIN)       \
   X(RO_REF) X(EAVG) X(EMAX) X(ECONT) X(VRMS) X(VMAX) X(VREF) X(TCOOL_MIN) X(SFEX1) X(SFEXH)   \
-  X(FLOWRAT)
+  X(FLOWRAT) X(YPLAM) X(YPTURB) X(CFWALL)
 #define ISCAL(X) X(IM) X(JM) X(KM) X(NROWS) X(IMID) X(KMID) X(NSTEP) X(NSTEPS_MAX) X(NCHANGE) \
   X(NRSMTH) X(NSFEXIT) X(IN_PRESS) X(IN_VTAN) X(IN_VR) X(IPOUT) X(IFMIX) X(IR) X(JR) X(KR)    \
   X(ILOS) X(NLOS) X(NMIXUP) X(NIB1) X(NJB1) X(NKB1) X(NIB2) X(NJB2) X(NKB2) X(NSBLK) X(IFEND) \
@@ -131,7 +132,7 @@ int orc_start(orc_t *o) {
       o->IF_REPEAT != 0 || o->IF_RESTART != 0)
     return -2;
   if (o->ILOS != 0 && o->ILOS != 10) return -3;
-  if (o->ILOS == 10 && o->YPLUSWALL < 0.0f) return -3;
+  if (o->ILOS == 10 && o->YPLUSWALL < -0.001f) return -3;
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMID = o->IMID, KMID = o->KMID;
   (void)IM;
   o->RO_REF = A1(PO1, KMID) / (o->RGAS * A1(TO1, KMID));
@@ -775,7 +776,428 @@ void orc_new_mixplan(orc_t *o) {
   free(WMACH_JP2); free(TISENT); free(WIS_JP2);
 }
 
-void orc_loss(orc_t *o) { (void)o; }
+/* ----------------------------------------------------------------------- */
+/* LOSS, mf:8058-9066: mixing-length eddy viscosity and log-law wall        */
+/* friction turned into body forces by a thin-shear-layer approximation.    */
+/* Restated for YPLUSWALL >= -0.001 (the Shih wall functions, mf:8269-8280, */
+/* and SET_PWALLGRAD are not) and KM > 2.  The entropy-generation rate      */
+/* (SGEN/TEMP4, mf:8538-8549 and 9013-9049) and the Reynolds-number and     */
+/* VISC_RAT diagnostics feed printed output only and are left out.          */
+
+#define LMAX 1024
+
+/* Skin-friction coefficient of a wall cell, mf:8287-8329: log-law curve fit
+ * (with the roughness correction when ROUGH > 1e-7), at least laminar.
+ * RO and WB are the density and speed the wall stress is formed with. */
+static float loss_cf(float RE, float PERP, float ROUGH, float RO, float WB, float VISLAM) {
+  float RELOG = 1.0f / logf(RE);
+  float CF;
+  if (ROUGH > 1.0e-7f) {
+    if (ROUGH > PERP) ROUGH = PERP;
+    float REK = RE * ROUGH / PERP;
+    REK = REK - 80.0f;
+    if (REK < 0.0f) REK = 0.0f;
+    float REKSQ = REK * REK;
+    float A1 = -.00178493f + .0000814923f * REK + .000000150445f * REKSQ;
+    float A2 = .029072f - .001584f * REK - .00000225194f * REKSQ;
+    float A3 = .270313f + .0091409f * REK + .00000451537f * REKSQ + .00000000464767f * REKSQ * REK;
+    CF = A1 + A2 * RELOG + A3 * RELOG * RELOG;
+  } else {
+    CF = -0.00178493f + 0.029072f * RELOG + 0.270313f * RELOG * RELOG;
+  }
+  float CFLAM = 2.0f / RE;
+  CF = fmaxf(CF, CFLAM);
+  if (RE < 125.f) CF = CFLAM;
+  if (ROUGH > 1.0e-7f) {
+    float TW = 0.5f * CF * RO * WB * WB;
+    float VSTAR = sqrtf(TW / RO);
+    float PLUSK = ROUGH * VSTAR * RO / VISLAM;
+    float RELOG10 = log10f(PERP / ROUGH);
+    float FUNCN = 5.75f * RELOG10 + 8.5f;
+    float CF_FULL_ROUGH = 2.0f / (FUNCN * FUNCN);
+    if (PLUSK > 45.f && CF > CF_FULL_ROUGH) CF = CF_FULL_ROUGH;
+  }
+  return CF;
+}
+
+/* damping of the turbulent viscosity in the sublayer and buffer region,
+ * mf:8496-8501 (FYPLUS keeps its previous value when Y_PLUS is NaN) */
+static float loss_fyplus(float yp, float YPLAM, float YPTURB, float prev) {
+  float f = prev;
+  if (yp <= YPLAM) f = 0.0f;
+  if (yp > YPLAM) {
+    float XFAC = (yp - YPLAM) / (YPTURB - YPLAM);
+    if (XFAC > 1.0f) XFAC = 1.0f;
+    f = XFAC * XFAC * (3.0f - 2.0f * XFAC);
+  }
+  return f;
+}
+
+void orc_loss(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID, KMID = o->KMID;
+  float AREA[LMAX], VXAVG[LMAX], VRAVG[LMAX], VTAVG[LMAX], WTAVG[LMAX], ROAVG[LMAX], TAVG[LMAX];
+  float RAVG[LMAX], WABS[LMAX], WTB[LMAX], WBOUND[LMAX], XSTRES[LMAX], RSTRES[LMAX], TSTRES[LMAX];
+  float QFLOW[LMAX], WVISC[LMAX], VLAM[LMAX], VTURB[LMAX];
+  float FYPLUS = 0.0f;
+  /* laminar viscosity over the first quarter of the run, mf:8082-8109 */
+  if (o->NSTEP == 1 || o->NSTEP < o->NSTEPS_MAX / 4) {
+    if (o->REYNO > 100.f) {
+      int J1 = A1(JLE, 1), J2 = A1(JTE, 1);
+      if (A1(JLE, 1) > JM) J1 = 1;
+      if (A1(JTE, 1) > JM) J2 = JM;
+      float XCHORD = AJK(SMERID, J2, KMID) - AJK(SMERID, J1, KMID);
+      float a = A3(ROVX, IMID, J2, KMID), b = A3(ROVR, IMID, J2, KMID), c = A3(ROWT, IMID, J2, KMID);
+      float ROW2 = sqrtf(a * a + b * b + c * c);
+      o->VISLAM = XCHORD * ROW2 / o->REYNO;
+    }
+    if (o->REYNO > 0.0f && o->REYNO < 99.99f) o->VISLAM = o->REYNO / 100000.f;
+    if (o->REYNO < 0.0f) o->VISLAM = -o->REYNO * 1.0e-5f;
+    o->TCOND = o->CP * o->VISLAM / o->PRANDTL;
+    o->FTCOND = o->TCOND / o->VISLAM;
+  }
+  float VISLAM = o->VISLAM, FTCOND = o->FTCOND;
+  const float RF_VIS = o->RF_VIS, RF_VIS1 = 1.f - RF_VIS, FMIXUP = o->FMIXUP;
+  /* mixing-length factors, first calls only, mf:8125-8183 */
+  if (o->NSTEP <= o->NLOS) {
+    for (int NRW = 1; NRW <= o->NROWS; NRW++) {
+      float SUMF = 0.0f;
+      for (int I = 2; I <= IMM1; I++) {
+        SUMF = SUMF + A1(FP, I - 1);
+        float XLIM = A1(XLLIM_I1, NRW) * (1.0f - SUMF) + A1(XLLIM_IM, NRW) * SUMF;
+        float FPITCH = SUMF * (1.f - SUMF);
+        if (FPITCH > XLIM) FPITCH = XLIM;
+        float DF = A1(FP, I) + A1(FP, I - 1);
+        ANR(FILAM, NRW, I) = A1(FP, I) / DF;
+        ANR(FITURB, NRW, I) = 0.16f * FPITCH * FPITCH / (DF * DF);
+        ANR(FIWAKE, NRW, I) = 0.16f * A1(XLLIM_DWN, NRW) * A1(XLLIM_DWN, NRW) / (DF * DF);
+        ANR(FIUP, NRW, I) = 0.16f * A1(XLLIM_UP, NRW) * A1(XLLIM_UP, NRW) / (DF * DF);
+      }
+      float DF = A1(FP, 1) + A1(FP, IMM1);
+      ANR(FILAM, NRW, 1) = A1(FP, 1) / DF;
+      ANR(FIWAKE, NRW, 1) = 0.16f * A1(XLLIM_DWN, NRW) * A1(XLLIM_DWN, NRW) / (DF * DF);
+      ANR(FIUP, NRW, 1) = 0.16f * A1(XLLIM_UP, NRW) * A1(XLLIM_UP, NRW) / (DF * DF);
+      ANR(FITURB, NRW, 1) = ANR(FITURB, NRW, 2);
+      ANR(FILAM, NRW, IM) = A1(FP, IMM1) / DF;
+      ANR(FIWAKE, NRW, IM) = 0.16f * A1(XLLIM_DWN, NRW) * A1(XLLIM_DWN, NRW) / (DF * DF);
+      ANR(FIUP, NRW, IM) = 0.16f * A1(XLLIM_UP, NRW) * A1(XLLIM_UP, NRW) / (DF * DF);
+      ANR(FITURB, NRW, IM) = ANR(FITURB, NRW, IMM1);
+      float AVGSPAN = 0.0f;
+      const int JMID = (int)(0.5f * (float)(A1(JLE, NRW) + A1(JTE, NRW)));
+      for (int K = 2; K <= KM; K++) {
+        float XDIF = AJK(X, JMID, K) - AJK(X, JMID, K - 1);
+        float RDIF = AJK(R, JMID, K) - AJK(R, JMID, K - 1);
+        AVGSPAN = AVGSPAN + sqrtf(XDIF * XDIF + RDIF * RDIF);
+      }
+      float AVGPIT = 2.0f * 3.1415926f * AJK(R, JMID, KMID) / (float)A1(NBLADE, JMID);
+      float XLIMH = A1(XLLIM_K1, NRW) * AVGPIT / AVGSPAN;
+      float XLIMT = A1(XLLIM_KM, NRW) * AVGPIT / AVGSPAN;
+      SUMF = 0.0f;
+      for (int K = 2; K <= KMM1; K++) {
+        SUMF = SUMF + A1(FR, K - 1);
+        float XLIM = XLIMH * (1.0f - SUMF) + XLIMT * SUMF;
+        float FCSPAN = SUMF * (1.f - SUMF);
+        if (FCSPAN > XLIM) FCSPAN = XLIM;
+        float DF2 = A1(FR, K) + A1(FR, K - 1);
+        ANR(FKLAM, NRW, K) = A1(FR, K) / DF2;
+        ANR(FKTURB, NRW, K) = 0.16f * FCSPAN * FCSPAN / (DF2 * DF2);
+      }
+    }
+  }
+  /* stresses on the streamwise (K = const) faces, mf:8201-8556 */
+  for (int J = 2; J <= JM; J++) {
+    const int NRW = A1(NROW, J), J1 = A1(JSTART, NRW), JROW = J - J1 + 1;
+    const int JTRHUB = A1(JTRAN_K1, NRW), JTRTIP = A1(JTRAN_KM, NRW);
+    const float WREL = A1(WRAD, J);
+    for (int I = 1; I <= IMM1; I++) {
+      for (int K = 1; K <= KM; K++) {
+        float ax = A3(ASX, I, J, K), ar = A3(ASR, I, J, K);
+        AREA[K] = sqrtf(ax * ax + ar * ar);
+        VXAVG[K] = 0.25f * (A3(VX, I, J, K) + A3(VX, I, J - 1, K) + A3(VX, I + 1, J, K) + A3(VX, I + 1, J - 1, K));
+        VRAVG[K] = 0.25f * (A3(VR, I, J, K) + A3(VR, I, J - 1, K) + A3(VR, I + 1, J, K) + A3(VR, I + 1, J - 1, K));
+        VTAVG[K] = 0.25f * (A3(VT, I, J, K) + A3(VT, I, J - 1, K) + A3(VT, I + 1, J, K) + A3(VT, I + 1, J - 1, K));
+        ROAVG[K] = 0.25f * (A3(RO, I, J, K) + A3(RO, I, J - 1, K) + A3(RO, I + 1, J, K) + A3(RO, I + 1, J - 1, K));
+        TAVG[K] = 0.25f * (A3(T_STATIC, I, J, K) + A3(T_STATIC, I, J - 1, K) + A3(T_STATIC, I + 1, J, K) +
+                           A3(T_STATIC, I + 1, J - 1, K));
+        RAVG[K] = 0.5f * (AJK(R, J, K) + AJK(R, J - 1, K));
+        WTAVG[K] = VTAVG[K] - WREL * RAVG[K];
+        float WABSQ = VXAVG[K] * VXAVG[K] + VRAVG[K] * VRAVG[K] + WTAVG[K] * WTAVG[K];
+        WABS[K] = sqrtf(WABSQ);
+        WTB[K] = VTAVG[K] - A1(WHUB, J) * RAVG[K];
+        if (K > KMID) WTB[K] = VTAVG[K] - A1(WTIP, J) * RAVG[K];
+        WBOUND[K] = sqrtf(VXAVG[K] * VXAVG[K] + VRAVG[K] * VRAVG[K] + WTB[K] * WTB[K]);
+      }
+      if (o->REYNO < 0.0001f) {
+        VISLAM = (fabsf(o->REYNO) / 100000.0f) * powf(TAVG[KMID] / 288.0f, 0.62f);
+        float TCOND = o->CP * VISLAM / o->PRANDTL;
+        FTCOND = TCOND / VISLAM;
+      }
+      float TWALLK1, TWALLKM;
+      if (o->YPLUSWALL < 5.f) {
+        float PERP = A3(VOL, I, J, 1) / AREA[1];
+        float RE = PERP * ROAVG[2] * WBOUND[2] / VISLAM;
+        float CF = loss_cf(RE, PERP, A1(ROUGH_H, NRW), ROAVG[2], WBOUND[2], VISLAM);
+        TWALLK1 = 0.5f * CF * ROAVG[2] * WBOUND[2] * WBOUND[2];
+      } else {
+        TWALLK1 = o->CFWALL * ROAVG[1] * WBOUND[1] * WBOUND[1];
+      }
+      if (o->YPLUSWALL < 5.f) {
+        float PERP = A3(VOL, I, J, KMM1) / AREA[KM];
+        float RE = PERP * ROAVG[KMM1] * WBOUND[KMM1] / VISLAM;
+        float CF = loss_cf(RE, PERP, A1(ROUGH_T, NRW), ROAVG[KMM1], WBOUND[KMM1], VISLAM);
+        TWALLKM = 0.5f * CF * ROAVG[KMM1] * WBOUND[KMM1] * WBOUND[KMM1];
+      } else {
+        TWALLKM = o->CFWALL * ROAVG[KM] * WBOUND[KM] * WBOUND[KM];
+      }
+      /* y+ from the end walls, mf:8420-8435 */
+      float VSTAR = sqrtf(TWALLK1 / ROAVG[1]);
+      for (int K = 1; K <= KMID; K++) {
+        float PERP = sqrtf(A3(DWALLSQ, I, J, K));
+        float YPLSK1 = VSTAR * PERP * ROAVG[1] / VISLAM;
+        A3(Y_PLUS, I, J, K) = fminf(YPLSK1, 1000.0f);
+      }
+      VSTAR = sqrtf(TWALLKM / ROAVG[KM]);
+      for (int K = KMID + 1; K <= KMM1; K++) {
+        float PERP = sqrtf(A3(DWALLSQ, I, J, K));
+        float YPLS = VSTAR * PERP * ROAVG[KM] / VISLAM;
+        A3(Y_PLUS, I, J, K) = fminf(YPLS, 1000.0f);
+      }
+      float FMULT = TWALLK1 * AREA[1] / WBOUND[1];
+      if (o->IBOUND == 1 || o->IBOUND > 2) FMULT = 0.0f;
+      XSTRES[1] = FMULT * VXAVG[1];
+      RSTRES[1] = FMULT * VRAVG[1];
+      TSTRES[1] = FMULT * WTB[1];
+      FMULT = -TWALLKM * AREA[KM] / WBOUND[KM];
+      if (o->IBOUND >= 2) FMULT = 0.0f;
+      XSTRES[KM] = FMULT * VXAVG[KM];
+      RSTRES[KM] = FMULT * VRAVG[KM];
+      TSTRES[KM] = FMULT * WTB[KM];
+      WVISC[1] = TSTRES[1] * A1(WHUB, J) * RAVG[1];
+      QFLOW[1] = 0.0f;
+      WVISC[KM] = TSTRES[KM] * A1(WTIP, J) * RAVG[KM];
+      QFLOW[KM] = 0.0f;
+      for (int K = 2; K <= KMM1; K++) {
+        FYPLUS = loss_fyplus(A3(Y_PLUS, I, J, K), o->YPLAM, o->YPTURB, FYPLUS);
+        float DPERP = A3(VOL, I, J, K) / AREA[K];
+        VLAM[K] = ANR(FKLAM, NRW, K) * VISLAM / DPERP;
+        VTURB[K] = ANR(FKTURB, NRW, K) * ROAVG[K] * fabsf(WABS[K + 1] - WABS[K - 1]) * FMIXUP;
+        VTURB[K] = VTURB[K] * FYPLUS;
+      }
+      /* transition, mf:8474-8500 */
+      int keep = 0;
+      if (JROW >= JTRHUB) {
+        float RMAX = 0.0f;
+        for (int K = 2; K <= KMID; K++) {
+          float RATVIS = VTURB[K] / VLAM[K];
+          if (RATVIS < RMAX) continue;
+          RMAX = RATVIS;
+        }
+        keep = RMAX > o->FTRANS;
+      }
+      if (!keep)
+        for (int K = 2; K <= KMID; K++) VTURB[K] = 0.0f;
+      keep = 0;
+      if (JROW >= JTRTIP) {
+        float RMAX = 0.0f;
+        for (int K = KMID; K <= KMM1; K++) {
+          float RATVIS = VTURB[K] / VLAM[K];
+          if (RATVIS < RMAX) continue;
+          RMAX = RATVIS;
+        }
+        keep = RMAX > o->FTRANS;
+      }
+      if (!keep)
+        for (int K = KMID; K <= KMM1; K++) VTURB[K] = 0.0f;
+      for (int K = 2; K <= KMM1; K++) {
+        float FM = AREA[K] * (VLAM[K] + VTURB[K]);
+        XSTRES[K] = FM * (VXAVG[K + 1] - VXAVG[K - 1]);
+        RSTRES[K] = FM * (VRAVG[K + 1] - VRAVG[K - 1]);
+        float VISTOT = VISLAM * (1.0f + VTURB[K] / VLAM[K]);
+        TSTRES[K] = FM * (VTAVG[K + 1] - VTAVG[K - 1]) - VISTOT * AREA[K] * VTAVG[K] / RAVG[K];
+        QFLOW[K] = FM * FTCOND * (TAVG[K + 1] - TAVG[K - 1]);
+        WVISC[K] = XSTRES[K] * VXAVG[K] + TSTRES[K] * VTAVG[K] + RSTRES[K] * VRAVG[K];
+      }
+      for (int K = 1; K <= KMM1; K++) {
+        A3(XFORCE, I, J, K) = RF_VIS1 * A3(XFORCE, I, J, K) + RF_VIS * (XSTRES[K] - XSTRES[K + 1]);
+        A3(RFORCE, I, J, K) = RF_VIS1 * A3(RFORCE, I, J, K) + RF_VIS * (RSTRES[K] - RSTRES[K + 1]);
+        A3(TFORCE, I, J, K) =
+            RF_VIS1 * A3(TFORCE, I, J, K) + RF_VIS * (TSTRES[K] * RAVG[K] - TSTRES[K + 1] * RAVG[K + 1]);
+        A3(QSOURCE, I, J, K) =
+            RF_VIS1 * A3(QSOURCE, I, J, K) + RF_VIS * (QFLOW[K] - QFLOW[K + 1] + WVISC[K] - WVISC[K + 1]);
+      }
+    }
+  }
+  /* stresses on the blade surfaces and bladewise (I = const) faces,
+   * mf:8562-9001 */
+  for (int J = 2; J <= JM; J++) {
+    const int NRW = A1(NROW, J), J1 = A1(JSTART, NRW), JROW = J - J1 + 1;
+    const int JTRLOW = A1(JTRAN_I1, NRW), JTRUP = A1(JTRAN_IM, NRW);
+    const int JLEDGE = A1(JLE, NRW), JTEDGE = A1(JTE, NRW);
+    const float WREL = A1(WRAD, J);
+    for (int K = 1; K <= KMM1; K++) {
+      const float RAVGK = AJK(RAVG_CELL, J, K);
+      const float abt = AJK(ABT, J, K);
+      const bool tipgap = (A1(KTIPS, NRW) > 0) && (K >= A1(KTIPS, NRW)) && (K < A1(KTIPE, NRW));
+      for (int I = 1; I <= IM; I++) {
+        float bx = A3(ABX, I, J, K), br = A3(ABR, I, J, K);
+        AREA[I] = sqrtf(bx * bx + br * br + abt * abt);
+        VXAVG[I] = 0.25f * (A3(VX, I, J, K) + A3(VX, I, J - 1, K) + A3(VX, I, J - 1, K + 1) + A3(VX, I, J, K + 1));
+        VRAVG[I] = 0.25f * (A3(VR, I, J, K) + A3(VR, I, J - 1, K) + A3(VR, I, J - 1, K + 1) + A3(VR, I, J, K + 1));
+        VTAVG[I] = 0.25f * (A3(VT, I, J, K) + A3(VT, I, J - 1, K) + A3(VT, I, J - 1, K + 1) + A3(VT, I, J, K + 1));
+        WTAVG[I] = 0.25f * (A3(WT, I, J, K) + A3(WT, I, J - 1, K) + A3(WT, I, J - 1, K + 1) + A3(WT, I, J, K + 1));
+        ROAVG[I] = 0.25f * (A3(RO, I, J, K) + A3(RO, I, J - 1, K) + A3(RO, I, J - 1, K + 1) + A3(RO, I, J, K + 1));
+        TAVG[I] = 0.25f * (A3(T_STATIC, I, J, K) + A3(T_STATIC, I, J - 1, K) + A3(T_STATIC, I, J - 1, K + 1) +
+                           A3(T_STATIC, I, J, K + 1));
+        float WABSQ = VXAVG[I] * VXAVG[I] + VRAVG[I] * VRAVG[I] + WTAVG[I] * WTAVG[I];
+        WABS[I] = sqrtf(WABSQ);
+        WBOUND[I] = WABS[I];
+      }
+      if (o->REYNO < 0.0001f) {
+        VISLAM = (fabsf(o->REYNO) / 100000.0f) * powf(TAVG[IMID] / 288.0f, 0.62f);
+        float TCOND = o->CP * VISLAM / o->PRANDTL;
+        FTCOND = TCOND / VISLAM;
+      }
+      if (!(J <= JLEDGE || J > JTEDGE)) {
+        /* within the blade row */
+        if (!tipgap) {
+          float TWALLI1, TWALLIM;
+          if (o->YPLUSWALL < 5.f) {
+            float PERP = A3(VOL, 1, J, K) / AREA[1];
+            float RE = ROAVG[2] * WABS[2] * PERP / VISLAM;
+            float CF = loss_cf(RE, PERP, A1(ROUGH_L, NRW), ROAVG[2], WBOUND[2], VISLAM);
+            TWALLI1 = 0.5f * CF * ROAVG[2] * WBOUND[2] * WBOUND[2];
+          } else {
+            TWALLI1 = o->CFWALL * ROAVG[1] * WABS[1] * WABS[1];
+          }
+          if (o->YPLUSWALL < 5.f) {
+            float PERP = A3(VOL, IMM1, J, K) / AREA[IM];
+            float RE = ROAVG[IMM1] * WABS[IMM1] * PERP / VISLAM;
+            float CF = loss_cf(RE, PERP, A1(ROUGH_U, NRW), ROAVG[IMM1], WBOUND[IMM1], VISLAM);
+            TWALLIM = 0.5f * CF * ROAVG[IMM1] * WBOUND[IMM1] * WBOUND[IMM1];
+          } else {
+            TWALLIM = o->CFWALL * ROAVG[IM] * WABS[IM] * WABS[IM];
+          }
+          /* y+ from the blade surfaces, mf:8768-8783 */
+          float VSTAR = sqrtf(TWALLI1 / ROAVG[1]);
+          for (int I = 1; I <= IMID; I++) {
+            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * ROAVG[1] / VISLAM;
+            float YPLSP = A3(Y_PLUS, I, J, K);
+            A3(Y_PLUS, I, J, K) = fminf(YPLS, YPLSP);
+          }
+          VSTAR = sqrtf(TWALLIM / ROAVG[IM]);
+          for (int I = IMID + 1; I <= IMM1; I++) {
+            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * ROAVG[IM] / VISLAM;
+            float YPLSP = A3(Y_PLUS, I, J, K);
+            A3(Y_PLUS, I, J, K) = fminf(YPLS, YPLSP);
+          }
+          float FM = TWALLI1 * AREA[1] / WABS[1];
+          XSTRES[1] = FM * VXAVG[1];
+          RSTRES[1] = FM * VRAVG[1];
+          TSTRES[1] = FM * WTAVG[1];
+          FM = -TWALLIM * AREA[IM] / WABS[IM];
+          XSTRES[IM] = FM * VXAVG[IM];
+          RSTRES[IM] = FM * VRAVG[IM];
+          TSTRES[IM] = FM * WTAVG[IM];
+          QFLOW[1] = 0.0f;
+          WVISC[1] = TSTRES[1] * WREL * RAVGK;
+          QFLOW[IM] = 0.0f;
+          WVISC[IM] = TSTRES[IM] * WREL * RAVGK;
+        }
+        for (int I = 2; I <= IMM1; I++) {
+          FYPLUS = loss_fyplus(A3(Y_PLUS, I, J, K), o->YPLAM, o->YPTURB, FYPLUS);
+          float DPERP = A3(VOL, I, J, K) / AREA[I];
+          VLAM[I] = ANR(FILAM, NRW, I) * VISLAM / DPERP;
+          VTURB[I] = ANR(FITURB, NRW, I) * ROAVG[I] * fabsf(WABS[I + 1] - WABS[I - 1]) * FMIXUP;
+          VTURB[I] = VTURB[I] * FYPLUS;
+        }
+        int keep = 0;
+        if (JROW >= JTRLOW) {
+          float RMAX = 0.0f;
+          for (int I = 2; I <= IMID; I++) {
+            float RATVIS = VTURB[I] / VLAM[I];
+            if (RATVIS < RMAX) continue;
+            RMAX = RATVIS;
+          }
+          keep = RMAX > o->FTRANS;
+        }
+        if (!keep)
+          for (int I = 2; I <= IMID; I++) VTURB[I] = 0.0f;
+        keep = 0;
+        if (JROW >= JTRUP) {
+          float RMAX = 0.0f;
+          for (int I = IMID; I <= IMM1; I++) {
+            float RATVIS = VTURB[I] / VLAM[I];
+            if (RATVIS < RMAX) continue;
+            RMAX = RATVIS;
+          }
+          keep = RMAX > o->FTRANS;
+        }
+        if (!keep)
+          for (int I = IMID; I <= IMM1; I++) VTURB[I] = 0.0f;
+        for (int I = 2; I <= IMM1; I++) {
+          float FM = (VLAM[I] + VTURB[I]) * AREA[I];
+          XSTRES[I] = FM * (VXAVG[I + 1] - VXAVG[I - 1]);
+          RSTRES[I] = FM * (VRAVG[I + 1] - VRAVG[I - 1]);
+          TSTRES[I] = FM * (WTAVG[I + 1] - WTAVG[I - 1]);
+          QFLOW[I] = FM * FTCOND * (TAVG[I + 1] - TAVG[I - 1]);
+          WVISC[I] = XSTRES[I] * VXAVG[I] + RSTRES[I] * VRAVG[I] + TSTRES[I] * VTAVG[I];
+        }
+        if (tipgap) {
+          XSTRES[1] = 0.5f * (XSTRES[2] + XSTRES[IMM1]);
+          RSTRES[1] = 0.5f * (RSTRES[2] + RSTRES[IMM1]);
+          TSTRES[1] = 0.5f * (TSTRES[2] + TSTRES[IMM1]);
+          XSTRES[IM] = XSTRES[1];
+          TSTRES[IM] = TSTRES[1];
+          RSTRES[IM] = RSTRES[1];
+          QFLOW[1] = 0.5f * (QFLOW[2] + QFLOW[IMM1]);
+          WVISC[1] = 0.5f * (WVISC[2] + WVISC[IMM1]);
+          QFLOW[IM] = QFLOW[1];
+          WVISC[IM] = WVISC[1];
+        }
+      } else {
+        /* upstream of the leading edge and in the wake, mf:8905-8977 */
+        const bool wake = J > JTEDGE;
+        float FACJ = 0.1f * (float)(J - JTEDGE);
+        if (FACJ > 1.f) FACJ = 1.f;
+        for (int I = 1; I <= IM; I++) {
+          const int ISUB = (I == IM) ? IMM1 : I;
+          const int IM1 = (I == 1) ? IMM1 : I - 1, IP1 = (I == IM) ? 2 : I + 1;
+          float DPERP = A3(VOL, ISUB, J, K) / AREA[I];
+          VLAM[I] = ANR(FILAM, NRW, I) * VISLAM / DPERP;
+          if (wake) {
+            float FBLEND = (FACJ * ANR(FIWAKE, NRW, I) + (1.f - FACJ) * ANR(FITURB, NRW, I)) * FMIXUP;
+            VTURB[I] = FBLEND * ROAVG[I] * fabsf(WABS[IP1] - WABS[IM1]);
+          } else {
+            VTURB[I] = FMIXUP * ANR(FIUP, NRW, I) * ROAVG[I] * fabsf(WABS[IP1] - WABS[IM1]);
+          }
+          float FM = AREA[I] * (VLAM[I] + VTURB[I]);
+          XSTRES[I] = FM * (VXAVG[IP1] - VXAVG[IM1]);
+          RSTRES[I] = FM * (VRAVG[IP1] - VRAVG[IM1]);
+          TSTRES[I] = FM * (WTAVG[IP1] - WTAVG[IM1]);
+          QFLOW[I] = FM * FTCOND * (TAVG[IP1] - TAVG[IM1]);
+          WVISC[I] = XSTRES[I] * VXAVG[I] + TSTRES[I] * VTAVG[I] + RSTRES[I] * VRAVG[I];
+        }
+        XSTRES[1] = 0.5f * (XSTRES[IM] + XSTRES[1]);
+        RSTRES[1] = 0.5f * (RSTRES[IM] + RSTRES[1]);
+        TSTRES[1] = 0.5f * (TSTRES[IM] + TSTRES[1]);
+        XSTRES[IM] = XSTRES[1];
+        RSTRES[IM] = RSTRES[1];
+        TSTRES[IM] = TSTRES[1];
+        QFLOW[1] = 0.5f * (QFLOW[IM] + QFLOW[1]);
+        WVISC[1] = 0.5f * (WVISC[IM] + WVISC[1]);
+        QFLOW[IM] = QFLOW[1];
+        WVISC[IM] = WVISC[1];
+      }
+      for (int I = 1; I <= IMM1; I++) {
+        A3(XFORCE, I, J, K) = A3(XFORCE, I, J, K) + RF_VIS * (XSTRES[I] - XSTRES[I + 1]);
+        A3(RFORCE, I, J, K) = A3(RFORCE, I, J, K) + RF_VIS * (RSTRES[I] - RSTRES[I + 1]);
+        A3(TFORCE, I, J, K) = A3(TFORCE, I, J, K) + RF_VIS * (TSTRES[I] - TSTRES[I + 1]) * RAVGK;
+        A3(QSOURCE, I, J, K) = A3(QSOURCE, I, J, K) + RF_VIS * (QFLOW[I] - QFLOW[I + 1] + WVISC[I] - WVISC[I + 1]);
+      }
+    }
+  }
+  o->VISLAM = VISLAM;
+  o->FTCOND = FTCOND;
+}
 
 /* ----------------------------------------------------------------------- */
 /* one time step: the body of LOOP from label 5000 to the GO TO 5000        */

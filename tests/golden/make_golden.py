@@ -34,6 +34,8 @@ CASES = {
     "tiny1_inv": lambda n: deckgen.deck_tiny(ilos=0, nrows=1, nsteps=n),
     "tiny3_inv": lambda n: deckgen.deck_tiny(ilos=0, nrows=3, nsteps=n),
     "tiny3_ml": lambda n: deckgen.deck_tiny(ilos=10, nrows=3, nsteps=n),
+    "tiny3_ml2": lambda n: deckgen.deck_tiny_ml_variant(nsteps=n),
+    "tiny1_mlw": lambda n: deckgen.deck_tiny_wall_variant(nsteps=n),
 }
 STEPS = (1, 100)
 

@@ -10,10 +10,16 @@ import pytest
 import orc
 from conftest import load_ref, load_state
 
-VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP"]
+VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP",
+        "XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
+
+# inviscid decks, and mixing-length (LOSS) decks covering the chord-Reynolds
+# and power-law laminar viscosity, smooth and rough walls, a rotor tip gap,
+# the fixed wall y+ and a shear-free hub
+CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw"]
 
 
-@pytest.mark.parametrize("case", ["tiny1_inv", "tiny3_inv"])
+@pytest.mark.parametrize("case", CASES)
 @pytest.mark.parametrize("nsteps", [1, 100])
 def test_oracle_matches_reference_bitwise(case, nsteps):
     st = load_state(case)
