@@ -57,7 +57,8 @@ typedef struct mh_resid {
 } mh_resid;
 
 /* Array ids.  Shapes: N = (IM,JM,KM) node/cell arrays; JK = (JM,KM);
- * IK = (IM,KM); I, K, J = vectors; R = per row (NROWS+1); B1/B2/SB =
+ * IK = (IM,KM); I, K, J = vectors; R = per row (NROWS+1); RN = per row
+ * (NROWS); B1/B2/SB =
  * multigrid step arrays STEP1 (NIB1,NJB1+1,NKB1), STEP2, STEPSBK. */
 enum mh_array_id {
   /* conserved variables and SSS history (BKPRIM, BKDIFF) -- N */
@@ -90,6 +91,15 @@ enum mh_array_id {
   MH_KB1, MH_KB2,                                   /* K */
   MH_NBLADE, MH_IND, MH_INDLE, MH_INDMIX, MH_INDTE, MH_NROW, MH_JSBLK, MH_JB1, MH_JB2, /* J */
   MH_JSTART, MH_JMIX, MH_JLE, MH_JTE, MH_JLED, MH_KTIPS, MH_KTIPE, MH_KS1, MH_KS2,  /* R */
+  /* mixing-length viscous model, LOSS (BKLOSS, BKNEWLOS, BKTIP): squared
+   * cell wall distance DWALLSQ (SET_XLENGTH, mf:12498-12508) and the y+ it
+   * yields -- N; end-wall rotation rates -- J; per-row mixing-length limits,
+   * wall roughnesses (m) and transition points -- RN = NROWS entries */
+  MH_DWALLSQ, MH_Y_PLUS,                            /* N */
+  MH_WHUB, MH_WTIP,                                 /* J */
+  MH_XLLIM_I1, MH_XLLIM_IM, MH_XLLIM_K1, MH_XLLIM_KM, MH_XLLIM_DWN, MH_XLLIM_UP,
+  MH_ROUGH_H, MH_ROUGH_T, MH_ROUGH_L, MH_ROUGH_U,   /* RN, float */
+  MH_JTRAN_I1, MH_JTRAN_IM, MH_JTRAN_K1, MH_JTRAN_KM, /* RN, int */
   MH_NARRAYS
 };
 
@@ -103,6 +113,12 @@ enum mh_scalar_id {
   MH_S_IN_VTAN, MH_S_IN_VR, MH_S_IPOUT, MH_S_IFMIX, MH_S_IR, MH_S_JR, MH_S_KR, MH_S_ILOS,
   MH_S_NLOS, MH_S_NMIXUP, MH_S_ITIMST, MH_S_IF_RESTART, MH_S_IFGAS, MH_S_IN_FLOW,
   MH_S_IF_REPEAT, MH_S_IRBB, MH_S_KRBB,
+  /* viscous model (BKLOSS, BKGAS, BKNEWLOS) */
+  MH_S_REYNO, MH_S_RF_VIS, MH_S_FTRANS, MH_S_PRANDTL, MH_S_YPLUSWALL, MH_S_YPLAM, MH_S_YPTURB,
+  MH_S_CFWALL, MH_S_IBOUND,
+  /* optional state: laminar viscosity and conductivity ratio left by the
+   * last LOSS call (default 0; mh_loss_init sets them) */
+  MH_S_VISLAM, MH_S_FTCOND,
   /* library controls (no reference counterpart) */
   MH_S_EXACT_ORDER,   /* 1: reductions in the reference's sequential order */
   MH_NSCALARS
@@ -130,6 +146,12 @@ int mh_set_array_strided(mh_ctx *ctx, int id, const void *host, int ld_i, int ld
 int mh_get_array_strided(mh_ctx *ctx, int id, void *host, int ld_i, int ld_j);
 int mh_set_scalar(mh_ctx *ctx, int id, double value);
 double mh_get_scalar(const mh_ctx *ctx, int id);
+
+/* The viscous-force initialisation SETUP performs before LOOP
+ * (mf:7346-7357): one LOSS call (mf:8058-9066) with RF_VIS = 1, FMIXUP = 1
+ * and NSTEP = 1, which also sets VISLAM/FTCOND.  ILOS = 10 only; a no-op for
+ * ILOS = 0.  Call after all inputs are set, before or after mh_start. */
+int mh_loss_init(mh_ctx *ctx);
 
 /* LOOP prologue (mf:3674-3714).  Call once after all inputs are set. */
 int mh_start(mh_ctx *ctx);

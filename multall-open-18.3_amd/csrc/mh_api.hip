@@ -54,6 +54,9 @@ __global__ void k_stepup_temp(Geo, Dev, Params);
 __global__ void k_stepup_step(Geo, Dev, float);
 __global__ void k_stepup_mix(Geo, Dev);
 __global__ void k_integrals(Geo, Dev, Params);
+__global__ void k_loss_vislam(Geo, Dev, Params);
+__global__ void k_loss_span(Geo, Dev, Params, float, float);
+__global__ void k_loss_pitch(Geo, Dev, Params, float, float);
 
 const ArrayInfo kArrays[MH_NARRAYS] = {
     {"RO", K_N, false}, {"ROVX", K_N, false}, {"ROVR", K_N, false}, {"RORVT", K_N, false}, {"ROE", K_N, false},
@@ -79,6 +82,11 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"NROW", K_J, true}, {"JSBLK", K_J, true}, {"JB1", K_J, true}, {"JB2", K_J, true},
     {"JSTART", K_R, true}, {"JMIX", K_R, true}, {"JLE", K_R, true}, {"JTE", K_R, true}, {"JLED", K_R, true},
     {"KTIPS", K_R, true}, {"KTIPE", K_R, true}, {"KS1", K_R, true}, {"KS2", K_R, true},
+    {"DWALLSQ", K_N, false}, {"Y_PLUS", K_N, false}, {"WHUB", K_J, false}, {"WTIP", K_J, false},
+    {"XLLIM_I1", K_RN, false}, {"XLLIM_IM", K_RN, false}, {"XLLIM_K1", K_RN, false}, {"XLLIM_KM", K_RN, false},
+    {"XLLIM_DWN", K_RN, false}, {"XLLIM_UP", K_RN, false}, {"ROUGH_H", K_RN, false}, {"ROUGH_T", K_RN, false},
+    {"ROUGH_L", K_RN, false}, {"ROUGH_U", K_RN, false}, {"JTRAN_I1", K_RN, true}, {"JTRAN_IM", K_RN, true},
+    {"JTRAN_K1", K_RN, true}, {"JTRAN_KM", K_RN, true},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -86,7 +94,8 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "SFEXIT", "FSMTHB", "RFIN", "PDOWN_HUB", "PDOWN_TIP", "FEXTRAP", "RFMIX", "FANGLE", "CP", "GA", "RGAS", "CV",
     "FGA", "GA1", "RCP", "VSOUND", "FACMIXUP", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
     "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS", "NMIXUP", "ITIMST",
-    "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "EXACT_ORDER",
+    "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
+    "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "VISLAM", "FTCOND", "EXACT_ORDER",
 };
 
 }  // namespace mh
@@ -109,6 +118,7 @@ long long kind_len(const mh_dims &d, Kind k) {
     case K_B1: return (long long)d.nib1 * (d.njb1 + 1) * d.nkb1;
     case K_B2: return (long long)d.nib2 * (d.njb2 + 1) * d.nkb2;
     case K_SB: return d.nsblk;
+    case K_RN: return d.nrows;
   }
   return 0;
 }
@@ -132,6 +142,10 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.INDLE, (void **)&d.INDMIX, (void **)&d.INDTE, (void **)&d.NROW, (void **)&d.JSBLK,
       (void **)&d.JB1, (void **)&d.JB2, (void **)&d.JSTART, (void **)&d.JMIX, (void **)&d.JLE, (void **)&d.JTE,
       (void **)&d.JLED, (void **)&d.KTIPS, (void **)&d.KTIPE, (void **)&d.KS1, (void **)&d.KS2,
+      (void **)&d.DWALLSQ, (void **)&d.Y_PLUS, (void **)&d.WHUB, (void **)&d.WTIP, (void **)&d.XLLIM_I1,
+      (void **)&d.XLLIM_IM, (void **)&d.XLLIM_K1, (void **)&d.XLLIM_KM, (void **)&d.XLLIM_DWN,
+      (void **)&d.XLLIM_UP, (void **)&d.ROUGH_H, (void **)&d.ROUGH_T, (void **)&d.ROUGH_L, (void **)&d.ROUGH_U,
+      (void **)&d.JTRAN_I1, (void **)&d.JTRAN_IM, (void **)&d.JTRAN_K1, (void **)&d.JTRAN_KM,
   };
   return base[id];
 }
@@ -192,6 +206,9 @@ Params params_of(const mh_ctx *c) {
   p.ITIMST = si(c, MH_S_ITIMST); p.IF_RESTART = si(c, MH_S_IF_RESTART); p.IFGAS = si(c, MH_S_IFGAS);
   p.IN_FLOW = si(c, MH_S_IN_FLOW); p.IF_REPEAT = si(c, MH_S_IF_REPEAT); p.IRBB = si(c, MH_S_IRBB);
   p.KRBB = si(c, MH_S_KRBB); p.EXACT = si(c, MH_S_EXACT_ORDER);
+  p.REYNO = sf(c, MH_S_REYNO); p.RF_VIS = sf(c, MH_S_RF_VIS); p.FTRANS = sf(c, MH_S_FTRANS);
+  p.PRANDTL = sf(c, MH_S_PRANDTL); p.YPLUSWALL = sf(c, MH_S_YPLUSWALL); p.YPLAM = sf(c, MH_S_YPLAM);
+  p.YPTURB = sf(c, MH_S_YPTURB); p.CFWALL = sf(c, MH_S_CFWALL); p.IBOUND = si(c, MH_S_IBOUND);
   return p;
 }
 
@@ -419,6 +436,93 @@ int stepup(mh_ctx *c, float relax) {
   return MH_OK;
 }
 
+// Mixing-length factors of LOSS (mf:8125-8183), formed on the host in FP32
+// in the reference's statement order.  They depend on the grid only, so they
+// are formed once (the reference recomputes the same values while NSTEP <=
+// NLOS).
+int loss_tables(mh_ctx *c) {
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const int IM = g.IM, KM = g.KM, JM = g.JM, IMM1 = g.IMM1, KMM1 = g.KMM1, NR = g.NROWS, KMID = g.KMID;
+  std::vector<float> FP(IM), FR(KM), X((size_t)JM * KM), R((size_t)JM * KM);
+  std::vector<float> xi1(NR), xim(NR), xk1(NR), xkm(NR), xdw(NR), xup(NR);
+  std::vector<int> NB(JM), JLE(NR + 1), JTE(NR + 1);
+  int r;
+  if ((r = d2h(c, FP.data(), d.FP, IM * 4)) || (r = d2h(c, FR.data(), d.FR, KM * 4)) ||
+      (r = d2h(c, X.data(), d.X, X.size() * 4)) || (r = d2h(c, R.data(), d.R, R.size() * 4)) ||
+      (r = d2h(c, NB.data(), d.NBLADE, JM * 4)) || (r = d2h(c, JLE.data(), d.JLE, NR * 4)) ||
+      (r = d2h(c, JTE.data(), d.JTE, NR * 4)) || (r = d2h(c, xi1.data(), d.XLLIM_I1, NR * 4)) ||
+      (r = d2h(c, xim.data(), d.XLLIM_IM, NR * 4)) || (r = d2h(c, xk1.data(), d.XLLIM_K1, NR * 4)) ||
+      (r = d2h(c, xkm.data(), d.XLLIM_KM, NR * 4)) || (r = d2h(c, xdw.data(), d.XLLIM_DWN, NR * 4)) ||
+      (r = d2h(c, xup.data(), d.XLLIM_UP, NR * 4)))
+    return r;
+  std::vector<float> t((size_t)NR * (4 * IM + 2 * KM), 0.0f);
+  auto P = [&](int tab, int nr, int I) -> float & { return t[lt_pitch(tab, nr, I, NR, IM)]; };
+  auto S = [&](int tab, int nr, int K) -> float & { return t[lt_span(tab, nr, K, NR, IM, KM)]; };
+  auto AJK = [&](const std::vector<float> &a, int J, int K) { return a[(size_t)(J - 1) + (size_t)JM * (K - 1)]; };
+  for (int nrw = 1; nrw <= NR; nrw++) {
+    const float dw = xdw[nrw - 1], up = xup[nrw - 1];
+    float SUMF = 0.0f;
+    for (int I = 2; I <= IMM1; I++) {
+      SUMF = SUMF + FP[I - 2];
+      const float XLIM = xi1[nrw - 1] * (1.0f - SUMF) + xim[nrw - 1] * SUMF;
+      float FPITCH = SUMF * (1.f - SUMF);
+      if (FPITCH > XLIM) FPITCH = XLIM;
+      const float DF = FP[I - 1] + FP[I - 2];
+      P(LT_FILAM, nrw, I) = FP[I - 1] / DF;
+      P(LT_FITURB, nrw, I) = 0.16f * FPITCH * FPITCH / (DF * DF);
+      P(LT_FIWAKE, nrw, I) = 0.16f * dw * dw / (DF * DF);
+      P(LT_FIUP, nrw, I) = 0.16f * up * up / (DF * DF);
+    }
+    const float DF = FP[0] + FP[IMM1 - 1];
+    P(LT_FILAM, nrw, 1) = FP[0] / DF;
+    P(LT_FIWAKE, nrw, 1) = 0.16f * dw * dw / (DF * DF);
+    P(LT_FIUP, nrw, 1) = 0.16f * up * up / (DF * DF);
+    P(LT_FITURB, nrw, 1) = P(LT_FITURB, nrw, 2);
+    P(LT_FILAM, nrw, IM) = FP[IMM1 - 1] / DF;
+    P(LT_FIWAKE, nrw, IM) = 0.16f * dw * dw / (DF * DF);
+    P(LT_FIUP, nrw, IM) = 0.16f * up * up / (DF * DF);
+    P(LT_FITURB, nrw, IM) = P(LT_FITURB, nrw, IMM1);
+    float AVGSPAN = 0.0f;
+    const int JMID = (int)(0.5f * (float)(JLE[nrw - 1] + JTE[nrw - 1]));
+    if (JMID < 1 || JMID > JM) return fail(c, MH_E_ARG, "LOSS: row leading/trailing edge outside the grid");
+    for (int K = 2; K <= KM; K++) {
+      const float XDIF = AJK(X, JMID, K) - AJK(X, JMID, K - 1);
+      const float RDIF = AJK(R, JMID, K) - AJK(R, JMID, K - 1);
+      AVGSPAN = AVGSPAN + sqrtf(XDIF * XDIF + RDIF * RDIF);
+    }
+    const float AVGPIT = 2.0f * 3.1415926f * AJK(R, JMID, KMID) / (float)NB[JMID - 1];
+    const float XLIMH = xk1[nrw - 1] * AVGPIT / AVGSPAN;
+    const float XLIMT = xkm[nrw - 1] * AVGPIT / AVGSPAN;
+    SUMF = 0.0f;
+    for (int K = 2; K <= KMM1; K++) {
+      SUMF = SUMF + FR[K - 2];
+      const float XLIM = XLIMH * (1.0f - SUMF) + XLIMT * SUMF;
+      float FCSPAN = SUMF * (1.f - SUMF);
+      if (FCSPAN > XLIM) FCSPAN = XLIM;
+      const float DF2 = FR[K - 1] + FR[K - 2];
+      S(LT_FKLAM, nrw, K) = FR[K - 1] / DF2;
+      S(LT_FKTURB, nrw, K) = 0.16f * FCSPAN * FCSPAN / (DF2 * DF2);
+    }
+  }
+  return h2d(c, d.LTAB, t.data(), t.size() * 4);
+}
+
+// One LOSS call (mf:8058-9066) with the given relaxation factor, FMIXUP and
+// step number (the laminar viscosity is re-evaluated while NSTEP = 1 or
+// NSTEP < NSTEPS_MAX/4, mf:8082).
+int loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const Params p = params_of(c);
+  if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_loss_span, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis,
+         fmixup);
+  LAUNCH(k_loss_pitch, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p, rf_vis, fmixup);
+  LCHK("loss");
+  return MH_OK;
+}
+
 // One time step: the body of LOOP from label 5000 (mf:3753-5454).
 int step_once(mh_ctx *c) {
   const Geo &g = c->g;
@@ -450,6 +554,8 @@ int step_once(mh_ctx *c) {
     LAUNCH(k_vrms_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("vrms");
   }
+  // viscous forces every NLOS steps, mf:3935
+  if (p.ILOS == 10 && c->nstep % p.NLOS == 0 && (r = loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
   if ((r = tstep(c, d.RO, d.DRO, 1))) return r;
@@ -487,6 +593,8 @@ int step_once(mh_ctx *c) {
     LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
     LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("convergence");
+    // forces brought up to date before the final output, mf:5338-5339
+    if (c->ifend == 1 && p.ILOS == 10 && (r = loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
     if ((r = stepup(c, 0.25f))) return r;
     float sc[8];
     HIPCHK(hipMemcpyAsync(sc, d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
@@ -570,6 +678,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.TMP, N)) || (r = dalloc(c, &d.TEMP1, N)) ||
       (r = dalloc(c, &d.B1CHG, kind_len(dm, K_B1))) || (r = dalloc(c, &d.B2CHG, kind_len(dm, K_B2))) ||
       (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)3 * dm.km)) ||
+      (r = dalloc(c, &d.LTAB, (size_t)dm.nrows * (4 * dm.im + 2 * dm.km))) ||
       (r = dalloc(c, &d.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))) ||
       (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
@@ -655,22 +764,56 @@ double mh_get_scalar(const mh_ctx *c, int id) {
   return c->scal[id];
 }
 
-int mh_start(mh_ctx *c) {
-  if (!c) return MH_E_ARG;
+// Every reference scalar must be set; VISLAM, FTCOND and the library
+// controls are optional.  Then the options this library implements.
+static int check_inputs(mh_ctx *c) {
   for (int i = 0; i < MH_NSCALARS; i++)
-    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER)
+    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_VISLAM && i != MH_S_FTCOND)
       return fail(c, MH_E_STATE, std::string("scalar not set: ") + kScalarNames[i]);
   const Params p = params_of(c);
   if (p.IFGAS != 0) return fail(c, MH_E_UNSUPPORTED, "real-gas properties (IFGAS=1)");
   if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
   if (p.IN_FLOW != 0 || p.IF_REPEAT != 0 || p.IF_RESTART != 0)
     return fail(c, MH_E_UNSUPPORTED, "IN_FLOW, IF_REPEAT or IF_RESTART set");
-  if (p.ILOS != 0) return fail(c, MH_E_UNSUPPORTED, "viscous models (ILOS != 0)");
+  if (p.ILOS != 0 && p.ILOS != 10)
+    return fail(c, MH_E_UNSUPPORTED, "viscous model other than ILOS = 0 or 10 (mixing length)");
+  if (p.ILOS == 10 && p.YPLUSWALL < -0.001f)
+    return fail(c, MH_E_UNSUPPORTED, "Shih wall functions (YPLUSWALL < 0)");
+  if (p.ILOS == 10 && p.NLOS < 1) return fail(c, MH_E_ARG, "NLOS < 1");
   if (p.IMID < 1 || p.IMID > c->g.IM || p.KMID < 1 || p.KMID > c->g.KM) return fail(c, MH_E_ARG, "IMID/KMID");
   c->g.IMID = p.IMID;
   c->g.KMID = p.KMID;
-  int r = build_lists(c);
+  return MH_OK;
+}
+
+int mh_loss_init(mh_ctx *c) {
+  if (!c) return MH_E_ARG;
+  int r = check_inputs(c);
   if (r) return r;
+  const Params p = params_of(c);
+  if (p.ILOS == 0) return MH_OK;
+  if ((r = loss_tables(c))) return r;
+  float vf[2] = {(float)c->scal[MH_S_VISLAM], (float)c->scal[MH_S_FTCOND]};
+  if ((r = h2d(c, c->d.SC + SC_VISLAM, vf, sizeof vf))) return r;
+  if ((r = loss(c, 1.0f, 1.0f, 1))) return r;
+  if ((r = d2h(c, vf, c->d.SC + SC_VISLAM, sizeof vf))) return r;
+  c->scal[MH_S_VISLAM] = vf[0];
+  c->scal[MH_S_FTCOND] = vf[1];
+  c->scal_set[MH_S_VISLAM] = c->scal_set[MH_S_FTCOND] = true;
+  return MH_OK;
+}
+
+int mh_start(mh_ctx *c) {
+  if (!c) return MH_E_ARG;
+  int r = check_inputs(c);
+  if (r) return r;
+  const Params p = params_of(c);
+  if ((r = build_lists(c))) return r;
+  if (p.ILOS == 10) {
+    if ((r = loss_tables(c))) return r;
+    float vf[2] = {(float)c->scal[MH_S_VISLAM], (float)c->scal[MH_S_FTCOND]};
+    if ((r = h2d(c, c->d.SC + SC_VISLAM, vf, sizeof vf))) return r;
+  }
   // LOOP prologue, mf:3674-3714
   const Geo &g = c->g;
   std::vector<float> po1(g.KM), p1(g.KM);

@@ -19,7 +19,7 @@
 
 namespace mh {
 
-enum Kind : int { K_N, K_JK, K_IK, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB };
+enum Kind : int { K_N, K_JK, K_IK, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB, K_RN };
 
 struct ArrayInfo {
   const char *name;
@@ -50,10 +50,15 @@ struct Dev {
   float *STEP1, *STEP2, *STEPSBK;
   int *IB1, *IB2, *KB1, *KB2, *NBLADE, *IND, *INDLE, *INDMIX, *INDTE, *NROW, *JSBLK, *JB1, *JB2;
   int *JSTART, *JMIX, *JLE, *JTE, *JLED, *KTIPS, *KTIPE, *KS1, *KS2;
+  float *DWALLSQ, *Y_PLUS, *WHUB, *WTIP;
+  float *XLLIM_I1, *XLLIM_IM, *XLLIM_K1, *XLLIM_KM, *XLLIM_DWN, *XLLIM_UP;
+  float *ROUGH_H, *ROUGH_T, *ROUGH_L, *ROUGH_U;
+  int *JTRAN_I1, *JTRAN_IM, *JTRAN_K1, *JTRAN_KM;
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
   float *BCW;                        // exit/inlet pitch averages per K (3*KM)
+  float *LTAB;                       // LOSS mixing-length factors, see LT_* below
   double *RED;                       // FP64 partial sums of the reductions
   float *SC;                         // device-side scalars (see SC_* below)
   int *FLAGS;                        // per (I,K) line flags for the density limiter
@@ -63,7 +68,7 @@ struct Dev {
 
 // Slots of Dev::SC, the step's device-resident scalars.
 enum {
-  SC_VRMS = 0, SC_VMAX, SC_EMAX, SC_EAVG, SC_ECONT, SC_VREF, SC_FLOWRAT, SC_FLOWOUT,
+  SC_VRMS = 0, SC_VMAX, SC_EMAX, SC_EAVG, SC_ECONT, SC_VREF, SC_FLOWRAT, SC_FLOWOUT, SC_VISLAM, SC_FTCOND,
   SC_AVG_CHG = 16,          // [NROWS]
   SC_AVG_BLK = 16 + 64,     // [NROWS]
   SC_VLIMIT = 16 + 128,     // [NROWS]
@@ -71,6 +76,18 @@ enum {
   SC_PLIM = 16 + 256,       // [KM], PLIM(K) fixed at the start of LOOP (mf:3684-3686)
   SC_COUNT = 16 + 256       // + KM
 };
+
+// Layout of Dev::LTAB, the per-row mixing-length factors of LOSS
+// (mf:8125-8183): FILAM, FITURB, FIWAKE, FIUP as [NROWS][IM] (pitch tables
+// 0-3), then FKLAM, FKTURB as [NROWS][KM] (span tables 0-1).
+enum { LT_FILAM = 0, LT_FITURB, LT_FIWAKE, LT_FIUP };
+enum { LT_FKLAM = 0, LT_FKTURB };
+__host__ __device__ inline int lt_pitch(int tab, int NR, int I, int NROWS, int IM) {
+  return (tab * NROWS + NR - 1) * IM + I - 1;
+}
+__host__ __device__ inline int lt_span(int tab, int NR, int K, int NROWS, int IM, int KM) {
+  return 4 * NROWS * IM + (tab * NROWS + NR - 1) * KM + K - 1;
+}
 
 // Per-step host scalars handed to the kernels.
 struct StepScalars {
@@ -85,6 +102,8 @@ struct Params {
       VSOUND, FACMIXUP;
   int IMID, KMID, NSTEPS_MAX, NCHANGE, NRSMTH, NSFEXIT, IN_PRESS, IN_VTAN, IN_VR, IPOUT, IFMIX, IR,
       JR, KR, ILOS, NLOS, NMIXUP, ITIMST, IF_RESTART, IFGAS, IN_FLOW, IF_REPEAT, IRBB, KRBB, EXACT;
+  float REYNO, RF_VIS, FTRANS, PRANDTL, YPLUSWALL, YPLAM, YPTURB, CFWALL;
+  int IBOUND;
 };
 
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid

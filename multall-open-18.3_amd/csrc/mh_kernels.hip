@@ -893,6 +893,435 @@ __global__ void k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Param
 }
 
 // ---------------------------------------------------------------------------
+// LOSS, mf:8058-9066: mixing-length eddy viscosity and log-law wall friction
+// turned into the body forces XFORCE, RFORCE, TFORCE, QSOURCE by a thin
+// shear layer approximation.  YPLUSWALL >= -0.001 (no Shih wall functions),
+// KM > 2.  k_loss_span forms the stresses on the K = const faces (one thread
+// per (I,J) column, streaming up the column three times), k_loss_pitch those
+// on the I = const faces and the blade surfaces (one wavefront per (J,K)
+// line, the line held in LDS).  The entropy-generation and Reynolds-number
+// diagnostics of the reference feed printed output only and are not formed.
+
+// laminar viscosity and FTCOND while NSTEP = 1 or NSTEP < NSTEPS_MAX/4,
+// mf:8082-8109 (one thread)
+__global__ void k_loss_vislam(Geo g, Dev d, Params p) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float VISLAM = d.SC[SC_VISLAM];
+  if (p.REYNO > 100.f) {
+    int J1 = d.JLE[0], J2 = d.JTE[0];
+    if (d.JLE[0] > g.JM) J1 = 1;
+    if (d.JTE[0] > g.JM) J2 = g.JM;
+    const float XCHORD = d.SMERID[JKX(J2, p.KMID)] - d.SMERID[JKX(J1, p.KMID)];
+    const unsigned n = IX(p.IMID, J2, p.KMID);
+    const float a = d.ROVX[n], b = d.ROVR[n], c = d.ROWT[n];
+    const float ROW2 = sqrtf(a * a + b * b + c * c);
+    VISLAM = XCHORD * ROW2 / p.REYNO;
+  }
+  if (p.REYNO > 0.0f && p.REYNO < 99.99f) VISLAM = p.REYNO / 100000.f;
+  if (p.REYNO < 0.0f) VISLAM = -p.REYNO * 1.0e-5f;
+  const float TCOND = p.CP * VISLAM / p.PRANDTL;
+  d.SC[SC_VISLAM] = VISLAM;
+  d.SC[SC_FTCOND] = TCOND / VISLAM;
+}
+
+// Skin-friction coefficient of a wall cell, mf:8287-8329 (log-law curve fit,
+// roughness correction when ROUGH > 1e-7, at least laminar).  RO and WB are
+// the density and speed the wall stress is formed with.
+__device__ float loss_cf(float RE, float PERP, float ROUGH, float RO, float WB, float VISLAM) {
+  const float RELOG = 1.0f / logf(RE);
+  float CF;
+  if (ROUGH > 1.0e-7f) {
+    if (ROUGH > PERP) ROUGH = PERP;
+    float REK = RE * ROUGH / PERP;
+    REK = REK - 80.0f;
+    if (REK < 0.0f) REK = 0.0f;
+    const float REKSQ = REK * REK;
+    const float A1 = -.00178493f + .0000814923f * REK + .000000150445f * REKSQ;
+    const float A2 = .029072f - .001584f * REK - .00000225194f * REKSQ;
+    const float A3 = .270313f + .0091409f * REK + .00000451537f * REKSQ + .00000000464767f * REKSQ * REK;
+    CF = A1 + A2 * RELOG + A3 * RELOG * RELOG;
+  } else {
+    CF = -0.00178493f + 0.029072f * RELOG + 0.270313f * RELOG * RELOG;
+  }
+  const float CFLAM = 2.0f / RE;
+  CF = fmaxf(CF, CFLAM);
+  if (RE < 125.f) CF = CFLAM;
+  if (ROUGH > 1.0e-7f) {
+    const float TW = 0.5f * CF * RO * WB * WB;
+    const float VSTAR = sqrtf(TW / RO);
+    const float PLUSK = ROUGH * VSTAR * RO / VISLAM;
+    const float RELOG10 = log10f(PERP / ROUGH);
+    const float FUNCN = 5.75f * RELOG10 + 8.5f;
+    const float CF_FULL_ROUGH = 2.0f / (FUNCN * FUNCN);
+    if (PLUSK > 45.f && CF > CF_FULL_ROUGH) CF = CF_FULL_ROUGH;
+  }
+  return CF;
+}
+
+// turbulent viscosity damping in the sublayer and buffer region, mf:8496-8501
+__device__ __forceinline__ float loss_fyplus(float yp, float YPLAM, float YPTURB, float prev) {
+  float f = prev;
+  if (yp <= YPLAM) f = 0.0f;
+  if (yp > YPLAM) {
+    float XFAC = (yp - YPLAM) / (YPTURB - YPLAM);
+    if (XFAC > 1.0f) XFAC = 1.0f;
+    f = XFAC * XFAC * (3.0f - 2.0f * XFAC);
+  }
+  return f;
+}
+
+// face averages of one K level of an (I,J) column, mf:8220-8235
+struct LossQ {
+  float vx, vr, vt, ro, t, area, ravg, wabs;
+};
+
+__device__ __forceinline__ LossQ loss_q_span(const Geo &g, const Dev &d, int I, int J, int K, float WREL) {
+  LossQ q;
+  const unsigned a = IX(I, J, K), b = IX(I, J - 1, K), c = IX(I + 1, J, K), e = IX(I + 1, J - 1, K);
+  const float ax = d.ASX[a], ar = d.ASR[a];
+  q.area = sqrtf(ax * ax + ar * ar);
+  q.vx = 0.25f * (d.VX[a] + d.VX[b] + d.VX[c] + d.VX[e]);
+  q.vr = 0.25f * (d.VR[a] + d.VR[b] + d.VR[c] + d.VR[e]);
+  q.vt = 0.25f * (d.VT[a] + d.VT[b] + d.VT[c] + d.VT[e]);
+  q.ro = 0.25f * (d.RO[a] + d.RO[b] + d.RO[c] + d.RO[e]);
+  q.t = 0.25f * (d.T_STATIC[a] + d.T_STATIC[b] + d.T_STATIC[c] + d.T_STATIC[e]);
+  q.ravg = 0.5f * (d.R[JKX(J, K)] + d.R[JKX(J - 1, K)]);
+  const float wt = q.vt - WREL * q.ravg;
+  q.wabs = sqrtf(q.vx * q.vx + q.vr * q.vr + wt * wt);
+  return q;
+}
+
+// speed relative to the end wall, mf:8231-8234
+__device__ __forceinline__ float loss_wbound(const LossQ &q, float W) {
+  const float wtb = q.vt - W * q.ravg;
+  return sqrtf(q.vx * q.vx + q.vr * q.vr + wtb * wtb);
+}
+
+struct LossS {
+  float x, r, t, q, w;
+};
+
+// Stresses on the K = const faces, forces relaxed by RF_VIS, mf:8201-8556.
+__global__ void k_loss_span(Geo g, Dev d, Params p, float RF_VIS, float FMIXUP) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
+  if (I > g.IMM1 || J > g.JM) return;
+  const int KM = g.KM, KMM1 = g.KMM1, KMID = p.KMID;
+  const int NRW = d.NROW[J - 1], JROW = J - d.JSTART[NRW - 1] + 1;
+  const float WREL = d.WRAD[J - 1], WHUB = d.WHUB[J - 1], WTIP = d.WTIP[J - 1];
+  const float RF_VIS1 = 1.f - RF_VIS;
+  // end-wall friction, mf:8238-8418
+  const LossQ q1 = loss_q_span(g, d, I, J, 1, WREL), q2 = loss_q_span(g, d, I, J, 2, WREL);
+  const LossQ qm1 = loss_q_span(g, d, I, J, KMM1, WREL), qm = loss_q_span(g, d, I, J, KM, WREL);
+  float VISLAM = d.SC[SC_VISLAM], FTCOND = d.SC[SC_FTCOND];
+  if (p.REYNO < 0.0001f) {
+    const LossQ qk = loss_q_span(g, d, I, J, KMID, WREL);
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * powf(qk.t / 288.0f, 0.62f);
+    const float TCOND = p.CP * VISLAM / p.PRANDTL;
+    FTCOND = TCOND / VISLAM;
+  }
+  const float wb1 = loss_wbound(q1, WHUB), wb2 = loss_wbound(q2, 2 > KMID ? WTIP : WHUB);
+  const float wbm1 = loss_wbound(qm1, KMM1 > KMID ? WTIP : WHUB), wbm = loss_wbound(qm, WTIP);
+  float TWALLK1, TWALLKM;
+  if (p.YPLUSWALL < 5.f) {
+    const float PERP = d.VOL[IX(I, J, 1)] / q1.area;
+    const float RE = PERP * q2.ro * wb2 / VISLAM;
+    const float CF = loss_cf(RE, PERP, d.ROUGH_H[NRW - 1], q2.ro, wb2, VISLAM);
+    TWALLK1 = 0.5f * CF * q2.ro * wb2 * wb2;
+    const float PERPM = d.VOL[IX(I, J, KMM1)] / qm.area;
+    const float REM = PERPM * qm1.ro * wbm1 / VISLAM;
+    const float CFM = loss_cf(REM, PERPM, d.ROUGH_T[NRW - 1], qm1.ro, wbm1, VISLAM);
+    TWALLKM = 0.5f * CFM * qm1.ro * wbm1 * wbm1;
+  } else {
+    TWALLK1 = p.CFWALL * q1.ro * wb1 * wb1;
+    TWALLKM = p.CFWALL * qm.ro * wbm * wbm;
+  }
+  const float VSTAR1 = sqrtf(TWALLK1 / q1.ro), VSTARM = sqrtf(TWALLKM / qm.ro);
+  LossS s1, sm;
+  {
+    float FMULT = TWALLK1 * q1.area / wb1;
+    if (p.IBOUND == 1 || p.IBOUND > 2) FMULT = 0.0f;
+    s1.x = FMULT * q1.vx;
+    s1.r = FMULT * q1.vr;
+    s1.t = FMULT * (q1.vt - WHUB * q1.ravg);
+    FMULT = -TWALLKM * qm.area / wbm;
+    if (p.IBOUND >= 2) FMULT = 0.0f;
+    sm.x = FMULT * qm.vx;
+    sm.r = FMULT * qm.vr;
+    sm.t = FMULT * (qm.vt - WTIP * qm.ravg);
+    s1.w = s1.t * WHUB * q1.ravg;
+    s1.q = 0.0f;
+    sm.w = sm.t * WTIP * qm.ravg;
+    sm.q = 0.0f;
+  }
+  // y+ from the end walls (mf:8420-8435), the viscosities and the
+  // transition test (mf:8437-8500)
+  auto yplus = [&](int K) -> float {
+    const float PERP = sqrtf(d.DWALLSQ[IX(I, J, K)]);
+    return (K <= KMID) ? fminf(VSTAR1 * PERP * q1.ro / VISLAM, 1000.0f) : fminf(VSTARM * PERP * qm.ro / VISLAM, 1000.0f);
+  };
+  d.Y_PLUS[IX(I, J, 1)] = yplus(1);
+  float RMAXH = 0.0f, RMAXT = 0.0f, RKMID = 0.0f;
+  {
+    LossQ qa = q1, qb = q2;
+    float fy = 0.0f;
+    for (int K = 2; K <= KMM1; K++) {
+      const LossQ qc = loss_q_span(g, d, I, J, K + 1, WREL);
+      const float yp = yplus(K);
+      d.Y_PLUS[IX(I, J, K)] = yp;
+      fy = loss_fyplus(yp, p.YPLAM, p.YPTURB, fy);
+      const float DPERP = d.VOL[IX(I, J, K)] / qb.area;
+      const float VLAM = d.LTAB[lt_span(LT_FKLAM, NRW, K, g.NROWS, g.IM, KM)] * VISLAM / DPERP;
+      float VTURB = d.LTAB[lt_span(LT_FKTURB, NRW, K, g.NROWS, g.IM, KM)] * qb.ro * fabsf(qc.wabs - qa.wabs) * FMIXUP;
+      VTURB = VTURB * fy;
+      const float RATVIS = VTURB / VLAM;
+      if (K <= KMID && !(RATVIS < RMAXH)) RMAXH = RATVIS;
+      if (K == KMID) RKMID = RATVIS;
+      if (K > KMID && !(RATVIS < RMAXT)) RMAXT = RATVIS;
+      qa = qb;
+      qb = qc;
+    }
+  }
+  const bool keep_hub = (JROW >= d.JTRAN_K1[NRW - 1]) && (RMAXH > p.FTRANS);
+  if (keep_hub && !(RKMID < RMAXT)) RMAXT = RKMID;  // the casing test re-reads VTURB(KMID)
+  const bool keep_tip = (JROW >= d.JTRAN_KM[NRW - 1]) && (RMAXT > p.FTRANS);
+  // stresses (mf:8503-8522) and forces (mf:8526-8535), streaming up in K
+  LossS sk = s1;
+  float ravgk = q1.ravg;
+  LossQ qa = q1, qb = q2;
+  float fy = 0.0f;
+  for (int K = 1; K <= KMM1; K++) {
+    LossS sn;
+    float ravgn;
+    if (K + 1 <= KMM1) {
+      const int KN = K + 1;
+      const LossQ qc = loss_q_span(g, d, I, J, KN + 1, WREL);
+      const float yp = yplus(KN);
+      fy = loss_fyplus(yp, p.YPLAM, p.YPTURB, fy);
+      const float DPERP = d.VOL[IX(I, J, KN)] / qb.area;
+      const float VLAM = d.LTAB[lt_span(LT_FKLAM, NRW, KN, g.NROWS, g.IM, KM)] * VISLAM / DPERP;
+      float VTURB = d.LTAB[lt_span(LT_FKTURB, NRW, KN, g.NROWS, g.IM, KM)] * qb.ro * fabsf(qc.wabs - qa.wabs) * FMIXUP;
+      VTURB = VTURB * fy;
+      if ((KN <= KMID && !keep_hub) || (KN >= KMID && !keep_tip)) VTURB = 0.0f;
+      const float FM = qb.area * (VLAM + VTURB);
+      sn.x = FM * (qc.vx - qa.vx);
+      sn.r = FM * (qc.vr - qa.vr);
+      const float VISTOT = VISLAM * (1.0f + VTURB / VLAM);
+      sn.t = FM * (qc.vt - qa.vt) - VISTOT * qb.area * qb.vt / qb.ravg;
+      sn.q = FM * FTCOND * (qc.t - qa.t);
+      sn.w = sn.x * qb.vx + sn.t * qb.vt + sn.r * qb.vr;
+      ravgn = qb.ravg;
+      qa = qb;
+      qb = qc;
+    } else {
+      sn = sm;
+      ravgn = qm.ravg;
+    }
+    const unsigned n = IX(I, J, K);
+    d.XFORCE[n] = RF_VIS1 * d.XFORCE[n] + RF_VIS * (sk.x - sn.x);
+    d.RFORCE[n] = RF_VIS1 * d.RFORCE[n] + RF_VIS * (sk.r - sn.r);
+    d.TFORCE[n] = RF_VIS1 * d.TFORCE[n] + RF_VIS * (sk.t * ravgk - sn.t * ravgn);
+    d.QSOURCE[n] = RF_VIS1 * d.QSOURCE[n] + RF_VIS * (sk.q - sn.q + sk.w - sn.w);
+    sk = sn;
+    ravgk = ravgn;
+  }
+}
+
+// wavefront max of x over lanes (non-NaN values; the order does not matter)
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float y = __shfl_xor(x, off, 64);
+    x = (y > x) ? y : x;
+  }
+  return x;
+}
+
+// Stresses on the blade surfaces and the I = const faces, added to the
+// forces, mf:8562-9001.  One 64-thread workgroup per (J,K) line.
+constexpr int kLossLine = 260;  // IM <= 257
+
+__global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float RF_VIS, float FMIXUP) {
+  __shared__ float AREA[kLossLine], VXA[kLossLine], VRA[kLossLine], VTA[kLossLine], WTA[kLossLine];
+  __shared__ float ROA[kLossLine], TA[kLossLine], WABS[kLossLine];
+  __shared__ float SX[kLossLine], SR[kLossLine], ST[kLossLine], SQ[kLossLine], SW[kLossLine];
+  __shared__ float VLM[kLossLine], VTU[kLossLine], YP[kLossLine];
+  const int K = blockIdx.x + 1, J = blockIdx.y + 2;
+  const int IM = g.IM, IMM1 = g.IMM1, IMID = p.IMID, lane = threadIdx.x;
+  const int NRW = d.NROW[J - 1], JROW = J - d.JSTART[NRW - 1] + 1;
+  const int JLEDGE = d.JLE[NRW - 1], JTEDGE = d.JTE[NRW - 1];
+  const float WREL = d.WRAD[J - 1], RAVGK = d.RAVG_CELL[JKX(J, K)], abt = d.ABT[JKX(J, K)];
+  const int kts = d.KTIPS[NRW - 1];
+  const bool tipgap = kts > 0 && K >= kts && K < d.KTIPE[NRW - 1];
+  for (int I = lane + 1; I <= IM; I += 64) {
+    const unsigned a = IX(I, J, K), b = IX(I, J - 1, K), c = IX(I, J - 1, K + 1), e = IX(I, J, K + 1);
+    const float bx = d.ABX[a], br = d.ABR[a];
+    AREA[I] = sqrtf(bx * bx + br * br + abt * abt);
+    const float vx = 0.25f * (d.VX[a] + d.VX[b] + d.VX[c] + d.VX[e]);
+    const float vr = 0.25f * (d.VR[a] + d.VR[b] + d.VR[c] + d.VR[e]);
+    const float wt = 0.25f * (d.WT[a] + d.WT[b] + d.WT[c] + d.WT[e]);
+    VXA[I] = vx;
+    VRA[I] = vr;
+    VTA[I] = 0.25f * (d.VT[a] + d.VT[b] + d.VT[c] + d.VT[e]);
+    WTA[I] = wt;
+    ROA[I] = 0.25f * (d.RO[a] + d.RO[b] + d.RO[c] + d.RO[e]);
+    TA[I] = 0.25f * (d.T_STATIC[a] + d.T_STATIC[b] + d.T_STATIC[c] + d.T_STATIC[e]);
+    WABS[I] = sqrtf(vx * vx + vr * vr + wt * wt);
+  }
+  __syncthreads();
+  float VISLAM = d.SC[SC_VISLAM], FTCOND = d.SC[SC_FTCOND];
+  if (p.REYNO < 0.0001f) {
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * powf(TA[IMID] / 288.0f, 0.62f);
+    const float TCOND = p.CP * VISLAM / p.PRANDTL;
+    FTCOND = TCOND / VISLAM;
+  }
+  if (!(J <= JLEDGE || J > JTEDGE)) {
+    // blade-surface friction and y+, mf:8633-8807
+    float VSTAR1 = 0.0f, VSTARM = 0.0f;
+    if (!tipgap) {
+      float TWALLI1, TWALLIM;
+      if (p.YPLUSWALL < 5.f) {
+        const float PERP = d.VOL[IX(1, J, K)] / AREA[1];
+        const float RE = ROA[2] * WABS[2] * PERP / VISLAM;
+        const float CF = loss_cf(RE, PERP, d.ROUGH_L[NRW - 1], ROA[2], WABS[2], VISLAM);
+        TWALLI1 = 0.5f * CF * ROA[2] * WABS[2] * WABS[2];
+        const float PERPM = d.VOL[IX(IMM1, J, K)] / AREA[IM];
+        const float REM = ROA[IMM1] * WABS[IMM1] * PERPM / VISLAM;
+        const float CFM = loss_cf(REM, PERPM, d.ROUGH_U[NRW - 1], ROA[IMM1], WABS[IMM1], VISLAM);
+        TWALLIM = 0.5f * CFM * ROA[IMM1] * WABS[IMM1] * WABS[IMM1];
+      } else {
+        TWALLI1 = p.CFWALL * ROA[1] * WABS[1] * WABS[1];
+        TWALLIM = p.CFWALL * ROA[IM] * WABS[IM] * WABS[IM];
+      }
+      VSTAR1 = sqrtf(TWALLI1 / ROA[1]);
+      VSTARM = sqrtf(TWALLIM / ROA[IM]);
+      if (lane == 0) {
+        float FM = TWALLI1 * AREA[1] / WABS[1];
+        SX[1] = FM * VXA[1];
+        SR[1] = FM * VRA[1];
+        ST[1] = FM * WTA[1];
+        FM = -TWALLIM * AREA[IM] / WABS[IM];
+        SX[IM] = FM * VXA[IM];
+        SR[IM] = FM * VRA[IM];
+        ST[IM] = FM * WTA[IM];
+        SQ[1] = 0.0f;
+        SW[1] = ST[1] * WREL * RAVGK;
+        SQ[IM] = 0.0f;
+        SW[IM] = ST[IM] * WREL * RAVGK;
+      }
+    }
+    // y+ including the blade surfaces (mf:8768-8783), kept in LDS
+    for (int I = lane + 1; I <= IMM1; I += 64) {
+      const unsigned n = IX(I, J, K);
+      float yp = d.Y_PLUS[n];
+      if (!tipgap) {
+        const float YPLS = (I <= IMID) ? VSTAR1 * sqrtf(d.DWALLSQ[n]) * ROA[1] / VISLAM
+                                       : VSTARM * sqrtf(d.DWALLSQ[n]) * ROA[IM] / VISLAM;
+        yp = fminf(YPLS, yp);
+        d.Y_PLUS[n] = yp;
+      }
+      YP[I] = yp;
+    }
+    __syncthreads();
+    // viscosities and transition, mf:8813-8869
+    float rlo = 0.0f, rhi = 0.0f, rmid = 0.0f;
+    for (int I = lane + 2; I <= IMM1; I += 64) {
+      const unsigned n = IX(I, J, K);
+      const float fy = loss_fyplus(YP[I], p.YPLAM, p.YPTURB, 0.0f);
+      const float DPERP = d.VOL[n] / AREA[I];
+      const float VLAM = d.LTAB[lt_pitch(LT_FILAM, NRW, I, g.NROWS, IM)] * VISLAM / DPERP;
+      float VTURB = d.LTAB[lt_pitch(LT_FITURB, NRW, I, g.NROWS, IM)] * ROA[I] * fabsf(WABS[I + 1] - WABS[I - 1]) * FMIXUP;
+      VTURB = VTURB * fy;
+      VLM[I] = VLAM;
+      VTU[I] = VTURB;
+      const float RATVIS = VTURB / VLAM;
+      if (I <= IMID) rlo = (RATVIS > rlo) ? RATVIS : rlo;
+      if (I == IMID) rmid = RATVIS;
+      if (I > IMID) rhi = (RATVIS > rhi) ? RATVIS : rhi;
+    }
+    rlo = wave_max(rlo);
+    rhi = wave_max(rhi);
+    rmid = wave_max(rmid);  // only one lane holds I = IMID; the others hold 0
+    const bool keep_lo = (JROW >= d.JTRAN_I1[NRW - 1]) && (rlo > p.FTRANS);
+    if (keep_lo) rhi = (rmid > rhi) ? rmid : rhi;  // the upper test re-reads VTURB(IMID)
+    const bool keep_hi = (JROW >= d.JTRAN_IM[NRW - 1]) && (rhi > p.FTRANS);
+    // stresses, mf:8873-8883
+    for (int I = lane + 2; I <= IMM1; I += 64) {
+      float VTURB = VTU[I];
+      if ((I <= IMID && !keep_lo) || (I >= IMID && !keep_hi)) VTURB = 0.0f;
+      const float FM = (VLM[I] + VTURB) * AREA[I];
+      SX[I] = FM * (VXA[I + 1] - VXA[I - 1]);
+      SR[I] = FM * (VRA[I + 1] - VRA[I - 1]);
+      ST[I] = FM * (WTA[I + 1] - WTA[I - 1]);
+      SQ[I] = FM * FTCOND * (TA[I + 1] - TA[I - 1]);
+      SW[I] = SX[I] * VXA[I] + SR[I] * VRA[I] + ST[I] * VTA[I];
+    }
+    __syncthreads();
+    if (tipgap && lane == 0) {  // periodic stresses above the tip, mf:8889-8905
+      SX[1] = 0.5f * (SX[2] + SX[IMM1]);
+      SR[1] = 0.5f * (SR[2] + SR[IMM1]);
+      ST[1] = 0.5f * (ST[2] + ST[IMM1]);
+      SX[IM] = SX[1];
+      ST[IM] = ST[1];
+      SR[IM] = SR[1];
+      SQ[1] = 0.5f * (SQ[2] + SQ[IMM1]);
+      SW[1] = 0.5f * (SW[2] + SW[IMM1]);
+      SQ[IM] = SQ[1];
+      SW[IM] = SW[1];
+    }
+  } else {
+    // upstream of the leading edge and in the wake, mf:8912-8977
+    const bool wake = J > JTEDGE;
+    float FACJ = 0.1f * (float)(J - JTEDGE);
+    if (FACJ > 1.f) FACJ = 1.f;
+    for (int I = lane + 1; I <= IM; I += 64) {
+      const int ISUB = (I == IM) ? IMM1 : I;
+      const int IM1 = (I == 1) ? IMM1 : I - 1, IP1 = (I == IM) ? 2 : I + 1;
+      const float DPERP = d.VOL[IX(ISUB, J, K)] / AREA[I];
+      const float VLAM = d.LTAB[lt_pitch(LT_FILAM, NRW, I, g.NROWS, IM)] * VISLAM / DPERP;
+      float VTURB;
+      if (wake) {
+        const float FBLEND = (FACJ * d.LTAB[lt_pitch(LT_FIWAKE, NRW, I, g.NROWS, IM)] +
+                              (1.f - FACJ) * d.LTAB[lt_pitch(LT_FITURB, NRW, I, g.NROWS, IM)]) * FMIXUP;
+        VTURB = FBLEND * ROA[I] * fabsf(WABS[IP1] - WABS[IM1]);
+      } else {
+        VTURB = FMIXUP * d.LTAB[lt_pitch(LT_FIUP, NRW, I, g.NROWS, IM)] * ROA[I] * fabsf(WABS[IP1] - WABS[IM1]);
+      }
+      const float FM = AREA[I] * (VLAM + VTURB);
+      const float sx = FM * (VXA[IP1] - VXA[IM1]), sr = FM * (VRA[IP1] - VRA[IM1]);
+      const float st = FM * (WTA[IP1] - WTA[IM1]);
+      SX[I] = sx;
+      SR[I] = sr;
+      ST[I] = st;
+      SQ[I] = FM * FTCOND * (TA[IP1] - TA[IM1]);
+      SW[I] = sx * VXA[I] + st * VTA[I] + sr * VRA[I];
+    }
+    __syncthreads();
+    if (lane == 0) {  // periodic, mf:8982-8991
+      SX[1] = 0.5f * (SX[IM] + SX[1]);
+      SR[1] = 0.5f * (SR[IM] + SR[1]);
+      ST[1] = 0.5f * (ST[IM] + ST[1]);
+      SX[IM] = SX[1];
+      SR[IM] = SR[1];
+      ST[IM] = ST[1];
+      SQ[1] = 0.5f * (SQ[IM] + SQ[1]);
+      SW[1] = 0.5f * (SW[IM] + SW[1]);
+      SQ[IM] = SQ[1];
+      SW[IM] = SW[1];
+    }
+  }
+  __syncthreads();
+  // forces, mf:9005-9012
+  for (int I = lane + 1; I <= IMM1; I += 64) {
+    const unsigned n = IX(I, J, K);
+    d.XFORCE[n] = d.XFORCE[n] + RF_VIS * (SX[I] - SX[I + 1]);
+    d.RFORCE[n] = d.RFORCE[n] + RF_VIS * (SR[I] - SR[I + 1]);
+    d.TFORCE[n] = d.TFORCE[n] + RF_VIS * (ST[I] - ST[I + 1]) * RAVGK;
+    d.QSOURCE[n] = d.QSOURCE[n] + RF_VIS * (SQ[I] - SQ[I + 1] + SW[I] - SW[I + 1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // limiters, mf:5115-5168
 
 __global__ void k_fill_int(int *p, int n, int v) {

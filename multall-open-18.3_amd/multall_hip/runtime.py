@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmultall_hip.so")
 
 EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar_id", "mh_array_len",
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
-           "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_start", "mh_step",
+           "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_start", "mh_step",
            "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report"]
 
 
@@ -70,6 +70,7 @@ def load(path: str = LIB_PATH):
     L.mh_get_scalar.argtypes = [vp, i]
     L.mh_get_scalar.restype = ctypes.c_double
     L.mh_start.argtypes = [vp]
+    L.mh_loss_init.argtypes = [vp]
     L.mh_step.argtypes = [vp, i, ctypes.POINTER(MhResid)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
     L.mh_sync.argtypes = [vp]
@@ -122,8 +123,15 @@ class Solver:
             msg = self.L.mh_last_error(self.h).decode(errors="replace") if self.h else ""
             raise MhError(f"{what}: rc={rc} {msg}")
 
+    def loss_init(self):
+        """The SETUP-time viscous-force initialisation (one LOSS call)."""
+        self._check(self.L.mh_loss_init(self.h), "mh_loss_init")
+
     def start(self):
         self._check(self.L.mh_start(self.h), "mh_start")
+
+    def scalar(self, name: str) -> float:
+        return float(self.L.mh_get_scalar(self.h, self.L.mh_scalar_id(name.encode())))
 
     def step(self, n: int) -> Dict:
         r = MhResid()

@@ -17,7 +17,12 @@ import orc
 from conftest import load_ref, load_state
 
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP"]
+FORCES = ["XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
 TOL = 1e-5
+INVISCID = ["tiny1_inv", "tiny3_inv"]
+# mixing-length (LOSS) decks: chord-Reynolds and power-law viscosity, smooth
+# and rough walls, rotor tip gap, fixed wall y+ with a shear-free hub
+VISCOUS = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw"]
 
 
 def _solver(st, exact):
@@ -40,7 +45,7 @@ def _scales(ref):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["tiny1_inv", "tiny3_inv"])
+@pytest.mark.parametrize("case", INVISCID + VISCOUS)
 def test_one_step_vs_oracle_exact_order(case):
     st = load_state(case)
     st["scalars"]["NSTEPS_MAX"] = 1
@@ -55,11 +60,41 @@ def test_one_step_vs_oracle_exact_order(case):
         err = _nerr(a, b)
         print(f"{case} {v}: {ndiff} differing values, normalized max err {err:.2e}")
         assert err <= 2e-6, (v, err)
+    # the step ends the run, so LOSS has refreshed the forces (logf and powf
+    # differ by ulps between device and host libm)
+    for v in FORCES:
+        err = _nerr(g.get(v), o[v])
+        print(f"{case} {v}: normalized max err {err:.2e}")
+        assert err <= 1e-5, (v, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", VISCOUS)
+def test_loss_call_vs_oracle(case):
+    """One LOSS call from the post-SETUP state (RF_VIS = 1, FMIXUP = 1,
+    NSTEP = 1, as SETUP makes it): forces, y+ and the laminar viscosity."""
+    st = load_state(case)
+    st["scalars"]["NSTEPS_MAX"] = 100
+    o = orc.Oracle(st)
+    for k, v in (("RF_VIS", 1.0), ("FMIXUP", 1.0), ("NSTEP", 1)):
+        o.lib.orc_scalar(o.h, k.encode(), float(v))
+    o.lib.orc_loss(o.h)
+    from multall_hip.runtime import Solver
+    g = Solver(st, exact_order=True)
+    g.loss_init()
+    vis = o.scalar("VISLAM")
+    if st["scalars"]["REYNO"] >= 0.0001:
+        assert abs(g.scalar("VISLAM") - vis) <= 1e-6 * abs(vis), (g.scalar("VISLAM"), vis)
+    for v in FORCES + ["Y_PLUS"]:
+        a, b = g.get(v), o[v]
+        err = _nerr(a, b)
+        print(f"{case} {v}: {int(np.count_nonzero(a != b))} differing values, normalized max err {err:.2e}")
+        assert err <= 1e-5, (v, err)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("exact", [False, True])
-@pytest.mark.parametrize("case", ["tiny1_inv", "tiny3_inv"])
+@pytest.mark.parametrize("case", INVISCID + VISCOUS)
 @pytest.mark.parametrize("nsteps", [1, 100])
 def test_vs_reference(case, nsteps, exact):
     st = load_state(case)
