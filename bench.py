@@ -110,6 +110,7 @@ def main():
     os.unlink(deck_path)
     assert st["scalars"]["IM"] * st["scalars"]["JM"] * st["scalars"]["KM"] == nodes
     g = Solver(st, exact_order=False)
+    g.loss_init()          # SETUP's initial viscous forces (no-op when inviscid)
     g.start()
     g.step(args.warmup)
     g.sync()

@@ -120,6 +120,9 @@ contains
     case ('RGAS'); v = RGAS
     case ('VSOUND'); v = VSOUND
     case ('VORT_MAX'); v = VORT_MAX
+    case ('YPLAM'); v = YPLAM
+    case ('YPTURB'); v = YPTURB
+    case ('CFWALL'); v = CFWALL
     case default
       v = 0; mhh_scalar = -1
     end select
@@ -176,6 +179,25 @@ contains
     case ('FLOWX'); call putr3(FLOWX)
     case ('FLOWT'); call putr3(FLOWT)
     case ('FLOWR'); call putr3(FLOWR)
+    case ('DWALLSQ'); call putr3(DWALLSQ)
+    case ('XLENGTH'); call putr3(XLENGTH)
+    case ('FST_RAT'); call putr3(FST_RAT)
+    case ('WHUB'); call putr1(WHUB)
+    case ('WTIP'); call putr1(WTIP)
+    case ('XLLIM_I1'); call putr1(XLLIM_I1)
+    case ('XLLIM_IM'); call putr1(XLLIM_IM)
+    case ('XLLIM_K1'); call putr1(XLLIM_K1)
+    case ('XLLIM_KM'); call putr1(XLLIM_KM)
+    case ('XLLIM_DWN'); call putr1(XLLIM_DWN)
+    case ('XLLIM_UP'); call putr1(XLLIM_UP)
+    case ('ROUGH_H'); call putr1(ROUGH_H)
+    case ('ROUGH_T'); call putr1(ROUGH_T)
+    case ('ROUGH_L'); call putr1(ROUGH_L)
+    case ('ROUGH_U'); call putr1(ROUGH_U)
+    case ('JTRAN_I1'); call puti(JTRAN_I1)
+    case ('JTRAN_IM'); call puti(JTRAN_IM)
+    case ('JTRAN_K1'); call puti(JTRAN_K1)
+    case ('JTRAN_KM'); call puti(JTRAN_KM)
     case ('THETA'); call putr3(THETA)
     case ('RTHETA'); call putr3(RTHETA)
     case ('AQTOT'); call putr3(AQTOT)

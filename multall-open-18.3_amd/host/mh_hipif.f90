@@ -54,6 +54,10 @@ module mh_hipif
       integer(c_int), value :: id
       real(c_double), value :: v
     end function mh_set_scalar
+    integer(c_int) function mh_loss_init(ctx) bind(C, name='mh_loss_init')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function mh_loss_init
     integer(c_int) function mh_start(ctx) bind(C, name='mh_start')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

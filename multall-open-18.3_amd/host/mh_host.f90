@@ -77,6 +77,8 @@ module mh_host
 
   ! ---- multigrid block time steps ------------------------------------------
   real, allocatable, save, public :: STEP1(:,:,:), STEP2(:,:,:), STEPSBK(:)
+  ! wall distances and mixing lengths (SET_XLENGTH)
+  real, allocatable, save, public :: DWALLSQ(:,:,:), XLENGTH(:,:,:), FST_RAT(:,:,:), XLIMIT(:)
 
 contains
 
@@ -1342,7 +1344,186 @@ contains
     dxm = X(JM, KM) - X(JM, 1)
     avgspan = 0.5*(sqrt(dx1*dx1 + dr1*dr1) + sqrt(dxm*dxm + drm*drm))
     VORT_MAX = 5000.*VM1(KMID)/avgspan
+    ! wall distances and mixing lengths; the viscous models use them
+    ! (mf:7340), an inviscid run does not
+    if (ILOS /= 0) then
+      call set_xlength()
+    else
+      allocate (DWALLSQ(IM, JM, KM), XLENGTH(IM, JM, KM), FST_RAT(IM, JM, KM), XLIMIT(JM))
+      DWALLSQ = 0.0; XLENGTH = 0.0; FST_RAT = 0.0; XLIMIT = 0.0
+    end if
   end function setup
+
+  !> Distance of every node to the nearest end wall and blade surface, the
+  !> mixing lengths derived from it and the squared cell wall distance,
+  !> restating SET_XLENGTH (mf:12194-12517): the nearest wall point is sought
+  !> over J +/- 20 (and K +/- 5 on the blades), and the distance is taken
+  !> normal to the wall there.
+  subroutine set_xlength()
+    integer, parameter :: jrange = 20, krange = 5
+    real, allocatable :: dist_min(:,:,:), xl(:,:,:), t3(:,:,:)
+    integer :: i, j, k, nr, iwall, j1, j2, k1, k2, jwall, jsurf, ksurf, jmin, kmin, knorm, if_found, iside, isurf
+    real :: pitch, dist_ref, hubdist, tipdist, dmin, xdif, rdif, distsq, atot, xnorm, rnorm, tnorm, tdif, tdifsq
+    real :: endwall_dist, blade_dist, sdist(2), distsqrt, xllim, dblend, expon, fblend, x_length, ratio
+    real :: avg_xlength, avg_fstrat, avgdist
+    allocate (dist_min(IM, JM, KM), xl(IM, JM, KM), t3(IM, JM, KM), XLIMIT(JM))
+    allocate (DWALLSQ(IM, JM, KM), XLENGTH(IM, JM, KM), FST_RAT(IM, JM, KM))
+    DWALLSQ = 0.0; XLENGTH = 0.0; FST_RAT = 0.0; XLIMIT = 0.0
+    xllim = 0.0; jmin = 1; kmin = 1
+    do k = 1, KM
+      do j = 1, JM
+        nr = NROW(j)
+        pitch = 6.283185*R(j, k)/NBLADE(j)
+        dist_ref = 0.5*pitch
+        do i = 1, IM
+          ! end walls: nearest hub / casing point over J +/- jrange, then the
+          ! distance normal to the wall there (mf:12227-12287)
+          iwall = IMID
+          j1 = max(j - jrange, 2)
+          j2 = min(j + jrange, JM)
+          dmin = 1.0E10
+          do jwall = j1, j2
+            xdif = X(j, k) - X(jwall, 1)
+            rdif = R(j, k) - R(jwall, 1)
+            distsq = xdif*xdif + rdif*rdif
+            if (distsq < dmin) then
+              dmin = distsq
+              jmin = jwall
+            end if
+          end do
+          atot = sqrt(ASX(iwall, jmin, 1)*ASX(iwall, jmin, 1) + ASR(iwall, jmin, 1)*ASR(iwall, jmin, 1))
+          xnorm = ASX(iwall, jmin, 1)/atot
+          rnorm = ASR(iwall, jmin, 1)/atot
+          xdif = X(j, k) - X(jmin, 1)
+          rdif = R(j, k) - R(jmin, 1)
+          hubdist = abs(xdif*xnorm + rdif*rnorm)
+          if (IBOUND == 1 .or. IBOUND > 2) hubdist = dist_ref
+          dmin = 1.0E10
+          do jwall = j1, j2
+            xdif = X(j, k) - X(jwall, KM)
+            rdif = R(j, k) - R(jwall, KM)
+            distsq = xdif*xdif + rdif*rdif
+            if (distsq < dmin) then
+              dmin = distsq
+              jmin = jwall
+            end if
+          end do
+          atot = sqrt(ASX(iwall, jmin, KM)*ASX(iwall, jmin, KM) + ASR(iwall, jmin, KM)*ASR(iwall, jmin, KM))
+          xnorm = ASX(iwall, jmin, KM)/atot
+          rnorm = ASR(iwall, jmin, KM)/atot
+          xdif = X(j, k) - X(jmin, KM)
+          rdif = R(j, k) - R(jmin, KM)
+          tipdist = abs(xdif*xnorm + rdif*rnorm)
+          if (IBOUND >= 2) tipdist = dist_ref
+          endwall_dist = hubdist*tipdist/(hubdist + tipdist)
+          ! blade surfaces I = 1 and I = IM of this row (mf:12296-12402)
+          j1 = j - jrange
+          if (j1 < 2) j1 = 2
+          if (j1 < JSTART(nr)) j1 = JSTART(nr)
+          if (j1 >= JTE(nr)) j1 = JTE(nr) - 1
+          j2 = j + jrange
+          if (j2 > JM) j2 = JM
+          if (j2 > JMIX(nr)) j2 = JMIX(nr)
+          if (j2 <= JLE(nr)) j2 = JLE(nr) + 1
+          k1 = max(k - krange, 1)
+          k2 = min(k + krange, KM)
+          do iside = 1, 2
+            isurf = 1
+            if (iside == 2) isurf = IM
+            dmin = 1.0E10
+            if_found = 0
+            do jsurf = j1, j2
+              do ksurf = k1, k2
+                xdif = X(j, k) - X(jsurf, ksurf)
+                rdif = R(j, k) - R(jsurf, ksurf)
+                tdifsq = 4.*R(j, k)*R(jsurf, ksurf)*sin((THETA(i, j, k) - THETA(isurf, jsurf, ksurf))/2.)**2
+                distsq = xdif*xdif + rdif*rdif + tdifsq
+                if (distsq < dmin) then
+                  dmin = distsq
+                  jmin = jsurf
+                  kmin = ksurf
+                  if_found = 1
+                end if
+              end do
+            end do
+            if (if_found == 1) then
+              knorm = kmin
+              if (kmin == KM) knorm = KM - 1
+              atot = sqrt(ABX(isurf, jmin, knorm)*ABX(isurf, jmin, knorm) + ABR(isurf, jmin, knorm)* &
+                          ABR(isurf, jmin, knorm) + ABT(jmin, knorm)*ABT(jmin, knorm))
+              xnorm = ABX(isurf, jmin, knorm)/atot
+              rnorm = ABR(isurf, jmin, knorm)/atot
+              tnorm = ABT(jmin, knorm)/atot
+              xdif = X(j, k) - X(jmin, kmin)
+              rdif = R(j, k) - R(jmin, kmin)
+              tdif = 2.*sqrt(R(j, k)*R(jmin, kmin))*sin((THETA(i, j, k) - THETA(isurf, jmin, kmin))/2.)
+              sdist(iside) = abs(xdif*xnorm + rdif*rnorm + tdif*tnorm)
+              if (jmin <= JLE(nr) .or. jmin >= JTE(nr)) sdist(iside) = sqrt(xdif*xdif + rdif*rdif + tdif*tdif)
+            else
+              sdist(iside) = dist_ref
+            end if
+          end do
+          blade_dist = sdist(1)*sdist(2)/(sdist(1) + sdist(2))
+          ! combined distance (mf:12408-12424)
+          if (k > 1 .and. k < KM) then
+            distsqrt = sqrt(endwall_dist*endwall_dist + blade_dist*blade_dist)
+            if (distsqrt < 1.0E-10) distsqrt = 1.0E-10
+            dist_min(i, j, k) = blade_dist*endwall_dist/distsqrt
+          else
+            dist_min(i, j, k) = 0.0
+          end if
+          if (KM == 2) dist_min(i, j, k) = blade_dist
+          if (IM == 2) dist_min(i, j, k) = endwall_dist
+          ! mixing-length limit varying along the row (mf:12428-12449)
+          if (i == IMID .and. k == KMID) then
+            if (j <= JLE(nr)) xllim = XLLIM_IN(nr) + (SMERID(j, k) - SMERID(JSTART(nr), k)) &
+                                      /(SMERID(JLE(nr), k) - SMERID(JSTART(nr), k))*(XLLIM_LE(nr) - XLLIM_IN(nr))
+            if (j > JLE(nr) .and. j <= JTE(nr)) xllim = XLLIM_LE(nr) + (SMERID(j, k) - SMERID(JLE(nr), k)) &
+                                      /(SMERID(JTE(nr), k) - SMERID(JLE(nr), k))*(XLLIM_TE(nr) - XLLIM_LE(nr))
+            if (j > JTE(nr)) xllim = XLLIM_TE(nr) + (SMERID(j, k) - SMERID(JTE(nr), k)) &
+                                      /(SMERID(JMIX(nr), k) - SMERID(JTE(nr), k))*(XLLIM_DN(nr) - XLLIM_TE(nr))
+            XLIMIT(j) = 2.0*dist_min(i, j, k)*xllim
+          end if
+        end do
+      end do
+    end do
+    ! blended mixing length and the free-stream damping function (mf:12460-12477)
+    dblend = 0.6666666
+    expon = dblend/(1.0 - dblend)
+    fblend = (1.0 - dblend)*dblend**expon
+    do j = 1, JM
+      do k = 1, KM
+        do i = 1, IM
+          x_length = dist_min(i, j, k)
+          ratio = x_length/XLIMIT(j)
+          if (ratio > dblend) x_length = XLIMIT(j)*(1.0 - fblend*ratio**(-expon))
+          xl(i, j, k) = x_length
+          t3(i, j, k) = (x_length/XLIMIT(j))**4
+        end do
+      end do
+    end do
+    ! cell averages (mf:12482-12508)
+    do k = 1, KMM1
+      do j = 2, JM
+        nr = NROW(j)
+        do i = 1, IMM1
+          avg_xlength = 0.125*(xl(i, j, k) + xl(i, j, k + 1) + xl(i, j - 1, k) + xl(i, j - 1, k + 1) + xl(i + 1, j, k) &
+                               + xl(i + 1, j, k + 1) + xl(i + 1, j - 1, k) + xl(i + 1, j - 1, k + 1))
+          XLENGTH(i, j, k) = 0.41*0.41*avg_xlength*avg_xlength
+          if (j == JSTART(nr)) XLENGTH(i, j, k) = 0.0
+          avg_fstrat = 0.125*(t3(i, j, k) + t3(i, j, k + 1) + t3(i, j - 1, k) + t3(i, j - 1, k + 1) + t3(i + 1, j, k) &
+                              + t3(i + 1, j, k + 1) + t3(i + 1, j - 1, k) + t3(i + 1, j - 1, k + 1))
+          FST_RAT(i, j, k) = avg_fstrat*FSTURB(nr)
+          if (j == JSTART(nr)) FST_RAT(i, j, k) = 0.0
+          avgdist = 0.125*(dist_min(i, j, k) + dist_min(i, j, k + 1) + dist_min(i, j - 1, k) + dist_min(i, j - 1, k + 1) &
+                           + dist_min(i + 1, j, k) + dist_min(i + 1, j, k + 1) + dist_min(i + 1, j - 1, k) &
+                           + dist_min(i + 1, j - 1, k + 1))
+          DWALLSQ(i, j, k) = avgdist*avgdist
+        end do
+      end do
+    end do
+    deallocate (dist_min, xl, t3)
+  end subroutine set_xlength
 
   subroutine reset()
     ! release everything from a previous load
@@ -1371,6 +1552,7 @@ contains
     if (allocated(KB1)) deallocate (KB1, KB2, IB1, IB2, JB1, JB2, JSBLK)
     if (allocated(STEP1)) deallocate (STEP1, STEP2, STEPSBK)
     if (allocated(XFORCE)) deallocate (XFORCE, RFORCE, TFORCE, QSOURCE, FLOWX, FLOWT, FLOWR)
+    if (allocated(DWALLSQ)) deallocate (DWALLSQ, XLENGTH, FST_RAT, XLIMIT)
   end subroutine reset
 
   !> Read a deck from an open unit and build the case.  Returns .true. on

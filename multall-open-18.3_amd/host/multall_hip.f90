@@ -38,6 +38,9 @@ program multall_hip
   rc = mh_create(dims, ctx)
   call check(rc, 'mh_create')
   call upload()
+  ! SETUP's initial viscous forces (mf:7346-7357); a no-op when ILOS = 0
+  rc = mh_loss_init(ctx)
+  call check(rc, 'mh_loss_init')
   rc = mh_start(ctx)
   call check(rc, 'mh_start')
   open (unit=4, file='stage.log')
@@ -159,6 +162,16 @@ contains
     call sets('ITIMST', dble(ITIMST)); call sets('IF_RESTART', dble(IF_RESTART)); call sets('IFGAS', dble(IFGAS))
     call sets('IN_FLOW', dble(IN_FLOW)); call sets('IF_REPEAT', dble(IF_REPEAT)); call sets('IRBB', dble(IRBB))
     call sets('KRBB', dble(KRBB))
+    ! viscous model inputs (LOSS)
+    call put3('DWALLSQ', DWALLSQ); call put1('WHUB', WHUB); call put1('WTIP', WTIP)
+    call put1('XLLIM_I1', XLLIM_I1); call put1('XLLIM_IM', XLLIM_IM); call put1('XLLIM_K1', XLLIM_K1)
+    call put1('XLLIM_KM', XLLIM_KM); call put1('XLLIM_DWN', XLLIM_DWN); call put1('XLLIM_UP', XLLIM_UP)
+    call put1('ROUGH_H', ROUGH_H); call put1('ROUGH_T', ROUGH_T); call put1('ROUGH_L', ROUGH_L)
+    call put1('ROUGH_U', ROUGH_U); call puti('JTRAN_I1', JTRAN_I1); call puti('JTRAN_IM', JTRAN_IM)
+    call puti('JTRAN_K1', JTRAN_K1); call puti('JTRAN_KM', JTRAN_KM)
+    call sets('REYNO', dble(REYNO)); call sets('RF_VIS', dble(RF_VIS)); call sets('FTRANS', dble(FTRANS))
+    call sets('PRANDTL', dble(PRANDTL)); call sets('YPLUSWALL', dble(YPLUSWALL)); call sets('YPLAM', dble(YPLAM))
+    call sets('YPTURB', dble(YPTURB)); call sets('CFWALL', dble(CFWALL)); call sets('IBOUND', dble(IBOUND))
   end subroutine upload
 
   subroutine get3(name, a)

@@ -18,12 +18,15 @@ HOST_LIB = os.path.join(PKG_ROOT, "lib", "libmultall_host.so")
 N3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "ROVT", "ROWT",
       "P", "HO", "PEFF", "VX", "VR", "VT", "WT", "T_STATIC", "AQX", "AQR", "ABX", "ABR", "ASX", "ASR",
       "VOL", "VOLOR", "STEP", "BSTEP", "RSTEP", "XFORCE", "RFORCE", "TFORCE", "QSOURCE", "FLOWX",
-      "FLOWT", "FLOWR"]
+      "FLOWT", "FLOWR", "DWALLSQ"]
 JK = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID"]
 IK = ["FTL", "FBL", "FTR", "FBR"]
 VI = ["FP", "FU", "FD"]
 VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD", "VM1"]
-VJ = ["FACUP", "FACDWN", "WRAD"]
+VJ = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
+RN = ["XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM", "XLLIM_DWN", "XLLIM_UP", "ROUGH_H", "ROUGH_T", "ROUGH_L",
+      "ROUGH_U"]
+RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM"]
 MG = ["STEP1", "STEP2", "STEPSBK"]
 II = ["IB1", "IB2"]
 IKK = ["KB1", "KB2"]
@@ -36,7 +39,7 @@ SCALARS = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NIB1", "NJB1", "NKB1", "N
            "F3", "RSMTH", "CFL", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "SFTIN", "FAC_4TH",
            "FBLK1", "FBLK2", "FBLK3", "RFMIX", "FEXTRAP", "FSMTHB", "FANGLE", "RFIN", "SFEXIT", "REYNO",
            "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL", "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP",
-           "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS", "VSOUND", "VORT_MAX"]
+           "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS", "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL"]
 
 _lib = None
 
@@ -76,7 +79,7 @@ def load_deck(path: str) -> Dict:
                  "SFXIN", "SFTIN", "FAC_4TH", "FBLK1", "FBLK2", "FBLK3", "RFMIX", "FEXTRAP", "FSMTHB",
                  "FANGLE", "RFIN", "SFEXIT", "REYNO", "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL",
                  "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP", "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS",
-                 "VSOUND", "VORT_MAX"):
+                 "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL"):
             continue
         sc[n] = int(sc[n])
     im, jm, km = sc["IM"], sc["JM"], sc["KM"]
@@ -112,4 +115,10 @@ def load_deck(path: str) -> Dict:
         st[n] = arr(n, (jm,), np.int32)
     for n in IR_:
         st[n] = arr(n, (sc["NROWS"] + 1,), np.int32)
+    for n in RN:
+        st[n] = arr(n, (sc["NROWS"],), np.float32)
+    for n in RNI:
+        st[n] = arr(n, (sc["NROWS"],), np.int32)
+    # the viscous-force state LOSS fills in (mh_loss_init) starts at zero
+    st["Y_PLUS"] = np.zeros((km, jm, im), np.float32)
     return st

@@ -110,3 +110,33 @@ def test_vs_reference(case, nsteps, exact):
         assert err <= TOL, (v, err)
     for v in ["DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP"]:
         print(f"{case} {nsteps} exact={exact} {v}: normalized max err {_nerr(g.get(v), ref[v]):.2e} (reported)")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", INVISCID + VISCOUS)
+def test_host_path_vs_reference(case):
+    """The whole drop-in path: the deck read and set up by the Fortran host,
+    the SETUP-time LOSS call on the GPU (forces against the reference's
+    post-SETUP state), then 100 steps against the reference."""
+    import os
+    from conftest import GOLDEN
+    from multall_hip import host
+    from multall_hip.runtime import Solver
+    st = host.load_deck(os.path.join(GOLDEN, "decks", f"{case}.dat"))
+    assert st["scalars"]["NSTEPS_MAX"] == 100
+    g = Solver(st, exact_order=False)
+    g.loss_init()
+    ref0 = load_state(case)
+    for v in FORCES:
+        err = _nerr(g.get(v), ref0[v])
+        print(f"{case} set-up {v}: normalized max err {err:.2e}")
+        assert err <= 1e-5, (v, err)
+    g.start()
+    r = g.step(110)
+    assert r["nstep"] == 100 and r["ifend"] == 1
+    ref = load_ref(case, 100)
+    sc = _scales(ref)
+    for v in ["RO", "ROVX", "ROVR", "RORVT", "ROE"]:
+        err = _nerr(g.get(v), ref[v], sc[v])
+        print(f"{case} host path 100 steps {v}: scaled max err {err:.2e}")
+        assert err <= TOL, (v, err)
