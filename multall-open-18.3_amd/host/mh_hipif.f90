@@ -48,6 +48,16 @@ module mh_hipif
       integer(c_int), value :: id
       integer(c_long_long), value :: n
     end function mh_get_array
+    integer(c_int) function mh_set_array_strided(ctx, id, host, ld_i, ld_j) bind(C, name='mh_set_array_strided')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: id, ld_i, ld_j
+    end function mh_set_array_strided
+    integer(c_int) function mh_get_array_strided(ctx, id, host, ld_i, ld_j) bind(C, name='mh_get_array_strided')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: id, ld_i, ld_j
+    end function mh_get_array_strided
     integer(c_int) function mh_set_scalar(ctx, id, v) bind(C, name='mh_set_scalar')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
