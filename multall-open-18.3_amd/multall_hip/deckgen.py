@@ -384,6 +384,30 @@ def deck_tiny_wall_variant(nsteps: int = 100) -> DeckSpec:
     return spec
 
 
+def deck_tiny_sa(nsteps: int = 100) -> DeckSpec:
+    """Three-row tiny cascade with the Spalart-Allmaras model (ILOS=200) and
+    a rotor tip gap over the two outer K-cells."""
+    spec = deck_tiny(ilos=200, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=200 SPALART-ALLMARAS, TIP GAP"
+    r = spec.rows[1]
+    r.ktips, r.ktipe, r.fractip = 7, 9, 0.02
+    return spec
+
+
+def deck_tiny_sa_variant(nsteps: int = 100) -> DeckSpec:
+    """The Spalart-Allmaras tiny cascade with power-law laminar viscosity,
+    rough walls, a shear-free casing (IBOUND = 2) and the mixing-length
+    relaxation source switched on (FAC_STMIX = 1)."""
+    spec = deck_tiny_sa(nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=200 POWER-LAW VISCOSITY, ROUGH, IBOUND=2"
+    spec.reyno = -2.0
+    spec.if_rough = 1
+    spec.rough = (20.0, 30.0, 10.0, 15.0)
+    spec.ibound = 2
+    spec.sa_factors = (1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 0.0, 0.0)
+    return spec
+
+
 def deck_config0(nsteps: int = 2000) -> DeckSpec:
     """Single axial turbine stator row, 37x33x97, 60 blades, hub/tip 0.30/0.35 m,
     camber 0->70 deg, 20% NACA thickness, p0=1e5 Pa, T0=300 K, p_exit=0.8e5 Pa,

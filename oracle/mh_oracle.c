@@ -28,14 +28,16 @@
   X(VX) X(VR) X(VT) X(WT) X(T_STATIC) X(DRO) X(DROVX) X(DROVR) X(DRORVT) X(DROE)          \
   X(AQX) X(AQR) X(ABX) X(ABR) X(ASX) X(ASR) X(VOL) X(VOLOR) X(STEP) X(BSTEP) X(RSTEP)     \
   X(XFORCE) X(RFORCE) X(TFORCE) X(QSOURCE) X(FLOWX) X(FLOWT) X(FLOWR) X(XFLUX) X(TFLUX)  \
-  X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN) X(DWALLSQ) X(Y_PLUS)
+  X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN) X(DWALLSQ) X(Y_PLUS) X(TRANS_DYN_VIS) X(DEL_DYNVIS)      \
+  X(TRANS_KVIS) X(T_SOURCE) X(ROAVG_CELL) X(VISC_LAM) X(VISC_RAT) X(VISC_TURB) X(XLENGTH) X(FST_RAT)      \
+  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT)
 #define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID)
 #define FIK(X) X(FTL) X(FBL) X(FTR) X(FBR)
 #define F1D(X) X(FP) X(FU) X(FD) X(FR) X(FKU) X(FKD) X(PO1) X(TO1) X(RPO1) X(BSSIN) X(BSCOS) \
   X(BRSIN) X(BRCOS) X(VTIN) X(PD) X(PLIM) X(VM1) X(FACUP) X(FACDWN) X(WRAD) X(WHUB)         \
   X(WTIP) X(VLIMIT) X(ROLIMIT) X(AVG_CHG) X(AVG_BLK) X(STEP1) X(STEP2) X(STEPSBK)           \
   X(XLLIM_I1) X(XLLIM_IM) X(XLLIM_K1) X(XLLIM_KM) X(XLLIM_DWN) X(XLLIM_UP) X(CHORD)         \
-  X(ROUGH_H) X(ROUGH_T) X(ROUGH_L) X(ROUGH_U) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
+  X(ROUGH_H) X(ROUGH_T) X(ROUGH_L) X(ROUGH_U) X(FSTURB) X(TURBVIS_DAMP) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
   X(REYNOLDS) X(ROVEXIT)
 #define I1D(X) X(IB1) X(IB2) X(KB1) X(KB2) X(NBLADE) X(IND) X(INDLE) X(INDMIX) X(INDTE)   \
   X(NROW) X(JSBLK) X(JB1) X(JB2) X(JSTART) X(JMIX) X(JLE) X(JTE) X(JLED) X(KTIPS) X(KTIPE) \
@@ -47,7 +49,8 @@
   X(FGA) X(GA1) X(RCP) X(VSOUND) X(FACMIXUP) X(FMIXUP) X(REYNO) X(RF_VIS) X(FTRANS)           \
   X(PRANDTL) X(YPLUSWALL) X(TURBVIS_LIM) X(VISLAM) X(FTCOND) X(PREFF) X(ROLIM) X(ROMIN)       \
   X(RO_REF) X(EAVG) X(EMAX) X(ECONT) X(VRMS) X(VMAX) X(VREF) X(TCOOL_MIN) X(SFEX1) X(SFEXH)   \
-  X(FLOWRAT) X(YPLAM) X(YPTURB) X(CFWALL)
+  X(FLOWRAT) X(YPLAM) X(YPTURB) X(CFWALL) X(FAC_STMIX) X(FAC_ST0) X(FAC_ST1) X(FAC_ST2)       \
+  X(FAC_ST3) X(FAC_SFVIS) X(FAC_VORT) X(FAC_PGRAD) X(VORT_MAX)
 #define ISCAL(X) X(IM) X(JM) X(KM) X(NROWS) X(IMID) X(KMID) X(NSTEP) X(NSTEPS_MAX) X(NCHANGE) \
   X(NRSMTH) X(NSFEXIT) X(IN_PRESS) X(IN_VTAN) X(IN_VR) X(IPOUT) X(IFMIX) X(IR) X(JR) X(KR)    \
   X(ILOS) X(NLOS) X(NMIXUP) X(NIB1) X(NJB1) X(NKB1) X(NIB2) X(NJB2) X(NKB2) X(NSBLK) X(IFEND) \
@@ -131,8 +134,9 @@ int orc_start(orc_t *o) {
   if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 || o->IN_FLOW != 0 ||
       o->IF_REPEAT != 0 || o->IF_RESTART != 0)
     return -2;
-  if (o->ILOS != 0 && o->ILOS != 10) return -3;
-  if (o->ILOS == 10 && o->YPLUSWALL < -0.001f) return -3;
+  if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 200) return -3;
+  if (o->ILOS != 0 && o->YPLUSWALL < -0.001f) return -3;
+  if (o->ILOS >= 200 && (o->FAC_VORT > 0.001f || o->FAC_PGRAD > 0.001f)) return -3;
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMID = o->IMID, KMID = o->KMID;
   (void)IM;
   o->RO_REF = A1(PO1, KMID) / (o->RGAS * A1(TO1, KMID));
@@ -373,10 +377,9 @@ void orc_smooth_var(orc_t *o, float *D) {
 }
 
 /* ----------------------------------------------------------------------- */
-/* TSTEP, mf:5470-5932 (NCALL=2, the SA branch, is not restated here)      */
+/* TSTEP, mf:5470-5932                                                      */
 
 void orc_tstep(orc_t *o, float *D, float *DIFF, int NCALL) {
-  (void)NCALL;
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID;
   const int NRWSM1 = o->NROWS - 1;
   float *STORE = o->STORE;
@@ -518,6 +521,27 @@ void orc_tstep(orc_t *o, float *D, float *DIFF, int NCALL) {
         }
       }
   }
+  int ILAST = IM;
+  if (NCALL == 2) {
+    /* the SA variable is cell-centred: add the change, carry the pitch
+     * average across each mixing plane and smooth, mf:5736-5777 */
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IMM1; I++) D3(D, I, J, K) = D3(D, I, J, K) + D3(STORE, I, J, K);
+    for (int NR = 1; NR <= o->NROWS - 1; NR++) {
+      int J = A1(JMIX, NR);
+      float FMULT = A1(TURBVIS_DAMP, NR);
+      for (int K = 1; K <= KMM1; K++) {
+        float AVG = 0.0f;
+        for (int I = 1; I <= IMM1; I++) AVG = AVG + A1(FP, I) * D3(D, I, J, K);
+        for (int JAV = J + 1; JAV <= J + 2; JAV++)
+          for (int I = 1; I <= IMM1; I++) D3(D, I, JAV, K) = FMULT * AVG;
+      }
+    }
+    smooth_resid(o, D, o->FAC_SFVIS * o->SFT, 1);
+    ILAST = IMM1;
+    goto bcs;
+  }
   /* second pitchwise averaging at mixing planes, mf:5786-5818 */
   for (int NR = 1; NR <= NRWSM1; NR++) {
     int J = A1(JMIX, NR);
@@ -553,8 +577,8 @@ void orc_tstep(orc_t *o, float *D, float *DIFF, int NCALL) {
         D3(D, I + 1, J - 1, K + 1) = D3(D, I + 1, J - 1, K + 1) + ADD * AIK(FTR, I, K) * A1(FACUP, J);
       }
   orc_smooth_var(o, D);
+bcs:
   /* periodicity, tip gap and mixing-plane averages of D, mf:5858-5926 */
-  int ILAST = IM;
   for (int J = 1; J <= JM; J++) {
     if ((A1(IND, J) == 1) || (A1(INDLE, J) == 1)) continue;
     for (int K = 1; K <= KM; K++) {
@@ -774,6 +798,624 @@ void orc_new_mixplan(orc_t *o) {
   }
   free(ALPHAA); free(GAMA); free(WREF_JP2); free(ROREF_JP2); free(PISENT); free(ROISENT);
   free(WMACH_JP2); free(TISENT); free(WIS_JP2);
+}
+
+/* ----------------------------------------------------------------------- */
+/* Spalart-Allmaras model: SPAL_LOSS (mf:13586-14827) with GRADVEL and      */
+/* GRADCELL (mf:13468-13580).  Restated without the Shih wall functions     */
+/* (YPLUSWALL >= -0.001), the pressure-gradient and helicity source factors */
+/* (FAC_PGRAD, FAC_VORT <= 0.001) and KM = 2.  The Reynolds-number          */
+/* diagnostics feed printed output only and are left out.                   */
+
+/* V of a corner-stored field: Gauss-theorem gradient of cell (I,J,K),
+ * mf:13468-13510 */
+static void gradvel(orc_t *o, const float *V, int I, int J, int K, float *DVDX, float *DVDR, float *DVDT) {
+  const int IM = o->IM, JM = o->JM;
+  (void)IM; (void)JM;
+  int IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
+  float RVOL = 0.25f / A3(VOL, I, J, K);
+  float VAVGJ1 = D3(V, I, JM1, K) + D3(V, IP1, JM1, K) + D3(V, IP1, JM1, KP1) + D3(V, I, JM1, KP1);
+  float VAVGJ2 = D3(V, I, J, K) + D3(V, IP1, J, K) + D3(V, IP1, J, KP1) + D3(V, I, J, KP1);
+  float VAVGI1 = D3(V, I, J, K) + D3(V, I, JM1, K) + D3(V, I, JM1, KP1) + D3(V, I, J, KP1);
+  float VAVGI2 = D3(V, IP1, J, K) + D3(V, IP1, JM1, K) + D3(V, IP1, JM1, KP1) + D3(V, IP1, J, KP1);
+  float VAVGK1 = D3(V, I, J, K) + D3(V, IP1, J, K) + D3(V, IP1, JM1, K) + D3(V, I, JM1, K);
+  float VAVGK2 = D3(V, I, J, KP1) + D3(V, IP1, J, KP1) + D3(V, IP1, JM1, KP1) + D3(V, I, JM1, KP1);
+  float VAVGCELL = 0.125f * (VAVGJ1 + VAVGJ2);
+  *DVDX = -(VAVGJ2 * A3(AQX, I, J, K) - VAVGJ1 * A3(AQX, I, JM1, K) + VAVGI1 * A3(ABX, I, J, K) -
+            VAVGI2 * A3(ABX, IP1, J, K) + VAVGK1 * A3(ASX, I, J, K) - VAVGK2 * A3(ASX, I, J, KP1)) * RVOL;
+  *DVDR = -(VAVGJ2 * A3(AQR, I, J, K) - VAVGJ1 * A3(AQR, I, JM1, K) + VAVGI1 * A3(ABR, I, J, K) -
+            VAVGI2 * A3(ABR, IP1, J, K) + VAVGK1 * A3(ASR, I, J, K) - VAVGK2 * A3(ASR, I, J, KP1)) * RVOL -
+          VAVGCELL / AJK(RAVG_CELL, J, K);
+  *DVDT = -(VAVGI1 - VAVGI2) * AJK(ABT, J, K) * RVOL;
+}
+
+/* Gradient of a cell-centred field, mf:13516-13580.  Cells beyond J = JM
+ * read as zero (the reference reads the unused, zero J = JM+1 layer of its
+ * padded array there). */
+static float cellv(orc_t *o, const float *V, int I, int J, int K) {
+  if (J > o->JM || J < 1) return 0.0f;
+  return D3(V, I, J, K);
+}
+
+static void gradcell(orc_t *o, const float *V, int I, int J, int K, float *DVDX, float *DVDR, float *DVDT) {
+  const int IMM1 = o->IM - 1, KMM1 = o->KM - 1;
+  int IP1 = I + 1, JP1 = J + 1, KP1 = K + 1, IM1 = I - 1, JM1 = J - 1, KM1 = K - 1;
+  if (I == 1) IM1 = IMM1;
+  if (I == IMM1) IP1 = 1;
+  if (K == 1) KM1 = 1;
+  if (K == KMM1) KP1 = KMM1;
+  float RVOL = 0.5f / A3(VOL, I, J, K);
+  float v0 = cellv(o, V, I, J, K);
+  float VAVGJ1 = v0 + cellv(o, V, I, JM1, K);
+  float VAVGJ2 = v0 + cellv(o, V, I, JP1, K);
+  float VAVGI1 = v0 + cellv(o, V, IM1, J, K);
+  float VAVGI2 = v0 + cellv(o, V, IP1, J, K);
+  float VAVGK1 = v0 + cellv(o, V, I, J, KM1);
+  float VAVGK2 = v0 + cellv(o, V, I, J, KP1);
+  if (A1(IND, J) == 1) {
+    if (I == 1) VAVGI1 = 0.0f;
+    if (I == IMM1) VAVGI2 = 0.0f;
+  }
+  if (K == 1 && (o->IBOUND == 0 || o->IBOUND == 2)) VAVGK1 = 0.0f;
+  if (K == KMM1 && o->IBOUND <= 1) VAVGK2 = 0.0f;
+  *DVDX = -(VAVGJ2 * A3(AQX, I, J, K) - VAVGJ1 * A3(AQX, I, JM1, K) + VAVGI1 * A3(ABX, I, J, K) -
+            VAVGI2 * A3(ABX, IP1, J, K) + VAVGK1 * A3(ASX, I, J, K) - VAVGK2 * A3(ASX, I, J, KP1)) * RVOL;
+  *DVDR = -(VAVGJ2 * A3(AQR, I, J, K) - VAVGJ1 * A3(AQR, I, JM1, K) + VAVGI1 * A3(ABR, I, J, K) -
+            VAVGI2 * A3(ABR, IP1, J, K) + VAVGK1 * A3(ASR, I, J, K) - VAVGK2 * A3(ASR, I, J, KP1)) * RVOL -
+          v0 / AJK(RAVG_CELL, J, K);
+  *DVDT = -(VAVGI1 - VAVGI2) * AJK(ABT, J, K) * RVOL;
+}
+
+/* log-law skin friction of SPAL_LOSS's wall cells (sums of four node values,
+ * hence the 1/16 and 1/128 factors), mf:14221-14268 */
+static float sa_cf(float RE, float DPERP, float ROUGH, float ROAVG4, float WAVG, float VISLAM) {
+  float RELOG = 1.f / logf(RE);
+  float CF;
+  if (ROUGH > 1.0e-7f) {
+    if (ROUGH > DPERP) ROUGH = DPERP;
+    float REK = RE * ROUGH / DPERP;
+    REK = REK - 80.0f;
+    if (REK < 0.0f) REK = 0.0f;
+    float REKSQ = REK * REK;
+    float A1 = -.00178493f + .0000814923f * REK + .000000150445f * REKSQ;
+    float A2 = .029072f - .001584f * REK - .00000225194f * REKSQ;
+    float A3 = .270313f + .0091409f * REK + .00000451537f * REKSQ + .00000000464767f * REKSQ * REK;
+    CF = A1 + A2 * RELOG + A3 * RELOG * RELOG;
+  } else {
+    CF = -0.00178493f + 0.029072f * RELOG + 0.270313f * RELOG * RELOG;
+  }
+  float CFLAM = 2.0f / RE;
+  CF = fmaxf(CF, CFLAM);
+  if (RE < 125.f) CF = CFLAM;
+  if (ROUGH > 1.0e-7f) {
+    float TW = CF * ROAVG4 * WAVG * WAVG / 128.f;
+    float VSTAR = sqrtf(TW / (0.25f * ROAVG4));
+    float PLUSK = ROUGH * VSTAR * ROAVG4 / VISLAM / 4.f;
+    float RELOG10 = log10f(DPERP / ROUGH);
+    float FUNCN = 5.75f * RELOG10 + 8.5f;
+    float CF_FULL_ROUGH = 2.0f / (FUNCN * FUNCN);
+    if (PLUSK > 45.f && CF > CF_FULL_ROUGH) CF = CF_FULL_ROUGH;
+  }
+  return CF;
+}
+
+void orc_spal_loss(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID, KMID = o->KMID;
+  const size_t N = (size_t)IM * JM * KM;
+  float *TXX = calloc(N, 4), *TXR = calloc(N, 4), *TXT = calloc(N, 4), *TRR = calloc(N, 4), *TRT = calloc(N, 4);
+  float *TTT = calloc(N, 4), *QXX = calloc(N, 4), *QRR = calloc(N, 4), *QTT = calloc(N, 4);
+  float *TRX = calloc(N, 4), *TTX = calloc(N, 4), *TTR = calloc(N, 4);
+  float *FORCEX = calloc(N, 4), *FORCER = calloc(N, 4), *FORCET = calloc(N, 4), *ESOURCE = calloc(N, 4);
+  const float RF_VIS = o->RF_VIS, RF_VIS1 = 1.0f - RF_VIS;
+  const int NDAMP = 100000;
+  float VISLAM = o->VISLAM;
+  int NRW = 0, JM1 = 0, KP1 = 0, IP1 = 0;
+  /* laminar viscosity over the first quarter of the run, mf:13611-13635 */
+  if (o->NSTEP == 1 || o->NSTEP < o->NSTEPS_MAX / 4) {
+    if (o->REYNO > 100.f) {
+      int J1 = A1(JLE, 1), J2 = A1(JTE, 1);
+      if (A1(JLE, 1) > JM) J1 = 1;
+      if (A1(JTE, 1) > JM) J2 = JM;
+      float XCHORD = AJK(SMERID, J2, KMID) - AJK(SMERID, J1, KMID);
+      float a = A3(ROVX, IMID, J2, KMID), b = A3(ROVR, IMID, J2, KMID), c = A3(ROWT, IMID, J2, KMID);
+      float ROW2 = sqrtf(a * a + b * b + c * c);
+      VISLAM = XCHORD * ROW2 / o->REYNO;
+    }
+    if (o->REYNO > 0.0f && o->REYNO < 99.99f) VISLAM = o->REYNO / 100000.f;
+    if (o->REYNO < 0.0f) VISLAM = -o->REYNO * 1.0e-5f;
+    float THCOND = o->CP * VISLAM / o->PRANDTL;
+    o->FTCOND = THCOND / VISLAM;
+  }
+  /* turbulent viscosity of each cell, mf:13662-13888 */
+  for (int K = 1; K <= KMM1; K++) {
+    KP1 = K + 1;
+    for (int J = 2; J <= JM; J++) {
+      JM1 = J - 1;
+      NRW = A1(NROW, J);
+      for (int I = 1; I <= IMM1; I++) {
+        IP1 = I + 1;
+        float WTAVG = 0.125f * (A3(WT, I, J, K) + A3(WT, IP1, J, K) + A3(WT, IP1, J, KP1) + A3(WT, I, J, KP1) +
+                                A3(WT, I, JM1, K) + A3(WT, IP1, JM1, K) + A3(WT, IP1, JM1, KP1) + A3(WT, I, JM1, KP1));
+        float ROAVG = A3(ROAVG_CELL, I, J, K);
+        float TSAVG = 0.125f * (A3(T_STATIC, I, J, K) + A3(T_STATIC, IP1, J, K) + A3(T_STATIC, IP1, J, KP1) +
+                                A3(T_STATIC, I, J, KP1) + A3(T_STATIC, I, JM1, K) + A3(T_STATIC, IP1, JM1, K) +
+                                A3(T_STATIC, IP1, JM1, KP1) + A3(T_STATIC, I, JM1, KP1));
+        float RAVG = AJK(RAVG_CELL, J, K);
+        float DVXDX, DVXDR, DVXDT, DVRDX, DVRDR, DVRDT, DWTDX, DWTDR, DWTDT, DTDX, DTDR, DTDT;
+        gradvel(o, o->VX, I, J, K, &DVXDX, &DVXDR, &DVXDT);
+        gradvel(o, o->VR, I, J, K, &DVRDX, &DVRDR, &DVRDT);
+        gradvel(o, o->WT, I, J, K, &DWTDX, &DWTDR, &DWTDT);
+        gradvel(o, o->T_STATIC, I, J, K, &DTDX, &DTDR, &DTDT);
+        D3(TXR, I, J, K) = DVXDR + DVRDX;
+        D3(TXX, I, J, K) = DVXDX;
+        D3(TXT, I, J, K) = DWTDX + DVXDT;
+        D3(TRR, I, J, K) = DVRDR;
+        D3(TRT, I, J, K) = DWTDR + DVRDT - WTAVG / RAVG;
+        D3(TTT, I, J, K) = DWTDT;
+        D3(QXX, I, J, K) = DTDX;
+        D3(QRR, I, J, K) = DTDR;
+        D3(QTT, I, J, K) = DTDT;
+        float VORTX = DVRDT - DWTDR - WTAVG / RAVG;
+        float VORTR = DWTDX - DVXDT;
+        float VORTT = DVXDR - DVRDX;
+        float VORT = sqrtf(VORTX * VORTX + VORTR * VORTR + VORTT * VORTT);
+        if (J == A1(JSTART, NRW)) VORT = 0.0f;
+        if (VORT > o->VORT_MAX) VORT = o->VORT_MAX;
+        if (A1(INDMIX, J) == 1) VORT = A3(ABS_VORT, I, J - 1, K);
+        A3(ABS_VORT, I, J, K) = VORT;
+        float TURBVIS_WALL = ROAVG * A3(XLENGTH, I, J, K) * VORT * o->FMIXUP;
+        if (o->REYNO < 0.0001f) VISLAM = (fabsf(o->REYNO) / 100000.0f) * powf(TSAVG / 288.0f, 0.62f);
+        float VISTOT = VISLAM * (1 + A3(FST_RAT, I, J, K)) + TURBVIS_WALL;
+        float VISLIM = o->TURBVIS_LIM * VISLAM;
+        if (VISTOT > VISLIM) VISTOT = VISLIM;
+        float VISKLAM = VISLAM / ROAVG;
+        A3(DYN_VIS_ML, I, J, K) = (VISTOT - VISLAM);
+        if (o->IF_RESTART == 0 && o->NSTEP == 1) {
+          float TRANSVISIN;
+          if (A1(FSTURB, 1) < 1.0f) TRANSVISIN = 4.35f * VISLAM * powf(A1(FSTURB, 1), 0.25f);
+          else TRANSVISIN = VISLAM * (3.5f + A1(FSTURB, 1) * 0.85f);
+          A3(TRANS_DYN_VIS, I, J, K) = TRANSVISIN;
+        }
+        if (o->NSTEP == 1) A3(TRANS_KVIS, I, J, K) = A3(TRANS_DYN_VIS, I, J, K) / ROAVG;
+        float TRANSKVIS = A3(TRANS_KVIS, I, J, K);
+        float XI = TRANSKVIS / VISKLAM;
+        if (XI < 1.1f) XI = 1.1f;
+        if (XI > o->TURBVIS_LIM) XI = o->TURBVIS_LIM;
+        float XI3 = XI * XI * XI;
+        float FV1 = XI3 / (XI3 + 357.9f);
+        float TRANSVISLIM = VISLIM / ROAVG / FV1;
+        if (TRANSKVIS > TRANSVISLIM) TRANSKVIS = TRANSVISLIM;
+        float VISTRU = TRANSKVIS * FV1 * ROAVG;
+        A3(TRANS_VORT, I, J, K) = A3(ABS_VORT, I, J, K);
+        VISTOT = VISTRU + VISLAM;
+        VISLIM = o->TURBVIS_LIM * VISLAM;
+        if (VISTOT > VISLIM) VISTOT = VISLIM;
+        A3(VISC_RAT, I, J, K) = VISTOT / VISLAM;
+        A3(VISC_LAM, I, J, K) = VISLAM;
+        A3(VISC_TURB, I, J, K) = VISTOT - VISLAM;
+      }
+    }
+  }
+  /* gradient of the turbulent viscosity, mf:13893-13911 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM - 1; J++) {
+      NRW = A1(NROW, J);
+      for (int I = 1; I <= IMM1; I++) {
+        float DX_, DR_, DT_;
+        gradcell(o, o->TRANS_KVIS, I, J, K, &DX_, &DR_, &DT_);
+        A3(GRAD_TVIS, I, J, K) = DX_ * DX_ + DR_ * DR_ + DT_ * DT_;
+        float VISCEFF = A3(TRANS_KVIS, I, J, K) + A3(VISC_LAM, I, J, K);
+        A3(TVISGRADX, I, J, K) = VISCEFF * DX_;
+        A3(TVISGRADR, I, J, K) = VISCEFF * DR_;
+        A3(TVISGRADT, I, J, K) = VISCEFF * DT_;
+        if (J == A1(JSTART, NRW) || J == A1(JSTART, NRW) - 1) {
+          A3(GRAD_TVIS, I, J, K) = 0.0f;
+          A3(TVISGRADX, I, J, K) = 0.0f;
+          A3(TVISGRADR, I, J, K) = 0.0f;
+          A3(TVISGRADT, I, J, K) = 0.0f;
+        }
+      }
+    }
+  /* source terms, mf:13916-13997 */
+  {
+    const float CW2 = 0.3f, SIGMA = 0.6667f, CB2 = 0.622f, CB1 = 0.1355f, CW1 = 0.80607f + 2.433f;
+    const float FST0 = o->FAC_ST0 * CB1, FST1 = o->FAC_ST1 / SIGMA, FST2 = o->FAC_ST2 * CB2 / SIGMA;
+    const float FST3 = o->FAC_ST3 * CW1, FSTMIX = o->FAC_STMIX / (float)NDAMP;
+    float FYPLUS = 0.0f;
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IMM1; I++) {
+          float yp = A3(Y_PLUS, I, J, K);
+          if (yp <= o->YPLAM) FYPLUS = 0.0f;
+          if (yp > o->YPLAM) {
+            float XFAC = (yp - o->YPLAM) / (o->YPTURB - o->YPLAM);
+            if (XFAC > 1.0f) XFAC = 1.0f;
+            FYPLUS = XFAC * XFAC * (3.0f - 2.0f * XFAC);
+          }
+          float TVISDDX, TVISDDR, TVISDDT, d1, d2;
+          gradcell(o, o->TVISGRADX, I, J, K, &TVISDDX, &d1, &d2);
+          gradcell(o, o->TVISGRADR, I, J, K, &d1, &TVISDDR, &d2);
+          gradcell(o, o->TVISGRADT, I, J, K, &d1, &d2, &TVISDDT);
+          float ROAVG = A3(ROAVG_CELL, I, J, K);
+          float TVIS = A3(TRANS_KVIS, I, J, K);
+          float ST0 = FST0 * A3(TRANS_VORT, I, J, K) * TVIS;
+          ST0 = ST0 * FYPLUS;
+          float ST1 = FST1 * (TVISDDX + TVISDDR + TVISDDT);
+          float ST2 = FST2 * A3(GRAD_TVIS, I, J, K);
+          float TAU = 5.9488f * TVIS / A3(TRANS_VORT, I, J, K) / A3(DWALLSQ, I, J, K);
+          if (TAU > 1.2f) TAU = 1.2f;
+          if (TAU < 0.001f) TAU = 0.001f;
+          float G = TAU + CW2 * (TAU * TAU * TAU * TAU * TAU * TAU - TAU);
+          float G6CW6 = G * G * G * G * G * G / 64.0f;
+          float FW = G * powf((1 + 0.015625f) / (1 + G6CW6), 0.166667f);
+          float ST3 = FST3 * FW * TVIS * TVIS / A3(DWALLSQ, I, J, K);
+          float ST_MIXL = FSTMIX * (A3(DYN_VIS_ML, I, J, K) - A3(VISC_TURB, I, J, K)) / A3(STEP, I, J, K);
+          A3(T_SOURCE, I, J, K) = RF_VIS1 * A3(T_SOURCE, I, J, K) +
+                                  RF_VIS * (ROAVG * (ST0 + ST1 + ST2 - ST3) * A3(VOL, I, J, K) + ST_MIXL);
+          if (J > 2 && A1(INDMIX, J - 1) == 1) A3(T_SOURCE, I, J, K) = 0.0f;
+          if (J > 2) {
+            if (A1(INDMIX, J - 2) == 1) A3(T_SOURCE, I, J, K) = 0.0f;
+          }
+        }
+  }
+  /* transition: damp the source towards laminar in laminar regions,
+   * mf:14004-14116 */
+  {
+    const int IEND1 = IMID / 2, IENDM = IMM1 - IEND1, KEND1 = KMID / 2, KENDM = KMM1 - KEND1;
+    const float FTRANS = o->FTRANS;
+    for (int J = 2; J <= JM; J++) {
+      NRW = A1(NROW, J);
+      const int J1 = A1(JSTART, NRW), JROW = J - J1 + 1;
+      const int JTRHUB = A1(JTRAN_K1, NRW), JTRTIP = A1(JTRAN_KM, NRW);
+      const int JTRLOW = A1(JTRAN_I1, NRW), JTRUP = A1(JTRAN_IM, NRW), JLEROW = A1(JLE, NRW);
+      for (int I = 1; I <= IMM1; I++) {
+        int damp = 0;
+        if (!(JTRHUB <= 1 && FTRANS < 0.1f)) {
+          if (JROW < JTRHUB) damp = 1;
+          else {
+            float RMAX = 0.0f;
+            for (int K = 1; K <= KEND1; K++) {
+              float RATVIS = A3(VISC_RAT, I, J, K);
+              if (RATVIS < RMAX) continue;
+              RMAX = RATVIS;
+            }
+            damp = !(RMAX > FTRANS);
+          }
+        }
+        if (damp) {
+          float TVEDGE = A3(VISC_TURB, I, J, KEND1);
+          for (int K = 1; K <= KEND1; K++) {
+            float FK = (float)(KEND1 + 1 - K) / (float)KEND1;
+            FK = FK * FK * (3 - 2 * FK);
+            float TVSPEC = FK * TVEDGE;
+            A3(T_SOURCE, I, J, K) =
+                A3(T_SOURCE, I, J, K) - (A3(VISC_TURB, I, J, K) - TVSPEC) / A3(STEP, I, J, K) / (float)NDAMP;
+          }
+        }
+        damp = 0;
+        if (!(JTRTIP <= 1 && FTRANS < 0.1f)) {
+          if (JROW < JTRTIP) damp = 1;
+          else {
+            float RMAX = 0.0f;
+            for (int K = KENDM; K <= KMM1; K++) {
+              float RATVIS = A3(VISC_RAT, I, J, K);
+              if (RATVIS < RMAX) continue;
+              RMAX = RATVIS;
+            }
+            damp = !(RMAX > FTRANS);
+          }
+        }
+        if (damp) {
+          float TVEDGE = A3(VISC_TURB, I, J, KENDM);
+          for (int K = KENDM; K <= KMM1; K++) {
+            float FK = (float)(K + 1 - KENDM) / (float)(KM - KENDM);
+            FK = FK * FK * (3 - 2 * FK);
+            float TVSPEC = FK * TVEDGE;
+            A3(T_SOURCE, I, J, K) =
+                A3(T_SOURCE, I, J, K) - (A3(VISC_TURB, I, J, K) - TVSPEC) / A3(STEP, I, J, K) / (float)NDAMP;
+          }
+        }
+      }
+      for (int K = 1; K <= KMM1; K++) {
+        int damp = 0;
+        if (!(JTRLOW <= 1 && FTRANS < 0.1f) && !(J <= JLEROW)) {
+          if (JROW < JTRLOW || J < JLEROW) damp = 1;
+          else {
+            float RMAX = 0.0f;
+            for (int I = 1; I <= IEND1; I++) {
+              float RATVIS = A3(VISC_RAT, I, J, K);
+              if (RATVIS < RMAX) continue;
+              RMAX = RATVIS;
+            }
+            damp = !(RMAX > FTRANS);
+          }
+        }
+        if (damp) {
+          float TVEDGE = A3(VISC_TURB, IEND1, J, K);
+          for (int I = 1; I <= IEND1; I++) {
+            float FI = (float)(IEND1 + 1 - I) / (float)IEND1;
+            FI = FI * FI * (3 - 2 * FI);
+            float TVSPEC = FI * TVEDGE;
+            A3(T_SOURCE, I, J, K) =
+                A3(T_SOURCE, I, J, K) - (A3(VISC_TURB, I, J, K) - TVSPEC) / A3(STEP, I, J, K) / (float)NDAMP;
+          }
+        }
+        damp = 0;
+        if (!(JTRUP <= 1 && FTRANS < 0.1f) && !(J <= JLEROW)) {
+          if (JROW < JTRUP) damp = 1;
+          else {
+            float RMAX = 0.0f;
+            for (int I = IENDM; I <= IMM1; I++) {
+              float RATVIS = A3(VISC_RAT, I, J, K);
+              if (RATVIS < RMAX) continue;
+              RMAX = RATVIS;
+            }
+            damp = !(RMAX > FTRANS);
+          }
+        }
+        if (damp) {
+          float TVEDGE = A3(VISC_TURB, IENDM, J, K);
+          for (int I = IENDM; I <= IMM1; I++) {
+            float FI = (float)(I + 1 - IENDM) / (float)(IM - IENDM);
+            FI = FI * FI * (3 - 2 * FI);
+            float TVSPEC = FI * TVEDGE;
+            A3(T_SOURCE, I, J, K) =
+                A3(T_SOURCE, I, J, K) - (A3(VISC_TURB, I, J, K) - TVSPEC) / A3(STEP, I, J, K) / (float)NDAMP;
+          }
+        }
+      }
+    }
+  }
+  /* stresses and heat flows with the final viscosity, mf:14123-14147 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        VISLAM = A3(VISC_LAM, I, J, K);
+        float VISTOT = A3(VISC_RAT, I, J, K) * VISLAM;
+        float THCOND = o->CP * VISTOT / o->PRANDTL;
+        float VISTOT2 = 2.0f * VISTOT;
+        D3(TXR, I, J, K) = VISTOT * D3(TXR, I, J, K);
+        D3(TXX, I, J, K) = VISTOT2 * D3(TXX, I, J, K);
+        D3(TXT, I, J, K) = VISTOT * D3(TXT, I, J, K);
+        D3(TRX, I, J, K) = D3(TXR, I, J, K);
+        D3(TRR, I, J, K) = VISTOT2 * D3(TRR, I, J, K);
+        D3(TRT, I, J, K) = VISTOT * D3(TRT, I, J, K);
+        D3(TTX, I, J, K) = D3(TXT, I, J, K);
+        D3(TTT, I, J, K) = VISTOT2 * D3(TTT, I, J, K);
+        D3(TTR, I, J, K) = D3(TRT, I, J, K);
+        D3(QXX, I, J, K) = -THCOND * D3(QXX, I, J, K);
+        D3(QRR, I, J, K) = -THCOND * D3(QRR, I, J, K);
+        D3(QTT, I, J, K) = -THCOND * D3(QTT, I, J, K);
+      }
+  const float VISLAM16 = VISLAM * 16.0f;
+  /* blade surfaces and the I = const faces, mf:14159-14500.  NRW and JM1
+   * keep their last values where the reference does not reset them. */
+  for (int K = 1; K <= KMM1; K++) {
+    KP1 = K + 1;
+    for (int J = 2; J <= JM; J++) {
+      if (A1(IND, J) != 0) {
+        JM1 = J - 1;
+        NRW = A1(NROW, J);
+        if (!((A1(KTIPS, NRW) > 0) && (K >= A1(KTIPS, NRW)) && (K < A1(KTIPE, NRW)))) {
+          /* I = 1 surface */
+          float VXAVG1 = A3(VX, 2, J, K) + A3(VX, 2, JM1, K) + A3(VX, 2, JM1, KP1) + A3(VX, 2, J, KP1);
+          float VRAVG1 = A3(VR, 2, J, K) + A3(VR, 2, JM1, K) + A3(VR, 2, JM1, KP1) + A3(VR, 2, J, KP1);
+          float WTAVG1 = A3(WT, 2, J, K) + A3(WT, 2, JM1, K) + A3(WT, 2, JM1, KP1) + A3(WT, 2, J, KP1);
+          float ROAVG1 = A3(RO, 2, J, K) + A3(RO, 2, JM1, K) + A3(RO, 2, JM1, KP1) + A3(RO, 2, J, KP1);
+          float WAVG = sqrtf(WTAVG1 * WTAVG1 + VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1);
+          float bx = A3(ABX, 1, J, K), br = A3(ABR, 1, J, K), bt = AJK(ABT, J, K);
+          float AREA1 = sqrtf(bx * bx + br * br + bt * bt);
+          float DPERP = A3(VOL, 1, J, K) / AREA1;
+          float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+          float CF = sa_cf(RE, DPERP, A1(ROUGH_L, NRW), ROAVG1, WAVG, VISLAM);
+          float TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+          float VSTAR = sqrtf(TWALLI1 / (0.25f * ROAVG1));
+          for (int I = 1; I <= IMID; I++) {
+            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
+            A3(Y_PLUS, I, J, K) = fminf(1000.0f, YPLS);
+          }
+          float FMULT = -TWALLI1 * AREA1 / WAVG;
+          float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
+          float WVISC1 = TFORCE1 * A1(WRAD, J) * AJK(RAVG_CELL, J, K);
+          D3(FORCEX, 1, J, K) = D3(FORCEX, 1, J, K) + XFORCE1;
+          D3(FORCER, 1, J, K) = D3(FORCER, 1, J, K) + RFORCE1;
+          D3(FORCET, 1, J, K) = D3(FORCET, 1, J, K) + TFORCE1;
+          D3(ESOURCE, 1, J, K) = D3(ESOURCE, 1, J, K) + WVISC1;
+          /* I = IM surface */
+          float VXAVGIM = A3(VX, IMM1, J, K) + A3(VX, IMM1, JM1, K) + A3(VX, IMM1, JM1, KP1) + A3(VX, IMM1, J, KP1);
+          float VRAVGIM = A3(VR, IMM1, J, K) + A3(VR, IMM1, JM1, K) + A3(VR, IMM1, JM1, KP1) + A3(VR, IMM1, J, KP1);
+          float WTAVGIM = A3(WT, IMM1, J, K) + A3(WT, IMM1, JM1, K) + A3(WT, IMM1, JM1, KP1) + A3(WT, IMM1, J, KP1);
+          float ROAVGIM = A3(RO, IMM1, J, K) + A3(RO, IMM1, JM1, K) + A3(RO, IMM1, JM1, KP1) + A3(RO, IMM1, J, KP1);
+          WAVG = sqrtf(WTAVGIM * WTAVGIM + VXAVGIM * VXAVGIM + VRAVGIM * VRAVGIM);
+          bx = A3(ABX, IM, J, K); br = A3(ABR, IM, J, K);
+          float AREAIM = sqrtf(bx * bx + br * br + bt * bt);
+          DPERP = A3(VOL, IMM1, J, K) / AREAIM;
+          RE = DPERP * ROAVGIM * WAVG / VISLAM16;
+          CF = sa_cf(RE, DPERP, A1(ROUGH_U, NRW), ROAVGIM, WAVG, VISLAM);
+          float TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+          VSTAR = sqrtf(TWALLIM / (0.25f * ROAVGIM));
+          for (int I = IMID + 1; I <= IM; I++) {
+            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGIM) / VISLAM;
+            A3(Y_PLUS, I, J, K) = fminf(1000.f, YPLS);
+          }
+          FMULT = -TWALLIM * AREAIM / WAVG;
+          float XFORCEIM = FMULT * VXAVGIM, RFORCEIM = FMULT * VRAVGIM, TFORCEIM = FMULT * WTAVGIM;
+          float WVISCIM = TFORCEIM * A1(WRAD, J) * AJK(RAVG_CELL, J, K);
+          D3(FORCEX, IMM1, J, K) = D3(FORCEX, IMM1, J, K) + XFORCEIM;
+          D3(FORCER, IMM1, J, K) = D3(FORCER, IMM1, J, K) + RFORCEIM;
+          D3(FORCET, IMM1, J, K) = D3(FORCET, IMM1, J, K) + TFORCEIM;
+          D3(ESOURCE, IMM1, J, K) = D3(ESOURCE, IMM1, J, K) + WVISCIM;
+        }
+      }
+      /* periodic cells I = 1 and IM-1 (skipped on solid blade surfaces),
+       * mf:14434-14486.  The test reads (IND(J)=1 .AND. K<KTIPS) .OR.
+       * K>KTIPE, as the reference's operator precedence makes it. */
+      if (!(((A1(IND, J) == 1) && (K < A1(KTIPS, NRW))) || (K > A1(KTIPE, NRW)))) {
+        int I = 1, IM1 = IMM1;
+        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TXT, I, J, K) + D3(TXT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TXX, I, J, K) + D3(TXX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCEX, IM1, J, K) = D3(FORCEX, IM1, J, K) + XFOR;
+        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
+        float AVG = 0.5f * (D3(FORCEX, I, J, K) + D3(FORCEX, IM1, J, K));
+        D3(FORCEX, I, J, K) = AVG;
+        D3(FORCEX, IM1, J, K) = AVG;
+        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TRT, I, J, K) + D3(TRT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TRX, I, J, K) + D3(TRX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCER, IM1, J, K) = D3(FORCER, IM1, J, K) + RFOR;
+        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
+        AVG = 0.5f * (D3(FORCER, I, J, K) + D3(FORCER, IM1, J, K));
+        D3(FORCER, I, J, K) = AVG;
+        D3(FORCER, IM1, J, K) = AVG;
+        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TTT, I, J, K) + D3(TTT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TTX, I, J, K) + D3(TTX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCET, IM1, J, K) = D3(FORCET, IM1, J, K) + TFOR;
+        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
+        AVG = 0.5f * (D3(FORCET, I, J, K) + D3(FORCET, IM1, J, K));
+        D3(FORCET, I, J, K) = AVG;
+        D3(FORCET, IM1, J, K) = AVG;
+        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, IM1, J, K)) * A3(ABX, I, J, K) +
+                              (D3(QRR, I, J, K) + D3(QRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                              (D3(QTT, I, J, K) + D3(QTT, IM1, J, K)) * AJK(ABT, J, K));
+        float VXAVG = A3(VX, I, J, K) + A3(VX, I, JM1, K) + A3(VX, I, JM1, KP1) + A3(VX, I, J, KP1);
+        float VRAVG = A3(VR, I, J, K) + A3(VR, I, JM1, K) + A3(VR, I, JM1, KP1) + A3(VR, I, J, KP1);
+        float VTAVG = A3(VT, I, J, K) + A3(VT, I, JM1, K) + A3(VT, I, JM1, KP1) + A3(VT, I, J, KP1);
+        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
+        D3(ESOURCE, IM1, J, K) = D3(ESOURCE, IM1, J, K) - QFLOW - WVISC;
+        AVG = 0.5f * (D3(ESOURCE, I, J, K) + D3(ESOURCE, IM1, J, K));
+        D3(ESOURCE, I, J, K) = AVG;
+        D3(ESOURCE, IM1, J, K) = AVG;
+      }
+      for (int I = 2; I <= IMM1; I++) {
+        int IM1 = I - 1;
+        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TXT, I, J, K) + D3(TXT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TXX, I, J, K) + D3(TXX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCEX, IM1, J, K) = D3(FORCEX, IM1, J, K) + XFOR;
+        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
+        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TRT, I, J, K) + D3(TRT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TRX, I, J, K) + D3(TRX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCER, IM1, J, K) = D3(FORCER, IM1, J, K) + RFOR;
+        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
+        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TTT, I, J, K) + D3(TTT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TTX, I, J, K) + D3(TTX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCET, IM1, J, K) = D3(FORCET, IM1, J, K) + TFOR;
+        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
+        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, IM1, J, K)) * A3(ABX, I, J, K) +
+                              (D3(QRR, I, J, K) + D3(QRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                              (D3(QTT, I, J, K) + D3(QTT, IM1, J, K)) * AJK(ABT, J, K));
+        float VXAVG = A3(VX, I, J, K) + A3(VX, I, JM1, K) + A3(VX, I, JM1, KP1) + A3(VX, I, J, KP1);
+        float VRAVG = A3(VR, I, J, K) + A3(VR, I, JM1, K) + A3(VR, I, JM1, KP1) + A3(VR, I, J, KP1);
+        float VTAVG = A3(VT, I, J, K) + A3(VT, I, JM1, K) + A3(VT, I, JM1, KP1) + A3(VT, I, J, KP1);
+        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
+        D3(ESOURCE, IM1, J, K) = D3(ESOURCE, IM1, J, K) - QFLOW - WVISC;
+      }
+    }
+  }
+  /* hub and casing surfaces and the K = const faces, mf:14505-14805 */
+  for (int J = 2; J <= JM; J++) {
+    JM1 = J - 1;
+    for (int I = 1; I <= IMM1; I++) {
+      IP1 = I + 1;
+      float VXAVG1 = A3(VX, I, J, 2) + A3(VX, IP1, J, 2) + A3(VX, IP1, JM1, 2) + A3(VX, I, JM1, 2);
+      float VRAVG1 = A3(VR, I, J, 2) + A3(VR, IP1, J, 2) + A3(VR, IP1, JM1, 2) + A3(VR, I, JM1, 2);
+      float VTAVG1 = A3(VT, I, J, 2) + A3(VT, IP1, J, 2) + A3(VT, IP1, JM1, 2) + A3(VT, I, JM1, 2);
+      float ROAVG1 = A3(RO, I, J, 2) + A3(RO, IP1, J, 2) + A3(RO, IP1, JM1, 2) + A3(RO, I, JM1, 2);
+      float RAVG1 = AJK(R, J, 2) + AJK(R, J, 2) + AJK(R, JM1, 2) + AJK(R, JM1, 2);
+      float WTAVG1 = VTAVG1 - A1(WHUB, J) * RAVG1;
+      float WAVG = sqrtf(VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1 + WTAVG1 * WTAVG1);
+      float sx = A3(ASX, I, J, 1), sr = A3(ASR, I, J, 1);
+      float AREA1 = sqrtf(sx * sx + sr * sr);
+      float DPERP = A3(VOL, I, J, 1) / AREA1;
+      float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+      float CF = sa_cf(RE, DPERP, A1(ROUGH_H, NRW), ROAVG1, WAVG, VISLAM);
+      float TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      if (o->IBOUND == 0 || o->IBOUND == 2) {
+        for (int K = 1; K <= KMID; K++) {
+          float VSTAR = sqrtf(TWALLK1 / (0.25f * ROAVG1));
+          float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
+          A3(Y_PLUS, I, J, K) = fminf(YPLS, A3(Y_PLUS, I, J, K));
+        }
+      }
+      float FMULT = -TWALLK1 * AREA1 / WAVG;
+      if (o->IBOUND == 1 || o->IBOUND > 2) FMULT = 0.0f;
+      float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
+      float WVISC1 = TFORCE1 * 0.25f * A1(WHUB, J) * RAVG1;
+      D3(FORCEX, I, J, 1) = D3(FORCEX, I, J, 1) + XFORCE1;
+      D3(FORCER, I, J, 1) = D3(FORCER, I, J, 1) + RFORCE1;
+      D3(FORCET, I, J, 1) = D3(FORCET, I, J, 1) + TFORCE1;
+      D3(ESOURCE, I, J, 1) = D3(ESOURCE, I, J, 1) + WVISC1;
+      float VXAVGKM = A3(VX, I, J, KMM1) + A3(VX, IP1, J, KMM1) + A3(VX, IP1, JM1, KMM1) + A3(VX, I, JM1, KMM1);
+      float VRAVGKM = A3(VR, I, J, KMM1) + A3(VR, IP1, J, KMM1) + A3(VR, IP1, JM1, KMM1) + A3(VR, I, JM1, KMM1);
+      float VTAVGKM = A3(VT, I, J, KMM1) + A3(VT, IP1, J, KMM1) + A3(VT, IP1, JM1, KMM1) + A3(VT, I, JM1, KMM1);
+      float ROAVGKM = A3(RO, I, J, KMM1) + A3(RO, IP1, J, KMM1) + A3(RO, IP1, JM1, KMM1) + A3(RO, I, JM1, KMM1);
+      float RAVGKM = AJK(R, J, KMM1) + AJK(R, J, KMM1) + AJK(R, JM1, KMM1) + AJK(R, JM1, KMM1);
+      float WTAVGKM = VTAVGKM - A1(WTIP, J) * RAVGKM;
+      WAVG = sqrtf(VXAVGKM * VXAVGKM + VRAVGKM * VRAVGKM + WTAVGKM * WTAVGKM);
+      sx = A3(ASX, I, J, KM); sr = A3(ASR, I, J, KM);
+      float AREAKM = sqrtf(sx * sx + sr * sr);
+      DPERP = A3(VOL, I, J, KMM1) / AREAKM;
+      RE = DPERP * ROAVGKM * WAVG / VISLAM16;
+      CF = sa_cf(RE, DPERP, A1(ROUGH_T, NRW), ROAVGKM, WAVG, VISLAM);
+      float TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+      if (o->IBOUND == 0 || o->IBOUND == 1) {
+        for (int K = KMID + 1; K <= KMM1; K++) {
+          float VSTAR = sqrtf(TWALLKM / (0.25f * ROAVGKM));
+          float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGKM) / VISLAM;
+          A3(Y_PLUS, I, J, K) = fminf(YPLS, A3(Y_PLUS, I, J, K));
+        }
+      }
+      FMULT = -TWALLKM * AREAKM / WAVG;
+      if (o->IBOUND >= 2) FMULT = 0.0f;
+      float XFORCEKM = FMULT * VXAVGKM, RFORCEKM = FMULT * VRAVGKM, TFORCEKM = FMULT * WTAVGKM;
+      float WVISCKM = TFORCEKM * 0.25f * A1(WTIP, J) * RAVGKM;
+      D3(FORCEX, I, J, KMM1) = D3(FORCEX, I, J, KMM1) + XFORCEKM;
+      D3(FORCER, I, J, KMM1) = D3(FORCER, I, J, KMM1) + RFORCEKM;
+      D3(FORCET, I, J, KMM1) = D3(FORCET, I, J, KMM1) + TFORCEKM;
+      D3(ESOURCE, I, J, KMM1) = D3(ESOURCE, I, J, KMM1) + WVISCKM;
+      for (int K = 2; K <= KMM1; K++) {
+        int KM1 = K - 1;
+        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, I, J, KM1)) * A3(ASR, I, J, K) +
+                             (D3(TXX, I, J, K) + D3(TXX, I, J, KM1)) * A3(ASX, I, J, K));
+        D3(FORCEX, I, J, KM1) = D3(FORCEX, I, J, KM1) + XFOR;
+        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
+        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, I, J, KM1)) * A3(ASR, I, J, K) +
+                             (D3(TRX, I, J, K) + D3(TRX, I, J, KM1)) * A3(ASX, I, J, K));
+        D3(FORCER, I, J, KM1) = D3(FORCER, I, J, KM1) + RFOR;
+        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
+        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, I, J, KM1)) * A3(ASR, I, J, K) +
+                             (D3(TTX, I, J, K) + D3(TTX, I, J, KM1)) * A3(ASX, I, J, K));
+        D3(FORCET, I, J, KM1) = D3(FORCET, I, J, KM1) + TFOR;
+        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
+        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, I, J, KM1)) * A3(ASX, I, J, K) +
+                              (D3(QRR, I, J, K) + D3(QRR, I, J, KM1)) * A3(ASR, I, J, K));
+        float VXAVG = A3(VX, I, J, K) + A3(VX, IP1, J, K) + A3(VX, IP1, JM1, K) + A3(VX, I, JM1, K);
+        float VRAVG = A3(VR, I, J, K) + A3(VR, IP1, J, K) + A3(VR, IP1, JM1, K) + A3(VR, I, JM1, K);
+        float VTAVG = A3(VT, I, J, K) + A3(VT, IP1, J, K) + A3(VT, IP1, JM1, K) + A3(VT, I, JM1, K);
+        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
+        D3(ESOURCE, I, J, KM1) = D3(ESOURCE, I, J, KM1) - QFLOW - WVISC;
+      }
+    }
+  }
+  /* body forces, mf:14812-14819 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        A3(XFORCE, I, J, K) = RF_VIS1 * A3(XFORCE, I, J, K) - RF_VIS * D3(FORCEX, I, J, K);
+        A3(TFORCE, I, J, K) = RF_VIS1 * A3(TFORCE, I, J, K) - RF_VIS * D3(FORCET, I, J, K) * AJK(RAVG_CELL, J, K);
+        A3(RFORCE, I, J, K) = RF_VIS1 * A3(RFORCE, I, J, K) - RF_VIS * D3(FORCER, I, J, K);
+        A3(QSOURCE, I, J, K) = RF_VIS1 * A3(QSOURCE, I, J, K) - RF_VIS * D3(ESOURCE, I, J, K);
+      }
+  o->VISLAM = VISLAM;
+  free(TXX); free(TXR); free(TXT); free(TRR); free(TRT); free(TTT); free(QXX); free(QRR); free(QTT);
+  free(TRX); free(TTX); free(TTR); free(FORCEX); free(FORCER); free(FORCET); free(ESOURCE);
 }
 
 /* ----------------------------------------------------------------------- */
@@ -1228,6 +1870,7 @@ static void step_once(orc_t *o) {
 
   orc_primitives(o);
   if (o->ILOS == 10 && o->NSTEP % o->NLOS == 0) orc_loss(o);
+  if (o->ILOS >= 200 && o->NSTEP % o->NLOS == 0) orc_spal_loss(o);
 
   /* mass fluxes, mf:3960-4190 */
   for (int K = 1; K <= KMM1; K++)
@@ -1290,6 +1933,57 @@ static void step_once(orc_t *o) {
       A3(RFLUX, I, J, KM) = 0.0f;
     }
   orc_tstep(o, o->RO, o->DRO, 1);
+
+  if (o->ILOS >= 100) { /* cell-average density, mf:4260-4270 */
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IMM1; I++) {
+          float ROAVG = 0.125f * (A3(RO, I, J, K) + A3(RO, I + 1, J, K) + A3(RO, I + 1, J, K + 1) + A3(RO, I, J, K + 1) +
+                                  A3(RO, I, J - 1, K) + A3(RO, I + 1, J - 1, K) + A3(RO, I + 1, J - 1, K + 1) +
+                                  A3(RO, I, J - 1, K + 1));
+          A3(ROAVG_CELL, I, J, K) = ROAVG;
+        }
+  }
+  if (o->ILOS >= 200) { /* SA transport: doubled fluxes and source, TSTEP NCALL=2, mf:4279-4351 */
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 1; J <= JM; J++) {
+        int JP1 = J + 1;
+        if (J == JM) JP1 = JM;
+        if (J == 1) JP1 = 1;
+        if (A1(INDMIX, J) == 1) JP1 = J;
+        for (int I = 1; I <= IMM1; I++)
+          A3(XFLUX, I, J, K) = A3(FLOWX, I, J, K) * (A3(TRANS_KVIS, I, J, K) + A3(TRANS_KVIS, I, JP1, K));
+      }
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IMM1; I++) A3(SOURCE, I, J, K) = -2.0f * A3(T_SOURCE, I, J, K);
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 2; I <= IMM1; I++)
+          A3(TFLUX, I, J, K) = A3(FLOWT, I, J, K) * (A3(TRANS_KVIS, I - 1, J, K) + A3(TRANS_KVIS, I, J, K));
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++) {
+        A3(TFLUX, 1, J, K) = A3(FLOWT, 1, J, K) * (A3(TRANS_KVIS, IMM1, J, K) + A3(TRANS_KVIS, 1, J, K));
+        A3(TFLUX, IM, J, K) = A3(TFLUX, 1, J, K);
+      }
+    for (int K = 2; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IMM1; I++)
+          A3(RFLUX, I, J, K) = A3(FLOWR, I, J, K) * (A3(TRANS_KVIS, I, J, K - 1) + A3(TRANS_KVIS, I, J, K));
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        A3(RFLUX, I, J, 1) = 0.0f;
+        A3(RFLUX, I, J, KM) = 0.0f;
+      }
+    orc_tstep(o, o->TRANS_DYN_VIS, o->DEL_DYNVIS, 2);
+    float TRANSVISMIN = 0.1f * A3(VISC_LAM, IMID, 2, KMID);
+    for (int K = 1; K <= KMM1; K++)
+      for (int J = 2; J <= JM; J++)
+        for (int I = 1; I <= IMM1; I++) {
+          if (A3(TRANS_DYN_VIS, I, J, K) < TRANSVISMIN) A3(TRANS_DYN_VIS, I, J, K) = TRANSVISMIN;
+          A3(TRANS_KVIS, I, J, K) = A3(TRANS_DYN_VIS, I, J, K) / A3(ROAVG_CELL, I, J, K);
+        }
+  }
 
   /* energy fluxes, mf:4357-4456 */
   for (int K = 1; K <= KMM1; K++)
@@ -1395,6 +2089,15 @@ static void step_once(orc_t *o) {
         A3(ROVX, I, 1, K) = A3(RO, I, 1, K) * A3(VX, I, 1, K);
         A3(ROE, I, 1, K) = A3(P, I, 1, K) / o->GA1 + A3(RO, I, 1, K) * 0.5f * VABSQ;
         A3(HO, I, 1, K) = o->CP * A1(TO1, K);
+        if (o->ILOS >= 200) { /* inlet turbulent viscosity, mf:4679-4690 */
+          float TRANSVISIN;
+          if (A1(FSTURB, 1) < 1.0f) TRANSVISIN = A3(VISC_LAM, I, 2, K) * 4.35f * powf(A1(FSTURB, 1), 0.25f);
+          else TRANSVISIN = A3(VISC_LAM, I, 2, K) * (3.5f + A1(FSTURB, 1) * 0.85f);
+          A3(TRANS_DYN_VIS, I, 1, K) = TRANSVISIN;
+          A3(TRANS_DYN_VIS, I, 2, K) = A3(TRANS_DYN_VIS, I, 1, K);
+          A3(TRANS_KVIS, I, 1, K) = A3(TRANS_DYN_VIS, I, 1, K) / A3(RO, IMID, 1, KMID);
+          A3(TRANS_KVIS, I, 2, K) = A3(TRANS_KVIS, I, 1, K);
+        }
         /* exit, mf:4690-4730 */
         float EKE = .5f * (A3(VR, I, JM, K) * A3(VR, I, JM, K) + A3(VT, I, JM, K) * A3(VT, I, JM, K) +
                            A3(VX, I, JM, K) * A3(VX, I, JM, K));
@@ -1625,6 +2328,7 @@ static void step_once(orc_t *o) {
     o->VREF = VREF;
     if (o->IFPRINT == 1 || o->IFEND == 1) {
       if (o->ILOS == 10) orc_loss(o);
+      if (o->ILOS >= 200) orc_spal_loss(o);
     }
     orc_stepup(o, 0.25f);
   }

@@ -3,7 +3,7 @@
  * Sequential CPU restatement of the reference's time-marching loop
  * (multall-open-18.3.f: LOOP :3664-5463, TSTEP :5470-5932, SMOOTH_VAR
  * :15352-15594, SMOOTH_RESID :14833-14917, STEPUP :7916-8002, NEW_MIXPLAN
- * :14922-15251, LOSS :8058-9066).  Arithmetic is FP32 in the reference's
+ * :14922-15251, LOSS :8058-9066, SPAL_LOSS :13586-14827).  Arithmetic is FP32 in the reference's
  * statement order so the result can be compared bit-for-bit with the
  * compiled reference on the same host.
  *
@@ -41,6 +41,7 @@ void orc_smooth_var(orc_t *o, float *d);
 void orc_stepup(orc_t *o, float relax);
 void orc_new_mixplan(orc_t *o);
 void orc_loss(orc_t *o);
+void orc_spal_loss(orc_t *o);
 
 #ifdef __cplusplus
 }

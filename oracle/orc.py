@@ -22,7 +22,9 @@ NODE3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "ROVT", "ROWT", "P", "HO", "PEFF"
          "WT", "T_STATIC", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "AQX", "AQR", "ABX", "ABR",
          "ASX", "ASR", "VOL", "VOLOR", "STEP", "BSTEP", "RSTEP", "XFORCE", "RFORCE", "TFORCE",
          "QSOURCE", "FLOWX", "FLOWT", "FLOWR", "XFLUX", "TFLUX", "RFLUX", "SOURCE", "STORE", "TEMP1",
-         "SGEN", "DWALLSQ", "Y_PLUS"]
+         "SGEN", "DWALLSQ", "Y_PLUS", "TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "ROAVG_CELL",
+         "VISC_LAM", "VISC_RAT", "VISC_TURB", "XLENGTH", "FST_RAT", "DYN_VIS_ML", "ABS_VORT", "TRANS_VORT",
+         "GRAD_TVIS", "TVISGRADX", "TVISGRADR", "TVISGRADT"]
 JK2 = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID"]
 IK2 = ["FTL", "FBL", "FTR", "FBR"]
 VEC_I = ["FP", "FU", "FD"]
@@ -30,7 +32,7 @@ VEC_K = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "B
          "PLIM", "VM1"]
 VEC_J = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 VEC_NR = ["VLIMIT", "ROLIMIT", "AVG_CHG", "AVG_BLK", "XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM",
-          "XLLIM_DWN", "XLLIM_UP", "CHORD", "ROUGH_H", "ROUGH_T", "ROUGH_L", "ROUGH_U", "VISCOSY", "REYNOLDS",
+          "XLLIM_DWN", "XLLIM_UP", "CHORD", "ROUGH_H", "ROUGH_T", "ROUGH_L", "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "VISCOSY", "REYNOLDS",
           "ROVEXIT"]
 NRTAB = ["FILAM", "FITURB", "FIWAKE", "FIUP", "FKLAM", "FKTURB"]     # (NRS, MAXKI) tables
 INT_I = ["IB1", "IB2"]
@@ -42,7 +44,8 @@ SCALARS_F = ["CFL", "DAMP", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "
              "FAC_4TH", "F1", "F2", "F3", "RSMTH", "SFEXIT", "FSMTHB", "RFIN", "PDOWN_HUB",
              "PDOWN_TIP", "FEXTRAP", "RFMIX", "FANGLE", "CP", "GA", "RGAS", "CV", "FGA", "GA1", "RCP",
              "VSOUND", "FACMIXUP", "REYNO", "RF_VIS", "FTRANS", "PRANDTL", "YPLUSWALL",
-             "TURBVIS_LIM", "VISLAM", "FTCOND", "YPLAM", "YPTURB", "CFWALL"]
+             "TURBVIS_LIM", "VISLAM", "FTCOND", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0",
+             "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX"]
 SCALARS_I = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
              "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS",
              "NMIXUP", "NIB1", "NJB1", "NKB1", "NIB2", "NJB2", "NKB2", "NSBLK", "ITIMST",

@@ -14,6 +14,7 @@ and trimmed to the logical extents where the caller asks for it.
 from __future__ import annotations
 
 import os
+import resource
 import re
 import shutil
 import subprocess
@@ -136,8 +137,10 @@ def run_reference(deck_text: str, variant: str = "fp32", workdir: Optional[str] 
     try:
         with open(os.path.join(wd, "intype"), "w") as f:
             f.write("N\n")
+        def unlimit_stack():  # SPAL_LOSS keeps four (ID,JD,KD) work arrays on the stack
+            resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
         p = subprocess.run([exe], input=deck_text, cwd=wd, capture_output=True, text=True,
-                           timeout=timeout)
+                           timeout=timeout, preexec_fn=unlimit_stack)
         if p.returncode != 0 or not os.path.exists(os.path.join(wd, "final.bin")):
             raise RuntimeError(f"reference failed rc={p.returncode}\n{p.stdout[-3000:]}\n{p.stderr[-2000:]}")
         layout = parse_commall()

@@ -29,6 +29,7 @@ from multall_hip import deckgen  # noqa: E402
 
 OUT_VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP",
             "XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
+SA_VARS = ["TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "Y_PLUS"]   # ILOS >= 200 decks
 
 CASES = {
     "tiny1_inv": lambda n: deckgen.deck_tiny(ilos=0, nrows=1, nsteps=n),
@@ -36,6 +37,8 @@ CASES = {
     "tiny3_ml": lambda n: deckgen.deck_tiny(ilos=10, nrows=3, nsteps=n),
     "tiny3_ml2": lambda n: deckgen.deck_tiny_ml_variant(nsteps=n),
     "tiny1_mlw": lambda n: deckgen.deck_tiny_wall_variant(nsteps=n),
+    "tiny3_sa": lambda n: deckgen.deck_tiny_sa(nsteps=n),
+    "tiny3_sa2": lambda n: deckgen.deck_tiny_sa_variant(nsteps=n),
 }
 STEPS = (1, 100)
 
@@ -73,7 +76,8 @@ def make(case):
                     assert v == setup_ref["scalars"], case
                 else:
                     assert np.array_equal(v, setup_ref[k]), (case, k)
-        out = {k: final.node(k).astype(np.float32) for k in OUT_VARS}
+        names = OUT_VARS + (SA_VARS if st["scalars"]["ILOS"] >= 200 else [])
+        out = {k: final.node(k).astype(np.float32) for k in names}
         out["NSTEP"] = np.array(int(final["NSTEP"]))
         np.savez_compressed(os.path.join(HERE, f"{case}_ref{n}.npz"), **out)
         print(f"{case}: {n} steps done")

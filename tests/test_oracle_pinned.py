@@ -16,7 +16,11 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 # inviscid decks, and mixing-length (LOSS) decks covering the chord-Reynolds
 # and power-law laminar viscosity, smooth and rough walls, a rotor tip gap,
 # the fixed wall y+ and a shear-free hub
-CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw"]
+CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2"]
+# Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
+# shear-free casing and the mixing-length relaxation source) also pin the
+# turbulence state
+SA_VARS = ["TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "Y_PLUS"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -28,7 +32,7 @@ def test_oracle_matches_reference_bitwise(case, nsteps):
     o.start()
     assert o.step(nsteps + 10) == nsteps          # the run ends at NSTEPS_MAX like the reference
     ref = load_ref(case, nsteps)
-    for v in VARS:
+    for v in VARS + [v for v in SA_VARS if v in ref]:
         a, b = o[v], ref[v]
         bad = np.count_nonzero(a.view(np.uint32) != b.view(np.uint32))
         assert bad == 0, f"{case} {nsteps} steps: {v} differs in {bad} of {a.size} values"
