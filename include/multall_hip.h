@@ -100,6 +100,14 @@ enum mh_array_id {
   MH_XLLIM_I1, MH_XLLIM_IM, MH_XLLIM_K1, MH_XLLIM_KM, MH_XLLIM_DWN, MH_XLLIM_UP,
   MH_ROUGH_H, MH_ROUGH_T, MH_ROUGH_L, MH_ROUGH_U,   /* RN, float */
   MH_JTRAN_I1, MH_JTRAN_IM, MH_JTRAN_K1, MH_JTRAN_KM, /* RN, int */
+  /* Spalart-Allmaras model, SPAL_LOSS (BKSPALOS, BKNEWLOS): the transported
+   * turbulent viscosity and its SSS history, its kinematic value and source,
+   * the cell density, laminar viscosity and viscosity ratio, the mixing
+   * length and free-stream factor of SET_XLENGTH -- N (cell-centred); the
+   * free-stream turbulence ratio and mixing-plane decay per row -- RN */
+  MH_TRANS_DYN_VIS, MH_DEL_DYNVIS, MH_TRANS_KVIS, MH_T_SOURCE, MH_ROAVG_CELL, MH_VISC_LAM, MH_VISC_RAT,
+  MH_XLENGTH, MH_FST_RAT,
+  MH_FSTURB, MH_TURBVIS_DAMP,
   MH_NARRAYS
 };
 
@@ -116,6 +124,9 @@ enum mh_scalar_id {
   /* viscous model (BKLOSS, BKGAS, BKNEWLOS) */
   MH_S_REYNO, MH_S_RF_VIS, MH_S_FTRANS, MH_S_PRANDTL, MH_S_YPLUSWALL, MH_S_YPLAM, MH_S_YPTURB,
   MH_S_CFWALL, MH_S_IBOUND,
+  /* Spalart-Allmaras source factors and limits (BKSPALOS) */
+  MH_S_FAC_STMIX, MH_S_FAC_ST0, MH_S_FAC_ST1, MH_S_FAC_ST2, MH_S_FAC_ST3, MH_S_FAC_SFVIS, MH_S_FAC_VORT,
+  MH_S_FAC_PGRAD, MH_S_VORT_MAX, MH_S_TURBVIS_LIM,
   /* optional state: laminar viscosity and conductivity ratio left by the
    * last LOSS call (default 0; mh_loss_init sets them) */
   MH_S_VISLAM, MH_S_FTCOND,
@@ -149,8 +160,10 @@ double mh_get_scalar(const mh_ctx *ctx, int id);
 
 /* The viscous-force initialisation SETUP performs before LOOP
  * (mf:7346-7357): one LOSS call (mf:8058-9066) with RF_VIS = 1, FMIXUP = 1
- * and NSTEP = 1, which also sets VISLAM/FTCOND.  ILOS = 10 only; a no-op for
- * ILOS = 0.  Call after all inputs are set, before or after mh_start. */
+ * and NSTEP = 1, which also sets VISLAM/FTCOND; with ILOS >= 200 the call is
+ * SPAL_LOSS (mf:13586-14827), which also sets the initial turbulent
+ * viscosity.  A no-op for ILOS = 0.  Call after all inputs are set, before
+ * or after mh_start. */
 int mh_loss_init(mh_ctx *ctx);
 
 /* LOOP prologue (mf:3674-3714).  Call once after all inputs are set. */

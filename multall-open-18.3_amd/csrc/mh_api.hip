@@ -23,7 +23,7 @@ __global__ void k_flux_energy(Geo, Dev);
 __global__ void k_flux_axial(Geo, Dev);
 __global__ void k_flux_angular(Geo, Dev);
 __global__ void k_flux_radial(Geo, Dev);
-__global__ void k_bc_sums(Geo, Dev, Params);
+__global__ void k_bc_sums(Geo, Dev, Params, StepScalars);
 __global__ void k_bc_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_peff(Geo, Dev, Params, StepScalars);
 __global__ void k_fextrap(Geo, Dev, Params);
@@ -57,6 +57,20 @@ __global__ void k_integrals(Geo, Dev, Params);
 __global__ void k_loss_vislam(Geo, Dev, Params);
 __global__ void k_loss_span(Geo, Dev, Params, float, float);
 __global__ void k_loss_pitch(Geo, Dev, Params, float, float);
+__global__ void k_sa_cellavg(Geo, Dev);
+__global__ void k_sa_fluxes(Geo, Dev);
+__global__ void k_sa_addstore(Geo, Dev, float *);
+__global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
+__global__ void k_sa_dbc(Geo, Dev, Params, float *);
+__global__ void k_sa_kvis(Geo, Dev, Params);
+__global__ void k_sa_grad(Geo, Dev, Params);
+__global__ void k_sa_visc(Geo, Dev, Params, float, int);
+__global__ void k_sa_gradtvis(Geo, Dev, Params);
+__global__ void k_sa_trcol(Geo, Dev, Params);
+__global__ void k_sa_trline(Geo, Dev, Params);
+__global__ void k_sa_source(Geo, Dev, Params, float);
+__global__ void k_sa_iforces(Geo, Dev, Params);
+__global__ void k_sa_kforces(Geo, Dev, Params, float);
 
 const ArrayInfo kArrays[MH_NARRAYS] = {
     {"RO", K_N, false}, {"ROVX", K_N, false}, {"ROVR", K_N, false}, {"RORVT", K_N, false}, {"ROE", K_N, false},
@@ -87,6 +101,9 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"XLLIM_DWN", K_RN, false}, {"XLLIM_UP", K_RN, false}, {"ROUGH_H", K_RN, false}, {"ROUGH_T", K_RN, false},
     {"ROUGH_L", K_RN, false}, {"ROUGH_U", K_RN, false}, {"JTRAN_I1", K_RN, true}, {"JTRAN_IM", K_RN, true},
     {"JTRAN_K1", K_RN, true}, {"JTRAN_KM", K_RN, true},
+    {"TRANS_DYN_VIS", K_N, false}, {"DEL_DYNVIS", K_N, false}, {"TRANS_KVIS", K_N, false},
+    {"T_SOURCE", K_N, false}, {"ROAVG_CELL", K_N, false}, {"VISC_LAM", K_N, false}, {"VISC_RAT", K_N, false},
+    {"XLENGTH", K_N, false}, {"FST_RAT", K_N, false}, {"FSTURB", K_RN, false}, {"TURBVIS_DAMP", K_RN, false},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -95,7 +112,8 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "FGA", "GA1", "RCP", "VSOUND", "FACMIXUP", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
     "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS", "NMIXUP", "ITIMST",
     "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
-    "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "VISLAM", "FTCOND", "EXACT_ORDER",
+    "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "FAC_STMIX", "FAC_ST0", "FAC_ST1", "FAC_ST2", "FAC_ST3",
+    "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "VISLAM", "FTCOND", "EXACT_ORDER",
 };
 
 }  // namespace mh
@@ -146,6 +164,9 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.XLLIM_IM, (void **)&d.XLLIM_K1, (void **)&d.XLLIM_KM, (void **)&d.XLLIM_DWN,
       (void **)&d.XLLIM_UP, (void **)&d.ROUGH_H, (void **)&d.ROUGH_T, (void **)&d.ROUGH_L, (void **)&d.ROUGH_U,
       (void **)&d.JTRAN_I1, (void **)&d.JTRAN_IM, (void **)&d.JTRAN_K1, (void **)&d.JTRAN_KM,
+      (void **)&d.TRANS_DYN_VIS, (void **)&d.DEL_DYNVIS, (void **)&d.TRANS_KVIS, (void **)&d.T_SOURCE,
+      (void **)&d.ROAVG_CELL, (void **)&d.VISC_LAM, (void **)&d.VISC_RAT, (void **)&d.XLENGTH, (void **)&d.FST_RAT,
+      (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP,
   };
   return base[id];
 }
@@ -209,6 +230,10 @@ Params params_of(const mh_ctx *c) {
   p.REYNO = sf(c, MH_S_REYNO); p.RF_VIS = sf(c, MH_S_RF_VIS); p.FTRANS = sf(c, MH_S_FTRANS);
   p.PRANDTL = sf(c, MH_S_PRANDTL); p.YPLUSWALL = sf(c, MH_S_YPLUSWALL); p.YPLAM = sf(c, MH_S_YPLAM);
   p.YPTURB = sf(c, MH_S_YPTURB); p.CFWALL = sf(c, MH_S_CFWALL); p.IBOUND = si(c, MH_S_IBOUND);
+  p.FAC_STMIX = sf(c, MH_S_FAC_STMIX); p.FAC_ST0 = sf(c, MH_S_FAC_ST0); p.FAC_ST1 = sf(c, MH_S_FAC_ST1);
+  p.FAC_ST2 = sf(c, MH_S_FAC_ST2); p.FAC_ST3 = sf(c, MH_S_FAC_ST3); p.FAC_SFVIS = sf(c, MH_S_FAC_SFVIS);
+  p.FAC_VORT = sf(c, MH_S_FAC_VORT); p.FAC_PGRAD = sf(c, MH_S_FAC_PGRAD); p.VORT_MAX = sf(c, MH_S_VORT_MAX);
+  p.TURBVIS_LIM = sf(c, MH_S_TURBVIS_LIM);
   return p;
 }
 
@@ -412,6 +437,21 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
     LAUNCH(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
     LCHK("damp");
   }
+  if (ncall == 2) {
+    // the cell-centred turbulent viscosity (mf:5736-5777, then 8700):
+    // add the change, carry the pitch average over each mixing plane,
+    // smooth once with FAC_SFVIS*SFT, then the periodic/tip/mixing-plane
+    // conditions with ILAST = IM-1
+    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, c->stream, g, d, D);
+    if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, c->stream, g, d, p, D);
+    const float sm = p.FAC_SFVIS * s.SFT;
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.TMP, sm, 0);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.TMP, d.DSC, sm, 1);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.DSC, D, sm, 2);
+    LAUNCH(k_sa_dbc, dim3(g.KM, g.JM), dim3(64), 0, c->stream, g, d, p, D);
+    LCHK("sa_tstep");
+    return MH_OK;
+  }
   if (nmp > 0) {
     LAUNCH(k_mpavg, dim3(nmp), dim3(64), 0, c->stream, g, d, p, 2);
     LCHK("mpavg2");
@@ -523,6 +563,56 @@ int loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   return MH_OK;
 }
 
+// Per-J values of the variables SPAL_LOSS's periodic-force block reads
+// without setting them (its J-1 and row number keep the values of the last
+// blade J its K, J loop visited; before the first one, those left by its
+// earlier loops), and the row its hub/casing loop then reads.
+int sa_maps(mh_ctx *c) {
+  const Geo &g = c->g;
+  const int JM = g.JM;
+  std::vector<int> IND(JM), NROW(JM), INDMIX(JM), map(4 * JM + 4, 0);
+  int r;
+  if ((r = d2h(c, IND.data(), c->d.IND, JM * 4)) || (r = d2h(c, NROW.data(), c->d.NROW, JM * 4)) ||
+      (r = d2h(c, INDMIX.data(), c->d.INDMIX, JM * 4)))
+    return r;
+  for (int J = 2; J <= JM; J++)
+    if (INDMIX[J - 1] == 1 && INDMIX[J - 2] == 1) return fail(c, MH_E_UNSUPPORTED, "adjacent mixing-plane J");
+  int last = 0;
+  for (int J = 2; J <= JM; J++)
+    if (IND[J - 1] == 1) last = J;
+  for (int pass = 0; pass < 2; pass++) {  // K = 1, then K > 1
+    int jm1 = JM - 1, nrw = NROW[JM - 1];
+    if (pass == 1 && last) { jm1 = last - 1; nrw = NROW[last - 1]; }
+    for (int J = 2; J <= JM; J++) {
+      if (IND[J - 1] == 1) { jm1 = J - 1; nrw = NROW[J - 1]; }
+      map[pass * JM + J - 1] = jm1;
+      map[(2 + pass) * JM + J - 1] = nrw;
+    }
+  }
+  map[4 * JM] = last ? NROW[last - 1] : NROW[JM - 1];
+  return h2d(c, c->d.SAJMAP, map.data(), map.size() * 4);
+}
+
+// One SPAL_LOSS call (mf:13586-14827) with the given relaxation factor,
+// FMIXUP and step number.
+int spal_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const Params p = params_of(c);
+  const dim3 grid = node_grid(g);
+  if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_grad, grid, kNodeBlock, 0, c->stream, g, d, p);
+  LAUNCH(k_sa_visc, grid, kNodeBlock, 0, c->stream, g, d, p, fmixup, nstep);
+  LAUNCH(k_sa_gradtvis, grid, kNodeBlock, 0, c->stream, g, d, p);
+  LAUNCH(k_sa_trcol, dim3((g.IMM1 + 63) / 64, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_trline, dim3((g.KMM1 + 63) / 64, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_source, grid, kNodeBlock, 0, c->stream, g, d, p, rf_vis);
+  LAUNCH(k_sa_iforces, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_kforces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis);
+  LCHK("spal_loss");
+  return MH_OK;
+}
+
 // One time step: the body of LOOP from label 5000 (mf:3753-5454).
 int step_once(mh_ctx *c) {
   const Geo &g = c->g;
@@ -554,15 +644,24 @@ int step_once(mh_ctx *c) {
     LAUNCH(k_vrms_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("vrms");
   }
-  // viscous forces every NLOS steps, mf:3935
+  // viscous forces every NLOS steps, mf:3935-3942
   if (p.ILOS == 10 && c->nstep % p.NLOS == 0 && (r = loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
+  if (p.ILOS >= 200 && c->nstep % p.NLOS == 0 && (r = spal_loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
   if ((r = tstep(c, d.RO, d.DRO, 1))) return r;
+  if (p.ILOS >= 200) {
+    // transport of the turbulent viscosity, mf:4260-4351
+    LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
+    LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
+    if ((r = tstep(c, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2))) return r;
+    LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
+    LCHK("sa_transport");
+  }
   LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_energy");
   if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
-  LAUNCH(k_bc_sums, dim3(1), dim3(1024), 6 * g.KM * sizeof(float), c->stream, g, d, p);
+  LAUNCH(k_bc_sums, dim3(1), dim3(1024), 6 * g.KM * sizeof(float), c->stream, g, d, p, s);
   LAUNCH(k_bc_apply, dim3((g.IM * g.KM + 255) / 256), dim3(256), 0, c->stream, g, d, p, s);
   LCHK("bc");
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
@@ -595,6 +694,7 @@ int step_once(mh_ctx *c) {
     LCHK("convergence");
     // forces brought up to date before the final output, mf:5338-5339
     if (c->ifend == 1 && p.ILOS == 10 && (r = loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
+    if (c->ifend == 1 && p.ILOS >= 200 && (r = spal_loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
     if ((r = stepup(c, 0.25f))) return r;
     float sc[8];
     HIPCHK(hipMemcpyAsync(sc, d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
@@ -676,9 +776,18 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   if ((r = dalloc(c, &d.XFLUX, N)) || (r = dalloc(c, &d.TFLUX, N)) || (r = dalloc(c, &d.RFLUX, N)) ||
       (r = dalloc(c, &d.SOURCE, N)) || (r = dalloc(c, &d.STORE, N)) || (r = dalloc(c, &d.DSC, N)) ||
       (r = dalloc(c, &d.TMP, N)) || (r = dalloc(c, &d.TEMP1, N)) ||
+      (r = dalloc(c, &d.SXX, N)) || (r = dalloc(c, &d.SXR, N)) || (r = dalloc(c, &d.SXT, N)) ||
+      (r = dalloc(c, &d.SRR, N)) || (r = dalloc(c, &d.SRT, N)) || (r = dalloc(c, &d.STT, N)) ||
+      (r = dalloc(c, &d.GQX, N)) || (r = dalloc(c, &d.GQR, N)) || (r = dalloc(c, &d.GQT, N)) ||
+      (r = dalloc(c, &d.ABS_VORT, N)) || (r = dalloc(c, &d.VISC_TURB, N)) || (r = dalloc(c, &d.DYN_VIS_ML, N)) ||
+      (r = dalloc(c, &d.GRAD_TVIS, N)) || (r = dalloc(c, &d.TVGX, N)) || (r = dalloc(c, &d.TVGR, N)) ||
+      (r = dalloc(c, &d.TVGT, N)) || (r = dalloc(c, &d.FORCEX, N)) || (r = dalloc(c, &d.FORCER, N)) ||
+      (r = dalloc(c, &d.FORCET, N)) || (r = dalloc(c, &d.ESOURCE, N)) ||
       (r = dalloc(c, &d.B1CHG, kind_len(dm, K_B1))) || (r = dalloc(c, &d.B2CHG, kind_len(dm, K_B2))) ||
-      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)3 * dm.km)) ||
+      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)3 * dm.km + 4)) ||
       (r = dalloc(c, &d.LTAB, (size_t)dm.nrows * (4 * dm.im + 2 * dm.km))) ||
+      (r = dalloc(c, &d.SAFLAG, (size_t)dm.im * dm.jm + (size_t)dm.jm * dm.km)) ||
+      (r = dalloc(c, &d.SAJMAP, (size_t)4 * dm.jm + 4)) ||
       (r = dalloc(c, &d.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))) ||
       (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
@@ -775,11 +884,13 @@ static int check_inputs(mh_ctx *c) {
   if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
   if (p.IN_FLOW != 0 || p.IF_REPEAT != 0 || p.IF_RESTART != 0)
     return fail(c, MH_E_UNSUPPORTED, "IN_FLOW, IF_REPEAT or IF_RESTART set");
-  if (p.ILOS != 0 && p.ILOS != 10)
-    return fail(c, MH_E_UNSUPPORTED, "viscous model other than ILOS = 0 or 10 (mixing length)");
-  if (p.ILOS == 10 && p.YPLUSWALL < -0.001f)
+  if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 200)
+    return fail(c, MH_E_UNSUPPORTED, "viscous model other than ILOS = 0, 10 (mixing length) or >= 200 (SA)");
+  if (p.ILOS != 0 && p.YPLUSWALL < -0.001f)
     return fail(c, MH_E_UNSUPPORTED, "Shih wall functions (YPLUSWALL < 0)");
-  if (p.ILOS == 10 && p.NLOS < 1) return fail(c, MH_E_ARG, "NLOS < 1");
+  if (p.ILOS >= 200 && (p.FAC_VORT > 0.001f || p.FAC_PGRAD > 0.001f))
+    return fail(c, MH_E_UNSUPPORTED, "SA helicity or pressure-gradient source factors (FAC_VORT, FAC_PGRAD)");
+  if (p.ILOS != 0 && p.NLOS < 1) return fail(c, MH_E_ARG, "NLOS < 1");
   if (p.IMID < 1 || p.IMID > c->g.IM || p.KMID < 1 || p.KMID > c->g.KM) return fail(c, MH_E_ARG, "IMID/KMID");
   c->g.IMID = p.IMID;
   c->g.KMID = p.KMID;
@@ -792,10 +903,13 @@ int mh_loss_init(mh_ctx *c) {
   if (r) return r;
   const Params p = params_of(c);
   if (p.ILOS == 0) return MH_OK;
-  if ((r = loss_tables(c))) return r;
   float vf[2] = {(float)c->scal[MH_S_VISLAM], (float)c->scal[MH_S_FTCOND]};
   if ((r = h2d(c, c->d.SC + SC_VISLAM, vf, sizeof vf))) return r;
-  if ((r = loss(c, 1.0f, 1.0f, 1))) return r;
+  if (p.ILOS == 10) {
+    if ((r = loss_tables(c)) || (r = loss(c, 1.0f, 1.0f, 1))) return r;
+  } else {
+    if ((r = sa_maps(c)) || (r = spal_loss(c, 1.0f, 1.0f, 1))) return r;
+  }
   if ((r = d2h(c, vf, c->d.SC + SC_VISLAM, sizeof vf))) return r;
   c->scal[MH_S_VISLAM] = vf[0];
   c->scal[MH_S_FTCOND] = vf[1];
@@ -809,8 +923,9 @@ int mh_start(mh_ctx *c) {
   if (r) return r;
   const Params p = params_of(c);
   if ((r = build_lists(c))) return r;
-  if (p.ILOS == 10) {
-    if ((r = loss_tables(c))) return r;
+  if (p.ILOS != 0) {
+    if (p.ILOS == 10 && (r = loss_tables(c))) return r;
+    if (p.ILOS >= 200 && (r = sa_maps(c))) return r;
     float vf[2] = {(float)c->scal[MH_S_VISLAM], (float)c->scal[MH_S_FTCOND]};
     if ((r = h2d(c, c->d.SC + SC_VISLAM, vf, sizeof vf))) return r;
   }

@@ -54,11 +54,22 @@ struct Dev {
   float *XLLIM_I1, *XLLIM_IM, *XLLIM_K1, *XLLIM_KM, *XLLIM_DWN, *XLLIM_UP;
   float *ROUGH_H, *ROUGH_T, *ROUGH_L, *ROUGH_U;
   int *JTRAN_I1, *JTRAN_IM, *JTRAN_K1, *JTRAN_KM;
+  float *TRANS_DYN_VIS, *DEL_DYNVIS, *TRANS_KVIS, *T_SOURCE, *ROAVG_CELL, *VISC_LAM, *VISC_RAT, *XLENGTH, *FST_RAT;
+  float *FSTURB, *TURBVIS_DAMP;
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
   float *BCW;                        // exit/inlet pitch averages per K (3*KM)
   float *LTAB;                       // LOSS mixing-length factors, see LT_* below
+  // Spalart-Allmaras work arrays (cell-centred, N): the velocity and
+  // temperature gradients of SPAL_LOSS (strain rates, before the viscosity),
+  // vorticity, turbulent and mixing-length viscosities, the gradient of the
+  // kinematic turbulent viscosity, and the force/energy accumulators
+  float *SXX, *SXR, *SXT, *SRR, *SRT, *STT, *GQX, *GQR, *GQT;
+  float *ABS_VORT, *VISC_TURB, *DYN_VIS_ML, *GRAD_TVIS, *TVGX, *TVGR, *TVGT;
+  float *FORCEX, *FORCER, *FORCET, *ESOURCE;
+  int *SAFLAG;                       // transition flags per (I,J) column and (J,K) line
+  int *SAJMAP;                       // per J: the J-1 and row the reference's periodic-force block reads
   double *RED;                       // FP64 partial sums of the reductions
   float *SC;                         // device-side scalars (see SC_* below)
   int *FLAGS;                        // per (I,K) line flags for the density limiter
@@ -104,6 +115,7 @@ struct Params {
       JR, KR, ILOS, NLOS, NMIXUP, ITIMST, IF_RESTART, IFGAS, IN_FLOW, IF_REPEAT, IRBB, KRBB, EXACT;
   float REYNO, RF_VIS, FTRANS, PRANDTL, YPLUSWALL, YPLAM, YPTURB, CFWALL;
   int IBOUND;
+  float FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD, VORT_MAX, TURBVIS_LIM;
 };
 
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid

@@ -296,7 +296,30 @@ __global__ void k_flux_radial(Geo g, Dev d) {
 // k_bc_apply updates the inlet and exit nodes in parallel.  d.BCW carries
 // PAVG2, PAVG3 and PAVGOUT (KM each) from the first to the second.
 
-__global__ void k_bc_sums(Geo g, Dev d, Params p) {
+// inlet pressure and static temperature of node (I,1,K) from the state
+// before the update, mf:4566-4606
+__device__ __forceinline__ void inlet_pt(const Geo &g, const Dev &d, const Params &p, const StepScalars &s,
+                                         const float *PAVG2, const float *PAVG3, int I, int K, float &P1,
+                                         float &TSTATIC) {
+  const unsigned n1 = IX(I, 1, K);
+  const float PO1 = d.PO1[K - 1], TO1 = d.TO1[K - 1];
+  float RO_STAG = PO1 / p.RGAS / TO1;
+  float RO_IN = d.RO[n1];
+  if (RO_IN > 0.9999f * RO_STAG) RO_IN = 0.9999f * RO_STAG;
+  float PNEW = PO1 * powf(RO_IN / RO_STAG, p.GA);
+  P1 = d.P[n1];
+  if (p.IN_PRESS == 0) P1 = s.RFIN1 * P1 + p.RFIN * PNEW;
+  if (abs(p.IN_PRESS) == 1) P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(I, 2, K)] - p.RFIN * d.P[IX(I, 3, K)];
+  if (abs(p.IN_PRESS) == 3) P1 = s.RFIN1 * P1 + s.RFIN2 * PAVG2[K - 1] - p.RFIN * PAVG3[K - 1];
+  if (abs(p.IN_PRESS) == 4)
+    P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(p.IMID, 2, p.KMID)] - p.RFIN * d.P[IX(p.IMID, 3, p.KMID)];
+  const float plim = d.SC[SC_PLIM + K - 1];
+  if (P1 > plim) P1 = 0.9f * P1 + 0.1f * plim;
+  if (P1 > 0.99999f * PO1) P1 = PO1 * 0.99999f;
+  TSTATIC = TO1 * powf(P1 * d.RPO1[K - 1], p.FGA);
+}
+
+__global__ void k_bc_sums(Geo g, Dev d, Params p, StepScalars s) {
   extern __shared__ float sh[];
   float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *PAVGOUT = d.BCW + 2 * g.KM, *SUMK = sh + 3 * g.KM;
   const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
@@ -351,6 +374,14 @@ __global__ void k_bc_sums(Geo g, Dev d, Params p) {
   }
   __syncthreads();
   for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) d.PD[K - 1] = PDL[K - 1];
+  if (threadIdx.x == 0 && p.ILOS >= 200) {
+    // RO(IMID,1,KMID) before and after its inlet update (the SA inlet
+    // condition reads whichever the reference's loop has reached)
+    float P1, TSTATIC;
+    inlet_pt(g, d, p, s, PAVG2, PAVG3, p.IMID, p.KMID, P1, TSTATIC);
+    d.BCW[3 * g.KM] = d.RO[IX(p.IMID, 1, p.KMID)];
+    d.BCW[3 * g.KM + 1] = P1 / (p.RGAS * TSTATIC);
+  }
 }
 
 __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
@@ -361,22 +392,10 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
   {
     const int I = t % IM + 1, K = t / IM + 1;
     const size_t n1 = IX(I, 1, K);
-    const float PO1 = d.PO1[K - 1], TO1 = d.TO1[K - 1];
-    float RO_STAG = PO1 / p.RGAS / TO1;
-    float RO_IN = d.RO[n1];
-    if (RO_IN > 0.9999f * RO_STAG) RO_IN = 0.9999f * RO_STAG;
-    float PNEW = PO1 * powf(RO_IN / RO_STAG, p.GA);
-    float P1 = d.P[n1];
-    if (p.IN_PRESS == 0) P1 = s.RFIN1 * P1 + p.RFIN * PNEW;
-    if (abs(p.IN_PRESS) == 1) P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(I, 2, K)] - p.RFIN * d.P[IX(I, 3, K)];
-    if (abs(p.IN_PRESS) == 3) P1 = s.RFIN1 * P1 + s.RFIN2 * PAVG2[K - 1] - p.RFIN * PAVG3[K - 1];
-    if (abs(p.IN_PRESS) == 4)
-      P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(p.IMID, 2, p.KMID)] - p.RFIN * d.P[IX(p.IMID, 3, p.KMID)];
-    const float plim = d.SC[SC_PLIM + K - 1];
-    if (P1 > plim) P1 = 0.9f * P1 + 0.1f * plim;
-    if (P1 > 0.99999f * PO1) P1 = PO1 * 0.99999f;
+    const float TO1 = d.TO1[K - 1];
+    float P1, TSTATIC;
+    inlet_pt(g, d, p, s, PAVG2, PAVG3, I, K, P1, TSTATIC);
     d.P[n1] = P1;
-    float TSTATIC = TO1 * powf(P1 * d.RPO1[K - 1], p.FGA);
     float VABSQ = 2 * p.CP * (TO1 - TSTATIC);
     d.T_STATIC[n1] = TSTATIC;
     float VABS = sqrtf(VABSQ);
@@ -419,6 +438,20 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
     d.ROVX[n1] = ro1 * vx1;
     d.ROE[n1] = P1 / p.GA1 + ro1 * 0.5f * VABSQ;
     d.HO[n1] = p.CP * TO1;
+    if (p.ILOS >= 200) {
+      // inlet turbulent viscosity, mf:4679-4690; the reference divides by
+      // RO(IMID,1,KMID) as its (K, I) loop has left it: already updated
+      // for (K,I) at or after (KMID,IMID), k_bc_sums kept both values
+      const float FST = d.FSTURB[0], VL = d.VISC_LAM[IX(I, 2, K)];
+      const float TRANSVISIN = (FST < 1.0f) ? VL * 4.35f * powf(FST, 0.25f) : VL * (3.5f + FST * 0.85f);
+      const bool after = K > p.KMID || (K == p.KMID && I >= p.IMID);
+      const float roref = after ? d.BCW[3 * g.KM + 1] : d.BCW[3 * g.KM];
+      d.TRANS_DYN_VIS[n1] = TRANSVISIN;
+      d.TRANS_DYN_VIS[IX(I, 2, K)] = TRANSVISIN;
+      const float tk = TRANSVISIN / roref;
+      d.TRANS_KVIS[n1] = tk;
+      d.TRANS_KVIS[IX(I, 2, K)] = tk;
+    }
     // exit, mf:4690-4730
     const size_t nm = IX(I, JM, K);
     float vr = d.VR[nm], vt = d.VT[nm], vx = d.VX[nm];
@@ -1318,6 +1351,767 @@ __global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float
     d.RFORCE[n] = d.RFORCE[n] + RF_VIS * (SR[I] - SR[I + 1]);
     d.TFORCE[n] = d.TFORCE[n] + RF_VIS * (ST[I] - ST[I + 1]) * RAVGK;
     d.QSOURCE[n] = d.QSOURCE[n] + RF_VIS * (SQ[I] - SQ[I + 1] + SW[I] - SW[I + 1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spalart-Allmaras model (ILOS >= 200): the per-step transport of the
+// turbulent viscosity (mf:4260-4351, TSTEP NCALL=2 mf:5736-5777) and
+// SPAL_LOSS (mf:13586-14827).  The turbulent viscosity is cell-centred.
+// Restated without the Shih wall functions, FAC_PGRAD and FAC_VORT.
+
+#define CELL_IJK \
+  NODE_IJK       \
+  if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
+
+// cell-average density, mf:4260-4270
+__global__ void k_sa_cellavg(Geo g, Dev d) {
+  CELL_IJK
+  const float *RO = d.RO;
+  d.ROAVG_CELL[IX(I, J, K)] = 0.125f * (RO[IX(I, J, K)] + RO[IX(I + 1, J, K)] + RO[IX(I + 1, J, K + 1)] +
+                                        RO[IX(I, J, K + 1)] + RO[IX(I, J - 1, K)] + RO[IX(I + 1, J - 1, K)] +
+                                        RO[IX(I + 1, J - 1, K + 1)] + RO[IX(I, J - 1, K + 1)]);
+}
+
+// doubled face fluxes and source of the turbulent viscosity, mf:4279-4330
+__global__ void k_sa_fluxes(Geo g, Dev d) {
+  NODE_IJK
+  const int IMM1 = g.IMM1, KMM1 = g.KMM1;
+  const float *TK = d.TRANS_KVIS;
+  const unsigned n = IX(I, J, K);
+  if (K == g.KM) {
+    if (J >= 2 && I <= IMM1) d.RFLUX[n] = 0.0f;
+    return;
+  }
+  if (I <= IMM1) {
+    int JP1 = J + 1;
+    if (J == g.JM) JP1 = g.JM;
+    if (J == 1) JP1 = 1;
+    if (d.INDMIX[J - 1] == 1) JP1 = J;
+    d.XFLUX[n] = d.FLOWX[n] * (TK[n] + TK[IX(I, JP1, K)]);
+  }
+  if (J < 2) return;
+  if (I <= IMM1) {
+    d.SOURCE[n] = -2.0f * d.T_SOURCE[n];
+    d.RFLUX[n] = (K == 1) ? 0.0f : d.FLOWR[n] * (TK[IX(I, J, K - 1)] + TK[n]);
+  }
+  if (I >= 2 && I <= IMM1) d.TFLUX[n] = d.FLOWT[n] * (TK[IX(I - 1, J, K)] + TK[n]);
+  if (I == 1 || I == g.IM) {
+    const unsigned n1 = IX(1, J, K);
+    d.TFLUX[n] = d.FLOWT[n1] * (TK[IX(IMM1, J, K)] + TK[n1]);
+  }
+  (void)KMM1;
+}
+
+// D += STORE on the cells, mf:5740-5744
+__global__ void k_sa_addstore(Geo g, Dev d, float *D) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  D[n] = D[n] + d.STORE[n];
+}
+
+// pitch average carried across each mixing plane, mf:5748-5762 (one
+// wavefront per (interface, K) line)
+__global__ void k_sa_mixtransfer(Geo g, Dev d, Params p, float *D) {
+  const int t = blockIdx.x;
+  const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
+  if (NR > g.NROWS - 1) return;
+  const int J = d.JMIX[NR - 1], lane = threadIdx.x;
+  const float FMULT = d.TURBVIS_DAMP[NR - 1];
+  float v[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    v[c] = (I <= g.IMM1) ? d.FP[I - 1] * D[IX(I, J, K)] : 0.0f;
+  }
+  const float AVG = line_sum(v, g.IMM1, p.EXACT);
+  for (int JAV = J + 1; JAV <= J + 2; JAV++)
+    for (int I = lane + 1; I <= g.IMM1; I += 64) D[IX(I, JAV, K)] = FMULT * AVG;
+}
+
+// periodic, tip-gap and mixing-plane conditions on the cell-centred
+// variable (ILAST = IM-1), mf:5858-5926.  One wavefront per (J,K) line,
+// the line held in LDS.
+__global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D) {
+  __shared__ float L[260];
+  const int K = blockIdx.x + 1, J = blockIdx.y + 1, lane = threadIdx.x;
+  const int IM = g.IM, IMM1 = g.IMM1, KM = g.KM;
+  const int NR = d.NROW[J - 1];
+  const bool per = !((d.IND[J - 1] == 1) || (d.INDLE[J - 1] == 1));
+  bool tip = false;
+  if (d.KTIPS[NR - 1] > 0) {
+    int K1 = d.KTIPS[NR - 1], K2 = d.KTIPE[NR - 1];
+    if (K1 == 1) K2 = K2 - 1;
+    if (K2 == KM) K1 = K1 + 1;
+    tip = K >= K1 && K <= K2;
+  }
+  const bool mp = p.IFMIX != 0 && ((J == d.JMIX[NR - 1] && NR < g.NROWS) || (NR > 1 && J == d.JSTART[NR - 1]));
+  if (!per && !tip && !mp) return;
+  for (int I = lane + 1; I <= IM; I += 64) L[I] = D[IX(I, J, K)];
+  __syncthreads();
+  if (lane == 0) {
+    if (per) {
+      L[1] = 0.5f * (L[1] + L[IMM1]);
+      L[IMM1] = L[1];
+    }
+    if (tip) {
+      L[1] = 0.5f * (L[1] + L[IMM1]);
+      L[IMM1] = L[1];
+    }
+  }
+  __syncthreads();
+  if (mp) {
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      v[c] = (I <= IMM1) ? d.FP[I - 1] * (L[I] + L[I + 1]) : 0.0f;
+    }
+    const float AVG = 0.5f * line_sum(v, IMM1, p.EXACT);
+    for (int I = lane + 1; I <= IM; I += 64) D[IX(I, J, K)] = AVG;
+  } else if (lane == 0) {
+    D[IX(1, J, K)] = L[1];
+    D[IX(IMM1, J, K)] = L[IMM1];
+  }
+}
+
+// lower limit and kinematic value of the turbulent viscosity, mf:4339-4348
+__global__ void k_sa_kvis(Geo g, Dev d, Params p) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  const float TRANSVISMIN = 0.1f * d.VISC_LAM[IX(p.IMID, 2, p.KMID)];
+  float v = d.TRANS_DYN_VIS[n];
+  if (v < TRANSVISMIN) v = TRANSVISMIN;
+  d.TRANS_DYN_VIS[n] = v;
+  d.TRANS_KVIS[n] = v / d.ROAVG_CELL[n];
+}
+
+// Gauss-theorem gradient of a corner-stored field over cell (I,J,K),
+// mf:13468-13510
+__device__ __forceinline__ void sa_gradvel(const Geo &g, const Dev &d, const float *V, int I, int J, int K,
+                                           float &DVDX, float &DVDR, float &DVDT) {
+  const int IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
+  const float RVOL = 0.25f / d.VOL[IX(I, J, K)];
+  const float VAVGJ1 = V[IX(I, JM1, K)] + V[IX(IP1, JM1, K)] + V[IX(IP1, JM1, KP1)] + V[IX(I, JM1, KP1)];
+  const float VAVGJ2 = V[IX(I, J, K)] + V[IX(IP1, J, K)] + V[IX(IP1, J, KP1)] + V[IX(I, J, KP1)];
+  const float VAVGI1 = V[IX(I, J, K)] + V[IX(I, JM1, K)] + V[IX(I, JM1, KP1)] + V[IX(I, J, KP1)];
+  const float VAVGI2 = V[IX(IP1, J, K)] + V[IX(IP1, JM1, K)] + V[IX(IP1, JM1, KP1)] + V[IX(IP1, J, KP1)];
+  const float VAVGK1 = V[IX(I, J, K)] + V[IX(IP1, J, K)] + V[IX(IP1, JM1, K)] + V[IX(I, JM1, K)];
+  const float VAVGK2 = V[IX(I, J, KP1)] + V[IX(IP1, J, KP1)] + V[IX(IP1, JM1, KP1)] + V[IX(I, JM1, KP1)];
+  const float VAVGCELL = 0.125f * (VAVGJ1 + VAVGJ2);
+  DVDX = -(VAVGJ2 * d.AQX[IX(I, J, K)] - VAVGJ1 * d.AQX[IX(I, JM1, K)] + VAVGI1 * d.ABX[IX(I, J, K)] -
+           VAVGI2 * d.ABX[IX(IP1, J, K)] + VAVGK1 * d.ASX[IX(I, J, K)] - VAVGK2 * d.ASX[IX(I, J, KP1)]) * RVOL;
+  DVDR = -(VAVGJ2 * d.AQR[IX(I, J, K)] - VAVGJ1 * d.AQR[IX(I, JM1, K)] + VAVGI1 * d.ABR[IX(I, J, K)] -
+           VAVGI2 * d.ABR[IX(IP1, J, K)] + VAVGK1 * d.ASR[IX(I, J, K)] - VAVGK2 * d.ASR[IX(I, J, KP1)]) * RVOL -
+         VAVGCELL / d.RAVG_CELL[JKX(J, K)];
+  DVDT = -(VAVGI1 - VAVGI2) * d.ABT[JKX(J, K)] * RVOL;
+}
+
+// Gradient of a cell-centred field, mf:13516-13580; cells beyond J = JM
+// read as zero (the reference's unused, zero J = JM+1 layer).
+__device__ __forceinline__ void sa_gradcell(const Geo &g, const Dev &d, const float *V, int I, int J, int K,
+                                            int p_IBOUND, float &DVDX, float &DVDR, float &DVDT) {
+  const int IMM1 = g.IMM1, KMM1 = g.KMM1;
+  int IP1 = I + 1, JP1 = J + 1, KP1 = K + 1, IM1 = I - 1, JM1 = J - 1, KM1 = K - 1;
+  if (I == 1) IM1 = IMM1;
+  if (I == IMM1) IP1 = 1;
+  if (K == 1) KM1 = 1;
+  if (K == KMM1) KP1 = KMM1;
+  const float RVOL = 0.5f / d.VOL[IX(I, J, K)];
+  const float v0 = V[IX(I, J, K)];
+  const float vjp = (JP1 <= g.JM) ? V[IX(I, JP1, K)] : 0.0f;
+  float VAVGJ1 = v0 + V[IX(I, JM1, K)];
+  float VAVGJ2 = v0 + vjp;
+  float VAVGI1 = v0 + V[IX(IM1, J, K)];
+  float VAVGI2 = v0 + V[IX(IP1, J, K)];
+  float VAVGK1 = v0 + V[IX(I, J, KM1)];
+  float VAVGK2 = v0 + V[IX(I, J, KP1)];
+  if (d.IND[J - 1] == 1) {
+    if (I == 1) VAVGI1 = 0.0f;
+    if (I == IMM1) VAVGI2 = 0.0f;
+  }
+  if (K == 1 && (p_IBOUND == 0 || p_IBOUND == 2)) VAVGK1 = 0.0f;
+  if (K == KMM1 && p_IBOUND <= 1) VAVGK2 = 0.0f;
+  DVDX = -(VAVGJ2 * d.AQX[IX(I, J, K)] - VAVGJ1 * d.AQX[IX(I, JM1, K)] + VAVGI1 * d.ABX[IX(I, J, K)] -
+           VAVGI2 * d.ABX[IX(IP1, J, K)] + VAVGK1 * d.ASX[IX(I, J, K)] - VAVGK2 * d.ASX[IX(I, J, KP1)]) * RVOL;
+  DVDR = -(VAVGJ2 * d.AQR[IX(I, J, K)] - VAVGJ1 * d.AQR[IX(I, JM1, K)] + VAVGI1 * d.ABR[IX(I, J, K)] -
+           VAVGI2 * d.ABR[IX(IP1, J, K)] + VAVGK1 * d.ASR[IX(I, J, K)] - VAVGK2 * d.ASR[IX(I, J, KP1)]) * RVOL -
+         v0 / d.RAVG_CELL[JKX(J, K)];
+  DVDT = -(VAVGI1 - VAVGI2) * d.ABT[JKX(J, K)] * RVOL;
+}
+
+// velocity and temperature gradients, strain rates and vorticity of each
+// cell, mf:13666-13718 (the vorticity before the mixing-plane substitution)
+__global__ void k_sa_grad(Geo g, Dev d, Params p) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  const int IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
+  const float *WT = d.WT;
+  const float WTAVG = 0.125f * (WT[IX(I, J, K)] + WT[IX(IP1, J, K)] + WT[IX(IP1, J, KP1)] + WT[IX(I, J, KP1)] +
+                                WT[IX(I, JM1, K)] + WT[IX(IP1, JM1, K)] + WT[IX(IP1, JM1, KP1)] + WT[IX(I, JM1, KP1)]);
+  const float RAVG = d.RAVG_CELL[JKX(J, K)];
+  float DVXDX, DVXDR, DVXDT, DVRDX, DVRDR, DVRDT, DWTDX, DWTDR, DWTDT, DTDX, DTDR, DTDT;
+  sa_gradvel(g, d, d.VX, I, J, K, DVXDX, DVXDR, DVXDT);
+  sa_gradvel(g, d, d.VR, I, J, K, DVRDX, DVRDR, DVRDT);
+  sa_gradvel(g, d, d.WT, I, J, K, DWTDX, DWTDR, DWTDT);
+  sa_gradvel(g, d, d.T_STATIC, I, J, K, DTDX, DTDR, DTDT);
+  d.SXR[n] = DVXDR + DVRDX;
+  d.SXX[n] = DVXDX;
+  d.SXT[n] = DWTDX + DVXDT;
+  d.SRR[n] = DVRDR;
+  d.SRT[n] = DWTDR + DVRDT - WTAVG / RAVG;
+  d.STT[n] = DWTDT;
+  d.GQX[n] = DTDX;
+  d.GQR[n] = DTDR;
+  d.GQT[n] = DTDT;
+  const float VORTX = DVRDT - DWTDR - WTAVG / RAVG;
+  const float VORTR = DWTDX - DVXDT;
+  const float VORTT = DVXDR - DVRDX;
+  float VORT = sqrtf(VORTX * VORTX + VORTR * VORTR + VORTT * VORTT);
+  const int NRW = d.NROW[J - 1];
+  if (J == d.JSTART[NRW - 1]) VORT = 0.0f;
+  if (VORT > p.VORT_MAX) VORT = p.VORT_MAX;
+  d.ABS_VORT[n] = VORT;
+}
+
+// turbulent viscosity of each cell, mf:13719-13888 (mixing-plane cells
+// take the vorticity of the cell upstream; no two such J are adjacent)
+__global__ void k_sa_visc(Geo g, Dev d, Params p, float FMIXUP, int NSTEP) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  float VORT = d.ABS_VORT[n];
+  if (d.INDMIX[J - 1] == 1) {
+    VORT = d.ABS_VORT[IX(I, J - 1, K)];
+    d.ABS_VORT[n] = VORT;
+  }
+  const float ROAVG = d.ROAVG_CELL[n];
+  const float TURBVIS_WALL = ROAVG * d.XLENGTH[n] * VORT * FMIXUP;
+  float VISLAM = d.SC[SC_VISLAM];
+  if (p.REYNO < 0.0001f) {
+    const int IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
+    const float *T = d.T_STATIC;
+    const float TSAVG = 0.125f * (T[IX(I, J, K)] + T[IX(IP1, J, K)] + T[IX(IP1, J, KP1)] + T[IX(I, J, KP1)] +
+                                  T[IX(I, JM1, K)] + T[IX(IP1, JM1, K)] + T[IX(IP1, JM1, KP1)] + T[IX(I, JM1, KP1)]);
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * powf(TSAVG / 288.0f, 0.62f);
+  }
+  float VISTOT = VISLAM * (1 + d.FST_RAT[n]) + TURBVIS_WALL;
+  float VISLIM = p.TURBVIS_LIM * VISLAM;
+  if (VISTOT > VISLIM) VISTOT = VISLIM;
+  const float VISKLAM = VISLAM / ROAVG;
+  d.DYN_VIS_ML[n] = (VISTOT - VISLAM);
+  if (p.IF_RESTART == 0 && NSTEP == 1) {
+    const float FST = d.FSTURB[0];
+    const float TRANSVISIN = (FST < 1.0f) ? 4.35f * VISLAM * powf(FST, 0.25f) : VISLAM * (3.5f + FST * 0.85f);
+    d.TRANS_DYN_VIS[n] = TRANSVISIN;
+  }
+  if (NSTEP == 1) d.TRANS_KVIS[n] = d.TRANS_DYN_VIS[n] / ROAVG;
+  float TRANSKVIS = d.TRANS_KVIS[n];
+  float XI = TRANSKVIS / VISKLAM;
+  if (XI < 1.1f) XI = 1.1f;
+  if (XI > p.TURBVIS_LIM) XI = p.TURBVIS_LIM;
+  const float XI3 = XI * XI * XI;
+  const float FV1 = XI3 / (XI3 + 357.9f);
+  const float TRANSVISLIM = VISLIM / ROAVG / FV1;
+  if (TRANSKVIS > TRANSVISLIM) TRANSKVIS = TRANSVISLIM;
+  const float VISTRU = TRANSKVIS * FV1 * ROAVG;
+  VISTOT = VISTRU + VISLAM;
+  VISLIM = p.TURBVIS_LIM * VISLAM;
+  if (VISTOT > VISLIM) VISTOT = VISLIM;
+  d.VISC_RAT[n] = VISTOT / VISLAM;
+  d.VISC_LAM[n] = VISLAM;
+  d.VISC_TURB[n] = VISTOT - VISLAM;
+}
+
+// gradient of the kinematic turbulent viscosity, mf:13893-13911 (J <= JM-1;
+// the J = JM layer stays zero as in the reference)
+__global__ void k_sa_gradtvis(Geo g, Dev d, Params p) {
+  CELL_IJK
+  if (J > g.JMM1) return;
+  const unsigned n = IX(I, J, K);
+  float DX_, DR_, DT_;
+  sa_gradcell(g, d, d.TRANS_KVIS, I, J, K, p.IBOUND, DX_, DR_, DT_);
+  const int NRW = d.NROW[J - 1];
+  if (J == d.JSTART[NRW - 1] || J == d.JSTART[NRW - 1] - 1) {
+    d.GRAD_TVIS[n] = 0.0f;
+    d.TVGX[n] = 0.0f;
+    d.TVGR[n] = 0.0f;
+    d.TVGT[n] = 0.0f;
+    return;
+  }
+  d.GRAD_TVIS[n] = DX_ * DX_ + DR_ * DR_ + DT_ * DT_;
+  const float VISCEFF = d.TRANS_KVIS[n] + d.VISC_LAM[n];
+  d.TVGX[n] = VISCEFF * DX_;
+  d.TVGR[n] = VISCEFF * DR_;
+  d.TVGT[n] = VISCEFF * DT_;
+}
+
+// Transition tests, mf:14004-14111: per (I,J) column (hub, casing) and per
+// (J,K) line (I = 1 and I = IM surfaces); bit 0/1 = damp the source there.
+__global__ void k_sa_trcol(Geo g, Dev d, Params p) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y + 2;
+  if (I > g.IMM1 || J > g.JM) return;
+  const int KEND1 = p.KMID / 2, KENDM = g.KMM1 - KEND1;
+  const int NRW = d.NROW[J - 1], JROW = J - d.JSTART[NRW - 1] + 1;
+  const int JTRHUB = d.JTRAN_K1[NRW - 1], JTRTIP = d.JTRAN_KM[NRW - 1];
+  int flags = 0;
+  if (!(JTRHUB <= 1 && p.FTRANS < 0.1f)) {
+    bool damp = JROW < JTRHUB;
+    if (!damp) {
+      float RMAX = 0.0f;
+      for (int K = 1; K <= KEND1; K++) {
+        const float r = d.VISC_RAT[IX(I, J, K)];
+        if (!(r < RMAX)) RMAX = r;
+      }
+      damp = !(RMAX > p.FTRANS);
+    }
+    if (damp) flags |= 1;
+  }
+  if (!(JTRTIP <= 1 && p.FTRANS < 0.1f)) {
+    bool damp = JROW < JTRTIP;
+    if (!damp) {
+      float RMAX = 0.0f;
+      for (int K = KENDM; K <= g.KMM1; K++) {
+        const float r = d.VISC_RAT[IX(I, J, K)];
+        if (!(r < RMAX)) RMAX = r;
+      }
+      damp = !(RMAX > p.FTRANS);
+    }
+    if (damp) flags |= 2;
+  }
+  d.SAFLAG[(I - 1) + g.IM * (J - 1)] = flags;
+}
+
+__global__ void k_sa_trline(Geo g, Dev d, Params p) {
+  const int K = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y + 2;
+  if (K > g.KMM1 || J > g.JM) return;
+  const int IEND1 = p.IMID / 2, IENDM = g.IMM1 - IEND1;
+  const int NRW = d.NROW[J - 1], JROW = J - d.JSTART[NRW - 1] + 1, JLEROW = d.JLE[NRW - 1];
+  const int JTRLOW = d.JTRAN_I1[NRW - 1], JTRUP = d.JTRAN_IM[NRW - 1];
+  int flags = 0;
+  if (!(JTRLOW <= 1 && p.FTRANS < 0.1f) && !(J <= JLEROW)) {
+    bool damp = JROW < JTRLOW || J < JLEROW;
+    if (!damp) {
+      float RMAX = 0.0f;
+      for (int I = 1; I <= IEND1; I++) {
+        const float r = d.VISC_RAT[IX(I, J, K)];
+        if (!(r < RMAX)) RMAX = r;
+      }
+      damp = !(RMAX > p.FTRANS);
+    }
+    if (damp) flags |= 1;
+  }
+  if (!(JTRUP <= 1 && p.FTRANS < 0.1f) && !(J <= JLEROW)) {
+    bool damp = JROW < JTRUP;
+    if (!damp) {
+      float RMAX = 0.0f;
+      for (int I = IENDM; I <= g.IMM1; I++) {
+        const float r = d.VISC_RAT[IX(I, J, K)];
+        if (!(r < RMAX)) RMAX = r;
+      }
+      damp = !(RMAX > p.FTRANS);
+    }
+    if (damp) flags |= 2;
+  }
+  d.SAFLAG[g.IM * g.JM + JKX(J, K)] = flags;
+}
+
+// source of the turbulent viscosity, relaxed into T_SOURCE, then damped
+// towards laminar where the tests say so, mf:13916-13997 and 14004-14116
+__global__ void k_sa_source(Geo g, Dev d, Params p, float RF_VIS) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  const float CW2 = 0.3f, SIGMA = 0.6667f, CB2 = 0.622f, CB1 = 0.1355f, CW1 = 0.80607f + 2.433f;
+  const float FST0 = p.FAC_ST0 * CB1, FST1 = p.FAC_ST1 / SIGMA, FST2 = p.FAC_ST2 * CB2 / SIGMA;
+  const float FST3 = p.FAC_ST3 * CW1, FSTMIX = p.FAC_STMIX / 100000.0f;
+  const float RF_VIS1 = 1.0f - RF_VIS;
+  const float yp = d.Y_PLUS[n];
+  float FYPLUS = 0.0f;
+  if (yp > p.YPLAM) {
+    float XFAC = (yp - p.YPLAM) / (p.YPTURB - p.YPLAM);
+    if (XFAC > 1.0f) XFAC = 1.0f;
+    FYPLUS = XFAC * XFAC * (3.0f - 2.0f * XFAC);
+  }
+  float TVISDDX, TVISDDR, TVISDDT, d1, d2;
+  sa_gradcell(g, d, d.TVGX, I, J, K, p.IBOUND, TVISDDX, d1, d2);
+  sa_gradcell(g, d, d.TVGR, I, J, K, p.IBOUND, d1, TVISDDR, d2);
+  sa_gradcell(g, d, d.TVGT, I, J, K, p.IBOUND, d1, d2, TVISDDT);
+  const float ROAVG = d.ROAVG_CELL[n];
+  const float TVIS = d.TRANS_KVIS[n];
+  const float TV = d.ABS_VORT[n];
+  float ST0 = FST0 * TV * TVIS;
+  ST0 = ST0 * FYPLUS;
+  const float ST1 = FST1 * (TVISDDX + TVISDDR + TVISDDT);
+  const float ST2 = FST2 * d.GRAD_TVIS[n];
+  const float DW = d.DWALLSQ[n];
+  float TAU = 5.9488f * TVIS / TV / DW;
+  if (TAU > 1.2f) TAU = 1.2f;
+  if (TAU < 0.001f) TAU = 0.001f;
+  const float G = TAU + CW2 * (TAU * TAU * TAU * TAU * TAU * TAU - TAU);
+  const float G6CW6 = G * G * G * G * G * G / 64.0f;
+  const float FW = G * powf((1 + 0.015625f) / (1 + G6CW6), 0.166667f);
+  const float ST3 = FST3 * FW * TVIS * TVIS / DW;
+  const float ST_MIXL = FSTMIX * (d.DYN_VIS_ML[n] - d.VISC_TURB[n]) / d.STEP[n];
+  float ts = RF_VIS1 * d.T_SOURCE[n] + RF_VIS * (ROAVG * (ST0 + ST1 + ST2 - ST3) * d.VOL[n] + ST_MIXL);
+  if (J > 2 && d.INDMIX[J - 2] == 1) ts = 0.0f;
+  if (J > 2 && d.INDMIX[J - 3] == 1) ts = 0.0f;
+  // transition: hub and casing first, then the blade surfaces (mf:14004-14116)
+  const int KEND1 = p.KMID / 2, KENDM = g.KMM1 - KEND1, IEND1 = p.IMID / 2, IENDM = g.IMM1 - IEND1;
+  const float VT = d.VISC_TURB[n], ST = d.STEP[n];
+  const int fc = d.SAFLAG[(I - 1) + g.IM * (J - 1)], fl = d.SAFLAG[g.IM * g.JM + JKX(J, K)];
+  if ((fc & 1) && K <= KEND1) {
+    const float TVEDGE = d.VISC_TURB[IX(I, J, KEND1)];
+    float FK = (float)(KEND1 + 1 - K) / (float)KEND1;
+    FK = FK * FK * (3 - 2 * FK);
+    const float TVSPEC = FK * TVEDGE;
+    ts = ts - (VT - TVSPEC) / ST / 100000.0f;
+  }
+  if ((fc & 2) && K >= KENDM) {
+    const float TVEDGE = d.VISC_TURB[IX(I, J, KENDM)];
+    float FK = (float)(K + 1 - KENDM) / (float)(g.KM - KENDM);
+    FK = FK * FK * (3 - 2 * FK);
+    const float TVSPEC = FK * TVEDGE;
+    ts = ts - (VT - TVSPEC) / ST / 100000.0f;
+  }
+  if ((fl & 1) && I <= IEND1) {
+    const float TVEDGE = d.VISC_TURB[IX(IEND1, J, K)];
+    float FI = (float)(IEND1 + 1 - I) / (float)IEND1;
+    FI = FI * FI * (3 - 2 * FI);
+    const float TVSPEC = FI * TVEDGE;
+    ts = ts - (VT - TVSPEC) / ST / 100000.0f;
+  }
+  if ((fl & 2) && I >= IENDM) {
+    const float TVEDGE = d.VISC_TURB[IX(IENDM, J, K)];
+    float FI = (float)(I + 1 - IENDM) / (float)(g.IM - IENDM);
+    FI = FI * FI * (3 - 2 * FI);
+    const float TVSPEC = FI * TVEDGE;
+    ts = ts - (VT - TVSPEC) / ST / 100000.0f;
+  }
+  d.T_SOURCE[n] = ts;
+}
+
+// wall friction of SPAL_LOSS's wall cells (sums of four node values: the
+// 1/16 and 1/128 factors), mf:14221-14268
+__device__ float sa_cf(float RE, float DPERP, float ROUGH, float ROAVG4, float WAVG, float VISLAM) {
+  const float RELOG = 1.f / logf(RE);
+  float CF;
+  if (ROUGH > 1.0e-7f) {
+    if (ROUGH > DPERP) ROUGH = DPERP;
+    float REK = RE * ROUGH / DPERP;
+    REK = REK - 80.0f;
+    if (REK < 0.0f) REK = 0.0f;
+    const float REKSQ = REK * REK;
+    const float A1 = -.00178493f + .0000814923f * REK + .000000150445f * REKSQ;
+    const float A2 = .029072f - .001584f * REK - .00000225194f * REKSQ;
+    const float A3 = .270313f + .0091409f * REK + .00000451537f * REKSQ + .00000000464767f * REKSQ * REK;
+    CF = A1 + A2 * RELOG + A3 * RELOG * RELOG;
+  } else {
+    CF = -0.00178493f + 0.029072f * RELOG + 0.270313f * RELOG * RELOG;
+  }
+  const float CFLAM = 2.0f / RE;
+  CF = fmaxf(CF, CFLAM);
+  if (RE < 125.f) CF = CFLAM;
+  if (ROUGH > 1.0e-7f) {
+    const float TW = CF * ROAVG4 * WAVG * WAVG / 128.f;
+    const float VSTAR = sqrtf(TW / (0.25f * ROAVG4));
+    const float PLUSK = ROUGH * VSTAR * ROAVG4 / VISLAM / 4.f;
+    const float RELOG10 = log10f(DPERP / ROUGH);
+    const float FUNCN = 5.75f * RELOG10 + 8.5f;
+    const float CF_FULL_ROUGH = 2.0f / (FUNCN * FUNCN);
+    if (PLUSK > 45.f && CF > CF_FULL_ROUGH) CF = CF_FULL_ROUGH;
+  }
+  return CF;
+}
+
+// the nine stresses and heat fluxes of a cell with its final viscosity,
+// mf:14123-14147
+struct SaStress {
+  float xr, xx, xt, rr, rt, tt, qx, qr, qt;
+};
+
+__device__ __forceinline__ SaStress sa_stress(const Geo &g, const Dev &d, const Params &p, unsigned n) {
+  const float VISLAM = d.VISC_LAM[n];
+  const float VISTOT = d.VISC_RAT[n] * VISLAM;
+  const float THCOND = p.CP * VISTOT / p.PRANDTL;
+  const float VISTOT2 = 2.0f * VISTOT;
+  SaStress s;
+  s.xr = VISTOT * d.SXR[n];
+  s.xx = VISTOT2 * d.SXX[n];
+  s.xt = VISTOT * d.SXT[n];
+  s.rr = VISTOT2 * d.SRR[n];
+  s.rt = VISTOT * d.SRT[n];
+  s.tt = VISTOT2 * d.STT[n];
+  s.qx = -THCOND * d.GQX[n];
+  s.qr = -THCOND * d.GQR[n];
+  s.qt = -THCOND * d.GQT[n];
+  return s;
+}
+
+// Forces from the blade surfaces and the I = const faces, mf:14159-14500.
+// One wavefront per (J,K) line.  d.SAJMAP gives, per J, the J-1 and the row
+// the reference's periodic-force block reads (its loop variables keep the
+// values of the last blade J visited: [0..JM) and [JM..2JM) the J-1 for
+// K = 1 and K > 1, [2JM..4JM) the row likewise).
+constexpr int kSaLine = 260;
+
+__global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
+  __shared__ float TXR[kSaLine], TXT[kSaLine], TXX[kSaLine], TRR[kSaLine], TRT[kSaLine], TTT[kSaLine];
+  __shared__ float QXX[kSaLine], QRR[kSaLine], QTT[kSaLine];
+  __shared__ float XF[kSaLine], RF[kSaLine], TF[kSaLine], QF[kSaLine], WF[kSaLine];
+  __shared__ float F0[4], F1[4], VS[2];
+  const int K = blockIdx.x + 1, J = blockIdx.y + 2, lane = threadIdx.x;
+  const int IM = g.IM, IMM1 = g.IMM1, IMID = p.IMID, KP1 = K + 1, JM = g.JM;
+  const float VISLAM = d.VISC_LAM[IX(IMM1, JM, g.KMM1)], VISLAM16 = VISLAM * 16.0f;
+  const float ABTJK = d.ABT[JKX(J, K)];
+  for (int I = lane + 1; I <= IMM1; I += 64) {
+    const SaStress s = sa_stress(g, d, p, IX(I, J, K));
+    TXR[I] = s.xr; TXX[I] = s.xx; TXT[I] = s.xt; TRR[I] = s.rr; TRT[I] = s.rt; TTT[I] = s.tt;
+    QXX[I] = s.qx; QRR[I] = s.qr; QTT[I] = s.qt;
+  }
+  if (lane < 4) { F0[lane] = 0.0f; F1[lane] = 0.0f; }
+  __syncthreads();
+  // blade-surface friction and y+ (mf:14159-14424)
+  const bool blade = d.IND[J - 1] != 0;
+  const int NRWJ = d.NROW[J - 1];
+  const bool tipcell = (d.KTIPS[NRWJ - 1] > 0) && (K >= d.KTIPS[NRWJ - 1]) && (K < d.KTIPE[NRWJ - 1]);
+  if (blade && !tipcell) {
+    if (lane == 0) {
+      const int JM1 = J - 1;
+      const float *VX = d.VX, *VR = d.VR, *WT = d.WT, *RO = d.RO;
+      float VXAVG1 = VX[IX(2, J, K)] + VX[IX(2, JM1, K)] + VX[IX(2, JM1, KP1)] + VX[IX(2, J, KP1)];
+      float VRAVG1 = VR[IX(2, J, K)] + VR[IX(2, JM1, K)] + VR[IX(2, JM1, KP1)] + VR[IX(2, J, KP1)];
+      float WTAVG1 = WT[IX(2, J, K)] + WT[IX(2, JM1, K)] + WT[IX(2, JM1, KP1)] + WT[IX(2, J, KP1)];
+      float ROAVG1 = RO[IX(2, J, K)] + RO[IX(2, JM1, K)] + RO[IX(2, JM1, KP1)] + RO[IX(2, J, KP1)];
+      float WAVG = sqrtf(WTAVG1 * WTAVG1 + VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1);
+      float bx = d.ABX[IX(1, J, K)], br = d.ABR[IX(1, J, K)];
+      const float AREA1 = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
+      float DPERP = d.VOL[IX(1, J, K)] / AREA1;
+      float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+      float CF = sa_cf(RE, DPERP, d.ROUGH_L[NRWJ - 1], ROAVG1, WAVG, VISLAM);
+      const float TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      VS[0] = sqrtf(TWALLI1 / (0.25f * ROAVG1));
+      F0[3] = ROAVG1;
+      float FMULT = -TWALLI1 * AREA1 / WAVG;
+      float TFORCE1 = FMULT * WTAVG1;
+      F0[0] = 0.0f + FMULT * VXAVG1;
+      F0[1] = 0.0f + FMULT * VRAVG1;
+      F0[2] = 0.0f + TFORCE1;
+      const float WVISC1 = TFORCE1 * d.WRAD[J - 1] * d.RAVG_CELL[JKX(J, K)];
+      float VXAVGIM = VX[IX(IMM1, J, K)] + VX[IX(IMM1, JM1, K)] + VX[IX(IMM1, JM1, KP1)] + VX[IX(IMM1, J, KP1)];
+      float VRAVGIM = VR[IX(IMM1, J, K)] + VR[IX(IMM1, JM1, K)] + VR[IX(IMM1, JM1, KP1)] + VR[IX(IMM1, J, KP1)];
+      float WTAVGIM = WT[IX(IMM1, J, K)] + WT[IX(IMM1, JM1, K)] + WT[IX(IMM1, JM1, KP1)] + WT[IX(IMM1, J, KP1)];
+      float ROAVGIM = RO[IX(IMM1, J, K)] + RO[IX(IMM1, JM1, K)] + RO[IX(IMM1, JM1, KP1)] + RO[IX(IMM1, J, KP1)];
+      WAVG = sqrtf(WTAVGIM * WTAVGIM + VXAVGIM * VXAVGIM + VRAVGIM * VRAVGIM);
+      bx = d.ABX[IX(IM, J, K)];
+      br = d.ABR[IX(IM, J, K)];
+      const float AREAIM = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
+      DPERP = d.VOL[IX(IMM1, J, K)] / AREAIM;
+      RE = DPERP * ROAVGIM * WAVG / VISLAM16;
+      CF = sa_cf(RE, DPERP, d.ROUGH_U[NRWJ - 1], ROAVGIM, WAVG, VISLAM);
+      const float TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+      VS[1] = sqrtf(TWALLIM / (0.25f * ROAVGIM));
+      F1[3] = ROAVGIM;
+      FMULT = -TWALLIM * AREAIM / WAVG;
+      const float TFORCEIM = FMULT * WTAVGIM;
+      F1[0] = FMULT * VXAVGIM;
+      F1[1] = FMULT * VRAVGIM;
+      F1[2] = TFORCEIM;
+      const float WVISCIM = TFORCEIM * d.WRAD[J - 1] * d.RAVG_CELL[JKX(J, K)];
+      // energy accumulators of the wall cells in QF/WF slots 0 and IM (not faces)
+      XF[0] = WVISC1;
+      XF[IM] = WVISCIM;
+    }
+    __syncthreads();
+    for (int I = lane + 1; I <= IM; I += 64) {
+      const unsigned n = IX(I, J, K);
+      const float YPLS = (I <= IMID) ? VS[0] * sqrtf(d.DWALLSQ[n]) * (0.25f * F0[3]) / VISLAM
+                                     : VS[1] * sqrtf(d.DWALLSQ[n]) * (0.25f * F1[3]) / VISLAM;
+      d.Y_PLUS[n] = fminf(1000.0f, YPLS);
+    }
+  }
+  // running sums of cells 1 and IM-1 before the I-face loop
+  float fx1 = 0.f, fr1 = 0.f, ft1 = 0.f, es1 = 0.f, fxm = 0.f, frm = 0.f, ftm = 0.f, esm = 0.f;
+  if (blade && !tipcell) {
+    fx1 = F0[0]; fr1 = F0[1]; ft1 = F0[2]; es1 = 0.0f + XF[0];
+    fxm = 0.0f + F1[0]; frm = 0.0f + F1[1]; ftm = 0.0f + F1[2]; esm = 0.0f + XF[IM];
+  }
+  __syncthreads();
+  // periodic cells I = 1 and IM-1, mf:14434-14486 (the reference's
+  // J-1 and row here are those of the last blade J it visited)
+  const int JM1E = (K == 1) ? d.SAJMAP[J - 1] : d.SAJMAP[JM + J - 1];
+  const int NRWE = (K == 1) ? d.SAJMAP[2 * JM + J - 1] : d.SAJMAP[3 * JM + J - 1];
+  const bool skipper = ((d.IND[J - 1] == 1) && (K < d.KTIPS[NRWE - 1])) || (K > d.KTIPE[NRWE - 1]);
+  if (!skipper) {
+    const int I = 1, IM1 = IMM1;
+    const float abr = d.ABR[IX(I, J, K)], abx = d.ABX[IX(I, J, K)];
+    const float XFOR = 0.5f * ((TXR[I] + TXR[IM1]) * abr + (TXT[I] + TXT[IM1]) * ABTJK + (TXX[I] + TXX[IM1]) * abx);
+    fxm = fxm + XFOR;
+    fx1 = fx1 - XFOR;
+    float AVG = 0.5f * (fx1 + fxm);
+    fx1 = AVG; fxm = AVG;
+    const float RFOR = 0.5f * ((TRR[I] + TRR[IM1]) * abr + (TRT[I] + TRT[IM1]) * ABTJK + (TXR[I] + TXR[IM1]) * abx);
+    frm = frm + RFOR;
+    fr1 = fr1 - RFOR;
+    AVG = 0.5f * (fr1 + frm);
+    fr1 = AVG; frm = AVG;
+    const float TFOR = 0.5f * ((TRT[I] + TRT[IM1]) * abr + (TTT[I] + TTT[IM1]) * ABTJK + (TXT[I] + TXT[IM1]) * abx);
+    ftm = ftm + TFOR;
+    ft1 = ft1 - TFOR;
+    AVG = 0.5f * (ft1 + ftm);
+    ft1 = AVG; ftm = AVG;
+    const float QFLOW = 0.5f * ((QXX[I] + QXX[IM1]) * abx + (QRR[I] + QRR[IM1]) * abr + (QTT[I] + QTT[IM1]) * ABTJK);
+    const float *VX = d.VX, *VR = d.VR, *VT = d.VT;
+    const float VXAVG = VX[IX(I, J, K)] + VX[IX(I, JM1E, K)] + VX[IX(I, JM1E, KP1)] + VX[IX(I, J, KP1)];
+    const float VRAVG = VR[IX(I, J, K)] + VR[IX(I, JM1E, K)] + VR[IX(I, JM1E, KP1)] + VR[IX(I, J, KP1)];
+    const float VTAVG = VT[IX(I, J, K)] + VT[IX(I, JM1E, K)] + VT[IX(I, JM1E, KP1)] + VT[IX(I, J, KP1)];
+    const float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+    es1 = es1 + QFLOW + WVISC;
+    esm = esm - QFLOW - WVISC;
+    AVG = 0.5f * (es1 + esm);
+    es1 = AVG; esm = AVG;
+  }
+  // the I = const faces, mf:14491-14498
+  for (int I = lane + 2; I <= IMM1; I += 64) {
+    const int IM1 = I - 1;
+    const unsigned n = IX(I, J, K);
+    const float abr = d.ABR[n], abx = d.ABX[n];
+    const float XFOR = 0.5f * ((TXR[I] + TXR[IM1]) * abr + (TXT[I] + TXT[IM1]) * ABTJK + (TXX[I] + TXX[IM1]) * abx);
+    const float RFOR = 0.5f * ((TRR[I] + TRR[IM1]) * abr + (TRT[I] + TRT[IM1]) * ABTJK + (TXR[I] + TXR[IM1]) * abx);
+    const float TFOR = 0.5f * ((TRT[I] + TRT[IM1]) * abr + (TTT[I] + TTT[IM1]) * ABTJK + (TXT[I] + TXT[IM1]) * abx);
+    const float QFLOW = 0.5f * ((QXX[I] + QXX[IM1]) * abx + (QRR[I] + QRR[IM1]) * abr + (QTT[I] + QTT[IM1]) * ABTJK);
+    const float *VX = d.VX, *VR = d.VR, *VT = d.VT;
+    const float VXAVG = VX[n] + VX[IX(I, JM1E, K)] + VX[IX(I, JM1E, KP1)] + VX[IX(I, J, KP1)];
+    const float VRAVG = VR[n] + VR[IX(I, JM1E, K)] + VR[IX(I, JM1E, KP1)] + VR[IX(I, J, KP1)];
+    const float VTAVG = VT[n] + VT[IX(I, JM1E, K)] + VT[IX(I, JM1E, KP1)] + VT[IX(I, J, KP1)];
+    XF[I] = XFOR;
+    RF[I] = RFOR;
+    TF[I] = TFOR;
+    QF[I] = QFLOW;
+    WF[I] = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+  }
+  __syncthreads();
+  // each cell: its start value, minus its own face, plus the next face (the
+  // reference's loop adds in this order)
+  for (int c = lane + 1; c <= IMM1; c += 64) {
+    float fx = 0.f, fr = 0.f, ft = 0.f, es = 0.f;
+    if (c == 1) { fx = fx1; fr = fr1; ft = ft1; es = es1; }
+    if (c == IMM1) { fx = fxm; fr = frm; ft = ftm; es = esm; }
+    if (c >= 2) {
+      fx = fx - XF[c];
+      fr = fr - RF[c];
+      ft = ft - TF[c];
+      es = es + QF[c] + WF[c];
+    }
+    if (c + 1 <= IMM1) {
+      fx = fx + XF[c + 1];
+      fr = fr + RF[c + 1];
+      ft = ft + TF[c + 1];
+      es = es - QF[c + 1] - WF[c + 1];
+    }
+    const unsigned n = IX(c, J, K);
+    d.FORCEX[n] = fx;
+    d.FORCER[n] = fr;
+    d.FORCET[n] = ft;
+    d.ESOURCE[n] = es;
+  }
+}
+
+// Hub and casing friction and the K = const faces, then the body forces,
+// mf:14505-14819.  One thread per (I,J) column, streaming up in K.
+__global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
+  if (I > g.IMM1 || J > g.JM) return;
+  const int IP1 = I + 1, JM1 = J - 1, KMM1 = g.KMM1, KM = g.KM, KMID = p.KMID;
+  const float VISLAM = d.VISC_LAM[IX(g.IMM1, g.JM, KMM1)], VISLAM16 = VISLAM * 16.0f;
+  const int NRWHC = d.SAJMAP[4 * g.JM];
+  const float *VX = d.VX, *VR = d.VR, *VT = d.VT, *RO = d.RO;
+  // hub, mf:14510-14650
+  float VXAVG1 = VX[IX(I, J, 2)] + VX[IX(IP1, J, 2)] + VX[IX(IP1, JM1, 2)] + VX[IX(I, JM1, 2)];
+  float VRAVG1 = VR[IX(I, J, 2)] + VR[IX(IP1, J, 2)] + VR[IX(IP1, JM1, 2)] + VR[IX(I, JM1, 2)];
+  float VTAVG1 = VT[IX(I, J, 2)] + VT[IX(IP1, J, 2)] + VT[IX(IP1, JM1, 2)] + VT[IX(I, JM1, 2)];
+  float ROAVG1 = RO[IX(I, J, 2)] + RO[IX(IP1, J, 2)] + RO[IX(IP1, JM1, 2)] + RO[IX(I, JM1, 2)];
+  float RAVG1 = d.R[JKX(J, 2)] + d.R[JKX(J, 2)] + d.R[JKX(JM1, 2)] + d.R[JKX(JM1, 2)];
+  float WTAVG1 = VTAVG1 - d.WHUB[J - 1] * RAVG1;
+  float WAVG = sqrtf(VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1 + WTAVG1 * WTAVG1);
+  float sx = d.ASX[IX(I, J, 1)], sr = d.ASR[IX(I, J, 1)];
+  const float AREA1 = sqrtf(sx * sx + sr * sr);
+  float DPERP = d.VOL[IX(I, J, 1)] / AREA1;
+  float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+  float CF = sa_cf(RE, DPERP, d.ROUGH_H[NRWHC - 1], ROAVG1, WAVG, VISLAM);
+  const float TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+  if (p.IBOUND == 0 || p.IBOUND == 2) {
+    for (int K = 1; K <= KMID; K++) {
+      const unsigned n = IX(I, J, K);
+      const float VSTAR = sqrtf(TWALLK1 / (0.25f * ROAVG1));
+      const float YPLS = VSTAR * sqrtf(d.DWALLSQ[n]) * (0.25f * ROAVG1) / VISLAM;
+      d.Y_PLUS[n] = fminf(YPLS, d.Y_PLUS[n]);
+    }
+  }
+  float FMULT = -TWALLK1 * AREA1 / WAVG;
+  if (p.IBOUND == 1 || p.IBOUND > 2) FMULT = 0.0f;
+  const float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
+  const float WVISC1 = TFORCE1 * 0.25f * d.WHUB[J - 1] * RAVG1;
+  // casing, mf:14652-14766
+  float VXAVGKM = VX[IX(I, J, KMM1)] + VX[IX(IP1, J, KMM1)] + VX[IX(IP1, JM1, KMM1)] + VX[IX(I, JM1, KMM1)];
+  float VRAVGKM = VR[IX(I, J, KMM1)] + VR[IX(IP1, J, KMM1)] + VR[IX(IP1, JM1, KMM1)] + VR[IX(I, JM1, KMM1)];
+  float VTAVGKM = VT[IX(I, J, KMM1)] + VT[IX(IP1, J, KMM1)] + VT[IX(IP1, JM1, KMM1)] + VT[IX(I, JM1, KMM1)];
+  float ROAVGKM = RO[IX(I, J, KMM1)] + RO[IX(IP1, J, KMM1)] + RO[IX(IP1, JM1, KMM1)] + RO[IX(I, JM1, KMM1)];
+  float RAVGKM = d.R[JKX(J, KMM1)] + d.R[JKX(J, KMM1)] + d.R[JKX(JM1, KMM1)] + d.R[JKX(JM1, KMM1)];
+  float WTAVGKM = VTAVGKM - d.WTIP[J - 1] * RAVGKM;
+  WAVG = sqrtf(VXAVGKM * VXAVGKM + VRAVGKM * VRAVGKM + WTAVGKM * WTAVGKM);
+  sx = d.ASX[IX(I, J, KM)];
+  sr = d.ASR[IX(I, J, KM)];
+  const float AREAKM = sqrtf(sx * sx + sr * sr);
+  DPERP = d.VOL[IX(I, J, KMM1)] / AREAKM;
+  RE = DPERP * ROAVGKM * WAVG / VISLAM16;
+  CF = sa_cf(RE, DPERP, d.ROUGH_T[NRWHC - 1], ROAVGKM, WAVG, VISLAM);
+  const float TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+  if (p.IBOUND == 0 || p.IBOUND == 1) {
+    for (int K = KMID + 1; K <= KMM1; K++) {
+      const unsigned n = IX(I, J, K);
+      const float VSTAR = sqrtf(TWALLKM / (0.25f * ROAVGKM));
+      const float YPLS = VSTAR * sqrtf(d.DWALLSQ[n]) * (0.25f * ROAVGKM) / VISLAM;
+      d.Y_PLUS[n] = fminf(YPLS, d.Y_PLUS[n]);
+    }
+  }
+  FMULT = -TWALLKM * AREAKM / WAVG;
+  if (p.IBOUND >= 2) FMULT = 0.0f;
+  const float XFORCEKM = FMULT * VXAVGKM, RFORCEKM = FMULT * VRAVGKM, TFORCEKM = FMULT * WTAVGKM;
+  const float WVISCKM = TFORCEKM * 0.25f * d.WTIP[J - 1] * RAVGKM;
+  // K faces and the final forces, mf:14770-14819
+  const float RF_VIS1 = 1.0f - RF_VIS;
+  SaStress sb = sa_stress(g, d, p, IX(I, J, 1));
+  float xfK = 0.f, rfK = 0.f, tfK = 0.f, qfK = 0.f, wfK = 0.f;  // face K (between K-1 and K)
+  for (int K = 1; K <= KMM1; K++) {
+    const unsigned n = IX(I, J, K);
+    float xfN = 0.f, rfN = 0.f, tfN = 0.f, qfN = 0.f, wfN = 0.f;  // face K+1
+    if (K + 1 <= KMM1) {
+      const unsigned m = IX(I, J, K + 1);
+      const SaStress st = sa_stress(g, d, p, m);
+      const float asr = d.ASR[m], asx = d.ASX[m];
+      xfN = 0.5f * ((st.xr + sb.xr) * asr + (st.xx + sb.xx) * asx);
+      rfN = 0.5f * ((st.rr + sb.rr) * asr + (st.xr + sb.xr) * asx);
+      tfN = 0.5f * ((st.rt + sb.rt) * asr + (st.xt + sb.xt) * asx);
+      qfN = 0.5f * ((st.qx + sb.qx) * asx + (st.qr + sb.qr) * asr);
+      const int KN = K + 1;
+      const float VXAVG = VX[IX(I, J, KN)] + VX[IX(IP1, J, KN)] + VX[IX(IP1, JM1, KN)] + VX[IX(I, JM1, KN)];
+      const float VRAVG = VR[IX(I, J, KN)] + VR[IX(IP1, J, KN)] + VR[IX(IP1, JM1, KN)] + VR[IX(I, JM1, KN)];
+      const float VTAVG = VT[IX(I, J, KN)] + VT[IX(IP1, J, KN)] + VT[IX(IP1, JM1, KN)] + VT[IX(I, JM1, KN)];
+      wfN = -0.25f * (xfN * VXAVG + rfN * VRAVG + tfN * VTAVG);
+      sb = st;
+    }
+    float fx = d.FORCEX[n], fr = d.FORCER[n], ft = d.FORCET[n], es = d.ESOURCE[n];
+    if (K == 1) { fx = fx + XFORCE1; fr = fr + RFORCE1; ft = ft + TFORCE1; es = es + WVISC1; }
+    if (K == KMM1) { fx = fx + XFORCEKM; fr = fr + RFORCEKM; ft = ft + TFORCEKM; es = es + WVISCKM; }
+    if (K >= 2) { fx = fx - xfK; fr = fr - rfK; ft = ft - tfK; es = es + qfK + wfK; }
+    if (K + 1 <= KMM1) { fx = fx + xfN; fr = fr + rfN; ft = ft + tfN; es = es - qfN - wfN; }
+    d.XFORCE[n] = RF_VIS1 * d.XFORCE[n] - RF_VIS * fx;
+    d.TFORCE[n] = RF_VIS1 * d.TFORCE[n] - RF_VIS * ft * d.RAVG_CELL[JKX(J, K)];
+    d.RFORCE[n] = RF_VIS1 * d.RFORCE[n] - RF_VIS * fr;
+    d.QSOURCE[n] = RF_VIS1 * d.QSOURCE[n] - RF_VIS * es;
+    xfK = xfN; rfK = rfN; tfK = tfN; qfK = qfN; wfK = wfN;
   }
 }
 

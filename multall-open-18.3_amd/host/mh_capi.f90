@@ -123,6 +123,14 @@ contains
     case ('YPLAM'); v = YPLAM
     case ('YPTURB'); v = YPTURB
     case ('CFWALL'); v = CFWALL
+    case ('FAC_STMIX'); v = FAC_STMIX
+    case ('FAC_ST0'); v = FAC_ST0
+    case ('FAC_ST1'); v = FAC_ST1
+    case ('FAC_ST2'); v = FAC_ST2
+    case ('FAC_ST3'); v = FAC_ST3
+    case ('FAC_SFVIS'); v = FAC_SFVIS
+    case ('FAC_VORT'); v = FAC_VORT
+    case ('FAC_PGRAD'); v = FAC_PGRAD
     case default
       v = 0; mhh_scalar = -1
     end select
@@ -198,6 +206,9 @@ contains
     case ('JTRAN_IM'); call puti(JTRAN_IM)
     case ('JTRAN_K1'); call puti(JTRAN_K1)
     case ('JTRAN_KM'); call puti(JTRAN_KM)
+    case ('FSTURB'); call putr1(FSTURB)
+    case ('TURBVIS_DAMP'); call putr1(TURBVIS_DAMP)
+    case ('ROAVG_CELL'); call putr3(ROAVG_CELL)
     case ('THETA'); call putr3(THETA)
     case ('RTHETA'); call putr3(RTHETA)
     case ('AQTOT'); call putr3(AQTOT)

@@ -172,6 +172,20 @@ contains
     call sets('REYNO', dble(REYNO)); call sets('RF_VIS', dble(RF_VIS)); call sets('FTRANS', dble(FTRANS))
     call sets('PRANDTL', dble(PRANDTL)); call sets('YPLUSWALL', dble(YPLUSWALL)); call sets('YPLAM', dble(YPLAM))
     call sets('YPTURB', dble(YPTURB)); call sets('CFWALL', dble(CFWALL)); call sets('IBOUND', dble(IBOUND))
+    ! Spalart-Allmaras inputs and the turbulence state SETUP starts from
+    ! (mf:6903-6909)
+    call put3('XLENGTH', XLENGTH); call put3('FST_RAT', FST_RAT); call put3('ROAVG_CELL', ROAVG_CELL)
+    call put1('FSTURB', FSTURB); call put1('TURBVIS_DAMP', TURBVIS_DAMP)
+    allocate (buf(IM, JM, KM))
+    buf = 25.0
+    call put3('Y_PLUS', buf)
+    buf = 1.0
+    call put3('VISC_RAT', buf)
+    deallocate (buf)
+    call sets('FAC_STMIX', dble(FAC_STMIX)); call sets('FAC_ST0', dble(FAC_ST0)); call sets('FAC_ST1', dble(FAC_ST1))
+    call sets('FAC_ST2', dble(FAC_ST2)); call sets('FAC_ST3', dble(FAC_ST3)); call sets('FAC_SFVIS', dble(FAC_SFVIS))
+    call sets('FAC_VORT', dble(FAC_VORT)); call sets('FAC_PGRAD', dble(FAC_PGRAD))
+    call sets('VORT_MAX', dble(VORT_MAX)); call sets('TURBVIS_LIM', dble(TURBVIS_LIM))
   end subroutine upload
 
   subroutine get3(name, a)

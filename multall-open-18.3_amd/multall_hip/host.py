@@ -18,14 +18,14 @@ HOST_LIB = os.path.join(PKG_ROOT, "lib", "libmultall_host.so")
 N3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "ROVT", "ROWT",
       "P", "HO", "PEFF", "VX", "VR", "VT", "WT", "T_STATIC", "AQX", "AQR", "ABX", "ABR", "ASX", "ASR",
       "VOL", "VOLOR", "STEP", "BSTEP", "RSTEP", "XFORCE", "RFORCE", "TFORCE", "QSOURCE", "FLOWX",
-      "FLOWT", "FLOWR", "DWALLSQ"]
+      "FLOWT", "FLOWR", "DWALLSQ", "XLENGTH", "FST_RAT", "ROAVG_CELL"]
 JK = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID"]
 IK = ["FTL", "FBL", "FTR", "FBR"]
 VI = ["FP", "FU", "FD"]
 VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD", "VM1"]
 VJ = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 RN = ["XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM", "XLLIM_DWN", "XLLIM_UP", "ROUGH_H", "ROUGH_T", "ROUGH_L",
-      "ROUGH_U"]
+      "ROUGH_U", "FSTURB", "TURBVIS_DAMP"]
 RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM"]
 MG = ["STEP1", "STEP2", "STEPSBK"]
 II = ["IB1", "IB2"]
@@ -39,7 +39,8 @@ SCALARS = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NIB1", "NJB1", "NKB1", "N
            "F3", "RSMTH", "CFL", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "SFTIN", "FAC_4TH",
            "FBLK1", "FBLK2", "FBLK3", "RFMIX", "FEXTRAP", "FSMTHB", "FANGLE", "RFIN", "SFEXIT", "REYNO",
            "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL", "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP",
-           "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS", "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL"]
+           "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS", "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0",
+           "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD"]
 
 _lib = None
 
@@ -79,7 +80,8 @@ def load_deck(path: str) -> Dict:
                  "SFXIN", "SFTIN", "FAC_4TH", "FBLK1", "FBLK2", "FBLK3", "RFMIX", "FEXTRAP", "FSMTHB",
                  "FANGLE", "RFIN", "SFEXIT", "REYNO", "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL",
                  "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP", "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS",
-                 "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL"):
+                 "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0", "FAC_ST1",
+                 "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD"):
             continue
         sc[n] = int(sc[n])
     im, jm, km = sc["IM"], sc["JM"], sc["KM"]
@@ -119,6 +121,10 @@ def load_deck(path: str) -> Dict:
         st[n] = arr(n, (sc["NROWS"],), np.float32)
     for n in RNI:
         st[n] = arr(n, (sc["NROWS"],), np.int32)
-    # the viscous-force state LOSS fills in (mh_loss_init) starts at zero
-    st["Y_PLUS"] = np.zeros((km, jm, im), np.float32)
+    # the turbulence state SETUP starts from (mf:6903-6909); the viscous
+    # models' first call (mh_loss_init) fills in the rest
+    st["Y_PLUS"] = np.full((km, jm, im), 25.0, np.float32)
+    st["VISC_RAT"] = np.ones((km, jm, im), np.float32)
+    for n in ("TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "VISC_LAM"):
+        st[n] = np.zeros((km, jm, im), np.float32)
     return st
