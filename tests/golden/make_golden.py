@@ -55,13 +55,35 @@ def load_state(path):
     return st
 
 
+def read_global_plt(path):
+    """The records EFICOOL writes to unit 11 at the end of a run
+    (mf:10748-10755): (1,1), then BLADE_FLOW, SUMPO, SUMTO, SUMRVT,
+    SUM_ENTPY per J, after what OUTPUT wrote there.  Fortran sequential
+    records with 4-byte markers."""
+    raw = open(path, "rb").read()
+    recs, pos = [], 0
+    while pos < len(raw):
+        n = int(np.frombuffer(raw, np.int32, 1, pos)[0])
+        recs.append(raw[pos + 4: pos + 4 + n])
+        pos += 8 + n
+    k = max(i for i, r in enumerate(recs) if len(r) == 8)   # EFICOOL's (1,1) record
+    return np.stack([np.frombuffer(r, np.float32) for r in recs[k + 1:k + 6]])
+
+
 def make(case):
+    import tempfile
+    import shutil
     import refdump, orc
     setup_ref = None
     for n in STEPS:
         spec = CASES[case](n)
         txt = deckgen.write_deck(spec)
-        setup, final, _ = refdump.run_reference(txt, "fp32")
+        wd = tempfile.mkdtemp(prefix="mhgold_")
+        try:
+            setup, final, _ = refdump.run_reference(txt, "fp32", workdir=wd)
+            sumflx = read_global_plt(os.path.join(wd, "global.plt"))
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
         st = orc.from_dump(setup)
         st["scalars"].pop("NSTEPS_MAX")
         st.pop("NOUT")
@@ -79,6 +101,7 @@ def make(case):
         names = OUT_VARS + (SA_VARS if st["scalars"]["ILOS"] >= 200 else [])
         out = {k: final.node(k).astype(np.float32) for k in names}
         out["NSTEP"] = np.array(int(final["NSTEP"]))
+        out["SUMFLX"] = sumflx          # SUMFLX's per-J integrals at the end of the run
         np.savez_compressed(os.path.join(HERE, f"{case}_ref{n}.npz"), **out)
         print(f"{case}: {n} steps done")
 
