@@ -6,6 +6,7 @@
 // launches per step in the reference's phase order.  The every-5-step
 // convergence figures come back to the host (one small copy) because the
 // next step's stop test needs them, exactly when the reference evaluates it.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -59,7 +60,14 @@ __global__ void k_loss_span(Geo, Dev, Params, float, float);
 __global__ void k_loss_pitch(Geo, Dev, Params, float, float);
 __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
-__global__ void k_sa_addstore(Geo, Dev, float *);
+__global__ void k_sa_addstore(Geo, Dev, float *, float, int);
+__global__ void k_resid_line(Geo, Dev, float *, Params);
+__global__ void k_mg_part(Geo, Dev, MgFast, int, int);
+__global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, const int *, const int *,
+                            float *, float *, float *);
+__global__ void k_mg_apply_lines(Geo, Dev, double *);
+__global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
+__global__ void k_damp_scatter(Geo, Dev, Params, const float *, float *, float, int, int);
 __global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
 __global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
@@ -367,53 +375,104 @@ int build_lists(mh_ctx *c) {
   if ((r = up(p1, &L.b1_ptr)) || (r = up(l1, &L.b1_list)) || (r = up(p2, &L.b2_ptr)) || (r = up(l2, &L.b2_list)) ||
       (r = up(ps, &L.sb_ptr)) || (r = up(ls, &L.sb_list)) || (r = up(pr, &L.row_ptr)) || (r = up(lr, &L.row_list)))
     return r;
+  // Fast-mode multigrid sums need every level-1 I (K) block of IR (KR) <= 4
+  // cells, inside one level-2 I (K) block.
+  std::vector<int> ib1(dm.im), ib2(dm.im), kb1(dm.km), kb2(dm.km);
+  if ((r = d2h(c, ib1.data(), c->d.IB1, dm.im * 4)) || (r = d2h(c, ib2.data(), c->d.IB2, dm.im * 4)) ||
+      (r = d2h(c, kb1.data(), c->d.KB1, dm.km * 4)) || (r = d2h(c, kb2.data(), c->d.KB2, dm.km * 4)))
+    return r;
+  auto nest = [](const std::vector<int> &a, const std::vector<int> &b, int ncell, int ir, int na, int nb,
+                 std::vector<int> &ptr, std::vector<int> &list) -> bool {
+    std::vector<int> m(na + 1, 0);
+    for (int x = 1; x <= ncell; x++) {
+      const int u = a[x - 1], v = b[x - 1];
+      if (u < 1 || u > na || v < 1 || v > nb || u != (x - 1) / ir + 1) return false;
+      if (m[u] && m[u] != v) return false;
+      m[u] = v;
+    }
+    ptr.assign(nb + 1, 0);
+    list.clear();
+    for (int v = 1; v <= nb; v++) {
+      ptr[v - 1] = (int)list.size();
+      for (int u = 1; u <= na; u++)
+        if (m[u] == v) list.push_back(u);
+    }
+    ptr[nb] = (int)list.size();
+    return true;
+  };
+  const Params pp = params_of(c);
+  std::vector<int> i2p, i2l, k2p, k2l;
+  L.mgf_ok = pp.IR >= 1 && pp.IR <= 4 && pp.KR >= 1 && pp.KR <= 4 &&
+             nest(ib1, ib2, dm.im - 1, pp.IR, dm.nib1, dm.nib2, i2p, i2l) &&
+             nest(kb1, kb2, dm.km - 1, pp.KR, dm.nkb1, dm.nkb2, k2p, k2l);
+  if (L.mgf_ok) {
+    MgFast &m = L.mgf;
+    int *t[4];
+    if ((r = up(i2p, &t[0])) || (r = up(i2l, &t[1])) || (r = up(k2p, &t[2])) || (r = up(k2l, &t[3])) ||
+        (r = dalloc(c, &m.P, (size_t)dm.nib1 * dm.jm * dm.nkb1)) || (r = dalloc(c, &m.PT, (size_t)dm.jm * dm.nkb1)))
+      return r;
+    m.i2_ptr = t[0], m.i2_list = t[1], m.k2_ptr = t[2], m.k2_list = t[3];
+  }
+  // the fused line kernels take rows as contiguous J ranges joined at JMIX
+  std::vector<int> nrow(dm.jm);
+  if ((r = d2h(c, nrow.data(), c->d.NROW, dm.jm * 4))) return r;
+  for (int nr = 1; nr < dm.nrows; nr++) {
+    const int J = jmix[nr - 1];
+    if (J + 2 > dm.jm || nrow[J - 1] != nr || nrow[J] != nr + 1 || nrow[J + 1] != nr + 1)
+      return fail(c, MH_E_UNSUPPORTED, "rows not contiguous at a mixing plane");
+  }
   return MH_OK;
 }
 
-// TSTEP (mf:5470-5932) for one conserved variable.
+// TSTEP (mf:5470-5932) for one conserved variable.  The launch chain per
+// call: the fused line kernel (flux extrapolation, residual, first
+// mixing-plane average), the multigrid sums, the multigrid update (with the
+// damping line sums in fast mode), the row averages and second
+// mixing-plane averages, the fused damping + corner distribution, and
+// SMOOTH_VAR with D's boundary conditions.
 int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   const Geo &g = c->g;
   Dev d = c->d;
   const Params p = params_of(c);
   const StepScalars s = step_scalars(c);
   const dim3 grid = node_grid(g);
+  const dim3 lgrid((g.JM - 1 + 3) / 4, g.KMM1), lblock(64, 4);
   const int nmp = (g.NROWS - 1) * g.KMM1;
-  if (!(p.FEXTRAP < 0.001f) && nmp > 0) {
-    LAUNCH(k_fextrap, dim3(nmp), dim3(64), 0, c->stream, g, d, p);
-    LCHK("fextrap");
-  }
-  LAUNCH(k_residual, grid, kNodeBlock, 0, c->stream, g, d, D, DIFF, p);
-  LCHK("residual");
-  if (nmp > 0) {
-    LAUNCH(k_mpavg, dim3(nmp), dim3(64), 0, c->stream, g, d, p, 1);
-    LCHK("mpavg1");
-  }
-  const bool mg = !(p.IR <= 1 && p.JR <= 1 && p.KR <= 1);
   const Lists &L = c->lists;
+  const bool damp = !(s.DAMP < 2.0f || s.DAMP > 100);
+  // fast mode without residual smoothing takes the damping line sums from
+  // the multigrid update
+  const bool fused_apply = !p.EXACT && p.NRSMTH == 0 && damp;
+  LAUNCH(k_resid_line, lgrid, lblock, 0, c->stream, g, d, DIFF, p);
+  LCHK("resid_line");
+  const bool mg = !(p.IR <= 1 && p.JR <= 1 && p.KR <= 1);
   if (mg) {
     const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
-    LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list,
-                       g.NIB1, g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
-    LCHK("mg1");
     if (p.EXACT) {
-      LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list,
-                         g.NIB2, g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
-      LCHK("mg2s");
-      LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr,
-                         L.sb_list, g.NSBLK, d.SBCHG, 0, 0);
-      LCHK("sbs");
+      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list, g.NIB1,
+             g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
+      LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
+             g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
+      LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr, L.sb_list,
+             g.NSBLK, d.SBCHG, 0, 0);
+    } else if (L.mgf_ok) {
+      const int nw = (g.JM - 1) * g.NKB1;
+      LAUNCH(k_mg_part, dim3((nw + 3) / 4), dim3(256), 0, c->stream, g, d, L.mgf, p.IR, p.KR);
+      LAUNCH(k_mg_levels, dim3((n1 + 255) / 256 + (n2 + 3) / 4 + g.NSBLK), dim3(256), 0, c->stream, g, L.mgf,
+             L.b1_ptr, L.b1_list, L.b2_ptr, L.b2_list, L.sb_ptr, L.sb_list, d.B1CHG, d.B2CHG, d.SBCHG);
     } else {
-      LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
-                         g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
-      LCHK("mg2w");
+      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list, g.NIB1,
+             g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
+      LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2, g.NJB2 + 1,
+             g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
       LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, c->stream, g, d, d.RED, 0);
-      LCHK("jplanes");
-      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr,
-                         L.sb_list, g.NSBLK, d.SBCHG, 0);
-      LCHK("sbw");
+      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr, L.sb_list,
+             g.NSBLK, d.SBCHG, 0);
     }
+    LCHK("mg_sums");
   }
-  LAUNCH(k_mg_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  if (fused_apply) LAUNCH(k_mg_apply_lines, lgrid, lblock, 0, c->stream, g, d, d.RED);
+  else LAUNCH(k_mg_apply, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("mgapply");
   if (p.NRSMTH > 0) {
     for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
@@ -425,24 +484,27 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
     }
     LCHK("smooth_resid");
   }
-  if (!(s.DAMP < 2.0f || s.DAMP > 100)) {
+  // row averages of |change| for the damping (mf:5677-5711)
+  if (fused_apply) {
+    LAUNCH(k_row_avg, dim3(g.NROWS), dim3(1024), 0, c->stream, g, d, (const double *)d.RED, L.row_ptr, L.row_list);
+    LCHK("row_avg");
+  } else if (damp) {
     if (p.EXACT) {
       LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, c->stream, g, d, L.row_ptr, L.row_list, g.NROWS,
-                         d.SC + SC_AVG_CHG, 1, 1);
+             d.SC + SC_AVG_CHG, 1, 1);
     } else {
       LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, c->stream, g, d, d.RED, 1);
-      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr,
-                         L.row_list, g.NROWS, d.SC + SC_AVG_CHG, 1);
+      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr, L.row_list,
+             g.NROWS, d.SC + SC_AVG_CHG, 1);
     }
-    LAUNCH(k_damp_apply, grid, kNodeBlock, 0, c->stream, g, d, s.DAMP);
-    LCHK("damp");
+    LCHK("damp_avg");
   }
   if (ncall == 2) {
     // the cell-centred turbulent viscosity (mf:5736-5777, then 8700):
-    // add the change, carry the pitch average over each mixing plane,
-    // smooth once with FAC_SFVIS*SFT, then the periodic/tip/mixing-plane
-    // conditions with ILAST = IM-1
-    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, c->stream, g, d, D);
+    // add the (damped) change, carry the pitch average over each mixing
+    // plane, smooth once with FAC_SFVIS*SFT, then the periodic/tip/
+    // mixing-plane conditions with ILAST = IM-1
+    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, c->stream, g, d, D, s.DAMP, damp ? 1 : 0);
     if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, c->stream, g, d, p, D);
     const float sm = p.FAC_SFVIS * s.SFT;
     LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.TMP, sm, 0);
@@ -452,14 +514,13 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
     LCHK("sa_tstep");
     return MH_OK;
   }
-  if (nmp > 0) {
-    LAUNCH(k_mpavg, dim3(nmp), dim3(64), 0, c->stream, g, d, p, 2);
-    LCHK("mpavg2");
-  }
-  LAUNCH(k_scatter, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.DSC);
-  LCHK("scatter");
-  LAUNCH(k_smooth_plane, dim3(g.JM), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
-                     s, ncall);
+  // damping, second mixing-plane averages (mf:5786-5818) and the corner
+  // distribution
+  LAUNCH(k_damp_scatter, dim3((g.JM + 3) / 4, (g.KM + 3) / 4), dim3(256),
+         (size_t)45 * 64 * ((g.IMM1 + 63) / 64) * sizeof(float), c->stream, g, d, p, (const float *)D,
+         d.DSC, s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+  LCHK("damp_scatter");
+  LAUNCH(k_smooth_plane, dim3(g.JM), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p, s, ncall);
   LCHK("smooth_plane");
   return MH_OK;
 }

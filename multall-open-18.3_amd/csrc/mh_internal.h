@@ -121,9 +121,19 @@ struct Params {
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid
 // blocks (JB1, JB2 incl. the mixing-plane dummy block), the superblocks
 // (JSBLK) and the row interiors used by the damping average.
+// Fast-mode multigrid sums: per-(level-1 I block, J, level-1 K block) sums
+// P and per-(J, level-1 K block) line totals PT, with the level-1 I (K)
+// blocks of each level-2 I (K) block.
+struct MgFast {
+  const int *i2_ptr = nullptr, *i2_list = nullptr, *k2_ptr = nullptr, *k2_list = nullptr;
+  double *P = nullptr, *PT = nullptr;
+};
+
 struct Lists {
   int *b1_ptr = nullptr, *b1_list = nullptr, *b2_ptr = nullptr, *b2_list = nullptr;
   int *sb_ptr = nullptr, *sb_list = nullptr, *row_ptr = nullptr, *row_list = nullptr;
+  MgFast mgf;             // fast-mode multigrid sums (valid when mgf_ok)
+  bool mgf_ok = false;
   float *mixscratch = nullptr;
 };
 

@@ -751,6 +751,397 @@ __global__ void k_scatter(Geo g, Dev d, const float *D, float *DOUT) {
   DOUT[n] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Fused TSTEP stages.  Each keeps the reference's arithmetic; only the
+// launch structure differs from the phase-per-kernel form.
+
+// mixing-plane role of J: 1 = JMIX(m), 2 = JMIX(m)+1, 3 = JMIX(m)+2 for an
+// interface m < NROWS (rows are contiguous in J), with m returned
+__device__ __forceinline__ int mp_role(const Geo &g, const Dev &d, int J, int &m) {
+  if (g.NROWS < 2) return 0;
+  const int NR = d.NROW[J - 1];
+  if (NR < g.NROWS && J == d.JMIX[NR - 1]) { m = NR; return 1; }
+  if (NR > 1 && J == d.JMIX[NR - 2] + 1) { m = NR - 1; return 2; }
+  if (NR > 1 && J == d.JMIX[NR - 2] + 2) { m = NR - 1; return 3; }
+  return 0;
+}
+
+// One (J,K) line per wavefront: the upstream-face flux extrapolation at the
+// mixing planes applied to this line's source (mf:5516-5555), the flux sum
+// and SSS history with the tip-gap TFLUX balance (mf:5503-5510, 5560-5570),
+// and the first pitchwise STORE average on the mixing-plane lines
+// (mf:5578-5607).
+__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p) {
+  const int lane = threadIdx.x, J = blockIdx.x * 4 + threadIdx.y + 2, K = blockIdx.y + 1;
+  if (J > g.JM) return;
+  const int IM = g.IM, IMM1 = g.IMM1;
+  const int NR = d.NROW[J - 1];
+  const float RATPITCH = (float)d.NBLADE[J - 2] / (float)d.NBLADE[J - 1];
+  const int kts = d.KTIPS[NR - 1];
+  const bool tipbal = kts > 0 && K >= kts && K <= d.KTIPE[NR - 1] - 1;
+  const float tavg = tipbal ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
+  int m = 0;
+  const int role = mp_role(g, d, J, m);
+  float FLOWDIRN = 0.0f;
+  if (role) FLOWDIRN = -d.FLOWX[IX(p.IMID, d.JMIX[m - 1], K)];
+  const bool fex = !(p.FEXTRAP < 0.001f);
+  const bool fexline = fex && ((role == 1 && FLOWDIRN > 0.0f) || (role == 3 && !(FLOWDIRN > 0.0f)));
+  float xf[4] = {0.f, 0.f, 0.f, 0.f}, AVFL = 0.0f;
+  if (fexline) {
+    const int jx = (role == 1) ? J - 1 : J;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      xf[c] = (I <= IMM1) ? d.XFLUX[IX(I, jx, K)] : 0.0f;
+    }
+    AVFL = line_sum(xf, IMM1, p.EXACT);
+  }
+  float st[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    st[c] = 0.0f;
+    if (I > IMM1) continue;
+    const unsigned n = IX(I, J, K);
+    float TF = d.TFLUX[n], TFP = d.TFLUX[n + 1];
+    if (tipbal) {
+      if (I == 1) TF = tavg;
+      if (I + 1 == IM) TFP = tavg;
+    }
+    float src = d.SOURCE[n];
+    if (fexline) {
+      const float DFLUX = (xf[c] - AVFL * d.FP[I - 1]) * p.FEXTRAP;
+      src = (role == 1) ? src - DFLUX : src + DFLUX;
+    }
+    if (fex && role == 2) src = 0.0f;
+    const float DELTA = d.XFLUX[n] - d.XFLUX[IX(I, J - 1, K)] * RATPITCH + d.RFLUX[n] - d.RFLUX[IX(I, J, K + 1)] + TF -
+                        TFP - src;
+    const float df = DIFF[n];
+    st[c] = p.F1 * DELTA + p.F2 * df;
+    DIFF[n] = DELTA + p.F3 * df;
+  }
+  const bool avg1 = (role == 3 && (FLOWDIRN > 0.0f || p.FEXTRAP < 0.001f)) ||
+                    (role == 1 && (FLOWDIRN < 0.0f || p.FEXTRAP < 0.001f));
+  if (avg1) {
+    const float SUM = line_sum(st, IMM1, p.EXACT);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      if (I <= IMM1) st[c] = SUM * d.FP[I - 1];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    if (I <= IMM1) d.STORE[IX(I, J, K)] = st[c];
+  }
+}
+
+// Fast-mode multigrid sums (mf:5621-5640) in two passes.  Pass 1: one
+// wavefront per (J, level-1 K block); each lane sums its I column over the
+// block's K planes (FP64), then the lanes of each level-1 I block are added
+// into P[(ib, J, kb)] and all lanes into PT[(J, kb)].  Needs IR, KR <= 4.
+#define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
+__global__ void __launch_bounds__(256) k_mg_part(Geo g, Dev d, MgFast m, int ir, int kr) {
+  __shared__ double sh[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wid = blockIdx.x * 4 + w;
+  const int J = wid % (g.JM - 1) + 2, kb = wid / (g.JM - 1) + 1;
+  if (kb > g.NKB1) return;  // whole wavefronts only: no barrier below spans wavefronts
+  const int k0 = (kb - 1) * kr + 1, k1 = min(kb * kr, g.KMM1);
+  float x[4][4];
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1, K = k0 + kk;
+      x[kk][c] = (K <= k1 && I <= g.IMM1) ? d.STORE[IX(I, J, K)] : 0.0f;
+    }
+  double tot = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double a = (((double)x[0][c] + (double)x[1][c]) + (double)x[2][c]) + (double)x[3][c];
+    sh[w][64 * c + lane] = a;
+    tot += a;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+  if (lane == 0) m.PT[JKX(J, kb)] = tot;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  for (int ib = lane + 1; ib <= g.NIB1; ib += 64) {
+    const int i0 = (ib - 1) * ir, i1 = min(ib * ir, g.IMM1);
+    double s = 0.0;
+    for (int i = i0; i < i1; i++) s += sh[w][i];
+    m.P[PX(ib, J, kb)] = s;
+  }
+}
+
+// Pass 2, by block range: one thread per level-1 block (its J members from
+// the level-1 list), one wavefront per level-2 block (its level-1 I and K
+// blocks and J members), one workgroup per superblock (its J members, all
+// level-1 K blocks); fixed-order trees throughout.
+__global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b1p, const int *b1l, const int *b2p,
+                                                   const int *b2l, const int *sbp, const int *sbl, float *B1, float *B2,
+                                                   float *SB) {
+  const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
+  const int nblk1 = (n1 + 255) / 256, nblk2 = (n2 + 3) / 4;
+  const int bx = blockIdx.x;
+  if (bx < nblk1) {
+    const int t = bx * 256 + threadIdx.x;
+    if (t >= n1) return;
+    const int nj = g.NJB1 + 1;
+    const int ib = t % g.NIB1 + 1, jb = (t / g.NIB1) % nj + 1, kb = t / (g.NIB1 * nj) + 1;
+    const int q0 = b1p[jb - 1], q1 = b1p[jb];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = (q0 + u < q1) ? m.P[PX(ib, b1l[q0 + u], kb)] : 0.0;
+    double s = ((v[0] + v[1]) + v[2]) + v[3];
+    for (int q = q0 + 4; q < q1; q++) s += m.P[PX(ib, b1l[q], kb)];
+    B1[t] = (float)s;
+    return;
+  }
+  if (bx < nblk1 + nblk2) {
+    const int lane = threadIdx.x & 63;
+    const int u = (bx - nblk1) * 4 + (threadIdx.x >> 6);
+    if (u >= n2) return;
+    const int nj = g.NJB2 + 1;
+    const int ib = u % g.NIB2, jb = (u / g.NIB2) % nj, kb = u / (g.NIB2 * nj);
+    const int ia = m.i2_ptr[ib], ni = m.i2_ptr[ib + 1] - ia, ka = m.k2_ptr[kb], nk = m.k2_ptr[kb + 1] - ka;
+    const int ja = b2p[jb], njj = b2p[jb + 1] - ja;
+    const int tot = ni * njj * nk;
+    double s = 0.0;
+    for (int c = lane; c < tot; c += 64) {
+      const int ii = c % ni, r = c / ni;
+      s += m.P[PX(m.i2_list[ia + ii], b2l[ja + r % njj], m.k2_list[ka + r / njj])];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) B2[u] = (float)s;
+    return;
+  }
+  __shared__ double sh[256];
+  const int sb = bx - nblk1 - nblk2;
+  const int ja = sbp[sb], nj = sbp[sb + 1] - ja, tot = nj * g.NKB1;
+  double s = 0.0;
+  for (int c0 = 0; c0 < tot; c0 += 1024) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = c0 + 256 * u + threadIdx.x;
+      v[u] = (c < tot) ? m.PT[JKX(sbl[ja + c % nj], c / nj + 1)] : 0.0;
+    }
+    s += ((v[0] + v[1]) + v[2]) + v[3];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) SB[sb] = (float)sh[0];
+}
+#undef PX
+
+// multigrid update (mf:5645-5660), one (J,K) line per wavefront, leaving
+// the FP64 line sum of |STORE| for the damping average in red[(J,K)]
+__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *red) {
+  const int lane = threadIdx.x, J = blockIdx.x * 4 + threadIdx.y + 2, K = blockIdx.y + 1;
+  if (J > g.JM) return;
+  const int K1 = d.KB1[K - 1], K2 = d.KB2[K - 1];
+  const int J1 = d.JB1[J - 1], J2 = d.JB2[J - 1], JSB = d.JSBLK[J - 1];
+  const float sbs = d.SBCHG[JSB - 1] * d.STEPSBK[JSB - 1];
+  double s = 0.0;
+  for (int I = lane + 1; I <= g.IMM1; I += 64) {
+    const unsigned n = IX(I, J, K);
+    const int I1 = d.IB1[I - 1], I2 = d.IB2[I - 1];
+    const unsigned b1 = (unsigned)(I1 - 1) + (unsigned)g.NIB1 * ((unsigned)(J1 - 1) + (unsigned)(g.NJB1 + 1) * (K1 - 1));
+    const unsigned b2 = (unsigned)(I2 - 1) + (unsigned)g.NIB2 * ((unsigned)(J2 - 1) + (unsigned)(g.NJB2 + 1) * (K2 - 1));
+    const float DELTA = d.STORE[n] * d.STEP[n] + (d.B1CHG[b1] * d.STEP1[b1] + d.B2CHG[b2] * d.STEP2[b2] + sbs) * d.RSTEP[n];
+    d.STORE[n] = DELTA;
+    s += (double)fabsf(DELTA);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) red[JKX(J, K)] = s;
+}
+
+// per-row average |change| from the (J,K) line sums (mf:5677-5693): one
+// workgroup per row (a contiguous J range), fixed-order tree
+__global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *red, const int *rptr, const int *rlist) {
+  __shared__ double sh[1024];
+  const int r = blockIdx.x, t = threadIdx.x;
+  const int q0 = rptr[r], nj = rptr[r + 1] - q0, ja = rlist[q0], tot = nj * g.KMM1;
+  double s = 0.0;
+  for (int c0 = 0; c0 < tot; c0 += 4096) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = c0 + 1024 * u + t;
+      v[u] = (c < tot) ? red[JKX(ja + c % nj, c / nj + 1)] : 0.0;
+    }
+    s += ((v[0] + v[1]) + v[2]) + v[3];
+  }
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (t < off) sh[t] += sh[t + off];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const float f = (float)sh[0];
+    d.SC[SC_AVG_CHG + r] = f / (float)(g.IMM1 * g.KMM1 * nj);
+  }
+}
+
+// the damped change of one cell (mf:5716-5724)
+__device__ __forceinline__ float damped(const Geo &g, const Dev &d, const float *AC, float DELTA, int J, float DAMP) {
+  const float ABSCHG = fabsf(DELTA);
+  const float FDAMP = ABSCHG / avg_blk(AC, d.NROW[J - 1], g.NROWS);
+  return DELTA / (1.f + FDAMP / DAMP);
+}
+
+// Damping (mf:5716-5724), the second pitchwise average of the damped change
+// on the mixing-plane lines (mf:5786-5818) and the corner distribution
+// (mf:5828-5842).  One workgroup (4 wavefronts) per tile of kTJ x kTK node
+// lines (all I).  The tile's cells (J..J+kTJ, K-1..K+kTK-1) are damped once
+// into LDS and the averaged mixing-plane lines replaced by their mean there;
+// the corner weights of those K planes are staged in LDS beside them.  Each
+// node then gathers from cells I-1..I, J..J+1, K-1..K in the order the
+// reference's cell loop (K, J, I) visits them.  Dynamic LDS: 45 lines of
+// `pitch` floats (pitch = IM-1 rounded up to 64).
+constexpr int kTJ = 4, kTK = 4, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4;
+__global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, const float *D, float *DOUT, float DAMP,
+                                                      int damp, int mpav) {
+  extern __shared__ float lds[];
+  __shared__ float avg[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
+  __shared__ int flag[kTL];
+  const int j0 = blockIdx.x * kTJ + 1, k0 = blockIdx.y * kTK + 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, t = threadIdx.x;
+  const int IM = g.IM, IMM1 = g.IMM1, pitch = 64 * ((IMM1 + 63) / 64);
+  float *C = lds;                  // [kTL][pitch] cell changes, line (a, b) = J j0+a, K k0-1+b
+  float *W = lds + kTL * pitch;    // [4][kTK+1][pitch]: FTL, FBL, FTR, FBR at K k0-1+b
+  // per-line and per-J metadata
+  if (t < kTL) {
+    const int jc = j0 + t % (kTJ + 1), kc = k0 - 1 + t / (kTJ + 1);
+    int f = 0;
+    if (jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1) {
+      f = 1;
+      if (mpav) {
+        int m = 0;
+        const int role = mp_role(g, d, jc, m);
+        if (role == 1 || role == 3) {
+          const float FLOWDIRN = -d.FLOWX[IX(p.IMID, d.JMIX[m - 1], kc)];
+          const bool on = (role == 3) ? ((FLOWDIRN > 0.0f) || (p.FEXTRAP < 0.001f))
+                                      : ((FLOWDIRN < 0.0f) || (p.FEXTRAP < 0.001f));
+          if (on) f = 3;
+        }
+      }
+    }
+    flag[t] = f;
+  } else if (t >= 64 && t < 64 + kTJ + 1) {
+    const int jc = j0 + t - 64;
+    if (jc >= 2 && jc <= g.JM) {
+      avg[t - 64] = damp ? avg_blk(d.SC + SC_AVG_CHG, d.NROW[jc - 1], g.NROWS) : 1.0f;
+      fdn[t - 64] = d.FACDWN[jc - 1];
+      fup[t - 64] = d.FACUP[jc - 1];
+    }
+  }
+  // this wavefront's cell lines: ln = w + 4*u
+  float x[kMaxLn][4];
+#pragma unroll
+  for (int u = 0; u < kMaxLn; u++) {
+    const int ln = w + 4 * u;
+    const int jc = j0 + ln % (kTJ + 1), kc = k0 - 1 + ln / (kTJ + 1);
+    const bool in = ln < kTL && jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      x[u][c] = (in && I <= IMM1) ? d.STORE[IX(I, jc, kc)] : 0.0f;
+    }
+  }
+  // corner weights: 4 arrays x (kTK+1) K planes = 20 lines, 5 per wavefront
+#pragma unroll
+  for (int u = 0; u < 5; u++) {
+    const int wl = w + 4 * u, arr = wl / (kTK + 1), kc = k0 - 1 + wl % (kTK + 1);
+    const float *src = arr == 0 ? d.FTL : arr == 1 ? d.FBL : arr == 2 ? d.FTR : d.FBR;
+    if (kc >= 1 && kc <= g.KMM1)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int I = 64 * c + lane + 1;
+        if (I <= IMM1) W[wl * pitch + I - 1] = src[IKX(I, kc)];
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kMaxLn; u++) {
+    const int ln = w + 4 * u;
+    if (ln >= kTL) continue;
+    const int f = flag[ln];
+    if (!f) continue;
+    float v[4];
+    const float A = avg[ln % (kTJ + 1)];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      float y = x[u][c];
+      if (damp) {
+        const float FDAMP = fabsf(y) / A;
+        y = y / (1.f + FDAMP / DAMP);
+      }
+      v[c] = y;
+    }
+    if (f == 3) {
+      const float SUM = line_sum(v, IMM1, p.EXACT) / (float)IMM1;
+#pragma unroll
+      for (int c = 0; c < 4; c++) v[c] = SUM;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      if (I <= IMM1) C[ln * pitch + I - 1] = v[c];
+    }
+  }
+  __syncthreads();
+  // gather: wavefront w takes node lines J = j0 + w, K = k0 .. k0+kTK-1
+  const int J = j0 + w;
+  if (J > g.JM) return;
+  for (int I = lane + 1; I <= IM; I += 64) {
+    float dv[kTK];
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk++) dv[kk] = (k0 + kk <= g.KM) ? D[IX(I, J, k0 + kk)] : 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk++) {
+      const int K = k0 + kk;
+      if (K > g.KM) break;
+      float v = dv[kk];
+#pragma unroll
+      for (int b2 = 0; b2 < 2; b2++) {
+        const int kc = K - 1 + b2;
+        if (kc < 1 || kc > g.KMM1) continue;
+        const int b = kk + b2;  // kc - (k0 - 1)
+        const bool top = (b2 == 0);  // node is the cell's K+1 corner
+#pragma unroll
+        for (int a2 = 0; a2 < 2; a2++) {
+          const int jc = J + a2;
+          if (jc < 2 || jc > g.JM) continue;
+          const int a = w + a2;
+          const float fac = a2 == 0 ? fdn[a] : fup[a];
+          const float *Lc = C + (a + (kTJ + 1) * b) * pitch;
+#pragma unroll
+          for (int i2 = 0; i2 < 2; i2++) {
+            const int ic = I - 1 + i2;
+            if (ic < 1 || ic > IMM1) continue;
+            const bool right = (i2 == 0);  // node is the cell's I+1 corner
+            const int arr = top ? (right ? 2 : 0) : (right ? 3 : 1);
+            v = v + Lc[ic - 1] * W[(arr * (kTK + 1) + b) * pitch + ic - 1] * fac;
+          }
+        }
+      }
+      DOUT[IX(I, J, K)] = v;
+    }
+  }
+}
+
 // SMOOTH_VAR (mf:15352-15594) followed by TSTEP's periodic, tip-gap and
 // mixing-plane conditions on D (mf:5858-5926).  One workgroup per J plane:
 // the streamwise stencil reads planes J-2..J+2 of the pre-smoothing field
@@ -1403,11 +1794,13 @@ __global__ void k_sa_fluxes(Geo g, Dev d) {
   (void)KMM1;
 }
 
-// D += STORE on the cells, mf:5740-5744
-__global__ void k_sa_addstore(Geo g, Dev d, float *D) {
+// D += STORE (damped as TSTEP's negative feedback leaves it) on the cells,
+// mf:5716-5744
+__global__ void k_sa_addstore(Geo g, Dev d, float *D, float DAMP, int damp) {
   CELL_IJK
   const unsigned n = IX(I, J, K);
-  D[n] = D[n] + d.STORE[n];
+  const float x = d.STORE[n];
+  D[n] = D[n] + (damp ? damped(g, d, d.SC + SC_AVG_CHG, x, J, DAMP) : x);
 }
 
 // pitch average carried across each mixing plane, mf:5748-5762 (one

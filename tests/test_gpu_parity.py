@@ -127,13 +127,18 @@ def test_vs_reference(case, nsteps, exact):
         assert err <= tol, (v, err)
     if nsteps == 100:
         # SUMFLX's per-J integrals against what the reference's EFICOOL wrote
-        # at the end of the same run (FP64 here, FP32 sequential there)
+        # at the end of the same run (FP64 here, FP32 sequential there).
+        # The entropy function
+        # CP*ln(T/TO1) - R*ln(p/PO1) is a small difference of two terms that
+        # are ~100x larger in a nearly loss-free (inviscid) run, so the
+        # state's few-1e-6 rounding differences reach its integral amplified
+        # by that factor: 2e-4 there, 5e-5 for the other integrals.
         integ = g.integrals()
         names = ["BLADE_FLOW", "SUMPO", "SUMTO", "SUMRVT", "SUM_ENTPY"]
         for q, name in enumerate(names):
             err = _nerr(integ[q], ref["SUMFLX"][q])
             print(f"{case} exact={exact} {name}: normalized max err {err:.2e}")
-            assert err <= 5e-5, (name, err)
+            assert err <= (2e-4 if name == "SUM_ENTPY" else 5e-5), (name, err)
     for v in ["DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP"]:
         print(f"{case} {nsteps} exact={exact} {v}: normalized max err {_nerr(g.get(v), ref[v]):.2e} (reported)")
 
