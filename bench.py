@@ -38,11 +38,44 @@ B_ALG_STEP = 132.0          # bytes per node-step of a fused FP32 step (U, histo
 KERNEL_BYTES_PER_NODE = {
     "k_prim": 4 * (5 + 7), "k_massflux": 4 * (10 + 9), "k_flux_energy": 4 * (6 + 4),
     "k_flux_axial": 4 * (9 + 4), "k_flux_angular": 4 * (6 + 4), "k_flux_radial": 4 * (12 + 4),
-    "k_residual": 4 * (5 + 2), "k_mg_apply": 4 * (3 + 1), "k_damp_apply": 4 * 2,
-    "k_scatter": 4 * (2 + 1), "k_smooth_plane": 4 * 2, "k_peff": 4 * 2, "k_mg_sum_seq": 4,
-    "k_mg_sum_wave": 4, "k_jplane_sums": 4, "k_rolim_check": 4, "k_mach_apply": 4 * 4,
-    "k_stepup_temp": 4 * 6, "k_stepup_step": 4 * 5, "k_smooth_resid": 4 * 2,
+    "k_resid_line": 4 * (5 + 2), "k_mg_part": 4, "k_mg_apply_lines": 4 * (3 + 1), "k_mg_apply": 4 * (3 + 1),
+    "k_damp_scatter1": 4 * (2 + 1), "k_damp_scatter4": 4 * (2 + 1), "k_smooth_plane": 4 * 2, "k_peff": 4 * 2,
+    "k_mg_sum_seq": 4, "k_mg_sum_wave": 4, "k_jplane_sums": 4, "k_rolim_check": 4, "k_mach_apply": 4 * 4,
+    "k_stepup_temp": 4 * 6, "k_stepup_step": 4 * 5, "k_smooth_resid": 4 * 2, "k_sa_addstore": 4 * 3,
 }
+
+# HBM traffic per launch of the TSTEP kernels from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes over this workload, each scaled by what the same counter
+# reads for a copy of known size (see --calibrate-copy); written by hand from
+# those runs into this file, read here when present.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tstep_kernels_config1_sa_pmc_traffic.json")
+
+
+def traffic_bytes(kernel: str):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    cal = t["calibration"]
+    return (k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]) * 1024.0
+
+
+def calibrate_copy() -> None:
+    """One device-to-device copy of 256 MiB, three times, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes exactly
+    268435456 bytes)."""
+    import torch
+    n = 1 << 26
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "copies": 3}))
 
 
 def workload_deck(nsteps: int, ilos: int):
@@ -88,10 +121,15 @@ def main():
     ap.add_argument("--ilos", type=int, default=200,
                     help="viscous model of the deck: 200 Spalart-Allmaras (the workload's), 10 mixing length, 0 inviscid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device copy (for scaling the PMC byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: multi-GPU runs are not part of this build "
-              "(rows and operating points would shard across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(rows and operating points would shard across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     total = args.warmup + args.steps + 1000
@@ -145,7 +183,7 @@ def main():
                    if args.ilos >= 200 else "1.5-stage axial turbine, 3 rows x 64x64x193 nodes, FP32 plain form",
                    "nodes": nodes, "viscous_model": f"ILOS={args.ilos}", "scheme": "scree, multigrid 3/9"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(dom),
                      "alg_bytes_per_launch": bytes_launch, "avg_launch_us": avg_s * 1e6},
         "step_roofline": {"alg_bytes_per_node_step": B_ALG_STEP,
                           "achieved_GBs": value * B_ALG_STEP / 1e9,

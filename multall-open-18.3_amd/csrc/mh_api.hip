@@ -67,7 +67,8 @@ __global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, 
                             float *, float *, float *);
 __global__ void k_mg_apply_lines(Geo, Dev, double *);
 __global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
-__global__ void k_damp_scatter(Geo, Dev, Params, const float *, float *, float, int, int);
+__global__ void k_damp_scatter1(Geo, Dev, Params, const float *, float *, float, int, int);
+__global__ void k_damp_scatter4(Geo, Dev, Params, const float *, float *, float, int, int);
 __global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
 __global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
@@ -416,12 +417,14 @@ int build_lists(mh_ctx *c) {
   // the fused line kernels take rows as contiguous J ranges joined at JMIX
   std::vector<int> nrow(dm.jm);
   if ((r = d2h(c, nrow.data(), c->d.NROW, dm.jm * 4))) return r;
+  std::vector<int> jmp(dm.jm, 0);
   for (int nr = 1; nr < dm.nrows; nr++) {
     const int J = jmix[nr - 1];
     if (J + 2 > dm.jm || nrow[J - 1] != nr || nrow[J] != nr + 1 || nrow[J + 1] != nr + 1)
       return fail(c, MH_E_UNSUPPORTED, "rows not contiguous at a mixing plane");
+    for (int q = 0; q < 3; q++) jmp[J - 1 + q] = (q + 1) | (J << 2);
   }
-  return MH_OK;
+  return h2d(c, c->d.JMP, jmp.data(), jmp.size() * 4);
 }
 
 // TSTEP (mf:5470-5932) for one conserved variable.  The launch chain per
@@ -436,7 +439,7 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   const Params p = params_of(c);
   const StepScalars s = step_scalars(c);
   const dim3 grid = node_grid(g);
-  const dim3 lgrid((g.JM - 1 + 3) / 4, g.KMM1), lblock(64, 4);
+  const dim3 lgrid((g.JM - 1 + 3) / 4, (g.KMM1 + 3) / 4), lblock(256);
   const int nmp = (g.NROWS - 1) * g.KMM1;
   const Lists &L = c->lists;
   const bool damp = !(s.DAMP < 2.0f || s.DAMP > 100);
@@ -516,9 +519,15 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   }
   // damping, second mixing-plane averages (mf:5786-5818) and the corner
   // distribution
-  LAUNCH(k_damp_scatter, dim3((g.JM + 3) / 4, (g.KM + 3) / 4), dim3(256),
-         (size_t)45 * 64 * ((g.IMM1 + 63) / 64) * sizeof(float), c->stream, g, d, p, (const float *)D,
-         d.DSC, s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+  {
+    const dim3 sg((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + kScatTK - 1) / kScatTK);
+    if (g.IMM1 <= 64)
+      LAUNCH(k_damp_scatter1, sg, dim3(256), scatter_lds_bytes(g.IMM1), c->stream, g, d, p, (const float *)D, d.DSC,
+             s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+    else
+      LAUNCH(k_damp_scatter4, sg, dim3(256), scatter_lds_bytes(g.IMM1), c->stream, g, d, p, (const float *)D, d.DSC,
+             s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+  }
   LCHK("damp_scatter");
   LAUNCH(k_smooth_plane, dim3(g.JM), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p, s, ncall);
   LCHK("smooth_plane");
@@ -848,7 +857,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)3 * dm.km + 4)) ||
       (r = dalloc(c, &d.LTAB, (size_t)dm.nrows * (4 * dm.im + 2 * dm.km))) ||
       (r = dalloc(c, &d.SAFLAG, (size_t)dm.im * dm.jm + (size_t)dm.jm * dm.km)) ||
-      (r = dalloc(c, &d.SAJMAP, (size_t)4 * dm.jm + 4)) ||
+      (r = dalloc(c, &d.SAJMAP, (size_t)4 * dm.jm + 4)) || (r = dalloc(c, &d.JMP, (size_t)dm.jm)) ||
       (r = dalloc(c, &d.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))) ||
       (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
@@ -856,10 +865,12 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
     return bail(r);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
-  c->lds_plane = (size_t)3 * dm.im * dm.km * sizeof(float);
+  c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
   if (c->lds_plane > 160 * 1024) return bail(MH_E_UNSUPPORTED);
   if (hipFuncSetAttribute((const void *)k_smooth_plane, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)c->lds_plane) != hipSuccess)
+                          (int)c->lds_plane) != hipSuccess ||
+      hipFuncSetAttribute((const void *)k_damp_scatter4, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)scatter_lds_bytes(dm.im - 1)) != hipSuccess)
     return bail(MH_E_HIP);
   *out = c;
   return MH_OK;

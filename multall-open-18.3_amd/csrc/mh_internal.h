@@ -69,6 +69,7 @@ struct Dev {
   float *ABS_VORT, *VISC_TURB, *DYN_VIS_ML, *GRAD_TVIS, *TVGX, *TVGR, *TVGT;
   float *FORCEX, *FORCER, *FORCET, *ESOURCE;
   int *SAFLAG;                       // transition flags per (I,J) column and (J,K) line
+  int *JMP;                          // per J: mixing-plane role (low 2 bits) and JMIX of its interface
   int *SAJMAP;                       // per J: the J-1 and row the reference's periodic-force block reads
   double *RED;                       // FP64 partial sums of the reductions
   float *SC;                         // device-side scalars (see SC_* below)
@@ -121,6 +122,14 @@ struct Params {
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid
 // blocks (JB1, JB2 incl. the mixing-plane dummy block), the superblocks
 // (JSBLK) and the row interiors used by the damping average.
+// Node tile of the damping + corner-distribution kernel (J x K lines, all
+// I) and its dynamic LDS: cell lines (J..J+TJ, K-1..K+TK-1) and the four
+// corner-weight arrays on K-1..K+TK-1, each `pitch` = IM-1 rounded up to 64.
+constexpr int kScatTJ = 4, kScatTK = 8;
+inline size_t scatter_lds_bytes(int imm1) {
+  return (size_t)((kScatTJ + 1) * (kScatTK + 1) + 4 * (kScatTK + 1)) * 64 * ((imm1 + 63) / 64) * sizeof(float);
+}
+
 // Fast-mode multigrid sums: per-(level-1 I block, J, level-1 K block) sums
 // P and per-(J, level-1 K block) line totals PT, with the level-1 I (K)
 // blocks of each level-2 I (K) block.

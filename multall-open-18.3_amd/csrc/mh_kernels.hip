@@ -49,6 +49,21 @@ __device__ __forceinline__ float line_sum(const float (&v)[4], int n, int exact)
   return (float)s;
 }
 
+// The wavefront's index in its workgroup, as a scalar (SGPR) value, so that
+// every line index derived from it is computed once per wavefront on the
+// scalar unit rather than per lane.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// XCD-aware workgroup order.  Workgroups are dispatched round-robin over the
+// 8 XCDs, each with its own L2; this bijection on [0, n) gives XCD x the
+// contiguous range of logical ids it would otherwise share with the other
+// seven, so neighbouring J planes (stencil halos) meet in one L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, r = n % kXcd, x = b % kXcd, l = b / kXcd;
+  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
 // ---------------------------------------------------------------------------
 // primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
 
@@ -323,7 +338,7 @@ __global__ void k_bc_sums(Geo g, Dev d, Params p, StepScalars s) {
   extern __shared__ float sh[];
   float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *PAVGOUT = d.BCW + 2 * g.KM, *SUMK = sh + 3 * g.KM;
   const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id(), nwave = blockDim.x >> 6;
   // pitchwise sums per K in the reference's order over I, mf:4512-4521
   for (int K = wave + 1; K <= g.KM; K += nwave) {
     float v2[4], v3[4], vo[4], vs[4];
@@ -755,44 +770,34 @@ __global__ void k_scatter(Geo g, Dev d, const float *D, float *DOUT) {
 // Fused TSTEP stages.  Each keeps the reference's arithmetic; only the
 // launch structure differs from the phase-per-kernel form.
 
-// mixing-plane role of J: 1 = JMIX(m), 2 = JMIX(m)+1, 3 = JMIX(m)+2 for an
-// interface m < NROWS (rows are contiguous in J), with m returned
-__device__ __forceinline__ int mp_role(const Geo &g, const Dev &d, int J, int &m) {
-  if (g.NROWS < 2) return 0;
-  const int NR = d.NROW[J - 1];
-  if (NR < g.NROWS && J == d.JMIX[NR - 1]) { m = NR; return 1; }
-  if (NR > 1 && J == d.JMIX[NR - 2] + 1) { m = NR - 1; return 2; }
-  if (NR > 1 && J == d.JMIX[NR - 2] + 2) { m = NR - 1; return 3; }
-  return 0;
+// mixing-plane role of J from the per-J table d.JMP (built at start-up):
+// 1 = JMIX(m), 2 = JMIX(m)+1, 3 = JMIX(m)+2 for an interface m < NROWS,
+// 0 elsewhere; jx returns JMIX(m)
+__device__ __forceinline__ int mp_role(const Dev &d, int J, int &jx) {
+  const int v = d.JMP[J - 1];
+  jx = v >> 2;
+  return v & 3;
 }
 
-// One (J,K) line per wavefront: the upstream-face flux extrapolation at the
-// mixing planes applied to this line's source (mf:5516-5555), the flux sum
-// and SSS history with the tip-gap TFLUX balance (mf:5503-5510, 5560-5570),
-// and the first pitchwise STORE average on the mixing-plane lines
-// (mf:5578-5607).
-__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p) {
-  const int lane = threadIdx.x, J = blockIdx.x * 4 + threadIdx.y + 2, K = blockIdx.y + 1;
-  if (J > g.JM) return;
+// Residual of one (J,K) line on one wavefront: the upstream-face flux
+// extrapolation at the mixing planes applied to the line's source
+// (mf:5516-5555), the flux sum and SSS history with the tip-gap TFLUX
+// balance (mf:5503-5510, 5560-5570), and the first pitchwise STORE average
+// on the mixing-plane lines (mf:5578-5607).
+__device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float *DIFF, const Params &p, int J, int K,
+                                              int lane, float RATPITCH, bool tipbal, int role, int jx) {
   const int IM = g.IM, IMM1 = g.IMM1;
-  const int NR = d.NROW[J - 1];
-  const float RATPITCH = (float)d.NBLADE[J - 2] / (float)d.NBLADE[J - 1];
-  const int kts = d.KTIPS[NR - 1];
-  const bool tipbal = kts > 0 && K >= kts && K <= d.KTIPE[NR - 1] - 1;
   const float tavg = tipbal ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
-  int m = 0;
-  const int role = mp_role(g, d, J, m);
-  float FLOWDIRN = 0.0f;
-  if (role) FLOWDIRN = -d.FLOWX[IX(p.IMID, d.JMIX[m - 1], K)];
+  const float FLOWDIRN = -d.FLOWX[IX(p.IMID, jx, K)];
   const bool fex = !(p.FEXTRAP < 0.001f);
   const bool fexline = fex && ((role == 1 && FLOWDIRN > 0.0f) || (role == 3 && !(FLOWDIRN > 0.0f)));
   float xf[4] = {0.f, 0.f, 0.f, 0.f}, AVFL = 0.0f;
   if (fexline) {
-    const int jx = (role == 1) ? J - 1 : J;
+    const int jj = (role == 1) ? J - 1 : J;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const int I = 64 * c + lane + 1;
-      xf[c] = (I <= IMM1) ? d.XFLUX[IX(I, jx, K)] : 0.0f;
+      xf[c] = (I <= IMM1) ? d.XFLUX[IX(I, jj, K)] : 0.0f;
     }
     AVFL = line_sum(xf, IMM1, p.EXACT);
   }
@@ -837,6 +842,69 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
   }
 }
 
+// TSTEP's residual (mf:5503-5607).  One wavefront per (J, 4 consecutive K
+// lines), four wavefronts along J per workgroup.  Lines away from the
+// mixing planes take the plain flux sum with all four lines' loads in
+// flight together; the mixing-plane lines go through resid_mp_line.
+__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
+  if (J > g.JM) return;
+  const int IM = g.IM, IMM1 = g.IMM1;
+  const int NR = d.NROW[J - 1];
+  const float RATPITCH = (float)d.NBLADE[J - 2] / (float)d.NBLADE[J - 1];
+  const int kts = d.KTIPS[NR - 1], kte = d.KTIPE[NR - 1];
+  int jx = 0;
+  const int role = g.NROWS > 1 ? mp_role(d, J, jx) : 0;
+  const int nk = min(4, g.KMM1 - kq + 1);
+  if (role) {
+    for (int kk = 0; kk < nk; kk++) {
+      const int K = kq + kk;
+      resid_mp_line(g, d, DIFF, p, J, K, lane, RATPITCH, kts > 0 && K >= kts && K <= kte - 1, role, jx);
+    }
+    return;
+  }
+  bool tb[4];
+  float tavg[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) {
+    const int K = kq + kk;
+    tb[kk] = kk < nk && kts > 0 && K >= kts && K <= kte - 1;
+    tavg[kk] = tb[kk] ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
+  }
+  // all loads of the four lines before any store (the arrays may alias as
+  // far as the compiler knows, so a store would hold back later loads)
+  for (int I = lane + 1; I <= IMM1; I += 64) {
+    float rf[5], xf[4], xm[4], tf[4], tp[4], sr[4], df[4];
+#pragma unroll
+    for (int kk = 0; kk <= 4; kk++) rf[kk] = d.RFLUX[IX(I, J, min(kq + kk, g.KM))];
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const unsigned n = IX(I, J, min(kq + kk, g.KMM1));
+      xf[kk] = d.XFLUX[n];
+      xm[kk] = d.XFLUX[n - IM];
+      tf[kk] = d.TFLUX[n];
+      tp[kk] = d.TFLUX[n + 1];
+      sr[kk] = d.SOURCE[n];
+      df[kk] = DIFF[n];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      if (kk >= nk) break;
+      float TF = tf[kk], TFP = tp[kk];
+      if (tb[kk]) {
+        if (I == 1) TF = tavg[kk];
+        if (I + 1 == IM) TFP = tavg[kk];
+      }
+      const float DELTA = xf[kk] - xm[kk] * RATPITCH + rf[kk] - rf[kk + 1] + TF - TFP - sr[kk];
+      const unsigned n = IX(I, J, kq + kk);
+      d.STORE[n] = p.F1 * DELTA + p.F2 * df[kk];
+      DIFF[n] = DELTA + p.F3 * df[kk];
+    }
+  }
+}
+
 // Fast-mode multigrid sums (mf:5621-5640) in two passes.  Pass 1: one
 // wavefront per (J, level-1 K block); each lane sums its I column over the
 // block's K planes (FP64), then the lanes of each level-1 I block are added
@@ -844,7 +912,7 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
 #define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
 __global__ void __launch_bounds__(256) k_mg_part(Geo g, Dev d, MgFast m, int ir, int kr) {
   __shared__ double sh[4][256];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = wave_id();
   const int wid = blockIdx.x * 4 + w;
   const int J = wid % (g.JM - 1) + 2, kb = wid / (g.JM - 1) + 1;
   if (kb > g.NKB1) return;  // whole wavefronts only: no barrier below spans wavefronts
@@ -903,7 +971,7 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b
   }
   if (bx < nblk1 + nblk2) {
     const int lane = threadIdx.x & 63;
-    const int u = (bx - nblk1) * 4 + (threadIdx.x >> 6);
+    const int u = (bx - nblk1) * 4 + wave_id();
     if (u >= n2) return;
     const int nj = g.NJB2 + 1;
     const int ib = u % g.NIB2, jb = (u / g.NIB2) % nj, kb = u / (g.NIB2 * nj);
@@ -943,27 +1011,57 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b
 }
 #undef PX
 
-// multigrid update (mf:5645-5660), one (J,K) line per wavefront, leaving
-// the FP64 line sum of |STORE| for the damping average in red[(J,K)]
+// multigrid update (mf:5645-5660), one wavefront per (J, 4 consecutive K
+// lines), leaving each line's FP64 sum of |STORE| for the damping average
+// in red[(J,K)]
 __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *red) {
-  const int lane = threadIdx.x, J = blockIdx.x * 4 + threadIdx.y + 2, K = blockIdx.y + 1;
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
   if (J > g.JM) return;
-  const int K1 = d.KB1[K - 1], K2 = d.KB2[K - 1];
+  const int nk = min(4, g.KMM1 - kq + 1);
   const int J1 = d.JB1[J - 1], J2 = d.JB2[J - 1], JSB = d.JSBLK[J - 1];
   const float sbs = d.SBCHG[JSB - 1] * d.STEPSBK[JSB - 1];
-  double s = 0.0;
+  unsigned o1[4], o2[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) {
+    const int K = min(kq + kk, g.KMM1);
+    o1[kk] = (unsigned)g.NIB1 * ((unsigned)(J1 - 1) + (unsigned)(g.NJB1 + 1) * (unsigned)(d.KB1[K - 1] - 1));
+    o2[kk] = (unsigned)g.NIB2 * ((unsigned)(J2 - 1) + (unsigned)(g.NJB2 + 1) * (unsigned)(d.KB2[K - 1] - 1));
+  }
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
   for (int I = lane + 1; I <= g.IMM1; I += 64) {
-    const unsigned n = IX(I, J, K);
-    const int I1 = d.IB1[I - 1], I2 = d.IB2[I - 1];
-    const unsigned b1 = (unsigned)(I1 - 1) + (unsigned)g.NIB1 * ((unsigned)(J1 - 1) + (unsigned)(g.NJB1 + 1) * (K1 - 1));
-    const unsigned b2 = (unsigned)(I2 - 1) + (unsigned)g.NIB2 * ((unsigned)(J2 - 1) + (unsigned)(g.NJB2 + 1) * (K2 - 1));
-    const float DELTA = d.STORE[n] * d.STEP[n] + (d.B1CHG[b1] * d.STEP1[b1] + d.B2CHG[b2] * d.STEP2[b2] + sbs) * d.RSTEP[n];
-    d.STORE[n] = DELTA;
-    s += (double)fabsf(DELTA);
+    const unsigned i1 = (unsigned)(d.IB1[I - 1] - 1), i2 = (unsigned)(d.IB2[I - 1] - 1);
+    float st[4], sp[4], rs[4], c1[4], s1[4], c2[4], s2[4];
+    // all loads of the four lines before any store
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const unsigned n = IX(I, J, min(kq + kk, g.KMM1)), b1 = i1 + o1[kk], b2 = i2 + o2[kk];
+      st[kk] = d.STORE[n];
+      sp[kk] = d.STEP[n];
+      rs[kk] = d.RSTEP[n];
+      c1[kk] = d.B1CHG[b1];
+      s1[kk] = d.STEP1[b1];
+      c2[kk] = d.B2CHG[b2];
+      s2[kk] = d.STEP2[b2];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      if (kk >= nk) break;
+      const float DELTA = st[kk] * sp[kk] + (c1[kk] * s1[kk] + c2[kk] * s2[kk] + sbs) * rs[kk];
+      d.STORE[IX(I, J, kq + kk)] = DELTA;
+      s[kk] += (double)fabsf(DELTA);
+    }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) red[JKX(J, K)] = s;
+  for (int kk = 0; kk < 4; kk++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s[kk] += __shfl_xor(s[kk], off, 64);
+  }
+  if (lane < nk) {
+    const double v = lane == 0 ? s[0] : lane == 1 ? s[1] : lane == 2 ? s[2] : s[3];
+    red[JKX(J, kq + lane)] = v;
+  }
 }
 
 // per-row average |change| from the (J,K) line sums (mf:5677-5693): one
@@ -1008,19 +1106,41 @@ __device__ __forceinline__ float damped(const Geo &g, const Dev &d, const float 
 // into LDS and the averaged mixing-plane lines replaced by their mean there;
 // the corner weights of those K planes are staged in LDS beside them.  Each
 // node then gathers from cells I-1..I, J..J+1, K-1..K in the order the
-// reference's cell loop (K, J, I) visits them.  Dynamic LDS: 45 lines of
-// `pitch` floats (pitch = IM-1 rounded up to 64).
-constexpr int kTJ = 4, kTK = 4, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4;
-__global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, const float *D, float *DOUT, float DAMP,
-                                                      int damp, int mpav) {
+// reference's cell loop (K, J, I) visits them.  Dynamic LDS: kTL + kWL
+// lines of `pitch` floats (pitch = IM-1 rounded up to 64).
+constexpr int kTJ = kScatTJ, kTK = kScatTK, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4, kWL = 4 * (kTK + 1);
+static_assert(kTJ == 4, "one wavefront per tile J line");
+template <int NC>
+__device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *D, float *DOUT,
+                                             float DAMP, int damp, int mpav) {
   extern __shared__ float lds[];
   __shared__ float avg[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
   __shared__ int flag[kTL];
-  const int j0 = blockIdx.x * kTJ + 1, k0 = blockIdx.y * kTK + 1;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, t = threadIdx.x;
+  const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int j0 = (bl % gridDim.x) * kTJ + 1, k0 = (bl / gridDim.x) * kTK + 1;
+  const int lane = threadIdx.x & 63, w = wave_id(), t = threadIdx.x;
   const int IM = g.IM, IMM1 = g.IMM1, pitch = 64 * ((IMM1 + 63) / 64);
   float *C = lds;                  // [kTL][pitch] cell changes, line (a, b) = J j0+a, K k0-1+b
   float *W = lds + kTL * pitch;    // [4][kTK+1][pitch]: FTL, FBL, FTR, FBR at K k0-1+b
+  // this wavefront's cell lines (ln = w + 4*u) and its first chunk of
+  // node values, loaded before anything waits
+  float x[kMaxLn][4];
+  static_assert(NC >= 1 && NC <= 4, "IM-1 <= 256");
+#pragma unroll
+  for (int u = 0; u < kMaxLn; u++) {
+    const int ln = w + 4 * u;
+    const int jc = j0 + ln % (kTJ + 1), kc = k0 - 1 + ln / (kTJ + 1);
+    const bool in = ln < kTL && jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      x[u][c] = (c < NC && in && I <= IMM1) ? d.STORE[IX(I, jc, kc)] : 0.0f;
+    }
+  }
+  const int J = j0 + w;
+  float dv0[kTK];
+#pragma unroll
+  for (int kk = 0; kk < kTK; kk++) dv0[kk] = (J <= g.JM && k0 + kk <= g.KM && lane < IM) ? D[IX(lane + 1, J, k0 + kk)] : 0.0f;
   // per-line and per-J metadata
   if (t < kTL) {
     const int jc = j0 + t % (kTJ + 1), kc = k0 - 1 + t / (kTJ + 1);
@@ -1028,10 +1148,10 @@ __global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, co
     if (jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1) {
       f = 1;
       if (mpav) {
-        int m = 0;
-        const int role = mp_role(g, d, jc, m);
+        int jx = 0;
+        const int role = mp_role(d, jc, jx);
         if (role == 1 || role == 3) {
-          const float FLOWDIRN = -d.FLOWX[IX(p.IMID, d.JMIX[m - 1], kc)];
+          const float FLOWDIRN = -d.FLOWX[IX(p.IMID, jx, kc)];
           const bool on = (role == 3) ? ((FLOWDIRN > 0.0f) || (p.FEXTRAP < 0.001f))
                                       : ((FLOWDIRN < 0.0f) || (p.FEXTRAP < 0.001f));
           if (on) f = 3;
@@ -1047,27 +1167,14 @@ __global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, co
       fup[t - 64] = d.FACUP[jc - 1];
     }
   }
-  // this wavefront's cell lines: ln = w + 4*u
-  float x[kMaxLn][4];
+  // corner weights: 4 arrays x (kTK+1) K planes, kWL / 4 lines per wavefront
 #pragma unroll
-  for (int u = 0; u < kMaxLn; u++) {
-    const int ln = w + 4 * u;
-    const int jc = j0 + ln % (kTJ + 1), kc = k0 - 1 + ln / (kTJ + 1);
-    const bool in = ln < kTL && jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int I = 64 * c + lane + 1;
-      x[u][c] = (in && I <= IMM1) ? d.STORE[IX(I, jc, kc)] : 0.0f;
-    }
-  }
-  // corner weights: 4 arrays x (kTK+1) K planes = 20 lines, 5 per wavefront
-#pragma unroll
-  for (int u = 0; u < 5; u++) {
+  for (int u = 0; u < kWL / 4; u++) {
     const int wl = w + 4 * u, arr = wl / (kTK + 1), kc = k0 - 1 + wl % (kTK + 1);
     const float *src = arr == 0 ? d.FTL : arr == 1 ? d.FBL : arr == 2 ? d.FTR : d.FBR;
     if (kc >= 1 && kc <= g.KMM1)
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
+      for (int c = 0; c < NC; c++) {
         const int I = 64 * c + lane + 1;
         if (I <= IMM1) W[wl * pitch + I - 1] = src[IKX(I, kc)];
       }
@@ -1079,10 +1186,10 @@ __global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, co
     if (ln >= kTL) continue;
     const int f = flag[ln];
     if (!f) continue;
-    float v[4];
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
     const float A = avg[ln % (kTJ + 1)];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NC; c++) {
       float y = x[u][c];
       if (damp) {
         const float FDAMP = fabsf(y) / A;
@@ -1093,22 +1200,22 @@ __global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, co
     if (f == 3) {
       const float SUM = line_sum(v, IMM1, p.EXACT) / (float)IMM1;
 #pragma unroll
-      for (int c = 0; c < 4; c++) v[c] = SUM;
+      for (int c = 0; c < NC; c++) v[c] = SUM;
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NC; c++) {
       const int I = 64 * c + lane + 1;
       if (I <= IMM1) C[ln * pitch + I - 1] = v[c];
     }
   }
   __syncthreads();
   // gather: wavefront w takes node lines J = j0 + w, K = k0 .. k0+kTK-1
-  const int J = j0 + w;
   if (J > g.JM) return;
   for (int I = lane + 1; I <= IM; I += 64) {
     float dv[kTK];
 #pragma unroll
-    for (int kk = 0; kk < kTK; kk++) dv[kk] = (k0 + kk <= g.KM) ? D[IX(I, J, k0 + kk)] : 0.0f;
+    for (int kk = 0; kk < kTK; kk++)
+      dv[kk] = (I == lane + 1) ? dv0[kk] : (k0 + kk <= g.KM) ? D[IX(I, J, k0 + kk)] : 0.0f;
 #pragma unroll
     for (int kk = 0; kk < kTK; kk++) {
       const int K = k0 + kk;
@@ -1142,79 +1249,133 @@ __global__ void __launch_bounds__(256) k_damp_scatter(Geo g, Dev d, Params p, co
   }
 }
 
+__global__ void __launch_bounds__(256) k_damp_scatter1(Geo g, Dev d, Params p, const float *D, float *DOUT,
+                                                       float DAMP, int damp, int mpav) {
+  damp_scatter<1>(g, d, p, D, DOUT, DAMP, damp, mpav);
+}
+__global__ void __launch_bounds__(256) k_damp_scatter4(Geo g, Dev d, Params p, const float *D, float *DOUT,
+                                                       float DAMP, int damp, int mpav) {
+  damp_scatter<4>(g, d, p, D, DOUT, DAMP, damp, mpav);
+}
+
 // SMOOTH_VAR (mf:15352-15594) followed by TSTEP's periodic, tip-gap and
-// mixing-plane conditions on D (mf:5858-5926).  One workgroup per J plane:
-// the streamwise stencil reads planes J-2..J+2 of the pre-smoothing field
-// SRC (global memory, L2-resident); the rest of the work is plane-local and
-// runs in LDS.  Writes the finished plane to DST.
-__global__ void k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Params p, StepScalars s, int ncall) {
+// mixing-plane conditions on D (mf:5858-5926).  One workgroup (8
+// wavefronts) per J plane: wavefront w takes the K lines w+1, w+9, ...,
+// lane l the I points l+1, l+65, ...  The streamwise stencil reads the
+// pre-smoothing field SRC (global memory, L2-resident) four K lines at a
+// time with all loads in flight; the rest of the work is plane-local and
+// runs in LDS, with the pitchwise and spanwise coefficients staged there.
+// Writes the finished plane to DST.  Dynamic LDS: 3*IM*KM + 2*(IM+KM)
+// floats.
+__global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Params p,
+                                                         StepScalars s, int ncall) {
   extern __shared__ float lds[];
-  const int J = blockIdx.x + 1;
+  const int J = xcd_block(blockIdx.x, gridDim.x) + 1;
+  const int lane = threadIdx.x & 63, w = wave_id();
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
   const int npl = IM * KM;
-  float *PL = lds;          // the plane, [K][I]
-  float *AV = lds + npl;    // AVG scratch
+  float *PL = lds;            // the plane, [K][I]
+  float *AV = lds + npl;      // AVG scratch
   float *CU = lds + 2 * npl;  // CURVE scratch
+  float *cFD = lds + 3 * npl, *cFU = cFD + IM, *cFKD = cFU + IM, *cFKU = cFKD + KM;
 #define PLX(I, K) PL[((I) - 1) + IM * ((K) - 1)]
 #define AVX(I, K) AV[((I) - 1) + IM * ((K) - 1)]
 #define CUX(I, K) CU[((I) - 1) + IM * ((K) - 1)]
+#define FOR_PLANE for (int K = w + 1; K <= KM; K += 8) for (int I = lane + 1; I <= IM; I += 64)
+  for (int t = threadIdx.x; t < IM; t += 512) {
+    cFD[t] = d.FD[t];
+    cFU[t] = d.FU[t];
+  }
+  for (int t = threadIdx.x; t < KM; t += 512) {
+    cFKD[t] = d.FKD[t];
+    cFKU[t] = d.FKU[t];
+  }
   const float SFX = s.SFX, SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
   const float SFXB = p.FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
   const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
   const float SFT = s.SFT, SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
   const int NR = d.NROW[J - 1];
   const int J1 = d.JSTART[NR - 1], J2 = d.JMIX[NR - 1], JLED = d.JLED[NR - 1];
-  // streamwise smoothing, mf:15369-15445
-  for (int t = threadIdx.x; t < npl; t += blockDim.x) {
-    const int I = t % IM + 1, K = t / IM + 1;
-#define S(JJ) SRC[IX(I, (JJ), K)]
-    float v = S(J);
-    if (!(SFX < 0.0001f)) {
-      if (J >= J1 + 3 && J <= J2 - 3) {
-        float AVERG = 0.5f * (S(J + 1) + S(J - 1));
-        float CURV = AVERG - 0.5f * S(J) - 0.25f * (S(J - 2) + S(J + 2));
-        v = SFX1 * S(J) + SFX * (AVERG + FAC_4TH * CURV);
-      }
-      if (J == J1 + 2) v = SFXB1 * S(J1 + 2) + SFXBH * (S(J1 + 3) + S(J1 + 1));
-      if (J == J1 + 1) v = SFXB1 * S(J1 + 1) + SFXB * (1.25f * S(J1 + 2) + 0.5f * S(J1 + 3) - 0.75f * S(J1 + 4));
-      if (J == J2 - 2) v = SFXB1 * S(J2 - 2) + SFXBH * (S(J2 - 1) + S(J2 - 3));
-      if (J == J2 - 1) v = SFXB1 * S(J2 - 1) + SFXB * (1.25f * S(J2 - 2) + 0.5f * S(J2 - 3) - 0.75f * S(J2 - 4));
-      if (I == 1 || I == IM) {
-        const int Io = (I == 1) ? IM : 1;
-        if (J == JLED - 2 || J == JLED - 1)
-          v = SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
-        if (J == JLED) v = SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)]);
-      }
-    }
-#undef S
-    PLX(I, K) = v;
+  // streamwise smoothing, mf:15369-15445.  The case depends on J only.
+  const bool smx = !(SFX < 0.0001f);
+  int cs = 0;  // 0 copy, 1 interior, 2 J1+2, 3 J1+1, 4 J2-2, 5 J2-1
+  if (smx) {
+    if (J >= J1 + 3 && J <= J2 - 3) cs = 1;
+    if (J == J1 + 2) cs = 2;
+    if (J == J1 + 1) cs = 3;
+    if (J == J2 - 2) cs = 4;
+    if (J == J2 - 1) cs = 5;
   }
+  // four K lines per wavefront per pass; K is clamped so every load is
+  // unconditional and the loads of all four lines issue together
+  auto stream = [&](auto f) {
+    for (int K0 = w + 1; K0 <= KM; K0 += 32)
+      for (int I = lane + 1; I <= IM; I += 64) {
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = f(I, min(K0 + 8 * q, KM));
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (K0 + 8 * q <= KM) PLX(I, K0 + 8 * q) = v[q];
+      }
+  };
+#define S(JJ) SRC[IX(I, (JJ), K)]
+  switch (cs) {
+    case 1:
+      stream([&](int I, int K) {
+        const float AVERG = 0.5f * (S(J + 1) + S(J - 1));
+        const float CURV = AVERG - 0.5f * S(J) - 0.25f * (S(J - 2) + S(J + 2));
+        return SFX1 * S(J) + SFX * (AVERG + FAC_4TH * CURV);
+      });
+      break;
+    case 2: stream([&](int I, int K) { return SFXB1 * S(J1 + 2) + SFXBH * (S(J1 + 3) + S(J1 + 1)); }); break;
+    case 3:
+      stream([&](int I, int K) {
+        return SFXB1 * S(J1 + 1) + SFXB * (1.25f * S(J1 + 2) + 0.5f * S(J1 + 3) - 0.75f * S(J1 + 4));
+      });
+      break;
+    case 4: stream([&](int I, int K) { return SFXB1 * S(J2 - 2) + SFXBH * (S(J2 - 1) + S(J2 - 3)); }); break;
+    case 5:
+      stream([&](int I, int K) {
+        return SFXB1 * S(J2 - 1) + SFXB * (1.25f * S(J2 - 2) + 0.5f * S(J2 - 3) - 0.75f * S(J2 - 4));
+      });
+      break;
+    default: stream([&](int I, int K) { return S(J); }); break;
+  }
+  // leading-edge points I = 1 and IM (mf:15417-15434), after the above
+  if (smx && (J == JLED - 2 || J == JLED - 1 || J == JLED)) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * KM; t += blockDim.x) {
+      const int K = t % KM + 1, I = (t < KM) ? 1 : IM, Io = (I == 1) ? IM : 1;
+      PLX(I, K) = (J == JLED) ? SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)])
+                              : SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
+    }
+  }
+#undef S
   __syncthreads();
   // pitchwise and spanwise smoothing and corners, mf:15460-15541 (J >= 2)
   if (J >= 2 && !(SFT < 0.0001f)) {
     const bool blade = (d.IND[J - 1] == 1 || d.INDLE[J - 1] == 1);
-    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
-      const int I = t % IM + 1, K = t / IM + 1;
+    FOR_PLANE {
       float avg, cur;
       if (I >= 2 && I <= IMM1) {
-        avg = d.FD[I - 1] * PLX(I + 1, K) + d.FU[I - 1] * PLX(I - 1, K);
+        avg = cFD[I - 1] * PLX(I + 1, K) + cFU[I - 1] * PLX(I - 1, K);
         cur = PLX(I, K) - avg;
       } else if (!blade) {
         avg = 0.5f * (PLX(2, K) + PLX(IMM1, K));
         cur = PLX(1, K) - avg;
       } else {
-        avg = (I == 1) ? PLX(2, K) + d.FU[0] * (PLX(2, K) - PLX(3, K))
-                       : PLX(IMM1, K) + d.FU[IM - 1] * (PLX(IMM1, K) - PLX(IMM2, K));
+        avg = (I == 1) ? PLX(2, K) + cFU[0] * (PLX(2, K) - PLX(3, K))
+                       : PLX(IMM1, K) + cFU[IM - 1] * (PLX(IMM1, K) - PLX(IMM2, K));
         cur = 0.0f;
       }
       AVX(I, K) = avg;
       CUX(I, K) = cur;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
-      const int I = t % IM + 1, K = t / IM + 1;
+    FOR_PLANE {
       float sc;
-      if (I >= 2 && I <= IMM1) sc = d.FU[I - 1] * CUX(I - 1, K) + d.FD[I - 1] * CUX(I + 1, K);
+      if (I >= 2 && I <= IMM1) sc = cFU[I - 1] * CUX(I - 1, K) + cFD[I - 1] * CUX(I + 1, K);
       else if (!blade) sc = 0.5f * (CUX(2, K) + CUX(IMM1, K));
       else sc = 0.0f;
       // AVG(I,K) is read only by this thread, so the result can replace it
@@ -1223,26 +1384,24 @@ __global__ void k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Param
     __syncthreads();
     { float *t = PL; PL = AV; AV = t; }
     // spanwise
-    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
-      const int I = t % IM + 1, K = t / IM + 1;
+    FOR_PLANE {
       float avg, cur;
       if (K >= 2 && K <= KMM1) {
-        avg = d.FKD[K - 1] * PLX(I, K + 1) + d.FKU[K - 1] * PLX(I, K - 1);
+        avg = cFKD[K - 1] * PLX(I, K + 1) + cFKU[K - 1] * PLX(I, K - 1);
         cur = PLX(I, K) - avg;
       } else if (K == 1) {
-        avg = PLX(I, 2) + 0.5f * d.FKU[0] * (PLX(I, 2) - PLX(I, 3));
+        avg = PLX(I, 2) + 0.5f * cFKU[0] * (PLX(I, 2) - PLX(I, 3));
         cur = 0.0f;
       } else {
-        avg = PLX(I, KMM1) + 0.5f * d.FKU[KM - 1] * (PLX(I, KMM1) - PLX(I, KMM2));
+        avg = PLX(I, KMM1) + 0.5f * cFKU[KM - 1] * (PLX(I, KMM1) - PLX(I, KMM2));
         cur = 0.0f;
       }
       AVX(I, K) = avg;
       CUX(I, K) = cur;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < npl; t += blockDim.x) {
-      const int I = t % IM + 1, K = t / IM + 1;
-      float sc = (K >= 2 && K <= KMM1) ? d.FKU[K - 1] * CUX(I, K - 1) + d.FKD[K - 1] * CUX(I, K + 1) : 0.0f;
+    FOR_PLANE {
+      const float sc = (K >= 2 && K <= KMM1) ? cFKU[K - 1] * CUX(I, K - 1) + cFKD[K - 1] * CUX(I, K + 1) : 0.0f;
       AVX(I, K) = SFT1 * PLX(I, K) + SFT * (AVX(I, K) + FAC_4TH * sc);
     }
     __syncthreads();
@@ -1298,19 +1457,23 @@ __global__ void k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Param
   if (p.IFMIX != 0) {
     bool mp = (J == J2 && NR < g.NROWS) || (NR > 1 && J == J1);
     if (mp) {
-      for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
-        float AVG = 0.0f;
-        for (int I = 1; I <= IMM1; I++) AVG = AVG + d.FP[I - 1] * (PLX(I, K) + PLX(I + 1, K));
-        AVG = 0.5f * AVG;
-        for (int I = 1; I <= IM; I++) PLX(I, K) = AVG;
+      // one wavefront per K line: the terms in parallel, summed in the
+      // reference's order (EXACT) or by the FP64 butterfly
+      for (int K = w + 1; K <= KM; K += 8) {
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int I = 64 * c + lane + 1;
+          v[c] = (I <= IMM1) ? d.FP[I - 1] * (PLX(I, K) + PLX(I + 1, K)) : 0.0f;
+        }
+        const float AVG = 0.5f * line_sum(v, IMM1, p.EXACT);
+        for (int I = lane + 1; I <= IM; I += 64) PLX(I, K) = AVG;
       }
       __syncthreads();
     }
   }
-  for (int t = threadIdx.x; t < npl; t += blockDim.x) {
-    const int I = t % IM + 1, K = t / IM + 1;
-    DST[IX(I, J, K)] = PLX(I, K);
-  }
+  FOR_PLANE DST[IX(I, J, K)] = PLX(I, K);
+#undef FOR_PLANE
 #undef PLX
 #undef AVX
 #undef CUX
