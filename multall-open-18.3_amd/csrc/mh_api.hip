@@ -39,7 +39,12 @@ __global__ void k_mg_apply(Geo, Dev);
 __global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int);
 __global__ void k_damp_apply(Geo, Dev, float);
 __global__ void k_scatter(Geo, Dev, const float *, float *);
-__global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int);
+__global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+__global__ void k_smooth_reg1(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+__global__ void k_smooth_reg2(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+__global__ void k_smooth_reg4(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+__global__ void k_smooth_reg8(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+__global__ void k_smooth_reg16(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
 __global__ void k_fill_int(int *, int, int);
 __global__ void k_rolim_check(Geo, Dev, StepScalars);
 __global__ void k_rolim_fix(Geo, Dev, StepScalars);
@@ -61,10 +66,9 @@ __global__ void k_loss_pitch(Geo, Dev, Params, float, float);
 __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
 __global__ void k_sa_addstore(Geo, Dev, float *, float, int);
-__global__ void k_resid_line(Geo, Dev, float *, Params);
-__global__ void k_mg_part(Geo, Dev, MgFast, int, int);
-__global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, const int *, const int *,
-                            float *, float *, float *);
+__global__ void k_resid_line(Geo, Dev, float *, Params, MgFast, int, int, int);
+__global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, float *, float *,
+                            float *);
 __global__ void k_mg_apply_lines(Geo, Dev, double *);
 __global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
 __global__ void k_damp_scatter1(Geo, Dev, Params, const float *, float *, float, int, int);
@@ -408,11 +412,29 @@ int build_lists(mh_ctx *c) {
              nest(kb1, kb2, dm.km - 1, pp.KR, dm.nkb1, dm.nkb2, k2p, k2l);
   if (L.mgf_ok) {
     MgFast &m = L.mgf;
-    int *t[4];
-    if ((r = up(i2p, &t[0])) || (r = up(i2l, &t[1])) || (r = up(k2p, &t[2])) || (r = up(k2l, &t[3])) ||
+    // each level-2 block's members as offsets into P[(ib, J, kb)], in the
+    // order K block, J, I block
+    std::vector<int> b2p, b2o;
+    const int nj2 = dm.njb2 + 1;
+    for (int kb2 = 0; kb2 < dm.nkb2; kb2++)
+      for (int jb2 = 0; jb2 < nj2; jb2++)
+        for (int ib2 = 0; ib2 < dm.nib2; ib2++) {
+          b2p.push_back((int)b2o.size());
+          for (int kq = k2p[kb2]; kq < k2p[kb2 + 1]; kq++)
+            for (int jq = p2[jb2]; jq < p2[jb2 + 1]; jq++)
+              for (int iq = i2p[ib2]; iq < i2p[ib2 + 1]; iq++)
+                b2o.push_back((i2l[iq] - 1) + dm.nib1 * ((l2[jq] - 1) + dm.jm * (k2l[kq] - 1)));
+        }
+    b2p.push_back((int)b2o.size());
+    int *t[2];
+    if ((r = up(b2p, &t[0])) || (r = up(b2o, &t[1])) ||
         (r = dalloc(c, &m.P, (size_t)dm.nib1 * dm.jm * dm.nkb1)) || (r = dalloc(c, &m.PT, (size_t)dm.jm * dm.nkb1)))
       return r;
-    m.i2_ptr = t[0], m.i2_list = t[1], m.k2_ptr = t[2], m.k2_list = t[3];
+    // level-1 K blocks past the last cell are never written and stay zero
+    if (hipMemsetAsync(m.P, 0, (size_t)dm.nib1 * dm.jm * dm.nkb1 * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(m.PT, 0, (size_t)dm.jm * dm.nkb1 * sizeof(double), c->stream) != hipSuccess)
+      return fail(c, MH_E_HIP, "hipMemsetAsync");
+    m.b2_ptr = t[0], m.b2_off = t[1];
   }
   // the fused line kernels take rows as contiguous J ranges joined at JMIX
   std::vector<int> nrow(dm.jm);
@@ -446,9 +468,13 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   // fast mode without residual smoothing takes the damping line sums from
   // the multigrid update
   const bool fused_apply = !p.EXACT && p.NRSMTH == 0 && damp;
-  LAUNCH(k_resid_line, lgrid, lblock, 0, c->stream, g, d, DIFF, p);
-  LCHK("resid_line");
   const bool mg = !(p.IR <= 1 && p.JR <= 1 && p.KR <= 1);
+  // fast mode: the residual kernel also leaves the multigrid partial sums
+  const bool fastmg = mg && !p.EXACT && L.mgf_ok;
+  const int kg = fastmg ? p.KR : 4;
+  LAUNCH(k_resid_line, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, c->stream, g, d, DIFF, p, L.mgf,
+         kg, p.IR, fastmg ? 1 : 0);
+  LCHK("resid_line");
   if (mg) {
     const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
     if (p.EXACT) {
@@ -458,11 +484,9 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
              g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
       LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr, L.sb_list,
              g.NSBLK, d.SBCHG, 0, 0);
-    } else if (L.mgf_ok) {
-      const int nw = (g.JM - 1) * g.NKB1;
-      LAUNCH(k_mg_part, dim3((nw + 3) / 4), dim3(256), 0, c->stream, g, d, L.mgf, p.IR, p.KR);
+    } else if (fastmg) {
       LAUNCH(k_mg_levels, dim3((n1 + 255) / 256 + (n2 + 3) / 4 + g.NSBLK), dim3(256), 0, c->stream, g, L.mgf,
-             L.b1_ptr, L.b1_list, L.b2_ptr, L.b2_list, L.sb_ptr, L.sb_list, d.B1CHG, d.B2CHG, d.SBCHG);
+             L.b1_ptr, L.b1_list, L.sb_ptr, L.sb_list, d.B1CHG, d.B2CHG, d.SBCHG);
     } else {
       LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list, g.NIB1,
              g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
@@ -529,7 +553,24 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
              s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
-  LAUNCH(k_smooth_plane, dim3(g.JM), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p, s, ncall);
+  // SMOOTH_VAR: planes in registers where IM <= 64 and KM <= 128, the LDS
+  // form elsewhere and on the exit-smoothing planes
+  const int jexit = (!(p.SFEXIT < 0.0001f) && p.NSFEXIT > 0) ? std::max(1, g.JM + 1 - p.NSFEXIT) : g.JM + 1;
+  int jreg = 1;  // first plane for the LDS form
+  if (g.IM <= 64 && g.KM <= 128 && jexit > 1) {
+    const int lines = (g.KM + 7) / 8;
+    const dim3 rg(jexit - 1);
+    const float *src = d.DSC;
+    if (lines <= 1) LAUNCH(k_smooth_reg1, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
+    else if (lines <= 2) LAUNCH(k_smooth_reg2, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
+    else if (lines <= 4) LAUNCH(k_smooth_reg4, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
+    else if (lines <= 8) LAUNCH(k_smooth_reg8, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
+    else LAUNCH(k_smooth_reg16, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
+    jreg = jexit;
+  }
+  if (jreg <= g.JM)
+    LAUNCH(k_smooth_plane, dim3(g.JM - jreg + 1), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
+           s, ncall, jreg);
   LCHK("smooth_plane");
   return MH_OK;
 }

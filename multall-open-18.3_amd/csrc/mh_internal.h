@@ -131,10 +131,10 @@ inline size_t scatter_lds_bytes(int imm1) {
 }
 
 // Fast-mode multigrid sums: per-(level-1 I block, J, level-1 K block) sums
-// P and per-(J, level-1 K block) line totals PT, with the level-1 I (K)
-// blocks of each level-2 I (K) block.
+// P and per-(J, level-1 K block) line totals PT, with each level-2 block's
+// members as offsets into P.
 struct MgFast {
-  const int *i2_ptr = nullptr, *i2_list = nullptr, *k2_ptr = nullptr, *k2_list = nullptr;
+  const int *b2_ptr = nullptr, *b2_off = nullptr;  // level-2 block -> offsets into P of its members
   double *P = nullptr, *PT = nullptr;
 };
 

@@ -54,6 +54,18 @@ __device__ __forceinline__ float line_sum(const float (&v)[4], int n, int exact)
 // scalar unit rather than per lane.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// FP32 transcendentals as the FP64 function rounded once to FP32: the
+// correctly rounded result except at rounding ties, which is what the host
+// libm's FP32 functions (the reference's) return for all but rare inputs.
+// A 1-ulp difference can be amplified downstream: the inlet temperature's
+// difference from TO1 gives the inlet speed, for one.
+__device__ __forceinline__ float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ float log_r(float x) { return (float)log((double)x); }
+__device__ __forceinline__ float sin_r(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cos_r(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float asin_r(float x) { return (float)asin((double)x); }
+__device__ __forceinline__ float atan2_r(float y, float x) { return (float)atan2((double)y, (double)x); }
+
 // XCD-aware workgroup order.  Workgroups are dispatched round-robin over the
 // 8 XCDs, each with its own L2; this bijection on [0, n) gives XCD x the
 // contiguous range of logical ids it would otherwise share with the other
@@ -313,6 +325,7 @@ __global__ void k_flux_radial(Geo g, Dev d) {
 
 // inlet pressure and static temperature of node (I,1,K) from the state
 // before the update, mf:4566-4606
+
 __device__ __forceinline__ void inlet_pt(const Geo &g, const Dev &d, const Params &p, const StepScalars &s,
                                          const float *PAVG2, const float *PAVG3, int I, int K, float &P1,
                                          float &TSTATIC) {
@@ -321,7 +334,7 @@ __device__ __forceinline__ void inlet_pt(const Geo &g, const Dev &d, const Param
   float RO_STAG = PO1 / p.RGAS / TO1;
   float RO_IN = d.RO[n1];
   if (RO_IN > 0.9999f * RO_STAG) RO_IN = 0.9999f * RO_STAG;
-  float PNEW = PO1 * powf(RO_IN / RO_STAG, p.GA);
+  float PNEW = PO1 * pow_r(RO_IN / RO_STAG, p.GA);
   P1 = d.P[n1];
   if (p.IN_PRESS == 0) P1 = s.RFIN1 * P1 + p.RFIN * PNEW;
   if (abs(p.IN_PRESS) == 1) P1 = s.RFIN1 * P1 + s.RFIN2 * d.P[IX(I, 2, K)] - p.RFIN * d.P[IX(I, 3, K)];
@@ -331,7 +344,7 @@ __device__ __forceinline__ void inlet_pt(const Geo &g, const Dev &d, const Param
   const float plim = d.SC[SC_PLIM + K - 1];
   if (P1 > plim) P1 = 0.9f * P1 + 0.1f * plim;
   if (P1 > 0.99999f * PO1) P1 = PO1 * 0.99999f;
-  TSTATIC = TO1 * powf(P1 * d.RPO1[K - 1], p.FGA);
+  TSTATIC = TO1 * pow_r(P1 * d.RPO1[K - 1], p.FGA);
 }
 
 __global__ void k_bc_sums(Geo g, Dev d, Params p, StepScalars s) {
@@ -458,7 +471,7 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
       // RO(IMID,1,KMID) as its (K, I) loop has left it: already updated
       // for (K,I) at or after (KMID,IMID), k_bc_sums kept both values
       const float FST = d.FSTURB[0], VL = d.VISC_LAM[IX(I, 2, K)];
-      const float TRANSVISIN = (FST < 1.0f) ? VL * 4.35f * powf(FST, 0.25f) : VL * (3.5f + FST * 0.85f);
+      const float TRANSVISIN = (FST < 1.0f) ? VL * 4.35f * pow_r(FST, 0.25f) : VL * (3.5f + FST * 0.85f);
       const bool after = K > p.KMID || (K == p.KMID && I >= p.IMID);
       const float roref = after ? d.BCW[3 * g.KM + 1] : d.BCW[3 * g.KM];
       d.TRANS_DYN_VIS[n1] = TRANSVISIN;
@@ -785,7 +798,8 @@ __device__ __forceinline__ int mp_role(const Dev &d, int J, int &jx) {
 // balance (mf:5503-5510, 5560-5570), and the first pitchwise STORE average
 // on the mixing-plane lines (mf:5578-5607).
 __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float *DIFF, const Params &p, int J, int K,
-                                              int lane, float RATPITCH, bool tipbal, int role, int jx) {
+                                              int lane, float RATPITCH, bool tipbal, int role, int jx,
+                                              float (&out)[4]) {
   const int IM = g.IM, IMM1 = g.IMM1;
   const float tavg = tipbal ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
   const float FLOWDIRN = -d.FLOWX[IX(p.IMID, jx, K)];
@@ -839,17 +853,25 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float 
   for (int c = 0; c < 4; c++) {
     const int I = 64 * c + lane + 1;
     if (I <= IMM1) d.STORE[IX(I, J, K)] = st[c];
+    out[c] = st[c];
   }
 }
 
-// TSTEP's residual (mf:5503-5607).  One wavefront per (J, 4 consecutive K
-// lines), four wavefronts along J per workgroup.  Lines away from the
-// mixing planes take the plain flux sum with all four lines' loads in
-// flight together; the mixing-plane lines go through resid_mp_line.
-__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p) {
+// TSTEP's residual (mf:5503-5607).  One wavefront per (J, kg consecutive
+// K lines), four wavefronts along J per workgroup.  Lines away from the
+// mixing planes take the plain flux sum with all lines' loads in flight
+// together; the mixing-plane lines go through resid_mp_line.  With part
+// set, kg = KR and the wavefront's lines are one level-1 multigrid K block:
+// it also leaves the fast-mode multigrid partial sums of the new STORE
+// (mf:5621-5640), FP64 per level-1 I block in P[(ib, J, kb)] and per line
+// group in PT[(J, kb)], lane columns summed over the block's K lines first.
+#define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
+__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p, MgFast m, int kg, int ir,
+                                                    int part) {
+  __shared__ double sh[4][256];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-  const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
+  const int J = (bl % gridDim.x) * 4 + w + 2, kb = bl / gridDim.x, kq = kb * kg + 1;
   if (J > g.JM) return;
   const int IM = g.IM, IMM1 = g.IMM1;
   const int NR = d.NROW[J - 1];
@@ -857,101 +879,93 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
   const int kts = d.KTIPS[NR - 1], kte = d.KTIPE[NR - 1];
   int jx = 0;
   const int role = g.NROWS > 1 ? mp_role(d, J, jx) : 0;
-  const int nk = min(4, g.KMM1 - kq + 1);
-  if (role) {
-    for (int kk = 0; kk < nk; kk++) {
-      const int K = kq + kk;
-      resid_mp_line(g, d, DIFF, p, J, K, lane, RATPITCH, kts > 0 && K >= kts && K <= kte - 1, role, jx);
-    }
-    return;
-  }
-  bool tb[4];
-  float tavg[4];
-#pragma unroll
-  for (int kk = 0; kk < 4; kk++) {
-    const int K = kq + kk;
-    tb[kk] = kk < nk && kts > 0 && K >= kts && K <= kte - 1;
-    tavg[kk] = tb[kk] ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
-  }
-  // all loads of the four lines before any store (the arrays may alias as
-  // far as the compiler knows, so a store would hold back later loads)
-  for (int I = lane + 1; I <= IMM1; I += 64) {
-    float rf[5], xf[4], xm[4], tf[4], tp[4], sr[4], df[4];
-#pragma unroll
-    for (int kk = 0; kk <= 4; kk++) rf[kk] = d.RFLUX[IX(I, J, min(kq + kk, g.KM))];
-#pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-      const unsigned n = IX(I, J, min(kq + kk, g.KMM1));
-      xf[kk] = d.XFLUX[n];
-      xm[kk] = d.XFLUX[n - IM];
-      tf[kk] = d.TFLUX[n];
-      tp[kk] = d.TFLUX[n + 1];
-      sr[kk] = d.SOURCE[n];
-      df[kk] = DIFF[n];
-    }
-#pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-      if (kk >= nk) break;
-      float TF = tf[kk], TFP = tp[kk];
-      if (tb[kk]) {
-        if (I == 1) TF = tavg[kk];
-        if (I + 1 == IM) TFP = tavg[kk];
-      }
-      const float DELTA = xf[kk] - xm[kk] * RATPITCH + rf[kk] - rf[kk + 1] + TF - TFP - sr[kk];
-      const unsigned n = IX(I, J, kq + kk);
-      d.STORE[n] = p.F1 * DELTA + p.F2 * df[kk];
-      DIFF[n] = DELTA + p.F3 * df[kk];
-    }
-  }
-}
-
-// Fast-mode multigrid sums (mf:5621-5640) in two passes.  Pass 1: one
-// wavefront per (J, level-1 K block); each lane sums its I column over the
-// block's K planes (FP64), then the lanes of each level-1 I block are added
-// into P[(ib, J, kb)] and all lanes into PT[(J, kb)].  Needs IR, KR <= 4.
-#define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
-__global__ void __launch_bounds__(256) k_mg_part(Geo g, Dev d, MgFast m, int ir, int kr) {
-  __shared__ double sh[4][256];
-  const int lane = threadIdx.x & 63, w = wave_id();
-  const int wid = blockIdx.x * 4 + w;
-  const int J = wid % (g.JM - 1) + 2, kb = wid / (g.JM - 1) + 1;
-  if (kb > g.NKB1) return;  // whole wavefronts only: no barrier below spans wavefronts
-  const int k0 = (kb - 1) * kr + 1, k1 = min(kb * kr, g.KMM1);
-  float x[4][4];
+  const int nk = min(kg, g.KMM1 - kq + 1);
+  float st[4][4];  // [line][I chunk]: the new STORE, zero off the grid
 #pragma unroll
   for (int kk = 0; kk < 4; kk++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int I = 64 * c + lane + 1, K = k0 + kk;
-      x[kk][c] = (K <= k1 && I <= g.IMM1) ? d.STORE[IX(I, J, K)] : 0.0f;
+    for (int c = 0; c < 4; c++) st[kk][c] = 0.0f;
+  if (role) {
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int K = kq + kk;
+      if (kk < nk) resid_mp_line(g, d, DIFF, p, J, K, lane, RATPITCH, kts > 0 && K >= kts && K <= kte - 1, role, jx,
+                                 st[kk]);
     }
+  } else {
+    bool tb[4];
+    float tavg[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int K = kq + kk;
+      tb[kk] = kk < nk && kts > 0 && K >= kts && K <= kte - 1;
+      tavg[kk] = tb[kk] ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      if (64 * c >= IMM1) break;
+      if (I > IMM1) continue;
+      // all loads of the lines before any store (the arrays may alias as
+      // far as the compiler knows, so a store would hold back later loads)
+      float rf[5], xf[4], xm[4], tf[4], tp[4], sr[4], df[4];
+#pragma unroll
+      for (int kk = 0; kk <= 4; kk++) rf[kk] = d.RFLUX[IX(I, J, min(kq + kk, g.KM))];
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) {
+        const unsigned n = IX(I, J, min(kq + kk, g.KMM1));
+        xf[kk] = d.XFLUX[n];
+        xm[kk] = d.XFLUX[n - IM];
+        tf[kk] = d.TFLUX[n];
+        tp[kk] = d.TFLUX[n + 1];
+        sr[kk] = d.SOURCE[n];
+        df[kk] = DIFF[n];
+      }
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) {
+        if (kk >= nk) break;
+        float TF = tf[kk], TFP = tp[kk];
+        if (tb[kk]) {
+          if (I == 1) TF = tavg[kk];
+          if (I + 1 == IM) TFP = tavg[kk];
+        }
+        const float DELTA = xf[kk] - xm[kk] * RATPITCH + rf[kk] - rf[kk + 1] + TF - TFP - sr[kk];
+        const unsigned n = IX(I, J, kq + kk);
+        const float v = p.F1 * DELTA + p.F2 * df[kk];
+        d.STORE[n] = v;
+        DIFF[n] = DELTA + p.F3 * df[kk];
+        st[kk][c] = v;
+      }
+    }
+  }
+  if (!part) return;
   double tot = 0.0;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    const double a = (((double)x[0][c] + (double)x[1][c]) + (double)x[2][c]) + (double)x[3][c];
+    const double a = (((double)st[0][c] + (double)st[1][c]) + (double)st[2][c]) + (double)st[3][c];
     sh[w][64 * c + lane] = a;
     tot += a;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-  if (lane == 0) m.PT[JKX(J, kb)] = tot;
+  if (lane == 0) m.PT[JKX(J, kb + 1)] = tot;
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   for (int ib = lane + 1; ib <= g.NIB1; ib += 64) {
-    const int i0 = (ib - 1) * ir, i1 = min(ib * ir, g.IMM1);
-    double s = 0.0;
-    for (int i = i0; i < i1; i++) s += sh[w][i];
-    m.P[PX(ib, J, kb)] = s;
+    const int i0 = (ib - 1) * ir, i1 = min(ib * ir, IMM1);
+    double sum = 0.0;
+    for (int i = i0; i < i1; i++) sum += sh[w][i];
+    m.P[PX(ib, J, kb + 1)] = sum;
   }
 }
 
-// Pass 2, by block range: one thread per level-1 block (its J members from
+// Fast-mode multigrid sums (mf:5621-5640), pass 2 (pass 1 is in
+// k_resid_line), by block range: one thread per level-1 block (its J members from
 // the level-1 list), one wavefront per level-2 block (its level-1 I and K
 // blocks and J members), one workgroup per superblock (its J members, all
 // level-1 K blocks); fixed-order trees throughout.
-__global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b1p, const int *b1l, const int *b2p,
-                                                   const int *b2l, const int *sbp, const int *sbl, float *B1, float *B2,
-                                                   float *SB) {
+__global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b1p, const int *b1l, const int *sbp,
+                                                   const int *sbl, float *B1, float *B2, float *SB) {
   const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
   const int nblk1 = (n1 + 255) / 256, nblk2 = (n2 + 3) / 4;
   const int bx = blockIdx.x;
@@ -973,15 +987,14 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b
     const int lane = threadIdx.x & 63;
     const int u = (bx - nblk1) * 4 + wave_id();
     if (u >= n2) return;
-    const int nj = g.NJB2 + 1;
-    const int ib = u % g.NIB2, jb = (u / g.NIB2) % nj, kb = u / (g.NIB2 * nj);
-    const int ia = m.i2_ptr[ib], ni = m.i2_ptr[ib + 1] - ia, ka = m.k2_ptr[kb], nk = m.k2_ptr[kb + 1] - ka;
-    const int ja = b2p[jb], njj = b2p[jb + 1] - ja;
-    const int tot = ni * njj * nk;
+    // members: host-built P offsets of the block's (level-1 I block, J,
+    // level-1 K block) partial sums
+    const int q0 = m.b2_ptr[u], q1 = m.b2_ptr[u + 1];
     double s = 0.0;
-    for (int c = lane; c < tot; c += 64) {
-      const int ii = c % ni, r = c / ni;
-      s += m.P[PX(m.i2_list[ia + ii], b2l[ja + r % njj], m.k2_list[ka + r / njj])];
+    for (int q = q0 + lane; q < q1; q += 128) {
+      const double a = m.P[m.b2_off[q]];
+      const double b = (q + 64 < q1) ? m.P[m.b2_off[q + 64]] : 0.0;
+      s += a + b;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -1065,10 +1078,11 @@ __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *re
 }
 
 // per-row average |change| from the (J,K) line sums (mf:5677-5693): one
-// workgroup per row (a contiguous J range), fixed-order tree
+// workgroup of 16 wavefronts per row (a contiguous J range); each lane sums
+// its share, then fixed-order butterflies within and across wavefronts
 __global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *red, const int *rptr, const int *rlist) {
-  __shared__ double sh[1024];
-  const int r = blockIdx.x, t = threadIdx.x;
+  __shared__ double sh[16];
+  const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, w = wave_id();
   const int q0 = rptr[r], nj = rptr[r + 1] - q0, ja = rlist[q0], tot = nj * g.KMM1;
   double s = 0.0;
   for (int c0 = 0; c0 < tot; c0 += 4096) {
@@ -1080,15 +1094,18 @@ __global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *re
     }
     s += ((v[0] + v[1]) + v[2]) + v[3];
   }
-  sh[t] = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) sh[w] = s;
   __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (t < off) sh[t] += sh[t + off];
-    __syncthreads();
-  }
-  if (t == 0) {
-    const float f = (float)sh[0];
-    d.SC[SC_AVG_CHG + r] = f / (float)(g.IMM1 * g.KMM1 * nj);
+  if (w == 0) {
+    double x = lane < 16 ? sh[lane] : 0.0;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) {
+      const float f = (float)x;
+      d.SC[SC_AVG_CHG + r] = f / (float)(g.IMM1 * g.KMM1 * nj);
+    }
   }
 }
 
@@ -1268,9 +1285,9 @@ __global__ void __launch_bounds__(256) k_damp_scatter4(Geo g, Dev d, Params p, c
 // Writes the finished plane to DST.  Dynamic LDS: 3*IM*KM + 2*(IM+KM)
 // floats.
 __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Params p,
-                                                         StepScalars s, int ncall) {
+                                                         StepScalars s, int ncall, int j0) {
   extern __shared__ float lds[];
-  const int J = xcd_block(blockIdx.x, gridDim.x) + 1;
+  const int J = xcd_block(blockIdx.x, gridDim.x) + j0;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
   const int npl = IM * KM;
@@ -1479,6 +1496,213 @@ __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const flo
 #undef CUX
 }
 
+// SMOOTH_VAR and TSTEP's D conditions as k_smooth_plane, for IM <= 64 and
+// planes without exit smoothing, with the plane held in registers: wavefront
+// w owns K lines w*L+1 .. w*L+L, lane l the point I = l+1.  The pitchwise
+// stencil and the line conditions use lane shuffles; only the spanwise
+// stencil goes through LDS (the whole plane, one barrier), and the corner
+// update reads lines 2 and KM-1 from LDS.
+template <int L>
+__device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const float *SRC, float *DST, const Params &p,
+                                           const StepScalars &s, int ncall, int j0) {
+  __shared__ float PLs[128 * 64];
+  __shared__ float C2[2][64];
+  const int J = xcd_block(blockIdx.x, gridDim.x) + j0;
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
+  const int I = lane + 1, Ic = min(I, IM);
+  const int kbase = w * L + 1;
+  const float SFX = s.SFX, SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
+  const float SFXB = p.FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
+  const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
+  const float SFT = s.SFT, SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
+  const int NR = d.NROW[J - 1];
+  const int J1 = d.JSTART[NR - 1], J2 = d.JMIX[NR - 1], JLED = d.JLED[NR - 1];
+  const bool smx = !(SFX < 0.0001f);
+  int cs = 0;  // 0 copy, 1 interior, 2 J1+2, 3 J1+1, 4 J2-2, 5 J2-1
+  if (smx) {
+    if (J >= J1 + 3 && J <= J2 - 3) cs = 1;
+    if (J == J1 + 2) cs = 2;
+    if (J == J1 + 1) cs = 3;
+    if (J == J2 - 2) cs = 4;
+    if (J == J2 - 1) cs = 5;
+  }
+  // streamwise smoothing, mf:15369-15445 (loads clamped into the grid)
+  float v[L];
+  auto stream = [&](auto f) {
+#pragma unroll
+    for (int l = 0; l < L; l++) v[l] = f(min(kbase + l, KM));
+  };
+#define S(JJ) SRC[IX(Ic, (JJ), K)]
+  switch (cs) {
+    case 1:
+      stream([&](int K) {
+        const float AVERG = 0.5f * (S(J + 1) + S(J - 1));
+        const float CURV = AVERG - 0.5f * S(J) - 0.25f * (S(J - 2) + S(J + 2));
+        return SFX1 * S(J) + SFX * (AVERG + FAC_4TH * CURV);
+      });
+      break;
+    case 2: stream([&](int K) { return SFXB1 * S(J1 + 2) + SFXBH * (S(J1 + 3) + S(J1 + 1)); }); break;
+    case 3:
+      stream([&](int K) {
+        return SFXB1 * S(J1 + 1) + SFXB * (1.25f * S(J1 + 2) + 0.5f * S(J1 + 3) - 0.75f * S(J1 + 4));
+      });
+      break;
+    case 4: stream([&](int K) { return SFXB1 * S(J2 - 2) + SFXBH * (S(J2 - 1) + S(J2 - 3)); }); break;
+    case 5:
+      stream([&](int K) {
+        return SFXB1 * S(J2 - 1) + SFXB * (1.25f * S(J2 - 2) + 0.5f * S(J2 - 3) - 0.75f * S(J2 - 4));
+      });
+      break;
+    default: stream([&](int K) { return S(J); }); break;
+  }
+  // leading-edge points I = 1 and IM (mf:15417-15434)
+  if (smx && (J == JLED - 2 || J == JLED - 1 || J == JLED) && (I == 1 || I == IM)) {
+    const int Io = (I == 1) ? IM : 1;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const int K = min(kbase + l, KM);
+      v[l] = (J == JLED) ? SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)])
+                         : SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
+    }
+  }
+#undef S
+  // pitchwise and spanwise smoothing and corners, mf:15460-15541 (J >= 2)
+  if (J >= 2 && !(SFT < 0.0001f)) {
+    const bool blade = (d.IND[J - 1] == 1 || d.INDLE[J - 1] == 1);
+    const float FDI = d.FD[Ic - 1], FUI = d.FU[Ic - 1], FU1 = d.FU[0], FUM = d.FU[IM - 1];
+    const bool mid = I >= 2 && I <= IMM1;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const float x = v[l];
+      const float xr = __shfl(x, lane + 1, 64), xl = __shfl(x, lane - 1, 64);
+      const float x1 = __shfl(x, 0, 64), x2 = __shfl(x, 1, 64), x3 = __shfl(x, 2, 64);
+      const float xm1 = __shfl(x, IMM1 - 1, 64), xm2 = __shfl(x, IMM2 - 1, 64);
+      float avg, cur;
+      if (mid) {
+        avg = FDI * xr + FUI * xl;
+        cur = x - avg;
+      } else if (!blade) {
+        avg = 0.5f * (x2 + xm1);
+        cur = x1 - avg;
+      } else {
+        avg = (I == 1) ? x2 + FU1 * (x2 - x3) : xm1 + FUM * (xm1 - xm2);
+        cur = 0.0f;
+      }
+      const float cr = __shfl(cur, lane + 1, 64), cl = __shfl(cur, lane - 1, 64);
+      const float c2 = __shfl(cur, 1, 64), cm1 = __shfl(cur, IMM1 - 1, 64);
+      float sc;
+      if (mid) sc = FUI * cl + FDI * cr;
+      else if (!blade) sc = 0.5f * (c2 + cm1);
+      else sc = 0.0f;
+      v[l] = SFT1 * x + SFT * (avg + FAC_4TH * sc);
+    }
+    // spanwise, from the whole pitchwise-smoothed plane in LDS
+#pragma unroll
+    for (int l = 0; l < L; l++)
+      if (kbase + l <= KM && I <= IM) PLs[(kbase + l - 1) * 64 + lane] = v[l];
+    __syncthreads();
+#define PK(KK) PLs[((KK) - 1) * 64 + lane]
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const int K = kbase + l;
+      if (K > KM) break;
+      const float x = v[l];
+      float avg;
+      float sc = 0.0f;
+      if (K >= 2 && K <= KMM1) {
+        avg = d.FKD[K - 1] * PK(K + 1) + d.FKU[K - 1] * PK(K - 1);
+        // CURVE at K-1 and K+1 (zero on the end lines)
+        float cm = 0.0f, cp = 0.0f;
+        if (K - 1 >= 2) cm = PK(K - 1) - (d.FKD[K - 2] * PK(K) + d.FKU[K - 2] * PK(K - 2));
+        if (K + 1 <= KMM1) cp = PK(K + 1) - (d.FKD[K] * PK(K + 2) + d.FKU[K] * PK(K));
+        sc = d.FKU[K - 1] * cm + d.FKD[K - 1] * cp;
+      } else if (K == 1) {
+        avg = PK(2) + 0.5f * d.FKU[0] * (PK(2) - PK(3));
+      } else {
+        avg = PK(KMM1) + 0.5f * d.FKU[KM - 1] * (PK(KMM1) - PK(KMM2));
+      }
+      v[l] = SFT1 * x + SFT * (avg + FAC_4TH * sc);
+    }
+#undef PK
+    // corners: lines 2 and KM-1 to LDS, then the owners of lines 1 and KM
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const int K = kbase + l;
+      if (K == 2) C2[0][lane] = v[l];
+      if (K == KMM1) C2[1][lane] = v[l];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const int K = kbase + l;
+      if (K != 1 && K != KM) continue;
+      const float x = v[l];
+      const float xn = __shfl(x, lane + 1, 64), xp = __shfl(x, lane - 1, 64);
+      const float y = (K == 1) ? C2[0][lane] : C2[1][lane];
+      if (K == 1) {
+        if (I == 1) v[l] = SFT1 * x + SFTH * (xn + y);
+        if (I == IM) v[l] = SFT1 * x + SFTH * (xp + y);
+      } else {
+        if (I == 1) v[l] = SFT1 * x + SFTH * (y + xn);
+        if (I == IM) v[l] = SFT1 * x + SFTH * (xp + y);
+      }
+    }
+  }
+  // periodicity off the blade and across the tip gap, mf:5858-5891
+  const int ILAST = (ncall == 2) ? IMM1 : IM;
+  const bool per = !((d.IND[J - 1] == 1) || (d.INDLE[J - 1] == 1));
+  int K1 = 1, K2 = 0;
+  if (d.KTIPS[NR - 1] > 0) {
+    K1 = d.KTIPS[NR - 1];
+    K2 = d.KTIPE[NR - 1];
+    if (K1 == 1) K2 = K2 - 1;
+    if (K2 == KM) K1 = K1 + 1;
+  }
+#pragma unroll
+  for (int l = 0; l < L; l++) {
+    const int K = kbase + l;
+    float x = v[l];
+    if (per) {
+      const float a = 0.5f * (__shfl(x, 0, 64) + __shfl(x, ILAST - 1, 64));
+      if (I == 1 || I == ILAST) x = a;
+    }
+    if (K >= K1 && K <= K2) {
+      const float a = 0.5f * (__shfl(x, 0, 64) + __shfl(x, ILAST - 1, 64));
+      if (I == 1 || I == ILAST) x = a;
+    }
+    v[l] = x;
+  }
+  // pitchwise-uniform D on both faces of a mixing plane, mf:5901-5926
+  if (p.IFMIX != 0 && ((J == J2 && NR < g.NROWS) || (NR > 1 && J == J1))) {
+    const float FPI = (I <= IMM1) ? d.FP[I - 1] : 0.0f;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const float x = v[l];
+      const float xr = __shfl(x, lane + 1, 64);
+      float t[4] = {(I <= IMM1) ? FPI * (x + xr) : 0.0f, 0.f, 0.f, 0.f};
+      v[l] = 0.5f * line_sum(t, IMM1, p.EXACT);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < L; l++) {
+    const int K = kbase + l;
+    if (K <= KM && I <= IM) DST[IX(I, J, K)] = v[l];
+  }
+}
+
+#define SMOOTH_REG(L)                                                                                         \
+  __global__ void __launch_bounds__(512) k_smooth_reg##L(Geo g, Dev d, const float *SRC, float *DST, Params p, \
+                                                         StepScalars s, int ncall, int j0) {                   \
+    smooth_reg<L>(g, d, SRC, DST, p, s, ncall, j0);                                                           \
+  }
+SMOOTH_REG(1)
+SMOOTH_REG(2)
+SMOOTH_REG(4)
+SMOOTH_REG(8)
+SMOOTH_REG(16)
+#undef SMOOTH_REG
+
 // ---------------------------------------------------------------------------
 // LOSS, mf:8058-9066: mixing-length eddy viscosity and log-law wall friction
 // turned into the body forces XFORCE, RFORCE, TFORCE, QSOURCE by a thin
@@ -1515,7 +1739,7 @@ __global__ void k_loss_vislam(Geo g, Dev d, Params p) {
 // roughness correction when ROUGH > 1e-7, at least laminar).  RO and WB are
 // the density and speed the wall stress is formed with.
 __device__ float loss_cf(float RE, float PERP, float ROUGH, float RO, float WB, float VISLAM) {
-  const float RELOG = 1.0f / logf(RE);
+  const float RELOG = 1.0f / log_r(RE);
   float CF;
   if (ROUGH > 1.0e-7f) {
     if (ROUGH > PERP) ROUGH = PERP;
@@ -1603,7 +1827,7 @@ __global__ void k_loss_span(Geo g, Dev d, Params p, float RF_VIS, float FMIXUP) 
   float VISLAM = d.SC[SC_VISLAM], FTCOND = d.SC[SC_FTCOND];
   if (p.REYNO < 0.0001f) {
     const LossQ qk = loss_q_span(g, d, I, J, KMID, WREL);
-    VISLAM = (fabsf(p.REYNO) / 100000.0f) * powf(qk.t / 288.0f, 0.62f);
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * pow_r(qk.t / 288.0f, 0.62f);
     const float TCOND = p.CP * VISLAM / p.PRANDTL;
     FTCOND = TCOND / VISLAM;
   }
@@ -1758,7 +1982,7 @@ __global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float
   __syncthreads();
   float VISLAM = d.SC[SC_VISLAM], FTCOND = d.SC[SC_FTCOND];
   if (p.REYNO < 0.0001f) {
-    VISLAM = (fabsf(p.REYNO) / 100000.0f) * powf(TA[IMID] / 288.0f, 0.62f);
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * pow_r(TA[IMID] / 288.0f, 0.62f);
     const float TCOND = p.CP * VISLAM / p.PRANDTL;
     FTCOND = TCOND / VISLAM;
   }
@@ -2148,7 +2372,7 @@ __global__ void k_sa_visc(Geo g, Dev d, Params p, float FMIXUP, int NSTEP) {
     const float *T = d.T_STATIC;
     const float TSAVG = 0.125f * (T[IX(I, J, K)] + T[IX(IP1, J, K)] + T[IX(IP1, J, KP1)] + T[IX(I, J, KP1)] +
                                   T[IX(I, JM1, K)] + T[IX(IP1, JM1, K)] + T[IX(IP1, JM1, KP1)] + T[IX(I, JM1, KP1)]);
-    VISLAM = (fabsf(p.REYNO) / 100000.0f) * powf(TSAVG / 288.0f, 0.62f);
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * pow_r(TSAVG / 288.0f, 0.62f);
   }
   float VISTOT = VISLAM * (1 + d.FST_RAT[n]) + TURBVIS_WALL;
   float VISLIM = p.TURBVIS_LIM * VISLAM;
@@ -2157,7 +2381,7 @@ __global__ void k_sa_visc(Geo g, Dev d, Params p, float FMIXUP, int NSTEP) {
   d.DYN_VIS_ML[n] = (VISTOT - VISLAM);
   if (p.IF_RESTART == 0 && NSTEP == 1) {
     const float FST = d.FSTURB[0];
-    const float TRANSVISIN = (FST < 1.0f) ? 4.35f * VISLAM * powf(FST, 0.25f) : VISLAM * (3.5f + FST * 0.85f);
+    const float TRANSVISIN = (FST < 1.0f) ? 4.35f * VISLAM * pow_r(FST, 0.25f) : VISLAM * (3.5f + FST * 0.85f);
     d.TRANS_DYN_VIS[n] = TRANSVISIN;
   }
   if (NSTEP == 1) d.TRANS_KVIS[n] = d.TRANS_DYN_VIS[n] / ROAVG;
@@ -2306,7 +2530,7 @@ __global__ void k_sa_source(Geo g, Dev d, Params p, float RF_VIS) {
   if (TAU < 0.001f) TAU = 0.001f;
   const float G = TAU + CW2 * (TAU * TAU * TAU * TAU * TAU * TAU - TAU);
   const float G6CW6 = G * G * G * G * G * G / 64.0f;
-  const float FW = G * powf((1 + 0.015625f) / (1 + G6CW6), 0.166667f);
+  const float FW = G * pow_r((1 + 0.015625f) / (1 + G6CW6), 0.166667f);
   const float ST3 = FST3 * FW * TVIS * TVIS / DW;
   const float ST_MIXL = FSTMIX * (d.DYN_VIS_ML[n] - d.VISC_TURB[n]) / d.STEP[n];
   float ts = RF_VIS1 * d.T_SOURCE[n] + RF_VIS * (ROAVG * (ST0 + ST1 + ST2 - ST3) * d.VOL[n] + ST_MIXL);
@@ -2350,7 +2574,7 @@ __global__ void k_sa_source(Geo g, Dev d, Params p, float RF_VIS) {
 // wall friction of SPAL_LOSS's wall cells (sums of four node values: the
 // 1/16 and 1/128 factors), mf:14221-14268
 __device__ float sa_cf(float RE, float DPERP, float ROUGH, float ROAVG4, float WAVG, float VISLAM) {
-  const float RELOG = 1.f / logf(RE);
+  const float RELOG = 1.f / log_r(RE);
   float CF;
   if (ROUGH > 1.0e-7f) {
     if (ROUGH > DPERP) ROUGH = DPERP;
@@ -2717,7 +2941,7 @@ __device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Param
   float GM1 = p.GA - 1.0f;
   float TRATIO = 1.0f + p.MACHLIM * p.MACHLIM * 0.5f * GM1;
   float RFGM1 = 1.0f / GM1;
-  float RORATIO = powf(TRATIO, RFGM1);
+  float RORATIO = pow_r(TRATIO, RFGM1);
   const int JREF = d.JLE[NR - 1];
   const size_t n = IX(IMID, JREF, KMID);
   float TREF = d.T_STATIC[n];
@@ -2726,7 +2950,7 @@ __device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Param
   float TLIMIT = TOREL_REF / TRATIO;
   if (TLIMIT > TCOOL_MIN) TLIMIT = TCOOL_MIN;
   vlimit = sqrtf(2 * p.CP * (TOREL_REF - TLIMIT));
-  float ROSTAG_REL = d.RO[n] * powf(TOREL_REF / TREF, RFGM1);
+  float ROSTAG_REL = d.RO[n] * pow_r(TOREL_REF / TREF, RFGM1);
   rolimit = ROSTAG_REL / RORATIO;
 }
 
@@ -2799,8 +3023,8 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
   const float WTMIX = VTMIX - UJP1;
   const float WMIXSQ = VMSQ + WTMIX * WTMIX;
   const float WMIX = sqrtf(WMIXSQ);
-  const float ALPHAMIX = asinf(WTMIX / WMIX);
-  const float GAMAMIX = atan2f(VRMIX, VXMIX);
+  const float ALPHAMIX = asin_r(WTMIX / WMIX);
+  const float GAMAMIX = atan2_r(VRMIX, VXMIX);
   const float TMIX = (ROEMIX / ROMIX - 0.5f * VABSQ) / p.CV;
   const float ROTHALPY = p.CP * TMIX + 0.5f * (WMIXSQ - UJP1 * UJP1);
   const float GAGAS = p.GA;
@@ -2828,10 +3052,10 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
     float ROVMSQJP3 = ROVXJP3 * ROVXJP3 + ROVRJP3 * ROVRJP3;
     float ROWRELSQJP3 = ROVMSQJP3 + ROWTJP3 * ROWTJP3;
     float ROWRELJP3 = sqrtf(ROWRELSQJP3);
-    float ALPHAJP3 = asinf(ROWTJP3 / ROWRELJP3);
-    float GAMAJP3 = atan2f(ROVRJP3, ROVXJP3);
+    float ALPHAJP3 = asin_r(ROWTJP3 / ROWRELJP3);
+    float GAMAJP3 = atan2_r(ROVRJP3, ROVXJP3);
     float PSTAT = ROJP2 * p.RGAS * T_ACT_JP2;
-    float TISENT_JP2 = TMIX * powf(PSTAT / PMIX, p.FGA);
+    float TISENT_JP2 = TMIX * pow_r(PSTAT / PMIX, p.FGA);
     float WIS_SQ_JP2 = 2 * (ROTHALPY - p.CP * TISENT_JP2 + 0.5f * UNOWJP2 * UNOWJP2);
     float ROISENT_JP2 = PSTAT / p.RGAS / TISENT_JP2;
     ROISENT[I - 1] = ROISENT_JP2;
@@ -2884,10 +3108,10 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
   for (int I = lane + 1; I <= IM; I += 64) {
     const size_t n2 = IX(I, JP2, K);
     float RO_JP2 = ROISENT[I - 1];
-    float WT_JP2 = WISV[I - 1] * sinf(ALPHAA[I - 1]);
-    float VM_JP2 = WISV[I - 1] * cosf(ALPHAA[I - 1]);
-    float VX_JP2 = VM_JP2 * cosf(GAMA[I - 1]);
-    float VR_JP2 = VM_JP2 * sinf(GAMA[I - 1]);
+    float WT_JP2 = WISV[I - 1] * sin_r(ALPHAA[I - 1]);
+    float VM_JP2 = WISV[I - 1] * cos_r(ALPHAA[I - 1]);
+    float VX_JP2 = VM_JP2 * cos_r(GAMA[I - 1]);
+    float VR_JP2 = VM_JP2 * sin_r(GAMA[I - 1]);
     float VT_JP2 = WT_JP2 + UNOWJP2;
     float V_JP2_SQ = VM_JP2 * VM_JP2 + VT_JP2 * VT_JP2;
     float ROE_JP2 = ROISENT[I - 1] * (p.CV * TISENT[I - 1] + 0.5f * V_JP2_SQ);
@@ -3184,10 +3408,10 @@ __global__ void k_integrals(Geo g, Dev d, Params p) {
     if (TS < 0.1f * TO1M) TS = 0.1f * TO1M;
     float PSTAT = d.P[n];
     if (PSTAT < 0.02f * PO1M) PSTAT = 0.02f * PO1M;
-    float PSTAG = PSTAT * powf(TSTAG / TS, 1.0f / p.FGA);
+    float PSTAG = PSTAT * pow_r(TSTAG / TS, 1.0f / p.FGA);
     float TRAT = TS / TO1M, PRAT = PSTAT / PO1M, ENTRPY = 0.0f;
     if (!((PRAT > 0.99999f && PRAT < 1.00001f) || (TRAT > 0.99999f && TRAT < 1.00001f)))
-      ENTRPY = p.CP * logf(TRAT) - p.RGAS * logf(PRAT);
+      ENTRPY = p.CP * log_r(TRAT) - p.RGAS * log_r(PRAT);
     tstag = TSTAG; prat0 = PSTAG / PO1M; rvt = d.RORVT[n] / d.RO[n]; ent = ENTRPY; ts = TS; ps = d.P[n];
   };
   for (int c = threadIdx.x; c < g.IMM1 * g.KMM1; c += blockDim.x) {

@@ -1553,6 +1553,10 @@ contains
     if (allocated(STEP1)) deallocate (STEP1, STEP2, STEPSBK)
     if (allocated(XFORCE)) deallocate (XFORCE, RFORCE, TFORCE, QSOURCE, FLOWX, FLOWT, FLOWR)
     if (allocated(DWALLSQ)) deallocate (DWALLSQ, XLENGTH, FST_RAT, XLIMIT)
+    ! the Spalart-Allmaras factors are read only for ILOS >= 200; otherwise
+    ! they keep the zero of a fresh COMMON block (multall-open-18.3.f:3338)
+    FAC_STMIX = 0.0; FAC_ST0 = 0.0; FAC_ST1 = 0.0; FAC_ST2 = 0.0; FAC_ST3 = 0.0; FAC_SFVIS = 0.0
+    FAC_VORT = 0.0; FAC_PGRAD = 0.0
   end subroutine reset
 
   !> Read a deck from an open unit and build the case.  Returns .true. on

@@ -408,6 +408,30 @@ def deck_tiny_sa_variant(nsteps: int = 100) -> DeckSpec:
     return spec
 
 
+def deck_tiny_exit(nsteps: int = 100) -> DeckSpec:
+    """The three-row mixing-length tiny cascade with exit smoothing on the
+    last three J planes (SFEXIT = 0.1, NSFEXIT = 3)."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=10 EXIT SMOOTHING"
+    spec.sfexit = 0.1
+    spec.nsfexit = 3
+    return spec
+
+
+def deck_tiny_wide(nsteps: int = 100) -> DeckSpec:
+    """A three-row mixing-length cascade with IM=72 (more than 64 pitchwise
+    cells per line) and short rows of 21 J-points, with exit smoothing on the
+    last three J planes (SFEXIT = 0.1, NSFEXIT = 3)."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=10 IM=72 EXIT SMOOTHING"
+    spec.im = 72
+    for r in spec.rows:
+        r.jm, r.jle, r.jte = 21, 6, 16
+    spec.sfexit = 0.1
+    spec.nsfexit = 3
+    return spec
+
+
 def deck_config0(nsteps: int = 2000) -> DeckSpec:
     """Single axial turbine stator row, 37x33x97, 60 blades, hub/tip 0.30/0.35 m,
     camber 0->70 deg, 20% NACA thickness, p0=1e5 Pa, T0=300 K, p_exit=0.8e5 Pa,

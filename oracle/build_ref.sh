@@ -11,13 +11,15 @@
 # for the compile (FORTRAN INCLUDE resolves next to the including file first)
 # and deleted afterwards; only the executables stay in oracle/_ref/.
 #
-# Usage: oracle/build_ref.sh [fp32|fp64|all]
+# Usage: oracle/build_ref.sh [fp32|fp64|fp32w|all]
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 SRC=/root/reference/Multall-open-18.3/multall-18.3
 OUT="$HERE/_ref"
 FC=${FC:-amdflang}
 DIMS="ID=65,JD=600,KD=65,MAXKI=65,NRS=21,IG1=30,"
+# a second FP32 build for pitchwise-wide decks (IM <= 96, JM <= 200, KM <= 33)
+DIMS_WIDE="ID=96,JD=200,KD=33,MAXKI=96,NRS=21,IG1=30,"
 if [ ! -f "$SRC/multall-open-18.3.f" ]; then
   echo "build_ref: reference sources not present; skipping" >&2
   exit 0
@@ -25,13 +27,13 @@ fi
 mkdir -p "$OUT"
 
 build_one() {
-  local v=$1 flags=$2
+  local v=$1 flags=$2 dims=${3:-$DIMS}
   local B="$OUT/build_$v"
   rm -rf "$B"; mkdir -p "$B"
   trap "rm -rf $B" ERR
   cp "$SRC/multall-open-18.3.f" "$SRC/commall-open-18.3" "$B/"
-  sed -i "s/PARAMETER(ID=55,JD=2500,KD=74,MAXKI=74,NRS=21,IG1=30,/PARAMETER($DIMS/" "$B/commall-open-18.3"
-  grep -q "PARAMETER($DIMS" "$B/commall-open-18.3"
+  sed -i "s/PARAMETER(ID=55,JD=2500,KD=74,MAXKI=74,NRS=21,IG1=30,/PARAMETER($dims/" "$B/commall-open-18.3"
+  grep -q "PARAMETER($dims" "$B/commall-open-18.3"
   cp "$HERE/ref_harness.f" "$B/"
   (cd "$B" && $FC -O3 $flags -c multall-open-18.3.f -o mf.o \
      && $FC -O3 $flags -c ref_harness.f -o harness.o)
@@ -46,7 +48,9 @@ build_one() {
 case "${1:-all}" in
   fp32) build_one fp32 "" ;;
   fp64) build_one fp64 "-fdefault-real-8 -fdefault-double-8" ;;
+  fp32w) build_one fp32w "" "$DIMS_WIDE" ;;
   all)  build_one fp32 "" & p1=$!
         build_one fp64 "-fdefault-real-8 -fdefault-double-8" & p2=$!
-        wait $p1; wait $p2 ;;
+        build_one fp32w "" "$DIMS_WIDE" & p3=$!
+        wait $p1; wait $p2; wait $p3 ;;
 esac

@@ -31,6 +31,9 @@ COMMALL = "/root/reference/Multall-open-18.3/multall-18.3/commall-open-18.3"
 # except the first five).
 BUILD_PARAMS = dict(ID=65, JD=600, KD=65, MAXKI=65, NRS=21, IG1=30, JG1=1000, KG1=30,
                     IG2=25, JG2=500, KG2=20, JG3=100, NCWL=100, NBLED=25)
+# per executable: fp32w is the pitchwise-wide build (DIMS_WIDE in build_ref.sh)
+VARIANT_PARAMS = {"fp32": BUILD_PARAMS, "fp64": BUILD_PARAMS,
+                  "fp32w": dict(BUILD_PARAMS, ID=96, JD=200, KD=33, MAXKI=96)}
 
 _EXPLICIT_REAL = {"MACHLIM", "MACHREL", "MACHCOOL"}
 
@@ -143,7 +146,7 @@ def run_reference(deck_text: str, variant: str = "fp32", workdir: Optional[str] 
                            timeout=timeout, preexec_fn=unlimit_stack)
         if p.returncode != 0 or not os.path.exists(os.path.join(wd, "final.bin")):
             raise RuntimeError(f"reference failed rc={p.returncode}\n{p.stdout[-3000:]}\n{p.stderr[-2000:]}")
-        layout = parse_commall()
+        layout = parse_commall(params=VARIANT_PARAMS[variant])
         return Dump(os.path.join(wd, "setup.bin"), layout), Dump(os.path.join(wd, "final.bin"), layout), p.stdout
     finally:
         if own:

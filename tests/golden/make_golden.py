@@ -1,8 +1,8 @@
 """Generate the golden fixtures under tests/golden/ from the reference solver.
 
 Runs the reference built by oracle/build_ref.sh (this container only; the
-reference is not available where the tests run) on the synthetic decks and
-stores, per case:
+reference is not available where the tests run) on the synthetic decks (the
+IM = 80 deck on its pitchwise-wide build) and stores, per case:
 
   <case>.dat          the NEW-format input deck
   <case>_setup.npz    COMMON state after the reference's SETUP (dense arrays
@@ -39,7 +39,11 @@ CASES = {
     "tiny1_mlw": lambda n: deckgen.deck_tiny_wall_variant(nsteps=n),
     "tiny3_sa": lambda n: deckgen.deck_tiny_sa(nsteps=n),
     "tiny3_sa2": lambda n: deckgen.deck_tiny_sa_variant(nsteps=n),
+    "tiny3_exit": lambda n: deckgen.deck_tiny_exit(nsteps=n),
+    "tiny3_wide": lambda n: deckgen.deck_tiny_wide(nsteps=n),
 }
+# decks wider than the main reference build (IM > 65) run on its wide build
+VARIANT = {"tiny3_wide": "fp32w"}
 STEPS = (1, 100)
 
 
@@ -80,7 +84,7 @@ def make(case):
         txt = deckgen.write_deck(spec)
         wd = tempfile.mkdtemp(prefix="mhgold_")
         try:
-            setup, final, _ = refdump.run_reference(txt, "fp32", workdir=wd)
+            setup, final, _ = refdump.run_reference(txt, VARIANT.get(case, "fp32"), workdir=wd)
             sumflx = read_global_plt(os.path.join(wd, "global.plt"))
         finally:
             shutil.rmtree(wd, ignore_errors=True)

@@ -5,10 +5,17 @@
   limiter, asinf/atan2f/sinf/cosf in NEW_MIXPLAN) enters, which device and
   host libm round differently; those cases get a 2e-6 relative bound.
 - both modes, 1 and 100 steps against the reference's own output:
-  max|gpu-ref| / scale <= 1e-5, where scale is max|ref| of the variable for
-  RO and ROE, of the momentum magnitude |rho V| for ROVX and ROVR, and of
-  |RORVT| for RORVT (a near-zero component such as the radial momentum is
-  measured against the momentum it is part of, not against itself).
+  max|gpu-ref| / scale, where scale is max|ref| of the variable for RO and
+  ROE, of the momentum magnitude |rho V| for ROVX and ROVR, and of |RORVT|
+  for RORVT (a near-zero component such as the radial momentum is measured
+  against the momentum it is part of, not against itself).  After 1 step the
+  bound is 1e-5 (observed ~1e-7).  After 100 steps it is TOL_100 = 2.5e-5:
+  FP32 time marching carries a single-ulp change of one initial value to
+  2.5e-6 .. 1.1e-5 of the scale after 100 steps of the reference's own
+  arithmetic (test_oracle_pinned.py::test_hundred_step_ulp_spread measures it
+  with the oracle and holds it below TOL_100 / 2), so a few ulps of
+  difference anywhere (transcendentals, or the fast mode's FP64 sums) cannot
+  be told apart from the reference beyond that.
 """
 import numpy as np
 import pytest
@@ -19,10 +26,12 @@ from conftest import load_ref, load_state
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP"]
 FORCES = ["XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
 TOL = 1e-5
+TOL_100 = 2.5e-5
 INVISCID = ["tiny1_inv", "tiny3_inv"]
 # mixing-length (LOSS) decks: chord-Reynolds and power-law viscosity, smooth
-# and rough walls, rotor tip gap, fixed wall y+ with a shear-free hub
-MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw"]
+# and rough walls, rotor tip gap, fixed wall y+ with a shear-free hub, exit
+# smoothing on the last three J planes
+MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide"]
 # Spalart-Allmaras decks: rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing, mixing-length relaxation source
 SA = ["tiny3_sa", "tiny3_sa2"]
@@ -116,7 +125,7 @@ def test_vs_reference(case, nsteps, exact):
     for v in ["RO", "ROVX", "ROVR", "RORVT", "ROE"]:
         err = _nerr(g.get(v), ref[v], sc[v])
         print(f"{case} {nsteps} exact={exact} {v}: scaled max err {err:.2e}")
-        assert err <= TOL, (v, err)
+        assert err <= (TOL if nsteps == 1 else TOL_100), (v, err)
     for v in [v for v in SA_VARS if v in ref]:
         # T_SOURCE is production minus destruction of the turbulent
         # viscosity: the cancellation amplifies the state's ~1e-6 rounding
@@ -170,4 +179,4 @@ def test_host_path_vs_reference(case):
     for v in ["RO", "ROVX", "ROVR", "RORVT", "ROE"]:
         err = _nerr(g.get(v), ref[v], sc[v])
         print(f"{case} host path 100 steps {v}: scaled max err {err:.2e}")
-        assert err <= TOL, (v, err)
+        assert err <= TOL_100, (v, err)

@@ -16,7 +16,7 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 # inviscid decks, and mixing-length (LOSS) decks covering the chord-Reynolds
 # and power-law laminar viscosity, smooth and rough walls, a rotor tip gap,
 # the fixed wall y+ and a shear-free hub
-CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2"]
+CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide"]
 # Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing and the mixing-length relaxation source) also pin the
 # turbulence state
@@ -55,3 +55,34 @@ def test_oracle_freestream_preserved():
     assert np.abs(xa).max() < 1e-5 * scale
     # radial closure leaves exactly -8*VOLOR (mf:6694-6700)
     assert np.allclose(-0.125 * ra, st["VOLOR"][:-1, 1:, :-1], rtol=1e-5, atol=1e-6 * np.abs(ra).max())
+
+
+@pytest.mark.parametrize("case", ["tiny3_inv", "tiny3_ml", "tiny3_sa", "tiny3_wide"])
+def test_hundred_step_ulp_spread(case):
+    """How far 100 FP32 steps of the reference's arithmetic (the oracle) carry
+    a one-ulp change of a single initial value: the noise floor below which a
+    100-step difference from the reference means nothing.  The GPU parity
+    tests' 100-step bound (2.5e-5 of each variable's scale) must stay at least
+    twice this spread."""
+    from test_gpu_parity import TOL_100, _scales
+    st = load_state(case)
+    st["scalars"]["NSTEPS_MAX"] = 100
+    o = orc.Oracle(st)
+    o.start()
+    assert o.step(110) == 100
+    base = {v: o[v].copy() for v in ["RO", "ROVX", "ROVR", "RORVT", "ROE"]}
+    sc = _scales(base)
+    spread = 0.0
+    for field, (k, j, i) in (("RO", (4, 30, 5)), ("ROE", (2, 10, 3)), ("ROVX", (6, 50, 7))):
+        st2 = {n: (np.array(v) if n != "scalars" else dict(v)) for n, v in st.items()}
+        a = st2[field]
+        k, j, i = min(k, a.shape[0] - 1), min(j, a.shape[1] - 1), min(i, a.shape[2] - 1)
+        a[k, j, i] = np.nextafter(a[k, j, i], np.float32(np.inf))
+        o2 = orc.Oracle(st2)
+        o2.start()
+        o2.step(110)
+        for v in base:
+            spread = max(spread, float(np.abs(o2[v].astype(np.float64) - base[v]).max() / sc[v]))
+    print(f"{case}: one-ulp spread after 100 steps {spread:.2e}")
+    assert 1e-7 < spread <= TOL_100 / 2, spread
+
