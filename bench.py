@@ -40,6 +40,8 @@ KERNEL_BYTES_PER_NODE = {
     "k_flux_axial": 4 * (9 + 4), "k_flux_angular": 4 * (6 + 4), "k_flux_radial": 4 * (12 + 4),
     "k_resid_line": 4 * (5 + 2), "k_mg_part": 4, "k_mg_apply_lines": 4 * (3 + 1), "k_mg_apply": 4 * (3 + 1),
     "k_damp_scatter1": 4 * (2 + 1), "k_damp_scatter4": 4 * (2 + 1), "k_smooth_plane": 4 * 2, "k_peff": 4 * 2,
+    "k_smooth_reg1": 4 * 2, "k_smooth_reg2": 4 * 2, "k_smooth_reg4": 4 * 2, "k_smooth_reg8": 4 * 2,
+    "k_smooth_reg16": 4 * 2,
     "k_mg_sum_seq": 4, "k_mg_sum_wave": 4, "k_jplane_sums": 4, "k_rolim_check": 4, "k_mach_apply": 4 * 4,
     "k_stepup_temp": 4 * 6, "k_stepup_step": 4 * 5, "k_smooth_resid": 4 * 2, "k_sa_addstore": 4 * 3,
 }
@@ -161,7 +163,8 @@ def main():
     dt = t1 - t0
     value = nodes * args.steps / dt
 
-    # kernel durations with HIP events on the library stream, same workload
+    # kernel durations from dispatch-timestamped HIP events on the library
+    # stream (hipExtLaunchKernel), same workload
     g.profile(True)
     g.step(args.steps)
     rep = g.profile_report()

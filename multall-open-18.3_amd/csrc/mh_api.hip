@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "mh_internal.h"
 
 namespace mh {
@@ -268,28 +270,28 @@ hipEvent_t prof_event(mh_ctx *c) {
   return e;
 }
 
-// Brackets one kernel launch with events on the context stream when
-// profiling is on (mh_profile_enable).
-struct ProfScope {
-  mh_ctx *c;
+// When profiling is on (mh_profile_enable), a launch goes through
+// hipExtLaunchKernel with a start and a stop event that the dispatch itself
+// timestamps, so the recorded time is the kernel's own execution (the same
+// begin/end that rocprofv3's kernel trace reports).
+mh_prof_rec prof_rec(mh_ctx *c, const char *name) {
   mh_prof_rec rec{};
-  ProfScope(mh_ctx *c_, const char *name) : c(c_) {
-    if (!c->prof_on) return;
-    rec.name = name;
-    rec.t0 = prof_event(c);
-    rec.t1 = prof_event(c);
-    if (rec.t0) (void)hipEventRecord(rec.t0, c->stream);
-  }
-  ~ProfScope() {
-    if (!c->prof_on || !rec.t0 || !rec.t1) return;
-    (void)hipEventRecord(rec.t1, c->stream);
-    c->prof_pending.push_back(rec);
-  }
-};
-#define LAUNCH(kern, ...)                   \
-  do {                                      \
-    ProfScope ps_(c, #kern);                \
-    hipLaunchKernelGGL(kern, __VA_ARGS__);  \
+  rec.name = name;
+  rec.t0 = prof_event(c);
+  rec.t1 = prof_event(c);
+  return rec;
+}
+#define LAUNCH(kern, grid, block, shmem, stream, ...)                                              \
+  do {                                                                                             \
+    if (c->prof_on) {                                                                              \
+      mh_prof_rec r_ = prof_rec(c, #kern);                                                         \
+      if (r_.t0 && r_.t1) {                                                                        \
+        hipExtLaunchKernelGGL(kern, grid, block, shmem, stream, r_.t0, r_.t1, 0, __VA_ARGS__);   \
+        c->prof_pending.push_back(r_);                                                             \
+        break;                                                                                     \
+      }                                                                                            \
+    }                                                                                              \
+    hipLaunchKernelGGL(kern, grid, block, shmem, stream, __VA_ARGS__);                             \
   } while (0)
 
 // Reduce finished launch timings into the per-kernel totals.
