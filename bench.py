@@ -9,10 +9,13 @@ Fortran host set-up and marched by libmultall_hip in FP32.  A "step" is one time
 of the reference's LOOP) over all 2,371,584 nodes.
 
 Prints one JSON line: node-steps/s (whole job), the roofline of the
-dominant kernel (HIP-event durations on the library's stream), the
-step-level fraction of the 8 TB/s HBM roofline at the 132 B/node-step
-algorithmic traffic of a fused FP32 step, and the reference solver timed on
-this host's CPU (single-threaded, as built by oracle/build_ref.sh).
+dominant kernel (dispatch-timestamped event durations on the library's
+stream, HBM traffic from the committed counter runs), the step-level fraction
+of the 8 TB/s HBM roofline at the 132 B/node-step algorithmic traffic of a
+fused FP32 step, the reference solver timed on this host's CPU
+(single-threaded, as built by oracle/build_ref.sh), and the time to
+converge Rotor 37 (the reference's own test deck, plain Spalart-Allmaras) by
+the reference's stop test.
 """
 from __future__ import annotations
 
@@ -64,6 +67,34 @@ def traffic_bytes(kernel: str):
         return None
     cal = t["calibration"]
     return (k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]) * 1024.0
+
+
+CONVERGED_DECK = os.path.join(ROOT, "tests", "golden", "decks", "converged_r37sa.dat")
+
+
+def time_to_converged() -> dict:
+    """Rotor 37 (the reference's own test deck, plain Spalart-Allmaras,
+    45 x 150 x 49 nodes) marched from the reference's initial guess until the
+    reference's stop test ends the run (mf:3762-3764): wall seconds for the
+    steps alone and including the deck read and host set-up."""
+    from multall_hip import host
+    from multall_hip.runtime import Solver
+    t0 = time.perf_counter()
+    st = host.load_deck(CONVERGED_DECK)
+    g = Solver(st, exact_order=False)
+    g.loss_init()
+    g.start()
+    g.sync()
+    t1 = time.perf_counter()
+    r = g.step(int(st["scalars"]["NSTEPS_MAX"]) + 10)
+    g.sync()
+    t2 = time.perf_counter()
+    nodes = int(st["scalars"]["IM"] * st["scalars"]["JM"] * st["scalars"]["KM"])
+    return {"workload": f"Rotor 37, reference test deck with plain Spalart-Allmaras, {nodes} nodes",
+            "converged": bool(r["ifend"] == 1 and r["nstep"] < int(st["scalars"]["NSTEPS_MAX"])),
+            "steps": int(r["nstep"]), "seconds": t2 - t1, "seconds_with_setup": t2 - t0,
+            "node_steps_per_s": nodes * r["nstep"] / (t2 - t1),
+            "final": {"eavg": r["eavg"], "emax": r["emax"], "econt": r["econt"]}}
 
 
 def calibrate_copy() -> None:
@@ -123,6 +154,7 @@ def main():
     ap.add_argument("--ilos", type=int, default=200,
                     help="viscous model of the deck: 200 Spalart-Allmaras (the workload's), 10 mixing length, 0 inviscid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-converged", action="store_true", help="skip the time-to-converged run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (for scaling the PMC byte counters)")
     args = ap.parse_args()
@@ -195,6 +227,7 @@ def main():
         "kernels_ms_per_step": {k: round(ms / args.steps, 4) for k, (c, ms) in
                                 sorted(kern.items(), key=lambda kv: -kv[1][1])},
         "cpu_baseline": cpu,
+        "time_to_converged": None if args.no_converged else time_to_converged(),
     }
     print(json.dumps(out))
 

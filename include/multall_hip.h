@@ -100,7 +100,9 @@ enum mh_array_id {
   MH_XLLIM_I1, MH_XLLIM_IM, MH_XLLIM_K1, MH_XLLIM_KM, MH_XLLIM_DWN, MH_XLLIM_UP,
   MH_ROUGH_H, MH_ROUGH_T, MH_ROUGH_L, MH_ROUGH_U,   /* RN, float */
   MH_JTRAN_I1, MH_JTRAN_IM, MH_JTRAN_K1, MH_JTRAN_KM, /* RN, int */
-  /* Spalart-Allmaras model, SPAL_LOSS (BKSPALOS, BKNEWLOS): the transported
+  /* Spalart-Allmaras model, SPAL_LOSS (BKSPALOS, BKNEWLOS; NEW_LOSS uses
+   * the cell density, laminar viscosity, viscosity ratio, mixing length and
+   * free-stream factor): the transported
    * turbulent viscosity and its SSS history, its kinematic value and source,
    * the cell density, laminar viscosity and viscosity ratio, the mixing
    * length and free-stream factor of SET_XLENGTH -- N (cell-centred); the
@@ -160,9 +162,10 @@ double mh_get_scalar(const mh_ctx *ctx, int id);
 
 /* The viscous-force initialisation SETUP performs before LOOP
  * (mf:7346-7357): one LOSS call (mf:8058-9066) with RF_VIS = 1, FMIXUP = 1
- * and NSTEP = 1, which also sets VISLAM/FTCOND; with ILOS >= 200 the call is
- * SPAL_LOSS (mf:13586-14827), which also sets the initial turbulent
- * viscosity.  A no-op for ILOS = 0.  Call after all inputs are set, before
+ * and NSTEP = 1, which also sets VISLAM/FTCOND; with 100 <= ILOS < 200 the
+ * call is NEW_LOSS (mf:12522-13462), with ILOS >= 200 SPAL_LOSS
+ * (mf:13586-14827), which also sets the initial turbulent viscosity.  A
+ * no-op for ILOS = 0.  Call after all inputs are set, before
  * or after mh_start. */
 int mh_loss_init(mh_ctx *ctx);
 

@@ -49,7 +49,7 @@ __global__ void k_smooth_reg8(Geo, Dev, const float *, float *, Params, StepScal
 __global__ void k_smooth_reg16(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
 __global__ void k_fill_int(int *, int, int);
 __global__ void k_rolim_check(Geo, Dev, StepScalars);
-__global__ void k_rolim_fix(Geo, Dev, StepScalars);
+__global__ void k_rolim_fix(Geo, Dev, StepScalars, Params);
 __global__ void k_mach_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_mixplan(Geo, Dev, Params, float *);
 __global__ void k_vrms_part(Geo, Dev, double *);
@@ -80,6 +80,8 @@ __global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
 __global__ void k_sa_grad(Geo, Dev, Params);
 __global__ void k_sa_visc(Geo, Dev, Params, float, int);
+__global__ void k_nl_visc(Geo, Dev, Params, float);
+__global__ void k_nl_trans(Geo, Dev, Params);
 __global__ void k_sa_gradtvis(Geo, Dev, Params);
 __global__ void k_sa_trcol(Geo, Dev, Params);
 __global__ void k_sa_trline(Geo, Dev, Params);
@@ -676,8 +678,8 @@ int loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   return MH_OK;
 }
 
-// Per-J values of the variables SPAL_LOSS's periodic-force block reads
-// without setting them (its J-1 and row number keep the values of the last
+// Per-J values of the variables the periodic-force block of SPAL_LOSS and
+// NEW_LOSS reads without setting them (its J-1 and row number keep the values of the last
 // blade J its K, J loop visited; before the first one, those left by its
 // earlier loops), and the row its hub/casing loop then reads.
 int sa_maps(mh_ctx *c) {
@@ -688,8 +690,9 @@ int sa_maps(mh_ctx *c) {
   if ((r = d2h(c, IND.data(), c->d.IND, JM * 4)) || (r = d2h(c, NROW.data(), c->d.NROW, JM * 4)) ||
       (r = d2h(c, INDMIX.data(), c->d.INDMIX, JM * 4)))
     return r;
-  for (int J = 2; J <= JM; J++)
-    if (INDMIX[J - 1] == 1 && INDMIX[J - 2] == 1) return fail(c, MH_E_UNSUPPORTED, "adjacent mixing-plane J");
+  if (params_of(c).ILOS >= 200)  // k_sa_visc copies the vorticity from J-1 at a mixing plane
+    for (int J = 2; J <= JM; J++)
+      if (INDMIX[J - 1] == 1 && INDMIX[J - 2] == 1) return fail(c, MH_E_UNSUPPORTED, "adjacent mixing-plane J");
   int last = 0;
   for (int J = 2; J <= JM; J++)
     if (IND[J - 1] == 1) last = J;
@@ -726,6 +729,32 @@ int spal_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   return MH_OK;
 }
 
+// One NEW_LOSS call (mf:12522-13462) with the given relaxation factor,
+// FMIXUP and step number: SPAL_LOSS's gradient, stress and force kernels
+// around the mixing-length viscosity and the transition test.
+int new_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const Params p = params_of(c);
+  const dim3 grid = node_grid(g);
+  if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_grad, grid, kNodeBlock, 0, c->stream, g, d, p);
+  LAUNCH(k_nl_visc, grid, kNodeBlock, 0, c->stream, g, d, p, fmixup);
+  LAUNCH(k_nl_trans, dim3(g.JM - 1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_iforces, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_kforces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis);
+  LCHK("new_loss");
+  return MH_OK;
+}
+
+// The viscous model's call for ILOS (mf:3935-3942, 5339-5341, 7354-7356).
+int viscous(mh_ctx *c, int ilos, float rf_vis, float fmixup, int nstep) {
+  if (ilos == 10) return loss(c, rf_vis, fmixup, nstep);
+  if (ilos >= 100 && ilos < 200) return new_loss(c, rf_vis, fmixup, nstep);
+  if (ilos >= 200) return spal_loss(c, rf_vis, fmixup, nstep);
+  return MH_OK;
+}
+
 // One time step: the body of LOOP from label 5000 (mf:3753-5454).
 int step_once(mh_ctx *c) {
   const Geo &g = c->g;
@@ -758,11 +787,15 @@ int step_once(mh_ctx *c) {
     LCHK("vrms");
   }
   // viscous forces every NLOS steps, mf:3935-3942
-  if (p.ILOS == 10 && c->nstep % p.NLOS == 0 && (r = loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
-  if (p.ILOS >= 200 && c->nstep % p.NLOS == 0 && (r = spal_loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
+  if (p.ILOS != 0 && c->nstep % p.NLOS == 0 && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
   if ((r = tstep(c, d.RO, d.DRO, 1))) return r;
+  if (p.ILOS >= 100 && p.ILOS < 200) {
+    // cell-average density for NEW_LOSS, mf:4260-4270
+    LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("cellavg");
+  }
   if (p.ILOS >= 200) {
     // transport of the turbulent viscosity, mf:4260-4351
     LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
@@ -791,7 +824,7 @@ int step_once(mh_ctx *c) {
   d = c->d;
   const int nline = g.IM * g.KM;
   LAUNCH(k_rolim_check, grid, kNodeBlock, 0, c->stream, g, d, s);
-  LAUNCH(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s);
+  LAUNCH(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s, p);
   LAUNCH(k_mach_apply, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   LCHK("limiters");
   if (g.IM > 2 && p.IFMIX > 0 && p.RFMIX > 1.0e-3f && g.NROWS > 1) {
@@ -806,8 +839,7 @@ int step_once(mh_ctx *c) {
     LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("convergence");
     // forces brought up to date before the final output, mf:5338-5339
-    if (c->ifend == 1 && p.ILOS == 10 && (r = loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
-    if (c->ifend == 1 && p.ILOS >= 200 && (r = spal_loss(c, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
+    if (c->ifend == 1 && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
     if ((r = stepup(c, 0.25f))) return r;
     float sc[8];
     HIPCHK(hipMemcpyAsync(sc, d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
@@ -999,8 +1031,8 @@ static int check_inputs(mh_ctx *c) {
   if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
   if (p.IN_FLOW != 0 || p.IF_REPEAT != 0 || p.IF_RESTART != 0)
     return fail(c, MH_E_UNSUPPORTED, "IN_FLOW, IF_REPEAT or IF_RESTART set");
-  if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 200)
-    return fail(c, MH_E_UNSUPPORTED, "viscous model other than ILOS = 0, 10 (mixing length) or >= 200 (SA)");
+  if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 100)
+    return fail(c, MH_E_UNSUPPORTED, "ILOS other than 0, 10 (LOSS), 100-199 (NEW_LOSS) or >= 200 (SPAL_LOSS)");
   if (p.ILOS != 0 && p.YPLUSWALL < -0.001f)
     return fail(c, MH_E_UNSUPPORTED, "Shih wall functions (YPLUSWALL < 0)");
   if (p.ILOS >= 200 && (p.FAC_VORT > 0.001f || p.FAC_PGRAD > 0.001f))
@@ -1023,7 +1055,7 @@ int mh_loss_init(mh_ctx *c) {
   if (p.ILOS == 10) {
     if ((r = loss_tables(c)) || (r = loss(c, 1.0f, 1.0f, 1))) return r;
   } else {
-    if ((r = sa_maps(c)) || (r = spal_loss(c, 1.0f, 1.0f, 1))) return r;
+    if ((r = sa_maps(c)) || (r = viscous(c, p.ILOS, 1.0f, 1.0f, 1))) return r;
   }
   if ((r = d2h(c, vf, c->d.SC + SC_VISLAM, sizeof vf))) return r;
   c->scal[MH_S_VISLAM] = vf[0];
@@ -1040,7 +1072,7 @@ int mh_start(mh_ctx *c) {
   if ((r = build_lists(c))) return r;
   if (p.ILOS != 0) {
     if (p.ILOS == 10 && (r = loss_tables(c))) return r;
-    if (p.ILOS >= 200 && (r = sa_maps(c))) return r;
+    if (p.ILOS >= 100 && (r = sa_maps(c))) return r;
     float vf[2] = {(float)c->scal[MH_S_VISLAM], (float)c->scal[MH_S_FTCOND]};
     if ((r = h2d(c, c->d.SC + SC_VISLAM, vf, sizeof vf))) return r;
   }

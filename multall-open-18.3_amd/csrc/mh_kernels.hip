@@ -2321,7 +2321,9 @@ __device__ __forceinline__ void sa_gradcell(const Geo &g, const Dev &d, const fl
 }
 
 // velocity and temperature gradients, strain rates and vorticity of each
-// cell, mf:13666-13718 (the vorticity before the mixing-plane substitution)
+// cell, mf:13666-13718 (the vorticity before the mixing-plane substitution);
+// NEW_LOSS forms the same quantities (mf:12586-12632) but keeps the
+// vorticity of a row's first J, which SPAL_LOSS zeroes
 __global__ void k_sa_grad(Geo g, Dev d, Params p) {
   CELL_IJK
   const unsigned n = IX(I, J, K);
@@ -2349,7 +2351,7 @@ __global__ void k_sa_grad(Geo g, Dev d, Params p) {
   const float VORTT = DVXDR - DVRDX;
   float VORT = sqrtf(VORTX * VORTX + VORTR * VORTR + VORTT * VORTT);
   const int NRW = d.NROW[J - 1];
-  if (J == d.JSTART[NRW - 1]) VORT = 0.0f;
+  if (p.ILOS >= 200 && J == d.JSTART[NRW - 1]) VORT = 0.0f;
   if (VORT > p.VORT_MAX) VORT = p.VORT_MAX;
   d.ABS_VORT[n] = VORT;
 }
@@ -2400,6 +2402,106 @@ __global__ void k_sa_visc(Geo g, Dev d, Params p, float FMIXUP, int NSTEP) {
   d.VISC_RAT[n] = VISTOT / VISLAM;
   d.VISC_LAM[n] = VISLAM;
   d.VISC_TURB[n] = VISTOT - VISLAM;
+}
+
+// NEW_LOSS's viscosity of each cell, mf:12636-12672: the mixing length times
+// the vorticity, damped through the buffer layer by the cell's y+ from the
+// previous call, plus the laminar and free-stream-turbulence parts, limited
+// to TURBVIS_LIM times laminar
+__global__ void k_nl_visc(Geo g, Dev d, Params p, float FMIXUP) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  float TURBVIS_WALL = d.ROAVG_CELL[n] * d.XLENGTH[n] * d.ABS_VORT[n] * FMIXUP;
+  const float yp = d.Y_PLUS[n];
+  float FYPLUS = 0.0f;
+  if (yp > p.YPLAM) {
+    float XFAC = (yp - p.YPLAM) / (p.YPTURB - p.YPLAM);
+    if (XFAC > 1.0f) XFAC = 1.0f;
+    FYPLUS = XFAC * XFAC * (3.0f - 2.0f * XFAC);
+  }
+  TURBVIS_WALL = TURBVIS_WALL * FYPLUS;
+  float VISLAM = d.SC[SC_VISLAM];
+  if (p.REYNO < 0.0001f) {
+    const int IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
+    const float *T = d.T_STATIC;
+    const float TSAVG = 0.125f * (T[IX(I, J, K)] + T[IX(IP1, J, K)] + T[IX(IP1, J, KP1)] + T[IX(I, J, KP1)] +
+                                  T[IX(I, JM1, K)] + T[IX(IP1, JM1, K)] + T[IX(IP1, JM1, KP1)] + T[IX(I, JM1, KP1)]);
+    VISLAM = (fabsf(p.REYNO) / 100000.0f) * pow_r(TSAVG / 288.0f, 0.62f);
+  }
+  float VISTOT = VISLAM * (1 + d.FST_RAT[n]) + TURBVIS_WALL;
+  const float VISLIM = p.TURBVIS_LIM * VISLAM;
+  if (VISTOT > VISLIM) VISTOT = VISLIM;
+  d.VISC_RAT[n] = VISTOT / VISLAM;
+  d.VISC_LAM[n] = VISLAM;
+}
+
+// NEW_LOSS's transition test, mf:12677-12772: a surface's cells within half
+// the span (hub, casing) or half the pitch (I = 1 and I = IM surfaces) are
+// laminar (viscosity ratio 1) upstream of its transition point and wherever
+// the largest ratio there has not passed FTRANS.  One workgroup per J: the
+// hub and casing tests of every (I) column come first, then the blade tests
+// of every (K) line, which read what the column tests set.  The casing range
+// starts at KMID and the I = IM range at IMID, so each re-reads the cell
+// the test before it may have reset.
+__global__ void __launch_bounds__(64) k_nl_trans(Geo g, Dev d, Params p) {
+  const int J = blockIdx.x + 2, lane = threadIdx.x;
+  const int IMM1 = g.IMM1, KMM1 = g.KMM1, IMID = p.IMID, KMID = p.KMID;
+  const int NRW = d.NROW[J - 1];
+  const int JROW = J - d.JSTART[NRW - 1] + 1;
+  float *VR = d.VISC_RAT;
+  if (g.KM != 2) {
+    const int JTRHUB = d.JTRAN_K1[NRW - 1], JTRTIP = d.JTRAN_KM[NRW - 1];
+    for (int I = lane + 1; I <= IMM1; I += 64) {
+      bool lam = true;
+      if (JROW >= JTRHUB) {
+        float RMAX = 0.0f;
+        for (int K = 1; K <= KMID; K++) {
+          const float v = VR[IX(I, J, K)];
+          if (!(v < RMAX)) RMAX = v;
+        }
+        lam = !(RMAX > p.FTRANS);
+      }
+      if (lam)
+        for (int K = 1; K <= KMID; K++) VR[IX(I, J, K)] = 1.0f;
+      lam = true;
+      if (JROW >= JTRTIP) {
+        float RMAX = 0.0f;
+        for (int K = KMID; K <= KMM1; K++) {
+          const float v = VR[IX(I, J, K)];
+          if (!(v < RMAX)) RMAX = v;
+        }
+        lam = !(RMAX > p.FTRANS);
+      }
+      if (lam)
+        for (int K = KMID; K <= KMM1; K++) VR[IX(I, J, K)] = 1.0f;
+    }
+  }
+  __syncthreads();
+  const int JTRLOW = d.JTRAN_I1[NRW - 1], JTRUP = d.JTRAN_IM[NRW - 1];
+  for (int K = lane + 1; K <= KMM1; K += 64) {
+    bool lam = true;
+    if (JROW >= JTRLOW) {
+      float RMAX = 0.0f;
+      for (int I = 1; I <= IMID; I++) {
+        const float v = VR[IX(I, J, K)];
+        if (!(v < RMAX)) RMAX = v;
+      }
+      lam = !(RMAX > p.FTRANS);
+    }
+    if (lam)
+      for (int I = 1; I <= IMID; I++) VR[IX(I, J, K)] = 1.0f;
+    lam = true;
+    if (JROW >= JTRUP) {
+      float RMAX = 0.0f;
+      for (int I = IMID; I <= IMM1; I++) {
+        const float v = VR[IX(I, J, K)];
+        if (!(v < RMAX)) RMAX = v;
+      }
+      lam = !(RMAX > p.FTRANS);
+    }
+    if (lam)
+      for (int I = IMID; I <= IMM1; I++) VR[IX(I, J, K)] = 1.0f;
+  }
 }
 
 // gradient of the kinematic turbulent viscosity, mf:13893-13911 (J <= JM-1;
@@ -2703,7 +2805,9 @@ __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
       XF[IM] = WVISCIM;
     }
     __syncthreads();
-    for (int I = lane + 1; I <= IM; I += 64) {
+    // SPAL_LOSS sets y+ up to I = IM (mf:14404), NEW_LOSS up to IM-1 (mf:13035)
+    const int iend = (p.ILOS >= 200) ? IM : IMM1;
+    for (int I = lane + 1; I <= iend; I += 64) {
       const unsigned n = IX(I, J, K);
       const float YPLS = (I <= IMID) ? VS[0] * sqrtf(d.DWALLSQ[n]) * (0.25f * F0[3]) / VISLAM
                                      : VS[1] * sqrtf(d.DWALLSQ[n]) * (0.25f * F1[3]) / VISLAM;
@@ -2805,7 +2909,9 @@ __global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
   if (I > g.IMM1 || J > g.JM) return;
   const int IP1 = I + 1, JM1 = J - 1, KMM1 = g.KMM1, KM = g.KM, KMID = p.KMID;
   const float VISLAM = d.VISC_LAM[IX(g.IMM1, g.JM, KMM1)], VISLAM16 = VISLAM * 16.0f;
-  const int NRWHC = d.SAJMAP[4 * g.JM];
+  // SPAL_LOSS's hub/casing loop reads the row its blade loop left
+  // (mf:14536); NEW_LOSS's takes the row of its own J (mf:13169)
+  const int NRWHC = (p.ILOS >= 200) ? d.SAJMAP[4 * g.JM] : d.NROW[J - 1];
   const float *VX = d.VX, *VR = d.VR, *VT = d.VT, *RO = d.RO;
   // hub, mf:14510-14650
   float VXAVG1 = VX[IX(I, J, 2)] + VX[IX(IP1, J, 2)] + VX[IX(IP1, JM1, 2)] + VX[IX(I, JM1, 2)];
@@ -2923,17 +3029,6 @@ __global__ void k_rolim_check(Geo g, Dev d, StepScalars s) {
   if (nv != ro) d.FLAGS[IKX(I, K)] = 1;
 }
 
-__global__ void k_rolim_fix(Geo g, Dev d, StepScalars s) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.IM * g.KM) return;
-  if (d.FLAGS[t] == 0) return;
-  d.FLAGS[t] = 0;  // ready for the next step's check
-  const int I = t % g.IM + 1, K = t / g.IM + 1;
-  for (int J = 3; J <= g.JM - 3; J++)
-    d.RO[IX(I, J, K)] = rolim_one(d.RO[IX(I, J, K)], d.RO[IX(I, J - 2, K)], d.RO[IX(I, J - 1, K)], d.RO[IX(I, J + 1, K)],
-                                  d.RO[IX(I, J + 2, K)], s.ROLIM);
-}
-
 // reference relative Mach conditions of row NR, mf:5127-5145
 __device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Params &p, float TCOOL_MIN, int NR,
                                          float &vlimit, float &rolimit) {
@@ -2954,17 +3049,32 @@ __device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Param
   rolimit = ROSTAG_REL / RORATIO;
 }
 
-// relative Mach number and mass-flux limiter, mf:5150-5168; each workgroup
-// evaluates the per-row limits (mf:5127-5145) into LDS first.
+// Density limiter on the flagged (I,K) lines, mf:5115-5122.  The thread of
+// the line (IMID, KMID), which holds every row's reference node, then forms
+// the per-row relative Mach limits (mf:5127-5145) into d.SC for
+// k_mach_apply, after its own line's fix.
+__global__ void k_rolim_fix(Geo g, Dev d, StepScalars s, Params p) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.IM * g.KM) return;
+  const int I = t % g.IM + 1, K = t / g.IM + 1;
+  if (d.FLAGS[t] != 0) {
+    d.FLAGS[t] = 0;  // ready for the next step's check
+    for (int J = 3; J <= g.JM - 3; J++)
+      d.RO[IX(I, J, K)] = rolim_one(d.RO[IX(I, J, K)], d.RO[IX(I, J - 2, K)], d.RO[IX(I, J - 1, K)],
+                                    d.RO[IX(I, J + 1, K)], d.RO[IX(I, J + 2, K)], s.ROLIM);
+  }
+  if (I == p.IMID && K == p.KMID)
+    for (int NR = 1; NR <= g.NROWS; NR++)
+      mach_ref(g, d, p, s.TCOOL_MIN, NR, d.SC[SC_VLIMIT + NR - 1], d.SC[SC_ROLIMIT + NR - 1]);
+}
+
+// relative Mach number and mass-flux limiter, mf:5150-5168, with the per-row
+// limits k_rolim_fix left in d.SC
 __global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
-  __shared__ float lim[2 * 64];
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  if (tid < g.NROWS) mach_ref(g, d, p, st.TCOOL_MIN, tid + 1, lim[tid], lim[64 + tid]);
-  __syncthreads();
   NODE_IJK
   const size_t n = IX(I, J, K);
   const int NR = d.NROW[J - 1];
-  const float V_LIMIT = lim[NR - 1], RO_LIMIT = lim[64 + NR - 1];
+  const float V_LIMIT = d.SC[SC_VLIMIT + NR - 1], RO_LIMIT = d.SC[SC_ROLIMIT + NR - 1];
   float ro = d.RO[n];
   if (ro < RO_LIMIT) {
     ro = RO_LIMIT;

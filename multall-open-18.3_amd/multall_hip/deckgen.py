@@ -42,6 +42,7 @@ class RowSpec:
     ktips: int = 0                 # tip-gap start K (0 = no gap)
     ktipe: int = 0
     fractip: float = 0.0           # tip gap as a fraction of span
+    jtran: tuple = (0, 0, 0, 0)    # transition J from the row start: I=1, I=IM, hub, casing surfaces
     pguess: tuple = (1.0e5, 1.0e5, 0.8e5, 0.8e5)   # PUP, PLE, PTE, PDN
 
 
@@ -278,7 +279,7 @@ def write_deck(spec: DeckSpec) -> str:
             a("  BLADE THINNING")
             a(_fmt([1.0] * spec.km, 16, "{:5.1f}"))
         a("       BOUNDARY LAYER TRANSITION POINTS")
-        a("         0         0         0         0")
+        a("".join(f"{j:10d}" for j in row.jtran))
         a("  SET NEWGRID = 1 TO GENERATE A NEW GRID")
         a("         0")
         a("   RPMROW,    RPMHUB")
@@ -405,6 +406,33 @@ def deck_tiny_sa_variant(nsteps: int = 100) -> DeckSpec:
     spec.rough = (20.0, 30.0, 10.0, 15.0)
     spec.ibound = 2
     spec.sa_factors = (1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 0.0, 0.0)
+    return spec
+
+
+def deck_tiny_nl(nsteps: int = 100) -> DeckSpec:
+    """Three-row tiny cascade with the vorticity mixing-length model of
+    NEW_LOSS (ILOS=100) and a rotor tip gap over the two outer K-cells."""
+    spec = deck_tiny(ilos=100, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=100 NEW MIXING LENGTH, TIP GAP"
+    r = spec.rows[1]
+    r.ktips, r.ktipe, r.fractip = 7, 9, 0.02
+    return spec
+
+
+def deck_tiny_nl_variant(nsteps: int = 100) -> DeckSpec:
+    """The ILOS=100 tiny cascade with power-law laminar viscosity, rough
+    walls, a shear-free hub (IBOUND = 1), transition points part-way along
+    every surface of each row and a transition threshold (FTRANS) that
+    leaves part of the flow laminar downstream of them."""
+    spec = deck_tiny_nl(nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=100 POWER-LAW VISCOSITY, ROUGH, IBOUND=1, TRANSITION"
+    spec.reyno = -2.0
+    spec.if_rough = 1
+    spec.rough = (20.0, 30.0, 10.0, 15.0)
+    spec.ibound = 1
+    spec.ftrans = 4.0
+    for n, r in enumerate(spec.rows):
+        r.jtran = (14 + 2 * n, 18, 9, 24 - 2 * n)
     return spec
 
 

@@ -11,7 +11,9 @@
  * (IFGAS=0), ITIMST=3 (scree; SSS and residual smoothing through F1/F2/F3
  * and NRSMTH), IM>2, KM>2, no cooling/bleed/shroud, IN_FLOW=0,
  * IF_REPEAT=0, no exit throttle or plate loss, IF_CUSP=0, IF_RESTART=0,
- * ILOS=0 (inviscid) or ILOS=10 (mixing-length LOSS with YPLUSWALL=0).
+ * ILOS=0 (inviscid), ILOS=10 (mixing-length LOSS), 100 <= ILOS < 200
+ * (NEW_LOSS) or ILOS >= 200 (SPAL_LOSS without the helicity and pressure-
+ * gradient source factors), each with YPLUSWALL >= -0.001.
  */
 #include "mh_oracle.h"
 
@@ -134,7 +136,7 @@ int orc_start(orc_t *o) {
   if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 || o->IN_FLOW != 0 ||
       o->IF_REPEAT != 0 || o->IF_RESTART != 0)
     return -2;
-  if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 200) return -3;
+  if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 100) return -3;
   if (o->ILOS != 0 && o->YPLUSWALL < -0.001f) return -3;
   if (o->ILOS >= 200 && (o->FAC_VORT > 0.001f || o->FAC_PGRAD > 0.001f)) return -3;
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMID = o->IMID, KMID = o->KMID;
@@ -899,13 +901,285 @@ static float sa_cf(float RE, float DPERP, float ROUGH, float ROAVG4, float WAVG,
   return CF;
 }
 
+/* The part SPAL_LOSS and NEW_LOSS share: stresses and heat flows with the
+ * final viscosity, wall friction on the blade, hub and casing surfaces, the
+ * I- and K-face sums and the body-force update (SPAL_LOSS mf:14123-14819,
+ * NEW_LOSS mf:12787-13455).  TXR..QTT hold the strain rates and temperature
+ * gradients of each cell and are overwritten with the stresses and heat
+ * flows.  The routines differ in two places, selected by newloss: NEW_LOSS's
+ * y+ loop on the I = IM surface stops at IM-1 (mf:13035; SPAL_LOSS's,
+ * mf:14404, runs to IM), and its hub/casing loop takes the row of its own J
+ * (mf:13169) where SPAL_LOSS's reads the row its blade loop left
+ * (mf:14536).  Both enter with J-1 = JM-1 and the row of J = JM, the values
+ * their earlier loops leave.  Returns the laminar viscosity of the last cell,
+ * the value the routines' VISLAM holds on return. */
+static float visc_forces(orc_t *o, float *TXR, float *TXX, float *TXT, float *TRR, float *TRT, float *TTT,
+                         float *QXX, float *QRR, float *QTT, int newloss) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID, KMID = o->KMID;
+  const size_t N = (size_t)IM * JM * KM;
+  float *TRX = calloc(N, 4), *TTX = calloc(N, 4), *TTR = calloc(N, 4);
+  float *FORCEX = calloc(N, 4), *FORCER = calloc(N, 4), *FORCET = calloc(N, 4), *ESOURCE = calloc(N, 4);
+  const float RF_VIS = o->RF_VIS, RF_VIS1 = 1.0f - RF_VIS;
+  float VISLAM = o->VISLAM;
+  int NRW = A1(NROW, JM), JM1 = JM - 1, KP1 = 0, IP1 = 0;
+  (void)KMID;
+  /* stresses and heat flows with the final viscosity, mf:14123-14147 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        VISLAM = A3(VISC_LAM, I, J, K);
+        float VISTOT = A3(VISC_RAT, I, J, K) * VISLAM;
+        float THCOND = o->CP * VISTOT / o->PRANDTL;
+        float VISTOT2 = 2.0f * VISTOT;
+        D3(TXR, I, J, K) = VISTOT * D3(TXR, I, J, K);
+        D3(TXX, I, J, K) = VISTOT2 * D3(TXX, I, J, K);
+        D3(TXT, I, J, K) = VISTOT * D3(TXT, I, J, K);
+        D3(TRX, I, J, K) = D3(TXR, I, J, K);
+        D3(TRR, I, J, K) = VISTOT2 * D3(TRR, I, J, K);
+        D3(TRT, I, J, K) = VISTOT * D3(TRT, I, J, K);
+        D3(TTX, I, J, K) = D3(TXT, I, J, K);
+        D3(TTT, I, J, K) = VISTOT2 * D3(TTT, I, J, K);
+        D3(TTR, I, J, K) = D3(TRT, I, J, K);
+        D3(QXX, I, J, K) = -THCOND * D3(QXX, I, J, K);
+        D3(QRR, I, J, K) = -THCOND * D3(QRR, I, J, K);
+        D3(QTT, I, J, K) = -THCOND * D3(QTT, I, J, K);
+      }
+  const float VISLAM16 = VISLAM * 16.0f;
+  /* blade surfaces and the I = const faces, mf:14159-14500.  NRW and JM1
+   * keep their last values where the reference does not reset them. */
+  for (int K = 1; K <= KMM1; K++) {
+    KP1 = K + 1;
+    for (int J = 2; J <= JM; J++) {
+      if (A1(IND, J) != 0) {
+        JM1 = J - 1;
+        NRW = A1(NROW, J);
+        if (!((A1(KTIPS, NRW) > 0) && (K >= A1(KTIPS, NRW)) && (K < A1(KTIPE, NRW)))) {
+          /* I = 1 surface */
+          float VXAVG1 = A3(VX, 2, J, K) + A3(VX, 2, JM1, K) + A3(VX, 2, JM1, KP1) + A3(VX, 2, J, KP1);
+          float VRAVG1 = A3(VR, 2, J, K) + A3(VR, 2, JM1, K) + A3(VR, 2, JM1, KP1) + A3(VR, 2, J, KP1);
+          float WTAVG1 = A3(WT, 2, J, K) + A3(WT, 2, JM1, K) + A3(WT, 2, JM1, KP1) + A3(WT, 2, J, KP1);
+          float ROAVG1 = A3(RO, 2, J, K) + A3(RO, 2, JM1, K) + A3(RO, 2, JM1, KP1) + A3(RO, 2, J, KP1);
+          float WAVG = sqrtf(WTAVG1 * WTAVG1 + VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1);
+          float bx = A3(ABX, 1, J, K), br = A3(ABR, 1, J, K), bt = AJK(ABT, J, K);
+          float AREA1 = sqrtf(bx * bx + br * br + bt * bt);
+          float DPERP = A3(VOL, 1, J, K) / AREA1;
+          float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+          float CF = sa_cf(RE, DPERP, A1(ROUGH_L, NRW), ROAVG1, WAVG, VISLAM);
+          float TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+          float VSTAR = sqrtf(TWALLI1 / (0.25f * ROAVG1));
+          for (int I = 1; I <= IMID; I++) {
+            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
+            A3(Y_PLUS, I, J, K) = fminf(1000.0f, YPLS);
+          }
+          float FMULT = -TWALLI1 * AREA1 / WAVG;
+          float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
+          float WVISC1 = TFORCE1 * A1(WRAD, J) * AJK(RAVG_CELL, J, K);
+          D3(FORCEX, 1, J, K) = D3(FORCEX, 1, J, K) + XFORCE1;
+          D3(FORCER, 1, J, K) = D3(FORCER, 1, J, K) + RFORCE1;
+          D3(FORCET, 1, J, K) = D3(FORCET, 1, J, K) + TFORCE1;
+          D3(ESOURCE, 1, J, K) = D3(ESOURCE, 1, J, K) + WVISC1;
+          /* I = IM surface */
+          float VXAVGIM = A3(VX, IMM1, J, K) + A3(VX, IMM1, JM1, K) + A3(VX, IMM1, JM1, KP1) + A3(VX, IMM1, J, KP1);
+          float VRAVGIM = A3(VR, IMM1, J, K) + A3(VR, IMM1, JM1, K) + A3(VR, IMM1, JM1, KP1) + A3(VR, IMM1, J, KP1);
+          float WTAVGIM = A3(WT, IMM1, J, K) + A3(WT, IMM1, JM1, K) + A3(WT, IMM1, JM1, KP1) + A3(WT, IMM1, J, KP1);
+          float ROAVGIM = A3(RO, IMM1, J, K) + A3(RO, IMM1, JM1, K) + A3(RO, IMM1, JM1, KP1) + A3(RO, IMM1, J, KP1);
+          WAVG = sqrtf(WTAVGIM * WTAVGIM + VXAVGIM * VXAVGIM + VRAVGIM * VRAVGIM);
+          bx = A3(ABX, IM, J, K); br = A3(ABR, IM, J, K);
+          float AREAIM = sqrtf(bx * bx + br * br + bt * bt);
+          DPERP = A3(VOL, IMM1, J, K) / AREAIM;
+          RE = DPERP * ROAVGIM * WAVG / VISLAM16;
+          CF = sa_cf(RE, DPERP, A1(ROUGH_U, NRW), ROAVGIM, WAVG, VISLAM);
+          float TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+          VSTAR = sqrtf(TWALLIM / (0.25f * ROAVGIM));
+          for (int I = IMID + 1; I <= (newloss ? IMM1 : IM); I++) {
+            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGIM) / VISLAM;
+            A3(Y_PLUS, I, J, K) = fminf(1000.f, YPLS);
+          }
+          FMULT = -TWALLIM * AREAIM / WAVG;
+          float XFORCEIM = FMULT * VXAVGIM, RFORCEIM = FMULT * VRAVGIM, TFORCEIM = FMULT * WTAVGIM;
+          float WVISCIM = TFORCEIM * A1(WRAD, J) * AJK(RAVG_CELL, J, K);
+          D3(FORCEX, IMM1, J, K) = D3(FORCEX, IMM1, J, K) + XFORCEIM;
+          D3(FORCER, IMM1, J, K) = D3(FORCER, IMM1, J, K) + RFORCEIM;
+          D3(FORCET, IMM1, J, K) = D3(FORCET, IMM1, J, K) + TFORCEIM;
+          D3(ESOURCE, IMM1, J, K) = D3(ESOURCE, IMM1, J, K) + WVISCIM;
+        }
+      }
+      /* periodic cells I = 1 and IM-1 (skipped on solid blade surfaces),
+       * mf:14434-14486.  The test reads (IND(J)=1 .AND. K<KTIPS) .OR.
+       * K>KTIPE, as the reference's operator precedence makes it. */
+      if (!(((A1(IND, J) == 1) && (K < A1(KTIPS, NRW))) || (K > A1(KTIPE, NRW)))) {
+        int I = 1, IM1 = IMM1;
+        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TXT, I, J, K) + D3(TXT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TXX, I, J, K) + D3(TXX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCEX, IM1, J, K) = D3(FORCEX, IM1, J, K) + XFOR;
+        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
+        float AVG = 0.5f * (D3(FORCEX, I, J, K) + D3(FORCEX, IM1, J, K));
+        D3(FORCEX, I, J, K) = AVG;
+        D3(FORCEX, IM1, J, K) = AVG;
+        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TRT, I, J, K) + D3(TRT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TRX, I, J, K) + D3(TRX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCER, IM1, J, K) = D3(FORCER, IM1, J, K) + RFOR;
+        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
+        AVG = 0.5f * (D3(FORCER, I, J, K) + D3(FORCER, IM1, J, K));
+        D3(FORCER, I, J, K) = AVG;
+        D3(FORCER, IM1, J, K) = AVG;
+        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TTT, I, J, K) + D3(TTT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TTX, I, J, K) + D3(TTX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCET, IM1, J, K) = D3(FORCET, IM1, J, K) + TFOR;
+        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
+        AVG = 0.5f * (D3(FORCET, I, J, K) + D3(FORCET, IM1, J, K));
+        D3(FORCET, I, J, K) = AVG;
+        D3(FORCET, IM1, J, K) = AVG;
+        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, IM1, J, K)) * A3(ABX, I, J, K) +
+                              (D3(QRR, I, J, K) + D3(QRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                              (D3(QTT, I, J, K) + D3(QTT, IM1, J, K)) * AJK(ABT, J, K));
+        float VXAVG = A3(VX, I, J, K) + A3(VX, I, JM1, K) + A3(VX, I, JM1, KP1) + A3(VX, I, J, KP1);
+        float VRAVG = A3(VR, I, J, K) + A3(VR, I, JM1, K) + A3(VR, I, JM1, KP1) + A3(VR, I, J, KP1);
+        float VTAVG = A3(VT, I, J, K) + A3(VT, I, JM1, K) + A3(VT, I, JM1, KP1) + A3(VT, I, J, KP1);
+        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
+        D3(ESOURCE, IM1, J, K) = D3(ESOURCE, IM1, J, K) - QFLOW - WVISC;
+        AVG = 0.5f * (D3(ESOURCE, I, J, K) + D3(ESOURCE, IM1, J, K));
+        D3(ESOURCE, I, J, K) = AVG;
+        D3(ESOURCE, IM1, J, K) = AVG;
+      }
+      for (int I = 2; I <= IMM1; I++) {
+        int IM1 = I - 1;
+        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TXT, I, J, K) + D3(TXT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TXX, I, J, K) + D3(TXX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCEX, IM1, J, K) = D3(FORCEX, IM1, J, K) + XFOR;
+        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
+        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TRT, I, J, K) + D3(TRT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TRX, I, J, K) + D3(TRX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCER, IM1, J, K) = D3(FORCER, IM1, J, K) + RFOR;
+        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
+        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, IM1, J, K)) * A3(ABR, I, J, K) +
+                             (D3(TTT, I, J, K) + D3(TTT, IM1, J, K)) * AJK(ABT, J, K) +
+                             (D3(TTX, I, J, K) + D3(TTX, IM1, J, K)) * A3(ABX, I, J, K));
+        D3(FORCET, IM1, J, K) = D3(FORCET, IM1, J, K) + TFOR;
+        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
+        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, IM1, J, K)) * A3(ABX, I, J, K) +
+                              (D3(QRR, I, J, K) + D3(QRR, IM1, J, K)) * A3(ABR, I, J, K) +
+                              (D3(QTT, I, J, K) + D3(QTT, IM1, J, K)) * AJK(ABT, J, K));
+        float VXAVG = A3(VX, I, J, K) + A3(VX, I, JM1, K) + A3(VX, I, JM1, KP1) + A3(VX, I, J, KP1);
+        float VRAVG = A3(VR, I, J, K) + A3(VR, I, JM1, K) + A3(VR, I, JM1, KP1) + A3(VR, I, J, KP1);
+        float VTAVG = A3(VT, I, J, K) + A3(VT, I, JM1, K) + A3(VT, I, JM1, KP1) + A3(VT, I, J, KP1);
+        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
+        D3(ESOURCE, IM1, J, K) = D3(ESOURCE, IM1, J, K) - QFLOW - WVISC;
+      }
+    }
+  }
+  /* hub and casing surfaces and the K = const faces, mf:14505-14805 */
+  for (int J = 2; J <= JM; J++) {
+    JM1 = J - 1;
+    if (newloss) NRW = A1(NROW, J);
+    for (int I = 1; I <= IMM1; I++) {
+      IP1 = I + 1;
+      float VXAVG1 = A3(VX, I, J, 2) + A3(VX, IP1, J, 2) + A3(VX, IP1, JM1, 2) + A3(VX, I, JM1, 2);
+      float VRAVG1 = A3(VR, I, J, 2) + A3(VR, IP1, J, 2) + A3(VR, IP1, JM1, 2) + A3(VR, I, JM1, 2);
+      float VTAVG1 = A3(VT, I, J, 2) + A3(VT, IP1, J, 2) + A3(VT, IP1, JM1, 2) + A3(VT, I, JM1, 2);
+      float ROAVG1 = A3(RO, I, J, 2) + A3(RO, IP1, J, 2) + A3(RO, IP1, JM1, 2) + A3(RO, I, JM1, 2);
+      float RAVG1 = AJK(R, J, 2) + AJK(R, J, 2) + AJK(R, JM1, 2) + AJK(R, JM1, 2);
+      float WTAVG1 = VTAVG1 - A1(WHUB, J) * RAVG1;
+      float WAVG = sqrtf(VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1 + WTAVG1 * WTAVG1);
+      float sx = A3(ASX, I, J, 1), sr = A3(ASR, I, J, 1);
+      float AREA1 = sqrtf(sx * sx + sr * sr);
+      float DPERP = A3(VOL, I, J, 1) / AREA1;
+      float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+      float CF = sa_cf(RE, DPERP, A1(ROUGH_H, NRW), ROAVG1, WAVG, VISLAM);
+      float TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      if (o->IBOUND == 0 || o->IBOUND == 2) {
+        for (int K = 1; K <= KMID; K++) {
+          float VSTAR = sqrtf(TWALLK1 / (0.25f * ROAVG1));
+          float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
+          A3(Y_PLUS, I, J, K) = fminf(YPLS, A3(Y_PLUS, I, J, K));
+        }
+      }
+      float FMULT = -TWALLK1 * AREA1 / WAVG;
+      if (o->IBOUND == 1 || o->IBOUND > 2) FMULT = 0.0f;
+      float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
+      float WVISC1 = TFORCE1 * 0.25f * A1(WHUB, J) * RAVG1;
+      D3(FORCEX, I, J, 1) = D3(FORCEX, I, J, 1) + XFORCE1;
+      D3(FORCER, I, J, 1) = D3(FORCER, I, J, 1) + RFORCE1;
+      D3(FORCET, I, J, 1) = D3(FORCET, I, J, 1) + TFORCE1;
+      D3(ESOURCE, I, J, 1) = D3(ESOURCE, I, J, 1) + WVISC1;
+      float VXAVGKM = A3(VX, I, J, KMM1) + A3(VX, IP1, J, KMM1) + A3(VX, IP1, JM1, KMM1) + A3(VX, I, JM1, KMM1);
+      float VRAVGKM = A3(VR, I, J, KMM1) + A3(VR, IP1, J, KMM1) + A3(VR, IP1, JM1, KMM1) + A3(VR, I, JM1, KMM1);
+      float VTAVGKM = A3(VT, I, J, KMM1) + A3(VT, IP1, J, KMM1) + A3(VT, IP1, JM1, KMM1) + A3(VT, I, JM1, KMM1);
+      float ROAVGKM = A3(RO, I, J, KMM1) + A3(RO, IP1, J, KMM1) + A3(RO, IP1, JM1, KMM1) + A3(RO, I, JM1, KMM1);
+      float RAVGKM = AJK(R, J, KMM1) + AJK(R, J, KMM1) + AJK(R, JM1, KMM1) + AJK(R, JM1, KMM1);
+      float WTAVGKM = VTAVGKM - A1(WTIP, J) * RAVGKM;
+      WAVG = sqrtf(VXAVGKM * VXAVGKM + VRAVGKM * VRAVGKM + WTAVGKM * WTAVGKM);
+      sx = A3(ASX, I, J, KM); sr = A3(ASR, I, J, KM);
+      float AREAKM = sqrtf(sx * sx + sr * sr);
+      DPERP = A3(VOL, I, J, KMM1) / AREAKM;
+      RE = DPERP * ROAVGKM * WAVG / VISLAM16;
+      CF = sa_cf(RE, DPERP, A1(ROUGH_T, NRW), ROAVGKM, WAVG, VISLAM);
+      float TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+      if (o->IBOUND == 0 || o->IBOUND == 1) {
+        for (int K = KMID + 1; K <= KMM1; K++) {
+          float VSTAR = sqrtf(TWALLKM / (0.25f * ROAVGKM));
+          float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGKM) / VISLAM;
+          A3(Y_PLUS, I, J, K) = fminf(YPLS, A3(Y_PLUS, I, J, K));
+        }
+      }
+      FMULT = -TWALLKM * AREAKM / WAVG;
+      if (o->IBOUND >= 2) FMULT = 0.0f;
+      float XFORCEKM = FMULT * VXAVGKM, RFORCEKM = FMULT * VRAVGKM, TFORCEKM = FMULT * WTAVGKM;
+      float WVISCKM = TFORCEKM * 0.25f * A1(WTIP, J) * RAVGKM;
+      D3(FORCEX, I, J, KMM1) = D3(FORCEX, I, J, KMM1) + XFORCEKM;
+      D3(FORCER, I, J, KMM1) = D3(FORCER, I, J, KMM1) + RFORCEKM;
+      D3(FORCET, I, J, KMM1) = D3(FORCET, I, J, KMM1) + TFORCEKM;
+      D3(ESOURCE, I, J, KMM1) = D3(ESOURCE, I, J, KMM1) + WVISCKM;
+      for (int K = 2; K <= KMM1; K++) {
+        int KM1 = K - 1;
+        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, I, J, KM1)) * A3(ASR, I, J, K) +
+                             (D3(TXX, I, J, K) + D3(TXX, I, J, KM1)) * A3(ASX, I, J, K));
+        D3(FORCEX, I, J, KM1) = D3(FORCEX, I, J, KM1) + XFOR;
+        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
+        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, I, J, KM1)) * A3(ASR, I, J, K) +
+                             (D3(TRX, I, J, K) + D3(TRX, I, J, KM1)) * A3(ASX, I, J, K));
+        D3(FORCER, I, J, KM1) = D3(FORCER, I, J, KM1) + RFOR;
+        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
+        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, I, J, KM1)) * A3(ASR, I, J, K) +
+                             (D3(TTX, I, J, K) + D3(TTX, I, J, KM1)) * A3(ASX, I, J, K));
+        D3(FORCET, I, J, KM1) = D3(FORCET, I, J, KM1) + TFOR;
+        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
+        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, I, J, KM1)) * A3(ASX, I, J, K) +
+                              (D3(QRR, I, J, K) + D3(QRR, I, J, KM1)) * A3(ASR, I, J, K));
+        float VXAVG = A3(VX, I, J, K) + A3(VX, IP1, J, K) + A3(VX, IP1, JM1, K) + A3(VX, I, JM1, K);
+        float VRAVG = A3(VR, I, J, K) + A3(VR, IP1, J, K) + A3(VR, IP1, JM1, K) + A3(VR, I, JM1, K);
+        float VTAVG = A3(VT, I, J, K) + A3(VT, IP1, J, K) + A3(VT, IP1, JM1, K) + A3(VT, I, JM1, K);
+        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
+        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
+        D3(ESOURCE, I, J, KM1) = D3(ESOURCE, I, J, KM1) - QFLOW - WVISC;
+      }
+    }
+  }
+  /* body forces, mf:14812-14819 */
+  for (int K = 1; K <= KMM1; K++)
+    for (int J = 2; J <= JM; J++)
+      for (int I = 1; I <= IMM1; I++) {
+        A3(XFORCE, I, J, K) = RF_VIS1 * A3(XFORCE, I, J, K) - RF_VIS * D3(FORCEX, I, J, K);
+        A3(TFORCE, I, J, K) = RF_VIS1 * A3(TFORCE, I, J, K) - RF_VIS * D3(FORCET, I, J, K) * AJK(RAVG_CELL, J, K);
+        A3(RFORCE, I, J, K) = RF_VIS1 * A3(RFORCE, I, J, K) - RF_VIS * D3(FORCER, I, J, K);
+        A3(QSOURCE, I, J, K) = RF_VIS1 * A3(QSOURCE, I, J, K) - RF_VIS * D3(ESOURCE, I, J, K);
+      }
+  free(TRX); free(TTX); free(TTR); free(FORCEX); free(FORCER); free(FORCET); free(ESOURCE);
+  return VISLAM;
+}
+
 void orc_spal_loss(orc_t *o) {
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID, KMID = o->KMID;
   const size_t N = (size_t)IM * JM * KM;
   float *TXX = calloc(N, 4), *TXR = calloc(N, 4), *TXT = calloc(N, 4), *TRR = calloc(N, 4), *TRT = calloc(N, 4);
   float *TTT = calloc(N, 4), *QXX = calloc(N, 4), *QRR = calloc(N, 4), *QTT = calloc(N, 4);
-  float *TRX = calloc(N, 4), *TTX = calloc(N, 4), *TTR = calloc(N, 4);
-  float *FORCEX = calloc(N, 4), *FORCER = calloc(N, 4), *FORCET = calloc(N, 4), *ESOURCE = calloc(N, 4);
   const float RF_VIS = o->RF_VIS, RF_VIS1 = 1.0f - RF_VIS;
   const int NDAMP = 100000;
   float VISLAM = o->VISLAM;
@@ -1166,256 +1440,161 @@ void orc_spal_loss(orc_t *o) {
       }
     }
   }
-  /* stresses and heat flows with the final viscosity, mf:14123-14147 */
-  for (int K = 1; K <= KMM1; K++)
-    for (int J = 2; J <= JM; J++)
-      for (int I = 1; I <= IMM1; I++) {
-        VISLAM = A3(VISC_LAM, I, J, K);
-        float VISTOT = A3(VISC_RAT, I, J, K) * VISLAM;
-        float THCOND = o->CP * VISTOT / o->PRANDTL;
-        float VISTOT2 = 2.0f * VISTOT;
-        D3(TXR, I, J, K) = VISTOT * D3(TXR, I, J, K);
-        D3(TXX, I, J, K) = VISTOT2 * D3(TXX, I, J, K);
-        D3(TXT, I, J, K) = VISTOT * D3(TXT, I, J, K);
-        D3(TRX, I, J, K) = D3(TXR, I, J, K);
-        D3(TRR, I, J, K) = VISTOT2 * D3(TRR, I, J, K);
-        D3(TRT, I, J, K) = VISTOT * D3(TRT, I, J, K);
-        D3(TTX, I, J, K) = D3(TXT, I, J, K);
-        D3(TTT, I, J, K) = VISTOT2 * D3(TTT, I, J, K);
-        D3(TTR, I, J, K) = D3(TRT, I, J, K);
-        D3(QXX, I, J, K) = -THCOND * D3(QXX, I, J, K);
-        D3(QRR, I, J, K) = -THCOND * D3(QRR, I, J, K);
-        D3(QTT, I, J, K) = -THCOND * D3(QTT, I, J, K);
-      }
-  const float VISLAM16 = VISLAM * 16.0f;
-  /* blade surfaces and the I = const faces, mf:14159-14500.  NRW and JM1
-   * keep their last values where the reference does not reset them. */
-  for (int K = 1; K <= KMM1; K++) {
-    KP1 = K + 1;
-    for (int J = 2; J <= JM; J++) {
-      if (A1(IND, J) != 0) {
-        JM1 = J - 1;
-        NRW = A1(NROW, J);
-        if (!((A1(KTIPS, NRW) > 0) && (K >= A1(KTIPS, NRW)) && (K < A1(KTIPE, NRW)))) {
-          /* I = 1 surface */
-          float VXAVG1 = A3(VX, 2, J, K) + A3(VX, 2, JM1, K) + A3(VX, 2, JM1, KP1) + A3(VX, 2, J, KP1);
-          float VRAVG1 = A3(VR, 2, J, K) + A3(VR, 2, JM1, K) + A3(VR, 2, JM1, KP1) + A3(VR, 2, J, KP1);
-          float WTAVG1 = A3(WT, 2, J, K) + A3(WT, 2, JM1, K) + A3(WT, 2, JM1, KP1) + A3(WT, 2, J, KP1);
-          float ROAVG1 = A3(RO, 2, J, K) + A3(RO, 2, JM1, K) + A3(RO, 2, JM1, KP1) + A3(RO, 2, J, KP1);
-          float WAVG = sqrtf(WTAVG1 * WTAVG1 + VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1);
-          float bx = A3(ABX, 1, J, K), br = A3(ABR, 1, J, K), bt = AJK(ABT, J, K);
-          float AREA1 = sqrtf(bx * bx + br * br + bt * bt);
-          float DPERP = A3(VOL, 1, J, K) / AREA1;
-          float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-          float CF = sa_cf(RE, DPERP, A1(ROUGH_L, NRW), ROAVG1, WAVG, VISLAM);
-          float TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
-          float VSTAR = sqrtf(TWALLI1 / (0.25f * ROAVG1));
-          for (int I = 1; I <= IMID; I++) {
-            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
-            A3(Y_PLUS, I, J, K) = fminf(1000.0f, YPLS);
-          }
-          float FMULT = -TWALLI1 * AREA1 / WAVG;
-          float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
-          float WVISC1 = TFORCE1 * A1(WRAD, J) * AJK(RAVG_CELL, J, K);
-          D3(FORCEX, 1, J, K) = D3(FORCEX, 1, J, K) + XFORCE1;
-          D3(FORCER, 1, J, K) = D3(FORCER, 1, J, K) + RFORCE1;
-          D3(FORCET, 1, J, K) = D3(FORCET, 1, J, K) + TFORCE1;
-          D3(ESOURCE, 1, J, K) = D3(ESOURCE, 1, J, K) + WVISC1;
-          /* I = IM surface */
-          float VXAVGIM = A3(VX, IMM1, J, K) + A3(VX, IMM1, JM1, K) + A3(VX, IMM1, JM1, KP1) + A3(VX, IMM1, J, KP1);
-          float VRAVGIM = A3(VR, IMM1, J, K) + A3(VR, IMM1, JM1, K) + A3(VR, IMM1, JM1, KP1) + A3(VR, IMM1, J, KP1);
-          float WTAVGIM = A3(WT, IMM1, J, K) + A3(WT, IMM1, JM1, K) + A3(WT, IMM1, JM1, KP1) + A3(WT, IMM1, J, KP1);
-          float ROAVGIM = A3(RO, IMM1, J, K) + A3(RO, IMM1, JM1, K) + A3(RO, IMM1, JM1, KP1) + A3(RO, IMM1, J, KP1);
-          WAVG = sqrtf(WTAVGIM * WTAVGIM + VXAVGIM * VXAVGIM + VRAVGIM * VRAVGIM);
-          bx = A3(ABX, IM, J, K); br = A3(ABR, IM, J, K);
-          float AREAIM = sqrtf(bx * bx + br * br + bt * bt);
-          DPERP = A3(VOL, IMM1, J, K) / AREAIM;
-          RE = DPERP * ROAVGIM * WAVG / VISLAM16;
-          CF = sa_cf(RE, DPERP, A1(ROUGH_U, NRW), ROAVGIM, WAVG, VISLAM);
-          float TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
-          VSTAR = sqrtf(TWALLIM / (0.25f * ROAVGIM));
-          for (int I = IMID + 1; I <= IM; I++) {
-            float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGIM) / VISLAM;
-            A3(Y_PLUS, I, J, K) = fminf(1000.f, YPLS);
-          }
-          FMULT = -TWALLIM * AREAIM / WAVG;
-          float XFORCEIM = FMULT * VXAVGIM, RFORCEIM = FMULT * VRAVGIM, TFORCEIM = FMULT * WTAVGIM;
-          float WVISCIM = TFORCEIM * A1(WRAD, J) * AJK(RAVG_CELL, J, K);
-          D3(FORCEX, IMM1, J, K) = D3(FORCEX, IMM1, J, K) + XFORCEIM;
-          D3(FORCER, IMM1, J, K) = D3(FORCER, IMM1, J, K) + RFORCEIM;
-          D3(FORCET, IMM1, J, K) = D3(FORCET, IMM1, J, K) + TFORCEIM;
-          D3(ESOURCE, IMM1, J, K) = D3(ESOURCE, IMM1, J, K) + WVISCIM;
-        }
-      }
-      /* periodic cells I = 1 and IM-1 (skipped on solid blade surfaces),
-       * mf:14434-14486.  The test reads (IND(J)=1 .AND. K<KTIPS) .OR.
-       * K>KTIPE, as the reference's operator precedence makes it. */
-      if (!(((A1(IND, J) == 1) && (K < A1(KTIPS, NRW))) || (K > A1(KTIPE, NRW)))) {
-        int I = 1, IM1 = IMM1;
-        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, IM1, J, K)) * A3(ABR, I, J, K) +
-                             (D3(TXT, I, J, K) + D3(TXT, IM1, J, K)) * AJK(ABT, J, K) +
-                             (D3(TXX, I, J, K) + D3(TXX, IM1, J, K)) * A3(ABX, I, J, K));
-        D3(FORCEX, IM1, J, K) = D3(FORCEX, IM1, J, K) + XFOR;
-        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
-        float AVG = 0.5f * (D3(FORCEX, I, J, K) + D3(FORCEX, IM1, J, K));
-        D3(FORCEX, I, J, K) = AVG;
-        D3(FORCEX, IM1, J, K) = AVG;
-        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, IM1, J, K)) * A3(ABR, I, J, K) +
-                             (D3(TRT, I, J, K) + D3(TRT, IM1, J, K)) * AJK(ABT, J, K) +
-                             (D3(TRX, I, J, K) + D3(TRX, IM1, J, K)) * A3(ABX, I, J, K));
-        D3(FORCER, IM1, J, K) = D3(FORCER, IM1, J, K) + RFOR;
-        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
-        AVG = 0.5f * (D3(FORCER, I, J, K) + D3(FORCER, IM1, J, K));
-        D3(FORCER, I, J, K) = AVG;
-        D3(FORCER, IM1, J, K) = AVG;
-        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, IM1, J, K)) * A3(ABR, I, J, K) +
-                             (D3(TTT, I, J, K) + D3(TTT, IM1, J, K)) * AJK(ABT, J, K) +
-                             (D3(TTX, I, J, K) + D3(TTX, IM1, J, K)) * A3(ABX, I, J, K));
-        D3(FORCET, IM1, J, K) = D3(FORCET, IM1, J, K) + TFOR;
-        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
-        AVG = 0.5f * (D3(FORCET, I, J, K) + D3(FORCET, IM1, J, K));
-        D3(FORCET, I, J, K) = AVG;
-        D3(FORCET, IM1, J, K) = AVG;
-        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, IM1, J, K)) * A3(ABX, I, J, K) +
-                              (D3(QRR, I, J, K) + D3(QRR, IM1, J, K)) * A3(ABR, I, J, K) +
-                              (D3(QTT, I, J, K) + D3(QTT, IM1, J, K)) * AJK(ABT, J, K));
-        float VXAVG = A3(VX, I, J, K) + A3(VX, I, JM1, K) + A3(VX, I, JM1, KP1) + A3(VX, I, J, KP1);
-        float VRAVG = A3(VR, I, J, K) + A3(VR, I, JM1, K) + A3(VR, I, JM1, KP1) + A3(VR, I, J, KP1);
-        float VTAVG = A3(VT, I, J, K) + A3(VT, I, JM1, K) + A3(VT, I, JM1, KP1) + A3(VT, I, J, KP1);
-        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
-        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
-        D3(ESOURCE, IM1, J, K) = D3(ESOURCE, IM1, J, K) - QFLOW - WVISC;
-        AVG = 0.5f * (D3(ESOURCE, I, J, K) + D3(ESOURCE, IM1, J, K));
-        D3(ESOURCE, I, J, K) = AVG;
-        D3(ESOURCE, IM1, J, K) = AVG;
-      }
-      for (int I = 2; I <= IMM1; I++) {
-        int IM1 = I - 1;
-        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, IM1, J, K)) * A3(ABR, I, J, K) +
-                             (D3(TXT, I, J, K) + D3(TXT, IM1, J, K)) * AJK(ABT, J, K) +
-                             (D3(TXX, I, J, K) + D3(TXX, IM1, J, K)) * A3(ABX, I, J, K));
-        D3(FORCEX, IM1, J, K) = D3(FORCEX, IM1, J, K) + XFOR;
-        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
-        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, IM1, J, K)) * A3(ABR, I, J, K) +
-                             (D3(TRT, I, J, K) + D3(TRT, IM1, J, K)) * AJK(ABT, J, K) +
-                             (D3(TRX, I, J, K) + D3(TRX, IM1, J, K)) * A3(ABX, I, J, K));
-        D3(FORCER, IM1, J, K) = D3(FORCER, IM1, J, K) + RFOR;
-        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
-        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, IM1, J, K)) * A3(ABR, I, J, K) +
-                             (D3(TTT, I, J, K) + D3(TTT, IM1, J, K)) * AJK(ABT, J, K) +
-                             (D3(TTX, I, J, K) + D3(TTX, IM1, J, K)) * A3(ABX, I, J, K));
-        D3(FORCET, IM1, J, K) = D3(FORCET, IM1, J, K) + TFOR;
-        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
-        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, IM1, J, K)) * A3(ABX, I, J, K) +
-                              (D3(QRR, I, J, K) + D3(QRR, IM1, J, K)) * A3(ABR, I, J, K) +
-                              (D3(QTT, I, J, K) + D3(QTT, IM1, J, K)) * AJK(ABT, J, K));
-        float VXAVG = A3(VX, I, J, K) + A3(VX, I, JM1, K) + A3(VX, I, JM1, KP1) + A3(VX, I, J, KP1);
-        float VRAVG = A3(VR, I, J, K) + A3(VR, I, JM1, K) + A3(VR, I, JM1, KP1) + A3(VR, I, J, KP1);
-        float VTAVG = A3(VT, I, J, K) + A3(VT, I, JM1, K) + A3(VT, I, JM1, KP1) + A3(VT, I, J, KP1);
-        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
-        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
-        D3(ESOURCE, IM1, J, K) = D3(ESOURCE, IM1, J, K) - QFLOW - WVISC;
-      }
-    }
-  }
-  /* hub and casing surfaces and the K = const faces, mf:14505-14805 */
-  for (int J = 2; J <= JM; J++) {
-    JM1 = J - 1;
-    for (int I = 1; I <= IMM1; I++) {
-      IP1 = I + 1;
-      float VXAVG1 = A3(VX, I, J, 2) + A3(VX, IP1, J, 2) + A3(VX, IP1, JM1, 2) + A3(VX, I, JM1, 2);
-      float VRAVG1 = A3(VR, I, J, 2) + A3(VR, IP1, J, 2) + A3(VR, IP1, JM1, 2) + A3(VR, I, JM1, 2);
-      float VTAVG1 = A3(VT, I, J, 2) + A3(VT, IP1, J, 2) + A3(VT, IP1, JM1, 2) + A3(VT, I, JM1, 2);
-      float ROAVG1 = A3(RO, I, J, 2) + A3(RO, IP1, J, 2) + A3(RO, IP1, JM1, 2) + A3(RO, I, JM1, 2);
-      float RAVG1 = AJK(R, J, 2) + AJK(R, J, 2) + AJK(R, JM1, 2) + AJK(R, JM1, 2);
-      float WTAVG1 = VTAVG1 - A1(WHUB, J) * RAVG1;
-      float WAVG = sqrtf(VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1 + WTAVG1 * WTAVG1);
-      float sx = A3(ASX, I, J, 1), sr = A3(ASR, I, J, 1);
-      float AREA1 = sqrtf(sx * sx + sr * sr);
-      float DPERP = A3(VOL, I, J, 1) / AREA1;
-      float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-      float CF = sa_cf(RE, DPERP, A1(ROUGH_H, NRW), ROAVG1, WAVG, VISLAM);
-      float TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
-      if (o->IBOUND == 0 || o->IBOUND == 2) {
-        for (int K = 1; K <= KMID; K++) {
-          float VSTAR = sqrtf(TWALLK1 / (0.25f * ROAVG1));
-          float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
-          A3(Y_PLUS, I, J, K) = fminf(YPLS, A3(Y_PLUS, I, J, K));
-        }
-      }
-      float FMULT = -TWALLK1 * AREA1 / WAVG;
-      if (o->IBOUND == 1 || o->IBOUND > 2) FMULT = 0.0f;
-      float XFORCE1 = FMULT * VXAVG1, RFORCE1 = FMULT * VRAVG1, TFORCE1 = FMULT * WTAVG1;
-      float WVISC1 = TFORCE1 * 0.25f * A1(WHUB, J) * RAVG1;
-      D3(FORCEX, I, J, 1) = D3(FORCEX, I, J, 1) + XFORCE1;
-      D3(FORCER, I, J, 1) = D3(FORCER, I, J, 1) + RFORCE1;
-      D3(FORCET, I, J, 1) = D3(FORCET, I, J, 1) + TFORCE1;
-      D3(ESOURCE, I, J, 1) = D3(ESOURCE, I, J, 1) + WVISC1;
-      float VXAVGKM = A3(VX, I, J, KMM1) + A3(VX, IP1, J, KMM1) + A3(VX, IP1, JM1, KMM1) + A3(VX, I, JM1, KMM1);
-      float VRAVGKM = A3(VR, I, J, KMM1) + A3(VR, IP1, J, KMM1) + A3(VR, IP1, JM1, KMM1) + A3(VR, I, JM1, KMM1);
-      float VTAVGKM = A3(VT, I, J, KMM1) + A3(VT, IP1, J, KMM1) + A3(VT, IP1, JM1, KMM1) + A3(VT, I, JM1, KMM1);
-      float ROAVGKM = A3(RO, I, J, KMM1) + A3(RO, IP1, J, KMM1) + A3(RO, IP1, JM1, KMM1) + A3(RO, I, JM1, KMM1);
-      float RAVGKM = AJK(R, J, KMM1) + AJK(R, J, KMM1) + AJK(R, JM1, KMM1) + AJK(R, JM1, KMM1);
-      float WTAVGKM = VTAVGKM - A1(WTIP, J) * RAVGKM;
-      WAVG = sqrtf(VXAVGKM * VXAVGKM + VRAVGKM * VRAVGKM + WTAVGKM * WTAVGKM);
-      sx = A3(ASX, I, J, KM); sr = A3(ASR, I, J, KM);
-      float AREAKM = sqrtf(sx * sx + sr * sr);
-      DPERP = A3(VOL, I, J, KMM1) / AREAKM;
-      RE = DPERP * ROAVGKM * WAVG / VISLAM16;
-      CF = sa_cf(RE, DPERP, A1(ROUGH_T, NRW), ROAVGKM, WAVG, VISLAM);
-      float TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
-      if (o->IBOUND == 0 || o->IBOUND == 1) {
-        for (int K = KMID + 1; K <= KMM1; K++) {
-          float VSTAR = sqrtf(TWALLKM / (0.25f * ROAVGKM));
-          float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGKM) / VISLAM;
-          A3(Y_PLUS, I, J, K) = fminf(YPLS, A3(Y_PLUS, I, J, K));
-        }
-      }
-      FMULT = -TWALLKM * AREAKM / WAVG;
-      if (o->IBOUND >= 2) FMULT = 0.0f;
-      float XFORCEKM = FMULT * VXAVGKM, RFORCEKM = FMULT * VRAVGKM, TFORCEKM = FMULT * WTAVGKM;
-      float WVISCKM = TFORCEKM * 0.25f * A1(WTIP, J) * RAVGKM;
-      D3(FORCEX, I, J, KMM1) = D3(FORCEX, I, J, KMM1) + XFORCEKM;
-      D3(FORCER, I, J, KMM1) = D3(FORCER, I, J, KMM1) + RFORCEKM;
-      D3(FORCET, I, J, KMM1) = D3(FORCET, I, J, KMM1) + TFORCEKM;
-      D3(ESOURCE, I, J, KMM1) = D3(ESOURCE, I, J, KMM1) + WVISCKM;
-      for (int K = 2; K <= KMM1; K++) {
-        int KM1 = K - 1;
-        float XFOR = 0.5f * ((D3(TXR, I, J, K) + D3(TXR, I, J, KM1)) * A3(ASR, I, J, K) +
-                             (D3(TXX, I, J, K) + D3(TXX, I, J, KM1)) * A3(ASX, I, J, K));
-        D3(FORCEX, I, J, KM1) = D3(FORCEX, I, J, KM1) + XFOR;
-        D3(FORCEX, I, J, K) = D3(FORCEX, I, J, K) - XFOR;
-        float RFOR = 0.5f * ((D3(TRR, I, J, K) + D3(TRR, I, J, KM1)) * A3(ASR, I, J, K) +
-                             (D3(TRX, I, J, K) + D3(TRX, I, J, KM1)) * A3(ASX, I, J, K));
-        D3(FORCER, I, J, KM1) = D3(FORCER, I, J, KM1) + RFOR;
-        D3(FORCER, I, J, K) = D3(FORCER, I, J, K) - RFOR;
-        float TFOR = 0.5f * ((D3(TTR, I, J, K) + D3(TTR, I, J, KM1)) * A3(ASR, I, J, K) +
-                             (D3(TTX, I, J, K) + D3(TTX, I, J, KM1)) * A3(ASX, I, J, K));
-        D3(FORCET, I, J, KM1) = D3(FORCET, I, J, KM1) + TFOR;
-        D3(FORCET, I, J, K) = D3(FORCET, I, J, K) - TFOR;
-        float QFLOW = 0.5f * ((D3(QXX, I, J, K) + D3(QXX, I, J, KM1)) * A3(ASX, I, J, K) +
-                              (D3(QRR, I, J, K) + D3(QRR, I, J, KM1)) * A3(ASR, I, J, K));
-        float VXAVG = A3(VX, I, J, K) + A3(VX, IP1, J, K) + A3(VX, IP1, JM1, K) + A3(VX, I, JM1, K);
-        float VRAVG = A3(VR, I, J, K) + A3(VR, IP1, J, K) + A3(VR, IP1, JM1, K) + A3(VR, I, JM1, K);
-        float VTAVG = A3(VT, I, J, K) + A3(VT, IP1, J, K) + A3(VT, IP1, JM1, K) + A3(VT, I, JM1, K);
-        float WVISC = -0.25f * (XFOR * VXAVG + RFOR * VRAVG + TFOR * VTAVG);
-        D3(ESOURCE, I, J, K) = D3(ESOURCE, I, J, K) + QFLOW + WVISC;
-        D3(ESOURCE, I, J, KM1) = D3(ESOURCE, I, J, KM1) - QFLOW - WVISC;
-      }
-    }
-  }
-  /* body forces, mf:14812-14819 */
-  for (int K = 1; K <= KMM1; K++)
-    for (int J = 2; J <= JM; J++)
-      for (int I = 1; I <= IMM1; I++) {
-        A3(XFORCE, I, J, K) = RF_VIS1 * A3(XFORCE, I, J, K) - RF_VIS * D3(FORCEX, I, J, K);
-        A3(TFORCE, I, J, K) = RF_VIS1 * A3(TFORCE, I, J, K) - RF_VIS * D3(FORCET, I, J, K) * AJK(RAVG_CELL, J, K);
-        A3(RFORCE, I, J, K) = RF_VIS1 * A3(RFORCE, I, J, K) - RF_VIS * D3(FORCER, I, J, K);
-        A3(QSOURCE, I, J, K) = RF_VIS1 * A3(QSOURCE, I, J, K) - RF_VIS * D3(ESOURCE, I, J, K);
-      }
+  VISLAM = visc_forces(o, TXR, TXX, TXT, TRR, TRT, TTT, QXX, QRR, QTT, 0);
   o->VISLAM = VISLAM;
   free(TXX); free(TXR); free(TXT); free(TRR); free(TRT); free(TTT); free(QXX); free(QRR); free(QTT);
-  free(TRX); free(TTX); free(TTR); free(FORCEX); free(FORCER); free(FORCET); free(ESOURCE);
+}
+
+/* ----------------------------------------------------------------------- */
+/* NEW_LOSS, mf:12522-13462: mixing-length eddy viscosity from the cell      */
+/* vorticity, damped through the buffer layer by y+, laminar regions set by  */
+/* the transition test, then the stress, wall-friction and force assembly    */
+/* it shares with SPAL_LOSS.  Restated for YPLUSWALL >= -0.001 (its Shih     */
+/* wall functions, mf:12849-12862, and SET_PWALLGRAD, mf:12812, are not).    */
+/* The Reynolds-number diagnostics (VISCOSY, ROVEXIT, REYNOLDS,              */
+/* mf:12645-12660) feed printed output only and are left out.                */
+
+void orc_new_loss(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID, KMID = o->KMID;
+  const size_t N = (size_t)IM * JM * KM;
+  float *TXX = calloc(N, 4), *TXR = calloc(N, 4), *TXT = calloc(N, 4), *TRR = calloc(N, 4), *TRT = calloc(N, 4);
+  float *TTT = calloc(N, 4), *QXX = calloc(N, 4), *QRR = calloc(N, 4), *QTT = calloc(N, 4);
+  float VISLAM = o->VISLAM;
+  /* laminar viscosity over the first quarter of the run, mf:12544-12570 */
+  if (o->NSTEP == 1 || o->NSTEP < o->NSTEPS_MAX / 4) {
+    if (o->REYNO > 100.f) {
+      int J1 = A1(JLE, 1), J2 = A1(JTE, 1);
+      if (A1(JLE, 1) > JM) J1 = 1;
+      if (A1(JTE, 1) > JM) J2 = JM;
+      float XCHORD = AJK(SMERID, J2, KMID) - AJK(SMERID, J1, KMID);
+      float a = A3(ROVX, IMID, J2, KMID), b = A3(ROVR, IMID, J2, KMID), c = A3(ROWT, IMID, J2, KMID);
+      float ROW2 = sqrtf(a * a + b * b + c * c);
+      VISLAM = XCHORD * ROW2 / o->REYNO;
+    }
+    if (o->REYNO > 0.0f && o->REYNO < 99.99f) VISLAM = o->REYNO / 100000.f;
+    if (o->REYNO < 0.0f) VISLAM = -o->REYNO * 1.0e-5f;
+    float THCOND = o->CP * VISLAM / o->PRANDTL;
+    o->FTCOND = THCOND / VISLAM;
+  }
+  /* strain rates and the turbulent viscosity of each cell, mf:12586-12672 */
+  float FYPLUS = 0.0f;
+  for (int K = 1; K <= KMM1; K++) {
+    const int KP1 = K + 1;
+    for (int J = 2; J <= JM; J++) {
+      const int JM1 = J - 1;
+      for (int I = 1; I <= IMM1; I++) {
+        const int IP1 = I + 1;
+        float WTAVG = 0.125f * (A3(WT, I, J, K) + A3(WT, IP1, J, K) + A3(WT, IP1, J, KP1) + A3(WT, I, J, KP1) +
+                                A3(WT, I, JM1, K) + A3(WT, IP1, JM1, K) + A3(WT, IP1, JM1, KP1) + A3(WT, I, JM1, KP1));
+        float ROAVG = A3(ROAVG_CELL, I, J, K);
+        float TSAVG = 0.125f * (A3(T_STATIC, I, J, K) + A3(T_STATIC, IP1, J, K) + A3(T_STATIC, IP1, J, KP1) +
+                                A3(T_STATIC, I, J, KP1) + A3(T_STATIC, I, JM1, K) + A3(T_STATIC, IP1, JM1, K) +
+                                A3(T_STATIC, IP1, JM1, KP1) + A3(T_STATIC, I, JM1, KP1));
+        float RAVG = AJK(RAVG_CELL, J, K);
+        float DVXDX, DVXDR, DVXDT, DVRDX, DVRDR, DVRDT, DWTDX, DWTDR, DWTDT, DTDX, DTDR, DTDT;
+        gradvel(o, o->VX, I, J, K, &DVXDX, &DVXDR, &DVXDT);
+        gradvel(o, o->VR, I, J, K, &DVRDX, &DVRDR, &DVRDT);
+        gradvel(o, o->WT, I, J, K, &DWTDX, &DWTDR, &DWTDT);
+        gradvel(o, o->T_STATIC, I, J, K, &DTDX, &DTDR, &DTDT);
+        D3(TXR, I, J, K) = DVXDR + DVRDX;
+        D3(TXX, I, J, K) = DVXDX;
+        D3(TXT, I, J, K) = DWTDX + DVXDT;
+        D3(TRR, I, J, K) = DVRDR;
+        D3(TRT, I, J, K) = DWTDR + DVRDT - WTAVG / RAVG;
+        D3(TTT, I, J, K) = DWTDT;
+        D3(QXX, I, J, K) = DTDX;
+        D3(QRR, I, J, K) = DTDR;
+        D3(QTT, I, J, K) = DTDT;
+        float VORTX = DVRDT - DWTDR - WTAVG / RAVG;
+        float VORTR = DWTDX - DVXDT;
+        float VORTT = DVXDR - DVRDX;
+        float VORT = sqrtf(VORTX * VORTX + VORTR * VORTR + VORTT * VORTT);
+        if (VORT > o->VORT_MAX) VORT = o->VORT_MAX;
+        A3(ABS_VORT, I, J, K) = VORT;
+        float TURBVIS_WALL = ROAVG * A3(XLENGTH, I, J, K) * VORT * o->FMIXUP;
+        float yp = A3(Y_PLUS, I, J, K);
+        if (yp <= o->YPLAM) FYPLUS = 0.0f;
+        if (yp > o->YPLAM) {
+          float XFAC = (yp - o->YPLAM) / (o->YPTURB - o->YPLAM);
+          if (XFAC > 1.0f) XFAC = 1.0f;
+          FYPLUS = XFAC * XFAC * (3.0f - 2.0f * XFAC);
+        }
+        TURBVIS_WALL = TURBVIS_WALL * FYPLUS;
+        if (o->REYNO < 0.0001f) VISLAM = (fabsf(o->REYNO) / 100000.0f) * powf(TSAVG / 288.0f, 0.62f);
+        float VISTOT = VISLAM * (1 + A3(FST_RAT, I, J, K)) + TURBVIS_WALL;
+        float VISLIM = o->TURBVIS_LIM * VISLAM;
+        if (VISTOT > VISLIM) VISTOT = VISLIM;
+        A3(VISC_RAT, I, J, K) = VISTOT / VISLAM;
+        A3(VISC_LAM, I, J, K) = VISLAM;
+      }
+    }
+  }
+  /* transition: laminar (viscosity ratio 1) upstream of the transition
+   * point of each surface and wherever the largest ratio within half the
+   * span or pitch of the wall has not exceeded FTRANS, mf:12677-12772.
+   * The hub and casing tests of a column come before the blade-surface
+   * tests of the lines, which read what they set. */
+  for (int J = 2; J <= JM; J++) {
+    const int NRW = A1(NROW, J), JROW = J - A1(JSTART, NRW) + 1;
+    const int JTRHUB = A1(JTRAN_K1, NRW), JTRTIP = A1(JTRAN_KM, NRW);
+    const int JTRLOW = A1(JTRAN_I1, NRW), JTRUP = A1(JTRAN_IM, NRW);
+    if (KM != 2) {
+      for (int I = 1; I <= IMM1; I++) {
+        int lam = 1;
+        if (JROW >= JTRHUB) {
+          float RMAX = 0.0f;
+          for (int K = 1; K <= KMID; K++) {
+            float RATVIS = A3(VISC_RAT, I, J, K);
+            if (RATVIS < RMAX) continue;
+            RMAX = RATVIS;
+          }
+          lam = !(RMAX > o->FTRANS);
+        }
+        if (lam)
+          for (int K = 1; K <= KMID; K++) A3(VISC_RAT, I, J, K) = 1.0f;
+        lam = 1;
+        if (JROW >= JTRTIP) {
+          float RMAX = 0.0f;
+          for (int K = KMID; K <= KMM1; K++) {
+            float RATVIS = A3(VISC_RAT, I, J, K);
+            if (RATVIS < RMAX) continue;
+            RMAX = RATVIS;
+          }
+          lam = !(RMAX > o->FTRANS);
+        }
+        if (lam)
+          for (int K = KMID; K <= KMM1; K++) A3(VISC_RAT, I, J, K) = 1.0f;
+      }
+    }
+    for (int K = 1; K <= KMM1; K++) {
+      int lam = 1;
+      if (JROW >= JTRLOW) {
+        float RMAX = 0.0f;
+        for (int I = 1; I <= IMID; I++) {
+          float RATVIS = A3(VISC_RAT, I, J, K);
+          if (RATVIS < RMAX) continue;
+          RMAX = RATVIS;
+        }
+        lam = !(RMAX > o->FTRANS);
+      }
+      if (lam)
+        for (int I = 1; I <= IMID; I++) A3(VISC_RAT, I, J, K) = 1.0f;
+      lam = 1;
+      if (JROW >= JTRUP) {
+        float RMAX = 0.0f;
+        for (int I = IMID; I <= IMM1; I++) {
+          float RATVIS = A3(VISC_RAT, I, J, K);
+          if (RATVIS < RMAX) continue;
+          RMAX = RATVIS;
+        }
+        lam = !(RMAX > o->FTRANS);
+      }
+      if (lam)
+        for (int I = IMID; I <= IMM1; I++) A3(VISC_RAT, I, J, K) = 1.0f;
+    }
+  }
+  o->VISLAM = visc_forces(o, TXR, TXX, TXT, TRR, TRT, TTT, QXX, QRR, QTT, 1);
+  free(TXX); free(TXR); free(TXT); free(TRR); free(TRT); free(TTT); free(QXX); free(QRR); free(QTT);
 }
 
 /* ----------------------------------------------------------------------- */
@@ -1870,6 +2049,7 @@ static void step_once(orc_t *o) {
 
   orc_primitives(o);
   if (o->ILOS == 10 && o->NSTEP % o->NLOS == 0) orc_loss(o);
+  if (o->ILOS >= 100 && o->ILOS < 200 && o->NSTEP % o->NLOS == 0) orc_new_loss(o);
   if (o->ILOS >= 200 && o->NSTEP % o->NLOS == 0) orc_spal_loss(o);
 
   /* mass fluxes, mf:3960-4190 */
@@ -2328,6 +2508,7 @@ static void step_once(orc_t *o) {
     o->VREF = VREF;
     if (o->IFPRINT == 1 || o->IFEND == 1) {
       if (o->ILOS == 10) orc_loss(o);
+      if (o->ILOS >= 100 && o->ILOS < 200) orc_new_loss(o);
       if (o->ILOS >= 200) orc_spal_loss(o);
     }
     orc_stepup(o, 0.25f);

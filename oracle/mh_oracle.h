@@ -42,6 +42,7 @@ void orc_stepup(orc_t *o, float relax);
 void orc_new_mixplan(orc_t *o);
 void orc_loss(orc_t *o);
 void orc_spal_loss(orc_t *o);
+void orc_new_loss(orc_t *o);
 
 #ifdef __cplusplus
 }

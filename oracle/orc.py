@@ -115,7 +115,7 @@ class Oracle:
         L.orc_start.argtypes = [ctypes.c_void_p]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
-        for fn in ("orc_primitives", "orc_new_mixplan", "orc_loss", "orc_spal_loss"):
+        for fn in ("orc_primitives", "orc_new_mixplan", "orc_loss", "orc_spal_loss", "orc_new_loss"):
             getattr(L, fn).argtypes = [ctypes.c_void_p]
         L.orc_stepup.argtypes = [ctypes.c_void_p, ctypes.c_float]
         L.orc_tstep.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]

@@ -106,7 +106,10 @@ class Dump:
             for (nm, is_int, dims) in layout[bname]:
                 n = int(np.prod(dims)) if dims else 1
                 dt = np.int32 if is_int else rdt
-                sz = np.dtype(dt).itemsize * n
+                isz = np.dtype(dt).itemsize
+                if pos % isz:  # the compiler aligns an 8-byte REAL after a 4-byte INTEGER
+                    pos += isz - pos % isz
+                sz = isz * n
                 arr = np.frombuffer(data, dt, n, pos)
                 pos += sz
                 self.vars[nm] = arr.reshape(dims[::-1]) if dims else arr[0]

@@ -1,0 +1,80 @@
+"""Generate the converged-run fixtures under tests/golden/ from the reference.
+
+Runs the reference built by oracle/build_ref.sh (this container only) in its
+FP64 build (-fdefault-real-8) and its FP32 build on a deck for the deck's
+whole step budget or until its own convergence test stops it (the
+reference's own Rotor 37 deck with plain Spalart-Allmaras), and stores what EFICOOL wrote at the end of the
+run (mf:10748-10755): the per-J SUMFLX integrals BLADE_FLOW, SUMPO, SUMTO,
+SUMRVT, SUM_ENTPY, with the step the run stopped at.
+
+  converged_<case>.json   {"deck": <deck file>, "fp64": {...}, "fp32": {...}}
+
+Usage: python tests/golden/make_converged.py [case ...]
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "multall-open-18.3_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+from make_golden import read_global_plt  # noqa: E402
+
+R37 = "/root/reference/Multall-open-18.3/multall-test-cases/r37+samods-17.5.dat"
+
+
+def r37_plain_sa() -> str:
+    """The reference's own Rotor 37 Spalart-Allmaras deck (multall-test-cases/
+    r37+samods-17.5.dat) with its two SA modifications (FAC_VORT, FAC_PGRAD)
+    and the Shih wall functions (YPLUSWALL = -11) switched off: plain SA with
+    the log-law wall treatment."""
+    lines = open(R37).read().split("\n")
+    i = next(n for n, l in enumerate(lines) if l.strip().upper().startswith("REYNOLDS NUMBER"))
+    v = lines[i + 1].split()
+    assert len(v) == 6 and float(v[5]) == -11.0, v
+    lines[i + 1] = "  " + "    ".join(v[:5] + ["0.0000"])
+    f = lines[i + 3].split()
+    assert len(f) == 8, f
+    lines[i + 3] = "    " + "    ".join(f[:6] + ["0.0", "0.0"])
+    return "\n".join(lines)
+
+
+CASES = {
+    "r37sa": r37_plain_sa,
+}
+
+
+def make(case):
+    import refdump
+    txt = CASES[case]()
+    deck = f"converged_{case}.dat"
+    with open(os.path.join(HERE, "decks", deck), "w") as f:
+        f.write(txt)
+    out = {"deck": f"decks/{deck}"}
+    for variant, real in (("fp64", np.float64), ("fp32", np.float32)):
+        wd = tempfile.mkdtemp(prefix="mhconv_")
+        t0 = time.time()
+        try:
+            _, final, _ = refdump.run_reference(txt, variant, workdir=wd, timeout=7200)
+            integ = read_global_plt(os.path.join(wd, "global.plt"), real)
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+        out[variant] = {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist()}
+        print(f"{case} {variant}: stopped at step {out[variant]['nstep']} ({time.time() - t0:.0f} s)", flush=True)
+    with open(os.path.join(HERE, f"converged_{case}.json"), "w") as f:
+        json.dump(out, f)
+
+
+if __name__ == "__main__":
+    for c in (sys.argv[1:] or list(CASES)):
+        make(c)

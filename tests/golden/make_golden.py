@@ -41,6 +41,8 @@ CASES = {
     "tiny3_sa2": lambda n: deckgen.deck_tiny_sa_variant(nsteps=n),
     "tiny3_exit": lambda n: deckgen.deck_tiny_exit(nsteps=n),
     "tiny3_wide": lambda n: deckgen.deck_tiny_wide(nsteps=n),
+    "tiny3_nl": lambda n: deckgen.deck_tiny_nl(nsteps=n),
+    "tiny3_nl2": lambda n: deckgen.deck_tiny_nl_variant(nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}
@@ -59,11 +61,11 @@ def load_state(path):
     return st
 
 
-def read_global_plt(path):
+def read_global_plt(path, real=np.float32):
     """The records EFICOOL writes to unit 11 at the end of a run
     (mf:10748-10755): (1,1), then BLADE_FLOW, SUMPO, SUMTO, SUMRVT,
     SUM_ENTPY per J, after what OUTPUT wrote there.  Fortran sequential
-    records with 4-byte markers."""
+    records with 4-byte markers; REAL is 4 or 8 bytes by build."""
     raw = open(path, "rb").read()
     recs, pos = [], 0
     while pos < len(raw):
@@ -71,7 +73,7 @@ def read_global_plt(path):
         recs.append(raw[pos + 4: pos + 4 + n])
         pos += 8 + n
     k = max(i for i, r in enumerate(recs) if len(r) == 8)   # EFICOOL's (1,1) record
-    return np.stack([np.frombuffer(r, np.float32) for r in recs[k + 1:k + 6]])
+    return np.stack([np.frombuffer(r, real) for r in recs[k + 1:k + 6]])
 
 
 def make(case):

@@ -36,7 +36,10 @@ MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide
 # shear-free casing, mixing-length relaxation source
 SA = ["tiny3_sa", "tiny3_sa2"]
 SA_VARS = ["TRANS_DYN_VIS", "TRANS_KVIS", "T_SOURCE"]
-VISCOUS = MIXING_LENGTH + SA
+# NEW_LOSS decks: rotor tip gap; power-law viscosity, rough walls, shear-free
+# hub, transition points and a threshold that leaves part of the flow laminar
+NEW_LOSS = ["tiny3_nl", "tiny3_nl2"]
+VISCOUS = MIXING_LENGTH + NEW_LOSS + SA
 
 
 def _solver(st, exact):
@@ -85,9 +88,10 @@ def test_one_step_vs_oracle_exact_order(case):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", VISCOUS)
 def test_loss_call_vs_oracle(case):
-    """One LOSS / SPAL_LOSS call from the post-SETUP state (RF_VIS = 1,
-    FMIXUP = 1, NSTEP = 1, as SETUP makes it): forces, y+, the laminar
-    viscosity and (SA) the turbulent viscosity and its source."""
+    """One LOSS / NEW_LOSS / SPAL_LOSS call from the post-SETUP state
+    (RF_VIS = 1, FMIXUP = 1, NSTEP = 1, as SETUP makes it): forces, y+, the
+    laminar viscosity, (NEW_LOSS, SA) the viscosity ratio after the
+    transition test and (SA) the turbulent viscosity and its source."""
     st = load_state(case)
     st["scalars"]["NSTEPS_MAX"] = 100
     o = orc.Oracle(st)
@@ -95,6 +99,8 @@ def test_loss_call_vs_oracle(case):
         o.lib.orc_scalar(o.h, k.encode(), float(v))
     if case in SA:
         o.lib.orc_spal_loss(o.h)
+    elif case in NEW_LOSS:
+        o.lib.orc_new_loss(o.h)
     else:
         o.lib.orc_loss(o.h)
     from multall_hip.runtime import Solver
@@ -103,7 +109,8 @@ def test_loss_call_vs_oracle(case):
     vis = o.scalar("VISLAM")
     if st["scalars"]["REYNO"] >= 0.0001:
         assert abs(g.scalar("VISLAM") - vis) <= 1e-6 * abs(vis), (g.scalar("VISLAM"), vis)
-    for v in FORCES + ["Y_PLUS"] + (SA_VARS + ["VISC_RAT", "VISC_LAM"] if case in SA else []):
+    extra = SA_VARS + ["VISC_RAT", "VISC_LAM"] if case in SA else ["VISC_RAT", "VISC_LAM"] if case in NEW_LOSS else []
+    for v in FORCES + ["Y_PLUS"] + extra:
         a, b = g.get(v), o[v]
         err = _nerr(a, b)
         print(f"{case} {v}: {int(np.count_nonzero(a != b))} differing values, normalized max err {err:.2e}")
