@@ -129,6 +129,9 @@ enum mh_scalar_id {
   /* Spalart-Allmaras source factors and limits (BKSPALOS) */
   MH_S_FAC_STMIX, MH_S_FAC_ST0, MH_S_FAC_ST1, MH_S_FAC_ST2, MH_S_FAC_ST3, MH_S_FAC_SFVIS, MH_S_FAC_VORT,
   MH_S_FAC_PGRAD, MH_S_VORT_MAX, MH_S_TURBVIS_LIM,
+  /* mass-flow forcing of SETFLO (BKINPT): target inlet flow and relaxation
+   * factor, read only when IN_FLOW != 0 (required then, default 0) */
+  MH_S_FLOWIN, MH_S_RFLOW,
   /* optional state: laminar viscosity and conductivity ratio left by the
    * last LOSS call (default 0; mh_loss_init sets them) */
   MH_S_VISLAM, MH_S_FTCOND,

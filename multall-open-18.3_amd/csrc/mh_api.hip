@@ -82,6 +82,9 @@ __global__ void k_sa_grad(Geo, Dev, Params);
 __global__ void k_sa_visc(Geo, Dev, Params, float, int);
 __global__ void k_nl_visc(Geo, Dev, Params, float);
 __global__ void k_nl_trans(Geo, Dev, Params);
+__global__ void k_setflo_flow(Geo, Dev, int);
+__global__ void k_setflo_delta(Geo, Dev, Params, int);
+__global__ void k_setflo_apply(Geo, Dev);
 __global__ void k_sa_gradtvis(Geo, Dev, Params);
 __global__ void k_sa_trcol(Geo, Dev, Params);
 __global__ void k_sa_trline(Geo, Dev, Params);
@@ -130,7 +133,8 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS", "NMIXUP", "ITIMST",
     "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
     "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "FAC_STMIX", "FAC_ST0", "FAC_ST1", "FAC_ST2", "FAC_ST3",
-    "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "VISLAM", "FTCOND", "EXACT_ORDER",
+    "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "FLOWIN", "RFLOW", "VISLAM", "FTCOND",
+    "EXACT_ORDER",
 };
 
 }  // namespace mh
@@ -251,6 +255,7 @@ Params params_of(const mh_ctx *c) {
   p.FAC_ST2 = sf(c, MH_S_FAC_ST2); p.FAC_ST3 = sf(c, MH_S_FAC_ST3); p.FAC_SFVIS = sf(c, MH_S_FAC_SFVIS);
   p.FAC_VORT = sf(c, MH_S_FAC_VORT); p.FAC_PGRAD = sf(c, MH_S_FAC_PGRAD); p.VORT_MAX = sf(c, MH_S_VORT_MAX);
   p.TURBVIS_LIM = sf(c, MH_S_TURBVIS_LIM);
+  p.FLOWIN = sf(c, MH_S_FLOWIN); p.RFLOW = sf(c, MH_S_RFLOW);
   return p;
 }
 
@@ -810,6 +815,13 @@ int step_once(mh_ctx *c) {
   LAUNCH(k_bc_sums, dim3(1), dim3(1024), 6 * g.KM * sizeof(float), c->stream, g, d, p, s);
   LAUNCH(k_bc_apply, dim3((g.IM * g.KM + 255) / 256), dim3(256), 0, c->stream, g, d, p, s);
   LCHK("bc");
+  if (p.IN_FLOW != 0) {
+    // mass-flow forcing, mf:4746
+    LAUNCH(k_setflo_flow, dim3(g.JM), dim3(64), 0, c->stream, g, d, p.EXACT);
+    LAUNCH(k_setflo_delta, dim3(1), dim3(256), 0, c->stream, g, d, p, p.EXACT);
+    LAUNCH(k_setflo_apply, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("setflo");
+  }
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_axial");
@@ -936,6 +948,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))) ||
       (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
+      (r = dalloc(c, &d.SETFLO, (size_t)2 * dm.jm)) ||
       (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) ||
       (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
     return bail(r);
@@ -1021,16 +1034,18 @@ double mh_get_scalar(const mh_ctx *c, int id) {
 }
 
 // Every reference scalar must be set; VISLAM, FTCOND and the library
-// controls are optional.  Then the options this library implements.
+// controls are optional, and FLOWIN and RFLOW are needed only with
+// IN_FLOW != 0.  Then the options this library implements.
 static int check_inputs(mh_ctx *c) {
+  const bool forcing = c->scal_set[MH_S_IN_FLOW] && si(c, MH_S_IN_FLOW) != 0;
   for (int i = 0; i < MH_NSCALARS; i++)
-    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_VISLAM && i != MH_S_FTCOND)
+    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_VISLAM && i != MH_S_FTCOND &&
+        (forcing || (i != MH_S_FLOWIN && i != MH_S_RFLOW)))
       return fail(c, MH_E_STATE, std::string("scalar not set: ") + kScalarNames[i]);
   const Params p = params_of(c);
   if (p.IFGAS != 0) return fail(c, MH_E_UNSUPPORTED, "real-gas properties (IFGAS=1)");
   if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
-  if (p.IN_FLOW != 0 || p.IF_REPEAT != 0 || p.IF_RESTART != 0)
-    return fail(c, MH_E_UNSUPPORTED, "IN_FLOW, IF_REPEAT or IF_RESTART set");
+  if (p.IF_REPEAT != 0 || p.IF_RESTART != 0) return fail(c, MH_E_UNSUPPORTED, "IF_REPEAT or IF_RESTART set");
   if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 100)
     return fail(c, MH_E_UNSUPPORTED, "ILOS other than 0, 10 (LOSS), 100-199 (NEW_LOSS) or >= 200 (SPAL_LOSS)");
   if (p.ILOS != 0 && p.YPLUSWALL < -0.001f)

@@ -75,6 +75,7 @@ struct Dev {
   float *SC;                         // device-side scalars (see SC_* below)
   int *FLAGS;                        // per (I,K) line flags for the density limiter
   float *CHECK_FLOW, *BLADE_FLOW;    // per J
+  float *SETFLO;                     // per J: SETFLO's FLOW(J), then its DELTA(J)
   double *INTEG;                     // 7*JM FP64 integrals
 };
 
@@ -117,6 +118,7 @@ struct Params {
   float REYNO, RF_VIS, FTRANS, PRANDTL, YPLUSWALL, YPLAM, YPTURB, CFWALL;
   int IBOUND;
   float FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD, VORT_MAX, TURBVIS_LIM;
+  float FLOWIN, RFLOW;
 };
 
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid

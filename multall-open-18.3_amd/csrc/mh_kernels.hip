@@ -3329,6 +3329,80 @@ __global__ void k_checkflow(Geo g, Dev d, int exact) {
   d.CHECK_FLOW[J - 1] = (SUMAS + 0.0f) * NB + 0.0f - 0.0f;
 }
 
+// SETFLO, mf:8006-8052: a body force that pulls the mass flow of every J
+// towards FLOWIN (IN_FLOW = 3) or towards the mean over J (otherwise).
+// Per J, the flow through the plane scaled to the first row's pitch,
+// FLOW(J) = -sum FLOWX * NBLADE(J) / NBLADE(1), summed K-major as the
+// reference does (exact) or in FP64 (fast); one workgroup per J.
+__global__ void __launch_bounds__(64) k_setflo_flow(Geo g, Dev d, int exact) {
+  const int J = blockIdx.x + 1;
+  const float NBJ = (float)d.NBLADE[J - 1], NB1 = (float)d.NBLADE[0];
+  if (exact) {
+    if (threadIdx.x != 0) return;
+    float F = 0.0f;
+    for (int K = 1; K <= g.KMM1; K++)
+      for (int I = 1; I <= g.IMM1; I++) F = F - d.FLOWX[IX(I, J, K)] * NBJ / NB1;
+    d.SETFLO[J - 1] = F;
+    return;
+  }
+  double s = 0.0;
+  for (int c = threadIdx.x; c < g.IMM1 * g.KMM1; c += 64)
+    s -= (double)(d.FLOWX[IX(c % g.IMM1 + 1, J, c / g.IMM1 + 1)] * NBJ / NB1);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) d.SETFLO[J - 1] = (float)s;
+}
+
+// The target flow and, per J, the meridional mass-flux change DELTA(J) at
+// the mid-passage node (mf:8022-8036), into d.SETFLO[JM + J - 1].  One
+// workgroup; the mean over J is sequential (exact) or FP64 (fast).
+__global__ void __launch_bounds__(256) k_setflo_delta(Geo g, Dev d, Params p, int exact) {
+  __shared__ double part[256];
+  __shared__ float avflow;
+  const int JM = g.JM, t = threadIdx.x;
+  if (p.IN_FLOW == 3) {
+    if (t == 0) avflow = p.FLOWIN / (float)d.NBLADE[0];
+  } else if (exact) {
+    if (t == 0) {
+      float A = 0.0f;
+      for (int J = 1; J <= JM; J++) A = A + d.SETFLO[J - 1];
+      avflow = A / (float)JM;
+    }
+  } else {
+    double s = 0.0;
+    for (int J = t + 1; J <= JM; J += 256) s += (double)d.SETFLO[J - 1];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (t < off) part[t] += part[t + off];
+      __syncthreads();
+    }
+    if (t == 0) avflow = (float)part[0] / (float)JM;
+  }
+  __syncthreads();
+  for (int J = t + 1; J <= JM; J += 256) {
+    const size_t n = IX(p.IMID, J, p.KMID);
+    const float ROVMER = sqrtf(d.ROVX[n] * d.ROVX[n] + d.ROVR[n] * d.ROVR[n]);
+    float DELTA = (avflow / d.SETFLO[J - 1] - 1.0f) * ROVMER;
+    DELTA = DELTA * p.RFLOW;
+    d.SETFLO[JM + J - 1] = DELTA;
+  }
+}
+
+// The change along the local meridional direction of every node, with the
+// relative swirl momentum scaled alike, mf:8037-8049
+__global__ void k_setflo_apply(Geo g, Dev d) {
+  NODE_IJK
+  const unsigned n = IX(I, J, K);
+  const float DELTA = d.SETFLO[g.JM + J - 1];
+  const float rovx = d.ROVX[n], rovr = d.ROVR[n];
+  float ROVMER = sqrtf(rovx * rovx + rovr * rovr);
+  if (rovx < 0.0f) ROVMER = -ROVMER;
+  d.ROVX[n] = rovx + rovx / ROVMER * DELTA;
+  d.ROVR[n] = rovr + rovr / ROVMER * DELTA;
+  const float RO_WT = d.ROVT[n] - d.UBLADE[JKX(J, K)] * d.RO[n];
+  d.ROVT[n] = d.ROVT[n] + RO_WT / ROVMER * DELTA;
+}
+
 __global__ void k_econt(Geo g, Dev d) {
   __shared__ float sm[256];
   float e = 0.0f;

@@ -103,6 +103,8 @@ contains
     case ('FSMTHB'); v = FSMTHB
     case ('FANGLE'); v = FANGLE
     case ('RFIN'); v = RFIN
+    case ('FLOWIN'); v = FLOWIN
+    case ('RFLOW'); v = RFLOW
     case ('SFEXIT'); v = SFEXIT
     case ('REYNO'); v = REYNO
     case ('RF_VIS'); v = RF_VIS

@@ -33,6 +33,7 @@ module mh_host
   real, save, public :: CP, GA, F1, F2, F3, F2EFF, RSMTH, CFL, DAMPIN, MACHLIM, F_PDOWN, CONLIM
   real, save, public :: SFXIN, SFTIN, FAC_4TH, FBLK1, FBLK2, FBLK3, RFMIX, FEXTRAP, FSMTHB, FANGLE
   real, save, public :: RFIN, SFEXIT, PLATE_LOSS, THROTTLE_EXIT, REYNO, RF_VIS, FTRANS, TURBVIS_LIM
+  real, save, public :: FLOWIN = 0.0, RFLOW = 0.0
   real, save, public :: PRANDTL, YPLUSWALL, YPLAM, YPTURB, PDOWN_HUB, PDOWN_TIP, FACMIXUP, CFWALL
   real, save, public :: FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD
   real, save, public :: GA1, FGA, RFGA, RCP, CV, RCV, RGAS, SFEX1, SFEXH, VSOUND, DSMIN, VORT_MAX, PREFF
@@ -238,8 +239,8 @@ contains
     IN_PRESS = 0; IN_VTAN = 0; IN_VR = 1; IN_FLOW = 0; IF_REPEAT = 0; RFIN = 0.1
     call skip(u)
     read (u, *, iostat=ios) IN_PRESS, IN_VTAN, IN_VR, IN_FLOW, IF_REPEAT, RFIN
-    if (IN_FLOW /= 0 .or. IF_REPEAT /= 0) then
-      bad = fail('IN_FLOW and IF_REPEAT are not supported'); return
+    if (IF_REPEAT /= 0) then
+      bad = fail('IF_REPEAT is not supported'); return
     end if
     IPOUT = 1; SFEXIT = 0.0; NSFEXIT = 0
     call skip(u)
@@ -249,6 +250,12 @@ contains
     read (u, *, iostat=ios) PLATE_LOSS, THROTTLE_EXIT
     if (THROTTLE_EXIT > 0.001 .or. PLATE_LOSS > 0.001) then
       bad = fail('exit throttle and plate loss are not supported'); return
+    end if
+    ! mass-flow forcing target and relaxation factor, mf:567-572
+    FLOWIN = 0.0; RFLOW = 0.0
+    if (IN_FLOW /= 0) then
+      call skip(u)
+      read (u, *) FLOWIN, RFLOW
     end if
     ILOS = 100; NLOS = 5; IBOUND = 0
     call skip(u)

@@ -149,6 +149,7 @@ contains
     call sets('SFTIN', dble(SFTIN)); call sets('FAC_4TH', dble(FAC_4TH)); call sets('F1', dble(F1))
     call sets('F2', dble(F2)); call sets('F3', dble(F3)); call sets('RSMTH', dble(RSMTH))
     call sets('SFEXIT', dble(SFEXIT)); call sets('FSMTHB', dble(FSMTHB)); call sets('RFIN', dble(RFIN))
+    call sets('FLOWIN', dble(FLOWIN)); call sets('RFLOW', dble(RFLOW))
     call sets('PDOWN_HUB', dble(PDOWN_HUB)); call sets('PDOWN_TIP', dble(PDOWN_TIP))
     call sets('FEXTRAP', dble(FEXTRAP)); call sets('RFMIX', dble(RFMIX)); call sets('FANGLE', dble(FANGLE))
     call sets('CP', dble(CP)); call sets('GA', dble(GA)); call sets('RGAS', dble(RGAS)); call sets('CV', dble(CV))

@@ -78,6 +78,9 @@ class DeckSpec:
     in_vtan: int = 0
     in_vr: int = 1
     rfin: float = 0.1
+    in_flow: int = 0               # mass-flow forcing: 3 towards FLOWIN, 2 towards the mean over J
+    flowin: float = 0.0            # kg/s through the whole annulus
+    rflow: float = 0.0
     ipout: int = 0
     sfexit: float = 0.0
     nsfexit: int = 0
@@ -228,11 +231,14 @@ def write_deck(spec: DeckSpec) -> str:
     a("       NSECS_IN")
     a(f"{spec.nsecs:10d}")
     a("       IN_PRESS  IN_VTAN   IN_VR    IN_FLOW  IF_REPEAT  RFIN")
-    a(f"{spec.in_press:10d}{spec.in_vtan:10d}{spec.in_vr:10d}         0         0{spec.rfin:12.5f}")
+    a(f"{spec.in_press:10d}{spec.in_vtan:10d}{spec.in_vr:10d}{spec.in_flow:10d}         0{spec.rfin:12.5f}")
     a("  IPOUT SFEXIT  NSFEXIT")
     a(f"{spec.ipout:5d}{spec.sfexit:12.6f}{spec.nsfexit:5d}")
     a("  PLATE_LOSS THROTTLE_EXIT")
     a("  0.000000  0.000000")
+    if spec.in_flow != 0:
+        a("  FLOWIN  RFLOW")
+        a(f"{spec.flowin:12.5f}{spec.rflow:10.4f}")
     a("        ILOS      NLOS      IBOUND")
     a(f"{spec.ilos:10d}{spec.nlos:10d}{spec.ibound:10d}")
     if spec.ilos != 0:
@@ -433,6 +439,29 @@ def deck_tiny_nl_variant(nsteps: int = 100) -> DeckSpec:
     spec.ftrans = 4.0
     for n, r in enumerate(spec.rows):
         r.jtran = (14 + 2 * n, 18, 9, 24 - 2 * n)
+    return spec
+
+
+def deck_tiny_sss(nsteps: int = 100) -> DeckSpec:
+    """The three-row mixing-length tiny cascade with the SSS scheme
+    (ITIMST = -3: F2 = -1.65, F3 = -0.65 and one residual-smoothing pass,
+    NRSMTH = 1, on every conserved variable)."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=10 SSS ITIMST=-3"
+    spec.itimst = -3
+    return spec
+
+
+def deck_tiny_flow(in_flow: int = 2, nsteps: int = 100) -> DeckSpec:
+    """The three-row mixing-length tiny cascade with SETFLO's mass-flow
+    forcing: towards the mean flow over J (IN_FLOW = 2), or towards a fixed
+    annulus flow of 12 kg/s, within the range the unforced cascade passes in
+    its first 100 steps (IN_FLOW = 3)."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = f"TINY CASCADE ILOS=10 MASS-FLOW FORCING IN_FLOW={in_flow}"
+    spec.in_flow = in_flow
+    spec.flowin = 12.0
+    spec.rflow = 0.05
     return spec
 
 

@@ -40,7 +40,7 @@ SCALARS = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NIB1", "NJB1", "NKB1", "N
            "FBLK1", "FBLK2", "FBLK3", "RFMIX", "FEXTRAP", "FSMTHB", "FANGLE", "RFIN", "SFEXIT", "REYNO",
            "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL", "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP",
            "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS", "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0",
-           "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD"]
+           "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "FLOWIN", "RFLOW"]
 
 _lib = None
 
@@ -81,7 +81,7 @@ def load_deck(path: str) -> Dict:
                  "FANGLE", "RFIN", "SFEXIT", "REYNO", "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL",
                  "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP", "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS",
                  "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0", "FAC_ST1",
-                 "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD"):
+                 "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "FLOWIN", "RFLOW"):
             continue
         sc[n] = int(sc[n])
     im, jm, km = sc["IM"], sc["JM"], sc["KM"]

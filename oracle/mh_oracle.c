@@ -9,7 +9,7 @@
  *
  * Supported options (others are rejected by orc_start): ideal gas
  * (IFGAS=0), ITIMST=3 (scree; SSS and residual smoothing through F1/F2/F3
- * and NRSMTH), IM>2, KM>2, no cooling/bleed/shroud, IN_FLOW=0,
+ * and NRSMTH), IM>2, KM>2, no cooling/bleed/shroud,
  * IF_REPEAT=0, no exit throttle or plate loss, IF_CUSP=0, IF_RESTART=0,
  * ILOS=0 (inviscid), ILOS=10 (mixing-length LOSS), 100 <= ILOS < 200
  * (NEW_LOSS) or ILOS >= 200 (SPAL_LOSS without the helicity and pressure-
@@ -52,7 +52,7 @@
   X(PRANDTL) X(YPLUSWALL) X(TURBVIS_LIM) X(VISLAM) X(FTCOND) X(PREFF) X(ROLIM) X(ROMIN)       \
   X(RO_REF) X(EAVG) X(EMAX) X(ECONT) X(VRMS) X(VMAX) X(VREF) X(TCOOL_MIN) X(SFEX1) X(SFEXH)   \
   X(FLOWRAT) X(YPLAM) X(YPTURB) X(CFWALL) X(FAC_STMIX) X(FAC_ST0) X(FAC_ST1) X(FAC_ST2)       \
-  X(FAC_ST3) X(FAC_SFVIS) X(FAC_VORT) X(FAC_PGRAD) X(VORT_MAX)
+  X(FAC_ST3) X(FAC_SFVIS) X(FAC_VORT) X(FAC_PGRAD) X(VORT_MAX) X(FLOWIN) X(RFLOW)
 #define ISCAL(X) X(IM) X(JM) X(KM) X(NROWS) X(IMID) X(KMID) X(NSTEP) X(NSTEPS_MAX) X(NCHANGE) \
   X(NRSMTH) X(NSFEXIT) X(IN_PRESS) X(IN_VTAN) X(IN_VR) X(IPOUT) X(IFMIX) X(IR) X(JR) X(KR)    \
   X(ILOS) X(NLOS) X(NMIXUP) X(NIB1) X(NJB1) X(NKB1) X(NIB2) X(NJB2) X(NKB2) X(NSBLK) X(IFEND) \
@@ -133,7 +133,7 @@ int orc_start(orc_t *o) {
 #define X(n) if (!o->n) return -100;
   F3D(X) FJK(X) FIK(X)
 #undef X
-  if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 || o->IN_FLOW != 0 ||
+  if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 ||
       o->IF_REPEAT != 0 || o->IF_RESTART != 0)
     return -2;
   if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 100) return -3;
@@ -2023,6 +2023,41 @@ void orc_loss(orc_t *o) {
 /* ----------------------------------------------------------------------- */
 /* one time step: the body of LOOP from label 5000 to the GO TO 5000        */
 
+/* SETFLO, mf:8006-8052: pull the mass flow of every J towards FLOWIN
+ * (IN_FLOW = 3) or the mean over J, through the meridional mass flux. */
+static void setflo(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, IMID = o->IMID, KMID = o->KMID;
+  float *FLOW = calloc(JM + 1, sizeof(float));
+  for (int J = 1; J <= JM; J++) {
+    FLOW[J] = 0.0f;
+    for (int K = 1; K <= KMM1; K++)
+      for (int I = 1; I <= IMM1; I++)
+        FLOW[J] = FLOW[J] - A3(FLOWX, I, J, K) * (float)A1(NBLADE, J) / (float)A1(NBLADE, 1);
+  }
+  float AVFLOW = o->FLOWIN / (float)A1(NBLADE, 1);
+  if (o->IN_FLOW != 3) {
+    AVFLOW = 0.0f;
+    for (int J = 1; J <= JM; J++) AVFLOW = AVFLOW + FLOW[J];
+    AVFLOW = AVFLOW / (float)JM;
+  }
+  for (int J = 1; J <= JM; J++) {
+    float ROVMER = sqrtf(A3(ROVX, IMID, J, KMID) * A3(ROVX, IMID, J, KMID) +
+                         A3(ROVR, IMID, J, KMID) * A3(ROVR, IMID, J, KMID));
+    float DELTA = (AVFLOW / FLOW[J] - 1.0f) * ROVMER;
+    DELTA = DELTA * o->RFLOW;
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++) {
+        ROVMER = sqrtf(A3(ROVX, I, J, K) * A3(ROVX, I, J, K) + A3(ROVR, I, J, K) * A3(ROVR, I, J, K));
+        if (A3(ROVX, I, J, K) < 0.0f) ROVMER = -ROVMER;
+        A3(ROVX, I, J, K) = A3(ROVX, I, J, K) + A3(ROVX, I, J, K) / ROVMER * DELTA;
+        A3(ROVR, I, J, K) = A3(ROVR, I, J, K) + A3(ROVR, I, J, K) / ROVMER * DELTA;
+        float RO_WT = A3(ROVT, I, J, K) - AJK(UBLADE, J, K) * A3(RO, I, J, K);
+        A3(ROVT, I, J, K) = A3(ROVT, I, J, K) + RO_WT / ROVMER * DELTA;
+      }
+  }
+  free(FLOW);
+}
+
 static void step_once(orc_t *o) {
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, JMM1 = JM - 1;
   const int IMID = o->IMID, KMID = o->KMID;
@@ -2289,6 +2324,7 @@ static void step_once(orc_t *o) {
       }
     }
   }
+  if (o->IN_FLOW != 0) setflo(o);
   for (int K = 1; K <= KM; K++)
     for (int J = 1; J <= JM; J++) {
       int JP1 = J + 1;

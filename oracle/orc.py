@@ -45,7 +45,7 @@ SCALARS_F = ["CFL", "DAMP", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "
              "PDOWN_TIP", "FEXTRAP", "RFMIX", "FANGLE", "CP", "GA", "RGAS", "CV", "FGA", "GA1", "RCP",
              "VSOUND", "FACMIXUP", "REYNO", "RF_VIS", "FTRANS", "PRANDTL", "YPLUSWALL",
              "TURBVIS_LIM", "VISLAM", "FTCOND", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0",
-             "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX"]
+             "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "FLOWIN", "RFLOW"]
 SCALARS_I = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
              "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS",
              "NMIXUP", "NIB1", "NJB1", "NKB1", "NIB2", "NJB2", "NKB2", "NSBLK", "ITIMST",

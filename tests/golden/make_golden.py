@@ -43,6 +43,9 @@ CASES = {
     "tiny3_wide": lambda n: deckgen.deck_tiny_wide(nsteps=n),
     "tiny3_nl": lambda n: deckgen.deck_tiny_nl(nsteps=n),
     "tiny3_nl2": lambda n: deckgen.deck_tiny_nl_variant(nsteps=n),
+    "tiny3_sss": lambda n: deckgen.deck_tiny_sss(nsteps=n),
+    "tiny3_flow2": lambda n: deckgen.deck_tiny_flow(in_flow=2, nsteps=n),
+    "tiny3_flow3": lambda n: deckgen.deck_tiny_flow(in_flow=3, nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}

@@ -30,8 +30,11 @@ TOL_100 = 2.5e-5
 INVISCID = ["tiny1_inv", "tiny3_inv"]
 # mixing-length (LOSS) decks: chord-Reynolds and power-law viscosity, smooth
 # and rough walls, rotor tip gap, fixed wall y+ with a shear-free hub, exit
-# smoothing on the last three J planes
-MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide"]
+# smoothing on the last three J planes, IM = 72, the SSS scheme with
+# residual smoothing (ITIMST = -3), and mass-flow forcing towards the mean
+# flow (IN_FLOW = 2) and towards a fixed flow (IN_FLOW = 3)
+MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide", "tiny3_sss", "tiny3_flow2",
+                 "tiny3_flow3"]
 # Spalart-Allmaras decks: rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing, mixing-length relaxation source
 SA = ["tiny3_sa", "tiny3_sa2"]

@@ -15,12 +15,14 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 
 # inviscid decks; mixing-length (LOSS) decks covering the chord-Reynolds and
 # power-law laminar viscosity, smooth and rough walls, a rotor tip gap, the
-# fixed wall y+, a shear-free hub, exit smoothing and IM = 72; NEW_LOSS decks
+# fixed wall y+, a shear-free hub, exit smoothing, IM = 72, the SSS scheme
+# with residual smoothing (ITIMST = -3) and mass-flow forcing (IN_FLOW = 2
+# and 3); NEW_LOSS decks
 # (tip gap; power-law viscosity, rough walls, shear-free hub and transition
 # points with a threshold that leaves part of the flow laminar); and the
 # Spalart-Allmaras decks below
 CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide",
-         "tiny3_nl", "tiny3_nl2"]
+         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3"]
 # Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing and the mixing-length relaxation source) also pin the
 # turbulence state
