@@ -110,6 +110,14 @@ enum mh_array_id {
   MH_TRANS_DYN_VIS, MH_DEL_DYNVIS, MH_TRANS_KVIS, MH_T_SOURCE, MH_ROAVG_CELL, MH_VISC_LAM, MH_VISC_RAT,
   MH_XLENGTH, MH_FST_RAT,
   MH_FSTURB, MH_TURBVIS_DAMP,
+  /* Shih et al. wall functions (YPLUSWALL < -0.001; BKLOSS, BKNEWLOS): the
+   * smoothed streamwise wall pressure gradient -- N (cell-centred); the y+
+   * each wall cell's function iterates on -- JK for the I = 1 and I = IM
+   * blade surfaces, IJ (I fastest, IM x JM) for the hub and casing.  They
+   * start at SETUP's values (0 and 25, mf:6904-6910) when not set.  The
+   * blade chord of each row, for the trailing-edge Reynolds number that
+   * scales the SA pressure-gradient source factor -- RN */
+  MH_DPDS_CELL, MH_YPLUS_I1, MH_YPLUS_IM, MH_YPLUS_K1, MH_YPLUS_KM, MH_CHORD,
   MH_NARRAYS
 };
 
@@ -157,7 +165,8 @@ int mh_array_is_int(int id);
 int mh_set_array(mh_ctx *ctx, int id, const void *host, long long n);
 int mh_get_array(mh_ctx *ctx, int id, void *host, long long n);
 /* Same, from/to a padded COMMON array with leading dimensions ld_i (ID) and
- * ld_j (JD) for N arrays, ld_j (JD) for JK arrays, ld_i (ID) for IK arrays. */
+ * ld_j (JD) for N arrays, ld_j (JD) for JK arrays, ld_i (ID) for IK and IJ
+ * arrays. */
 int mh_set_array_strided(mh_ctx *ctx, int id, const void *host, int ld_i, int ld_j);
 int mh_get_array_strided(mh_ctx *ctx, int id, void *host, int ld_i, int ld_j);
 int mh_set_scalar(mh_ctx *ctx, int id, double value);

@@ -82,6 +82,9 @@ __global__ void k_sa_grad(Geo, Dev, Params);
 __global__ void k_sa_visc(Geo, Dev, Params, float, int);
 __global__ void k_nl_visc(Geo, Dev, Params, float);
 __global__ void k_nl_trans(Geo, Dev, Params);
+__global__ void k_pwallgrad_cell(Geo, Dev, Params);
+__global__ void k_pwallgrad_smooth(Geo, Dev);
+__global__ void k_sa_factors(Geo, Dev, Params);
 __global__ void k_setflo_flow(Geo, Dev, int);
 __global__ void k_setflo_delta(Geo, Dev, Params, int);
 __global__ void k_setflo_apply(Geo, Dev);
@@ -124,6 +127,8 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"TRANS_DYN_VIS", K_N, false}, {"DEL_DYNVIS", K_N, false}, {"TRANS_KVIS", K_N, false},
     {"T_SOURCE", K_N, false}, {"ROAVG_CELL", K_N, false}, {"VISC_LAM", K_N, false}, {"VISC_RAT", K_N, false},
     {"XLENGTH", K_N, false}, {"FST_RAT", K_N, false}, {"FSTURB", K_RN, false}, {"TURBVIS_DAMP", K_RN, false},
+    {"DPDS_CELL", K_N, false}, {"YPLUS_I1", K_JK, false}, {"YPLUS_IM", K_JK, false}, {"YPLUS_K1", K_IJ, false},
+    {"YPLUS_KM", K_IJ, false}, {"CHORD", K_RN, false},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -150,6 +155,7 @@ long long kind_len(const mh_dims &d, Kind k) {
     case K_N: return (long long)d.im * d.jm * d.km;
     case K_JK: return (long long)d.jm * d.km;
     case K_IK: return (long long)d.im * d.km;
+    case K_IJ: return (long long)d.im * d.jm;
     case K_I: return d.im;
     case K_K: return d.km;
     case K_J: return d.jm;
@@ -187,7 +193,8 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.JTRAN_I1, (void **)&d.JTRAN_IM, (void **)&d.JTRAN_K1, (void **)&d.JTRAN_KM,
       (void **)&d.TRANS_DYN_VIS, (void **)&d.DEL_DYNVIS, (void **)&d.TRANS_KVIS, (void **)&d.T_SOURCE,
       (void **)&d.ROAVG_CELL, (void **)&d.VISC_LAM, (void **)&d.VISC_RAT, (void **)&d.XLENGTH, (void **)&d.FST_RAT,
-      (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP,
+      (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP, (void **)&d.DPDS_CELL, (void **)&d.YPLUS_I1,
+      (void **)&d.YPLUS_IM, (void **)&d.YPLUS_K1, (void **)&d.YPLUS_KM, (void **)&d.CHORD,
   };
   return base[id];
 }
@@ -668,6 +675,20 @@ int loss_tables(mh_ctx *c) {
   return h2d(c, d.LTAB, t.data(), t.size() * 4);
 }
 
+// SET_PWALLGRAD (mf:16052-16114): the smoothed streamwise pressure gradient
+// the Shih wall functions (YPLUSWALL < -10) and the SA pressure-gradient
+// source factor read.  A pure function of PEFF and the velocities, which a
+// viscous call does not change, so one evaluation serves each call.
+int pwallgrad(mh_ctx *c) {
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const Params p = params_of(c);
+  LAUNCH(k_pwallgrad_cell, node_grid(g), kNodeBlock, 0, c->stream, g, d, p);
+  LAUNCH(k_pwallgrad_smooth, dim3((g.IMM1 * g.KMM1 + 63) / 64), dim3(64), 0, c->stream, g, d);
+  LCHK("pwallgrad");
+  return MH_OK;
+}
+
 // One LOSS call (mf:8058-9066) with the given relaxation factor, FMIXUP and
 // step number (the laminar viscosity is re-evaluated while NSTEP = 1 or
 // NSTEP < NSTEPS_MAX/4, mf:8082).
@@ -676,6 +697,8 @@ int loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   const Dev &d = c->d;
   const Params p = params_of(c);
   if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  int r;
+  if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c))) return r;  // mf:8197
   LAUNCH(k_loss_span, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis,
          fmixup);
   LAUNCH(k_loss_pitch, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p, rf_vis, fmixup);
@@ -722,8 +745,12 @@ int spal_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   const Params p = params_of(c);
   const dim3 grid = node_grid(g);
   if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  int r;
+  // mf:13656-13658 and 14175
+  if ((p.FAC_PGRAD > 0.001f || p.YPLUSWALL < -10.0f) && (r = pwallgrad(c))) return r;
   LAUNCH(k_sa_grad, grid, kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_sa_visc, grid, kNodeBlock, 0, c->stream, g, d, p, fmixup, nstep);
+  if (p.FAC_VORT > 0.001f || p.FAC_PGRAD > 0.001f) LAUNCH(k_sa_factors, dim3(1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_gradtvis, grid, kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_sa_trcol, dim3((g.IMM1 + 63) / 64, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_trline, dim3((g.KMM1 + 63) / 64, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
@@ -746,6 +773,8 @@ int new_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   LAUNCH(k_sa_grad, grid, kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_nl_visc, grid, kNodeBlock, 0, c->stream, g, d, p, fmixup);
   LAUNCH(k_nl_trans, dim3(g.JM - 1), dim3(64), 0, c->stream, g, d, p);
+  int r;
+  if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c))) return r;  // mf:12812
   LAUNCH(k_sa_iforces, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_kforces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis);
   LCHK("new_loss");
@@ -930,6 +959,15 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   }
   const size_t N = (size_t)dm.im * dm.jm * dm.km;
   Dev &d = c->d;
+  {
+    // the wall functions' y+ state starts at SETUP's 25 (mf:6906-6910)
+    const std::vector<float> y25((size_t)std::max(dm.jm * dm.km, dm.im * dm.jm), 25.0f);
+    if ((r = h2d(c, d.YPLUS_I1, y25.data(), (size_t)dm.jm * dm.km * 4)) ||
+        (r = h2d(c, d.YPLUS_IM, y25.data(), (size_t)dm.jm * dm.km * 4)) ||
+        (r = h2d(c, d.YPLUS_K1, y25.data(), (size_t)dm.im * dm.jm * 4)) ||
+        (r = h2d(c, d.YPLUS_KM, y25.data(), (size_t)dm.im * dm.jm * 4)))
+      return bail(r);
+  }
   if ((r = dalloc(c, &d.XFLUX, N)) || (r = dalloc(c, &d.TFLUX, N)) || (r = dalloc(c, &d.RFLUX, N)) ||
       (r = dalloc(c, &d.SOURCE, N)) || (r = dalloc(c, &d.STORE, N)) || (r = dalloc(c, &d.DSC, N)) ||
       (r = dalloc(c, &d.TMP, N)) || (r = dalloc(c, &d.TEMP1, N)) ||
@@ -997,6 +1035,7 @@ static int copy_array(mh_ctx *c, int id, void *host, long long n, int ld_i, int 
     case K_B2: w = dm.nib2; h = dm.njb2 + 1; planes = dm.nkb2; spitch = ld_i; splane = (size_t)ld_i * ld_j; break;
     case K_JK: w = dm.jm; h = dm.km; spitch = ld_j; break;
     case K_IK: w = dm.im; h = dm.km; spitch = ld_i; break;
+    case K_IJ: w = dm.im; h = dm.jm; spitch = ld_i; break;
     default: w = (size_t)len; h = 1; spitch = w; break;
   }
   if (spitch < w || (planes > 1 && (size_t)ld_j < h)) return fail(c, MH_E_ARG, "leading dimensions too small");
@@ -1048,10 +1087,6 @@ static int check_inputs(mh_ctx *c) {
   if (p.IF_REPEAT != 0 || p.IF_RESTART != 0) return fail(c, MH_E_UNSUPPORTED, "IF_REPEAT or IF_RESTART set");
   if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 100)
     return fail(c, MH_E_UNSUPPORTED, "ILOS other than 0, 10 (LOSS), 100-199 (NEW_LOSS) or >= 200 (SPAL_LOSS)");
-  if (p.ILOS != 0 && p.YPLUSWALL < -0.001f)
-    return fail(c, MH_E_UNSUPPORTED, "Shih wall functions (YPLUSWALL < 0)");
-  if (p.ILOS >= 200 && (p.FAC_VORT > 0.001f || p.FAC_PGRAD > 0.001f))
-    return fail(c, MH_E_UNSUPPORTED, "SA helicity or pressure-gradient source factors (FAC_VORT, FAC_PGRAD)");
   if (p.ILOS != 0 && p.NLOS < 1) return fail(c, MH_E_ARG, "NLOS < 1");
   if (p.IMID < 1 || p.IMID > c->g.IM || p.KMID < 1 || p.KMID > c->g.KM) return fail(c, MH_E_ARG, "IMID/KMID");
   c->g.IMID = p.IMID;

@@ -19,7 +19,7 @@
 
 namespace mh {
 
-enum Kind : int { K_N, K_JK, K_IK, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB, K_RN };
+enum Kind : int { K_N, K_JK, K_IK, K_IJ, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB, K_RN };
 
 struct ArrayInfo {
   const char *name;
@@ -56,6 +56,7 @@ struct Dev {
   int *JTRAN_I1, *JTRAN_IM, *JTRAN_K1, *JTRAN_KM;
   float *TRANS_DYN_VIS, *DEL_DYNVIS, *TRANS_KVIS, *T_SOURCE, *ROAVG_CELL, *VISC_LAM, *VISC_RAT, *XLENGTH, *FST_RAT;
   float *FSTURB, *TURBVIS_DAMP;
+  float *DPDS_CELL, *YPLUS_I1, *YPLUS_IM, *YPLUS_K1, *YPLUS_KM, *CHORD;
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
@@ -82,6 +83,7 @@ struct Dev {
 // Slots of Dev::SC, the step's device-resident scalars.
 enum {
   SC_VRMS = 0, SC_VMAX, SC_EMAX, SC_EAVG, SC_ECONT, SC_VREF, SC_FLOWRAT, SC_FLOWOUT, SC_VISLAM, SC_FTCOND,
+  SC_VORTFAC, SC_PGRADFAC,  // SPAL_LOSS's helicity and pressure-gradient source factors
   SC_AVG_CHG = 16,          // [NROWS]
   SC_AVG_BLK = 16 + 64,     // [NROWS]
   SC_VLIMIT = 16 + 128,     // [NROWS]

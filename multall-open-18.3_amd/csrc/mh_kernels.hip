@@ -18,6 +18,7 @@ namespace mh {
 #define IX(I, J, K) ((size_t)((I) - 1) + (size_t)g.IM * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((K) - 1)))
 #define JKX(J, K) ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((K) - 1))
 #define IKX(I, K) ((size_t)((I) - 1) + (size_t)g.IM * (size_t)((K) - 1))
+#define IJX(I, J) ((size_t)((I) - 1) + (size_t)g.IM * (size_t)((J) - 1))
 #define NODE_IJK                                           \
   const int I = blockIdx.x * blockDim.x + threadIdx.x + 1; \
   const int J = blockIdx.y * blockDim.y + threadIdx.y + 1; \
@@ -65,6 +66,7 @@ __device__ __forceinline__ float sin_r(float x) { return (float)sin((double)x); 
 __device__ __forceinline__ float cos_r(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float asin_r(float x) { return (float)asin((double)x); }
 __device__ __forceinline__ float atan2_r(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float tanh_r(float x) { return (float)tanh((double)x); }
 
 // XCD-aware workgroup order.  Workgroups are dispatched round-robin over the
 // 8 XCDs, each with its own L2; this bijection on [0, n) gives XCD x the
@@ -1715,6 +1717,49 @@ SMOOTH_REG(16)
 
 // laminar viscosity and FTCOND while NSTEP = 1 or NSTEP < NSTEPS_MAX/4,
 // mf:8082-8109 (one thread)
+// Shih et al. wall function (NASA/TM-1999-209398), mf:15980-16047: the wall
+// shear stress from the y+ of the previous call (*yplus, replaced by the
+// new one, capped at 1000 as every caller does), with the streamwise
+// pressure-gradient velocity when YPLUSWALL <= -10.  There |DPDS| < 1 is
+// taken as 1 and stored back (the reference's DPDS is a by-reference
+// argument); only the thread passing a non-null dpds_out stores.
+__device__ float wallfun(float YPLUSWALL, float VISLAM, float PERP, float DPDS, float *dpds_out, float DENSITY,
+                         float *yplus, float WREL_WALL, bool store) {
+  const float YPLUS_OLD = *yplus;
+  float SGN_DPDS = 0.0f, VPRES = 0.0f, VSTAR = 0.0f, FYPLUSP = 0.0f, FYPLUS = 0.0f, FRACP;
+  if (!(YPLUSWALL > -10.0f)) {
+    if (fabsf(DPDS) < 1.0f) {
+      DPDS = 1.0f;
+      if (store) *dpds_out = 1.0f;
+    }
+    SGN_DPDS = fabsf(DPDS) / DPDS;
+    VPRES = pow_r(VISLAM / DENSITY / DENSITY * fabsf(DPDS), 0.33333f);
+    VSTAR = YPLUS_OLD * VISLAM / (DENSITY * PERP);
+    if (SGN_DPDS < 0.0f && VPRES > 0.33f * VSTAR) VPRES = 0.33f * VSTAR;
+    if (SGN_DPDS > 0.0f && VPRES > 2.0f * VSTAR) VPRES = 2.0f * VSTAR;
+    const float YP1 = VPRES * PERP * DENSITY / VISLAM, YP2 = YP1 * YP1, YP3 = YP2 * YP1, YP4 = YP2 * YP2;
+    if (YP1 <= 4.0f) FYPLUSP = 0.5f * YP2 - 7.31E-3f * YP3;
+    if (YP1 > 4.0f && YP1 <= 15.0f)
+      FYPLUSP = -15.138f + 8.4688f * YP1 - 0.81976f * YP2 + 3.7292E-2f * YP3 - 6.3866E-4f * YP4;
+    if (YP1 > 15.0f && YP1 <= 30.0f)
+      FYPLUSP = 11.925f + 0.934f * YP1 - 2.7805E-2f * YP2 + 4.6262E-4f * YP3 - 3.1442E-6f * YP4;
+    if (YP1 > 30.0f) FYPLUSP = 5.0f * log_r(YP1) + 8.0f;
+  }
+  const float YP1 = YPLUS_OLD, YP2 = YP1 * YP1, YP3 = YP2 * YP1, YP4 = YP2 * YP2;
+  if (YP1 <= 5.0f) FYPLUS = YP1 + 1.0E-2f * YP2 - 2.9E-03f * YP3;
+  if (YP1 > 5.0f && YP1 <= 30.0f) FYPLUS = -0.872f + 1.465f * YP1 - 7.02E-2f * YP2 + 1.66E-03f * YP3 - 1.495E-5f * YP4;
+  if (YP1 > 30.0f && YP1 <= 140.0f)
+    FYPLUS = 8.6f + 0.1864f * YP1 - 2.006E-3f * YP2 + 1.144E-5f * YP3 - 2.551E-8f * YP4;
+  if (YP1 > 140.0f) FYPLUS = 2.439f * log_r(YP1) + 5.0f;
+  if (YPLUSWALL > -10.0f) FRACP = 0.0f;
+  else FRACP = SGN_DPDS * (VPRES / VSTAR) * FYPLUSP / FYPLUS;
+  const float VPLUS = FYPLUS * (1.0f + FRACP);
+  VSTAR = WREL_WALL / VPLUS;
+  const float TWALL = DENSITY * VSTAR * VSTAR;
+  if (store) *yplus = fminf(1000.0f, DENSITY * VSTAR * PERP / VISLAM);
+  return TWALL;
+}
+
 __global__ void k_loss_vislam(Geo g, Dev d, Params p) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   float VISLAM = d.SC[SC_VISLAM];
@@ -1834,7 +1879,14 @@ __global__ void k_loss_span(Geo g, Dev d, Params p, float RF_VIS, float FMIXUP) 
   const float wb1 = loss_wbound(q1, WHUB), wb2 = loss_wbound(q2, 2 > KMID ? WTIP : WHUB);
   const float wbm1 = loss_wbound(qm1, KMM1 > KMID ? WTIP : WHUB), wbm = loss_wbound(qm, WTIP);
   float TWALLK1, TWALLKM;
-  if (p.YPLUSWALL < 5.f) {
+  if (p.YPLUSWALL < -0.001f) {
+    // Shih wall functions, mf:8269-8280 and 8348-8358
+    const unsigned c1 = IX(I, J, 1), cm = IX(I, J, KMM1);
+    TWALLK1 = wallfun(p.YPLUSWALL, VISLAM, d.VOL[c1] / q1.area, d.DPDS_CELL[c1], &d.DPDS_CELL[c1], q1.ro,
+                      &d.YPLUS_K1[IJX(I, J)], wb1, true);
+    TWALLKM = wallfun(p.YPLUSWALL, VISLAM, d.VOL[cm] / qm.area, d.DPDS_CELL[cm], &d.DPDS_CELL[cm], qm1.ro,
+                      &d.YPLUS_KM[IJX(I, J)], wbm1, true);
+  } else if (p.YPLUSWALL < 5.f) {
     const float PERP = d.VOL[IX(I, J, 1)] / q1.area;
     const float RE = PERP * q2.ro * wb2 / VISLAM;
     const float CF = loss_cf(RE, PERP, d.ROUGH_H[NRW - 1], q2.ro, wb2, VISLAM);
@@ -1991,7 +2043,21 @@ __global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float
     float VSTAR1 = 0.0f, VSTARM = 0.0f;
     if (!tipgap) {
       float TWALLI1, TWALLIM;
-      if (p.YPLUSWALL < 5.f) {
+      if (p.YPLUSWALL < -0.001f) {
+        // Shih wall functions, mf:8621-8631 and 8696-8706; every lane
+        // forms the stress, lane 0 keeps the wall state
+        const unsigned c1 = IX(1, J, K), cm = IX(IMM1, J, K);
+        const float dp1 = d.DPDS_CELL[c1], dpm = d.DPDS_CELL[cm];
+        float yp1 = d.YPLUS_I1[JKX(J, K)], ypm = d.YPLUS_IM[JKX(J, K)];
+        const bool keep = lane == 0;
+        TWALLI1 = wallfun(p.YPLUSWALL, VISLAM, d.VOL[c1] / AREA[1], dp1, &d.DPDS_CELL[c1], ROA[1], &yp1, WABS[2], keep);
+        TWALLIM = wallfun(p.YPLUSWALL, VISLAM, d.VOL[cm] / AREA[IM], dpm, &d.DPDS_CELL[cm], ROA[IMM1], &ypm,
+                          WABS[IMM1], keep);
+        if (keep) {
+          d.YPLUS_I1[JKX(J, K)] = yp1;
+          d.YPLUS_IM[JKX(J, K)] = ypm;
+        }
+      } else if (p.YPLUSWALL < 5.f) {
         const float PERP = d.VOL[IX(1, J, K)] / AREA[1];
         const float RE = ROA[2] * WABS[2] * PERP / VISLAM;
         const float CF = loss_cf(RE, PERP, d.ROUGH_L[NRW - 1], ROA[2], WABS[2], VISLAM);
@@ -2320,6 +2386,71 @@ __device__ __forceinline__ void sa_gradcell(const Geo &g, const Dev &d, const fl
   DVDT = -(VAVGI1 - VAVGI2) * d.ABT[JKX(J, K)] * RVOL;
 }
 
+// SET_PWALLGRAD, mf:16052-16114.  The streamwise pressure gradient of each
+// cell (PEFF's Gauss gradient along the 8-node velocity sum), limited.
+__global__ void k_pwallgrad_cell(Geo g, Dev d, Params p) {
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  // the limit: the mean pressure change per row over a tenth of the chord,
+  // as the reference writes it (mf:16065-16067)
+  float DPDS_LIM = 20.0f * (d.PO1[p.KMID - 1] - 0.5f * (p.PDOWN_HUB + p.PDOWN_TIP) / (float)g.NROWS /
+                                                    (d.CHORD[0] + d.CHORD[g.NROWS - 1]));
+  DPDS_LIM = fabsf(DPDS_LIM);
+  float DPDX, DPDR, DPDT;
+  sa_gradvel(g, d, d.PEFF, I, J, K, DPDX, DPDR, DPDT);
+  const float *VX = d.VX, *VR = d.VR, *WT = d.WT;
+  const float VXCELL = VX[IX(I, J, K)] + VX[IX(I, J - 1, K)] + VX[IX(I + 1, J, K)] + VX[IX(I + 1, J - 1, K)] +
+                       VX[IX(I, J, K + 1)] + VX[IX(I, J - 1, K + 1)] + VX[IX(I + 1, J, K + 1)] + VX[IX(I + 1, J - 1, K + 1)];
+  const float VRCELL = VR[IX(I, J, K)] + VR[IX(I, J - 1, K)] + VR[IX(I + 1, J, K)] + VR[IX(I + 1, J - 1, K)] +
+                       VR[IX(I, J, K + 1)] + VR[IX(I, J - 1, K + 1)] + VR[IX(I + 1, J, K + 1)] + VR[IX(I + 1, J - 1, K + 1)];
+  const float WTCELL = WT[IX(I, J, K)] + WT[IX(I, J - 1, K)] + WT[IX(I + 1, J, K)] + WT[IX(I + 1, J - 1, K)] +
+                       WT[IX(I, J, K + 1)] + WT[IX(I, J - 1, K + 1)] + WT[IX(I + 1, J, K + 1)] + WT[IX(I + 1, J - 1, K + 1)];
+  const float WREL_CELL = sqrtf(VXCELL * VXCELL + VRCELL * VRCELL + WTCELL * WTCELL);
+  float DPDS = (DPDX * VXCELL + DPDR * VRCELL + DPDT * WTCELL) / WREL_CELL;
+  if (fabsf(DPDS) > DPDS_LIM) DPDS = DPDS_LIM * DPDS / fabsf(DPDS);
+  d.DPDS_CELL[n] = DPDS;
+}
+
+// The two smoothing passes along J within each row, with the row's end
+// planes extrapolated before and after, mf:16089-16110; one thread per
+// (I,K) line walks the rows in order (a row's end treatment writes the
+// next row's second plane, which that row's smoothing reads).  Planes
+// beyond JM read as zero and are not stored (the reference's unused, zero
+// array padding).
+__global__ void k_pwallgrad_smooth(Geo g, Dev d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.IMM1 * g.KMM1) return;
+  const int I = t % g.IMM1 + 1, K = t / g.IMM1 + 1, JM = g.JM;
+  float *D = d.DPDS_CELL;
+  auto dp = [&](int J) -> float { return J <= JM ? D[IX(I, J, K)] : 0.0f; };
+  auto set = [&](int J, float v) {
+    if (J <= JM) D[IX(I, J, K)] = v;
+  };
+  for (int NR = 1; NR <= g.NROWS; NR++) {
+    const int J1 = d.JSTART[NR - 1], J2 = d.JMIX[NR - 1];
+    set(J2 + 1, 0.0f);
+    set(J2 + 2, 2 * dp(J2 + 3) - dp(J2 + 4));
+    set(J2, 2 * dp(J2 - 1) - dp(J2 - 2));
+    // TEMPP(J) = 0.5 D(J) + 0.25 (D(J-1) + D(J+1)) for J1+2..J2-1, then
+    // D(J) = 0.5 TEMPP(J) + 0.25 (TEMPP(J-1) + TEMPP(J+1)) for J1+3..J2-2,
+    // streamed with a three-plane window of each
+    float dm1 = dp(J1 + 1), d0 = dp(J1 + 2), dn = dp(J1 + 3);
+    float tm1 = 0.0f, t0 = 0.5f * d0 + 0.25f * (dm1 + dn);  // TEMPP(J1+2)
+    for (int J = J1 + 3; J <= J2 - 2; J++) {
+      dm1 = d0;
+      d0 = dn;
+      dn = dp(J + 1);
+      tm1 = t0;
+      t0 = 0.5f * d0 + 0.25f * (dm1 + dn);                  // TEMPP(J)
+      const float tn = 0.5f * dn + 0.25f * (d0 + dp(J + 2));  // TEMPP(J+1)
+      D[IX(I, J, K)] = 0.5f * t0 + 0.25f * (tm1 + tn);
+    }
+    set(J2 + 1, 0.0f);
+    set(J2 + 2, 2 * dp(J2 + 3) - dp(J2 + 4));
+    set(J2, 2 * dp(J2 - 1) - dp(J2 - 2));
+  }
+}
+
 // velocity and temperature gradients, strain rates and vorticity of each
 // cell, mf:13666-13718 (the vorticity before the mixing-plane substitution);
 // NEW_LOSS forms the same quantities (mf:12586-12632) but keeps the
@@ -2504,6 +2635,62 @@ __global__ void __launch_bounds__(64) k_nl_trans(Geo g, Dev d, Params p) {
   }
 }
 
+// SPAL_LOSS's helicity and pressure-gradient source factors,
+// mf:13725-13780.  The reference forms them as scalars in its cell loop and
+// its source loop applies the values left by the loop's last cell
+// (IM-1, JM, KM-1) to every cell (mf:13954-13955), so one thread forms that
+// cell's.  The pressure-gradient factor scales with the Reynolds number of
+// the last row's trailing edge, which the cell loop forms on its way
+// (mf:13795-13806), and uses the laminar viscosity the loop holds on
+// reaching the cell: the previous cell's when it varies with temperature.
+__global__ void k_sa_factors(Geo g, Dev d, Params p) {
+  if (threadIdx.x != 0) return;
+  const int I = g.IMM1, J = g.JM, K = g.KMM1, IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
+  const unsigned n = IX(I, J, K);
+  const float *VX = d.VX, *VR = d.VR, *WT = d.WT;
+  const float WTSUM = WT[IX(I, J, K)] + WT[IX(IP1, J, K)] + WT[IX(IP1, J, KP1)] + WT[IX(I, J, KP1)] +
+                      WT[IX(I, JM1, K)] + WT[IX(IP1, JM1, K)] + WT[IX(IP1, JM1, KP1)] + WT[IX(I, JM1, KP1)];
+  const float VXSUM = VX[IX(I, J, K)] + VX[IX(IP1, J, K)] + VX[IX(IP1, J, KP1)] + VX[IX(I, J, KP1)] +
+                      VX[IX(I, JM1, K)] + VX[IX(IP1, JM1, K)] + VX[IX(IP1, JM1, KP1)] + VX[IX(I, JM1, KP1)];
+  const float VRSUM = VR[IX(I, J, K)] + VR[IX(IP1, J, K)] + VR[IX(IP1, J, KP1)] + VR[IX(I, J, KP1)] +
+                      VR[IX(I, JM1, K)] + VR[IX(IP1, JM1, K)] + VR[IX(IP1, JM1, KP1)] + VR[IX(I, JM1, KP1)];
+  float WREL = sqrtf(VXSUM * VXSUM + VRSUM * VRSUM + WTSUM * WTSUM);
+  if (p.FAC_VORT > 0.001f) {
+    const float WTAVG = 0.125f * WTSUM;
+    const float RAVG = d.RAVG_CELL[JKX(J, K)];
+    float DVXDX, DVXDR, DVXDT, DVRDX, DVRDR, DVRDT, DWTDX, DWTDR, DWTDT;
+    sa_gradvel(g, d, d.VX, I, J, K, DVXDX, DVXDR, DVXDT);
+    sa_gradvel(g, d, d.VR, I, J, K, DVRDX, DVRDR, DVRDT);
+    sa_gradvel(g, d, d.WT, I, J, K, DWTDX, DWTDR, DWTDT);
+    const float VORTX = DVRDT - DWTDR - WTAVG / RAVG;
+    const float VORTR = DWTDX - DVXDT;
+    const float VORTT = DVXDR - DVRDX;
+    const float HELFAC = (VXSUM * VORTX + VRSUM * VORTR + WTSUM * VORTT) / (WREL * d.ABS_VORT[n]);
+    d.SC[SC_VORTFAC] = (1.0f + p.FAC_VORT * tanh_r(3.0f * HELFAC * HELFAC));
+  }
+  if (p.FAC_PGRAD > 0.001f) {
+    const int NRW = d.NROW[J - 1], JTE = d.JTE[NRW - 1];
+    float REYNOLDS = 0.0f;
+    if (JTE >= 2 && JTE <= g.JM && p.IMID <= g.IMM1 && p.KMID <= g.KMM1) {
+      const unsigned m = IX(p.IMID, JTE, p.KMID);
+      const float WABS = sqrtf(VX[m] * VX[m] + VR[m] * VR[m] + WT[m] * WT[m]);
+      const float ROVEXIT = d.RO[m] * WABS;
+      REYNOLDS = d.CHORD[NRW - 1] * ROVEXIT / d.VISC_LAM[m];
+    }
+    const unsigned c = IX(p.IMID, J, K);
+    const float WLIM = 0.1f * sqrtf(VX[c] * VX[c] + VR[c] * VR[c] + WT[c] * WT[c]);
+    if (WREL < WLIM) WREL = WLIM;
+    WREL = 0.125f * WREL;
+    float PGRAD = d.DPDS_CELL[n];
+    if (PGRAD < 0.0f) PGRAD = 0.0f;
+    const float VISLAM = (p.REYNO < 0.0001f) ? d.VISC_LAM[IX(I - 1, J, K)] : d.SC[SC_VISLAM];
+    const float ROAVG = d.ROAVG_CELL[n];
+    float PTERM = PGRAD * VISLAM / (ROAVG * ROAVG * WREL * WREL * WREL);
+    PTERM = PTERM * REYNOLDS;
+    d.SC[SC_PGRADFAC] = (1.0f + p.FAC_PGRAD * tanh_r(PTERM * PTERM));
+  }
+}
+
 // gradient of the kinematic turbulent viscosity, mf:13893-13911 (J <= JM-1;
 // the J = JM layer stays zero as in the reference)
 __global__ void k_sa_gradtvis(Geo g, Dev d, Params p) {
@@ -2623,6 +2810,8 @@ __global__ void k_sa_source(Geo g, Dev d, Params p, float RF_VIS) {
   const float TVIS = d.TRANS_KVIS[n];
   const float TV = d.ABS_VORT[n];
   float ST0 = FST0 * TV * TVIS;
+  if (p.FAC_VORT > 0.001f) ST0 = ST0 * d.SC[SC_VORTFAC];
+  if (p.FAC_PGRAD > 0.001f) ST0 = ST0 * d.SC[SC_PGRADFAC];
   ST0 = ST0 * FYPLUS;
   const float ST1 = FST1 * (TVISDDX + TVISDDR + TVISDDT);
   const float ST2 = FST2 * d.GRAD_TVIS[n];
@@ -2769,9 +2958,16 @@ __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
       float bx = d.ABX[IX(1, J, K)], br = d.ABR[IX(1, J, K)];
       const float AREA1 = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
       float DPERP = d.VOL[IX(1, J, K)] / AREA1;
-      float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-      float CF = sa_cf(RE, DPERP, d.ROUGH_L[NRWJ - 1], ROAVG1, WAVG, VISLAM);
-      const float TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      float TWALLI1;
+      if (p.YPLUSWALL < -0.001f) {  // Shih wall functions, mf:14209-14219
+        const unsigned c1 = IX(1, J, K);
+        TWALLI1 = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[c1], &d.DPDS_CELL[c1], 0.25f * ROAVG1,
+                          &d.YPLUS_I1[JKX(J, K)], 0.25f * WAVG, true);
+      } else {
+        const float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+        const float CF = sa_cf(RE, DPERP, d.ROUGH_L[NRWJ - 1], ROAVG1, WAVG, VISLAM);
+        TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      }
       VS[0] = sqrtf(TWALLI1 / (0.25f * ROAVG1));
       F0[3] = ROAVG1;
       float FMULT = -TWALLI1 * AREA1 / WAVG;
@@ -2789,9 +2985,16 @@ __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
       br = d.ABR[IX(IM, J, K)];
       const float AREAIM = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
       DPERP = d.VOL[IX(IMM1, J, K)] / AREAIM;
-      RE = DPERP * ROAVGIM * WAVG / VISLAM16;
-      CF = sa_cf(RE, DPERP, d.ROUGH_U[NRWJ - 1], ROAVGIM, WAVG, VISLAM);
-      const float TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+      float TWALLIM;
+      if (p.YPLUSWALL < -0.001f) {  // mf:14326-14336
+        const unsigned cm = IX(IMM1, J, K);
+        TWALLIM = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[cm], &d.DPDS_CELL[cm], 0.25f * ROAVGIM,
+                          &d.YPLUS_IM[JKX(J, K)], 0.25f * WAVG, true);
+      } else {
+        const float RE = DPERP * ROAVGIM * WAVG / VISLAM16;
+        const float CF = sa_cf(RE, DPERP, d.ROUGH_U[NRWJ - 1], ROAVGIM, WAVG, VISLAM);
+        TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+      }
       VS[1] = sqrtf(TWALLIM / (0.25f * ROAVGIM));
       F1[3] = ROAVGIM;
       FMULT = -TWALLIM * AREAIM / WAVG;
@@ -2924,9 +3127,16 @@ __global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
   float sx = d.ASX[IX(I, J, 1)], sr = d.ASR[IX(I, J, 1)];
   const float AREA1 = sqrtf(sx * sx + sr * sr);
   float DPERP = d.VOL[IX(I, J, 1)] / AREA1;
-  float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-  float CF = sa_cf(RE, DPERP, d.ROUGH_H[NRWHC - 1], ROAVG1, WAVG, VISLAM);
-  const float TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+  float TWALLK1;
+  if (p.YPLUSWALL < -0.001f) {  // Shih wall functions, mf:14556-14566
+    const unsigned c1 = IX(I, J, 1);
+    TWALLK1 = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[c1], &d.DPDS_CELL[c1], 0.25f * ROAVG1,
+                      &d.YPLUS_K1[IJX(I, J)], 0.25f * WAVG, true);
+  } else {
+    const float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+    const float CF = sa_cf(RE, DPERP, d.ROUGH_H[NRWHC - 1], ROAVG1, WAVG, VISLAM);
+    TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+  }
   if (p.IBOUND == 0 || p.IBOUND == 2) {
     for (int K = 1; K <= KMID; K++) {
       const unsigned n = IX(I, J, K);
@@ -2951,9 +3161,16 @@ __global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
   sr = d.ASR[IX(I, J, KM)];
   const float AREAKM = sqrtf(sx * sx + sr * sr);
   DPERP = d.VOL[IX(I, J, KMM1)] / AREAKM;
-  RE = DPERP * ROAVGKM * WAVG / VISLAM16;
-  CF = sa_cf(RE, DPERP, d.ROUGH_T[NRWHC - 1], ROAVGKM, WAVG, VISLAM);
-  const float TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+  float TWALLKM;
+  if (p.YPLUSWALL < -0.001f) {  // mf:14671-14681
+    const unsigned cm = IX(I, J, KMM1);
+    TWALLKM = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[cm], &d.DPDS_CELL[cm], 0.25f * ROAVGKM,
+                      &d.YPLUS_KM[IJX(I, J)], 0.25f * WAVG, true);
+  } else {
+    const float RE = DPERP * ROAVGKM * WAVG / VISLAM16;
+    const float CF = sa_cf(RE, DPERP, d.ROUGH_T[NRWHC - 1], ROAVGKM, WAVG, VISLAM);
+    TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+  }
   if (p.IBOUND == 0 || p.IBOUND == 1) {
     for (int K = KMID + 1; K <= KMM1; K++) {
       const unsigned n = IX(I, J, K);

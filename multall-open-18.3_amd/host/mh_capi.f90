@@ -204,6 +204,7 @@ contains
     case ('ROUGH_T'); call putr1(ROUGH_T)
     case ('ROUGH_L'); call putr1(ROUGH_L)
     case ('ROUGH_U'); call putr1(ROUGH_U)
+    case ('CHORD'); call putr1(CHORD)
     case ('JTRAN_I1'); call puti(JTRAN_I1)
     case ('JTRAN_IM'); call puti(JTRAN_IM)
     case ('JTRAN_K1'); call puti(JTRAN_K1)

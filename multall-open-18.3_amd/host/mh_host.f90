@@ -1560,6 +1560,11 @@ contains
     if (allocated(STEP1)) deallocate (STEP1, STEP2, STEPSBK)
     if (allocated(XFORCE)) deallocate (XFORCE, RFORCE, TFORCE, QSOURCE, FLOWX, FLOWT, FLOWR)
     if (allocated(DWALLSQ)) deallocate (DWALLSQ, XLENGTH, FST_RAT, XLIMIT)
+    ! the mixing-plane parameters are read only for IFMIX /= 0 (mf:476-489)
+    ! and the y+ damping range only for ILOS /= 0 (mf:3642-3649); otherwise
+    ! they keep the zero of a fresh COMMON block, whatever a deck loaded
+    ! before set
+    RFMIX = 0.0; FEXTRAP = 0.0; FSMTHB = 0.0; FANGLE = 0.0; YPLAM = 0.0; YPTURB = 0.0
     ! the Spalart-Allmaras factors are read only for ILOS >= 200; otherwise
     ! they keep the zero of a fresh COMMON block (multall-open-18.3.f:3338)
     FAC_STMIX = 0.0; FAC_ST0 = 0.0; FAC_ST1 = 0.0; FAC_ST2 = 0.0; FAC_ST3 = 0.0; FAC_SFVIS = 0.0

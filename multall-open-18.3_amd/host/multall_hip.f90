@@ -170,6 +170,7 @@ contains
     call put1('ROUGH_H', ROUGH_H); call put1('ROUGH_T', ROUGH_T); call put1('ROUGH_L', ROUGH_L)
     call put1('ROUGH_U', ROUGH_U); call puti('JTRAN_I1', JTRAN_I1); call puti('JTRAN_IM', JTRAN_IM)
     call puti('JTRAN_K1', JTRAN_K1); call puti('JTRAN_KM', JTRAN_KM)
+    call put1('CHORD', CHORD)
     call sets('REYNO', dble(REYNO)); call sets('RF_VIS', dble(RF_VIS)); call sets('FTRANS', dble(FTRANS))
     call sets('PRANDTL', dble(PRANDTL)); call sets('YPLUSWALL', dble(YPLUSWALL)); call sets('YPLAM', dble(YPLAM))
     call sets('YPTURB', dble(YPTURB)); call sets('CFWALL', dble(CFWALL)); call sets('IBOUND', dble(IBOUND))

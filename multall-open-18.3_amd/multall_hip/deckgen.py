@@ -465,6 +465,27 @@ def deck_tiny_flow(in_flow: int = 2, nsteps: int = 100) -> DeckSpec:
     return spec
 
 
+def deck_tiny_wallfun(ilos: int, nsteps: int = 100) -> DeckSpec:
+    """Three-row tiny cascades with the Shih et al. wall functions: without
+    the pressure-gradient term for the mixing-length model (ILOS = 10,
+    YPLUSWALL = -6), with it (YPLUSWALL = -11) for NEW_LOSS (ILOS = 100)
+    and for Spalart-Allmaras (ILOS = 200).  The Spalart-Allmaras deck also
+    takes the Rotor 37 test deck's helicity and pressure-gradient source
+    factors (FAC_VORT = FAC_PGRAD = 1) and transition points with a
+    threshold that leaves part of the flow laminar."""
+    spec = deck_tiny(ilos=ilos, nrows=3, nsteps=nsteps)
+    spec.ypluswall = -6.0 if ilos == 10 else -11.0
+    spec.title = f"TINY CASCADE ILOS={ilos} SHIH WALL FUNCTIONS YPLUSWALL={spec.ypluswall:g}"
+    r = spec.rows[1]
+    r.ktips, r.ktipe, r.fractip = 7, 9, 0.02
+    if ilos >= 200:
+        spec.sa_factors = (0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+        spec.ftrans = 4.0
+        for n, row in enumerate(spec.rows):
+            row.jtran = (14 + 2 * n, 18, 9, 24 - 2 * n)
+    return spec
+
+
 def deck_tiny_exit(nsteps: int = 100) -> DeckSpec:
     """The three-row mixing-length tiny cascade with exit smoothing on the
     last three J planes (SFEXIT = 0.1, NSFEXIT = 3)."""

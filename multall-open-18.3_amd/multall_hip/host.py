@@ -25,7 +25,7 @@ VI = ["FP", "FU", "FD"]
 VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD", "VM1"]
 VJ = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 RN = ["XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM", "XLLIM_DWN", "XLLIM_UP", "ROUGH_H", "ROUGH_T", "ROUGH_L",
-      "ROUGH_U", "FSTURB", "TURBVIS_DAMP"]
+      "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "CHORD"]
 RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM"]
 MG = ["STEP1", "STEP2", "STEPSBK"]
 II = ["IB1", "IB2"]

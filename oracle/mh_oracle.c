@@ -12,8 +12,8 @@
  * and NRSMTH), IM>2, KM>2, no cooling/bleed/shroud,
  * IF_REPEAT=0, no exit throttle or plate loss, IF_CUSP=0, IF_RESTART=0,
  * ILOS=0 (inviscid), ILOS=10 (mixing-length LOSS), 100 <= ILOS < 200
- * (NEW_LOSS) or ILOS >= 200 (SPAL_LOSS without the helicity and pressure-
- * gradient source factors), each with YPLUSWALL >= -0.001.
+ * (NEW_LOSS) or ILOS >= 200 (SPAL_LOSS), each with the log-law, fixed-y+ or
+ * Shih wall functions (YPLUSWALL).
  */
 #include "mh_oracle.h"
 
@@ -32,8 +32,9 @@
   X(XFORCE) X(RFORCE) X(TFORCE) X(QSOURCE) X(FLOWX) X(FLOWT) X(FLOWR) X(XFLUX) X(TFLUX)  \
   X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN) X(DWALLSQ) X(Y_PLUS) X(TRANS_DYN_VIS) X(DEL_DYNVIS)      \
   X(TRANS_KVIS) X(T_SOURCE) X(ROAVG_CELL) X(VISC_LAM) X(VISC_RAT) X(VISC_TURB) X(XLENGTH) X(FST_RAT)      \
-  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT)
-#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID)
+  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT) X(DPDS_CELL)
+#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID) X(YPLUS_I1) X(YPLUS_IM)
+#define FIJ(X) X(YPLUS_K1) X(YPLUS_KM)
 #define FIK(X) X(FTL) X(FBL) X(FTR) X(FBR)
 #define F1D(X) X(FP) X(FU) X(FD) X(FR) X(FKU) X(FKD) X(PO1) X(TO1) X(RPO1) X(BSSIN) X(BSCOS) \
   X(BRSIN) X(BRCOS) X(VTIN) X(PD) X(PLIM) X(VM1) X(FACUP) X(FACDWN) X(WRAD) X(WHUB)         \
@@ -60,7 +61,7 @@
 
 struct orc {
 #define X(n) float *n;
-  F3D(X) FJK(X) FIK(X) F1D(X)
+  F3D(X) FJK(X) FIK(X) FIJ(X) F1D(X)
 #undef X
 #define X(n) int *n;
   I1D(X)
@@ -85,7 +86,7 @@ void orc_destroy(orc_t *o) {
 
 int orc_bind(orc_t *o, const char *name, void *ptr) {
 #define X(n) if (!strcmp(name, #n)) { o->n = (float *)ptr; return 0; }
-  F3D(X) FJK(X) FIK(X) F1D(X)
+  F3D(X) FJK(X) FIK(X) FIJ(X) F1D(X)
 #undef X
 #define X(n) if (!strcmp(name, #n)) { o->n = (int *)ptr; return 0; }
   I1D(X)
@@ -118,6 +119,7 @@ double orc_get_scalar(orc_t *o, const char *name) {
 #define D3(p, i, j, k) ((p)[((i) - 1) + (size_t)IM_ * (((j) - 1) + (size_t)JM_ * ((k) - 1))])
 #define AJK(a, j, k) (o->a[((j) - 1) + (size_t)JM_ * ((k) - 1)])
 #define AIK(a, i, k) (o->a[((i) - 1) + (size_t)IM_ * ((k) - 1)])
+#define AIJ(a, i, j) (o->a[((i) - 1) + (size_t)IM_ * ((j) - 1)])
 #define A1(a, n) (o->a[(n) - 1])
 #define NRSZ 21 /* leading dimension of the (NRS,MAXKI) LOSS tables, mf:commall :73-75 */
 #define ANR(a, nr, i) (o->a[((nr) - 1) + (size_t)NRSZ * ((i) - 1)])
@@ -131,14 +133,12 @@ double orc_get_scalar(orc_t *o, const char *name) {
 
 int orc_start(orc_t *o) {
 #define X(n) if (!o->n) return -100;
-  F3D(X) FJK(X) FIK(X)
+  F3D(X) FJK(X) FIK(X) FIJ(X)
 #undef X
   if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 ||
       o->IF_REPEAT != 0 || o->IF_RESTART != 0)
     return -2;
   if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 100) return -3;
-  if (o->ILOS != 0 && o->YPLUSWALL < -0.001f) return -3;
-  if (o->ILOS >= 200 && (o->FAC_VORT > 0.001f || o->FAC_PGRAD > 0.001f)) return -3;
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMID = o->IMID, KMID = o->KMID;
   (void)IM;
   o->RO_REF = A1(PO1, KMID) / (o->RGAS * A1(TO1, KMID));
@@ -804,10 +804,9 @@ void orc_new_mixplan(orc_t *o) {
 
 /* ----------------------------------------------------------------------- */
 /* Spalart-Allmaras model: SPAL_LOSS (mf:13586-14827) with GRADVEL and      */
-/* GRADCELL (mf:13468-13580).  Restated without the Shih wall functions     */
-/* (YPLUSWALL >= -0.001), the pressure-gradient and helicity source factors */
-/* (FAC_PGRAD, FAC_VORT <= 0.001) and KM = 2.  The Reynolds-number          */
-/* diagnostics feed printed output only and are left out.                   */
+/* GRADCELL (mf:13468-13580).  Restated for KM > 2.  Of the Reynolds-number */
+/* diagnostics only REYNOLDS of each row's trailing edge is kept, which the */
+/* pressure-gradient source factor reads.                                   */
 
 /* V of a corner-stored field: Gauss-theorem gradient of cell (I,J,K),
  * mf:13468-13510 */
@@ -866,6 +865,98 @@ static void gradcell(orc_t *o, const float *V, int I, int J, int K, float *DVDX,
             VAVGI2 * A3(ABR, IP1, J, K) + VAVGK1 * A3(ASR, I, J, K) - VAVGK2 * A3(ASR, I, J, KP1)) * RVOL -
           v0 / AJK(RAVG_CELL, J, K);
   *DVDT = -(VAVGI1 - VAVGI2) * AJK(ABT, J, K) * RVOL;
+}
+
+/* Shih et al. wall function (NASA/TM-1999-209398), mf:15980-16047: the wall
+ * shear stress from the y+ of the previous call, with the streamwise
+ * pressure-gradient velocity when YPLUSWALL <= -10.  There |DPDS| < 1 is
+ * taken as 1 and written back through DPDS, as the reference's
+ * by-reference argument is.  VISLAM is the caller's laminar viscosity.
+ * Returns TWALL; *YPLUS_NEW is the y+ the next call starts from. */
+static float wallfun(const orc_t *o, float VISLAM, float PERP, float *DPDS, float DENSITY, float YPLUS_OLD,
+                     float WREL_WALL, float *YPLUS_NEW) {
+  float SGN_DPDS = 0.0f, VPRES = 0.0f, VSTAR = 0.0f, FYPLUSP = 0.0f, FYPLUS = 0.0f, FRACP;
+  if (!(o->YPLUSWALL > -10.0f)) {
+    if (fabsf(*DPDS) < 1.0f) *DPDS = 1.0f;
+    SGN_DPDS = fabsf(*DPDS) / *DPDS;
+    VPRES = powf(VISLAM / DENSITY / DENSITY * fabsf(*DPDS), 0.33333f);
+    VSTAR = YPLUS_OLD * VISLAM / (DENSITY * PERP);
+    if (SGN_DPDS < 0.0f && VPRES > 0.33f * VSTAR) VPRES = 0.33f * VSTAR;
+    if (SGN_DPDS > 0.0f && VPRES > 2.0f * VSTAR) VPRES = 2.0f * VSTAR;
+    float YPLUS_PRES = VPRES * PERP * DENSITY / VISLAM;
+    float YP1 = YPLUS_PRES, YP2 = YP1 * YP1, YP3 = YP2 * YP1, YP4 = YP2 * YP2;
+    if (YP1 <= 4.0f) FYPLUSP = 0.5f * YP2 - 7.31E-3f * YP3;
+    if (YP1 > 4.0f && YP1 <= 15.0f)
+      FYPLUSP = -15.138f + 8.4688f * YP1 - 0.81976f * YP2 + 3.7292E-2f * YP3 - 6.3866E-4f * YP4;
+    if (YP1 > 15.0f && YP1 <= 30.0f)
+      FYPLUSP = 11.925f + 0.934f * YP1 - 2.7805E-2f * YP2 + 4.6262E-4f * YP3 - 3.1442E-6f * YP4;
+    if (YP1 > 30.0f) FYPLUSP = 5.0f * logf(YP1) + 8.0f;
+  }
+  float YP1 = YPLUS_OLD, YP2 = YP1 * YP1, YP3 = YP2 * YP1, YP4 = YP2 * YP2;
+  if (YP1 <= 5.0f) FYPLUS = YP1 + 1.0E-2f * YP2 - 2.9E-03f * YP3;
+  if (YP1 > 5.0f && YP1 <= 30.0f) FYPLUS = -0.872f + 1.465f * YP1 - 7.02E-2f * YP2 + 1.66E-03f * YP3 - 1.495E-5f * YP4;
+  if (YP1 > 30.0f && YP1 <= 140.0f)
+    FYPLUS = 8.6f + 0.1864f * YP1 - 2.006E-3f * YP2 + 1.144E-5f * YP3 - 2.551E-8f * YP4;
+  if (YP1 > 140.0f) FYPLUS = 2.439f * logf(YP1) + 5.0f;
+  if (o->YPLUSWALL > -10.0f) FRACP = 0.0f;
+  else FRACP = SGN_DPDS * (VPRES / VSTAR) * FYPLUSP / FYPLUS;
+  float VPLUS = FYPLUS * (1.0f + FRACP);
+  VSTAR = WREL_WALL / VPLUS;
+  float TWALL = DENSITY * VSTAR * VSTAR;
+  *YPLUS_NEW = DENSITY * VSTAR * PERP / VISLAM;
+  return TWALL;
+}
+
+/* SET_PWALLGRAD, mf:16052-16114: the streamwise pressure gradient of every
+ * cell (limited to DPDS_LIM), then smoothed twice along J within each row,
+ * with the row's last J planes extrapolated.  The planes beyond JM that the
+ * last row's end treatment touches are the reference's unused, zero array
+ * padding: they read as zero here and are not stored. */
+static void set_pwallgrad(orc_t *o) {
+  const int IM = o->IM, JM = o->JM, KM = o->KM, IMM1 = IM - 1, KMM1 = KM - 1, KMID = o->KMID;
+  (void)IM;
+  float DPDS_LIM = 20.0f * (A1(PO1, KMID) - 0.5f * (o->PDOWN_HUB + o->PDOWN_TIP) / (float)o->NROWS /
+                                                (A1(CHORD, 1) + A1(CHORD, o->NROWS)));
+  DPDS_LIM = fabsf(DPDS_LIM);
+  for (int J = 2; J <= JM; J++)
+    for (int K = 1; K <= KMM1; K++)
+      for (int I = 1; I <= IMM1; I++) {
+        float DPDX, DPDR, DPDT;
+        gradvel(o, o->PEFF, I, J, K, &DPDX, &DPDR, &DPDT);
+        float VXCELL = A3(VX, I, J, K) + A3(VX, I, J - 1, K) + A3(VX, I + 1, J, K) + A3(VX, I + 1, J - 1, K) +
+                       A3(VX, I, J, K + 1) + A3(VX, I, J - 1, K + 1) + A3(VX, I + 1, J, K + 1) + A3(VX, I + 1, J - 1, K + 1);
+        float VRCELL = A3(VR, I, J, K) + A3(VR, I, J - 1, K) + A3(VR, I + 1, J, K) + A3(VR, I + 1, J - 1, K) +
+                       A3(VR, I, J, K + 1) + A3(VR, I, J - 1, K + 1) + A3(VR, I + 1, J, K + 1) + A3(VR, I + 1, J - 1, K + 1);
+        float WTCELL = A3(WT, I, J, K) + A3(WT, I, J - 1, K) + A3(WT, I + 1, J, K) + A3(WT, I + 1, J - 1, K) +
+                       A3(WT, I, J, K + 1) + A3(WT, I, J - 1, K + 1) + A3(WT, I + 1, J, K + 1) + A3(WT, I + 1, J - 1, K + 1);
+        float WREL_CELL = sqrtf(VXCELL * VXCELL + VRCELL * VRCELL + WTCELL * WTCELL);
+        float DPDS = (DPDX * VXCELL + DPDR * VRCELL + DPDT * WTCELL) / WREL_CELL;
+        if (fabsf(DPDS) > DPDS_LIM) DPDS = DPDS_LIM * DPDS / fabsf(DPDS);
+        A3(DPDS_CELL, I, J, K) = DPDS;
+      }
+  float *TEMPP = calloc(JM + 8, sizeof(float));
+#define DP(J_) ((J_) <= JM ? A3(DPDS_CELL, I, (J_), K) : 0.0f)
+#define DPSET(J_, v) do { float v_ = (v); if ((J_) <= JM) A3(DPDS_CELL, I, (J_), K) = v_; } while (0)
+  for (int NR = 1; NR <= o->NROWS; NR++) {
+    const int J1 = A1(JSTART, NR), J2 = A1(JMIX, NR);
+    for (int K = 1; K <= KMM1; K++)
+      for (int I = 1; I <= IMM1; I++) {
+        for (int pass = 0; pass < 2; pass++) {
+          if (pass == 1) {
+            for (int J = J1 + 2; J <= J2 - 1; J++) TEMPP[J] = 0.5f * DP(J) + 0.25f * (DP(J - 1) + DP(J + 1));
+            TEMPP[J2 + 1] = 0.0f;
+            for (int J = J1 + 3; J <= J2 - 2; J++)
+              A3(DPDS_CELL, I, J, K) = 0.5f * TEMPP[J] + 0.25f * (TEMPP[J - 1] + TEMPP[J + 1]);
+          }
+          DPSET(J2 + 1, 0.0f);
+          DPSET(J2 + 2, 2 * DP(J2 + 3) - DP(J2 + 4));
+          DPSET(J2, 2 * DP(J2 - 1) - DP(J2 - 2));
+        }
+      }
+  }
+#undef DP
+#undef DPSET
+  free(TEMPP);
 }
 
 /* log-law skin friction of SPAL_LOSS's wall cells (sums of four node values,
@@ -945,6 +1036,7 @@ static float visc_forces(orc_t *o, float *TXR, float *TXX, float *TXT, float *TR
         D3(QTT, I, J, K) = -THCOND * D3(QTT, I, J, K);
       }
   const float VISLAM16 = VISLAM * 16.0f;
+  if (o->YPLUSWALL < -10.0f) set_pwallgrad(o); /* mf:12812, 14175 */
   /* blade surfaces and the I = const faces, mf:14159-14500.  NRW and JM1
    * keep their last values where the reference does not reset them. */
   for (int K = 1; K <= KMM1; K++) {
@@ -963,9 +1055,17 @@ static float visc_forces(orc_t *o, float *TXR, float *TXX, float *TXT, float *TR
           float bx = A3(ABX, 1, J, K), br = A3(ABR, 1, J, K), bt = AJK(ABT, J, K);
           float AREA1 = sqrtf(bx * bx + br * br + bt * bt);
           float DPERP = A3(VOL, 1, J, K) / AREA1;
-          float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-          float CF = sa_cf(RE, DPERP, A1(ROUGH_L, NRW), ROAVG1, WAVG, VISLAM);
-          float TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+          float TWALLI1;
+          if (o->YPLUSWALL < -0.001f) { /* Shih wall functions, mf:14209-14219 */
+            float YPN;
+            TWALLI1 = wallfun(o, VISLAM, DPERP, &A3(DPDS_CELL, 1, J, K), 0.25f * ROAVG1, AJK(YPLUS_I1, J, K),
+                              0.25f * WAVG, &YPN);
+            AJK(YPLUS_I1, J, K) = fminf(1000.0f, YPN);
+          } else {
+            float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+            float CF = sa_cf(RE, DPERP, A1(ROUGH_L, NRW), ROAVG1, WAVG, VISLAM);
+            TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+          }
           float VSTAR = sqrtf(TWALLI1 / (0.25f * ROAVG1));
           for (int I = 1; I <= IMID; I++) {
             float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVG1) / VISLAM;
@@ -987,9 +1087,17 @@ static float visc_forces(orc_t *o, float *TXR, float *TXX, float *TXT, float *TR
           bx = A3(ABX, IM, J, K); br = A3(ABR, IM, J, K);
           float AREAIM = sqrtf(bx * bx + br * br + bt * bt);
           DPERP = A3(VOL, IMM1, J, K) / AREAIM;
-          RE = DPERP * ROAVGIM * WAVG / VISLAM16;
-          CF = sa_cf(RE, DPERP, A1(ROUGH_U, NRW), ROAVGIM, WAVG, VISLAM);
-          float TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+          float TWALLIM;
+          if (o->YPLUSWALL < -0.001f) { /* mf:14326-14336 */
+            float YPN;
+            TWALLIM = wallfun(o, VISLAM, DPERP, &A3(DPDS_CELL, IMM1, J, K), 0.25f * ROAVGIM, AJK(YPLUS_IM, J, K),
+                              0.25f * WAVG, &YPN);
+            AJK(YPLUS_IM, J, K) = fminf(1000.0f, YPN);
+          } else {
+            float RE = DPERP * ROAVGIM * WAVG / VISLAM16;
+            float CF = sa_cf(RE, DPERP, A1(ROUGH_U, NRW), ROAVGIM, WAVG, VISLAM);
+            TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
+          }
           VSTAR = sqrtf(TWALLIM / (0.25f * ROAVGIM));
           for (int I = IMID + 1; I <= (newloss ? IMM1 : IM); I++) {
             float YPLS = VSTAR * sqrtf(A3(DWALLSQ, I, J, K)) * (0.25f * ROAVGIM) / VISLAM;
@@ -1091,9 +1199,17 @@ static float visc_forces(orc_t *o, float *TXR, float *TXX, float *TXT, float *TR
       float sx = A3(ASX, I, J, 1), sr = A3(ASR, I, J, 1);
       float AREA1 = sqrtf(sx * sx + sr * sr);
       float DPERP = A3(VOL, I, J, 1) / AREA1;
-      float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-      float CF = sa_cf(RE, DPERP, A1(ROUGH_H, NRW), ROAVG1, WAVG, VISLAM);
-      float TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      float TWALLK1;
+      if (o->YPLUSWALL < -0.001f) { /* Shih wall functions, mf:14556-14566 */
+        float YPN;
+        TWALLK1 = wallfun(o, VISLAM, DPERP, &A3(DPDS_CELL, I, J, 1), 0.25f * ROAVG1, AIJ(YPLUS_K1, I, J),
+                          0.25f * WAVG, &YPN);
+        AIJ(YPLUS_K1, I, J) = fminf(1000.0f, YPN);
+      } else {
+        float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
+        float CF = sa_cf(RE, DPERP, A1(ROUGH_H, NRW), ROAVG1, WAVG, VISLAM);
+        TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+      }
       if (o->IBOUND == 0 || o->IBOUND == 2) {
         for (int K = 1; K <= KMID; K++) {
           float VSTAR = sqrtf(TWALLK1 / (0.25f * ROAVG1));
@@ -1119,9 +1235,17 @@ static float visc_forces(orc_t *o, float *TXR, float *TXX, float *TXT, float *TR
       sx = A3(ASX, I, J, KM); sr = A3(ASR, I, J, KM);
       float AREAKM = sqrtf(sx * sx + sr * sr);
       DPERP = A3(VOL, I, J, KMM1) / AREAKM;
-      RE = DPERP * ROAVGKM * WAVG / VISLAM16;
-      CF = sa_cf(RE, DPERP, A1(ROUGH_T, NRW), ROAVGKM, WAVG, VISLAM);
-      float TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+      float TWALLKM;
+      if (o->YPLUSWALL < -0.001f) { /* mf:14671-14681 */
+        float YPN;
+        TWALLKM = wallfun(o, VISLAM, DPERP, &A3(DPDS_CELL, I, J, KMM1), 0.25f * ROAVGKM, AIJ(YPLUS_KM, I, J),
+                          0.25f * WAVG, &YPN);
+        AIJ(YPLUS_KM, I, J) = fminf(1000.0f, YPN);
+      } else {
+        float RE = DPERP * ROAVGKM * WAVG / VISLAM16;
+        float CF = sa_cf(RE, DPERP, A1(ROUGH_T, NRW), ROAVGKM, WAVG, VISLAM);
+        TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
+      }
       if (o->IBOUND == 0 || o->IBOUND == 1) {
         for (int K = KMID + 1; K <= KMM1; K++) {
           float VSTAR = sqrtf(TWALLKM / (0.25f * ROAVGKM));
@@ -1200,6 +1324,11 @@ void orc_spal_loss(orc_t *o) {
     float THCOND = o->CP * VISLAM / o->PRANDTL;
     o->FTCOND = THCOND / VISLAM;
   }
+  if (o->FAC_PGRAD > 0.001f) set_pwallgrad(o); /* mf:13656-13658 */
+  /* The helicity and pressure-gradient factors of the source (mf:13725-13780)
+   * are scalars the cell loop overwrites: the source loop applies the last
+   * cell's values to every cell (mf:13954-13955). */
+  float VORTICITY_FAC = 1.0f, PGRAD_FAC = 1.0f;
   /* turbulent viscosity of each cell, mf:13662-13888 */
   for (int K = 1; K <= KMM1; K++) {
     KP1 = K + 1;
@@ -1237,8 +1366,39 @@ void orc_spal_loss(orc_t *o) {
         if (VORT > o->VORT_MAX) VORT = o->VORT_MAX;
         if (A1(INDMIX, J) == 1) VORT = A3(ABS_VORT, I, J - 1, K);
         A3(ABS_VORT, I, J, K) = VORT;
+        if (o->FAC_VORT > 0.001f || o->FAC_PGRAD > 0.001f) { /* mf:13725-13780 */
+          float WTSUM = A3(WT, I, J, K) + A3(WT, IP1, J, K) + A3(WT, IP1, J, KP1) + A3(WT, I, J, KP1) +
+                        A3(WT, I, JM1, K) + A3(WT, IP1, JM1, K) + A3(WT, IP1, JM1, KP1) + A3(WT, I, JM1, KP1);
+          float VXSUM = A3(VX, I, J, K) + A3(VX, IP1, J, K) + A3(VX, IP1, J, KP1) + A3(VX, I, J, KP1) +
+                        A3(VX, I, JM1, K) + A3(VX, IP1, JM1, K) + A3(VX, IP1, JM1, KP1) + A3(VX, I, JM1, KP1);
+          float VRSUM = A3(VR, I, J, K) + A3(VR, IP1, J, K) + A3(VR, IP1, J, KP1) + A3(VR, I, J, KP1) +
+                        A3(VR, I, JM1, K) + A3(VR, IP1, JM1, K) + A3(VR, IP1, JM1, KP1) + A3(VR, I, JM1, KP1);
+          float WREL = sqrtf(VXSUM * VXSUM + VRSUM * VRSUM + WTSUM * WTSUM);
+          if (o->FAC_VORT > 0.001f) {
+            float HELFAC = (VXSUM * VORTX + VRSUM * VORTR + WTSUM * VORTT) / (WREL * VORT);
+            VORTICITY_FAC = (1.0f + o->FAC_VORT * tanhf(3.0f * HELFAC * HELFAC));
+          }
+          if (o->FAC_PGRAD > 0.001f) {
+            float a = A3(VX, IMID, J, K), b = A3(VR, IMID, J, K), c = A3(WT, IMID, J, K);
+            float WLIM = 0.1f * sqrtf(a * a + b * b + c * c);
+            if (WREL < WLIM) WREL = WLIM;
+            WREL = 0.125f * WREL;
+            float PGRAD = A3(DPDS_CELL, I, J, K);
+            if (PGRAD < 0.0f) PGRAD = 0.0f;
+            float PTERM = PGRAD * VISLAM / (ROAVG * ROAVG * WREL * WREL * WREL);
+            PTERM = PTERM * A1(REYNOLDS, NRW);
+            PGRAD_FAC = (1.0f + o->FAC_PGRAD * tanhf(PTERM * PTERM));
+          }
+        }
         float TURBVIS_WALL = ROAVG * A3(XLENGTH, I, J, K) * VORT * o->FMIXUP;
         if (o->REYNO < 0.0001f) VISLAM = (fabsf(o->REYNO) / 100000.0f) * powf(TSAVG / 288.0f, 0.62f);
+        if (I == IMID && K == KMID && J == A1(JTE, NRW)) { /* mf:13795-13806 */
+          float a = A3(VX, I, J, K), b = A3(VR, I, J, K), c = A3(WT, I, J, K);
+          float WABS = sqrtf(a * a + b * b + c * c);
+          A1(ROVEXIT, NRW) = A3(RO, I, J, K) * WABS;
+          A1(VISCOSY, NRW) = VISLAM;
+          A1(REYNOLDS, NRW) = A1(CHORD, NRW) * A1(ROVEXIT, NRW) / VISLAM;
+        }
         float VISTOT = VISLAM * (1 + A3(FST_RAT, I, J, K)) + TURBVIS_WALL;
         float VISLIM = o->TURBVIS_LIM * VISLAM;
         if (VISTOT > VISLIM) VISTOT = VISLIM;
@@ -1313,6 +1473,8 @@ void orc_spal_loss(orc_t *o) {
           float ROAVG = A3(ROAVG_CELL, I, J, K);
           float TVIS = A3(TRANS_KVIS, I, J, K);
           float ST0 = FST0 * A3(TRANS_VORT, I, J, K) * TVIS;
+          if (o->FAC_VORT > 0.001f) ST0 = ST0 * VORTICITY_FAC;
+          if (o->FAC_PGRAD > 0.001f) ST0 = ST0 * PGRAD_FAC;
           ST0 = ST0 * FYPLUS;
           float ST1 = FST1 * (TVISDDX + TVISDDR + TVISDDT);
           float ST2 = FST2 * A3(GRAD_TVIS, I, J, K);
@@ -1724,6 +1886,7 @@ void orc_loss(orc_t *o) {
       }
     }
   }
+  if (o->YPLUSWALL < -10.0f) set_pwallgrad(o); /* mf:8197 */
   /* stresses on the streamwise (K = const) faces, mf:8201-8556 */
   for (int J = 2; J <= JM; J++) {
     const int NRW = A1(NROW, J), J1 = A1(JSTART, NRW), JROW = J - J1 + 1;
@@ -1753,7 +1916,12 @@ void orc_loss(orc_t *o) {
         FTCOND = TCOND / VISLAM;
       }
       float TWALLK1, TWALLKM;
-      if (o->YPLUSWALL < 5.f) {
+      if (o->YPLUSWALL < -0.001f) { /* Shih wall functions, mf:8269-8280 */
+        float YPN;
+        TWALLK1 = wallfun(o, VISLAM, A3(VOL, I, J, 1) / AREA[1], &A3(DPDS_CELL, I, J, 1), ROAVG[1], AIJ(YPLUS_K1, I, J),
+                          WBOUND[1], &YPN);
+        AIJ(YPLUS_K1, I, J) = fminf(1000.0f, YPN);
+      } else if (o->YPLUSWALL < 5.f) {
         float PERP = A3(VOL, I, J, 1) / AREA[1];
         float RE = PERP * ROAVG[2] * WBOUND[2] / VISLAM;
         float CF = loss_cf(RE, PERP, A1(ROUGH_H, NRW), ROAVG[2], WBOUND[2], VISLAM);
@@ -1761,7 +1929,12 @@ void orc_loss(orc_t *o) {
       } else {
         TWALLK1 = o->CFWALL * ROAVG[1] * WBOUND[1] * WBOUND[1];
       }
-      if (o->YPLUSWALL < 5.f) {
+      if (o->YPLUSWALL < -0.001f) { /* mf:8348-8358 */
+        float YPN;
+        TWALLKM = wallfun(o, VISLAM, A3(VOL, I, J, KMM1) / AREA[KM], &A3(DPDS_CELL, I, J, KMM1), ROAVG[KMM1],
+                          AIJ(YPLUS_KM, I, J), WBOUND[KMM1], &YPN);
+        AIJ(YPLUS_KM, I, J) = fminf(1000.0f, YPN);
+      } else if (o->YPLUSWALL < 5.f) {
         float PERP = A3(VOL, I, J, KMM1) / AREA[KM];
         float RE = PERP * ROAVG[KMM1] * WBOUND[KMM1] / VISLAM;
         float CF = loss_cf(RE, PERP, A1(ROUGH_T, NRW), ROAVG[KMM1], WBOUND[KMM1], VISLAM);
@@ -1881,7 +2054,12 @@ void orc_loss(orc_t *o) {
         /* within the blade row */
         if (!tipgap) {
           float TWALLI1, TWALLIM;
-          if (o->YPLUSWALL < 5.f) {
+          if (o->YPLUSWALL < -0.001f) { /* Shih wall functions, mf:8621-8631 */
+            float YPN;
+            TWALLI1 = wallfun(o, VISLAM, A3(VOL, 1, J, K) / AREA[1], &A3(DPDS_CELL, 1, J, K), ROAVG[1],
+                              AJK(YPLUS_I1, J, K), WBOUND[2], &YPN);
+            AJK(YPLUS_I1, J, K) = fminf(1000.0f, YPN);
+          } else if (o->YPLUSWALL < 5.f) {
             float PERP = A3(VOL, 1, J, K) / AREA[1];
             float RE = ROAVG[2] * WABS[2] * PERP / VISLAM;
             float CF = loss_cf(RE, PERP, A1(ROUGH_L, NRW), ROAVG[2], WBOUND[2], VISLAM);
@@ -1889,7 +2067,12 @@ void orc_loss(orc_t *o) {
           } else {
             TWALLI1 = o->CFWALL * ROAVG[1] * WABS[1] * WABS[1];
           }
-          if (o->YPLUSWALL < 5.f) {
+          if (o->YPLUSWALL < -0.001f) { /* mf:8696-8706 */
+            float YPN;
+            TWALLIM = wallfun(o, VISLAM, A3(VOL, IMM1, J, K) / AREA[IM], &A3(DPDS_CELL, IMM1, J, K), ROAVG[IMM1],
+                              AJK(YPLUS_IM, J, K), WBOUND[IMM1], &YPN);
+            AJK(YPLUS_IM, J, K) = fminf(1000.0f, YPN);
+          } else if (o->YPLUSWALL < 5.f) {
             float PERP = A3(VOL, IMM1, J, K) / AREA[IM];
             float RE = ROAVG[IMM1] * WABS[IMM1] * PERP / VISLAM;
             float CF = loss_cf(RE, PERP, A1(ROUGH_U, NRW), ROAVG[IMM1], WBOUND[IMM1], VISLAM);

@@ -24,8 +24,13 @@ NODE3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "ROVT", "ROWT", "P", "HO", "PEFF"
          "QSOURCE", "FLOWX", "FLOWT", "FLOWR", "XFLUX", "TFLUX", "RFLUX", "SOURCE", "STORE", "TEMP1",
          "SGEN", "DWALLSQ", "Y_PLUS", "TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "ROAVG_CELL",
          "VISC_LAM", "VISC_RAT", "VISC_TURB", "XLENGTH", "FST_RAT", "DYN_VIS_ML", "ABS_VORT", "TRANS_VORT",
-         "GRAD_TVIS", "TVISGRADX", "TVISGRADR", "TVISGRADT"]
-JK2 = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID"]
+         "GRAD_TVIS", "TVISGRADX", "TVISGRADR", "TVISGRADT", "DPDS_CELL"]
+JK2 = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID", "YPLUS_I1", "YPLUS_IM"]
+IJ2 = ["YPLUS_K1", "YPLUS_KM"]
+# SETUP's initial values of the wall-function state (mf:6904-6910), for
+# fixtures written before these arrays were part of the state
+WALL_STATE_DEFAULTS = {"DPDS_CELL": ("N", 0.0), "YPLUS_I1": ("JK", 25.0), "YPLUS_IM": ("JK", 25.0),
+                       "YPLUS_K1": ("IJ", 25.0), "YPLUS_KM": ("IJ", 25.0)}
 IK2 = ["FTL", "FBL", "FTR", "FBR"]
 VEC_I = ["FP", "FU", "FD"]
 VEC_K = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD",
@@ -52,6 +57,18 @@ SCALARS_I = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE",
              "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IBOUND", "IRBB", "JRBB", "KRBB"]
 
 
+def with_wall_defaults(state: Dict) -> Dict:
+    """The state with SETUP's initial wall-function arrays added where absent."""
+    sc = state["scalars"]
+    im, jm, km = int(sc["IM"]), int(sc["JM"]), int(sc["KM"])
+    shape = {"N": (km, jm, im), "JK": (km, jm), "IJ": (jm, im)}
+    out = dict(state)
+    for n, (kind, v) in WALL_STATE_DEFAULTS.items():
+        if n not in out:
+            out[n] = np.full(shape[kind], v, np.float32)
+    return out
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
@@ -68,6 +85,8 @@ def from_dump(d) -> Dict:
         st[n] = d.jk(n).astype(rdt)
     for n in IK2:
         st[n] = np.ascontiguousarray(d[n][:km, :im]).astype(rdt)
+    for n in IJ2:
+        st[n] = np.ascontiguousarray(d[n][:jm, :im]).astype(rdt)
     for n in VEC_I:
         st[n] = np.array(d[n][:im], rdt)
     for n in VEC_K:
@@ -122,6 +141,7 @@ class Oracle:
         L.orc_smooth_var.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.h = L.orc_create()
         self.a: Dict[str, np.ndarray] = {}
+        state = with_wall_defaults(state)
         for k, v in state.items():
             if k == "scalars":
                 continue

@@ -3,8 +3,8 @@
 Runs the reference built by oracle/build_ref.sh (this container only) in its
 FP64 build (-fdefault-real-8) and its FP32 build on a deck for the deck's
 whole step budget or until its own convergence test stops it (the
-reference's own Rotor 37 deck with plain Spalart-Allmaras), and stores what EFICOOL wrote at the end of the
-run (mf:10748-10755): the per-J SUMFLX integrals BLADE_FLOW, SUMPO, SUMTO,
+reference's own Rotor 37 deck, as it ships and with plain Spalart-Allmaras),
+and stores what EFICOOL wrote at the end of the run (mf:10748-10755): the per-J SUMFLX integrals BLADE_FLOW, SUMPO, SUMTO,
 SUMRVT, SUM_ENTPY, with the step the run stopped at.
 
   converged_<case>.json   {"deck": <deck file>, "fp64": {...}, "fp32": {...}}
@@ -49,8 +49,17 @@ def r37_plain_sa() -> str:
     return "\n".join(lines)
 
 
+def r37_samods() -> str:
+    """The reference's own Rotor 37 Spalart-Allmaras deck as it ships: the
+    helicity and pressure-gradient source factors (FAC_VORT = FAC_PGRAD = 1)
+    and the Shih wall functions with the pressure-gradient term
+    (YPLUSWALL = -11)."""
+    return open(R37).read()
+
+
 CASES = {
     "r37sa": r37_plain_sa,
+    "r37samods": r37_samods,
 }
 
 

@@ -46,6 +46,9 @@ CASES = {
     "tiny3_sss": lambda n: deckgen.deck_tiny_sss(nsteps=n),
     "tiny3_flow2": lambda n: deckgen.deck_tiny_flow(in_flow=2, nsteps=n),
     "tiny3_flow3": lambda n: deckgen.deck_tiny_flow(in_flow=3, nsteps=n),
+    "tiny3_ml3": lambda n: deckgen.deck_tiny_wallfun(ilos=10, nsteps=n),
+    "tiny3_nl3": lambda n: deckgen.deck_tiny_wallfun(ilos=100, nsteps=n),
+    "tiny3_sa3": lambda n: deckgen.deck_tiny_wallfun(ilos=200, nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}
@@ -108,7 +111,15 @@ def make(case):
                 else:
                     assert np.array_equal(v, setup_ref[k]), (case, k)
         names = OUT_VARS + (SA_VARS if st["scalars"]["ILOS"] >= 200 else [])
+        if st["scalars"]["YPLUSWALL"] < -0.001:
+            names = names + ["DPDS_CELL"] + (["Y_PLUS"] if st["scalars"]["ILOS"] < 200 else [])
         out = {k: final.node(k).astype(np.float32) for k in names}
+        if st["scalars"]["YPLUSWALL"] < -0.001:   # the Shih wall functions' y+ state
+            jm, im = st["scalars"]["JM"], st["scalars"]["IM"]
+            for k in ("YPLUS_I1", "YPLUS_IM"):
+                out[k] = final.jk(k).astype(np.float32)
+            for k in ("YPLUS_K1", "YPLUS_KM"):
+                out[k] = np.ascontiguousarray(final[k][:jm, :im]).astype(np.float32)
         out["NSTEP"] = np.array(int(final["NSTEP"]))
         out["SUMFLX"] = sumflx          # SUMFLX's per-J integrals at the end of the run
         np.savez_compressed(os.path.join(HERE, f"{case}_ref{n}.npz"), **out)

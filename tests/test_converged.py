@@ -1,0 +1,86 @@
+"""Converged accuracy against the FP64 build of the reference.
+
+The library is run to convergence (the reference's own stop test) on the
+reference's Rotor 37 deck (NASA transonic compressor rotor, 45 x 150 x 49
+nodes, 36 blades) with plain Spalart-Allmaras, and its mass flow, total
+pressure ratio and total-to-total isentropic efficiency are compared with the
+FP64 build of the reference (-fdefault-real-8) run on the same deck: within
+0.05 % for mass flow and pressure ratio and 0.05 points for efficiency.
+
+Fixture: tests/golden/converged_r37sa.json holds the per-J SUMFLX integrals
+(BLADE_FLOW, SUMPO, SUMTO, SUMRVT, SUM_ENTPY) that EFICOOL wrote at the end of
+the FP64 and FP32 reference runs (mf:10748-10755), made by
+tests/golden/make_converged.py.  The quantities are formed from those
+integrals the way EFICOOL mass-averages them (mf:10761-10770).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+FIXTURE = os.path.join(GOLDEN, "converged_r37sa.json")
+BOUND_REL = 5e-4     # 0.05 % (mass flow, pressure ratio)
+BOUND_ETA = 5e-4     # 0.05 points of efficiency
+
+
+def quantities(integrals, ga):
+    """Exit mass flow, mass-averaged total pressure and total temperature
+    ratios (exit / inlet) and the total-to-total isentropic efficiency of a
+    compressor, from the per-J integrals."""
+    flow, spo, sto = (np.asarray(integrals[q], np.float64) for q in range(3))
+    p0r = (spo[-1] / flow[-1]) / (spo[0] / flow[0])
+    t0r = (sto[-1] / flow[-1]) / (sto[0] / flow[0])
+    eta = (p0r ** ((ga - 1.0) / ga) - 1.0) / (t0r - 1.0)
+    return {"mass_flow": float(flow[-1]), "p0_ratio": float(p0r), "t0_ratio": float(t0r), "eta": float(eta)}
+
+
+def _fixture():
+    with open(FIXTURE) as f:
+        return json.load(f)
+
+
+def _ga():
+    from multall_hip import host
+    st = host.load_deck(os.path.join(GOLDEN, _fixture()["deck"]))
+    return st, float(st["scalars"]["GA"])
+
+
+def _check(q, ref, label):
+    for k in ("mass_flow", "p0_ratio"):
+        err = abs(q[k] - ref[k]) / abs(ref[k])
+        print(f"{label} {k}: {q[k]:.7g} vs FP64 {ref[k]:.7g}, relative error {err:.2e}")
+        assert err <= BOUND_REL, (label, k, err)
+    err = abs(q["eta"] - ref["eta"])
+    print(f"{label} efficiency: {q['eta']:.6f} vs FP64 {ref['eta']:.6f}, error {100 * err:.4f} points")
+    assert err <= BOUND_ETA, (label, err)
+
+
+def test_fp32_reference_within_bounds_of_fp64():
+    """The bounds are met by the reference's own FP32 build: FP32 arithmetic
+    itself is not the limit at convergence."""
+    d = _fixture()
+    _, ga = _ga()
+    q64 = quantities(d["fp64"]["integrals"], ga)
+    q32 = quantities(d["fp32"]["integrals"], ga)
+    assert 1.5 < q64["p0_ratio"] < 2.5 and 0.5 < q64["eta"] < 1.0, q64
+    _check(q32, q64, "reference FP32")
+
+
+@pytest.mark.gpu
+def test_converged_rotor37_vs_fp64_reference():
+    from multall_hip.runtime import Solver
+    d = _fixture()
+    st, ga = _ga()
+    g = Solver(st, exact_order=False)
+    g.loss_init()
+    g.start()
+    r = g.step(int(st["scalars"]["NSTEPS_MAX"]) + 10)
+    assert r["ifend"] == 1
+    print(f"GPU stopped at step {r['nstep']} (EAVG {r['eavg']:.3g}, ECONT {r['econt']:.3g}); "
+          f"reference FP64 at {d['fp64']['nstep']}, FP32 at {d['fp32']['nstep']}")
+    q = quantities(g.integrals()[:5], ga)
+    q64 = quantities(d["fp64"]["integrals"], ga)
+    _check(q, q64, "GPU (fast mode)")

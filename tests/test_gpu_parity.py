@@ -31,17 +31,25 @@ INVISCID = ["tiny1_inv", "tiny3_inv"]
 # mixing-length (LOSS) decks: chord-Reynolds and power-law viscosity, smooth
 # and rough walls, rotor tip gap, fixed wall y+ with a shear-free hub, exit
 # smoothing on the last three J planes, IM = 72, the SSS scheme with
-# residual smoothing (ITIMST = -3), and mass-flow forcing towards the mean
-# flow (IN_FLOW = 2) and towards a fixed flow (IN_FLOW = 3)
+# residual smoothing (ITIMST = -3), mass-flow forcing towards the mean flow
+# (IN_FLOW = 2) and towards a fixed flow (IN_FLOW = 3), and the Shih wall
+# functions without the pressure-gradient term (YPLUSWALL = -6)
 MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide", "tiny3_sss", "tiny3_flow2",
-                 "tiny3_flow3"]
+                 "tiny3_flow3", "tiny3_ml3"]
 # Spalart-Allmaras decks: rotor tip gap; power-law viscosity, rough walls,
-# shear-free casing, mixing-length relaxation source
-SA = ["tiny3_sa", "tiny3_sa2"]
+# shear-free casing, mixing-length relaxation source; the Rotor 37 test
+# deck's options (Shih wall functions with the pressure-gradient term,
+# YPLUSWALL = -11, and the helicity and pressure-gradient source factors)
+# with transition points
+SA = ["tiny3_sa", "tiny3_sa2", "tiny3_sa3"]
 SA_VARS = ["TRANS_DYN_VIS", "TRANS_KVIS", "T_SOURCE"]
 # NEW_LOSS decks: rotor tip gap; power-law viscosity, rough walls, shear-free
-# hub, transition points and a threshold that leaves part of the flow laminar
-NEW_LOSS = ["tiny3_nl", "tiny3_nl2"]
+# hub, transition points and a threshold that leaves part of the flow
+# laminar; the Shih wall functions with the pressure-gradient term
+NEW_LOSS = ["tiny3_nl", "tiny3_nl2", "tiny3_nl3"]
+# the wall functions' smoothed pressure gradient and y+ state
+WALL_VARS = ["DPDS_CELL", "YPLUS_I1", "YPLUS_IM", "YPLUS_K1", "YPLUS_KM"]
+WALLFUN = ["tiny3_ml3", "tiny3_nl3", "tiny3_sa3"]
 VISCOUS = MIXING_LENGTH + NEW_LOSS + SA
 
 
@@ -53,6 +61,7 @@ def _solver(st, exact):
 
 
 def _nerr(a, b, scale=None):
+    a = np.asarray(a).reshape(np.shape(b))   # the library returns non-node arrays flat
     scale = np.abs(b).max() if scale is None else scale
     return float(np.abs(a.astype(np.float64) - b).max() / max(scale, 1e-30))
 
@@ -82,7 +91,7 @@ def test_one_step_vs_oracle_exact_order(case):
         assert err <= 2e-6, (v, err)
     # the step ends the run, so LOSS has refreshed the forces (logf and powf
     # differ by ulps between device and host libm)
-    for v in FORCES + (SA_VARS if case in SA else []):
+    for v in FORCES + (SA_VARS if case in SA else []) + (WALL_VARS if case in WALLFUN else []):
         err = _nerr(g.get(v), o[v])
         print(f"{case} {v}: normalized max err {err:.2e}")
         assert err <= 1e-5, (v, err)
@@ -113,8 +122,10 @@ def test_loss_call_vs_oracle(case):
     if st["scalars"]["REYNO"] >= 0.0001:
         assert abs(g.scalar("VISLAM") - vis) <= 1e-6 * abs(vis), (g.scalar("VISLAM"), vis)
     extra = SA_VARS + ["VISC_RAT", "VISC_LAM"] if case in SA else ["VISC_RAT", "VISC_LAM"] if case in NEW_LOSS else []
+    extra = extra + (WALL_VARS if case in WALLFUN else [])
     for v in FORCES + ["Y_PLUS"] + extra:
-        a, b = g.get(v), o[v]
+        b = o[v]
+        a = g.get(v).reshape(b.shape)
         err = _nerr(a, b)
         print(f"{case} {v}: {int(np.count_nonzero(a != b))} differing values, normalized max err {err:.2e}")
         assert err <= 1e-5, (v, err)

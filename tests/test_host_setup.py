@@ -14,7 +14,7 @@ import pytest
 from conftest import GOLDEN, load_state
 
 CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide",
-         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3"]
+         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3", "tiny3_ml3", "tiny3_nl3", "tiny3_sa3"]
 LOSS_STATE = {"XFORCE", "RFORCE", "TFORCE", "QSOURCE", "Y_PLUS", "VISC_RAT", "VISC_LAM", "TRANS_DYN_VIS",
               "TRANS_KVIS", "T_SOURCE"}
 
@@ -44,3 +44,20 @@ def test_host_setup_matches_reference(case):
     for k, v in st["scalars"].items():
         if k in ref["scalars"] and k not in ("NSTEPS_MAX", "VISLAM", "FTCOND"):
             assert np.float32(v) == np.float32(ref["scalars"][k]), (k, v, ref["scalars"][k])
+
+
+def test_reload_keeps_no_state_of_the_previous_deck():
+    """A deck read after another one sets up exactly as in a fresh process:
+    options a deck leaves unread (the mixing-plane parameters of a one-row
+    machine, IFMIX = 0; the Spalart-Allmaras factors of a mixing-length run)
+    keep the reference's fresh-COMMON zero rather than the previous deck's
+    values.  Rotor 37 (IFMIX = 0) after a three-row Spalart-Allmaras deck."""
+    from multall_hip import host
+    r37 = os.path.join(GOLDEN, "decks", "converged_r37sa.dat")
+    fresh = host.load_deck(r37)
+    host.load_deck(os.path.join(GOLDEN, "decks", "tiny3_sa2.dat"))
+    again = host.load_deck(r37)
+    assert fresh["scalars"] == again["scalars"]
+    for name, a in fresh.items():
+        if name != "scalars":
+            assert np.array_equal(a, again[name]), name

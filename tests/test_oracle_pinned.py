@@ -22,11 +22,14 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 # points with a threshold that leaves part of the flow laminar); and the
 # Spalart-Allmaras decks below
 CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide",
-         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3"]
+         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3", "tiny3_ml3", "tiny3_nl3", "tiny3_sa3"]
 # Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing and the mixing-length relaxation source) also pin the
 # turbulence state
 SA_VARS = ["TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "Y_PLUS"]
+# decks with the Shih wall functions also pin the smoothed wall pressure
+# gradient, y+ and the wall functions' own y+ state
+WALL_VARS = ["DPDS_CELL", "YPLUS_I1", "YPLUS_IM", "YPLUS_K1", "YPLUS_KM"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -38,7 +41,7 @@ def test_oracle_matches_reference_bitwise(case, nsteps):
     o.start()
     assert o.step(nsteps + 10) == nsteps          # the run ends at NSTEPS_MAX like the reference
     ref = load_ref(case, nsteps)
-    for v in VARS + [v for v in SA_VARS if v in ref]:
+    for v in VARS + [v for v in SA_VARS + WALL_VARS if v in ref]:
         a, b = o[v], ref[v]
         bad = np.count_nonzero(a.view(np.uint32) != b.view(np.uint32))
         assert bad == 0, f"{case} {nsteps} steps: {v} differs in {bad} of {a.size} values"
