@@ -93,7 +93,8 @@ __global__ void k_sa_trcol(Geo, Dev, Params);
 __global__ void k_sa_trline(Geo, Dev, Params);
 __global__ void k_sa_source(Geo, Dev, Params, float);
 __global__ void k_sa_iforces(Geo, Dev, Params);
-__global__ void k_sa_kforces(Geo, Dev, Params, float);
+__global__ void k_sa_kwalls(Geo, Dev, Params);
+__global__ void k_sa_kfaces(Geo, Dev, Params, float);
 
 const ArrayInfo kArrays[MH_NARRAYS] = {
     {"RO", K_N, false}, {"ROVX", K_N, false}, {"ROVR", K_N, false}, {"RORVT", K_N, false}, {"ROE", K_N, false},
@@ -756,7 +757,9 @@ int spal_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   LAUNCH(k_sa_trline, dim3((g.KMM1 + 63) / 64, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_source, grid, kNodeBlock, 0, c->stream, g, d, p, rf_vis);
   LAUNCH(k_sa_iforces, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
-  LAUNCH(k_sa_kforces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis);
+  LAUNCH(k_sa_kwalls, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_kfaces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4, (g.KMM1 + kKSeg - 1) / kKSeg), dim3(64, 4), 0,
+         c->stream, g, d, p, rf_vis);
   LCHK("spal_loss");
   return MH_OK;
 }
@@ -776,7 +779,9 @@ int new_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   int r;
   if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c))) return r;  // mf:12812
   LAUNCH(k_sa_iforces, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
-  LAUNCH(k_sa_kforces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis);
+  LAUNCH(k_sa_kwalls, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p);
+  LAUNCH(k_sa_kfaces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4, (g.KMM1 + kKSeg - 1) / kKSeg), dim3(64, 4), 0,
+         c->stream, g, d, p, rf_vis);
   LCHK("new_loss");
   return MH_OK;
 }
@@ -987,6 +992,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.SC, (size_t)SC_COUNT + dm.km)) || (r = dalloc(c, &d.FLAGS, (size_t)dm.im * dm.km)) ||
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
       (r = dalloc(c, &d.SETFLO, (size_t)2 * dm.jm)) ||
+      (r = dalloc(c, &d.KWALL, (size_t)kKWall * dm.im * dm.jm)) ||
       (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) ||
       (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
     return bail(r);

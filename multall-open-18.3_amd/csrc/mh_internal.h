@@ -77,8 +77,15 @@ struct Dev {
   int *FLAGS;                        // per (I,K) line flags for the density limiter
   float *CHECK_FLOW, *BLADE_FLOW;    // per J
   float *SETFLO;                     // per J: SETFLO's FLOW(J), then its DELTA(J)
+  float *KWALL;                      // kKWall planes of (IM-1) x (JM-1) end-wall column terms
   double *INTEG;                     // 7*JM FP64 integrals
 };
+
+// SA/NEW_LOSS end-wall terms per (I,J) column (forces, energy, friction
+// velocity and density for hub and casing), and the cells of a column one
+// k_sa_kfaces thread covers.
+constexpr int kKWall = 12;
+constexpr int kKSeg = 16;
 
 // Slots of Dev::SC, the step's device-resident scalars.
 enum {

@@ -2947,65 +2947,39 @@ __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
   const int NRWJ = d.NROW[J - 1];
   const bool tipcell = (d.KTIPS[NRWJ - 1] > 0) && (K >= d.KTIPS[NRWJ - 1]) && (K < d.KTIPE[NRWJ - 1]);
   if (blade && !tipcell) {
-    if (lane == 0) {
-      const int JM1 = J - 1;
+    // the I = 1 wall on lane 0 and the I = IM wall on lane 1 (mf:14189-14394)
+    if (lane < 2) {
+      const bool lo = lane == 0;
+      const int In = lo ? 2 : IMM1, Ic = lo ? 1 : IMM1, Ia = lo ? 1 : IM, JM1 = J - 1;
       const float *VX = d.VX, *VR = d.VR, *WT = d.WT, *RO = d.RO;
-      float VXAVG1 = VX[IX(2, J, K)] + VX[IX(2, JM1, K)] + VX[IX(2, JM1, KP1)] + VX[IX(2, J, KP1)];
-      float VRAVG1 = VR[IX(2, J, K)] + VR[IX(2, JM1, K)] + VR[IX(2, JM1, KP1)] + VR[IX(2, J, KP1)];
-      float WTAVG1 = WT[IX(2, J, K)] + WT[IX(2, JM1, K)] + WT[IX(2, JM1, KP1)] + WT[IX(2, J, KP1)];
-      float ROAVG1 = RO[IX(2, J, K)] + RO[IX(2, JM1, K)] + RO[IX(2, JM1, KP1)] + RO[IX(2, J, KP1)];
-      float WAVG = sqrtf(WTAVG1 * WTAVG1 + VXAVG1 * VXAVG1 + VRAVG1 * VRAVG1);
-      float bx = d.ABX[IX(1, J, K)], br = d.ABR[IX(1, J, K)];
-      const float AREA1 = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
-      float DPERP = d.VOL[IX(1, J, K)] / AREA1;
-      float TWALLI1;
-      if (p.YPLUSWALL < -0.001f) {  // Shih wall functions, mf:14209-14219
-        const unsigned c1 = IX(1, J, K);
-        TWALLI1 = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[c1], &d.DPDS_CELL[c1], 0.25f * ROAVG1,
-                          &d.YPLUS_I1[JKX(J, K)], 0.25f * WAVG, true);
+      const float VXAVG = VX[IX(In, J, K)] + VX[IX(In, JM1, K)] + VX[IX(In, JM1, KP1)] + VX[IX(In, J, KP1)];
+      const float VRAVG = VR[IX(In, J, K)] + VR[IX(In, JM1, K)] + VR[IX(In, JM1, KP1)] + VR[IX(In, J, KP1)];
+      const float WTAVG = WT[IX(In, J, K)] + WT[IX(In, JM1, K)] + WT[IX(In, JM1, KP1)] + WT[IX(In, J, KP1)];
+      const float ROAVG = RO[IX(In, J, K)] + RO[IX(In, JM1, K)] + RO[IX(In, JM1, KP1)] + RO[IX(In, J, KP1)];
+      const float WAVG = sqrtf(WTAVG * WTAVG + VXAVG * VXAVG + VRAVG * VRAVG);
+      const float bx = d.ABX[IX(Ia, J, K)], br = d.ABR[IX(Ia, J, K)];
+      const float AREA = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
+      const float DPERP = d.VOL[IX(Ic, J, K)] / AREA;
+      float TWALL;
+      if (p.YPLUSWALL < -0.001f) {  // Shih wall functions, mf:14209-14219, 14326-14336
+        const unsigned c = IX(Ic, J, K);
+        TWALL = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[c], &d.DPDS_CELL[c], 0.25f * ROAVG,
+                        &(lo ? d.YPLUS_I1 : d.YPLUS_IM)[JKX(J, K)], 0.25f * WAVG, true);
       } else {
-        const float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
-        const float CF = sa_cf(RE, DPERP, d.ROUGH_L[NRWJ - 1], ROAVG1, WAVG, VISLAM);
-        TWALLI1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
+        const float RE = DPERP * ROAVG * WAVG / VISLAM16;
+        const float CF = sa_cf(RE, DPERP, (lo ? d.ROUGH_L : d.ROUGH_U)[NRWJ - 1], ROAVG, WAVG, VISLAM);
+        TWALL = CF * ROAVG * WAVG * WAVG / 128.f;
       }
-      VS[0] = sqrtf(TWALLI1 / (0.25f * ROAVG1));
-      F0[3] = ROAVG1;
-      float FMULT = -TWALLI1 * AREA1 / WAVG;
-      float TFORCE1 = FMULT * WTAVG1;
-      F0[0] = 0.0f + FMULT * VXAVG1;
-      F0[1] = 0.0f + FMULT * VRAVG1;
-      F0[2] = 0.0f + TFORCE1;
-      const float WVISC1 = TFORCE1 * d.WRAD[J - 1] * d.RAVG_CELL[JKX(J, K)];
-      float VXAVGIM = VX[IX(IMM1, J, K)] + VX[IX(IMM1, JM1, K)] + VX[IX(IMM1, JM1, KP1)] + VX[IX(IMM1, J, KP1)];
-      float VRAVGIM = VR[IX(IMM1, J, K)] + VR[IX(IMM1, JM1, K)] + VR[IX(IMM1, JM1, KP1)] + VR[IX(IMM1, J, KP1)];
-      float WTAVGIM = WT[IX(IMM1, J, K)] + WT[IX(IMM1, JM1, K)] + WT[IX(IMM1, JM1, KP1)] + WT[IX(IMM1, J, KP1)];
-      float ROAVGIM = RO[IX(IMM1, J, K)] + RO[IX(IMM1, JM1, K)] + RO[IX(IMM1, JM1, KP1)] + RO[IX(IMM1, J, KP1)];
-      WAVG = sqrtf(WTAVGIM * WTAVGIM + VXAVGIM * VXAVGIM + VRAVGIM * VRAVGIM);
-      bx = d.ABX[IX(IM, J, K)];
-      br = d.ABR[IX(IM, J, K)];
-      const float AREAIM = sqrtf(bx * bx + br * br + ABTJK * ABTJK);
-      DPERP = d.VOL[IX(IMM1, J, K)] / AREAIM;
-      float TWALLIM;
-      if (p.YPLUSWALL < -0.001f) {  // mf:14326-14336
-        const unsigned cm = IX(IMM1, J, K);
-        TWALLIM = wallfun(p.YPLUSWALL, VISLAM, DPERP, d.DPDS_CELL[cm], &d.DPDS_CELL[cm], 0.25f * ROAVGIM,
-                          &d.YPLUS_IM[JKX(J, K)], 0.25f * WAVG, true);
-      } else {
-        const float RE = DPERP * ROAVGIM * WAVG / VISLAM16;
-        const float CF = sa_cf(RE, DPERP, d.ROUGH_U[NRWJ - 1], ROAVGIM, WAVG, VISLAM);
-        TWALLIM = CF * ROAVGIM * WAVG * WAVG / 128.f;
-      }
-      VS[1] = sqrtf(TWALLIM / (0.25f * ROAVGIM));
-      F1[3] = ROAVGIM;
-      FMULT = -TWALLIM * AREAIM / WAVG;
-      const float TFORCEIM = FMULT * WTAVGIM;
-      F1[0] = FMULT * VXAVGIM;
-      F1[1] = FMULT * VRAVGIM;
-      F1[2] = TFORCEIM;
-      const float WVISCIM = TFORCEIM * d.WRAD[J - 1] * d.RAVG_CELL[JKX(J, K)];
-      // energy accumulators of the wall cells in QF/WF slots 0 and IM (not faces)
-      XF[0] = WVISC1;
-      XF[IM] = WVISCIM;
+      VS[lane] = sqrtf(TWALL / (0.25f * ROAVG));
+      float *F = lo ? F0 : F1;
+      F[3] = ROAVG;
+      const float FMULT = -TWALL * AREA / WAVG;
+      const float TFORCE = FMULT * WTAVG;
+      F[0] = 0.0f + FMULT * VXAVG;
+      F[1] = 0.0f + FMULT * VRAVG;
+      F[2] = 0.0f + TFORCE;
+      // energy of the wall cells in XF slots 0 and IM (not faces)
+      XF[lo ? 0 : IM] = TFORCE * d.WRAD[J - 1] * d.RAVG_CELL[JKX(J, K)];
     }
     __syncthreads();
     // SPAL_LOSS sets y+ up to I = IM (mf:14404), NEW_LOSS up to IM-1 (mf:13035)
@@ -3104,13 +3078,16 @@ __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
   }
 }
 
-// Hub and casing friction and the K = const faces, then the body forces,
-// mf:14505-14819.  One thread per (I,J) column, streaming up in K.
-__global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
+// Hub and casing friction of each (I,J) column, mf:14505-14766 (SPAL_LOSS;
+// NEW_LOSS mf:13167-13400): the forces and energy source the two wall
+// cells receive, and the friction velocity and density the column's y+ is
+// formed with, into d.KWALL (kKWall quantities, each a plane of
+// (IM-1) x (JM-1) columns).  One thread per column.
+__global__ void k_sa_kwalls(Geo g, Dev d, Params p) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
   if (I > g.IMM1 || J > g.JM) return;
-  const int IP1 = I + 1, JM1 = J - 1, KMM1 = g.KMM1, KM = g.KM, KMID = p.KMID;
+  const int IP1 = I + 1, JM1 = J - 1, KMM1 = g.KMM1, KM = g.KM;
   const float VISLAM = d.VISC_LAM[IX(g.IMM1, g.JM, KMM1)], VISLAM16 = VISLAM * 16.0f;
   // SPAL_LOSS's hub/casing loop reads the row its blade loop left
   // (mf:14536); NEW_LOSS's takes the row of its own J (mf:13169)
@@ -3136,14 +3113,6 @@ __global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
     const float RE = DPERP * ROAVG1 * WAVG / VISLAM16;
     const float CF = sa_cf(RE, DPERP, d.ROUGH_H[NRWHC - 1], ROAVG1, WAVG, VISLAM);
     TWALLK1 = CF * ROAVG1 * WAVG * WAVG / 128.f;
-  }
-  if (p.IBOUND == 0 || p.IBOUND == 2) {
-    for (int K = 1; K <= KMID; K++) {
-      const unsigned n = IX(I, J, K);
-      const float VSTAR = sqrtf(TWALLK1 / (0.25f * ROAVG1));
-      const float YPLS = VSTAR * sqrtf(d.DWALLSQ[n]) * (0.25f * ROAVG1) / VISLAM;
-      d.Y_PLUS[n] = fminf(YPLS, d.Y_PLUS[n]);
-    }
   }
   float FMULT = -TWALLK1 * AREA1 / WAVG;
   if (p.IBOUND == 1 || p.IBOUND > 2) FMULT = 0.0f;
@@ -3171,43 +3140,87 @@ __global__ void k_sa_kforces(Geo g, Dev d, Params p, float RF_VIS) {
     const float CF = sa_cf(RE, DPERP, d.ROUGH_T[NRWHC - 1], ROAVGKM, WAVG, VISLAM);
     TWALLKM = CF * ROAVGKM * WAVG * WAVG / 128.f;
   }
-  if (p.IBOUND == 0 || p.IBOUND == 1) {
-    for (int K = KMID + 1; K <= KMM1; K++) {
-      const unsigned n = IX(I, J, K);
-      const float VSTAR = sqrtf(TWALLKM / (0.25f * ROAVGKM));
-      const float YPLS = VSTAR * sqrtf(d.DWALLSQ[n]) * (0.25f * ROAVGKM) / VISLAM;
-      d.Y_PLUS[n] = fminf(YPLS, d.Y_PLUS[n]);
-    }
-  }
   FMULT = -TWALLKM * AREAKM / WAVG;
   if (p.IBOUND >= 2) FMULT = 0.0f;
   const float XFORCEKM = FMULT * VXAVGKM, RFORCEKM = FMULT * VRAVGKM, TFORCEKM = FMULT * WTAVGKM;
   const float WVISCKM = TFORCEKM * 0.25f * d.WTIP[J - 1] * RAVGKM;
-  // K faces and the final forces, mf:14770-14819
+  const size_t ncol = (size_t)g.IMM1 * (g.JM - 1), col = (size_t)(I - 1) + (size_t)g.IMM1 * (J - 2);
+  float *W = d.KWALL;
+  W[0 * ncol + col] = XFORCE1;
+  W[1 * ncol + col] = RFORCE1;
+  W[2 * ncol + col] = TFORCE1;
+  W[3 * ncol + col] = WVISC1;
+  W[4 * ncol + col] = sqrtf(TWALLK1 / (0.25f * ROAVG1));
+  W[5 * ncol + col] = ROAVG1;
+  W[6 * ncol + col] = XFORCEKM;
+  W[7 * ncol + col] = RFORCEKM;
+  W[8 * ncol + col] = TFORCEKM;
+  W[9 * ncol + col] = WVISCKM;
+  W[10 * ncol + col] = sqrtf(TWALLKM / (0.25f * ROAVGKM));
+  W[11 * ncol + col] = ROAVGKM;
+}
+
+// The end walls' y+ (mf:14640-14648, 14756-14764), the K = const faces and
+// the body forces (mf:14770-14819) for a segment of kKSeg cells of one
+// (I,J) column per thread; the face below the segment is formed again from
+// the cell under it.
+__global__ void k_sa_kfaces(Geo g, Dev d, Params p, float RF_VIS) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
+  const int k0 = blockIdx.z * kKSeg + 1;
+  if (I > g.IMM1 || J > g.JM || k0 > g.KMM1) return;
+  const int KMM1 = g.KMM1, KMID = p.KMID, IP1 = I + 1, JM1 = J - 1;
+  const int k1 = min(k0 + kKSeg - 1, KMM1);
+  const float VISLAM = d.VISC_LAM[IX(g.IMM1, g.JM, KMM1)];
+  const size_t ncol = (size_t)g.IMM1 * (g.JM - 1), col = (size_t)(I - 1) + (size_t)g.IMM1 * (J - 2);
+  const float *W = d.KWALL;
+  const float VSTAR1 = W[4 * ncol + col], ROAVG1 = W[5 * ncol + col];
+  const float VSTARKM = W[10 * ncol + col], ROAVGKM = W[11 * ncol + col];
+  for (int K = k0; K <= k1; K++) {
+    const unsigned n = IX(I, J, K);
+    if (K <= KMID && (p.IBOUND == 0 || p.IBOUND == 2)) {
+      const float YPLS = VSTAR1 * sqrtf(d.DWALLSQ[n]) * (0.25f * ROAVG1) / VISLAM;
+      d.Y_PLUS[n] = fminf(YPLS, d.Y_PLUS[n]);
+    }
+    if (K > KMID && (p.IBOUND == 0 || p.IBOUND == 1)) {
+      const float YPLS = VSTARKM * sqrtf(d.DWALLSQ[n]) * (0.25f * ROAVGKM) / VISLAM;
+      d.Y_PLUS[n] = fminf(YPLS, d.Y_PLUS[n]);
+    }
+  }
+  const float *VX = d.VX, *VR = d.VR, *VT = d.VT;
+  // the face between cells K-1 and K (K >= 2) and its viscous work
+  auto face = [&](int K, const SaStress &lo, const SaStress &hi, float &xf, float &rf, float &tf, float &qf,
+                  float &wf) {
+    const unsigned m = IX(I, J, K);
+    const float asr = d.ASR[m], asx = d.ASX[m];
+    xf = 0.5f * ((hi.xr + lo.xr) * asr + (hi.xx + lo.xx) * asx);
+    rf = 0.5f * ((hi.rr + lo.rr) * asr + (hi.xr + lo.xr) * asx);
+    tf = 0.5f * ((hi.rt + lo.rt) * asr + (hi.xt + lo.xt) * asx);
+    qf = 0.5f * ((hi.qx + lo.qx) * asx + (hi.qr + lo.qr) * asr);
+    const float VXAVG = VX[IX(I, J, K)] + VX[IX(IP1, J, K)] + VX[IX(IP1, JM1, K)] + VX[IX(I, JM1, K)];
+    const float VRAVG = VR[IX(I, J, K)] + VR[IX(IP1, J, K)] + VR[IX(IP1, JM1, K)] + VR[IX(I, JM1, K)];
+    const float VTAVG = VT[IX(I, J, K)] + VT[IX(IP1, J, K)] + VT[IX(IP1, JM1, K)] + VT[IX(I, JM1, K)];
+    wf = -0.25f * (xf * VXAVG + rf * VRAVG + tf * VTAVG);
+  };
   const float RF_VIS1 = 1.0f - RF_VIS;
-  SaStress sb = sa_stress(g, d, p, IX(I, J, 1));
+  SaStress sb = sa_stress(g, d, p, IX(I, J, k0));
   float xfK = 0.f, rfK = 0.f, tfK = 0.f, qfK = 0.f, wfK = 0.f;  // face K (between K-1 and K)
-  for (int K = 1; K <= KMM1; K++) {
+  if (k0 >= 2) face(k0, sa_stress(g, d, p, IX(I, J, k0 - 1)), sb, xfK, rfK, tfK, qfK, wfK);
+  for (int K = k0; K <= k1; K++) {
     const unsigned n = IX(I, J, K);
     float xfN = 0.f, rfN = 0.f, tfN = 0.f, qfN = 0.f, wfN = 0.f;  // face K+1
     if (K + 1 <= KMM1) {
-      const unsigned m = IX(I, J, K + 1);
-      const SaStress st = sa_stress(g, d, p, m);
-      const float asr = d.ASR[m], asx = d.ASX[m];
-      xfN = 0.5f * ((st.xr + sb.xr) * asr + (st.xx + sb.xx) * asx);
-      rfN = 0.5f * ((st.rr + sb.rr) * asr + (st.xr + sb.xr) * asx);
-      tfN = 0.5f * ((st.rt + sb.rt) * asr + (st.xt + sb.xt) * asx);
-      qfN = 0.5f * ((st.qx + sb.qx) * asx + (st.qr + sb.qr) * asr);
-      const int KN = K + 1;
-      const float VXAVG = VX[IX(I, J, KN)] + VX[IX(IP1, J, KN)] + VX[IX(IP1, JM1, KN)] + VX[IX(I, JM1, KN)];
-      const float VRAVG = VR[IX(I, J, KN)] + VR[IX(IP1, J, KN)] + VR[IX(IP1, JM1, KN)] + VR[IX(I, JM1, KN)];
-      const float VTAVG = VT[IX(I, J, KN)] + VT[IX(IP1, J, KN)] + VT[IX(IP1, JM1, KN)] + VT[IX(I, JM1, KN)];
-      wfN = -0.25f * (xfN * VXAVG + rfN * VRAVG + tfN * VTAVG);
+      const SaStress st = sa_stress(g, d, p, IX(I, J, K + 1));
+      face(K + 1, sb, st, xfN, rfN, tfN, qfN, wfN);
       sb = st;
     }
     float fx = d.FORCEX[n], fr = d.FORCER[n], ft = d.FORCET[n], es = d.ESOURCE[n];
-    if (K == 1) { fx = fx + XFORCE1; fr = fr + RFORCE1; ft = ft + TFORCE1; es = es + WVISC1; }
-    if (K == KMM1) { fx = fx + XFORCEKM; fr = fr + RFORCEKM; ft = ft + TFORCEKM; es = es + WVISCKM; }
+    if (K == 1) {
+      fx = fx + W[0 * ncol + col]; fr = fr + W[1 * ncol + col]; ft = ft + W[2 * ncol + col]; es = es + W[3 * ncol + col];
+    }
+    if (K == KMM1) {
+      fx = fx + W[6 * ncol + col]; fr = fr + W[7 * ncol + col]; ft = ft + W[8 * ncol + col]; es = es + W[9 * ncol + col];
+    }
     if (K >= 2) { fx = fx - xfK; fr = fr - rfK; ft = ft - tfK; es = es + qfK + wfK; }
     if (K + 1 <= KMM1) { fx = fx + xfN; fr = fr + rfN; ft = ft + tfN; es = es - qfN - wfN; }
     d.XFORCE[n] = RF_VIS1 * d.XFORCE[n] - RF_VIS * fx;
