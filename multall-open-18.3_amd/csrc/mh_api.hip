@@ -26,7 +26,8 @@ __global__ void k_flux_energy(Geo, Dev);
 __global__ void k_flux_axial(Geo, Dev);
 __global__ void k_flux_angular(Geo, Dev);
 __global__ void k_flux_radial(Geo, Dev);
-__global__ void k_bc_sums(Geo, Dev, Params, StepScalars);
+__global__ void k_bc_lines(Geo, Dev, Params);
+__global__ void k_bc_pd(Geo, Dev, Params, StepScalars);
 __global__ void k_bc_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_peff(Geo, Dev, Params, StepScalars);
 __global__ void k_fextrap(Geo, Dev, Params);
@@ -846,7 +847,8 @@ int step_once(mh_ctx *c) {
   LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_energy");
   if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
-  LAUNCH(k_bc_sums, dim3(1), dim3(1024), 6 * g.KM * sizeof(float), c->stream, g, d, p, s);
+  LAUNCH(k_bc_lines, dim3(g.KM), dim3(64), 0, c->stream, g, d, p);
+  LAUNCH(k_bc_pd, dim3(1), dim3(128), 0, c->stream, g, d, p, s);
   LAUNCH(k_bc_apply, dim3((g.IM * g.KM + 255) / 256), dim3(256), 0, c->stream, g, d, p, s);
   LCHK("bc");
   if (p.IN_FLOW != 0) {
@@ -942,8 +944,9 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   if (!dims || !out) return MH_E_ARG;
   *out = nullptr;
   const mh_dims &dm = *dims;
-  // pitchwise lines are summed on one wavefront in up to 4 chunks (IM-1 <= 256)
-  if (dm.im < 4 || dm.im > 257 || dm.km < 4 || dm.jm < 10 || dm.nrows < 1 || dm.nrows > 64 || dm.nib1 < 1 || dm.njb1 < 1 ||
+  // pitchwise lines are summed on one wavefront in up to 4 chunks (IM-1 <= 256);
+  // the exit-pressure recursion stages up to 512 K values in LDS
+  if (dm.im < 4 || dm.im > 257 || dm.km < 4 || dm.km > 512 || dm.jm < 10 || dm.nrows < 1 || dm.nrows > 64 || dm.nib1 < 1 || dm.njb1 < 1 ||
       dm.nkb1 < 1 || dm.nib2 < 1 || dm.njb2 < 1 || dm.nkb2 < 1 || dm.nsblk < 1)
     return MH_E_ARG;
   mh_ctx *c = new mh_ctx();
@@ -984,7 +987,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.TVGT, N)) || (r = dalloc(c, &d.FORCEX, N)) || (r = dalloc(c, &d.FORCER, N)) ||
       (r = dalloc(c, &d.FORCET, N)) || (r = dalloc(c, &d.ESOURCE, N)) ||
       (r = dalloc(c, &d.B1CHG, kind_len(dm, K_B1))) || (r = dalloc(c, &d.B2CHG, kind_len(dm, K_B2))) ||
-      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)3 * dm.km + 4)) ||
+      (r = dalloc(c, &d.SBCHG, (size_t)dm.nsblk)) || (r = dalloc(c, &d.BCW, (size_t)4 * dm.km + 4)) ||
       (r = dalloc(c, &d.LTAB, (size_t)dm.nrows * (4 * dm.im + 2 * dm.km))) ||
       (r = dalloc(c, &d.SAFLAG, (size_t)dm.im * dm.jm + (size_t)dm.jm * dm.km)) ||
       (r = dalloc(c, &d.SAJMAP, (size_t)4 * dm.jm + 4)) || (r = dalloc(c, &d.JMP, (size_t)dm.jm)) ||

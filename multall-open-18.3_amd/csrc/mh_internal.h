@@ -60,7 +60,7 @@ struct Dev {
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
-  float *BCW;                        // exit/inlet pitch averages per K (3*KM)
+  float *BCW;                        // inlet/exit pitch averages per K, SA inlet density, swirl sums (4*KM+4)
   float *LTAB;                       // LOSS mixing-length factors, see LT_* below
   // Spalart-Allmaras work arrays (cell-centred, N): the velocity and
   // temperature gradients of SPAL_LOSS (strain rates, before the viscosity),

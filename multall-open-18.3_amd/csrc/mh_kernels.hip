@@ -320,10 +320,10 @@ __global__ void k_flux_radial(Geo g, Dev d) {
 
 // ---------------------------------------------------------------------------
 // inlet and exit boundary conditions, mf:4476-4734 (IFGAS=0, ITIMST<5,
-// no throttle or plate loss).  k_bc_sums (one workgroup) forms the pitchwise
-// sums per K and the exit-pressure recursion PD(K), which is serial in K;
+// no throttle or plate loss).  k_bc_lines forms the pitchwise sums per K,
+// k_bc_pd the exit-pressure recursion PD(K), which is serial in K, and
 // k_bc_apply updates the inlet and exit nodes in parallel.  d.BCW carries
-// PAVG2, PAVG3 and PAVGOUT (KM each) from the first to the second.
+// PAVG2, PAVG3 and PAVGOUT (KM each) between them.
 
 // inlet pressure and static temperature of node (I,1,K) from the state
 // before the update, mf:4566-4606
@@ -349,62 +349,73 @@ __device__ __forceinline__ void inlet_pt(const Geo &g, const Dev &d, const Param
   TSTATIC = TO1 * pow_r(P1 * d.RPO1[K - 1], p.FGA);
 }
 
-__global__ void k_bc_sums(Geo g, Dev d, Params p, StepScalars s) {
-  extern __shared__ float sh[];
-  float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *PAVGOUT = d.BCW + 2 * g.KM, *SUMK = sh + 3 * g.KM;
+// Pitchwise sums per K in the reference's order over I, mf:4512-4521: the
+// static pressure averages on J = 2, 3 (inlet) and JM-1 (exit) and the
+// swirl term of the exit's radial equilibrium.  One wavefront per K line,
+// into d.BCW (PAVG2, PAVG3, PAVGOUT at 0, KM, 2 KM; the swirl sums at
+// 3 KM + 4).
+__global__ void __launch_bounds__(64) k_bc_lines(Geo g, Dev d, Params p) {
+  const int K = blockIdx.x + 1, lane = threadIdx.x;
   const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
-  const int lane = threadIdx.x & 63, wave = wave_id(), nwave = blockDim.x >> 6;
-  // pitchwise sums per K in the reference's order over I, mf:4512-4521
-  for (int K = wave + 1; K <= g.KM; K += nwave) {
-    float v2[4], v3[4], vo[4], vs[4];
-    const float rjm = d.R[JKX(JM, K)];
+  float v2[4], v3[4], vo[4], vs[4];
+  const float rjm = d.R[JKX(JM, K)];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int I = 64 * c + lane + 1;
-      if (I <= IMM1) {
-        const float fp = d.FP[I - 1];
-        v2[c] = 0.5f * (d.P[IX(I, 2, K)] + d.P[IX(I + 1, 2, K)]) * fp;
-        v3[c] = 0.5f * (d.P[IX(I, 3, K)] + d.P[IX(I + 1, 3, K)]) * fp;
-        vo[c] = 0.5f * (d.P[IX(I, JMM1, K)] + d.P[IX(I + 1, JMM1, K)]) * fp;
-        vs[c] = 0.5f * fp *
-                (d.ROVT[IX(I, JM, K)] * d.VT[IX(I, JM, K)] + d.ROVT[IX(I + 1, JM, K)] * d.VT[IX(I + 1, JM, K)]) / rjm;
-      } else {
-        v2[c] = v3[c] = vo[c] = vs[c] = 0.0f;
-      }
-    }
-    const float a2 = line_sum(v2, IMM1, p.EXACT), a3 = line_sum(v3, IMM1, p.EXACT);
-    const float ao = line_sum(vo, IMM1, p.EXACT), sm = line_sum(vs, IMM1, p.EXACT);
-    if (lane == 0) {
-      PAVG2[K - 1] = a2;
-      PAVG3[K - 1] = a3;
-      PAVGOUT[K - 1] = ao;
-      SUMK[K - 1] = sm;
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    if (I <= IMM1) {
+      const float fp = d.FP[I - 1];
+      v2[c] = 0.5f * (d.P[IX(I, 2, K)] + d.P[IX(I + 1, 2, K)]) * fp;
+      v3[c] = 0.5f * (d.P[IX(I, 3, K)] + d.P[IX(I + 1, 3, K)]) * fp;
+      vo[c] = 0.5f * (d.P[IX(I, JMM1, K)] + d.P[IX(I + 1, JMM1, K)]) * fp;
+      vs[c] = 0.5f * fp *
+              (d.ROVT[IX(I, JM, K)] * d.VT[IX(I, JM, K)] + d.ROVT[IX(I + 1, JM, K)] * d.VT[IX(I + 1, JM, K)]) / rjm;
+    } else {
+      v2[c] = v3[c] = vo[c] = vs[c] = 0.0f;
     }
   }
-  __syncthreads();
-  // exit pressure from radial equilibrium, serial in K, mf:4531-4542
-  float *PDL = sh + 4 * g.KM, *RJM = sh + 5 * g.KM;
-  for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) {
-    PDL[K - 1] = d.PD[K - 1];
-    RJM[K - 1] = d.R[JKX(JM, K)];
+  const float a2 = line_sum(v2, IMM1, p.EXACT), a3 = line_sum(v3, IMM1, p.EXACT);
+  const float ao = line_sum(vo, IMM1, p.EXACT), sm = line_sum(vs, IMM1, p.EXACT);
+  if (lane == 0) {
+    d.BCW[K - 1] = a2;
+    d.BCW[g.KM + K - 1] = a3;
+    d.BCW[2 * g.KM + K - 1] = ao;
+    d.BCW[3 * g.KM + 4 + K - 1] = sm;
+  }
+}
+
+// Exit pressure from radial equilibrium, serial in K (mf:4531-4542).  The
+// workgroup stages the per-K inputs in LDS; thread 0 runs the recursion
+// with the running value in a register, reading only LDS, while thread 64
+// forms the SA inlet density of the mid-passage node.  The reference resets
+// PD(1) to PDOWN_HUB on every pass of its K loop, so after the loop PD(1)
+// is PDOWN_HUB and the K = 2 update reads it without the IPOUT = -1 shift.
+constexpr int kBcMaxK = 512;
+__global__ void __launch_bounds__(128) k_bc_pd(Geo g, Dev d, Params p, StepScalars s) {
+  __shared__ float sumk[kBcMaxK], rjm[kBcMaxK], pdold[kBcMaxK], pdnew[kBcMaxK];
+  const float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *SUMK = d.BCW + 3 * g.KM + 4;
+  const int KM = g.KM, JM = g.JM;
+  for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
+    sumk[K - 1] = SUMK[K - 1];
+    rjm[K - 1] = d.R[JKX(JM, K)];
+    pdold[K - 1] = d.PD[K - 1];
   }
   __syncthreads();
   if (threadIdx.x == 0 && p.IPOUT <= 0) {
-    float PDIFF = 0.0f, SUMP = 0.0f;
-    if (p.IPOUT == -1) PDIFF = p.PDOWN_TIP - PDL[g.KM - 1];
-    for (int K = 1; K <= g.KM; K++) {
-      PDL[0] = p.PDOWN_HUB;
-      if (K != 1) {
-        float DP = (SUMK[K - 1] + SUMP) * 0.5f * (RJM[K - 1] - RJM[K - 2]);
-        PDL[K - 1] = 0.9f * PDL[K - 1] + 0.1f * (PDL[K - 2] + DP);
-      }
-      SUMP = SUMK[K - 1];
-      if (p.IPOUT == -1) PDL[K - 1] = PDL[K - 1] + PDIFF;
+    float PDIFF = 0.0f;
+    if (p.IPOUT == -1) PDIFF = p.PDOWN_TIP - pdold[KM - 1];
+    float prev = p.PDOWN_HUB, SUMP = sumk[0];  // PD(K-1) and SUMK(K-1) as the K-th pass reads them
+#pragma unroll 8
+    for (int K = 2; K <= KM; K++) {
+      const float DP = (sumk[K - 1] + SUMP) * 0.5f * (rjm[K - 1] - rjm[K - 2]);
+      float v = 0.9f * pdold[K - 1] + 0.1f * (prev + DP);
+      if (p.IPOUT == -1) v = v + PDIFF;
+      pdnew[K - 1] = v;
+      SUMP = sumk[K - 1];
+      prev = v;
     }
+    pdnew[0] = (KM >= 2) ? p.PDOWN_HUB : p.PDOWN_HUB + PDIFF;
   }
-  __syncthreads();
-  for (int K = threadIdx.x + 1; K <= g.KM; K += blockDim.x) d.PD[K - 1] = PDL[K - 1];
-  if (threadIdx.x == 0 && p.ILOS >= 200) {
+  if (threadIdx.x == 64 && p.ILOS >= 200) {
     // RO(IMID,1,KMID) before and after its inlet update (the SA inlet
     // condition reads whichever the reference's loop has reached)
     float P1, TSTATIC;
@@ -412,6 +423,9 @@ __global__ void k_bc_sums(Geo g, Dev d, Params p, StepScalars s) {
     d.BCW[3 * g.KM] = d.RO[IX(p.IMID, 1, p.KMID)];
     d.BCW[3 * g.KM + 1] = P1 / (p.RGAS * TSTATIC);
   }
+  __syncthreads();
+  if (p.IPOUT <= 0)
+    for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) d.PD[K - 1] = pdnew[K - 1];
 }
 
 __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
@@ -471,7 +485,7 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
     if (p.ILOS >= 200) {
       // inlet turbulent viscosity, mf:4679-4690; the reference divides by
       // RO(IMID,1,KMID) as its (K, I) loop has left it: already updated
-      // for (K,I) at or after (KMID,IMID), k_bc_sums kept both values
+      // for (K,I) at or after (KMID,IMID), k_bc_pd kept both values
       const float FST = d.FSTURB[0], VL = d.VISC_LAM[IX(I, 2, K)];
       const float TRANSVISIN = (FST < 1.0f) ? VL * 4.35f * pow_r(FST, 0.25f) : VL * (3.5f + FST * 0.85f);
       const bool after = K > p.KMID || (K == p.KMID && I >= p.IMID);
@@ -3279,27 +3293,26 @@ __device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Param
   rolimit = ROSTAG_REL / RORATIO;
 }
 
-// Density limiter on the flagged (I,K) lines, mf:5115-5122.  The thread of
-// the line (IMID, KMID), which holds every row's reference node, then forms
-// the per-row relative Mach limits (mf:5127-5145) into d.SC for
-// k_mach_apply, after its own line's fix.
+// Density limiter on the flagged (I,K) lines, mf:5115-5122.  The workgroup
+// holding the line (IMID, KMID), on which every row's reference node lies,
+// then forms the per-row relative Mach limits (mf:5127-5145) into d.SC for
+// k_mach_apply, one row per thread, after that line's fix.
 __global__ void k_rolim_fix(Geo g, Dev d, StepScalars s, Params p) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.IM * g.KM) return;
-  const int I = t % g.IM + 1, K = t / g.IM + 1;
-  if (d.FLAGS[t] != 0) {
+  if (t < g.IM * g.KM && d.FLAGS[t] != 0) {
+    const int I = t % g.IM + 1, K = t / g.IM + 1;
     d.FLAGS[t] = 0;  // ready for the next step's check
     for (int J = 3; J <= g.JM - 3; J++)
       d.RO[IX(I, J, K)] = rolim_one(d.RO[IX(I, J, K)], d.RO[IX(I, J - 2, K)], d.RO[IX(I, J - 1, K)],
                                     d.RO[IX(I, J + 1, K)], d.RO[IX(I, J + 2, K)], s.ROLIM);
   }
-  if (I == p.IMID && K == p.KMID)
-    for (int NR = 1; NR <= g.NROWS; NR++)
-      mach_ref(g, d, p, s.TCOOL_MIN, NR, d.SC[SC_VLIMIT + NR - 1], d.SC[SC_ROLIMIT + NR - 1]);
+  const int tmid = (p.IMID - 1) + g.IM * (p.KMID - 1);
+  if (tmid / (int)blockDim.x != (int)blockIdx.x) return;
+  __syncthreads();
+  for (int NR = threadIdx.x + 1; NR <= g.NROWS; NR += blockDim.x)
+    mach_ref(g, d, p, s.TCOOL_MIN, NR, d.SC[SC_VLIMIT + NR - 1], d.SC[SC_ROLIMIT + NR - 1]);
 }
 
-// relative Mach number and mass-flux limiter, mf:5150-5168, with the per-row
-// limits k_rolim_fix left in d.SC
 __global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
   NODE_IJK
   const size_t n = IX(I, J, K);
