@@ -3488,13 +3488,19 @@ __global__ void k_vrms_part(Geo g, Dev d, double *part) {
   const size_t N = (size_t)g.IM * g.JM * g.KM;
   double s = 0.0;
   float m = 0.0f;
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + tid; n < N; n += (size_t)gridDim.x * blockDim.x) {
-    const int J = (int)((n / g.IM) % g.JM) + 1;
+  // wavefronts walk whole (J,K) lines, lanes the I points of a line
+  const int lane = tid & 63, nw = blockDim.x >> 6, nlines = g.JM * g.KM;
+  for (int ln = blockIdx.x * nw + wave_id(); ln < nlines; ln += gridDim.x * nw) {
+    const int J = ln % g.JM + 1, K = ln / g.JM + 1;
     if (J < 2) continue;
-    float EKE = d.VR[n] * d.VR[n] + d.WT[n] * d.WT[n] + d.VX[n] * d.VX[n];
-    s += (double)EKE;
-    if (EKE > m) m = EKE;
+    for (int I = lane + 1; I <= g.IM; I += 64) {
+      const size_t n = IX(I, J, K);
+      float EKE = d.VR[n] * d.VR[n] + d.WT[n] * d.WT[n] + d.VX[n] * d.VX[n];
+      s += (double)EKE;
+      if (EKE > m) m = EKE;
+    }
   }
+  (void)N;
   ssum[tid] = s;
   smax[tid] = m;
   __syncthreads();
@@ -3561,8 +3567,10 @@ __global__ void k_checkflow(Geo g, Dev d, int exact) {
     for (int I = 1; I <= g.IMM1; I++)
       for (int K = 1; K <= g.KMM1; K++) SUMAS = SUMAS - d.FLOWX[IX(I, J, K)];
   } else {
+    // lanes over I, K lines in turn, FP64
     double s = 0.0;
-    for (int c = threadIdx.x; c < g.IMM1 * g.KMM1; c += 64) s -= (double)d.FLOWX[IX(c % g.IMM1 + 1, J, c / g.IMM1 + 1)];
+    for (int K = 1; K <= g.KMM1; K++)
+      for (int I = threadIdx.x + 1; I <= g.IMM1; I += 64) s -= (double)d.FLOWX[IX(I, J, K)];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (threadIdx.x != 0) return;
     SUMAS = (float)s;
@@ -3589,8 +3597,8 @@ __global__ void __launch_bounds__(64) k_setflo_flow(Geo g, Dev d, int exact) {
     return;
   }
   double s = 0.0;
-  for (int c = threadIdx.x; c < g.IMM1 * g.KMM1; c += 64)
-    s -= (double)(d.FLOWX[IX(c % g.IMM1 + 1, J, c / g.IMM1 + 1)] * NBJ / NB1);
+  for (int K = 1; K <= g.KMM1; K++)
+    for (int I = threadIdx.x + 1; I <= g.IMM1; I += 64) s -= (double)(d.FLOWX[IX(I, J, K)] * NBJ / NB1);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (threadIdx.x == 0) d.SETFLO[J - 1] = (float)s;
 }
@@ -3689,13 +3697,19 @@ __global__ void k_emax_part(Geo g, Dev d, double *part) {
   double s = 0.0;
   float ba = -1.0f, bv = 0.0f;
   long long bi = -1;
-  for (size_t n = (size_t)blockIdx.x * blockDim.x + tid; n < N; n += (size_t)gridDim.x * blockDim.x) {
-    const int I = (int)(n % g.IM) + 1, J = (int)((n / g.IM) % g.JM) + 1, K = (int)(n / ((size_t)g.IM * g.JM)) + 1;
+  // wavefronts walk whole (J,K) lines, lanes the I points of a line
+  const int lane = tid & 63, nw = blockDim.x >> 6, nlines = g.JM * g.KM;
+  for (int ln = blockIdx.x * nw + wave_id(); ln < nlines; ln += gridDim.x * nw) {
+    const int J = ln % g.JM + 1, K = ln / g.JM + 1;
     if (J < 2) continue;
-    float v = dvmer_at(g, d, I, J, K, RVEF);
-    s += (double)fabsf(v);
-    if (fabsf(v) > ba) { ba = fabsf(v); bv = v; bi = (long long)n; }
+    for (int I = lane + 1; I <= g.IM; I += 64) {
+      const size_t n = IX(I, J, K);
+      float v = dvmer_at(g, d, I, J, K, RVEF);
+      s += (double)fabsf(v);
+      if (fabsf(v) > ba) { ba = fabsf(v); bv = v; bi = (long long)n; }
+    }
   }
+  (void)N;
   ssum[tid] = s; sabs[tid] = ba; sval[tid] = bv; sidx[tid] = bi;
   __syncthreads();
   for (int off = blockDim.x / 2; off > 0; off >>= 1) {
