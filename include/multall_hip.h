@@ -186,6 +186,17 @@ int mh_start(mh_ctx *ctx);
 /* Run up to nsteps time steps (fewer if the run ends).  *out (optional)
  * receives the latest convergence summary. */
 int mh_step(mh_ctx *ctx, int nsteps, mh_resid *out);
+
+/* Several independent machines (operating points of a speedline, say) at
+ * once: each of the n distinct contexts advances up to nsteps steps, as
+ * mh_step would, driven by a host thread of its own on its own stream, so
+ * the kernels of different contexts overlap on the GPU.  out[i] receives
+ * context i's residuals.  Returns 0, or the first failing context's code
+ * with its index in *failed (the message is that context's
+ * mh_last_error).  No counterpart in the reference, which runs one
+ * operating point per process; the drivers of a speedline or of
+ * operating-point sweeps call it. */
+int mh_step_batch(mh_ctx *const *ctxs, int n, int nsteps, mh_resid *out, int *failed);
 /* Per-J mass-flux-weighted sums of SUMFLX (mf:7883-7910) in FP64:
  * out[7*JM] = {flow, sum_po, sum_to, sum_rvt, sum_entropy, sum_tstat,
  * sum_pstat} per J, with p0/po1 and T0 from the current state. */

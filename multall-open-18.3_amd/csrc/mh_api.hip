@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -1174,6 +1175,26 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
     if (c->prof_on && c->prof_pending.size() > 4096 && (r = prof_collect(c))) return r;
   }
   if (out) *out = c->last;
+  return MH_OK;
+}
+
+int mh_step_batch(mh_ctx *const *ctxs, int n, int nsteps, mh_resid *out, int *failed) {
+  if (!ctxs || !out || n < 1 || nsteps < 0) return MH_E_ARG;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i]) return MH_E_ARG;
+    for (int k = 0; k < i; k++)
+      if (ctxs[k] == ctxs[i]) return fail(ctxs[i], MH_E_ARG, "mh_step_batch: a context appears twice");
+  }
+  std::vector<int> rc(n, MH_OK);
+  std::vector<std::thread> th;
+  th.reserve(n);
+  for (int i = 0; i < n; i++) th.emplace_back([&, i] { rc[i] = mh_step(ctxs[i], nsteps, &out[i]); });
+  for (auto &t : th) t.join();
+  for (int i = 0; i < n; i++)
+    if (rc[i] != MH_OK) {
+      if (failed) *failed = i;
+      return rc[i];
+    }
   return MH_OK;
 }
 

@@ -537,3 +537,38 @@ def deck_config1(nsteps: int = 5000, ilos: int = 200) -> DeckSpec:
     return DeckSpec(title="CONFIG1 1.5-STAGE TURBINE 3X64X64X193", im=64, km=64, rows=[s1, r1, s2],
                     ilos=ilos, nsteps=nsteps, conlim=0.005, nchange=1000, seed=1,
                     po1=2.0e5, to1=1000.0, vm1=150.0, pdown_hub=1.1e5, pdown_tip=1.1e5)
+
+
+def speedline_sizes(n_points: int = 16, seed: int = 3):
+    """The seeded grid-size draw of the speedline: (IM, KM, JM per row)."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([37, 49, 65])), int(rng.choice([33, 49, 65])), int(rng.choice([121, 161, 193])))
+            for _ in range(n_points)]
+
+
+def deck_speedline_point(i: int, n_points: int = 16, nsteps: int = 3000, seed: int = 3) -> DeckSpec:
+    """Point i of the batched speedline: a single-stage axial compressor
+    (rotor, 36 blades at 9000 rpm, then stator, 48 blades), hub/tip radii
+    0.25/0.35 m, inlet p0 = 1.013e5 Pa, T0 = 288 K, axial, with the exit
+    static pressure linearly spaced from 1.2 to 1.6 times the inlet total
+    pressure over the points (choke towards stall).  Grid sizes come from a
+    seeded draw: IM from {37, 49, 65}, KM from {33, 49, 65}, JM per row from
+    {121, 161, 193}.  Circular-arc camber, 8 % thickness, mixing-length
+    viscous model, SCREE, multigrid 3/9."""
+    im, km, jm = speedline_sizes(n_points, seed)[i]
+    frac = i / max(1, n_points - 1)
+    po1 = 1.013e5
+    pexit = po1 * (1.2 + 0.4 * frac)
+    jle, jte = jm // 4 + 1, jm - jm // 4
+    pin = 0.88e5
+    pmid = 0.5 * (pin + pexit)
+    rotor = RowSpec(name="ROTOR", nblades=36, jm=jm, jle=jle, jte=jte, chord_x=0.04,
+                    alpha_in=-60.0, alpha_out=-35.0, tmax=0.08, rpm=9000.0,
+                    pguess=(pin, pin, pmid, pmid))
+    stator = RowSpec(name="STATOR", nblades=48, jm=jm, jle=jle, jte=jte, chord_x=0.04,
+                     alpha_in=50.0, alpha_out=0.0, tmax=0.08,
+                     pguess=(pmid, pmid, pexit, pexit))
+    return DeckSpec(title=f"SPEEDLINE POINT {i + 1} OF {n_points}: COMPRESSOR STAGE {im}X{km}X2X{jm} PEXIT {pexit:.0f}",
+                    im=im, km=km, rows=[rotor, stator], r_hub=0.25, r_tip=0.35,
+                    ilos=10, nsteps=nsteps, conlim=0.005, nchange=500, seed=seed + i,
+                    po1=po1, to1=288.0, vm1=150.0, pdown_hub=pexit, pdown_tip=pexit)

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmultall_hip.so")
 EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar_id", "mh_array_len",
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
            "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_start", "mh_step",
-           "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report"]
+           "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch"]
 
 
 class MhDims(ctypes.Structure):
@@ -72,6 +72,7 @@ def load(path: str = LIB_PATH):
     L.mh_start.argtypes = [vp]
     L.mh_loss_init.argtypes = [vp]
     L.mh_step.argtypes = [vp, i, ctypes.POINTER(MhResid)]
+    L.mh_step_batch.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(MhResid), ctypes.POINTER(i)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
     L.mh_sync.argtypes = [vp]
     L.mh_profile_enable.argtypes = [vp, i]
@@ -186,3 +187,19 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def step_batch(solvers, n: int):
+    """Advance several solvers (distinct machines) by up to n steps each,
+    concurrently on the GPU (mh_step_batch); returns their residual dicts."""
+    L = load()
+    k = len(solvers)
+    hs = (ctypes.c_void_p * k)(*[s.h.value for s in solvers])
+    out = (MhResid * k)()
+    failed = ctypes.c_int(-1)
+    rc = L.mh_step_batch(hs, k, int(n), out, ctypes.byref(failed))
+    if rc != 0:
+        i = failed.value
+        msg = L.mh_last_error(solvers[i].h).decode(errors="replace") if 0 <= i < k else ""
+        raise MhError(f"mh_step_batch: context {i} rc={rc} {msg}")
+    return [out[i].as_dict() for i in range(k)]

@@ -2,12 +2,15 @@
 
 The library is run to convergence (the reference's own stop test) on the
 reference's Rotor 37 deck (NASA transonic compressor rotor, 45 x 150 x 49
-nodes, 36 blades) with plain Spalart-Allmaras, and its mass flow, total
-pressure ratio and total-to-total isentropic efficiency are compared with the
-FP64 build of the reference (-fdefault-real-8) run on the same deck: within
-0.05 % for mass flow and pressure ratio and 0.05 points for efficiency.
+nodes, 36 blades), as it ships (Spalart-Allmaras with its helicity and
+pressure-gradient source factors and the Shih wall functions with the
+pressure-gradient term) and with plain Spalart-Allmaras and the log-law
+walls, and its mass flow, total pressure ratio and total-to-total isentropic
+efficiency are compared with the FP64 build of the reference
+(-fdefault-real-8) run on the same deck: within 0.05 % for mass flow and
+pressure ratio and 0.05 points for efficiency.
 
-Fixture: tests/golden/converged_r37sa.json holds the per-J SUMFLX integrals
+Fixtures: tests/golden/converged_<case>.json hold the per-J SUMFLX integrals
 (BLADE_FLOW, SUMPO, SUMTO, SUMRVT, SUM_ENTPY) that EFICOOL wrote at the end of
 the FP64 and FP32 reference runs (mf:10748-10755), made by
 tests/golden/make_converged.py.  The quantities are formed from those
@@ -21,7 +24,7 @@ import pytest
 
 from conftest import GOLDEN
 
-FIXTURE = os.path.join(GOLDEN, "converged_r37sa.json")
+CASES = ["r37samods", "r37sa"]
 BOUND_REL = 5e-4     # 0.05 % (mass flow, pressure ratio)
 BOUND_ETA = 5e-4     # 0.05 points of efficiency
 
@@ -37,14 +40,14 @@ def quantities(integrals, ga):
     return {"mass_flow": float(flow[-1]), "p0_ratio": float(p0r), "t0_ratio": float(t0r), "eta": float(eta)}
 
 
-def _fixture():
-    with open(FIXTURE) as f:
+def _fixture(case):
+    with open(os.path.join(GOLDEN, f"converged_{case}.json")) as f:
         return json.load(f)
 
 
-def _ga():
+def _ga(case):
     from multall_hip import host
-    st = host.load_deck(os.path.join(GOLDEN, _fixture()["deck"]))
+    st = host.load_deck(os.path.join(GOLDEN, _fixture(case)["deck"]))
     return st, float(st["scalars"]["GA"])
 
 
@@ -58,11 +61,12 @@ def _check(q, ref, label):
     assert err <= BOUND_ETA, (label, err)
 
 
-def test_fp32_reference_within_bounds_of_fp64():
+@pytest.mark.parametrize("case", CASES)
+def test_fp32_reference_within_bounds_of_fp64(case):
     """The bounds are met by the reference's own FP32 build: FP32 arithmetic
     itself is not the limit at convergence."""
-    d = _fixture()
-    _, ga = _ga()
+    d = _fixture(case)
+    _, ga = _ga(case)
     q64 = quantities(d["fp64"]["integrals"], ga)
     q32 = quantities(d["fp32"]["integrals"], ga)
     assert 1.5 < q64["p0_ratio"] < 2.5 and 0.5 < q64["eta"] < 1.0, q64
@@ -70,10 +74,11 @@ def test_fp32_reference_within_bounds_of_fp64():
 
 
 @pytest.mark.gpu
-def test_converged_rotor37_vs_fp64_reference():
+@pytest.mark.parametrize("case", CASES)
+def test_converged_rotor37_vs_fp64_reference(case):
     from multall_hip.runtime import Solver
-    d = _fixture()
-    st, ga = _ga()
+    d = _fixture(case)
+    st, ga = _ga(case)
     g = Solver(st, exact_order=False)
     g.loss_init()
     g.start()
