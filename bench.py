@@ -13,9 +13,11 @@ dominant kernel (dispatch-timestamped event durations on the library's
 stream, HBM traffic from the committed counter runs), the step-level fraction
 of the 8 TB/s HBM roofline at the 132 B/node-step algorithmic traffic of a
 fused FP32 step, the reference solver timed on this host's CPU
-(single-threaded, as built by oracle/build_ref.sh), and the time to
-converge Rotor 37 (the reference's own test deck, plain Spalart-Allmaras) by
-the reference's stop test.
+(single-threaded, as built by oracle/build_ref.sh), the time to converge
+Rotor 37 (the reference's own test deck as it ships) by the reference's stop
+test, and the batched speedline: 16 operating points of a single-stage
+compressor with uneven grids (11.5 M nodes in all) stepped concurrently
+(mh_step_batch) against the same points stepped one after another.
 """
 from __future__ import annotations
 
@@ -99,6 +101,69 @@ def time_to_converged() -> dict:
             "final": {"eavg": r["eavg"], "emax": r["emax"], "econt": r["econt"]}}
 
 
+def _speedline_state(i: int):
+    """Deck read and host set-up of speedline point i (in a worker process:
+    the Fortran host holds one deck at a time)."""
+    from multall_hip import deckgen, host
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, f"point{i}.dat")
+        with open(path, "w") as f:
+            f.write(deckgen.write_deck(deckgen.deck_speedline_point(i)))
+        return host.load_deck(path)
+
+
+def speedline_states(points: int = 16) -> list:
+    """The speedline's set-up states, read in worker processes before this
+    process touches the GPU."""
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(min(8, points)) as ex:
+        return list(ex.map(_speedline_state, range(points)))
+
+
+def speedline(states: list, steps: int = 100, warmup: int = 5) -> dict:
+    """The batched speedline: all points stepped concurrently by
+    mh_step_batch (one host thread and HIP stream per point) against the same
+    points, freshly set up, stepped one after another.  Both runs start from
+    the reference's initial guess after `warmup` steps; their results are
+    bit-identical."""
+    from multall_hip.runtime import Solver, step_batch
+    points = len(states)
+    nodes = [int(s["scalars"]["IM"] * s["scalars"]["JM"] * s["scalars"]["KM"]) for s in states]
+
+    def fresh():
+        sol = [Solver(s) for s in states]
+        for g in sol:
+            g.loss_init()
+            g.start()
+        return sol
+
+    seq = fresh()
+    for g in seq:
+        g.step(warmup)
+        g.sync()
+    t0 = time.perf_counter()
+    r_seq = [g.step(steps) for g in seq]
+    for g in seq:
+        g.sync()
+    t_seq = time.perf_counter() - t0
+    del seq
+    bat = fresh()
+    step_batch(bat, warmup)
+    for g in bat:
+        g.sync()
+    t0 = time.perf_counter()
+    r_bat = step_batch(bat, steps)
+    for g in bat:
+        g.sync()
+    t_bat = time.perf_counter() - t0
+    total = sum(nodes)
+    return {"workload": f"{points}-point speedline, single-stage axial compressor, exit p 1.2-1.6 x inlet p0, "
+                        f"mixing length, {total} nodes ({min(nodes)}-{max(nodes)} per point)",
+            "points": points, "steps": steps, "node_steps_per_s_batched": total * steps / t_bat,
+            "node_steps_per_s_sequential": total * steps / t_seq, "speedup": t_seq / t_bat,
+            "identical": r_seq == r_bat}
+
+
 def calibrate_copy() -> None:
     """One device-to-device copy of 256 MiB, three times, for scaling the
     FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes exactly
@@ -157,6 +222,7 @@ def main():
                     help="viscous model of the deck: 200 Spalart-Allmaras (the workload's), 10 mixing length, 0 inviscid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converged", action="store_true", help="skip the time-to-converged run")
+    ap.add_argument("--no-speedline", action="store_true", help="skip the batched speedline run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (for scaling the PMC byte counters)")
     args = ap.parse_args()
@@ -168,6 +234,7 @@ def main():
               "(rows and operating points would shard across GPUs)", file=sys.stderr)
         sys.exit(2)
 
+    sl_states = None if args.no_speedline else speedline_states()
     total = args.warmup + args.steps + 1000
     spec, deck = workload_deck(total, args.ilos)
     nodes = sum(r.jm for r in spec.rows) * spec.im * spec.km
@@ -230,6 +297,7 @@ def main():
                                 sorted(kern.items(), key=lambda kv: -kv[1][1])},
         "cpu_baseline": cpu,
         "time_to_converged": None if args.no_converged else time_to_converged(),
+        "speedline": None if args.no_speedline else speedline(sl_states),
     }
     print(json.dumps(out))
 
