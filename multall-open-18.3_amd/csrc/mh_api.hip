@@ -474,15 +474,35 @@ int build_lists(mh_ctx *c) {
 // damping line sums in fast mode), the row averages and second
 // mixing-plane averages, the fused damping + corner distribution, and
 // SMOOTH_VAR with D's boundary conditions.
-int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
-  const Geo &g = c->g;
+// The Dev a lane's kernels see: the context's arrays with the lane's own
+// flux and TSTEP scratch (lane 0 is the context's stream and scratch).
+Dev lane_dev(const mh_ctx *c, int lane) {
   Dev d = c->d;
+  if (lane == 0) return d;
+  const Lane &l = c->lanes[lane - 1];
+  d.XFLUX = l.XFLUX; d.TFLUX = l.TFLUX; d.RFLUX = l.RFLUX; d.SOURCE = l.SOURCE;
+  d.STORE = l.STORE; d.DSC = l.DSC; d.TMP = l.TMP;
+  d.B1CHG = l.B1CHG; d.B2CHG = l.B2CHG; d.SBCHG = l.SBCHG; d.AVG = l.AVG; d.RED = l.RED;
+  return d;
+}
+hipStream_t lane_stream(const mh_ctx *c, int lane) { return lane == 0 ? c->stream : c->lanes[lane - 1].stream; }
+
+// One TSTEP call (mf:5470-5932) on lane `lane`.
+int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
+  const Geo &g = c->g;
+  Dev d = lane_dev(c, lane);
+  hipStream_t st = lane_stream(c, lane);
   const Params p = params_of(c);
   const StepScalars s = step_scalars(c);
   const dim3 grid = node_grid(g);
   const dim3 lgrid((g.JM - 1 + 3) / 4, (g.KMM1 + 3) / 4), lblock(256);
   const int nmp = (g.NROWS - 1) * g.KMM1;
   const Lists &L = c->lists;
+  MgFast mgf = L.mgf;
+  if (lane) {
+    mgf.P = c->lanes[lane - 1].mgf.P;
+    mgf.PT = c->lanes[lane - 1].mgf.PT;
+  }
   const bool damp = !(s.DAMP < 2.0f || s.DAMP > 100);
   // fast mode without residual smoothing takes the damping line sums from
   // the multigrid update
@@ -491,57 +511,58 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   // fast mode: the residual kernel also leaves the multigrid partial sums
   const bool fastmg = mg && !p.EXACT && L.mgf_ok;
   const int kg = fastmg ? p.KR : 4;
-  LAUNCH(k_resid_line, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, c->stream, g, d, DIFF, p, L.mgf,
+  LAUNCH(k_resid_line, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, st, g, d, DIFF, p, mgf,
          kg, p.IR, fastmg ? 1 : 0);
   LCHK("resid_line");
   if (mg) {
     const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
     if (p.EXACT) {
-      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list, g.NIB1,
+      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, st, g, d, L.b1_ptr, L.b1_list, g.NIB1,
              g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
-      LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2,
+      LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, st, g, d, L.b2_ptr, L.b2_list, g.NIB2,
              g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
-      LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, c->stream, g, d, L.sb_ptr, L.sb_list,
+      LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, st, g, d, L.sb_ptr, L.sb_list,
              g.NSBLK, d.SBCHG, 0, 0);
     } else if (fastmg) {
-      LAUNCH(k_mg_levels, dim3((n1 + 255) / 256 + (n2 + 3) / 4 + g.NSBLK), dim3(256), 0, c->stream, g, L.mgf,
+      LAUNCH(k_mg_levels, dim3((n1 + 255) / 256 + (n2 + 3) / 4 + g.NSBLK), dim3(256), 0, st, g, mgf,
              L.b1_ptr, L.b1_list, L.sb_ptr, L.sb_list, d.B1CHG, d.B2CHG, d.SBCHG);
     } else {
-      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, c->stream, g, d, L.b1_ptr, L.b1_list, g.NIB1,
+      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, st, g, d, L.b1_ptr, L.b1_list, g.NIB1,
              g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
-      LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, c->stream, g, d, L.b2_ptr, L.b2_list, g.NIB2, g.NJB2 + 1,
+      LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, st, g, d, L.b2_ptr, L.b2_list, g.NIB2, g.NJB2 + 1,
              g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
-      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, c->stream, g, d, d.RED, 0);
-      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, c->stream, g, (const double *)d.RED, L.sb_ptr, L.sb_list,
+      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 0);
+      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, st, g, (const double *)d.RED, L.sb_ptr, L.sb_list,
              g.NSBLK, d.SBCHG, 0);
     }
     LCHK("mg_sums");
   }
-  if (fused_apply) LAUNCH(k_mg_apply_lines, lgrid, lblock, 0, c->stream, g, d, d.RED);
-  else LAUNCH(k_mg_apply, grid, kNodeBlock, 0, c->stream, g, d);
+  if (fused_apply) LAUNCH(k_mg_apply_lines, lgrid, lblock, 0, st, g, d, d.RED);
+  else LAUNCH(k_mg_apply, grid, kNodeBlock, 0, st, g, d);
   LCHK("mgapply");
   if (p.NRSMTH > 0) {
     for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0);
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.TMP, d.STORE, p.RSMTH, 1);
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 2);
-      std::swap(c->d.STORE, c->d.TMP);
-      d = c->d;
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.STORE, p.RSMTH, 1);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 2);
+      if (lane == 0) std::swap(c->d.STORE, c->d.TMP);
+      else std::swap(c->lanes[lane - 1].STORE, c->lanes[lane - 1].TMP);
+      d = lane_dev(c, lane);
     }
     LCHK("smooth_resid");
   }
   // row averages of |change| for the damping (mf:5677-5711)
   if (fused_apply) {
-    LAUNCH(k_row_avg, dim3(g.NROWS), dim3(1024), 0, c->stream, g, d, (const double *)d.RED, L.row_ptr, L.row_list);
+    LAUNCH(k_row_avg, dim3(g.NROWS), dim3(1024), 0, st, g, d, (const double *)d.RED, L.row_ptr, L.row_list);
     LCHK("row_avg");
   } else if (damp) {
     if (p.EXACT) {
-      LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, c->stream, g, d, L.row_ptr, L.row_list, g.NROWS,
-             d.SC + SC_AVG_CHG, 1, 1);
+      LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, st, g, d, L.row_ptr, L.row_list, g.NROWS,
+             d.AVG, 1, 1);
     } else {
-      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, c->stream, g, d, d.RED, 1);
-      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, c->stream, g, (const double *)d.RED, L.row_ptr, L.row_list,
-             g.NROWS, d.SC + SC_AVG_CHG, 1);
+      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 1);
+      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, st, g, (const double *)d.RED, L.row_ptr, L.row_list,
+             g.NROWS, d.AVG, 1);
     }
     LCHK("damp_avg");
   }
@@ -550,13 +571,13 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
     // add the (damped) change, carry the pitch average over each mixing
     // plane, smooth once with FAC_SFVIS*SFT, then the periodic/tip/
     // mixing-plane conditions with ILAST = IM-1
-    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, c->stream, g, d, D, s.DAMP, damp ? 1 : 0);
-    if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, c->stream, g, d, p, D);
+    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, st, g, d, D, s.DAMP, damp ? 1 : 0);
+    if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, st, g, d, p, D);
     const float sm = p.FAC_SFVIS * s.SFT;
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)D, d.TMP, sm, 0);
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.TMP, d.DSC, sm, 1);
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, c->stream, g, d, (const float *)d.DSC, D, sm, 2);
-    LAUNCH(k_sa_dbc, dim3(g.KM, g.JM), dim3(64), 0, c->stream, g, d, p, D);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)D, d.TMP, sm, 0);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, sm, 1);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, D, sm, 2);
+    LAUNCH(k_sa_dbc, dim3(g.KM, g.JM), dim3(64), 0, st, g, d, p, D);
     LCHK("sa_tstep");
     return MH_OK;
   }
@@ -565,10 +586,10 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
   {
     const dim3 sg((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + kScatTK - 1) / kScatTK);
     if (g.IMM1 <= 64)
-      LAUNCH(k_damp_scatter1, sg, dim3(256), scatter_lds_bytes(g.IMM1), c->stream, g, d, p, (const float *)D, d.DSC,
+      LAUNCH(k_damp_scatter1, sg, dim3(256), scatter_lds_bytes(g.IMM1), st, g, d, p, (const float *)D, d.DSC,
              s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
     else
-      LAUNCH(k_damp_scatter4, sg, dim3(256), scatter_lds_bytes(g.IMM1), c->stream, g, d, p, (const float *)D, d.DSC,
+      LAUNCH(k_damp_scatter4, sg, dim3(256), scatter_lds_bytes(g.IMM1), st, g, d, p, (const float *)D, d.DSC,
              s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
@@ -580,15 +601,15 @@ int tstep(mh_ctx *c, float *D, float *DIFF, int ncall) {
     const int lines = (g.KM + 7) / 8;
     const dim3 rg(jexit - 1);
     const float *src = d.DSC;
-    if (lines <= 1) LAUNCH(k_smooth_reg1, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
-    else if (lines <= 2) LAUNCH(k_smooth_reg2, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
-    else if (lines <= 4) LAUNCH(k_smooth_reg4, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
-    else if (lines <= 8) LAUNCH(k_smooth_reg8, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
-    else LAUNCH(k_smooth_reg16, rg, dim3(512), 0, c->stream, g, d, src, D, p, s, ncall, 1);
+    if (lines <= 1) LAUNCH(k_smooth_reg1, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
+    else if (lines <= 2) LAUNCH(k_smooth_reg2, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
+    else if (lines <= 4) LAUNCH(k_smooth_reg4, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
+    else if (lines <= 8) LAUNCH(k_smooth_reg8, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
+    else LAUNCH(k_smooth_reg16, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
     jreg = jexit;
   }
   if (jreg <= g.JM)
-    LAUNCH(k_smooth_plane, dim3(g.JM - jreg + 1), dim3(512), c->lds_plane, c->stream, g, d, (const float *)d.DSC, D, p,
+    LAUNCH(k_smooth_plane, dim3(g.JM - jreg + 1), dim3(512), c->lds_plane, st, g, d, (const float *)d.DSC, D, p,
            s, ncall, jreg);
   LCHK("smooth_plane");
   return MH_OK;
@@ -796,6 +817,33 @@ int viscous(mh_ctx *c, int ilos, float rf_vis, float fmixup, int nstep) {
   return MH_OK;
 }
 
+// The extra streams and scratch of the concurrent time steps (fast mode),
+// made on the first step that uses them.
+static int make_lanes(mh_ctx *c) {
+  const mh_dims &dm = c->dims;
+  const size_t N = (size_t)dm.im * dm.jm * dm.km;
+  int r;
+  for (int l = 0; l < kLanes; l++) {
+    Lane &L = c->lanes[l];
+    if (!L.stream) HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    if ((r = dalloc(c, &L.XFLUX, N)) || (r = dalloc(c, &L.TFLUX, N)) || (r = dalloc(c, &L.RFLUX, N)) ||
+        (r = dalloc(c, &L.SOURCE, N)) || (r = dalloc(c, &L.STORE, N)) || (r = dalloc(c, &L.DSC, N)) ||
+        (r = dalloc(c, &L.TMP, N)) || (r = dalloc(c, &L.B1CHG, (size_t)kind_len(dm, K_B1))) ||
+        (r = dalloc(c, &L.B2CHG, (size_t)kind_len(dm, K_B2))) || (r = dalloc(c, &L.SBCHG, (size_t)dm.nsblk)) ||
+        (r = dalloc(c, &L.AVG, (size_t)64)) ||
+        (r = dalloc(c, &L.RED, std::max<size_t>(4 * kRedBlocks, (size_t)dm.jm * dm.km))))
+      return r;
+    if (c->lists.mgf_ok &&
+        ((r = dalloc(c, &L.mgf.P, (size_t)dm.nib1 * dm.jm * dm.nkb1)) || (r = dalloc(c, &L.mgf.PT, (size_t)dm.jm * dm.nkb1))))
+      return r;
+  }
+  for (auto &e : c->ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nlanes = kLanes;
+  return MH_OK;
+}
+
 // One time step: the body of LOOP from label 5000 (mf:3753-5454).
 int step_once(mh_ctx *c) {
   const Geo &g = c->g;
@@ -831,7 +879,24 @@ int step_once(mh_ctx *c) {
   if (p.ILOS != 0 && c->nstep % p.NLOS == 0 && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
-  if ((r = tstep(c, d.RO, d.DRO, 1))) return r;
+  // With lanes, the energy equation (its fluxes read only the mass fluxes,
+  // HO and QSOURCE) runs on lane 1 beside continuity and the turbulence
+  // transport, and the three momentum equations (fluxes from the step's
+  // primitives, PEFF and the body forces) on lanes 0, 1 and 2.  Each lane
+  // has its own flux and TSTEP scratch; the boundary conditions and the
+  // limiters, which read several variables, wait for all of them.
+  if (!p.EXACT && c->lanes_on && c->nlanes == 0 && (r = make_lanes(c))) return r;
+  const bool lanes = c->nlanes == kLanes && c->lanes_on;
+  if (lanes) {
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(hipStreamWaitEvent(c->lanes[0].stream, c->ev[0], 0));
+    const Dev d1 = lane_dev(c, 1);
+    LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->lanes[0].stream, g, d1);
+    LCHK("flux_energy");
+    if ((r = tstep(c, 1, d.ROE, d.DROE, 3))) return r;
+    HIPCHK(hipEventRecord(c->ev[1], c->lanes[0].stream));
+  }
+  if ((r = tstep(c, 0, d.RO, d.DRO, 1))) return r;
   if (p.ILOS >= 100 && p.ILOS < 200) {
     // cell-average density for NEW_LOSS, mf:4260-4270
     LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
@@ -841,13 +906,17 @@ int step_once(mh_ctx *c) {
     // transport of the turbulent viscosity, mf:4260-4351
     LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
     LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
-    if ((r = tstep(c, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2))) return r;
+    if ((r = tstep(c, 0, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2))) return r;
     LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
     LCHK("sa_transport");
   }
-  LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
-  LCHK("flux_energy");
-  if ((r = tstep(c, d.ROE, d.DROE, 3))) return r;
+  if (lanes) {
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev[1], 0));
+  } else {
+    LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("flux_energy");
+    if ((r = tstep(c, 0, d.ROE, d.DROE, 3))) return r;
+  }
   LAUNCH(k_bc_lines, dim3(g.KM), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_bc_pd, dim3(1), dim3(128), 0, c->stream, g, d, p, s);
   LAUNCH(k_bc_apply, dim3((g.IM * g.KM + 255) / 256), dim3(256), 0, c->stream, g, d, p, s);
@@ -860,15 +929,33 @@ int step_once(mh_ctx *c) {
     LCHK("setflo");
   }
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
+  if (lanes) {
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    for (int l = 0; l < kLanes; l++) HIPCHK(hipStreamWaitEvent(c->lanes[l].stream, c->ev[2], 0));
+    const Dev d1 = lane_dev(c, 1), d2 = lane_dev(c, 2);
+    LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->lanes[0].stream, g, d1);
+    LCHK("flux_angular");
+    if ((r = tstep(c, 1, d.RORVT, d.DRORVT, 5))) return r;
+    HIPCHK(hipEventRecord(c->ev[3], c->lanes[0].stream));
+    LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->lanes[1].stream, g, d2);
+    LCHK("flux_radial");
+    if ((r = tstep(c, 2, d.ROVR, d.DROVR, 6))) return r;
+    HIPCHK(hipEventRecord(c->ev[4], c->lanes[1].stream));
+  }
   LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_axial");
-  if ((r = tstep(c, d.ROVX, d.DROVX, 4))) return r;
-  LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
-  LCHK("flux_angular");
-  if ((r = tstep(c, d.RORVT, d.DRORVT, 5))) return r;
-  LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
-  LCHK("flux_radial");
-  if ((r = tstep(c, d.ROVR, d.DROVR, 6))) return r;
+  if ((r = tstep(c, 0, d.ROVX, d.DROVX, 4))) return r;
+  if (lanes) {
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev[3], 0));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev[4], 0));
+  } else {
+    LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("flux_angular");
+    if ((r = tstep(c, 0, d.RORVT, d.DRORVT, 5))) return r;
+    LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("flux_radial");
+    if ((r = tstep(c, 0, d.ROVR, d.DROVR, 6))) return r;
+  }
   // limiters, mf:5115-5168
   d = c->d;
   const int nline = g.IM * g.KM;
@@ -1000,6 +1087,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) ||
       (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
     return bail(r);
+  d.AVG = d.SC + SC_AVG_CHG;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
   if (c->lds_plane > 160 * 1024) return bail(MH_E_UNSUPPORTED);
@@ -1020,6 +1108,13 @@ int mh_destroy(mh_ctx *c) {
     (void)hipEventDestroy(r.t1);
   }
   for (auto e : c->prof_pool) (void)hipEventDestroy(e);
+  for (int l = 0; l < kLanes; l++)
+    if (c->lanes[l].stream) {
+      (void)hipStreamSynchronize(c->lanes[l].stream);
+      (void)hipStreamDestroy(c->lanes[l].stream);
+    }
+  for (auto e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1185,10 +1280,17 @@ int mh_step_batch(mh_ctx *const *ctxs, int n, int nsteps, mh_resid *out, int *fa
     for (int k = 0; k < i; k++)
       if (ctxs[k] == ctxs[i]) return fail(ctxs[i], MH_E_ARG, "mh_step_batch: a context appears twice");
   }
+  // whole contexts overlap here: each keeps to its one stream (more streams
+  // than the GPU has hardware queues would serialise on the lanes' waits)
   std::vector<int> rc(n, MH_OK);
   std::vector<std::thread> th;
   th.reserve(n);
-  for (int i = 0; i < n; i++) th.emplace_back([&, i] { rc[i] = mh_step(ctxs[i], nsteps, &out[i]); });
+  for (int i = 0; i < n; i++)
+    th.emplace_back([&, i] {
+      ctxs[i]->lanes_on = n == 1;
+      rc[i] = mh_step(ctxs[i], nsteps, &out[i]);
+      ctxs[i]->lanes_on = true;
+    });
   for (auto &t : th) t.join();
   for (int i = 0; i < n; i++)
     if (rc[i] != MH_OK) {

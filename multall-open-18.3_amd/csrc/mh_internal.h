@@ -77,6 +77,7 @@ struct Dev {
   int *FLAGS;                        // per (I,K) line flags for the density limiter
   float *CHECK_FLOW, *BLADE_FLOW;    // per J
   float *SETFLO;                     // per J: SETFLO's FLOW(J), then its DELTA(J)
+  float *AVG;                        // per row: the damping's mean |change| of the current TSTEP call
   float *KWALL;                      // kKWall planes of (IM-1) x (JM-1) end-wall column terms
   double *INTEG;                     // 7*JM FP64 integrals
 };
@@ -165,6 +166,20 @@ struct mh_prof_rec {
   hipEvent_t t0, t1;
 };
 
+namespace mh {
+// A second or third HIP stream with its own flux and TSTEP scratch, so the
+// time steps of equations that do not read each other's results run
+// concurrently (continuity beside energy; the three momentum equations).
+struct Lane {
+  hipStream_t stream = nullptr;
+  float *XFLUX = nullptr, *TFLUX = nullptr, *RFLUX = nullptr, *SOURCE = nullptr, *STORE = nullptr, *DSC = nullptr,
+        *TMP = nullptr, *B1CHG = nullptr, *B2CHG = nullptr, *SBCHG = nullptr, *AVG = nullptr;
+  double *RED = nullptr;
+  MgFast mgf;
+};
+constexpr int kLanes = 2;
+}  // namespace mh
+
 struct mh_ctx {
   bool prof_on = false;
   std::vector<mh_prof_rec> prof_pending;      // launches not yet reduced
@@ -187,4 +202,9 @@ struct mh_ctx {
   float EAVG = 1e6f, EMAX = 1e6f, ECONT = 1.f;
   mh_resid last{};
   std::string err;
+  // concurrent time steps (fast mode): extra streams and their fork/join events
+  int nlanes = 0;
+  bool lanes_on = true;  // off while mh_step_batch overlaps whole contexts instead
+  mh::Lane lanes[mh::kLanes];
+  hipEvent_t ev[5] = {};
 };

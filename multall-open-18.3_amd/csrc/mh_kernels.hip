@@ -766,7 +766,7 @@ __global__ void k_damp_apply(Geo g, Dev d, float DAMP) {
   const int NR = d.NROW[J - 1];
   float DELTA = d.STORE[n];
   float ABSCHG = fabsf(DELTA);
-  float FDAMP = ABSCHG / avg_blk(d.SC + SC_AVG_CHG, NR, g.NROWS);
+  float FDAMP = ABSCHG / avg_blk(d.AVG, NR, g.NROWS);
   d.STORE[n] = DELTA / (1.f + FDAMP / DAMP);
 }
 
@@ -1120,7 +1120,7 @@ __global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *re
     for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if (lane == 0) {
       const float f = (float)x;
-      d.SC[SC_AVG_CHG + r] = f / (float)(g.IMM1 * g.KMM1 * nj);
+      d.AVG[r] = f / (float)(g.IMM1 * g.KMM1 * nj);
     }
   }
 }
@@ -1195,7 +1195,7 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   } else if (t >= 64 && t < 64 + kTJ + 1) {
     const int jc = j0 + t - 64;
     if (jc >= 2 && jc <= g.JM) {
-      avg[t - 64] = damp ? avg_blk(d.SC + SC_AVG_CHG, d.NROW[jc - 1], g.NROWS) : 1.0f;
+      avg[t - 64] = damp ? avg_blk(d.AVG, d.NROW[jc - 1], g.NROWS) : 1.0f;
       fdn[t - 64] = d.FACDWN[jc - 1];
       fup[t - 64] = d.FACUP[jc - 1];
     }
@@ -2267,7 +2267,7 @@ __global__ void k_sa_addstore(Geo g, Dev d, float *D, float DAMP, int damp) {
   CELL_IJK
   const unsigned n = IX(I, J, K);
   const float x = d.STORE[n];
-  D[n] = D[n] + (damp ? damped(g, d, d.SC + SC_AVG_CHG, x, J, DAMP) : x);
+  D[n] = D[n] + (damp ? damped(g, d, d.AVG, x, J, DAMP) : x);
 }
 
 // pitch average carried across each mixing plane, mf:5748-5762 (one
