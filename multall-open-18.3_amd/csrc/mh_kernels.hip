@@ -1204,7 +1204,12 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
 #pragma unroll
   for (int u = 0; u < kWL / 4; u++) {
     const int wl = w + 4 * u, arr = wl / (kTK + 1), kc = k0 - 1 + wl % (kTK + 1);
-    const float *src = arr == 0 ? d.FTL : arr == 1 ? d.FBL : arr == 2 ? d.FTR : d.FBR;
+    // wavefront-uniform selects (an indexed table of the four pointers
+    // would be built in scratch memory, in every lane)
+    const float *src = d.FTL;
+    if (arr == 1) src = d.FBL;
+    if (arr == 2) src = d.FTR;
+    if (arr == 3) src = d.FBR;
     if (kc >= 1 && kc <= g.KMM1)
 #pragma unroll
       for (int c = 0; c < NC; c++) {
