@@ -8,9 +8,10 @@
 !> :7801-7836, STEPUP :7916-8002) in the reference's FP32 statement order,
 !> so the arrays handed to the GPU equal the reference's own COMMON state
 !> after SETUP bit-for-bit.  Supported deck options: ideal gas, ITIMST 3/-3/4,
-!> no cooling, bleed, shroud, restart, throughflow or Q3D, ISHIFT=0,
-!> INSURF=0, NEWGRID=0, IF_DESIGN/IF_STAGGER/IF_LEAN=0, IF_CUSP=0,
-!> IF_ANGLES=0, IN_FLOW=0, IF_REPEAT=0, no exit throttle.  Anything else is
+!> no cooling, bleed, shroud, restart, throughflow or Q3D, ISHIFT 0 to 4
+!> (the inter-row gap grid of GRID_UP/GRID_DOWN, cusps IF_CUSP 0 or 1),
+!> INSURF=0, NEWGRID=0, IF_DESIGN/IF_STAGGER/IF_LEAN=0, IF_REPEAT=0, no
+!> exit throttle.  Anything else is
 !> reported through mh_host_error and the load fails.
 !>
 !> Arrays are allocated to the logical extents (IM,JM,KM), column-major with
@@ -44,6 +45,8 @@ module mh_host
   integer, allocatable, save, public :: KS1(:), KS2(:), NSTAGE(:), IF_CUSP(:), IF_ANGLES(:)
   integer, allocatable, save, public :: JTRAN_I1(:), JTRAN_IM(:), JTRAN_K1(:), JTRAN_KM(:)
   real, allocatable, save, public :: FRACTIP1(:), FRACTIP2(:), FTHICK(:,:), CHORD(:)
+  integer, allocatable, save, public :: ICUSP(:), LCUSP(:), LCUSPUP(:)
+  real, allocatable, save, public :: BETAUP(:,:), BETADWN1(:,:), BETADWN2(:,:)
   real, allocatable, save, public :: XLLIM_I1(:), XLLIM_IM(:), XLLIM_K1(:), XLLIM_KM(:), XLLIM_DWN(:)
   real, allocatable, save, public :: XLLIM_UP(:), XLLIM_IN(:), XLLIM_LE(:), XLLIM_TE(:), XLLIM_DN(:)
   real, allocatable, save, public :: FSTURB(:), TURBVIS_DAMP(:), ROUGH_H(:), ROUGH_T(:), ROUGH_L(:), ROUGH_U(:)
@@ -147,7 +150,8 @@ contains
     integer, parameter :: JCAP = 20000, KCAP = 400
     integer :: ios, nr, k, j, i, n, j1, j2, jmrow, jlerow, jterow, nblades_in_row, new_grid, jrow
     integer :: jroths, jrothe, jrotts, jrotte, nsecs_row, insurf, if_design, if_stagger, if_lean, jmidrw
-    integer :: kin_mid, iout(20)
+    integer :: kin_mid, iout(20), n_angles, j_mid
+    real :: fracn_span(KCAP), angl_up(KCAP), angl_dwn1(KCAP), angl_dwn2(KCAP), qspan(KCAP), xdif, rdif
     integer, allocatable :: kout(:)
     real :: rpmrow, rpmhub, puprow, plerow, pterow, pdnrow, fac1, xshift, fac2, tshift, fac3, fac4, rshift
     real :: xd, rd, srange, slim, xspan, rspan, f_span, xarg, rarg, frrat, frmax, frev
@@ -279,8 +283,8 @@ contains
     ISHIFT = 2; NEXTRAP_LE = 10; NEXTRAP_TE = 10
     call skip(u)
     read (u, *, iostat=ios) ISHIFT, NEXTRAP_LE, NEXTRAP_TE
-    if (ISHIFT /= 0) then
-      bad = fail('only ISHIFT=0 decks are supported (the deck supplies the inter-row grid)'); return
+    if (ISHIFT < 0 .or. ISHIFT > 4) then
+      bad = fail('ISHIFT must be 0 to 4'); return
     end if
     allocate (NSTAGE(NROWS + 1))
     do n = 1, NROWS
@@ -301,6 +305,9 @@ contains
     allocate (KTIPS(NROWS + 1), KTIPE(NROWS + 1), KS1(NROWS + 1), KS2(NROWS + 1), IF_CUSP(NROWS))
     allocate (IF_ANGLES(NROWS), FRACTIP1(NROWS), FRACTIP2(NROWS), FTHICK(NROWS, KM), CHORD(NROWS))
     allocate (JTRAN_I1(NROWS), JTRAN_IM(NROWS), JTRAN_K1(NROWS), JTRAN_KM(NROWS))
+    allocate (ICUSP(NROWS), LCUSP(NROWS), LCUSPUP(NROWS))
+    allocate (BETAUP(NROWS, max(NSECS_IN, 2)), BETADWN1(NROWS, max(NSECS_IN, 2)), BETADWN2(NROWS, max(NSECS_IN, 2)))
+    ICUSP = 0; LCUSP = 0; LCUSPUP = 0; BETAUP = 0.0; BETADWN1 = 0.0; BETADWN2 = 0.0
     allocate (t_ind(JCAP), t_indle(JCAP), t_indte(JCAP), t_indmix(JCAP), t_indmid(JCAP), t_nblade(JCAP))
     allocate (t_nrow(JCAP), t_wrad(JCAP), t_whub(JCAP), t_wtip(JCAP), t_pguess(JCAP))
     allocate (XSURF(JCAP, max(NSECS_IN, 2)), RSURF(JCAP, max(NSECS_IN, 2)))
@@ -364,9 +371,11 @@ contains
       IF_CUSP(nr) = 0; IF_ANGLES(nr) = 0
       call skip(u)
       read (u, *, iostat=ios) IF_CUSP(nr), IF_ANGLES(nr)
-      if (IF_CUSP(nr) /= 0 .or. IF_ANGLES(nr) /= 0) then
-        bad = fail('IF_CUSP and IF_ANGLES are not supported'); return
+      if (IF_CUSP(nr) < 0 .or. IF_CUSP(nr) > 1 .or. IF_ANGLES(nr) < 0) then
+        bad = fail('IF_CUSP must be 0 or 1 (the IF_CUSP=2 trailing-edge body force is not supported)'); return
       end if
+      ! cusp shape, mf:980-984
+      if (IF_CUSP(nr) == 1) read (u, *) ICUSP(nr), LCUSP(nr), LCUSPUP(nr)
       j2 = j1 + jmrow - 1
       do k = 1, nsecs_row
         if_design = 0; if_stagger = 0; if_lean = 0
@@ -445,6 +454,30 @@ contains
           end do
         end if
       end do
+      ! upstream and downstream grid angles on the sections, mf:1342-1374
+      if (IF_ANGLES(nr) > 0) then
+        call skip(u)
+        read (u, *) n_angles
+        read (u, *) (fracn_span(n), n = 1, n_angles)
+        read (u, *) (angl_up(n), n = 1, n_angles)
+        read (u, *) (angl_dwn1(n), n = 1, n_angles)
+        read (u, *) (angl_dwn2(n), n = 1, n_angles)
+        qspan(1) = 0.0
+        j_mid = j1 + (jlerow + jterow)/2
+        do k = 2, NSECS_IN
+          xdif = XSURF(j_mid, k) - XSURF(j_mid, k - 1)
+          rdif = RSURF(j_mid, k) - RSURF(j_mid, k - 1)
+          qspan(k) = qspan(k - 1) + sqrt(xdif*xdif + rdif*rdif)
+        end do
+        do k = 1, NSECS_IN
+          qspan(k) = qspan(k)/qspan(NSECS_IN)
+        end do
+        do k = 1, NSECS_IN
+          call intp(n_angles, fracn_span, angl_up, qspan(k), BETAUP(nr, k))
+          call intp(n_angles, fracn_span, angl_dwn1, qspan(k), BETADWN1(nr, k))
+          call intp(n_angles, fracn_span, angl_dwn2, qspan(k), BETADWN2(nr, k))
+        end do
+      end if
       ! markers, rotation and pressure guess for the row, mf:1765-1840
       JLE(nr) = j1 + jlerow - 1
       JTE(nr) = j1 + jterow - 1
@@ -505,6 +538,10 @@ contains
     IND = t_ind(1:JM); INDLE = t_indle(1:JM); INDTE = t_indte(1:JM); INDMIX = t_indmix(1:JM)
     INDMID = t_indmid(1:JM); NBLADE = t_nblade(1:JM); NROW = t_nrow(1:JM); WRAD = t_wrad(1:JM)
     WHUB = t_whub(1:JM); WTIP = t_wtip(1:JM); PGUESS = t_pguess(1:JM)
+    if (ISHIFT /= 0) then
+      bad = grid_gaps()
+      if (bad) return
+    end if
 
     ! inlet and exit boundary data, mf:1838-1940
     call skip(u)
@@ -607,6 +644,359 @@ contains
       end do
     end if
   end function read_deck
+
+  ! ---------------------------------------------------------------------------
+  ! The grid between blade rows, NEW_READIN mf:1546-1837.  ISHIFT = 1 moves
+  ! each row axially so the hub X spacing runs on across the mixing plane;
+  ! ISHIFT = 2 to 4 re-grid every gap (upstream of the first row, between
+  ! rows, downstream of the last) on each section: the gap's meridional
+  ! curve (kept, straightened, or straightened away from hub and casing),
+  ! smoothed, then laid out by GRID_UP / GRID_DOWN.
+  logical function grid_gaps() result(bad)
+    real :: sm(JM), xint(JM), rint(JM), sdist(JM)
+    real :: xshift, diff1, diff2, dlast, xgap, rgap, gap, xd, rd, proj, dist, sfgap, smid, ste, sle
+    real :: anglup, angldwn1, angldwn2
+    integer :: j, k, ngap, jst, jend, jmid, nrw, nrwm1, npoints, ngaps, ns, nn
+    bad = .false.
+    if (ISHIFT == 1) then
+      xshift = 0.0
+      do j = 2, JM
+        if (INDMIX(j - 1) == 1) then
+          diff1 = XSURF(j - 1, 1) - XSURF(j - 2, 1)
+          diff2 = XSURF(j + 1, 1) - XSURF(j, 1)
+          xshift = XSURF(j - 1, 1) + 0.01*(diff1 + diff2) - XSURF(j, 1)
+        end if
+        do k = 1, NSECS_IN
+          XSURF(j, k) = XSURF(j, k) + xshift
+        end do
+      end do
+      return
+    end if
+    jmid = 0
+    do ngap = 1, NROWS + 1
+      if (ngap == 1) then
+        nrw = 1; jst = 1; jend = JLE(1)
+      end if
+      if (ngap > 1 .and. ngap /= NROWS + 1) then
+        nrw = ngap; jst = JTE(nrw - 1); jend = JLE(nrw); jmid = JMIX(nrw - 1)
+      end if
+      if (ngap == NROWS + 1) then
+        nrw = NROWS + 1; jst = JTE(NROWS); jend = JM
+      end if
+      nrwm1 = nrw - 1
+      do k = 1, NSECS_IN
+        sm(1) = 0.0
+        do j = 2, JM
+          xd = XSURF(j, k) - XSURF(j - 1, k)
+          rd = RSURF(j, k) - RSURF(j - 1, k)
+          sm(j) = sm(j - 1) + sqrt(xd*xd + rd*rd)
+        end do
+        ! the points of the gap's meridional curve
+        dlast = -1.0
+        npoints = 1
+        xgap = XSURF(jend, k) - XSURF(jst, k)
+        rgap = RSURF(jend, k) - RSURF(jst, k)
+        gap = sqrt(xgap*xgap + rgap*rgap)
+        do j = jst, jend
+          xd = XSURF(j, k) - XSURF(jst, k)
+          rd = RSURF(j, k) - RSURF(jst, k)
+          proj = (xd*xgap + rd*rgap)
+          if (ISHIFT >= 3) proj = proj/gap
+          dist = sqrt(xd*xd + rd*rd)
+          if (proj < 0.0) cycle
+          if (dist > (1.00001*gap)) then
+            xint(npoints) = XSURF(jend, k)
+            rint(npoints) = RSURF(jend, k)
+            npoints = npoints + 1
+            exit
+          end if
+          if (dist < dlast) cycle
+          dlast = dist
+          if (ISHIFT == 3 .or. (ISHIFT == 4 .and. k /= 1 .and. k /= NSECS_IN)) then
+            xint(npoints) = XSURF(jst, k) + proj*xgap/gap
+            rint(npoints) = RSURF(jst, k) + proj*rgap/gap
+          else
+            xint(npoints) = XSURF(j, k)
+            rint(npoints) = RSURF(j, k)
+          end if
+          npoints = npoints + 1
+        end do
+        ! ten smoothing passes along the curve, then its arc length
+        ngaps = npoints - 1
+        sfgap = 0.2
+        do ns = 1, 10
+          do nn = 2, ngaps - 1
+            xint(nn) = (1. - sfgap)*xint(nn) + sfgap*0.5*(xint(nn - 1) + xint(nn + 1))
+            rint(nn) = (1. - sfgap)*rint(nn) + sfgap*0.5*(rint(nn - 1) + rint(nn + 1))
+          end do
+        end do
+        sdist(1) = sm(jst)
+        do nn = 2, ngaps
+          xd = xint(nn) - xint(nn - 1)
+          rd = rint(nn) - rint(nn - 1)
+          sdist(nn) = sdist(nn - 1) + sqrt(xd*xd + rd*rd)
+        end do
+        smid = 0.5*(sdist(1) + sdist(ngaps))
+        ste = sdist(1)
+        sle = sdist(ngaps)
+        anglup = 0.0
+        if (nrw <= NROWS) anglup = BETAUP(nrw, k)
+        if (nrw == 1) then
+          angldwn1 = BETADWN1(1, k); angldwn2 = BETADWN2(1, k)
+        else
+          angldwn1 = BETADWN1(nrwm1, k); angldwn2 = BETADWN2(nrwm1, k)
+        end if
+        if (nrw == NROWS + 1) then
+          angldwn1 = BETADWN1(NROWS, k); angldwn2 = BETADWN2(NROWS, k)
+        end if
+        if (nrw == 1) then
+          bad = grid_up(k, jst, jend, ste, sle, ngaps, sdist, xint, rint, sm, NEXTRAP_LE, anglup, IF_ANGLES(1))
+        else if (nrw /= NROWS + 1) then
+          bad = grid_down(k, jst, jmid, ste, smid, ngaps, sdist, xint, rint, sm, NEXTRAP_TE, angldwn1, angldwn2, &
+                          IF_CUSP(nrwm1), ICUSP(nrwm1), LCUSP(nrwm1), LCUSPUP(nrwm1), IF_ANGLES(nrwm1))
+          if (bad) return
+          bad = grid_up(k, jmid + 1, jend, smid, sle, ngaps, sdist, xint, rint, sm, NEXTRAP_LE, anglup, &
+                        IF_ANGLES(nrw))
+        else
+          bad = grid_down(k, jst, jend, ste, sle, ngaps, sdist, xint, rint, sm, NEXTRAP_TE, angldwn1, angldwn2, &
+                          IF_CUSP(NROWS), ICUSP(NROWS), LCUSP(NROWS), LCUSPUP(NROWS), IF_ANGLES(NROWS))
+        end if
+        if (bad) return
+      end do
+    end do
+  end function grid_gaps
+
+  ! SOLVE, mf:9513-9543: the ratio RAT of a geometric progression of NPUP
+  ! spacings, the first DX1, that adds up to XUP.
+  subroutine solve(npup, xup, dx1, rat)
+    integer, intent(in) :: npup
+    real, intent(in) :: xup, dx1
+    real, intent(out) :: rat
+    real :: rup, rhs, rnew, diff
+    integer :: m
+    rup = xup/dx1
+    rat = 1.2
+    if (rup < npup) rat = 0.9
+    m = 1
+    do
+      rhs = (rat - 1.)*rup
+      if (rup >= npup) rnew = (rhs + 1.)**(1./npup)
+      if (rup < npup) rnew = (rat**npup - 1.)/rup + 1.
+      diff = abs(rnew/rat - 1.0)
+      rat = rnew
+      m = m + 1
+      if (diff < 0.00001) exit
+      if (m > 250) exit
+    end do
+  end subroutine solve
+
+  ! GRID_UP, mf:9433-9508: J1..J2 upstream of a leading edge at J2, spaced
+  ! in a geometric progression from the first blade spacing, placed on the
+  ! smoothed gap curve; the blade's camber line is run on upstream at the
+  ! slope of its first NEXTRAP points (or the input angle).
+  logical function grid_up(k, j1, j2, s1, s2, ngap, sdist, xint, rint, sm, nextrap, beta_up, ifangles) result(bad)
+    integer, intent(in) :: k, j1, j2, ngap, nextrap, ifangles
+    real, intent(in) :: s1, s2, sdist(:), xint(:), rint(:), sm(:), beta_up
+    real :: snew(JM), ansx(JM), ansr(JM), th_upp(JM), th_thick(JM)
+    real :: sdiff, r1, rat, xd, slopet, rtheta_le
+    integer :: j, jsub, nint
+    bad = .false.
+    if (j2 + nextrap > JM .or. j2 - j1 < 1) then
+      bad = fail('GRID_UP: the row has fewer than NEXTRAP_LE points after its leading edge'); return
+    end if
+    snew(j1) = s1
+    snew(j2) = s2
+    sdiff = s2 - s1
+    nint = j2 - j1
+    r1 = sm(j2 + 1) - sm(j2)
+    call solve(nint, sdiff, r1, rat)
+    xd = r1
+    do j = j1 + 1, j2 - 1
+      jsub = j2 + j1 - j
+      snew(jsub) = snew(jsub + 1) - xd
+      xd = xd*rat
+    end do
+    if (j1 /= 1) snew(j1) = snew(j1) + 0.01*(snew(j1 + 1) - snew(j1))
+    do j = j1, j2
+      call intp(ngap, sdist, xint, snew(j), ansx(j))
+      call intp(ngap, sdist, rint, snew(j), ansr(j))
+    end do
+    do j = j1, j2
+      XSURF(j, k) = ansx(j)
+      RSURF(j, k) = ansr(j)
+    end do
+    do j = j1, j2 + nextrap
+      th_upp(j) = RT_UPP(j, k)/RSURF(j, k)
+      th_thick(j) = RT_THICK(j, k)/RSURF(j, k)
+    end do
+    if (ifangles == 0) then
+      slopet = (th_upp(j2) - 0.5*th_thick(j2) - th_upp(j2 + nextrap) &
+                + 0.5*th_thick(j2 + nextrap))/(sm(j2) - sm(j2 + nextrap))
+    else
+      slopet = tan(beta_up*DEGRAD)/RSURF(j2, k)
+    end if
+    rtheta_le = RT_UPP(j2, k) - 0.5*RT_THICK(j2, k)
+    do j = j1, j2 - 1
+      RT_UPP(j, k) = rtheta_le + RSURF(j, k)*slopet*(snew(j) - snew(j2))
+    end do
+  end function grid_up
+
+  ! GRID_DOWN, mf:9201-9431: J1..J2 downstream of a trailing edge at J1, a
+  ! geometric progression from the last blade spacing on the gap curve; the
+  ! blade's own cusp is kept and the zero-thickness wake line runs on at the
+  ! camber-line slope of the last NEXTRAP points (IFCUSP = 0), or a new cusp
+  ! of LCUSP points replaces the trailing edge LCUSPUP points upstream of it
+  ! (IFCUSP = 1; ICUSP = 0 centred, 1 or -1 keeping the I = 1 or I = IM
+  ! surface straight).
+  logical function grid_down(k, j1, j2, s1, s2, ngap, sdist, xint, rint, sm, nextrap, beta_dwn1, beta_dwn2, &
+                             ifcusp, icusp_r, lcusp_r, lcuspup_r, ifangles) result(bad)
+    integer, intent(in) :: k, j1, j2, ngap, nextrap, ifcusp, icusp_r, lcusp_r, lcuspup_r, ifangles
+    real, intent(in) :: s1, s2, sdist(:), xint(:), rint(:), sm(:), beta_dwn1, beta_dwn2
+    integer, parameter :: LUP = 20
+    real :: snew(0:JM), ansx(JM), ansr(JM), th_upp(JM), th_thick(JM), th_mid(JM)
+    real :: sdiff, r1, rat, xd, rd, dsmer, slopes, slopetk, slopep, slopecent, betadown1, betadown2
+    real :: fdown, betadown, dsdist, rthik, tkgrad, ygrad, ythickte, ymidte, ymid, clength, plength, ytnew
+    integer :: j, js, jstrt, jcs, nint, jcuspend
+    bad = .false.
+    if (j2 - j1 < 1 .or. j1 - nextrap < 1 .or. &
+        (ifcusp /= 0 .and. (j1 - LUP - 1 < 1 .or. lcuspup_r > LUP - 2 .or. lcuspup_r < 0 .or. lcusp_r < 1 &
+                            .or. j1 - lcuspup_r + lcusp_r > j2))) then
+      bad = fail('GRID_DOWN: the row or its gap is too short for NEXTRAP_TE or the cusp'); return
+    end if
+    snew(j1) = s1
+    snew(j2) = s2
+    sdiff = s2 - s1
+    nint = j2 - j1
+    r1 = sm(j1) - sm(j1 - 1)
+    call solve(nint, sdiff, r1, rat)
+    xd = r1
+    do j = j1 + 1, j2 - 1
+      snew(j) = snew(j - 1) + xd
+      xd = xd*rat
+    end do
+    do j = j1, j2
+      call intp(ngap, sdist, xint, snew(j), ansx(j))
+      call intp(ngap, sdist, rint, snew(j), ansr(j))
+    end do
+    do j = j1, j2
+      XSURF(j, k) = ansx(j)
+      RSURF(j, k) = ansr(j)
+    end do
+    if (ifcusp == 0) then
+      ! the arc length restarts from zero one point before J1-NEXTRAP
+      jstrt = j1 - nextrap
+      snew(jstrt - 1) = 0.0
+      do j = jstrt, j2
+        xd = XSURF(j, k) - XSURF(j - 1, k)
+        rd = RSURF(j, k) - RSURF(j - 1, k)
+        snew(j) = snew(j - 1) + sqrt(xd*xd + rd*rd)
+        th_upp(j) = RT_UPP(j, k)/RSURF(j, k)
+        th_thick(j) = RT_THICK(j, k)/RSURF(j, k)
+        th_mid(j) = th_upp(j) - 0.5*th_thick(j)
+      end do
+      if (ifangles == 0) then
+        dsmer = snew(j1) - snew(jstrt)
+        slopes = (th_upp(j1) - th_upp(jstrt))/dsmer
+        slopetk = (th_thick(j1) - th_thick(jstrt))/dsmer
+        slopep = slopes - slopetk
+        slopecent = 0.5*(slopes + slopep)
+        betadown1 = slopecent
+        betadown2 = betadown1
+      else
+        betadown1 = tan(beta_dwn1*DEGRAD)/RSURF(j1, k)
+        betadown2 = tan(beta_dwn2*DEGRAD)/RSURF(j2, k)
+      end if
+      ! the end of the deck's own cusp: the first point of zero thickness
+      js = j1
+      do j = j1, j2
+        if (RT_THICK(j, k) < 1.0E-4*sdiff) then
+          js = j
+          exit
+        end if
+      end do
+      do j = js + 1, j2
+        fdown = (snew(j) - snew(j1))/(snew(j2) - snew(j1))
+        betadown = betadown1 + fdown*(betadown2 - betadown1)
+        dsdist = snew(j) - snew(j - 1)
+        RT_UPP(j, k) = RT_UPP(j - 1, k) + RSURF(j, k)*betadown*dsdist
+        RT_THICK(j, k) = 0.0
+      end do
+      return
+    end if
+    ! a new cusp
+    jstrt = j1 - LUP
+    snew(jstrt - 1) = 0.0
+    do j = jstrt, j2
+      xd = XSURF(j, k) - XSURF(j - 1, k)
+      rd = RSURF(j, k) - RSURF(j - 1, k)
+      snew(j) = snew(j - 1) + sqrt(xd*xd + rd*rd)
+      th_upp(j) = RT_UPP(j, k)/RSURF(j, k)
+      th_thick(j) = RT_THICK(j, k)/RSURF(j, k)
+      th_mid(j) = th_upp(j) - 0.5*th_thick(j)
+    end do
+    js = jstrt + 1
+    if (js > j1) js = j1
+    ! the start of the trailing-edge thinning
+    jcs = j1
+    do j = js, j1
+      rthik = th_thick(j)/th_thick(j - 1)
+      if (rthik < 0.9) then
+        jcs = j - 1
+        exit
+      end if
+    end do
+    if (jcs - 2 < jstrt) then
+      bad = fail('GRID_DOWN: the trailing-edge thinning starts too close to J1-20'); return
+    end if
+    ! the blade run on straight (camber line and thickness) from JCS to J1
+    tkgrad = (th_thick(jcs) - th_thick(jcs - 2))/(snew(jcs) - snew(jcs - 2))
+    ygrad = (th_mid(jcs) - th_mid(jcs - 2))/(snew(jcs) - snew(jcs - 2))
+    do j = jcs, j1
+      th_mid(j) = th_mid(jcs) + ygrad*(snew(j) - snew(jcs))
+      th_thick(j) = th_thick(jcs) + tkgrad*(snew(j) - snew(jcs))
+      if (th_thick(j) < 0.0) th_thick(j) = 0.0
+      th_upp(j) = th_mid(j) + 0.5*th_thick(j)
+      RT_UPP(j, k) = RSURF(j, k)*th_upp(j)
+      RT_THICK(j, k) = RSURF(j, k)*th_thick(j)
+    end do
+    ! slopes at the new trailing edge, LCUSPUP points before J1
+    jstrt = j1 - lcuspup_r
+    dsmer = snew(jstrt) - snew(jstrt - 2)
+    slopes = (th_upp(jstrt) - th_upp(jstrt - 2))/dsmer
+    slopetk = (th_thick(jstrt) - th_thick(jstrt - 2))/dsmer
+    slopep = slopes - slopetk
+    slopecent = 0.5*(slopes + slopep)
+    if (ifangles == 0) then
+      betadown1 = slopecent
+      betadown2 = slopecent
+    else
+      betadown1 = tan(beta_dwn1*DEGRAD)/RSURF(jstrt, k)
+      betadown2 = tan(beta_dwn2*DEGRAD)/RSURF(jstrt, k)
+    end if
+    ythickte = RT_THICK(jstrt, k)
+    if (ythickte < 0.0) ythickte = 0.0
+    ymidte = RT_UPP(jstrt, k) - 0.5*ythickte
+    do j = jstrt + 1, j2
+      fdown = (snew(j) - snew(jstrt))/(snew(j2) - snew(jstrt))
+      betadown = betadown1 + fdown*(betadown2 - betadown1)
+      dsdist = snew(j) - snew(j - 1)
+      ymid = ymidte + RSURF(j, k)*slopecent*(snew(j) - snew(jstrt))
+      clength = snew(jstrt + lcusp_r) - snew(jstrt)
+      plength = snew(j) - snew(jstrt)
+      if (plength > clength) plength = clength
+      jcuspend = jstrt + lcusp_r
+      if ((jcuspend - j) >= 0) then
+        ytnew = ythickte*(1. - plength/clength)
+        RT_THICK(j, k) = ytnew
+        if (icusp_r == 0) RT_UPP(j, k) = ymid + 0.5*ytnew
+        if (icusp_r == 1) RT_UPP(j, k) = RT_UPP(j - 1, k) + RSURF(j, k)*slopes*dsdist
+        if (icusp_r == -1) RT_UPP(j, k) = RT_UPP(j - 1, k) - RT_THICK(j - 1, k) + RSURF(j, k)*slopep*dsdist + ytnew
+      else
+        RT_UPP(j, k) = RT_UPP(j - 1, k) + RSURF(j, k)*betadown*dsdist
+        RT_THICK(j, k) = 0.0
+      end if
+    end do
+  end function grid_down
 
   ! INPINT, mf:9073-9195: inlet conditions from the KIN input points onto KM
   ! stream surfaces.
@@ -1537,7 +1927,8 @@ contains
     if (allocated(FP)) deallocate (FP, FR, FU, FD, FKU, FKD, FSPAN)
     if (allocated(NSTAGE)) deallocate (NSTAGE)
     if (allocated(JSTART)) deallocate (JSTART, JMIX, JLE, JTE, JLED, KTIPS, KTIPE, KS1, KS2, IF_CUSP, IF_ANGLES, &
-                                       FRACTIP1, FRACTIP2, FTHICK, CHORD, JTRAN_I1, JTRAN_IM, JTRAN_K1, JTRAN_KM)
+                                       FRACTIP1, FRACTIP2, FTHICK, CHORD, JTRAN_I1, JTRAN_IM, JTRAN_K1, JTRAN_KM, &
+                                       ICUSP, LCUSP, LCUSPUP, BETAUP, BETADWN1, BETADWN2)
     if (allocated(XSURF)) deallocate (XSURF, RSURF)
     if (allocated(RT_UPP)) deallocate (RT_UPP, RT_THICK)
     if (allocated(IND)) deallocate (IND, INDLE, INDTE, INDMIX, INDMID, NBLADE, NROW, WRAD, WHUB, WTIP, PGUESS)

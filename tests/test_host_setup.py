@@ -6,6 +6,7 @@ SETUP for the deck tests/golden/decks/<case>.dat.  The reference's SETUP also
 makes the first LOSS call (forces, y+, VISLAM); the host leaves that to the
 library (mh_loss_init), so those arrays are compared on the GPU instead.
 """
+import json
 import os
 
 import numpy as np
@@ -61,3 +62,31 @@ def test_reload_keeps_no_state_of_the_previous_deck():
     for name, a in fresh.items():
         if name != "scalars":
             assert np.array_equal(a, again[name]), name
+
+
+REF_DIGESTS = os.path.join(GOLDEN, "refdeck_setup_digests.json")
+# the reference's own test decks that fit its FP32 build: one- to three-row
+# axial and radial machines re-gridded between rows (ISHIFT = 2) with and
+# without new trailing-edge cusps (IF_CUSP = 1: the radial inflow turbine and
+# Harrison's cascade), a KM = 5 inviscid shock case and Rotor 37 (ISHIFT = 0)
+REF_DECKS = sorted(json.load(open(REF_DIGESTS))) if os.path.exists(REF_DIGESTS) else []
+
+
+@pytest.mark.parametrize("deck", REF_DECKS)
+def test_host_setup_of_reference_decks(deck):
+    """The host's set-up of the reference's own test decks equals the
+    reference's post-SETUP COMMON state bit for bit, array by array (SHA-256
+    of the FP32/INT32 values; golden digests written by
+    tests/golden/make_refdeck_digests.py from the reference itself)."""
+    import hashlib
+    from multall_hip import host
+    want = json.load(open(REF_DIGESTS))[deck]
+    st = host.load_deck(os.path.join(GOLDEN, "decks", "ref", deck))
+    for name, w in want["arrays"].items():
+        a = np.ascontiguousarray(st[name])
+        assert list(a.shape) == w["shape"] and str(a.dtype) == w["dtype"], name
+        got = hashlib.sha256(a.tobytes()).hexdigest()
+        assert got == w["sha256"], f"{deck}: {name} differs (sum {a.astype(np.float64).sum()!r} vs {w['sum']!r})"
+    for k, v in want["scalars"].items():
+        assert np.float32(st["scalars"][k]) == np.float32(v), (deck, k, st["scalars"][k], v)
+    assert len(want["arrays"]) >= 100
