@@ -883,18 +883,19 @@ contains
       RSURF(j, k) = ansr(j)
     end do
     if (ifcusp == 0) then
-      ! the arc length restarts from zero one point before J1-NEXTRAP
-      jstrt = j1 - nextrap
-      snew(jstrt - 1) = 0.0
-      do j = jstrt, j2
-        xd = XSURF(j, k) - XSURF(j - 1, k)
-        rd = RSURF(j, k) - RSURF(j - 1, k)
-        snew(j) = snew(j - 1) + sqrt(xd*xd + rd*rd)
-        th_upp(j) = RT_UPP(j, k)/RSURF(j, k)
-        th_thick(j) = RT_THICK(j, k)/RSURF(j, k)
-        th_mid(j) = th_upp(j) - 0.5*th_thick(j)
-      end do
       if (ifangles == 0) then
+        ! the arc length restarts from zero one point before J1-NEXTRAP;
+        ! with input angles the progression's SNEW stays
+        jstrt = j1 - nextrap
+        snew(jstrt - 1) = 0.0
+        do j = jstrt, j2
+          xd = XSURF(j, k) - XSURF(j - 1, k)
+          rd = RSURF(j, k) - RSURF(j - 1, k)
+          snew(j) = snew(j - 1) + sqrt(xd*xd + rd*rd)
+          th_upp(j) = RT_UPP(j, k)/RSURF(j, k)
+          th_thick(j) = RT_THICK(j, k)/RSURF(j, k)
+          th_mid(j) = th_upp(j) - 0.5*th_thick(j)
+        end do
         dsmer = snew(j1) - snew(jstrt)
         slopes = (th_upp(j1) - th_upp(jstrt))/dsmer
         slopetk = (th_thick(j1) - th_thick(jstrt))/dsmer

@@ -44,6 +44,9 @@ class RowSpec:
     fractip: float = 0.0           # tip gap as a fraction of span
     jtran: tuple = (0, 0, 0, 0)    # transition J from the row start: I=1, I=IM, hub, casing surfaces
     pguess: tuple = (1.0e5, 1.0e5, 0.8e5, 0.8e5)   # PUP, PLE, PTE, PDN
+    cusp: Optional[tuple] = None   # (ICUSP, LCUSP, LCUSPUP): IF_CUSP = 1, a new trailing-edge cusp
+    # (span fractions, upstream, downstream-start, downstream-end grid angles in degrees): IF_ANGLES = 1
+    angles: Optional[tuple] = None
 
 
 @dataclass
@@ -78,6 +81,8 @@ class DeckSpec:
     in_vtan: int = 0
     in_vr: int = 1
     rfin: float = 0.1
+    ishift: int = 0                # 2: the reader re-grids the gaps between rows (GRID_UP/GRID_DOWN)
+    nextrap: tuple = (10, 10)      # NEXTRAP_LE, NEXTRAP_TE
     in_flow: int = 0               # mass-flow forcing: 3 towards FLOWIN, 2 towards the mean over J
     flowin: float = 0.0            # kg/s through the whole annulus
     rflow: float = 0.0
@@ -252,7 +257,7 @@ def write_deck(spec: DeckSpec) -> str:
         a("   YPLAM      YPTURB")
         a(f"{spec.yplam:12.6f}{spec.ypturb:12.6f}")
     a("      ISHIFT    NEXTRAP_LE  NEXTRAP_TE")
-    a("         0        10        10")
+    a(f"{spec.ishift:10d}{spec.nextrap[0]:10d}{spec.nextrap[1]:10d}")
     a("  NSTG(N),N=1,NROWS")
     a("".join(f"{1 + n // 2:10d}" for n in range(len(spec.rows))))
     a("  5  TIME STEPS FOR PRINTOUT")
@@ -298,7 +303,9 @@ def write_deck(spec: DeckSpec) -> str:
         a("      NSECS_ROW   INSURF")
         a(f"{spec.nsecs:10d}         0")
         a("  IF_CUSP   IFANGLES")
-        a("         0         0")
+        a(f"{0 if row.cusp is None else 1:10d}{0 if row.angles is None else 1:10d}")
+        if row.cusp is not None:
+            a("".join(f"{v:10d}" for v in row.cusp))
         for ks in range(spec.nsecs):
             fr = ks / (spec.nsecs - 1)
             r = spec.r_hub + fr * span
@@ -313,6 +320,11 @@ def write_deck(spec: DeckSpec) -> str:
             a(_fmt(rtt))
             a("   1.00000   0.00000")
             a(_fmt(np.full(row.jm, r)))
+        if row.angles is not None:
+            a("  GRID ANGLES: N, SPAN FRACTIONS, UPSTREAM, DOWNSTREAM START AND END")
+            a(f"{len(row.angles[0]):10d}")
+            for vals in row.angles:
+                a("".join(f"{v:10.4f}" for v in vals))
     kin = 2
     a("  STARTING INLET BOUNDARY CONDITION DATA .")
     a("  NUMBER OF POINTS FOR INLET BOUNDARY CONDITIONS")
@@ -365,6 +377,23 @@ def deck_tiny(ilos: int = 0, nrows: int = 1, nsteps: int = 100) -> DeckSpec:
     return DeckSpec(title=f"TINY CASCADE ILOS={ilos} ROWS={nrows}", im=9, km=9, rows=rows,
                     ilos=ilos, nsteps=nsteps, conlim=1e-9, nchange=50, nmixup=50,
                     pdown_hub=0.8e5, pdown_tip=0.8e5, seed=1)
+
+
+def deck_tiny_gap(nsteps: int = 100) -> DeckSpec:
+    """The three-row mixing-length cascade with the gaps re-gridded by the
+    reader (ISHIFT = 2, NEXTRAP 6 and 8): a new centred cusp on the first
+    stator, input grid angles varying over the span on the rotor, and a cusp
+    that keeps the I = IM surface straight with input angles on the last
+    stator."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE REGRIDDED GAPS"
+    spec.ishift = 2
+    spec.nextrap = (6, 8)
+    spec.rows[0].cusp = (0, 6, 2)
+    spec.rows[1].angles = ((0.0, 0.5, 1.0), (-15.0, -18.0, -22.0), (-55.0, -58.0, -62.0), (-50.0, -52.0, -56.0))
+    spec.rows[2].cusp = (-1, 4, 1)
+    spec.rows[2].angles = ((0.0, 1.0), (2.0, 4.0), (60.0, 63.0), (55.0, 58.0))
+    return spec
 
 
 def deck_tiny_ml_variant(nsteps: int = 100) -> DeckSpec:

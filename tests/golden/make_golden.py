@@ -49,6 +49,7 @@ CASES = {
     "tiny3_ml3": lambda n: deckgen.deck_tiny_wallfun(ilos=10, nsteps=n),
     "tiny3_nl3": lambda n: deckgen.deck_tiny_wallfun(ilos=100, nsteps=n),
     "tiny3_sa3": lambda n: deckgen.deck_tiny_wallfun(ilos=200, nsteps=n),
+    "tiny3_gap": lambda n: deckgen.deck_tiny_gap(nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}

@@ -35,7 +35,7 @@ INVISCID = ["tiny1_inv", "tiny3_inv"]
 # (IN_FLOW = 2) and towards a fixed flow (IN_FLOW = 3), and the Shih wall
 # functions without the pressure-gradient term (YPLUSWALL = -6)
 MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide", "tiny3_sss", "tiny3_flow2",
-                 "tiny3_flow3", "tiny3_ml3"]
+                 "tiny3_flow3", "tiny3_ml3", "tiny3_gap"]
 # Spalart-Allmaras decks: rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing, mixing-length relaxation source; the Rotor 37 test
 # deck's options (Shih wall functions with the pressure-gradient term,

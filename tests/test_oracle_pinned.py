@@ -17,12 +17,14 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 # power-law laminar viscosity, smooth and rough walls, a rotor tip gap, the
 # fixed wall y+, a shear-free hub, exit smoothing, IM = 72, the SSS scheme
 # with residual smoothing (ITIMST = -3) and mass-flow forcing (IN_FLOW = 2
-# and 3); NEW_LOSS decks
+# and 3), a deck whose inter-row gaps the reader re-grids (ISHIFT = 2, new
+# trailing-edge cusps, input grid angles); NEW_LOSS decks
 # (tip gap; power-law viscosity, rough walls, shear-free hub and transition
 # points with a threshold that leaves part of the flow laminar); and the
 # Spalart-Allmaras decks below
 CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide",
-         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3", "tiny3_ml3", "tiny3_nl3", "tiny3_sa3"]
+         "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3", "tiny3_ml3", "tiny3_nl3", "tiny3_sa3",
+         "tiny3_gap"]
 # Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing and the mixing-length relaxation source) also pin the
 # turbulence state
