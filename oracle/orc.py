@@ -69,6 +69,22 @@ def with_wall_defaults(state: Dict) -> Dict:
     return out
 
 
+def with_workspace(state: Dict) -> Dict:
+    """A state set up by the Fortran host (which hands over only what the
+    library takes) completed with zeros, as in a fresh COMMON block, for the
+    oracle's work arrays and the per-row LOSS outputs its viscous calls
+    recompute before reading."""
+    sc = state["scalars"]
+    im, jm, km, nr = int(sc["IM"]), int(sc["JM"]), int(sc["KM"]), int(sc["NROWS"])
+    out = with_wall_defaults(state)
+    for names, shape in ((NODE3, (km, jm, im)), (VEC_K, (km,)), (VEC_NR, (nr,)),
+                         (NRTAB, (max(im, km, 65), 21))):
+        for n in names:
+            if n not in out:
+                out[n] = np.zeros(shape, np.float32)
+    return out
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
