@@ -924,11 +924,15 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
       if (I > IMM1) continue;
       // all loads of the lines before any store (the arrays may alias as
       // far as the compiler knows, so a store would hold back later loads)
-      float rf[5], xf[4], xm[4], tf[4], tp[4], sr[4], df[4];
+      // (only the wavefront's own nk lines: with kg = 3 the fourth belongs
+      // to the next wavefront)
+      float rf[5] = {}, xf[4] = {}, xm[4] = {}, tf[4] = {}, tp[4] = {}, sr[4] = {}, df[4] = {};
 #pragma unroll
-      for (int kk = 0; kk <= 4; kk++) rf[kk] = d.RFLUX[IX(I, J, min(kq + kk, g.KM))];
+      for (int kk = 0; kk <= 4; kk++)
+        if (kk <= nk) rf[kk] = d.RFLUX[IX(I, J, min(kq + kk, g.KM))];
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) {
+        if (kk >= nk) break;
         const unsigned n = IX(I, J, min(kq + kk, g.KMM1));
         xf[kk] = d.XFLUX[n];
         xm[kk] = d.XFLUX[n - IM];
