@@ -181,6 +181,20 @@ double mh_get_scalar(const mh_ctx *ctx, int id);
  * or after mh_start. */
 int mh_loss_init(mh_ctx *ctx);
 
+/* SET_XLENGTH (multall-open-18.3.f:12194-12517) on the device: the wall
+ * distances and mixing-length scales DWALLSQ, XLENGTH and FST_RAT from the
+ * grid already set (X, R, SMERID, ASX, ASR, ABX, ABR, ABT, the row tables,
+ * FSTURB and every scalar), instead of passing them in.  theta: the nodal
+ * THETA(I,J,K) of the reference's COMMON, dense (IM,JM,KM), I fastest;
+ * xllim_in/le/te/dn: the per-row mixing-length limits XLLIM_IN, XLLIM_LE,
+ * XLLIM_TE, XLLIM_DN, NROWS
+ * values each (XLLIM_* read at mf:2072).  The reference calls SET_XLENGTH from
+ * SETUP (mf:7340); the replacement is one call after the uploads and before
+ * mh_loss_init.  The values equal the reference's within float rounding (the
+ * device's sinf and powf). */
+int mh_set_xlength(mh_ctx *ctx, const float *theta, const float *xllim_in, const float *xllim_le,
+                   const float *xllim_te, const float *xllim_dn);
+
 /* LOOP prologue (mf:3674-3714).  Call once after all inputs are set. */
 int mh_start(mh_ctx *ctx);
 /* Run up to nsteps time steps (fewer if the run ends).  *out (optional)

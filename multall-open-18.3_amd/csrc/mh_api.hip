@@ -74,6 +74,9 @@ __global__ void k_resid_line(Geo, Dev, float *, Params, MgFast, int, int, int);
 __global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, float *, float *,
                             float *);
 __global__ void k_mg_apply_lines(Geo, Dev, double *);
+__global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
+__global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
+__global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
 __global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
 __global__ void k_damp_scatter1(Geo, Dev, Params, const float *, float *, float, int, int);
 __global__ void k_damp_scatter4(Geo, Dev, Params, const float *, float *, float, int, int);
@@ -1216,6 +1219,49 @@ int mh_loss_init(mh_ctx *c) {
   c->scal[MH_S_VISLAM] = vf[0];
   c->scal[MH_S_FTCOND] = vf[1];
   c->scal_set[MH_S_VISLAM] = c->scal_set[MH_S_FTCOND] = true;
+  return MH_OK;
+}
+
+// SET_XLENGTH on the device.  The work buffers (the nodal wall distance, the
+// per-row limits, the per-J limit) live for this call only.
+int mh_set_xlength(mh_ctx *c, const float *theta, const float *xllim_in, const float *xllim_le,
+                   const float *xllim_te, const float *xllim_dn) {
+  if (!c) return MH_E_ARG;
+  if (!theta || !xllim_in || !xllim_le || !xllim_te || !xllim_dn) return fail(c, MH_E_ARG, "null argument");
+  int r = check_inputs(c);
+  if (r) return r;
+  const Params p = params_of(c);
+  const Geo g = c->g;
+  const size_t n = (size_t)g.IM * g.JM * g.KM;
+  const int nr = g.NROWS;
+  struct Bufs {
+    float *th = nullptr, *dist = nullptr, *xl4 = nullptr, *xlim = nullptr;
+    ~Bufs() {
+      (void)hipFree(th);
+      (void)hipFree(dist);
+      (void)hipFree(xl4);
+      (void)hipFree(xlim);
+    }
+  } b;
+  HIPCHK(hipMalloc(&b.th, n * 4));
+  HIPCHK(hipMalloc(&b.dist, n * 4));
+  HIPCHK(hipMalloc(&b.xl4, 4 * (size_t)nr * 4));
+  HIPCHK(hipMalloc(&b.xlim, (size_t)g.JM * 4));
+  const hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(b.th, theta, n * 4, hipMemcpyHostToDevice, st));
+  const float *rows[4] = {xllim_in, xllim_le, xllim_te, xllim_dn};
+  for (int q = 0; q < 4; q++)
+    HIPCHK(hipMemcpyAsync(b.xl4 + (size_t)q * nr, rows[q], (size_t)nr * 4, hipMemcpyHostToDevice, st));
+  const Dev &d = c->d;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  LAUNCH(k_xl_dist, dim3(nb), dim3(256), 0, st, g, d, b.th, b.dist, p.IBOUND);
+  LCHK("xl_dist");
+  LAUNCH(k_xl_limit, dim3((g.JM + 63) / 64), dim3(64), 0, st, g, d, b.dist, b.xl4, b.xlim);
+  LCHK("xl_limit");
+  const float dblend = 0.6666666f, expon = dblend / (1.0f - dblend), fblend = (1.0f - dblend) * powf(dblend, expon);
+  LAUNCH(k_xl_cells, dim3(nb), dim3(256), 0, st, g, d, b.dist, b.xlim, dblend, expon, fblend);
+  LCHK("xl_cells");
+  HIPCHK(hipStreamSynchronize(st));
   return MH_OK;
 }
 

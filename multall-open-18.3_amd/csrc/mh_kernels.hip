@@ -3889,3 +3889,162 @@ __global__ void k_integrals(Geo g, Dev d, Params p) {
 }
 
 }  // namespace mh
+
+namespace mh {
+
+// ---------------------------------------------------------------------------
+// SET_XLENGTH (mf:12194-12517): wall distances and mixing-length scales, once
+// per run.  k_xl_dist: one thread per node.  The end walls: the nearest hub
+// (casing) grid point over J +/- 20, then the distance along that wall
+// point's face normal (mf:12227-12287); the blade surfaces I = 1 and I = IM
+// of the node's own row: the nearest surface point over J +/- 20 (clipped to
+// the row) and K +/- 5 in true 3D distance, then the normal distance there,
+// or the point distance off the blade (mf:12296-12402); the two combined as
+// the reference does (mf:12408-12424).  FP32 in the reference's statement
+// order; sinf is the device's.
+__global__ void __launch_bounds__(256) k_xl_dist(Geo g, Dev d, const float *TH, float *DIST, int IBOUND) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.IM * g.JM * g.KM) return;
+  const int I = n % g.IM + 1, J = (n / g.IM) % g.JM + 1, K = n / (g.IM * g.JM) + 1;
+  const int NR = d.NROW[J - 1];
+  const float XP = d.X[JKX(J, K)], RP = d.R[JKX(J, K)];
+  const float PITCH = 6.283185f * RP / (float)d.NBLADE[J - 1];
+  const float DREF = 0.5f * PITCH;
+  const int IW = g.IMID, j1 = max(J - 20, 2), j2 = min(J + 20, g.JM);
+  auto endwall = [&](int KW) {
+    float dmin = 1.0e10f;
+    int jmin = j1;
+    for (int jw = j1; jw <= j2; jw++) {
+      const float xd = XP - d.X[JKX(jw, KW)], rd = RP - d.R[JKX(jw, KW)];
+      const float ds = xd * xd + rd * rd;
+      if (ds < dmin) {
+        dmin = ds;
+        jmin = jw;
+      }
+    }
+    const float ax = d.ASX[IX(IW, jmin, KW)], ar = d.ASR[IX(IW, jmin, KW)];
+    const float atot = sqrtf(ax * ax + ar * ar);
+    const float xn = ax / atot, rn = ar / atot;
+    const float xd = XP - d.X[JKX(jmin, KW)], rd = RP - d.R[JKX(jmin, KW)];
+    return fabsf(xd * xn + rd * rn);
+  };
+  float hub = endwall(1);
+  if (IBOUND == 1 || IBOUND > 2) hub = DREF;
+  float tip = endwall(g.KM);
+  if (IBOUND >= 2) tip = DREF;
+  const float EW = hub * tip / (hub + tip);
+  const int jst = d.JSTART[NR - 1], jle = d.JLE[NR - 1], jte = d.JTE[NR - 1], jmx = d.JMIX[NR - 1];
+  int b1 = J - 20;
+  if (b1 < 2) b1 = 2;
+  if (b1 < jst) b1 = jst;
+  if (b1 >= jte) b1 = jte - 1;
+  int b2 = J + 20;
+  if (b2 > g.JM) b2 = g.JM;
+  if (b2 > jmx) b2 = jmx;
+  if (b2 <= jle) b2 = jle + 1;
+  const int k1 = max(K - 5, 1), k2 = min(K + 5, g.KM);
+  const float TP = TH[n];
+  float sd[2];
+  for (int side = 0; side < 2; side++) {
+    const int IS = side ? g.IM : 1;
+    float dmin = 1.0e10f;
+    int jm = 0, km = 0;
+    for (int js = b1; js <= b2; js++)
+      for (int ks = k1; ks <= k2; ks++) {
+        const float RS = d.R[JKX(js, ks)];
+        const float xd = XP - d.X[JKX(js, ks)], rd = RP - RS;
+        const float s = sinf((TP - TH[IX(IS, js, ks)]) / 2.0f);
+        const float td = 4.0f * RP * RS * (s * s);
+        const float ds = xd * xd + rd * rd + td;
+        if (ds < dmin) {
+          dmin = ds;
+          jm = js;
+          km = ks;
+        }
+      }
+    if (jm > 0) {
+      const int kn = km == g.KM ? g.KM - 1 : km;
+      const float bx = d.ABX[IX(IS, jm, kn)], br = d.ABR[IX(IS, jm, kn)], bt = d.ABT[JKX(jm, kn)];
+      const float atot = sqrtf(bx * bx + br * br + bt * bt);
+      const float xn = bx / atot, rn = br / atot, tn = bt / atot;
+      const float RS = d.R[JKX(jm, km)];
+      const float xd = XP - d.X[JKX(jm, km)], rd = RP - RS;
+      const float tdif = 2.0f * sqrtf(RP * RS) * sinf((TP - TH[IX(IS, jm, km)]) / 2.0f);
+      sd[side] = fabsf(xd * xn + rd * rn + tdif * tn);
+      if (jm <= jle || jm >= jte) sd[side] = sqrtf(xd * xd + rd * rd + tdif * tdif);
+    } else {
+      sd[side] = DREF;
+    }
+  }
+  const float BD = sd[0] * sd[1] / (sd[0] + sd[1]);
+  float dm = 0.0f;
+  if (K > 1 && K < g.KM) {
+    float q = sqrtf(EW * EW + BD * BD);
+    if (q < 1.0e-10f) q = 1.0e-10f;
+    dm = BD * EW / q;
+  }
+  if (g.KM == 2) dm = BD;
+  if (g.IM == 2) dm = EW;
+  DIST[n] = dm;
+  (void)I;
+}
+
+// The mixing-length limit along each row at (IMID, KMID), varying linearly in
+// meridional distance from inlet to leading edge, leading to trailing edge,
+// trailing edge to mixing plane (mf:12428-12449).  XL4 = [IN | LE | TE | DN]
+// per row.  One thread per J.
+__global__ void k_xl_limit(Geo g, Dev d, const float *DIST, const float *XL4, float *XLIMIT) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (J > g.JM) return;
+  const int NR = d.NROW[J - 1], K = g.KMID, nr = g.NROWS;
+  const float xin = XL4[NR - 1], xle = XL4[nr + NR - 1], xte = XL4[2 * nr + NR - 1], xdn = XL4[3 * nr + NR - 1];
+  const int jst = d.JSTART[NR - 1], jle = d.JLE[NR - 1], jte = d.JTE[NR - 1], jmx = d.JMIX[NR - 1];
+  const float *S = d.SMERID;
+  float xllim = 0.0f;
+  if (J <= jle) xllim = xin + (S[JKX(J, K)] - S[JKX(jst, K)]) / (S[JKX(jle, K)] - S[JKX(jst, K)]) * (xle - xin);
+  if (J > jle && J <= jte)
+    xllim = xle + (S[JKX(J, K)] - S[JKX(jle, K)]) / (S[JKX(jte, K)] - S[JKX(jle, K)]) * (xte - xle);
+  if (J > jte) xllim = xte + (S[JKX(J, K)] - S[JKX(jte, K)]) / (S[JKX(jmx, K)] - S[JKX(jte, K)]) * (xdn - xte);
+  XLIMIT[J - 1] = 2.0f * DIST[IX(g.IMID, J, K)] * xllim;
+}
+
+// The blended mixing length, the free-stream damping ratio and their cell
+// averages (mf:12460-12508): XLENGTH = (0.41 l)^2, FST_RAT, DWALLSQ per cell,
+// zero on the planes that hold no cell and on each row's first J.  One thread
+// per node.
+__global__ void __launch_bounds__(256) k_xl_cells(Geo g, Dev d, const float *DIST, const float *XLIMIT, float dblend,
+                                                  float expon, float fblend) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.IM * g.JM * g.KM) return;
+  const int I = n % g.IM + 1, J = (n / g.IM) % g.JM + 1, K = n / (g.IM * g.JM) + 1;
+  if (I > g.IMM1 || J < 2 || K > g.KMM1) {
+    d.XLENGTH[n] = 0.0f;
+    d.FST_RAT[n] = 0.0f;
+    d.DWALLSQ[n] = 0.0f;
+    return;
+  }
+  float xl[8], t3[8], dd[8];
+  const int di[8] = {0, 0, 0, 0, 1, 1, 1, 1}, dj[8] = {0, 0, -1, -1, 0, 0, -1, -1}, dk[8] = {0, 1, 0, 1, 0, 1, 0, 1};
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int Jq = J + dj[q];
+    const float lim = XLIMIT[Jq - 1];
+    float x = DIST[IX(I + di[q], Jq, K + dk[q])];
+    dd[q] = x;
+    const float ratio = x / lim;
+    if (ratio > dblend) x = lim * (1.0f - fblend * powf(ratio, -expon));
+    xl[q] = x;
+    const float r = x / lim;
+    t3[q] = ((r * r) * r) * r;  // the reference compiler's order for r**4
+  }
+  const float axl = 0.125f * (xl[0] + xl[1] + xl[2] + xl[3] + xl[4] + xl[5] + xl[6] + xl[7]);
+  const float aft = 0.125f * (t3[0] + t3[1] + t3[2] + t3[3] + t3[4] + t3[5] + t3[6] + t3[7]);
+  const float ads = 0.125f * (dd[0] + dd[1] + dd[2] + dd[3] + dd[4] + dd[5] + dd[6] + dd[7]);
+  const int NR = d.NROW[J - 1];
+  const bool first = J == d.JSTART[NR - 1];
+  d.XLENGTH[n] = first ? 0.0f : 0.41f * 0.41f * axl * axl;
+  d.FST_RAT[n] = first ? 0.0f : aft * d.FSTURB[NR - 1];
+  d.DWALLSQ[n] = ads * ads;
+}
+
+}  // namespace mh

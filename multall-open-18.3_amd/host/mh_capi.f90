@@ -205,6 +205,11 @@ contains
     case ('ROUGH_L'); call putr1(ROUGH_L)
     case ('ROUGH_U'); call putr1(ROUGH_U)
     case ('CHORD'); call putr1(CHORD)
+    case ('THETA'); call putr3(THETA)
+    case ('XLLIM_IN'); call putr1(XLLIM_IN)
+    case ('XLLIM_LE'); call putr1(XLLIM_LE)
+    case ('XLLIM_TE'); call putr1(XLLIM_TE)
+    case ('XLLIM_DN'); call putr1(XLLIM_DN)
     case ('JTRAN_I1'); call puti(JTRAN_I1)
     case ('JTRAN_IM'); call puti(JTRAN_IM)
     case ('JTRAN_K1'); call puti(JTRAN_K1)
@@ -212,7 +217,6 @@ contains
     case ('FSTURB'); call putr1(FSTURB)
     case ('TURBVIS_DAMP'); call putr1(TURBVIS_DAMP)
     case ('ROAVG_CELL'); call putr3(ROAVG_CELL)
-    case ('THETA'); call putr3(THETA)
     case ('RTHETA'); call putr3(RTHETA)
     case ('AQTOT'); call putr3(AQTOT)
     case ('ABTOT'); call putr3(ABTOT)

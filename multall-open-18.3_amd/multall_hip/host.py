@@ -62,6 +62,17 @@ def _load():
     return _lib
 
 
+def array(name: str, shape, dtype=np.float32) -> np.ndarray:
+    """One more array of the deck loaded last (e.g. THETA, XLLIM_IN), by its
+    reference name."""
+    L = _load()
+    out = np.empty(int(np.prod(shape)), dtype)
+    n = L.mhh_array(name.encode(), out.ctypes.data, out.size)
+    if n != out.size:
+        raise RuntimeError(f"{name}: host returned {n}, expected {out.size}")
+    return out.reshape(shape)
+
+
 def load_deck(path: str) -> Dict:
     """Read and set up a deck; return the state dict (dense [k, j, i] arrays)."""
     L = _load()

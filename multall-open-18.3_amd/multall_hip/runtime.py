@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmultall_hip.so")
 
 EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar_id", "mh_array_len",
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
-           "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_start", "mh_step",
+           "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_set_xlength", "mh_start", "mh_step",
            "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch"]
 
 
@@ -71,6 +71,7 @@ def load(path: str = LIB_PATH):
     L.mh_get_scalar.restype = ctypes.c_double
     L.mh_start.argtypes = [vp]
     L.mh_loss_init.argtypes = [vp]
+    L.mh_set_xlength.argtypes = [vp] + [ctypes.c_void_p] * 5
     L.mh_step.argtypes = [vp, i, ctypes.POINTER(MhResid)]
     L.mh_step_batch.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(MhResid), ctypes.POINTER(i)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
@@ -127,6 +128,12 @@ class Solver:
     def loss_init(self):
         """The SETUP-time viscous-force initialisation (one LOSS call)."""
         self._check(self.L.mh_loss_init(self.h), "mh_loss_init")
+
+    def set_xlength(self, theta, xllim_in, xllim_le, xllim_te, xllim_dn):
+        """SET_XLENGTH on the device (wall distances, XLENGTH, FST_RAT) from
+        the nodal THETA and the per-row mixing-length limits."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (theta, xllim_in, xllim_le, xllim_te, xllim_dn)]
+        self._check(self.L.mh_set_xlength(self.h, *[a.ctypes.data for a in arrs]), "mh_set_xlength")
 
     def start(self):
         self._check(self.L.mh_start(self.h), "mh_start")
