@@ -80,12 +80,16 @@ def time_to_converged() -> dict:
     functions; 45 x 150 x 49 nodes) marched from the reference's initial
     guess until the reference's stop test ends the run (mf:3762-3764): wall
     seconds for the steps alone and including the deck read and host
-    set-up."""
+    set-up (the wall distances and mixing lengths formed on the device)."""
     from multall_hip import host
     from multall_hip.runtime import Solver
     t0 = time.perf_counter()
-    st = host.load_deck(CONVERGED_DECK)
+    st = host.load_deck(CONVERGED_DECK, xlength_on_host=False)
+    sc = st["scalars"]
     g = Solver(st, exact_order=False)
+    # the wall distances and mixing lengths on the device (SET_XLENGTH)
+    g.set_xlength(host.array("THETA", (sc["KM"], sc["JM"], sc["IM"])),
+                  *[host.array(n, (sc["NROWS"],)) for n in ("XLLIM_IN", "XLLIM_LE", "XLLIM_TE", "XLLIM_DN")])
     g.loss_init()
     g.start()
     g.sync()

@@ -26,6 +26,13 @@ contains
     if (mh_host_load_path(cstr(path))) mhh_load = -1
   end function mhh_load
 
+  !> 1: the next loads form DWALLSQ/XLENGTH/FST_RAT on the host (SET_XLENGTH,
+  !> the default); 0: they leave them zero for the library's mh_set_xlength.
+  subroutine mhh_set_host_xlength(flag) bind(C, name='mhh_set_host_xlength')
+    integer(c_int), value :: flag
+    host_xlength = flag /= 0
+  end subroutine mhh_set_host_xlength
+
   !> Copy the last error message (NUL-terminated, truncated to n bytes).
   subroutine mhh_error(buf, n) bind(C, name='mhh_error')
     character(kind=c_char), intent(out) :: buf(*)

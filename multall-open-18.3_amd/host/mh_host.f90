@@ -39,6 +39,9 @@ module mh_host
   real, save, public :: FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD
   real, save, public :: GA1, FGA, RFGA, RCP, CV, RCV, RGAS, SFEX1, SFEXH, VSOUND, DSMIN, VORT_MAX, PREFF
   integer, save, public :: NOUT(5)
+  ! form the wall distances and mixing lengths here (SET_XLENGTH); off when
+  ! the caller uses the library's mh_set_xlength instead
+  logical, save, public :: host_xlength = .true.
 
   ! ---- per row --------------------------------------------------------------
   integer, allocatable, save, public :: JSTART(:), JMIX(:), JLE(:), JTE(:), JLED(:), KTIPS(:), KTIPE(:)
@@ -1743,8 +1746,9 @@ contains
     avgspan = 0.5*(sqrt(dx1*dx1 + dr1*dr1) + sqrt(dxm*dxm + drm*drm))
     VORT_MAX = 5000.*VM1(KMID)/avgspan
     ! wall distances and mixing lengths; the viscous models use them
-    ! (mf:7340), an inviscid run does not
-    if (ILOS /= 0) then
+    ! (mf:7340), an inviscid run does not; with host_xlength off the
+    ! caller forms them on the device (mh_set_xlength) and they stay zero here
+    if (ILOS /= 0 .and. host_xlength) then
       call set_xlength()
     else
       allocate (DWALLSQ(IM, JM, KM), XLENGTH(IM, JM, KM), FST_RAT(IM, JM, KM), XLIMIT(JM))

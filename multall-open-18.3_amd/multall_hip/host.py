@@ -58,6 +58,7 @@ def _load():
         L.mhh_scalar.restype = ctypes.c_int
         L.mhh_array.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong]
         L.mhh_array.restype = ctypes.c_longlong
+        L.mhh_set_host_xlength.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -73,9 +74,12 @@ def array(name: str, shape, dtype=np.float32) -> np.ndarray:
     return out.reshape(shape)
 
 
-def load_deck(path: str) -> Dict:
-    """Read and set up a deck; return the state dict (dense [k, j, i] arrays)."""
+def load_deck(path: str, xlength_on_host: bool = True) -> Dict:
+    """Read and set up a deck; return the state dict (dense [k, j, i] arrays).
+    With xlength_on_host False the wall distances and mixing lengths
+    (DWALLSQ, XLENGTH, FST_RAT) are left zero for ``Solver.set_xlength``."""
     L = _load()
+    L.mhh_set_host_xlength(1 if xlength_on_host else 0)
     if L.mhh_load(os.fspath(path).encode()) != 0:
         buf = ctypes.create_string_buffer(512)
         L.mhh_error(buf, 512)
