@@ -5,8 +5,11 @@ reference's Rotor 37 deck (NASA transonic compressor rotor, 45 x 150 x 49
 nodes, 36 blades), as it ships (Spalart-Allmaras with its helicity and
 pressure-gradient source factors and the Shih wall functions with the
 pressure-gradient term) and with plain Spalart-Allmaras and the log-law
-walls, and its mass flow, total pressure ratio and total-to-total isentropic
-efficiency are compared with the FP64 build of the reference
+walls, and on two of the reference's one-stage machines as they ship (the
+axial compressor and the axial turbine: rotor and stator across a mixing
+plane, gaps re-gridded by the reader, mixing-length model, 37 x 222 x 37 and
+46 x 282 x 46 nodes).  Mass flow, total pressure ratio and total-to-total
+isentropic efficiency are compared with the FP64 build of the reference
 (-fdefault-real-8) run on the same deck: within 0.05 % for mass flow and
 pressure ratio and 0.05 points for efficiency.
 
