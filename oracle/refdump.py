@@ -33,7 +33,9 @@ BUILD_PARAMS = dict(ID=65, JD=600, KD=65, MAXKI=65, NRS=21, IG1=30, JG1=1000, KG
                     IG2=25, JG2=500, KG2=20, JG3=100, NCWL=100, NBLED=25)
 # per executable: fp32w is the pitchwise-wide build (DIMS_WIDE in build_ref.sh)
 VARIANT_PARAMS = {"fp32": BUILD_PARAMS, "fp64": BUILD_PARAMS,
-                  "fp32w": dict(BUILD_PARAMS, ID=96, JD=200, KD=33, MAXKI=96)}
+                  "fp32w": dict(BUILD_PARAMS, ID=96, JD=200, KD=33, MAXKI=96),
+                  "fp32l": dict(BUILD_PARAMS, ID=38, JD=1300, KD=38, MAXKI=38),
+                  "fp64l": dict(BUILD_PARAMS, ID=38, JD=1300, KD=38, MAXKI=38)}
 
 _EXPLICIT_REAL = {"MACHLIM", "MACHREL", "MACHCOOL"}
 

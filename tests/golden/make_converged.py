@@ -75,7 +75,11 @@ CASES = {
     # stator across a mixing plane, gaps re-gridded by the reader)
     "axialcompr": shipped("simple-axial-compr-17.1.dat"),
     "axialturb": shipped("Single-stage-turbine-17.1.dat"),
+    # the six-stage compressor: twelve rows, eleven mixing planes
+    "sixstage": shipped("Six-stage-compr-17.1.dat"),
 }
+# decks longer than the main builds' JD = 600 run on the long builds
+LONG = {"sixstage"}
 
 
 def make(case):
@@ -85,7 +89,8 @@ def make(case):
     with open(os.path.join(HERE, "decks", deck), "w") as f:
         f.write(txt)
     out = {"deck": f"decks/{deck}"}
-    for variant, real in (("fp64", np.float64), ("fp32", np.float32)):
+    sfx = "l" if case in LONG else ""
+    for variant, real in (("fp64" + sfx, np.float64), ("fp32" + sfx, np.float32)):
         wd = tempfile.mkdtemp(prefix="mhconv_")
         t0 = time.time()
         try:
@@ -93,8 +98,8 @@ def make(case):
             integ = read_global_plt(os.path.join(wd, "global.plt"), real)
         finally:
             shutil.rmtree(wd, ignore_errors=True)
-        out[variant] = {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist()}
-        print(f"{case} {variant}: stopped at step {out[variant]['nstep']} ({time.time() - t0:.0f} s)", flush=True)
+        out[variant.rstrip("l")] = {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist()}
+        print(f"{case} {variant}: stopped at step {out[variant.rstrip('l')]['nstep']} ({time.time() - t0:.0f} s)", flush=True)
     with open(os.path.join(HERE, f"converged_{case}.json"), "w") as f:
         json.dump(out, f)
 

@@ -1,10 +1,12 @@
 """The HIP library on the reference's own test decks against the oracle.
 
 Each deck under tests/golden/decks/ref/ (the reference's multall-test-cases,
-verbatim: one- to three-row axial and radial machines whose inter-row gaps
-the reader re-grids, new trailing-edge cusps, a KM = 5 inviscid shock case,
-Rotor 37) is set up by the Fortran host, whose set-up equals the
-reference's bit for bit (test_host_setup.py).  The library's SETUP-time
+verbatim: one- to three-row axial and radial machines and the twelve-row
+six-stage compressor, inter-row gaps re-gridded by the reader, new
+trailing-edge cusps, a KM = 5 inviscid shock case, Rotor 37) is set up by the
+Fortran host, whose set-up equals the reference's bit for bit
+(test_host_setup.py; the six-stage compressor to the last bits of its last
+wake line).  The library's SETUP-time
 viscous call (mh_loss_init) forms the forces and turbulence state; the
 oracle starts from that same state, and one time step in the reference's
 statement order (exact-order mode) is compared: conserved variables,
