@@ -8,7 +8,8 @@ pressure-gradient term) and with plain Spalart-Allmaras and the log-law
 walls, and on two of the reference's one-stage machines as they ship (the
 axial compressor and the axial turbine: rotor and stator across a mixing
 plane, gaps re-gridded by the reader, mixing-length model, 37 x 222 x 37 and
-46 x 282 x 46 nodes).  Mass flow, total pressure ratio and total-to-total
+46 x 282 x 46 nodes) and on its six-stage compressor (twelve rows, eleven
+mixing planes, 37 x 1282 x 37 nodes; reference on its long build).  Mass flow, total pressure ratio and total-to-total
 isentropic efficiency are compared with the FP64 build of the reference
 (-fdefault-real-8) run on the same deck: within 0.05 % for mass flow and
 pressure ratio and 0.05 points for efficiency.
@@ -27,7 +28,7 @@ import pytest
 
 from conftest import GOLDEN
 
-CASES = ["r37samods", "r37sa", "axialcompr", "axialturb"]
+CASES = ["r37samods", "r37sa", "axialcompr", "axialturb", "sixstage"]
 BOUND_REL = 5e-4     # 0.05 % (mass flow, pressure ratio)
 BOUND_ETA = 5e-4     # 0.05 points of efficiency
 
