@@ -68,6 +68,12 @@ module mh_hipif
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function mh_loss_init
+    integer(c_int) function mh_set_xlength(ctx, theta, xllim_in, xllim_le, xllim_te, xllim_dn) &
+        bind(C, name='mh_set_xlength')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: theta(*), xllim_in(*), xllim_le(*), xllim_te(*), xllim_dn(*)
+    end function mh_set_xlength
     integer(c_int) function mh_start(ctx) bind(C, name='mh_start')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

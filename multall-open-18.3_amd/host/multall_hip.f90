@@ -29,6 +29,8 @@ program multall_hip
   open (unit=12, file='stopit')
   write (12, *) 0
   close (12)
+  ! the wall distances and mixing lengths are formed on the device below
+  host_xlength = .false.
   if (mh_host_load_unit(5)) then
     write (*, *) ' DECK/SETUP FAILED: ', trim(mh_host_err)
     stop 2
@@ -38,7 +40,12 @@ program multall_hip
   rc = mh_create(dims, ctx)
   call check(rc, 'mh_create')
   call upload()
-  ! SETUP's initial viscous forces (mf:7346-7357); a no-op when ILOS = 0
+  ! SETUP's SET_XLENGTH (mf:7340) on the device, then its initial viscous
+  ! forces (mf:7346-7357); both no-ops when ILOS = 0
+  if (ILOS /= 0) then
+    rc = mh_set_xlength(ctx, THETA, XLLIM_IN, XLLIM_LE, XLLIM_TE, XLLIM_DN)
+    call check(rc, 'mh_set_xlength')
+  end if
   rc = mh_loss_init(ctx)
   call check(rc, 'mh_loss_init')
   rc = mh_start(ctx)
