@@ -88,9 +88,10 @@ def make(name: str) -> dict:
 
 
 if __name__ == "__main__":
-    # the six-stage compressor is stepped on the GPU but not digested: on the
-    # reference's long build its wake line behind the last row differs from
-    # the host's in the last bits (an open item)
+    # the six-stage compressor is stepped on the GPU but not digested: the
+    # reference's long build reads an unset local in GRID_DOWN
+    # (SNEW(J1-NEXTRAP-1)), so its wake line behind the last row differs from
+    # the host's (which takes the value as zero) in the last bits
     names = sys.argv[1:] or sorted(n for n in os.listdir(DECKS) if not n.startswith("Six-stage"))
     out = json.load(open(OUT)) if os.path.exists(OUT) else {}
     for n in names:
