@@ -118,6 +118,11 @@ enum mh_array_id {
    * blade chord of each row, for the trailing-edge Reynolds number that
    * scales the SA pressure-gradient source factor -- RN */
   MH_DPDS_CELL, MH_YPLUS_I1, MH_YPLUS_IM, MH_YPLUS_K1, MH_YPLUS_KM, MH_CHORD,
+  /* the five printout steps NOUT (BKCNTL, read at mf:723): a step that is
+   * one of them and ends on a convergence check refreshes the viscous
+   * forces there, as the reference does before OUTPUT (mf:3767-3769,
+   * 5338-5341) -- 5 ints; unset (0) means none inside the run */
+  MH_NOUT,
   MH_NARRAYS
 };
 
@@ -200,6 +205,12 @@ int mh_start(mh_ctx *ctx);
 /* Run up to nsteps time steps (fewer if the run ends).  *out (optional)
  * receives the latest convergence summary. */
 int mh_step(mh_ctx *ctx, int nsteps, mh_resid *out);
+/* A stop request, as the reference's `stopit` file (read at the start of
+ * every step N with N divisible by 10, mf:3774-3779): the next such step
+ * becomes the run's last (IFEND = 1), with its closing viscous call and
+ * convergence check, and NSTEPS_MAX keeps its value.  The host polls the
+ * file between mh_step calls and calls this when it holds a value >= 1. */
+int mh_request_stop(mh_ctx *ctx);
 
 /* Several independent machines (operating points of a speedline, say) at
  * once: each of the n distinct contexts advances up to nsteps steps, as

@@ -134,7 +134,7 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"T_SOURCE", K_N, false}, {"ROAVG_CELL", K_N, false}, {"VISC_LAM", K_N, false}, {"VISC_RAT", K_N, false},
     {"XLENGTH", K_N, false}, {"FST_RAT", K_N, false}, {"FSTURB", K_RN, false}, {"TURBVIS_DAMP", K_RN, false},
     {"DPDS_CELL", K_N, false}, {"YPLUS_I1", K_JK, false}, {"YPLUS_IM", K_JK, false}, {"YPLUS_K1", K_IJ, false},
-    {"YPLUS_KM", K_IJ, false}, {"CHORD", K_RN, false},
+    {"YPLUS_KM", K_IJ, false}, {"CHORD", K_RN, false}, {"NOUT", K_NOUT, true},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -155,6 +155,7 @@ using namespace mh;
 namespace {
 
 constexpr int kRedBlocks = 1024;
+constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
 long long kind_len(const mh_dims &d, Kind k) {
   switch (k) {
@@ -170,6 +171,7 @@ long long kind_len(const mh_dims &d, Kind k) {
     case K_B2: return (long long)d.nib2 * (d.njb2 + 1) * d.nkb2;
     case K_SB: return d.nsblk;
     case K_RN: return d.nrows;
+    case K_NOUT: return 5;
   }
   return 0;
 }
@@ -200,7 +202,7 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.TRANS_DYN_VIS, (void **)&d.DEL_DYNVIS, (void **)&d.TRANS_KVIS, (void **)&d.T_SOURCE,
       (void **)&d.ROAVG_CELL, (void **)&d.VISC_LAM, (void **)&d.VISC_RAT, (void **)&d.XLENGTH, (void **)&d.FST_RAT,
       (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP, (void **)&d.DPDS_CELL, (void **)&d.YPLUS_I1,
-      (void **)&d.YPLUS_IM, (void **)&d.YPLUS_K1, (void **)&d.YPLUS_KM, (void **)&d.CHORD,
+      (void **)&d.YPLUS_IM, (void **)&d.YPLUS_K1, (void **)&d.YPLUS_KM, (void **)&d.CHORD, (void **)&d.NOUT,
   };
   return base[id];
 }
@@ -858,6 +860,9 @@ int step_once(mh_ctx *c) {
   if (c->nstep == p.NSTEPS_MAX) c->ifend = 1;
   if (c->EAVG < p.CONLIM && c->ECONT < 0.01f) c->ifend = 1;
   if (fabsf(c->EMAX) < (4 * p.CONLIM) && c->ECONT < 0.01f) c->ifend = 1;
+  bool ifprint = false;
+  for (int i = 0; i < 5; i++) ifprint = ifprint || c->nstep == c->nout[i];
+  if (c->nstep % 10 == 0 && c->stop_req) c->ifend = 1;  // stopit, mf:3774-3779
   float FCHANGE = 1.0f - (float)(c->nstep - 1) / (float)p.NCHANGE;
   if (FCHANGE > 0.0f) {
     c->SFT = c->SFTIN_s + 0.02f * FCHANGE;
@@ -977,8 +982,9 @@ int step_once(mh_ctx *c) {
     LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
     LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("convergence");
-    // forces brought up to date before the final output, mf:5338-5339
-    if (c->ifend == 1 && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
+    // forces brought up to date before a printout or the final output,
+    // mf:5338-5341
+    if ((ifprint || c->ifend == 1) && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
     if ((r = stepup(c, 0.25f))) return r;
     float sc[8];
     HIPCHK(hipMemcpyAsync(sc, d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
@@ -1093,12 +1099,16 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   d.AVG = d.SC + SC_AVG_CHG;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
-  if (c->lds_plane > 160 * 1024) return bail(MH_E_UNSUPPORTED);
-  if (hipFuncSetAttribute((const void *)k_smooth_plane, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)c->lds_plane) != hipSuccess ||
-      hipFuncSetAttribute((const void *)k_damp_scatter4, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)scatter_lds_bytes(dm.im - 1)) != hipSuccess)
-    return bail(MH_E_HIP);
+  if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1) > kMaxLds) return bail(MH_E_UNSUPPORTED);
+  // The dynamic-LDS limit is a per-function, process-wide attribute: set it
+  // once to the largest size any context may ask for, so a later context
+  // never lowers it below what an earlier one launches with.
+  static const bool lds_ok =
+      hipFuncSetAttribute((const void *)k_smooth_plane, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) ==
+          hipSuccess &&
+      hipFuncSetAttribute((const void *)k_damp_scatter4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) ==
+          hipSuccess;
+  if (!lds_ok) return bail(MH_E_HIP);
   *out = c;
   return MH_OK;
 }
@@ -1302,6 +1312,8 @@ int mh_start(mh_ctx *c) {
   c->EMAX = 1.0e06f;
   c->ifend = 0;
   c->nstep = 0;
+  c->stop_req = false;
+  if ((r = d2h(c, c->nout, c->d.NOUT, sizeof c->nout))) return r;
   c->last = mh_resid{};
   c->started = true;
   return MH_OK;
@@ -1316,6 +1328,13 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
     if (c->prof_on && c->prof_pending.size() > 4096 && (r = prof_collect(c))) return r;
   }
   if (out) *out = c->last;
+  return MH_OK;
+}
+
+int mh_request_stop(mh_ctx *c) {
+  if (!c) return MH_E_ARG;
+  if (!c->started) return fail(c, MH_E_STATE, "mh_request_stop before mh_start");
+  c->stop_req = true;
   return MH_OK;
 }
 

@@ -19,7 +19,7 @@
 
 namespace mh {
 
-enum Kind : int { K_N, K_JK, K_IK, K_IJ, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB, K_RN };
+enum Kind : int { K_N, K_JK, K_IK, K_IJ, K_I, K_K, K_J, K_R, K_B1, K_B2, K_SB, K_RN, K_NOUT };
 
 struct ArrayInfo {
   const char *name;
@@ -57,6 +57,7 @@ struct Dev {
   float *TRANS_DYN_VIS, *DEL_DYNVIS, *TRANS_KVIS, *T_SOURCE, *ROAVG_CELL, *VISC_LAM, *VISC_RAT, *XLENGTH, *FST_RAT;
   float *FSTURB, *TURBVIS_DAMP;
   float *DPDS_CELL, *YPLUS_I1, *YPLUS_IM, *YPLUS_K1, *YPLUS_KM, *CHORD;
+  int *NOUT;
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
@@ -197,6 +198,8 @@ struct mh_ctx {
   bool started = false;
   // loop state (mf:3674-3714 and the per-step bookkeeping)
   int nstep = 0, ifend = 0;
+  int nout[5] = {};       // printout steps (read from the device at mh_start)
+  bool stop_req = false;  // mh_request_stop: end at the next step divisible by 10
   float SFT = 0, SFX = 0, DAMP = 0, FMIXUP = 1, PREFF = 0, ROLIM = 0, ROMIN = 0;
   float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
   float EAVG = 1e6f, EMAX = 1e6f, ECONT = 1.f;

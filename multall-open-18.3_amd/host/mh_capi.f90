@@ -287,6 +287,7 @@ contains
     case ('KTIPE'); call puti(KTIPE(1:NROWS + 1))
     case ('KS1'); call puti(KS1(1:NROWS + 1))
     case ('KS2'); call puti(KS2(1:NROWS + 1))
+    case ('NOUT'); call puti(NOUT)
     end select
   contains
     subroutine putr1(a)

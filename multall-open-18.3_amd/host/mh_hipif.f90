@@ -84,6 +84,10 @@ module mh_hipif
       integer(c_int), value :: n
       type(mh_resid), intent(out) :: r
     end function mh_step
+    integer(c_int) function mh_request_stop(ctx) bind(C, name='mh_request_stop')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function mh_request_stop
     integer(c_int) function mh_integrals(ctx, out, n) bind(C, name='mh_integrals')
       import :: c_int, c_ptr, c_double, c_long_long
       type(c_ptr), value :: ctx

@@ -2,8 +2,10 @@
 !> with `CALL LOOP` replaced by the MI355X library.  Reads a NEW-format deck
 !> on standard input, builds the grid with mh_host, uploads it, and marches
 !> in chunks of 5 steps -- writing the reference's convergence line to
-!> stage.log (mf:5437-5438) and polling the `stopit` file every 10 steps
-!> (mf:3774-3779) -- until NSTEPS_MAX or convergence.  Writes `flow_out` in
+!> stage.log (mf:5437-5438) and polling the `stopit` file between chunks; a
+!> request ends the run at the next step divisible by 10, the step at which
+!> the reference reads the file (mf:3774-3779) -- until NSTEPS_MAX or
+!> convergence.  Writes `flow_out` in
 !> the reference's record layout (mf:5951-5976) at the end.
 program multall_hip
   use iso_c_binding
@@ -64,13 +66,12 @@ program multall_hip
       write (*, '(i6,3f9.4,2f10.2,2f12.4)') res%nstep, res%emax, res%eavg, res%econt, res%vref, res%vmax, &
       res%flow_in, res%flow_out
     if (res%ifend == 1) exit
-    if (mod(res%nstep, 10) == 0) then
-      open (unit=12, file='stopit')
-      read (12, *, iostat=ios) ifstop
-      close (12)
-      if (ios == 0 .and. ifstop >= 1) then
-        rc = mh_set_scalar(ctx, mh_scalar_id(cname('NSTEPS_MAX')), dble(res%nstep + 1))
-      end if
+    open (unit=12, file='stopit')
+    read (12, *, iostat=ios) ifstop
+    close (12)
+    if (ios == 0 .and. ifstop >= 1) then
+      rc = mh_request_stop(ctx)
+      call check(rc, 'mh_request_stop')
     end if
   end do
   close (4)
@@ -151,6 +152,7 @@ contains
     call puti('JLE', JLE(1:NROWS + 1)); call puti('JTE', JTE(1:NROWS + 1)); call puti('JLED', JLED(1:NROWS + 1))
     call puti('KTIPS', KTIPS(1:NROWS + 1)); call puti('KTIPE', KTIPE(1:NROWS + 1))
     call puti('KS1', KS1(1:NROWS + 1)); call puti('KS2', KS2(1:NROWS + 1))
+    call puti('NOUT', NOUT)
     call sets('CFL', dble(CFL)); call sets('DAMPIN', dble(DAMPIN)); call sets('MACHLIM', dble(MACHLIM))
     call sets('F_PDOWN', dble(F_PDOWN)); call sets('CONLIM', dble(CONLIM)); call sets('SFXIN', dble(SFXIN))
     call sets('SFTIN', dble(SFTIN)); call sets('FAC_4TH', dble(FAC_4TH)); call sets('F1', dble(F1))

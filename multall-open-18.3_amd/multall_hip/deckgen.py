@@ -108,6 +108,11 @@ class DeckSpec:
     po1: float = 1.0e5
     to1: float = 300.0
     vm1: float = 100.0
+    vtin: float = 0.0              # inlet tangential velocity (read with IN_VTAN = 1), m/s
+    yaw: float = 0.0               # inlet yaw angle BS, degrees
+    pitch: float = 0.0             # inlet meridional pitch angle BR, degrees
+    pd_exit: Optional[tuple] = None   # IPOUT = 3: exit static pressure at the KIN inlet-data points
+    nout: Optional[tuple] = None   # the 5 printout steps (default: past the run)
     pdown_hub: float = 0.8e5
     pdown_tip: float = 0.8e5
     fp_ratio: float = 1.0          # geometric expansion of FP towards mid-pitch
@@ -262,7 +267,7 @@ def write_deck(spec: DeckSpec) -> str:
     a("".join(f"{1 + n // 2:10d}" for n in range(len(spec.rows))))
     a("  5  TIME STEPS FOR PRINTOUT")
     big = spec.nsteps + 10
-    a("".join(f"{big:10d}" for _ in range(5)))
+    a("".join(f"{n:10d}" for n in (spec.nout or (big,) * 5)))
     a("  MARKER FOR VARIABLES TO BE SENT TO THE OUTPUT FILE.")
     a(" " + _ints([0] * 13))
     a("  STREAM SURFACES ON WHICH RESULTS ARE TO BE SENT TO   THE OUTPUT FILE")
@@ -333,16 +338,19 @@ def write_deck(spec: DeckSpec) -> str:
     a(_fmt([1.0] * (kin - 1)))
     a("   INLET STAGNATION PRESSURES")
     a(_fmt([spec.po1] * kin, 8, "{:12.1f}"))
+    if spec.ipout == 3:
+        a("   EXIT STATIC PRESSURES")
+        a(_fmt(spec.pd_exit, 8, "{:12.1f}"))
     a("   INLET STAGNATION TEMPERATURES")
     a(_fmt([spec.to1] * kin, 8, "{:12.3f}"))
     a("   INLET TANGENTIAL VELOCITY")
-    a(_fmt([0.0] * kin, 8, "{:12.4f}"))
+    a(_fmt([spec.vtin] * kin, 8, "{:12.4f}"))
     a("   INLET MERIDIONAL VELOCITY")
     a(_fmt([spec.vm1] * kin, 8, "{:12.4f}"))
     a("   INLET YAW ANGLE")
-    a(_fmt([0.0] * kin, 8, "{:12.4f}"))
+    a(_fmt([spec.yaw] * kin, 8, "{:12.4f}"))
     a("   INLET PITCH ANGLE")
-    a(_fmt([0.0] * kin, 8, "{:12.4f}"))
+    a(_fmt([spec.pitch] * kin, 8, "{:12.4f}"))
     a("   PDOWN_HUB   PDOWN_TIP")
     a(f"{spec.pdown_hub:12.1f}{spec.pdown_tip:12.1f}")
     if spec.ilos != 0:
@@ -601,3 +609,76 @@ def deck_speedline_point(i: int, n_points: int = 16, nsteps: int = 3000, seed: i
                     im=im, km=km, rows=[rotor, stator], r_hub=0.25, r_tip=0.35,
                     ilos=10, nsteps=nsteps, conlim=0.005, nchange=500, seed=seed + i,
                     po1=po1, to1=288.0, vm1=150.0, pdown_hub=pexit, pdown_tip=pexit)
+
+
+def deck_tiny_bc(variant: str, nsteps: int = 100) -> DeckSpec:
+    """Tiny mixing-length cascades for the inlet and exit condition options
+    (LOOP, multall-open-18.3.f:4506-4757):
+
+    - ``bc1``: the reference's Rotor 37 test deck's choice: exit pressure
+      held at its set-up values (IPOUT = 1), inlet pressure extrapolated
+      from the pitch-averaged pressures of J = 2 and 3 (IN_PRESS = -3) and
+      the inlet radial velocity taken from J = 2 (IN_VR = -3), 10 degrees of
+      inlet yaw;
+    - ``bc2``: radial equilibrium shifted to the casing pressure
+      (IPOUT = -1, with hub and casing pressures apart), the inlet pressure
+      extrapolated node by node (IN_PRESS = 1), a fixed inlet tangential
+      velocity (IN_VTAN = 1) and a 5 degree inlet pitch angle (IN_VR = 1);
+    - ``bc3``: a rotor first (rotor and stator), a spanwise exit pressure
+      profile from the deck (IPOUT = 3), the inlet pressure extrapolated from
+      mid-pitch, mid-span (IN_PRESS = 4), a fixed relative inlet flow angle
+      against the blade speed at J = 1 (IN_VTAN = 2) and the inlet radial
+      velocity from J = 2 (IN_VR = 0)."""
+    if variant == "bc3":
+        spec = deck_tiny(ilos=10, nrows=2, nsteps=nsteps)
+        rot, sta = spec.rows
+        rot.name, rot.nblades, rot.alpha_in, rot.alpha_out, rot.rpm = "ROTOR", 48, -45.0, -20.0, 6000.0
+        sta.name, sta.nblades, sta.alpha_in, sta.alpha_out, sta.rpm = "STATOR", 40, 30.0, 0.0, 0.0
+        rot.pguess = (1.0e5, 0.97e5, 0.93e5, 0.92e5)
+        sta.pguess = (0.92e5, 0.91e5, 0.90e5, 0.90e5)
+        spec.ipout, spec.pd_exit = 3, (0.89e5, 0.91e5)
+        spec.in_press, spec.in_vtan, spec.in_vr, spec.yaw = 4, 2, 0, 5.0
+        spec.pdown_hub = spec.pdown_tip = 0.9e5
+    else:
+        spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+        if variant == "bc1":
+            spec.ipout, spec.in_press, spec.in_vr, spec.yaw = 1, -3, -3, 10.0
+        elif variant == "bc2":
+            spec.ipout, spec.in_press, spec.in_vtan, spec.vtin, spec.pitch = -1, 1, 1, 20.0, 5.0
+            spec.pdown_hub, spec.pdown_tip = 0.79e5, 0.81e5
+        else:
+            raise ValueError(variant)
+    spec.title = (f"TINY CASCADE ILOS=10 IPOUT={spec.ipout} IN_PRESS={spec.in_press} "
+                  f"IN_VTAN={spec.in_vtan} IN_VR={spec.in_vr}")
+    return spec
+
+
+def deck_tiny_supersonic(nsteps: int = 100) -> DeckSpec:
+    """A three-row inviscid cascade whose first stator turns the flow to 70
+    degrees at a high subsonic exit Mach number (low guessed and imposed exit
+    pressures) ahead of a rotor turning against it (negative speed), so the
+    flow relative to the rotor is supersonic at JMIX + 2 and NEW_MIXPLAN's
+    Prandtl-Meyer angle sweeps (multall-open-18.3.f:15139-15182) run."""
+    spec = deck_tiny(ilos=0, nrows=3, nsteps=nsteps)
+    s1, r1, s2 = spec.rows
+    s1.alpha_out = 70.0
+    s1.pguess = (1.0e5, 0.95e5, 0.45e5, 0.42e5)
+    r1.alpha_in, r1.alpha_out, r1.rpm = 65.0, 30.0, -6000.0
+    r1.pguess = (0.42e5, 0.41e5, 0.36e5, 0.35e5)
+    s2.alpha_in, s2.alpha_out = 20.0, 40.0
+    s2.pguess = (0.35e5, 0.35e5, 0.34e5, 0.34e5)
+    spec.pdown_hub = spec.pdown_tip = 0.34e5
+    spec.title = "TINY CASCADE INVISCID SUPERSONIC MIXING PLANES"
+    return spec
+
+
+def deck_tiny_nout(nsteps: int = 100) -> DeckSpec:
+    """The three-row mixing-length cascade with printout steps inside the
+    run: NOUT = 10, 25, 40 and 55 land on convergence checks, so the
+    reference refreshes the viscous forces there
+    (multall-open-18.3.f:5338-5341); NOUT = 73 does not (the printout block
+    runs only on the every-5-step check)."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.title = "TINY CASCADE ILOS=10 PRINTOUT AT STEPS 10 25 40 55 73"
+    spec.nout = (10, 25, 40, 55, 73)
+    return spec

@@ -136,6 +136,8 @@ def load_deck(path: str, xlength_on_host: bool = True) -> Dict:
         st[n] = arr(n, (sc["NROWS"],), np.float32)
     for n in RNI:
         st[n] = arr(n, (sc["NROWS"],), np.int32)
+    st["NOUT"] = arr("NOUT", (5,), np.int32)     # the printout steps
+
     # the turbulence state SETUP starts from (mf:6903-6909); the viscous
     # models' first call (mh_loss_init) fills in the rest
     st["Y_PLUS"] = np.full((km, jm, im), 25.0, np.float32)

@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmultall_hip.so")
 EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar_id", "mh_array_len",
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
            "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_set_xlength", "mh_start", "mh_step",
-           "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch"]
+           "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch",
+           "mh_request_stop"]
 
 
 class MhDims(ctypes.Structure):
@@ -73,6 +74,7 @@ def load(path: str = LIB_PATH):
     L.mh_loss_init.argtypes = [vp]
     L.mh_set_xlength.argtypes = [vp] + [ctypes.c_void_p] * 5
     L.mh_step.argtypes = [vp, i, ctypes.POINTER(MhResid)]
+    L.mh_request_stop.argtypes = [vp]
     L.mh_step_batch.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(MhResid), ctypes.POINTER(i)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
     L.mh_sync.argtypes = [vp]
@@ -145,6 +147,10 @@ class Solver:
         r = MhResid()
         self._check(self.L.mh_step(self.h, int(n), ctypes.byref(r)), "mh_step")
         return r.as_dict()
+
+    def request_stop(self):
+        """The `stopit` request: the next step divisible by 10 ends the run."""
+        self._check(self.L.mh_request_stop(self.h), "mh_request_stop")
 
     def get(self, name: str, shape=None) -> np.ndarray:
         aid = self.L.mh_array_id(name.encode())

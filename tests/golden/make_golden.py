@@ -6,7 +6,9 @@ IM = 80 deck on its pitchwise-wide build) and stores, per case:
 
   <case>.dat          the NEW-format input deck
   <case>_setup.npz    COMMON state after the reference's SETUP (dense arrays
-                      plus a JSON list of scalars): the input of every test
+                      plus a JSON list of scalars, and the printout steps
+                      NOUT where they fall inside the run): the input of
+                      every test
   <case>_ref<N>.npz   conserved variables, SSS histories and time steps after
                       N time steps of the reference (N = 1 and 100)
 
@@ -29,6 +31,9 @@ from multall_hip import deckgen  # noqa: E402
 
 OUT_VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP",
             "XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
+# ROVT as the reference leaves it (refreshed from RORVT at the start of the
+# last step, mf:3833): the angular-momentum record of its flow_out
+OUT_VARS_LATER = ["ROVT"]
 SA_VARS = ["TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "Y_PLUS"]   # ILOS >= 200 decks
 
 CASES = {
@@ -50,6 +55,11 @@ CASES = {
     "tiny3_nl3": lambda n: deckgen.deck_tiny_wallfun(ilos=100, nsteps=n),
     "tiny3_sa3": lambda n: deckgen.deck_tiny_wallfun(ilos=200, nsteps=n),
     "tiny3_gap": lambda n: deckgen.deck_tiny_gap(nsteps=n),
+    "tiny3_bc1": lambda n: deckgen.deck_tiny_bc("bc1", nsteps=n),
+    "tiny3_bc2": lambda n: deckgen.deck_tiny_bc("bc2", nsteps=n),
+    "tiny2_bc3": lambda n: deckgen.deck_tiny_bc("bc3", nsteps=n),
+    "tiny3_sup": lambda n: deckgen.deck_tiny_supersonic(nsteps=n),
+    "tiny3_nout": lambda n: deckgen.deck_tiny_nout(nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}
@@ -99,7 +109,10 @@ def make(case):
             shutil.rmtree(wd, ignore_errors=True)
         st = orc.from_dump(setup)
         st["scalars"].pop("NSTEPS_MAX")
-        st.pop("NOUT")
+        # the printout steps matter only where a deck sets them inside the
+        # run (the others put them just past NSTEPS_MAX, which differs by run)
+        if spec.nout is None:
+            st.pop("NOUT")
         if setup_ref is None:
             setup_ref = st
             with open(os.path.join(HERE, "decks", f"{case}.dat"), "w") as f:
@@ -111,7 +124,7 @@ def make(case):
                     assert v == setup_ref["scalars"], case
                 else:
                     assert np.array_equal(v, setup_ref[k]), (case, k)
-        names = OUT_VARS + (SA_VARS if st["scalars"]["ILOS"] >= 200 else [])
+        names = OUT_VARS + OUT_VARS_LATER + (SA_VARS if st["scalars"]["ILOS"] >= 200 else [])
         if st["scalars"]["YPLUSWALL"] < -0.001:
             names = names + ["DPDS_CELL"] + (["Y_PLUS"] if st["scalars"]["ILOS"] < 200 else [])
         out = {k: final.node(k).astype(np.float32) for k in names}

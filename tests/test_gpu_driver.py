@@ -3,9 +3,13 @@ with LOOP replaced by the library, the deck on standard input) on a
 three-row mixing-length deck: it reads and sets up the deck (the wall
 distances formed on the device by mh_set_xlength), marches the deck's 100
 steps in chunks of 5 and writes flow_out in the reference's record layout
-(multall-open-18.3.f:5951-5976).  The density and energy it writes match the
-reference's own state after 100 steps within the 100-step bound of the
-parity tests."""
+(multall-open-18.3.f:5951-5976).  The density, energy and angular momentum
+it writes match the reference's own state after 100 steps within the
+100-step bound of the parity tests.  The angular-momentum record is ROVT as
+the reference leaves it: refreshed from RORVT at the start of the last step
+(mf:3833) and then not updated by that step's TSTEP, so it lags the marched
+RORVT by one step away from the inlet and the limiters; the test checks both
+that it matches the reference's ROVT and that it differs from RORVT/R."""
 import os
 import subprocess
 
@@ -39,10 +43,18 @@ def test_driver_program_marches_a_deck(tmp_path):
     ref = load_ref("tiny3_ml", 100)
     shape = ref["RO"].shape
     got = {"RO": np.frombuffer(rec[1], np.float32).reshape(shape),
-           "ROE": np.frombuffer(rec[5], np.float32).reshape(shape)}
+           "ROE": np.frombuffer(rec[5], np.float32).reshape(shape),
+           "ROVT": np.frombuffer(rec[4], np.float32).reshape(shape)}
     sc = _scales(ref)
+    sc["ROVT"] = float(np.abs(ref["ROVT"]).max())
     for v, a in got.items():
         err = float(np.abs(a.astype(np.float64) - ref[v]).max() / sc[v])
         print(f"driver {v}: normalized max err {err:.2e} after 100 steps")
         assert err <= TOL_100, (v, err)
+    # the stale record: one step behind the marched angular momentum
+    r = np.load(os.path.join(GOLDEN, "tiny3_ml_setup.npz"))["R"]      # (KM, JM)
+    marched = ref["RORVT"].astype(np.float64) / r[:, :, None]
+    lag = float(np.abs(marched - ref["ROVT"])[:, 2:-2, :].max() / sc["ROVT"])
+    print(f"ROVT lags RORVT/R by {lag:.2e} of its scale")
+    assert lag > 10 * TOL_100
     assert (tmp_path / "stage.log").exists()

@@ -27,15 +27,21 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 FORCES = ["XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
 TOL = 1e-5
 TOL_100 = 2.5e-5
-INVISCID = ["tiny1_inv", "tiny3_inv"]
+# inviscid decks; the last one's flow relative to the second row is
+# supersonic at the first mixing plane, so NEW_MIXPLAN's Prandtl-Meyer angle
+# sweeps run
+INVISCID = ["tiny1_inv", "tiny3_inv", "tiny3_sup"]
 # mixing-length (LOSS) decks: chord-Reynolds and power-law viscosity, smooth
 # and rough walls, rotor tip gap, fixed wall y+ with a shear-free hub, exit
 # smoothing on the last three J planes, IM = 72, the SSS scheme with
 # residual smoothing (ITIMST = -3), mass-flow forcing towards the mean flow
-# (IN_FLOW = 2) and towards a fixed flow (IN_FLOW = 3), and the Shih wall
-# functions without the pressure-gradient term (YPLUSWALL = -6)
+# (IN_FLOW = 2) and towards a fixed flow (IN_FLOW = 3), the Shih wall
+# functions without the pressure-gradient term (YPLUSWALL = -6), re-gridded
+# gaps, the inlet and exit condition options (IPOUT = 1, -1, 3; IN_PRESS =
+# -3, 1, 4; IN_VTAN = 1, 2; IN_VR = -3, 0; inlet yaw and pitch) and printout
+# steps inside the run (viscous refreshes at NOUT)
 MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide", "tiny3_sss", "tiny3_flow2",
-                 "tiny3_flow3", "tiny3_ml3", "tiny3_gap"]
+                 "tiny3_flow3", "tiny3_ml3", "tiny3_gap", "tiny3_bc1", "tiny3_bc2", "tiny2_bc3", "tiny3_nout"]
 # Spalart-Allmaras decks: rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing, mixing-length relaxation source; the Rotor 37 test
 # deck's options (Shih wall functions with the pressure-gradient term,

@@ -8,11 +8,15 @@ Fortran host, whose set-up equals the reference's bit for bit
 (test_host_setup.py; the six-stage compressor to the last bits of its last
 wake line).  The library's SETUP-time
 viscous call (mh_loss_init) forms the forces and turbulence state; the
-oracle starts from that same state, and one time step in the reference's
-statement order (exact-order mode) is compared: conserved variables,
-changes and time steps within 2e-6 of each variable's largest magnitude (the
-transcendentals device and host libm round differently), the forces that
-the run's closing viscous call refreshes within 1e-5.
+oracle makes the same call itself (LOSS, NEW_LOSS or SPAL_LOSS with
+RF_VIS = 1, FMIXUP = 1, NSTEP = 1, as SETUP does, mf:7346-7357) and the two
+are compared within 1e-5 of each array's largest magnitude.  Then, to test
+the step apart from the viscous call, the oracle takes the library's
+post-call state and one time step in the reference's statement order
+(exact-order mode) is compared: conserved variables, changes and time steps
+within 2e-6 of each variable's largest magnitude (the transcendentals device
+and host libm round differently), the forces that the run's closing viscous
+call refreshes within 1e-5.
 """
 import os
 
@@ -43,6 +47,20 @@ def test_reference_deck_one_step_vs_oracle(deck):
     ost = orc.with_workspace(dict(st))
     if sc["ILOS"] != 0:
         g.loss_init()
+        own = orc.Oracle(ost)
+        for k, v in (("RF_VIS", 1.0), ("FMIXUP", 1.0), ("NSTEP", 1)):
+            own.lib.orc_scalar(own.h, k.encode(), float(v))
+        if sc["ILOS"] >= 200:
+            own.lib.orc_spal_loss(own.h)
+        elif sc["ILOS"] >= 100:
+            own.lib.orc_new_loss(own.h)
+        else:
+            own.lib.orc_loss(own.h)
+        outs = FORCES + (["TRANS_DYN_VIS", "TRANS_KVIS"] if sc["ILOS"] >= 200 else [])
+        for v in outs:
+            err = _nerr(g.get(v), own[v])
+            print(f"{deck} set-up call {v}: normalized max err {err:.2e}")
+            assert err <= 1e-5, (deck, v, err)
         for n in LOSS_OUT:
             if n in ost:
                 ost[n] = g.get(n).reshape(np.shape(ost[n])).astype(np.float32)

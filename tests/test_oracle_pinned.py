@@ -18,13 +18,18 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 # fixed wall y+, a shear-free hub, exit smoothing, IM = 72, the SSS scheme
 # with residual smoothing (ITIMST = -3) and mass-flow forcing (IN_FLOW = 2
 # and 3), a deck whose inter-row gaps the reader re-grids (ISHIFT = 2, new
-# trailing-edge cusps, input grid angles); NEW_LOSS decks
+# trailing-edge cusps, input grid angles); the inlet and exit condition
+# options (IPOUT = 1, -1, 3; IN_PRESS = -3, 1, 4; IN_VTAN = 1, 2; IN_VR = -3,
+# 0; inlet yaw and pitch angles), an inviscid deck whose flow relative to the
+# second row is supersonic at the first mixing plane (NEW_MIXPLAN's
+# Prandtl-Meyer sweeps) and a deck with a printout step inside the run (the
+# viscous refresh at a printout step); NEW_LOSS decks
 # (tip gap; power-law viscosity, rough walls, shear-free hub and transition
 # points with a threshold that leaves part of the flow laminar); and the
 # Spalart-Allmaras decks below
 CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide",
          "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3", "tiny3_ml3", "tiny3_nl3", "tiny3_sa3",
-         "tiny3_gap"]
+         "tiny3_gap", "tiny3_bc1", "tiny3_bc2", "tiny2_bc3", "tiny3_sup", "tiny3_nout"]
 # Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing and the mixing-length relaxation source) also pin the
 # turbulence state
