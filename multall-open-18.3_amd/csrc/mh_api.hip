@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -156,6 +157,27 @@ namespace {
 
 constexpr int kRedBlocks = 1024;
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
+
+// The dynamic-LDS limit is a per-function, process-wide attribute.  Each
+// kernel's limit only grows: a context raises it to what it launches with,
+// and a later context with smaller planes never lowers it below what an
+// earlier one still needs.
+std::mutex g_lds_mu;
+std::vector<std::pair<const void *, size_t>> g_lds_limit;
+
+int raise_lds_limit(const void *fn, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_lds_mu);
+  for (auto &e : g_lds_limit)
+    if (e.first == fn) {
+      if (bytes <= e.second) return 0;
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 1;
+      e.second = bytes;
+      return 0;
+    }
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return 1;
+  g_lds_limit.push_back({fn, bytes});
+  return 0;
+}
 
 long long kind_len(const mh_dims &d, Kind k) {
   switch (k) {
@@ -1100,15 +1122,9 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
   if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1) > kMaxLds) return bail(MH_E_UNSUPPORTED);
-  // The dynamic-LDS limit is a per-function, process-wide attribute: set it
-  // once to the largest size any context may ask for, so a later context
-  // never lowers it below what an earlier one launches with.
-  static const bool lds_ok =
-      hipFuncSetAttribute((const void *)k_smooth_plane, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) ==
-          hipSuccess &&
-      hipFuncSetAttribute((const void *)k_damp_scatter4, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) ==
-          hipSuccess;
-  if (!lds_ok) return bail(MH_E_HIP);
+  if (raise_lds_limit((const void *)k_smooth_plane, c->lds_plane) ||
+      raise_lds_limit((const void *)k_damp_scatter4, scatter_lds_bytes(dm.im - 1)))
+    return bail(fail(c, MH_E_HIP, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
   *out = c;
   return MH_OK;
 }

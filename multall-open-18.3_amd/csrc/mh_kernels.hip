@@ -388,7 +388,10 @@ __global__ void __launch_bounds__(64) k_bc_lines(Geo g, Dev d, Params p) {
 // with the running value in a register, reading only LDS, while thread 64
 // forms the SA inlet density of the mid-passage node.  The reference resets
 // PD(1) to PDOWN_HUB on every pass of its K loop, so after the loop PD(1)
-// is PDOWN_HUB and the K = 2 update reads it without the IPOUT = -1 shift.
+// is PDOWN_HUB and the K = 2 update reads it without the IPOUT = -1 shift;
+// but the exit nodes of K = 1 are set inside the K = 1 pass, when PD(1)
+// still holds PDOWN_HUB + PDIFF.  That value goes to d.BCW[3 KM + 2] for
+// k_bc_apply.
 constexpr int kBcMaxK = 512;
 __global__ void __launch_bounds__(128) k_bc_pd(Geo g, Dev d, Params p, StepScalars s) {
   __shared__ float sumk[kBcMaxK], rjm[kBcMaxK], pdold[kBcMaxK], pdnew[kBcMaxK];
@@ -414,6 +417,9 @@ __global__ void __launch_bounds__(128) k_bc_pd(Geo g, Dev d, Params p, StepScala
       prev = v;
     }
     pdnew[0] = (KM >= 2) ? p.PDOWN_HUB : p.PDOWN_HUB + PDIFF;
+    d.BCW[3 * KM + 2] = p.PDOWN_HUB + PDIFF;
+  } else if (threadIdx.x == 0) {
+    d.BCW[3 * KM + 2] = pdold[0];
   }
   if (threadIdx.x == 64 && p.ILOS >= 200) {
     // RO(IMID,1,KMID) before and after its inlet update (the SA inlet
@@ -500,7 +506,8 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
     const size_t nm = IX(I, JM, K);
     float vr = d.VR[nm], vt = d.VT[nm], vx = d.VX[nm];
     float EKE = .5f * (vr * vr + vt * vt + vx * vx);
-    float PEX = PDL[K - 1] + p.FEXTRAP * (d.P[IX(I, JMM1, K)] - PAVGOUT[K - 1]);
+    const float pdk = K == 1 ? d.BCW[3 * g.KM + 2] : PDL[K - 1];  // PD(1) as the K = 1 pass leaves it
+    float PEX = pdk + p.FEXTRAP * (d.P[IX(I, JMM1, K)] - PAVGOUT[K - 1]);
     float HOEX = 2.0f * d.HO[IX(I, JMM1, K)] - d.HO[IX(I, JM - 2, K)];
     float TEXIT = (HOEX - EKE) / p.CP;
     float roex = PEX / p.RGAS / TEXIT;

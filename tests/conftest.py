@@ -13,6 +13,16 @@ for p in (os.path.join(ROOT, "multall-open-18.3_amd"), os.path.join(ROOT, "oracl
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# The long comparison runs 100 steps, except on the supersonic cascade: there
+# a one-ulp change of any initial value grows to ~1e-3 of the solution
+# between steps 20 and 30 (a sonic switch flips at one node), so it is
+# compared after 20 steps (tests/golden/make_golden.py STEPS_OF).
+LONG_STEPS = {"tiny3_sup": 20}
+
+
+def long_steps(case):
+    return LONG_STEPS.get(case, 100)
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")

@@ -10,7 +10,8 @@ IM = 80 deck on its pitchwise-wide build) and stores, per case:
                       NOUT where they fall inside the run): the input of
                       every test
   <case>_ref<N>.npz   conserved variables, SSS histories and time steps after
-                      N time steps of the reference (N = 1 and 100)
+                      N time steps of the reference (N = 1 and 100, or the
+                      STEPS_OF pair)
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
@@ -64,6 +65,11 @@ CASES = {
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}
 STEPS = (1, 100)
+# the supersonic cascade is compared after 20 steps: between steps 20 and
+# 30 a one-ulp change of any single initial value grows to ~1e-3 of the
+# solution (a limiter or sonic switch flips at one node), so agreement after
+# 100 steps would mean nothing there
+STEPS_OF = {"tiny3_sup": (1, 20)}
 
 
 def save_state(path, st):
@@ -98,7 +104,8 @@ def make(case):
     import shutil
     import refdump, orc
     setup_ref = None
-    for n in STEPS:
+    steps = STEPS_OF.get(case, STEPS)
+    for n in steps:
         spec = CASES[case](n)
         txt = deckgen.write_deck(spec)
         wd = tempfile.mkdtemp(prefix="mhgold_")
@@ -116,7 +123,7 @@ def make(case):
         if setup_ref is None:
             setup_ref = st
             with open(os.path.join(HERE, "decks", f"{case}.dat"), "w") as f:
-                f.write(deckgen.write_deck(CASES[case](max(STEPS))))
+                f.write(deckgen.write_deck(CASES[case](max(steps))))
             save_state(os.path.join(HERE, f"{case}_setup.npz"), st)
         else:
             for k, v in st.items():

@@ -21,7 +21,7 @@ import numpy as np
 import pytest
 
 import orc
-from conftest import load_ref, load_state
+from conftest import load_ref, load_state, long_steps
 
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP"]
 FORCES = ["XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
@@ -142,6 +142,7 @@ def test_loss_call_vs_oracle(case):
 @pytest.mark.parametrize("case", INVISCID + VISCOUS)
 @pytest.mark.parametrize("nsteps", [1, 100])
 def test_vs_reference(case, nsteps, exact):
+    nsteps = long_steps(case) if nsteps == 100 else nsteps
     st = load_state(case)
     st["scalars"]["NSTEPS_MAX"] = nsteps
     g = _solver(st, exact=exact)
@@ -161,7 +162,7 @@ def test_vs_reference(case, nsteps, exact):
         err = _nerr(g.get(v), ref[v])
         print(f"{case} {nsteps} exact={exact} {v}: normalized max err {err:.2e}")
         assert err <= tol, (v, err)
-    if nsteps == 100:
+    if nsteps > 1:
         # SUMFLX's per-J integrals against what the reference's EFICOOL wrote
         # at the end of the same run (FP64 here, FP32 sequential there).
         # The entropy function
@@ -184,13 +185,15 @@ def test_vs_reference(case, nsteps, exact):
 def test_host_path_vs_reference(case):
     """The whole drop-in path: the deck read and set up by the Fortran host,
     the SETUP-time LOSS call on the GPU (forces against the reference's
-    post-SETUP state), then 100 steps against the reference."""
+    post-SETUP state), then 100 steps (the supersonic cascade 20) against
+    the reference."""
     import os
     from conftest import GOLDEN
     from multall_hip import host
     from multall_hip.runtime import Solver
     st = host.load_deck(os.path.join(GOLDEN, "decks", f"{case}.dat"))
-    assert st["scalars"]["NSTEPS_MAX"] == 100
+    n = long_steps(case)
+    assert st["scalars"]["NSTEPS_MAX"] == n
     g = Solver(st, exact_order=False)
     g.loss_init()
     ref0 = load_state(case)
@@ -200,10 +203,10 @@ def test_host_path_vs_reference(case):
         assert err <= 1e-5, (v, err)
     g.start()
     r = g.step(110)
-    assert r["nstep"] == 100 and r["ifend"] == 1
-    ref = load_ref(case, 100)
+    assert r["nstep"] == n and r["ifend"] == 1
+    ref = load_ref(case, n)
     sc = _scales(ref)
     for v in ["RO", "ROVX", "ROVR", "RORVT", "ROE"]:
         err = _nerr(g.get(v), ref[v], sc[v])
-        print(f"{case} host path 100 steps {v}: scaled max err {err:.2e}")
+        print(f"{case} host path {n} steps {v}: scaled max err {err:.2e}")
         assert err <= TOL_100, (v, err)

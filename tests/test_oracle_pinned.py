@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import orc
-from conftest import load_ref, load_state
+from conftest import load_ref, load_state, long_steps
 
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "STEP",
         "XFORCE", "RFORCE", "TFORCE", "QSOURCE"]
@@ -42,6 +42,7 @@ WALL_VARS = ["DPDS_CELL", "YPLUS_I1", "YPLUS_IM", "YPLUS_K1", "YPLUS_KM"]
 @pytest.mark.parametrize("case", CASES)
 @pytest.mark.parametrize("nsteps", [1, 100])
 def test_oracle_matches_reference_bitwise(case, nsteps):
+    nsteps = long_steps(case) if nsteps == 100 else nsteps
     st = load_state(case)
     st["scalars"]["NSTEPS_MAX"] = nsteps
     o = orc.Oracle(st)
