@@ -200,6 +200,22 @@ int mh_loss_init(mh_ctx *ctx);
 int mh_set_xlength(mh_ctx *ctx, const float *theta, const float *xllim_in, const float *xllim_le,
                    const float *xllim_te, const float *xllim_dn);
 
+/* mh_set_xlength without allocating: the same computation (the same
+ * kernels, bit-identical results), its work buffers in a file-scope device
+ * array of the library instead of device memory allocated per call.  No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (under HIP graph capture, in a serving loop, in harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc/hipFree.  Limits: at most
+ * MH_XLENGTH_SCRATCH_NODES nodes (IM*JM*KM) and MH_XLENGTH_SCRATCH_J
+ * streamwise points; larger grids get MH_E_ARG (use mh_set_xlength).  The
+ * array (about 32 MiB) is held on each device the library runs on until the
+ * process ends and is shared by all contexts on that device: calls that
+ * overlap on one GPU race, so the caller serialises them. */
+#define MH_XLENGTH_SCRATCH_NODES 4194304
+#define MH_XLENGTH_SCRATCH_J 65536
+int mh_set_xlength_noalloc(mh_ctx *ctx, const float *theta, const float *xllim_in, const float *xllim_le,
+                           const float *xllim_te, const float *xllim_dn);
+
 /* LOOP prologue (mf:3674-3714).  Call once after all inputs are set. */
 int mh_start(mh_ctx *ctx);
 /* Run up to nsteps time steps (fewer if the run ends).  *out (optional)

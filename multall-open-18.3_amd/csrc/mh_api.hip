@@ -153,9 +153,38 @@ const char *const kScalarNames[MH_NSCALARS] = {
 
 using namespace mh;
 
+// The scratch array of mh_set_xlength_noalloc (see scratch_address below).
+constexpr size_t kXlRowSlots = 4 * 64;
+constexpr size_t kXlScratchFloats = 2 * (size_t)MH_XLENGTH_SCRATCH_NODES + kXlRowSlots + MH_XLENGTH_SCRATCH_J;
+static __device__ __align__(256) float scratch[kXlScratchFloats];
+
 namespace {
 
 constexpr int kRedBlocks = 1024;
+
+// Scratch of mh_set_xlength_noalloc, one array per device, held for the
+// life of the process: the nodal THETA and wall distance (NODES floats
+// each), the four per-row limits (kXlRowSlots = 4 x 64 rows) and the per-J
+// limit (J floats), about 32 MiB.  Host code never passes the array's name
+// to a kernel (it is a host-side stand-in there): it takes the device
+// address hipGetSymbolAddress gives, per device, at mh_create.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch`, looked up once per device (with that
+// device current, as it is on the calling thread).
+int scratch_address(int dev, float **out) {
+  if (dev < 0 || dev >= kMaxDevices) return MH_E_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MH_E_HIP;
+    g_scratch_ptr[dev] = (float *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MH_OK;
+}
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 
 // The dynamic-LDS limit is a per-function, process-wide attribute.  Each
@@ -1070,6 +1099,10 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
     return MH_E_ARG;
   mh_ctx *c = new mh_ctx();
   c->dims = dm;
+  if (hipGetDevice(&c->device) != hipSuccess || scratch_address(c->device, &c->xl_scratch) != MH_OK) {
+    delete c;
+    return MH_E_HIP;
+  }
   Geo &g = c->g;
   g.IM = dm.im; g.JM = dm.jm; g.KM = dm.km; g.IMM1 = dm.im - 1; g.JMM1 = dm.jm - 1; g.KMM1 = dm.km - 1;
   g.NROWS = dm.nrows; g.NIB1 = dm.nib1; g.NJB1 = dm.njb1; g.NKB1 = dm.nkb1; g.NIB2 = dm.nib2; g.NJB2 = dm.njb2;
@@ -1248,18 +1281,43 @@ int mh_loss_init(mh_ctx *c) {
   return MH_OK;
 }
 
-// SET_XLENGTH on the device.  The work buffers (the nodal wall distance, the
-// per-row limits, the per-J limit) live for this call only.
+// SET_XLENGTH's launches on work buffers the caller provides: th (the nodal
+// THETA, n floats), dist (the nodal wall distance, n), xl4 (the four per-row
+// limits, 4 NROWS) and xlim (the per-J limit, JM).  Every kernel writes each
+// work value before it reads it, so the buffers need no clearing.
+static int xlength_run(mh_ctx *c, float *th, float *dist, float *xl4, float *xlim, const float *theta,
+                       const float *xllim_in, const float *xllim_le, const float *xllim_te, const float *xllim_dn) {
+  const Params p = params_of(c);
+  const Geo g = c->g;
+  const size_t n = (size_t)g.IM * g.JM * g.KM;
+  const int nr = g.NROWS;
+  const hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(th, theta, n * 4, hipMemcpyHostToDevice, st));
+  const float *rows[4] = {xllim_in, xllim_le, xllim_te, xllim_dn};
+  for (int q = 0; q < 4; q++)
+    HIPCHK(hipMemcpyAsync(xl4 + (size_t)q * nr, rows[q], (size_t)nr * 4, hipMemcpyHostToDevice, st));
+  const Dev &d = c->d;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  LAUNCH(k_xl_dist, dim3(nb), dim3(256), 0, st, g, d, th, dist, p.IBOUND);
+  LCHK("xl_dist");
+  LAUNCH(k_xl_limit, dim3((g.JM + 63) / 64), dim3(64), 0, st, g, d, dist, xl4, xlim);
+  LCHK("xl_limit");
+  const float dblend = 0.6666666f, expon = dblend / (1.0f - dblend), fblend = (1.0f - dblend) * powf(dblend, expon);
+  LAUNCH(k_xl_cells, dim3(nb), dim3(256), 0, st, g, d, dist, xlim, dblend, expon, fblend);
+  LCHK("xl_cells");
+  HIPCHK(hipStreamSynchronize(st));
+  return MH_OK;
+}
+
+// SET_XLENGTH on the device.  The work buffers live for this call only.
 int mh_set_xlength(mh_ctx *c, const float *theta, const float *xllim_in, const float *xllim_le,
                    const float *xllim_te, const float *xllim_dn) {
   if (!c) return MH_E_ARG;
   if (!theta || !xllim_in || !xllim_le || !xllim_te || !xllim_dn) return fail(c, MH_E_ARG, "null argument");
   int r = check_inputs(c);
   if (r) return r;
-  const Params p = params_of(c);
   const Geo g = c->g;
   const size_t n = (size_t)g.IM * g.JM * g.KM;
-  const int nr = g.NROWS;
   struct Bufs {
     float *th = nullptr, *dist = nullptr, *xl4 = nullptr, *xlim = nullptr;
     ~Bufs() {
@@ -1271,24 +1329,31 @@ int mh_set_xlength(mh_ctx *c, const float *theta, const float *xllim_in, const f
   } b;
   HIPCHK(hipMalloc(&b.th, n * 4));
   HIPCHK(hipMalloc(&b.dist, n * 4));
-  HIPCHK(hipMalloc(&b.xl4, 4 * (size_t)nr * 4));
+  HIPCHK(hipMalloc(&b.xl4, 4 * (size_t)g.NROWS * 4));
   HIPCHK(hipMalloc(&b.xlim, (size_t)g.JM * 4));
-  const hipStream_t st = c->stream;
-  HIPCHK(hipMemcpyAsync(b.th, theta, n * 4, hipMemcpyHostToDevice, st));
-  const float *rows[4] = {xllim_in, xllim_le, xllim_te, xllim_dn};
-  for (int q = 0; q < 4; q++)
-    HIPCHK(hipMemcpyAsync(b.xl4 + (size_t)q * nr, rows[q], (size_t)nr * 4, hipMemcpyHostToDevice, st));
-  const Dev &d = c->d;
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  LAUNCH(k_xl_dist, dim3(nb), dim3(256), 0, st, g, d, b.th, b.dist, p.IBOUND);
-  LCHK("xl_dist");
-  LAUNCH(k_xl_limit, dim3((g.JM + 63) / 64), dim3(64), 0, st, g, d, b.dist, b.xl4, b.xlim);
-  LCHK("xl_limit");
-  const float dblend = 0.6666666f, expon = dblend / (1.0f - dblend), fblend = (1.0f - dblend) * powf(dblend, expon);
-  LAUNCH(k_xl_cells, dim3(nb), dim3(256), 0, st, g, d, b.dist, b.xlim, dblend, expon, fblend);
-  LCHK("xl_cells");
-  HIPCHK(hipStreamSynchronize(st));
-  return MH_OK;
+  return xlength_run(c, b.th, b.dist, b.xl4, b.xlim, theta, xllim_in, xllim_le, xllim_te, xllim_dn);
+}
+
+// The same on this device's file-scope scratch array: no allocation.  The
+// array is shared by every context on the device, so overlapping calls on
+// one GPU race (the caller serialises them).
+int mh_set_xlength_noalloc(mh_ctx *c, const float *theta, const float *xllim_in, const float *xllim_le,
+                           const float *xllim_te, const float *xllim_dn) {
+  if (!c) return MH_E_ARG;
+  if (!theta || !xllim_in || !xllim_le || !xllim_te || !xllim_dn) return fail(c, MH_E_ARG, "null argument");
+  const Geo g = c->g;
+  const size_t n = (size_t)g.IM * g.JM * g.KM;
+  if (n > (size_t)MH_XLENGTH_SCRATCH_NODES || g.JM > MH_XLENGTH_SCRATCH_J)
+    return fail(c, MH_E_ARG,
+                "mh_set_xlength_noalloc: " + std::to_string(n) + " nodes, JM = " + std::to_string(g.JM) +
+                    ": the scratch array holds at most " + std::to_string(MH_XLENGTH_SCRATCH_NODES) +
+                    " nodes and JM <= " + std::to_string(MH_XLENGTH_SCRATCH_J) + " (use mh_set_xlength)");
+  if (!c->xl_scratch) return fail(c, MH_E_STATE, "mh_set_xlength_noalloc: no scratch address for this device");
+  int r = check_inputs(c);
+  if (r) return r;
+  float *th = c->xl_scratch, *dist = th + MH_XLENGTH_SCRATCH_NODES;
+  float *xl4 = dist + MH_XLENGTH_SCRATCH_NODES, *xlim = xl4 + kXlRowSlots;
+  return xlength_run(c, th, dist, xl4, xlim, theta, xllim_in, xllim_le, xllim_te, xllim_dn);
 }
 
 int mh_start(mh_ctx *c) {

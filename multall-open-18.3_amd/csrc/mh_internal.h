@@ -200,6 +200,8 @@ struct mh_ctx {
   int nstep = 0, ifend = 0;
   int nout[5] = {};       // printout steps (read from the device at mh_start)
   bool stop_req = false;  // mh_request_stop: end at the next step divisible by 10
+  int device = 0;             // the HIP device the context was created on
+  float *xl_scratch = nullptr;  // that device's address of mh_set_xlength_noalloc's scratch
   float SFT = 0, SFX = 0, DAMP = 0, FMIXUP = 1, PREFF = 0, ROLIM = 0, ROMIN = 0;
   float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
   float EAVG = 1e6f, EMAX = 1e6f, ECONT = 1.f;

@@ -21,7 +21,7 @@ EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
            "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_set_xlength", "mh_start", "mh_step",
            "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch",
-           "mh_request_stop"]
+           "mh_request_stop", "mh_set_xlength_noalloc"]
 
 
 class MhDims(ctypes.Structure):
@@ -73,6 +73,7 @@ def load(path: str = LIB_PATH):
     L.mh_start.argtypes = [vp]
     L.mh_loss_init.argtypes = [vp]
     L.mh_set_xlength.argtypes = [vp] + [ctypes.c_void_p] * 5
+    L.mh_set_xlength_noalloc.argtypes = [vp] + [ctypes.c_void_p] * 5
     L.mh_step.argtypes = [vp, i, ctypes.POINTER(MhResid)]
     L.mh_request_stop.argtypes = [vp]
     L.mh_step_batch.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(MhResid), ctypes.POINTER(i)]
@@ -136,6 +137,12 @@ class Solver:
         the nodal THETA and the per-row mixing-length limits."""
         arrs = [np.ascontiguousarray(a, np.float32) for a in (theta, xllim_in, xllim_le, xllim_te, xllim_dn)]
         self._check(self.L.mh_set_xlength(self.h, *[a.ctypes.data for a in arrs]), "mh_set_xlength")
+
+    def set_xlength_noalloc(self, theta, xllim_in, xllim_le, xllim_te, xllim_dn):
+        """set_xlength on the library's per-device scratch array (no device
+        allocation; grids up to MH_XLENGTH_SCRATCH_NODES nodes)."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (theta, xllim_in, xllim_le, xllim_te, xllim_dn)]
+        self._check(self.L.mh_set_xlength_noalloc(self.h, *[a.ctypes.data for a in arrs]), "mh_set_xlength_noalloc")
 
     def start(self):
         self._check(self.L.mh_start(self.h), "mh_start")
