@@ -1,0 +1,166 @@
+"""Golden summaries of the reference on the benchmark workloads.
+
+For the single stator row (37 x 33 x 97, mixing length), the 1.5-stage
+turbine (3 rows of 64 x 64 x 193, Spalart-Allmaras) and the first and last
+points of the 16-point compressor speedline, runs the reference built by
+oracle/build_ref.sh for 1 and 100 time steps from the deck's initial guess
+and stores, per case:
+
+  decks/<case>.dat        the NEW-format deck (100 steps)
+  <case>_summary<N>.npz   after N steps, for each conserved variable
+                          (RO, ROVX, ROVR, RORVT, ROE): the pitchwise mean of
+                          every (J, K) line (FP64 sums, stored FP32), the
+                          whole K = KMID plane, and the FP64 sum of the field;
+                          the reference's own SUMFLX integrals (global.plt);
+                          and `spread`, how far one step-count of the
+                          reference's arithmetic carries a one-ulp change of
+                          one initial density (the C restatement of the
+                          oracle run twice from the reference's set-up),
+                          in each variable's scale
+  <case>_summary100_fp64.npz  the same after 100 steps of the reference's
+                          FP64 build (-fdefault-real-8), the yardstick for the
+                          library's fast mode: its FP64 reductions differ from
+                          the FP32 reference's sequential FP32 sums (over up to
+                          ~1e5 cells for the superblocks of the turbine), so it
+                          is held to be no farther from the FP64 reference than
+                          the FP32 reference itself is
+  config_setup_digests.json  SHA-256 digests of the reference's post-SETUP
+                          arrays, for the host set-up test
+
+The full fields are too large to keep (the turbine is 2.37 M nodes); every
+node still enters the line means, and a wrong node anywhere moves its line's
+mean.
+
+Usage: python tests/golden/make_configs.py [case ...]
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "multall-open-18.3_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+from multall_hip import deckgen  # noqa: E402
+
+VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE"]
+CASES = {
+    "config0": lambda n: deckgen.deck_config0(nsteps=n),
+    "config1": lambda n: deckgen.deck_config1(nsteps=n),
+    "speedline0": lambda n: deckgen.deck_speedline_point(0, nsteps=n),
+    "speedline15": lambda n: deckgen.deck_speedline_point(15, nsteps=n),
+}
+STEPS = (1, 100)
+DIGESTS = os.path.join(HERE, "config_setup_digests.json")
+
+
+def summary(field: dict, kmid: int) -> dict:
+    """Line means, mid-span plane and field sums of the conserved variables
+    (arrays [k, j, i])."""
+    out = {}
+    for v in VARS:
+        a = np.asarray(field[v], np.float64)
+        out[f"{v}_lines"] = a.mean(axis=2).astype(np.float32)          # [K, J]
+        out[f"{v}_plane"] = np.asarray(field[v][kmid - 1], np.float32)  # [J, I]
+        out[f"{v}_sum"] = np.array(a.sum())
+    return out
+
+
+def scales(field: dict) -> dict:
+    mom = np.sqrt(np.asarray(field["ROVX"], np.float64) ** 2 + np.asarray(field["ROVR"], np.float64) ** 2).max()
+    return {"RO": float(np.abs(field["RO"]).max()), "ROE": float(np.abs(field["ROE"]).max()), "ROVX": mom,
+            "ROVR": mom, "RORVT": float(np.abs(field["RORVT"]).max())}
+
+
+def ulp_spread(setup_state: dict, n: int) -> float:
+    """The oracle from the reference's set-up, n steps, with and without a
+    one-ulp change of one interior density: the largest difference in each
+    variable's scale."""
+    import orc
+    st = {k: (np.array(v) if k != "scalars" else dict(v)) for k, v in setup_state.items()}
+    st["scalars"]["NSTEPS_MAX"] = n
+    o = orc.Oracle(st)
+    o.start()
+    o.step(n + 10)
+    base = {v: o[v].copy() for v in VARS}
+    del o
+    sc = scales(base)
+    ro = st["RO"]
+    k, j, i = ro.shape[0] // 2, ro.shape[1] // 3, ro.shape[2] // 2
+    ro[k, j, i] = np.nextafter(ro[k, j, i], np.float32(np.inf))
+    o = orc.Oracle(st)
+    o.start()
+    o.step(n + 10)
+    return max(float(np.abs(o[v].astype(np.float64) - base[v]).max() / sc[v]) for v in VARS)
+
+
+def make(case: str) -> None:
+    import make_golden
+    import make_refdeck_digests as mrd
+    import orc
+    import refdump
+    from multall_hip import host
+    digests = json.load(open(DIGESTS)) if os.path.exists(DIGESTS) else {}
+    for n in STEPS:
+        spec = CASES[case](n)
+        txt = deckgen.write_deck(spec)
+        wd = tempfile.mkdtemp(prefix="mhcfg_")
+        try:
+            setup, final, _ = refdump.run_reference(txt, "fp32", workdir=wd)
+            sumflx = make_golden.read_global_plt(os.path.join(wd, "global.plt"))
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+        ref_setup = orc.from_dump(setup)
+        kmid = int(ref_setup["scalars"]["KMID"])
+        field = {v: final.node(v) for v in VARS}
+        out = summary(field, kmid)
+        out["NSTEP"] = np.array(int(final["NSTEP"]))
+        out["SUMFLX"] = sumflx
+        out["spread"] = np.array(ulp_spread(ref_setup, n) if n > 1 else 0.0)
+        np.savez_compressed(os.path.join(HERE, f"{case}_summary{n}.npz"), **out)
+        print(f"{case}: {n} steps, NSTEP {int(final['NSTEP'])}, one-ulp spread {float(out['spread']):.2e}",
+              flush=True)
+        if n == max(STEPS):
+            path = os.path.join(HERE, "decks", f"{case}.dat")
+            with open(path, "w") as f:
+                f.write(txt)
+            st = host.load_deck(path)
+            skip = set(mrd.LOSS_STATE)
+            arrays = {}
+            for k, a in st.items():
+                if k == "scalars" or k in skip or k not in ref_setup or k == "NOUT":
+                    continue
+                b = mrd.normalise(np.asarray(a), ref_setup[k])
+                arrays[k] = {"sha256": mrd.digest(b), "shape": list(b.shape), "dtype": str(b.dtype),
+                             "sum": float(np.asarray(b, np.float64).sum())}
+            scal = {k: float(ref_setup["scalars"][k]) for k in st["scalars"]
+                    if k in ref_setup["scalars"] and k not in ("NSTEPS_MAX", "VISLAM", "FTCOND")}
+            digests[case] = {"arrays": arrays, "scalars": scal}
+    with open(DIGESTS, "w") as f:
+        json.dump(digests, f, indent=0, sort_keys=True)
+
+
+def make_fp64(case: str, n: int = 100) -> None:
+    import orc
+    import refdump
+    txt = deckgen.write_deck(CASES[case](n))
+    _, final, _ = refdump.run_reference(txt, "fp64")
+    kmid = int(orc.from_dump(final)["scalars"]["KMID"])
+    out = summary({v: final.node(v) for v in VARS}, kmid)
+    out["NSTEP"] = np.array(int(final["NSTEP"]))
+    np.savez_compressed(os.path.join(HERE, f"{case}_summary{n}_fp64.npz"), **out)
+    print(f"{case}: {n} steps of the FP64 build", flush=True)
+
+
+if __name__ == "__main__":
+    for c in (sys.argv[1:] or list(CASES)):
+        make(c)
+        make_fp64(c)

@@ -91,3 +91,28 @@ def test_host_setup_of_reference_decks(deck):
     for k, v in want["scalars"].items():
         assert np.float32(st["scalars"][k]) == np.float32(v), (deck, k, st["scalars"][k], v)
     assert len(want["arrays"]) >= 100
+
+
+CFG_DIGESTS = os.path.join(GOLDEN, "config_setup_digests.json")
+# the benchmark workloads: the single stator row, the 1.5-stage turbine and
+# the first and last speedline points (tests/golden/make_configs.py)
+CFG_DECKS = sorted(json.load(open(CFG_DIGESTS))) if os.path.exists(CFG_DIGESTS) else []
+
+
+@pytest.mark.parametrize("deck", CFG_DECKS)
+def test_host_setup_of_config_decks(deck):
+    """The host's set-up of the benchmark decks equals the reference's
+    post-SETUP state bit for bit (SHA-256 of every array the library takes),
+    so the GPU tests on these decks start where the reference starts."""
+    import hashlib
+    from multall_hip import host
+    want = json.load(open(CFG_DIGESTS))[deck]
+    st = host.load_deck(os.path.join(GOLDEN, "decks", f"{deck}.dat"))
+    for name, w in want["arrays"].items():
+        a = np.ascontiguousarray(st[name])
+        assert list(a.shape) == w["shape"] and str(a.dtype) == w["dtype"], name
+        got = hashlib.sha256(a.tobytes()).hexdigest()
+        assert got == w["sha256"], f"{deck}: {name} differs (sum {a.astype(np.float64).sum()!r} vs {w['sum']!r})"
+    for k, v in want["scalars"].items():
+        assert np.float32(st["scalars"][k]) == np.float32(v), (deck, k, st["scalars"][k], v)
+    assert len(want["arrays"]) >= 80
