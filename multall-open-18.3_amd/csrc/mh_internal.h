@@ -137,10 +137,13 @@ struct Params {
 // (JSBLK) and the row interiors used by the damping average.
 // Node tile of the damping + corner-distribution kernel (J x K lines, all
 // I) and its dynamic LDS: cell lines (J..J+TJ, K-1..K+TK-1) and the four
-// corner-weight arrays on K-1..K+TK-1, each `pitch` = IM-1 rounded up to 64.
+// corner-weight arrays on K-1..K+TK-1, each `pitch` = 64 NC floats (NC = 1
+// for IM-1 <= 64, else 4) plus a zero pad column at each end.
 constexpr int kScatTJ = 4, kScatTK = 8;
+inline int scatter_chunks(int imm1) { return imm1 <= 64 ? 1 : 4; }
 inline size_t scatter_lds_bytes(int imm1) {
-  return (size_t)((kScatTJ + 1) * (kScatTK + 1) + 4 * (kScatTK + 1)) * 64 * ((imm1 + 63) / 64) * sizeof(float);
+  return (size_t)((kScatTJ + 1) * (kScatTK + 1) + 4 * (kScatTK + 1)) * (64 * scatter_chunks(imm1) + 2) *
+         sizeof(float);
 }
 
 // Fast-mode multigrid sums: per-(level-1 I block, J, level-1 K block) sums

@@ -1148,48 +1148,59 @@ __device__ __forceinline__ float damped(const Geo &g, const Dev &d, const float 
 // (mf:5828-5842).  One workgroup (4 wavefronts) per tile of kTJ x kTK node
 // lines (all I).  The tile's cells (J..J+kTJ, K-1..K+kTK-1) are damped once
 // into LDS and the averaged mixing-plane lines replaced by their mean there;
-// the corner weights of those K planes are staged in LDS beside them.  Each
-// node then gathers from cells I-1..I, J..J+1, K-1..K in the order the
-// reference's cell loop (K, J, I) visits them.  Dynamic LDS: kTL + kWL
-// lines of `pitch` floats (pitch = IM-1 rounded up to 64).
+// the corner weights of those K planes are staged in LDS beside them.  Cells
+// and weights off the grid (J < 2 or > JM, K < 1 or > KM-1, I < 1 or > IM-1)
+// are stored as zeros in padded lines (column c holds cell I = c, columns 0
+// and IM are pads), so each node gathers its eight corner contributions from
+// cells I-1..I, J..J+1, K-1..K without a bounds test, in the order the
+// reference's cell loop (K, J, I) visits them (an absent cell adds zero).
+// The fast mode damps with one reciprocal per value, y / (1 + |y| / (A DAMP))
+// as y * rcp(1 + |y| rAD) with rAD = 1 / (A DAMP) per J line; the exact mode
+// keeps the reference's two divisions.  Dynamic LDS: kTL + kWL lines of
+// `pitch` = 64 NC + 2 floats.
 constexpr int kTJ = kScatTJ, kTK = kScatTK, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4, kWL = 4 * (kTK + 1);
 static_assert(kTJ == 4, "one wavefront per tile J line");
 template <int NC>
 __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *D, float *DOUT,
                                              float DAMP, int damp, int mpav) {
   extern __shared__ float lds[];
-  __shared__ float avg[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
+  __shared__ float avg[kTJ + 1], rad[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
   __shared__ int flag[kTL];
+  constexpr int pitch = 64 * NC + 2;
   const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int j0 = (bl % gridDim.x) * kTJ + 1, k0 = (bl / gridDim.x) * kTK + 1;
   const int lane = threadIdx.x & 63, w = wave_id(), t = threadIdx.x;
-  const int IM = g.IM, IMM1 = g.IMM1, pitch = 64 * ((IMM1 + 63) / 64);
+  const int IM = g.IM, IMM1 = g.IMM1, JM = g.JM, KM = g.KM, KMM1 = g.KMM1;
+  const unsigned plane = (unsigned)IM * (unsigned)JM;
   float *C = lds;                  // [kTL][pitch] cell changes, line (a, b) = J j0+a, K k0-1+b
   float *W = lds + kTL * pitch;    // [4][kTK+1][pitch]: FTL, FBL, FTR, FBR at K k0-1+b
   // this wavefront's cell lines (ln = w + 4*u) and its first chunk of
   // node values, loaded before anything waits
-  float x[kMaxLn][4];
+  float x[kMaxLn][NC];
   static_assert(NC >= 1 && NC <= 4, "IM-1 <= 256");
 #pragma unroll
   for (int u = 0; u < kMaxLn; u++) {
     const int ln = w + 4 * u;
     const int jc = j0 + ln % (kTJ + 1), kc = k0 - 1 + ln / (kTJ + 1);
-    const bool in = ln < kTL && jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1;
+    const bool in = ln < kTL && jc >= 2 && jc <= JM && kc >= 1 && kc <= KMM1;
+    const unsigned base = in ? (unsigned)(jc - 1) * (unsigned)IM + (unsigned)(kc - 1) * plane : 0u;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NC; c++) {
       const int I = 64 * c + lane + 1;
-      x[u][c] = (c < NC && in && I <= IMM1) ? d.STORE[IX(I, jc, kc)] : 0.0f;
+      x[u][c] = (in && I <= IMM1) ? d.STORE[base + (unsigned)(I - 1)] : 0.0f;
     }
   }
   const int J = j0 + w;
+  const bool jin = J <= JM;
+  const unsigned nbase = jin ? (unsigned)(J - 1) * (unsigned)IM + (unsigned)(k0 - 1) * plane : 0u;
   float dv0[kTK];
 #pragma unroll
-  for (int kk = 0; kk < kTK; kk++) dv0[kk] = (J <= g.JM && k0 + kk <= g.KM && lane < IM) ? D[IX(lane + 1, J, k0 + kk)] : 0.0f;
+  for (int kk = 0; kk < kTK; kk++) dv0[kk] = (jin && k0 + kk <= KM && lane < IM) ? D[nbase + kk * plane + lane] : 0.0f;
   // per-line and per-J metadata
   if (t < kTL) {
     const int jc = j0 + t % (kTJ + 1), kc = k0 - 1 + t / (kTJ + 1);
     int f = 0;
-    if (jc >= 2 && jc <= g.JM && kc >= 1 && kc <= g.KMM1) {
+    if (jc >= 2 && jc <= JM && kc >= 1 && kc <= KMM1) {
       f = 1;
       if (mpav) {
         int jx = 0;
@@ -1205,13 +1216,15 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
     flag[t] = f;
   } else if (t >= 64 && t < 64 + kTJ + 1) {
     const int jc = j0 + t - 64;
-    if (jc >= 2 && jc <= g.JM) {
-      avg[t - 64] = damp ? avg_blk(d.AVG, d.NROW[jc - 1], g.NROWS) : 1.0f;
-      fdn[t - 64] = d.FACDWN[jc - 1];
-      fup[t - 64] = d.FACUP[jc - 1];
-    }
+    const bool jok = jc >= 2 && jc <= JM;
+    const float A = (jok && damp) ? avg_blk(d.AVG, d.NROW[jc - 1], g.NROWS) : 1.0f;
+    avg[t - 64] = A;
+    rad[t - 64] = 1.0f / (A * DAMP);
+    fdn[t - 64] = jok ? d.FACDWN[jc - 1] : 0.0f;
+    fup[t - 64] = jok ? d.FACUP[jc - 1] : 0.0f;
   }
-  // corner weights: 4 arrays x (kTK+1) K planes, kWL / 4 lines per wavefront
+  // corner weights: 4 arrays x (kTK+1) K planes, kWL / 4 lines per
+  // wavefront, zero off the grid and in the pad columns
 #pragma unroll
   for (int u = 0; u < kWL / 4; u++) {
     const int wl = w + 4 * u, arr = wl / (kTK + 1), kc = k0 - 1 + wl % (kTK + 1);
@@ -1221,12 +1234,18 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
     if (arr == 1) src = d.FBL;
     if (arr == 2) src = d.FTR;
     if (arr == 3) src = d.FBR;
-    if (kc >= 1 && kc <= g.KMM1)
+    const bool kin = kc >= 1 && kc <= KMM1;
+    const unsigned kb = kin ? (unsigned)(kc - 1) * (unsigned)IM : 0u;
+    float *Wl = W + wl * pitch;
 #pragma unroll
-      for (int c = 0; c < NC; c++) {
-        const int I = 64 * c + lane + 1;
-        if (I <= IMM1) W[wl * pitch + I - 1] = src[IKX(I, kc)];
-      }
+    for (int c = 0; c < NC; c++) {
+      const int I = 64 * c + lane + 1;
+      Wl[I] = (kin && I <= IMM1) ? src[kb + (unsigned)(I - 1)] : 0.0f;
+    }
+    if (lane == 0) {
+      Wl[0] = 0.0f;
+      Wl[64 * NC + 1] = 0.0f;
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -1234,70 +1253,77 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
     const int ln = w + 4 * u;
     if (ln >= kTL) continue;
     const int f = flag[ln];
-    if (!f) continue;
     float v[4] = {0.f, 0.f, 0.f, 0.f};
-    const float A = avg[ln % (kTJ + 1)];
+    if (f) {
+      const int a = ln % (kTJ + 1);
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-      float y = x[u][c];
-      if (damp) {
-        const float FDAMP = fabsf(y) / A;
-        y = y / (1.f + FDAMP / DAMP);
+      for (int c = 0; c < NC; c++) {
+        float y = x[u][c];
+        if (damp) {
+          if (p.EXACT) {
+            const float FDAMP = fabsf(y) / avg[a];
+            y = y / (1.f + FDAMP / DAMP);
+          } else {
+            y = y * __builtin_amdgcn_rcpf(1.f + fabsf(y) * rad[a]);
+          }
+        }
+        v[c] = y;
       }
-      v[c] = y;
-    }
-    if (f == 3) {
-      const float SUM = line_sum(v, IMM1, p.EXACT) / (float)IMM1;
+      if (f == 3) {
+        const float SUM = line_sum(v, IMM1, p.EXACT) / (float)IMM1;
 #pragma unroll
-      for (int c = 0; c < NC; c++) v[c] = SUM;
+        for (int c = 0; c < NC; c++) {
+          const int I = 64 * c + lane + 1;
+          v[c] = I <= IMM1 ? SUM : 0.0f;
+        }
+      }
     }
+    float *Cl = C + ln * pitch;
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-      const int I = 64 * c + lane + 1;
-      if (I <= IMM1) C[ln * pitch + I - 1] = v[c];
+    for (int c = 0; c < NC; c++) Cl[64 * c + lane + 1] = v[c];
+    if (lane == 0) {
+      Cl[0] = 0.0f;
+      Cl[64 * NC + 1] = 0.0f;
     }
   }
   __syncthreads();
   // gather: wavefront w takes node lines J = j0 + w, K = k0 .. k0+kTK-1
-  if (J > g.JM) return;
-  for (int I = lane + 1; I <= IM; I += 64) {
+  if (!jin) return;
+  const int nk = min(kTK, KM - k0 + 1);
+  const float fd0 = fdn[w], fu1 = fup[w + 1];
+  // node chunks: IM may be 64 NC + 1 (the last node's cells are I-1 and the
+  // right-hand pad)
+#pragma unroll
+  for (int c = 0; c <= NC; c++) {
+    const int I = 64 * c + lane + 1;
+    if (64 * c >= IM) break;
     float dv[kTK];
 #pragma unroll
     for (int kk = 0; kk < kTK; kk++)
-      dv[kk] = (I == lane + 1) ? dv0[kk] : (k0 + kk <= g.KM) ? D[IX(I, J, k0 + kk)] : 0.0f;
+      dv[kk] = c == 0 ? dv0[kk] : (kk < nk && I <= IM) ? D[nbase + kk * plane + (unsigned)(I - 1)] : 0.0f;
 #pragma unroll
     for (int kk = 0; kk < kTK; kk++) {
-      const int K = k0 + kk;
-      if (K > g.KM) break;
+      if (kk >= nk) break;
       float v = dv[kk];
 #pragma unroll
       for (int b2 = 0; b2 < 2; b2++) {
-        const int kc = K - 1 + b2;
-        if (kc < 1 || kc > g.KMM1) continue;
-        const int b = kk + b2;  // kc - (k0 - 1)
+        const int b = kk + b2;        // cell K line kc = k0 - 1 + b = K - 1 + b2
         const bool top = (b2 == 0);  // node is the cell's K+1 corner
+        // weights of the cells I-1 (node is their I+1 corner) and I
+        const float wr = W[((top ? 2 : 3) * (kTK + 1) + b) * pitch + I - 1];
+        const float wl = W[((top ? 0 : 1) * (kTK + 1) + b) * pitch + I];
 #pragma unroll
         for (int a2 = 0; a2 < 2; a2++) {
-          const int jc = J + a2;
-          if (jc < 2 || jc > g.JM) continue;
-          const int a = w + a2;
-          const float fac = a2 == 0 ? fdn[a] : fup[a];
-          const float *Lc = C + (a + (kTJ + 1) * b) * pitch;
-#pragma unroll
-          for (int i2 = 0; i2 < 2; i2++) {
-            const int ic = I - 1 + i2;
-            if (ic < 1 || ic > IMM1) continue;
-            const bool right = (i2 == 0);  // node is the cell's I+1 corner
-            const int arr = top ? (right ? 2 : 0) : (right ? 3 : 1);
-            v = v + Lc[ic - 1] * W[(arr * (kTK + 1) + b) * pitch + ic - 1] * fac;
-          }
+          const float fac = a2 == 0 ? fd0 : fu1;
+          const float *Lc = C + (w + a2 + (kTJ + 1) * b) * pitch;
+          v = v + Lc[I - 1] * wr * fac;
+          v = v + Lc[I] * wl * fac;
         }
       }
-      DOUT[IX(I, J, K)] = v;
+      if (I <= IM) DOUT[nbase + kk * plane + (unsigned)(I - 1)] = v;
     }
   }
 }
-
 __global__ void __launch_bounds__(256) k_damp_scatter1(Geo g, Dev d, Params p, const float *D, float *DOUT,
                                                        float DAMP, int damp, int mpav) {
   damp_scatter<1>(g, d, p, D, DOUT, DAMP, damp, mpav);
@@ -1531,9 +1557,21 @@ __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const flo
 // SMOOTH_VAR and TSTEP's D conditions as k_smooth_plane, for IM <= 64 and
 // planes without exit smoothing, with the plane held in registers: wavefront
 // w owns K lines w*L+1 .. w*L+L, lane l the point I = l+1.  The pitchwise
-// stencil and the line conditions use lane shuffles; only the spanwise
-// stencil goes through LDS (the whole plane, one barrier), and the corner
-// update reads lines 2 and KM-1 from LDS.
+// stencil and the line conditions use lane moves (DPP shifts to the
+// neighbours, readlane for the fixed lanes 1, 2, 3, IM-2, IM-1 and the last
+// point); only the spanwise stencil goes through LDS (the whole plane, one
+// barrier), and the corner update reads lines 2 and KM-1 from LDS.  Offsets
+// are 32-bit (the grid has < 2^31 nodes).
+__device__ __forceinline__ float lane_next(float x) {  // lane l gets lane l+1 (lane 63: itself)
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float x) {  // lane l gets lane l-1 (lane 0: itself)
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_at(float x, int l) {  // lane l's value, wavefront-uniform l
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
 template <int L>
 __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const float *SRC, float *DST, const Params &p,
                                            const StepScalars &s, int ncall, int j0) {
@@ -1544,6 +1582,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
   const int I = lane + 1, Ic = min(I, IM);
   const int kbase = w * L + 1;
+  const unsigned plane = (unsigned)IM * (unsigned)g.JM;
   const float SFX = s.SFX, SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
   const float SFXB = p.FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
   const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
@@ -1559,34 +1598,46 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
     if (J == J2 - 2) cs = 4;
     if (J == J2 - 1) cs = 5;
   }
+  // the spanwise coefficients of this wavefront's lines and their
+  // neighbours (FKU, FKD at K-2 .. K+L), read before the plane loads
+  float fku[L + 2], fkd[L + 2];
+#pragma unroll
+  for (int l = 0; l < L + 2; l++) {
+    const int K = min(max(kbase + l - 1, 1), KM);
+    fku[l] = d.FKU[K - 1];
+    fkd[l] = d.FKD[K - 1];
+  }
   // streamwise smoothing, mf:15369-15445 (loads clamped into the grid)
   float v[L];
   auto stream = [&](auto f) {
 #pragma unroll
-    for (int l = 0; l < L; l++) v[l] = f(min(kbase + l, KM));
+    for (int l = 0; l < L; l++) {
+      const unsigned o = (unsigned)(min(kbase + l, KM) - 1) * plane + (unsigned)(Ic - 1);
+      v[l] = f(o);
+    }
   };
-#define S(JJ) SRC[IX(Ic, (JJ), K)]
+#define S(JJ) SRC[o + (unsigned)((JJ) - 1) * (unsigned)IM]
   switch (cs) {
     case 1:
-      stream([&](int K) {
+      stream([&](unsigned o) {
         const float AVERG = 0.5f * (S(J + 1) + S(J - 1));
         const float CURV = AVERG - 0.5f * S(J) - 0.25f * (S(J - 2) + S(J + 2));
         return SFX1 * S(J) + SFX * (AVERG + FAC_4TH * CURV);
       });
       break;
-    case 2: stream([&](int K) { return SFXB1 * S(J1 + 2) + SFXBH * (S(J1 + 3) + S(J1 + 1)); }); break;
+    case 2: stream([&](unsigned o) { return SFXB1 * S(J1 + 2) + SFXBH * (S(J1 + 3) + S(J1 + 1)); }); break;
     case 3:
-      stream([&](int K) {
+      stream([&](unsigned o) {
         return SFXB1 * S(J1 + 1) + SFXB * (1.25f * S(J1 + 2) + 0.5f * S(J1 + 3) - 0.75f * S(J1 + 4));
       });
       break;
-    case 4: stream([&](int K) { return SFXB1 * S(J2 - 2) + SFXBH * (S(J2 - 1) + S(J2 - 3)); }); break;
+    case 4: stream([&](unsigned o) { return SFXB1 * S(J2 - 2) + SFXBH * (S(J2 - 1) + S(J2 - 3)); }); break;
     case 5:
-      stream([&](int K) {
+      stream([&](unsigned o) {
         return SFXB1 * S(J2 - 1) + SFXB * (1.25f * S(J2 - 2) + 0.5f * S(J2 - 3) - 0.75f * S(J2 - 4));
       });
       break;
-    default: stream([&](int K) { return S(J); }); break;
+    default: stream([&](unsigned o) { return S(J); }); break;
   }
   // leading-edge points I = 1 and IM (mf:15417-15434)
   if (smx && (J == JLED - 2 || J == JLED - 1 || J == JLED) && (I == 1 || I == IM)) {
@@ -1594,6 +1645,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
 #pragma unroll
     for (int l = 0; l < L; l++) {
       const int K = min(kbase + l, KM);
+      const unsigned o = (unsigned)(K - 1) * plane + (unsigned)(Ic - 1);
       v[l] = (J == JLED) ? SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)])
                          : SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
     }
@@ -1607,9 +1659,9 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
 #pragma unroll
     for (int l = 0; l < L; l++) {
       const float x = v[l];
-      const float xr = __shfl(x, lane + 1, 64), xl = __shfl(x, lane - 1, 64);
-      const float x1 = __shfl(x, 0, 64), x2 = __shfl(x, 1, 64), x3 = __shfl(x, 2, 64);
-      const float xm1 = __shfl(x, IMM1 - 1, 64), xm2 = __shfl(x, IMM2 - 1, 64);
+      const float xr = lane_next(x), xl = lane_prev(x);
+      const float x1 = lane_at(x, 0), x2 = lane_at(x, 1), x3 = lane_at(x, 2);
+      const float xm1 = lane_at(x, IMM1 - 1), xm2 = lane_at(x, IMM2 - 1);
       float avg, cur;
       if (mid) {
         avg = FDI * xr + FUI * xl;
@@ -1621,8 +1673,8 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
         avg = (I == 1) ? x2 + FU1 * (x2 - x3) : xm1 + FUM * (xm1 - xm2);
         cur = 0.0f;
       }
-      const float cr = __shfl(cur, lane + 1, 64), cl = __shfl(cur, lane - 1, 64);
-      const float c2 = __shfl(cur, 1, 64), cm1 = __shfl(cur, IMM1 - 1, 64);
+      const float cr = lane_next(cur), cl = lane_prev(cur);
+      const float c2 = lane_at(cur, 1), cm1 = lane_at(cur, IMM1 - 1);
       float sc;
       if (mid) sc = FUI * cl + FDI * cr;
       else if (!blade) sc = 0.5f * (c2 + cm1);
@@ -1642,17 +1694,18 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
       const float x = v[l];
       float avg;
       float sc = 0.0f;
+      // coefficients at K-1, K, K+1: fk?[l], fk?[l+1], fk?[l+2]
       if (K >= 2 && K <= KMM1) {
-        avg = d.FKD[K - 1] * PK(K + 1) + d.FKU[K - 1] * PK(K - 1);
+        avg = fkd[l + 1] * PK(K + 1) + fku[l + 1] * PK(K - 1);
         // CURVE at K-1 and K+1 (zero on the end lines)
         float cm = 0.0f, cp = 0.0f;
-        if (K - 1 >= 2) cm = PK(K - 1) - (d.FKD[K - 2] * PK(K) + d.FKU[K - 2] * PK(K - 2));
-        if (K + 1 <= KMM1) cp = PK(K + 1) - (d.FKD[K] * PK(K + 2) + d.FKU[K] * PK(K));
-        sc = d.FKU[K - 1] * cm + d.FKD[K - 1] * cp;
+        if (K - 1 >= 2) cm = PK(K - 1) - (fkd[l] * PK(K) + fku[l] * PK(K - 2));
+        if (K + 1 <= KMM1) cp = PK(K + 1) - (fkd[l + 2] * PK(K + 2) + fku[l + 2] * PK(K));
+        sc = fku[l + 1] * cm + fkd[l + 1] * cp;
       } else if (K == 1) {
-        avg = PK(2) + 0.5f * d.FKU[0] * (PK(2) - PK(3));
+        avg = PK(2) + 0.5f * fku[l + 1] * (PK(2) - PK(3));
       } else {
-        avg = PK(KMM1) + 0.5f * d.FKU[KM - 1] * (PK(KMM1) - PK(KMM2));
+        avg = PK(KMM1) + 0.5f * fku[l + 1] * (PK(KMM1) - PK(KMM2));
       }
       v[l] = SFT1 * x + SFT * (avg + FAC_4TH * sc);
     }
@@ -1670,7 +1723,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
       const int K = kbase + l;
       if (K != 1 && K != KM) continue;
       const float x = v[l];
-      const float xn = __shfl(x, lane + 1, 64), xp = __shfl(x, lane - 1, 64);
+      const float xn = lane_next(x), xp = lane_prev(x);
       const float y = (K == 1) ? C2[0][lane] : C2[1][lane];
       if (K == 1) {
         if (I == 1) v[l] = SFT1 * x + SFTH * (xn + y);
@@ -1696,11 +1749,11 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
     const int K = kbase + l;
     float x = v[l];
     if (per) {
-      const float a = 0.5f * (__shfl(x, 0, 64) + __shfl(x, ILAST - 1, 64));
+      const float a = 0.5f * (lane_at(x, 0) + lane_at(x, ILAST - 1));
       if (I == 1 || I == ILAST) x = a;
     }
     if (K >= K1 && K <= K2) {
-      const float a = 0.5f * (__shfl(x, 0, 64) + __shfl(x, ILAST - 1, 64));
+      const float a = 0.5f * (lane_at(x, 0) + lane_at(x, ILAST - 1));
       if (I == 1 || I == ILAST) x = a;
     }
     v[l] = x;
@@ -1711,15 +1764,16 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
 #pragma unroll
     for (int l = 0; l < L; l++) {
       const float x = v[l];
-      const float xr = __shfl(x, lane + 1, 64);
+      const float xr = lane_next(x);
       float t[4] = {(I <= IMM1) ? FPI * (x + xr) : 0.0f, 0.f, 0.f, 0.f};
       v[l] = 0.5f * line_sum(t, IMM1, p.EXACT);
     }
   }
+  const unsigned ob = (unsigned)(J - 1) * (unsigned)IM + (unsigned)lane;
 #pragma unroll
   for (int l = 0; l < L; l++) {
     const int K = kbase + l;
-    if (K <= KM && I <= IM) DST[IX(I, J, K)] = v[l];
+    if (K <= KM && I <= IM) DST[ob + (unsigned)(K - 1) * plane] = v[l];
   }
 }
 
