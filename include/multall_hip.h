@@ -150,6 +150,8 @@ enum mh_scalar_id {
   MH_S_VISLAM, MH_S_FTCOND,
   /* library controls (no reference counterpart) */
   MH_S_EXACT_ORDER,   /* 1: reductions in the reference's sequential order */
+  MH_S_USE_GRAPHS,    /* 1 (default): steps queued as captured HIP graphs, 0: launch by launch */
+  MH_S_LANES,         /* 1 (default): in the fast mode, independent equations' time steps on three streams */
   MH_NSCALARS
 };
 

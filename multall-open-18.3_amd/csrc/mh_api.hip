@@ -38,11 +38,11 @@ __global__ void k_mpavg(Geo, Dev, Params, int);
 __global__ void k_mg_sum_seq(Geo, Dev, const int *, const int *, int, int, int, int, int, float *);
 __global__ void k_mg_sum_wave(Geo, Dev, const int *, const int *, int, int, int, int, int, float *);
 __global__ void k_jplane_sums(Geo, Dev, double *, int);
-__global__ void k_range_sums(Geo, const double *, const int *, const int *, int, float *, int);
+__global__ void k_range_sums(Geo, const double *, const int *, const int *, int, float *, int, const int *);
 __global__ void k_range_sums_seq(Geo, Dev, const int *, const int *, int, float *, int, int);
 __global__ void k_mg_apply(Geo, Dev);
-__global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int);
-__global__ void k_damp_apply(Geo, Dev, float);
+__global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int, int);
+__global__ void k_damp_apply(Geo, Dev);
 __global__ void k_scatter(Geo, Dev, const float *, float *);
 __global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
 __global__ void k_smooth_reg1(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
@@ -66,11 +66,11 @@ __global__ void k_stepup_step(Geo, Dev, float);
 __global__ void k_stepup_mix(Geo, Dev);
 __global__ void k_integrals(Geo, Dev, Params);
 __global__ void k_loss_vislam(Geo, Dev, Params);
-__global__ void k_loss_span(Geo, Dev, Params, float, float);
-__global__ void k_loss_pitch(Geo, Dev, Params, float, float);
+__global__ void k_loss_span(Geo, Dev, Params, float);
+__global__ void k_loss_pitch(Geo, Dev, Params, float);
 __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
-__global__ void k_sa_addstore(Geo, Dev, float *, float, int);
+__global__ void k_sa_addstore(Geo, Dev, float *, int);
 __global__ void k_resid_line(Geo, Dev, float *, Params, MgFast, int, int, int);
 __global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, float *, float *,
                             float *);
@@ -79,14 +79,14 @@ __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
 __global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
 __global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
 __global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
-__global__ void k_damp_scatter1(Geo, Dev, Params, const float *, float *, float, int, int);
-__global__ void k_damp_scatter4(Geo, Dev, Params, const float *, float *, float, int, int);
+__global__ void k_damp_scatter1(Geo, Dev, Params, const float *, float *, int, int);
+__global__ void k_damp_scatter4(Geo, Dev, Params, const float *, float *, int, int);
 __global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
 __global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
 __global__ void k_sa_grad(Geo, Dev, Params);
-__global__ void k_sa_visc(Geo, Dev, Params, float, int);
-__global__ void k_nl_visc(Geo, Dev, Params, float);
+__global__ void k_sa_visc(Geo, Dev, Params, int);
+__global__ void k_nl_visc(Geo, Dev, Params);
 __global__ void k_nl_trans(Geo, Dev, Params);
 __global__ void k_pwallgrad_cell(Geo, Dev, Params);
 __global__ void k_pwallgrad_smooth(Geo, Dev);
@@ -101,6 +101,8 @@ __global__ void k_sa_source(Geo, Dev, Params, float);
 __global__ void k_sa_iforces(Geo, Dev, Params);
 __global__ void k_sa_kwalls(Geo, Dev, Params);
 __global__ void k_sa_kfaces(Geo, Dev, Params, float);
+__global__ void k_step_begin(Geo, Dev, Params);
+__global__ void k_nan_check(Geo, Dev);
 
 const ArrayInfo kArrays[MH_NARRAYS] = {
     {"RO", K_N, false}, {"ROVX", K_N, false}, {"ROVR", K_N, false}, {"RORVT", K_N, false}, {"ROE", K_N, false},
@@ -146,7 +148,7 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
     "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "FAC_STMIX", "FAC_ST0", "FAC_ST1", "FAC_ST2", "FAC_ST3",
     "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "FLOWIN", "RFLOW", "VISLAM", "FTCOND",
-    "EXACT_ORDER",
+    "EXACT_ORDER", "USE_GRAPHS", "LANES",
 };
 
 }  // namespace mh
@@ -403,12 +405,50 @@ int last_launch(mh_ctx *c, const char *what) {
 
 StepScalars step_scalars(const mh_ctx *c) {
   StepScalars s{};
-  s.SFT = c->SFT; s.SFX = c->SFX; s.DAMP = c->DAMP; s.FMIXUP = c->FMIXUP; s.PREFF = c->PREFF;
+  s.PREFF = c->PREFF;
   s.ROLIM = c->ROLIM; s.TCOOL_MIN = 1.0e6f;
   s.SFEX1 = 1.f - sf(c, MH_S_SFEXIT); s.SFEXH = 0.5f * sf(c, MH_S_SFEXIT);
   s.RFIN1 = 1.0f - sf(c, MH_S_RFIN); s.RFIN2 = 2.f * sf(c, MH_S_RFIN);
-  s.NSTEP = c->nstep; s.IFEND = c->ifend;
   return s;
+}
+
+// What the host knows of step n before the device takes it: which launches
+// the step makes.  Everything that changes from step to step as a value (the
+// ramps, the step count, the stop tests) is on the device; the structure of
+// the step depends on n alone, except the convergence stop, which the
+// device can only find on a step just after a check (n = 5k+1): there the
+// check block is launched gated on the device's IFEND.
+struct StepPlan {
+  bool check5 = false;   // the every-5-step check runs (mf:5240)
+  bool cond = false;     // the check runs only if the device ends the run here
+  bool visc = false;     // the viscous forces are refreshed (every NLOS steps)
+  bool vislam = false;   // ... re-evaluating the laminar viscosity (NSTEP = 1 or < NSTEPS_MAX/4)
+  bool refresh = false;  // the closing / printout viscous refresh (mf:5338-5341)
+  bool damp = false;     // DAMP within [2, 100]: TSTEP damps
+  bool first = false;    // NSTEP = 1
+  int code() const {
+    return check5 | cond << 1 | visc << 2 | vislam << 3 | refresh << 4 | damp << 5 | first << 6;
+  }
+};
+
+// Plan step n (host mirror of the device's ramp for the damping flag; the
+// host's DAMP follows the same FP32 statements as k_step_begin).
+StepPlan plan_step(mh_ctx *c, int n) {
+  const Params p = params_of(c);
+  StepPlan q;
+  const bool end_known = n == p.NSTEPS_MAX || (c->stop_req && n % 10 == 0);
+  q.check5 = n % 5 == 0 || end_known;
+  q.cond = !q.check5 && n > 1 && n % 5 == 1;
+  q.visc = p.ILOS != 0 && n % p.NLOS == 0;
+  q.vislam = n == 1 || n < p.NSTEPS_MAX / 4;
+  bool ifprint = false;
+  for (int i = 0; i < 5; i++) ifprint = ifprint || n == c->nout[i];
+  q.refresh = (q.check5 && ifprint) || end_known;
+  const float FCHANGE = 1.0f - (float)(n - 1) / (float)p.NCHANGE;
+  if (FCHANGE > 0.0f) c->DAMP = p.DAMPIN * (1.f - 0.75f * FCHANGE);
+  q.damp = !(c->DAMP < 2.0f || c->DAMP > 100);
+  q.first = n == 1;
+  return q;
 }
 
 // Build the CSR J-lists on the host from the integer maps.
@@ -532,8 +572,9 @@ int build_lists(mh_ctx *c) {
 // SMOOTH_VAR with D's boundary conditions.
 // The Dev a lane's kernels see: the context's arrays with the lane's own
 // flux and TSTEP scratch (lane 0 is the context's stream and scratch).
-Dev lane_dev(const mh_ctx *c, int lane) {
+Dev lane_dev(const mh_ctx *c, int lane, int gate) {
   Dev d = c->d;
+  d.gate = gate;
   if (lane == 0) return d;
   const Lane &l = c->lanes[lane - 1];
   d.XFLUX = l.XFLUX; d.TFLUX = l.TFLUX; d.RFLUX = l.RFLUX; d.SOURCE = l.SOURCE;
@@ -543,10 +584,10 @@ Dev lane_dev(const mh_ctx *c, int lane) {
 }
 hipStream_t lane_stream(const mh_ctx *c, int lane) { return lane == 0 ? c->stream : c->lanes[lane - 1].stream; }
 
-// One TSTEP call (mf:5470-5932) on lane `lane`.
-int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
+// One TSTEP call (mf:5470-5932) on lane `lane`, gated on CTL slot `gate`.
+int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan &q, int gate) {
   const Geo &g = c->g;
-  Dev d = lane_dev(c, lane);
+  Dev d = lane_dev(c, lane, gate);
   hipStream_t st = lane_stream(c, lane);
   const Params p = params_of(c);
   const StepScalars s = step_scalars(c);
@@ -559,7 +600,9 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
     mgf.P = c->lanes[lane - 1].mgf.P;
     mgf.PT = c->lanes[lane - 1].mgf.PT;
   }
-  const bool damp = !(s.DAMP < 2.0f || s.DAMP > 100);
+  mgf.gate = c->d.CTL + gate;
+  const int *gp = c->d.CTL + gate;
+  const bool damp = q.damp;
   // fast mode without residual smoothing takes the damping line sums from
   // the multigrid update
   const bool fused_apply = !p.EXACT && p.NRSMTH == 0 && damp;
@@ -589,7 +632,7 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
              g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
       LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 0);
       LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, st, g, (const double *)d.RED, L.sb_ptr, L.sb_list,
-             g.NSBLK, d.SBCHG, 0);
+             g.NSBLK, d.SBCHG, 0, gp);
     }
     LCHK("mg_sums");
   }
@@ -597,13 +640,12 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
   else LAUNCH(k_mg_apply, grid, kNodeBlock, 0, st, g, d);
   LCHK("mgapply");
   if (p.NRSMTH > 0) {
+    // STORE -> TMP -> DSC -> STORE (DSC is free until the corner
+    // distribution), so the buffers keep their roles from call to call
     for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0);
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.STORE, p.RSMTH, 1);
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 2);
-      if (lane == 0) std::swap(c->d.STORE, c->d.TMP);
-      else std::swap(c->lanes[lane - 1].STORE, c->lanes[lane - 1].TMP);
-      d = lane_dev(c, lane);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0, 0);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, p.RSMTH, 0, 1);
+      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, d.STORE, p.RSMTH, 0, 2);
     }
     LCHK("smooth_resid");
   }
@@ -618,7 +660,7 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
     } else {
       LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 1);
       LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, st, g, (const double *)d.RED, L.row_ptr, L.row_list,
-             g.NROWS, d.AVG, 1);
+             g.NROWS, d.AVG, 1, gp);
     }
     LCHK("damp_avg");
   }
@@ -627,12 +669,12 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
     // add the (damped) change, carry the pitch average over each mixing
     // plane, smooth once with FAC_SFVIS*SFT, then the periodic/tip/
     // mixing-plane conditions with ILAST = IM-1
-    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, st, g, d, D, s.DAMP, damp ? 1 : 0);
+    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, st, g, d, D, damp ? 1 : 0);
     if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, st, g, d, p, D);
-    const float sm = p.FAC_SFVIS * s.SFT;
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)D, d.TMP, sm, 0);
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, sm, 1);
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, D, sm, 2);
+    // smoothing factor FAC_SFVIS * SFT, the step's SFT taken on the device
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)D, d.TMP, p.FAC_SFVIS, 1, 0);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, p.FAC_SFVIS, 1, 1);
+    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, D, p.FAC_SFVIS, 1, 2);
     LAUNCH(k_sa_dbc, dim3(g.KM, g.JM), dim3(64), 0, st, g, d, p, D);
     LCHK("sa_tstep");
     return MH_OK;
@@ -643,10 +685,10 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
     const dim3 sg((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + kScatTK - 1) / kScatTK);
     if (g.IMM1 <= 64)
       LAUNCH(k_damp_scatter1, sg, dim3(256), scatter_lds_bytes(g.IMM1), st, g, d, p, (const float *)D, d.DSC,
-             s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+             damp ? 1 : 0, nmp > 0 ? 1 : 0);
     else
       LAUNCH(k_damp_scatter4, sg, dim3(256), scatter_lds_bytes(g.IMM1), st, g, d, p, (const float *)D, d.DSC,
-             s.DAMP, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+             damp ? 1 : 0, nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
   // SMOOTH_VAR: planes in registers where IM <= 64 and KM <= 128, the LDS
@@ -671,9 +713,8 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
   return MH_OK;
 }
 
-int stepup(mh_ctx *c, float relax) {
+int stepup(mh_ctx *c, const Dev &d, float relax) {
   const Geo &g = c->g;
-  const Dev &d = c->d;
   const Params p = params_of(c);
   LAUNCH(k_stepup_temp, node_grid(g), kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_stepup_step, node_grid(g), kNodeBlock, 0, c->stream, g, d, relax);
@@ -759,9 +800,8 @@ int loss_tables(mh_ctx *c) {
 // the Shih wall functions (YPLUSWALL < -10) and the SA pressure-gradient
 // source factor read.  A pure function of PEFF and the velocities, which a
 // viscous call does not change, so one evaluation serves each call.
-int pwallgrad(mh_ctx *c) {
+int pwallgrad(mh_ctx *c, const Dev &d) {
   const Geo &g = c->g;
-  const Dev &d = c->d;
   const Params p = params_of(c);
   LAUNCH(k_pwallgrad_cell, node_grid(g), kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_pwallgrad_smooth, dim3((g.IMM1 * g.KMM1 + 63) / 64), dim3(64), 0, c->stream, g, d);
@@ -769,19 +809,17 @@ int pwallgrad(mh_ctx *c) {
   return MH_OK;
 }
 
-// One LOSS call (mf:8058-9066) with the given relaxation factor, FMIXUP and
-// step number (the laminar viscosity is re-evaluated while NSTEP = 1 or
-// NSTEP < NSTEPS_MAX/4, mf:8082).
-int loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
+// One LOSS call (mf:8058-9066) with the given relaxation factor; FMIXUP is
+// the device's (SC_FMIXUP).  vislam: the laminar viscosity is re-evaluated
+// (NSTEP = 1 or NSTEP < NSTEPS_MAX/4, mf:8082).
+int loss(mh_ctx *c, const Dev &d, float rf_vis, bool vislam) {
   const Geo &g = c->g;
-  const Dev &d = c->d;
   const Params p = params_of(c);
-  if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  if (vislam) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
   int r;
-  if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c))) return r;  // mf:8197
-  LAUNCH(k_loss_span, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis,
-         fmixup);
-  LAUNCH(k_loss_pitch, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p, rf_vis, fmixup);
+  if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c, d))) return r;  // mf:8197
+  LAUNCH(k_loss_span, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p, rf_vis);
+  LAUNCH(k_loss_pitch, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p, rf_vis);
   LCHK("loss");
   return MH_OK;
 }
@@ -817,19 +855,18 @@ int sa_maps(mh_ctx *c) {
   return h2d(c, c->d.SAJMAP, map.data(), map.size() * 4);
 }
 
-// One SPAL_LOSS call (mf:13586-14827) with the given relaxation factor,
-// FMIXUP and step number.
-int spal_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
+// One SPAL_LOSS call (mf:13586-14827) with the given relaxation factor;
+// first: NSTEP = 1.
+int spal_loss(mh_ctx *c, const Dev &d, float rf_vis, bool vislam, bool first) {
   const Geo &g = c->g;
-  const Dev &d = c->d;
   const Params p = params_of(c);
   const dim3 grid = node_grid(g);
-  if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  if (vislam) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
   int r;
   // mf:13656-13658 and 14175
-  if ((p.FAC_PGRAD > 0.001f || p.YPLUSWALL < -10.0f) && (r = pwallgrad(c))) return r;
+  if ((p.FAC_PGRAD > 0.001f || p.YPLUSWALL < -10.0f) && (r = pwallgrad(c, d))) return r;
   LAUNCH(k_sa_grad, grid, kNodeBlock, 0, c->stream, g, d, p);
-  LAUNCH(k_sa_visc, grid, kNodeBlock, 0, c->stream, g, d, p, fmixup, nstep);
+  LAUNCH(k_sa_visc, grid, kNodeBlock, 0, c->stream, g, d, p, first ? 1 : 0);
   if (p.FAC_VORT > 0.001f || p.FAC_PGRAD > 0.001f) LAUNCH(k_sa_factors, dim3(1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_gradtvis, grid, kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_sa_trcol, dim3((g.IMM1 + 63) / 64, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
@@ -843,20 +880,19 @@ int spal_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
   return MH_OK;
 }
 
-// One NEW_LOSS call (mf:12522-13462) with the given relaxation factor,
-// FMIXUP and step number: SPAL_LOSS's gradient, stress and force kernels
-// around the mixing-length viscosity and the transition test.
-int new_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
+// One NEW_LOSS call (mf:12522-13462) with the given relaxation factor:
+// SPAL_LOSS's gradient, stress and force kernels around the mixing-length
+// viscosity and the transition test.
+int new_loss(mh_ctx *c, const Dev &d, float rf_vis, bool vislam) {
   const Geo &g = c->g;
-  const Dev &d = c->d;
   const Params p = params_of(c);
   const dim3 grid = node_grid(g);
-  if (nstep == 1 || nstep < p.NSTEPS_MAX / 4) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
+  if (vislam) LAUNCH(k_loss_vislam, dim3(1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_grad, grid, kNodeBlock, 0, c->stream, g, d, p);
-  LAUNCH(k_nl_visc, grid, kNodeBlock, 0, c->stream, g, d, p, fmixup);
+  LAUNCH(k_nl_visc, grid, kNodeBlock, 0, c->stream, g, d, p);
   LAUNCH(k_nl_trans, dim3(g.JM - 1), dim3(64), 0, c->stream, g, d, p);
   int r;
-  if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c))) return r;  // mf:12812
+  if (p.YPLUSWALL < -10.0f && (r = pwallgrad(c, d))) return r;  // mf:12812
   LAUNCH(k_sa_iforces, dim3(g.KMM1, g.JM - 1), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_sa_kwalls, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4), dim3(64, 4), 0, c->stream, g, d, p);
   LAUNCH(k_sa_kfaces, dim3((g.IMM1 + 63) / 64, (g.JM - 1 + 3) / 4, (g.KMM1 + kKSeg - 1) / kKSeg), dim3(64, 4), 0,
@@ -866,11 +902,18 @@ int new_loss(mh_ctx *c, float rf_vis, float fmixup, int nstep) {
 }
 
 // The viscous model's call for ILOS (mf:3935-3942, 5339-5341, 7354-7356).
-int viscous(mh_ctx *c, int ilos, float rf_vis, float fmixup, int nstep) {
-  if (ilos == 10) return loss(c, rf_vis, fmixup, nstep);
-  if (ilos >= 100 && ilos < 200) return new_loss(c, rf_vis, fmixup, nstep);
-  if (ilos >= 200) return spal_loss(c, rf_vis, fmixup, nstep);
+int viscous(mh_ctx *c, const Dev &d, int ilos, float rf_vis, bool vislam, bool first) {
+  if (ilos == 10) return loss(c, d, rf_vis, vislam);
+  if (ilos >= 100 && ilos < 200) return new_loss(c, d, rf_vis, vislam);
+  if (ilos >= 200) return spal_loss(c, d, rf_vis, vislam, first);
   return MH_OK;
+}
+
+// Forget the captured step graphs (after a scalar changes, at a new start,
+// at destroy).
+void drop_graphs(mh_ctx *c) {
+  for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
 }
 
 // The extra streams and scratch of the concurrent time steps (fast mode),
@@ -900,42 +943,34 @@ static int make_lanes(mh_ctx *c) {
   return MH_OK;
 }
 
-// One time step: the body of LOOP from label 5000 (mf:3753-5454).
-int step_once(mh_ctx *c) {
+// One time step: the body of LOOP from label 5000 (mf:3753-5454), queued on
+// the context's streams without waiting for anything.  The step's
+// bookkeeping runs on the device (k_step_begin); every launch after it is
+// gated on the device's CTL_SKIP (the run has ended), and the check block of
+// a step that may end the run by convergence (q.cond) on CTL_NOTEND.
+int enqueue_step(mh_ctx *c, const StepPlan &q) {
   const Geo &g = c->g;
   const Params p = params_of(c);
   const dim3 grid = node_grid(g);
   int r;
-  // bookkeeping, stop tests and the smoothing/damping ramp, mf:3760-3824
-  c->nstep = c->nstep + 1;
-  if (c->nstep == p.NSTEPS_MAX) c->ifend = 1;
-  if (c->EAVG < p.CONLIM && c->ECONT < 0.01f) c->ifend = 1;
-  if (fabsf(c->EMAX) < (4 * p.CONLIM) && c->ECONT < 0.01f) c->ifend = 1;
-  bool ifprint = false;
-  for (int i = 0; i < 5; i++) ifprint = ifprint || c->nstep == c->nout[i];
-  if (c->nstep % 10 == 0 && c->stop_req) c->ifend = 1;  // stopit, mf:3774-3779
-  float FCHANGE = 1.0f - (float)(c->nstep - 1) / (float)p.NCHANGE;
-  if (FCHANGE > 0.0f) {
-    c->SFT = c->SFTIN_s + 0.02f * FCHANGE;
-    c->SFX = c->SFXIN_s + 0.02f * FCHANGE;
-    c->DAMP = p.DAMPIN * (1.f - 0.75f * FCHANGE);
-  }
-  float FMIXUP = 1.0f + (p.FACMIXUP - 1.f) * (float)(p.NMIXUP - c->nstep) / (float)p.NMIXUP;
-  if (FMIXUP < 1.0f) FMIXUP = 1.0f;
-  c->FMIXUP = FMIXUP * FMIXUP;
-  const bool check5 = (c->nstep % 5 == 0) || c->ifend == 1;
-  Dev d = c->d;
+  const Dev d = lane_dev(c, 0, CTL_SKIP);
+  const Dev dn = lane_dev(c, 0, CTL_NOTEND);     // launches only the run's last step makes
+  const Dev &dc = q.check5 ? d : dn;               // the check block
   const StepScalars s = step_scalars(c);
-
+  {
+    Dev d0 = c->d;
+    d0.gate = -1;
+    LAUNCH(k_step_begin, dim3(1), dim3(64), 0, c->stream, g, d0, p);
+  }
   LAUNCH(k_prim, grid, kNodeBlock, 0, c->stream, g, d, p);
   LCHK("prim");
-  if (check5) {
-    LAUNCH(k_vrms_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
-    LAUNCH(k_vrms_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+  if (q.check5 || q.cond) {
+    LAUNCH(k_vrms_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, dc, d.RED);
+    LAUNCH(k_vrms_final, dim3(1), dim3(256), 0, c->stream, g, dc, (const double *)d.RED, kRedBlocks, p.EXACT);
     LCHK("vrms");
   }
   // viscous forces every NLOS steps, mf:3935-3942
-  if (p.ILOS != 0 && c->nstep % p.NLOS == 0 && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
+  if (q.visc && (r = viscous(c, d, p.ILOS, p.RF_VIS, q.vislam, q.first))) return r;
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
   // With lanes, the energy equation (its fluxes read only the mass fluxes,
@@ -944,18 +979,17 @@ int step_once(mh_ctx *c) {
   // primitives, PEFF and the body forces) on lanes 0, 1 and 2.  Each lane
   // has its own flux and TSTEP scratch; the boundary conditions and the
   // limiters, which read several variables, wait for all of them.
-  if (!p.EXACT && c->lanes_on && c->nlanes == 0 && (r = make_lanes(c))) return r;
-  const bool lanes = c->nlanes == kLanes && c->lanes_on;
+  const bool lanes = c->nlanes == kLanes && c->lanes_on && !p.EXACT && c->scal[MH_S_LANES] != 0.0;
   if (lanes) {
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     HIPCHK(hipStreamWaitEvent(c->lanes[0].stream, c->ev[0], 0));
-    const Dev d1 = lane_dev(c, 1);
+    const Dev d1 = lane_dev(c, 1, CTL_SKIP);
     LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->lanes[0].stream, g, d1);
     LCHK("flux_energy");
-    if ((r = tstep(c, 1, d.ROE, d.DROE, 3))) return r;
+    if ((r = tstep(c, 1, d.ROE, d.DROE, 3, q, CTL_SKIP))) return r;
     HIPCHK(hipEventRecord(c->ev[1], c->lanes[0].stream));
   }
-  if ((r = tstep(c, 0, d.RO, d.DRO, 1))) return r;
+  if ((r = tstep(c, 0, d.RO, d.DRO, 1, q, CTL_SKIP))) return r;
   if (p.ILOS >= 100 && p.ILOS < 200) {
     // cell-average density for NEW_LOSS, mf:4260-4270
     LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
@@ -965,7 +999,7 @@ int step_once(mh_ctx *c) {
     // transport of the turbulent viscosity, mf:4260-4351
     LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
     LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
-    if ((r = tstep(c, 0, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2))) return r;
+    if ((r = tstep(c, 0, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
     LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
     LCHK("sa_transport");
   }
@@ -974,7 +1008,7 @@ int step_once(mh_ctx *c) {
   } else {
     LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("flux_energy");
-    if ((r = tstep(c, 0, d.ROE, d.DROE, 3))) return r;
+    if ((r = tstep(c, 0, d.ROE, d.DROE, 3, q, CTL_SKIP))) return r;
   }
   LAUNCH(k_bc_lines, dim3(g.KM), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_bc_pd, dim3(1), dim3(128), 0, c->stream, g, d, p, s);
@@ -991,32 +1025,31 @@ int step_once(mh_ctx *c) {
   if (lanes) {
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     for (int l = 0; l < kLanes; l++) HIPCHK(hipStreamWaitEvent(c->lanes[l].stream, c->ev[2], 0));
-    const Dev d1 = lane_dev(c, 1), d2 = lane_dev(c, 2);
+    const Dev d1 = lane_dev(c, 1, CTL_SKIP), d2 = lane_dev(c, 2, CTL_SKIP);
     LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->lanes[0].stream, g, d1);
     LCHK("flux_angular");
-    if ((r = tstep(c, 1, d.RORVT, d.DRORVT, 5))) return r;
+    if ((r = tstep(c, 1, d.RORVT, d.DRORVT, 5, q, CTL_SKIP))) return r;
     HIPCHK(hipEventRecord(c->ev[3], c->lanes[0].stream));
     LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->lanes[1].stream, g, d2);
     LCHK("flux_radial");
-    if ((r = tstep(c, 2, d.ROVR, d.DROVR, 6))) return r;
+    if ((r = tstep(c, 2, d.ROVR, d.DROVR, 6, q, CTL_SKIP))) return r;
     HIPCHK(hipEventRecord(c->ev[4], c->lanes[1].stream));
   }
   LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("flux_axial");
-  if ((r = tstep(c, 0, d.ROVX, d.DROVX, 4))) return r;
+  if ((r = tstep(c, 0, d.ROVX, d.DROVX, 4, q, CTL_SKIP))) return r;
   if (lanes) {
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev[3], 0));
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev[4], 0));
   } else {
     LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("flux_angular");
-    if ((r = tstep(c, 0, d.RORVT, d.DRORVT, 5))) return r;
+    if ((r = tstep(c, 0, d.RORVT, d.DRORVT, 5, q, CTL_SKIP))) return r;
     LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("flux_radial");
-    if ((r = tstep(c, 0, d.ROVR, d.DROVR, 6))) return r;
+    if ((r = tstep(c, 0, d.ROVR, d.DROVR, 6, q, CTL_SKIP))) return r;
   }
   // limiters, mf:5115-5168
-  d = c->d;
   const int nline = g.IM * g.KM;
   LAUNCH(k_rolim_check, grid, kNodeBlock, 0, c->stream, g, d, s);
   LAUNCH(k_rolim_fix, dim3((nline + 63) / 64), dim3(64), 0, c->stream, g, d, s, p);
@@ -1027,34 +1060,80 @@ int step_once(mh_ctx *c) {
     LAUNCH(k_mixplan, dim3(nl), dim3(64), 9 * g.IM * sizeof(float), c->stream, g, d, p, c->lists.mixscratch);
     LCHK("mixplan");
   }
-  if (check5) {
-    LAUNCH(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, d, p.EXACT);
-    LAUNCH(k_econt, dim3(1), dim3(256), 0, c->stream, g, d);
-    LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, d, d.RED);
-    LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, d, (const double *)d.RED, kRedBlocks, p.EXACT);
+  if (q.check5 || q.cond) {
+    LAUNCH(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, dc, p.EXACT);
+    LAUNCH(k_econt, dim3(1), dim3(256), 0, c->stream, g, dc);
+    LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, dc, d.RED);
+    LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, dc, (const double *)d.RED, kRedBlocks, p.EXACT);
+    LAUNCH(k_nan_check, dim3(1), dim3(64), 0, c->stream, g, dc);
     LCHK("convergence");
     // forces brought up to date before a printout or the final output,
-    // mf:5338-5341
-    if ((ifprint || c->ifend == 1) && (r = viscous(c, p.ILOS, p.RF_VIS, c->FMIXUP, c->nstep))) return r;
-    if ((r = stepup(c, 0.25f))) return r;
-    float sc[8];
-    HIPCHK(hipMemcpyAsync(sc, d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->EMAX = sc[SC_EMAX];
-    c->EAVG = sc[SC_EAVG];
-    c->ECONT = sc[SC_ECONT];
-    c->last.emax = sc[SC_EMAX];
-    c->last.eavg = sc[SC_EAVG];
-    c->last.econt = sc[SC_ECONT];
-    c->last.vref = sc[SC_VREF];
-    c->last.vmax = sc[SC_VMAX];
-    c->last.flow_in = sc[SC_FLOWRAT];
-    c->last.flow_out = sc[SC_FLOWOUT];
-    if (!std::isfinite(sc[SC_VMAX]) || !std::isfinite(sc[SC_FLOWRAT]))
-      return fail(c, MH_E_NAN, "non-finite flow field at step " + std::to_string(c->nstep));
+    // mf:5338-5341 (on a step that may end by convergence: if it does)
+    if (q.refresh && (r = viscous(c, dc, p.ILOS, p.RF_VIS, q.vislam, q.first))) return r;
+    if (q.cond && !q.refresh && (r = viscous(c, dn, p.ILOS, p.RF_VIS, q.vislam, q.first))) return r;
+    if ((r = stepup(c, dc, 0.25f))) return r;
   }
+  return MH_OK;
+}
+
+// Steps n0+1 .. n0+m as one captured HIP graph (built on first use and kept
+// per step pattern), or queued launch by launch when graphs are off or the
+// kernels are being timed.
+int enqueue_chunk(mh_ctx *c, const std::vector<StepPlan> &plan) {
+  const bool graphs = c->scal[MH_S_USE_GRAPHS] != 0.0 && !c->prof_on;
+  if (!graphs) {
+    for (const StepPlan &q : plan) {
+      int r = enqueue_step(c, q);
+      if (r) return r;
+    }
+    return MH_OK;
+  }
+  std::string key = c->lanes_on ? "L" : "S";
+  for (const StepPlan &q : plan) key += std::to_string(q.code()) + ",";
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = MH_OK;
+    for (const StepPlan &q : plan)
+      if ((r = enqueue_step(c, q))) break;
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (r) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return r;
+    }
+    if (e != hipSuccess || !gr) return fail(c, MH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e2 != hipSuccess) return fail(c, MH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+  return MH_OK;
+}
+
+// The run's state as the device left it: step count, end flag and the last
+// check's figures (one small copy).
+int read_back(mh_ctx *c) {
+  int ctl[CTL_COUNT];
+  float sc[16];
+  HIPCHK(hipMemcpyAsync(ctl, c->d.CTL, sizeof ctl, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(sc, c->d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nstep = ctl[CTL_NSTEP];
+  c->ifend = (ctl[CTL_IFEND] || ctl[CTL_DONE]) ? 1 : 0;
   c->last.nstep = c->nstep;
   c->last.ifend = c->ifend;
+  c->last.emax = sc[SC_EMAX];
+  c->last.eavg = sc[SC_EAVG];
+  c->last.econt = sc[SC_ECONT];
+  c->last.vref = sc[SC_VREF];
+  c->last.vmax = sc[SC_VMAX];
+  c->last.flow_in = sc[SC_FLOWRAT];
+  c->last.flow_out = sc[SC_FLOWOUT];
+  if (ctl[CTL_NANSTEP])
+    return fail(c, MH_E_NAN, "non-finite flow field at step " + std::to_string(ctl[CTL_NANSTEP]));
   return MH_OK;
 }
 
@@ -1148,10 +1227,16 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.CHECK_FLOW, (size_t)dm.jm)) || (r = dalloc(c, &d.BLADE_FLOW, (size_t)dm.jm)) ||
       (r = dalloc(c, &d.SETFLO, (size_t)2 * dm.jm)) ||
       (r = dalloc(c, &d.KWALL, (size_t)kKWall * dm.im * dm.jm)) ||
-      (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) ||
+      (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) || (r = dalloc(c, &d.CTL, (size_t)CTL_COUNT)) ||
       (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
     return bail(r);
   d.AVG = d.SC + SC_AVG_CHG;
+  d.gate = -1;
+  c->scal[MH_S_USE_GRAPHS] = 1.0;  // default: steps run as captured graphs
+  c->scal[MH_S_LANES] = 1.0;       // default: concurrent equation streams (fast mode)
+  if (hipHostMalloc((void **)&c->ctl_host, CTL_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_poll, hipEventDisableTiming) != hipSuccess)
+    return bail(MH_E_HIP);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
   if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1) > kMaxLds) return bail(MH_E_UNSUPPORTED);
@@ -1177,6 +1262,9 @@ int mh_destroy(mh_ctx *c) {
     }
   for (auto e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  drop_graphs(c);
+  if (c->ev_poll) (void)hipEventDestroy(c->ev_poll);
+  if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1231,6 +1319,7 @@ int mh_set_scalar(mh_ctx *c, int id, double v) {
   if (!c || id < 0 || id >= MH_NSCALARS) return fail(c, MH_E_ARG, "bad scalar id");
   c->scal[id] = v;
   c->scal_set[id] = true;
+  drop_graphs(c);  // captured launches hold the scalars by value
   return MH_OK;
 }
 
@@ -1245,7 +1334,8 @@ double mh_get_scalar(const mh_ctx *c, int id) {
 static int check_inputs(mh_ctx *c) {
   const bool forcing = c->scal_set[MH_S_IN_FLOW] && si(c, MH_S_IN_FLOW) != 0;
   for (int i = 0; i < MH_NSCALARS; i++)
-    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_VISLAM && i != MH_S_FTCOND &&
+    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_USE_GRAPHS && i != MH_S_LANES && i != MH_S_VISLAM &&
+        i != MH_S_FTCOND &&
         (forcing || (i != MH_S_FLOWIN && i != MH_S_RFLOW)))
       return fail(c, MH_E_STATE, std::string("scalar not set: ") + kScalarNames[i]);
   const Params p = params_of(c);
@@ -1269,10 +1359,12 @@ int mh_loss_init(mh_ctx *c) {
   if (p.ILOS == 0) return MH_OK;
   float vf[2] = {(float)c->scal[MH_S_VISLAM], (float)c->scal[MH_S_FTCOND]};
   if ((r = h2d(c, c->d.SC + SC_VISLAM, vf, sizeof vf))) return r;
+  const float one = 1.0f;  // SETUP's call has FMIXUP = 1 (mf:7350)
+  if ((r = h2d(c, c->d.SC + SC_FMIXUP, &one, sizeof one))) return r;
   if (p.ILOS == 10) {
-    if ((r = loss_tables(c)) || (r = loss(c, 1.0f, 1.0f, 1))) return r;
+    if ((r = loss_tables(c)) || (r = loss(c, c->d, 1.0f, true))) return r;
   } else {
-    if ((r = sa_maps(c)) || (r = viscous(c, p.ILOS, 1.0f, 1.0f, 1))) return r;
+    if ((r = sa_maps(c)) || (r = viscous(c, c->d, p.ILOS, 1.0f, true, true))) return r;
   }
   if ((r = d2h(c, vf, c->d.SC + SC_VISLAM, sizeof vf))) return r;
   c->scal[MH_S_VISLAM] = vf[0];
@@ -1388,35 +1480,84 @@ int mh_start(mh_ctx *c) {
   c->PREFF = 0.5f * (pa + pb);
   c->SFXIN_s = sf(c, MH_S_SFXIN) * sf(c, MH_S_CFL) / 0.5f;
   c->SFTIN_s = sf(c, MH_S_SFTIN) * sf(c, MH_S_CFL) / 0.5f;
-  c->ECONT = 1.f;
-  c->EAVG = 1.0e06f;
-  c->EMAX = 1.0e06f;
   c->ifend = 0;
   c->nstep = 0;
+  c->hstep = 0;
+  c->DAMP = 0.0f;
   c->stop_req = false;
+  drop_graphs(c);  // they hold this run's scalars
+  {
+    // the device's step state: NSTEP = 0, the stop tests' figures as
+    // before any check (mf:3697-3699), the ramps not yet started
+    const std::vector<int> ctl(CTL_COUNT, 0);
+    float sc[16] = {};
+    if ((r = d2h(c, sc, c->d.SC, sizeof sc))) return r;
+    sc[SC_EMAX] = 1.0e06f;
+    sc[SC_EAVG] = 1.0e06f;
+    sc[SC_ECONT] = 1.f;
+    sc[SC_SFT] = sc[SC_SFX] = sc[SC_DAMP] = 0.0f;
+    sc[SC_FMIXUP] = 1.0f;
+    if ((r = h2d(c, c->d.CTL, ctl.data(), CTL_COUNT * sizeof(int))) || (r = h2d(c, c->d.SC, sc, sizeof sc))) return r;
+  }
   if ((r = d2h(c, c->nout, c->d.NOUT, sizeof c->nout))) return r;
   c->last = mh_resid{};
   c->started = true;
   return MH_OK;
 }
 
+// Steps are queued in chunks that end on the every-5-step checks, each chunk
+// one graph launch; nothing waits for the device until the call returns.
+// While queuing, the device's end flag is read back without blocking (a
+// copy into pinned memory, polled with an event), so queuing stops soon
+// after a run ends by convergence; the steps queued beyond it are not taken
+// on the device.
 int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
   if (!c) return MH_E_ARG;
   if (!c->started) return fail(c, MH_E_STATE, "mh_step before mh_start");
-  for (int i = 0; i < nsteps && !c->ifend; i++) {
-    int r = step_once(c);
-    if (r) return r;
+  if (nsteps < 0) return fail(c, MH_E_ARG, "nsteps < 0");
+  const Params p = params_of(c);
+  int r;
+  if (!p.EXACT && c->lanes_on && c->nlanes == 0 && (r = make_lanes(c))) return r;
+  bool end = c->ifend != 0, polling = false;
+  int queued = 0;
+  std::vector<StepPlan> plan;
+  while (queued < nsteps && !end) {
+    const int hn = c->hstep;
+    const int len = std::min(nsteps - queued, 5 - hn % 5);
+    plan.clear();
+    for (int i = 1; i <= len; i++) {
+      const int n = hn + i;
+      plan.push_back(plan_step(c, n));
+      if (n == p.NSTEPS_MAX || (c->stop_req && n % 10 == 0)) {
+        end = true;
+        break;
+      }
+    }
+    if ((r = enqueue_chunk(c, plan))) return r;
+    c->hstep = hn + (int)plan.size();
+    queued += (int)plan.size();
+    if (!polling) {
+      HIPCHK(hipMemcpyAsync(c->ctl_host, c->d.CTL, CTL_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipEventRecord(c->ev_poll, c->stream));
+      polling = true;
+    } else if (hipEventQuery(c->ev_poll) == hipSuccess) {
+      polling = false;
+      if (c->ctl_host[CTL_DONE] || c->ctl_host[CTL_IFEND]) end = true;
+    }
     if (c->prof_on && c->prof_pending.size() > 4096 && (r = prof_collect(c))) return r;
   }
+  r = read_back(c);
+  c->hstep = c->nstep;  // the device may have ended the run before the host's last queued step
   if (out) *out = c->last;
-  return MH_OK;
+  return r;
 }
 
 int mh_request_stop(mh_ctx *c) {
   if (!c) return MH_E_ARG;
   if (!c->started) return fail(c, MH_E_STATE, "mh_request_stop before mh_start");
   c->stop_req = true;
-  return MH_OK;
+  const int one = 1;
+  return h2d(c, c->d.CTL + CTL_STOPREQ, &one, sizeof one);
 }
 
 int mh_step_batch(mh_ctx *const *ctxs, int n, int nsteps, mh_resid *out, int *failed) {

@@ -12,6 +12,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -81,7 +82,27 @@ struct Dev {
   float *AVG;                        // per row: the damping's mean |change| of the current TSTEP call
   float *KWALL;                      // kKWall planes of (IM-1) x (JM-1) end-wall column terms
   double *INTEG;                     // 7*JM FP64 integrals
+  int *CTL;                          // device-resident step control (see CTL_* below)
+  int gate;                          // CTL slot whose non-zero value makes a kernel return at once (-1: none)
 };
+
+// Slots of Dev::CTL, the step bookkeeping of LOOP (mf:3753-3824) kept on the
+// device so that steps can be queued (and captured in graphs) without the
+// host reading the convergence figures back.
+enum {
+  CTL_NSTEP = 0,   // NSTEP
+  CTL_IFEND,       // IFEND of the current step
+  CTL_DONE,        // the run ended at an earlier step (or on a non-finite check)
+  CTL_SKIP,        // this step is not taken (the run has ended)
+  CTL_NOTEND,      // SKIP, or this step is not the run's last: gates launches that only the last step makes
+  CTL_STOPREQ,     // a stop request (stopit) is pending
+  CTL_NANSTEP,     // the step whose convergence check found a non-finite value (0: none)
+  CTL_COUNT = 16
+};
+
+// A kernel launched as part of a time step returns at once when its gate is
+// set (the run has ended, or a launch that only a run's last step makes).
+__device__ __forceinline__ bool gated(const Dev &d) { return d.gate >= 0 && d.CTL[d.gate] != 0; }
 
 // SA/NEW_LOSS end-wall terms per (I,J) column (forces, energy, friction
 // velocity and density for hub and casing), and the cells of a column one
@@ -93,6 +114,8 @@ constexpr int kKSeg = 16;
 enum {
   SC_VRMS = 0, SC_VMAX, SC_EMAX, SC_EAVG, SC_ECONT, SC_VREF, SC_FLOWRAT, SC_FLOWOUT, SC_VISLAM, SC_FTCOND,
   SC_VORTFAC, SC_PGRADFAC,  // SPAL_LOSS's helicity and pressure-gradient source factors
+  SC_SFT, SC_SFX, SC_DAMP,  // the step's smoothing and damping factors (ramped over NCHANGE steps)
+  SC_FMIXUP,                // the step's FMIXUP (squared), which the viscous calls read
   SC_AVG_CHG = 16,          // [NROWS]
   SC_AVG_BLK = 16 + 64,     // [NROWS]
   SC_VLIMIT = 16 + 128,     // [NROWS]
@@ -113,10 +136,10 @@ __host__ __device__ inline int lt_span(int tab, int NR, int K, int NROWS, int IM
   return 4 * NROWS * IM + (tab * NROWS + NR - 1) * KM + K - 1;
 }
 
-// Per-step host scalars handed to the kernels.
+// Scalars fixed for a run (set at the start of LOOP) handed to the kernels;
+// those that change from step to step live on the device (SC_SFT ..).
 struct StepScalars {
-  float SFT, SFX, DAMP, FMIXUP, PREFF, ROLIM, TCOOL_MIN, SFEX1, SFEXH, RFIN1, RFIN2;
-  int NSTEP, IFEND;
+  float PREFF, ROLIM, TCOOL_MIN, SFEX1, SFEXH, RFIN1, RFIN2;
 };
 
 // Copy of the scalar table used by the kernels (floats and ints).
@@ -152,6 +175,7 @@ inline size_t scatter_lds_bytes(int imm1) {
 struct MgFast {
   const int *b2_ptr = nullptr, *b2_off = nullptr;  // level-2 block -> offsets into P of its members
   double *P = nullptr, *PT = nullptr;
+  const int *gate = nullptr;                       // the launch's gate flag (null: none)
 };
 
 struct Lists {
@@ -201,13 +225,17 @@ struct mh_ctx {
   bool started = false;
   // loop state (mf:3674-3714 and the per-step bookkeeping)
   int nstep = 0, ifend = 0;
+  int hstep = 0;            // steps the host has queued (the device may stop earlier)
+  std::map<std::string, hipGraphExec_t> graphs;  // captured step chunks, by step pattern
+  int *ctl_host = nullptr;  // pinned copy of the device's CTL, for polling the end flag
+  hipEvent_t ev_poll = nullptr;
   int nout[5] = {};       // printout steps (read from the device at mh_start)
   bool stop_req = false;  // mh_request_stop: end at the next step divisible by 10
   int device = 0;             // the HIP device the context was created on
   float *xl_scratch = nullptr;  // that device's address of mh_set_xlength_noalloc's scratch
-  float SFT = 0, SFX = 0, DAMP = 0, FMIXUP = 1, PREFF = 0, ROLIM = 0, ROMIN = 0;
+  float DAMP = 0;                            // host mirror of the device's ramped DAMP (launch choices)
+  float PREFF = 0, ROLIM = 0, ROMIN = 0;
   float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
-  float EAVG = 1e6f, EMAX = 1e6f, ECONT = 1.f;
   mh_resid last{};
   std::string err;
   // concurrent time steps (fast mode): extra streams and their fork/join events

@@ -82,6 +82,7 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 // primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
 
 __global__ void k_prim(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   NODE_IJK
   if (J < 2) return;
   const size_t n = IX(I, J, K);
@@ -123,6 +124,7 @@ __device__ __forceinline__ float flowt_raw(const Geo &g, const Dev &d, int I, in
 }
 
 __global__ void k_massflux(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
@@ -174,6 +176,7 @@ __device__ __forceinline__ float tflux_energy(const Geo &g, const Dev &d, int I,
 }
 
 __global__ void k_flux_energy(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
@@ -205,6 +208,7 @@ __device__ __forceinline__ float tflux_axial(const Geo &g, const Dev &d, int I, 
 }
 
 __global__ void k_flux_axial(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
@@ -245,6 +249,7 @@ __device__ __forceinline__ float tflux_angular(const Geo &g, const Dev &d, int I
 }
 
 __global__ void k_flux_angular(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
@@ -272,6 +277,7 @@ __global__ void k_flux_angular(Geo g, Dev d) {
 // radial momentum fluxes and source, mf:5003-5095
 
 __global__ void k_flux_radial(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
@@ -355,6 +361,7 @@ __device__ __forceinline__ void inlet_pt(const Geo &g, const Dev &d, const Param
 // into d.BCW (PAVG2, PAVG3, PAVGOUT at 0, KM, 2 KM; the swirl sums at
 // 3 KM + 4).
 __global__ void __launch_bounds__(64) k_bc_lines(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int K = blockIdx.x + 1, lane = threadIdx.x;
   const int JM = g.JM, JMM1 = g.JMM1, IMM1 = g.IMM1;
   float v2[4], v3[4], vo[4], vs[4];
@@ -394,6 +401,7 @@ __global__ void __launch_bounds__(64) k_bc_lines(Geo g, Dev d, Params p) {
 // k_bc_apply.
 constexpr int kBcMaxK = 512;
 __global__ void __launch_bounds__(128) k_bc_pd(Geo g, Dev d, Params p, StepScalars s) {
+  if (gated(d)) return;
   __shared__ float sumk[kBcMaxK], rjm[kBcMaxK], pdold[kBcMaxK], pdnew[kBcMaxK];
   const float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *SUMK = d.BCW + 3 * g.KM + 4;
   const int KM = g.KM, JM = g.JM;
@@ -435,6 +443,7 @@ __global__ void __launch_bounds__(128) k_bc_pd(Geo g, Dev d, Params p, StepScala
 }
 
 __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
+  if (gated(d)) return;
   const float *PAVG2 = d.BCW, *PAVG3 = d.BCW + g.KM, *PAVGOUT = d.BCW + 2 * g.KM, *PDL = d.PD;
   const int JM = g.JM, JMM1 = g.JMM1, IM = g.IM;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -520,6 +529,7 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
 
 // PEFF, mf:4751-4757
 __global__ void k_peff(Geo g, Dev d, Params p, StepScalars s) {
+  if (gated(d)) return;
   NODE_IJK
   int JP1 = J + 1;
   if (J == g.JM) JP1 = g.JM;
@@ -532,6 +542,7 @@ __global__ void k_peff(Geo g, Dev d, Params p, StepScalars s) {
 
 // upstream-face flux extrapolation at mixing planes, mf:5516-5555
 __global__ void k_fextrap(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int t = blockIdx.x;  // one wavefront per (mixing plane, K) line
   const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
   if (NR > g.NROWS - 1) return;
@@ -565,6 +576,7 @@ __global__ void k_fextrap(Geo g, Dev d, Params p) {
 // flux sum, scree/SSS history (mf:5560-5570), with the tip-gap TFLUX
 // balance of mf:5503-5510 applied to the two periodic faces it touches.
 __global__ void k_residual(Geo g, Dev d, float *D, float *DIFF, Params p) {
+  if (gated(d)) return;
   NODE_IJK
   if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
   const size_t n = IX(I, J, K);
@@ -587,6 +599,7 @@ __global__ void k_residual(Geo g, Dev d, float *D, float *DIFF, Params p) {
 // pitchwise averaging of STORE at the mixing planes, mf:5578-5607 (pass 1,
 // FP-weighted) and mf:5786-5818 (pass 2, plain mean)
 __global__ void k_mpavg(Geo g, Dev d, Params p, int pass) {
+  if (gated(d)) return;
   const int t = blockIdx.x;  // one wavefront per (mixing plane, K) line
   const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
   if (NR > g.NROWS - 1) return;
@@ -620,6 +633,7 @@ __global__ void k_mpavg(Geo g, Dev d, Params p, int pass) {
 // Level 1 (3x3x3 cells): one thread per block, reference order K, J, I.
 __global__ void k_mg_sum_seq(Geo g, Dev d, const int *jptr, const int *jlist, int ni, int nj, int nk, int ir, int kr,
                              float *out) {
+  if (gated(d)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ni * nj * nk) return;
   const int ib = t % ni + 1, jb = (t / ni) % nj + 1, kb = t / (ni * nj) + 1;
@@ -638,6 +652,7 @@ __global__ void k_mg_sum_seq(Geo g, Dev d, const int *jptr, const int *jlist, in
 // block's cells, fixed-order tree at the end (deterministic).
 __global__ void k_mg_sum_wave(Geo g, Dev d, const int *jptr, const int *jlist, int ni, int nj, int nk, int ir, int kr,
                               float *out) {
+  if (gated(d)) return;
   const int t = blockIdx.x;
   if (t >= ni * nj * nk) return;
   const int ib = t % ni + 1, jb = (t / ni) % nj + 1, kb = t / (ni * nj) + 1;
@@ -658,6 +673,7 @@ __global__ void k_mg_sum_wave(Geo g, Dev d, const int *jptr, const int *jlist, i
 // workgroup per J plane sums its cells (optionally |STORE|) in FP64 with a
 // fixed-order tree.
 __global__ void k_jplane_sums(Geo g, Dev d, double *jpart, int absval) {
+  if (gated(d)) return;
   __shared__ double sh[256];
   const int J = blockIdx.x + 1;
   double s = 0.0;
@@ -680,7 +696,8 @@ __global__ void k_jplane_sums(Geo g, Dev d, double *jpart, int absval) {
 
 // stage 2: one wavefront per J range sums the per-J partials
 __global__ void k_range_sums(Geo g, const double *jpart, const int *jptr, const int *jlist, int nranges, float *out,
-                             int divide_by_cells) {
+                             int divide_by_cells, const int *gate) {
+  if (gate && *gate) return;
   const int r = blockIdx.x;
   if (r >= nranges) return;
   const int q0 = jptr[r], nj = jptr[r + 1] - q0;
@@ -697,6 +714,7 @@ __global__ void k_range_sums(Geo g, const double *jpart, const int *jptr, const 
 // sequential-order versions for EXACT mode (reference order K, J, I)
 __global__ void k_range_sums_seq(Geo g, Dev d, const int *jptr, const int *jlist, int nranges, float *out,
                                  int absval, int divide_by_cells) {
+  if (gated(d)) return;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nranges) return;
   float s = 0.0f;
@@ -714,6 +732,7 @@ __global__ void k_range_sums_seq(Geo g, Dev d, const int *jptr, const int *jlist
 
 // multigrid update, mf:5645-5660
 __global__ void k_mg_apply(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
   const size_t n = IX(I, J, K);
@@ -728,7 +747,10 @@ __global__ void k_mg_apply(Geo g, Dev d) {
 
 // SMOOTH_RESID, mf:14833-14917: one pass in J, then I, then K, within each
 // row (rows never share cells).  dir 0/1/2 = J/I/K; src -> dst.
-__global__ void k_smooth_resid(Geo g, Dev d, const float *src, float *dst, float SMFAC, int dir) {
+// SMFAC is the smoothing factor, times the step's SFT when sft is set
+__global__ void k_smooth_resid(Geo g, Dev d, const float *src, float *dst, float SMFAC, int sft, int dir) {
+  if (gated(d)) return;
+  if (sft) SMFAC = SMFAC * d.SC[SC_SFT];
   NODE_IJK
   if (I > g.IMM1 || K > g.KMM1) return;
   const int NR = d.NROW[J - 1];
@@ -766,7 +788,9 @@ __device__ __forceinline__ float avg_blk(const float *AC, int NR, int NROWS) {
   return 0.25f * (AC[NR - 2] + AC[NR]) + 0.5f * AC[NR - 1];
 }
 
-__global__ void k_damp_apply(Geo g, Dev d, float DAMP) {
+__global__ void k_damp_apply(Geo g, Dev d) {
+  if (gated(d)) return;
+  const float DAMP = d.SC[SC_DAMP];
   NODE_IJK
   if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
   const size_t n = IX(I, J, K);
@@ -781,6 +805,7 @@ __global__ void k_damp_apply(Geo g, Dev d, float DAMP) {
 // gather: node (I,J,K) receives from cells I-1..I, J..J+1, K-1..K, summed in
 // the order the reference's cell loop (K, J, I) visits them.
 __global__ void k_scatter(Geo g, Dev d, const float *D, float *DOUT) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   float v = D[n];
@@ -891,6 +916,7 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float 
 #define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
 __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p, MgFast m, int kg, int ir,
                                                     int part) {
+  if (gated(d)) return;
   __shared__ double sh[4][256];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -993,6 +1019,7 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
 // level-1 K blocks); fixed-order trees throughout.
 __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b1p, const int *b1l, const int *sbp,
                                                    const int *sbl, float *B1, float *B2, float *SB) {
+  if (m.gate && *m.gate) return;
   const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
   const int nblk1 = (n1 + 255) / 256, nblk2 = (n2 + 3) / 4;
   const int bx = blockIdx.x;
@@ -1055,6 +1082,7 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b
 // lines), leaving each line's FP64 sum of |STORE| for the damping average
 // in red[(J,K)]
 __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *red) {
+  if (gated(d)) return;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
@@ -1108,6 +1136,7 @@ __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *re
 // workgroup of 16 wavefronts per row (a contiguous J range); each lane sums
 // its share, then fixed-order butterflies within and across wavefronts
 __global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *red, const int *rptr, const int *rlist) {
+  if (gated(d)) return;
   __shared__ double sh[16];
   const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, w = wave_id();
   const int q0 = rptr[r], nj = rptr[r + 1] - q0, ja = rlist[q0], tot = nj * g.KMM1;
@@ -1162,7 +1191,8 @@ constexpr int kTJ = kScatTJ, kTK = kScatTK, kTL = (kTJ + 1) * (kTK + 1), kMaxLn 
 static_assert(kTJ == 4, "one wavefront per tile J line");
 template <int NC>
 __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *D, float *DOUT,
-                                             float DAMP, int damp, int mpav) {
+                                             int damp, int mpav) {
+  const float DAMP = d.SC[SC_DAMP];
   extern __shared__ float lds[];
   __shared__ float avg[kTJ + 1], rad[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
   __shared__ int flag[kTL];
@@ -1325,12 +1355,14 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   }
 }
 __global__ void __launch_bounds__(256) k_damp_scatter1(Geo g, Dev d, Params p, const float *D, float *DOUT,
-                                                       float DAMP, int damp, int mpav) {
-  damp_scatter<1>(g, d, p, D, DOUT, DAMP, damp, mpav);
+                                                       int damp, int mpav) {
+  if (gated(d)) return;
+  damp_scatter<1>(g, d, p, D, DOUT, damp, mpav);
 }
 __global__ void __launch_bounds__(256) k_damp_scatter4(Geo g, Dev d, Params p, const float *D, float *DOUT,
-                                                       float DAMP, int damp, int mpav) {
-  damp_scatter<4>(g, d, p, D, DOUT, DAMP, damp, mpav);
+                                                       int damp, int mpav) {
+  if (gated(d)) return;
+  damp_scatter<4>(g, d, p, D, DOUT, damp, mpav);
 }
 
 // SMOOTH_VAR (mf:15352-15594) followed by TSTEP's periodic, tip-gap and
@@ -1344,6 +1376,7 @@ __global__ void __launch_bounds__(256) k_damp_scatter4(Geo g, Dev d, Params p, c
 // floats.
 __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Params p,
                                                          StepScalars s, int ncall, int j0) {
+  if (gated(d)) return;
   extern __shared__ float lds[];
   const int J = xcd_block(blockIdx.x, gridDim.x) + j0;
   const int lane = threadIdx.x & 63, w = wave_id();
@@ -1365,10 +1398,10 @@ __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const flo
     cFKD[t] = d.FKD[t];
     cFKU[t] = d.FKU[t];
   }
-  const float SFX = s.SFX, SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
+  const float SFX = d.SC[SC_SFX], SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
   const float SFXB = p.FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
   const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
-  const float SFT = s.SFT, SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
+  const float SFT = d.SC[SC_SFT], SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
   const int NR = d.NROW[J - 1];
   const int J1 = d.JSTART[NR - 1], J2 = d.JMIX[NR - 1], JLED = d.JLED[NR - 1];
   // streamwise smoothing, mf:15369-15445.  The case depends on J only.
@@ -1583,10 +1616,10 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
   const int I = lane + 1, Ic = min(I, IM);
   const int kbase = w * L + 1;
   const unsigned plane = (unsigned)IM * (unsigned)g.JM;
-  const float SFX = s.SFX, SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
+  const float SFX = d.SC[SC_SFX], SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
   const float SFXB = p.FSMTHB * SFX, SFXB1 = 1.0f - SFXB, SFXBH = 0.5f * SFXB;
   const float SFXH = 0.5f * SFX, SFXHM1 = 1.f - SFXH, SFX14 = 0.25f * SFX;
-  const float SFT = s.SFT, SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
+  const float SFT = d.SC[SC_SFT], SFT1 = 1.0f - SFT, SFTH = 0.5f * SFT;
   const int NR = d.NROW[J - 1];
   const int J1 = d.JSTART[NR - 1], J2 = d.JMIX[NR - 1], JLED = d.JLED[NR - 1];
   const bool smx = !(SFX < 0.0001f);
@@ -1780,6 +1813,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
 #define SMOOTH_REG(L)                                                                                         \
   __global__ void __launch_bounds__(512) k_smooth_reg##L(Geo g, Dev d, const float *SRC, float *DST, Params p, \
                                                          StepScalars s, int ncall, int j0) {                   \
+    if (gated(d)) return;                                                                                     \
     smooth_reg<L>(g, d, SRC, DST, p, s, ncall, j0);                                                           \
   }
 SMOOTH_REG(1)
@@ -1845,6 +1879,7 @@ __device__ float wallfun(float YPLUSWALL, float VISLAM, float PERP, float DPDS, 
 }
 
 __global__ void k_loss_vislam(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   float VISLAM = d.SC[SC_VISLAM];
   if (p.REYNO > 100.f) {
@@ -1942,7 +1977,9 @@ struct LossS {
 };
 
 // Stresses on the K = const faces, forces relaxed by RF_VIS, mf:8201-8556.
-__global__ void k_loss_span(Geo g, Dev d, Params p, float RF_VIS, float FMIXUP) {
+__global__ void k_loss_span(Geo g, Dev d, Params p, float RF_VIS) {
+  if (gated(d)) return;
+  const float FMIXUP = d.SC[SC_FMIXUP];
   const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
   if (I > g.IMM1 || J > g.JM) return;
@@ -2088,7 +2125,9 @@ __device__ __forceinline__ float wave_max(float x) {
 // forces, mf:8562-9001.  One 64-thread workgroup per (J,K) line.
 constexpr int kLossLine = 260;  // IM <= 257
 
-__global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float RF_VIS, float FMIXUP) {
+__global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float RF_VIS) {
+  if (gated(d)) return;
+  const float FMIXUP = d.SC[SC_FMIXUP];
   __shared__ float AREA[kLossLine], VXA[kLossLine], VRA[kLossLine], VTA[kLossLine], WTA[kLossLine];
   __shared__ float ROA[kLossLine], TA[kLossLine], WABS[kLossLine];
   __shared__ float SX[kLossLine], SR[kLossLine], ST[kLossLine], SQ[kLossLine], SW[kLossLine];
@@ -2294,6 +2333,7 @@ __global__ void __launch_bounds__(64) k_loss_pitch(Geo g, Dev d, Params p, float
 
 // cell-average density, mf:4260-4270
 __global__ void k_sa_cellavg(Geo g, Dev d) {
+  if (gated(d)) return;
   CELL_IJK
   const float *RO = d.RO;
   d.ROAVG_CELL[IX(I, J, K)] = 0.125f * (RO[IX(I, J, K)] + RO[IX(I + 1, J, K)] + RO[IX(I + 1, J, K + 1)] +
@@ -2303,6 +2343,7 @@ __global__ void k_sa_cellavg(Geo g, Dev d) {
 
 // doubled face fluxes and source of the turbulent viscosity, mf:4279-4330
 __global__ void k_sa_fluxes(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const int IMM1 = g.IMM1, KMM1 = g.KMM1;
   const float *TK = d.TRANS_KVIS;
@@ -2333,7 +2374,9 @@ __global__ void k_sa_fluxes(Geo g, Dev d) {
 
 // D += STORE (damped as TSTEP's negative feedback leaves it) on the cells,
 // mf:5716-5744
-__global__ void k_sa_addstore(Geo g, Dev d, float *D, float DAMP, int damp) {
+__global__ void k_sa_addstore(Geo g, Dev d, float *D, int damp) {
+  if (gated(d)) return;
+  const float DAMP = d.SC[SC_DAMP];
   CELL_IJK
   const unsigned n = IX(I, J, K);
   const float x = d.STORE[n];
@@ -2343,6 +2386,7 @@ __global__ void k_sa_addstore(Geo g, Dev d, float *D, float DAMP, int damp) {
 // pitch average carried across each mixing plane, mf:5748-5762 (one
 // wavefront per (interface, K) line)
 __global__ void k_sa_mixtransfer(Geo g, Dev d, Params p, float *D) {
+  if (gated(d)) return;
   const int t = blockIdx.x;
   const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
   if (NR > g.NROWS - 1) return;
@@ -2363,6 +2407,7 @@ __global__ void k_sa_mixtransfer(Geo g, Dev d, Params p, float *D) {
 // variable (ILAST = IM-1), mf:5858-5926.  One wavefront per (J,K) line,
 // the line held in LDS.
 __global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D) {
+  if (gated(d)) return;
   __shared__ float L[260];
   const int K = blockIdx.x + 1, J = blockIdx.y + 1, lane = threadIdx.x;
   const int IM = g.IM, IMM1 = g.IMM1, KM = g.KM;
@@ -2407,6 +2452,7 @@ __global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D)
 
 // lower limit and kinematic value of the turbulent viscosity, mf:4339-4348
 __global__ void k_sa_kvis(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   CELL_IJK
   const unsigned n = IX(I, J, K);
   const float TRANSVISMIN = 0.1f * d.VISC_LAM[IX(p.IMID, 2, p.KMID)];
@@ -2473,6 +2519,7 @@ __device__ __forceinline__ void sa_gradcell(const Geo &g, const Dev &d, const fl
 // SET_PWALLGRAD, mf:16052-16114.  The streamwise pressure gradient of each
 // cell (PEFF's Gauss gradient along the 8-node velocity sum), limited.
 __global__ void k_pwallgrad_cell(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   CELL_IJK
   const unsigned n = IX(I, J, K);
   // the limit: the mean pressure change per row over a tenth of the chord,
@@ -2502,6 +2549,7 @@ __global__ void k_pwallgrad_cell(Geo g, Dev d, Params p) {
 // beyond JM read as zero and are not stored (the reference's unused, zero
 // array padding).
 __global__ void k_pwallgrad_smooth(Geo g, Dev d) {
+  if (gated(d)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.IMM1 * g.KMM1) return;
   const int I = t % g.IMM1 + 1, K = t / g.IMM1 + 1, JM = g.JM;
@@ -2540,6 +2588,7 @@ __global__ void k_pwallgrad_smooth(Geo g, Dev d) {
 // NEW_LOSS forms the same quantities (mf:12586-12632) but keeps the
 // vorticity of a row's first J, which SPAL_LOSS zeroes
 __global__ void k_sa_grad(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   CELL_IJK
   const unsigned n = IX(I, J, K);
   const int IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
@@ -2573,7 +2622,11 @@ __global__ void k_sa_grad(Geo g, Dev d, Params p) {
 
 // turbulent viscosity of each cell, mf:13719-13888 (mixing-plane cells
 // take the vorticity of the cell upstream; no two such J are adjacent)
-__global__ void k_sa_visc(Geo g, Dev d, Params p, float FMIXUP, int NSTEP) {
+// first: the call is the first step's (NSTEP = 1)
+__global__ void k_sa_visc(Geo g, Dev d, Params p, int first) {
+  if (gated(d)) return;
+  const float FMIXUP = d.SC[SC_FMIXUP];
+  const int NSTEP = first ? 1 : 2;
   CELL_IJK
   const unsigned n = IX(I, J, K);
   float VORT = d.ABS_VORT[n];
@@ -2623,7 +2676,9 @@ __global__ void k_sa_visc(Geo g, Dev d, Params p, float FMIXUP, int NSTEP) {
 // the vorticity, damped through the buffer layer by the cell's y+ from the
 // previous call, plus the laminar and free-stream-turbulence parts, limited
 // to TURBVIS_LIM times laminar
-__global__ void k_nl_visc(Geo g, Dev d, Params p, float FMIXUP) {
+__global__ void k_nl_visc(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
+  const float FMIXUP = d.SC[SC_FMIXUP];
   CELL_IJK
   const unsigned n = IX(I, J, K);
   float TURBVIS_WALL = d.ROAVG_CELL[n] * d.XLENGTH[n] * d.ABS_VORT[n] * FMIXUP;
@@ -2659,6 +2714,7 @@ __global__ void k_nl_visc(Geo g, Dev d, Params p, float FMIXUP) {
 // starts at KMID and the I = IM range at IMID, so each re-reads the cell
 // the test before it may have reset.
 __global__ void __launch_bounds__(64) k_nl_trans(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int J = blockIdx.x + 2, lane = threadIdx.x;
   const int IMM1 = g.IMM1, KMM1 = g.KMM1, IMID = p.IMID, KMID = p.KMID;
   const int NRW = d.NROW[J - 1];
@@ -2728,6 +2784,7 @@ __global__ void __launch_bounds__(64) k_nl_trans(Geo g, Dev d, Params p) {
 // (mf:13795-13806), and uses the laminar viscosity the loop holds on
 // reaching the cell: the previous cell's when it varies with temperature.
 __global__ void k_sa_factors(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   if (threadIdx.x != 0) return;
   const int I = g.IMM1, J = g.JM, K = g.KMM1, IP1 = I + 1, JM1 = J - 1, KP1 = K + 1;
   const unsigned n = IX(I, J, K);
@@ -2778,6 +2835,7 @@ __global__ void k_sa_factors(Geo g, Dev d, Params p) {
 // gradient of the kinematic turbulent viscosity, mf:13893-13911 (J <= JM-1;
 // the J = JM layer stays zero as in the reference)
 __global__ void k_sa_gradtvis(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   CELL_IJK
   if (J > g.JMM1) return;
   const unsigned n = IX(I, J, K);
@@ -2801,6 +2859,7 @@ __global__ void k_sa_gradtvis(Geo g, Dev d, Params p) {
 // Transition tests, mf:14004-14111: per (I,J) column (hub, casing) and per
 // (J,K) line (I = 1 and I = IM surfaces); bit 0/1 = damp the source there.
 __global__ void k_sa_trcol(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int J = blockIdx.y + 2;
   if (I > g.IMM1 || J > g.JM) return;
@@ -2836,6 +2895,7 @@ __global__ void k_sa_trcol(Geo g, Dev d, Params p) {
 }
 
 __global__ void k_sa_trline(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int K = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int J = blockIdx.y + 2;
   if (K > g.KMM1 || J > g.JM) return;
@@ -2873,6 +2933,7 @@ __global__ void k_sa_trline(Geo g, Dev d, Params p) {
 // source of the turbulent viscosity, relaxed into T_SOURCE, then damped
 // towards laminar where the tests say so, mf:13916-13997 and 14004-14116
 __global__ void k_sa_source(Geo g, Dev d, Params p, float RF_VIS) {
+  if (gated(d)) return;
   CELL_IJK
   const unsigned n = IX(I, J, K);
   const float CW2 = 0.3f, SIGMA = 0.6667f, CB2 = 0.622f, CB1 = 0.1355f, CW1 = 0.80607f + 2.433f;
@@ -3011,6 +3072,7 @@ __device__ __forceinline__ SaStress sa_stress(const Geo &g, const Dev &d, const 
 constexpr int kSaLine = 260;
 
 __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   __shared__ float TXR[kSaLine], TXT[kSaLine], TXX[kSaLine], TRR[kSaLine], TRT[kSaLine], TTT[kSaLine];
   __shared__ float QXX[kSaLine], QRR[kSaLine], QTT[kSaLine];
   __shared__ float XF[kSaLine], RF[kSaLine], TF[kSaLine], QF[kSaLine], WF[kSaLine];
@@ -3168,6 +3230,7 @@ __global__ void __launch_bounds__(64) k_sa_iforces(Geo g, Dev d, Params p) {
 // formed with, into d.KWALL (kKWall quantities, each a plane of
 // (IM-1) x (JM-1) columns).  One thread per column.
 __global__ void k_sa_kwalls(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
   if (I > g.IMM1 || J > g.JM) return;
@@ -3249,6 +3312,7 @@ __global__ void k_sa_kwalls(Geo g, Dev d, Params p) {
 // (I,J) column per thread; the face below the segment is formed again from
 // the cell under it.
 __global__ void k_sa_kfaces(Geo g, Dev d, Params p, float RF_VIS) {
+  if (gated(d)) return;
   const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int J = blockIdx.y * blockDim.y + threadIdx.y + 2;
   const int k0 = blockIdx.z * kKSeg + 1;
@@ -3318,6 +3382,54 @@ __global__ void k_sa_kfaces(Geo g, Dev d, Params p, float RF_VIS) {
 // ---------------------------------------------------------------------------
 // limiters, mf:5115-5168
 
+// The bookkeeping at the top of LOOP's step (mf:3757-3824) on the device:
+// the step count, the stop tests (NSTEPS_MAX; the convergence figures of the
+// last check; a pending stop request on a step divisible by 10), and the
+// smoothing/damping and FMIXUP ramps.  A run that has ended (IFEND set by
+// the previous step, or a non-finite check) marks this step and all later
+// ones as not taken; every launch of the step is gated on that.
+__global__ void k_step_begin(Geo g, Dev d, Params p) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int *C = d.CTL;
+  float *S = d.SC;
+  if (C[CTL_DONE] || C[CTL_IFEND]) {
+    C[CTL_DONE] = 1;
+    C[CTL_SKIP] = 1;
+    C[CTL_NOTEND] = 1;
+    return;
+  }
+  const int NSTEP = C[CTL_NSTEP] + 1;
+  C[CTL_NSTEP] = NSTEP;
+  int IFEND = 0;
+  if (NSTEP == p.NSTEPS_MAX) IFEND = 1;
+  if (S[SC_EAVG] < p.CONLIM && S[SC_ECONT] < 0.01f) IFEND = 1;
+  if (fabsf(S[SC_EMAX]) < (4 * p.CONLIM) && S[SC_ECONT] < 0.01f) IFEND = 1;
+  if (NSTEP % 10 == 0 && C[CTL_STOPREQ]) IFEND = 1;
+  C[CTL_IFEND] = IFEND;
+  C[CTL_SKIP] = 0;
+  C[CTL_NOTEND] = IFEND ? 0 : 1;
+  const float FCHANGE = 1.0f - (float)(NSTEP - 1) / (float)p.NCHANGE;
+  if (FCHANGE > 0.0f) {
+    S[SC_SFT] = p.SFTIN + 0.02f * FCHANGE;
+    S[SC_SFX] = p.SFXIN + 0.02f * FCHANGE;
+    S[SC_DAMP] = p.DAMPIN * (1.f - 0.75f * FCHANGE);
+  }
+  float FMIXUP = 1.0f + (p.FACMIXUP - 1.f) * (float)(p.NMIXUP - NSTEP) / (float)p.NMIXUP;
+  if (FMIXUP < 1.0f) FMIXUP = 1.0f;
+  S[SC_FMIXUP] = FMIXUP * FMIXUP;
+}
+
+// After a convergence check: a non-finite maximum velocity or inlet flow
+// ends the run here and records the step for the host's error report.
+__global__ void k_nan_check(Geo g, Dev d) {
+  if (gated(d)) return;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (!isfinite(d.SC[SC_VMAX]) || !isfinite(d.SC[SC_FLOWRAT])) {
+    d.CTL[CTL_NANSTEP] = d.CTL[CTL_NSTEP];
+    d.CTL[CTL_DONE] = 1;
+  }
+}
+
 __global__ void k_fill_int(int *p, int n, int v) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -3336,6 +3448,7 @@ __device__ __forceinline__ float rolim_one(float ro, float rom2, float rom1, flo
 // every node with the unmodified field and flags the (I,K) lines where any
 // node would change; pass 2 replays only those lines serially in J.
 __global__ void k_rolim_check(Geo g, Dev d, StepScalars s) {
+  if (gated(d)) return;
   NODE_IJK
   if (J < 3 || J > g.JM - 3) return;
   float ro = d.RO[IX(I, J, K)];
@@ -3368,6 +3481,7 @@ __device__ __forceinline__ void mach_ref(const Geo &g, const Dev &d, const Param
 // then forms the per-row relative Mach limits (mf:5127-5145) into d.SC for
 // k_mach_apply, one row per thread, after that line's fix.
 __global__ void k_rolim_fix(Geo g, Dev d, StepScalars s, Params p) {
+  if (gated(d)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < g.IM * g.KM && d.FLAGS[t] != 0) {
     const int I = t % g.IM + 1, K = t / g.IM + 1;
@@ -3384,6 +3498,7 @@ __global__ void k_rolim_fix(Geo g, Dev d, StepScalars s, Params p) {
 }
 
 __global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   const int NR = d.NROW[J - 1];
@@ -3413,6 +3528,7 @@ __global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
 // (mixing plane, K) line; the supersonic Prandtl-Meyer sweeps are serial in I.
 
 __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
+  if (gated(d)) return;
   extern __shared__ float ml[];  // 9 * IM floats: per-I work arrays of this line
   const int t = blockIdx.x;      // one wavefront per (mixing plane, K) line
   const int NR = t / g.KM + 1, K = t % g.KM + 1;
@@ -3552,6 +3668,7 @@ __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
 
 // Sum and max of W^2 over J>=2: stage 1 per block in FP64.
 __global__ void k_vrms_part(Geo g, Dev d, double *part) {
+  if (gated(d)) return;
   __shared__ double ssum[256];
   __shared__ float smax[256];
   const int tid = threadIdx.x;
@@ -3588,6 +3705,7 @@ __global__ void k_vrms_part(Geo g, Dev d, double *part) {
 }
 
 __global__ void k_vrms_final(Geo g, Dev d, const double *part, int nparts, int exact) {
+  if (gated(d)) return;
   __shared__ double ss[256];
   __shared__ double sm[256];
   if (exact) {
@@ -3628,6 +3746,7 @@ __global__ void k_vrms_final(Geo g, Dev d, const double *part, int nparts, int e
 
 // per-J mass flow, reference order over I then K (exact) or FP64 (fast)
 __global__ void k_checkflow(Geo g, Dev d, int exact) {
+  if (gated(d)) return;
   const int J = blockIdx.x + 1;
   if (J > g.JM) return;
   float SUMAS;
@@ -3656,6 +3775,7 @@ __global__ void k_checkflow(Geo g, Dev d, int exact) {
 // FLOW(J) = -sum FLOWX * NBLADE(J) / NBLADE(1), summed K-major as the
 // reference does (exact) or in FP64 (fast); one workgroup per J.
 __global__ void __launch_bounds__(64) k_setflo_flow(Geo g, Dev d, int exact) {
+  if (gated(d)) return;
   const int J = blockIdx.x + 1;
   const float NBJ = (float)d.NBLADE[J - 1], NB1 = (float)d.NBLADE[0];
   if (exact) {
@@ -3677,6 +3797,7 @@ __global__ void __launch_bounds__(64) k_setflo_flow(Geo g, Dev d, int exact) {
 // the mid-passage node (mf:8022-8036), into d.SETFLO[JM + J - 1].  One
 // workgroup; the mean over J is sequential (exact) or FP64 (fast).
 __global__ void __launch_bounds__(256) k_setflo_delta(Geo g, Dev d, Params p, int exact) {
+  if (gated(d)) return;
   __shared__ double part[256];
   __shared__ float avflow;
   const int JM = g.JM, t = threadIdx.x;
@@ -3712,6 +3833,7 @@ __global__ void __launch_bounds__(256) k_setflo_delta(Geo g, Dev d, Params p, in
 // The change along the local meridional direction of every node, with the
 // relative swirl momentum scaled alike, mf:8037-8049
 __global__ void k_setflo_apply(Geo g, Dev d) {
+  if (gated(d)) return;
   NODE_IJK
   const unsigned n = IX(I, J, K);
   const float DELTA = d.SETFLO[g.JM + J - 1];
@@ -3725,6 +3847,7 @@ __global__ void k_setflo_apply(Geo g, Dev d) {
 }
 
 __global__ void k_econt(Geo g, Dev d) {
+  if (gated(d)) return;
   __shared__ float sm[256];
   float e = 0.0f;
   const float cf1 = d.CHECK_FLOW[0];
@@ -3755,6 +3878,7 @@ __device__ __forceinline__ float dvmer_at(const Geo &g, const Dev &d, int I, int
 
 // stage 1: per block FP64 sum of |DVMER| and the first max-|DVMER| node
 __global__ void k_emax_part(Geo g, Dev d, double *part) {
+  if (gated(d)) return;
   __shared__ double ssum[256];
   __shared__ float sabs[256];
   __shared__ long long sidx[256];
@@ -3802,6 +3926,7 @@ __global__ void k_emax_part(Geo g, Dev d, double *part) {
 }
 
 __global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int exact) {
+  if (gated(d)) return;
   __shared__ double ssum[256], sabs[256], sidx[256];
   __shared__ float sval[256];
   const float RIJKM = 2.f / (float)(g.IM * g.JM * g.KM);
@@ -3853,6 +3978,7 @@ __global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int e
 // STEPUP, mf:7916-8002 (ideal gas, ITIMST<5)
 
 __global__ void k_stepup_temp(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   float TLIM = 0.1f * d.TO1[p.KMID - 1];
@@ -3872,6 +3998,7 @@ __global__ void k_stepup_temp(Geo g, Dev d, Params p) {
 }
 
 __global__ void k_stepup_step(Geo g, Dev d, float RELAX) {
+  if (gated(d)) return;
   NODE_IJK
   if (I > g.IMM1 || K > g.KMM1 || J < 2) return;
   const int IP1 = I + 1, KP1 = K + 1;
@@ -3889,6 +4016,7 @@ __global__ void k_stepup_step(Geo g, Dev d, float RELAX) {
 }
 
 __global__ void k_stepup_mix(Geo g, Dev d) {
+  if (gated(d)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int per = g.IMM1 * g.KMM1;
   const int NR = t / per + 1, c = t % per;
@@ -3904,6 +4032,7 @@ __global__ void k_stepup_mix(Geo g, Dev d) {
 // mf:7883-7910), FP64 per-J sums: one workgroup per J
 
 __global__ void k_integrals(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
   const int J = blockIdx.x + 1;
   if (J > g.JM) return;
   __shared__ double sh[7][256];
@@ -3964,6 +4093,7 @@ namespace mh {
 // the reference does (mf:12408-12424).  FP32 in the reference's statement
 // order; sinf is the device's.
 __global__ void __launch_bounds__(256) k_xl_dist(Geo g, Dev d, const float *TH, float *DIST, int IBOUND) {
+  if (gated(d)) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.IM * g.JM * g.KM) return;
   const int I = n % g.IM + 1, J = (n / g.IM) % g.JM + 1, K = n / (g.IM * g.JM) + 1;
@@ -4055,6 +4185,7 @@ __global__ void __launch_bounds__(256) k_xl_dist(Geo g, Dev d, const float *TH, 
 // trailing edge to mixing plane (mf:12428-12449).  XL4 = [IN | LE | TE | DN]
 // per row.  One thread per J.
 __global__ void k_xl_limit(Geo g, Dev d, const float *DIST, const float *XL4, float *XLIMIT) {
+  if (gated(d)) return;
   const int J = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (J > g.JM) return;
   const int NR = d.NROW[J - 1], K = g.KMID, nr = g.NROWS;
@@ -4075,6 +4206,7 @@ __global__ void k_xl_limit(Geo g, Dev d, const float *DIST, const float *XL4, fl
 // per node.
 __global__ void __launch_bounds__(256) k_xl_cells(Geo g, Dev d, const float *DIST, const float *XLIMIT, float dblend,
                                                   float expon, float fblend) {
+  if (gated(d)) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.IM * g.JM * g.KM) return;
   const int I = n % g.IM + 1, J = (n / g.IM) % g.JM + 1, K = n / (g.IM * g.JM) + 1;

@@ -147,6 +147,9 @@ class Solver:
     def start(self):
         self._check(self.L.mh_start(self.h), "mh_start")
 
+    def set_scalar(self, name: str, value: float):
+        self._check(self.L.mh_set_scalar(self.h, self.L.mh_scalar_id(name.encode()), float(value)), f"scalar {name}")
+
     def scalar(self, name: str) -> float:
         return float(self.L.mh_get_scalar(self.h, self.L.mh_scalar_id(name.encode())))
 

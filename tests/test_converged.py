@@ -90,7 +90,8 @@ def test_converged_vs_fp64_reference(case):
     g.loss_init()
     g.start()
     r = g.step(int(st["scalars"]["NSTEPS_MAX"]) + 10)
-    assert r["ifend"] == 1
+    # the run ends by the reference's convergence test, found on the device
+    assert r["ifend"] == 1 and r["nstep"] < int(st["scalars"]["NSTEPS_MAX"])
     print(f"GPU stopped at step {r['nstep']} (EAVG {r['eavg']:.3g}, ECONT {r['econt']:.3g}); "
           f"reference FP64 at {d['fp64']['nstep']}, FP32 at {d['fp32']['nstep']}")
     q = quantities(g.integrals()[:5], ga)
