@@ -152,6 +152,11 @@ enum mh_scalar_id {
   MH_S_EXACT_ORDER,   /* 1: reductions in the reference's sequential order */
   MH_S_USE_GRAPHS,    /* 1 (default): steps queued as captured HIP graphs, 0: launch by launch */
   MH_S_LANES,         /* 1 (default): in the fast mode, independent equations' time steps on three streams */
+  MH_S_MIXED,         /* 1: the mixed form (fast mode only, read at mh_start): the SSS histories DRO..DROE and
+                         DEL_DYNVIS stored in BF16, all arithmetic in FP32; 0 (default): FP32 storage */
+  MH_S_FLOW_CONV,     /* > 0: also end the run once the inlet mass flow has changed by less than this
+                         fraction between each two of the last 10 every-5-step checks (50 steps) and
+                         ECONT < 0.01 (default 0: off; the reference's own stop tests always apply) */
   MH_NSCALARS
 };
 

@@ -72,6 +72,9 @@ __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
 __global__ void k_sa_addstore(Geo, Dev, float *, int);
 __global__ void k_resid_line(Geo, Dev, float *, Params, MgFast, int, int, int);
+__global__ void k_resid_line_bf(Geo, Dev, unsigned short *, Params, MgFast, int, int, int);
+__global__ void k_to_bf16(const float *, unsigned short *, unsigned);
+__global__ void k_from_bf16(const unsigned short *, float *, unsigned);
 __global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, float *, float *,
                             float *);
 __global__ void k_mg_apply_lines(Geo, Dev, double *);
@@ -102,7 +105,7 @@ __global__ void k_sa_iforces(Geo, Dev, Params);
 __global__ void k_sa_kwalls(Geo, Dev, Params);
 __global__ void k_sa_kfaces(Geo, Dev, Params, float);
 __global__ void k_step_begin(Geo, Dev, Params);
-__global__ void k_nan_check(Geo, Dev);
+__global__ void k_nan_check(Geo, Dev, Params);
 
 const ArrayInfo kArrays[MH_NARRAYS] = {
     {"RO", K_N, false}, {"ROVX", K_N, false}, {"ROVR", K_N, false}, {"RORVT", K_N, false}, {"ROE", K_N, false},
@@ -148,7 +151,7 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
     "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "FAC_STMIX", "FAC_ST0", "FAC_ST1", "FAC_ST2", "FAC_ST3",
     "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "FLOWIN", "RFLOW", "VISLAM", "FTCOND",
-    "EXACT_ORDER", "USE_GRAPHS", "LANES",
+    "EXACT_ORDER", "USE_GRAPHS", "LANES", "MIXED", "FLOW_CONV",
 };
 
 }  // namespace mh
@@ -324,6 +327,7 @@ Params params_of(const mh_ctx *c) {
   p.FAC_VORT = sf(c, MH_S_FAC_VORT); p.FAC_PGRAD = sf(c, MH_S_FAC_PGRAD); p.VORT_MAX = sf(c, MH_S_VORT_MAX);
   p.TURBVIS_LIM = sf(c, MH_S_TURBVIS_LIM);
   p.FLOWIN = sf(c, MH_S_FLOWIN); p.RFLOW = sf(c, MH_S_RFLOW);
+  p.FLOW_CONV = sf(c, MH_S_FLOW_CONV);
   return p;
 }
 
@@ -584,6 +588,30 @@ Dev lane_dev(const mh_ctx *c, int lane, int gate) {
 }
 hipStream_t lane_stream(const mh_ctx *c, int lane) { return lane == 0 ? c->stream : c->lanes[lane - 1].stream; }
 
+// The mixed form's BF16 copy of a history array (DRO, DROVX, DROVR, DRORVT,
+// DROE, DEL_DYNVIS), or null for any other array.
+int hist_slot(const mh_ctx *c, const float *a) {
+  const float *h[kHist] = {c->d.DRO, c->d.DROVX, c->d.DROVR, c->d.DRORVT, c->d.DROE, c->d.DEL_DYNVIS};
+  for (int i = 0; i < kHist; i++)
+    if (a == h[i]) return i;
+  return -1;
+}
+unsigned short *hist16(const mh_ctx *c, const float *a) {
+  const int i = hist_slot(c, a);
+  return i < 0 ? nullptr : c->h16[i];
+}
+
+// FP32 history <-> its BF16 copy (synchronous; outside the step path).
+int hist_convert(mh_ctx *c, int i, bool to_bf16) {
+  float *h[kHist] = {c->d.DRO, c->d.DROVX, c->d.DROVR, c->d.DRORVT, c->d.DROE, c->d.DEL_DYNVIS};
+  const unsigned n = (unsigned)((size_t)c->g.IM * c->g.JM * c->g.KM);
+  if (to_bf16) LAUNCH(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float *)h[i], c->h16[i], n);
+  else LAUNCH(k_from_bf16, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const unsigned short *)c->h16[i], h[i], n);
+  LCHK("bf16 convert");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MH_OK;
+}
+
 // One TSTEP call (mf:5470-5932) on lane `lane`, gated on CTL slot `gate`.
 int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan &q, int gate) {
   const Geo &g = c->g;
@@ -610,8 +638,12 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan 
   // fast mode: the residual kernel also leaves the multigrid partial sums
   const bool fastmg = mg && !p.EXACT && L.mgf_ok;
   const int kg = fastmg ? p.KR : 4;
-  LAUNCH(k_resid_line, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, st, g, d, DIFF, p, mgf,
-         kg, p.IR, fastmg ? 1 : 0);
+  if (c->mixed)
+    LAUNCH(k_resid_line_bf, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, st, g, d, hist16(c, DIFF), p,
+           mgf, kg, p.IR, fastmg ? 1 : 0);
+  else
+    LAUNCH(k_resid_line, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, st, g, d, DIFF, p, mgf, kg,
+           p.IR, fastmg ? 1 : 0);
   LCHK("resid_line");
   if (mg) {
     const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
@@ -1065,7 +1097,7 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LAUNCH(k_econt, dim3(1), dim3(256), 0, c->stream, g, dc);
     LAUNCH(k_emax_part, dim3(kRedBlocks), dim3(256), 0, c->stream, g, dc, d.RED);
     LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, dc, (const double *)d.RED, kRedBlocks, p.EXACT);
-    LAUNCH(k_nan_check, dim3(1), dim3(64), 0, c->stream, g, dc);
+    LAUNCH(k_nan_check, dim3(1), dim3(64), 0, c->stream, g, dc, p);
     LCHK("convergence");
     // forces brought up to date before a printout or the final output,
     // mf:5338-5341 (on a step that may end by convergence: if it does)
@@ -1117,7 +1149,7 @@ int enqueue_chunk(mh_ctx *c, const std::vector<StepPlan> &plan) {
 // check's figures (one small copy).
 int read_back(mh_ctx *c) {
   int ctl[CTL_COUNT];
-  float sc[16];
+  float sc[SC_NHEAD];
   HIPCHK(hipMemcpyAsync(ctl, c->d.CTL, sizeof ctl, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(sc, c->d.SC, sizeof sc, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1237,6 +1269,8 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   if (hipHostMalloc((void **)&c->ctl_host, CTL_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_poll, hipEventDisableTiming) != hipSuccess)
     return bail(MH_E_HIP);
+  for (auto &e : c->ev_ahead)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail(MH_E_HIP);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
   if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1) > kMaxLds) return bail(MH_E_UNSUPPORTED);
@@ -1264,6 +1298,8 @@ int mh_destroy(mh_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->ev_poll) (void)hipEventDestroy(c->ev_poll);
+  for (auto e : c->ev_ahead)
+    if (e) (void)hipEventDestroy(e);
   if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1271,8 +1307,21 @@ int mh_destroy(mh_ctx *c) {
   return MH_OK;
 }
 
+static int copy_array_raw(mh_ctx *c, int id, void *host, long long n, int ld_i, int ld_j, bool to_dev);
+
+// In the mixed form the library's copy of a history array is its BF16
+// array: reads widen it first, writes narrow it after.
 static int copy_array(mh_ctx *c, int id, void *host, long long n, int ld_i, int ld_j, bool to_dev) {
   if (!c || id < 0 || id >= MH_NARRAYS || !host) return fail(c, MH_E_ARG, "bad argument");
+  const int h = (c->mixed && kArrays[id].kind == K_N) ? hist_slot(c, (const float *)*slot_of(c->d, id)) : -1;
+  int r;
+  if (h >= 0 && !to_dev && (r = hist_convert(c, h, false))) return r;
+  if ((r = copy_array_raw(c, id, host, n, ld_i, ld_j, to_dev))) return r;
+  if (h >= 0 && to_dev && (r = hist_convert(c, h, true))) return r;
+  return MH_OK;
+}
+
+static int copy_array_raw(mh_ctx *c, int id, void *host, long long n, int ld_i, int ld_j, bool to_dev) {
   const mh_dims &dm = c->dims;
   const Kind k = kArrays[id].kind;
   const long long len = kind_len(dm, k);
@@ -1334,7 +1383,9 @@ double mh_get_scalar(const mh_ctx *c, int id) {
 static int check_inputs(mh_ctx *c) {
   const bool forcing = c->scal_set[MH_S_IN_FLOW] && si(c, MH_S_IN_FLOW) != 0;
   for (int i = 0; i < MH_NSCALARS; i++)
-    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_USE_GRAPHS && i != MH_S_LANES && i != MH_S_VISLAM &&
+    if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_USE_GRAPHS && i != MH_S_LANES && i != MH_S_MIXED &&
+        i != MH_S_FLOW_CONV &&
+        i != MH_S_VISLAM &&
         i != MH_S_FTCOND &&
         (forcing || (i != MH_S_FLOWIN && i != MH_S_RFLOW)))
       return fail(c, MH_E_STATE, std::string("scalar not set: ") + kScalarNames[i]);
@@ -1480,6 +1531,21 @@ int mh_start(mh_ctx *c) {
   c->PREFF = 0.5f * (pa + pb);
   c->SFXIN_s = sf(c, MH_S_SFXIN) * sf(c, MH_S_CFL) / 0.5f;
   c->SFTIN_s = sf(c, MH_S_SFTIN) * sf(c, MH_S_CFL) / 0.5f;
+  // the mixed form: the SSS histories kept in BF16 from here on
+  const bool mixed = c->scal[MH_S_MIXED] != 0.0;
+  if (mixed && p.EXACT) return fail(c, MH_E_UNSUPPORTED, "the mixed form (MIXED = 1) needs EXACT_ORDER = 0");
+  if (c->mixed && !mixed)  // back to FP32 storage: take the histories as they are
+    for (int i = 0; i < kHist; i++)
+      if ((r = hist_convert(c, i, false))) return r;
+  c->mixed = false;
+  if (mixed) {
+    const size_t n = (size_t)g.IM * g.JM * g.KM;
+    for (int i = 0; i < kHist; i++) {
+      if (!c->h16[i] && (r = dalloc(c, &c->h16[i], n))) return r;
+      if ((r = hist_convert(c, i, true))) return r;
+    }
+    c->mixed = true;
+  }
   c->ifend = 0;
   c->nstep = 0;
   c->hstep = 0;
@@ -1490,13 +1556,15 @@ int mh_start(mh_ctx *c) {
     // the device's step state: NSTEP = 0, the stop tests' figures as
     // before any check (mf:3697-3699), the ramps not yet started
     const std::vector<int> ctl(CTL_COUNT, 0);
-    float sc[16] = {};
+    float sc[SC_NHEAD] = {};
     if ((r = d2h(c, sc, c->d.SC, sizeof sc))) return r;
     sc[SC_EMAX] = 1.0e06f;
     sc[SC_EAVG] = 1.0e06f;
     sc[SC_ECONT] = 1.f;
     sc[SC_SFT] = sc[SC_SFX] = sc[SC_DAMP] = 0.0f;
     sc[SC_FMIXUP] = 1.0f;
+    sc[SC_FLOWPREV] = 0.0f;
+    sc[SC_DFLOW] = 1.0e6f;
     if ((r = h2d(c, c->d.CTL, ctl.data(), CTL_COUNT * sizeof(int))) || (r = h2d(c, c->d.SC, sc, sizeof sc))) return r;
   }
   if ((r = d2h(c, c->nout, c->d.NOUT, sizeof c->nout))) return r;
@@ -1506,9 +1574,10 @@ int mh_start(mh_ctx *c) {
 }
 
 // Steps are queued in chunks that end on the every-5-step checks, each chunk
-// one graph launch; nothing waits for the device until the call returns.
-// While queuing, the device's end flag is read back without blocking (a
-// copy into pinned memory, polled with an event), so queuing stops soon
+// one graph launch.  The host runs at most kAhead chunks ahead of the device
+// (it waits on the event of the chunk kAhead back), which bounds the queue
+// depth; while queuing, the device's end flag is read back without blocking
+// (a copy into pinned memory, polled with an event), so queuing stops soon
 // after a run ends by convergence; the steps queued beyond it are not taken
 // on the device.
 int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
@@ -1533,7 +1602,11 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
         break;
       }
     }
+    hipEvent_t &slot = c->ev_ahead[c->nchunk % kAhead];
+    if (c->nchunk >= kAhead) HIPCHK(hipEventSynchronize(slot));
     if ((r = enqueue_chunk(c, plan))) return r;
+    HIPCHK(hipEventRecord(slot, c->stream));
+    c->nchunk++;
     c->hstep = hn + (int)plan.size();
     queued += (int)plan.size();
     if (!polling) {
@@ -1547,6 +1620,7 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
     if (c->prof_on && c->prof_pending.size() > 4096 && (r = prof_collect(c))) return r;
   }
   r = read_back(c);
+  c->nchunk = 0;        // the stream is drained: no chunk is outstanding
   c->hstep = c->nstep;  // the device may have ended the run before the host's last queued step
   if (out) *out = c->last;
   return r;

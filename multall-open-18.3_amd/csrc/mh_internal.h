@@ -97,6 +97,7 @@ enum {
   CTL_NOTEND,      // SKIP, or this step is not the run's last: gates launches that only the last step makes
   CTL_STOPREQ,     // a stop request (stopit) is pending
   CTL_NANSTEP,     // the step whose convergence check found a non-finite value (0: none)
+  CTL_FLOWOK,      // consecutive checks whose inlet-flow change met FLOW_CONV
   CTL_COUNT = 16
 };
 
@@ -116,12 +117,15 @@ enum {
   SC_VORTFAC, SC_PGRADFAC,  // SPAL_LOSS's helicity and pressure-gradient source factors
   SC_SFT, SC_SFX, SC_DAMP,  // the step's smoothing and damping factors (ramped over NCHANGE steps)
   SC_FMIXUP,                // the step's FMIXUP (squared), which the viscous calls read
-  SC_AVG_CHG = 16,          // [NROWS]
-  SC_AVG_BLK = 16 + 64,     // [NROWS]
-  SC_VLIMIT = 16 + 128,     // [NROWS]
-  SC_ROLIMIT = 16 + 192,    // [NROWS]
-  SC_PLIM = 16 + 256,       // [KM], PLIM(K) fixed at the start of LOOP (mf:3684-3686)
-  SC_COUNT = 16 + 256       // + KM
+  SC_FLOWPREV,              // inlet mass flow at the previous check
+  SC_DFLOW,                 // its relative change between the last two checks (FLOW_CONV stop test)
+  SC_NHEAD = 32,            // scalar slots before the per-row and per-K tables
+  SC_AVG_CHG = SC_NHEAD,            // [NROWS]
+  SC_AVG_BLK = SC_NHEAD + 64,       // [NROWS]
+  SC_VLIMIT = SC_NHEAD + 128,       // [NROWS]
+  SC_ROLIMIT = SC_NHEAD + 192,      // [NROWS]
+  SC_PLIM = SC_NHEAD + 256,         // [KM], PLIM(K) fixed at the start of LOOP (mf:3684-3686)
+  SC_COUNT = SC_NHEAD + 256         // + KM
 };
 
 // Layout of Dev::LTAB, the per-row mixing-length factors of LOSS
@@ -153,6 +157,7 @@ struct Params {
   int IBOUND;
   float FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD, VORT_MAX, TURBVIS_LIM;
   float FLOWIN, RFLOW;
+  float FLOW_CONV;  // library stop test on the inlet mass flow's relative change per check (0: off)
 };
 
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid
@@ -195,6 +200,8 @@ struct mh_prof_rec {
 };
 
 namespace mh {
+constexpr int kHist = 6;  // history arrays with a BF16 copy in the mixed form
+constexpr int kAhead = 8; // chunks (of up to 5 steps) the host may queue ahead of the device
 // A second or third HIP stream with its own flux and TSTEP scratch, so the
 // time steps of equations that do not read each other's results run
 // concurrently (continuity beside energy; the three momentum equations).
@@ -229,11 +236,15 @@ struct mh_ctx {
   std::map<std::string, hipGraphExec_t> graphs;  // captured step chunks, by step pattern
   int *ctl_host = nullptr;  // pinned copy of the device's CTL, for polling the end flag
   hipEvent_t ev_poll = nullptr;
+  hipEvent_t ev_ahead[8] = {};  // completion of the last kAhead queued chunks
+  long long nchunk = 0;         // chunks queued since the stream was last drained
   int nout[5] = {};       // printout steps (read from the device at mh_start)
   bool stop_req = false;  // mh_request_stop: end at the next step divisible by 10
   int device = 0;             // the HIP device the context was created on
   float *xl_scratch = nullptr;  // that device's address of mh_set_xlength_noalloc's scratch
   float DAMP = 0;                            // host mirror of the device's ramped DAMP (launch choices)
+  bool mixed = false;                       // the SSS histories stored in BF16 (set at mh_start)
+  unsigned short *h16[6] = {};              // their BF16 copies: DRO, DROVX, DROVR, DRORVT, DROE, DEL_DYNVIS
   float PREFF = 0, ROLIM = 0, ROMIN = 0;
   float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
   mh_resid last{};

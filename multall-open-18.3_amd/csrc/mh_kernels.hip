@@ -845,7 +845,39 @@ __device__ __forceinline__ int mp_role(const Dev &d, int J, int &jx) {
 // (mf:5516-5555), the flux sum and SSS history with the tip-gap TFLUX
 // balance (mf:5503-5510, 5560-5570), and the first pitchwise STORE average
 // on the mixing-plane lines (mf:5578-5607).
-__device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float *DIFF, const Params &p, int J, int K,
+// The SSS history of one conserved variable: FP32, or (the mixed form) BF16
+// storage rounded to nearest even, the arithmetic in FP32 either way.
+__device__ __forceinline__ unsigned short f32_to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+template <bool BF>
+struct Hist {
+  void *p;
+  __device__ __forceinline__ float ld(unsigned n) const {
+    if constexpr (BF) return bf16_to_f32(((const unsigned short *)p)[n]);
+    else return ((const float *)p)[n];
+  }
+  __device__ __forceinline__ void st(unsigned n, float v) const {
+    if constexpr (BF) ((unsigned short *)p)[n] = f32_to_bf16(v);
+    else ((float *)p)[n] = v;
+  }
+};
+
+__global__ void k_to_bf16(const float *src, unsigned short *dst, unsigned n) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = f32_to_bf16(src[i]);
+}
+__global__ void k_from_bf16(const unsigned short *src, float *dst, unsigned n) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = bf16_to_f32(src[i]);
+}
+
+template <bool BF>
+__device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<BF> DIFF, const Params &p, int J, int K,
                                               int lane, float RATPITCH, bool tipbal, int role, int jx,
                                               float (&out)[4]) {
   const int IM = g.IM, IMM1 = g.IMM1;
@@ -883,9 +915,9 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float 
     if (fex && role == 2) src = 0.0f;
     const float DELTA = d.XFLUX[n] - d.XFLUX[IX(I, J - 1, K)] * RATPITCH + d.RFLUX[n] - d.RFLUX[IX(I, J, K + 1)] + TF -
                         TFP - src;
-    const float df = DIFF[n];
+    const float df = DIFF.ld(n);
     st[c] = p.F1 * DELTA + p.F2 * df;
-    DIFF[n] = DELTA + p.F3 * df;
+    DIFF.st(n, DELTA + p.F3 * df);
   }
   const bool avg1 = (role == 3 && (FLOWDIRN > 0.0f || p.FEXTRAP < 0.001f)) ||
                     (role == 1 && (FLOWDIRN < 0.0f || p.FEXTRAP < 0.001f));
@@ -914,8 +946,9 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, float 
 // (mf:5621-5640), FP64 per level-1 I block in P[(ib, J, kb)] and per line
 // group in PT[(J, kb)], lane columns summed over the block's K lines first.
 #define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
-__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p, MgFast m, int kg, int ir,
-                                                    int part) {
+template <bool BF>
+__device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> DIFF, const Params &p, const MgFast &m,
+                                           int kg, int ir, int part) {
   if (gated(d)) return;
   __shared__ double sh[4][256];
   const int lane = threadIdx.x & 63, w = wave_id();
@@ -972,7 +1005,7 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
         tf[kk] = d.TFLUX[n];
         tp[kk] = d.TFLUX[n + 1];
         sr[kk] = d.SOURCE[n];
-        df[kk] = DIFF[n];
+        df[kk] = DIFF.ld(n);
       }
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) {
@@ -986,7 +1019,7 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
         const unsigned n = IX(I, J, kq + kk);
         const float v = p.F1 * DELTA + p.F2 * df[kk];
         d.STORE[n] = v;
-        DIFF[n] = DELTA + p.F3 * df[kk];
+        DIFF.st(n, DELTA + p.F3 * df[kk]);
         st[kk][c] = v;
       }
     }
@@ -1010,6 +1043,17 @@ __global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, P
     for (int i = i0; i < i1; i++) sum += sh[w][i];
     m.P[PX(ib, J, kb + 1)] = sum;
   }
+}
+__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p, MgFast m, int kg, int ir,
+                                                    int part) {
+  if (gated(d)) return;
+  resid_line<false>(g, d, Hist<false>{DIFF}, p, m, kg, ir, part);
+}
+// the same with the SSS history stored in BF16 (the mixed form)
+__global__ void __launch_bounds__(256) k_resid_line_bf(Geo g, Dev d, unsigned short *DIFF, Params p, MgFast m, int kg,
+                                                       int ir, int part) {
+  if (gated(d)) return;
+  resid_line<true>(g, d, Hist<true>{DIFF}, p, m, kg, ir, part);
 }
 
 // Fast-mode multigrid sums (mf:5621-5640), pass 2 (pass 1 is in
@@ -3388,6 +3432,7 @@ __global__ void k_sa_kfaces(Geo g, Dev d, Params p, float RF_VIS) {
 // smoothing/damping and FMIXUP ramps.  A run that has ended (IFEND set by
 // the previous step, or a non-finite check) marks this step and all later
 // ones as not taken; every launch of the step is gated on that.
+constexpr int kFlowChecks = 10;  // checks (50 steps) the FLOW_CONV test must hold for
 __global__ void k_step_begin(Geo g, Dev d, Params p) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int *C = d.CTL;
@@ -3405,6 +3450,9 @@ __global__ void k_step_begin(Geo g, Dev d, Params p) {
   if (S[SC_EAVG] < p.CONLIM && S[SC_ECONT] < 0.01f) IFEND = 1;
   if (fabsf(S[SC_EMAX]) < (4 * p.CONLIM) && S[SC_ECONT] < 0.01f) IFEND = 1;
   if (NSTEP % 10 == 0 && C[CTL_STOPREQ]) IFEND = 1;
+  // the library's optional stop test on the inlet mass flow (FLOW_CONV > 0):
+  // steady to FLOW_CONV over the last kFlowChecks checks, and ECONT < 0.01
+  if (p.FLOW_CONV > 0.0f && C[CTL_FLOWOK] >= kFlowChecks && S[SC_ECONT] < 0.01f) IFEND = 1;
   C[CTL_IFEND] = IFEND;
   C[CTL_SKIP] = 0;
   C[CTL_NOTEND] = IFEND ? 0 : 1;
@@ -3420,14 +3468,21 @@ __global__ void k_step_begin(Geo g, Dev d, Params p) {
 }
 
 // After a convergence check: a non-finite maximum velocity or inlet flow
-// ends the run here and records the step for the host's error report.
-__global__ void k_nan_check(Geo g, Dev d) {
+// ends the run here and records the step for the host's error report; the
+// relative change of the inlet mass flow since the previous check (FP64)
+// feeds the optional FLOW_CONV stop test.
+__global__ void k_nan_check(Geo g, Dev d, Params p) {
   if (gated(d)) return;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (!isfinite(d.SC[SC_VMAX]) || !isfinite(d.SC[SC_FLOWRAT])) {
+  float *S = d.SC;
+  if (!isfinite(S[SC_VMAX]) || !isfinite(S[SC_FLOWRAT])) {
     d.CTL[CTL_NANSTEP] = d.CTL[CTL_NSTEP];
     d.CTL[CTL_DONE] = 1;
   }
+  const double f = S[SC_FLOWRAT], f0 = S[SC_FLOWPREV];
+  S[SC_DFLOW] = f != 0.0 ? (float)(fabs(f - f0) / fabs(f)) : 1.0e6f;
+  S[SC_FLOWPREV] = S[SC_FLOWRAT];
+  d.CTL[CTL_FLOWOK] = S[SC_DFLOW] < p.FLOW_CONV ? d.CTL[CTL_FLOWOK] + 1 : 0;
 }
 
 __global__ void k_fill_int(int *p, int n, int v) {

@@ -19,6 +19,7 @@ the streamwise grid spacing and to the tangential camber coordinate.
 from __future__ import annotations
 
 import math
+import re
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -576,41 +577,6 @@ def deck_config1(nsteps: int = 5000, ilos: int = 200) -> DeckSpec:
                     po1=2.0e5, to1=1000.0, vm1=150.0, pdown_hub=1.1e5, pdown_tip=1.1e5)
 
 
-def speedline_sizes(n_points: int = 16, seed: int = 3):
-    """The seeded grid-size draw of the speedline: (IM, KM, JM per row)."""
-    rng = np.random.default_rng(seed)
-    return [(int(rng.choice([37, 49, 65])), int(rng.choice([33, 49, 65])), int(rng.choice([121, 161, 193])))
-            for _ in range(n_points)]
-
-
-def deck_speedline_point(i: int, n_points: int = 16, nsteps: int = 3000, seed: int = 3) -> DeckSpec:
-    """Point i of the batched speedline: a single-stage axial compressor
-    (rotor, 36 blades at 9000 rpm, then stator, 48 blades), hub/tip radii
-    0.25/0.35 m, inlet p0 = 1.013e5 Pa, T0 = 288 K, axial, with the exit
-    static pressure linearly spaced from 1.2 to 1.6 times the inlet total
-    pressure over the points (choke towards stall).  Grid sizes come from a
-    seeded draw: IM from {37, 49, 65}, KM from {33, 49, 65}, JM per row from
-    {121, 161, 193}.  Circular-arc camber, 8 % thickness, mixing-length
-    viscous model, SCREE, multigrid 3/9."""
-    im, km, jm = speedline_sizes(n_points, seed)[i]
-    frac = i / max(1, n_points - 1)
-    po1 = 1.013e5
-    pexit = po1 * (1.2 + 0.4 * frac)
-    jle, jte = jm // 4 + 1, jm - jm // 4
-    pin = 0.88e5
-    pmid = 0.5 * (pin + pexit)
-    rotor = RowSpec(name="ROTOR", nblades=36, jm=jm, jle=jle, jte=jte, chord_x=0.04,
-                    alpha_in=-60.0, alpha_out=-35.0, tmax=0.08, rpm=9000.0,
-                    pguess=(pin, pin, pmid, pmid))
-    stator = RowSpec(name="STATOR", nblades=48, jm=jm, jle=jle, jte=jte, chord_x=0.04,
-                     alpha_in=50.0, alpha_out=0.0, tmax=0.08,
-                     pguess=(pmid, pmid, pexit, pexit))
-    return DeckSpec(title=f"SPEEDLINE POINT {i + 1} OF {n_points}: COMPRESSOR STAGE {im}X{km}X2X{jm} PEXIT {pexit:.0f}",
-                    im=im, km=km, rows=[rotor, stator], r_hub=0.25, r_tip=0.35,
-                    ilos=10, nsteps=nsteps, conlim=0.005, nchange=500, seed=seed + i,
-                    po1=po1, to1=288.0, vm1=150.0, pdown_hub=pexit, pdown_tip=pexit)
-
-
 def deck_tiny_bc(variant: str, nsteps: int = 100) -> DeckSpec:
     """Tiny mixing-length cascades for the inlet and exit condition options
     (LOOP, multall-open-18.3.f:4506-4757):
@@ -682,3 +648,87 @@ def deck_tiny_nout(nsteps: int = 100) -> DeckSpec:
     spec.title = "TINY CASCADE ILOS=10 PRINTOUT AT STEPS 10 25 40 55 73"
     spec.nout = (10, 25, 40, 55, 73)
     return spec
+
+
+def speedline_sizes(n_points: int = 16, seed: int = 3):
+    """The seeded grid-size draw of the compressor speedline: (IM, KM), IM
+    from {37, 49} and KM from {33, 49, 65}.  (At IM = 65 this machine's
+    pitchwise spacing leaves a steady continuity error, ECONT, of 1.4 % on
+    the choke side, above the 1 % a converged point is held to.)"""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([37, 49])), int(rng.choice([33, 49, 65]))) for _ in range(n_points)]
+
+
+def speedline_pexit(i: int, n_points: int = 16, lo: float = 0.95, hi: float = 1.08,
+                    design: float = 109140.9) -> float:
+    """Exit static pressure of point i: linearly spaced from `lo` (towards
+    choke) to `hi` (towards stall) times the design exit pressure of the
+    reference's one-stage axial compressor deck."""
+    return design * (lo + (hi - lo) * i / max(1, n_points - 1))
+
+
+def regrid_deck(text: str, im: int, km: int, pexit: Optional[float] = None, nsteps: Optional[int] = None) -> str:
+    """A NEW-format deck with its pitchwise and spanwise grid counts changed
+    (IM, KM and the FP / FR spacing lists, geometric towards mid-passage like
+    the reference's own decks, and the KM stream-surface output flags), and
+    optionally the exit static pressure (PDOWN_HUB = PDOWN_TIP) and
+    NSTEPS_MAX.  The blade geometry is given per section and interpolated
+    onto the grid by the reader, so any IM and KM describe the same machine."""
+    lines = text.split("\n")
+    out = []
+    i = 0
+
+    def spacing(n):
+        w = np.ones(n)
+        for q in range(1, n):
+            w[q] = min(w[q - 1] * 1.25, 20.0)
+        return np.minimum(w, w[::-1])
+
+    while i < len(lines):
+        ln = lines[i]
+        up = ln.upper().strip()
+        if up.startswith("IM") and "KM" in up and i + 1 < len(lines):
+            out += [ln, f"{im:10d}{km:10d}"]
+            i += 2
+            continue
+        if up.startswith("FP(I)") or up.startswith("FR(K)"):
+            n = (im if up.startswith("FP") else km) - 1
+            out.append(ln)
+            out.append(_fmt(spacing(n)))
+            i += 1
+            while i < len(lines) and re.match(r"^\s*[-0-9.]", lines[i]):
+                i += 1
+            continue
+        if up.startswith("STREAM SURFACES"):
+            out += [ln, " " + _ints([0] * km)]
+            i += 2
+            continue
+        if up.startswith("PDOWN_HUB") and pexit is not None:
+            out += [ln, f"{pexit:10.1f}{pexit:10.1f}"]
+            i += 2
+            continue
+        if up.startswith("NSTEPS_MAX") and nsteps is not None:
+            rest = lines[i + 1].split()
+            out += [ln, f"{nsteps:10d}  " + "  ".join(rest[1:])]
+            i += 2
+            continue
+        out.append(ln)
+        i += 1
+    return "\n".join(out)
+
+
+SPEEDLINE_BASE = "simple-axial-compr-17.1.dat"   # the reference's one-stage axial compressor (test deck)
+
+
+def speedline_deck(base_text: str, i: int, n_points: int = 16, nsteps: int = 6000, seed: int = 3) -> str:
+    """Point i of the 16-point compressor speedline: the reference's
+    one-stage axial compressor (rotor 50 blades at 5000 rpm, stator; 2 rows
+    of 111 J-points; mixing length) with the pitchwise and spanwise grid of a
+    seeded draw, IM from {37, 49} and KM from {33, 49, 65}, and the exit
+    static pressure linearly spaced from 0.95 to 1.08 times the deck's own
+    (choke side towards stall).  A comment line opening with "(" loses that
+    parenthesis, so that flang's list-directed read of the comment into a
+    CHARACTER variable takes the whole line; no input value changes."""
+    im, km = speedline_sizes(n_points, seed)[i]
+    text = regrid_deck(base_text, im, km, speedline_pexit(i, n_points), nsteps)
+    return "\n".join(ln.replace("(", " ", 1) if ln.strip().startswith("(") else ln for ln in text.split("\n"))

@@ -2,7 +2,9 @@
 
 For the single stator row (37 x 33 x 97, mixing length), the 1.5-stage
 turbine (3 rows of 64 x 64 x 193, Spalart-Allmaras) and the first and last
-points of the 16-point compressor speedline, runs the reference built by
+points of the 16-point compressor speedline (the reference's one-stage
+axial compressor re-gridded, multall_hip.deckgen.speedline_deck), runs the
+reference built by
 oracle/build_ref.sh for 1 and 100 time steps from the deck's initial guess
 and stores, per case:
 
@@ -52,11 +54,12 @@ sys.path.insert(0, HERE)
 from multall_hip import deckgen  # noqa: E402
 
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE"]
+SPEEDLINE_BASE = os.path.join(HERE, "decks", "ref", deckgen.SPEEDLINE_BASE)
 CASES = {
-    "config0": lambda n: deckgen.deck_config0(nsteps=n),
-    "config1": lambda n: deckgen.deck_config1(nsteps=n),
-    "speedline0": lambda n: deckgen.deck_speedline_point(0, nsteps=n),
-    "speedline15": lambda n: deckgen.deck_speedline_point(15, nsteps=n),
+    "config0": lambda n: deckgen.write_deck(deckgen.deck_config0(nsteps=n)),
+    "config1": lambda n: deckgen.write_deck(deckgen.deck_config1(nsteps=n)),
+    "speedline0": lambda n: deckgen.speedline_deck(open(SPEEDLINE_BASE).read(), 0, nsteps=n),
+    "speedline15": lambda n: deckgen.speedline_deck(open(SPEEDLINE_BASE).read(), 15, nsteps=n),
 }
 STEPS = (1, 100)
 DIGESTS = os.path.join(HERE, "config_setup_digests.json")
@@ -110,8 +113,7 @@ def make(case: str) -> None:
     from multall_hip import host
     digests = json.load(open(DIGESTS)) if os.path.exists(DIGESTS) else {}
     for n in STEPS:
-        spec = CASES[case](n)
-        txt = deckgen.write_deck(spec)
+        txt = CASES[case](n)
         wd = tempfile.mkdtemp(prefix="mhcfg_")
         try:
             setup, final, _ = refdump.run_reference(txt, "fp32", workdir=wd)
@@ -151,7 +153,7 @@ def make(case: str) -> None:
 def make_fp64(case: str, n: int = 100) -> None:
     import orc
     import refdump
-    txt = deckgen.write_deck(CASES[case](n))
+    txt = CASES[case](n)
     _, final, _ = refdump.run_reference(txt, "fp64")
     kmid = int(orc.from_dump(final)["scalars"]["KMID"])
     out = summary({v: final.node(v) for v in VARS}, kmid)

@@ -81,12 +81,16 @@ def test_fp32_reference_within_bounds_of_fp64(case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("case", CASES)
-def test_converged_vs_fp64_reference(case):
+def test_converged_vs_fp64_reference(case, mixed):
+    """Plain FP32 and the mixed form (SSS histories stored in BF16, FP32
+    arithmetic): both within the bounds of the FP64 reference."""
     from multall_hip.runtime import Solver
     d = _fixture(case)
     st, ga = _ga(case)
     g = Solver(st, exact_order=False)
+    g.set_scalar("MIXED", 1 if mixed else 0)
     g.loss_init()
     g.start()
     r = g.step(int(st["scalars"]["NSTEPS_MAX"]) + 10)
@@ -96,4 +100,4 @@ def test_converged_vs_fp64_reference(case):
           f"reference FP64 at {d['fp64']['nstep']}, FP32 at {d['fp32']['nstep']}")
     q = quantities(g.integrals()[:5], ga)
     q64 = quantities(d["fp64"]["integrals"], ga)
-    _check(q, q64, "GPU (fast mode)")
+    _check(q, q64, "GPU (mixed form)" if mixed else "GPU (fast mode)")

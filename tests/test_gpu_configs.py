@@ -1,7 +1,9 @@
 """The benchmark workloads against the reference: the single turbine stator
 row (37 x 33 x 97, mixing length), the 1.5-stage turbine (3 rows of
 64 x 64 x 193, Spalart-Allmaras: the bench's own workload) and the first and
-last points of the 16-point compressor speedline, each read and set up by
+last points of the 16-point compressor speedline (the reference's one-stage
+axial compressor deck re-gridded, at its choke- and stall-side exit
+pressures), each read and set up by
 the Fortran host (bit for bit the reference's set-up, test_host_setup.py),
 marched 1 and 100 steps in both reduction modes and compared with the
 reference's summaries (tests/golden/make_configs.py): the pitchwise mean of
@@ -116,11 +118,12 @@ def test_speedline_batch_equals_one_by_one():
     from multall_hip.runtime import Solver, step_batch
     import tempfile
     states = []
+    base = open(os.path.join(GOLDEN, "decks", "ref", deckgen.SPEEDLINE_BASE)).read()
     with tempfile.TemporaryDirectory() as d:
         for i in range(16):
             path = os.path.join(d, f"p{i}.dat")
             with open(path, "w") as f:
-                f.write(deckgen.write_deck(deckgen.deck_speedline_point(i, nsteps=100)))
+                f.write(deckgen.speedline_deck(base, i, nsteps=100))
             st = host.load_deck(path, xlength_on_host=False)
             s = st["scalars"]
             st["_xl"] = (host.array("THETA", (s["KM"], s["JM"], s["IM"])),
