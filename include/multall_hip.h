@@ -14,7 +14,9 @@
  *   - mh_set_array / mh_get_array stand for the COMMON blocks the loop
  *     reads and writes (commall-open-18.3:13-203);
  *   - mh_integrals replaces SUMFLX (mf:7873-7912) as called by EFICOOL
- *     (mf:10741), accumulating in FP64.
+ *     (mf:10741), accumulating in FP64;
+ *   - mh_eficool replaces EFICOOL's stage and machine performance
+ *     (mf:11253-11548), in FP64.
  *
  * Conventions: arrays use the reference's names and its I-fastest,
  * column-major order.  Dense arrays are packed to the logical extents
@@ -123,6 +125,10 @@ enum mh_array_id {
    * forces there, as the reference does before OUTPUT (mf:3767-3769,
    * 5338-5341) -- 5 ints; unset (0) means none inside the run */
   MH_NOUT,
+  /* the stage number of each blade row (BKSTAG NSTAGE, read at mf:710-711)
+   * -- RN ints; unset means the reference's default 1 + (N-1)/2 (mf:704-706).
+   * Used only by mh_eficool. */
+  MH_NSTAGE,
   MH_NARRAYS
 };
 
@@ -157,6 +163,11 @@ enum mh_scalar_id {
   MH_S_FLOW_CONV,     /* > 0: also end the run once the inlet mass flow has changed by less than this
                          fraction between each two of the last 10 every-5-step checks (50 steps) and
                          ECONT < 0.01 (default 0: off; the reference's own stop tests always apply) */
+  MH_S_TUNE,          /* 1 (default): at mh_start, time the tile shapes of the damping/corner-distribution
+                         and SMOOTH_VAR kernels on this grid and keep the fastest (remembered per grid
+                         shape for the process); 0: the fixed shapes; every shape gives the same bits */
+  MH_S_SCAT_TK,       /* 2, 4 or 8: force the damping/corner tile's K lines (0, default: the tuner's) */
+  MH_S_SMOOTH_WAVES,  /* 8 or 16: force SMOOTH_VAR's wavefronts per J plane (0, default: the tuner's) */
   MH_NSCALARS
 };
 
@@ -249,6 +260,37 @@ int mh_step_batch(mh_ctx *const *ctxs, int n, int nsteps, mh_resid *out, int *fa
  * out[7*JM] = {flow, sum_po, sum_to, sum_rvt, sum_entropy, sum_tstat,
  * sum_pstat} per J, with p0/po1 and T0 from the current state. */
 int mh_integrals(mh_ctx *ctx, double *out, long long n);
+
+/* Stage and machine performance from mass averages: EFICOOL's uncooled
+ * branch (mf:11253-11548; IFCOOL = IFBLEED = 0, IFGAS = 0), in FP64 on the
+ * device from the SUMFLX integrals of the current state.  One record per
+ * stage (the rows of one NSTAGE value; mf:3600-3614), then one for the whole
+ * machine (stage = 0) when there is more than one stage, as the reference
+ * prints them.  A stage's planes are J1 = its first J and J2M1 = the J after
+ * its last row's JMIX (JM - 1 for the last stage); the whole machine's are
+ * J = 2 and JM - 1.  Quantities at a plane are mass averages (SUMFLX sums
+ * over BLADE_FLOW); pressures in Pa, temperatures in K, flows in kg/s for
+ * the whole annulus. */
+typedef struct {
+  int stage;            /* 1 .. NSTAGES; 0 = the whole machine */
+  int j1, j2m1, jmix;   /* inlet, exit and mid-stage planes (1-based J) */
+  int compressor;       /* 1 when the stagnation temperature rises (DELTO >= 0) */
+  int pad_;
+  double po_in, po_out;       /* stagnation pressure (SUMPO x PO1(KMID)) */
+  double ps_in, ps_out;       /* static pressure */
+  double to_in, to_out;       /* stagnation temperature */
+  double flow_in, flow_out;   /* mass flow at J1 and J2M1 (the machine: J = 1 and JM - 1) */
+  double po_ratio;            /* po_out / po_in, not below 0.01 (PORAT) */
+  double eta_tt, eta_ts;      /* total-to-total and total-to-static isentropic efficiency */
+  double eta_poly;            /* total-to-total polytropic efficiency */
+  double reaction;            /* from the mass-averaged static temperatures at J1, JMIX, J2M1 (stages) */
+  double reaction_hub, reaction_mid, reaction_tip; /* from FP-weighted pitchwise static pressures at
+                                                      K = 1, KM/2, KM (stages) */
+  double work_kw;             /* CP (flow_in to_in - flow_out to_out) / 1000, kW */
+} mh_perf;
+/* Fills out[0 .. *n-1]; MH_E_ARG when cap < the number of records (then *n
+ * still says how many there are). */
+int mh_eficool(mh_ctx *ctx, mh_perf *out, int cap, int *n);
 /* Wait for all queued work; returns MH_E_HIP on an asynchronous error. */
 int mh_sync(mh_ctx *ctx);
 

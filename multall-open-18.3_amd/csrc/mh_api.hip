@@ -7,9 +7,11 @@
 // convergence figures come back to the host (one small copy) because the
 // next step's stop test needs them, exactly when the reference evaluates it.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -45,11 +47,18 @@ __global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int, int
 __global__ void k_damp_apply(Geo, Dev);
 __global__ void k_scatter(Geo, Dev, const float *, float *);
 __global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
-__global__ void k_smooth_reg1(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
-__global__ void k_smooth_reg2(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
-__global__ void k_smooth_reg4(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
-__global__ void k_smooth_reg8(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
-__global__ void k_smooth_reg16(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+#define SMOOTH_REG_DECL(L, NW) \
+  __global__ void k_smooth_reg##L##w##NW(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+SMOOTH_REG_DECL(1, 8)
+SMOOTH_REG_DECL(2, 8)
+SMOOTH_REG_DECL(4, 8)
+SMOOTH_REG_DECL(8, 8)
+SMOOTH_REG_DECL(16, 8)
+SMOOTH_REG_DECL(1, 16)
+SMOOTH_REG_DECL(2, 16)
+SMOOTH_REG_DECL(4, 16)
+SMOOTH_REG_DECL(8, 16)
+#undef SMOOTH_REG_DECL
 __global__ void k_fill_int(int *, int, int);
 __global__ void k_rolim_check(Geo, Dev, StepScalars);
 __global__ void k_rolim_fix(Geo, Dev, StepScalars, Params);
@@ -65,6 +74,7 @@ __global__ void k_stepup_temp(Geo, Dev, Params);
 __global__ void k_stepup_step(Geo, Dev, float);
 __global__ void k_stepup_mix(Geo, Dev);
 __global__ void k_integrals(Geo, Dev, Params);
+__global__ void k_eficool(Geo, Dev, Params, const int *, int, double *);
 __global__ void k_loss_vislam(Geo, Dev, Params);
 __global__ void k_loss_span(Geo, Dev, Params, float);
 __global__ void k_loss_pitch(Geo, Dev, Params, float);
@@ -82,8 +92,15 @@ __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
 __global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
 __global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
 __global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
-__global__ void k_damp_scatter1(Geo, Dev, Params, const float *, float *, int, int);
-__global__ void k_damp_scatter4(Geo, Dev, Params, const float *, float *, int, int);
+#define DAMP_SCATTER_DECL(NC, TK) \
+  __global__ void k_damp_scatter##NC##_k##TK(Geo, Dev, Params, const float *, float *, int, int);
+DAMP_SCATTER_DECL(1, 2)
+DAMP_SCATTER_DECL(1, 4)
+DAMP_SCATTER_DECL(1, 8)
+DAMP_SCATTER_DECL(4, 2)
+DAMP_SCATTER_DECL(4, 4)
+DAMP_SCATTER_DECL(4, 8)
+#undef DAMP_SCATTER_DECL
 __global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
 __global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
@@ -141,6 +158,7 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"XLENGTH", K_N, false}, {"FST_RAT", K_N, false}, {"FSTURB", K_RN, false}, {"TURBVIS_DAMP", K_RN, false},
     {"DPDS_CELL", K_N, false}, {"YPLUS_I1", K_JK, false}, {"YPLUS_IM", K_JK, false}, {"YPLUS_K1", K_IJ, false},
     {"YPLUS_KM", K_IJ, false}, {"CHORD", K_RN, false}, {"NOUT", K_NOUT, true},
+    {"NSTAGE", K_RN, true},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -151,7 +169,7 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
     "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "FAC_STMIX", "FAC_ST0", "FAC_ST1", "FAC_ST2", "FAC_ST3",
     "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "FLOWIN", "RFLOW", "VISLAM", "FTCOND",
-    "EXACT_ORDER", "USE_GRAPHS", "LANES", "MIXED", "FLOW_CONV",
+    "EXACT_ORDER", "USE_GRAPHS", "LANES", "MIXED", "FLOW_CONV", "TUNE", "SCAT_TK", "SMOOTH_WAVES",
 };
 
 }  // namespace mh
@@ -259,6 +277,7 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.ROAVG_CELL, (void **)&d.VISC_LAM, (void **)&d.VISC_RAT, (void **)&d.XLENGTH, (void **)&d.FST_RAT,
       (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP, (void **)&d.DPDS_CELL, (void **)&d.YPLUS_I1,
       (void **)&d.YPLUS_IM, (void **)&d.YPLUS_K1, (void **)&d.YPLUS_KM, (void **)&d.CHORD, (void **)&d.NOUT,
+      (void **)&d.NSTAGE,
   };
   return base[id];
 }
@@ -360,10 +379,11 @@ mh_prof_rec prof_rec(mh_ctx *c, const char *name) {
   rec.t1 = prof_event(c);
   return rec;
 }
-#define LAUNCH(kern, grid, block, shmem, stream, ...)                                              \
+#define LAUNCH(kern, grid, block, shmem, stream, ...) LAUNCH_NAMED(kern, #kern, grid, block, shmem, stream, __VA_ARGS__)
+#define LAUNCH_NAMED(kern, name, grid, block, shmem, stream, ...)                                  \
   do {                                                                                             \
     if (c->prof_on) {                                                                              \
-      mh_prof_rec r_ = prof_rec(c, #kern);                                                         \
+      mh_prof_rec r_ = prof_rec(c, name);                                                          \
       if (r_.t0 && r_.t1) {                                                                        \
         hipExtLaunchKernelGGL(kern, grid, block, shmem, stream, r_.t0, r_.t1, 0, __VA_ARGS__);   \
         c->prof_pending.push_back(r_);                                                             \
@@ -449,7 +469,7 @@ StepPlan plan_step(mh_ctx *c, int n) {
   for (int i = 0; i < 5; i++) ifprint = ifprint || n == c->nout[i];
   q.refresh = (q.check5 && ifprint) || end_known;
   const float FCHANGE = 1.0f - (float)(n - 1) / (float)p.NCHANGE;
-  if (FCHANGE > 0.0f) c->DAMP = p.DAMPIN * (1.f - 0.75f * FCHANGE);
+  if (!(p.IF_RESTART != 0 && n > 100) && FCHANGE > 0.0f) c->DAMP = p.DAMPIN * (1.f - 0.75f * FCHANGE);
   q.damp = !(c->DAMP < 2.0f || c->DAMP > 100);
   q.first = n == 1;
   return q;
@@ -613,6 +633,123 @@ int hist_convert(mh_ctx *c, int i, bool to_bf16) {
 }
 
 // One TSTEP call (mf:5470-5932) on lane `lane`, gated on CTL slot `gate`.
+// ---------------------------------------------------------------------------
+// Tile-shape variants of the damping/corner-distribution kernel (TK node K
+// lines per workgroup) and of SMOOTH_VAR's register form (NW wavefronts per J
+// plane, L = the smallest power of two >= KM/NW K lines each).  Every variant
+// does the same arithmetic for every node, so all give the same bits; which
+// is fastest depends on the grid (how many workgroups the launch has next to
+// the 256 CUs, how much of each tile is halo), so autotune() times them.
+struct ScatVariant {
+  void (*k)(Geo, Dev, Params, const float *, float *, int, int);
+  const char *name;
+};
+const ScatVariant &scat_variant(int imm1, int tk) {
+  static const ScatVariant v[2][3] = {
+      {{k_damp_scatter1_k2, "k_damp_scatter1_k2"}, {k_damp_scatter1_k4, "k_damp_scatter1_k4"},
+       {k_damp_scatter1_k8, "k_damp_scatter1_k8"}},
+      {{k_damp_scatter4_k2, "k_damp_scatter4_k2"}, {k_damp_scatter4_k4, "k_damp_scatter4_k4"},
+       {k_damp_scatter4_k8, "k_damp_scatter4_k8"}}};
+  return v[scatter_chunks(imm1) == 1 ? 0 : 1][tk == 2 ? 0 : tk == 4 ? 1 : 2];
+}
+dim3 scat_grid(const Geo &g, int tk) { return dim3((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + tk - 1) / tk); }
+
+struct SmoothVariant {
+  void (*k)(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+  const char *name;
+  int nw;
+};
+const SmoothVariant &smooth_variant(int km, int nw) {
+  static const SmoothVariant w8[5] = {{k_smooth_reg1w8, "k_smooth_reg1w8", 8}, {k_smooth_reg2w8, "k_smooth_reg2w8", 8},
+                                      {k_smooth_reg4w8, "k_smooth_reg4w8", 8}, {k_smooth_reg8w8, "k_smooth_reg8w8", 8},
+                                      {k_smooth_reg16w8, "k_smooth_reg16w8", 8}};
+  static const SmoothVariant w16[4] = {{k_smooth_reg1w16, "k_smooth_reg1w16", 16},
+                                       {k_smooth_reg2w16, "k_smooth_reg2w16", 16},
+                                       {k_smooth_reg4w16, "k_smooth_reg4w16", 16},
+                                       {k_smooth_reg8w16, "k_smooth_reg8w16", 16}};
+  const int lines = (km + nw - 1) / nw;
+  const int li = lines <= 1 ? 0 : lines <= 2 ? 1 : lines <= 4 ? 2 : lines <= 8 ? 3 : 4;
+  return nw == 16 ? w16[std::min(li, 3)] : w8[li];
+}
+
+// The tuner: for each kernel, one launch to warm up, then three launches
+// between two events on the context's stream, for every shape; the fastest
+// is kept, per grid shape for the whole process.  The launches read the
+// context's own arrays and write only the scratch arrays (the corner
+// distribution's output DSC, and TMP in place of SMOOTH_VAR's output), so
+// the state is left as it was.
+int autotune(mh_ctx *c) {
+  static std::mutex mu;
+  static std::map<std::array<int, 5>, std::pair<int, int>> known;
+  const Geo &g = c->g;
+  const Params p = params_of(c);
+  const std::array<int, 5> key = {g.IM, g.JM, g.KM, g.NROWS, p.EXACT};
+  const int ftk = (int)c->scal[MH_S_SCAT_TK], fnw = (int)c->scal[MH_S_SMOOTH_WAVES];
+  if (ftk && ftk != 2 && ftk != 4 && ftk != 8) return fail(c, MH_E_ARG, "SCAT_TK must be 0, 2, 4 or 8");
+  if (fnw && fnw != 8 && fnw != 16) return fail(c, MH_E_ARG, "SMOOTH_WAVES must be 0, 8 or 16");
+  if (fnw == 16 && g.KM > 128) return fail(c, MH_E_ARG, "SMOOTH_WAVES = 16 needs KM <= 128");
+  c->scat_tk = ftk ? ftk : 8;
+  c->smooth_nw = fnw ? fnw : 8;
+  if (c->scal[MH_S_TUNE] == 0.0 || (ftk && fnw)) return MH_OK;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(key);
+    if (it != known.end()) {
+      if (!ftk) c->scat_tk = it->second.first;
+      if (!fnw) c->smooth_nw = it->second.second;
+      return MH_OK;
+    }
+  }
+  const bool prof = c->prof_on;
+  c->prof_on = false;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const Dev &d = c->d;
+  const StepScalars s = step_scalars(c);
+  auto time_it = [&](auto launch) -> float {
+    launch();
+    (void)hipEventRecord(e0, c->stream);
+    for (int i = 0; i < 3; i++) launch();
+    (void)hipEventRecord(e1, c->stream);
+    float ms = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return 1e30f;
+    return ms;
+  };
+  int best_tk = c->scat_tk, best_nw = c->smooth_nw;
+  if (!ftk) {
+    float best = 1e30f;
+    for (int tk : kScatTKs) {
+      const ScatVariant &v = scat_variant(g.IMM1, tk);
+      const float t = time_it([&] {
+        hipLaunchKernelGGL(v.k, scat_grid(g, tk), dim3(256), scatter_lds_bytes(g.IMM1, tk), c->stream, g, d, p,
+                           (const float *)d.DRO, d.DSC, 1, 0);
+      });
+      if (t < best) best = t, best_tk = tk;
+    }
+  }
+  if (!fnw && g.IM <= 64 && g.KM <= 128) {
+    float best = 1e30f;
+    for (int nw : kSmoothWaves) {
+      const SmoothVariant &v = smooth_variant(g.KM, nw);
+      const float t = time_it([&] {
+        hipLaunchKernelGGL(v.k, dim3(g.JM), dim3(64 * v.nw), 0, c->stream, g, d, (const float *)d.DSC, d.TMP, p, s,
+                           1, 1);
+      });
+      if (t < best) best = t, best_nw = nw;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->prof_on = prof;
+  HIPCHK(hipGetLastError());
+  c->scat_tk = best_tk;
+  c->smooth_nw = best_nw;
+  std::lock_guard<std::mutex> lk(mu);
+  known[key] = {best_tk, best_nw};
+  return MH_OK;
+}
+
 int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan &q, int gate) {
   const Geo &g = c->g;
   Dev d = lane_dev(c, lane, gate);
@@ -714,13 +851,9 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan 
   // damping, second mixing-plane averages (mf:5786-5818) and the corner
   // distribution
   {
-    const dim3 sg((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + kScatTK - 1) / kScatTK);
-    if (g.IMM1 <= 64)
-      LAUNCH(k_damp_scatter1, sg, dim3(256), scatter_lds_bytes(g.IMM1), st, g, d, p, (const float *)D, d.DSC,
-             damp ? 1 : 0, nmp > 0 ? 1 : 0);
-    else
-      LAUNCH(k_damp_scatter4, sg, dim3(256), scatter_lds_bytes(g.IMM1), st, g, d, p, (const float *)D, d.DSC,
-             damp ? 1 : 0, nmp > 0 ? 1 : 0);
+    const ScatVariant &v = scat_variant(g.IMM1, c->scat_tk);
+    LAUNCH_NAMED(v.k, v.name, scat_grid(g, c->scat_tk), dim3(256), scatter_lds_bytes(g.IMM1, c->scat_tk), st, g, d,
+                 p, (const float *)D, d.DSC, damp ? 1 : 0, nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
   // SMOOTH_VAR: planes in registers where IM <= 64 and KM <= 128, the LDS
@@ -728,14 +861,9 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan 
   const int jexit = (!(p.SFEXIT < 0.0001f) && p.NSFEXIT > 0) ? std::max(1, g.JM + 1 - p.NSFEXIT) : g.JM + 1;
   int jreg = 1;  // first plane for the LDS form
   if (g.IM <= 64 && g.KM <= 128 && jexit > 1) {
-    const int lines = (g.KM + 7) / 8;
-    const dim3 rg(jexit - 1);
-    const float *src = d.DSC;
-    if (lines <= 1) LAUNCH(k_smooth_reg1, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
-    else if (lines <= 2) LAUNCH(k_smooth_reg2, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
-    else if (lines <= 4) LAUNCH(k_smooth_reg4, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
-    else if (lines <= 8) LAUNCH(k_smooth_reg8, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
-    else LAUNCH(k_smooth_reg16, rg, dim3(512), 0, st, g, d, src, D, p, s, ncall, 1);
+    const SmoothVariant &v = smooth_variant(g.KM, c->smooth_nw);
+    LAUNCH_NAMED(v.k, v.name, dim3(jexit - 1), dim3(64 * v.nw), 0, st, g, d, (const float *)d.DSC, D, p, s, ncall,
+                 1);
     jreg = jexit;
   }
   if (jreg <= g.JM)
@@ -1266,6 +1394,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   d.gate = -1;
   c->scal[MH_S_USE_GRAPHS] = 1.0;  // default: steps run as captured graphs
   c->scal[MH_S_LANES] = 1.0;       // default: concurrent equation streams (fast mode)
+  c->scal[MH_S_TUNE] = 1.0;        // default: tile shapes timed at mh_start
   if (hipHostMalloc((void **)&c->ctl_host, CTL_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_poll, hipEventDisableTiming) != hipSuccess)
     return bail(MH_E_HIP);
@@ -1273,10 +1402,12 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail(MH_E_HIP);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
-  if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1) > kMaxLds) return bail(MH_E_UNSUPPORTED);
-  if (raise_lds_limit((const void *)k_smooth_plane, c->lds_plane) ||
-      raise_lds_limit((const void *)k_damp_scatter4, scatter_lds_bytes(dm.im - 1)))
+  if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1, 8) > kMaxLds) return bail(MH_E_UNSUPPORTED);
+  if (raise_lds_limit((const void *)k_smooth_plane, c->lds_plane))
     return bail(fail(c, MH_E_HIP, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
+  for (int tk : kScatTKs)
+    if (raise_lds_limit((const void *)scat_variant(dm.im - 1, tk).k, scatter_lds_bytes(dm.im - 1, tk)))
+      return bail(fail(c, MH_E_HIP, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
   *out = c;
   return MH_OK;
 }
@@ -1384,7 +1515,7 @@ static int check_inputs(mh_ctx *c) {
   const bool forcing = c->scal_set[MH_S_IN_FLOW] && si(c, MH_S_IN_FLOW) != 0;
   for (int i = 0; i < MH_NSCALARS; i++)
     if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_USE_GRAPHS && i != MH_S_LANES && i != MH_S_MIXED &&
-        i != MH_S_FLOW_CONV &&
+        i != MH_S_FLOW_CONV && i != MH_S_TUNE && i != MH_S_SCAT_TK && i != MH_S_SMOOTH_WAVES &&
         i != MH_S_VISLAM &&
         i != MH_S_FTCOND &&
         (forcing || (i != MH_S_FLOWIN && i != MH_S_RFLOW)))
@@ -1392,7 +1523,7 @@ static int check_inputs(mh_ctx *c) {
   const Params p = params_of(c);
   if (p.IFGAS != 0) return fail(c, MH_E_UNSUPPORTED, "real-gas properties (IFGAS=1)");
   if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
-  if (p.IF_REPEAT != 0 || p.IF_RESTART != 0) return fail(c, MH_E_UNSUPPORTED, "IF_REPEAT or IF_RESTART set");
+  if (p.IF_REPEAT != 0) return fail(c, MH_E_UNSUPPORTED, "repeating-stage conditions (IF_REPEAT) set");
   if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 100)
     return fail(c, MH_E_UNSUPPORTED, "ILOS other than 0, 10 (LOSS), 100-199 (NEW_LOSS) or >= 200 (SPAL_LOSS)");
   if (p.ILOS != 0 && p.NLOS < 1) return fail(c, MH_E_ARG, "NLOS < 1");
@@ -1546,6 +1677,7 @@ int mh_start(mh_ctx *c) {
     }
     c->mixed = true;
   }
+  if ((r = autotune(c))) return r;
   c->ifend = 0;
   c->nstep = 0;
   c->hstep = 0;
@@ -1668,6 +1800,79 @@ int mh_integrals(mh_ctx *c, double *out, long long n) {
   LCHK("integrals");
   HIPCHK(hipMemcpyAsync(out, c->d.INTEG, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return MH_OK;
+}
+
+// The stages of mf:3600-3614 from NSTAGE (the reference's default when it
+// was never set), each stage's planes as EFICOOL takes them (mf:11271-11295),
+// then k_integrals and k_eficool.
+int mh_eficool(mh_ctx *c, mh_perf *out, int cap, int *n) {
+  if (!c || !n || cap < 0 || (cap > 0 && !out)) return fail(c, MH_E_ARG, "mh_eficool: bad arguments");
+  const Geo &g = c->g;
+  const int NR = g.NROWS, JM = g.JM;
+  std::vector<int> nst(NR), jstart(NR + 1), jmix(NR + 1), nrow(JM);
+  int r;
+  if ((r = d2h(c, nst.data(), c->d.NSTAGE, NR * 4)) || (r = d2h(c, jstart.data(), c->d.JSTART, (NR + 1) * 4)) ||
+      (r = d2h(c, jmix.data(), c->d.JMIX, (NR + 1) * 4)) || (r = d2h(c, nrow.data(), c->d.NROW, JM * 4)))
+    return r;
+  if (std::all_of(nst.begin(), nst.end(), [](int v) { return v == 0; }))
+    for (int N = 1; N <= NR; N++) nst[N - 1] = 1 + (N - 1) / 2;
+  const int NSTAGES = nst[NR - 1];
+  if (NSTAGES < 1 || NSTAGES > NR) return fail(c, MH_E_ARG, "mh_eficool: NSTAGE of the last row out of 1..NROWS");
+  std::vector<int> js(NSTAGES + 2, 0), je(NSTAGES + 2, 0);
+  js[1] = 1;
+  for (int N = 1; N <= NR; N++) {
+    const int a = nst[N - 1], b = N < NR ? nst[N] : NSTAGES + 1;
+    if (a < 1 || a > NSTAGES || b < 1 || b > NSTAGES + 1) return fail(c, MH_E_ARG, "mh_eficool: NSTAGE out of range");
+    if (a != b) {
+      je[a] = jmix[N - 1];
+      js[b] = N < NR ? jstart[N] : JM + 1;
+    }
+  }
+  std::vector<int> tab;
+  auto add = [&](int j1, int j2m1, int jm, int stage) {
+    tab.insert(tab.end(), {j1, j2m1, jm, stage});
+  };
+  for (int S = 1; S <= NSTAGES; S++) {
+    const int J1 = js[S];
+    const int J2M1 = S == NSTAGES ? JM - 1 : je[S] + 1;
+    if (J1 < 1 || J1 + 1 > JM) return fail(c, MH_E_ARG, "mh_eficool: a stage's first plane is out of the grid");
+    const int NRW = nrow[J1];  // NROW(J1+1)
+    if (NRW < 1 || NRW > NR) return fail(c, MH_E_ARG, "mh_eficool: NROW");
+    add(J1, J2M1, jmix[NRW - 1], S);
+  }
+  if (NSTAGES > 1) add(2, JM - 1, (2 + JM) / 2, 0);
+  const int nrec = (int)tab.size() / 4;
+  for (int i = 0; i < nrec; i++)
+    for (int q = 0; q < 3; q++)
+      if (tab[4 * i + q] < 1 || tab[4 * i + q] > JM) return fail(c, MH_E_ARG, "mh_eficool: a plane is out of the grid");
+  *n = nrec;
+  if (cap < nrec) return fail(c, MH_E_ARG, "mh_eficool: room for " + std::to_string(nrec) + " records needed");
+  if (!c->eff_tab && ((r = dalloc(c, &c->eff_tab, 4 * (size_t)(NR + 1))) ||
+                      (r = dalloc(c, &c->eff_out, (size_t)kEffFields * (NR + 1)))))
+    return r;
+  if ((r = h2d(c, c->eff_tab, tab.data(), tab.size() * 4))) return r;
+  const Params p = params_of(c);
+  LAUNCH(k_integrals, dim3(JM), dim3(256), 0, c->stream, c->g, c->d, p);
+  LCHK("integrals");
+  LAUNCH(k_eficool, dim3(1), dim3(64), 0, c->stream, c->g, c->d, p, c->eff_tab, nrec, c->eff_out);
+  LCHK("eficool");
+  std::vector<double> v((size_t)kEffFields * nrec);
+  if ((r = d2h(c, v.data(), c->eff_out, v.size() * 8))) return r;
+  for (int i = 0; i < nrec; i++) {
+    const double *f = &v[(size_t)kEffFields * i];
+    mh_perf &o = out[i];
+    o = mh_perf{};
+    o.stage = tab[4 * i + 3];
+    o.j1 = tab[4 * i];
+    o.j2m1 = tab[4 * i + 1];
+    o.jmix = tab[4 * i + 2];
+    o.compressor = f[17] != 0.0;
+    o.po_in = f[0]; o.po_out = f[1]; o.ps_in = f[2]; o.ps_out = f[3]; o.to_in = f[4]; o.to_out = f[5];
+    o.flow_in = f[6]; o.flow_out = f[7]; o.po_ratio = f[8]; o.eta_tt = f[9]; o.eta_ts = f[10];
+    o.eta_poly = f[11]; o.reaction = f[12]; o.reaction_hub = f[13]; o.reaction_mid = f[14];
+    o.reaction_tip = f[15]; o.work_kw = f[16];
+  }
   return MH_OK;
 }
 

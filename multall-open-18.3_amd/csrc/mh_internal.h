@@ -59,6 +59,7 @@ struct Dev {
   float *FSTURB, *TURBVIS_DAMP;
   float *DPDS_CELL, *YPLUS_I1, *YPLUS_IM, *YPLUS_K1, *YPLUS_KM, *CHORD;
   int *NOUT;
+  int *NSTAGE;
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
@@ -163,16 +164,20 @@ struct Params {
 // Host-built J membership lists (CSR, 1-based J values) of the multigrid
 // blocks (JB1, JB2 incl. the mixing-plane dummy block), the superblocks
 // (JSBLK) and the row interiors used by the damping average.
-// Node tile of the damping + corner-distribution kernel (J x K lines, all
-// I) and its dynamic LDS: cell lines (J..J+TJ, K-1..K+TK-1) and the four
-// corner-weight arrays on K-1..K+TK-1, each `pitch` = 64 NC floats (NC = 1
-// for IM-1 <= 64, else 4) plus a zero pad column at each end.
-constexpr int kScatTJ = 4, kScatTK = 8;
+// Node tile of the damping + corner-distribution kernel (TJ x TK node
+// lines, all I; TK is tuned per grid from kScatTKs) and its dynamic LDS:
+// cell lines (J..J+TJ, K-1..K+TK-1) and the four corner-weight arrays on
+// K-1..K+TK-1, each `pitch` = 64 NC floats (NC = 1 for IM-1 <= 64, else 4)
+// plus a zero pad column at each end.
+constexpr int kScatTJ = 4;
+constexpr int kScatTKs[3] = {2, 4, 8};
 inline int scatter_chunks(int imm1) { return imm1 <= 64 ? 1 : 4; }
-inline size_t scatter_lds_bytes(int imm1) {
-  return (size_t)((kScatTJ + 1) * (kScatTK + 1) + 4 * (kScatTK + 1)) * (64 * scatter_chunks(imm1) + 2) *
-         sizeof(float);
+inline size_t scatter_lds_bytes(int imm1, int tk) {
+  return (size_t)((kScatTJ + 1) * (tk + 1) + 4 * (tk + 1)) * (64 * scatter_chunks(imm1) + 2) * sizeof(float);
 }
+// SMOOTH_VAR's register form: wavefronts per workgroup (one J plane per
+// workgroup), tuned per grid from kSmoothWaves.
+constexpr int kSmoothWaves[2] = {8, 16};
 
 // Fast-mode multigrid sums: per-(level-1 I block, J, level-1 K block) sums
 // P and per-(J, level-1 K block) line totals PT, with each level-2 block's
@@ -200,6 +205,10 @@ struct mh_prof_rec {
 };
 
 namespace mh {
+// doubles per record of k_eficool (mh_eficool): po in/out, ps in/out, to
+// in/out, flow in/out, PORAT, eta tt/ts/poly, reaction, reactions at hub,
+// mid and tip, work, compressor flag
+constexpr int kEffFields = 18;
 constexpr int kHist = 6;  // history arrays with a BF16 copy in the mixed form
 constexpr int kAhead = 8; // chunks (of up to 5 steps) the host may queue ahead of the device
 // A second or third HIP stream with its own flux and TSTEP scratch, so the
@@ -241,6 +250,9 @@ struct mh_ctx {
   int nout[5] = {};       // printout steps (read from the device at mh_start)
   bool stop_req = false;  // mh_request_stop: end at the next step divisible by 10
   int device = 0;             // the HIP device the context was created on
+  int scat_tk = 8, smooth_nw = 8;  // tile shapes in use (autotune at mh_start)
+  int *eff_tab = nullptr;        // mh_eficool's plane table and records (device)
+  double *eff_out = nullptr;
   float *xl_scratch = nullptr;  // that device's address of mh_set_xlength_noalloc's scratch
   float DAMP = 0;                            // host mirror of the device's ramped DAMP (launch choices)
   bool mixed = false;                       // the SSS histories stored in BF16 (set at mh_start)

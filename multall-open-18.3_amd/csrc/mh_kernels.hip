@@ -1231,11 +1231,11 @@ __device__ __forceinline__ float damped(const Geo &g, const Dev &d, const float 
 // as y * rcp(1 + |y| rAD) with rAD = 1 / (A DAMP) per J line; the exact mode
 // keeps the reference's two divisions.  Dynamic LDS: kTL + kWL lines of
 // `pitch` = 64 NC + 2 floats.
-constexpr int kTJ = kScatTJ, kTK = kScatTK, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4, kWL = 4 * (kTK + 1);
-static_assert(kTJ == 4, "one wavefront per tile J line");
-template <int NC>
+template <int NC, int kTK>
 __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *D, float *DOUT,
                                              int damp, int mpav) {
+  constexpr int kTJ = kScatTJ, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4, kWL = 4 * (kTK + 1);
+  static_assert(kTJ == 4, "one wavefront per tile J line");
   const float DAMP = d.SC[SC_DAMP];
   extern __shared__ float lds[];
   __shared__ float avg[kTJ + 1], rad[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
@@ -1398,16 +1398,19 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
     }
   }
 }
-__global__ void __launch_bounds__(256) k_damp_scatter1(Geo g, Dev d, Params p, const float *D, float *DOUT,
-                                                       int damp, int mpav) {
-  if (gated(d)) return;
-  damp_scatter<1>(g, d, p, D, DOUT, damp, mpav);
-}
-__global__ void __launch_bounds__(256) k_damp_scatter4(Geo g, Dev d, Params p, const float *D, float *DOUT,
-                                                       int damp, int mpav) {
-  if (gated(d)) return;
-  damp_scatter<4>(g, d, p, D, DOUT, damp, mpav);
-}
+#define DAMP_SCATTER(NC, TK)                                                                                 \
+  __global__ void __launch_bounds__(256) k_damp_scatter##NC##_k##TK(Geo g, Dev d, Params p, const float *D,   \
+                                                                    float *DOUT, int damp, int mpav) {        \
+    if (gated(d)) return;                                                                                    \
+    damp_scatter<NC, TK>(g, d, p, D, DOUT, damp, mpav);                                                      \
+  }
+DAMP_SCATTER(1, 2)
+DAMP_SCATTER(1, 4)
+DAMP_SCATTER(1, 8)
+DAMP_SCATTER(4, 2)
+DAMP_SCATTER(4, 4)
+DAMP_SCATTER(4, 8)
+#undef DAMP_SCATTER
 
 // SMOOTH_VAR (mf:15352-15594) followed by TSTEP's periodic, tip-gap and
 // mixing-plane conditions on D (mf:5858-5926).  One workgroup (8
@@ -1854,17 +1857,22 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
   }
 }
 
-#define SMOOTH_REG(L)                                                                                         \
-  __global__ void __launch_bounds__(512) k_smooth_reg##L(Geo g, Dev d, const float *SRC, float *DST, Params p, \
-                                                         StepScalars s, int ncall, int j0) {                   \
-    if (gated(d)) return;                                                                                     \
-    smooth_reg<L>(g, d, SRC, DST, p, s, ncall, j0);                                                           \
+// k_smooth_reg<L>w<NW>: NW wavefronts per J plane, L K lines each
+#define SMOOTH_REG(L, NW)                                                                                    \
+  __global__ void __launch_bounds__(64 * NW) k_smooth_reg##L##w##NW(Geo g, Dev d, const float *SRC, float *DST, \
+                                                                   Params p, StepScalars s, int ncall, int j0) { \
+    if (gated(d)) return;                                                                                    \
+    smooth_reg<L>(g, d, SRC, DST, p, s, ncall, j0);                                                          \
   }
-SMOOTH_REG(1)
-SMOOTH_REG(2)
-SMOOTH_REG(4)
-SMOOTH_REG(8)
-SMOOTH_REG(16)
+SMOOTH_REG(1, 8)
+SMOOTH_REG(2, 8)
+SMOOTH_REG(4, 8)
+SMOOTH_REG(8, 8)
+SMOOTH_REG(16, 8)
+SMOOTH_REG(1, 16)
+SMOOTH_REG(2, 16)
+SMOOTH_REG(4, 16)
+SMOOTH_REG(8, 16)
 #undef SMOOTH_REG
 
 // ---------------------------------------------------------------------------
@@ -3456,8 +3464,10 @@ __global__ void k_step_begin(Geo g, Dev d, Params p) {
   C[CTL_IFEND] = IFEND;
   C[CTL_SKIP] = 0;
   C[CTL_NOTEND] = IFEND ? 0 : 1;
+  // the smoothing and damping ramp (mf:3787-3798; from a restart file it
+  // stops after step 100)
   const float FCHANGE = 1.0f - (float)(NSTEP - 1) / (float)p.NCHANGE;
-  if (FCHANGE > 0.0f) {
+  if (!(p.IF_RESTART != 0 && NSTEP > 100) && FCHANGE > 0.0f) {
     S[SC_SFT] = p.SFTIN + 0.02f * FCHANGE;
     S[SC_SFX] = p.SFXIN + 0.02f * FCHANGE;
     S[SC_DAMP] = p.DAMPIN * (1.f - 0.75f * FCHANGE);
@@ -4131,6 +4141,71 @@ __global__ void k_integrals(Geo g, Dev d, Params p) {
   }
   if (threadIdx.x == 0)
     for (int q = 0; q < 7; q++) d.INTEG[(size_t)q * g.JM + J - 1] = sh[q][0];
+}
+
+// EFICOOL's stage and machine figures, uncooled branch (mf:11253-11548,
+// IFGAS = 0), in FP64 from the SUMFLX integrals in d.INTEG (k_integrals:
+// BLADE_FLOW, SUMPO, SUMTO, SUMRVT, SUM_ENTPY, SUMTSTAT, SUMPSTAT per J) and,
+// for the pressure reactions, FP-weighted pitchwise sums of P at K = 1,
+// KM/2 and KM (mf:11299-11357).  One wavefront.  Record r has the planes
+// tab[4r .. 4r+2] = J1, J2M1, JMIX and tab[4r+3] = its stage (0 = the whole
+// machine); it writes kEffFields doubles at out[r * kEffFields].
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__global__ void __launch_bounds__(64) k_eficool(Geo g, Dev d, Params p, const int *tab, int nrec, double *out) {
+  const int lane = threadIdx.x, JM = g.JM;
+  const double *FLOW = d.INTEG, *SPO = FLOW + JM, *STO = FLOW + 2 * JM, *STS = FLOW + 5 * JM, *SPS = FLOW + 6 * JM;
+  const double CP = p.CP, GA = p.GA, FGA = (GA - 1.0) / GA, PO1M = d.PO1[p.KMID - 1];
+  const int KMIDE = g.KM / 2 > 0 ? g.KM / 2 : 1;
+  const int ks[3] = {1, KMIDE, g.KM};
+  for (int r = 0; r < nrec; r++) {
+    const int J1 = tab[4 * r], J2M1 = tab[4 * r + 1], JMIX = tab[4 * r + 2], stage = tab[4 * r + 3];
+    // pitchwise static pressures: [plane J1, JMIX, J2M1][K = 1, KM/2, KM]
+    double pp[3][3];
+    const int js[3] = {J1, JMIX, J2M1};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double s = 0.0;
+        for (int I = lane + 1; I <= g.IMM1; I += 64)
+          s += 0.5 * ((double)d.P[IX(I, js[a], ks[b])] + (double)d.P[IX(I + 1, js[a], ks[b])]) * (double)d.FP[I - 1];
+        pp[a][b] = wave_sum_d(s);
+      }
+    if (lane != 0) continue;
+    const double POIN = SPO[J1 - 1] / FLOW[J1 - 1] * PO1M, PO2 = SPO[J2M1 - 1] / FLOW[J2M1 - 1] * PO1M;
+    const double PSIN = SPS[J1 - 1] / FLOW[J1 - 1], PSOUT = SPS[J2M1 - 1] / FLOW[J2M1 - 1];
+    const double TO_IN = STO[J1 - 1] / FLOW[J1 - 1], TO_OUT = STO[J2M1 - 1] / FLOW[J2M1 - 1];
+    const double TORAT = TO_OUT / TO_IN, DELTO = TO_OUT - TO_IN;
+    double PORAT = PO2 / POIN;
+    const double PSRAT = PSOUT / POIN;
+    if (PORAT < 0.01) PORAT = 0.01;
+    const double DELTOIS = pow(PORAT, FGA) * TO_IN - TO_IN, DELT_TOT_STAT = pow(PSRAT, FGA) * TO_IN - TO_IN;
+    double ETAIS = 1.0, ETATS = 1.0;
+    if (DELTO > 0.0) { ETAIS = DELTOIS / DELTO; ETATS = DELT_TOT_STAT / DELTO; }
+    if (DELTO < 0.0) { ETAIS = DELTO / DELTOIS; ETATS = DELTO / DELT_TOT_STAT; }
+    double EPOLY = log(TORAT) / log(PORAT) / FGA;
+    if (TORAT > 1.0) EPOLY = 1.0 / EPOLY;
+    double REACTION = NAN, RH = NAN, RM = NAN, RT = NAN, WORK;
+    if (stage != 0) {
+      const double tin = STS[J1 - 1] / FLOW[J1 - 1], tmid = STS[JMIX - 1] / FLOW[JMIX - 1],
+                   tout = STS[J2M1 - 1] / FLOW[J2M1 - 1];
+      REACTION = (tmid - tout) / (tin - tout);
+      if (PSOUT > PSIN) REACTION = 1.0 - REACTION;
+      RH = (pp[1][0] - pp[2][0]) / (pp[0][0] - pp[2][0]);
+      RM = (pp[1][1] - pp[2][1]) / (pp[0][1] - pp[2][1]);
+      RT = (pp[1][2] - pp[2][2]) / (pp[0][2] - pp[2][2]);
+      if (pp[2][1] > pp[0][1]) { RH = 1.0 - RH; RM = 1.0 - RM; RT = 1.0 - RT; }
+      WORK = CP * (FLOW[J1 - 1] * TO_IN - FLOW[J2M1 - 1] * TO_OUT) / 1000.0;
+    } else {
+      WORK = CP * (FLOW[0] * TO_IN - FLOW[JM - 2] * TO_OUT) / 1000.0;
+    }
+    double *o = out + (size_t)r * kEffFields;
+    const double v[kEffFields] = {POIN, PO2, PSIN, PSOUT, TO_IN, TO_OUT, FLOW[J1 - 1], FLOW[J2M1 - 1], PORAT,
+                                  ETAIS, ETATS, EPOLY, REACTION, RH, RM, RT, WORK, DELTO >= 0.0 ? 1.0 : 0.0};
+    for (int f = 0; f < kEffFields; f++) o[f] = v[f];
+  }
 }
 
 }  // namespace mh

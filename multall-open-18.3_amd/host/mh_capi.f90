@@ -224,6 +224,7 @@ contains
     case ('FSTURB'); call putr1(FSTURB)
     case ('TURBVIS_DAMP'); call putr1(TURBVIS_DAMP)
     case ('ROAVG_CELL'); call putr3(ROAVG_CELL)
+    case ('TRANS_DYN_VIS'); call putr3(TRANS_DYN_VIS)
     case ('RTHETA'); call putr3(RTHETA)
     case ('AQTOT'); call putr3(AQTOT)
     case ('ABTOT'); call putr3(ABTOT)
@@ -288,6 +289,7 @@ contains
     case ('KS1'); call puti(KS1(1:NROWS + 1))
     case ('KS2'); call puti(KS2(1:NROWS + 1))
     case ('NOUT'); call puti(NOUT)
+    case ('NSTAGE'); call puti(NSTAGE(1:NROWS))
     end select
   contains
     subroutine putr1(a)

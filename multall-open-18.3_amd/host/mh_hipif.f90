@@ -14,6 +14,12 @@ module mh_hipif
     real(c_float) :: emax, eavg, econt, vref, vmax, flow_in, flow_out
   end type mh_resid
 
+  type, bind(C) :: mh_perf
+    integer(c_int) :: stage, j1, j2m1, jmix, compressor, pad_
+    real(c_double) :: po_in, po_out, ps_in, ps_out, to_in, to_out, flow_in, flow_out, po_ratio
+    real(c_double) :: eta_tt, eta_ts, eta_poly, reaction, reaction_hub, reaction_mid, reaction_tip, work_kw
+  end type mh_perf
+
   interface
     integer(c_int) function mh_create(dims, ctx) bind(C, name='mh_create')
       import :: c_int, c_ptr, mh_dims
@@ -94,6 +100,13 @@ module mh_hipif
       real(c_double), intent(out) :: out(*)
       integer(c_long_long), value :: n
     end function mh_integrals
+    integer(c_int) function mh_eficool(ctx, out, cap, n) bind(C, name='mh_eficool')
+      import :: c_int, c_ptr, mh_perf
+      type(c_ptr), value :: ctx
+      type(mh_perf), intent(out) :: out(*)
+      integer(c_int), value :: cap
+      integer(c_int), intent(out) :: n
+    end function mh_eficool
   end interface
 
 contains

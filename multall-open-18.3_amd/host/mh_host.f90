@@ -81,6 +81,8 @@ module mh_host
   real, allocatable, save, public :: TEMP2(:,:,:), DRO(:,:,:), DROVX(:,:,:), DROVR(:,:,:), DRORVT(:,:,:)
   real, allocatable, save, public :: DROE(:,:,:), XFORCE(:,:,:), RFORCE(:,:,:), TFORCE(:,:,:), QSOURCE(:,:,:)
   real, allocatable, save, public :: FLOWX(:,:,:), FLOWT(:,:,:), FLOWR(:,:,:), ROAVG_CELL(:,:,:)
+  ! the Spalart-Allmaras variable: zero, or read from the restart file
+  real, allocatable, save, public :: TRANS_DYN_VIS(:,:,:)
 
   ! ---- multigrid block time steps ------------------------------------------
   real, allocatable, save, public :: STEP1(:,:,:), STEP2(:,:,:), STEPSBK(:)
@@ -198,15 +200,13 @@ contains
     IF_RESTART = 0
     call skip(u)
     read (u, *, iostat=ios) IF_RESTART
-    if (IF_RESTART /= 0) then
-      bad = fail('restart (IF_RESTART) is not supported'); return
-    end if
     NSTEPS_MAX = 5000; CONLIM = 0.005
     call skip(u)
     read (u, *, iostat=ios) NSTEPS_MAX, CONLIM
     SFXIN = 0.005; SFTIN = 0.005; FAC_4TH = 0.8; NCHANGE = NSTEPS_MAX/4
     call skip(u)
     read (u, *, iostat=ios) SFXIN, SFTIN, FAC_4TH, NCHANGE
+    if (IF_RESTART /= 0) NCHANGE = 100   ! mf:382
     call skip(u)
     read (u, *) NROWS
     call skip(u)
@@ -631,7 +631,10 @@ contains
       end do
       call skip(u)
       read (u, *, iostat=ios) FACMIXUP, NMIXUP
-      if (FACMIXUP < 1.0) FACMIXUP = 1.0
+      if (ios == 0) then    ! mf:3490-3492, skipped by a read error
+        if (FACMIXUP < 1.0) FACMIXUP = 1.0
+        if (IF_RESTART /= 0) FACMIXUP = 1.0
+      end if
     end if
     CFWALL = 0.0
     if (YPLUSWALL > 5.0) CFWALL = 1/(YPLUSWALL*YPLUSWALL)
@@ -1192,6 +1195,64 @@ contains
   end subroutine stepup
 
   ! ---------------------------------------------------------------------------
+  ! The restart file in place of the initial guess, mf:6919-6999 (ITIMST < 5,
+  ! IFGAS = 0): the conserved variables and the SA variable from the
+  ! plotting/restart file 'flow_out' of the current directory (the record
+  ! layout OUTPUT writes, mf:5951-5976; its step count and ROSUB are not
+  ! used), the secondary variables from them, and, for IPOUT <= 0, the exit
+  ! pressure profile shifted to the file's pitch-averaged exit pressures.
+  logical function read_restart() result(bad)
+    integer :: u, ios, i, j, k, nstep_file
+    real :: eke, tstatic, pdiff
+    real, allocatable :: rosub(:,:,:), pdown(:)
+    bad = .false.
+    allocate (rosub(IM, JM, KM), pdown(KM))
+    open (newunit=u, file='flow_out', form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      bad = fail('IF_RESTART set but no restart file flow_out'); return
+    end if
+    read (u, iostat=ios) nstep_file
+    if (ios == 0) read (u, iostat=ios) RO
+    if (ios == 0) read (u, iostat=ios) ROVX
+    if (ios == 0) read (u, iostat=ios) ROVR
+    if (ios == 0) read (u, iostat=ios) ROVT
+    if (ios == 0) read (u, iostat=ios) ROE
+    if (ios == 0) read (u, iostat=ios) rosub
+    if (ios == 0) read (u, iostat=ios) TRANS_DYN_VIS
+    close (u)
+    if (ios /= 0) then
+      bad = fail('restart file flow_out: short or of another grid'); return
+    end if
+    do k = 1, KM
+      do j = 1, JM
+        do i = 1, IM
+          VX(i, j, k) = ROVX(i, j, k)/RO(i, j, k)
+          VR(i, j, k) = ROVR(i, j, k)/RO(i, j, k)
+          VT(i, j, k) = ROVT(i, j, k)/RO(i, j, k)
+          RORVT(i, j, k) = ROVT(i, j, k)*R(j, k)
+          eke = 0.5*(VX(i, j, k)*VX(i, j, k) + VR(i, j, k)*VR(i, j, k) + VT(i, j, k)*VT(i, j, k))
+          tstatic = (ROE(i, j, k)/RO(i, j, k) - eke)/CV
+          T_STATIC(i, j, k) = tstatic
+          HO(i, j, k) = CP*tstatic + eke
+          P(i, j, k) = RO(i, j, k)*RGAS*tstatic
+          PEFF(i, j, k) = P(i, j, k)
+        end do
+      end do
+      pdown(k) = 0.0
+      do i = 1, IMM1
+        pdown(k) = pdown(k) + FP(i)*0.5*(P(i, JM, k) + P(i + 1, JM, k))
+      end do
+    end do
+    if (IPOUT <= 0) then
+      pdiff = 0.0
+      if (IPOUT == 0) pdiff = PD(1) - pdown(1)
+      if (IPOUT == -1) pdiff = PD(KM) - pdown(KM)
+      do k = 1, KM
+        PD(k) = pdown(k) + pdiff
+      end do
+    end if
+  end function read_restart
+
   ! SETUP, mf:6280-7487 (up to the viscous-model initialisation)
   logical function setup() result(bad)
     integer :: i, j, k, nr, jledge, jtedge, kavg, jstt, mark, jp1, nsb, jsb, i1, j1, k1, i2, j2, k2
@@ -1538,6 +1599,12 @@ contains
         end do
       end do
     end do
+    allocate (TRANS_DYN_VIS(IM, JM, KM))
+    TRANS_DYN_VIS = 0.0
+    if (IF_RESTART /= 0) then
+      bad = read_restart()
+      if (bad) return
+    end if
     ! time steps, mf:7012-7058
     VSOUND = sqrt(RGAS*GA*TO1(KMID))
     allocate (STEP(IM, JM, KM), BSTEP(IM, JM, KM), RSTEP(IM, JM, KM))
@@ -1956,6 +2023,7 @@ contains
     if (allocated(STEP1)) deallocate (STEP1, STEP2, STEPSBK)
     if (allocated(XFORCE)) deallocate (XFORCE, RFORCE, TFORCE, QSOURCE, FLOWX, FLOWT, FLOWR)
     if (allocated(DWALLSQ)) deallocate (DWALLSQ, XLENGTH, FST_RAT, XLIMIT)
+    if (allocated(TRANS_DYN_VIS)) deallocate (TRANS_DYN_VIS)
     ! the mixing-plane parameters are read only for IFMIX /= 0 (mf:476-489)
     ! and the y+ damping range only for ILOS /= 0 (mf:3642-3649); otherwise
     ! they keep the zero of a fresh COMMON block, whatever a deck loaded

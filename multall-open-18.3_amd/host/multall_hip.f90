@@ -76,6 +76,7 @@ program multall_hip
   end do
   close (4)
   call write_flow_out()
+  call print_performance()
   rc = mh_destroy(ctx)
   write (*, *) ' STOPPED AFTER ', res%nstep, ' STEPS'
 
@@ -152,7 +153,7 @@ contains
     call puti('JLE', JLE(1:NROWS + 1)); call puti('JTE', JTE(1:NROWS + 1)); call puti('JLED', JLED(1:NROWS + 1))
     call puti('KTIPS', KTIPS(1:NROWS + 1)); call puti('KTIPE', KTIPE(1:NROWS + 1))
     call puti('KS1', KS1(1:NROWS + 1)); call puti('KS2', KS2(1:NROWS + 1))
-    call puti('NOUT', NOUT)
+    call puti('NOUT', NOUT); call puti('NSTAGE', NSTAGE(1:NROWS))
     call sets('CFL', dble(CFL)); call sets('DAMPIN', dble(DAMPIN)); call sets('MACHLIM', dble(MACHLIM))
     call sets('F_PDOWN', dble(F_PDOWN)); call sets('CONLIM', dble(CONLIM)); call sets('SFXIN', dble(SFXIN))
     call sets('SFTIN', dble(SFTIN)); call sets('FAC_4TH', dble(FAC_4TH)); call sets('F1', dble(F1))
@@ -186,6 +187,7 @@ contains
     ! Spalart-Allmaras inputs and the turbulence state SETUP starts from
     ! (mf:6903-6909)
     call put3('XLENGTH', XLENGTH); call put3('FST_RAT', FST_RAT); call put3('ROAVG_CELL', ROAVG_CELL)
+    call put3('TRANS_DYN_VIS', TRANS_DYN_VIS)
     call put1('FSTURB', FSTURB); call put1('TURBVIS_DAMP', TURBVIS_DAMP)
     allocate (buf(IM, JM, KM))
     buf = 25.0
@@ -205,7 +207,50 @@ contains
     call check(mh_get_array(ctx, mh_array_id(cname(name)), c_loc(a), int(size(a), c_long_long)), name)
   end subroutine get3
 
-  ! flow_out, mf:5951-5976 (ITIMST<5: ROE as marched, ROSUB unset = 0)
+  ! EFICOOL's stage and machine figures at the end of the run, as the
+  ! reference prints them (mf:11497-11548; FP64 on the device)
+  subroutine print_performance()
+    type(mh_perf) :: pf(NROWS + 1)
+    integer(c_int) :: nrec
+    integer :: m
+    call check(mh_eficool(ctx, pf, int(NROWS + 1, c_int), nrec), 'mh_eficool')
+    do m = 1, nrec
+      write (*, *)
+      if (pf(m)%stage > 0) then
+        write (*, *) ' RESULTS FOR STAGE NUMBER ', pf(m)%stage
+      else
+        write (*, *) ' THE FOLLOWING RESULTS ARE FOR THE WHOLE MACHINE'
+      end if
+      if (pf(m)%compressor == 0) then
+        write (*, *) ' ********** THIS IS A TURBINE ********** '
+        write (*, *) ' PRESSURE RATIO, POinlet/POexit            = ', 1.0d0/pf(m)%po_ratio
+      else
+        write (*, *) ' ***********THIS IS A COMPRESSOR ********** '
+        write (*, *) ' PRESSURE RATIO, POexit/POinlet            = ', pf(m)%po_ratio
+      end if
+      write (*, *) ' EFFICIENCIES BASED ON VALUES AT J = ', pf(m)%j1, pf(m)%j2m1
+      write (*, *) ' JMIX  = ', pf(m)%jmix
+      write (*, *) ' INLET AND EXIT STAGNATION PRESSURES   = ', pf(m)%po_in, pf(m)%po_out
+      write (*, *) ' INLET AND EXIT STATIC PRESSURES       = ', pf(m)%ps_in, pf(m)%ps_out
+      write (*, *) ' INLET AND EXIT STAGNATION TEMPERATURES= ', pf(m)%to_in, pf(m)%to_out
+      write (*, *) ' INLET AND OUTLET MASS FLOW RATES      = ', pf(m)%flow_in, pf(m)%flow_out
+      write (*, *) ' TOTAL TO TOTAL ISENTROPIC EFFICIENCY       = ', pf(m)%eta_tt
+      write (*, *) ' TOTAL TO STATIC ISENTROPIC EFFICIENCY      = ', pf(m)%eta_ts
+      write (*, *) ' TOTAL TO TOTAL POLYTROPIC EFFICIENCY       = ', pf(m)%eta_poly
+      if (pf(m)%stage > 0) then
+        write (*, *) ' AVERAGE STAGE REACTION BASED ON MASS AVERAGED TEMPERATURES = ', pf(m)%reaction
+        write (*, *) ' HUB REACTION BASED ON PRESSURE CHANGES     ', pf(m)%reaction_hub
+        write (*, *) ' MID SPAN REACTION BASED ON PRESSURE CHANGES', pf(m)%reaction_mid
+        write (*, *) ' TIP REACTION BASED ON PRESSURE CHANGES     ', pf(m)%reaction_tip
+        write (*, *) ' STAGE WORK NEGLECTING ANY BLEED OR COOLING FLOWS = ', pf(m)%work_kw, ' KILOWATTS.'
+      else
+        write (*, *) ' OVERALL POWER OUTPUT             = ', pf(m)%work_kw, ' KILOWATTS.'
+      end if
+    end do
+  end subroutine print_performance
+
+  ! flow_out, mf:5951-5976 (ITIMST<5: ROE as marched, ROSUB unset = 0; the
+  ! SA variable with ILOS >= 200, else the wall pressure gradient x 1e-6)
   subroutine write_flow_out()
     integer :: u
     call get3('RO', RO); call get3('ROVX', ROVX); call get3('ROVR', ROVR); call get3('ROVT', ROVT)
@@ -220,9 +265,16 @@ contains
     allocate (buf(IM, JM, KM))
     buf = 0.0
     write (u) buf
+    if (ILOS >= 200) then
+      call get3('TRANS_DYN_VIS', buf)
+    else
+      call get3('DPDS_CELL', buf)
+      buf = buf*1.0e-6
+    end if
     write (u) buf
     write (u) PEFF + PREFF
     close (u)
+    deallocate (buf)
   end subroutine write_flow_out
 
 end program multall_hip

@@ -18,7 +18,7 @@ HOST_LIB = os.path.join(PKG_ROOT, "lib", "libmultall_host.so")
 N3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "ROVT", "ROWT",
       "P", "HO", "PEFF", "VX", "VR", "VT", "WT", "T_STATIC", "AQX", "AQR", "ABX", "ABR", "ASX", "ASR",
       "VOL", "VOLOR", "STEP", "BSTEP", "RSTEP", "XFORCE", "RFORCE", "TFORCE", "QSOURCE", "FLOWX",
-      "FLOWT", "FLOWR", "DWALLSQ", "XLENGTH", "FST_RAT", "ROAVG_CELL"]
+      "FLOWT", "FLOWR", "DWALLSQ", "XLENGTH", "FST_RAT", "ROAVG_CELL", "TRANS_DYN_VIS"]
 JK = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID"]
 IK = ["FTL", "FBL", "FTR", "FBR"]
 VI = ["FP", "FU", "FD"]
@@ -26,7 +26,7 @@ VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCO
 VJ = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 RN = ["XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM", "XLLIM_DWN", "XLLIM_UP", "ROUGH_H", "ROUGH_T", "ROUGH_L",
       "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "CHORD"]
-RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM"]
+RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM", "NSTAGE"]
 MG = ["STEP1", "STEP2", "STEPSBK"]
 II = ["IB1", "IB2"]
 IKK = ["KB1", "KB2"]
@@ -142,6 +142,7 @@ def load_deck(path: str, xlength_on_host: bool = True) -> Dict:
     # models' first call (mh_loss_init) fills in the rest
     st["Y_PLUS"] = np.full((km, jm, im), 25.0, np.float32)
     st["VISC_RAT"] = np.ones((km, jm, im), np.float32)
-    for n in ("TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "VISC_LAM"):
+    # (TRANS_DYN_VIS is the host's: zero, or the restart file's)
+    for n in ("DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "VISC_LAM"):
         st[n] = np.zeros((km, jm, im), np.float32)
     return st

@@ -21,7 +21,7 @@ EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
            "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_set_xlength", "mh_start", "mh_step",
            "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch",
-           "mh_request_stop", "mh_set_xlength_noalloc"]
+           "mh_request_stop", "mh_set_xlength_noalloc", "mh_eficool"]
 
 
 class MhDims(ctypes.Structure):
@@ -36,6 +36,18 @@ class MhResid(ctypes.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class MhPerf(ctypes.Structure):
+    """One stage's (stage >= 1) or the whole machine's (stage 0) performance
+    record of mh_eficool."""
+    _fields_ = [(n, ctypes.c_int) for n in ("stage", "j1", "j2m1", "jmix", "compressor", "pad_")] + \
+               [(n, ctypes.c_double) for n in ("po_in", "po_out", "ps_in", "ps_out", "to_in", "to_out", "flow_in",
+                                               "flow_out", "po_ratio", "eta_tt", "eta_ts", "eta_poly", "reaction",
+                                               "reaction_hub", "reaction_mid", "reaction_tip", "work_kw")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad_"}
 
 
 class MhError(RuntimeError):
@@ -78,6 +90,7 @@ def load(path: str = LIB_PATH):
     L.mh_request_stop.argtypes = [vp]
     L.mh_step_batch.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(MhResid), ctypes.POINTER(i)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
+    L.mh_eficool.argtypes = [vp, ctypes.POINTER(MhPerf), i, ctypes.POINTER(i)]
     L.mh_sync.argtypes = [vp]
     L.mh_profile_enable.argtypes = [vp, i]
     L.mh_profile_report.argtypes = [vp, ctypes.c_char_p, i]
@@ -181,6 +194,16 @@ class Solver:
         self._check(self.L.mh_integrals(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 7 * jm),
                     "mh_integrals")
         return out.reshape(7, jm)
+
+    def eficool(self) -> list:
+        """Stage and machine performance of the current state (mh_eficool):
+        a list of dicts, stages first, then the whole machine (stage 0)
+        when there is more than one stage."""
+        cap = int(self.dims.nrows) + 1
+        out = (MhPerf * cap)()
+        n = ctypes.c_int(0)
+        self._check(self.L.mh_eficool(self.h, out, cap, ctypes.byref(n)), "mh_eficool")
+        return [out[i].as_dict() for i in range(n.value)]
 
     def profile(self, on: bool):
         self._check(self.L.mh_profile_enable(self.h, 1 if on else 0), "mh_profile_enable")

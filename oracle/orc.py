@@ -57,6 +57,11 @@ SCALARS_I = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE",
              "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IBOUND", "IRBB", "JRBB", "KRBB"]
 
 
+
+# arrays a host state may hold that LOOP never reads (EFICOOL's stage
+# numbering of the rows, mf:11271-11295): not bound
+NOT_READ_BY_LOOP = {"NSTAGE"}
+
 def with_wall_defaults(state: Dict) -> Dict:
     """The state with SETUP's initial wall-function arrays added where absent."""
     sc = state["scalars"]
@@ -159,7 +164,7 @@ class Oracle:
         self.a: Dict[str, np.ndarray] = {}
         state = with_wall_defaults(state)
         for k, v in state.items():
-            if k == "scalars":
+            if k == "scalars" or k in NOT_READ_BY_LOOP:
                 continue
             arr = np.ascontiguousarray(v).copy()
             self.a[k] = arr

@@ -9,6 +9,12 @@ SUMRVT, SUM_ENTPY, with the step the run stopped at.
 
   converged_<case>.json   {"deck": <deck file>, "fp64": {...}, "fp32": {...}}
 
+and, per build, the last EFICOOL block the reference printed (mf:11253-11548):
+per stage and for the whole machine, the stagnation and static pressures,
+stagnation temperatures and mass flows at the stage's inlet and exit planes,
+the pressure ratio, the total-to-total, total-to-static and polytropic
+efficiencies, the reactions and the work.
+
 Usage: python tests/golden/make_converged.py [case ...]
 """
 from __future__ import annotations
@@ -29,6 +35,70 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, HERE)
 
 from make_golden import read_global_plt  # noqa: E402
+
+_NUM = r"[-+]?(?:NaN|Infinity|\d+\.?\d*|\.\d+)(?:[EeDd][-+]?\d+)?"
+
+
+def eficool_records(stdout: str) -> list:
+    """The last EFICOOL performance block the reference printed (the end of
+    the run; IFCOOL = 0, mf:11253-11548): one record per stage, then the
+    whole machine when there is more than one stage, each with the numbers
+    the reference printed (list-directed output: a line that runs out of
+    width continues on the next, which then holds numbers only)."""
+    import re
+    lines = stdout.split("\n")
+    starts = [i for i, l in enumerate(lines) if "RESULTS FOR STAGE NUMBER" in l or
+              "THE FOLLOWING RESULTS ARE FOR THE WHOLE MACHINE" in l]
+    if not starts:
+        return []
+    first = max(i for i in starts if "RESULTS FOR STAGE NUMBER" in lines[i] and
+                int(re.findall(_NUM, lines[i])[-1]) == 1)
+    numeric = re.compile(r"^\s*(?:" + _NUM + r"\s*)+$")
+
+    def vals(i, k, after="="):
+        """The first k numbers after `after` on line i and its continuation lines."""
+        text = lines[i].split(after, 1)[1] if after else lines[i]
+        out = [float(x.replace("D", "E").replace("d", "e")) for x in re.findall(_NUM, text)]
+        j = i + 1
+        while len(out) < k and j < len(lines) and numeric.match(lines[j]):
+            out += [float(x.replace("D", "E")) for x in re.findall(_NUM, lines[j])]
+            j += 1
+        return out[:k]
+    recs = []
+    keys = [("PRESSURE RATIO, POinlet/POexit", ("po_ratio_printed",)),
+            ("PRESSURE RATIO, POexit/POinlet", ("po_ratio_printed",)),
+            ("EFFICIENCIES BASED ON VALUES AT J", ("j1", "j2m1")), ("JMIX", ("jmix",)),
+            ("INLET AND EXIT STAGNATION PRESSURES", ("po_in", "po_out")),
+            ("INLET AND EXIT STATIC PRESSURES", ("ps_in", "ps_out")),
+            ("INLET AND EXIT STAGNATION TEMPERATURES", ("to_in", "to_out")),
+            ("INLET AND OUTLET MASS FLOW RATES", ("flow_in", "flow_out")),
+            ("TOTAL TO TOTAL ISENTROPIC EFFICIENCY", ("eta_tt",)),
+            ("TOTAL TO STATIC ISENTROPIC EFFICIENCY", ("eta_ts",)),
+            ("TOTAL TO TOTAL POLYTROPIC EFFICIENCY", ("eta_poly",)),
+            ("AVERAGE STAGE REACTION BASED ON MASS AVERAGED", ("reaction",)),
+            ("STAGE WORK NEGLECTING", ("work_kw",)), ("OVERALL POWER OUTPUT", ("work_kw",))]
+    pressure_keys = [("HUB REACTION BASED ON PRESSURE CHANGES", "reaction_hub"),
+                     ("MID SPAN REACTION BASED ON PRESSURE CHANGES", "reaction_mid"),
+                     ("TIP REACTION BASED ON PRESSURE CHANGES", "reaction_tip")]
+    for s0 in [x for x in starts if x >= first]:
+        r = {"stage": 0 if "WHOLE MACHINE" in lines[s0] else int(re.findall(_NUM, lines[s0])[-1])}
+        for i in range(s0 + 1, min(s0 + 90, len(lines))):
+            l = lines[i]
+            if "RESULTS FOR STAGE NUMBER" in l or "FOLLOWING RESULTS ARE FOR THE WHOLE MACHINE" in l:
+                break
+            for label, names in keys:
+                if l.strip().startswith(label) and names[0] not in r:
+                    if label.startswith("PRESSURE RATIO"):
+                        r["turbine"] = "POinlet/POexit" in label
+                    v = vals(i, len(names))
+                    for n, x in zip(names, v):
+                        r[n] = int(x) if n in ("j1", "j2m1", "jmix") else x
+                    break
+            for label, name in pressure_keys:
+                if l.strip().startswith(label):
+                    r[name] = vals(i, 1, after=label)[0]
+        recs.append(r)
+    return recs
 
 R37 = "/root/reference/Multall-open-18.3/multall-test-cases/r37+samods-17.5.dat"
 
@@ -94,11 +164,12 @@ def make(case):
         wd = tempfile.mkdtemp(prefix="mhconv_")
         t0 = time.time()
         try:
-            _, final, _ = refdump.run_reference(txt, variant, workdir=wd, timeout=7200)
+            _, final, stdout = refdump.run_reference(txt, variant, workdir=wd, timeout=7200)
             integ = read_global_plt(os.path.join(wd, "global.plt"), real)
         finally:
             shutil.rmtree(wd, ignore_errors=True)
-        out[variant.rstrip("l")] = {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist()}
+        out[variant.rstrip("l")] = {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist(),
+                                    "eficool": eficool_records(stdout)}
         print(f"{case} {variant}: stopped at step {out[variant.rstrip('l')]['nstep']} ({time.time() - t0:.0f} s)", flush=True)
     with open(os.path.join(HERE, f"converged_{case}.json"), "w") as f:
         json.dump(out, f)

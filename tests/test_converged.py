@@ -100,4 +100,25 @@ def test_converged_vs_fp64_reference(case, mixed):
           f"reference FP64 at {d['fp64']['nstep']}, FP32 at {d['fp32']['nstep']}")
     q = quantities(g.integrals()[:5], ga)
     q64 = quantities(d["fp64"]["integrals"], ga)
-    _check(q, q64, "GPU (mixed form)" if mixed else "GPU (fast mode)")
+    label = "GPU (mixed form)" if mixed else "GPU (fast mode)"
+    _check(q, q64, label)
+    # EFICOOL's stage (and machine) figures, mh_eficool
+    perf = g.eficool()
+    if d["fp64"].get("eficool"):
+        from test_gpu_eficool import compare
+        compare(perf, d["fp64"]["eficool"], d["fp32"]["eficool"], BOUND_REL, BOUND_ETA, f"{case} {label}")
+    else:
+        # no printout kept: each record's planes in the FP64 reference's integrals
+        f64 = np.asarray(d["fp64"]["integrals"], np.float64)
+        for rec in perf:
+            j1, j2 = rec["j1"] - 1, rec["j2m1"] - 1
+            pr = (f64[1][j2] / f64[0][j2]) / (f64[1][j1] / f64[0][j1])
+            tr = (f64[2][j2] / f64[0][j2]) / (f64[2][j1] / f64[0][j1])
+            x = (ga - 1.0) / ga
+            eta = (pr ** x - 1.0) / (tr - 1.0) if tr > 1.0 else (tr - 1.0) / (pr ** x - 1.0)
+            for name, a, b in (("flow_in", rec["flow_in"], f64[0][j1]), ("flow_out", rec["flow_out"], f64[0][j2]),
+                               ("po_ratio", rec["po_ratio"], pr), ("to ratio", rec["to_out"] / rec["to_in"], tr)):
+                print(f"{case} {label} stage {rec['stage']} {name}: {a:.7g} vs FP64 {b:.7g}")
+                assert abs(a - b) <= BOUND_REL * abs(b), (rec["stage"], name, a, b)
+            assert abs(rec["eta_tt"] - eta) <= BOUND_ETA, (rec["stage"], rec["eta_tt"], eta)
+        assert [r["stage"] for r in perf] == list(range(1, len(perf))) + [0]
