@@ -129,6 +129,15 @@ enum mh_array_id {
    * -- RN ints; unset means the reference's default 1 + (N-1)/2 (mf:704-706).
    * Used only by mh_eficool. */
   MH_NSTAGE,
+  /* the trailing-edge separation force of rows with IF_CUSP = 2 (BKSTAG and
+   * the reader's per-row inputs, mf:993-1004; applied each step,
+   * mf:5184-5233): the per-row option, the points upstream of the trailing
+   * edge where it starts on the I = 1 and I = IM surfaces, the wake points
+   * it extends over -- RN ints; the layer thickness as a fraction of the
+   * blade thickness and the momentum factor -- RN floats; the blade
+   * tangential thickness RT_THICK -- JK; the node pitchwise positions
+   * R*THETA (RTHETA) -- N.  Needed only when some row has IF_CUSP = 2. */
+  MH_IF_CUSP, MH_NUP_I1, MH_NUP_IM, MH_N_WAKE, MH_SEP_THIK, MH_SEP_DRAG, MH_RT_THICK, MH_RTHETA,
   MH_NARRAYS
 };
 

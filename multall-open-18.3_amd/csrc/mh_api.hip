@@ -64,6 +64,7 @@ __global__ void k_rolim_check(Geo, Dev, StepScalars);
 __global__ void k_rolim_fix(Geo, Dev, StepScalars, Params);
 __global__ void k_mach_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_mixplan(Geo, Dev, Params, float *);
+__global__ void k_te_sep(Geo, Dev);
 __global__ void k_vrms_part(Geo, Dev, double *);
 __global__ void k_vrms_final(Geo, Dev, const double *, int, int);
 __global__ void k_checkflow(Geo, Dev, int);
@@ -159,6 +160,8 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"DPDS_CELL", K_N, false}, {"YPLUS_I1", K_JK, false}, {"YPLUS_IM", K_JK, false}, {"YPLUS_K1", K_IJ, false},
     {"YPLUS_KM", K_IJ, false}, {"CHORD", K_RN, false}, {"NOUT", K_NOUT, true},
     {"NSTAGE", K_RN, true},
+    {"IF_CUSP", K_RN, true}, {"NUP_I1", K_RN, true}, {"NUP_IM", K_RN, true}, {"N_WAKE", K_RN, true},
+    {"SEP_THIK", K_RN, false}, {"SEP_DRAG", K_RN, false}, {"RT_THICK", K_JK, false}, {"RTHETA", K_N, false},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -277,7 +280,8 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.ROAVG_CELL, (void **)&d.VISC_LAM, (void **)&d.VISC_RAT, (void **)&d.XLENGTH, (void **)&d.FST_RAT,
       (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP, (void **)&d.DPDS_CELL, (void **)&d.YPLUS_I1,
       (void **)&d.YPLUS_IM, (void **)&d.YPLUS_K1, (void **)&d.YPLUS_KM, (void **)&d.CHORD, (void **)&d.NOUT,
-      (void **)&d.NSTAGE,
+      (void **)&d.NSTAGE, (void **)&d.IF_CUSP, (void **)&d.NUP_I1, (void **)&d.NUP_IM, (void **)&d.N_WAKE,
+      (void **)&d.SEP_THIK, (void **)&d.SEP_DRAG, (void **)&d.RT_THICK, (void **)&d.RTHETA,
   };
   return base[id];
 }
@@ -1220,6 +1224,10 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LAUNCH(k_mixplan, dim3(nl), dim3(64), 9 * g.IM * sizeof(float), c->stream, g, d, p, c->lists.mixscratch);
     LCHK("mixplan");
   }
+  if (c->te_sep) {
+    LAUNCH(k_te_sep, dim3(g.NROWS * g.KM), dim3(256), 0, c->stream, g, d);
+    LCHK("te_sep");
+  }
   if (q.check5 || q.cond) {
     LAUNCH(k_checkflow, dim3(g.JM), dim3(64), 0, c->stream, g, dc, p.EXACT);
     LAUNCH(k_econt, dim3(1), dim3(256), 0, c->stream, g, dc);
@@ -1630,12 +1638,37 @@ int mh_set_xlength_noalloc(mh_ctx *c, const float *theta, const float *xllim_in,
   return xlength_run(c, th, dist, xl4, xlim, theta, xllim_in, xllim_le, xllim_te, xllim_dn);
 }
 
+// Rows with IF_CUSP = 2: their separation force runs each step; every node
+// and J it reads must lie in the grid (I = 1..6 and IM-6..IM; J from one
+// before the start points to N_WAKE past the trailing edge).
+static int check_te_sep(mh_ctx *c) {
+  const Geo &g = c->g;
+  const int NR = g.NROWS;
+  std::vector<int> ifc(NR), nu1(NR), num(NR), nwk(NR), jte(NR + 1);
+  int r;
+  if ((r = d2h(c, ifc.data(), c->d.IF_CUSP, NR * 4)) || (r = d2h(c, nu1.data(), c->d.NUP_I1, NR * 4)) ||
+      (r = d2h(c, num.data(), c->d.NUP_IM, NR * 4)) || (r = d2h(c, nwk.data(), c->d.N_WAKE, NR * 4)) ||
+      (r = d2h(c, jte.data(), c->d.JTE, (NR + 1) * 4)))
+    return r;
+  c->te_sep = false;
+  for (int n = 0; n < NR; n++) {
+    if (ifc[n] != 2) continue;
+    const int j1 = jte[n] - nu1[n], j2 = jte[n] - num[n], jend = jte[n] + nwk[n];
+    if (g.IM < 7 || std::min(j1, j2) < 2 || std::max(j1, j2) + 1 > g.JM || jend > g.JM || nwk[n] < 0)
+      return fail(c, MH_E_ARG, "IF_CUSP = 2 row " + std::to_string(n + 1) +
+                                   ": NUP_I1, NUP_IM or N_WAKE reach outside the grid");
+    c->te_sep = true;
+  }
+  return MH_OK;
+}
+
 int mh_start(mh_ctx *c) {
   if (!c) return MH_E_ARG;
   int r = check_inputs(c);
   if (r) return r;
   const Params p = params_of(c);
   if ((r = build_lists(c))) return r;
+  if ((r = check_te_sep(c))) return r;
   if (p.ILOS != 0) {
     if (p.ILOS == 10 && (r = loss_tables(c))) return r;
     if (p.ILOS >= 100 && (r = sa_maps(c))) return r;

@@ -60,6 +60,8 @@ struct Dev {
   float *DPDS_CELL, *YPLUS_I1, *YPLUS_IM, *YPLUS_K1, *YPLUS_KM, *CHORD;
   int *NOUT;
   int *NSTAGE;
+  int *IF_CUSP, *NUP_I1, *NUP_IM, *N_WAKE;
+  float *SEP_THIK, *SEP_DRAG, *RT_THICK, *RTHETA;
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
@@ -251,6 +253,7 @@ struct mh_ctx {
   bool stop_req = false;  // mh_request_stop: end at the next step divisible by 10
   int device = 0;             // the HIP device the context was created on
   int scat_tk = 8, smooth_nw = 8;  // tile shapes in use (autotune at mh_start)
+  bool te_sep = false;             // some row has IF_CUSP = 2 (the separation force runs)
   int *eff_tab = nullptr;        // mh_eficool's plane table and records (device)
   double *eff_out = nullptr;
   float *xl_scratch = nullptr;  // that device's address of mh_set_xlength_noalloc's scratch

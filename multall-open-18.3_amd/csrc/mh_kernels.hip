@@ -3592,6 +3592,55 @@ __global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
 // NEW_MIXPLAN, mf:14922-15251 (ideal gas, ITIMST<5): one thread per
 // (mixing plane, K) line; the supersonic Prandtl-Meyer sweeps are serial in I.
 
+// The trailing-edge separation force of rows with IF_CUSP = 2
+// (mf:5184-5233), after NEW_MIXPLAN: one workgroup per (row, K).  From
+// NUP_I1 points upstream of the trailing edge to N_WAKE points downstream,
+// the nodes I = 1..6 that lie more than SEP_THIK x the blade thickness (at
+// the later of the two start points) below the I = 1 surface extrapolated
+// from its start point get their meridional momentum and their angular
+// momentum relative to the blade scaled by SEP_DRAG; then the nodes
+// I = IM-6..IM above the extrapolated I = IM surface, from NUP_IM points
+// upstream.  The two passes run in the reference's order (a node in both is
+// scaled twice).  The host has checked that every index stays in the grid.
+__global__ void __launch_bounds__(256) k_te_sep(Geo g, Dev d) {
+  if (gated(d)) return;
+  const int NR = blockIdx.x / g.KM + 1, K = blockIdx.x % g.KM + 1;
+  if (d.IF_CUSP[NR - 1] != 2) return;
+  const int IM = g.IM, JTEDGE = d.JTE[NR - 1], NWKE = d.N_WAKE[NR - 1];
+  const int JSEP_I1 = JTEDGE - d.NUP_I1[NR - 1], JSEP_IM = JTEDGE - d.NUP_IM[NR - 1];
+  const int JTHIK = max(JSEP_I1, JSEP_IM);
+  const float FDRAG = d.SEP_DRAG[NR - 1];
+  const float THIK_LIM = d.SEP_THIK[NR - 1] * d.RT_THICK[JKX(JTHIK, K)];
+  const float *RT = d.RTHETA, *X = d.X;
+  const float SSSLOPE = (RT[IX(1, JSEP_I1 + 1, K)] - RT[IX(1, JSEP_I1 - 1, K)]) /
+                        (X[JKX(JSEP_I1 + 1, K)] - X[JKX(JSEP_I1 - 1, K)]);
+  const float PSSLOPE = (RT[IX(IM, JSEP_IM + 1, K)] - RT[IX(IM, JSEP_IM - 1, K)]) /
+                        (X[JKX(JSEP_IM + 1, K)] - X[JKX(JSEP_IM - 1, K)]);
+  auto drag = [&](size_t n, int J) {
+    d.ROVX[n] = FDRAG * d.ROVX[n];
+    d.ROVR[n] = FDRAG * d.ROVR[n];
+    const float RO_VBLADE = d.RO[n] * d.UBLADE[JKX(J, K)];
+    const float RO_WT = d.ROVT[n] - RO_VBLADE;
+    const float RO_VT = FDRAG * RO_WT + RO_VBLADE;
+    d.RORVT[n] = RO_VT * d.R[JKX(J, K)];
+  };
+  const int n1 = (JTEDGE + NWKE - JSEP_I1 + 1) * 6;
+  for (int t = threadIdx.x; t < n1; t += blockDim.x) {
+    const int J = JSEP_I1 + t / 6, I = t % 6 + 1;
+    const float RTEXTRAP = RT[IX(1, JSEP_I1, K)] + SSSLOPE * (X[JKX(J, K)] - X[JKX(JSEP_I1, K)]);
+    const size_t n = IX(I, J, K);
+    if ((RTEXTRAP - RT[n]) > THIK_LIM) drag(n, J);
+  }
+  __syncthreads();
+  const int n2 = (JTEDGE + NWKE - JSEP_IM + 1) * 7;
+  for (int t = threadIdx.x; t < n2; t += blockDim.x) {
+    const int J = JSEP_IM + t / 7, I = IM - 6 + t % 7;
+    const float RTEXTRAP = RT[IX(IM, JSEP_IM, K)] + PSSLOPE * (X[JKX(J, K)] - X[JKX(JSEP_IM, K)]);
+    const size_t n = IX(I, J, K);
+    if ((RT[n] - RTEXTRAP) > THIK_LIM) drag(n, J);
+  }
+}
+
 __global__ void k_mixplan(Geo g, Dev d, Params p, float *scratch) {
   if (gated(d)) return;
   extern __shared__ float ml[];  // 9 * IM floats: per-I work arrays of this line

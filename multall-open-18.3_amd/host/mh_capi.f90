@@ -239,7 +239,6 @@ contains
     case ('DS'); call putr2(DS)
     case ('SMERID'); call putr2(SMERID)
     case ('RT_UPP'); call putr2(RT_UPP)
-    case ('RT_THICK'); call putr2(RT_THICK)
     case ('FTL'); call putr2(FTL)
     case ('FBL'); call putr2(FBL)
     case ('FTR'); call putr2(FTR)
@@ -290,6 +289,13 @@ contains
     case ('KS2'); call puti(KS2(1:NROWS + 1))
     case ('NOUT'); call puti(NOUT)
     case ('NSTAGE'); call puti(NSTAGE(1:NROWS))
+    case ('IF_CUSP'); call puti(IF_CUSP(1:NROWS))
+    case ('NUP_I1'); call puti(NUP_I1(1:NROWS))
+    case ('NUP_IM'); call puti(NUP_IM(1:NROWS))
+    case ('N_WAKE'); call puti(N_WAKE(1:NROWS))
+    case ('SEP_THIK'); call putr1(SEP_THIK(1:NROWS))
+    case ('SEP_DRAG'); call putr1(SEP_DRAG(1:NROWS))
+    case ('RT_THICK'); call putr2(RT_THICK(1:JM, 1:KM))
     end select
   contains
     subroutine putr1(a)

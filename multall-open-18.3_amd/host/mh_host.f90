@@ -49,6 +49,9 @@ module mh_host
   integer, allocatable, save, public :: JTRAN_I1(:), JTRAN_IM(:), JTRAN_K1(:), JTRAN_KM(:)
   real, allocatable, save, public :: FRACTIP1(:), FRACTIP2(:), FTHICK(:,:), CHORD(:)
   integer, allocatable, save, public :: ICUSP(:), LCUSP(:), LCUSPUP(:)
+  ! the trailing-edge separation force of IF_CUSP = 2 (mf:993-1004)
+  integer, allocatable, save, public :: NUP_I1(:), NUP_IM(:), N_WAKE(:)
+  real, allocatable, save, public :: SEP_THIK(:), SEP_DRAG(:)
   real, allocatable, save, public :: BETAUP(:,:), BETADWN1(:,:), BETADWN2(:,:)
   real, allocatable, save, public :: XLLIM_I1(:), XLLIM_IM(:), XLLIM_K1(:), XLLIM_KM(:), XLLIM_DWN(:)
   real, allocatable, save, public :: XLLIM_UP(:), XLLIM_IN(:), XLLIM_LE(:), XLLIM_TE(:), XLLIM_DN(:)
@@ -309,6 +312,8 @@ contains
     allocate (IF_ANGLES(NROWS), FRACTIP1(NROWS), FRACTIP2(NROWS), FTHICK(NROWS, KM), CHORD(NROWS))
     allocate (JTRAN_I1(NROWS), JTRAN_IM(NROWS), JTRAN_K1(NROWS), JTRAN_KM(NROWS))
     allocate (ICUSP(NROWS), LCUSP(NROWS), LCUSPUP(NROWS))
+    allocate (NUP_I1(NROWS), NUP_IM(NROWS), N_WAKE(NROWS), SEP_THIK(NROWS), SEP_DRAG(NROWS))
+    NUP_I1 = 0; NUP_IM = 0; N_WAKE = 0; SEP_THIK = 0.0; SEP_DRAG = 0.0
     allocate (BETAUP(NROWS, max(NSECS_IN, 2)), BETADWN1(NROWS, max(NSECS_IN, 2)), BETADWN2(NROWS, max(NSECS_IN, 2)))
     ICUSP = 0; LCUSP = 0; LCUSPUP = 0; BETAUP = 0.0; BETADWN1 = 0.0; BETADWN2 = 0.0
     allocate (t_ind(JCAP), t_indle(JCAP), t_indte(JCAP), t_indmix(JCAP), t_indmid(JCAP), t_nblade(JCAP))
@@ -374,11 +379,19 @@ contains
       IF_CUSP(nr) = 0; IF_ANGLES(nr) = 0
       call skip(u)
       read (u, *, iostat=ios) IF_CUSP(nr), IF_ANGLES(nr)
-      if (IF_CUSP(nr) < 0 .or. IF_CUSP(nr) > 1 .or. IF_ANGLES(nr) < 0) then
-        bad = fail('IF_CUSP must be 0 or 1 (the IF_CUSP=2 trailing-edge body force is not supported)'); return
+      if (IF_CUSP(nr) < 0 .or. IF_CUSP(nr) > 2 .or. IF_ANGLES(nr) < 0) then
+        bad = fail('IF_CUSP must be 0, 1 or 2'); return
       end if
       ! cusp shape, mf:980-984
       if (IF_CUSP(nr) == 1) read (u, *) ICUSP(nr), LCUSP(nr), LCUSPUP(nr)
+      ! the trailing-edge separation force, mf:993-1004
+      if (IF_CUSP(nr) == 2) then
+        if (ISHIFT >= 2) then
+          ! the re-gridded gap would form a new cusp of zero length there
+          bad = fail('IF_CUSP = 2 with ISHIFT >= 2 (gap re-gridding) is not supported'); return
+        end if
+        read (u, *) NUP_I1(nr), NUP_IM(nr), N_WAKE(nr), SEP_THIK(nr), SEP_DRAG(nr)
+      end if
       j2 = j1 + jmrow - 1
       do k = 1, nsecs_row
         if_design = 0; if_stagger = 0; if_lean = 0
@@ -2001,6 +2014,7 @@ contains
     if (allocated(JSTART)) deallocate (JSTART, JMIX, JLE, JTE, JLED, KTIPS, KTIPE, KS1, KS2, IF_CUSP, IF_ANGLES, &
                                        FRACTIP1, FRACTIP2, FTHICK, CHORD, JTRAN_I1, JTRAN_IM, JTRAN_K1, JTRAN_KM, &
                                        ICUSP, LCUSP, LCUSPUP, BETAUP, BETADWN1, BETADWN2)
+    if (allocated(NUP_I1)) deallocate (NUP_I1, NUP_IM, N_WAKE, SEP_THIK, SEP_DRAG)
     if (allocated(XSURF)) deallocate (XSURF, RSURF)
     if (allocated(RT_UPP)) deallocate (RT_UPP, RT_THICK)
     if (allocated(IND)) deallocate (IND, INDLE, INDTE, INDMIX, INDMID, NBLADE, NROW, WRAD, WHUB, WTIP, PGUESS)

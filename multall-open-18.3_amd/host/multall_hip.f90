@@ -154,6 +154,10 @@ contains
     call puti('KTIPS', KTIPS(1:NROWS + 1)); call puti('KTIPE', KTIPE(1:NROWS + 1))
     call puti('KS1', KS1(1:NROWS + 1)); call puti('KS2', KS2(1:NROWS + 1))
     call puti('NOUT', NOUT); call puti('NSTAGE', NSTAGE(1:NROWS))
+    ! the trailing-edge separation force of IF_CUSP = 2 rows
+    call puti('IF_CUSP', IF_CUSP(1:NROWS)); call puti('NUP_I1', NUP_I1); call puti('NUP_IM', NUP_IM)
+    call puti('N_WAKE', N_WAKE); call put1('SEP_THIK', SEP_THIK); call put1('SEP_DRAG', SEP_DRAG)
+    call put2('RT_THICK', RT_THICK(1:JM, 1:KM)); call put3('RTHETA', RTHETA)
     call sets('CFL', dble(CFL)); call sets('DAMPIN', dble(DAMPIN)); call sets('MACHLIM', dble(MACHLIM))
     call sets('F_PDOWN', dble(F_PDOWN)); call sets('CONLIM', dble(CONLIM)); call sets('SFXIN', dble(SFXIN))
     call sets('SFTIN', dble(SFTIN)); call sets('FAC_4TH', dble(FAC_4TH)); call sets('F1', dble(F1))

@@ -46,6 +46,8 @@ class RowSpec:
     jtran: tuple = (0, 0, 0, 0)    # transition J from the row start: I=1, I=IM, hub, casing surfaces
     pguess: tuple = (1.0e5, 1.0e5, 0.8e5, 0.8e5)   # PUP, PLE, PTE, PDN
     cusp: Optional[tuple] = None   # (ICUSP, LCUSP, LCUSPUP): IF_CUSP = 1, a new trailing-edge cusp
+    # (NUP_I1, NUP_IM, N_WAKE, SEP_THIK, SEP_DRAG): IF_CUSP = 2, the trailing-edge separation force
+    sep_force: Optional[tuple] = None
     # (span fractions, upstream, downstream-start, downstream-end grid angles in degrees): IF_ANGLES = 1
     angles: Optional[tuple] = None
 
@@ -309,9 +311,13 @@ def write_deck(spec: DeckSpec) -> str:
         a("      NSECS_ROW   INSURF")
         a(f"{spec.nsecs:10d}         0")
         a("  IF_CUSP   IFANGLES")
-        a(f"{0 if row.cusp is None else 1:10d}{0 if row.angles is None else 1:10d}")
-        if row.cusp is not None:
+        ifcusp = 2 if row.sep_force is not None else (0 if row.cusp is None else 1)
+        a(f"{ifcusp:10d}{0 if row.angles is None else 1:10d}")
+        if row.cusp is not None and row.sep_force is None:
             a("".join(f"{v:10d}" for v in row.cusp))
+        if row.sep_force is not None:
+            n1, nm, nw, thik, drag = row.sep_force
+            a(f"{n1:10d}{nm:10d}{nw:10d}{thik:10.4f}{drag:10.4f}")
         for ks in range(spec.nsecs):
             fr = ks / (spec.nsecs - 1)
             r = spec.r_hub + fr * span
@@ -635,6 +641,19 @@ def deck_tiny_supersonic(nsteps: int = 100) -> DeckSpec:
     s2.pguess = (0.35e5, 0.35e5, 0.34e5, 0.34e5)
     spec.pdown_hub = spec.pdown_tip = 0.34e5
     spec.title = "TINY CASCADE INVISCID SUPERSONIC MIXING PLANES"
+    return spec
+
+
+def deck_tiny_cusp2(nsteps: int = 100) -> DeckSpec:
+    """The three-row mixing-length cascade with the trailing-edge separation
+    force (IF_CUSP = 2, multall-open-18.3.f:5184-5233) on both stators: from
+    4 (I = 1 side) and 3 (I = IM side) points upstream of the trailing edge
+    to 3 points into the wake, a layer of 0.05 of the blade thickness,
+    momentum factor 0.97."""
+    spec = deck_tiny(ilos=10, nrows=3, nsteps=nsteps)
+    spec.rows[0].sep_force = (4, 3, 3, 0.05, 0.97)
+    spec.rows[2].sep_force = (3, 4, 2, 0.05, 0.97)
+    spec.title = "TINY CASCADE ILOS=10 IF_CUSP=2 ON THE STATORS"
     return spec
 
 

@@ -18,15 +18,15 @@ HOST_LIB = os.path.join(PKG_ROOT, "lib", "libmultall_host.so")
 N3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "DROE", "ROVT", "ROWT",
       "P", "HO", "PEFF", "VX", "VR", "VT", "WT", "T_STATIC", "AQX", "AQR", "ABX", "ABR", "ASX", "ASR",
       "VOL", "VOLOR", "STEP", "BSTEP", "RSTEP", "XFORCE", "RFORCE", "TFORCE", "QSOURCE", "FLOWX",
-      "FLOWT", "FLOWR", "DWALLSQ", "XLENGTH", "FST_RAT", "ROAVG_CELL", "TRANS_DYN_VIS"]
-JK = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID"]
+      "FLOWT", "FLOWR", "DWALLSQ", "XLENGTH", "FST_RAT", "ROAVG_CELL", "TRANS_DYN_VIS", "RTHETA"]
+JK = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID", "RT_THICK"]
 IK = ["FTL", "FBL", "FTR", "FBR"]
 VI = ["FP", "FU", "FD"]
 VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD", "VM1"]
 VJ = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 RN = ["XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM", "XLLIM_DWN", "XLLIM_UP", "ROUGH_H", "ROUGH_T", "ROUGH_L",
-      "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "CHORD"]
-RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM", "NSTAGE"]
+      "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "CHORD", "SEP_THIK", "SEP_DRAG"]
+RNI = ["JTRAN_I1", "JTRAN_IM", "JTRAN_K1", "JTRAN_KM", "NSTAGE", "IF_CUSP", "NUP_I1", "NUP_IM", "N_WAKE"]
 MG = ["STEP1", "STEP2", "STEPSBK"]
 II = ["IB1", "IB2"]
 IKK = ["KB1", "KB2"]

@@ -32,8 +32,10 @@
   X(XFORCE) X(RFORCE) X(TFORCE) X(QSOURCE) X(FLOWX) X(FLOWT) X(FLOWR) X(XFLUX) X(TFLUX)  \
   X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN) X(DWALLSQ) X(Y_PLUS) X(TRANS_DYN_VIS) X(DEL_DYNVIS)      \
   X(TRANS_KVIS) X(T_SOURCE) X(ROAVG_CELL) X(VISC_LAM) X(VISC_RAT) X(VISC_TURB) X(XLENGTH) X(FST_RAT)      \
-  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT) X(DPDS_CELL)
-#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID) X(YPLUS_I1) X(YPLUS_IM)
+  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT) X(DPDS_CELL) \
+  X(RTHETA)
+#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID) X(YPLUS_I1) X(YPLUS_IM) \
+  X(RT_THICK)
 #define FIJ(X) X(YPLUS_K1) X(YPLUS_KM)
 #define FIK(X) X(FTL) X(FBL) X(FTR) X(FBR)
 #define F1D(X) X(FP) X(FU) X(FD) X(FR) X(FKU) X(FKD) X(PO1) X(TO1) X(RPO1) X(BSSIN) X(BSCOS) \
@@ -41,10 +43,11 @@
   X(WTIP) X(VLIMIT) X(ROLIMIT) X(AVG_CHG) X(AVG_BLK) X(STEP1) X(STEP2) X(STEPSBK)           \
   X(XLLIM_I1) X(XLLIM_IM) X(XLLIM_K1) X(XLLIM_KM) X(XLLIM_DWN) X(XLLIM_UP) X(CHORD)         \
   X(ROUGH_H) X(ROUGH_T) X(ROUGH_L) X(ROUGH_U) X(FSTURB) X(TURBVIS_DAMP) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
-  X(REYNOLDS) X(ROVEXIT)
+  X(REYNOLDS) X(ROVEXIT) X(SEP_THIK) X(SEP_DRAG)
 #define I1D(X) X(IB1) X(IB2) X(KB1) X(KB2) X(NBLADE) X(IND) X(INDLE) X(INDMIX) X(INDTE)   \
   X(NROW) X(JSBLK) X(JB1) X(JB2) X(JSTART) X(JMIX) X(JLE) X(JTE) X(JLED) X(KTIPS) X(KTIPE) \
-  X(KS1) X(KS2) X(NOUT) X(JTRAN_I1) X(JTRAN_IM) X(JTRAN_K1) X(JTRAN_KM)
+  X(KS1) X(KS2) X(NOUT) X(JTRAN_I1) X(JTRAN_IM) X(JTRAN_K1) X(JTRAN_KM) X(IF_CUSP) X(NUP_I1) X(NUP_IM) \
+  X(N_WAKE)
 
 #define SCAL(X) X(CFL) X(DAMP) X(DAMPIN) X(MACHLIM) X(F_PDOWN) X(CONLIM) X(SFXIN) X(SFTIN)    \
   X(SFX) X(SFT) X(FAC_4TH) X(F1) X(F2) X(F3) X(RSMTH) X(SFEXIT) X(FSMTHB) X(RFIN) X(RFIN1)    \
@@ -2687,6 +2690,40 @@ static void step_once(orc_t *o) {
       }
   }
   if (o->IFMIX > 0 && o->RFMIX > 1.0e-3f) orc_new_mixplan(o);
+
+  /* the trailing-edge separation force of rows with IF_CUSP = 2, mf:5184-5233 */
+  for (int NR = 1; o->IF_CUSP && NR <= o->NROWS; NR++) {
+    if (A1(IF_CUSP, NR) != 2) continue;
+    const int JTEDGE = A1(JTE, NR), JSEP_I1 = JTEDGE - A1(NUP_I1, NR), JSEP_IM = JTEDGE - A1(NUP_IM, NR);
+    const int JTHIK = JSEP_I1 > JSEP_IM ? JSEP_I1 : JSEP_IM, NWKE = A1(N_WAKE, NR);
+    const float FDRAG = A1(SEP_DRAG, NR);
+    for (int K = 1; K <= KM; K++) {
+      const float THIK_LIM = A1(SEP_THIK, NR) * AJK(RT_THICK, JTHIK, K);
+      const float SSSLOPE = (A3(RTHETA, 1, JSEP_I1 + 1, K) - A3(RTHETA, 1, JSEP_I1 - 1, K)) /
+                            (AJK(X, JSEP_I1 + 1, K) - AJK(X, JSEP_I1 - 1, K));
+      const float PSSLOPE = (A3(RTHETA, IM, JSEP_IM + 1, K) - A3(RTHETA, IM, JSEP_IM - 1, K)) /
+                            (AJK(X, JSEP_IM + 1, K) - AJK(X, JSEP_IM - 1, K));
+      for (int side = 0; side < 2; side++) {
+        const int JS = side == 0 ? JSEP_I1 : JSEP_IM, IB = side == 0 ? 1 : IM;
+        const int I1 = side == 0 ? 1 : IM - 6, I2 = side == 0 ? 6 : IM;
+        const float SLOPE = side == 0 ? SSSLOPE : PSSLOPE;
+        for (int J = JS; J <= JTEDGE + NWKE; J++) {
+          const float RTEXTRAP = A3(RTHETA, IB, JS, K) + SLOPE * (AJK(X, J, K) - AJK(X, JS, K));
+          for (int I = I1; I <= I2; I++) {
+            const float d = side == 0 ? RTEXTRAP - A3(RTHETA, I, J, K) : A3(RTHETA, I, J, K) - RTEXTRAP;
+            if (d > THIK_LIM) {
+              A3(ROVX, I, J, K) = FDRAG * A3(ROVX, I, J, K);
+              A3(ROVR, I, J, K) = FDRAG * A3(ROVR, I, J, K);
+              const float RO_VBLADE = A3(RO, I, J, K) * AJK(UBLADE, J, K);
+              const float RO_WT = A3(ROVT, I, J, K) - RO_VBLADE;
+              const float RO_VT = FDRAG * RO_WT + RO_VBLADE;
+              A3(RORVT, I, J, K) = RO_VT * AJK(R, J, K);
+            }
+          }
+        }
+      }
+    }
+  }
 
   /* every 5 steps: convergence checks and the time-step update, mf:5252-5349 */
   if (o->NSTEP % 5 == 0 || o->IFEND == 1) {

@@ -24,8 +24,9 @@ NODE3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "ROVT", "ROWT", "P", "HO", "PEFF"
          "QSOURCE", "FLOWX", "FLOWT", "FLOWR", "XFLUX", "TFLUX", "RFLUX", "SOURCE", "STORE", "TEMP1",
          "SGEN", "DWALLSQ", "Y_PLUS", "TRANS_DYN_VIS", "DEL_DYNVIS", "TRANS_KVIS", "T_SOURCE", "ROAVG_CELL",
          "VISC_LAM", "VISC_RAT", "VISC_TURB", "XLENGTH", "FST_RAT", "DYN_VIS_ML", "ABS_VORT", "TRANS_VORT",
-         "GRAD_TVIS", "TVISGRADX", "TVISGRADR", "TVISGRADT", "DPDS_CELL"]
-JK2 = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID", "YPLUS_I1", "YPLUS_IM"]
+         "GRAD_TVIS", "TVISGRADX", "TVISGRADR", "TVISGRADT", "DPDS_CELL", "RTHETA"]
+JK2 = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID", "YPLUS_I1", "YPLUS_IM",
+       "RT_THICK"]
 IJ2 = ["YPLUS_K1", "YPLUS_KM"]
 # SETUP's initial values of the wall-function state (mf:6904-6910), for
 # fixtures written before these arrays were part of the state
@@ -38,13 +39,14 @@ VEC_K = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "B
 VEC_J = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 VEC_NR = ["VLIMIT", "ROLIMIT", "AVG_CHG", "AVG_BLK", "XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM",
           "XLLIM_DWN", "XLLIM_UP", "CHORD", "ROUGH_H", "ROUGH_T", "ROUGH_L", "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "VISCOSY", "REYNOLDS",
-          "ROVEXIT"]
+          "ROVEXIT", "SEP_THIK", "SEP_DRAG"]
 NRTAB = ["FILAM", "FITURB", "FIWAKE", "FIUP", "FKLAM", "FKTURB"]     # (NRS, MAXKI) tables
 INT_I = ["IB1", "IB2"]
 INT_K = ["KB1", "KB2"]
 INT_J = ["NBLADE", "IND", "INDLE", "INDMIX", "INDTE", "NROW", "JSBLK", "JB1", "JB2"]
 INT_NR = ["JSTART", "JMIX", "JLE", "JTE", "JLED", "KTIPS", "KTIPE", "KS1", "KS2", "JTRAN_I1",
           "JTRAN_IM", "JTRAN_K1", "JTRAN_KM"]
+INT_RN = ["IF_CUSP", "NUP_I1", "NUP_IM", "N_WAKE"]     # one per row
 SCALARS_F = ["CFL", "DAMP", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "SFTIN", "SFX", "SFT",
              "FAC_4TH", "F1", "F2", "F3", "RSMTH", "SFEXIT", "FSMTHB", "RFIN", "PDOWN_HUB",
              "PDOWN_TIP", "FEXTRAP", "RFMIX", "FANGLE", "CP", "GA", "RGAS", "CV", "FGA", "GA1", "RCP",
@@ -126,6 +128,8 @@ def from_dump(d) -> Dict:
         st[n] = np.array(d[n][:jm], np.int32)
     for n in INT_NR:
         st[n] = np.array(d[n][:nr + 1], np.int32)
+    for n in INT_RN:
+        st[n] = np.array(d[n][:nr], np.int32)
     st["NOUT"] = np.array(d["NOUT"][:5], np.int32)
     nib1, njb1, nkb1 = int(d["NIB1"]), int(d["NJB1"]), int(d["NKB1"])
     nib2, njb2, nkb2 = int(d["NIB2"]), int(d["NJB2"]), int(d["NKB2"])

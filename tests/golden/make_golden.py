@@ -61,6 +61,7 @@ CASES = {
     "tiny2_bc3": lambda n: deckgen.deck_tiny_bc("bc3", nsteps=n),
     "tiny3_sup": lambda n: deckgen.deck_tiny_supersonic(nsteps=n),
     "tiny3_nout": lambda n: deckgen.deck_tiny_nout(nsteps=n),
+    "tiny3_cusp2": lambda n: deckgen.deck_tiny_cusp2(nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}
