@@ -32,10 +32,10 @@
   X(XFORCE) X(RFORCE) X(TFORCE) X(QSOURCE) X(FLOWX) X(FLOWT) X(FLOWR) X(XFLUX) X(TFLUX)  \
   X(RFLUX) X(SOURCE) X(STORE) X(TEMP1) X(SGEN) X(DWALLSQ) X(Y_PLUS) X(TRANS_DYN_VIS) X(DEL_DYNVIS)      \
   X(TRANS_KVIS) X(T_SOURCE) X(ROAVG_CELL) X(VISC_LAM) X(VISC_RAT) X(VISC_TURB) X(XLENGTH) X(FST_RAT)      \
-  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT) X(DPDS_CELL) \
-  X(RTHETA)
-#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID) X(YPLUS_I1) X(YPLUS_IM) \
-  X(RT_THICK)
+  X(DYN_VIS_ML) X(ABS_VORT) X(TRANS_VORT) X(GRAD_TVIS) X(TVISGRADX) X(TVISGRADR) X(TVISGRADT) X(DPDS_CELL)
+#define FJK(X) X(R) X(X) X(RAVG_CELL) X(ABT) X(WRABT) X(UBLADE) X(DX) X(DR) X(DS) X(SMERID) X(YPLUS_I1) X(YPLUS_IM)
+/* read only for rows with IF_CUSP = 2: node R*THETA, blade thickness (J,K) */
+#define FOPT(X) X(RTHETA) X(RT_THICK)
 #define FIJ(X) X(YPLUS_K1) X(YPLUS_KM)
 #define FIK(X) X(FTL) X(FBL) X(FTR) X(FBR)
 #define F1D(X) X(FP) X(FU) X(FD) X(FR) X(FKU) X(FKD) X(PO1) X(TO1) X(RPO1) X(BSSIN) X(BSCOS) \
@@ -64,7 +64,7 @@
 
 struct orc {
 #define X(n) float *n;
-  F3D(X) FJK(X) FIK(X) FIJ(X) F1D(X)
+  F3D(X) FJK(X) FIK(X) FIJ(X) F1D(X) FOPT(X)
 #undef X
 #define X(n) int *n;
   I1D(X)
@@ -89,7 +89,7 @@ void orc_destroy(orc_t *o) {
 
 int orc_bind(orc_t *o, const char *name, void *ptr) {
 #define X(n) if (!strcmp(name, #n)) { o->n = (float *)ptr; return 0; }
-  F3D(X) FJK(X) FIK(X) FIJ(X) F1D(X)
+  F3D(X) FJK(X) FIK(X) FIJ(X) F1D(X) FOPT(X)
 #undef X
 #define X(n) if (!strcmp(name, #n)) { o->n = (int *)ptr; return 0; }
   I1D(X)
@@ -142,6 +142,10 @@ int orc_start(orc_t *o) {
       o->IF_REPEAT != 0 || o->IF_RESTART != 0)
     return -2;
   if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 100) return -3;
+  for (int NR = 1; o->IF_CUSP && NR <= o->NROWS; NR++)
+    if (o->IF_CUSP[NR - 1] == 2 && (!o->RTHETA || !o->RT_THICK || !o->NUP_I1 || !o->NUP_IM || !o->N_WAKE ||
+                                    !o->SEP_THIK || !o->SEP_DRAG))
+      return -100;
   const int IM = o->IM, JM = o->JM, KM = o->KM, IMID = o->IMID, KMID = o->KMID;
   (void)IM;
   o->RO_REF = A1(PO1, KMID) / (o->RGAS * A1(TO1, KMID));
