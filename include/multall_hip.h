@@ -56,6 +56,7 @@ typedef struct mh_resid {
   int ifend;        /* 1 once the run has ended (NSTEPS_MAX, converged) */
   float emax, eavg, econt, vref, vmax;
   float flow_in, flow_out; /* BLADE_FLOW(1), BLADE_FLOW(JM), kg/s */
+  int imax, jmax, kmax;     /* where the last check found EMAX (IMAX, JMAX, KMAX; 0 before any) */
 } mh_resid;
 
 /* Array ids.  Shapes: N = (IM,JM,KM) node/cell arrays; JK = (JM,KM);

@@ -1300,6 +1300,9 @@ int read_back(mh_ctx *c) {
   c->last.vmax = sc[SC_VMAX];
   c->last.flow_in = sc[SC_FLOWRAT];
   c->last.flow_out = sc[SC_FLOWOUT];
+  c->last.imax = ctl[CTL_IMAX];
+  c->last.jmax = ctl[CTL_JMAX];
+  c->last.kmax = ctl[CTL_KMAX];
   if (ctl[CTL_NANSTEP])
     return fail(c, MH_E_NAN, "non-finite flow field at step " + std::to_string(ctl[CTL_NANSTEP]));
   return MH_OK;

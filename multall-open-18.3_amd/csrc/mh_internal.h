@@ -101,6 +101,9 @@ enum {
   CTL_STOPREQ,     // a stop request (stopit) is pending
   CTL_NANSTEP,     // the step whose convergence check found a non-finite value (0: none)
   CTL_FLOWOK,      // consecutive checks whose inlet-flow change met FLOW_CONV
+  CTL_IMAX,        // where the last check found EMAX (1-based I, J, K)
+  CTL_JMAX,
+  CTL_KMAX,
   CTL_COUNT = 16
 };
 

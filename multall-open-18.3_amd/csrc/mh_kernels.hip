@@ -4049,16 +4049,21 @@ __global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int e
   if (exact) {
     if (threadIdx.x != 0) return;
     float EMAX = 0.0f, EAVG = 0.0f;
+    int IMAX = 0, JMAX = 0, KMAX = 0;
     for (int K = 1; K <= g.KM; K++)
       for (int J = 2; J <= g.JM; J++)
         for (int I = 1; I <= g.IM; I++) {
           float DVMER = dvmer_at(g, d, I, J, K, RVEF);
           EAVG = EAVG + fabsf(DVMER);
-          if (fabsf(DVMER) > fabsf(EMAX)) EMAX = DVMER;
+          if (fabsf(DVMER) > fabsf(EMAX)) {
+            EMAX = DVMER;
+            IMAX = I; JMAX = J; KMAX = K;
+          }
         }
     d.SC[SC_EAVG] = EAVG / (float)(g.IM * g.JMM1 * g.KM);
     d.SC[SC_EMAX] = EMAX;
     d.SC[SC_VREF] = VREF;
+    d.CTL[CTL_IMAX] = IMAX; d.CTL[CTL_JMAX] = JMAX; d.CTL[CTL_KMAX] = KMAX;
     return;
   }
   double s = 0.0, ba = -1.0, bi = -1.0;
@@ -4085,6 +4090,15 @@ __global__ void k_emax_final(Geo g, Dev d, const double *part, int nparts, int e
     d.SC[SC_EAVG] = (float)ssum[0] / (float)(g.IM * g.JMM1 * g.KM);
     d.SC[SC_EMAX] = sval[0];
     d.SC[SC_VREF] = VREF;
+    // the first node in the reference's loop order (K, J, I) with the
+    // largest |DVMER|: the smallest linear index among equal values; none
+    // when every value is zero (the reference keeps its last location then)
+    if (sidx[0] >= 0 && sabs[0] > 0.0) {
+      const long long n = (long long)sidx[0];
+      d.CTL[CTL_IMAX] = (int)(n % g.IM) + 1;
+      d.CTL[CTL_JMAX] = (int)((n / g.IM) % g.JM) + 1;
+      d.CTL[CTL_KMAX] = (int)(n / ((long long)g.IM * g.JM)) + 1;
+    }
   }
 }
 

@@ -26,6 +26,13 @@ contains
     if (mh_host_load_path(cstr(path))) mhh_load = -1
   end function mhh_load
 
+  !> Write the loaded deck's plotting grid file (grid_out) to path.
+  integer(c_int) function mhh_write_grid_out(path) bind(C, name='mhh_write_grid_out')
+    character(kind=c_char), intent(in) :: path(*)
+    mhh_write_grid_out = 0
+    if (write_grid_out(cstr(path))) mhh_write_grid_out = -1
+  end function mhh_write_grid_out
+
   !> 1: the next loads form DWALLSQ/XLENGTH/FST_RAT on the host (SET_XLENGTH,
   !> the default); 0: they leave them zero for the library's mh_set_xlength.
   subroutine mhh_set_host_xlength(flag) bind(C, name='mhh_set_host_xlength')

@@ -12,6 +12,7 @@ module mh_hipif
   type, bind(C) :: mh_resid
     integer(c_int) :: nstep, ifend
     real(c_float) :: emax, eavg, econt, vref, vmax, flow_in, flow_out
+    integer(c_int) :: imax, jmax, kmax
   end type mh_resid
 
   type, bind(C) :: mh_perf

@@ -20,7 +20,7 @@ module mh_host
   use iso_c_binding
   implicit none
   private
-  public :: mh_host_load_unit, mh_host_load_path
+  public :: mh_host_load_unit, mh_host_load_path, write_grid_out
 
   character(len=256), save, public :: mh_host_err = ''
   real, parameter :: PI = 3.14159265, DEGRAD = PI/180., RADDEG = 180./PI
@@ -2051,6 +2051,38 @@ contains
 
   !> Read a deck from an open unit and build the case.  Returns .true. on
   !> failure (message in mh_host_err).
+  !> The plotting grid file SETUP writes (mf:6585-6599): NSTEPS_MAX, the
+  !> grid size, CP and GA, the leading/trailing-edge marker, WRAD and NBLADE
+  !> per J, then X, R and the I line of R*THETA for every (J, K).
+  logical function write_grid_out(path) result(bad)
+    character(len=*), intent(in) :: path
+    integer :: u, ios, j, k, i
+    integer, allocatable :: indlete(:)
+    bad = .false.
+    open (newunit=u, file=path, form='unformatted', status='replace', iostat=ios)
+    if (ios /= 0) then
+      bad = fail('cannot open ' // trim(path)); return
+    end if
+    allocate (indlete(JM))
+    do j = 1, JM
+      indlete(j) = 0
+      if (INDLE(j) == 1) indlete(j) = 1
+      if (INDTE(j) == 1) indlete(j) = 1
+    end do
+    write (u) NSTEPS_MAX
+    write (u) IM, JM, KM
+    write (u) CP, GA
+    write (u) (indlete(j), j = 1, JM)
+    write (u) (WRAD(j), j = 1, JM)
+    write (u) (NBLADE(j), j = 1, JM)
+    do j = 1, JM
+      do k = 1, KM
+        write (u) X(j, k), R(j, k), (RTHETA(i, j, k), i = 1, IM)
+      end do
+    end do
+    close (u)
+  end function write_grid_out
+
   logical function mh_host_load_unit(u) result(bad)
     integer, intent(in) :: u
     call reset()

@@ -38,6 +38,8 @@ program multall_hip
     stop 2
   end if
   write (*, '(a,3i6,a,i3)') ' GRID IM,JM,KM =', IM, JM, KM, '   ROWS =', NROWS
+  ! the plotting grid file SETUP writes, mf:6585-6599
+  if (write_grid_out('grid_out')) write (*, *) ' COULD NOT WRITE grid_out: ', trim(mh_host_err)
   dims = mh_dims(IM, JM, KM, NROWS, NIB1, NJB1, NKB1, NIB2, NJB2, NKB2, NSBLK)
   rc = mh_create(dims, ctx)
   call check(rc, 'mh_create')
@@ -60,8 +62,9 @@ program multall_hip
     call check(rc, 'mh_step')
     if (res%nstep == nstep_done) exit
     nstep_done = res%nstep
-    write (4, '(a,3e12.5,a,e12.5,a,i5)') 'EMAX,EAVG,ECONT', res%emax, res%eavg, res%econt, ' FLOW', res%flow_in, &
-      'STEP', res%nstep
+    ! the convergence line of every check, mf:5437-5438
+    write (4, '(a,3e12.5,a,e12.5,a,i5,a,3i5)') 'EMAX,EAVG,ECONT', res%emax, res%eavg, res%econt, ' FLOW', &
+      res%flow_in, 'STEP', res%nstep, 'AT', res%imax, res%jmax, res%kmax
     if (mod(res%nstep, 50) == 0 .or. res%ifend == 1) &
       write (*, '(i6,3f9.4,2f10.2,2f12.4)') res%nstep, res%emax, res%eavg, res%econt, res%vref, res%vmax, &
       res%flow_in, res%flow_out

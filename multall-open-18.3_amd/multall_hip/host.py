@@ -59,8 +59,20 @@ def _load():
         L.mhh_array.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong]
         L.mhh_array.restype = ctypes.c_longlong
         L.mhh_set_host_xlength.argtypes = [ctypes.c_int]
+        L.mhh_write_grid_out.argtypes = [ctypes.c_char_p]
+        L.mhh_write_grid_out.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def write_grid_out(path: str) -> None:
+    """The plotting grid file of the deck loaded last, in the record layout
+    the reference's SETUP writes (grid_out)."""
+    L = _load()
+    if L.mhh_write_grid_out(os.fspath(path).encode()) != 0:
+        buf = ctypes.create_string_buffer(512)
+        L.mhh_error(buf, 512)
+        raise OSError(buf.value.decode(errors="replace"))
 
 
 def array(name: str, shape, dtype=np.float32) -> np.ndarray:

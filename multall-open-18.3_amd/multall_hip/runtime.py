@@ -32,7 +32,8 @@ class MhDims(ctypes.Structure):
 class MhResid(ctypes.Structure):
     _fields_ = [("nstep", ctypes.c_int), ("ifend", ctypes.c_int), ("emax", ctypes.c_float),
                 ("eavg", ctypes.c_float), ("econt", ctypes.c_float), ("vref", ctypes.c_float),
-                ("vmax", ctypes.c_float), ("flow_in", ctypes.c_float), ("flow_out", ctypes.c_float)]
+                ("vmax", ctypes.c_float), ("flow_in", ctypes.c_float), ("flow_out", ctypes.c_float),
+                ("imax", ctypes.c_int), ("jmax", ctypes.c_int), ("kmax", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

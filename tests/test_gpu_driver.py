@@ -57,4 +57,26 @@ def test_driver_program_marches_a_deck(tmp_path):
     lag = float(np.abs(marched - ref["ROVT"])[:, 2:-2, :].max() / sc["ROVT"])
     print(f"ROVT lags RORVT/R by {lag:.2e} of its scale")
     assert lag > 10 * TOL_100
-    assert (tmp_path / "stage.log").exists()
+    # grid_out as the reference's SETUP wrote it, byte for byte
+    assert (tmp_path / "grid_out").read_bytes() == open(os.path.join(GOLDEN, "tiny3_ml.grid_out"), "rb").read()
+    # stage.log: one line per check in the reference's format (mf:5437-5438),
+    # with its step, EMAX location and figures
+    mine = (tmp_path / "stage.log").read_text().splitlines()
+    want = open(os.path.join(GOLDEN, "tiny3_ml.stage_log")).read().splitlines()
+    assert len(mine) == len(want), (len(mine), len(want))
+
+    def fields(line):
+        head, at = line.split("AT")
+        step = int(head.split("STEP")[1])
+        nums = [float(x) for x in head.split("ECONT")[1].replace(" FLOW", " ").split("STEP")[0].split()]
+        return step, nums, [int(x) for x in at.split()]
+    same_at = 0
+    for a, b in zip(mine, want):
+        sa, na, la = fields(a)
+        sb, nb, lb = fields(b)
+        assert sa == sb
+        for x, y in zip(na, nb):
+            assert abs(x - y) <= 2e-3 * max(abs(y), 1e-3), (a, b)
+        same_at += la == lb
+    print(f"stage.log: {len(mine)} lines, EMAX location equal on {same_at}")
+    assert same_at >= len(want) - 2

@@ -116,3 +116,16 @@ def test_host_setup_of_config_decks(deck):
     for k, v in want["scalars"].items():
         assert np.float32(st["scalars"][k]) == np.float32(v), (deck, k, st["scalars"][k], v)
     assert len(want["arrays"]) >= 80
+
+
+@pytest.mark.parametrize("deck", ["tiny3_ml", "tiny3_gap"])
+def test_grid_out_equals_the_reference_file(deck, tmp_path):
+    """The plotting grid file the host writes (grid_out, mf:6585-6599) is,
+    byte for byte, the file the reference's SETUP wrote for the same deck
+    (tests/golden/make_outputs.py); tiny3_gap has its gaps re-gridded."""
+    from multall_hip import host
+    host.load_deck(os.path.join(GOLDEN, "decks", f"{deck}.dat"))
+    host.write_grid_out(str(tmp_path / "grid_out"))
+    got = (tmp_path / "grid_out").read_bytes()
+    want = open(os.path.join(GOLDEN, f"{deck}.grid_out"), "rb").read()
+    assert len(got) == len(want) and got == want
