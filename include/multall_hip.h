@@ -139,6 +139,10 @@ enum mh_array_id {
    * tangential thickness RT_THICK -- JK; the node pitchwise positions
    * R*THETA (RTHETA) -- N.  Needed only when some row has IF_CUSP = 2. */
   MH_IF_CUSP, MH_NUP_I1, MH_NUP_IM, MH_N_WAKE, MH_SEP_THIK, MH_SEP_DRAG, MH_RT_THICK, MH_RTHETA,
+  /* the inlet yaw and pitch angles in degrees (BS, BR; BKINPT) -- K; the
+   * repeating-stage condition relaxes BS (mf:12100) and recomputes the
+   * sines and cosines from both.  Needed only with IF_REPEAT != 0. */
+  MH_BS, MH_BR,
   MH_NARRAYS
 };
 
@@ -161,6 +165,11 @@ enum mh_scalar_id {
   /* mass-flow forcing of SETFLO (BKINPT): target inlet flow and relaxation
    * factor, read only when IN_FLOW != 0 (required then, default 0) */
   MH_S_FLOWIN, MH_S_RFLOW,
+  /* the repeating-stage condition (IF_REPEAT != 0; required then, default
+   * 0): steps between updates and relaxation factor (mf:575-585), and the
+   * deck's mid-span inlet stagnation pressure and tangential velocity
+   * (PO1 and VTIN at the middle input point, mf:1889, 1911) */
+  MH_S_NINMOD, MH_S_RFINBC, MH_S_PO_IN_MID, MH_S_VT_IN_MID,
   /* optional state: laminar viscosity and conductivity ratio left by the
    * last LOSS call (default 0; mh_loss_init sets them) */
   MH_S_VISLAM, MH_S_FTCOND,
@@ -301,6 +310,19 @@ typedef struct {
 /* Fills out[0 .. *n-1]; MH_E_ARG when cap < the number of records (then *n
  * still says how many there are). */
 int mh_eficool(mh_ctx *ctx, mh_perf *out, int cap, int *n);
+
+/* MIX_BCONDS (mf:11906-12080): the spanwise profiles of mass-averaged flow
+ * at the inlet (J = 1) and at each row's mixing plane or exit (J = JMIX),
+ * as the reference writes them to 'mixbconds' and the repeating-stage
+ * condition uses them.  Per plane p (0 = inlet, p = row number) and
+ * quantity q (0 stagnation pressure, 1 static pressure, 2 stagnation
+ * temperature, 3 static temperature, 4 absolute tangential velocity,
+ * 5 meridional velocity, 6 yaw angle, 7 pitch angle, both in degrees):
+ * out[(p*8 + q)*(KM+1) + K-1] for K = 1..KM (the cell-line mass averages
+ * moved to the nodes as MASS_AVG does, mf:11729-11760), and
+ * out[(p*8 + q)*(KM+1) + KM] the whole plane's mass average.  Weights are
+ * the cells' positive mass flows; sums in FP64.  n = (NROWS+1)*8*(KM+1). */
+int mh_mix_bconds(mh_ctx *ctx, double *out, long long n);
 /* Wait for all queued work; returns MH_E_HIP on an asynchronous error. */
 int mh_sync(mh_ctx *ctx);
 

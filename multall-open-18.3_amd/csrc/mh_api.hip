@@ -65,6 +65,8 @@ __global__ void k_rolim_fix(Geo, Dev, StepScalars, Params);
 __global__ void k_mach_apply(Geo, Dev, Params, StepScalars);
 __global__ void k_mixplan(Geo, Dev, Params, float *);
 __global__ void k_te_sep(Geo, Dev);
+__global__ void k_mix_bconds(Geo, Dev, Params);
+__global__ void k_newbconds(Geo, Dev, Params);
 __global__ void k_vrms_part(Geo, Dev, double *);
 __global__ void k_vrms_final(Geo, Dev, const double *, int, int);
 __global__ void k_checkflow(Geo, Dev, int);
@@ -162,6 +164,7 @@ const ArrayInfo kArrays[MH_NARRAYS] = {
     {"NSTAGE", K_RN, true},
     {"IF_CUSP", K_RN, true}, {"NUP_I1", K_RN, true}, {"NUP_IM", K_RN, true}, {"N_WAKE", K_RN, true},
     {"SEP_THIK", K_RN, false}, {"SEP_DRAG", K_RN, false}, {"RT_THICK", K_JK, false}, {"RTHETA", K_N, false},
+    {"BS", K_K, false}, {"BR", K_K, false},
 };
 
 const char *const kScalarNames[MH_NSCALARS] = {
@@ -171,7 +174,8 @@ const char *const kScalarNames[MH_NSCALARS] = {
     "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS", "NMIXUP", "ITIMST",
     "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IRBB", "KRBB", "REYNO", "RF_VIS", "FTRANS", "PRANDTL",
     "YPLUSWALL", "YPLAM", "YPTURB", "CFWALL", "IBOUND", "FAC_STMIX", "FAC_ST0", "FAC_ST1", "FAC_ST2", "FAC_ST3",
-    "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "FLOWIN", "RFLOW", "VISLAM", "FTCOND",
+    "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "TURBVIS_LIM", "FLOWIN", "RFLOW", "NINMOD", "RFINBC",
+    "PO_IN_MID", "VT_IN_MID", "VISLAM", "FTCOND",
     "EXACT_ORDER", "USE_GRAPHS", "LANES", "MIXED", "FLOW_CONV", "TUNE", "SCAT_TK", "SMOOTH_WAVES",
 };
 
@@ -281,7 +285,8 @@ void **slot_of(Dev &d, int id) {
       (void **)&d.FSTURB, (void **)&d.TURBVIS_DAMP, (void **)&d.DPDS_CELL, (void **)&d.YPLUS_I1,
       (void **)&d.YPLUS_IM, (void **)&d.YPLUS_K1, (void **)&d.YPLUS_KM, (void **)&d.CHORD, (void **)&d.NOUT,
       (void **)&d.NSTAGE, (void **)&d.IF_CUSP, (void **)&d.NUP_I1, (void **)&d.NUP_IM, (void **)&d.N_WAKE,
-      (void **)&d.SEP_THIK, (void **)&d.SEP_DRAG, (void **)&d.RT_THICK, (void **)&d.RTHETA,
+      (void **)&d.SEP_THIK, (void **)&d.SEP_DRAG, (void **)&d.RT_THICK, (void **)&d.RTHETA, (void **)&d.BS,
+      (void **)&d.BR,
   };
   return base[id];
 }
@@ -350,6 +355,8 @@ Params params_of(const mh_ctx *c) {
   p.FAC_VORT = sf(c, MH_S_FAC_VORT); p.FAC_PGRAD = sf(c, MH_S_FAC_PGRAD); p.VORT_MAX = sf(c, MH_S_VORT_MAX);
   p.TURBVIS_LIM = sf(c, MH_S_TURBVIS_LIM);
   p.FLOWIN = sf(c, MH_S_FLOWIN); p.RFLOW = sf(c, MH_S_RFLOW);
+  p.NINMOD = si(c, MH_S_NINMOD); p.RFINBC = sf(c, MH_S_RFINBC);
+  p.PO_IN_MID = sf(c, MH_S_PO_IN_MID); p.VT_IN_MID = sf(c, MH_S_VT_IN_MID);
   p.FLOW_CONV = sf(c, MH_S_FLOW_CONV);
   return p;
 }
@@ -454,8 +461,9 @@ struct StepPlan {
   bool refresh = false;  // the closing / printout viscous refresh (mf:5338-5341)
   bool damp = false;     // DAMP within [2, 100]: TSTEP damps
   bool first = false;    // NSTEP = 1
+  bool repeat = false;   // the repeating-stage update runs in the check (NSTEP divisible by NINMOD)
   int code() const {
-    return check5 | cond << 1 | visc << 2 | vislam << 3 | refresh << 4 | damp << 5 | first << 6;
+    return check5 | cond << 1 | visc << 2 | vislam << 3 | refresh << 4 | damp << 5 | first << 6 | repeat << 7;
   }
 };
 
@@ -476,6 +484,7 @@ StepPlan plan_step(mh_ctx *c, int n) {
   if (!(p.IF_RESTART != 0 && n > 100) && FCHANGE > 0.0f) c->DAMP = p.DAMPIN * (1.f - 0.75f * FCHANGE);
   q.damp = !(c->DAMP < 2.0f || c->DAMP > 100);
   q.first = n == 1;
+  q.repeat = p.IF_REPEAT > 0 && p.NINMOD > 0 && (q.check5 || q.cond) && n % p.NINMOD == 0;
   return q;
 }
 
@@ -1235,6 +1244,12 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LAUNCH(k_emax_final, dim3(1), dim3(256), 0, c->stream, g, dc, (const double *)d.RED, kRedBlocks, p.EXACT);
     LAUNCH(k_nan_check, dim3(1), dim3(64), 0, c->stream, g, dc, p);
     LCHK("convergence");
+    // the repeating-stage condition, mf:5323-5331: MIX_BCONDS, then NEWBCONDS
+    if (q.repeat) {
+      LAUNCH(k_mix_bconds, dim3((g.NROWS + 1) * 8), dim3(64), (g.KM + 1) * sizeof(double), c->stream, g, dc, p);
+      LAUNCH(k_newbconds, dim3((g.KM + 63) / 64), dim3(64), 0, c->stream, g, dc, p);
+      LCHK("repeating stage");
+    }
     // forces brought up to date before a printout or the final output,
     // mf:5338-5341 (on a step that may end by convergence: if it does)
     if (q.refresh && (r = viscous(c, dc, p.ILOS, p.RF_VIS, q.vislam, q.first))) return r;
@@ -1399,6 +1414,7 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
       (r = dalloc(c, &d.SETFLO, (size_t)2 * dm.jm)) ||
       (r = dalloc(c, &d.KWALL, (size_t)kKWall * dm.im * dm.jm)) ||
       (r = dalloc(c, &d.INTEG, (size_t)7 * dm.jm)) || (r = dalloc(c, &d.CTL, (size_t)CTL_COUNT)) ||
+      (r = dalloc(c, &d.MIXB, (size_t)(dm.nrows + 1) * 8 * (dm.km + 1))) ||
       (r = dalloc(c, &c->lists.mixscratch, (size_t)std::max(1, dm.nrows - 1) * dm.km * 9 * dm.im)))
     return bail(r);
   d.AVG = d.SC + SC_AVG_CHG;
@@ -1524,17 +1540,19 @@ double mh_get_scalar(const mh_ctx *c, int id) {
 // IN_FLOW != 0.  Then the options this library implements.
 static int check_inputs(mh_ctx *c) {
   const bool forcing = c->scal_set[MH_S_IN_FLOW] && si(c, MH_S_IN_FLOW) != 0;
+  const bool repeating = c->scal_set[MH_S_IF_REPEAT] && si(c, MH_S_IF_REPEAT) != 0;
   for (int i = 0; i < MH_NSCALARS; i++)
     if (!c->scal_set[i] && i != MH_S_EXACT_ORDER && i != MH_S_USE_GRAPHS && i != MH_S_LANES && i != MH_S_MIXED &&
         i != MH_S_FLOW_CONV && i != MH_S_TUNE && i != MH_S_SCAT_TK && i != MH_S_SMOOTH_WAVES &&
         i != MH_S_VISLAM &&
         i != MH_S_FTCOND &&
-        (forcing || (i != MH_S_FLOWIN && i != MH_S_RFLOW)))
+        (forcing || (i != MH_S_FLOWIN && i != MH_S_RFLOW)) &&
+        (repeating || (i != MH_S_NINMOD && i != MH_S_RFINBC && i != MH_S_PO_IN_MID && i != MH_S_VT_IN_MID)))
       return fail(c, MH_E_STATE, std::string("scalar not set: ") + kScalarNames[i]);
   const Params p = params_of(c);
   if (p.IFGAS != 0) return fail(c, MH_E_UNSUPPORTED, "real-gas properties (IFGAS=1)");
   if (p.ITIMST != 3) return fail(c, MH_E_UNSUPPORTED, "ITIMST other than 3/-3 (artificial compressibility)");
-  if (p.IF_REPEAT != 0) return fail(c, MH_E_UNSUPPORTED, "repeating-stage conditions (IF_REPEAT) set");
+  if (p.IF_REPEAT != 0 && p.NINMOD < 1) return fail(c, MH_E_ARG, "IF_REPEAT set with NINMOD < 1");
   if (p.ILOS != 0 && p.ILOS != 10 && p.ILOS < 100)
     return fail(c, MH_E_UNSUPPORTED, "ILOS other than 0, 10 (LOSS), 100-199 (NEW_LOSS) or >= 200 (SPAL_LOSS)");
   if (p.ILOS != 0 && p.NLOS < 1) return fail(c, MH_E_ARG, "NLOS < 1");
@@ -1909,6 +1927,18 @@ int mh_eficool(mh_ctx *c, mh_perf *out, int cap, int *n) {
     o.eta_poly = f[11]; o.reaction = f[12]; o.reaction_hub = f[13]; o.reaction_mid = f[14];
     o.reaction_tip = f[15]; o.work_kw = f[16];
   }
+  return MH_OK;
+}
+
+int mh_mix_bconds(mh_ctx *c, double *out, long long n) {
+  if (!c || !out || n != (long long)(c->g.NROWS + 1) * 8 * (c->g.KM + 1))
+    return fail(c, MH_E_ARG, "mh_mix_bconds: need (NROWS+1)*8*(KM+1) doubles");
+  const Params p = params_of(c);
+  LAUNCH(k_mix_bconds, dim3((c->g.NROWS + 1) * 8), dim3(64), (c->g.KM + 1) * sizeof(double), c->stream, c->g, c->d,
+         p);
+  LCHK("mix_bconds");
+  HIPCHK(hipMemcpyAsync(out, c->d.MIXB, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return MH_OK;
 }
 

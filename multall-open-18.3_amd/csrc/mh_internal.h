@@ -62,6 +62,8 @@ struct Dev {
   int *NSTAGE;
   int *IF_CUSP, *NUP_I1, *NUP_IM, *N_WAKE;
   float *SEP_THIK, *SEP_DRAG, *RT_THICK, *RTHETA;
+  float *BS, *BR;
+  double *MIXB;  // MIX_BCONDS profiles: (NROWS+1) planes x 8 quantities x (KM+1)
   // scratch (not part of the public array set)
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *TEMP1;
   float *B1CHG, *B2CHG, *SBCHG;      // multigrid block sums (FP32 results)
@@ -163,6 +165,8 @@ struct Params {
   int IBOUND;
   float FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD, VORT_MAX, TURBVIS_LIM;
   float FLOWIN, RFLOW;
+  int NINMOD;
+  float RFINBC, PO_IN_MID, VT_IN_MID;
   float FLOW_CONV;  // library stop test on the inlet mass flow's relative change per check (0: off)
 };
 

@@ -53,6 +53,12 @@ __device__ __forceinline__ float line_sum(const float (&v)[4], int n, int exact)
 // The wavefront's index in its workgroup, as a scalar (SGPR) value, so that
 // every line index derived from it is computed once per wavefront on the
 // scalar unit rather than per lane.
+// FP64 sum over the 64 lanes of a wavefront (every lane gets it)
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 // FP32 transcendentals as the FP64 function rounded once to FP32: the
@@ -3592,6 +3598,97 @@ __global__ void k_mach_apply(Geo g, Dev d, Params p, StepScalars st) {
 // NEW_MIXPLAN, mf:14922-15251 (ideal gas, ITIMST<5): one thread per
 // (mixing plane, K) line; the supersonic Prandtl-Meyer sweeps are serial in I.
 
+// MIX_BCONDS (mf:11906-12080) with MASS_AVG (mf:11729-11760): one wavefront
+// per (plane, quantity).  Plane 0 is J = 1, plane p >= 1 is J = JMIX(p).
+// The node quantity of each cell's four corners on the plane, weighted by
+// the cell's positive mass flow -FLOWX, summed over I per cell line K (FP64
+// lane sums), divided by the line's flow; the line values then moved to the
+// nodes by the reference's running average and end extrapolations, and the
+// whole plane's average.  The node quantities are the reference's FP32
+// expressions (transcendentals through FP64, rounded once).
+__device__ __forceinline__ float mixb_quantity(const Geo &g, const Dev &d, const Params &p, int q, size_t n) {
+  const float RADDEG = 180.f / 3.14159265f;
+  const float VX = d.VX[n], VT = d.VT[n], VR = d.VR[n];
+  switch (q) {
+    case 0: {
+      const float D = .5f * (VX * VX + VT * VT + VR * VR);
+      const float TSTAG = d.HO[n] / p.CP;
+      float TS = TSTAG - D / p.CP;
+      if (TS < 1.f) TS = 1.f;
+      return d.P[n] * pow_r(TSTAG / TS, p.GA / (p.GA - 1));
+    }
+    case 1: return d.P[n];
+    case 2: return d.HO[n] / p.CP;
+    case 3: {
+      const float D = .5f * (VX * VX + VT * VT + VR * VR);
+      return (d.HO[n] - D) / p.CP;
+    }
+    case 4: return VT;
+    case 5: return sqrtf(VX * VX + VR * VR);
+    case 6: return atan2_r(VT, sqrtf(VX * VX + VR * VR)) * RADDEG;
+    default: return atan2_r(VR, sqrtf(VX * VX + VR * VR)) * RADDEG;
+  }
+}
+__global__ void __launch_bounds__(64) k_mix_bconds(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
+  const int pl = blockIdx.x / 8, q = blockIdx.x % 8, lane = threadIdx.x;
+  const int J = pl == 0 ? 1 : d.JMIX[pl - 1];
+  const int KM = g.KM;
+  extern __shared__ double S[];  // KM + 1 line values
+  double sumall = 0.0, flowall = 0.0;
+  for (int K = 1; K <= KM - 1; K++) {
+    double sas = 0.0, sf = 0.0;
+    for (int I = lane + 1; I <= g.IMM1; I += 64) {
+      float DFLOW = -d.FLOWX[IX(I, J, K)];
+      if (DFLOW < 0.0f) DFLOW = 0.0f;
+      const float f = mixb_quantity(g, d, p, q, IX(I, J, K)) + mixb_quantity(g, d, p, q, IX(I + 1, J, K)) +
+                      mixb_quantity(g, d, p, q, IX(I + 1, J, K + 1)) + mixb_quantity(g, d, p, q, IX(I, J, K + 1));
+      sas += (double)DFLOW;
+      sf += (double)DFLOW * (double)f * 0.25;
+    }
+    sas = wave_sum_d(sas);
+    sf = wave_sum_d(sf);
+    if (lane == 0) {
+      sumall += sf;
+      flowall += sas;
+      S[K] = sf / sas;
+    }
+  }
+  if (lane != 0) return;
+  for (int K = 2; K <= KM - 1; K++) S[K] = 0.5 * (S[K] + S[K - 1]);
+  S[1] = 2.0 * S[2] - S[3];
+  S[KM] = 2.0 * S[KM - 1] - S[KM - 2];
+  double *o = d.MIXB + (size_t)blockIdx.x * (KM + 1);
+  for (int K = 1; K <= KM; K++) o[K - 1] = S[K];
+  o[KM] = sumall / flowall;
+}
+
+// NEWBCONDS (mf:12085-12115), the repeating-stage condition: the inlet
+// stagnation pressure, yaw angle and tangential velocity relaxed by RFINBC
+// towards the exit plane's profiles (MIX_BCONDS of the last plane, in the
+// reference's FP32), shifted to the deck's mid-span inlet values; the
+// sines and cosines of the inlet angles recomputed.  One thread per K.
+__global__ void k_newbconds(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
+  const int K = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (K > g.KM) return;
+  const float DEGRAD = 3.14159265f / 180.f;
+  const double *ex = d.MIXB + (size_t)g.NROWS * 8 * (g.KM + 1);  // the last plane
+  auto prof = [&](int q, int k) { return (float)ex[(size_t)q * (g.KM + 1) + k - 1]; };
+  const float RFINBC1 = 1.0f - p.RFINBC;
+  const float po = RFINBC1 * d.PO1[K - 1] + p.RFINBC * (p.PO_IN_MID + prof(0, K) - prof(0, p.KMID));
+  const float bs = RFINBC1 * d.BS[K - 1] + p.RFINBC * prof(6, K);
+  const float vt = RFINBC1 * d.VTIN[K - 1] + p.RFINBC * (p.VT_IN_MID + prof(4, K) - prof(4, p.KMID));
+  d.PO1[K - 1] = po;
+  d.BS[K - 1] = bs;
+  d.VTIN[K - 1] = vt;
+  d.RPO1[K - 1] = 1.f / po;
+  d.BSSIN[K - 1] = sin_r(bs * DEGRAD);
+  d.BSCOS[K - 1] = cos_r(bs * DEGRAD);
+  d.BRSIN[K - 1] = sin_r(d.BR[K - 1] * DEGRAD);
+  d.BRCOS[K - 1] = cos_r(d.BR[K - 1] * DEGRAD);
+}
+
 // The trailing-edge separation force of rows with IF_CUSP = 2
 // (mf:5184-5233), after NEW_MIXPLAN: one workgroup per (row, K).  From
 // NUP_I1 points upstream of the trailing edge to N_WAKE points downstream,
@@ -4213,11 +4310,6 @@ __global__ void k_integrals(Geo g, Dev d, Params p) {
 // KM/2 and KM (mf:11299-11357).  One wavefront.  Record r has the planes
 // tab[4r .. 4r+2] = J1, J2M1, JMIX and tab[4r+3] = its stage (0 = the whole
 // machine); it writes kEffFields doubles at out[r * kEffFields].
-__device__ __forceinline__ double wave_sum_d(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
 __global__ void __launch_bounds__(64) k_eficool(Geo g, Dev d, Params p, const int *tab, int nrec, double *out) {
   const int lane = threadIdx.x, JM = g.JM;
   const double *FLOW = d.INTEG, *SPO = FLOW + JM, *STO = FLOW + 2 * JM, *STS = FLOW + 5 * JM, *SPS = FLOW + 6 * JM;

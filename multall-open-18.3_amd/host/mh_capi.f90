@@ -117,6 +117,10 @@ contains
     case ('FSMTHB'); v = FSMTHB
     case ('FANGLE'); v = FANGLE
     case ('RFIN'); v = RFIN
+    case ('NINMOD'); v = NINMOD
+    case ('RFINBC'); v = RFINBC
+    case ('PO_IN_MID'); v = PO_IN_MID
+    case ('VT_IN_MID'); v = VT_IN_MID
     case ('FLOWIN'); v = FLOWIN
     case ('RFLOW'); v = RFLOW
     case ('SFEXIT'); v = SFEXIT
@@ -303,6 +307,8 @@ contains
     case ('SEP_THIK'); call putr1(SEP_THIK(1:NROWS))
     case ('SEP_DRAG'); call putr1(SEP_DRAG(1:NROWS))
     case ('RT_THICK'); call putr2(RT_THICK(1:JM, 1:KM))
+    case ('BS'); call putr1(BS(1:KM))
+    case ('BR'); call putr1(BR(1:KM))
     end select
   contains
     subroutine putr1(a)

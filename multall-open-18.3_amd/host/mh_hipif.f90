@@ -101,6 +101,12 @@ module mh_hipif
       real(c_double), intent(out) :: out(*)
       integer(c_long_long), value :: n
     end function mh_integrals
+    integer(c_int) function mh_mix_bconds(ctx, out, n) bind(C, name='mh_mix_bconds')
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(*)
+      integer(c_long_long), value :: n
+    end function mh_mix_bconds
     integer(c_int) function mh_eficool(ctx, out, cap, n) bind(C, name='mh_eficool')
       import :: c_int, c_ptr, mh_perf
       type(c_ptr), value :: ctx

@@ -35,6 +35,11 @@ module mh_host
   real, save, public :: SFXIN, SFTIN, FAC_4TH, FBLK1, FBLK2, FBLK3, RFMIX, FEXTRAP, FSMTHB, FANGLE
   real, save, public :: RFIN, SFEXIT, PLATE_LOSS, THROTTLE_EXIT, REYNO, RF_VIS, FTRANS, TURBVIS_LIM
   real, save, public :: FLOWIN = 0.0, RFLOW = 0.0
+  ! the repeating-stage condition (IF_REPEAT, mf:575-585): the steps between
+  ! its updates and their relaxation factor; the mid-span inlet values the
+  ! reader keeps (mf:1889, 1911)
+  integer, save, public :: NINMOD = 0
+  real, save, public :: RFINBC = 0.0, PO_IN_MID = 0.0, VT_IN_MID = 0.0
   real, save, public :: PRANDTL, YPLUSWALL, YPLAM, YPTURB, PDOWN_HUB, PDOWN_TIP, FACMIXUP, CFWALL
   real, save, public :: FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD
   real, save, public :: GA1, FGA, RFGA, RCP, CV, RCV, RGAS, SFEX1, SFEXH, VSOUND, DSMIN, VORT_MAX, PREFF
@@ -249,9 +254,6 @@ contains
     IN_PRESS = 0; IN_VTAN = 0; IN_VR = 1; IN_FLOW = 0; IF_REPEAT = 0; RFIN = 0.1
     call skip(u)
     read (u, *, iostat=ios) IN_PRESS, IN_VTAN, IN_VR, IN_FLOW, IF_REPEAT, RFIN
-    if (IF_REPEAT /= 0) then
-      bad = fail('IF_REPEAT is not supported'); return
-    end if
     IPOUT = 1; SFEXIT = 0.0; NSFEXIT = 0
     call skip(u)
     read (u, *, iostat=ios) IPOUT, SFEXIT, NSFEXIT
@@ -266,6 +268,13 @@ contains
     if (IN_FLOW /= 0) then
       call skip(u)
       read (u, *) FLOWIN, RFLOW
+    end if
+    ! the repeating-stage factors, mf:575-585
+    NINMOD = 0; RFINBC = 0.0
+    if (IF_REPEAT /= 0) then
+      NINMOD = 10; RFINBC = 0.025
+      call skip(u)
+      read (u, *, iostat=ios) NINMOD, RFINBC
     end if
     ILOS = 100; NLOS = 5; IBOUND = 0
     call skip(u)
@@ -571,6 +580,7 @@ contains
     read (u, *) (FR_IN(k), k = 1, KIN - 1)
     call skip(u)
     read (u, *) (PO1(k), k = 1, KIN)
+    PO_IN_MID = PO1(KIN_MID)
     if (IPOUT == 3) then
       call skip(u)
       read (u, *) (PD(k), k = 1, KIN)
@@ -579,6 +589,7 @@ contains
     read (u, *) (TO1(k), k = 1, KIN)
     call skip(u)
     read (u, *) (VTIN(k), k = 1, KIN)
+    VT_IN_MID = VTIN(KIN_MID)
     call skip(u)
     read (u, *) (VM1(k), k = 1, KIN)
     call skip(u)

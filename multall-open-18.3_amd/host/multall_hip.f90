@@ -80,6 +80,7 @@ program multall_hip
   close (4)
   call write_flow_out()
   call print_performance()
+  call write_mixbconds()
   rc = mh_destroy(ctx)
   write (*, *) ' STOPPED AFTER ', res%nstep, ' STEPS'
 
@@ -167,6 +168,10 @@ contains
     call sets('F2', dble(F2)); call sets('F3', dble(F3)); call sets('RSMTH', dble(RSMTH))
     call sets('SFEXIT', dble(SFEXIT)); call sets('FSMTHB', dble(FSMTHB)); call sets('RFIN', dble(RFIN))
     call sets('FLOWIN', dble(FLOWIN)); call sets('RFLOW', dble(RFLOW))
+    ! the repeating-stage condition
+    call sets('NINMOD', dble(NINMOD)); call sets('RFINBC', dble(RFINBC))
+    call sets('PO_IN_MID', dble(PO_IN_MID)); call sets('VT_IN_MID', dble(VT_IN_MID))
+    call put1('BS', BS(1:KM)); call put1('BR', BR(1:KM))
     call sets('PDOWN_HUB', dble(PDOWN_HUB)); call sets('PDOWN_TIP', dble(PDOWN_TIP))
     call sets('FEXTRAP', dble(FEXTRAP)); call sets('RFMIX', dble(RFMIX)); call sets('FANGLE', dble(FANGLE))
     call sets('CP', dble(CP)); call sets('GA', dble(GA)); call sets('RGAS', dble(RGAS)); call sets('CV', dble(CV))
@@ -255,6 +260,57 @@ contains
       end if
     end do
   end subroutine print_performance
+
+  ! 'mixbconds' at the end of the run (MIX_BCONDS(1), mf:5445 and
+  ! 11906-12080): the mass-averaged spanwise profiles at the inlet and at
+  ! each row's JMIX, in the reference's layout; the averages from the device
+  subroutine write_mixbconds()
+    real(c_double), allocatable :: mb(:)
+    integer :: u, nr, q, k, j, base
+    real :: fspan(KM)
+    character(len=32), parameter :: names(8) = [character(len=32) :: ' STAGNATION PRESSURE ', &
+      ' STATIC PRESSURE ', ' STAGNATION TEMPERATURE ', ' STATIC TEMPERATURE ', &
+      ' ABSOLUTE TANGENTIAL VELOCITY ', ' MERIDIONAL VELOCITY ', ' YAW ANGLE ', ' PITCH ANGLE ']
+    allocate (mb((NROWS + 1)*8*(KM + 1)))
+    call check(mh_mix_bconds(ctx, mb, int(size(mb), c_long_long)), 'mh_mix_bconds')
+    fspan(1) = 0.0
+    do k = 2, KM
+      fspan(k) = fspan(k - 1) + FR(k - 1)
+    end do
+    open (newunit=u, file='mixbconds')
+    do nr = 1, NROWS + 1
+      write (u, *)
+      write (u, *) ' BLADE ROW NUMBER ', nr, ' JMIX = ', JMIX(nr)
+      write (u, *)
+      if (nr == 1) then
+        write (u, *)
+        write (u, *) ' INLET BOUNDARY CONDITIONS TO ROW NUMBER 1.'
+        write (u, *)
+      else
+        j = JMIX(nr - 1)
+        write (u, *) ' ROW NUMBER ', nr - 1, ' JMIX  =  ', j
+        write (u, *) ' EXIT FLOW CONDITIONS FROM THIS BLADE ROW'
+        write (u, *) ' THESE CAN BE USED AS THE INLET BOUNDARY CONDITIONS FOR THE NEXT STAGE.'
+        write (u, *)
+      end if
+      do q = 1, 8
+        base = ((nr - 1)*8 + q - 1)*(KM + 1)
+        write (u, *) trim(names(q))
+        if (q <= 4) then
+          write (u, '(8F10.1)') (real(mb(base + k)), k = 1, KM)
+        else
+          write (u, '(8F10.4)') (real(mb(base + k)), k = 1, KM)
+        end if
+      end do
+      write (u, *) ' SPANWISE GRID SPACING '
+      write (u, '(8F10.6)') (FR(k), k = 1, KM - 1)
+      write (u, *) ' PITCHWISE GRID SPACING '
+      write (u, '(8F10.6)') (FP(k), k = 1, IM - 1)
+      write (u, *) ' SPANWISE GRID POSITIONS'
+      write (u, '(8F10.6)') (fspan(k), k = 1, KM)
+    end do
+    close (u)
+  end subroutine write_mixbconds
 
   ! flow_out, mf:5951-5976 (ITIMST<5: ROE as marched, ROSUB unset = 0; the
   ! SA variable with ILOS >= 200, else the wall pressure gradient x 1e-6)

@@ -87,6 +87,7 @@ class DeckSpec:
     ishift: int = 0                # 2: the reader re-grids the gaps between rows (GRID_UP/GRID_DOWN)
     nextrap: tuple = (10, 10)      # NEXTRAP_LE, NEXTRAP_TE
     in_flow: int = 0               # mass-flow forcing: 3 towards FLOWIN, 2 towards the mean over J
+    repeat: Optional[tuple] = None  # (NINMOD, RFINBC): IF_REPEAT = 1, the repeating-stage inlet condition
     flowin: float = 0.0            # kg/s through the whole annulus
     rflow: float = 0.0
     ipout: int = 0
@@ -244,7 +245,8 @@ def write_deck(spec: DeckSpec) -> str:
     a("       NSECS_IN")
     a(f"{spec.nsecs:10d}")
     a("       IN_PRESS  IN_VTAN   IN_VR    IN_FLOW  IF_REPEAT  RFIN")
-    a(f"{spec.in_press:10d}{spec.in_vtan:10d}{spec.in_vr:10d}{spec.in_flow:10d}         0{spec.rfin:12.5f}")
+    a(f"{spec.in_press:10d}{spec.in_vtan:10d}{spec.in_vr:10d}{spec.in_flow:10d}"
+      f"{0 if spec.repeat is None else 1:10d}{spec.rfin:12.5f}")
     a("  IPOUT SFEXIT  NSFEXIT")
     a(f"{spec.ipout:5d}{spec.sfexit:12.6f}{spec.nsfexit:5d}")
     a("  PLATE_LOSS THROTTLE_EXIT")
@@ -252,6 +254,9 @@ def write_deck(spec: DeckSpec) -> str:
     if spec.in_flow != 0:
         a("  FLOWIN  RFLOW")
         a(f"{spec.flowin:12.5f}{spec.rflow:10.4f}")
+    if spec.repeat is not None:
+        a("  NINMOD  RFINBC")
+        a(f"{spec.repeat[0]:10d}{spec.repeat[1]:10.4f}")
     a("        ILOS      NLOS      IBOUND")
     a(f"{spec.ilos:10d}{spec.nlos:10d}{spec.ibound:10d}")
     if spec.ilos != 0:
@@ -654,6 +659,24 @@ def deck_tiny_cusp2(nsteps: int = 100) -> DeckSpec:
     spec.rows[0].sep_force = (4, 3, 3, 0.05, 0.97)
     spec.rows[2].sep_force = (3, 4, 2, 0.05, 0.97)
     spec.title = "TINY CASCADE ILOS=10 IF_CUSP=2 ON THE STATORS"
+    return spec
+
+
+def deck_tiny_repeat(nsteps: int = 100) -> DeckSpec:
+    """A rotor-stator stage (the two-row cascade of ``deck_tiny_bc("bc3")``
+    with the plain inlet and exit conditions) under the repeating-stage
+    condition (IF_REPEAT = 1, multall-open-18.3.f:5323-5331, 12085-12115):
+    every 10 steps the inlet stagnation pressure, yaw angle and tangential
+    velocity are relaxed by 0.05 towards the exit profiles."""
+    spec = deck_tiny(ilos=10, nrows=2, nsteps=nsteps)
+    rot, sta = spec.rows
+    rot.name, rot.nblades, rot.alpha_in, rot.alpha_out, rot.rpm = "ROTOR", 48, -45.0, -20.0, 6000.0
+    sta.name, sta.nblades, sta.alpha_in, sta.alpha_out, sta.rpm = "STATOR", 40, 30.0, 0.0, 0.0
+    rot.pguess = (1.0e5, 0.97e5, 0.93e5, 0.92e5)
+    sta.pguess = (0.92e5, 0.91e5, 0.90e5, 0.90e5)
+    spec.pdown_hub = spec.pdown_tip = 0.9e5
+    spec.repeat = (10, 0.05)
+    spec.title = "TINY STAGE ILOS=10 REPEATING STAGE NINMOD=10 RFINBC=0.05"
     return spec
 
 

@@ -22,7 +22,7 @@ N3 = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT", "
 JK = ["R", "X", "RAVG_CELL", "ABT", "WRABT", "UBLADE", "DX", "DR", "DS", "SMERID", "RT_THICK"]
 IK = ["FTL", "FBL", "FTR", "FBR"]
 VI = ["FP", "FU", "FD"]
-VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD", "VM1"]
+VK = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD", "VM1", "BS", "BR"]
 VJ = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 RN = ["XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM", "XLLIM_DWN", "XLLIM_UP", "ROUGH_H", "ROUGH_T", "ROUGH_L",
       "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "CHORD", "SEP_THIK", "SEP_DRAG"]
@@ -40,7 +40,8 @@ SCALARS = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NIB1", "NJB1", "NKB1", "N
            "FBLK1", "FBLK2", "FBLK3", "RFMIX", "FEXTRAP", "FSMTHB", "FANGLE", "RFIN", "SFEXIT", "REYNO",
            "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL", "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP",
            "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS", "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0",
-           "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "FLOWIN", "RFLOW"]
+           "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "FLOWIN", "RFLOW",
+           "NINMOD", "RFINBC", "PO_IN_MID", "VT_IN_MID"]
 
 _lib = None
 
@@ -108,7 +109,8 @@ def load_deck(path: str, xlength_on_host: bool = True) -> Dict:
                  "FANGLE", "RFIN", "SFEXIT", "REYNO", "RF_VIS", "FTRANS", "TURBVIS_LIM", "PRANDTL",
                  "YPLUSWALL", "PDOWN_HUB", "PDOWN_TIP", "FACMIXUP", "GA1", "FGA", "RCP", "CV", "RGAS",
                  "VSOUND", "VORT_MAX", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0", "FAC_ST1",
-                 "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "FLOWIN", "RFLOW"):
+                 "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "FLOWIN", "RFLOW",
+                 "RFINBC", "PO_IN_MID", "VT_IN_MID"):
             continue
         sc[n] = int(sc[n])
     im, jm, km = sc["IM"], sc["JM"], sc["KM"]

@@ -21,7 +21,7 @@ EXPORTS = ["mh_create", "mh_destroy", "mh_last_error", "mh_array_id", "mh_scalar
            "mh_array_is_int", "mh_set_array", "mh_get_array", "mh_set_array_strided",
            "mh_get_array_strided", "mh_set_scalar", "mh_get_scalar", "mh_loss_init", "mh_set_xlength", "mh_start", "mh_step",
            "mh_integrals", "mh_sync", "mh_profile_enable", "mh_profile_report", "mh_step_batch",
-           "mh_request_stop", "mh_set_xlength_noalloc", "mh_eficool"]
+           "mh_request_stop", "mh_set_xlength_noalloc", "mh_eficool", "mh_mix_bconds"]
 
 
 class MhDims(ctypes.Structure):
@@ -92,6 +92,7 @@ def load(path: str = LIB_PATH):
     L.mh_step_batch.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(MhResid), ctypes.POINTER(i)]
     L.mh_integrals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
     L.mh_eficool.argtypes = [vp, ctypes.POINTER(MhPerf), i, ctypes.POINTER(i)]
+    L.mh_mix_bconds.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ll]
     L.mh_sync.argtypes = [vp]
     L.mh_profile_enable.argtypes = [vp, i]
     L.mh_profile_report.argtypes = [vp, ctypes.c_char_p, i]
@@ -205,6 +206,18 @@ class Solver:
         n = ctypes.c_int(0)
         self._check(self.L.mh_eficool(self.h, out, cap, ctypes.byref(n)), "mh_eficool")
         return [out[i].as_dict() for i in range(n.value)]
+
+    MIXB_NAMES = ["PO", "P", "TO", "TS", "VT", "VM", "YAW", "PITCH"]
+
+    def mix_bconds(self) -> np.ndarray:
+        """MIX_BCONDS profiles (mh_mix_bconds): [plane, quantity, K] for
+        K = 1..KM, then the plane's mass average at index KM; planes: inlet,
+        then each row's JMIX; quantities in MIXB_NAMES order."""
+        np_, km = int(self.dims.nrows) + 1, int(self.dims.km)
+        out = np.empty(np_ * 8 * (km + 1), np.float64)
+        self._check(self.L.mh_mix_bconds(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size),
+                    "mh_mix_bconds")
+        return out.reshape(np_, 8, km + 1)
 
     def profile(self, on: bool):
         self._check(self.L.mh_profile_enable(self.h, 1 if on else 0), "mh_profile_enable")

@@ -43,7 +43,7 @@
   X(WTIP) X(VLIMIT) X(ROLIMIT) X(AVG_CHG) X(AVG_BLK) X(STEP1) X(STEP2) X(STEPSBK)           \
   X(XLLIM_I1) X(XLLIM_IM) X(XLLIM_K1) X(XLLIM_KM) X(XLLIM_DWN) X(XLLIM_UP) X(CHORD)         \
   X(ROUGH_H) X(ROUGH_T) X(ROUGH_L) X(ROUGH_U) X(FSTURB) X(TURBVIS_DAMP) X(FILAM) X(FITURB) X(FIWAKE) X(FIUP) X(FKLAM) X(FKTURB) X(VISCOSY)  \
-  X(REYNOLDS) X(ROVEXIT) X(SEP_THIK) X(SEP_DRAG)
+  X(REYNOLDS) X(ROVEXIT) X(SEP_THIK) X(SEP_DRAG) X(BS) X(BR)
 #define I1D(X) X(IB1) X(IB2) X(KB1) X(KB2) X(NBLADE) X(IND) X(INDLE) X(INDMIX) X(INDTE)   \
   X(NROW) X(JSBLK) X(JB1) X(JB2) X(JSTART) X(JMIX) X(JLE) X(JTE) X(JLED) X(KTIPS) X(KTIPE) \
   X(KS1) X(KS2) X(NOUT) X(JTRAN_I1) X(JTRAN_IM) X(JTRAN_K1) X(JTRAN_KM) X(IF_CUSP) X(NUP_I1) X(NUP_IM) \
@@ -56,11 +56,12 @@
   X(PRANDTL) X(YPLUSWALL) X(TURBVIS_LIM) X(VISLAM) X(FTCOND) X(PREFF) X(ROLIM) X(ROMIN)       \
   X(RO_REF) X(EAVG) X(EMAX) X(ECONT) X(VRMS) X(VMAX) X(VREF) X(TCOOL_MIN) X(SFEX1) X(SFEXH)   \
   X(FLOWRAT) X(YPLAM) X(YPTURB) X(CFWALL) X(FAC_STMIX) X(FAC_ST0) X(FAC_ST1) X(FAC_ST2)       \
-  X(FAC_ST3) X(FAC_SFVIS) X(FAC_VORT) X(FAC_PGRAD) X(VORT_MAX) X(FLOWIN) X(RFLOW)
+  X(FAC_ST3) X(FAC_SFVIS) X(FAC_VORT) X(FAC_PGRAD) X(VORT_MAX) X(FLOWIN) X(RFLOW) X(RFINBC) X(PO_IN_MID)   \
+  X(VT_IN_MID)
 #define ISCAL(X) X(IM) X(JM) X(KM) X(NROWS) X(IMID) X(KMID) X(NSTEP) X(NSTEPS_MAX) X(NCHANGE) \
   X(NRSMTH) X(NSFEXIT) X(IN_PRESS) X(IN_VTAN) X(IN_VR) X(IPOUT) X(IFMIX) X(IR) X(JR) X(KR)    \
   X(ILOS) X(NLOS) X(NMIXUP) X(NIB1) X(NJB1) X(NKB1) X(NIB2) X(NJB2) X(NKB2) X(NSBLK) X(IFEND) \
-  X(IFPRINT) X(ITIMST) X(IF_RESTART) X(IFGAS) X(IN_FLOW) X(IF_REPEAT) X(IBOUND)
+  X(IFPRINT) X(ITIMST) X(IF_RESTART) X(IFGAS) X(IN_FLOW) X(IF_REPEAT) X(IBOUND) X(NINMOD)
 
 struct orc {
 #define X(n) float *n;
@@ -79,6 +80,8 @@ struct orc {
   float *CHECK_FLOW, *BLADE_FLOW; /* LOOP's SAVEd locals (mf:3672-3673) */
   float *B1CHG, *B2CHG, *SBCHG;   /* TSTEP's block accumulators (mf:5482) */
 };
+
+static void orc_repeating_stage(orc_t *o);
 
 orc_t *orc_create(void) { return (orc_t *)calloc(1, sizeof(orc_t)); }
 void orc_destroy(orc_t *o) {
@@ -138,8 +141,8 @@ int orc_start(orc_t *o) {
 #define X(n) if (!o->n) return -100;
   F3D(X) FJK(X) FIK(X) FIJ(X)
 #undef X
-  if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 ||
-      o->IF_REPEAT != 0 || o->IF_RESTART != 0)
+  if (o->IFGAS != 0 || o->ITIMST != 3 || o->IM <= 2 || o->KM <= 2 || o->IF_RESTART != 0 ||
+      (o->IF_REPEAT != 0 && (o->NINMOD < 1 || !o->BS || !o->BR)))
     return -2;
   if (o->ILOS != 0 && o->ILOS != 10 && o->ILOS < 100) return -3;
   for (int NR = 1; o->IF_CUSP && NR <= o->NROWS; NR++)
@@ -2766,6 +2769,8 @@ static void step_once(orc_t *o) {
     o->EAVG = EAVG / (float)(IM * JMM1 * KM);
     o->EMAX = EMAX;
     o->VREF = VREF;
+    /* the repeating-stage condition, mf:5323-5331 (N = NSTEP) */
+    if (o->IF_REPEAT > 0 && o->NSTEP % o->NINMOD == 0) orc_repeating_stage(o);
     if (o->IFPRINT == 1 || o->IFEND == 1) {
       if (o->ILOS == 10) orc_loss(o);
       if (o->ILOS >= 100 && o->ILOS < 200) orc_new_loss(o);
@@ -2773,6 +2778,98 @@ static void step_once(orc_t *o) {
     }
     orc_stepup(o, 0.25f);
   }
+}
+
+/* MASS_AVG (mf:11729-11760): the positive-mass-flow average of the node
+   field F over each cell line K of plane J, moved to the nodes by the
+   running average and the end extrapolations; the plane average. */
+static void mass_avg(orc_t *o, int J, const float *F, float *SUMFUN, float *AVGALL) {
+  const int IM = o->IM, KM = o->KM;
+#define F3(i, j, k) F[((i) - 1) + (size_t)IM * (((j) - 1) + (size_t)o->JM * ((k) - 1))]
+  float SUMALL = 0.0f, FLOWALL = 0.0f;
+  for (int K = 1; K <= KM - 1; K++) {
+    float SUMAS = 0.0f;
+    SUMFUN[K] = 0.0f;
+    for (int I = 1; I <= IM - 1; I++) {
+      float DFLOW = -A3(FLOWX, I, J, K);
+      if (DFLOW < 0.0f) DFLOW = 0.0f;
+      SUMAS = SUMAS + DFLOW;
+      SUMFUN[K] = SUMFUN[K] + DFLOW * (F3(I, J, K) + F3(I + 1, J, K) + F3(I + 1, J, K + 1) + F3(I, J, K + 1)) * 0.25f;
+    }
+    SUMALL = SUMALL + SUMFUN[K];
+    FLOWALL = FLOWALL + SUMAS;
+    SUMFUN[K] = SUMFUN[K] / SUMAS;
+  }
+#undef F3
+  for (int K = 2; K <= KM - 1; K++) SUMFUN[K] = 0.5f * (SUMFUN[K] + SUMFUN[K - 1]);
+  SUMFUN[1] = 2.0f * SUMFUN[2] - SUMFUN[3];
+  SUMFUN[KM] = 2.0f * SUMFUN[KM - 1] - SUMFUN[KM - 2];
+  *AVGALL = SUMALL / FLOWALL;
+}
+
+/* MIX_BCONDS(1) (mf:11906-12080) over the inlet and every row's JMIX, then
+   NEWBCONDS (mf:12085-12115) from the last plane's profiles.  STORE holds
+   each node quantity on the plane, as in the reference. */
+static void orc_repeating_stage(orc_t *o) {
+  const int IM = o->IM, KM = o->KM;
+  const float PI = 3.14159265f, DEGRAD = PI / 180.f, RADDEG = 180.f / PI;
+  float *po = calloc(KM + 1, sizeof(float)), *f = calloc(KM + 1, sizeof(float));
+  float *vt = calloc(KM + 1, sizeof(float)), *yaw = calloc(KM + 1, sizeof(float));
+  float avg;
+  for (int NR = 1; NR <= o->NROWS + 1; NR++) {
+    const int J = NR == 1 ? 1 : A1(JMIX, NR - 1);
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++) {
+        const float VX = A3(VX, I, J, K), VT = A3(VT, I, J, K), VR = A3(VR, I, J, K);
+        const float D = .5f * (VX * VX + VT * VT + VR * VR);
+        const float TSTAG = A3(HO, I, J, K) / o->CP;
+        float TS = TSTAG - D / o->CP;
+        if (TS < 1.f) TS = 1.f;
+        A3(STORE, I, J, K) = A3(P, I, J, K) * powf(TSTAG / TS, o->GA / (o->GA - 1));
+      }
+    mass_avg(o, J, o->STORE, po, &avg);
+    mass_avg(o, J, o->P, f, &avg);
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++) A3(STORE, I, J, K) = A3(HO, I, J, K) / o->CP;
+    mass_avg(o, J, o->STORE, f, &avg);
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++) {
+        const float VX = A3(VX, I, J, K), VT = A3(VT, I, J, K), VR = A3(VR, I, J, K);
+        const float D = .5f * (VX * VX + VT * VT + VR * VR);
+        A3(STORE, I, J, K) = (A3(HO, I, J, K) - D) / o->CP;
+      }
+    mass_avg(o, J, o->STORE, f, &avg);
+    mass_avg(o, J, o->VT, vt, &avg);
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++)
+        A3(STORE, I, J, K) = sqrtf(A3(VX, I, J, K) * A3(VX, I, J, K) + A3(VR, I, J, K) * A3(VR, I, J, K));
+    mass_avg(o, J, o->STORE, f, &avg);
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++) {
+        const float VMER = sqrtf(A3(VX, I, J, K) * A3(VX, I, J, K) + A3(VR, I, J, K) * A3(VR, I, J, K));
+        A3(STORE, I, J, K) = atan2f(A3(VT, I, J, K), VMER) * RADDEG;
+      }
+    mass_avg(o, J, o->STORE, yaw, &avg);
+    for (int I = 1; I <= IM; I++)
+      for (int K = 1; K <= KM; K++) {
+        const float VMER = sqrtf(A3(VX, I, J, K) * A3(VX, I, J, K) + A3(VR, I, J, K) * A3(VR, I, J, K));
+        A3(STORE, I, J, K) = atan2f(A3(VR, I, J, K), VMER) * RADDEG;
+      }
+    mass_avg(o, J, o->STORE, f, &avg);
+  }
+  const float RFINBC1 = 1.0f - o->RFINBC;
+  const int KMID = o->KMID;
+  for (int K = 1; K <= KM; K++) {
+    A1(PO1, K) = RFINBC1 * A1(PO1, K) + o->RFINBC * (o->PO_IN_MID + po[K] - po[KMID]);
+    A1(BS, K) = RFINBC1 * A1(BS, K) + o->RFINBC * yaw[K];
+    A1(VTIN, K) = RFINBC1 * A1(VTIN, K) + o->RFINBC * (o->VT_IN_MID + vt[K] - vt[KMID]);
+    A1(RPO1, K) = 1.f / A1(PO1, K);
+    A1(BSSIN, K) = sinf(A1(BS, K) * DEGRAD);
+    A1(BSCOS, K) = cosf(A1(BS, K) * DEGRAD);
+    A1(BRSIN, K) = sinf(A1(BR, K) * DEGRAD);
+    A1(BRCOS, K) = cosf(A1(BR, K) * DEGRAD);
+  }
+  free(po); free(f); free(vt); free(yaw);
 }
 
 int orc_step(orc_t *o, int n) {

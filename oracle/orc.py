@@ -35,7 +35,7 @@ WALL_STATE_DEFAULTS = {"DPDS_CELL": ("N", 0.0), "YPLUS_I1": ("JK", 25.0), "YPLUS
 IK2 = ["FTL", "FBL", "FTR", "FBR"]
 VEC_I = ["FP", "FU", "FD"]
 VEC_K = ["FR", "FKU", "FKD", "PO1", "TO1", "RPO1", "BSSIN", "BSCOS", "BRSIN", "BRCOS", "VTIN", "PD",
-         "PLIM", "VM1"]
+         "PLIM", "VM1", "BS", "BR"]
 VEC_J = ["FACUP", "FACDWN", "WRAD", "WHUB", "WTIP"]
 VEC_NR = ["VLIMIT", "ROLIMIT", "AVG_CHG", "AVG_BLK", "XLLIM_I1", "XLLIM_IM", "XLLIM_K1", "XLLIM_KM",
           "XLLIM_DWN", "XLLIM_UP", "CHORD", "ROUGH_H", "ROUGH_T", "ROUGH_L", "ROUGH_U", "FSTURB", "TURBVIS_DAMP", "VISCOSY", "REYNOLDS",
@@ -52,11 +52,12 @@ SCALARS_F = ["CFL", "DAMP", "DAMPIN", "MACHLIM", "F_PDOWN", "CONLIM", "SFXIN", "
              "PDOWN_TIP", "FEXTRAP", "RFMIX", "FANGLE", "CP", "GA", "RGAS", "CV", "FGA", "GA1", "RCP",
              "VSOUND", "FACMIXUP", "REYNO", "RF_VIS", "FTRANS", "PRANDTL", "YPLUSWALL",
              "TURBVIS_LIM", "VISLAM", "FTCOND", "YPLAM", "YPTURB", "CFWALL", "FAC_STMIX", "FAC_ST0",
-             "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "FLOWIN", "RFLOW"]
+             "FAC_ST1", "FAC_ST2", "FAC_ST3", "FAC_SFVIS", "FAC_VORT", "FAC_PGRAD", "VORT_MAX", "FLOWIN", "RFLOW",
+             "RFINBC", "PO_IN_MID", "VT_IN_MID"]
 SCALARS_I = ["IM", "JM", "KM", "NROWS", "IMID", "KMID", "NSTEPS_MAX", "NCHANGE", "NRSMTH", "NSFEXIT",
              "IN_PRESS", "IN_VTAN", "IN_VR", "IPOUT", "IFMIX", "IR", "JR", "KR", "ILOS", "NLOS",
              "NMIXUP", "NIB1", "NJB1", "NKB1", "NIB2", "NJB2", "NKB2", "NSBLK", "ITIMST",
-             "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IBOUND", "IRBB", "JRBB", "KRBB"]
+             "IF_RESTART", "IFGAS", "IN_FLOW", "IF_REPEAT", "IBOUND", "IRBB", "JRBB", "KRBB", "NINMOD"]
 
 
 

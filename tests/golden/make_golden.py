@@ -62,6 +62,7 @@ CASES = {
     "tiny3_sup": lambda n: deckgen.deck_tiny_supersonic(nsteps=n),
     "tiny3_nout": lambda n: deckgen.deck_tiny_nout(nsteps=n),
     "tiny3_cusp2": lambda n: deckgen.deck_tiny_cusp2(nsteps=n),
+    "tiny2_rep": lambda n: deckgen.deck_tiny_repeat(nsteps=n),
 }
 # decks wider than the main reference build (IM > 65) run on its wide build
 VARIANT = {"tiny3_wide": "fp32w"}

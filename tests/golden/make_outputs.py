@@ -6,6 +6,8 @@ its state:
 
   <case>.grid_out    the plotting grid file SETUP writes (mf:6585-6599)
   <case>.stage_log   stage.log, the convergence lines of every check (mf:5437-5438)
+  <case>.mixbconds   the mass-averaged profiles MIX_BCONDS writes at the end
+                     of the run (mf:5445, 11906-12080)
 
 Usage: python tests/golden/make_outputs.py [case ...]
 """
@@ -31,6 +33,7 @@ def make(case: str) -> None:
         refdump.run_reference(txt, "fp32", workdir=wd)
         shutil.copy(os.path.join(wd, "grid_out"), os.path.join(HERE, f"{case}.grid_out"))
         shutil.copy(os.path.join(wd, "stage.log"), os.path.join(HERE, f"{case}.stage_log"))
+        shutil.copy(os.path.join(wd, "mixbconds"), os.path.join(HERE, f"{case}.mixbconds"))
     finally:
         shutil.rmtree(wd, ignore_errors=True)
     print(f"{case}: grid_out and stage.log kept", flush=True)

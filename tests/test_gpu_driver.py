@@ -3,7 +3,7 @@ with LOOP replaced by the library, the deck on standard input) on a
 three-row mixing-length deck: it reads and sets up the deck (the wall
 distances formed on the device by mh_set_xlength), marches the deck's 100
 steps in chunks of 5 and writes flow_out in the reference's record layout
-(multall-open-18.3.f:5951-5976).  The density, energy and angular momentum
+(multall-open-18.3.f:5951-5976), grid_out, stage.log and mixbconds.  The density, energy and angular momentum
 it writes match the reference's own state after 100 steps within the
 100-step bound of the parity tests.  The angular-momentum record is ROVT as
 the reference leaves it: refreshed from RORVT at the start of the last step
@@ -80,3 +80,21 @@ def test_driver_program_marches_a_deck(tmp_path):
         same_at += la == lb
     print(f"stage.log: {len(mine)} lines, EMAX location equal on {same_at}")
     assert same_at >= len(want) - 2
+    # mixbconds: MIX_BCONDS's profiles at the inlet and every JMIX in the
+    # reference's layout; each number within twice its printed resolution
+    # or 1e-4 of its value
+    tm = (tmp_path / "mixbconds").read_text().split()
+    tw = open(os.path.join(GOLDEN, "tiny3_ml.mixbconds")).read().split()
+    assert len(tm) == len(tw), (len(tm), len(tw))
+    worst = 0.0
+    for a, b in zip(tm, tw):
+        try:
+            y = float(b)
+        except ValueError:
+            assert a == b, (a, b)
+            continue
+        res = 10.0 ** -(len(b.split(".")[1]) if "." in b else 0)
+        err = abs(float(a) - y)
+        worst = max(worst, err / (2 * res + 1e-4 * abs(y)))
+        assert err <= 2 * res + 1e-4 * abs(y), (a, b)
+    print(f"mixbconds: {len(tw)} tokens, worst error {worst:.2f} of its bound")

@@ -42,7 +42,7 @@ INVISCID = ["tiny1_inv", "tiny3_inv", "tiny3_sup"]
 # steps inside the run (viscous refreshes at NOUT)
 MIXING_LENGTH = ["tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_exit", "tiny3_wide", "tiny3_sss", "tiny3_flow2",
                  "tiny3_flow3", "tiny3_ml3", "tiny3_gap", "tiny3_bc1", "tiny3_bc2", "tiny2_bc3", "tiny3_nout",
-                 "tiny3_cusp2"]
+                 "tiny3_cusp2", "tiny2_rep"]
 # Spalart-Allmaras decks: rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing, mixing-length relaxation source; the Rotor 37 test
 # deck's options (Shih wall functions with the pressure-gradient term,

@@ -29,7 +29,7 @@ VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE", "DRO", "DROVX", "DROVR", "DRORVT",
 # Spalart-Allmaras decks below
 CASES = ["tiny1_inv", "tiny3_inv", "tiny3_ml", "tiny3_ml2", "tiny1_mlw", "tiny3_sa", "tiny3_sa2", "tiny3_exit", "tiny3_wide",
          "tiny3_nl", "tiny3_nl2", "tiny3_sss", "tiny3_flow2", "tiny3_flow3", "tiny3_ml3", "tiny3_nl3", "tiny3_sa3",
-         "tiny3_gap", "tiny3_bc1", "tiny3_bc2", "tiny2_bc3", "tiny3_sup", "tiny3_nout", "tiny3_cusp2"]
+         "tiny3_gap", "tiny3_bc1", "tiny3_bc2", "tiny2_bc3", "tiny3_sup", "tiny3_nout", "tiny3_cusp2", "tiny2_rep"]
 # Spalart-Allmaras decks (rotor tip gap; power-law viscosity, rough walls,
 # shear-free casing and the mixing-length relaxation source) also pin the
 # turbulence state
