@@ -161,15 +161,18 @@ def timed_form(st: dict, mixed: bool, warmup: int, steps: int) -> tuple:
 
 def check_vs_reference(st: dict, mixed: bool) -> dict:
     """The workload marched 100 steps from the reference's set-up and
-    compared with the reference's own 100-step summary of this deck
-    (tests/golden/config1_summary100_fp64.npz, the reference's FP64 build):
-    the largest difference of any pitchwise line mean, in each conserved
-    variable's scale."""
+    compared with the reference's own 100-step summaries of this deck
+    (tests/golden/config1_summary100_fp64.npz and config1_summary100.npz,
+    its FP64 and FP32 builds): the largest difference of any pitchwise line
+    mean from the FP64 build's, in each conserved variable's scale, beside
+    the FP32 build's own distance from it."""
     from multall_hip.runtime import Solver
     path = os.path.join(GOLDEN, "config1_summary100_fp64.npz")
-    if not os.path.exists(path):
+    path32 = os.path.join(GOLDEN, "config1_summary100.npz")
+    if not os.path.exists(path) or not os.path.exists(path32):
         return None
     ref = np.load(path, allow_pickle=False)
+    ref32 = np.load(path32, allow_pickle=False)
     st = _copy_state(st)
     st["scalars"]["NSTEPS_MAX"] = 100
     g = Solver(st, exact_order=False)
@@ -177,15 +180,17 @@ def check_vs_reference(st: dict, mixed: bool) -> dict:
     g.loss_init()
     g.start()
     r = g.step(110)
-    worst = 0.0
+    worst = worst32 = 0.0
     for v in ("RO", "ROVX", "ROVR", "RORVT", "ROE"):
         lines = g.get(v).astype(np.float64).mean(axis=2)
         want = ref[f"{v}_lines"].astype(np.float64)
         scale = np.abs(ref[f"{v}_plane"]).max() if v in ("RO", "ROE", "RORVT") else np.sqrt(
             ref["ROVX_plane"].astype(np.float64) ** 2 + ref["ROVR_plane"].astype(np.float64) ** 2).max()
         worst = max(worst, float(np.abs(lines - want).max() / scale))
+        worst32 = max(worst32, float(np.abs(ref32[f"{v}_lines"].astype(np.float64) - want).max() / scale))
     g.close()
-    return {"steps": int(r["nstep"]), "max_line_mean_err_vs_fp64_reference": worst}
+    return {"steps": int(r["nstep"]), "max_line_mean_err_vs_fp64_reference": worst,
+            "fp32_reference_vs_fp64_reference": worst32}
 
 
 CONVERGED_DECK = os.path.join(GOLDEN, "decks", "converged_r37samods.dat")
