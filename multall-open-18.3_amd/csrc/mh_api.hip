@@ -1147,11 +1147,14 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
   // With lanes, the energy equation (its fluxes read only the mass fluxes,
-  // HO and QSOURCE) runs on lane 1 beside continuity and the turbulence
-  // transport, and the three momentum equations (fluxes from the step's
-  // primitives, PEFF and the body forces) on lanes 0, 1 and 2.  Each lane
-  // has its own flux and TSTEP scratch; the boundary conditions and the
-  // limiters, which read several variables, wait for all of them.
+  // HO and QSOURCE) runs on lane 1 and the turbulence transport (its fluxes
+  // read only the mass fluxes, the previous step's kinematic viscosity and
+  // T_SOURCE) on lane 2, both beside continuity; only the kinematic value
+  // that closes the transport needs the new density.  The three momentum
+  // equations (fluxes from the step's primitives, PEFF and the body forces)
+  // run on lanes 0, 1 and 2.  Each lane has its own flux and TSTEP scratch;
+  // the boundary conditions and the limiters, which read several variables,
+  // wait for all of them.
   const bool lanes = c->nlanes == kLanes && c->lanes_on && !p.EXACT && c->scal[MH_S_LANES] != 0.0;
   if (lanes) {
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -1161,6 +1164,14 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LCHK("flux_energy");
     if ((r = tstep(c, 1, d.ROE, d.DROE, 3, q, CTL_SKIP))) return r;
     HIPCHK(hipEventRecord(c->ev[1], c->lanes[0].stream));
+    if (p.ILOS >= 200) {
+      // transport of the turbulent viscosity, mf:4275-4335
+      HIPCHK(hipStreamWaitEvent(c->lanes[1].stream, c->ev[0], 0));
+      const Dev d2 = lane_dev(c, 2, CTL_SKIP);
+      LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->lanes[1].stream, g, d2);
+      if ((r = tstep(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
+      HIPCHK(hipEventRecord(c->ev[5], c->lanes[1].stream));
+    }
   }
   if ((r = tstep(c, 0, d.RO, d.DRO, 1, q, CTL_SKIP))) return r;
   if (p.ILOS >= 100 && p.ILOS < 200) {
@@ -1169,10 +1180,15 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LCHK("cellavg");
   }
   if (p.ILOS >= 200) {
-    // transport of the turbulent viscosity, mf:4260-4351
+    // cell-average density (mf:4260-4270), the transport (mf:4275-4335) in
+    // order here without lanes, then the lower limit and kinematic value
     LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
-    LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
-    if ((r = tstep(c, 0, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
+    if (lanes) {
+      HIPCHK(hipStreamWaitEvent(c->stream, c->ev[5], 0));
+    } else {
+      LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
+      if ((r = tstep(c, 0, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
+    }
     LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
     LCHK("sa_transport");
   }

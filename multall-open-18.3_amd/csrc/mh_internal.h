@@ -275,5 +275,5 @@ struct mh_ctx {
   int nlanes = 0;
   bool lanes_on = true;  // off while mh_step_batch overlaps whole contexts instead
   mh::Lane lanes[mh::kLanes];
-  hipEvent_t ev[5] = {};
+  hipEvent_t ev[6] = {};
 };

@@ -58,3 +58,26 @@ def test_stop_request_ends_at_the_next_tenth_step():
     assert ra == rb
     for v in VARS:
         assert np.array_equal(a.get(v).view(np.uint32), b.get(v).view(np.uint32)), v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["tiny3_sa", "tiny3_sa2", "tiny3_nl", "tiny3_ml"])
+def test_concurrent_equations_equal_one_stream(case):
+    """Fast mode with the equations on three streams (LANES = 1, the default:
+    energy and the turbulence transport beside continuity, then the three
+    momentum equations side by side) against all of them in order on one
+    stream (LANES = 0), 100 steps, bit for bit."""
+    from multall_hip.runtime import Solver
+    runs = []
+    for lanes in (1, 0):
+        st = load_state(case)
+        st["scalars"]["NSTEPS_MAX"] = 100
+        g = Solver(st, exact_order=False)
+        g.set_scalar("LANES", lanes)
+        g.start()
+        runs.append((g, g.step(100)))
+    (a, ra), (b, rb) = runs
+    assert ra == rb and ra["nstep"] == 100
+    names = VARS + (["TRANS_DYN_VIS", "TRANS_KVIS"] if load_state(case)["scalars"]["ILOS"] >= 200 else [])
+    for v in names:
+        assert np.array_equal(a.get(v).view(np.uint32), b.get(v).view(np.uint32)), (case, v)
