@@ -46,9 +46,9 @@ __global__ void k_mg_apply(Geo, Dev);
 __global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int, int);
 __global__ void k_damp_apply(Geo, Dev);
 __global__ void k_scatter(Geo, Dev, const float *, float *);
-__global__ void k_smooth_plane(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+__global__ void k_smooth_plane(Geo, Dev, EqSet, Params, StepScalars, int);
 #define SMOOTH_REG_DECL(L, NW) \
-  __global__ void k_smooth_reg##L##w##NW(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+  __global__ void k_smooth_reg##L##w##NW(Geo, Dev, EqSet, Params, StepScalars, int);
 SMOOTH_REG_DECL(1, 8)
 SMOOTH_REG_DECL(2, 8)
 SMOOTH_REG_DECL(4, 8)
@@ -84,19 +84,19 @@ __global__ void k_loss_pitch(Geo, Dev, Params, float);
 __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
 __global__ void k_sa_addstore(Geo, Dev, float *, int);
-__global__ void k_resid_line(Geo, Dev, float *, Params, MgFast, int, int, int);
-__global__ void k_resid_line_bf(Geo, Dev, unsigned short *, Params, MgFast, int, int, int);
+__global__ void k_resid_line(Geo, Dev, EqSet, Params, MgFast, int, int, int);
+__global__ void k_resid_line_bf(Geo, Dev, EqSet, Params, MgFast, int, int, int);
 __global__ void k_to_bf16(const float *, unsigned short *, unsigned);
 __global__ void k_from_bf16(const unsigned short *, float *, unsigned);
-__global__ void k_mg_levels(Geo, MgFast, const int *, const int *, const int *, const int *, float *, float *,
-                            float *);
-__global__ void k_mg_apply_lines(Geo, Dev, double *);
+__global__ void k_mg_levels(Geo, MgFast, EqSet, const int *, const int *, const int *, const int *);
+__global__ void k_mg_apply_lines(Geo, Dev, EqSet);
+__global__ void k_flux_eq(Geo, Dev, EqSet);
 __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
 __global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
 __global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
-__global__ void k_row_avg(Geo, Dev, const double *, const int *, const int *);
+__global__ void k_row_avg(Geo, Dev, EqSet, const int *, const int *);
 #define DAMP_SCATTER_DECL(NC, TK) \
-  __global__ void k_damp_scatter##NC##_k##TK(Geo, Dev, Params, const float *, float *, int, int);
+  __global__ void k_damp_scatter##NC##_k##TK(Geo, Dev, EqSet, Params, int, int);
 DAMP_SCATTER_DECL(1, 2)
 DAMP_SCATTER_DECL(1, 4)
 DAMP_SCATTER_DECL(1, 8)
@@ -601,14 +601,8 @@ int build_lists(mh_ctx *c) {
   return h2d(c, c->d.JMP, jmp.data(), jmp.size() * 4);
 }
 
-// TSTEP (mf:5470-5932) for one conserved variable.  The launch chain per
-// call: the fused line kernel (flux extrapolation, residual, first
-// mixing-plane average), the multigrid sums, the multigrid update (with the
-// damping line sums in fast mode), the row averages and second
-// mixing-plane averages, the fused damping + corner distribution, and
-// SMOOTH_VAR with D's boundary conditions.
 // The Dev a lane's kernels see: the context's arrays with the lane's own
-// flux and TSTEP scratch (lane 0 is the context's stream and scratch).
+// flux and TSTEP scratch (lane 0: the context's own scratch).
 Dev lane_dev(const mh_ctx *c, int lane, int gate) {
   Dev d = c->d;
   d.gate = gate;
@@ -619,7 +613,6 @@ Dev lane_dev(const mh_ctx *c, int lane, int gate) {
   d.B1CHG = l.B1CHG; d.B2CHG = l.B2CHG; d.SBCHG = l.SBCHG; d.AVG = l.AVG; d.RED = l.RED;
   return d;
 }
-hipStream_t lane_stream(const mh_ctx *c, int lane) { return lane == 0 ? c->stream : c->lanes[lane - 1].stream; }
 
 // The mixed form's BF16 copy of a history array (DRO, DROVX, DROVR, DRORVT,
 // DROE, DEL_DYNVIS), or null for any other array.
@@ -634,6 +627,22 @@ unsigned short *hist16(const mh_ctx *c, const float *a) {
   return i < 0 ? nullptr : c->h16[i];
 }
 
+// A variable of a batched TSTEP or flux launch on lane `lane`.
+EqLane eq_lane(const mh_ctx *c, int lane, float *D, float *DIFF, int ncall, int kind) {
+  const Dev d = lane_dev(c, lane, -1);
+  const MgFast &m = lane ? c->lanes[lane - 1].mgf : c->lists.mgf;
+  EqLane q{};
+  q.D = D;
+  q.DIFF = DIFF;
+  q.DIFF16 = hist16(c, DIFF);
+  q.XFLUX = d.XFLUX; q.TFLUX = d.TFLUX; q.RFLUX = d.RFLUX; q.SOURCE = d.SOURCE;
+  q.STORE = d.STORE; q.DSC = d.DSC; q.TMP = d.TMP;
+  q.B1CHG = d.B1CHG; q.B2CHG = d.B2CHG; q.SBCHG = d.SBCHG; q.AVG = d.AVG;
+  q.RED = d.RED; q.P = m.P; q.PT = m.PT;
+  q.ncall = ncall;
+  q.kind = kind;
+  return q;
+}
 // FP32 history <-> its BF16 copy (synchronous; outside the step path).
 int hist_convert(mh_ctx *c, int i, bool to_bf16) {
   float *h[kHist] = {c->d.DRO, c->d.DROVX, c->d.DROVR, c->d.DRORVT, c->d.DROE, c->d.DEL_DYNVIS};
@@ -645,7 +654,6 @@ int hist_convert(mh_ctx *c, int i, bool to_bf16) {
   return MH_OK;
 }
 
-// One TSTEP call (mf:5470-5932) on lane `lane`, gated on CTL slot `gate`.
 // ---------------------------------------------------------------------------
 // Tile-shape variants of the damping/corner-distribution kernel (TK node K
 // lines per workgroup) and of SMOOTH_VAR's register form (NW wavefronts per J
@@ -654,7 +662,7 @@ int hist_convert(mh_ctx *c, int i, bool to_bf16) {
 // is fastest depends on the grid (how many workgroups the launch has next to
 // the 256 CUs, how much of each tile is halo), so autotune() times them.
 struct ScatVariant {
-  void (*k)(Geo, Dev, Params, const float *, float *, int, int);
+  void (*k)(Geo, Dev, EqSet, Params, int, int);
   const char *name;
 };
 const ScatVariant &scat_variant(int imm1, int tk) {
@@ -668,7 +676,7 @@ const ScatVariant &scat_variant(int imm1, int tk) {
 dim3 scat_grid(const Geo &g, int tk) { return dim3((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + tk - 1) / tk); }
 
 struct SmoothVariant {
-  void (*k)(Geo, Dev, const float *, float *, Params, StepScalars, int, int);
+  void (*k)(Geo, Dev, EqSet, Params, StepScalars, int);
   const char *name;
   int nw;
 };
@@ -720,6 +728,19 @@ int autotune(mh_ctx *c) {
   HIPCHK(hipEventCreate(&e1));
   const Dev &d = c->d;
   const StepScalars s = step_scalars(c);
+  // the shape the step launches: three variables at once in the fast mode
+  // (the lanes' own scratch), one in the exact mode; the corner
+  // distribution reads the conserved variables and writes DSC, SMOOTH_VAR
+  // reads DSC and writes TMP in place of the variable
+  const int nz = c->nlanes == kLanes ? 3 : 1;
+  float *const vars[3] = {d.ROVX, d.RORVT, d.ROVR};
+  EqSet es{}, et{};
+  for (int i = 0; i < kMaxEq; i++) {
+    const int l = i < nz ? i : 0;
+    es.q[i] = eq_lane(c, l, vars[l], d.DROVX, 4, -1);
+    et.q[i] = es.q[i];
+    et.q[i].D = et.q[i].TMP;
+  }
   auto time_it = [&](auto launch) -> float {
     launch();
     (void)hipEventRecord(e0, c->stream);
@@ -735,8 +756,9 @@ int autotune(mh_ctx *c) {
     for (int tk : kScatTKs) {
       const ScatVariant &v = scat_variant(g.IMM1, tk);
       const float t = time_it([&] {
-        hipLaunchKernelGGL(v.k, scat_grid(g, tk), dim3(256), scatter_lds_bytes(g.IMM1, tk), c->stream, g, d, p,
-                           (const float *)d.DRO, d.DSC, 1, 0);
+        dim3 sg = scat_grid(g, tk);
+        sg.z = nz;
+        hipLaunchKernelGGL(v.k, sg, dim3(256), scatter_lds_bytes(g.IMM1, tk), c->stream, g, d, es, p, 1, 0);
       });
       if (t < best) best = t, best_tk = tk;
     }
@@ -746,8 +768,7 @@ int autotune(mh_ctx *c) {
     for (int nw : kSmoothWaves) {
       const SmoothVariant &v = smooth_variant(g.KM, nw);
       const float t = time_it([&] {
-        hipLaunchKernelGGL(v.k, dim3(g.JM), dim3(64 * v.nw), 0, c->stream, g, d, (const float *)d.DSC, d.TMP, p, s,
-                           1, 1);
+        hipLaunchKernelGGL(v.k, dim3(g.JM, 1, nz), dim3(64 * v.nw), 0, c->stream, g, d, et, p, s, 1);
       });
       if (t < best) best = t, best_nw = nw;
     }
@@ -763,21 +784,33 @@ int autotune(mh_ctx *c) {
   return MH_OK;
 }
 
-int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan &q, int gate) {
+// One TSTEP call (mf:5470-5932) for each of n variables whose time steps do
+// not read each other's results (continuity, energy and the turbulence
+// transport; the three momentum equations), on the context's stream, gated
+// on CTL slot `gate`.  The chain: the fused line kernel (flux extrapolation,
+// residual, first mixing-plane average, multigrid partial sums), the
+// multigrid sums, the multigrid update (with the damping line sums in fast
+// mode), the row averages, the fused damping + second mixing-plane average +
+// corner distribution, and SMOOTH_VAR with D's boundary conditions.  Each
+// variable has its own flux and TSTEP scratch (lane lanes[i]); the hot
+// kernels take all n in one launch (grid z), the others run per variable.
+// A transport variable (ncall = 2) comes last: it leaves after the damping
+// averages for its own smoothing and conditions.
+int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan &q, int gate) {
   const Geo &g = c->g;
-  Dev d = lane_dev(c, lane, gate);
-  hipStream_t st = lane_stream(c, lane);
+  const hipStream_t st = c->stream;
   const Params p = params_of(c);
   const StepScalars s = step_scalars(c);
   const dim3 grid = node_grid(g);
-  const dim3 lgrid((g.JM - 1 + 3) / 4, (g.KMM1 + 3) / 4), lblock(256);
+  const dim3 lblock(256);
   const int nmp = (g.NROWS - 1) * g.KMM1;
   const Lists &L = c->lists;
+  if (n < 1 || n > kMaxEq) return fail(c, MH_E_ARG, "tstep: 1 to 3 variables");
+  EqSet e{};
+  for (int i = 0; i < kMaxEq; i++) e.q[i] = eqs[i < n ? i : 0];
+  const Dev d0 = lane_dev(c, 0, gate);
+  auto dev_of = [&](int i) { return lane_dev(c, lanes[i], gate); };
   MgFast mgf = L.mgf;
-  if (lane) {
-    mgf.P = c->lanes[lane - 1].mgf.P;
-    mgf.PT = c->lanes[lane - 1].mgf.PT;
-  }
   mgf.gate = c->d.CTL + gate;
   const int *gp = c->d.CTL + gate;
   const bool damp = q.damp;
@@ -788,69 +821,83 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan 
   // fast mode: the residual kernel also leaves the multigrid partial sums
   const bool fastmg = mg && !p.EXACT && L.mgf_ok;
   const int kg = fastmg ? p.KR : 4;
-  if (c->mixed)
-    LAUNCH(k_resid_line_bf, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, st, g, d, hist16(c, DIFF), p,
-           mgf, kg, p.IR, fastmg ? 1 : 0);
-  else
-    LAUNCH(k_resid_line, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg), lblock, 0, st, g, d, DIFF, p, mgf, kg,
-           p.IR, fastmg ? 1 : 0);
+  const dim3 rgrid((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg, n);
+  if (c->mixed) LAUNCH(k_resid_line_bf, rgrid, lblock, 0, st, g, d0, e, p, mgf, kg, p.IR, fastmg ? 1 : 0);
+  else LAUNCH(k_resid_line, rgrid, lblock, 0, st, g, d0, e, p, mgf, kg, p.IR, fastmg ? 1 : 0);
   LCHK("resid_line");
   if (mg) {
     const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
-    if (p.EXACT) {
-      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, st, g, d, L.b1_ptr, L.b1_list, g.NIB1,
-             g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
-      LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, st, g, d, L.b2_ptr, L.b2_list, g.NIB2,
-             g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
-      LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, st, g, d, L.sb_ptr, L.sb_list,
-             g.NSBLK, d.SBCHG, 0, 0);
-    } else if (fastmg) {
-      LAUNCH(k_mg_levels, dim3((n1 + 255) / 256 + (n2 + 3) / 4 + g.NSBLK), dim3(256), 0, st, g, mgf,
-             L.b1_ptr, L.b1_list, L.sb_ptr, L.sb_list, d.B1CHG, d.B2CHG, d.SBCHG);
+    if (fastmg) {
+      LAUNCH(k_mg_levels, dim3((n1 + 255) / 256 + (n2 + 3) / 4 + g.NSBLK, n), dim3(256), 0, st, g, mgf, e,
+             L.b1_ptr, L.b1_list, L.sb_ptr, L.sb_list);
     } else {
-      LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, st, g, d, L.b1_ptr, L.b1_list, g.NIB1,
-             g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
-      LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, st, g, d, L.b2_ptr, L.b2_list, g.NIB2, g.NJB2 + 1,
-             g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
-      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 0);
-      LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, st, g, (const double *)d.RED, L.sb_ptr, L.sb_list,
-             g.NSBLK, d.SBCHG, 0, gp);
+      for (int i = 0; i < n; i++) {
+        const Dev d = dev_of(i);
+        if (p.EXACT) {
+          LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, st, g, d, L.b1_ptr, L.b1_list, g.NIB1,
+                 g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
+          LAUNCH(k_mg_sum_seq, dim3((n2 + 63) / 64), dim3(64), 0, st, g, d, L.b2_ptr, L.b2_list, g.NIB2,
+                 g.NJB2 + 1, g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
+          LAUNCH(k_range_sums_seq, dim3((g.NSBLK + 63) / 64), dim3(64), 0, st, g, d, L.sb_ptr, L.sb_list,
+                 g.NSBLK, d.SBCHG, 0, 0);
+        } else {
+          LAUNCH(k_mg_sum_seq, dim3((n1 + 127) / 128), dim3(128), 0, st, g, d, L.b1_ptr, L.b1_list, g.NIB1,
+                 g.NJB1 + 1, g.NKB1, p.IR, p.KR, d.B1CHG);
+          LAUNCH(k_mg_sum_wave, dim3(n2), dim3(64), 0, st, g, d, L.b2_ptr, L.b2_list, g.NIB2, g.NJB2 + 1,
+                 g.NKB2, p.IRBB, p.KRBB, d.B2CHG);
+          LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 0);
+          LAUNCH(k_range_sums, dim3(g.NSBLK), dim3(64), 0, st, g, (const double *)d.RED, L.sb_ptr, L.sb_list,
+                 g.NSBLK, d.SBCHG, 0, gp);
+        }
+      }
     }
     LCHK("mg_sums");
   }
-  if (fused_apply) LAUNCH(k_mg_apply_lines, lgrid, lblock, 0, st, g, d, d.RED);
-  else LAUNCH(k_mg_apply, grid, kNodeBlock, 0, st, g, d);
+  if (fused_apply) {
+    LAUNCH(k_mg_apply_lines, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + 3) / 4, n), lblock, 0, st, g, d0, e);
+  } else {
+    for (int i = 0; i < n; i++) LAUNCH(k_mg_apply, grid, kNodeBlock, 0, st, g, dev_of(i));
+  }
   LCHK("mgapply");
   if (p.NRSMTH > 0) {
     // STORE -> TMP -> DSC -> STORE (DSC is free until the corner
     // distribution), so the buffers keep their roles from call to call
-    for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0, 0);
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, p.RSMTH, 0, 1);
-      LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, d.STORE, p.RSMTH, 0, 2);
+    for (int i = 0; i < n; i++) {
+      const Dev d = dev_of(i);
+      for (int nsm = 0; nsm < p.NRSMTH; nsm++) {
+        LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.STORE, d.TMP, p.RSMTH, 0, 0);
+        LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, p.RSMTH, 0, 1);
+        LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, d.STORE, p.RSMTH, 0, 2);
+      }
     }
     LCHK("smooth_resid");
   }
   // row averages of |change| for the damping (mf:5677-5711)
   if (fused_apply) {
-    LAUNCH(k_row_avg, dim3(g.NROWS), dim3(1024), 0, st, g, d, (const double *)d.RED, L.row_ptr, L.row_list);
+    LAUNCH(k_row_avg, dim3(g.NROWS, n), dim3(1024), 0, st, g, d0, e, L.row_ptr, L.row_list);
     LCHK("row_avg");
   } else if (damp) {
-    if (p.EXACT) {
-      LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, st, g, d, L.row_ptr, L.row_list, g.NROWS,
-             d.AVG, 1, 1);
-    } else {
-      LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 1);
-      LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, st, g, (const double *)d.RED, L.row_ptr, L.row_list,
-             g.NROWS, d.AVG, 1, gp);
+    for (int i = 0; i < n; i++) {
+      const Dev d = dev_of(i);
+      if (p.EXACT) {
+        LAUNCH(k_range_sums_seq, dim3(1), dim3(64), 0, st, g, d, L.row_ptr, L.row_list, g.NROWS, d.AVG, 1, 1);
+      } else {
+        LAUNCH(k_jplane_sums, dim3(g.JM), dim3(256), 0, st, g, d, d.RED, 1);
+        LAUNCH(k_range_sums, dim3(g.NROWS), dim3(64), 0, st, g, (const double *)d.RED, L.row_ptr, L.row_list,
+               g.NROWS, d.AVG, 1, gp);
+      }
     }
     LCHK("damp_avg");
   }
-  if (ncall == 2) {
+  int nv = n;
+  while (nv > 0 && eqs[nv - 1].ncall == 2) nv--;
+  for (int i = nv; i < n; i++) {
     // the cell-centred turbulent viscosity (mf:5736-5777, then 8700):
     // add the (damped) change, carry the pitch average over each mixing
     // plane, smooth once with FAC_SFVIS*SFT, then the periodic/tip/
     // mixing-plane conditions with ILAST = IM-1
+    const Dev d = dev_of(i);
+    float *D = eqs[i].D;
     LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, st, g, d, D, damp ? 1 : 0);
     if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, st, g, d, p, D);
     // smoothing factor FAC_SFVIS * SFT, the step's SFT taken on the device
@@ -859,14 +906,16 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan 
     LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, D, p.FAC_SFVIS, 1, 2);
     LAUNCH(k_sa_dbc, dim3(g.KM, g.JM), dim3(64), 0, st, g, d, p, D);
     LCHK("sa_tstep");
-    return MH_OK;
   }
+  if (nv == 0) return MH_OK;
   // damping, second mixing-plane averages (mf:5786-5818) and the corner
   // distribution
   {
     const ScatVariant &v = scat_variant(g.IMM1, c->scat_tk);
-    LAUNCH_NAMED(v.k, v.name, scat_grid(g, c->scat_tk), dim3(256), scatter_lds_bytes(g.IMM1, c->scat_tk), st, g, d,
-                 p, (const float *)D, d.DSC, damp ? 1 : 0, nmp > 0 ? 1 : 0);
+    dim3 sg = scat_grid(g, c->scat_tk);
+    sg.z = nv;
+    LAUNCH_NAMED(v.k, v.name, sg, dim3(256), scatter_lds_bytes(g.IMM1, c->scat_tk), st, g, d0, e, p, damp ? 1 : 0,
+                 nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
   // SMOOTH_VAR: planes in registers where IM <= 64 and KM <= 128, the LDS
@@ -875,15 +924,20 @@ int tstep(mh_ctx *c, int lane, float *D, float *DIFF, int ncall, const StepPlan 
   int jreg = 1;  // first plane for the LDS form
   if (g.IM <= 64 && g.KM <= 128 && jexit > 1) {
     const SmoothVariant &v = smooth_variant(g.KM, c->smooth_nw);
-    LAUNCH_NAMED(v.k, v.name, dim3(jexit - 1), dim3(64 * v.nw), 0, st, g, d, (const float *)d.DSC, D, p, s, ncall,
-                 1);
+    LAUNCH_NAMED(v.k, v.name, dim3(jexit - 1, 1, nv), dim3(64 * v.nw), 0, st, g, d0, e, p, s, 1);
     jreg = jexit;
   }
   if (jreg <= g.JM)
-    LAUNCH(k_smooth_plane, dim3(g.JM - jreg + 1), dim3(512), c->lds_plane, st, g, d, (const float *)d.DSC, D, p,
-           s, ncall, jreg);
+    LAUNCH(k_smooth_plane, dim3(g.JM - jreg + 1, 1, nv), dim3(512), c->lds_plane, st, g, d0, e, p, s, jreg);
   LCHK("smooth_plane");
   return MH_OK;
+}
+
+// TSTEP for one variable on lane 0
+int tstep1(mh_ctx *c, float *D, float *DIFF, int ncall, const StepPlan &q, int gate) {
+  const EqLane e = eq_lane(c, 0, D, DIFF, ncall, -1);
+  const int lane = 0;
+  return tstep(c, &e, &lane, 1, q, gate);
 }
 
 int stepup(mh_ctx *c, const Dev &d, float relax) {
@@ -1089,15 +1143,15 @@ void drop_graphs(mh_ctx *c) {
   c->graphs.clear();
 }
 
-// The extra streams and scratch of the concurrent time steps (fast mode),
-// made on the first step that uses them.
+// The flux and TSTEP scratch of the batched time steps' second and third
+// variables (fast mode), made once per context.
 static int make_lanes(mh_ctx *c) {
+  if (c->nlanes == kLanes) return MH_OK;
   const mh_dims &dm = c->dims;
   const size_t N = (size_t)dm.im * dm.jm * dm.km;
   int r;
   for (int l = 0; l < kLanes; l++) {
     Lane &L = c->lanes[l];
-    if (!L.stream) HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     if ((r = dalloc(c, &L.XFLUX, N)) || (r = dalloc(c, &L.TFLUX, N)) || (r = dalloc(c, &L.RFLUX, N)) ||
         (r = dalloc(c, &L.SOURCE, N)) || (r = dalloc(c, &L.STORE, N)) || (r = dalloc(c, &L.DSC, N)) ||
         (r = dalloc(c, &L.TMP, N)) || (r = dalloc(c, &L.B1CHG, (size_t)kind_len(dm, K_B1))) ||
@@ -1109,8 +1163,6 @@ static int make_lanes(mh_ctx *c) {
         ((r = dalloc(c, &L.mgf.P, (size_t)dm.nib1 * dm.jm * dm.nkb1)) || (r = dalloc(c, &L.mgf.PT, (size_t)dm.jm * dm.nkb1))))
       return r;
   }
-  for (auto &e : c->ev)
-    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->nlanes = kLanes;
   return MH_OK;
@@ -1146,58 +1198,54 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
   if (q.visc && (r = viscous(c, d, p.ILOS, p.RF_VIS, q.vislam, q.first))) return r;
   LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
   LCHK("massflux");
-  // With lanes, the energy equation (its fluxes read only the mass fluxes,
-  // HO and QSOURCE) runs on lane 1 and the turbulence transport (its fluxes
-  // read only the mass fluxes, the previous step's kinematic viscosity and
-  // T_SOURCE) on lane 2, both beside continuity; only the kinematic value
-  // that closes the transport needs the new density.  The three momentum
-  // equations (fluxes from the step's primitives, PEFF and the body forces)
-  // run on lanes 0, 1 and 2.  Each lane has its own flux and TSTEP scratch;
-  // the boundary conditions and the limiters, which read several variables,
-  // wait for all of them.
-  const bool lanes = c->nlanes == kLanes && c->lanes_on && !p.EXACT && c->scal[MH_S_LANES] != 0.0;
-  if (lanes) {
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(hipStreamWaitEvent(c->lanes[0].stream, c->ev[0], 0));
-    const Dev d1 = lane_dev(c, 1, CTL_SKIP);
-    LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->lanes[0].stream, g, d1);
-    LCHK("flux_energy");
-    if ((r = tstep(c, 1, d.ROE, d.DROE, 3, q, CTL_SKIP))) return r;
-    HIPCHK(hipEventRecord(c->ev[1], c->lanes[0].stream));
-    if (p.ILOS >= 200) {
-      // transport of the turbulent viscosity, mf:4275-4335
-      HIPCHK(hipStreamWaitEvent(c->lanes[1].stream, c->ev[0], 0));
-      const Dev d2 = lane_dev(c, 2, CTL_SKIP);
-      LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->lanes[1].stream, g, d2);
-      if ((r = tstep(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
-      HIPCHK(hipEventRecord(c->ev[5], c->lanes[1].stream));
+  // Batched (fast mode, LANES = 1): continuity, energy and the turbulence
+  // transport take their time steps together (their fluxes read only the
+  // mass fluxes, HO, QSOURCE, the previous step's kinematic viscosity and
+  // T_SOURCE; only the kinematic value that closes the transport needs the
+  // new density), then, after the boundary conditions, the three momentum
+  // equations (fluxes from the step's primitives, PEFF and the body forces).
+  // Each variable has its own flux arrays and TSTEP scratch (lanes 0-2;
+  // continuity's fluxes are the mass fluxes k_massflux leaves on lane 0).
+  // Otherwise one variable at a time in the reference's order.
+  const bool batched = c->nlanes == kLanes && !p.EXACT && c->scal[MH_S_LANES] != 0.0;
+  if (batched) {
+    EqLane eq[kMaxEq];
+    int ln[kMaxEq] = {0, 1, 2};
+    int n = 0;
+    eq[n++] = eq_lane(c, 0, d.RO, d.DRO, 1, -1);
+    eq[n++] = eq_lane(c, 1, d.ROE, d.DROE, 3, EQ_ENERGY);
+    if (p.ILOS >= 200) eq[n++] = eq_lane(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, EQ_SA);
+    EqSet ef{};
+    for (int i = 0; i < kMaxEq; i++) ef.q[i] = eq[std::min(i + 1, n - 1)];
+    LAUNCH(k_flux_eq, dim3(grid.x, grid.y, g.KM * (n - 1)), kNodeBlock, 0, c->stream, g, d, ef);
+    LCHK("flux_eq");
+    if ((r = tstep(c, eq, ln, n, q, CTL_SKIP))) return r;
+    if (p.ILOS >= 100) {
+      // cell-average density (mf:4260-4270), then the transport's lower
+      // limit and kinematic value
+      LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
+      if (p.ILOS >= 200) LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
+      LCHK("sa_transport");
     }
-  }
-  if ((r = tstep(c, 0, d.RO, d.DRO, 1, q, CTL_SKIP))) return r;
-  if (p.ILOS >= 100 && p.ILOS < 200) {
-    // cell-average density for NEW_LOSS, mf:4260-4270
-    LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
-    LCHK("cellavg");
-  }
-  if (p.ILOS >= 200) {
-    // cell-average density (mf:4260-4270), the transport (mf:4275-4335) in
-    // order here without lanes, then the lower limit and kinematic value
-    LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
-    if (lanes) {
-      HIPCHK(hipStreamWaitEvent(c->stream, c->ev[5], 0));
-    } else {
-      LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
-      if ((r = tstep(c, 0, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
-    }
-    LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
-    LCHK("sa_transport");
-  }
-  if (lanes) {
-    HIPCHK(hipStreamWaitEvent(c->stream, c->ev[1], 0));
   } else {
+    if ((r = tstep1(c, d.RO, d.DRO, 1, q, CTL_SKIP))) return r;
+    if (p.ILOS >= 100 && p.ILOS < 200) {
+      // cell-average density for NEW_LOSS, mf:4260-4270
+      LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
+      LCHK("cellavg");
+    }
+    if (p.ILOS >= 200) {
+      // cell-average density (mf:4260-4270), the transport (mf:4275-4335),
+      // then the lower limit and kinematic value
+      LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
+      LAUNCH(k_sa_fluxes, grid, kNodeBlock, 0, c->stream, g, d);
+      if ((r = tstep1(c, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, q, CTL_SKIP))) return r;
+      LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
+      LCHK("sa_transport");
+    }
     LAUNCH(k_flux_energy, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("flux_energy");
-    if ((r = tstep(c, 0, d.ROE, d.DROE, 3, q, CTL_SKIP))) return r;
+    if ((r = tstep1(c, d.ROE, d.DROE, 3, q, CTL_SKIP))) return r;
   }
   LAUNCH(k_bc_lines, dim3(g.KM), dim3(64), 0, c->stream, g, d, p);
   LAUNCH(k_bc_pd, dim3(1), dim3(128), 0, c->stream, g, d, p, s);
@@ -1211,32 +1259,25 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LCHK("setflo");
   }
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
-  if (lanes) {
-    HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    for (int l = 0; l < kLanes; l++) HIPCHK(hipStreamWaitEvent(c->lanes[l].stream, c->ev[2], 0));
-    const Dev d1 = lane_dev(c, 1, CTL_SKIP), d2 = lane_dev(c, 2, CTL_SKIP);
-    LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->lanes[0].stream, g, d1);
-    LCHK("flux_angular");
-    if ((r = tstep(c, 1, d.RORVT, d.DRORVT, 5, q, CTL_SKIP))) return r;
-    HIPCHK(hipEventRecord(c->ev[3], c->lanes[0].stream));
-    LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->lanes[1].stream, g, d2);
-    LCHK("flux_radial");
-    if ((r = tstep(c, 2, d.ROVR, d.DROVR, 6, q, CTL_SKIP))) return r;
-    HIPCHK(hipEventRecord(c->ev[4], c->lanes[1].stream));
-  }
-  LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
-  LCHK("flux_axial");
-  if ((r = tstep(c, 0, d.ROVX, d.DROVX, 4, q, CTL_SKIP))) return r;
-  if (lanes) {
-    HIPCHK(hipStreamWaitEvent(c->stream, c->ev[3], 0));
-    HIPCHK(hipStreamWaitEvent(c->stream, c->ev[4], 0));
+  if (batched) {
+    EqSet em{};
+    em.q[0] = eq_lane(c, 0, d.ROVX, d.DROVX, 4, EQ_AXIAL);
+    em.q[1] = eq_lane(c, 1, d.RORVT, d.DRORVT, 5, EQ_ANGULAR);
+    em.q[2] = eq_lane(c, 2, d.ROVR, d.DROVR, 6, EQ_RADIAL);
+    const int ln[kMaxEq] = {0, 1, 2};
+    LAUNCH(k_flux_eq, dim3(grid.x, grid.y, g.KM * 3), kNodeBlock, 0, c->stream, g, d, em);
+    LCHK("flux_eq");
+    if ((r = tstep(c, em.q, ln, 3, q, CTL_SKIP))) return r;
   } else {
+    LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("flux_axial");
+    if ((r = tstep1(c, d.ROVX, d.DROVX, 4, q, CTL_SKIP))) return r;
     LAUNCH(k_flux_angular, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("flux_angular");
-    if ((r = tstep(c, 0, d.RORVT, d.DRORVT, 5, q, CTL_SKIP))) return r;
+    if ((r = tstep1(c, d.RORVT, d.DRORVT, 5, q, CTL_SKIP))) return r;
     LAUNCH(k_flux_radial, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("flux_radial");
-    if ((r = tstep(c, 0, d.ROVR, d.DROVR, 6, q, CTL_SKIP))) return r;
+    if ((r = tstep1(c, d.ROVR, d.DROVR, 6, q, CTL_SKIP))) return r;
   }
   // limiters, mf:5115-5168
   const int nline = g.IM * g.KM;
@@ -1287,7 +1328,7 @@ int enqueue_chunk(mh_ctx *c, const std::vector<StepPlan> &plan) {
     }
     return MH_OK;
   }
-  std::string key = c->lanes_on ? "L" : "S";
+  std::string key;
   for (const StepPlan &q : plan) key += std::to_string(q.code()) + ",";
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
@@ -1463,13 +1504,6 @@ int mh_destroy(mh_ctx *c) {
     (void)hipEventDestroy(r.t1);
   }
   for (auto e : c->prof_pool) (void)hipEventDestroy(e);
-  for (int l = 0; l < kLanes; l++)
-    if (c->lanes[l].stream) {
-      (void)hipStreamSynchronize(c->lanes[l].stream);
-      (void)hipStreamDestroy(c->lanes[l].stream);
-    }
-  for (auto e : c->ev)
-    if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->ev_poll) (void)hipEventDestroy(c->ev_poll);
   for (auto e : c->ev_ahead)
@@ -1747,6 +1781,7 @@ int mh_start(mh_ctx *c) {
     }
     c->mixed = true;
   }
+  if (!p.EXACT && (r = make_lanes(c))) return r;
   if ((r = autotune(c))) return r;
   c->ifend = 0;
   c->nstep = 0;
@@ -1788,7 +1823,6 @@ int mh_step(mh_ctx *c, int nsteps, mh_resid *out) {
   if (nsteps < 0) return fail(c, MH_E_ARG, "nsteps < 0");
   const Params p = params_of(c);
   int r;
-  if (!p.EXACT && c->lanes_on && c->nlanes == 0 && (r = make_lanes(c))) return r;
   bool end = c->ifend != 0, polling = false;
   int queued = 0;
   std::vector<StepPlan> plan;
@@ -1843,17 +1877,11 @@ int mh_step_batch(mh_ctx *const *ctxs, int n, int nsteps, mh_resid *out, int *fa
     for (int k = 0; k < i; k++)
       if (ctxs[k] == ctxs[i]) return fail(ctxs[i], MH_E_ARG, "mh_step_batch: a context appears twice");
   }
-  // whole contexts overlap here: each keeps to its one stream (more streams
-  // than the GPU has hardware queues would serialise on the lanes' waits)
+  // whole contexts overlap here, each on its own stream
   std::vector<int> rc(n, MH_OK);
   std::vector<std::thread> th;
   th.reserve(n);
-  for (int i = 0; i < n; i++)
-    th.emplace_back([&, i] {
-      ctxs[i]->lanes_on = n == 1;
-      rc[i] = mh_step(ctxs[i], nsteps, &out[i]);
-      ctxs[i]->lanes_on = true;
-    });
+  for (int i = 0; i < n; i++) th.emplace_back([&, i] { rc[i] = mh_step(ctxs[i], nsteps, &out[i]); });
   for (auto &t : th) t.join();
   for (int i = 0; i < n; i++)
     if (rc[i] != MH_OK) {

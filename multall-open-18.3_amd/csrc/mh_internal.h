@@ -197,6 +197,24 @@ struct MgFast {
   const int *gate = nullptr;                       // the launch's gate flag (null: none)
 };
 
+// One conserved variable of a batched TSTEP launch: the variable, its SSS
+// history (FP32, or its BF16 copy in the mixed form), its face fluxes and
+// source, and its own TSTEP scratch.  A batched launch covers up to kMaxEq
+// variables whose time steps do not read each other's results, one grid
+// z-slice each; `kind` selects the flux formula of a batched flux launch.
+enum { EQ_ENERGY = 0, EQ_SA, EQ_AXIAL, EQ_ANGULAR, EQ_RADIAL };
+struct EqLane {
+  float *D, *DIFF;
+  unsigned short *DIFF16;
+  float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *B1CHG, *B2CHG, *SBCHG, *AVG;
+  double *RED, *P, *PT;
+  int ncall, kind;
+};
+constexpr int kMaxEq = 3;
+struct EqSet {
+  EqLane q[kMaxEq];
+};
+
 struct Lists {
   int *b1_ptr = nullptr, *b1_list = nullptr, *b2_ptr = nullptr, *b2_list = nullptr;
   int *sb_ptr = nullptr, *sb_list = nullptr, *row_ptr = nullptr, *row_list = nullptr;
@@ -220,11 +238,10 @@ namespace mh {
 constexpr int kEffFields = 18;
 constexpr int kHist = 6;  // history arrays with a BF16 copy in the mixed form
 constexpr int kAhead = 8; // chunks (of up to 5 steps) the host may queue ahead of the device
-// A second or third HIP stream with its own flux and TSTEP scratch, so the
-// time steps of equations that do not read each other's results run
-// concurrently (continuity beside energy; the three momentum equations).
+// The flux and TSTEP scratch of a batched launch's second or third variable
+// (time steps that do not read each other's results: continuity, energy and
+// the turbulence transport; the three momentum equations).
 struct Lane {
-  hipStream_t stream = nullptr;
   float *XFLUX = nullptr, *TFLUX = nullptr, *RFLUX = nullptr, *SOURCE = nullptr, *STORE = nullptr, *DSC = nullptr,
         *TMP = nullptr, *B1CHG = nullptr, *B2CHG = nullptr, *SBCHG = nullptr, *AVG = nullptr;
   double *RED = nullptr;
@@ -271,9 +288,7 @@ struct mh_ctx {
   float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
   mh_resid last{};
   std::string err;
-  // concurrent time steps (fast mode): extra streams and their fork/join events
+  // batched time steps (fast mode): the second and third variables' scratch
   int nlanes = 0;
-  bool lanes_on = true;  // off while mh_step_batch overlaps whole contexts instead
   mh::Lane lanes[mh::kLanes];
-  hipEvent_t ev[6] = {};
 };

@@ -84,6 +84,38 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
 }
 
+// Batched launches: grid z-slice z works on variable e.q[z].  The XCD-aware
+// order runs over the whole grid, so each XCD takes a contiguous range of
+// one variable's logical blocks.  Returns the logical block within the
+// slice; z is wavefront-uniform.
+__device__ __forceinline__ int eq_block(int &z) {
+  const int nxy = gridDim.x * gridDim.y;
+  const int b = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nxy * gridDim.z);
+  z = b / nxy;
+  return b - z * nxy;
+}
+// e.q[z] by uniform selects (a dynamic index would copy the set to scratch)
+__device__ __forceinline__ const EqLane &eq_pick(const EqSet &e, int z) {
+  return z == 0 ? e.q[0] : z == 1 ? e.q[1] : e.q[2];
+}
+// the context's arrays with variable q's flux and TSTEP scratch
+__device__ __forceinline__ Dev eq_dev(const Dev &d0, const EqLane &q) {
+  Dev d = d0;
+  d.XFLUX = q.XFLUX;
+  d.TFLUX = q.TFLUX;
+  d.RFLUX = q.RFLUX;
+  d.SOURCE = q.SOURCE;
+  d.STORE = q.STORE;
+  d.DSC = q.DSC;
+  d.TMP = q.TMP;
+  d.B1CHG = q.B1CHG;
+  d.B2CHG = q.B2CHG;
+  d.SBCHG = q.SBCHG;
+  d.AVG = q.AVG;
+  d.RED = q.RED;
+  return d;
+}
+
 // ---------------------------------------------------------------------------
 // primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
 
@@ -181,9 +213,7 @@ __device__ __forceinline__ float tflux_energy(const Geo &g, const Dev &d, int I,
   return (d.FLOWT[IX(I, J, K)] * AVGHO + d.WRABT[JKX(J, K)] * AVGPB) * 0.25f;
 }
 
-__global__ void k_flux_energy(Geo g, Dev d) {
-  if (gated(d)) return;
-  NODE_IJK
+__device__ __forceinline__ void flux_energy_at(const Geo &g, const Dev &d, int I, int J, int K) {
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
     float AVGHO = d.HO[IX(I, J, K)] + d.HO[IX(I, J, K + 1)] + d.HO[IX(I + 1, J, K)] + d.HO[IX(I + 1, J, K + 1)];
@@ -203,6 +233,11 @@ __global__ void k_flux_energy(Geo g, Dev d) {
     d.TFLUX[n] = tf;
   }
 }
+__global__ void k_flux_energy(Geo g, Dev d) {
+  if (gated(d)) return;
+  NODE_IJK
+  flux_energy_at(g, d, I, J, K);
+}
 
 // ---------------------------------------------------------------------------
 // axial momentum fluxes, mf:4818-4896
@@ -213,9 +248,7 @@ __device__ __forceinline__ float tflux_axial(const Geo &g, const Dev &d, int I, 
   return 0.25f * (d.FLOWT[IX(I, J, K)] * AVGVX + d.ABX[IX(I, J, K)] * AVGP);
 }
 
-__global__ void k_flux_axial(Geo g, Dev d) {
-  if (gated(d)) return;
-  NODE_IJK
+__device__ __forceinline__ void flux_axial_at(const Geo &g, const Dev &d, int I, int J, int K) {
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
     d.SOURCE[n] = d.XFORCE[n];
@@ -243,6 +276,11 @@ __global__ void k_flux_axial(Geo g, Dev d) {
     d.TFLUX[n] = tf;
   }
 }
+__global__ void k_flux_axial(Geo g, Dev d) {
+  if (gated(d)) return;
+  NODE_IJK
+  flux_axial_at(g, d, I, J, K);
+}
 
 // ---------------------------------------------------------------------------
 // angular momentum fluxes, mf:4913-4956
@@ -254,9 +292,7 @@ __device__ __forceinline__ float tflux_angular(const Geo &g, const Dev &d, int I
   return (d.FLOWT[IX(I, J, K)] * AVGVT + d.ABT[JKX(J, K)] * AVGP) * AVGR;
 }
 
-__global__ void k_flux_angular(Geo g, Dev d) {
-  if (gated(d)) return;
-  NODE_IJK
+__device__ __forceinline__ void flux_angular_at(const Geo &g, const Dev &d, int I, int J, int K) {
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
     float AVGR = 0.125f * (d.R[JKX(J, K)] + d.R[JKX(J, K + 1)]);
@@ -278,13 +314,16 @@ __global__ void k_flux_angular(Geo g, Dev d) {
     d.RFLUX[n] = d.FLOWR[n] * AVGVT * AVGR;
   }
 }
+__global__ void k_flux_angular(Geo g, Dev d) {
+  if (gated(d)) return;
+  NODE_IJK
+  flux_angular_at(g, d, I, J, K);
+}
 
 // ---------------------------------------------------------------------------
 // radial momentum fluxes and source, mf:5003-5095
 
-__global__ void k_flux_radial(Geo g, Dev d) {
-  if (gated(d)) return;
-  NODE_IJK
+__device__ __forceinline__ void flux_radial_at(const Geo &g, const Dev &d, int I, int J, int K) {
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
     float AVGVR = d.VR[IX(I, J, K)] + d.VR[IX(I, J, K + 1)] + d.VR[IX(I + 1, J, K)] + d.VR[IX(I + 1, J, K + 1)];
@@ -327,6 +366,33 @@ __global__ void k_flux_radial(Geo g, Dev d) {
                     d.PEFF[IX(I + 1, J, K + 1)];
       d.SOURCE[n] = d.RFORCE[n] - (AVGRVT * AVGVT + AVGP8) * d.VOLOR[n];
     }
+  }
+}
+__global__ void k_flux_radial(Geo g, Dev d) {
+  if (gated(d)) return;
+  NODE_IJK
+  flux_radial_at(g, d, I, J, K);
+}
+
+// The face fluxes and sources of up to three variables in one launch (grid
+// z = variable x K), each into its own flux arrays; the formula is the
+// variable's kind (energy mf:4357-4456, axial mf:4818-4896, angular
+// mf:4913-4956, radial mf:5003-5095; the turbulence transport's are below).
+__device__ __forceinline__ void sa_fluxes_at(const Geo &g, const Dev &d, int I, int J, int K);
+__global__ void k_flux_eq(Geo g, Dev d0, EqSet e) {
+  if (gated(d0)) return;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int z = blockIdx.z / g.KM, K = blockIdx.z - z * g.KM + 1;
+  if (I > g.IM || J > g.JM) return;
+  const EqLane &q = eq_pick(e, z);
+  const Dev d = eq_dev(d0, q);
+  switch (q.kind) {
+    case EQ_ENERGY: flux_energy_at(g, d, I, J, K); break;
+    case EQ_SA: sa_fluxes_at(g, d, I, J, K); break;
+    case EQ_AXIAL: flux_axial_at(g, d, I, J, K); break;
+    case EQ_ANGULAR: flux_angular_at(g, d, I, J, K); break;
+    default: flux_radial_at(g, d, I, J, K); break;
   }
 }
 
@@ -846,6 +912,7 @@ __device__ __forceinline__ int mp_role(const Dev &d, int J, int &jx) {
   return v & 3;
 }
 
+
 // Residual of one (J,K) line on one wavefront: the upstream-face flux
 // extrapolation at the mixing planes applied to the line's source
 // (mf:5516-5555), the flux sum and SSS history with the tip-gap TFLUX
@@ -954,11 +1021,9 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<B
 #define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
 template <bool BF>
 __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> DIFF, const Params &p, const MgFast &m,
-                                           int kg, int ir, int part) {
-  if (gated(d)) return;
+                                           int kg, int ir, int part, int bl) {
   __shared__ double sh[4][256];
   const int lane = threadIdx.x & 63, w = wave_id();
-  const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int J = (bl % gridDim.x) * 4 + w + 2, kb = bl / gridDim.x, kq = kb * kg + 1;
   if (J > g.JM) return;
   const int IM = g.IM, IMM1 = g.IMM1;
@@ -1050,16 +1115,31 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
     m.P[PX(ib, J, kb + 1)] = sum;
   }
 }
-__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, float *DIFF, Params p, MgFast m, int kg, int ir,
+// Batched over the variables of e (grid z), each with its own scratch and
+// multigrid partial sums.
+template <bool BF>
+__device__ __forceinline__ void resid_line_eq(const Geo &g, const Dev &d0, const EqSet &e, const Params &p,
+                                              const MgFast &m0, int kg, int ir, int part) {
+  int z;
+  const int bl = eq_block(z);
+  const EqLane &q = eq_pick(e, z);
+  const Dev d = eq_dev(d0, q);
+  MgFast m = m0;
+  m.P = q.P;
+  m.PT = q.PT;
+  if constexpr (BF) resid_line<true>(g, d, Hist<true>{q.DIFF16}, p, m, kg, ir, part, bl);
+  else resid_line<false>(g, d, Hist<false>{q.DIFF}, p, m, kg, ir, part, bl);
+}
+__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, EqSet e, Params p, MgFast m, int kg, int ir,
                                                     int part) {
   if (gated(d)) return;
-  resid_line<false>(g, d, Hist<false>{DIFF}, p, m, kg, ir, part);
+  resid_line_eq<false>(g, d, e, p, m, kg, ir, part);
 }
-// the same with the SSS history stored in BF16 (the mixed form)
-__global__ void __launch_bounds__(256) k_resid_line_bf(Geo g, Dev d, unsigned short *DIFF, Params p, MgFast m, int kg,
-                                                       int ir, int part) {
+// the same with the SSS histories stored in BF16 (the mixed form)
+__global__ void __launch_bounds__(256) k_resid_line_bf(Geo g, Dev d, EqSet e, Params p, MgFast m, int kg, int ir,
+                                                       int part) {
   if (gated(d)) return;
-  resid_line<true>(g, d, Hist<true>{DIFF}, p, m, kg, ir, part);
+  resid_line_eq<true>(g, d, e, p, m, kg, ir, part);
 }
 
 // Fast-mode multigrid sums (mf:5621-5640), pass 2 (pass 1 is in
@@ -1067,9 +1147,14 @@ __global__ void __launch_bounds__(256) k_resid_line_bf(Geo g, Dev d, unsigned sh
 // the level-1 list), one wavefront per level-2 block (its level-1 I and K
 // blocks and J members), one workgroup per superblock (its J members, all
 // level-1 K blocks); fixed-order trees throughout.
-__global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b1p, const int *b1l, const int *sbp,
-                                                   const int *sbl, float *B1, float *B2, float *SB) {
+__global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, EqSet e, const int *b1p, const int *b1l,
+                                                   const int *sbp, const int *sbl) {
   if (m.gate && *m.gate) return;
+  // grid y: the variable
+  const EqLane &q = eq_pick(e, blockIdx.y);
+  m.P = q.P;
+  m.PT = q.PT;
+  float *const B1 = q.B1CHG, *const B2 = q.B2CHG, *const SB = q.SBCHG;
   const int n1 = g.NIB1 * (g.NJB1 + 1) * g.NKB1, n2 = g.NIB2 * (g.NJB2 + 1) * g.NKB2;
   const int nblk1 = (n1 + 255) / 256, nblk2 = (n2 + 3) / 4;
   const int bx = blockIdx.x;
@@ -1131,10 +1216,13 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, const int *b
 // multigrid update (mf:5645-5660), one wavefront per (J, 4 consecutive K
 // lines), leaving each line's FP64 sum of |STORE| for the damping average
 // in red[(J,K)]
-__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *red) {
-  if (gated(d)) return;
+__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) {
+  if (gated(d0)) return;
   const int lane = threadIdx.x & 63, w = wave_id();
-  const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  int z;
+  const int bl = eq_block(z);
+  const Dev d = eq_dev(d0, eq_pick(e, z));
+  double *const red = d.RED;
   const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
   if (J > g.JM) return;
   const int nk = min(4, g.KMM1 - kq + 1);
@@ -1185,8 +1273,11 @@ __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d, double *re
 // per-row average |change| from the (J,K) line sums (mf:5677-5693): one
 // workgroup of 16 wavefronts per row (a contiguous J range); each lane sums
 // its share, then fixed-order butterflies within and across wavefronts
-__global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *red, const int *rptr, const int *rlist) {
+__global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, EqSet e, const int *rptr, const int *rlist) {
   if (gated(d)) return;
+  // grid y: the variable
+  const EqLane &q = eq_pick(e, blockIdx.y);
+  const double *const red = q.RED;
   __shared__ double sh[16];
   const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, w = wave_id();
   const int q0 = rptr[r], nj = rptr[r + 1] - q0, ja = rlist[q0], tot = nj * g.KMM1;
@@ -1210,7 +1301,7 @@ __global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, const double *re
     for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if (lane == 0) {
       const float f = (float)x;
-      d.AVG[r] = f / (float)(g.IMM1 * g.KMM1 * nj);
+      q.AVG[r] = f / (float)(g.IMM1 * g.KMM1 * nj);
     }
   }
 }
@@ -1238,8 +1329,9 @@ __device__ __forceinline__ float damped(const Geo &g, const Dev &d, const float 
 // keeps the reference's two divisions.  Dynamic LDS: kTL + kWL lines of
 // `pitch` = 64 NC + 2 floats.
 template <int NC, int kTK>
-__device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *D, float *DOUT,
-                                             int damp, int mpav) {
+__device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *STORE,
+                                             const float *AC, const float *D, float *DOUT, int damp, int mpav,
+                                             int bl) {
   constexpr int kTJ = kScatTJ, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4, kWL = 4 * (kTK + 1);
   static_assert(kTJ == 4, "one wavefront per tile J line");
   const float DAMP = d.SC[SC_DAMP];
@@ -1247,7 +1339,6 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   __shared__ float avg[kTJ + 1], rad[kTJ + 1], fdn[kTJ + 1], fup[kTJ + 1];
   __shared__ int flag[kTL];
   constexpr int pitch = 64 * NC + 2;
-  const int bl = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int j0 = (bl % gridDim.x) * kTJ + 1, k0 = (bl / gridDim.x) * kTK + 1;
   const int lane = threadIdx.x & 63, w = wave_id(), t = threadIdx.x;
   const int IM = g.IM, IMM1 = g.IMM1, JM = g.JM, KM = g.KM, KMM1 = g.KMM1;
@@ -1267,7 +1358,7 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const int I = 64 * c + lane + 1;
-      x[u][c] = (in && I <= IMM1) ? d.STORE[base + (unsigned)(I - 1)] : 0.0f;
+      x[u][c] = (in && I <= IMM1) ? STORE[base + (unsigned)(I - 1)] : 0.0f;
     }
   }
   const int J = j0 + w;
@@ -1297,7 +1388,7 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   } else if (t >= 64 && t < 64 + kTJ + 1) {
     const int jc = j0 + t - 64;
     const bool jok = jc >= 2 && jc <= JM;
-    const float A = (jok && damp) ? avg_blk(d.AVG, d.NROW[jc - 1], g.NROWS) : 1.0f;
+    const float A = (jok && damp) ? avg_blk(AC, d.NROW[jc - 1], g.NROWS) : 1.0f;
     avg[t - 64] = A;
     rad[t - 64] = 1.0f / (A * DAMP);
     fdn[t - 64] = jok ? d.FACDWN[jc - 1] : 0.0f;
@@ -1404,11 +1495,15 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
     }
   }
 }
+// batched over the variables of e (grid z): D = q.D, the result to q.DSC
 #define DAMP_SCATTER(NC, TK)                                                                                 \
-  __global__ void __launch_bounds__(256) k_damp_scatter##NC##_k##TK(Geo g, Dev d, Params p, const float *D,   \
-                                                                    float *DOUT, int damp, int mpav) {        \
-    if (gated(d)) return;                                                                                    \
-    damp_scatter<NC, TK>(g, d, p, D, DOUT, damp, mpav);                                                      \
+  __global__ void __launch_bounds__(256) k_damp_scatter##NC##_k##TK(Geo g, Dev d0, EqSet e, Params p, int damp, \
+                                                                    int mpav) {                              \
+    if (gated(d0)) return;                                                                                   \
+    int z;                                                                                                   \
+    const int bl = eq_block(z);                                                                              \
+    const EqLane &q = eq_pick(e, z);                                                                         \
+    damp_scatter<NC, TK>(g, d0, p, q.STORE, q.AVG, q.D, q.DSC, damp, mpav, bl);                             \
   }
 DAMP_SCATTER(1, 2)
 DAMP_SCATTER(1, 4)
@@ -1427,11 +1522,16 @@ DAMP_SCATTER(4, 8)
 // runs in LDS, with the pitchwise and spanwise coefficients staged there.
 // Writes the finished plane to DST.  Dynamic LDS: 3*IM*KM + 2*(IM+KM)
 // floats.
-__global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, const float *SRC, float *DST, Params p,
-                                                         StepScalars s, int ncall, int j0) {
+__global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, EqSet e, Params p, StepScalars s, int j0) {
   if (gated(d)) return;
   extern __shared__ float lds[];
-  const int J = xcd_block(blockIdx.x, gridDim.x) + j0;
+  // batched over the variables of e (grid z): q.DSC smoothed into q.D
+  int z;
+  const int J = eq_block(z) + j0;
+  const EqLane &q = eq_pick(e, z);
+  const float *const SRC = q.DSC;
+  float *const DST = q.D;
+  const int ncall = q.ncall;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
   const int npl = IM * KM;
@@ -1660,10 +1760,9 @@ __device__ __forceinline__ float lane_at(float x, int l) {  // lane l's value, w
 
 template <int L>
 __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const float *SRC, float *DST, const Params &p,
-                                           const StepScalars &s, int ncall, int j0) {
+                                           const StepScalars &s, int ncall, int J) {
   __shared__ float PLs[128 * 64];
   __shared__ float C2[2][64];
-  const int J = xcd_block(blockIdx.x, gridDim.x) + j0;
   const int lane = threadIdx.x & 63, w = wave_id();
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
   const int I = lane + 1, Ic = min(I, IM);
@@ -1864,11 +1963,15 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
 }
 
 // k_smooth_reg<L>w<NW>: NW wavefronts per J plane, L K lines each
+// batched over the variables of e (grid z): q.DSC smoothed into q.D
 #define SMOOTH_REG(L, NW)                                                                                    \
-  __global__ void __launch_bounds__(64 * NW) k_smooth_reg##L##w##NW(Geo g, Dev d, const float *SRC, float *DST, \
-                                                                   Params p, StepScalars s, int ncall, int j0) { \
+  __global__ void __launch_bounds__(64 * NW) k_smooth_reg##L##w##NW(Geo g, Dev d, EqSet e, Params p,         \
+                                                                   StepScalars s, int j0) {                  \
     if (gated(d)) return;                                                                                    \
-    smooth_reg<L>(g, d, SRC, DST, p, s, ncall, j0);                                                          \
+    int z;                                                                                                   \
+    const int J = eq_block(z) + j0;                                                                          \
+    const EqLane &q = eq_pick(e, z);                                                                         \
+    smooth_reg<L>(g, d, q.DSC, q.D, p, s, q.ncall, J);                                                       \
   }
 SMOOTH_REG(1, 8)
 SMOOTH_REG(2, 8)
@@ -2400,9 +2503,7 @@ __global__ void k_sa_cellavg(Geo g, Dev d) {
 }
 
 // doubled face fluxes and source of the turbulent viscosity, mf:4279-4330
-__global__ void k_sa_fluxes(Geo g, Dev d) {
-  if (gated(d)) return;
-  NODE_IJK
+__device__ __forceinline__ void sa_fluxes_at(const Geo &g, const Dev &d, int I, int J, int K) {
   const int IMM1 = g.IMM1, KMM1 = g.KMM1;
   const float *TK = d.TRANS_KVIS;
   const unsigned n = IX(I, J, K);
@@ -2428,6 +2529,11 @@ __global__ void k_sa_fluxes(Geo g, Dev d) {
     d.TFLUX[n] = d.FLOWT[n1] * (TK[IX(IMM1, J, K)] + TK[n1]);
   }
   (void)KMM1;
+}
+__global__ void k_sa_fluxes(Geo g, Dev d) {
+  if (gated(d)) return;
+  NODE_IJK
+  sa_fluxes_at(g, d, I, J, K);
 }
 
 // D += STORE (damped as TSTEP's negative feedback leaves it) on the cells,
