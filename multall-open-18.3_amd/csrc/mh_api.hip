@@ -90,7 +90,8 @@ __global__ void k_to_bf16(const float *, unsigned short *, unsigned);
 __global__ void k_from_bf16(const unsigned short *, float *, unsigned);
 __global__ void k_mg_levels(Geo, MgFast, EqSet, const int *, const int *, const int *, const int *);
 __global__ void k_mg_apply_lines(Geo, Dev, EqSet);
-__global__ void k_flux_eq(Geo, Dev, EqSet);
+__global__ void k_flux_mom(Geo, Dev, EqSet);
+__global__ void k_massflux_eq(Geo, Dev, EqSet, int);
 __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
 __global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
 __global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
@@ -628,7 +629,7 @@ unsigned short *hist16(const mh_ctx *c, const float *a) {
 }
 
 // A variable of a batched TSTEP or flux launch on lane `lane`.
-EqLane eq_lane(const mh_ctx *c, int lane, float *D, float *DIFF, int ncall, int kind) {
+EqLane eq_lane(const mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
   const Dev d = lane_dev(c, lane, -1);
   const MgFast &m = lane ? c->lanes[lane - 1].mgf : c->lists.mgf;
   EqLane q{};
@@ -640,7 +641,6 @@ EqLane eq_lane(const mh_ctx *c, int lane, float *D, float *DIFF, int ncall, int 
   q.B1CHG = d.B1CHG; q.B2CHG = d.B2CHG; q.SBCHG = d.SBCHG; q.AVG = d.AVG;
   q.RED = d.RED; q.P = m.P; q.PT = m.PT;
   q.ncall = ncall;
-  q.kind = kind;
   return q;
 }
 // FP32 history <-> its BF16 copy (synchronous; outside the step path).
@@ -737,7 +737,7 @@ int autotune(mh_ctx *c) {
   EqSet es{}, et{};
   for (int i = 0; i < kMaxEq; i++) {
     const int l = i < nz ? i : 0;
-    es.q[i] = eq_lane(c, l, vars[l], d.DROVX, 4, -1);
+    es.q[i] = eq_lane(c, l, vars[l], d.DROVX, 4);
     et.q[i] = es.q[i];
     et.q[i].D = et.q[i].TMP;
   }
@@ -935,7 +935,7 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
 
 // TSTEP for one variable on lane 0
 int tstep1(mh_ctx *c, float *D, float *DIFF, int ncall, const StepPlan &q, int gate) {
-  const EqLane e = eq_lane(c, 0, D, DIFF, ncall, -1);
+  const EqLane e = eq_lane(c, 0, D, DIFF, ncall);
   const int lane = 0;
   return tstep(c, &e, &lane, 1, q, gate);
 }
@@ -1196,8 +1196,6 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
   }
   // viscous forces every NLOS steps, mf:3935-3942
   if (q.visc && (r = viscous(c, d, p.ILOS, p.RF_VIS, q.vislam, q.first))) return r;
-  LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
-  LCHK("massflux");
   // Batched (fast mode, LANES = 1): continuity, energy and the turbulence
   // transport take their time steps together (their fluxes read only the
   // mass fluxes, HO, QSOURCE, the previous step's kinematic viscosity and
@@ -1205,20 +1203,25 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
   // new density), then, after the boundary conditions, the three momentum
   // equations (fluxes from the step's primitives, PEFF and the body forces).
   // Each variable has its own flux arrays and TSTEP scratch (lanes 0-2;
-  // continuity's fluxes are the mass fluxes k_massflux leaves on lane 0).
+  // continuity's fluxes are the mass fluxes, left on lane 0; the energy and
+  // transport fluxes are formed beside them, in the same launch).
   // Otherwise one variable at a time in the reference's order.
   const bool batched = c->nlanes == kLanes && !p.EXACT && c->scal[MH_S_LANES] != 0.0;
+  if (!batched) {
+    LAUNCH(k_massflux, grid, kNodeBlock, 0, c->stream, g, d);
+    LCHK("massflux");
+  }
   if (batched) {
     EqLane eq[kMaxEq];
     int ln[kMaxEq] = {0, 1, 2};
     int n = 0;
-    eq[n++] = eq_lane(c, 0, d.RO, d.DRO, 1, -1);
-    eq[n++] = eq_lane(c, 1, d.ROE, d.DROE, 3, EQ_ENERGY);
-    if (p.ILOS >= 200) eq[n++] = eq_lane(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2, EQ_SA);
+    eq[n++] = eq_lane(c, 0, d.RO, d.DRO, 1);
+    eq[n++] = eq_lane(c, 1, d.ROE, d.DROE, 3);
+    if (p.ILOS >= 200) eq[n++] = eq_lane(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2);
     EqSet ef{};
     for (int i = 0; i < kMaxEq; i++) ef.q[i] = eq[std::min(i + 1, n - 1)];
-    LAUNCH(k_flux_eq, dim3(grid.x, grid.y, g.KM * (n - 1)), kNodeBlock, 0, c->stream, g, d, ef);
-    LCHK("flux_eq");
+    LAUNCH(k_massflux_eq, grid, kNodeBlock, 0, c->stream, g, d, ef, n - 1);
+    LCHK("massflux_eq");
     if ((r = tstep(c, eq, ln, n, q, CTL_SKIP))) return r;
     if (p.ILOS >= 100) {
       // cell-average density (mf:4260-4270), then the transport's lower
@@ -1261,12 +1264,12 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
   LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   if (batched) {
     EqSet em{};
-    em.q[0] = eq_lane(c, 0, d.ROVX, d.DROVX, 4, EQ_AXIAL);
-    em.q[1] = eq_lane(c, 1, d.RORVT, d.DRORVT, 5, EQ_ANGULAR);
-    em.q[2] = eq_lane(c, 2, d.ROVR, d.DROVR, 6, EQ_RADIAL);
+    em.q[0] = eq_lane(c, 0, d.ROVX, d.DROVX, 4);
+    em.q[1] = eq_lane(c, 1, d.RORVT, d.DRORVT, 5);
+    em.q[2] = eq_lane(c, 2, d.ROVR, d.DROVR, 6);
     const int ln[kMaxEq] = {0, 1, 2};
-    LAUNCH(k_flux_eq, dim3(grid.x, grid.y, g.KM * 3), kNodeBlock, 0, c->stream, g, d, em);
-    LCHK("flux_eq");
+    LAUNCH(k_flux_mom, grid, kNodeBlock, 0, c->stream, g, d, em);
+    LCHK("flux_mom");
     if ((r = tstep(c, em.q, ln, 3, q, CTL_SKIP))) return r;
   } else {
     LAUNCH(k_flux_axial, grid, kNodeBlock, 0, c->stream, g, d);

@@ -201,14 +201,13 @@ struct MgFast {
 // history (FP32, or its BF16 copy in the mixed form), its face fluxes and
 // source, and its own TSTEP scratch.  A batched launch covers up to kMaxEq
 // variables whose time steps do not read each other's results, one grid
-// z-slice each; `kind` selects the flux formula of a batched flux launch.
-enum { EQ_ENERGY = 0, EQ_SA, EQ_AXIAL, EQ_ANGULAR, EQ_RADIAL };
+// z-slice each.
 struct EqLane {
   float *D, *DIFF;
   unsigned short *DIFF16;
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *B1CHG, *B2CHG, *SBCHG, *AVG;
   double *RED, *P, *PT;
-  int ncall, kind;
+  int ncall;
 };
 constexpr int kMaxEq = 3;
 struct EqSet {

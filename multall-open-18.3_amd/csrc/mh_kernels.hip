@@ -161,77 +161,125 @@ __device__ __forceinline__ float flowt_raw(const Geo &g, const Dev &d, int I, in
   return (AVGRVX * d.ABX[IX(I, J, K)] + AVGRVT * d.ABT[JKX(J, K)] + AVGRVR * d.ABR[IX(I, J, K)]) * 0.25f;
 }
 
-__global__ void k_massflux(Geo g, Dev d) {
-  if (gated(d)) return;
-  NODE_IJK
+// The node's mass fluxes (zero on the faces not formed there) and relative
+// swirl, formed without storing.
+struct MassFlux {
+  float fx = 0.f, ft = 0.f, fr = 0.f, rowt = 0.f, wt = 0.f;
+  bool hx = false, ht = false, hr = false;
+};
+__device__ __forceinline__ MassFlux massflux_vals(const Geo &g, const Dev &d, int I, int J, int K) {
+  MassFlux m;
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
     float AVGRVX = d.ROVX[IX(I, J, K)] + d.ROVX[IX(I, J, K + 1)] + d.ROVX[IX(I + 1, J, K)] + d.ROVX[IX(I + 1, J, K + 1)];
     float AVGRVR = d.ROVR[IX(I, J, K)] + d.ROVR[IX(I, J, K + 1)] + d.ROVR[IX(I + 1, J, K)] + d.ROVR[IX(I + 1, J, K + 1)];
-    float fx = 0.25f * (AVGRVX * d.AQX[n] + AVGRVR * d.AQR[n]);
-    d.FLOWX[n] = fx;
-    d.SOURCE[n] = 0.0f;
-    d.XFLUX[n] = fx;
+    m.fx = 0.25f * (AVGRVX * d.AQX[n] + AVGRVR * d.AQR[n]);
+    m.hx = true;
   }
-  {
-    float rowt = rowt_at(g, d, I, J, K);
-    d.ROWT[n] = rowt;
-    d.WT[n] = rowt / d.RO[n];
-  }
+  m.rowt = rowt_at(g, d, I, J, K);
+  m.wt = m.rowt / d.RO[n];
   if (J >= 2 && K <= g.KMM1) {
-    float ft;
     if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
-      ft = 0.5f * (flowt_raw(g, d, 1, J, K) + flowt_raw(g, d, g.IM, J, K));
+      m.ft = 0.5f * (flowt_raw(g, d, 1, J, K) + flowt_raw(g, d, g.IM, J, K));
     } else {
-      ft = flowt_raw(g, d, I, J, K);
+      m.ft = flowt_raw(g, d, I, J, K);
       if (d.IND[J - 1] == 1 && (I == 1 || I == g.IM)) {
         int NR = d.NROW[J - 1];
-        if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) ft = 0.0f;
+        if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) m.ft = 0.0f;
       }
     }
-    d.FLOWT[n] = ft;
-    d.TFLUX[n] = ft;
+    m.ht = true;
   }
   if (J >= 2 && I <= g.IMM1) {
-    float fr = 0.0f;
     if (K >= 2 && K <= g.KMM1) {
       float AVGRVX = d.ROVX[IX(I, J, K)] + d.ROVX[IX(I, J - 1, K)] + d.ROVX[IX(I + 1, J - 1, K)] + d.ROVX[IX(I + 1, J, K)];
       float AVGRVR = d.ROVR[IX(I, J, K)] + d.ROVR[IX(I, J - 1, K)] + d.ROVR[IX(I + 1, J - 1, K)] + d.ROVR[IX(I + 1, J, K)];
-      fr = 0.25f * (AVGRVX * d.ASX[n] + AVGRVR * d.ASR[n]);
+      m.fr = 0.25f * (AVGRVX * d.ASX[n] + AVGRVR * d.ASR[n]);
     }
-    d.FLOWR[n] = fr;
-    d.RFLUX[n] = fr;
+    m.hr = true;
   }
+  return m;
+}
+// FLOWX, FLOWT, FLOWR, ROWT, WT, and continuity's fluxes (the mass fluxes,
+// no source) in d's flux arrays
+__device__ __forceinline__ void massflux_store(const Geo &g, const Dev &d, const MassFlux &m, size_t n) {
+  if (m.hx) {
+    d.FLOWX[n] = m.fx;
+    d.SOURCE[n] = 0.0f;
+    d.XFLUX[n] = m.fx;
+  }
+  d.ROWT[n] = m.rowt;
+  d.WT[n] = m.wt;
+  if (m.ht) {
+    d.FLOWT[n] = m.ft;
+    d.TFLUX[n] = m.ft;
+  }
+  if (m.hr) {
+    d.FLOWR[n] = m.fr;
+    d.RFLUX[n] = m.fr;
+  }
+}
+__global__ void k_massflux(Geo g, Dev d) {
+  if (gated(d)) return;
+  NODE_IJK
+  massflux_store(g, d, massflux_vals(g, d, I, J, K), IX(I, J, K));
 }
 
 // ---------------------------------------------------------------------------
 // energy fluxes, mf:4357-4456
 
-__device__ __forceinline__ float tflux_energy(const Geo &g, const Dev &d, int I, int J, int K) {
+// FLT: FLOWT(I,J,K)
+__device__ __forceinline__ float tflux_energy(const Geo &g, const Dev &d, int I, int J, int K, float FLT) {
   float AVGHO = d.HO[IX(I, J, K)] + d.HO[IX(I, J - 1, K)] + d.HO[IX(I, J - 1, K + 1)] + d.HO[IX(I, J, K + 1)];
   float AVGPB = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
-  return (d.FLOWT[IX(I, J, K)] * AVGHO + d.WRABT[JKX(J, K)] * AVGPB) * 0.25f;
+  return (FLT * AVGHO + d.WRABT[JKX(J, K)] * AVGPB) * 0.25f;
 }
 
-__device__ __forceinline__ void flux_energy_at(const Geo &g, const Dev &d, int I, int J, int K) {
+// The face fluxes and source a flux formula forms at one node (has_*: the
+// faces it forms there), so that a thread can form several before storing.
+struct FaceFlux {
+  float x = 0.f, r = 0.f, t = 0.f, s = 0.f;
+  bool hx = false, hr = false, ht = false, hs = false;
+};
+__device__ __forceinline__ void store_faces(const FaceFlux &v, float *XF, float *RF, float *TF, float *SRC, size_t n) {
+  if (v.hx) XF[n] = v.x;
+  if (v.hr) RF[n] = v.r;
+  if (v.ht) TF[n] = v.t;
+  if (v.hs) SRC[n] = v.s;
+}
+
+// fx, ft, fr: the node's FLOWX, FLOWT, FLOWR.  On a periodic line FLOWT is
+// the same at I = 1 and IM (their average), so ft serves both ends.
+__device__ __forceinline__ FaceFlux energy_faces(const Geo &g, const Dev &d, int I, int J, int K, float fx, float ft,
+                                                 float fr) {
   const size_t n = IX(I, J, K);
+  FaceFlux v;
   if (I <= g.IMM1 && K <= g.KMM1) {
     float AVGHO = d.HO[IX(I, J, K)] + d.HO[IX(I, J, K + 1)] + d.HO[IX(I + 1, J, K)] + d.HO[IX(I + 1, J, K + 1)];
-    d.XFLUX[n] = d.FLOWX[n] * AVGHO * 0.25f;
-    if (J >= 2) d.SOURCE[n] = d.QSOURCE[n];
+    v.x = fx * AVGHO * 0.25f;
+    v.hx = true;
+    if (J >= 2) {
+      v.s = d.QSOURCE[n];
+      v.hs = true;
+    }
   }
   if (J >= 2 && I <= g.IMM1) {
     float AVGHO = 0.25f * (d.HO[IX(I, J, K)] + d.HO[IX(I, J - 1, K)] + d.HO[IX(I + 1, J - 1, K)] + d.HO[IX(I + 1, J, K)]);
-    d.RFLUX[n] = d.FLOWR[n] * AVGHO;
+    v.r = fr * AVGHO;
+    v.hr = true;
   }
   if (J >= 2 && K <= g.KMM1) {
-    float tf;
     if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM))
-      tf = 0.5f * (tflux_energy(g, d, 1, J, K) + tflux_energy(g, d, g.IM, J, K));
+      v.t = 0.5f * (tflux_energy(g, d, 1, J, K, ft) + tflux_energy(g, d, g.IM, J, K, ft));
     else
-      tf = tflux_energy(g, d, I, J, K);
-    d.TFLUX[n] = tf;
+      v.t = tflux_energy(g, d, I, J, K, ft);
+    v.ht = true;
   }
+  return v;
+}
+__device__ __forceinline__ void flux_energy_at(const Geo &g, const Dev &d, int I, int J, int K) {
+  const size_t n = IX(I, J, K);
+  store_faces(energy_faces(g, d, I, J, K, d.FLOWX[n], d.FLOWT[n], d.FLOWR[n]), d.XFLUX, d.RFLUX, d.TFLUX, d.SOURCE, n);
 }
 __global__ void k_flux_energy(Geo g, Dev d) {
   if (gated(d)) return;
@@ -246,6 +294,16 @@ __device__ __forceinline__ float tflux_axial(const Geo &g, const Dev &d, int I, 
   float AVGVX = d.VX[IX(I, J, K)] + d.VX[IX(I, J - 1, K)] + d.VX[IX(I, J - 1, K + 1)] + d.VX[IX(I, J, K + 1)];
   float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
   return 0.25f * (d.FLOWT[IX(I, J, K)] * AVGVX + d.ABX[IX(I, J, K)] * AVGP);
+}
+
+// the periodic line's value at I = 1 and IM off the blade
+__device__ __forceinline__ float tflux_axial_per(const Geo &g, const Dev &d, size_t n, int J, int K) {
+  const int IM = g.IM;
+  float AVGPCUSP = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)] +
+                   d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)];
+  float AVGVXCUSP = d.VX[IX(1, J, K)] + d.VX[IX(1, J - 1, K)] + d.VX[IX(1, J, K + 1)] + d.VX[IX(1, J - 1, K + 1)] +
+                    d.VX[IX(IM, J, K)] + d.VX[IX(IM, J - 1, K)] + d.VX[IX(IM, J, K + 1)] + d.VX[IX(IM, J - 1, K + 1)];
+  return 0.125f * (d.FLOWT[n] * AVGVXCUSP + AVGPCUSP * d.ABX[n]);
 }
 
 __device__ __forceinline__ void flux_axial_at(const Geo &g, const Dev &d, int I, int J, int K) {
@@ -264,12 +322,7 @@ __device__ __forceinline__ void flux_axial_at(const Geo &g, const Dev &d, int I,
   if (J >= 2 && K <= g.KMM1) {
     float tf;
     if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
-      const int IM = g.IM;
-      float AVGPCUSP = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)] +
-                       d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)];
-      float AVGVXCUSP = d.VX[IX(1, J, K)] + d.VX[IX(1, J - 1, K)] + d.VX[IX(1, J, K + 1)] + d.VX[IX(1, J - 1, K + 1)] +
-                        d.VX[IX(IM, J, K)] + d.VX[IX(IM, J - 1, K)] + d.VX[IX(IM, J, K + 1)] + d.VX[IX(IM, J - 1, K + 1)];
-      tf = 0.125f * (d.FLOWT[n] * AVGVXCUSP + AVGPCUSP * d.ABX[n]);
+      tf = tflux_axial_per(g, d, n, J, K);
     } else {
       tf = tflux_axial(g, d, I, J, K);
     }
@@ -323,6 +376,22 @@ __global__ void k_flux_angular(Geo g, Dev d) {
 // ---------------------------------------------------------------------------
 // radial momentum fluxes and source, mf:5003-5095
 
+// the periodic line's value at I = 1 and IM off the blade
+__device__ __forceinline__ float tflux_radial_per(const Geo &g, const Dev &d, size_t n, int J, int K) {
+  const int IM = g.IM;
+  float AVGVR1 = d.VR[IX(1, J, K)] + d.VR[IX(1, J - 1, K)] + d.VR[IX(1, J - 1, K + 1)] + d.VR[IX(1, J, K + 1)];
+  float AVGPB1 = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J - 1, K + 1)] + d.PEFF[IX(1, J, K + 1)];
+  float TF1 = .25f * (d.FLOWT[IX(1, J, K)] * AVGVR1 + AVGPB1 * d.ABR[IX(1, J, K)]);
+  float AVGVRM = d.VR[IX(IM, J, K)] + d.VR[IX(IM, J - 1, K)] + d.VR[IX(IM, J - 1, K + 1)] + d.VR[IX(IM, J, K + 1)];
+  float AVGPBM = d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J - 1, K + 1)] + d.PEFF[IX(IM, J, K + 1)];
+  float TFM = .25f * (d.FLOWT[IX(IM, J, K)] * AVGVRM + AVGPBM * d.ABR[IX(IM, J, K)]);
+  float AVGP1 = 0.25f * (d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)]);
+  float AVGPM = 0.25f * (d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)]);
+  float T1 = TF1 - AVGP1 * d.ABR[IX(1, J, K)];
+  float T2 = TFM - AVGPM * d.ABR[IX(IM, J, K)];
+  return 0.5f * ((T1 + T2) + (AVGP1 + AVGPM) * d.ABR[n]);
+}
+
 __device__ __forceinline__ void flux_radial_at(const Geo &g, const Dev &d, int I, int J, int K) {
   const size_t n = IX(I, J, K);
   if (I <= g.IMM1 && K <= g.KMM1) {
@@ -334,20 +403,7 @@ __device__ __forceinline__ void flux_radial_at(const Geo &g, const Dev &d, int I
     float AVGVR = d.VR[IX(I, J, K)] + d.VR[IX(I, J - 1, K)] + d.VR[IX(I, J - 1, K + 1)] + d.VR[IX(I, J, K + 1)];
     float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
     float tf = .25f * (d.FLOWT[n] * AVGVR + AVGP * d.ABR[n]);
-    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
-      const int IM = g.IM;
-      float AVGVR1 = d.VR[IX(1, J, K)] + d.VR[IX(1, J - 1, K)] + d.VR[IX(1, J - 1, K + 1)] + d.VR[IX(1, J, K + 1)];
-      float AVGPB1 = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J - 1, K + 1)] + d.PEFF[IX(1, J, K + 1)];
-      float TF1 = .25f * (d.FLOWT[IX(1, J, K)] * AVGVR1 + AVGPB1 * d.ABR[IX(1, J, K)]);
-      float AVGVRM = d.VR[IX(IM, J, K)] + d.VR[IX(IM, J - 1, K)] + d.VR[IX(IM, J - 1, K + 1)] + d.VR[IX(IM, J, K + 1)];
-      float AVGPBM = d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J - 1, K + 1)] + d.PEFF[IX(IM, J, K + 1)];
-      float TFM = .25f * (d.FLOWT[IX(IM, J, K)] * AVGVRM + AVGPBM * d.ABR[IX(IM, J, K)]);
-      float AVGP1 = 0.25f * (d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)]);
-      float AVGPM = 0.25f * (d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)]);
-      float T1 = TF1 - AVGP1 * d.ABR[IX(1, J, K)];
-      float T2 = TFM - AVGPM * d.ABR[IX(IM, J, K)];
-      tf = 0.5f * ((T1 + T2) + (AVGP1 + AVGPM) * d.ABR[n]);
-    }
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) tf = tflux_radial_per(g, d, n, J, K);
     d.TFLUX[n] = tf;
   }
   if (J >= 2 && I <= g.IMM1) {
@@ -374,25 +430,131 @@ __global__ void k_flux_radial(Geo g, Dev d) {
   flux_radial_at(g, d, I, J, K);
 }
 
-// The face fluxes and sources of up to three variables in one launch (grid
-// z = variable x K), each into its own flux arrays; the formula is the
-// variable's kind (energy mf:4357-4456, axial mf:4818-4896, angular
-// mf:4913-4956, radial mf:5003-5095; the turbulence transport's are below).
-__device__ __forceinline__ void sa_fluxes_at(const Geo &g, const Dev &d, int I, int J, int K);
-__global__ void k_flux_eq(Geo g, Dev d0, EqSet e) {
-  if (gated(d0)) return;
-  const int I = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int z = blockIdx.z / g.KM, K = blockIdx.z - z * g.KM + 1;
-  if (I > g.IM || J > g.JM) return;
-  const EqLane &q = eq_pick(e, z);
-  const Dev d = eq_dev(d0, q);
-  switch (q.kind) {
-    case EQ_ENERGY: flux_energy_at(g, d, I, J, K); break;
-    case EQ_SA: sa_fluxes_at(g, d, I, J, K); break;
-    case EQ_AXIAL: flux_axial_at(g, d, I, J, K); break;
-    case EQ_ANGULAR: flux_angular_at(g, d, I, J, K); break;
-    default: flux_radial_at(g, d, I, J, K); break;
+// The mass fluxes with the energy equation's fluxes (into e.q[0]) and, with
+// nq = 2, the turbulence transport's (into e.q[1]) at the same node: the
+// latter two take the node's mass fluxes from the thread instead of reading
+// them back.  FLOWT(1,J,K), which the transport's pitchwise faces at I = IM
+// read, is the node's own value on a periodic line and is formed again on a
+// blade line (mf:3960-4190, 4357-4456, 4275-4335).
+__device__ __forceinline__ FaceFlux sa_faces(const Geo &g, const Dev &d, int I, int J, int K, float fx, float ft,
+                                             float fr, float ft1);
+__global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
+  if (gated(d)) return;
+  NODE_IJK
+  const size_t n = IX(I, J, K);
+  const MassFlux m = massflux_vals(g, d, I, J, K);
+  const FaceFlux fe = energy_faces(g, d, I, J, K, m.fx, m.ft, m.fr);
+  FaceFlux fs;
+  if (nq > 1) {
+    float ft1 = m.ft;
+    if (I == g.IM && J >= 2 && K <= g.KMM1 && d.IND[J - 1] == 1) {
+      ft1 = flowt_raw(g, d, 1, J, K);
+      const int NR = d.NROW[J - 1];
+      if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) ft1 = 0.0f;
+    }
+    fs = sa_faces(g, d, I, J, K, m.fx, m.ft, m.fr, ft1);
+  }
+  massflux_store(g, d, m, n);
+  store_faces(fe, e.q[0].XFLUX, e.q[0].RFLUX, e.q[0].TFLUX, e.q[0].SOURCE, n);
+  if (nq > 1) store_faces(fs, e.q[1].XFLUX, e.q[1].RFLUX, e.q[1].TFLUX, e.q[1].SOURCE, n);
+}
+
+// The three momentum equations' face fluxes and sources at one node in one
+// thread (axial mf:4818-4896, angular mf:4913-4956, radial mf:5003-5095),
+// into the flux arrays of e.q[0], e.q[1] and e.q[2]: the node values they
+// share (PEFF at the eight corners, the mass fluxes) are loaded once and
+// every value before any store.  The expressions and their operand order
+// are those of k_flux_axial, k_flux_angular and k_flux_radial.
+__global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
+  if (gated(d)) return;
+  NODE_IJK
+  const int IM = g.IM;
+  const bool fx = I <= g.IMM1 && K <= g.KMM1, fr = J >= 2 && I <= g.IMM1, ft = J >= 2 && K <= g.KMM1;
+  const bool per = ft && d.IND[J - 1] != 1 && (I == 1 || I == IM);
+  // the corners (I..I+1, J-1..J, K..K+1) with indices clamped into the
+  // grid: a clamped corner is loaded but enters no face that is stored
+  const int Ip = min(I + 1, IM), Jm = max(J - 1, 1), Kp = min(K + 1, g.KM);
+  const unsigned n = IX(I, J, K);
+  const unsigned c001 = IX(I, J, Kp), c100 = IX(Ip, J, K), c101 = IX(Ip, J, Kp);
+  const unsigned cm00 = IX(I, Jm, K), cm01 = IX(I, Jm, Kp), cm10 = IX(Ip, Jm, K), cm11 = IX(Ip, Jm, Kp);
+#define C7(A, v)                                                                                            \
+  const float v##000 = d.A[n], v##001 = d.A[c001], v##100 = d.A[c100], v##101 = d.A[c101], v##m00 = d.A[cm00], \
+              v##m01 = d.A[cm01], v##m10 = d.A[cm10];
+#define C8(A, v) C7(A, v) const float v##m11 = d.A[cm11];
+  C8(PEFF, p)
+  C7(VX, x)
+  C8(VT, t)
+  C7(VR, r)
+  C8(ROVT, w)
+#undef C8
+#undef C7
+  const float flx = d.FLOWX[n], flt = d.FLOWT[n], flr = d.FLOWR[n];
+  const float R0 = d.R[JKX(J, K)], R1 = d.R[JKX(J, Kp)], Rm = d.R[JKX(Jm, K)];
+  float axX = 0.f, axS = 0.f, anX = 0.f, anS = 0.f, raX = 0.f;
+  if (fx) {
+    const float AVGP = p000 + p001 + p101 + p100;
+    const float AVGVX = x000 + x001 + x101 + x100;
+    axS = d.XFORCE[n];
+    axX = 0.25f * (AVGP * d.AQX[n] + flx * AVGVX);
+    const float AVGR = 0.125f * (R0 + R1);
+    const float AVGVT = t000 + t001 + t100 + t101;
+    anS = d.TFORCE[n];
+    anX = flx * AVGR * AVGVT;
+    const float AVGVR = r000 + r001 + r100 + r101;
+    raX = 0.25f * (flx * AVGVR + d.AQR[n] * AVGP);
+  }
+  float axR = 0.f, anR = 0.f, raR = 0.f, raS = 0.f;
+  if (fr) {
+    const float AVGP = p000 + pm00 + pm10 + p100;
+    const float AVGVX = x000 + xm00 + xm10 + x100;
+    axR = 0.25f * (AVGP * d.ASX[n] + flr * AVGVX);
+    const float AVGR = (R0 + Rm) * 0.125f;
+    const float AVGVT = t000 + tm00 + tm10 + t100;
+    anR = flr * AVGVT * AVGR;
+    const float AVGVR = r000 + rm00 + rm10 + r100;
+    raR = 0.25f * (flr * AVGVR + d.ASR[n] * AVGP);
+    if (K <= g.KMM1) {
+      const float AVGRVT = w000 + w001 + wm00 + wm01 + w100 + w101 + wm10 + wm11;
+      const float AVGVT8 = 0.125f * (t000 + tm00 + tm01 + t001 + t100 + tm10 + tm11 + t101);
+      const float AVGP8 = p000 + pm00 + pm01 + p001 + p100 + pm10 + pm11 + p101;
+      raS = d.RFORCE[n] - (AVGRVT * AVGVT8 + AVGP8) * d.VOLOR[n];
+    }
+  }
+  float axT = 0.f, anT = 0.f, raT = 0.f;
+  if (ft) {
+    const float AVGP = p000 + pm00 + pm01 + p001;
+    if (per) {
+      axT = tflux_axial_per(g, d, n, J, K);
+      anT = 0.5f * (tflux_angular(g, d, 1, J, K) + tflux_angular(g, d, IM, J, K));
+      raT = tflux_radial_per(g, d, n, J, K);
+    } else {
+      const float AVGVX = x000 + xm00 + xm01 + x001;
+      axT = 0.25f * (flt * AVGVX + d.ABX[n] * AVGP);
+      const float AVGR = 0.25f * d.RAVG_CELL[JKX(J, K)];
+      const float AVGVT = t000 + tm00 + tm01 + t001;
+      anT = (flt * AVGVT + d.ABT[JKX(J, K)] * AVGP) * AVGR;
+      const float AVGVR = r000 + rm00 + rm01 + r001;
+      raT = .25f * (flt * AVGVR + AVGP * d.ABR[n]);
+    }
+  }
+  const EqLane &qa = e.q[0], &qt = e.q[1], &qr = e.q[2];
+  if (fx) {
+    qa.SOURCE[n] = axS;
+    qa.XFLUX[n] = axX;
+    qt.SOURCE[n] = anS;
+    qt.XFLUX[n] = anX;
+    qr.XFLUX[n] = raX;
+  }
+  if (fr) {
+    qa.RFLUX[n] = axR;
+    qt.RFLUX[n] = anR;
+    qr.RFLUX[n] = raR;
+    if (K <= g.KMM1) qr.SOURCE[n] = raS;
+  }
+  if (ft) {
+    qa.TFLUX[n] = axT;
+    qt.TFLUX[n] = anT;
+    qr.TFLUX[n] = raT;
   }
 }
 
@@ -2503,32 +2665,46 @@ __global__ void k_sa_cellavg(Geo g, Dev d) {
 }
 
 // doubled face fluxes and source of the turbulent viscosity, mf:4279-4330
-__device__ __forceinline__ void sa_fluxes_at(const Geo &g, const Dev &d, int I, int J, int K) {
-  const int IMM1 = g.IMM1, KMM1 = g.KMM1;
+// fx, ft, fr: the node's FLOWX, FLOWT, FLOWR; ft1: FLOWT(1,J,K)
+__device__ __forceinline__ FaceFlux sa_faces(const Geo &g, const Dev &d, int I, int J, int K, float fx, float ft,
+                                             float fr, float ft1) {
+  const int IMM1 = g.IMM1;
   const float *TK = d.TRANS_KVIS;
   const unsigned n = IX(I, J, K);
+  FaceFlux v;
   if (K == g.KM) {
-    if (J >= 2 && I <= IMM1) d.RFLUX[n] = 0.0f;
-    return;
+    v.hr = J >= 2 && I <= IMM1;
+    return v;
   }
   if (I <= IMM1) {
     int JP1 = J + 1;
     if (J == g.JM) JP1 = g.JM;
     if (J == 1) JP1 = 1;
     if (d.INDMIX[J - 1] == 1) JP1 = J;
-    d.XFLUX[n] = d.FLOWX[n] * (TK[n] + TK[IX(I, JP1, K)]);
+    v.x = fx * (TK[n] + TK[IX(I, JP1, K)]);
+    v.hx = true;
   }
-  if (J < 2) return;
+  if (J < 2) return v;
   if (I <= IMM1) {
-    d.SOURCE[n] = -2.0f * d.T_SOURCE[n];
-    d.RFLUX[n] = (K == 1) ? 0.0f : d.FLOWR[n] * (TK[IX(I, J, K - 1)] + TK[n]);
+    v.s = -2.0f * d.T_SOURCE[n];
+    v.r = (K == 1) ? 0.0f : fr * (TK[IX(I, J, K - 1)] + TK[n]);
+    v.hs = v.hr = true;
   }
-  if (I >= 2 && I <= IMM1) d.TFLUX[n] = d.FLOWT[n] * (TK[IX(I - 1, J, K)] + TK[n]);
+  if (I >= 2 && I <= IMM1) {
+    v.t = ft * (TK[IX(I - 1, J, K)] + TK[n]);
+    v.ht = true;
+  }
   if (I == 1 || I == g.IM) {
     const unsigned n1 = IX(1, J, K);
-    d.TFLUX[n] = d.FLOWT[n1] * (TK[IX(IMM1, J, K)] + TK[n1]);
+    v.t = ft1 * (TK[IX(IMM1, J, K)] + TK[n1]);
+    v.ht = true;
   }
-  (void)KMM1;
+  return v;
+}
+__device__ __forceinline__ void sa_fluxes_at(const Geo &g, const Dev &d, int I, int J, int K) {
+  const size_t n = IX(I, J, K);
+  store_faces(sa_faces(g, d, I, J, K, d.FLOWX[n], d.FLOWT[n], d.FLOWR[n], d.FLOWT[IX(1, J, K)]), d.XFLUX, d.RFLUX,
+              d.TFLUX, d.SOURCE, n);
 }
 __global__ void k_sa_fluxes(Geo g, Dev d) {
   if (gated(d)) return;
