@@ -55,26 +55,54 @@ HBM_PEAK_GBS = 8000.0
 # histories in BF16: 2 B less each way per history.
 B_ALG = {("plain", False): 132.0, ("plain", True): 152.0, ("mixed", False): 112.0, ("mixed", True): 128.0}
 
-# Algorithmic bytes per node for one launch of each kernel: every array the
-# kernel reads or writes, counted once (stencil neighbours are re-use, not
-# traffic).  Cell arrays are node-shaped, so per-node == per-cell.
-KERNEL_BYTES_PER_NODE = {
-    "k_prim": 4 * (5 + 7), "k_massflux": 4 * (10 + 9), "k_flux_energy": 4 * (6 + 4),
-    "k_flux_axial": 4 * (9 + 4), "k_flux_angular": 4 * (6 + 4), "k_flux_radial": 4 * (12 + 4),
-    "k_resid_line": 4 * (5 + 2), "k_mg_part": 4, "k_mg_apply_lines": 4 * (3 + 1), "k_mg_apply": 4 * (3 + 1),
-    "k_damp_scatter1": 4 * (2 + 1), "k_damp_scatter4": 4 * (2 + 1), "k_smooth_plane": 4 * 2, "k_peff": 4 * 2,
-    "k_smooth_reg1": 4 * 2, "k_smooth_reg2": 4 * 2, "k_smooth_reg4": 4 * 2, "k_smooth_reg8": 4 * 2,
-    "k_smooth_reg16": 4 * 2,
-    "k_mg_sum_seq": 4, "k_mg_sum_wave": 4, "k_jplane_sums": 4, "k_rolim_check": 4, "k_mach_apply": 4 * 4,
-    "k_stepup_temp": 4 * 6, "k_stepup_step": 4 * 5, "k_smooth_resid": 4 * 2, "k_sa_addstore": 4 * 3,
+# Algorithmic bytes per node and step of each kernel family on the bench
+# workload (Spalart-Allmaras, the fast mode's batched launches): every array
+# a launch reads or writes, counted once (stencil neighbours are re-use, not
+# traffic; (J,K)- and I-shaped arrays are negligible), summed over the
+# step's launches.  Cell arrays are node-shaped, so per-node == per-cell.
+#   k_massflux_eq: ROVX ROVR RO ROVT and six face areas, HO PEFF QSOURCE,
+#     TRANS_KVIS T_SOURCE read; FLOWX/T/R ROWT WT and the energy and
+#     transport fluxes (4 each) written: 15 + 13 arrays
+#   k_flux_mom: PEFF VX VT VR ROVT FLOWX/T/R, eight face areas and VOLOR,
+#     three body forces read; 3 x 4 fluxes written: 18 + 12
+#   k_resid_line: per variable XFLUX TFLUX RFLUX SOURCE and the history
+#     read, STORE and the history written (7; the mixed form's BF16 history
+#     2 B each way), six variables a step in two launches
+#   k_mg_apply_lines: per variable STORE read and written, STEP and RSTEP
+#     once per launch (the variables of a launch share them in L2): 3 + 3
+#     variables
+#   k_damp_scatter*: per variable STORE and the variable read, DSC written,
+#     five variables; k_smooth_reg* / k_smooth_plane: DSC read, the variable
+#     written, five variables
+#   k_smooth_resid: 3 launches of one read, one write
+KERNEL_BYTES_PER_NODE_STEP = {
+    "k_prim": 4 * (5 + 7), "k_massflux_eq": 4 * (15 + 13), "k_flux_mom": 4 * (18 + 12),
+    "k_resid_line": 6 * 4 * 7, "k_resid_line_bf": 6 * (4 * 5 + 2 * 2),
+    "k_mg_apply_lines": 2 * (3 * 4 * 2 + 4 * 2), "k_damp_scatter1": 5 * 4 * 3, "k_damp_scatter4": 5 * 4 * 3,
+    "k_smooth_reg1": 5 * 4 * 2, "k_smooth_reg2": 5 * 4 * 2, "k_smooth_reg4": 5 * 4 * 2,
+    "k_smooth_reg8": 5 * 4 * 2, "k_smooth_reg16": 5 * 4 * 2, "k_smooth_plane": 5 * 4 * 2,
+    "k_smooth_resid": 3 * 4 * 2, "k_sa_cellavg_kvis": 4 * (2 + 3), "k_sa_addstore": 4 * 3, "k_peff": 4 * 2,
 }
 
 # HBM traffic per launch of the hot kernels on this workload, written by
 # `bench.py --pmc-table` from rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each
 # scaled by what the same counter reads for a copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tstep_kernels_config1_sa_pmc_traffic.json")
-PMC_KERNELS = ["k_resid_line", "k_damp_scatter1", "k_smooth_reg8", "k_mg_apply_lines", "k_massflux", "k_prim",
-               "k_flux_radial", "k_flux_axial", "k_flux_energy", "k_flux_angular", "k_smooth_resid"]
+# launches per step of each family on that workload
+LAUNCHES_PER_STEP = {"k_prim": 1, "k_massflux_eq": 1, "k_flux_mom": 1, "k_resid_line": 2, "k_resid_line_bf": 2,
+                     "k_mg_apply_lines": 2, "k_damp_scatter1": 2, "k_damp_scatter4": 2, "k_smooth_reg1": 2,
+                     "k_smooth_reg2": 2, "k_smooth_reg4": 2, "k_smooth_reg8": 2, "k_smooth_reg16": 2,
+                     "k_smooth_plane": 2, "k_smooth_resid": 3, "k_sa_cellavg_kvis": 1, "k_sa_addstore": 1, "k_peff": 1}
+
+
+def alg_bytes_per_launch(kernel: str, nodes: int) -> float:
+    fam = base_name(kernel)
+    n = LAUNCHES_PER_STEP.get(fam)
+    return KERNEL_BYTES_PER_NODE_STEP.get(fam, 0) * nodes / n if n else 0.0
+
+
+PMC_KERNELS = ["k_massflux_eq", "k_resid_line", "k_flux_mom", "k_damp_scatter1", "k_smooth_reg8", "k_mg_apply_lines",
+               "k_prim", "k_smooth_resid", "k_sa_cellavg_kvis"]
 NODES_C1 = 3 * 64 * 64 * 193
 
 
@@ -138,15 +166,20 @@ def _copy_state(st: dict) -> dict:
     return {k: (v if k != "scalars" else dict(v)) for k, v in st.items()}
 
 
-def timed_form(st: dict, mixed: bool, warmup: int, steps: int) -> tuple:
+def timed_form(st: dict, mixed: bool, warmup: int, steps: int, tiles=None) -> tuple:
     """One Solver on the workload: warm-up steps, then `steps` timed steps
     between two device synchronisations.  Returns (node-steps/s, ms/step,
-    the open Solver)."""
+    the open Solver).  tiles = (TK, NW) fixes the tile shapes instead of
+    timing them at the start (so a counter run sees only the step's
+    launches)."""
     from multall_hip.runtime import Solver
     s = st["scalars"]
     nodes = int(s["IM"] * s["JM"] * s["KM"])
     g = Solver(_copy_state(st), exact_order=False)
     g.set_scalar("MIXED", 1 if mixed else 0)
+    if tiles:
+        g.set_scalar("SCAT_TK", tiles[0])
+        g.set_scalar("SMOOTH_WAVES", tiles[1])
     g.loss_init()          # SETUP's initial viscous forces
     g.start()
     g.step(warmup)
@@ -458,7 +491,7 @@ def _counter_rows(path: str, counter: str) -> dict:
         for row in csv.DictReader(f):
             if row["Counter_Name"] != counter:
                 continue
-            name = row["Kernel_Name"].split("(")[0].split("<")[0].strip()
+            name = row["Kernel_Name"].split("(")[0].split("<")[0].strip().split("::")[-1]
             s, n = acc.get(name, (0.0, 0))
             acc[name] = (s + float(row["Counter_Value"]), n + 1)
     return {k: s / n for k, (s, n) in acc.items()}
@@ -481,7 +514,7 @@ def pmc_table(fetch_csv: str, write_csv: str, copy_fetch_csv: str, copy_write_cs
     rows = []
     for k, v in sorted(kernels.items(), key=lambda kv: -(kv[1]["fetch_kib"] + kv[1]["write_kib"])):
         mb = (v["fetch_kib"] * cal["fetch_scale"] + v["write_kib"] * cal["write_scale"]) * 1024 / 1e6
-        alg = KERNEL_BYTES_PER_NODE.get(base_name(k), 0) * NODES_C1 / 1e6
+        alg = alg_bytes_per_launch(k, NODES_C1) / 1e6
         rows.append(f"| {k} | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | {mb:.1f} | {alg:.1f} | "
                     f"{mb / alg if alg else float('nan'):.2f} |")
     md = TRAFFIC_FILE[:-5] + ".md"
@@ -491,7 +524,7 @@ def pmc_table(fetch_csv: str, write_csv: str, copy_fetch_csv: str, copy_write_cs
         f.write("Made by `python3 bench.py --pmc-table` from four rocprofv3 counter passes (from the repository "
                 "root on the MI355X box; `cd /tmp && export TMPDIR=/tmp` first):\n\n```\n")
         f.write("rocprofv3 --pmc FETCH_SIZE --kernel-include-regex '" + "|".join(PMC_KERNELS) + "' \\\n"
-                "    --output-format csv -d <out>/fetch -o run -- python3 bench.py --steps 3 --warmup 1 "
+                "    --output-format csv -d <out>/fetch -o run -- python3 bench.py --steps 3 --warmup 1 --tiles 8 8 "
                 "--no-cpu-baseline --no-converged --no-speedline --no-mixed --no-check\n"
                 "rocprofv3 --pmc WRITE_SIZE  (same filter and command, -d <out>/write)\n"
                 "rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'copy|Copy' --output-format csv -d <out>/cfetch "
@@ -513,7 +546,7 @@ def stats_table(stats_csv: str, steps: int) -> None:
     rows = []
     with open(stats_csv, newline="") as f:
         for row in csv.DictReader(f):
-            name = row["Name"].split("(")[0].split("<")[0].strip()
+            name = row["Name"].split("(")[0].split("<")[0].strip().split("::")[-1]
             rows.append((name, int(row["Calls"]), float(row["TotalDurationNs"]) / 1e3,
                          float(row["AverageNs"]) / 1e3, float(row["Percentage"])))
     tot = sum(r[2] for r in rows)
@@ -551,6 +584,8 @@ def main():
     ap.add_argument("--no-speedline", action="store_true", help="skip the speedline runs")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form run of the workload")
     ap.add_argument("--no-check", action="store_true", help="skip the 100-step checks against the reference")
+    ap.add_argument("--tiles", nargs=2, type=int, metavar=("TK", "NW"),
+                    help="fix the corner-distribution and SMOOTH_VAR tile shapes (no tuning launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (for scaling the PMC byte counters)")
     ap.add_argument("--pmc-table", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
@@ -580,7 +615,7 @@ def main():
     sa = args.ilos >= 100
 
     print("workload: plain FP32", file=sys.stderr, flush=True)
-    value, ms, g = timed_form(st, False, args.warmup, args.steps)
+    value, ms, g = timed_form(st, False, args.warmup, args.steps, args.tiles)
     # kernel durations from dispatch-timestamped HIP events on the library
     # stream (hipExtLaunchKernel), same workload, launched one by one
     g.profile(True)
@@ -592,13 +627,13 @@ def main():
     dom = max(kern, key=lambda k: kern[k][1])
     calls, tot_ms = kern[dom]
     avg_s = tot_ms / calls * 1e-3
-    bytes_launch = KERNEL_BYTES_PER_NODE.get(base_name(dom), 0) * nodes
+    bytes_launch = alg_bytes_per_launch(dom, nodes)
     achieved = bytes_launch / avg_s / 1e9
     gpu_ms_step = sum(ms_ for _, ms_ in kern.values()) / args.steps
     mixed = None
     if not args.no_mixed:
         print("workload: mixed form", file=sys.stderr, flush=True)
-        mv, mms, gm = timed_form(st, True, args.warmup, args.steps)
+        mv, mms, gm = timed_form(st, True, args.warmup, args.steps, args.tiles)
         gm.close()
         b = B_ALG[("mixed", sa)]
         mixed = {"value": mv, "ms_per_step": mms, "dtype": "f32 arithmetic, bf16 SSS histories",

@@ -108,6 +108,7 @@ DAMP_SCATTER_DECL(4, 8)
 __global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
 __global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
+__global__ void k_sa_cellavg_kvis(Geo, Dev, Params);
 __global__ void k_sa_grad(Geo, Dev, Params);
 __global__ void k_sa_visc(Geo, Dev, Params, int);
 __global__ void k_nl_visc(Geo, Dev, Params);
@@ -1150,6 +1151,9 @@ static int make_lanes(mh_ctx *c) {
   const mh_dims &dm = c->dims;
   const size_t N = (size_t)dm.im * dm.jm * dm.km;
   int r;
+  // a node array of zeros (dalloc clears it): continuity's source in the
+  // batched steps, never written
+  if ((r = dalloc(c, &c->zero, N))) return r;
   for (int l = 0; l < kLanes; l++) {
     Lane &L = c->lanes[l];
     if ((r = dalloc(c, &L.XFLUX, N)) || (r = dalloc(c, &L.TFLUX, N)) || (r = dalloc(c, &L.RFLUX, N)) ||
@@ -1203,8 +1207,8 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
   // new density), then, after the boundary conditions, the three momentum
   // equations (fluxes from the step's primitives, PEFF and the body forces).
   // Each variable has its own flux arrays and TSTEP scratch (lanes 0-2;
-  // continuity's fluxes are the mass fluxes, left on lane 0; the energy and
-  // transport fluxes are formed beside them, in the same launch).
+  // continuity's fluxes are the mass-flux arrays; the energy and transport
+  // fluxes are formed beside them, in the same launch).
   // Otherwise one variable at a time in the reference's order.
   const bool batched = c->nlanes == kLanes && !p.EXACT && c->scal[MH_S_LANES] != 0.0;
   if (!batched) {
@@ -1215,7 +1219,12 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     EqLane eq[kMaxEq];
     int ln[kMaxEq] = {0, 1, 2};
     int n = 0;
-    eq[n++] = eq_lane(c, 0, d.RO, d.DRO, 1);
+    // continuity's fluxes are the mass fluxes themselves, with no source
+    eq[n] = eq_lane(c, 0, d.RO, d.DRO, 1);
+    eq[n].XFLUX = d.FLOWX;
+    eq[n].TFLUX = d.FLOWT;
+    eq[n].RFLUX = d.FLOWR;
+    eq[n++].SOURCE = c->zero;
     eq[n++] = eq_lane(c, 1, d.ROE, d.DROE, 3);
     if (p.ILOS >= 200) eq[n++] = eq_lane(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2);
     EqSet ef{};
@@ -1226,8 +1235,8 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     if (p.ILOS >= 100) {
       // cell-average density (mf:4260-4270), then the transport's lower
       // limit and kinematic value
-      LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
-      if (p.ILOS >= 200) LAUNCH(k_sa_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
+      if (p.ILOS >= 200) LAUNCH(k_sa_cellavg_kvis, grid, kNodeBlock, 0, c->stream, g, d, p);
+      else LAUNCH(k_sa_cellavg, grid, kNodeBlock, 0, c->stream, g, d);
       LCHK("sa_transport");
     }
   } else {

@@ -290,4 +290,5 @@ struct mh_ctx {
   // batched time steps (fast mode): the second and third variables' scratch
   int nlanes = 0;
   mh::Lane lanes[mh::kLanes];
+  float *zero = nullptr;  // a node array of zeros (continuity's source)
 };

@@ -84,15 +84,18 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
 }
 
-// Batched launches: grid z-slice z works on variable e.q[z].  The XCD-aware
-// order runs over the whole grid, so each XCD takes a contiguous range of
-// one variable's logical blocks.  Returns the logical block within the
-// slice; z is wavefront-uniform.
+// Batched launches: a workgroup works on variable e.q[z] (gridDim.z
+// variables).  The XCD-aware order runs over the whole grid with the
+// variable fastest, so one XCD takes a contiguous range of tiles for all
+// variables at once and the arrays they share (the time steps, the grid)
+// are fetched into its L2 once.  Returns the logical tile; z is
+// wavefront-uniform.
 __device__ __forceinline__ int eq_block(int &z) {
-  const int nxy = gridDim.x * gridDim.y;
-  const int b = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nxy * gridDim.z);
-  z = b / nxy;
-  return b - z * nxy;
+  const int nz = gridDim.z;
+  const int b = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y * nz);
+  const int t = b / nz;
+  z = b - t * nz;
+  return t;
 }
 // e.q[z] by uniform selects (a dynamic index would copy the set to scratch)
 __device__ __forceinline__ const EqLane &eq_pick(const EqSet &e, int z) {
@@ -200,29 +203,31 @@ __device__ __forceinline__ MassFlux massflux_vals(const Geo &g, const Dev &d, in
   }
   return m;
 }
-// FLOWX, FLOWT, FLOWR, ROWT, WT, and continuity's fluxes (the mass fluxes,
-// no source) in d's flux arrays
-__device__ __forceinline__ void massflux_store(const Geo &g, const Dev &d, const MassFlux &m, size_t n) {
+// FLOWX, FLOWT, FLOWR, ROWT, WT and, with dup, continuity's fluxes (the
+// mass fluxes, no source) in d's flux arrays
+__device__ __forceinline__ void massflux_store(const Geo &g, const Dev &d, const MassFlux &m, size_t n, bool dup) {
   if (m.hx) {
     d.FLOWX[n] = m.fx;
-    d.SOURCE[n] = 0.0f;
-    d.XFLUX[n] = m.fx;
+    if (dup) {
+      d.SOURCE[n] = 0.0f;
+      d.XFLUX[n] = m.fx;
+    }
   }
   d.ROWT[n] = m.rowt;
   d.WT[n] = m.wt;
   if (m.ht) {
     d.FLOWT[n] = m.ft;
-    d.TFLUX[n] = m.ft;
+    if (dup) d.TFLUX[n] = m.ft;
   }
   if (m.hr) {
     d.FLOWR[n] = m.fr;
-    d.RFLUX[n] = m.fr;
+    if (dup) d.RFLUX[n] = m.fr;
   }
 }
 __global__ void k_massflux(Geo g, Dev d) {
   if (gated(d)) return;
   NODE_IJK
-  massflux_store(g, d, massflux_vals(g, d, I, J, K), IX(I, J, K));
+  massflux_store(g, d, massflux_vals(g, d, I, J, K), IX(I, J, K), true);
 }
 
 // ---------------------------------------------------------------------------
@@ -431,7 +436,8 @@ __global__ void k_flux_radial(Geo g, Dev d) {
 }
 
 // The mass fluxes with the energy equation's fluxes (into e.q[0]) and, with
-// nq = 2, the turbulence transport's (into e.q[1]) at the same node: the
+// nq = 2, the turbulence transport's (into e.q[1]) at the same node
+// (continuity reads the mass-flux arrays themselves as its fluxes): the
 // latter two take the node's mass fluxes from the thread instead of reading
 // them back.  FLOWT(1,J,K), which the transport's pitchwise faces at I = IM
 // read, is the node's own value on a periodic line and is formed again on a
@@ -454,7 +460,7 @@ __global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
     }
     fs = sa_faces(g, d, I, J, K, m.fx, m.ft, m.fr, ft1);
   }
-  massflux_store(g, d, m, n);
+  massflux_store(g, d, m, n, false);
   store_faces(fe, e.q[0].XFLUX, e.q[0].RFLUX, e.q[0].TFLUX, e.q[0].SOURCE, n);
   if (nq > 1) store_faces(fs, e.q[1].XFLUX, e.q[1].RFLUX, e.q[1].TFLUX, e.q[1].SOURCE, n);
 }
@@ -2791,15 +2797,30 @@ __global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D)
 }
 
 // lower limit and kinematic value of the turbulent viscosity, mf:4339-4348
-__global__ void k_sa_kvis(Geo g, Dev d, Params p) {
-  if (gated(d)) return;
-  CELL_IJK
-  const unsigned n = IX(I, J, K);
+__device__ __forceinline__ void sa_kvis_at(const Geo &g, const Dev &d, const Params &p, unsigned n, float roavg) {
   const float TRANSVISMIN = 0.1f * d.VISC_LAM[IX(p.IMID, 2, p.KMID)];
   float v = d.TRANS_DYN_VIS[n];
   if (v < TRANSVISMIN) v = TRANSVISMIN;
   d.TRANS_DYN_VIS[n] = v;
-  d.TRANS_KVIS[n] = v / d.ROAVG_CELL[n];
+  d.TRANS_KVIS[n] = v / roavg;
+}
+__global__ void k_sa_kvis(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
+  CELL_IJK
+  const unsigned n = IX(I, J, K);
+  sa_kvis_at(g, d, p, n, d.ROAVG_CELL[n]);
+}
+// the cell-average density (k_sa_cellavg) and then the above, per cell
+__global__ void k_sa_cellavg_kvis(Geo g, Dev d, Params p) {
+  if (gated(d)) return;
+  CELL_IJK
+  const float *RO = d.RO;
+  const unsigned n = IX(I, J, K);
+  const float roavg = 0.125f * (RO[IX(I, J, K)] + RO[IX(I + 1, J, K)] + RO[IX(I + 1, J, K + 1)] + RO[IX(I, J, K + 1)] +
+                                RO[IX(I, J - 1, K)] + RO[IX(I + 1, J - 1, K)] + RO[IX(I + 1, J - 1, K + 1)] +
+                                RO[IX(I, J - 1, K + 1)]);
+  d.ROAVG_CELL[n] = roavg;
+  sa_kvis_at(g, d, p, n, roavg);
 }
 
 // Gauss-theorem gradient of a corner-stored field over cell (I,J,K),
