@@ -83,7 +83,6 @@ __global__ void k_loss_span(Geo, Dev, Params, float);
 __global__ void k_loss_pitch(Geo, Dev, Params, float);
 __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
-__global__ void k_sa_addstore(Geo, Dev, float *, int);
 __global__ void k_resid_line(Geo, Dev, EqSet, Params, MgFast, int, int, int);
 __global__ void k_resid_line_bf(Geo, Dev, EqSet, Params, MgFast, int, int, int);
 __global__ void k_to_bf16(const float *, unsigned short *, unsigned);
@@ -105,10 +104,10 @@ DAMP_SCATTER_DECL(4, 2)
 DAMP_SCATTER_DECL(4, 4)
 DAMP_SCATTER_DECL(4, 8)
 #undef DAMP_SCATTER_DECL
-__global__ void k_sa_mixtransfer(Geo, Dev, Params, float *);
-__global__ void k_sa_dbc(Geo, Dev, Params, float *);
 __global__ void k_sa_kvis(Geo, Dev, Params);
 __global__ void k_sa_cellavg_kvis(Geo, Dev, Params);
+__global__ void k_sa_tail1(Geo, Dev, Params, const float *, int);
+__global__ void k_sa_tail2(Geo, Dev, Params, float *);
 __global__ void k_sa_grad(Geo, Dev, Params);
 __global__ void k_sa_visc(Geo, Dev, Params, int);
 __global__ void k_nl_visc(Geo, Dev, Params);
@@ -897,15 +896,15 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
     // add the (damped) change, carry the pitch average over each mixing
     // plane, smooth once with FAC_SFVIS*SFT, then the periodic/tip/
     // mixing-plane conditions with ILAST = IM-1
+    // (two line kernels: the new value, the pitch average over each mixing
+    // plane and the streamwise and pitchwise smoothing into TMP; then the
+    // spanwise smoothing and the line conditions back into D; the smoothing
+    // factor FAC_SFVIS * SFT with the step's SFT taken on the device)
     const Dev d = dev_of(i);
     float *D = eqs[i].D;
-    LAUNCH(k_sa_addstore, grid, kNodeBlock, 0, st, g, d, D, damp ? 1 : 0);
-    if (nmp > 0) LAUNCH(k_sa_mixtransfer, dim3(nmp), dim3(64), 0, st, g, d, p, D);
-    // smoothing factor FAC_SFVIS * SFT, the step's SFT taken on the device
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)D, d.TMP, p.FAC_SFVIS, 1, 0);
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.TMP, d.DSC, p.FAC_SFVIS, 1, 1);
-    LAUNCH(k_smooth_resid, grid, kNodeBlock, 0, st, g, d, (const float *)d.DSC, D, p.FAC_SFVIS, 1, 2);
-    LAUNCH(k_sa_dbc, dim3(g.KM, g.JM), dim3(64), 0, st, g, d, p, D);
+    const dim3 tg((g.KM + 3) / 4, g.JM);
+    LAUNCH(k_sa_tail1, tg, dim3(256), 0, st, g, d, p, (const float *)D, damp ? 1 : 0);
+    LAUNCH(k_sa_tail2, tg, dim3(256), 0, st, g, d, p, D);
     LCHK("sa_tstep");
   }
   if (nv == 0) return MH_OK;

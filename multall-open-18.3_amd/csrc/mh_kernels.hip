@@ -2720,44 +2720,136 @@ __global__ void k_sa_fluxes(Geo g, Dev d) {
 
 // D += STORE (damped as TSTEP's negative feedback leaves it) on the cells,
 // mf:5716-5744
-__global__ void k_sa_addstore(Geo g, Dev d, float *D, int damp) {
-  if (gated(d)) return;
-  const float DAMP = d.SC[SC_DAMP];
-  CELL_IJK
-  const unsigned n = IX(I, J, K);
-  const float x = d.STORE[n];
-  D[n] = D[n] + (damp ? damped(g, d, d.AVG, x, J, DAMP) : x);
-}
-
-// pitch average carried across each mixing plane, mf:5748-5762 (one
-// wavefront per (interface, K) line)
-__global__ void k_sa_mixtransfer(Geo g, Dev d, Params p, float *D) {
-  if (gated(d)) return;
-  const int t = blockIdx.x;
-  const int NR = t / g.KMM1 + 1, K = t % g.KMM1 + 1;
-  if (NR > g.NROWS - 1) return;
-  const int J = d.JMIX[NR - 1], lane = threadIdx.x;
-  const float FMULT = d.TURBVIS_DAMP[NR - 1];
-  float v[4];
+// The transport variable's TSTEP tail (the damped change added, the pitch
+// average carried over each mixing plane, SMOOTH_RESID's three directional
+// passes, the line conditions) in two line kernels: one wavefront per (J,K)
+// line, four lines per workgroup along K, each pass's statements in the
+// reference's order.
+// k_sa_tail1: the new value D' = D + (damped) STORE (mf:5736-5746) of the
+// lines J-1, J, J+1, with the pitch average carried over each mixing plane
+// (mf:5748-5762: D' of lines JMIX+1 and JMIX+2 is FMULT times the average of
+// line JMIX), then the streamwise and pitchwise smoothing passes of
+// SMOOTH_RESID with FAC_SFVIS*SFT (mf:8700), into TMP (cells only).
+__device__ __forceinline__ void sa_new_line(const Geo &g, const Dev &d, const Params &p, const float *D, int damp,
+                                            float DAMP, int J, int K, int lane, int nc, float (&v)[4]) {
+  int jx = 0;
+  const int role = g.NROWS > 1 ? mp_role(d, J, jx) : 0;
+  const int JS = (role >= 2) ? jx : J;  // the line whose D' this line takes
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const int I = 64 * c + lane + 1;
-    v[c] = (I <= g.IMM1) ? d.FP[I - 1] * D[IX(I, J, K)] : 0.0f;
+    v[c] = 0.0f;
+    if (c >= nc || I > g.IMM1) continue;
+    const unsigned n = IX(I, JS, K);
+    const float x = D[n];
+    if (JS < 2) {
+      v[c] = x;
+    } else {
+      const float st = d.STORE[n];
+      v[c] = x + (damp ? damped(g, d, d.AVG, st, JS, DAMP) : st);
+    }
   }
-  const float AVG = line_sum(v, g.IMM1, p.EXACT);
-  for (int JAV = J + 1; JAV <= J + 2; JAV++)
-    for (int I = lane + 1; I <= g.IMM1; I += 64) D[IX(I, JAV, K)] = FMULT * AVG;
+  if (role >= 2) {
+    const float FMULT = d.TURBVIS_DAMP[d.NROW[jx - 1] - 1];
+    float t[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int I = 64 * c + lane + 1;
+      t[c] = (c < nc && I <= g.IMM1) ? d.FP[I - 1] * v[c] : 0.0f;
+    }
+    const float AVG = line_sum(t, g.IMM1, p.EXACT);
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] = FMULT * AVG;
+  }
+}
+__global__ void __launch_bounds__(256) k_sa_tail1(Geo g, Dev d, Params p, const float *D, int damp) {
+  if (gated(d)) return;
+  __shared__ float sh[4][260];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int K = blockIdx.x * 4 + w + 1, J = blockIdx.y + 1;
+  if (K > g.KMM1) return;
+  const int IMM1 = g.IMM1, IMM2 = g.IM - 2, nc = (IMM1 + 63) / 64;
+  const float DAMP = d.SC[SC_DAMP];
+  const float SMFAC = p.FAC_SFVIS * d.SC[SC_SFT];
+  const float SFAC1 = 1.0f - SMFAC, SFACH = 0.5f * SMFAC;
+  const int NR = d.NROW[J - 1];
+  const int J1P1 = d.JSTART[NR - 1] + 1, J2 = d.JMIX[NR - 1];
+  float s0[4], u[4];
+  sa_new_line(g, d, p, D, damp, DAMP, J, K, lane, nc, s0);
+  if (J < J1P1) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) u[c] = s0[c];
+  } else {
+    // streamwise: J1+1 takes J1+2, JMIX takes JMIX-1, others J-1 and J+1
+    float a[4], b[4];
+    if (J == J1P1) {
+      sa_new_line(g, d, p, D, damp, DAMP, J + 1, K, lane, nc, a);
+#pragma unroll
+      for (int c = 0; c < 4; c++) u[c] = s0[c] + SFACH * (a[c] - s0[c]);
+    } else if (J == J2) {
+      sa_new_line(g, d, p, D, damp, DAMP, J - 1, K, lane, nc, a);
+#pragma unroll
+      for (int c = 0; c < 4; c++) u[c] = s0[c] + SFACH * (a[c] - s0[c]);
+    } else {
+      sa_new_line(g, d, p, D, damp, DAMP, J + 1, K, lane, nc, a);
+      sa_new_line(g, d, p, D, damp, DAMP, J - 1, K, lane, nc, b);
+#pragma unroll
+      for (int c = 0; c < 4; c++) u[c] = SFAC1 * s0[c] + SFACH * (a[c] + b[c]);
+    }
+  }
+  // pitchwise, from the line in LDS
+  float *L = sh[w];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    if (c < nc && I <= IMM1) L[I] = u[c];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int I = 64 * c + lane + 1;
+    if (c >= nc || I > IMM1) continue;
+    float v;
+    if (J < J1P1) v = L[I];
+    else if (I == 1) v = L[I] + SFACH * (L[2] - L[I]);
+    else if (I == IMM1) v = L[I] + SFACH * (L[IMM2] - L[I]);
+    else v = SFAC1 * L[I] + SFACH * (L[I + 1] + L[I - 1]);
+    d.TMP[IX(I, J, K)] = v;
+  }
 }
 
-// periodic, tip-gap and mixing-plane conditions on the cell-centred
-// variable (ILAST = IM-1), mf:5858-5926.  One wavefront per (J,K) line,
-// the line held in LDS.
-__global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D) {
+// k_sa_tail2: the spanwise smoothing pass from TMP, then the periodic,
+// tip-gap and mixing-plane conditions on the line with ILAST = IM-1
+// (mf:5858-5926), into D; lines K = KM (no cells) take only the conditions.
+__global__ void __launch_bounds__(256) k_sa_tail2(Geo g, Dev d, Params p, float *D) {
   if (gated(d)) return;
-  __shared__ float L[260];
-  const int K = blockIdx.x + 1, J = blockIdx.y + 1, lane = threadIdx.x;
-  const int IM = g.IM, IMM1 = g.IMM1, KM = g.KM;
+  __shared__ float sh[4][260];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int K = blockIdx.x * 4 + w + 1, J = blockIdx.y + 1;
+  if (K > g.KM) return;
+  const int IM = g.IM, IMM1 = g.IMM1, KM = g.KM, KMM1 = g.KMM1, KMM2 = KM - 2;
+  const float SMFAC = p.FAC_SFVIS * d.SC[SC_SFT];
+  const float SFAC1 = 1.0f - SMFAC, SFACH = 0.5f * SMFAC;
   const int NR = d.NROW[J - 1];
+  const int J1P1 = d.JSTART[NR - 1] + 1;
+  float *L = sh[w];
+  for (int I = lane + 1; I <= IM; I += 64) {
+    const unsigned n = IX(I, J, K);
+    float v;
+    if (K > KMM1 || I > IMM1) {
+      v = D[n];
+    } else {
+      const float *src = d.TMP;
+      if (J < J1P1) v = src[n];
+      else if (K == 1) v = src[n] + SFACH * (src[IX(I, J, 2)] - src[n]);
+      else if (K == KMM1) v = src[n] + SFACH * (src[IX(I, J, KMM2)] - src[n]);
+      else v = SFAC1 * src[n] + SFACH * (src[IX(I, J, K + 1)] + src[IX(I, J, K - 1)]);
+    }
+    L[I] = v;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   const bool per = !((d.IND[J - 1] == 1) || (d.INDLE[J - 1] == 1));
   bool tip = false;
   if (d.KTIPS[NR - 1] > 0) {
@@ -2767,9 +2859,6 @@ __global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D)
     tip = K >= K1 && K <= K2;
   }
   const bool mp = p.IFMIX != 0 && ((J == d.JMIX[NR - 1] && NR < g.NROWS) || (NR > 1 && J == d.JSTART[NR - 1]));
-  if (!per && !tip && !mp) return;
-  for (int I = lane + 1; I <= IM; I += 64) L[I] = D[IX(I, J, K)];
-  __syncthreads();
   if (lane == 0) {
     if (per) {
       L[1] = 0.5f * (L[1] + L[IMM1]);
@@ -2780,7 +2869,8 @@ __global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D)
       L[IMM1] = L[1];
     }
   }
-  __syncthreads();
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   if (mp) {
     float v[4];
 #pragma unroll
@@ -2790,9 +2880,8 @@ __global__ void __launch_bounds__(64) k_sa_dbc(Geo g, Dev d, Params p, float *D)
     }
     const float AVG = 0.5f * line_sum(v, IMM1, p.EXACT);
     for (int I = lane + 1; I <= IM; I += 64) D[IX(I, J, K)] = AVG;
-  } else if (lane == 0) {
-    D[IX(1, J, K)] = L[1];
-    D[IX(IMM1, J, K)] = L[IMM1];
+  } else {
+    for (int I = lane + 1; I <= IMM1; I += 64) D[IX(I, J, K)] = L[I];
   }
 }
 
