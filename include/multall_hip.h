@@ -176,7 +176,8 @@ enum mh_scalar_id {
   /* library controls (no reference counterpart) */
   MH_S_EXACT_ORDER,   /* 1: reductions in the reference's sequential order */
   MH_S_USE_GRAPHS,    /* 1 (default): steps queued as captured HIP graphs, 0: launch by launch */
-  MH_S_LANES,         /* 1 (default): in the fast mode, independent equations' time steps on three streams */
+  MH_S_LANES,         /* 1 (default): in the fast mode, the time steps of independent equations batched in one
+                         launch per kernel (continuity, energy, turbulence; the three momentum equations) */
   MH_S_MIXED,         /* 1: the mixed form (fast mode only, read at mh_start): the SSS histories DRO..DROE and
                          DEL_DYNVIS stored in BF16, all arithmetic in FP32; 0 (default): FP32 storage */
   MH_S_FLOW_CONV,     /* > 0: also end the run once the inlet mass flow has changed by less than this
