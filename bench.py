@@ -15,7 +15,10 @@ against the reference's 100-step summary of this deck, the roofline of the
 dominant kernel (HIP-event durations on the library's stream, HBM traffic
 from the committed counter table), the step-level fraction of the 8 TB/s HBM
 roofline at the step's algorithmic bytes per node, the reference solver
-timed on this host's CPU (one core, 100 steps differenced), the time to
+timed on this host's CPU (one core, 100 steps differenced), the throughput
+of the other machines of the benchmark set in both forms (the reference's
+six-stage compressor deck; its centrifugal compressor deck, impeller and
+vaned diffuser, on 97 x 81 with Spalart-Allmaras), the time to
 converge Rotor 37 (the reference's own deck as it ships), and the
 16-point compressor speedline run to per-point convergence: plain FP32 one
 point after another, batched FP32, batched mixed, and the reference on 16
@@ -228,6 +231,51 @@ def check_vs_reference(st: dict, mixed: bool) -> dict:
 
 
 CONVERGED_DECK = os.path.join(GOLDEN, "decks", "converged_r37samods.dat")
+# The other machines of the benchmark set, as the reference can state them
+# (one IM and KM for all rows): the multi-stage compressor is the reference's
+# own six-stage compressor test deck (12 rows of 37 x 37 on 1282 J-points,
+# mixing length, total pressure ratio 3.44 at convergence), the centrifugal
+# stage its centrifugal compressor deck (impeller and vaned diffuser) on a
+# uniform 97 x 81 grid with Spalart-Allmaras (deckgen.config4_deck).
+MULTISTAGE_DECK = os.path.join(GOLDEN, "decks", "converged_sixstage.dat")
+CENTRIFUGAL_DECK = os.path.join(GOLDEN, "decks", "config4.dat")
+
+
+def other_workload(path: str, title: str, warmup: int, steps: int) -> dict:
+    """Throughput of another machine: its deck read and set up by the host
+    (wall distances and mixing lengths on the device), then warm-up steps
+    and `steps` timed steps, plain FP32 and mixed, as for the main
+    workload."""
+    from multall_hip import host
+    from multall_hip.runtime import Solver
+    st = host.load_deck(path, xlength_on_host=False)
+    sc = st["scalars"]
+    sc["NSTEPS_MAX"] = warmup + steps + 1000
+    nodes = int(sc["IM"] * sc["JM"] * sc["KM"])
+    out = {"workload": title, "nodes": nodes, "grid": [int(sc["IM"]), int(sc["JM"]), int(sc["KM"])],
+           "rows": int(sc["NROWS"]), "viscous_model": f"ILOS={int(sc['ILOS'])}"}
+    sa = int(sc["ILOS"]) >= 100
+    for mixed in (False, True):
+        g = Solver(_copy_state(st), exact_order=False)
+        g.set_scalar("MIXED", 1 if mixed else 0)
+        if sa:
+            g.set_xlength(host.array("THETA", (sc["KM"], sc["JM"], sc["IM"])),
+                          *[host.array(n, (sc["NROWS"],)) for n in ("XLLIM_IN", "XLLIM_LE", "XLLIM_TE", "XLLIM_DN")])
+        g.loss_init()
+        g.start()
+        g.step(warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        r = g.step(steps)
+        g.sync()
+        dt = time.perf_counter() - t0
+        g.close()
+        assert r["nstep"] == warmup + steps, r
+        b = B_ALG[("mixed" if mixed else "plain", sa)]
+        out["mixed" if mixed else "plain"] = {
+            "value": nodes * steps / dt, "ms_per_step": dt / steps * 1e3,
+            "step_roofline": {"alg_bytes_per_node_step": b, "frac": nodes * steps / dt * b / 1e9 / HBM_PEAK_GBS}}
+    return out
 
 
 def time_to_converged(mixed: bool) -> dict:
@@ -585,6 +633,8 @@ def main():
     ap.add_argument("--no-speedline", action="store_true", help="skip the speedline runs")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form run of the workload")
     ap.add_argument("--no-check", action="store_true", help="skip the 100-step checks against the reference")
+    ap.add_argument("--no-machines", action="store_true",
+                    help="skip the multi-stage compressor and centrifugal stage throughput runs")
     ap.add_argument("--tiles", nargs=2, type=int, metavar=("TK", "NW"),
                     help="fix the corner-distribution and SMOOTH_VAR tile shapes (no tuning launches)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -644,6 +694,14 @@ def main():
     print("checks against the reference's 100-step summary", file=sys.stderr, flush=True)
     check = None if args.no_check else {"plain": check_vs_reference(st, False),
                                         "mixed": None if args.no_mixed else check_vs_reference(st, True)}
+    machines = None
+    if not args.no_machines:
+        print("other machines: multi-stage compressor, centrifugal stage", file=sys.stderr, flush=True)
+        machines = [other_workload(MULTISTAGE_DECK, "multi-stage axial compressor: the reference's six-stage "
+                                   "compressor deck, 12 rows", 10, 50),
+                    other_workload(CENTRIFUGAL_DECK, "centrifugal stage: the reference's centrifugal compressor "
+                                   "deck (impeller + vaned diffuser, mixing plane) on 97 x 81, Spalart-Allmaras",
+                                   10, 50)]
     ttc = None
     if not args.no_converged:
         print("time to converged: Rotor 37", file=sys.stderr, flush=True)
@@ -679,6 +737,7 @@ def main():
         "kernels_ms_per_step": {k: round(ms_ / args.steps, 4) for k, (c, ms_) in
                                 sorted(kern.items(), key=lambda kv: -kv[1][1])},
         "cpu_baseline": cpu,
+        "machines": machines,
         "time_to_converged": ttc,
         "speedline": sl,
     }

@@ -772,5 +772,64 @@ def speedline_deck(base_text: str, i: int, n_points: int = 16, nsteps: int = 600
     parenthesis, so that flang's list-directed read of the comment into a
     CHARACTER variable takes the whole line; no input value changes."""
     im, km = speedline_sizes(n_points, seed)[i]
-    text = regrid_deck(base_text, im, km, speedline_pexit(i, n_points), nsteps)
+    return _plain_comments(regrid_deck(base_text, im, km, speedline_pexit(i, n_points), nsteps))
+
+
+def _plain_comments(text: str) -> str:
+    """A comment line opening with "(" loses that parenthesis, so that
+    flang's list-directed read of the comment into a CHARACTER variable
+    takes the whole line; no input value changes."""
     return "\n".join(ln.replace("(", " ", 1) if ln.strip().startswith("(") else ln for ln in text.split("\n"))
+
+
+def with_spalart_allmaras(text: str, limits=(0.03, 0.03, 0.03, 0.03, 1.0, 0.5),
+                          factors=(0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0)) -> str:
+    """A mixing-length (ILOS = 10) NEW-format deck switched to the
+    Spalart-Allmaras model (ILOS = 200): the ILOS value, the source-factor
+    line the reader takes after the Reynolds-number line when ILOS >= 200
+    (FAC_STMIX .. FAC_PGRAD), and each row's limits line read as XLLIM_IN,
+    XLLIM_LE, XLLIM_TE, XLLIM_DN, FSTURB, TURBVIS_DAMP (multall-open-18.3.f:
+    2062-2075)."""
+    lines = text.split("\n")
+    out = []
+    i = 0
+    nlim = 0
+    while i < len(lines):
+        ln = lines[i]
+        up = ln.upper().strip()
+        if up.startswith("ILOS") and i + 1 < len(lines):
+            rest = lines[i + 1].split()
+            out += [ln, f"{200:10d}" + "".join(f"{int(v):10d}" for v in rest[1:])]
+            i += 2
+            continue
+        if up.startswith("REYNO") and i + 1 < len(lines):
+            out += [ln, lines[i + 1], "  FAC_STMIX, FAC_ST0, FAC_ST1, FAC_ST2, FAC_ST3, FAC_SFVIS, FAC_VORT, FAC_PGRAD",
+                    "".join(f"{v:10.4f}" for v in factors)]
+            i += 2
+            continue
+        if up.startswith("MIXING LENGTH LIMITS"):
+            out.append(ln)
+            i += 1
+            while i < len(lines) and re.match(r"^\s*[-0-9.]", lines[i]):
+                out.append("".join(f"{v:10.4f}" for v in limits))
+                nlim += 1
+                i += 1
+            continue
+        out.append(ln)
+        i += 1
+    assert nlim > 0, "no mixing-length limits block"
+    return "\n".join(out)
+
+
+C4_BASE = "centrif-compr-17.1.dat"   # the reference's centrifugal compressor (impeller + vaned diffuser)
+
+
+def config4_deck(base_text: str, nsteps: int = 100, im: int = 97, km: int = 81) -> str:
+    """The centrifugal-stage workload: the reference's centrifugal compressor
+    test deck (impeller and vaned diffuser, 2 rows of 141 and 146 J-points,
+    a mixing plane between them) with a uniform pitchwise and spanwise grid
+    of IM = 97 by KM = 81 on both rows and the Spalart-Allmaras model:
+    2,254,959 nodes.  (The reference reads IM and KM once for all rows, so
+    the impeller cannot keep a finer grid than the diffuser.)  Comment
+    lines lose a leading parenthesis, as in speedline_deck."""
+    return _plain_comments(with_spalart_allmaras(regrid_deck(base_text, im, km, nsteps=nsteps)))

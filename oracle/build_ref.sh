@@ -11,9 +11,10 @@
 # for the compile (FORTRAN INCLUDE resolves next to the including file first)
 # and deleted afterwards; only the executables stay in oracle/_ref/.
 #
-# Usage: oracle/build_ref.sh [fp32|fp64|fp32w|fp32l|fp64l|all]
+# Usage: oracle/build_ref.sh [fp32|fp64|fp32w|fp32l|fp64l|fp32c|fp64c|all]
 # (all = fp32, fp64, fp32w; the long builds fp32l/fp64l, JM <= 1300 with
-# IM, KM <= 38, are made on request for the longest test decks)
+# IM, KM <= 38, are made on request for the longest test decks, and
+# fp32c/fp64c, IM <= 97, KM <= 81, for the centrifugal-stage workload)
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 SRC=/root/reference/Multall-open-18.3/multall-18.3
@@ -24,6 +25,9 @@ DIMS="ID=65,JD=600,KD=65,MAXKI=65,NRS=21,IG1=30,"
 DIMS_WIDE="ID=96,JD=200,KD=33,MAXKI=96,NRS=21,IG1=30,"
 # long builds for decks of many rows (IM, KM <= 38, JM <= 1300)
 DIMS_LONG="ID=38,JD=1300,KD=38,MAXKI=38,NRS=21,IG1=30,"
+# builds for the centrifugal-stage workload (IM <= 97, JM <= 290, KM <= 81;
+# 32 level-1 multigrid blocks pitchwise at IM = 97)
+DIMS_CENT="ID=97,JD=290,KD=81,MAXKI=97,NRS=21,IG1=40,"
 if [ ! -f "$SRC/multall-open-18.3.f" ]; then
   echo "build_ref: reference sources not present; skipping" >&2
   exit 0
@@ -55,6 +59,8 @@ case "${1:-all}" in
   fp32w) build_one fp32w "" "$DIMS_WIDE" ;;
   fp32l) build_one fp32l "" "$DIMS_LONG" ;;
   fp64l) build_one fp64l "-fdefault-real-8 -fdefault-double-8" "$DIMS_LONG" ;;
+  fp32c) build_one fp32c "" "$DIMS_CENT" ;;
+  fp64c) build_one fp64c "-fdefault-real-8 -fdefault-double-8" "$DIMS_CENT" ;;
   all)  build_one fp32 "" & p1=$!
         build_one fp64 "-fdefault-real-8 -fdefault-double-8" & p2=$!
         build_one fp32w "" "$DIMS_WIDE" & p3=$!

@@ -35,7 +35,9 @@ BUILD_PARAMS = dict(ID=65, JD=600, KD=65, MAXKI=65, NRS=21, IG1=30, JG1=1000, KG
 VARIANT_PARAMS = {"fp32": BUILD_PARAMS, "fp64": BUILD_PARAMS,
                   "fp32w": dict(BUILD_PARAMS, ID=96, JD=200, KD=33, MAXKI=96),
                   "fp32l": dict(BUILD_PARAMS, ID=38, JD=1300, KD=38, MAXKI=38),
-                  "fp64l": dict(BUILD_PARAMS, ID=38, JD=1300, KD=38, MAXKI=38)}
+                  "fp64l": dict(BUILD_PARAMS, ID=38, JD=1300, KD=38, MAXKI=38),
+                  "fp32c": dict(BUILD_PARAMS, ID=97, JD=290, KD=81, MAXKI=97, IG1=40),
+                  "fp64c": dict(BUILD_PARAMS, ID=97, JD=290, KD=81, MAXKI=97, IG1=40)}
 
 _EXPLICIT_REAL = {"MACHLIM", "MACHREL", "MACHCOOL"}
 

@@ -1,10 +1,13 @@
 """Golden summaries of the reference on the benchmark workloads.
 
 For the single stator row (37 x 33 x 97, mixing length), the 1.5-stage
-turbine (3 rows of 64 x 64 x 193, Spalart-Allmaras) and the first and last
+turbine (3 rows of 64 x 64 x 193, Spalart-Allmaras), the first and last
 points of the 16-point compressor speedline (the reference's one-stage
-axial compressor re-gridded, multall_hip.deckgen.speedline_deck), runs the
-reference built by
+axial compressor re-gridded, multall_hip.deckgen.speedline_deck) and the
+centrifugal stage (the reference's centrifugal compressor, impeller and
+vaned diffuser, re-gridded to 97 x 81 with Spalart-Allmaras,
+multall_hip.deckgen.config4_deck; its own reference builds fp32c/fp64c),
+runs the reference built by
 oracle/build_ref.sh for 1 and 100 time steps from the deck's initial guess
 and stores, per case:
 
@@ -55,12 +58,16 @@ from multall_hip import deckgen  # noqa: E402
 
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE"]
 SPEEDLINE_BASE = os.path.join(HERE, "decks", "ref", deckgen.SPEEDLINE_BASE)
+C4_BASE = os.path.join(HERE, "decks", "ref", deckgen.C4_BASE)
 CASES = {
     "config0": lambda n: deckgen.write_deck(deckgen.deck_config0(nsteps=n)),
     "config1": lambda n: deckgen.write_deck(deckgen.deck_config1(nsteps=n)),
     "speedline0": lambda n: deckgen.speedline_deck(open(SPEEDLINE_BASE).read(), 0, nsteps=n),
     "speedline15": lambda n: deckgen.speedline_deck(open(SPEEDLINE_BASE).read(), 15, nsteps=n),
+    "config4": lambda n: deckgen.config4_deck(open(C4_BASE).read(), nsteps=n),
 }
+# the reference builds (oracle/build_ref.sh) a case needs: FP32, FP64
+BUILDS = {"config4": ("fp32c", "fp64c")}
 STEPS = (1, 100)
 DIGESTS = os.path.join(HERE, "config_setup_digests.json")
 
@@ -116,7 +123,7 @@ def make(case: str) -> None:
         txt = CASES[case](n)
         wd = tempfile.mkdtemp(prefix="mhcfg_")
         try:
-            setup, final, _ = refdump.run_reference(txt, "fp32", workdir=wd)
+            setup, final, _ = refdump.run_reference(txt, BUILDS.get(case, ("fp32", "fp64"))[0], workdir=wd)
             sumflx = make_golden.read_global_plt(os.path.join(wd, "global.plt"))
         finally:
             shutil.rmtree(wd, ignore_errors=True)
@@ -154,7 +161,7 @@ def make_fp64(case: str, n: int = 100) -> None:
     import orc
     import refdump
     txt = CASES[case](n)
-    _, final, _ = refdump.run_reference(txt, "fp64")
+    _, final, _ = refdump.run_reference(txt, BUILDS.get(case, ("fp32", "fp64"))[1])
     kmid = int(orc.from_dump(final)["scalars"]["KMID"])
     out = summary({v: final.node(v) for v in VARS}, kmid)
     out["NSTEP"] = np.array(int(final["NSTEP"]))

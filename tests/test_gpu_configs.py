@@ -1,9 +1,11 @@
 """The benchmark workloads against the reference: the single turbine stator
 row (37 x 33 x 97, mixing length), the 1.5-stage turbine (3 rows of
-64 x 64 x 193, Spalart-Allmaras: the bench's own workload) and the first and
+64 x 64 x 193, Spalart-Allmaras: the bench's own workload), the first and
 last points of the 16-point compressor speedline (the reference's one-stage
 axial compressor deck re-gridded, at its choke- and stall-side exit
-pressures), each read and set up by
+pressures) and the centrifugal stage (the reference's centrifugal
+compressor deck, impeller and vaned diffuser, on a uniform 97 x 81 grid with
+Spalart-Allmaras, 2,254,959 nodes), each read and set up by
 the Fortran host (bit for bit the reference's set-up, test_host_setup.py),
 marched 1 and 100 steps in both reduction modes and compared with the
 reference's summaries (tests/golden/make_configs.py): the pitchwise mean of
@@ -32,7 +34,14 @@ from conftest import GOLDEN
 from test_gpu_parity import TOL, TOL_100
 
 VARS = ["RO", "ROVX", "ROVR", "RORVT", "ROE"]
-CASES = ["config0", "config1", "speedline0", "speedline15"]
+CASES = ["config0", "config1", "speedline0", "speedline15", "config4"]
+# The fast mode's one-step bound where the reference's FP32 sums move the
+# result further than 1e-5: the centrifugal stage's superblocks hold about
+# 7e4 cells (96 x 80 x 9), whose sequential FP32 sum in the reference and
+# FP64 tree here differ by up to 1.5e-5 of a variable's scale after one step
+# (measured; the exact mode, summing in the reference's order, equals the
+# reference's mid-span plane to the last bit on this deck).
+TOL_FAST_1 = {"config4": 2e-5}
 
 
 def _summary(name, n):
@@ -90,6 +99,8 @@ def test_config_vs_reference(case, nsteps, exact):
         e = _errors(*mine, (ref[f"{v}_lines"], ref[f"{v}_plane"], float(ref[f"{v}_sum"]) / n), sc_[v], kmid)
         if r64 is None:
             tol = TOL if nsteps == 1 else max(TOL_100, 2.0 * float(ref["spread"]))
+            if nsteps == 1 and not exact:
+                tol = max(tol, TOL_FAST_1.get(case, 0.0))
             print(f"{case} {nsteps} {v}: lines {e[0]:.2e} plane {e[1]:.2e} sum {e[2]:.2e} (bound {tol:.1e})")
             assert max(e) <= tol, (v, e)
         else:
