@@ -82,7 +82,8 @@ B_ALG = {("plain", False): 132.0, ("plain", True): 152.0, ("mixed", False): 112.
 KERNEL_BYTES_PER_NODE_STEP = {
     "k_prim": 4 * (5 + 7), "k_massflux_eq": 4 * (15 + 13), "k_flux_mom": 4 * (18 + 12),
     "k_resid_line": 6 * 4 * 7, "k_resid_line_bf": 6 * (4 * 5 + 2 * 2),
-    "k_mg_apply_lines": 2 * (3 * 4 * 2 + 4 * 2), "k_damp_scatter1": 5 * 4 * 3, "k_damp_scatter4": 5 * 4 * 3,
+    "k_mg_apply_lines": 2 * (3 * 4 * 2 + 4 * 2), "k_damp_scatter1": 5 * 4 * 3, "k_damp_scatter2": 5 * 4 * 3,
+    "k_damp_scatter4": 5 * 4 * 3,
     "k_smooth_reg1": 5 * 4 * 2, "k_smooth_reg2": 5 * 4 * 2, "k_smooth_reg4": 5 * 4 * 2,
     "k_smooth_reg8": 5 * 4 * 2, "k_smooth_reg16": 5 * 4 * 2, "k_smooth_plane": 5 * 4 * 2,
     "k_sa_cellavg_kvis": 4 * (2 + 3), "k_sa_tail1": 4 * (2 + 1), "k_sa_tail2": 4 * 2, "k_peff": 4 * 2,
@@ -94,7 +95,7 @@ KERNEL_BYTES_PER_NODE_STEP = {
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tstep_kernels_config1_sa_pmc_traffic.json")
 # launches per step of each family on that workload
 LAUNCHES_PER_STEP = {"k_prim": 1, "k_massflux_eq": 1, "k_flux_mom": 1, "k_resid_line": 2, "k_resid_line_bf": 2,
-                     "k_mg_apply_lines": 2, "k_damp_scatter1": 2, "k_damp_scatter4": 2, "k_smooth_reg1": 2,
+                     "k_mg_apply_lines": 2, "k_damp_scatter1": 2, "k_damp_scatter2": 2, "k_damp_scatter4": 2, "k_smooth_reg1": 2,
                      "k_smooth_reg2": 2, "k_smooth_reg4": 2, "k_smooth_reg8": 2, "k_smooth_reg16": 2,
                      "k_smooth_plane": 2, "k_sa_cellavg_kvis": 1, "k_sa_tail1": 1, "k_sa_tail2": 1, "k_peff": 1}
 

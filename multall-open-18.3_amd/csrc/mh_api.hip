@@ -47,17 +47,21 @@ __global__ void k_smooth_resid(Geo, Dev, const float *, float *, float, int, int
 __global__ void k_damp_apply(Geo, Dev);
 __global__ void k_scatter(Geo, Dev, const float *, float *);
 __global__ void k_smooth_plane(Geo, Dev, EqSet, Params, StepScalars, int);
-#define SMOOTH_REG_DECL(L, NW) \
-  __global__ void k_smooth_reg##L##w##NW(Geo, Dev, EqSet, Params, StepScalars, int);
-SMOOTH_REG_DECL(1, 8)
-SMOOTH_REG_DECL(2, 8)
-SMOOTH_REG_DECL(4, 8)
-SMOOTH_REG_DECL(8, 8)
-SMOOTH_REG_DECL(16, 8)
-SMOOTH_REG_DECL(1, 16)
-SMOOTH_REG_DECL(2, 16)
-SMOOTH_REG_DECL(4, 16)
-SMOOTH_REG_DECL(8, 16)
+#define SMOOTH_REG_DECL(L, NW, SFX) \
+  __global__ void k_smooth_reg##L##w##NW##SFX(Geo, Dev, EqSet, Params, StepScalars, int);
+#define SMOOTH_REG_DECLS(SFX)  \
+  SMOOTH_REG_DECL(1, 8, SFX)   \
+  SMOOTH_REG_DECL(2, 8, SFX)   \
+  SMOOTH_REG_DECL(4, 8, SFX)   \
+  SMOOTH_REG_DECL(8, 8, SFX)   \
+  SMOOTH_REG_DECL(16, 8, SFX)  \
+  SMOOTH_REG_DECL(1, 16, SFX)  \
+  SMOOTH_REG_DECL(2, 16, SFX)  \
+  SMOOTH_REG_DECL(4, 16, SFX)  \
+  SMOOTH_REG_DECL(8, 16, SFX)
+SMOOTH_REG_DECLS()
+SMOOTH_REG_DECLS(c2)
+#undef SMOOTH_REG_DECLS
 #undef SMOOTH_REG_DECL
 __global__ void k_fill_int(int *, int, int);
 __global__ void k_rolim_check(Geo, Dev, StepScalars);
@@ -100,6 +104,9 @@ __global__ void k_row_avg(Geo, Dev, EqSet, const int *, const int *);
 DAMP_SCATTER_DECL(1, 2)
 DAMP_SCATTER_DECL(1, 4)
 DAMP_SCATTER_DECL(1, 8)
+DAMP_SCATTER_DECL(2, 2)
+DAMP_SCATTER_DECL(2, 4)
+DAMP_SCATTER_DECL(2, 8)
 DAMP_SCATTER_DECL(4, 2)
 DAMP_SCATTER_DECL(4, 4)
 DAMP_SCATTER_DECL(4, 8)
@@ -666,12 +673,15 @@ struct ScatVariant {
   const char *name;
 };
 const ScatVariant &scat_variant(int imm1, int tk) {
-  static const ScatVariant v[2][3] = {
+  static const ScatVariant v[3][3] = {
       {{k_damp_scatter1_k2, "k_damp_scatter1_k2"}, {k_damp_scatter1_k4, "k_damp_scatter1_k4"},
        {k_damp_scatter1_k8, "k_damp_scatter1_k8"}},
+      {{k_damp_scatter2_k2, "k_damp_scatter2_k2"}, {k_damp_scatter2_k4, "k_damp_scatter2_k4"},
+       {k_damp_scatter2_k8, "k_damp_scatter2_k8"}},
       {{k_damp_scatter4_k2, "k_damp_scatter4_k2"}, {k_damp_scatter4_k4, "k_damp_scatter4_k4"},
        {k_damp_scatter4_k8, "k_damp_scatter4_k8"}}};
-  return v[scatter_chunks(imm1) == 1 ? 0 : 1][tk == 2 ? 0 : tk == 4 ? 1 : 2];
+  const int nc = scatter_chunks(imm1);
+  return v[nc == 1 ? 0 : nc == 2 ? 1 : 2][tk == 2 ? 0 : tk == 4 ? 1 : 2];
 }
 dim3 scat_grid(const Geo &g, int tk) { return dim3((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + tk - 1) / tk); }
 
@@ -680,18 +690,23 @@ struct SmoothVariant {
   const char *name;
   int nw;
 };
-const SmoothVariant &smooth_variant(int km, int nw) {
-  static const SmoothVariant w8[5] = {{k_smooth_reg1w8, "k_smooth_reg1w8", 8}, {k_smooth_reg2w8, "k_smooth_reg2w8", 8},
-                                      {k_smooth_reg4w8, "k_smooth_reg4w8", 8}, {k_smooth_reg8w8, "k_smooth_reg8w8", 8},
-                                      {k_smooth_reg16w8, "k_smooth_reg16w8", 8}};
-  static const SmoothVariant w16[4] = {{k_smooth_reg1w16, "k_smooth_reg1w16", 16},
-                                       {k_smooth_reg2w16, "k_smooth_reg2w16", 16},
-                                       {k_smooth_reg4w16, "k_smooth_reg4w16", 16},
-                                       {k_smooth_reg8w16, "k_smooth_reg8w16", 16}};
+// nc: points per lane (1 for IM <= 64, 2 for IM <= 128)
+const SmoothVariant &smooth_variant(int km, int nw, int nc) {
+#define SV(L, NW, SFX) {k_smooth_reg##L##w##NW##SFX, "k_smooth_reg" #L "w" #NW #SFX, NW}
+  static const SmoothVariant w8[2][5] = {{SV(1, 8, ), SV(2, 8, ), SV(4, 8, ), SV(8, 8, ), SV(16, 8, )},
+                                         {SV(1, 8, c2), SV(2, 8, c2), SV(4, 8, c2), SV(8, 8, c2), SV(16, 8, c2)}};
+  static const SmoothVariant w16[2][4] = {{SV(1, 16, ), SV(2, 16, ), SV(4, 16, ), SV(8, 16, )},
+                                          {SV(1, 16, c2), SV(2, 16, c2), SV(4, 16, c2), SV(8, 16, c2)}};
+#undef SV
   const int lines = (km + nw - 1) / nw;
   const int li = lines <= 1 ? 0 : lines <= 2 ? 1 : lines <= 4 ? 2 : lines <= 8 ? 3 : 4;
-  return nw == 16 ? w16[std::min(li, 3)] : w8[li];
+  const int ci = nc > 1 ? 1 : 0;
+  return nw == 16 ? w16[ci][std::min(li, 3)] : w8[ci][li];
 }
+// the register form of SMOOTH_VAR holds IM <= 128 points of a line and
+// KM <= 128 lines of a plane
+bool smooth_reg_fits(const Geo &g) { return g.IM <= 128 && g.KM <= 128; }
+int smooth_chunks(const Geo &g) { return g.IM <= 64 ? 1 : 2; }
 
 // The tuner: for each kernel, one launch to warm up, then three launches
 // between two events on the context's stream, for every shape; the fastest
@@ -763,10 +778,10 @@ int autotune(mh_ctx *c) {
       if (t < best) best = t, best_tk = tk;
     }
   }
-  if (!fnw && g.IM <= 64 && g.KM <= 128) {
+  if (!fnw && smooth_reg_fits(g)) {
     float best = 1e30f;
     for (int nw : kSmoothWaves) {
-      const SmoothVariant &v = smooth_variant(g.KM, nw);
+      const SmoothVariant &v = smooth_variant(g.KM, nw, smooth_chunks(g));
       const float t = time_it([&] {
         hipLaunchKernelGGL(v.k, dim3(g.JM, 1, nz), dim3(64 * v.nw), 0, c->stream, g, d, et, p, s, 1);
       });
@@ -918,12 +933,12 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
                  nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
-  // SMOOTH_VAR: planes in registers where IM <= 64 and KM <= 128, the LDS
+  // SMOOTH_VAR: planes in registers where IM <= 128 and KM <= 128, the LDS
   // form elsewhere and on the exit-smoothing planes
   const int jexit = (!(p.SFEXIT < 0.0001f) && p.NSFEXIT > 0) ? std::max(1, g.JM + 1 - p.NSFEXIT) : g.JM + 1;
   int jreg = 1;  // first plane for the LDS form
-  if (g.IM <= 64 && g.KM <= 128 && jexit > 1) {
-    const SmoothVariant &v = smooth_variant(g.KM, c->smooth_nw);
+  if (smooth_reg_fits(g) && jexit > 1) {
+    const SmoothVariant &v = smooth_variant(g.KM, c->smooth_nw, smooth_chunks(g));
     LAUNCH_NAMED(v.k, v.name, dim3(jexit - 1, 1, nv), dim3(64 * v.nw), 0, st, g, d0, e, p, s, 1);
     jreg = jexit;
   }

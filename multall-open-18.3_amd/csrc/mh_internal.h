@@ -176,11 +176,11 @@ struct Params {
 // Node tile of the damping + corner-distribution kernel (TJ x TK node
 // lines, all I; TK is tuned per grid from kScatTKs) and its dynamic LDS:
 // cell lines (J..J+TJ, K-1..K+TK-1) and the four corner-weight arrays on
-// K-1..K+TK-1, each `pitch` = 64 NC floats (NC = 1 for IM-1 <= 64, else 4)
+// K-1..K+TK-1, each `pitch` = 64 NC floats (NC = 1, 2 or 4 for IM-1 <= 64, 128, 256)
 // plus a zero pad column at each end.
 constexpr int kScatTJ = 4;
 constexpr int kScatTKs[3] = {2, 4, 8};
-inline int scatter_chunks(int imm1) { return imm1 <= 64 ? 1 : 4; }
+inline int scatter_chunks(int imm1) { return imm1 <= 64 ? 1 : imm1 <= 128 ? 2 : 4; }
 inline size_t scatter_lds_bytes(int imm1, int tk) {
   return (size_t)((kScatTJ + 1) * (tk + 1) + 4 * (tk + 1)) * (64 * scatter_chunks(imm1) + 2) * sizeof(float);
 }

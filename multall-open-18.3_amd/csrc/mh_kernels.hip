@@ -1676,6 +1676,9 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
 DAMP_SCATTER(1, 2)
 DAMP_SCATTER(1, 4)
 DAMP_SCATTER(1, 8)
+DAMP_SCATTER(2, 2)
+DAMP_SCATTER(2, 4)
+DAMP_SCATTER(2, 8)
 DAMP_SCATTER(4, 2)
 DAMP_SCATTER(4, 4)
 DAMP_SCATTER(4, 8)
@@ -1926,14 +1929,47 @@ __device__ __forceinline__ float lane_at(float x, int l) {  // lane l's value, w
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-template <int L>
+template <int L, int NC>
 __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const float *SRC, float *DST, const Params &p,
                                            const StepScalars &s, int ncall, int J) {
-  __shared__ float PLs[128 * 64];
-  __shared__ float C2[2][64];
+  // NC chunks of 64 points per lane: point I = 64 c + lane + 1 of chunk c
+  __shared__ float PLs[128 * 64 * NC];
+  __shared__ float C2[2][64 * NC];
+  constexpr int P = 64 * NC;  // LDS line pitch
   const int lane = threadIdx.x & 63, w = wave_id();
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
-  const int I = lane + 1, Ic = min(I, IM);
+  int Iv[NC], Icv[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    Iv[c] = 64 * c + lane + 1;
+    Icv[c] = min(Iv[c], IM);
+  }
+  // the point I+1, I-1 of chunk c (across the chunk boundary by readlane)
+  auto nxt = [&](const float (&x)[NC], int c) -> float {
+    float r = lane_next(x[c]);
+    if (c + 1 < NC) {
+      const float h = lane_at(x[c + 1 < NC ? c + 1 : c], 0);
+      r = (lane == 63) ? h : r;
+    }
+    return r;
+  };
+  auto prv = [&](const float (&x)[NC], int c) -> float {
+    float r = lane_prev(x[c]);
+    if (c > 0) {
+      const float h = lane_at(x[c > 0 ? c - 1 : 0], 63);
+      r = (lane == 0) ? h : r;
+    }
+    return r;
+  };
+  // the value of point i + 1 (i wavefront-uniform)
+  auto at = [&](const float (&x)[NC], int i) -> float {
+    const int c = i >> 6, l = i & 63;
+    float r = lane_at(x[0], l);
+#pragma unroll
+    for (int q = 1; q < NC; q++)
+      if (c == q) r = lane_at(x[q], l);
+    return r;
+  };
   const int kbase = w * L + 1;
   const unsigned plane = (unsigned)IM * (unsigned)g.JM;
   const float SFX = d.SC[SC_SFX], SFX1 = 1.f - SFX, FAC_4TH = p.FAC_4TH;
@@ -1961,13 +1997,15 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
     fkd[l] = d.FKD[K - 1];
   }
   // streamwise smoothing, mf:15369-15445 (loads clamped into the grid)
-  float v[L];
+  float v[L][NC];
   auto stream = [&](auto f) {
 #pragma unroll
-    for (int l = 0; l < L; l++) {
-      const unsigned o = (unsigned)(min(kbase + l, KM) - 1) * plane + (unsigned)(Ic - 1);
-      v[l] = f(o);
-    }
+    for (int l = 0; l < L; l++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const unsigned o = (unsigned)(min(kbase + l, KM) - 1) * plane + (unsigned)(Icv[c] - 1);
+        v[l][c] = f(o);
+      }
   };
 #define S(JJ) SRC[o + (unsigned)((JJ) - 1) * (unsigned)IM]
   switch (cs) {
@@ -1993,97 +2031,132 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
     default: stream([&](unsigned o) { return S(J); }); break;
   }
   // leading-edge points I = 1 and IM (mf:15417-15434)
-  if (smx && (J == JLED - 2 || J == JLED - 1 || J == JLED) && (I == 1 || I == IM)) {
-    const int Io = (I == 1) ? IM : 1;
+  if (smx && (J == JLED - 2 || J == JLED - 1 || J == JLED)) {
 #pragma unroll
-    for (int l = 0; l < L; l++) {
-      const int K = min(kbase + l, KM);
-      const unsigned o = (unsigned)(K - 1) * plane + (unsigned)(Ic - 1);
-      v[l] = (J == JLED) ? SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)])
-                         : SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
+    for (int c = 0; c < NC; c++) {
+      const int I = Iv[c];
+      if (I != 1 && I != IM) continue;
+      const int Io = (I == 1) ? IM : 1;
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const int K = min(kbase + l, KM);
+        const unsigned o = (unsigned)(K - 1) * plane + (unsigned)(Icv[c] - 1);
+        v[l][c] = (J == JLED) ? SFXHM1 * S(J) + SFX14 * (SRC[IX(I, J + 1, K)] + SRC[IX(Io, J, K)])
+                              : SFXHM1 * S(J) + SFX14 * (SRC[IX(2, J, K)] + SRC[IX(IMM1, J, K)]);
+      }
     }
   }
 #undef S
   // pitchwise and spanwise smoothing and corners, mf:15460-15541 (J >= 2)
   if (J >= 2 && !(SFT < 0.0001f)) {
     const bool blade = (d.IND[J - 1] == 1 || d.INDLE[J - 1] == 1);
-    const float FDI = d.FD[Ic - 1], FUI = d.FU[Ic - 1], FU1 = d.FU[0], FUM = d.FU[IM - 1];
-    const bool mid = I >= 2 && I <= IMM1;
+    const float FU1 = d.FU[0], FUM = d.FU[IM - 1];
+    float FDI[NC], FUI[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      FDI[c] = d.FD[Icv[c] - 1];
+      FUI[c] = d.FU[Icv[c] - 1];
+    }
 #pragma unroll
     for (int l = 0; l < L; l++) {
-      const float x = v[l];
-      const float xr = lane_next(x), xl = lane_prev(x);
-      const float x1 = lane_at(x, 0), x2 = lane_at(x, 1), x3 = lane_at(x, 2);
-      const float xm1 = lane_at(x, IMM1 - 1), xm2 = lane_at(x, IMM2 - 1);
-      float avg, cur;
-      if (mid) {
-        avg = FDI * xr + FUI * xl;
-        cur = x - avg;
-      } else if (!blade) {
-        avg = 0.5f * (x2 + xm1);
-        cur = x1 - avg;
-      } else {
-        avg = (I == 1) ? x2 + FU1 * (x2 - x3) : xm1 + FUM * (xm1 - xm2);
-        cur = 0.0f;
+      const float x1 = at(v[l], 0), x2 = at(v[l], 1), x3 = at(v[l], 2);
+      const float xm1 = at(v[l], IMM1 - 1), xm2 = at(v[l], IMM2 - 1);
+      float avg[NC], cur[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int I = Iv[c];
+        const float x = v[l][c];
+        const float xr = nxt(v[l], c), xl = prv(v[l], c);
+        if (I >= 2 && I <= IMM1) {
+          avg[c] = FDI[c] * xr + FUI[c] * xl;
+          cur[c] = x - avg[c];
+        } else if (!blade) {
+          avg[c] = 0.5f * (x2 + xm1);
+          cur[c] = x1 - avg[c];
+        } else {
+          avg[c] = (I == 1) ? x2 + FU1 * (x2 - x3) : xm1 + FUM * (xm1 - xm2);
+          cur[c] = 0.0f;
+        }
       }
-      const float cr = lane_next(cur), cl = lane_prev(cur);
-      const float c2 = lane_at(cur, 1), cm1 = lane_at(cur, IMM1 - 1);
-      float sc;
-      if (mid) sc = FUI * cl + FDI * cr;
-      else if (!blade) sc = 0.5f * (c2 + cm1);
-      else sc = 0.0f;
-      v[l] = SFT1 * x + SFT * (avg + FAC_4TH * sc);
+      const float c2 = at(cur, 1), cm1 = at(cur, IMM1 - 1);
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int I = Iv[c];
+        const float cr = nxt(cur, c), cl = prv(cur, c);
+        float sc;
+        if (I >= 2 && I <= IMM1) sc = FUI[c] * cl + FDI[c] * cr;
+        else if (!blade) sc = 0.5f * (c2 + cm1);
+        else sc = 0.0f;
+        v[l][c] = SFT1 * v[l][c] + SFT * (avg[c] + FAC_4TH * sc);
+      }
     }
     // spanwise, from the whole pitchwise-smoothed plane in LDS
 #pragma unroll
     for (int l = 0; l < L; l++)
-      if (kbase + l <= KM && I <= IM) PLs[(kbase + l - 1) * 64 + lane] = v[l];
-    __syncthreads();
-#define PK(KK) PLs[((KK) - 1) * 64 + lane]
 #pragma unroll
-    for (int l = 0; l < L; l++) {
-      const int K = kbase + l;
-      if (K > KM) break;
-      const float x = v[l];
-      float avg;
-      float sc = 0.0f;
-      // coefficients at K-1, K, K+1: fk?[l], fk?[l+1], fk?[l+2]
-      if (K >= 2 && K <= KMM1) {
-        avg = fkd[l + 1] * PK(K + 1) + fku[l + 1] * PK(K - 1);
-        // CURVE at K-1 and K+1 (zero on the end lines)
-        float cm = 0.0f, cp = 0.0f;
-        if (K - 1 >= 2) cm = PK(K - 1) - (fkd[l] * PK(K) + fku[l] * PK(K - 2));
-        if (K + 1 <= KMM1) cp = PK(K + 1) - (fkd[l + 2] * PK(K + 2) + fku[l + 2] * PK(K));
-        sc = fku[l + 1] * cm + fkd[l + 1] * cp;
-      } else if (K == 1) {
-        avg = PK(2) + 0.5f * fku[l + 1] * (PK(2) - PK(3));
-      } else {
-        avg = PK(KMM1) + 0.5f * fku[l + 1] * (PK(KMM1) - PK(KMM2));
+      for (int c = 0; c < NC; c++)
+        if (kbase + l <= KM && Iv[c] <= IM) PLs[(kbase + l - 1) * P + 64 * c + lane] = v[l][c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+#define PK(KK) PLs[((KK) - 1) * P + 64 * c + lane]
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const int K = kbase + l;
+        if (K > KM) break;
+        const float x = v[l][c];
+        float avg;
+        float sc = 0.0f;
+        // coefficients at K-1, K, K+1: fk?[l], fk?[l+1], fk?[l+2]
+        if (K >= 2 && K <= KMM1) {
+          avg = fkd[l + 1] * PK(K + 1) + fku[l + 1] * PK(K - 1);
+          // CURVE at K-1 and K+1 (zero on the end lines)
+          float cm = 0.0f, cp = 0.0f;
+          if (K - 1 >= 2) cm = PK(K - 1) - (fkd[l] * PK(K) + fku[l] * PK(K - 2));
+          if (K + 1 <= KMM1) cp = PK(K + 1) - (fkd[l + 2] * PK(K + 2) + fku[l + 2] * PK(K));
+          sc = fku[l + 1] * cm + fkd[l + 1] * cp;
+        } else if (K == 1) {
+          avg = PK(2) + 0.5f * fku[l + 1] * (PK(2) - PK(3));
+        } else {
+          avg = PK(KMM1) + 0.5f * fku[l + 1] * (PK(KMM1) - PK(KMM2));
+        }
+        v[l][c] = SFT1 * x + SFT * (avg + FAC_4TH * sc);
       }
-      v[l] = SFT1 * x + SFT * (avg + FAC_4TH * sc);
-    }
 #undef PK
+    }
     // corners: lines 2 and KM-1 to LDS, then the owners of lines 1 and KM
 #pragma unroll
     for (int l = 0; l < L; l++) {
       const int K = kbase + l;
-      if (K == 2) C2[0][lane] = v[l];
-      if (K == KMM1) C2[1][lane] = v[l];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        if (K == 2) C2[0][64 * c + lane] = v[l][c];
+        if (K == KMM1) C2[1][64 * c + lane] = v[l][c];
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int l = 0; l < L; l++) {
       const int K = kbase + l;
       if (K != 1 && K != KM) continue;
-      const float x = v[l];
-      const float xn = lane_next(x), xp = lane_prev(x);
-      const float y = (K == 1) ? C2[0][lane] : C2[1][lane];
-      if (K == 1) {
-        if (I == 1) v[l] = SFT1 * x + SFTH * (xn + y);
-        if (I == IM) v[l] = SFT1 * x + SFTH * (xp + y);
-      } else {
-        if (I == 1) v[l] = SFT1 * x + SFTH * (y + xn);
-        if (I == IM) v[l] = SFT1 * x + SFTH * (xp + y);
+      float xn[NC], xp[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        xn[c] = nxt(v[l], c);
+        xp[c] = prv(v[l], c);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int I = Iv[c];
+        const float x = v[l][c];
+        const float y = (K == 1) ? C2[0][64 * c + lane] : C2[1][64 * c + lane];
+        if (K == 1) {
+          if (I == 1) v[l][c] = SFT1 * x + SFTH * (xn[c] + y);
+          if (I == IM) v[l][c] = SFT1 * x + SFTH * (xp[c] + y);
+        } else {
+          if (I == 1) v[l][c] = SFT1 * x + SFTH * (y + xn[c]);
+          if (I == IM) v[l][c] = SFT1 * x + SFTH * (xp[c] + y);
+        }
       }
     }
   }
@@ -2100,56 +2173,75 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
 #pragma unroll
   for (int l = 0; l < L; l++) {
     const int K = kbase + l;
-    float x = v[l];
     if (per) {
-      const float a = 0.5f * (lane_at(x, 0) + lane_at(x, ILAST - 1));
-      if (I == 1 || I == ILAST) x = a;
+      const float a = 0.5f * (at(v[l], 0) + at(v[l], ILAST - 1));
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        if (Iv[c] == 1 || Iv[c] == ILAST) v[l][c] = a;
     }
     if (K >= K1 && K <= K2) {
-      const float a = 0.5f * (lane_at(x, 0) + lane_at(x, ILAST - 1));
-      if (I == 1 || I == ILAST) x = a;
+      const float a = 0.5f * (at(v[l], 0) + at(v[l], ILAST - 1));
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        if (Iv[c] == 1 || Iv[c] == ILAST) v[l][c] = a;
     }
-    v[l] = x;
   }
   // pitchwise-uniform D on both faces of a mixing plane, mf:5901-5926
   if (p.IFMIX != 0 && ((J == J2 && NR < g.NROWS) || (NR > 1 && J == J1))) {
-    const float FPI = (I <= IMM1) ? d.FP[I - 1] : 0.0f;
 #pragma unroll
     for (int l = 0; l < L; l++) {
-      const float x = v[l];
-      const float xr = lane_next(x);
-      float t[4] = {(I <= IMM1) ? FPI * (x + xr) : 0.0f, 0.f, 0.f, 0.f};
-      v[l] = 0.5f * line_sum(t, IMM1, p.EXACT);
+      float t[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int I = Iv[c];
+        const float xr = nxt(v[l], c);
+        t[c] = (I <= IMM1) ? d.FP[I - 1] * (v[l][c] + xr) : 0.0f;
+      }
+      const float a = 0.5f * line_sum(t, IMM1, p.EXACT);
+#pragma unroll
+      for (int c = 0; c < NC; c++) v[l][c] = a;
     }
   }
   const unsigned ob = (unsigned)(J - 1) * (unsigned)IM + (unsigned)lane;
 #pragma unroll
   for (int l = 0; l < L; l++) {
     const int K = kbase + l;
-    if (K <= KM && I <= IM) DST[ob + (unsigned)(K - 1) * plane] = v[l];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+      if (K <= KM && Iv[c] <= IM) DST[ob + 64u * c + (unsigned)(K - 1) * plane] = v[l][c];
   }
 }
 
 // k_smooth_reg<L>w<NW>: NW wavefronts per J plane, L K lines each
 // batched over the variables of e (grid z): q.DSC smoothed into q.D
-#define SMOOTH_REG(L, NW)                                                                                    \
-  __global__ void __launch_bounds__(64 * NW) k_smooth_reg##L##w##NW(Geo g, Dev d, EqSet e, Params p,         \
-                                                                   StepScalars s, int j0) {                  \
+#define SMOOTH_REG(L, NW, NC, SFX)                                                                           \
+  __global__ void __launch_bounds__(64 * NW) k_smooth_reg##L##w##NW##SFX(Geo g, Dev d, EqSet e, Params p,    \
+                                                                        StepScalars s, int j0) {             \
     if (gated(d)) return;                                                                                    \
     int z;                                                                                                   \
     const int J = eq_block(z) + j0;                                                                          \
     const EqLane &q = eq_pick(e, z);                                                                         \
-    smooth_reg<L>(g, d, q.DSC, q.D, p, s, q.ncall, J);                                                       \
+    smooth_reg<L, NC>(g, d, q.DSC, q.D, p, s, q.ncall, J);                                                   \
   }
-SMOOTH_REG(1, 8)
-SMOOTH_REG(2, 8)
-SMOOTH_REG(4, 8)
-SMOOTH_REG(8, 8)
-SMOOTH_REG(16, 8)
-SMOOTH_REG(1, 16)
-SMOOTH_REG(2, 16)
-SMOOTH_REG(4, 16)
-SMOOTH_REG(8, 16)
+SMOOTH_REG(1, 8, 1, )
+SMOOTH_REG(2, 8, 1, )
+SMOOTH_REG(4, 8, 1, )
+SMOOTH_REG(8, 8, 1, )
+SMOOTH_REG(16, 8, 1, )
+SMOOTH_REG(1, 16, 1, )
+SMOOTH_REG(2, 16, 1, )
+SMOOTH_REG(4, 16, 1, )
+SMOOTH_REG(8, 16, 1, )
+// two points per lane (IM <= 128)
+SMOOTH_REG(1, 8, 2, c2)
+SMOOTH_REG(2, 8, 2, c2)
+SMOOTH_REG(4, 8, 2, c2)
+SMOOTH_REG(8, 8, 2, c2)
+SMOOTH_REG(16, 8, 2, c2)
+SMOOTH_REG(1, 16, 2, c2)
+SMOOTH_REG(2, 16, 2, c2)
+SMOOTH_REG(4, 16, 2, c2)
+SMOOTH_REG(8, 16, 2, c2)
 #undef SMOOTH_REG
 
 // ---------------------------------------------------------------------------

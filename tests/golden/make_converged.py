@@ -127,6 +127,11 @@ def r37_samods() -> str:
     return open(R37).read()
 
 
+def centrifugal() -> str:
+    from multall_hip import deckgen
+    return deckgen.config4_deck(open(os.path.join(os.path.dirname(R37), deckgen.C4_BASE)).read(), nsteps=9000)
+
+
 def shipped(name: str):
     """One of the reference's own multi-row test decks as it ships, except
     that a comment line opening with "(" loses that parenthesis (flang's
@@ -147,30 +152,39 @@ CASES = {
     "axialturb": shipped("Single-stage-turbine-17.1.dat"),
     # the six-stage compressor: twelve rows, eleven mixing planes
     "sixstage": shipped("Six-stage-compr-17.1.dat"),
+    # the centrifugal stage: the centrifugal compressor deck (impeller and
+    # vaned diffuser) on 97 x 81 with Spalart-Allmaras
+    "centrifugal": centrifugal,
 }
-# decks longer than the main builds' JD = 600 run on the long builds
-LONG = {"sixstage"}
+# the reference builds (FP64, FP32) a case runs on: decks longer than the
+# main builds' JD = 600 on the long builds, the centrifugal stage on its own
+BUILDS = {"sixstage": ("fp64l", "fp32l"), "centrifugal": ("fp64c", "fp32c")}
+
+
+def run_one(txt: str, variant: str, real) -> dict:
+    """One reference run to its stop: the step it stopped at, the final
+    per-J integrals and the last EFICOOL block it printed."""
+    import refdump
+    wd = tempfile.mkdtemp(prefix="mhconv_")
+    try:
+        _, final, stdout = refdump.run_reference(txt, variant, workdir=wd, timeout=14400)
+        integ = read_global_plt(os.path.join(wd, "global.plt"), real)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    return {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist(),
+            "eficool": eficool_records(stdout)}
 
 
 def make(case):
-    import refdump
     txt = CASES[case]()
     deck = f"converged_{case}.dat"
     with open(os.path.join(HERE, "decks", deck), "w") as f:
         f.write(txt)
     out = {"deck": f"decks/{deck}"}
-    sfx = "l" if case in LONG else ""
-    for variant, real in (("fp64" + sfx, np.float64), ("fp32" + sfx, np.float32)):
-        wd = tempfile.mkdtemp(prefix="mhconv_")
-        t0 = time.time()
-        try:
-            _, final, stdout = refdump.run_reference(txt, variant, workdir=wd, timeout=7200)
-            integ = read_global_plt(os.path.join(wd, "global.plt"), real)
-        finally:
-            shutil.rmtree(wd, ignore_errors=True)
-        out[variant.rstrip("l")] = {"nstep": int(final["NSTEP"]), "integrals": integ.astype(np.float64).tolist(),
-                                    "eficool": eficool_records(stdout)}
-        print(f"{case} {variant}: stopped at step {out[variant.rstrip('l')]['nstep']} ({time.time() - t0:.0f} s)", flush=True)
+    for (variant, real), key in zip(zip(BUILDS.get(case, ("fp64", "fp32")), (np.float64, np.float32)),
+                                    ("fp64", "fp32")):
+        out[key] = run_one(txt, variant, real)
+        print(f"{case} {variant}: stopped at step {out[key]['nstep']}", flush=True)
     with open(os.path.join(HERE, f"converged_{case}.json"), "w") as f:
         json.dump(out, f)
 
