@@ -70,7 +70,8 @@ B_ALG = {("plain", False): 132.0, ("plain", True): 152.0, ("mixed", False): 112.
 #     three body forces read; 3 x 4 fluxes written: 18 + 12
 #   k_resid_line: per variable XFLUX TFLUX RFLUX SOURCE and the history
 #     read, STORE and the history written (7; the mixed form's BF16 history
-#     2 B each way), six variables a step in two launches
+#     2 B each way), six variables a step in two launches; continuity has
+#     no source array
 #   k_mg_apply_lines: per variable STORE read and written, STEP and RSTEP
 #     once per launch (the variables of a launch share them in L2): 3 + 3
 #     variables
@@ -81,7 +82,7 @@ B_ALG = {("plain", False): 132.0, ("plain", True): 152.0, ("mixed", False): 112.
 #     k_sa_tail2: TMP read, the variable written
 KERNEL_BYTES_PER_NODE_STEP = {
     "k_prim": 4 * (5 + 7), "k_massflux_eq": 4 * (15 + 13), "k_flux_mom": 4 * (18 + 12),
-    "k_resid_line": 6 * 4 * 7, "k_resid_line_bf": 6 * (4 * 5 + 2 * 2),
+    "k_resid_line": 6 * 4 * 7 - 4, "k_resid_line_bf": 6 * (4 * 5 + 2 * 2) - 4,
     "k_mg_apply_lines": 2 * (3 * 4 * 2 + 4 * 2), "k_damp_scatter1": 5 * 4 * 3, "k_damp_scatter2": 5 * 4 * 3,
     "k_damp_scatter4": 5 * 4 * 3,
     "k_smooth_reg1": 5 * 4 * 2, "k_smooth_reg2": 5 * 4 * 2, "k_smooth_reg4": 5 * 4 * 2,

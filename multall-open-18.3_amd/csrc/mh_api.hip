@@ -95,6 +95,7 @@ __global__ void k_mg_levels(Geo, MgFast, EqSet, const int *, const int *, const 
 __global__ void k_mg_apply_lines(Geo, Dev, EqSet);
 __global__ void k_flux_mom(Geo, Dev, EqSet);
 __global__ void k_massflux_eq(Geo, Dev, EqSet, int);
+
 __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
 __global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
 __global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
@@ -1165,9 +1166,6 @@ static int make_lanes(mh_ctx *c) {
   const mh_dims &dm = c->dims;
   const size_t N = (size_t)dm.im * dm.jm * dm.km;
   int r;
-  // a node array of zeros (dalloc clears it): continuity's source in the
-  // batched steps, never written
-  if ((r = dalloc(c, &c->zero, N))) return r;
   for (int l = 0; l < kLanes; l++) {
     Lane &L = c->lanes[l];
     if ((r = dalloc(c, &L.XFLUX, N)) || (r = dalloc(c, &L.TFLUX, N)) || (r = dalloc(c, &L.RFLUX, N)) ||
@@ -1238,7 +1236,7 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     eq[n].XFLUX = d.FLOWX;
     eq[n].TFLUX = d.FLOWT;
     eq[n].RFLUX = d.FLOWR;
-    eq[n++].SOURCE = c->zero;
+    eq[n++].SOURCE = nullptr;
     eq[n++] = eq_lane(c, 1, d.ROE, d.DROE, 3);
     if (p.ILOS >= 200) eq[n++] = eq_lane(c, 2, d.TRANS_DYN_VIS, d.DEL_DYNVIS, 2);
     EqSet ef{};

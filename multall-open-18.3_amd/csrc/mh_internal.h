@@ -202,7 +202,7 @@ struct MgFast {
 // source, and its own TSTEP scratch.  A batched launch covers up to kMaxEq
 // variables whose time steps do not read each other's results, one grid
 // z-slice each.
-struct EqLane {
+struct EqLane {  // SOURCE may be null: no source term
   float *D, *DIFF;
   unsigned short *DIFF16;
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *B1CHG, *B2CHG, *SBCHG, *AVG;
@@ -290,5 +290,4 @@ struct mh_ctx {
   // batched time steps (fast mode): the second and third variables' scratch
   int nlanes = 0;
   mh::Lane lanes[mh::kLanes];
-  float *zero = nullptr;  // a node array of zeros (continuity's source)
 };

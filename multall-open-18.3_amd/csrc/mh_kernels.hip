@@ -1148,7 +1148,7 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<B
       if (I == 1) TF = tavg;
       if (I + 1 == IM) TFP = tavg;
     }
-    float src = d.SOURCE[n];
+    float src = d.SOURCE ? d.SOURCE[n] : 0.0f;  // no source array: continuity's
     if (fexline) {
       const float DFLUX = (xf[c] - AVFL * d.FP[I - 1]) * p.FEXTRAP;
       src = (role == 1) ? src - DFLUX : src + DFLUX;
@@ -1243,7 +1243,7 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
         xm[kk] = d.XFLUX[n - IM];
         tf[kk] = d.TFLUX[n];
         tp[kk] = d.TFLUX[n + 1];
-        sr[kk] = d.SOURCE[n];
+        sr[kk] = d.SOURCE ? d.SOURCE[n] : 0.0f;
         df[kk] = DIFF.ld(n);
       }
 #pragma unroll
