@@ -89,6 +89,9 @@ __global__ void k_sa_cellavg(Geo, Dev);
 __global__ void k_sa_fluxes(Geo, Dev);
 __global__ void k_resid_line(Geo, Dev, EqSet, Params, MgFast, int, int, int);
 __global__ void k_resid_line_bf(Geo, Dev, EqSet, Params, MgFast, int, int, int);
+__global__ void k_resid_line_bfs(Geo, Dev, EqSet, Params, MgFast, int, int, int);
+__global__ void k_mg_apply_lines_bfs(Geo, Dev, EqSet);
+__global__ void k_sa_tail1_bfs(Geo, Dev, Params, const float *, int, unsigned short *);
 __global__ void k_to_bf16(const float *, unsigned short *, unsigned);
 __global__ void k_from_bf16(const unsigned short *, float *, unsigned);
 __global__ void k_mg_levels(Geo, MgFast, EqSet, const int *, const int *, const int *, const int *);
@@ -100,8 +103,9 @@ __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
 __global__ void k_xl_limit(Geo, Dev, const float *, const float *, float *);
 __global__ void k_xl_cells(Geo, Dev, const float *, const float *, float, float, float);
 __global__ void k_row_avg(Geo, Dev, EqSet, const int *, const int *);
-#define DAMP_SCATTER_DECL(NC, TK) \
-  __global__ void k_damp_scatter##NC##_k##TK(Geo, Dev, EqSet, Params, int, int);
+#define DAMP_SCATTER_DECL(NC, TK)                                                  \
+  __global__ void k_damp_scatter##NC##_k##TK(Geo, Dev, EqSet, Params, int, int);   \
+  __global__ void k_damp_scatter##NC##_k##TK##_bfs(Geo, Dev, EqSet, Params, int, int);
 DAMP_SCATTER_DECL(1, 2)
 DAMP_SCATTER_DECL(1, 4)
 DAMP_SCATTER_DECL(1, 8)
@@ -644,6 +648,7 @@ EqLane eq_lane(const mh_ctx *c, int lane, float *D, float *DIFF, int ncall) {
   q.D = D;
   q.DIFF = DIFF;
   q.DIFF16 = hist16(c, DIFF);
+  q.STORE16 = lane ? c->lanes[lane - 1].STORE16 : c->store16;
   q.XFLUX = d.XFLUX; q.TFLUX = d.TFLUX; q.RFLUX = d.RFLUX; q.SOURCE = d.SOURCE;
   q.STORE = d.STORE; q.DSC = d.DSC; q.TMP = d.TMP;
   q.B1CHG = d.B1CHG; q.B2CHG = d.B2CHG; q.SBCHG = d.SBCHG; q.AVG = d.AVG;
@@ -673,16 +678,18 @@ struct ScatVariant {
   void (*k)(Geo, Dev, EqSet, Params, int, int);
   const char *name;
 };
-const ScatVariant &scat_variant(int imm1, int tk) {
-  static const ScatVariant v[3][3] = {
-      {{k_damp_scatter1_k2, "k_damp_scatter1_k2"}, {k_damp_scatter1_k4, "k_damp_scatter1_k4"},
-       {k_damp_scatter1_k8, "k_damp_scatter1_k8"}},
-      {{k_damp_scatter2_k2, "k_damp_scatter2_k2"}, {k_damp_scatter2_k4, "k_damp_scatter2_k4"},
-       {k_damp_scatter2_k8, "k_damp_scatter2_k8"}},
-      {{k_damp_scatter4_k2, "k_damp_scatter4_k2"}, {k_damp_scatter4_k4, "k_damp_scatter4_k4"},
-       {k_damp_scatter4_k8, "k_damp_scatter4_k8"}}};
+// bfs: STORE read from the mixed form's BF16 copy
+const ScatVariant &scat_variant(int imm1, int tk, bool bfs = false) {
+#define SV(NC, TK, SFX) {k_damp_scatter##NC##_k##TK##SFX, "k_damp_scatter" #NC "_k" #TK #SFX}
+  static const ScatVariant v[2][3][3] = {{{SV(1, 2, ), SV(1, 4, ), SV(1, 8, )},
+                                          {SV(2, 2, ), SV(2, 4, ), SV(2, 8, )},
+                                          {SV(4, 2, ), SV(4, 4, ), SV(4, 8, )}},
+                                         {{SV(1, 2, _bfs), SV(1, 4, _bfs), SV(1, 8, _bfs)},
+                                          {SV(2, 2, _bfs), SV(2, 4, _bfs), SV(2, 8, _bfs)},
+                                          {SV(4, 2, _bfs), SV(4, 4, _bfs), SV(4, 8, _bfs)}}};
+#undef SV
   const int nc = scatter_chunks(imm1);
-  return v[nc == 1 ? 0 : nc == 2 ? 1 : 2][tk == 2 ? 0 : tk == 4 ? 1 : 2];
+  return v[bfs ? 1 : 0][nc == 1 ? 0 : nc == 2 ? 1 : 2][tk == 2 ? 0 : tk == 4 ? 1 : 2];
 }
 dim3 scat_grid(const Geo &g, int tk) { return dim3((g.JM + kScatTJ - 1) / kScatTJ, (g.KM + tk - 1) / tk); }
 
@@ -837,8 +844,14 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
   // fast mode: the residual kernel also leaves the multigrid partial sums
   const bool fastmg = mg && !p.EXACT && L.mgf_ok;
   const int kg = fastmg ? p.KR : 4;
+  // the mixed form keeps STORE in BF16 through the fused fast-mode chain
+  // (residual, multigrid update with the damping line sums, damping and
+  // corner distribution); the other chains read it as FP32
+  bool bfs = c->mixed && fused_apply && fastmg;
+  for (int i = 0; i < n; i++) bfs = bfs && eqs[i].STORE16 != nullptr;
   const dim3 rgrid((g.JM - 1 + 3) / 4, (g.KMM1 + kg - 1) / kg, n);
-  if (c->mixed) LAUNCH(k_resid_line_bf, rgrid, lblock, 0, st, g, d0, e, p, mgf, kg, p.IR, fastmg ? 1 : 0);
+  if (bfs) LAUNCH(k_resid_line_bfs, rgrid, lblock, 0, st, g, d0, e, p, mgf, kg, p.IR, 1);
+  else if (c->mixed) LAUNCH(k_resid_line_bf, rgrid, lblock, 0, st, g, d0, e, p, mgf, kg, p.IR, fastmg ? 1 : 0);
   else LAUNCH(k_resid_line, rgrid, lblock, 0, st, g, d0, e, p, mgf, kg, p.IR, fastmg ? 1 : 0);
   LCHK("resid_line");
   if (mg) {
@@ -870,7 +883,9 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
     LCHK("mg_sums");
   }
   if (fused_apply) {
-    LAUNCH(k_mg_apply_lines, dim3((g.JM - 1 + 3) / 4, (g.KMM1 + 3) / 4, n), lblock, 0, st, g, d0, e);
+    const dim3 ag((g.JM - 1 + 3) / 4, (g.KMM1 + 3) / 4, n);
+    if (bfs) LAUNCH(k_mg_apply_lines_bfs, ag, lblock, 0, st, g, d0, e);
+    else LAUNCH(k_mg_apply_lines, ag, lblock, 0, st, g, d0, e);
   } else {
     for (int i = 0; i < n; i++) LAUNCH(k_mg_apply, grid, kNodeBlock, 0, st, g, dev_of(i));
   }
@@ -919,7 +934,8 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
     const Dev d = dev_of(i);
     float *D = eqs[i].D;
     const dim3 tg((g.KM + 3) / 4, g.JM);
-    LAUNCH(k_sa_tail1, tg, dim3(256), 0, st, g, d, p, (const float *)D, damp ? 1 : 0);
+    if (bfs) LAUNCH(k_sa_tail1_bfs, tg, dim3(256), 0, st, g, d, p, (const float *)D, damp ? 1 : 0, eqs[i].STORE16);
+    else LAUNCH(k_sa_tail1, tg, dim3(256), 0, st, g, d, p, (const float *)D, damp ? 1 : 0);
     LAUNCH(k_sa_tail2, tg, dim3(256), 0, st, g, d, p, D);
     LCHK("sa_tstep");
   }
@@ -927,7 +943,7 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
   // damping, second mixing-plane averages (mf:5786-5818) and the corner
   // distribution
   {
-    const ScatVariant &v = scat_variant(g.IMM1, c->scat_tk);
+    const ScatVariant &v = scat_variant(g.IMM1, c->scat_tk, bfs);
     dim3 sg = scat_grid(g, c->scat_tk);
     sg.z = nv;
     LAUNCH_NAMED(v.k, v.name, sg, dim3(256), scatter_lds_bytes(g.IMM1, c->scat_tk), st, g, d0, e, p, damp ? 1 : 0,
@@ -1803,9 +1819,14 @@ int mh_start(mh_ctx *c) {
       if (!c->h16[i] && (r = dalloc(c, &c->h16[i], n))) return r;
       if ((r = hist_convert(c, i, true))) return r;
     }
+    // STORE's BF16 copies (every call writes them before they are read)
+    if (!c->store16 && (r = dalloc(c, &c->store16, n))) return r;
     c->mixed = true;
   }
   if (!p.EXACT && (r = make_lanes(c))) return r;
+  if (c->mixed)
+    for (int l = 0; l < c->nlanes; l++)
+      if (!c->lanes[l].STORE16 && (r = dalloc(c, &c->lanes[l].STORE16, (size_t)g.IM * g.JM * g.KM))) return r;
   if ((r = autotune(c))) return r;
   c->ifend = 0;
   c->nstep = 0;

@@ -204,7 +204,7 @@ struct MgFast {
 // z-slice each.
 struct EqLane {  // SOURCE may be null: no source term
   float *D, *DIFF;
-  unsigned short *DIFF16;
+  unsigned short *DIFF16, *STORE16;  // the mixed form's BF16 history and STORE
   float *XFLUX, *TFLUX, *RFLUX, *SOURCE, *STORE, *DSC, *TMP, *B1CHG, *B2CHG, *SBCHG, *AVG;
   double *RED, *P, *PT;
   int ncall;
@@ -241,6 +241,7 @@ constexpr int kAhead = 8; // chunks (of up to 5 steps) the host may queue ahead 
 // (time steps that do not read each other's results: continuity, energy and
 // the turbulence transport; the three momentum equations).
 struct Lane {
+  unsigned short *STORE16 = nullptr;  // the mixed form's BF16 STORE
   float *XFLUX = nullptr, *TFLUX = nullptr, *RFLUX = nullptr, *SOURCE = nullptr, *STORE = nullptr, *DSC = nullptr,
         *TMP = nullptr, *B1CHG = nullptr, *B2CHG = nullptr, *SBCHG = nullptr, *AVG = nullptr;
   double *RED = nullptr;
@@ -283,6 +284,7 @@ struct mh_ctx {
   float DAMP = 0;                            // host mirror of the device's ramped DAMP (launch choices)
   bool mixed = false;                       // the SSS histories stored in BF16 (set at mh_start)
   unsigned short *h16[6] = {};              // their BF16 copies: DRO, DROVX, DROVR, DRORVT, DROE, DEL_DYNVIS
+  unsigned short *store16 = nullptr;        // the mixed form's BF16 STORE (lane 0; the lanes hold their own)
   float PREFF = 0, ROLIM = 0, ROMIN = 0;
   float SFXIN_s = 0, SFTIN_s = 0;           // scaled by CFL/0.5 at the start of LOOP
   mh_resid last{};

@@ -1117,10 +1117,10 @@ __global__ void k_from_bf16(const unsigned short *src, float *dst, unsigned n) {
   if (i < n) dst[i] = bf16_to_f32(src[i]);
 }
 
-template <bool BF>
-__device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<BF> DIFF, const Params &p, int J, int K,
-                                              int lane, float RATPITCH, bool tipbal, int role, int jx,
-                                              float (&out)[4]) {
+template <bool BF, bool BS>
+__device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<BF> DIFF, Hist<BS> SOUT,
+                                              const Params &p, int J, int K, int lane, float RATPITCH, bool tipbal,
+                                              int role, int jx, float (&out)[4]) {
   const int IM = g.IM, IMM1 = g.IMM1;
   const float tavg = tipbal ? 0.5f * (d.TFLUX[IX(1, J, K)] + d.TFLUX[IX(IM, J, K)]) : 0.0f;
   const float FLOWDIRN = -d.FLOWX[IX(p.IMID, jx, K)];
@@ -1173,7 +1173,7 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<B
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const int I = 64 * c + lane + 1;
-    if (I <= IMM1) d.STORE[IX(I, J, K)] = st[c];
+    if (I <= IMM1) SOUT.st(IX(I, J, K), st[c]);
     out[c] = st[c];
   }
 }
@@ -1187,8 +1187,9 @@ __device__ __forceinline__ void resid_mp_line(const Geo &g, const Dev &d, Hist<B
 // (mf:5621-5640), FP64 per level-1 I block in P[(ib, J, kb)] and per line
 // group in PT[(J, kb)], lane columns summed over the block's K lines first.
 #define PX(ib, J, kb) ((size_t)((ib) - 1) + (size_t)g.NIB1 * ((size_t)((J) - 1) + (size_t)g.JM * (size_t)((kb) - 1)))
-template <bool BF>
-__device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> DIFF, const Params &p, const MgFast &m,
+template <bool BF, bool BS>
+__device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> DIFF, Hist<BS> SOUT, const Params &p,
+                                           const MgFast &m,
                                            int kg, int ir, int part, int bl) {
   __shared__ double sh[4][256];
   const int lane = threadIdx.x & 63, w = wave_id();
@@ -1210,7 +1211,7 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       const int K = kq + kk;
-      if (kk < nk) resid_mp_line(g, d, DIFF, p, J, K, lane, RATPITCH, kts > 0 && K >= kts && K <= kte - 1, role, jx,
+      if (kk < nk) resid_mp_line(g, d, DIFF, SOUT, p, J, K, lane, RATPITCH, kts > 0 && K >= kts && K <= kte - 1, role, jx,
                                  st[kk]);
     }
   } else {
@@ -1257,7 +1258,7 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
         const float DELTA = xf[kk] - xm[kk] * RATPITCH + rf[kk] - rf[kk + 1] + TF - TFP - sr[kk];
         const unsigned n = IX(I, J, kq + kk);
         const float v = p.F1 * DELTA + p.F2 * df[kk];
-        d.STORE[n] = v;
+        SOUT.st(n, v);
         DIFF.st(n, DELTA + p.F3 * df[kk]);
         st[kk][c] = v;
       }
@@ -1284,8 +1285,9 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
   }
 }
 // Batched over the variables of e (grid z), each with its own scratch and
-// multigrid partial sums.
-template <bool BF>
+// multigrid partial sums.  BF: the SSS histories in BF16; BS: the new STORE
+// too (the mixed form's fused fast-mode chain).
+template <bool BF, bool BS>
 __device__ __forceinline__ void resid_line_eq(const Geo &g, const Dev &d0, const EqSet &e, const Params &p,
                                               const MgFast &m0, int kg, int ir, int part) {
   int z;
@@ -1295,20 +1297,22 @@ __device__ __forceinline__ void resid_line_eq(const Geo &g, const Dev &d0, const
   MgFast m = m0;
   m.P = q.P;
   m.PT = q.PT;
-  if constexpr (BF) resid_line<true>(g, d, Hist<true>{q.DIFF16}, p, m, kg, ir, part, bl);
-  else resid_line<false>(g, d, Hist<false>{q.DIFF}, p, m, kg, ir, part, bl);
+  const Hist<BF> DIFF{BF ? (void *)q.DIFF16 : (void *)q.DIFF};
+  const Hist<BS> SOUT{BS ? (void *)q.STORE16 : (void *)q.STORE};
+  resid_line<BF, BS>(g, d, DIFF, SOUT, p, m, kg, ir, part, bl);
 }
-__global__ void __launch_bounds__(256) k_resid_line(Geo g, Dev d, EqSet e, Params p, MgFast m, int kg, int ir,
-                                                    int part) {
-  if (gated(d)) return;
-  resid_line_eq<false>(g, d, e, p, m, kg, ir, part);
-}
-// the same with the SSS histories stored in BF16 (the mixed form)
-__global__ void __launch_bounds__(256) k_resid_line_bf(Geo g, Dev d, EqSet e, Params p, MgFast m, int kg, int ir,
-                                                       int part) {
-  if (gated(d)) return;
-  resid_line_eq<true>(g, d, e, p, m, kg, ir, part);
-}
+#define RESID_LINE(NAME, BF, BS)                                                                              \
+  __global__ void __launch_bounds__(256) NAME(Geo g, Dev d, EqSet e, Params p, MgFast m, int kg, int ir,       \
+                                              int part) {                                                     \
+    if (gated(d)) return;                                                                                     \
+    resid_line_eq<BF, BS>(g, d, e, p, m, kg, ir, part);                                                       \
+  }
+RESID_LINE(k_resid_line, false, false)
+// the same with the SSS histories stored in BF16 (the mixed form), and with
+// the new STORE in BF16 as well
+RESID_LINE(k_resid_line_bf, true, false)
+RESID_LINE(k_resid_line_bfs, true, true)
+#undef RESID_LINE
 
 // Fast-mode multigrid sums (mf:5621-5640), pass 2 (pass 1 is in
 // k_resid_line), by block range: one thread per level-1 block (its J members from
@@ -1384,12 +1388,14 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, EqSet e, con
 // multigrid update (mf:5645-5660), one wavefront per (J, 4 consecutive K
 // lines), leaving each line's FP64 sum of |STORE| for the damping average
 // in red[(J,K)]
-__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) {
-  if (gated(d0)) return;
+template <bool BS>
+__device__ __forceinline__ void mg_apply_lines(const Geo &g, const Dev &d0, const EqSet &e) {
   const int lane = threadIdx.x & 63, w = wave_id();
   int z;
   const int bl = eq_block(z);
-  const Dev d = eq_dev(d0, eq_pick(e, z));
+  const EqLane &q = eq_pick(e, z);
+  const Dev d = eq_dev(d0, q);
+  const Hist<BS> S{BS ? (void *)q.STORE16 : (void *)q.STORE};
   double *const red = d.RED;
   const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
   if (J > g.JM) return;
@@ -1411,7 +1417,7 @@ __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) 
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       const unsigned n = IX(I, J, min(kq + kk, g.KMM1)), b1 = i1 + o1[kk], b2 = i2 + o2[kk];
-      st[kk] = d.STORE[n];
+      st[kk] = S.ld(n);
       sp[kk] = d.STEP[n];
       rs[kk] = d.RSTEP[n];
       c1[kk] = d.B1CHG[b1];
@@ -1423,7 +1429,7 @@ __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) 
     for (int kk = 0; kk < 4; kk++) {
       if (kk >= nk) break;
       const float DELTA = st[kk] * sp[kk] + (c1[kk] * s1[kk] + c2[kk] * s2[kk] + sbs) * rs[kk];
-      d.STORE[IX(I, J, kq + kk)] = DELTA;
+      S.st(IX(I, J, kq + kk), DELTA);
       s[kk] += (double)fabsf(DELTA);
     }
   }
@@ -1436,6 +1442,15 @@ __global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) 
     const double v = lane == 0 ? s[0] : lane == 1 ? s[1] : lane == 2 ? s[2] : s[3];
     red[JKX(J, kq + lane)] = v;
   }
+}
+__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) {
+  if (gated(d0)) return;
+  mg_apply_lines<false>(g, d0, e);
+}
+// the mixed form's fused chain: STORE in BF16
+__global__ void __launch_bounds__(256) k_mg_apply_lines_bfs(Geo g, Dev d0, EqSet e) {
+  if (gated(d0)) return;
+  mg_apply_lines<true>(g, d0, e);
 }
 
 // per-row average |change| from the (J,K) line sums (mf:5677-5693): one
@@ -1496,8 +1511,8 @@ __device__ __forceinline__ float damped(const Geo &g, const Dev &d, const float 
 // as y * rcp(1 + |y| rAD) with rAD = 1 / (A DAMP) per J line; the exact mode
 // keeps the reference's two divisions.  Dynamic LDS: kTL + kWL lines of
 // `pitch` = 64 NC + 2 floats.
-template <int NC, int kTK>
-__device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, const float *STORE,
+template <int NC, int kTK, bool BS>
+__device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const Params &p, Hist<BS> STORE,
                                              const float *AC, const float *D, float *DOUT, int damp, int mpav,
                                              int bl) {
   constexpr int kTJ = kScatTJ, kTL = (kTJ + 1) * (kTK + 1), kMaxLn = (kTL + 3) / 4, kWL = 4 * (kTK + 1);
@@ -1526,7 +1541,7 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const int I = 64 * c + lane + 1;
-      x[u][c] = (in && I <= IMM1) ? STORE[base + (unsigned)(I - 1)] : 0.0f;
+      x[u][c] = (in && I <= IMM1) ? STORE.ld(base + (unsigned)(I - 1)) : 0.0f;
     }
   }
   const int J = j0 + w;
@@ -1664,15 +1679,18 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   }
 }
 // batched over the variables of e (grid z): D = q.D, the result to q.DSC
-#define DAMP_SCATTER(NC, TK)                                                                                 \
-  __global__ void __launch_bounds__(256) k_damp_scatter##NC##_k##TK(Geo g, Dev d0, EqSet e, Params p, int damp, \
-                                                                    int mpav) {                              \
+// (with BS, the mixed form's fused chain, STORE read from its BF16 copy)
+#define DAMP_SCATTER_BS(NC, TK, BS, SFX)                                                                     \
+  __global__ void __launch_bounds__(256) k_damp_scatter##NC##_k##TK##SFX(Geo g, Dev d0, EqSet e, Params p,    \
+                                                                         int damp, int mpav) {               \
     if (gated(d0)) return;                                                                                   \
     int z;                                                                                                   \
     const int bl = eq_block(z);                                                                              \
     const EqLane &q = eq_pick(e, z);                                                                         \
-    damp_scatter<NC, TK>(g, d0, p, q.STORE, q.AVG, q.D, q.DSC, damp, mpav, bl);                             \
+    const Hist<BS> S{BS ? (void *)q.STORE16 : (void *)q.STORE};                                             \
+    damp_scatter<NC, TK, BS>(g, d0, p, S, q.AVG, q.D, q.DSC, damp, mpav, bl);                                \
   }
+#define DAMP_SCATTER(NC, TK) DAMP_SCATTER_BS(NC, TK, false, ) DAMP_SCATTER_BS(NC, TK, true, _bfs)
 DAMP_SCATTER(1, 2)
 DAMP_SCATTER(1, 4)
 DAMP_SCATTER(1, 8)
@@ -1683,6 +1701,7 @@ DAMP_SCATTER(4, 2)
 DAMP_SCATTER(4, 4)
 DAMP_SCATTER(4, 8)
 #undef DAMP_SCATTER
+#undef DAMP_SCATTER_BS
 
 // SMOOTH_VAR (mf:15352-15594) followed by TSTEP's periodic, tip-gap and
 // mixing-plane conditions on D (mf:5858-5926).  One workgroup (8
@@ -2822,8 +2841,10 @@ __global__ void k_sa_fluxes(Geo g, Dev d) {
 // (mf:5748-5762: D' of lines JMIX+1 and JMIX+2 is FMULT times the average of
 // line JMIX), then the streamwise and pitchwise smoothing passes of
 // SMOOTH_RESID with FAC_SFVIS*SFT (mf:8700), into TMP (cells only).
-__device__ __forceinline__ void sa_new_line(const Geo &g, const Dev &d, const Params &p, const float *D, int damp,
-                                            float DAMP, int J, int K, int lane, int nc, float (&v)[4]) {
+template <bool BS>
+__device__ __forceinline__ void sa_new_line(const Geo &g, const Dev &d, Hist<BS> STORE, const Params &p,
+                                            const float *D, int damp, float DAMP, int J, int K, int lane, int nc,
+                                            float (&v)[4]) {
   int jx = 0;
   const int role = g.NROWS > 1 ? mp_role(d, J, jx) : 0;
   const int JS = (role >= 2) ? jx : J;  // the line whose D' this line takes
@@ -2837,7 +2858,7 @@ __device__ __forceinline__ void sa_new_line(const Geo &g, const Dev &d, const Pa
     if (JS < 2) {
       v[c] = x;
     } else {
-      const float st = d.STORE[n];
+      const float st = STORE.ld(n);
       v[c] = x + (damp ? damped(g, d, d.AVG, st, JS, DAMP) : st);
     }
   }
@@ -2854,8 +2875,9 @@ __device__ __forceinline__ void sa_new_line(const Geo &g, const Dev &d, const Pa
     for (int c = 0; c < 4; c++) v[c] = FMULT * AVG;
   }
 }
-__global__ void __launch_bounds__(256) k_sa_tail1(Geo g, Dev d, Params p, const float *D, int damp) {
-  if (gated(d)) return;
+template <bool BS>
+__device__ __forceinline__ void sa_tail1(const Geo &g, const Dev &d, const Params &p, const float *D, int damp,
+                                         Hist<BS> S) {
   __shared__ float sh[4][260];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int K = blockIdx.x * 4 + w + 1, J = blockIdx.y + 1;
@@ -2867,7 +2889,7 @@ __global__ void __launch_bounds__(256) k_sa_tail1(Geo g, Dev d, Params p, const 
   const int NR = d.NROW[J - 1];
   const int J1P1 = d.JSTART[NR - 1] + 1, J2 = d.JMIX[NR - 1];
   float s0[4], u[4];
-  sa_new_line(g, d, p, D, damp, DAMP, J, K, lane, nc, s0);
+  sa_new_line(g, d, S, p, D, damp, DAMP, J, K, lane, nc, s0);
   if (J < J1P1) {
 #pragma unroll
     for (int c = 0; c < 4; c++) u[c] = s0[c];
@@ -2875,16 +2897,16 @@ __global__ void __launch_bounds__(256) k_sa_tail1(Geo g, Dev d, Params p, const 
     // streamwise: J1+1 takes J1+2, JMIX takes JMIX-1, others J-1 and J+1
     float a[4], b[4];
     if (J == J1P1) {
-      sa_new_line(g, d, p, D, damp, DAMP, J + 1, K, lane, nc, a);
+      sa_new_line(g, d, S, p, D, damp, DAMP, J + 1, K, lane, nc, a);
 #pragma unroll
       for (int c = 0; c < 4; c++) u[c] = s0[c] + SFACH * (a[c] - s0[c]);
     } else if (J == J2) {
-      sa_new_line(g, d, p, D, damp, DAMP, J - 1, K, lane, nc, a);
+      sa_new_line(g, d, S, p, D, damp, DAMP, J - 1, K, lane, nc, a);
 #pragma unroll
       for (int c = 0; c < 4; c++) u[c] = s0[c] + SFACH * (a[c] - s0[c]);
     } else {
-      sa_new_line(g, d, p, D, damp, DAMP, J + 1, K, lane, nc, a);
-      sa_new_line(g, d, p, D, damp, DAMP, J - 1, K, lane, nc, b);
+      sa_new_line(g, d, S, p, D, damp, DAMP, J + 1, K, lane, nc, a);
+      sa_new_line(g, d, S, p, D, damp, DAMP, J - 1, K, lane, nc, b);
 #pragma unroll
       for (int c = 0; c < 4; c++) u[c] = SFAC1 * s0[c] + SFACH * (a[c] + b[c]);
     }
@@ -2909,6 +2931,16 @@ __global__ void __launch_bounds__(256) k_sa_tail1(Geo g, Dev d, Params p, const 
     else v = SFAC1 * L[I] + SFACH * (L[I + 1] + L[I - 1]);
     d.TMP[IX(I, J, K)] = v;
   }
+}
+__global__ void __launch_bounds__(256) k_sa_tail1(Geo g, Dev d, Params p, const float *D, int damp) {
+  if (gated(d)) return;
+  sa_tail1<false>(g, d, p, D, damp, Hist<false>{d.STORE});
+}
+// the mixed form's fused chain: STORE from its BF16 copy
+__global__ void __launch_bounds__(256) k_sa_tail1_bfs(Geo g, Dev d, Params p, const float *D, int damp,
+                                                      unsigned short *S16) {
+  if (gated(d)) return;
+  sa_tail1<true>(g, d, p, D, damp, Hist<true>{S16});
 }
 
 // k_sa_tail2: the spanwise smoothing pass from TMP, then the periodic,
