@@ -1196,12 +1196,46 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
   const int J = (bl % gridDim.x) * 4 + w + 2, kb = bl / gridDim.x, kq = kb * kg + 1;
   if (J > g.JM) return;
   const int IM = g.IM, IMM1 = g.IMM1;
+  const int nk = min(kg, g.KMM1 - kq + 1);
+  // the faces, source and history of the wavefront's lines (nk of them,
+  // the next line's RFLUX too) in one 64-point chunk of I: all loads of the
+  // lines before any store (the arrays may alias as far as the compiler
+  // knows, so a store would hold back later loads); only the wavefront's
+  // own nk lines (with kg = 3 the fourth belongs to the next wavefront)
+  struct Faces {
+    float rf[5], xf[4], xm[4], tf[4], tp[4], sr[4], df[4];
+  };
+  auto load = [&](int c, Faces &F) {
+    const int I = 64 * c + lane + 1;
+    const bool ok = I <= IMM1;
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) F.xf[kk] = F.xm[kk] = F.tf[kk] = F.tp[kk] = F.sr[kk] = F.df[kk] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk <= 4; kk++) F.rf[kk] = (ok && kk <= nk) ? d.RFLUX[IX(I, J, min(kq + kk, g.KM))] : 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      if (kk >= nk) break;
+      if (!ok) continue;
+      const unsigned n = IX(I, J, min(kq + kk, g.KMM1));
+      F.xf[kk] = d.XFLUX[n];
+      F.xm[kk] = d.XFLUX[n - IM];
+      F.tf[kk] = d.TFLUX[n];
+      F.tp[kk] = d.TFLUX[n + 1];
+      F.sr[kk] = d.SOURCE ? d.SOURCE[n] : 0.0f;
+      F.df[kk] = DIFF.ld(n);
+    }
+  };
+  // the first chunk's loads go out before the step's gate and the row
+  // lookups (their addresses depend on nothing loaded); a mixing-plane line
+  // does not use them
+  Faces F0;
+  load(0, F0);
+  if (gated(d)) return;
   const int NR = d.NROW[J - 1];
   const float RATPITCH = (float)d.NBLADE[J - 2] / (float)d.NBLADE[J - 1];
   const int kts = d.KTIPS[NR - 1], kte = d.KTIPE[NR - 1];
   int jx = 0;
   const int role = g.NROWS > 1 ? mp_role(d, J, jx) : 0;
-  const int nk = min(kg, g.KMM1 - kq + 1);
   float st[4][4];  // [line][I chunk]: the new STORE, zero off the grid
 #pragma unroll
   for (int kk = 0; kk < 4; kk++)
@@ -1227,26 +1261,13 @@ __device__ __forceinline__ void resid_line(const Geo &g, const Dev &d, Hist<BF> 
     for (int c = 0; c < 4; c++) {
       const int I = 64 * c + lane + 1;
       if (64 * c >= IMM1) break;
+      Faces Fc;
+      if (c > 0) load(c, Fc);
+      const Faces &F = c == 0 ? F0 : Fc;
       if (I > IMM1) continue;
-      // all loads of the lines before any store (the arrays may alias as
-      // far as the compiler knows, so a store would hold back later loads)
-      // (only the wavefront's own nk lines: with kg = 3 the fourth belongs
-      // to the next wavefront)
-      float rf[5] = {}, xf[4] = {}, xm[4] = {}, tf[4] = {}, tp[4] = {}, sr[4] = {}, df[4] = {};
-#pragma unroll
-      for (int kk = 0; kk <= 4; kk++)
-        if (kk <= nk) rf[kk] = d.RFLUX[IX(I, J, min(kq + kk, g.KM))];
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++) {
-        if (kk >= nk) break;
-        const unsigned n = IX(I, J, min(kq + kk, g.KMM1));
-        xf[kk] = d.XFLUX[n];
-        xm[kk] = d.XFLUX[n - IM];
-        tf[kk] = d.TFLUX[n];
-        tp[kk] = d.TFLUX[n + 1];
-        sr[kk] = d.SOURCE ? d.SOURCE[n] : 0.0f;
-        df[kk] = DIFF.ld(n);
-      }
+      const float(&rf)[5] = F.rf;
+      const float(&xf)[4] = F.xf, (&xm)[4] = F.xm, (&tf)[4] = F.tf, (&tp)[4] = F.tp, (&sr)[4] = F.sr,
+                 (&df)[4] = F.df;
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) {
         if (kk >= nk) break;
@@ -1301,10 +1322,10 @@ __device__ __forceinline__ void resid_line_eq(const Geo &g, const Dev &d0, const
   const Hist<BS> SOUT{BS ? (void *)q.STORE16 : (void *)q.STORE};
   resid_line<BF, BS>(g, d, DIFF, SOUT, p, m, kg, ir, part, bl);
 }
+// (the step's gate is checked inside, after the first loads)
 #define RESID_LINE(NAME, BF, BS)                                                                              \
   __global__ void __launch_bounds__(256) NAME(Geo g, Dev d, EqSet e, Params p, MgFast m, int kg, int ir,       \
                                               int part) {                                                     \
-    if (gated(d)) return;                                                                                     \
     resid_line_eq<BF, BS>(g, d, e, p, m, kg, ir, part);                                                       \
   }
 RESID_LINE(k_resid_line, false, false)
@@ -1550,6 +1571,9 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   float dv0[kTK];
 #pragma unroll
   for (int kk = 0; kk < kTK; kk++) dv0[kk] = (jin && k0 + kk <= KM && lane < IM) ? D[nbase + kk * plane + lane] : 0.0f;
+  // the step's gate only after these loads are out (their addresses depend
+  // on nothing loaded); the whole workgroup sees the same gate
+  if (gated(d)) return;
   // per-line and per-J metadata
   if (t < kTL) {
     const int jc = j0 + t % (kTJ + 1), kc = k0 - 1 + t / (kTJ + 1);
@@ -1679,11 +1703,11 @@ __device__ __forceinline__ void damp_scatter(const Geo &g, const Dev &d, const P
   }
 }
 // batched over the variables of e (grid z): D = q.D, the result to q.DSC
-// (with BS, the mixed form's fused chain, STORE read from its BF16 copy)
+// (with BS, the mixed form's fused chain, STORE read from its BF16 copy);
+// the step's gate is checked inside, after the first loads
 #define DAMP_SCATTER_BS(NC, TK, BS, SFX)                                                                     \
   __global__ void __launch_bounds__(256) k_damp_scatter##NC##_k##TK##SFX(Geo g, Dev d0, EqSet e, Params p,    \
                                                                          int damp, int mpav) {               \
-    if (gated(d0)) return;                                                                                   \
     int z;                                                                                                   \
     const int bl = eq_block(z);                                                                              \
     const EqLane &q = eq_pick(e, z);                                                                         \
@@ -1948,11 +1972,12 @@ __device__ __forceinline__ float lane_at(float x, int l) {  // lane l's value, w
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-template <int L, int NC>
+template <int L, int NW, int NC>
 __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const float *SRC, float *DST, const Params &p,
                                            const StepScalars &s, int ncall, int J) {
-  // NC chunks of 64 points per lane: point I = 64 c + lane + 1 of chunk c
-  __shared__ float PLs[128 * 64 * NC];
+  // NC chunks of 64 points per lane: point I = 64 c + lane + 1 of chunk c;
+  // the NW wavefronts' L lines each cover the plane's KM <= L NW lines
+  __shared__ float PLs[L * NW * 64 * NC];
   __shared__ float C2[2][64 * NC];
   constexpr int P = 64 * NC;  // LDS line pitch
   const int lane = threadIdx.x & 63, w = wave_id();
@@ -2240,7 +2265,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
     int z;                                                                                                   \
     const int J = eq_block(z) + j0;                                                                          \
     const EqLane &q = eq_pick(e, z);                                                                         \
-    smooth_reg<L, NC>(g, d, q.DSC, q.D, p, s, q.ncall, J);                                                   \
+    smooth_reg<L, NW, NC>(g, d, q.DSC, q.D, p, s, q.ncall, J);                                               \
   }
 SMOOTH_REG(1, 8, 1, )
 SMOOTH_REG(2, 8, 1, )
