@@ -122,11 +122,13 @@ __device__ __forceinline__ Dev eq_dev(const Dev &d0, const EqLane &q) {
 // ---------------------------------------------------------------------------
 // primitive variables, mf:3826-3835 and mf:3871-3897 (ideal gas)
 
+// (node kernels check the step's gate after their loads, before their
+// first store, so the loads do not wait on it)
 __global__ void k_prim(Geo g, Dev d, Params p) {
-  if (gated(d)) return;
   NODE_IJK
   if (J < 2) return;
   const size_t n = IX(I, J, K);
+  const float roe = d.ROE[n];
   float RRNOW = 1 / d.R[JKX(J, K)];
   float RONOW = d.RO[n];
   float RECIP = 1.0f / RONOW;
@@ -134,13 +136,14 @@ __global__ void k_prim(Geo g, Dev d, Params p) {
   float vr = d.ROVR[n] * RECIP;
   float rovt = d.RORVT[n] * RRNOW;
   float vt = rovt * RECIP;
+  if (gated(d)) return;
   d.VX[n] = vx;
   d.VR[n] = vr;
   d.ROVT[n] = rovt;
   d.VT[n] = vt;
   if (J <= g.JMM1) {
     float EKE = .5f * (vr * vr + vt * vt + vx * vx);
-    float TSTATIC = (d.ROE[n] / RONOW - EKE) / p.CV;
+    float TSTATIC = (roe / RONOW - EKE) / p.CV;
     d.P[n] = RONOW * p.RGAS * TSTATIC;
     d.HO[n] = p.CP * TSTATIC + EKE;
     d.T_STATIC[n] = TSTATIC;
@@ -445,7 +448,6 @@ __global__ void k_flux_radial(Geo g, Dev d) {
 __device__ __forceinline__ FaceFlux sa_faces(const Geo &g, const Dev &d, int I, int J, int K, float fx, float ft,
                                              float fr, float ft1);
 __global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
-  if (gated(d)) return;
   NODE_IJK
   const size_t n = IX(I, J, K);
   const MassFlux m = massflux_vals(g, d, I, J, K);
@@ -460,6 +462,7 @@ __global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
     }
     fs = sa_faces(g, d, I, J, K, m.fx, m.ft, m.fr, ft1);
   }
+  if (gated(d)) return;
   massflux_store(g, d, m, n, false);
   store_faces(fe, e.q[0].XFLUX, e.q[0].RFLUX, e.q[0].TFLUX, e.q[0].SOURCE, n);
   if (nq > 1) store_faces(fs, e.q[1].XFLUX, e.q[1].RFLUX, e.q[1].TFLUX, e.q[1].SOURCE, n);
@@ -472,7 +475,6 @@ __global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
 // every value before any store.  The expressions and their operand order
 // are those of k_flux_axial, k_flux_angular and k_flux_radial.
 __global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
-  if (gated(d)) return;
   NODE_IJK
   const int IM = g.IM;
   const bool fx = I <= g.IMM1 && K <= g.KMM1, fr = J >= 2 && I <= g.IMM1, ft = J >= 2 && K <= g.KMM1;
@@ -544,6 +546,7 @@ __global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
     }
   }
   const EqLane &qa = e.q[0], &qt = e.q[1], &qr = e.q[2];
+  if (gated(d)) return;
   if (fx) {
     qa.SOURCE[n] = axS;
     qa.XFLUX[n] = axX;
@@ -769,12 +772,13 @@ __global__ void k_bc_apply(Geo g, Dev d, Params p, StepScalars s) {
 
 // PEFF, mf:4751-4757
 __global__ void k_peff(Geo g, Dev d, Params p, StepScalars s) {
-  if (gated(d)) return;
   NODE_IJK
   int JP1 = J + 1;
   if (J == g.JM) JP1 = g.JM;
   const size_t n = IX(I, J, K);
-  d.PEFF[n] = d.P[n] - s.PREFF + p.F_PDOWN * (d.P[IX(I, JP1, K)] - d.P[n]);
+  const float v = d.P[n] - s.PREFF + p.F_PDOWN * (d.P[IX(I, JP1, K)] - d.P[n]);
+  if (gated(d)) return;
+  d.PEFF[n] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1421,6 +1425,20 @@ __device__ __forceinline__ void mg_apply_lines(const Geo &g, const Dev &d0, cons
   const int J = (bl % gridDim.x) * 4 + w + 2, kq = (bl / gridDim.x) * 4 + 1;
   if (J > g.JM) return;
   const int nk = min(4, g.KMM1 - kq + 1);
+  // the first chunk's node values and block indices, loaded before the
+  // step's gate and the J-line lookups (their addresses depend on nothing
+  // loaded)
+  float st0[4], sp0[4], rs0[4];
+  const int I0 = min(lane + 1, g.IMM1);
+  const int ib10 = d.IB1[I0 - 1], ib20 = d.IB2[I0 - 1];
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) {
+    const unsigned n = IX(I0, J, min(kq + kk, g.KMM1));
+    st0[kk] = S.ld(n);
+    sp0[kk] = d.STEP[n];
+    rs0[kk] = d.RSTEP[n];
+  }
+  if (gated(d)) return;
   const int J1 = d.JB1[J - 1], J2 = d.JB2[J - 1], JSB = d.JSBLK[J - 1];
   const float sbs = d.SBCHG[JSB - 1] * d.STEPSBK[JSB - 1];
   unsigned o1[4], o2[4];
@@ -1432,15 +1450,16 @@ __device__ __forceinline__ void mg_apply_lines(const Geo &g, const Dev &d0, cons
   }
   double s[4] = {0.0, 0.0, 0.0, 0.0};
   for (int I = lane + 1; I <= g.IMM1; I += 64) {
-    const unsigned i1 = (unsigned)(d.IB1[I - 1] - 1), i2 = (unsigned)(d.IB2[I - 1] - 1);
+    const bool first = I == lane + 1;
+    const unsigned i1 = (unsigned)((first ? ib10 : d.IB1[I - 1]) - 1), i2 = (unsigned)((first ? ib20 : d.IB2[I - 1]) - 1);
     float st[4], sp[4], rs[4], c1[4], s1[4], c2[4], s2[4];
     // all loads of the four lines before any store
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       const unsigned n = IX(I, J, min(kq + kk, g.KMM1)), b1 = i1 + o1[kk], b2 = i2 + o2[kk];
-      st[kk] = S.ld(n);
-      sp[kk] = d.STEP[n];
-      rs[kk] = d.RSTEP[n];
+      st[kk] = first ? st0[kk] : S.ld(n);
+      sp[kk] = first ? sp0[kk] : d.STEP[n];
+      rs[kk] = first ? rs0[kk] : d.RSTEP[n];
       c1[kk] = d.B1CHG[b1];
       s1[kk] = d.STEP1[b1];
       c2[kk] = d.B2CHG[b2];
@@ -1464,13 +1483,10 @@ __device__ __forceinline__ void mg_apply_lines(const Geo &g, const Dev &d0, cons
     red[JKX(J, kq + lane)] = v;
   }
 }
-__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) {
-  if (gated(d0)) return;
-  mg_apply_lines<false>(g, d0, e);
-}
+// (the step's gate is checked inside, after the first loads)
+__global__ void __launch_bounds__(256) k_mg_apply_lines(Geo g, Dev d0, EqSet e) { mg_apply_lines<false>(g, d0, e); }
 // the mixed form's fused chain: STORE in BF16
 __global__ void __launch_bounds__(256) k_mg_apply_lines_bfs(Geo g, Dev d0, EqSet e) {
-  if (gated(d0)) return;
   mg_apply_lines<true>(g, d0, e);
 }
 
