@@ -449,23 +449,140 @@ __device__ __forceinline__ FaceFlux sa_faces(const Geo &g, const Dev &d, int I, 
                                              float fr, float ft1);
 __global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
   NODE_IJK
-  const size_t n = IX(I, J, K);
-  const MassFlux m = massflux_vals(g, d, I, J, K);
-  const FaceFlux fe = energy_faces(g, d, I, J, K, m.fx, m.ft, m.fr);
-  FaceFlux fs;
-  if (nq > 1) {
-    float ft1 = m.ft;
-    if (I == g.IM && J >= 2 && K <= g.KMM1 && d.IND[J - 1] == 1) {
-      ft1 = flowt_raw(g, d, 1, J, K);
-      const int NR = d.NROW[J - 1];
-      if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) ft1 = 0.0f;
+  const int IM = g.IM, IMM1 = g.IMM1, KMM1 = g.KMM1;
+  // the corners (I..I+1, J-1..J, K..K+1) with indices clamped into the
+  // grid, every value loaded before anything waits: a clamped corner is
+  // loaded but enters no face that is stored.  The expressions and their
+  // operand order are those of massflux_vals, energy_faces and sa_faces;
+  // only the periodic-line and blade-line ends (I = 1, IM) load more.
+  const int Ip = min(I + 1, IM), Jm = max(J - 1, 1), Kp = min(K + 1, g.KM), Km = max(K - 1, 1);
+  const unsigned n = IX(I, J, K);
+  const unsigned c001 = IX(I, J, Kp), c100 = IX(Ip, J, K), c101 = IX(Ip, J, Kp);
+  const unsigned cm00 = IX(I, Jm, K), cm01 = IX(I, Jm, Kp), cm10 = IX(Ip, Jm, K);
+  const float x000 = d.ROVX[n], x001 = d.ROVX[c001], x100 = d.ROVX[c100], x101 = d.ROVX[c101], xm00 = d.ROVX[cm00],
+              xm01 = d.ROVX[cm01], xm10 = d.ROVX[cm10];
+  const float r000 = d.ROVR[n], r001 = d.ROVR[c001], r100 = d.ROVR[c100], r101 = d.ROVR[c101], rm00 = d.ROVR[cm00],
+              rm01 = d.ROVR[cm01], rm10 = d.ROVR[cm10];
+  const float w000 = d.ROVT[n], w001 = d.ROVT[c001], wm00 = d.ROVT[cm00], wm01 = d.ROVT[cm01];
+  const float o000 = d.RO[n], o001 = d.RO[c001], om00 = d.RO[cm00], om01 = d.RO[cm01];
+  const float u00 = d.UBLADE[JKX(J, K)], u01 = d.UBLADE[JKX(J, Kp)], um0 = d.UBLADE[JKX(Jm, K)],
+              um1 = d.UBLADE[JKX(Jm, Kp)];
+  const float h000 = d.HO[n], h001 = d.HO[c001], h100 = d.HO[c100], h101 = d.HO[c101], hm00 = d.HO[cm00],
+              hm01 = d.HO[cm01], hm10 = d.HO[cm10];
+  const float p000 = d.PEFF[n], p001 = d.PEFF[c001], pm00 = d.PEFF[cm00], pm01 = d.PEFF[cm01];
+  const float aqx = d.AQX[n], aqr = d.AQR[n], abx = d.ABX[n], abr = d.ABR[n], asx = d.ASX[n], asr = d.ASR[n];
+  const float abt = d.ABT[JKX(J, K)], wrabt = d.WRABT[JKX(J, K)], qs = d.QSOURCE[n];
+  const bool sa = nq > 1;
+  float tk0 = 0.f, tkjp = 0.f, tkkm = 0.f, tkim = 0.f, tsrc = 0.f;
+  if (sa) {
+    const float *TK = d.TRANS_KVIS;
+    tk0 = TK[n];
+    tkjp = TK[IX(I, min(J + 1, g.JM), K)];
+    tkkm = TK[IX(I, J, Km)];
+    tkim = TK[IX(max(I - 1, 1), J, K)];
+    tsrc = d.T_SOURCE[n];
+  }
+  const int ind = d.IND[J - 1];
+  const bool ends = I == 1 || I == IM;
+  // the mass fluxes (massflux_vals)
+  MassFlux m;
+  if (I <= IMM1 && K <= KMM1) {
+    float AVGRVX = x000 + x001 + x100 + x101;
+    float AVGRVR = r000 + r001 + r100 + r101;
+    m.fx = 0.25f * (AVGRVX * aqx + AVGRVR * aqr);
+    m.hx = true;
+  }
+  m.rowt = w000 - u00 * o000;
+  m.wt = m.rowt / o000;
+  if (J >= 2 && K <= KMM1) {
+    if (ind != 1 && ends) {
+      m.ft = 0.5f * (flowt_raw(g, d, 1, J, K) + flowt_raw(g, d, IM, J, K));
+    } else {
+      float AVGRVX = x000 + xm00 + xm01 + x001;
+      float AVGRVT = (w000 - u00 * o000) + (wm00 - um0 * om00) + (wm01 - um1 * om01) + (w001 - u01 * o001);
+      float AVGRVR = r000 + rm00 + rm01 + r001;
+      m.ft = (AVGRVX * abx + AVGRVT * abt + AVGRVR * abr) * 0.25f;
+      if (ind == 1 && ends) {
+        int NR = d.NROW[J - 1];
+        if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) m.ft = 0.0f;
+      }
     }
-    fs = sa_faces(g, d, I, J, K, m.fx, m.ft, m.fr, ft1);
+    m.ht = true;
+  }
+  if (J >= 2 && I <= IMM1) {
+    if (K >= 2 && K <= KMM1) {
+      float AVGRVX = x000 + xm00 + xm10 + x100;
+      float AVGRVR = r000 + rm00 + rm10 + r100;
+      m.fr = 0.25f * (AVGRVX * asx + AVGRVR * asr);
+    }
+    m.hr = true;
+  }
+  // the energy equation's faces (energy_faces)
+  FaceFlux fe;
+  if (I <= IMM1 && K <= KMM1) {
+    float AVGHO = h000 + h001 + h100 + h101;
+    fe.x = m.fx * AVGHO * 0.25f;
+    fe.hx = true;
+    if (J >= 2) {
+      fe.s = qs;
+      fe.hs = true;
+    }
+  }
+  if (J >= 2 && I <= IMM1) {
+    float AVGHO = 0.25f * (h000 + hm00 + hm10 + h100);
+    fe.r = m.fr * AVGHO;
+    fe.hr = true;
+  }
+  if (J >= 2 && K <= KMM1) {
+    if (ind != 1 && ends) {
+      fe.t = 0.5f * (tflux_energy(g, d, 1, J, K, m.ft) + tflux_energy(g, d, IM, J, K, m.ft));
+    } else {
+      float AVGHO = h000 + hm00 + hm01 + h001;
+      float AVGPB = p000 + pm00 + pm01 + p001;
+      fe.t = (m.ft * AVGHO + wrabt * AVGPB) * 0.25f;
+    }
+    fe.ht = true;
+  }
+  // the transport's faces (sa_faces); FLOWT(1,J,K), which the pitchwise
+  // face at I = IM reads, is the node's own value on a periodic line and is
+  // formed again on a blade line
+  FaceFlux fs;
+  if (sa) {
+    if (K == g.KM) {
+      fs.hr = J >= 2 && I <= IMM1;
+    } else {
+      if (I <= IMM1) {
+        const bool jp = J != g.JM && J != 1 && d.INDMIX[J - 1] != 1;  // JP1 = J + 1, else J
+        fs.x = m.fx * (tk0 + (jp ? tkjp : tk0));
+        fs.hx = true;
+      }
+      if (J >= 2) {
+        if (I <= IMM1) {
+          fs.s = -2.0f * tsrc;
+          fs.r = (K == 1) ? 0.0f : m.fr * (tkkm + tk0);
+          fs.hs = fs.hr = true;
+        }
+        if (I >= 2 && I <= IMM1) {
+          fs.t = m.ft * (tkim + tk0);
+          fs.ht = true;
+        }
+        if (ends) {
+          float ft1 = m.ft;
+          if (I == IM && K <= KMM1 && ind == 1) {
+            ft1 = flowt_raw(g, d, 1, J, K);
+            const int NR = d.NROW[J - 1];
+            if (K >= d.KS1[NR - 1] && K <= d.KS2[NR - 1]) ft1 = 0.0f;
+          }
+          fs.t = ft1 * (d.TRANS_KVIS[IX(IMM1, J, K)] + d.TRANS_KVIS[IX(1, J, K)]);
+          fs.ht = true;
+        }
+      }
+    }
   }
   if (gated(d)) return;
   massflux_store(g, d, m, n, false);
   store_faces(fe, e.q[0].XFLUX, e.q[0].RFLUX, e.q[0].TFLUX, e.q[0].SOURCE, n);
-  if (nq > 1) store_faces(fs, e.q[1].XFLUX, e.q[1].RFLUX, e.q[1].TFLUX, e.q[1].SOURCE, n);
+  if (sa) store_faces(fs, e.q[1].XFLUX, e.q[1].RFLUX, e.q[1].TFLUX, e.q[1].SOURCE, n);
 }
 
 // The three momentum equations' face fluxes and sources at one node in one
