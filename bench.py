@@ -575,8 +575,8 @@ def pmc_table(fetch_csv: str, write_csv: str, copy_fetch_csv: str, copy_write_cs
         f.write("Made by `python3 bench.py --pmc-table` from four rocprofv3 counter passes (from the repository "
                 "root on the MI355X box; `cd /tmp && export TMPDIR=/tmp` first):\n\n```\n")
         f.write("rocprofv3 --pmc FETCH_SIZE --kernel-include-regex '" + "|".join(PMC_KERNELS) + "' \\\n"
-                "    --output-format csv -d <out>/fetch -o run -- python3 bench.py --steps 3 --warmup 1 --tiles 8 8 "
-                "--no-cpu-baseline --no-converged --no-speedline --no-mixed --no-check\n"
+                "    --output-format csv -d <out>/fetch -o run -- python3 bench.py --steps 3 --warmup 1 --tiles 4 8 "
+                "--no-cpu-baseline --no-converged --no-speedline --no-mixed --no-check --no-machines\n"
                 "rocprofv3 --pmc WRITE_SIZE  (same filter and command, -d <out>/write)\n"
                 "rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'copy|Copy' --output-format csv -d <out>/cfetch "
                 "-o run -- python3 bench.py --calibrate-copy\n"
