@@ -1869,13 +1869,8 @@ DAMP_SCATTER(4, 8)
 // runs in LDS, with the pitchwise and spanwise coefficients staged there.
 // Writes the finished plane to DST.  Dynamic LDS: 3*IM*KM + 2*(IM+KM)
 // floats.
-__global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, EqSet e, Params p, StepScalars s, int j0) {
-  if (gated(d)) return;
-  extern __shared__ float lds[];
-  // batched over the variables of e (grid z): q.DSC smoothed into q.D
-  int z;
-  const int J = eq_block(z) + j0;
-  const EqLane &q = eq_pick(e, z);
+__device__ __forceinline__ void smooth_plane(const Geo &g, const Dev &d, const EqLane &q, const Params &p,
+                                             const StepScalars &s, int J, float *lds) {
   const float *const SRC = q.DSC;
   float *const DST = q.D;
   const int ncall = q.ncall;
@@ -2025,14 +2020,30 @@ __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, EqSet e, 
   // exit smoothing, mf:15553-15588: Gauss-Seidel sweeps in I, then in K
   if (!(p.SFEXIT < 0.0001f) && J >= g.JM + 1 - p.NSFEXIT) {
     const float SFEX1 = s.SFEX1, SFEXH = s.SFEXH, SFEXIT = p.SFEXIT;
+    // (the running value is carried in a register, so a point's update
+    // waits on the previous one's arithmetic and not on an LDS round trip)
     for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
-      for (int I = 2; I <= IMM1; I++) PLX(I, K) = SFEX1 * PLX(I, K) + SFEXH * (PLX(I + 1, K) + PLX(I - 1, K));
+      float prev = PLX(1, K), cur = PLX(2, K);
+      for (int I = 2; I <= IMM1; I++) {
+        const float nxt = PLX(I + 1, K);
+        const float v = SFEX1 * cur + SFEXH * (nxt + prev);
+        PLX(I, K) = v;
+        prev = v;
+        cur = nxt;
+      }
       PLX(1, K) = PLX(1, K) + SFEXIT * (2.f * PLX(2, K) - PLX(3, K) - PLX(1, K));
       PLX(IM, K) = PLX(IM, K) + SFEXIT * (2.f * PLX(IMM1, K) - PLX(IMM2, K) - PLX(IM, K));
     }
     __syncthreads();
     for (int I = threadIdx.x + 1; I <= IM; I += blockDim.x) {
-      for (int K = 2; K <= KMM1; K++) PLX(I, K) = SFEX1 * PLX(I, K) + SFEXH * (PLX(I, K - 1) + PLX(I, K + 1));
+      float prev = PLX(I, 1), cur = PLX(I, 2);
+      for (int K = 2; K <= KMM1; K++) {
+        const float nxt = PLX(I, K + 1);
+        const float v = SFEX1 * cur + SFEXH * (prev + nxt);
+        PLX(I, K) = v;
+        prev = v;
+        cur = nxt;
+      }
       PLX(I, 1) = PLX(I, 1) + SFEXIT * (2.f * PLX(I, 2) - PLX(I, 3) - PLX(I, 1));
       PLX(I, KM) = PLX(I, KM) + SFEXIT * (2.f * PLX(I, KMM1) - PLX(I, KMM2) - PLX(I, KM));
     }
@@ -2086,6 +2097,14 @@ __global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, EqSet e, 
 #undef AVX
 #undef CUX
 }
+__global__ void __launch_bounds__(512, 6) k_smooth_plane(Geo g, Dev d, EqSet e, Params p, StepScalars s, int j0) {
+  if (gated(d)) return;
+  extern __shared__ float lds[];
+  // batched over the variables of e (grid z): q.DSC smoothed into q.D
+  int z;
+  const int J = eq_block(z) + j0;
+  smooth_plane(g, d, eq_pick(e, z), p, s, J, lds);
+}
 
 // SMOOTH_VAR and TSTEP's D conditions as k_smooth_plane, for IM <= 64 and
 // planes without exit smoothing, with the plane held in registers: wavefront
@@ -2105,13 +2124,18 @@ __device__ __forceinline__ float lane_at(float x, int l) {  // lane l's value, w
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
+// LDS floats of smooth_reg<L, NW, NC>: the plane, then lines 2 and KM-1
+template <int L, int NW, int NC>
+constexpr int smooth_reg_lds() { return L * NW * 64 * NC + 2 * 64 * NC; }
+
 template <int L, int NW, int NC>
 __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const float *SRC, float *DST, const Params &p,
-                                           const StepScalars &s, int ncall, int J) {
+                                           const StepScalars &s, int ncall, int J, float *lds) {
   // NC chunks of 64 points per lane: point I = 64 c + lane + 1 of chunk c;
   // the NW wavefronts' L lines each cover the plane's KM <= L NW lines
-  __shared__ float PLs[L * NW * 64 * NC];
-  __shared__ float C2[2][64 * NC];
+  float *const PLs = lds;
+  float *const C2r = lds + L * NW * 64 * NC;
+  auto C2 = [&](int which) { return C2r + which * 64 * NC; };
   constexpr int P = 64 * NC;  // LDS line pitch
   const int lane = threadIdx.x & 63, w = wave_id();
   const int IM = g.IM, KM = g.KM, IMM1 = g.IMM1, IMM2 = IM - 2, KMM1 = g.KMM1, KMM2 = KM - 2;
@@ -2307,8 +2331,8 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
       const int K = kbase + l;
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        if (K == 2) C2[0][64 * c + lane] = v[l][c];
-        if (K == KMM1) C2[1][64 * c + lane] = v[l][c];
+        if (K == 2) C2(0)[64 * c + lane] = v[l][c];
+        if (K == KMM1) C2(1)[64 * c + lane] = v[l][c];
       }
     }
     __syncthreads();
@@ -2326,7 +2350,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
       for (int c = 0; c < NC; c++) {
         const int I = Iv[c];
         const float x = v[l][c];
-        const float y = (K == 1) ? C2[0][64 * c + lane] : C2[1][64 * c + lane];
+        const float y = (K == 1) ? C2(0)[64 * c + lane] : C2(1)[64 * c + lane];
         if (K == 1) {
           if (I == 1) v[l][c] = SFT1 * x + SFTH * (xn[c] + y);
           if (I == IM) v[l][c] = SFT1 * x + SFTH * (xp[c] + y);
@@ -2398,7 +2422,8 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
     int z;                                                                                                   \
     const int J = eq_block(z) + j0;                                                                          \
     const EqLane &q = eq_pick(e, z);                                                                         \
-    smooth_reg<L, NW, NC>(g, d, q.DSC, q.D, p, s, q.ncall, J);                                               \
+    __shared__ float lds[smooth_reg_lds<L, NW, NC>()];                                                       \
+    smooth_reg<L, NW, NC>(g, d, q.DSC, q.D, p, s, q.ncall, J, lds);                                          \
   }
 SMOOTH_REG(1, 8, 1, )
 SMOOTH_REG(2, 8, 1, )
@@ -2419,6 +2444,39 @@ SMOOTH_REG(1, 16, 2, c2)
 SMOOTH_REG(2, 16, 2, c2)
 SMOOTH_REG(4, 16, 2, c2)
 SMOOTH_REG(8, 16, 2, c2)
+
+// SMOOTH_VAR over all of a step's planes in one launch when the last
+// NSFEXIT planes take exit smoothing: the first nlead workgroups (a
+// multiple of 8, so the XCD order of the others is kept) take the exit
+// planes jexit..JM in the LDS form, and being dispatched first they run
+// beside the others, which take the planes 1..jexit-1 in registers (8
+// wavefronts of L lines).  nv variables (e), the variable fastest.
+// Dynamic LDS: the larger of the two forms' needs.
+#define SMOOTH_MIX(L, NC, SFX)                                                                                    \
+  __global__ void __launch_bounds__(512) k_smooth_mix##L##SFX(Geo g, Dev d, EqSet e, Params p, StepScalars s,   \
+                                                              int nv, int jexit, int nlead) {                  \
+    if (gated(d)) return;                                                                                       \
+    extern __shared__ float lds[];                                                                              \
+    const int b = blockIdx.x;                                                                                   \
+    if (b < nlead) {                                                                                            \
+      if (b >= (g.JM - jexit + 1) * nv) return;                                                                 \
+      smooth_plane(g, d, eq_pick(e, b % nv), p, s, jexit + b / nv, lds);                                        \
+      return;                                                                                                   \
+    }                                                                                                           \
+    const int t = xcd_block(b - nlead, (jexit - 1) * nv);                                                       \
+    const EqLane &q = eq_pick(e, t % nv);                                                                       \
+    smooth_reg<L, 8, NC>(g, d, q.DSC, q.D, p, s, q.ncall, t / nv + 1, lds);                                     \
+  }
+SMOOTH_MIX(1, 1, )
+SMOOTH_MIX(2, 1, )
+SMOOTH_MIX(4, 1, )
+SMOOTH_MIX(8, 1, )
+SMOOTH_MIX(16, 1, )
+SMOOTH_MIX(1, 2, c2)
+SMOOTH_MIX(2, 2, c2)
+SMOOTH_MIX(4, 2, c2)
+SMOOTH_MIX(8, 2, c2)
+SMOOTH_MIX(16, 2, c2)
 #undef SMOOTH_REG
 
 // ---------------------------------------------------------------------------
