@@ -117,7 +117,7 @@ __device__ __forceinline__ bool gated(const Dev &d) { return d.gate >= 0 && d.CT
 // velocity and density for hub and casing), and the cells of a column one
 // k_sa_kfaces thread covers.
 constexpr int kKWall = 12;
-constexpr int kKSeg = 16;
+constexpr int kKSeg = 4;
 
 // Slots of Dev::SC, the step's device-resident scalars.
 enum {

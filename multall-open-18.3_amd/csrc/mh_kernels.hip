@@ -1507,15 +1507,21 @@ __global__ void __launch_bounds__(256) k_mg_levels(Geo g, MgFast m, EqSet e, con
   __shared__ double sh[256];
   const int sb = bx - nblk1 - nblk2;
   const int ja = sbp[sb], nj = sbp[sb + 1] - ja, tot = nj * g.NKB1;
+  // groups of four members per thread summed in turn; four groups are
+  // fetched at once (indices, then values) before they are added in order
   double s = 0.0;
-  for (int c0 = 0; c0 < tot; c0 += 1024) {
-    double v[4];
+  for (int c0 = 0; c0 < tot; c0 += 4096) {
+    int o[16];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 16; u++) {
       const int c = c0 + 256 * u + threadIdx.x;
-      v[u] = (c < tot) ? m.PT[JKX(sbl[ja + c % nj], c / nj + 1)] : 0.0;
+      o[u] = (c < tot) ? (int)JKX(sbl[ja + c % nj], c / nj + 1) : -1;
     }
-    s += ((v[0] + v[1]) + v[2]) + v[3];
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = (o[u] >= 0) ? m.PT[o[u]] : 0.0;
+#pragma unroll
+    for (int t4 = 0; t4 < 4; t4++) s += ((v[4 * t4] + v[4 * t4 + 1]) + v[4 * t4 + 2]) + v[4 * t4 + 3];
   }
   sh[threadIdx.x] = s;
   __syncthreads();
@@ -1618,15 +1624,18 @@ __global__ void __launch_bounds__(1024) k_row_avg(Geo g, Dev d, EqSet e, const i
   __shared__ double sh[16];
   const int r = blockIdx.x, t = threadIdx.x, lane = t & 63, w = wave_id();
   const int q0 = rptr[r], nj = rptr[r + 1] - q0, ja = rlist[q0], tot = nj * g.KMM1;
+  // groups of four lines per thread summed in turn, four groups fetched at
+  // once before they are added in order
   double s = 0.0;
-  for (int c0 = 0; c0 < tot; c0 += 4096) {
-    double v[4];
+  for (int c0 = 0; c0 < tot; c0 += 16384) {
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 16; u++) {
       const int c = c0 + 1024 * u + t;
       v[u] = (c < tot) ? red[JKX(ja + c % nj, c / nj + 1)] : 0.0;
     }
-    s += ((v[0] + v[1]) + v[2]) + v[3];
+#pragma unroll
+    for (int t4 = 0; t4 < 4; t4++) s += ((v[4 * t4] + v[4 * t4 + 1]) + v[4 * t4 + 2]) + v[4 * t4 + 3];
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -3355,17 +3364,31 @@ __global__ void k_pwallgrad_smooth(Geo g, Dev d) {
     set(J2, 2 * dp(J2 - 1) - dp(J2 - 2));
     // TEMPP(J) = 0.5 D(J) + 0.25 (D(J-1) + D(J+1)) for J1+2..J2-1, then
     // D(J) = 0.5 TEMPP(J) + 0.25 (TEMPP(J-1) + TEMPP(J+1)) for J1+3..J2-2,
-    // streamed with a three-plane window of each
-    float dm1 = dp(J1 + 1), d0 = dp(J1 + 2), dn = dp(J1 + 3);
+    // streamed with a three-plane window of each.  Every D read is of a
+    // plane not yet written, so the planes are fetched kCh at a time ahead
+    // of the chunk's stores (one memory latency per chunk, not per plane;
+    // planes past J2 are fetched but not used).
+    constexpr int kCh = 16;
+    float dm1 = dp(J1 + 1), d0 = dp(J1 + 2), dn = dp(J1 + 3), nx = dp(J1 + 4);
     float tm1 = 0.0f, t0 = 0.5f * d0 + 0.25f * (dm1 + dn);  // TEMPP(J1+2)
-    for (int J = J1 + 3; J <= J2 - 2; J++) {
-      dm1 = d0;
-      d0 = dn;
-      dn = dp(J + 1);
-      tm1 = t0;
-      t0 = 0.5f * d0 + 0.25f * (dm1 + dn);                  // TEMPP(J)
-      const float tn = 0.5f * dn + 0.25f * (d0 + dp(J + 2));  // TEMPP(J+1)
-      D[IX(I, J, K)] = 0.5f * t0 + 0.25f * (tm1 + tn);
+    for (int Jc = J1 + 3; Jc <= J2 - 2; Jc += kCh) {
+      float w[kCh];  // D(J+2) of the chunk's planes J
+#pragma unroll
+      for (int q = 0; q < kCh; q++) w[q] = dp(Jc + q + 2);
+#pragma unroll
+      for (int q = 0; q < kCh; q++) {
+        const int J = Jc + q;
+        if (J <= J2 - 2) {
+          dm1 = d0;
+          d0 = dn;
+          dn = nx;  // D(J+1)
+          tm1 = t0;
+          t0 = 0.5f * d0 + 0.25f * (dm1 + dn);            // TEMPP(J)
+          const float tn = 0.5f * dn + 0.25f * (d0 + w[q]);  // TEMPP(J+1)
+          D[IX(I, J, K)] = 0.5f * t0 + 0.25f * (tm1 + tn);
+          nx = w[q];
+        }
+      }
     }
     set(J2 + 1, 0.0f);
     set(J2 + 2, 2 * dp(J2 + 3) - dp(J2 + 4));
