@@ -29,6 +29,8 @@ host cores at once.
       (writes the HBM-traffic table under profiles/ from rocprofv3 --pmc runs)
   python bench.py --stats-table KERNEL_STATS.csv STEPS
       (prints the per-kernel time table of a rocprofv3 --stats run)
+  python bench.py --converged-profile STEPS
+      (per-kernel HIP-event times of the time-to-converged workload, Rotor 37)
 """
 from __future__ import annotations
 
@@ -607,6 +609,42 @@ def stats_table(stats_csv: str, steps: int) -> None:
         print(f"| {name} | {calls} | {avg:.2f} | {tus / steps:.1f} | {pct:.2f} |")
 
 
+def converged_profile(steps: int) -> None:
+    """Per-kernel durations of the time-to-converged workload (Rotor 37 as
+    it ships, fast mode, plain FP32): 100 steps from the initial guess, the
+    wall time of 500 steps, then `steps` steps launched one by one with
+    dispatch-timestamped HIP events on the library's stream (a markdown
+    table: launches, mean duration, time per step)."""
+    from multall_hip import host
+    from multall_hip.runtime import Solver
+    st = host.load_deck(CONVERGED_DECK, xlength_on_host=False)
+    sc = st["scalars"]
+    g = Solver(st, exact_order=False)
+    g.set_xlength(host.array("THETA", (sc["KM"], sc["JM"], sc["IM"])),
+                  *[host.array(n, (sc["NROWS"],)) for n in ("XLLIM_IN", "XLLIM_LE", "XLLIM_TE", "XLLIM_DN")])
+    g.loss_init()
+    g.start()
+    g.step(100)
+    g.sync()
+    t = time.perf_counter()
+    g.step(500)
+    g.sync()
+    wall = (time.perf_counter() - t) / 500
+    g.profile(True)
+    g.step(steps)
+    rep = g.profile_report()
+    g.profile(False)
+    g.close()
+    busy = sum(ms for c, ms in rep.values()) / steps
+    launches = sum(c for c, ms in rep.values()) / steps
+    print(f"Grid {sc['IM']} x {sc['JM']} x {sc['KM']}; {wall * 1e3:.4f} ms/step wall over 500 steps as captured "
+          f"graphs; {steps} steps launched one by one: {busy * 1e3:.1f} us/step of kernel time, "
+          f"{launches:.1f} launches/step.\n")
+    print("| kernel | calls | avg us | us/step |\n|---|---:|---:|---:|")
+    for k, (c, ms) in sorted(rep.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {k} | {c} | {ms / c * 1e3:.2f} | {ms / steps * 1e3:.2f} |")
+
+
 def calibrate_copy() -> None:
     """One device-to-device copy of 256 MiB, three times, for scaling the
     FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes exactly
@@ -639,6 +677,8 @@ def main():
                     help="skip the multi-stage compressor and centrifugal stage throughput runs")
     ap.add_argument("--tiles", nargs=2, type=int, metavar=("TK", "NW"),
                     help="fix the corner-distribution and SMOOTH_VAR tile shapes (no tuning launches)")
+    ap.add_argument("--converged-profile", type=int, metavar="STEPS",
+                    help="print the per-kernel time table of the time-to-converged workload (Rotor 37)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (for scaling the PMC byte counters)")
     ap.add_argument("--pmc-table", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
@@ -654,6 +694,9 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.converged_profile:
+        converged_profile(args.converged_profile)
         return
     if args.gpus != 1:
         print("bench.py measures one MI355X: multi-GPU runs are not part of this build "

@@ -61,16 +61,6 @@ __global__ void k_smooth_plane(Geo, Dev, EqSet, Params, StepScalars, int);
   SMOOTH_REG_DECL(8, 16, SFX)
 SMOOTH_REG_DECLS()
 SMOOTH_REG_DECLS(c2)
-#define SMOOTH_MIX_DECL(L, SFX) \
-  __global__ void k_smooth_mix##L##SFX(Geo, Dev, EqSet, Params, StepScalars, int, int, int);
-#define SMOOTH_MIX_DECLS(SFX) \
-  SMOOTH_MIX_DECL(1, SFX)     \
-  SMOOTH_MIX_DECL(2, SFX)     \
-  SMOOTH_MIX_DECL(4, SFX)     \
-  SMOOTH_MIX_DECL(8, SFX)     \
-  SMOOTH_MIX_DECL(16, SFX)
-SMOOTH_MIX_DECLS()
-SMOOTH_MIX_DECLS(c2)
 #undef SMOOTH_REG_DECLS
 #undef SMOOTH_REG_DECL
 __global__ void k_fill_int(int *, int, int);
@@ -721,22 +711,6 @@ const SmoothVariant &smooth_variant(int km, int nw, int nc) {
   const int ci = nc > 1 ? 1 : 0;
   return nw == 16 ? w16[ci][std::min(li, 3)] : w8[ci][li];
 }
-// the one-launch form for steps with exit smoothing (8 wavefronts): the
-// kernel and its register planes' LDS floats
-struct MixVariant {
-  void (*k)(Geo, Dev, EqSet, Params, StepScalars, int, int, int);
-  const char *name;
-  int lds_floats;
-};
-const MixVariant &mix_variant(int km, int nc) {
-#define MV(L, NC, SFX) {k_smooth_mix##L##SFX, "k_smooth_mix" #L #SFX, (L * 8 * 64 + 2 * 64) * NC}
-  static const MixVariant v[2][5] = {{MV(1, 1, ), MV(2, 1, ), MV(4, 1, ), MV(8, 1, ), MV(16, 1, )},
-                                     {MV(1, 2, c2), MV(2, 2, c2), MV(4, 2, c2), MV(8, 2, c2), MV(16, 2, c2)}};
-#undef MV
-  const int lines = (km + 7) / 8;
-  const int li = lines <= 1 ? 0 : lines <= 2 ? 1 : lines <= 4 ? 2 : lines <= 8 ? 3 : 4;
-  return v[nc > 1 ? 1 : 0][li];
-}
 // the register form of SMOOTH_VAR holds IM <= 128 points of a line and
 // KM <= 128 lines of a plane
 bool smooth_reg_fits(const Geo &g) { return g.IM <= 128 && g.KM <= 128; }
@@ -753,8 +727,7 @@ int autotune(mh_ctx *c) {
   static std::map<std::array<int, 6>, std::pair<int, int>> known;
   const Geo &g = c->g;
   const Params p = params_of(c);
-  // with exit smoothing SMOOTH_VAR takes its one-launch form, 8 wavefronts
-  const bool exit_sm = !(p.SFEXIT < 0.0001f) && p.NSFEXIT > 0;
+  const bool exit_sm = !(p.SFEXIT < 0.0001f) && p.NSFEXIT > 0;  // the exit planes' sweeps change the timing
   const std::array<int, 6> key = {g.IM, g.JM, g.KM, g.NROWS, p.EXACT, exit_sm ? 1 : 0};
   const int ftk = (int)c->scal[MH_S_SCAT_TK], fnw = (int)c->scal[MH_S_SMOOTH_WAVES];
   if (ftk && ftk != 2 && ftk != 4 && ftk != 8) return fail(c, MH_E_ARG, "SCAT_TK must be 0, 2, 4 or 8");
@@ -814,7 +787,7 @@ int autotune(mh_ctx *c) {
       if (t < best) best = t, best_tk = tk;
     }
   }
-  if (!fnw && smooth_reg_fits(g) && !exit_sm) {
+  if (!fnw && smooth_reg_fits(g)) {
     float best = 1e30f;
     for (int nw : kSmoothWaves) {
       const SmoothVariant &v = smooth_variant(g.KM, nw, smooth_chunks(g));
@@ -978,27 +951,15 @@ int tstep(mh_ctx *c, const EqLane *eqs, const int *lanes, int n, const StepPlan 
                  nmp > 0 ? 1 : 0);
   }
   LCHK("damp_scatter");
-  // SMOOTH_VAR: planes in registers where IM <= 128 and KM <= 128, the LDS
-  // form elsewhere and on the exit-smoothing planes
-  const int jexit = (!(p.SFEXIT < 0.0001f) && p.NSFEXIT > 0) ? std::max(1, g.JM + 1 - p.NSFEXIT) : g.JM + 1;
-  int jreg = 1;  // first plane for the LDS form
-  if (smooth_reg_fits(g) && jexit > 1 && jexit <= g.JM && c->smooth_nw == 8) {
-    // exit planes and register planes in one launch
-    const MixVariant &v = mix_variant(g.KM, smooth_chunks(g));
-    const int nlead = ((g.JM - jexit + 1) * nv + 7) / 8 * 8;
-    const size_t lds = std::max(c->lds_plane, (size_t)v.lds_floats * sizeof(float));
-    LAUNCH_NAMED(v.k, v.name, dim3(nlead + (jexit - 1) * nv), dim3(512), lds, st, g, d0, e, p, s, nv, jexit, nlead);
-    LCHK("smooth_mix");
-    return MH_OK;
-  }
-  if (smooth_reg_fits(g) && jexit > 1) {
+  // SMOOTH_VAR: planes in registers (exit-smoothing planes included) where
+  // IM <= 128 and KM <= 128, the LDS form elsewhere
+  if (smooth_reg_fits(g)) {
     const SmoothVariant &v = smooth_variant(g.KM, c->smooth_nw, smooth_chunks(g));
-    LAUNCH_NAMED(v.k, v.name, dim3(jexit - 1, 1, nv), dim3(64 * v.nw), 0, st, g, d0, e, p, s, 1);
-    jreg = jexit;
+    LAUNCH_NAMED(v.k, v.name, dim3(g.JM, 1, nv), dim3(64 * v.nw), 0, st, g, d0, e, p, s, 1);
+  } else {
+    LAUNCH(k_smooth_plane, dim3(g.JM, 1, nv), dim3(512), c->lds_plane, st, g, d0, e, p, s, 1);
   }
-  if (jreg <= g.JM)
-    LAUNCH(k_smooth_plane, dim3(g.JM - jreg + 1, 1, nv), dim3(512), c->lds_plane, st, g, d0, e, p, s, jreg);
-  LCHK("smooth_plane");
+  LCHK("smooth");
   return MH_OK;
 }
 
@@ -1564,12 +1525,6 @@ int mh_create(const mh_dims *dims, mh_ctx **out) {
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(MH_E_HIP);
   c->lds_plane = ((size_t)3 * dm.im * dm.km + 2 * (size_t)(dm.im + dm.km)) * sizeof(float);
   if (c->lds_plane > kMaxLds || scatter_lds_bytes(dm.im - 1, 8) > kMaxLds) return bail(MH_E_UNSUPPORTED);
-  for (int nc = 1; nc <= 2; nc++)
-    for (int km : {8, 16, 32, 64, 128}) {
-      const MixVariant &v = mix_variant(km, nc);
-      if (raise_lds_limit((const void *)v.k, std::max(c->lds_plane, (size_t)v.lds_floats * sizeof(float))))
-        return bail(fail(c, MH_E_HIP, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
-    }
   if (raise_lds_limit((const void *)k_smooth_plane, c->lds_plane))
     return bail(fail(c, MH_E_HIP, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)"));
   for (int tk : kScatTKs)

@@ -2370,6 +2370,52 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
       }
     }
   }
+  // exit smoothing, mf:15553-15588: the plane through LDS for the
+  // Gauss-Seidel sweeps in I (a thread per K line), then in K (a thread per
+  // I line), the running value carried in a register; the pitchwise and
+  // spanwise stages are done with PLs (the corner stage's barrier)
+  if (!(p.SFEXIT < 0.0001f) && J >= g.JM + 1 - p.NSFEXIT) {
+    const float SFEX1 = s.SFEX1, SFEXH = s.SFEXH, SFEXIT = p.SFEXIT;
+#define PX(I, K) PLs[((K) - 1) * P + (I) - 1]
+#pragma unroll
+    for (int l = 0; l < L; l++)
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        if (kbase + l <= KM && Iv[c] <= IM) PX(Iv[c], kbase + l) = v[l][c];
+    __syncthreads();
+    for (int K = threadIdx.x + 1; K <= KM; K += blockDim.x) {
+      float prev = PX(1, K), cur = PX(2, K);
+      for (int I = 2; I <= IMM1; I++) {
+        const float nxt = PX(I + 1, K);
+        const float x = SFEX1 * cur + SFEXH * (nxt + prev);
+        PX(I, K) = x;
+        prev = x;
+        cur = nxt;
+      }
+      PX(1, K) = PX(1, K) + SFEXIT * (2.f * PX(2, K) - PX(3, K) - PX(1, K));
+      PX(IM, K) = PX(IM, K) + SFEXIT * (2.f * PX(IMM1, K) - PX(IMM2, K) - PX(IM, K));
+    }
+    __syncthreads();
+    for (int I = threadIdx.x + 1; I <= IM; I += blockDim.x) {
+      float prev = PX(I, 1), cur = PX(I, 2);
+      for (int K = 2; K <= KMM1; K++) {
+        const float nxt = PX(I, K + 1);
+        const float x = SFEX1 * cur + SFEXH * (prev + nxt);
+        PX(I, K) = x;
+        prev = x;
+        cur = nxt;
+      }
+      PX(I, 1) = PX(I, 1) + SFEXIT * (2.f * PX(I, 2) - PX(I, 3) - PX(I, 1));
+      PX(I, KM) = PX(I, KM) + SFEXIT * (2.f * PX(I, KMM1) - PX(I, KMM2) - PX(I, KM));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < L; l++)
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        if (kbase + l <= KM && Iv[c] <= IM) v[l][c] = PX(Iv[c], kbase + l);
+#undef PX
+  }
   // periodicity off the blade and across the tip gap, mf:5858-5891
   const int ILAST = (ncall == 2) ? IMM1 : IM;
   const bool per = !((d.IND[J - 1] == 1) || (d.INDLE[J - 1] == 1));
@@ -2429,7 +2475,7 @@ __device__ __forceinline__ void smooth_reg(const Geo &g, const Dev &d, const flo
                                                                         StepScalars s, int j0) {             \
     if (gated(d)) return;                                                                                    \
     int z;                                                                                                   \
-    const int J = eq_block(z) + j0;                                                                          \
+    const int J = j0 + (int)gridDim.x - 1 - eq_block(z); /* last planes (exit smoothing) first */             \
     const EqLane &q = eq_pick(e, z);                                                                         \
     __shared__ float lds[smooth_reg_lds<L, NW, NC>()];                                                       \
     smooth_reg<L, NW, NC>(g, d, q.DSC, q.D, p, s, q.ncall, J, lds);                                          \
@@ -2454,38 +2500,6 @@ SMOOTH_REG(2, 16, 2, c2)
 SMOOTH_REG(4, 16, 2, c2)
 SMOOTH_REG(8, 16, 2, c2)
 
-// SMOOTH_VAR over all of a step's planes in one launch when the last
-// NSFEXIT planes take exit smoothing: the first nlead workgroups (a
-// multiple of 8, so the XCD order of the others is kept) take the exit
-// planes jexit..JM in the LDS form, and being dispatched first they run
-// beside the others, which take the planes 1..jexit-1 in registers (8
-// wavefronts of L lines).  nv variables (e), the variable fastest.
-// Dynamic LDS: the larger of the two forms' needs.
-#define SMOOTH_MIX(L, NC, SFX)                                                                                    \
-  __global__ void __launch_bounds__(512) k_smooth_mix##L##SFX(Geo g, Dev d, EqSet e, Params p, StepScalars s,   \
-                                                              int nv, int jexit, int nlead) {                  \
-    if (gated(d)) return;                                                                                       \
-    extern __shared__ float lds[];                                                                              \
-    const int b = blockIdx.x;                                                                                   \
-    if (b < nlead) {                                                                                            \
-      if (b >= (g.JM - jexit + 1) * nv) return;                                                                 \
-      smooth_plane(g, d, eq_pick(e, b % nv), p, s, jexit + b / nv, lds);                                        \
-      return;                                                                                                   \
-    }                                                                                                           \
-    const int t = xcd_block(b - nlead, (jexit - 1) * nv);                                                       \
-    const EqLane &q = eq_pick(e, t % nv);                                                                       \
-    smooth_reg<L, 8, NC>(g, d, q.DSC, q.D, p, s, q.ncall, t / nv + 1, lds);                                     \
-  }
-SMOOTH_MIX(1, 1, )
-SMOOTH_MIX(2, 1, )
-SMOOTH_MIX(4, 1, )
-SMOOTH_MIX(8, 1, )
-SMOOTH_MIX(16, 1, )
-SMOOTH_MIX(1, 2, c2)
-SMOOTH_MIX(2, 2, c2)
-SMOOTH_MIX(4, 2, c2)
-SMOOTH_MIX(8, 2, c2)
-SMOOTH_MIX(16, 2, c2)
 #undef SMOOTH_REG
 
 // ---------------------------------------------------------------------------
