@@ -96,7 +96,7 @@ __global__ void k_to_bf16(const float *, unsigned short *, unsigned);
 __global__ void k_from_bf16(const unsigned short *, float *, unsigned);
 __global__ void k_mg_levels(Geo, MgFast, EqSet, const int *, const int *, const int *, const int *);
 __global__ void k_mg_apply_lines(Geo, Dev, EqSet);
-__global__ void k_flux_mom(Geo, Dev, EqSet);
+__global__ void k_flux_mom(Geo, Dev, EqSet, Params, StepScalars);
 __global__ void k_massflux_eq(Geo, Dev, EqSet, int);
 
 __global__ void k_xl_dist(Geo, Dev, const float *, float *, int);
@@ -1296,14 +1296,15 @@ int enqueue_step(mh_ctx *c, const StepPlan &q) {
     LAUNCH(k_setflo_apply, grid, kNodeBlock, 0, c->stream, g, d);
     LCHK("setflo");
   }
-  LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
+  // PEFF (mf:4751-4757): the batched momentum flux kernel forms it itself
+  if (!batched) LAUNCH(k_peff, grid, kNodeBlock, 0, c->stream, g, d, p, s);
   if (batched) {
     EqSet em{};
     em.q[0] = eq_lane(c, 0, d.ROVX, d.DROVX, 4);
     em.q[1] = eq_lane(c, 1, d.RORVT, d.DRORVT, 5);
     em.q[2] = eq_lane(c, 2, d.ROVR, d.DROVR, 6);
     const int ln[kMaxEq] = {0, 1, 2};
-    LAUNCH(k_flux_mom, grid, kNodeBlock, 0, c->stream, g, d, em);
+    LAUNCH(k_flux_mom, grid, kNodeBlock, 0, c->stream, g, d, em, p, s);
     LCHK("flux_mom");
     if ((r = tstep(c, em.q, ln, 3, q, CTL_SKIP))) return r;
   } else {

@@ -297,6 +297,22 @@ __global__ void k_flux_energy(Geo g, Dev d) {
 
 // ---------------------------------------------------------------------------
 // axial momentum fluxes, mf:4818-4896
+// PEFF at a node for the periodic momentum faces: the array k_peff wrote,
+// or (the batched momentum flux kernel) formed from P as k_peff forms it
+struct PeffArray {
+  const float *PEFF;
+  __device__ float operator()(const Geo &g, int I, int J, int K) const { return PEFF[IX(I, J, K)]; }
+};
+struct PeffFromP {
+  const float *P;
+  float PREFF, F_PDOWN;
+  __device__ float operator()(const Geo &g, int I, int J, int K) const {
+    const int JP1 = (J == g.JM) ? J : J + 1;
+    const size_t a = IX(I, J, K);
+    return P[a] - PREFF + F_PDOWN * (P[IX(I, JP1, K)] - P[a]);
+  }
+};
+
 
 __device__ __forceinline__ float tflux_axial(const Geo &g, const Dev &d, int I, int J, int K) {
   float AVGVX = d.VX[IX(I, J, K)] + d.VX[IX(I, J - 1, K)] + d.VX[IX(I, J - 1, K + 1)] + d.VX[IX(I, J, K + 1)];
@@ -305,10 +321,11 @@ __device__ __forceinline__ float tflux_axial(const Geo &g, const Dev &d, int I, 
 }
 
 // the periodic line's value at I = 1 and IM off the blade
-__device__ __forceinline__ float tflux_axial_per(const Geo &g, const Dev &d, size_t n, int J, int K) {
+template <class PE>
+__device__ __forceinline__ float tflux_axial_per(const Geo &g, const Dev &d, size_t n, int J, int K, const PE &pe) {
   const int IM = g.IM;
-  float AVGPCUSP = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)] +
-                   d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)];
+  float AVGPCUSP = pe(g, 1, J, K) + pe(g, 1, J - 1, K) + pe(g, 1, J, K + 1) + pe(g, 1, J - 1, K + 1) +
+                   pe(g, IM, J, K) + pe(g, IM, J - 1, K) + pe(g, IM, J, K + 1) + pe(g, IM, J - 1, K + 1);
   float AVGVXCUSP = d.VX[IX(1, J, K)] + d.VX[IX(1, J - 1, K)] + d.VX[IX(1, J, K + 1)] + d.VX[IX(1, J - 1, K + 1)] +
                     d.VX[IX(IM, J, K)] + d.VX[IX(IM, J - 1, K)] + d.VX[IX(IM, J, K + 1)] + d.VX[IX(IM, J - 1, K + 1)];
   return 0.125f * (d.FLOWT[n] * AVGVXCUSP + AVGPCUSP * d.ABX[n]);
@@ -330,7 +347,7 @@ __device__ __forceinline__ void flux_axial_at(const Geo &g, const Dev &d, int I,
   if (J >= 2 && K <= g.KMM1) {
     float tf;
     if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) {
-      tf = tflux_axial_per(g, d, n, J, K);
+      tf = tflux_axial_per(g, d, n, J, K, PeffArray{d.PEFF});
     } else {
       tf = tflux_axial(g, d, I, J, K);
     }
@@ -346,10 +363,11 @@ __global__ void k_flux_axial(Geo g, Dev d) {
 // ---------------------------------------------------------------------------
 // angular momentum fluxes, mf:4913-4956
 
-__device__ __forceinline__ float tflux_angular(const Geo &g, const Dev &d, int I, int J, int K) {
+template <class PE>
+__device__ __forceinline__ float tflux_angular(const Geo &g, const Dev &d, int I, int J, int K, const PE &pe) {
   float AVGR = 0.25f * d.RAVG_CELL[JKX(J, K)];
   float AVGVT = d.VT[IX(I, J, K)] + d.VT[IX(I, J - 1, K)] + d.VT[IX(I, J - 1, K + 1)] + d.VT[IX(I, J, K + 1)];
-  float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
+  float AVGP = pe(g, I, J, K) + pe(g, I, J - 1, K) + pe(g, I, J - 1, K + 1) + pe(g, I, J, K + 1);
   return (d.FLOWT[IX(I, J, K)] * AVGVT + d.ABT[JKX(J, K)] * AVGP) * AVGR;
 }
 
@@ -364,9 +382,9 @@ __device__ __forceinline__ void flux_angular_at(const Geo &g, const Dev &d, int 
   if (J >= 2 && K <= g.KMM1) {
     float tf;
     if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM))
-      tf = 0.5f * (tflux_angular(g, d, 1, J, K) + tflux_angular(g, d, g.IM, J, K));
+      tf = 0.5f * (tflux_angular(g, d, 1, J, K, PeffArray{d.PEFF}) + tflux_angular(g, d, g.IM, J, K, PeffArray{d.PEFF}));
     else
-      tf = tflux_angular(g, d, I, J, K);
+      tf = tflux_angular(g, d, I, J, K, PeffArray{d.PEFF});
     d.TFLUX[n] = tf;
   }
   if (J >= 2 && I <= g.IMM1) {
@@ -385,16 +403,17 @@ __global__ void k_flux_angular(Geo g, Dev d) {
 // radial momentum fluxes and source, mf:5003-5095
 
 // the periodic line's value at I = 1 and IM off the blade
-__device__ __forceinline__ float tflux_radial_per(const Geo &g, const Dev &d, size_t n, int J, int K) {
+template <class PE>
+__device__ __forceinline__ float tflux_radial_per(const Geo &g, const Dev &d, size_t n, int J, int K, const PE &pe) {
   const int IM = g.IM;
   float AVGVR1 = d.VR[IX(1, J, K)] + d.VR[IX(1, J - 1, K)] + d.VR[IX(1, J - 1, K + 1)] + d.VR[IX(1, J, K + 1)];
-  float AVGPB1 = d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J - 1, K + 1)] + d.PEFF[IX(1, J, K + 1)];
+  float AVGPB1 = pe(g, 1, J, K) + pe(g, 1, J - 1, K) + pe(g, 1, J - 1, K + 1) + pe(g, 1, J, K + 1);
   float TF1 = .25f * (d.FLOWT[IX(1, J, K)] * AVGVR1 + AVGPB1 * d.ABR[IX(1, J, K)]);
   float AVGVRM = d.VR[IX(IM, J, K)] + d.VR[IX(IM, J - 1, K)] + d.VR[IX(IM, J - 1, K + 1)] + d.VR[IX(IM, J, K + 1)];
-  float AVGPBM = d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J - 1, K + 1)] + d.PEFF[IX(IM, J, K + 1)];
+  float AVGPBM = pe(g, IM, J, K) + pe(g, IM, J - 1, K) + pe(g, IM, J - 1, K + 1) + pe(g, IM, J, K + 1);
   float TFM = .25f * (d.FLOWT[IX(IM, J, K)] * AVGVRM + AVGPBM * d.ABR[IX(IM, J, K)]);
-  float AVGP1 = 0.25f * (d.PEFF[IX(1, J, K)] + d.PEFF[IX(1, J - 1, K)] + d.PEFF[IX(1, J, K + 1)] + d.PEFF[IX(1, J - 1, K + 1)]);
-  float AVGPM = 0.25f * (d.PEFF[IX(IM, J, K)] + d.PEFF[IX(IM, J - 1, K)] + d.PEFF[IX(IM, J, K + 1)] + d.PEFF[IX(IM, J - 1, K + 1)]);
+  float AVGP1 = 0.25f * (pe(g, 1, J, K) + pe(g, 1, J - 1, K) + pe(g, 1, J, K + 1) + pe(g, 1, J - 1, K + 1));
+  float AVGPM = 0.25f * (pe(g, IM, J, K) + pe(g, IM, J - 1, K) + pe(g, IM, J, K + 1) + pe(g, IM, J - 1, K + 1));
   float T1 = TF1 - AVGP1 * d.ABR[IX(1, J, K)];
   float T2 = TFM - AVGPM * d.ABR[IX(IM, J, K)];
   return 0.5f * ((T1 + T2) + (AVGP1 + AVGPM) * d.ABR[n]);
@@ -411,7 +430,7 @@ __device__ __forceinline__ void flux_radial_at(const Geo &g, const Dev &d, int I
     float AVGVR = d.VR[IX(I, J, K)] + d.VR[IX(I, J - 1, K)] + d.VR[IX(I, J - 1, K + 1)] + d.VR[IX(I, J, K + 1)];
     float AVGP = d.PEFF[IX(I, J, K)] + d.PEFF[IX(I, J - 1, K)] + d.PEFF[IX(I, J - 1, K + 1)] + d.PEFF[IX(I, J, K + 1)];
     float tf = .25f * (d.FLOWT[n] * AVGVR + AVGP * d.ABR[n]);
-    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) tf = tflux_radial_per(g, d, n, J, K);
+    if (d.IND[J - 1] != 1 && (I == 1 || I == g.IM)) tf = tflux_radial_per(g, d, n, J, K, PeffArray{d.PEFF});
     d.TFLUX[n] = tf;
   }
   if (J >= 2 && I <= g.IMM1) {
@@ -590,8 +609,15 @@ __global__ void k_massflux_eq(Geo g, Dev d, EqSet e, int nq) {
 // into the flux arrays of e.q[0], e.q[1] and e.q[2]: the node values they
 // share (PEFF at the eight corners, the mass fluxes) are loaded once and
 // every value before any store.  The expressions and their operand order
-// are those of k_flux_axial, k_flux_angular and k_flux_radial.
-__global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
+// are those of k_flux_axial, k_flux_angular and k_flux_radial.  PEFF
+// (mf:4751-4757) is formed at the corners from P as k_peff forms it (P at
+// the corner and at the next J, the last J taking itself) and stored at the
+// thread's own node, so the step needs no separate PEFF launch.
+__device__ __forceinline__ float peff_of(const Dev &d, const Params &p, const StepScalars &s, unsigned a,
+                                         unsigned b) {
+  return d.P[a] - s.PREFF + p.F_PDOWN * (d.P[b] - d.P[a]);
+}
+__global__ void k_flux_mom(Geo g, Dev d, EqSet e, Params pr, StepScalars ss) {
   NODE_IJK
   const int IM = g.IM;
   const bool fx = I <= g.IMM1 && K <= g.KMM1, fr = J >= 2 && I <= g.IMM1, ft = J >= 2 && K <= g.KMM1;
@@ -606,7 +632,13 @@ __global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
   const float v##000 = d.A[n], v##001 = d.A[c001], v##100 = d.A[c100], v##101 = d.A[c101], v##m00 = d.A[cm00], \
               v##m01 = d.A[cm01], v##m10 = d.A[cm10];
 #define C8(A, v) C7(A, v) const float v##m11 = d.A[cm11];
-  C8(PEFF, p)
+  // the corners' next-J nodes: planes J+1 (J at the last plane) and J
+  const int Jn = (J == g.JM) ? J : J + 1;
+  const unsigned n001 = IX(I, Jn, Kp), n100 = IX(Ip, Jn, K), n101 = IX(Ip, Jn, Kp), n000 = IX(I, Jn, K);
+  const float p000 = peff_of(d, pr, ss, n, n000), p001 = peff_of(d, pr, ss, c001, n001),
+              p100 = peff_of(d, pr, ss, c100, n100), p101 = peff_of(d, pr, ss, c101, n101),
+              pm00 = peff_of(d, pr, ss, cm00, n), pm01 = peff_of(d, pr, ss, cm01, c001),
+              pm10 = peff_of(d, pr, ss, cm10, c100), pm11 = peff_of(d, pr, ss, cm11, c101);
   C7(VX, x)
   C8(VT, t)
   C7(VR, r)
@@ -649,9 +681,10 @@ __global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
   if (ft) {
     const float AVGP = p000 + pm00 + pm01 + p001;
     if (per) {
-      axT = tflux_axial_per(g, d, n, J, K);
-      anT = 0.5f * (tflux_angular(g, d, 1, J, K) + tflux_angular(g, d, IM, J, K));
-      raT = tflux_radial_per(g, d, n, J, K);
+      const PeffFromP pe{d.P, ss.PREFF, pr.F_PDOWN};
+      axT = tflux_axial_per(g, d, n, J, K, pe);
+      anT = 0.5f * (tflux_angular(g, d, 1, J, K, pe) + tflux_angular(g, d, IM, J, K, pe));
+      raT = tflux_radial_per(g, d, n, J, K, pe);
     } else {
       const float AVGVX = x000 + xm00 + xm01 + x001;
       axT = 0.25f * (flt * AVGVX + d.ABX[n] * AVGP);
@@ -664,6 +697,7 @@ __global__ void k_flux_mom(Geo g, Dev d, EqSet e) {
   }
   const EqLane &qa = e.q[0], &qt = e.q[1], &qr = e.q[2];
   if (gated(d)) return;
+  d.PEFF[n] = p000;
   if (fx) {
     qa.SOURCE[n] = axS;
     qa.XFLUX[n] = axX;
